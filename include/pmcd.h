@@ -1,0 +1,166 @@
+/*
+ * C ABI of the MI355X convection-diffusion assemble-and-solve library
+ * (libpmcd.so). Plain C types only; every entry point returns 0 on success
+ * and a non-zero status on failure, with the message available from
+ * pmcd_last_error() (the reference throws std::runtime_error through ErrThrow,
+ * include/General/MooNMD_Io.h:435-447; exceptions never cross this boundary).
+ *
+ * Each entry point names the reference interface it stands in for.
+ */
+#ifndef PMCD_H
+#define PMCD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pmcd_system pmcd_system;
+
+/* Geometry codes: default coarse meshes of the reference
+ * (src/Geometry/Tests.C:95,473,1893,1937) refined n_refine times, or
+ * structured N^3 cubes. */
+enum {
+  PMCD_UNIT_SQUARE = 0,
+  PMCD_TWO_TRIANGLES = 1,
+  PMCD_TWO_TRIANGLES2 = 2,
+  PMCD_CUBE_HEXA = 3,
+  PMCD_CUBE_TETRA = 4,
+  PMCD_STRUCTURED_HEXA = 10,
+  PMCD_STRUCTURED_TETRA = 11
+};
+
+/* Example codes (PDE data, include/Examples/CD_2D/SineLaplace.h,
+ * include/Examples/CD_3D/test_p0.h, test_p1.h, test_p2.h and the custom
+ * manufactured/constant-data examples of the benchmark configurations). */
+enum {
+  PMCD_EX_SINE_LAPLACE_2D = 0,
+  PMCD_EX_SINE_2D = 1,
+  PMCD_EX_SINE_3D = 2,
+  PMCD_EX_TEST_P0_3D = 3,
+  PMCD_EX_TEST_P1_3D = 4,
+  PMCD_EX_TEST_P2_3D = 5,
+  PMCD_EX_CONST_RHS = 6
+};
+
+typedef struct {
+  int geometry;     /* PMCD_* geometry code */
+  int n_refine;     /* uniform refinement steps (refinement_n_initial_steps) */
+  int structured_n; /* cells per edge for PMCD_STRUCTURED_* */
+  int order;        /* ANSATZ_ORDER: 1 or 2 */
+  int supg;         /* space_discretization_type: 0 galerkin, 1 supg */
+  int linear_quad;  /* INTERNAL_PROBLEM_LINEAR: quadrature degree 2p instead of 3p */
+  int mg_levels;    /* multigrid_n_levels (1: finest level only) */
+  int example;      /* PMCD_EX_* */
+  double eps;       /* diffusion coefficient */
+  double b[3];      /* convection */
+  double c;         /* reaction (PMCD_EX_CONST_RHS) */
+  double f0;        /* right-hand side (PMCD_EX_CONST_RHS) */
+} pmcd_problem_desc;
+
+enum { PMCD_DIRECT = 0, PMCD_CG = 1, PMCD_FGMRES = 2, PMCD_RICHARDSON = 3 };
+enum { PMCD_NO_PRECOND = 0, PMCD_JACOBI = 1, PMCD_MULTIGRID = 2 };
+
+typedef struct {
+  int method;          /* solver_type / iterative_solver_type */
+  int precond;         /* preconditioner */
+  int max_iter;        /* max_n_iterations */
+  int min_iter;        /* min_n_iterations */
+  double res_tol;      /* residual_tolerance */
+  double res_red;      /* residual_reduction */
+  double div_factor;   /* divergence_factor */
+  int restart;         /* gmres_restart */
+  int n_pre, n_post;   /* multigrid_n_pre_smooth, multigrid_n_post_smooth */
+  double jacobi_damp;  /* jacobi_damp */
+  double mg_damp;      /* multigrid_correction_damp_factor */
+  int mixed;           /* 1: multigrid V-cycle in FP32 inside the FP64 outer solver */
+} pmcd_solver_desc;
+
+typedef struct {
+  int iterations;
+  int converged;
+  double initial_residual;
+  double final_residual;
+  double seconds;
+} pmcd_solve_stats;
+
+/* Message of the last failure on the calling thread. */
+const char* pmcd_last_error(void);
+/* Number of visible GPUs. */
+int pmcd_device_count(int* n);
+
+/* Host-only setup (no GPU needed): mesh hierarchy, FE spaces and sparsity,
+ * the part of ConvectionDiffusion<d>'s constructor that stays on the host:
+ * TDomain::refine_and_get_hierarchy_of_collections (Domain.C:2796-2900),
+ * TFESpace3D::ConstructSpace + compute_various_numbers (FESpace3D.C:186-804,
+ * FESpace.C:1863-2099), TStructure(space) (Structure.C:226-618). Levels are
+ * numbered 0 = coarsest, -1 = finest; info as in pmcd_system_info. */
+typedef struct pmcd_host pmcd_host;
+int pmcd_host_create(const pmcd_problem_desc* desc, pmcd_host** out);
+int pmcd_host_destroy(pmcd_host* h);
+int pmcd_host_info(pmcd_host* h, int level, long long* info);
+int pmcd_host_dofmap(pmcd_host* h, int level, int* out);
+int pmcd_host_pattern(pmcd_host* h, int level, int* row_ptr, int* col);
+int pmcd_host_cell_coords(pmcd_host* h, int level, double* out);
+
+/* Mesh hierarchy, FE spaces, sparsity, device buffers.
+ * Replaces ConvectionDiffusion<d>::ConvectionDiffusion
+ * (src/System/ConvectionDiffusion.C:112-180): TDomain refinement,
+ * TFESpace2D/3D (FESpace3D.C:186-804), TStructure (Structure.C:226-618). */
+int pmcd_system_create(const pmcd_problem_desc* desc, pmcd_system** out);
+int pmcd_system_destroy(pmcd_system* s);
+
+/* info[0..7] = n_cells, n_dof, n_active, nnz, n_local, n_quad_points,
+ * n_levels, n_dirichlet of level `level` (0 = coarsest, -1 = finest). */
+int pmcd_system_info(pmcd_system* s, int level, long long* info);
+/* GlobalNumbers of the level's FE space (TFESpace::GetGlobalDOF),
+ * n_cells * n_local ints. */
+int pmcd_get_dofmap(pmcd_system* s, int level, int* out);
+/* CSR pattern (TStructure::get_row_ptr / get_vector_columns). */
+int pmcd_get_pattern(pmcd_system* s, int level, int* row_ptr, int* col);
+/* vertex coordinates per cell, n_cells * n_vertices_per_cell * 3 doubles */
+int pmcd_get_cell_coords(pmcd_system* s, int level, double* out);
+
+/* Assembly of matrix and right-hand side on every level.
+ * Replaces ConvectionDiffusion<d>::assemble (ConvectionDiffusion.C:238-254)
+ * -> LocalAssembling<d> + Assemble2D/Assemble3D (Assemble3D.C:39-917,
+ * Assemble2D.C:2798-3430). `ms` (may be NULL) receives the time. */
+int pmcd_assemble(pmcd_system* s, double* ms);
+int pmcd_get_matrix(pmcd_system* s, int level, double* val);
+int pmcd_get_rhs(pmcd_system* s, int level, double* rhs);
+
+/* Solve on the finest level. Replaces Solver<BlockFEMatrix,BlockVector>::solve
+ * (src/Solver/Solver.C:601-696) with CG (Iteration_cg.C:27-147), flexible
+ * GMRES (Iteration_gmres.C:430-620), Jacobi (Iteration_jacobi.C) and
+ * multigrid (Multigrid.C:157-410) preconditioning. */
+int pmcd_solve(pmcd_system* s, const pmcd_solver_desc* sd, pmcd_solve_stats* stats);
+int pmcd_get_solution(pmcd_system* s, double* out);
+int pmcd_set_solution(pmcd_system* s, const double* in);
+
+/* L2, H1-semi, SD, DG-volume and L-infinity errors of the finest solution.
+ * Replaces TFEFunction3D::GetErrors / TFEFunction2D::GetErrors with
+ * conv_diff_l2_h1_linf_error<d> (ConvDiff.C:2124-2183). */
+int pmcd_errors(pmcd_system* s, double* out5);
+
+/* y = A x (mode 0, BlockFEMatrix::apply) or y += a A x (mode 1,
+ * BlockFEMatrix::apply_scaled_add), BlockFEMatrix.C:969-1095, on host arrays
+ * of the level's length. */
+int pmcd_apply(pmcd_system* s, int level, const double* x, double* y, double a, int mode);
+/* One multigrid cycle z = MG(v) on the finest level with the given solver
+ * settings (Iteration_multigrid::apply, Iteration_multigrid.C:23-34). */
+int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, double* z);
+
+/* Device-pointer entry points (all pointers are device memory, `stream` a
+ * hipStream_t or NULL). FEMatrix::multiplyActive + addScaledNonActive:
+ * y[r] = beta*z[r] + alpha*(A x)[r] for r < n_active, beta*z[r] + alpha*x[r]
+ * otherwise. */
+int pmcd_spmv_device(int n, int n_active, const int* row_ptr, const int* col, const double* val,
+                     const double* x, const double* z, double* y, double alpha, double beta,
+                     void* stream);
+/* Ddot (Blas1.C:21-38): deterministic two-pass reduction, result to host. */
+int pmcd_dot_device(int n, const double* x, const double* y, double* result, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PMCD_H */

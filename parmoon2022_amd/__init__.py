@@ -1,0 +1,291 @@
+"""MI355X convection-diffusion assemble-and-solve path (ctypes binding of libpmcd.so).
+
+The classes mirror the reference's interface for this path:
+``ConvectionDiffusion`` (src/System/ConvectionDiffusion.C: constructor,
+assemble(), solve(), output(), get_L2_error() ...), driven by a parameter
+dictionary with the reference's parameter names (``ParameterDatabase`` keys of
+Solver<>::default_solver_database, Multigrid::default_multigrid_database,
+LocalAssembling::default_local_assembling_database).
+
+The compute path is the HIP library only: if libpmcd.so is missing, every
+entry point raises; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpmcd.so")
+_lib = None
+
+GEOMETRIES = {"UnitSquare": 0, "TwoTriangles": 1, "TwoTriangles2": 2,
+              "Default_UnitCube_Hexa": 3, "Default_UnitCube_Tetra": 4,
+              "StructuredHexa": 10, "StructuredTetra": 11}
+EXAMPLES = {"sine_laplace_2d": 0, "sine_2d": 1, "sine_3d": 2, "test_p0": 3,
+            "test_p1": 4, "test_p2": 5, "const_rhs": 6}
+METHODS = {"direct": 0, "cg": 1, "fgmres": 2, "richardson": 3}
+PRECONDS = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2}
+
+
+class ProblemDesc(ctypes.Structure):
+    _fields_ = [("geometry", ctypes.c_int), ("n_refine", ctypes.c_int),
+                ("structured_n", ctypes.c_int), ("order", ctypes.c_int),
+                ("supg", ctypes.c_int), ("linear_quad", ctypes.c_int),
+                ("mg_levels", ctypes.c_int), ("example", ctypes.c_int),
+                ("eps", ctypes.c_double), ("b", ctypes.c_double * 3),
+                ("c", ctypes.c_double), ("f0", ctypes.c_double)]
+
+
+class SolverDesc(ctypes.Structure):
+    _fields_ = [("method", ctypes.c_int), ("precond", ctypes.c_int),
+                ("max_iter", ctypes.c_int), ("min_iter", ctypes.c_int),
+                ("res_tol", ctypes.c_double), ("res_red", ctypes.c_double),
+                ("div_factor", ctypes.c_double), ("restart", ctypes.c_int),
+                ("n_pre", ctypes.c_int), ("n_post", ctypes.c_int),
+                ("jacobi_damp", ctypes.c_double), ("mg_damp", ctypes.c_double),
+                ("mixed", ctypes.c_int)]
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("initial_residual", ctypes.c_double), ("final_residual", ctypes.c_double),
+                ("seconds", ctypes.c_double)]
+
+
+EXPORTS = [
+    "pmcd_last_error", "pmcd_device_count",
+    "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
+    "pmcd_host_pattern", "pmcd_host_cell_coords",
+    "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
+    "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
+    "pmcd_get_rhs", "pmcd_solve", "pmcd_get_solution", "pmcd_set_solution", "pmcd_errors",
+    "pmcd_apply", "pmcd_vcycle", "pmcd_spmv_device", "pmcd_dot_device",
+]
+
+
+def lib():
+    """Load libpmcd.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libpmcd.so not built: run __graft_entry__.build() "
+                               "(python -m parmoon2022_amd.build)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.pmcd_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().pmcd_last_error().decode())
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def problem_desc(db):
+    """Build a ProblemDesc from a parameter dictionary (reference key names)."""
+    d = ProblemDesc()
+    d.geometry = GEOMETRIES[db.get("geo_file", "UnitSquare")]
+    d.n_refine = int(db.get("refinement_n_initial_steps", 0))
+    d.structured_n = int(db.get("structured_n", 1))
+    d.order = int(db.get("ANSATZ_ORDER", 1))
+    d.supg = 1 if db.get("space_discretization_type", "galerkin") == "supg" else 0
+    d.linear_quad = int(db.get("INTERNAL_PROBLEM_LINEAR", 0))
+    use_mg = db.get("preconditioner") == "multigrid" and db.get("solver_type") == "iterative"
+    d.mg_levels = int(db.get("multigrid_n_levels", 1)) if use_mg else 1
+    d.example = EXAMPLES[db.get("example", "sine_laplace_2d")]
+    d.eps = float(db.get("diffusion_coefficient", 1.0))
+    b = db.get("convection", (0.0, 0.0, 0.0))
+    for k in range(3):
+        d.b[k] = float(b[k]) if k < len(b) else 0.0
+    d.c = float(db.get("reaction", 0.0))
+    d.f0 = float(db.get("rhs_constant", 1.0))
+    return d
+
+
+def solver_desc(db):
+    """Build a SolverDesc from reference solver/multigrid parameter names."""
+    s = SolverDesc()
+    if db.get("solver_type", "direct") == "direct":
+        s.method = 0
+    else:
+        s.method = METHODS[db.get("iterative_solver_type", "fgmres")]
+    s.precond = PRECONDS[db.get("preconditioner", "no_preconditioner")]
+    s.max_iter = int(db.get("max_n_iterations", 100))
+    s.min_iter = int(db.get("min_n_iterations", 0))
+    s.res_tol = float(db.get("residual_tolerance", 1e-8))
+    s.res_red = float(db.get("residual_reduction", 0.0))
+    s.div_factor = float(db.get("divergence_factor", 1e10))
+    s.restart = int(db.get("gmres_restart", 20))
+    s.n_pre = int(db.get("multigrid_n_pre_smooth", 1))
+    s.n_post = int(db.get("multigrid_n_post_smooth", 1))
+    s.jacobi_damp = float(db.get("jacobi_damp", 1.0))
+    s.mg_damp = float(db.get("multigrid_correction_damp_factor", 1.0))
+    s.mixed = int(db.get("multigrid_mixed_precision", 0))
+    return s
+
+
+class HostSetup:
+    """Host-only mesh/FE-space/sparsity setup (no GPU)."""
+
+    def __init__(self, db):
+        self._h = ctypes.c_void_p()
+        self.desc = problem_desc(db)
+        _check(lib().pmcd_host_create(ctypes.byref(self.desc), ctypes.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().pmcd_host_destroy(self._h)
+            self._h = None
+
+    def info(self, level=-1):
+        a = (ctypes.c_longlong * 8)()
+        _check(lib().pmcd_host_info(self._h, level, a))
+        keys = ["n_cells", "n_dof", "n_active", "nnz", "n_local", "n_quad", "n_levels", "n_dirichlet"]
+        return dict(zip(keys, list(a)))
+
+    def dofmap(self, level=-1):
+        i = self.info(level)
+        out = np.empty(i["n_cells"] * i["n_local"], dtype=np.int32)
+        _check(lib().pmcd_host_dofmap(self._h, level, _ptr(out, ctypes.c_int)))
+        return out.reshape(i["n_cells"], i["n_local"])
+
+    def pattern(self, level=-1):
+        i = self.info(level)
+        rp = np.empty(i["n_dof"] + 1, dtype=np.int32)
+        col = np.empty(i["nnz"], dtype=np.int32)
+        _check(lib().pmcd_host_pattern(self._h, level, _ptr(rp, ctypes.c_int), _ptr(col, ctypes.c_int)))
+        return rp, col
+
+    def cell_coords(self, level=-1):
+        i = self.info(level)
+        nv = VERTICES_PER_CELL[self.desc.geometry]
+        out = np.empty(i["n_cells"] * nv * 3, dtype=np.float64)
+        _check(lib().pmcd_host_cell_coords(self._h, level, _ptr(out, ctypes.c_double)))
+        return out.reshape(i["n_cells"], nv, 3)
+
+
+VERTICES_PER_CELL = {0: 4, 1: 3, 2: 3, 3: 8, 4: 4, 10: 8, 11: 4}
+
+
+class ConvectionDiffusion:
+    """Device-resident CD problem: ConvectionDiffusion<d> of the reference."""
+
+    def __init__(self, db):
+        self.db = dict(db)
+        self._s = ctypes.c_void_p()
+        self.desc = problem_desc(self.db)
+        _check(lib().pmcd_system_create(ctypes.byref(self.desc), ctypes.byref(self._s)))
+        self.errors = np.zeros(5)
+        self.stats = SolveStats()
+        self.assemble_ms = 0.0
+
+    def close(self):
+        if getattr(self, "_s", None):
+            lib().pmcd_system_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self, level=-1):
+        a = (ctypes.c_longlong * 8)()
+        _check(lib().pmcd_system_info(self._s, level, a))
+        keys = ["n_cells", "n_dof", "n_active", "nnz", "n_local", "n_quad", "n_levels", "n_dirichlet"]
+        return dict(zip(keys, list(a)))
+
+    def assemble(self):
+        ms = ctypes.c_double()
+        _check(lib().pmcd_assemble(self._s, ctypes.byref(ms)))
+        self.assemble_ms = ms.value
+
+    def solve(self, db=None):
+        sd = solver_desc(self.db if db is None else db)
+        st = SolveStats()
+        _check(lib().pmcd_solve(self._s, ctypes.byref(sd), ctypes.byref(st)))
+        self.stats = st
+        return st
+
+    def output(self):
+        e = np.zeros(5)
+        _check(lib().pmcd_errors(self._s, _ptr(e, ctypes.c_double)))
+        self.errors = e
+        return e
+
+    def get_L2_error(self):
+        return self.errors[0]
+
+    def get_H1_semi_error(self):
+        return self.errors[1]
+
+    def get_SD_error(self):
+        return self.errors[2]
+
+    def get_L_inf_error(self):
+        return self.errors[4]
+
+    def dofmap(self, level=-1):
+        i = self.info(level)
+        out = np.empty(i["n_cells"] * i["n_local"], dtype=np.int32)
+        _check(lib().pmcd_get_dofmap(self._s, level, _ptr(out, ctypes.c_int)))
+        return out.reshape(i["n_cells"], i["n_local"])
+
+    def pattern(self, level=-1):
+        i = self.info(level)
+        rp = np.empty(i["n_dof"] + 1, dtype=np.int32)
+        col = np.empty(i["nnz"], dtype=np.int32)
+        _check(lib().pmcd_get_pattern(self._s, level, _ptr(rp, ctypes.c_int), _ptr(col, ctypes.c_int)))
+        return rp, col
+
+    def matrix_values(self, level=-1):
+        i = self.info(level)
+        v = np.empty(i["nnz"], dtype=np.float64)
+        _check(lib().pmcd_get_matrix(self._s, level, _ptr(v, ctypes.c_double)))
+        return v
+
+    def rhs(self, level=-1):
+        i = self.info(level)
+        v = np.empty(i["n_dof"], dtype=np.float64)
+        _check(lib().pmcd_get_rhs(self._s, level, _ptr(v, ctypes.c_double)))
+        return v
+
+    def solution(self):
+        i = self.info(-1)
+        v = np.empty(i["n_dof"], dtype=np.float64)
+        _check(lib().pmcd_get_solution(self._s, _ptr(v, ctypes.c_double)))
+        return v
+
+    def set_solution(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(lib().pmcd_set_solution(self._s, _ptr(v, ctypes.c_double)))
+
+    def apply(self, x, level=-1, y=None, a=1.0):
+        """BlockFEMatrix::apply (y None) or apply_scaled_add (y given)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        mode = 0 if y is None else 1
+        out = np.zeros_like(x) if y is None else np.array(y, dtype=np.float64)
+        _check(lib().pmcd_apply(self._s, level, _ptr(x, ctypes.c_double),
+                                _ptr(out, ctypes.c_double), ctypes.c_double(a), mode))
+        return out
+
+    def vcycle(self, v, db=None):
+        sd = solver_desc(self.db if db is None else db)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        z = np.zeros_like(v)
+        _check(lib().pmcd_vcycle(self._s, ctypes.byref(sd), _ptr(v, ctypes.c_double),
+                                 _ptr(z, ctypes.c_double)))
+        return z
+
+
+def gpu_available():
+    n = ctypes.c_int(0)
+    try:
+        if lib().pmcd_device_count(ctypes.byref(n)) != 0:
+            return False
+    except RuntimeError:
+        return False
+    return n.value > 0

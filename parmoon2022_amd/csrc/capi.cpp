@@ -1,0 +1,322 @@
+// extern "C" boundary of libpmcd.so (declarations in include/pmcd.h).
+#include "../../include/pmcd.h"
+#include "system.hpp"
+#include <cstring>
+#include <string>
+
+using namespace pmcd;
+
+struct pmcd_system {
+  hipStream_t stream = nullptr;
+  std::unique_ptr<CDSystem> sys;
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch(const std::exception& e) {
+    g_err = e.what();
+  } catch(...) {
+    g_err = "unknown error";
+  }
+  return 1;
+}
+
+Level& level_of(pmcd_system* s, int level) {
+  PMCD_CHECK(s && s->sys, "null system");
+  const int nl = s->sys->n_levels();
+  if(level < 0) level = nl + level;
+  PMCD_CHECK(level >= 0 && level < nl, "level ", level, " out of range");
+  return s->sys->level(level);
+}
+
+ProblemDesc to_desc(const pmcd_problem_desc& d) {
+  ProblemDesc p{};
+  p.geometry = d.geometry;
+  p.n_refine = d.n_refine;
+  p.structured_n = d.structured_n;
+  p.order = d.order;
+  p.supg = d.supg;
+  p.linear_quad = d.linear_quad;
+  p.mg_levels = d.mg_levels;
+  p.ex.kind = d.example;
+  p.ex.eps = d.eps;
+  for(int k = 0; k < 3; ++k) p.ex.b[k] = d.b[k];
+  p.ex.c = d.c;
+  p.ex.f0 = d.f0;
+  return p;
+}
+
+SolverDesc to_solver(const pmcd_solver_desc& s) {
+  SolverDesc d{};
+  d.method = s.method;
+  d.precond = s.precond;
+  d.max_iter = s.max_iter;
+  d.min_iter = s.min_iter;
+  d.res_tol = s.res_tol;
+  d.res_red = s.res_red;
+  d.div_factor = s.div_factor > 0 ? s.div_factor : 1e10;
+  d.restart = s.restart > 0 ? s.restart : 20;
+  d.n_pre = s.n_pre;
+  d.n_post = s.n_post;
+  d.jacobi_damp = s.jacobi_damp;
+  d.mg_damp = s.mg_damp;
+  d.mixed = s.mixed;
+  return d;
+}
+}  // namespace
+
+struct pmcd_host {
+  std::vector<std::unique_ptr<Level>> levels;
+  int n_quad = 0;
+};
+
+namespace {
+Level& host_level(pmcd_host* h, int level) {
+  PMCD_CHECK(h, "null handle");
+  const int nl = (int)h->levels.size();
+  if(level < 0) level = nl + level;
+  PMCD_CHECK(level >= 0 && level < nl, "level ", level, " out of range");
+  return *h->levels[level];
+}
+void fill_info(const Level& L, int n_quad, int n_levels, long long* info) {
+  info[0] = L.mesh.n_cells;
+  info[1] = L.space.n_dof;
+  info[2] = L.space.n_active;
+  info[3] = L.st.nnz;
+  info[4] = L.space.n_local;
+  info[5] = n_quad;
+  info[6] = n_levels;
+  info[7] = L.space.n_dirichlet;
+}
+void copy_coords(const Mesh& m, double* out) {
+  for(int c = 0; c < m.n_cells; ++c)
+    for(int a = 0; a < m.nv; ++a)
+      for(int k = 0; k < 3; ++k) out[((size_t)c * m.nv + a) * 3 + k] = m.coords[3 * m.cell(c)[a] + k];
+}
+}  // namespace
+
+extern "C" {
+
+const char* pmcd_last_error(void) { return g_err.c_str(); }
+
+int pmcd_host_create(const pmcd_problem_desc* desc, pmcd_host** out) {
+  return guarded([&] {
+    PMCD_CHECK(desc && out, "null argument");
+    auto h = std::make_unique<pmcd_host>();
+    ProblemDesc d = to_desc(*desc);
+    h->levels = build_levels(d);
+    const Shape s = h->levels.back()->mesh.shape;
+    h->n_quad = quad_formula(cell_rule_for(s, d.order, d.linear_quad != 0)).n();
+    *out = h.release();
+  });
+}
+
+int pmcd_host_destroy(pmcd_host* h) {
+  return guarded([&] { delete h; });
+}
+
+int pmcd_host_info(pmcd_host* h, int level, long long* info) {
+  return guarded([&] { fill_info(host_level(h, level), h->n_quad, (int)h->levels.size(), info); });
+}
+
+int pmcd_host_dofmap(pmcd_host* h, int level, int* out) {
+  return guarded([&] {
+    Level& L = host_level(h, level);
+    std::memcpy(out, L.space.dof.data(), L.space.dof.size() * sizeof(int));
+  });
+}
+
+int pmcd_host_pattern(pmcd_host* h, int level, int* row_ptr, int* col) {
+  return guarded([&] {
+    Level& L = host_level(h, level);
+    std::memcpy(row_ptr, L.st.row_ptr.data(), L.st.row_ptr.size() * sizeof(int));
+    std::memcpy(col, L.st.col.data(), L.st.col.size() * sizeof(int));
+  });
+}
+
+int pmcd_host_cell_coords(pmcd_host* h, int level, double* out) {
+  return guarded([&] { copy_coords(host_level(h, level).mesh, out); });
+}
+
+int pmcd_device_count(int* n) {
+  return guarded([&] { PMCD_HIP(hipGetDeviceCount(n)); });
+}
+
+int pmcd_system_create(const pmcd_problem_desc* desc, pmcd_system** out) {
+  return guarded([&] {
+    PMCD_CHECK(desc && out, "null argument");
+    int ndev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+      PMCD_FAIL("no GPU visible: the HIP path needs an MI355X device");
+    auto s = std::make_unique<pmcd_system>();
+    PMCD_HIP(hipStreamCreate(&s->stream));
+    try {
+      s->sys = std::make_unique<CDSystem>(to_desc(*desc), s->stream);
+    } catch(...) {
+      (void)hipStreamDestroy(s->stream);
+      throw;
+    }
+    *out = s.release();
+  });
+}
+
+int pmcd_system_destroy(pmcd_system* s) {
+  return guarded([&] {
+    if(!s) return;
+    s->sys.reset();
+    if(s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+  });
+}
+
+int pmcd_system_info(pmcd_system* s, int level, long long* info) {
+  return guarded([&] {
+    Level& L = level_of(s, level);
+    info[0] = L.mesh.n_cells;
+    info[1] = L.n();
+    info[2] = L.n_active();
+    info[3] = L.st.nnz;
+    info[4] = L.space.n_local;
+    info[5] = quad_formula(s->sys->cell_rule()).n();
+    info[6] = s->sys->n_levels();
+    info[7] = L.space.n_dirichlet;
+  });
+}
+
+int pmcd_get_dofmap(pmcd_system* s, int level, int* out) {
+  return guarded([&] {
+    Level& L = level_of(s, level);
+    std::memcpy(out, L.space.dof.data(), L.space.dof.size() * sizeof(int));
+  });
+}
+
+int pmcd_get_pattern(pmcd_system* s, int level, int* row_ptr, int* col) {
+  return guarded([&] {
+    Level& L = level_of(s, level);
+    std::memcpy(row_ptr, L.st.row_ptr.data(), L.st.row_ptr.size() * sizeof(int));
+    std::memcpy(col, L.st.col.data(), L.st.col.size() * sizeof(int));
+  });
+}
+
+int pmcd_get_cell_coords(pmcd_system* s, int level, double* out) {
+  return guarded([&] {
+    Level& L = level_of(s, level);
+    const Mesh& m = L.mesh;
+    for(int c = 0; c < m.n_cells; ++c)
+      for(int a = 0; a < m.nv; ++a)
+        for(int k = 0; k < 3; ++k) out[((size_t)c * m.nv + a) * 3 + k] = m.coords[3 * m.cell(c)[a] + k];
+  });
+}
+
+int pmcd_assemble(pmcd_system* s, double* ms) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->sys, "null system");
+    s->sys->assemble();
+    if(ms) *ms = s->sys->last_assemble_ms;
+  });
+}
+
+int pmcd_get_matrix(pmcd_system* s, int level, double* val) {
+  return guarded([&] {
+    Level& L = level_of(s, level);
+    L.val.download(val, L.val.n, s->stream);
+    PMCD_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
+int pmcd_get_rhs(pmcd_system* s, int level, double* rhs) {
+  return guarded([&] {
+    Level& L = level_of(s, level);
+    L.rhs.download(rhs, L.n(), s->stream);
+    PMCD_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
+int pmcd_solve(pmcd_system* s, const pmcd_solver_desc* sd, pmcd_solve_stats* stats) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->sys && sd, "null argument");
+    SolveStats st = s->sys->solve(to_solver(*sd));
+    if(stats) {
+      stats->iterations = st.iterations;
+      stats->converged = st.converged;
+      stats->initial_residual = st.initial_residual;
+      stats->final_residual = st.final_residual;
+      stats->seconds = st.seconds;
+    }
+  });
+}
+
+int pmcd_get_solution(pmcd_system* s, double* out) {
+  return guarded([&] {
+    Level& L = level_of(s, -1);
+    L.sol.download(out, L.n(), s->stream);
+    PMCD_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
+int pmcd_set_solution(pmcd_system* s, const double* in) {
+  return guarded([&] {
+    Level& L = level_of(s, -1);
+    PMCD_HIP(hipMemcpyAsync(L.sol.p, in, sizeof(double) * L.n(), hipMemcpyHostToDevice, s->stream));
+    PMCD_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
+int pmcd_errors(pmcd_system* s, double* out5) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->sys, "null system");
+    s->sys->errors(out5);
+  });
+}
+
+int pmcd_apply(pmcd_system* s, int level, const double* x, double* y, double a, int mode) {
+  return guarded([&] {
+    Level& L = level_of(s, level);
+    DevBuf<double> dx, dy;
+    dx.upload(x, L.n(), s->stream);
+    if(mode == 1) dy.upload(y, L.n(), s->stream);
+    else dy.alloc(L.n());
+    if(mode == 1) s->sys->apply_scaled_add(L, dx.p, dy.p, a);
+    else s->sys->apply(L, dx.p, dy.p);
+    dy.download(y, L.n(), s->stream);
+    PMCD_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
+int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, double* z) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->sys && sd, "null argument");
+    Level& F = s->sys->finest();
+    s->sys->prepare(to_solver(*sd));
+    DevBuf<double> dv, dz(F.n());
+    dv.upload(v, F.n(), s->stream);
+    s->sys->vcycle(dv.p, dz.p);
+    dz.download(z, F.n(), s->stream);
+    PMCD_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
+int pmcd_spmv_device(int n, int n_active, const int* row_ptr, const int* col, const double* val,
+                     const double* x, const double* z, double* y, double alpha, double beta,
+                     void* stream) {
+  return guarded([&] {
+    launch_spmv(n, n_active, row_ptr, col, val, x, z, y, alpha, beta, (hipStream_t)stream);
+  });
+}
+
+int pmcd_dot_device(int n, const double* x, const double* y, double* result, void* stream) {
+  return guarded([&] {
+    DevBuf<double> part(1024 + 8), out(1);
+    launch_dot(n, x, y, part.p, out.p, (hipStream_t)stream);
+    PMCD_HIP(hipMemcpyAsync(result, out.p, sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)stream));
+    PMCD_HIP(hipStreamSynchronize((hipStream_t)stream));
+  });
+}
+
+}  // extern "C"

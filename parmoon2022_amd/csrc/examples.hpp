@@ -1,0 +1,152 @@
+// PDE data of the convection-diffusion examples, evaluable on host and device.
+//
+// The reference passes host callbacks (CoeffFct2D/3D, BoundCondFunct,
+// BoundValueFunct; include/General/Constants.h:36-54) per cell. A device
+// kernel cannot call them, so the examples are enumerated with parameter
+// structs and evaluated inside the kernels. Coefficients follow the
+// reference layout (src/AssembleRoutines/ConvDiff.C:1797-1801,1858-1863):
+// eps, b, c, f and |b|.
+#pragma once
+#include <cmath>
+
+#ifndef __HIP_DEVICE_COMPILE__
+#ifndef __host__
+#define __host__
+#define __device__
+#endif
+#endif
+
+namespace pmcd {
+
+enum ExampleKind : int {
+  // include/Examples/CD_2D/SineLaplace.h (reference 2D example 0): u =
+  // sin(pi x) sin(pi y), eps = nu, b = c = 0, Neumann on component 1 (x = 1)
+  EX_SINE_LAPLACE_2D = 0,
+  // u = sin(pi x) sin(pi y), eps, b = (b1, b2), c = 0, Dirichlet everywhere
+  EX_SINE_2D = 1,
+  // u = sin(pi x) sin(pi y) sin(pi z), eps, b, c = 0, Dirichlet everywhere
+  EX_SINE_3D = 2,
+  // include/Examples/CD_3D/test_p0.h (example -1): u = 1
+  EX_TEST_P0_3D = 3,
+  // include/Examples/CD_3D/test_p1.h (example -2): u = 1 + 2x + 3y + 4z
+  EX_TEST_P1_3D = 4,
+  // include/Examples/CD_3D/test_p2.h (example -3): u = x^2 + y z
+  EX_TEST_P2_3D = 5,
+  // eps, b, c given, f = f0, homogeneous Dirichlet; no known solution
+  EX_CONST_RHS = 6,
+};
+
+struct ExampleParams {
+  int kind;
+  int dim;
+  double eps;
+  double b[3];
+  double c;
+  double f0;
+};
+
+// coefficients at a point: out = {eps, b1, b2, b3, c, f}
+__host__ __device__ inline void example_coeffs(const ExampleParams& ex, double x, double y,
+                                               double z, double* out) {
+  const double p = M_PI;
+  switch(ex.kind) {
+    case EX_SINE_LAPLACE_2D: {
+      // SineLaplace.h:89-108: f = -eps * laplace(u)
+      double lap = -2 * p * p * sin(p * x) * sin(p * y);
+      out[0] = ex.eps; out[1] = 0; out[2] = 0; out[3] = 0; out[4] = 0;
+      out[5] = -out[0] * lap;
+      break;
+    }
+    case EX_SINE_2D: {
+      double u = sin(p * x) * sin(p * y);
+      double ux = p * cos(p * x) * sin(p * y), uy = p * sin(p * x) * cos(p * y);
+      double lap = -2 * p * p * u;
+      out[0] = ex.eps; out[1] = ex.b[0]; out[2] = ex.b[1]; out[3] = 0; out[4] = 0;
+      out[5] = -out[0] * lap + out[1] * ux + out[2] * uy;
+      break;
+    }
+    case EX_SINE_3D: {
+      double sx = sin(p * x), sy = sin(p * y), sz = sin(p * z);
+      double u = sx * sy * sz;
+      double ux = p * cos(p * x) * sy * sz, uy = p * sx * cos(p * y) * sz,
+             uz = p * sx * sy * cos(p * z);
+      out[0] = ex.eps; out[1] = ex.b[0]; out[2] = ex.b[1]; out[3] = ex.b[2]; out[4] = 0;
+      out[5] = 3 * p * p * ex.eps * u + ex.b[0] * ux + ex.b[1] * uy + ex.b[2] * uz;
+      break;
+    }
+    case EX_TEST_P0_3D:
+      out[0] = 1; out[1] = 1; out[2] = 1; out[3] = 1; out[4] = 1; out[5] = 1;
+      break;
+    case EX_TEST_P1_3D:
+      out[0] = 1; out[1] = 1; out[2] = 2; out[3] = 3; out[4] = 0; out[5] = 1 * 2 + 2 * 3 + 3 * 4;
+      break;
+    case EX_TEST_P2_3D: {
+      double eps = 1;
+      out[0] = eps; out[1] = 4; out[2] = 3; out[3] = 2; out[4] = 1;
+      out[5] = -eps * 2 + 4 * 2 * x + 3 * z + 2 * y + 1 * (x * x + y * z);
+      break;
+    }
+    default:  // EX_CONST_RHS
+      out[0] = ex.eps; out[1] = ex.b[0]; out[2] = ex.b[1]; out[3] = ex.b[2];
+      out[4] = ex.c; out[5] = ex.f0;
+      break;
+  }
+}
+
+// exact solution: out = {u, u_x, u_y, u_z, laplace u}; returns false if unknown
+__host__ __device__ inline bool example_exact(const ExampleParams& ex, double x, double y,
+                                              double z, double* out) {
+  const double p = M_PI;
+  switch(ex.kind) {
+    case EX_SINE_LAPLACE_2D:
+    case EX_SINE_2D:
+      out[0] = sin(p * x) * sin(p * y);
+      out[1] = p * cos(p * x) * sin(p * y);
+      out[2] = p * sin(p * x) * cos(p * y);
+      out[3] = 0;
+      out[4] = -2 * p * p * sin(p * x) * sin(p * y);
+      return true;
+    case EX_SINE_3D: {
+      double sx = sin(p * x), sy = sin(p * y), sz = sin(p * z);
+      out[0] = sx * sy * sz;
+      out[1] = p * cos(p * x) * sy * sz;
+      out[2] = p * sx * cos(p * y) * sz;
+      out[3] = p * sx * sy * cos(p * z);
+      out[4] = -3 * p * p * sx * sy * sz;
+      return true;
+    }
+    case EX_TEST_P0_3D:
+      out[0] = 1; out[1] = 0; out[2] = 0; out[3] = 0; out[4] = 0;
+      return true;
+    case EX_TEST_P1_3D:
+      out[0] = 1 + 2 * x + 3 * y + 4 * z; out[1] = 2; out[2] = 3; out[3] = 4; out[4] = 0;
+      return true;
+    case EX_TEST_P2_3D:
+      out[0] = x * x + y * z; out[1] = 2 * x; out[2] = z; out[3] = y; out[4] = 2;
+      return true;
+    default:
+      out[0] = out[1] = out[2] = out[3] = out[4] = 0;
+      return false;
+  }
+}
+
+// boundary condition kind (0 Dirichlet, 1 Neumann) of component `comp`
+__host__ __device__ inline int example_bc(const ExampleParams& ex, int comp) {
+  if(ex.kind == EX_SINE_LAPLACE_2D && comp == 1) return 1;
+  return 0;
+}
+
+// boundary value at a boundary point of component `comp` (SineLaplace.h:67-84:
+// Dirichlet value u, Neumann value eps * n . grad u)
+__host__ __device__ inline double example_bvalue(const ExampleParams& ex, int comp, double x,
+                                                 double y, double z) {
+  double e[5];
+  if(!example_exact(ex, x, y, z, e)) return 0.0;
+  if(ex.kind == EX_SINE_LAPLACE_2D && comp == 1) {
+    // outer normal of x = 1 is (1, 0)
+    return ex.eps * (1.0 * e[1] + 0.0 * e[2]);
+  }
+  return e[0];
+}
+
+}  // namespace pmcd

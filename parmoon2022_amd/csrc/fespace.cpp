@@ -1,0 +1,99 @@
+#include "fespace.hpp"
+#include <algorithm>
+#include <unordered_map>
+#include <climits>
+
+namespace pmcd {
+
+namespace {
+struct Key3Hash {
+  size_t operator()(const std::array<int, 4>& k) const {
+    uint64_t h = 1469598103934665603ull;
+    for(int v : k) { h ^= (uint64_t)(uint32_t)v; h *= 1099511628211ull; }
+    return (size_t)h;
+  }
+};
+}  // namespace
+
+FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc) {
+  FESpace sp;
+  sp.element = element_for(mesh.shape, order);
+  const ElementInfo& info = element_info(sp.element);
+  sp.n_local = info.n_dof;
+  const int nl = sp.n_local;
+  const int nc = mesh.n_cells;
+
+  // class of each (cell, local dof), classes created in first-appearance order
+  std::vector<int> cls((size_t)nc * nl);
+  std::vector<int> vertex_class(mesh.n_vertices(), -1);
+  std::unordered_map<std::array<int, 4>, int, Key3Hash> entity_class;
+  entity_class.reserve((size_t)nc * 4);
+  int n_classes = 0;
+  const int full = shape_n_vertices(mesh.shape);
+  for(int c = 0; c < nc; ++c) {
+    const int* cv = mesh.cell(c);
+    for(int i = 0; i < nl; ++i) {
+      const auto& ev = info.entity_vertices[i];
+      int k;
+      if(ev.size() == 1) {
+        int& vc = vertex_class[cv[ev[0]]];
+        if(vc < 0) vc = n_classes++;
+        k = vc;
+      } else if((int)ev.size() == full) {
+        k = n_classes++;  // cell-interior DOF
+      } else {
+        std::array<int, 4> key = {-1, -1, -1, -1};
+        int g[4];
+        for(size_t a = 0; a < ev.size(); ++a) g[a] = cv[ev[a]];
+        std::sort(g, g + ev.size());
+        for(size_t a = 0; a < ev.size(); ++a) key[a] = g[a];
+        auto it = entity_class.find(key);
+        if(it == entity_class.end()) { k = n_classes++; entity_class.emplace(key, k); }
+        else k = it->second;
+      }
+      cls[(size_t)c * nl + i] = k;
+    }
+  }
+
+  // boundary kind of each class: the smallest BoundCond over boundary joints
+  const int INNER = INT_MAX;
+  std::vector<int> kind(n_classes, INNER);
+  const auto& faces = shape_faces(mesh.shape);
+  for(int c = 0; c < nc; ++c)
+    for(int j = 0; j < mesh.nj; ++j) {
+      int comp = mesh.joint_comp[(size_t)c * mesh.nj + j];
+      if(comp < 0) continue;
+      double cen[3] = {0, 0, 0};
+      for(int lv : faces[j])
+        for(int k = 0; k < 3; ++k) cen[k] += mesh.coords[3 * mesh.cell(c)[lv] + k];
+      for(int k = 0; k < 3; ++k) cen[k] /= faces[j].size();
+      int b = (int)bc(comp, cen);
+      for(int i : info.joint_dofs[j]) {
+        int k = cls[(size_t)c * nl + i];
+        kind[k] = std::min(kind[k], b);
+      }
+    }
+
+  // numbering: classes are already in first-appearance order
+  constexpr int NK = 8;
+  std::vector<int> count_kind(NK + 1, 0);  // [0..6] boundary kinds, [7] inner
+  auto slot = [&](int kd) { return kd == INNER ? NK - 1 : kd; };
+  for(int k = 0; k < n_classes; ++k) count_kind[slot(kind[k])]++;
+  sp.n_inner = count_kind[NK - 1];
+  sp.n_dirichlet = count_kind[DIRICHLET];
+  // offsets: inner, NEUMANN, ROBIN, ..., then DIRICHLET
+  std::vector<int> offset(NK, 0);
+  int off = 0;
+  offset[NK - 1] = off; off += count_kind[NK - 1];
+  for(int kd = 1; kd < NK - 1; ++kd) { offset[kd] = off; off += count_kind[kd]; }
+  sp.n_active = off;
+  offset[DIRICHLET] = off; off += count_kind[DIRICHLET];
+  sp.n_dof = off;
+  std::vector<int> number(n_classes);
+  for(int k = 0; k < n_classes; ++k) number[k] = offset[slot(kind[k])]++;
+  sp.dof.resize((size_t)nc * nl);
+  for(size_t p = 0; p < sp.dof.size(); ++p) sp.dof[p] = number[cls[p]];
+  return sp;
+}
+
+}  // namespace pmcd

@@ -1,0 +1,42 @@
+// Continuous Lagrange FE space on a mesh: local-to-global DOF map with the
+// reference's global numbering.
+//
+// Reference numbering (src/FE/FESpace3D.C:186-804 / FESpace2D.C ConstructSpace,
+// src/FE/FEMapper.C:456-552 map_single_dof, src/FE/FESpace.C:1863-2099
+// compute_various_numbers): local DOFs of face-neighbouring cells are merged
+// into classes whose representative is the class member with the smallest
+// (cell, local dof) position; classes are then numbered in order of that first
+// appearance, inner DOFs first, then the non-Dirichlet boundary kinds in the
+// order of the BoundCond enumeration, Dirichlet DOFs last
+// (include/FE/FESpace.h:71-92). A class lying on boundary joints of several
+// kinds takes the kind that comes first in BoundCond.
+//
+// This build identifies the classes directly by the mesh entity carrying each
+// nodal point (vertex, edge, face, cell interior) instead of by union-find
+// over faces; for conforming meshes the two give the same classes and the
+// same first appearances, hence the same numbers.
+#pragma once
+#include "common.hpp"
+#include "mesh.hpp"
+#include "elements.hpp"
+#include <functional>
+
+namespace pmcd {
+
+// boundary condition of joint with component `comp` and centroid x
+using BoundCondFn = std::function<BoundCond(int comp, const double* x)>;
+
+struct FESpace {
+  Element element;
+  int n_local = 0;
+  int n_dof = 0;
+  int n_active = 0;      // inner + non-Dirichlet boundary DOFs
+  int n_dirichlet = 0;
+  int n_inner = 0;
+  std::vector<int> dof;  // n_cells * n_local, global DOF numbers
+  const int* cell_dof(int c) const { return &dof[(size_t)c * n_local]; }
+};
+
+FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc);
+
+}  // namespace pmcd

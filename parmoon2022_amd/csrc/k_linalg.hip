@@ -1,0 +1,520 @@
+// Bandwidth-bound kernels of the solve: CSR SpMV with the reference's
+// non-active-row rule, the damped Jacobi sweep, transfer products, dense
+// coarse-grid products, deterministic reductions and vector updates.
+//
+// Reference: BlockFEMatrix::apply / apply_scaled_add
+// (src/Matrix/BlockFEMatrix.C:969-1095) -> FEMatrix::multiplyActive
+// (src/Matrix/FEMatrix.C:209-223) + BlockVector::addScaledNonActive;
+// JacobiSmoother::smooth (src/Multigrid/JacobiSmoother.C:22-30) with
+// Iteration_jacobi::apply (src/Solver/Iteration_jacobi.C:168-182);
+// Ddot/Dnorm (src/FE/Blas1.C:21-98).
+//
+// Reductions use a fixed two-pass tree (fixed grid, fixed block tree), so
+// repeated runs give identical bits; the order differs from the reference's
+// serial sums, which the parity tests bound by a rounding tolerance.
+#include "kernels.hpp"
+#include "device.hpp"
+
+#pragma clang fp contract(off)
+
+namespace pmcd {
+
+namespace {
+
+constexpr int kRedBlocks = 1024;
+
+// W lanes per row; lanes stride the row, then a butterfly over the W lanes.
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_spmv(int n, int n_active, const int* __restrict__ rp,
+                                              const int* __restrict__ col,
+                                              const T* __restrict__ val, const T* __restrict__ x,
+                                              const T* __restrict__ z, T* __restrict__ y,
+                                              T alpha, T beta) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int row = gt / W, l = gt % W;
+  if(row >= n) return;
+  if(row < n_active) {
+    T s = 0;
+    const int e = rp[row + 1];
+    for(int k = rp[row] + l; k < e; k += W) s += val[k] * x[col[k]];
+#pragma unroll
+    for(int off = W / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, W);
+    if(l == 0) y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * s;
+  } else if(l == 0) {
+    y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * x[row];
+  }
+}
+
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_jacobi(int n, int n_active, const int* __restrict__ rp,
+                                                const int* __restrict__ col,
+                                                const T* __restrict__ val,
+                                                const T* __restrict__ diag,
+                                                const T* __restrict__ b, const T* __restrict__ x,
+                                                T* __restrict__ x_out, T omega) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int row = gt / W, l = gt % W;
+  if(row >= n) return;
+  T s;
+  if(row < n_active) {
+    s = 0;
+    const int e = rp[row + 1];
+    for(int k = rp[row] + l; k < e; k += W) s += val[k] * x[col[k]];
+#pragma unroll
+    for(int off = W / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, W);
+  } else {
+    s = x[row];
+  }
+  if(l == 0) {
+    T d = b[row] + T(-1) * s;
+    d = d / diag[row];
+    x_out[row] = x[row] + omega * d;
+  }
+}
+
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_csr(int n, const int* __restrict__ rp,
+                                             const int* __restrict__ col,
+                                             const T* __restrict__ val, const T* __restrict__ x,
+                                             T* __restrict__ y) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int row = gt / W, l = gt % W;
+  if(row >= n) return;
+  T s = 0;
+  const int e = rp[row + 1];
+  for(int k = rp[row] + l; k < e; k += W) s += val[k] * x[col[k]];
+#pragma unroll
+  for(int off = W / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, W);
+  if(l == 0) y[row] = s;
+}
+
+__device__ inline double block_sum(double v) {
+  __shared__ double s_w[4];
+  for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+  if(l == 0) s_w[w] = v;
+  __syncthreads();
+  double r = 0;
+  if(threadIdx.x == 0) r = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(256) void k_dot_partial(int n, const double* __restrict__ x,
+                                                     const double* __restrict__ y,
+                                                     double* __restrict__ partial) {
+  double s = 0;
+  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += x[i] * y[i];
+  s = block_sum(s);
+  if(threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_final_sum(int m, const double* __restrict__ partial,
+                                                   double* __restrict__ out) {
+  double s = 0;
+  for(int i = threadIdx.x; i < m; i += 256) s += partial[i];
+  s = block_sum(s);
+  if(threadIdx.x == 0) out[0] = s;
+}
+
+// per-cell error records: sum of fields 0..3, max of field 4
+__global__ __launch_bounds__(256) void k_sum_max_partial(int n, int stride,
+                                                         const double* __restrict__ v,
+                                                         double* __restrict__ partial) {
+  double s[4] = {0, 0, 0, 0};
+  double mx = 0;
+  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    for(int k = 0; k < 4; ++k) s[k] += v[(size_t)i * stride + k];
+    mx = fmax(mx, v[(size_t)i * stride + 4]);
+  }
+  for(int k = 0; k < 4; ++k) {
+    double t = block_sum(s[k]);
+    if(threadIdx.x == 0) partial[blockIdx.x * 5 + k] = t;
+  }
+  __shared__ double s_m[256];
+  s_m[threadIdx.x] = mx;
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    double m = 0;
+    for(int i = 0; i < 256; ++i) m = fmax(m, s_m[i]);
+    partial[blockIdx.x * 5 + 4] = m;
+  }
+}
+
+__global__ void k_sum_max_final(int m, const double* __restrict__ partial, double* __restrict__ out) {
+  if(threadIdx.x == 0 && blockIdx.x == 0) {
+    double s[4] = {0, 0, 0, 0}, mx = 0;
+    for(int i = 0; i < m; ++i) {
+      for(int k = 0; k < 4; ++k) s[k] += partial[i * 5 + k];
+      mx = fmax(mx, partial[i * 5 + 4]);
+    }
+    for(int k = 0; k < 4; ++k) out[k] = s[k];
+    out[4] = mx;
+  }
+}
+
+__global__ void k_axpy_dev(int n, const double* __restrict__ a, double scale,
+                           const double* __restrict__ x, double* __restrict__ y) {
+  const double al = scale * a[0];
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] += al * x[i];
+}
+
+__global__ void k_axpy(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+
+__global__ void k_scale_copy_dev(int n, const double* __restrict__ num, int invert,
+                                 const double* __restrict__ x, double* __restrict__ y) {
+  const double f = invert ? 1.0 / num[0] : num[0];
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] = x[i] * f;
+}
+
+template <class T>
+__global__ void k_fill(int n, T v, T* __restrict__ y) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = v;
+}
+
+__global__ void k_copy(int n, const double* __restrict__ x, double* __restrict__ y) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = x[i];
+}
+
+// r += (-h) v ; partial dot of the updated r with v_next
+__global__ __launch_bounds__(256) void k_mgs_step(int n, const double* __restrict__ h,
+                                                  double* __restrict__ r,
+                                                  const double* __restrict__ v,
+                                                  const double* __restrict__ vn,
+                                                  double* __restrict__ partial) {
+  const double mh = -h[0];
+  double s = 0;
+  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const double ri = r[i] + mh * v[i];
+    r[i] = ri;
+    if(vn) s += ri * vn[i];
+  }
+  if(vn) {
+    s = block_sum(s);
+    if(threadIdx.x == 0) partial[blockIdx.x] = s;
+  }
+}
+
+__global__ void k_d2f(int n, const double* __restrict__ x, float* __restrict__ y) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] = (float)x[i];
+}
+__global__ void k_f2d(int n, const float* __restrict__ x, double* __restrict__ y) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] = (double)x[i];
+}
+template <class T>
+__global__ void k_axpy_active(int n_active, T a, const T* __restrict__ x, T* __restrict__ y) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_active; i += gridDim.x * blockDim.x)
+    y[i] += a * x[i];
+}
+template <class T>
+__global__ void k_zero_tail(int n, int n_active, T* __restrict__ y) {
+  for(int i = n_active + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    y[i] = T(0);
+}
+
+// one wave per output row
+template <class T>
+__global__ __launch_bounds__(256) void k_gemv(int n, const T* __restrict__ M,
+                                              const T* __restrict__ x, T* __restrict__ y) {
+  const int row = blockIdx.x * 4 + threadIdx.x / 64, l = threadIdx.x % 64;
+  if(row >= n) return;
+  T s = 0;
+  const T* m = M + (size_t)row * n;
+  for(int k = l; k < n; k += 64) s += m[k] * x[k];
+  for(int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if(l == 0) y[row] = s;
+}
+
+inline int grid_for(int n, int per_block) {
+  long g = ((long)n + per_block - 1) / per_block;
+  if(g > 65535L * 16) g = 65535L * 16;
+  return (int)std::max<long>(g, 1);
+}
+
+template <class T>
+void dispatch_spmv(int width, int n, int n_active, const int* rp, const int* col, const T* val,
+                   const T* x, const T* z, T* y, T alpha, T beta, hipStream_t s) {
+  const long threads = (long)n * width;
+  const int blocks = (int)((threads + 255) / 256);
+  switch(width) {
+    case 4: k_spmv<4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    case 8: k_spmv<8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    case 16: k_spmv<16, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    default: k_spmv<32, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+  }
+}
+
+__global__ void k_scale_copy(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = x[i] * a;
+}
+__global__ void k_scale_inplace(int n, double a, double* __restrict__ y) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = y[i] * a;
+}
+__global__ void k_div(int n, const double* __restrict__ x, const double* __restrict__ d,
+                      double* __restrict__ y) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = x[i] / d[i];
+}
+
+__global__ void k_gather_idx(int n, const int* __restrict__ idx, const double* __restrict__ src,
+                             double* __restrict__ dst) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    dst[i] = src[idx[i]];
+}
+
+__global__ void k_csr_to_dense(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                               const double* __restrict__ val, double* __restrict__ M) {
+  const int r = blockIdx.x;
+  for(int k = rp[r] + threadIdx.x; k < rp[r + 1]; k += blockDim.x) M[(size_t)r * n + col[k]] = val[k];
+}
+
+// pivot search in column k, rows k..n-1 (largest magnitude, lowest row on ties)
+__global__ __launch_bounds__(256) void k_pivot(int n, int k, const double* __restrict__ A,
+                                               int* __restrict__ piv) {
+  __shared__ double s_v[256];
+  __shared__ int s_i[256];
+  double best = -1.0;
+  int bi = k;
+  for(int i = k + threadIdx.x; i < n; i += 256) {
+    double v = fabs(A[(size_t)i * n + k]);
+    if(v > best) { best = v; bi = i; }
+  }
+  s_v[threadIdx.x] = best;
+  s_i[threadIdx.x] = bi;
+  __syncthreads();
+  for(int off = 128; off > 0; off >>= 1) {
+    if(threadIdx.x < off) {
+      double v2 = s_v[threadIdx.x + off];
+      int i2 = s_i[threadIdx.x + off];
+      if(v2 > s_v[threadIdx.x] || (v2 == s_v[threadIdx.x] && i2 < s_i[threadIdx.x])) {
+        s_v[threadIdx.x] = v2;
+        s_i[threadIdx.x] = i2;
+      }
+    }
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) piv[0] = s_i[0];
+}
+
+// swap rows k and piv, then save column k and the pivot value
+__global__ void k_swap_save(int n, int k, double* __restrict__ A, double* __restrict__ I,
+                            const int* __restrict__ piv, double* __restrict__ colk) {
+  const int p = piv[0];
+  for(int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    if(p != k) {
+      double t = A[(size_t)k * n + j]; A[(size_t)k * n + j] = A[(size_t)p * n + j]; A[(size_t)p * n + j] = t;
+      t = I[(size_t)k * n + j]; I[(size_t)k * n + j] = I[(size_t)p * n + j]; I[(size_t)p * n + j] = t;
+    }
+  }
+}
+
+__global__ void k_save_col(int n, int k, const double* __restrict__ A, double* __restrict__ colk) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    colk[i] = A[(size_t)i * n + k];
+}
+
+__global__ void k_scale_row(int n, int k, double* __restrict__ A, double* __restrict__ I,
+                            const double* __restrict__ colk) {
+  const double pv = colk[k];
+  for(int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    A[(size_t)k * n + j] = A[(size_t)k * n + j] / pv;
+    I[(size_t)k * n + j] = I[(size_t)k * n + j] / pv;
+  }
+}
+
+__global__ void k_eliminate(int n, int k, double* __restrict__ A, double* __restrict__ I,
+                            const double* __restrict__ colk) {
+  const int i = blockIdx.y;
+  if(i == k) return;
+  const double f = colk[i];
+  if(f == 0.0) return;
+  for(int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
+    I[(size_t)i * n + j] -= f * I[(size_t)k * n + j];
+  }
+}
+
+__global__ void k_identity(int n, double* __restrict__ I) {
+  const size_t nn = (size_t)n * n;
+  for(size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nn; t += (size_t)gridDim.x * blockDim.x)
+    I[t] = (t / n == t % n) ? 1.0 : 0.0;
+}
+
+}  // namespace
+
+void launch_scale_copy(int n, double a, const double* x, double* y, hipStream_t s) {
+  if(n) k_scale_copy<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, x, y);
+}
+void launch_scale_inplace(int n, double a, double* y, hipStream_t s) {
+  if(n) k_scale_inplace<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, y);
+}
+void launch_div(int n, const double* x, const double* d, double* y, hipStream_t s) {
+  if(n) k_div<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, x, d, y);
+}
+
+void launch_gather_idx(int n, const int* idx, const double* src, double* dst, hipStream_t s) {
+  if(n) k_gather_idx<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, idx, src, dst);
+}
+
+void launch_csr_to_dense(int n, const int* rp, const int* col, const double* val, double* M,
+                         hipStream_t s) {
+  if(!n) return;
+  const long nn = (long)n * n;
+  k_fill<double><<<(int)std::min<long>((nn + 255) / 256, 65535), 256, 0, s>>>((int)nn, 0.0, M);
+  k_csr_to_dense<<<n, 64, 0, s>>>(n, rp, col, val, M);
+}
+
+void dense_inverse(int n, double* A, double* I, double* work, int* piv, hipStream_t s) {
+  if(!n) return;
+  const int g = std::max(1, std::min(1024, (n * n + 255) / 256));
+  k_identity<<<g, 256, 0, s>>>(n, I);
+  const int gx = (n + 255) / 256;
+  for(int k = 0; k < n; ++k) {
+    k_pivot<<<1, 256, 0, s>>>(n, k, A, piv);
+    k_swap_save<<<gx, 256, 0, s>>>(n, k, A, I, piv, work);
+    k_save_col<<<gx, 256, 0, s>>>(n, k, A, work);
+    k_scale_row<<<gx, 256, 0, s>>>(n, k, A, I, work);
+    k_eliminate<<<dim3(gx, n), 256, 0, s>>>(n, k, A, I, work);
+  }
+  PMCD_HIP(hipGetLastError());
+}
+
+// lanes per row from the mean row length
+static int spmv_width(int n, const int* /*rp*/, long nnz_hint) {
+  (void)n; (void)nnz_hint;
+  return 16;
+}
+
+static int g_spmv_width = 16;
+
+void launch_spmv(int n, int n_active, const int* rp, const int* col, const double* val,
+                 const double* x, const double* z, double* y, double alpha, double beta,
+                 hipStream_t s) {
+  if(n == 0) return;
+  (void)spmv_width;
+  dispatch_spmv<double>(g_spmv_width, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_spmv_f32(int n, int n_active, const int* rp, const int* col, const float* val,
+                     const float* x, const float* z, float* y, float alpha, float beta,
+                     hipStream_t s) {
+  if(n == 0) return;
+  dispatch_spmv<float>(g_spmv_width, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_jacobi(int n, int n_active, const int* rp, const int* col, const double* val,
+                   const double*, const double* diag, const double* b, const double* x,
+                   double* x_out, double omega, hipStream_t s) {
+  if(n == 0) return;
+  const int blocks = (int)(((long)n * 16 + 255) / 256);
+  k_jacobi<16, double><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_jacobi_f32(int n, int n_active, const int* rp, const int* col, const float* val,
+                       const float* diag, const float* b, const float* x, float* x_out,
+                       float omega, hipStream_t s) {
+  if(n == 0) return;
+  const int blocks = (int)(((long)n * 16 + 255) / 256);
+  k_jacobi<16, float><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_csr(int n, const int* rp, const int* col, const double* val, const double* x,
+                double* y, hipStream_t s) {
+  if(n == 0) return;
+  const int blocks = (int)(((long)n * 8 + 255) / 256);
+  k_csr<8, double><<<blocks, 256, 0, s>>>(n, rp, col, val, x, y);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_csr_f32(int n, const int* rp, const int* col, const float* val, const float* x,
+                    float* y, hipStream_t s) {
+  if(n == 0) return;
+  const int blocks = (int)(((long)n * 8 + 255) / 256);
+  k_csr<8, float><<<blocks, 256, 0, s>>>(n, rp, col, val, x, y);
+  PMCD_HIP(hipGetLastError());
+}
+
+int reduce_blocks(int n) {
+  int b = (n + 255) / 256;
+  return std::max(1, std::min(b, kRedBlocks));
+}
+
+void launch_dot(int n, const double* x, const double* y, double* partial, double* out,
+                hipStream_t s) {
+  const int b = reduce_blocks(n);
+  k_dot_partial<<<b, 256, 0, s>>>(n, x, y, partial);
+  k_final_sum<<<1, 256, 0, s>>>(b, partial, out);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_sum_max(int n, int stride, const double* v, double* partial, double* out,
+                    hipStream_t s) {
+  const int b = reduce_blocks(n);
+  k_sum_max_partial<<<b, 256, 0, s>>>(n, stride, v, partial);
+  k_sum_max_final<<<1, 64, 0, s>>>(b, partial, out);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_axpy_dev(int n, const double* a, double scale, const double* x, double* y,
+                     hipStream_t s) {
+  k_axpy_dev<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, scale, x, y);
+}
+void launch_axpy(int n, double a, const double* x, double* y, hipStream_t s) {
+  k_axpy<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, x, y);
+}
+void launch_scale_copy_dev(int n, const double* num, int invert, const double* x, double* y,
+                           hipStream_t s) {
+  k_scale_copy_dev<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, num, invert, x, y);
+}
+void launch_fill(int n, double v, double* y, hipStream_t s) {
+  if(n) k_fill<double><<<grid_for(n, 256 * 4), 256, 0, s>>>(n, v, y);
+}
+void launch_fill_f32(int n, float v, float* y, hipStream_t s) {
+  if(n) k_fill<float><<<grid_for(n, 256 * 4), 256, 0, s>>>(n, v, y);
+}
+void launch_copy(int n, const double* x, double* y, hipStream_t s) {
+  if(n) k_copy<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, x, y);
+}
+void launch_mgs_step(int n, const double* h, double* r, const double* v, const double* vn,
+                     double* partial, double* out_next, hipStream_t s) {
+  const int b = reduce_blocks(n);
+  k_mgs_step<<<b, 256, 0, s>>>(n, h, r, v, vn, partial);
+  if(vn) k_final_sum<<<1, 256, 0, s>>>(b, partial, out_next);
+}
+void launch_d2f(int n, const double* x, float* y, hipStream_t s) {
+  if(n) k_d2f<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, x, y);
+}
+void launch_f2d(int n, const float* x, double* y, hipStream_t s) {
+  if(n) k_f2d<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, x, y);
+}
+void launch_axpy_f32(int n, float a, const float* x, float* y, int n_active, hipStream_t s) {
+  (void)n;
+  if(n_active) k_axpy_active<float><<<grid_for(n_active, 256 * 4), 256, 0, s>>>(n_active, a, x, y);
+}
+void launch_axpy_active(int n_active, double a, const double* x, double* y, hipStream_t s) {
+  if(n_active) k_axpy_active<double><<<grid_for(n_active, 256 * 4), 256, 0, s>>>(n_active, a, x, y);
+}
+void launch_gemv(int n, const double* M, const double* x, double* y, hipStream_t s) {
+  if(n) k_gemv<double><<<(n + 3) / 4, 256, 0, s>>>(n, M, x, y);
+}
+void launch_gemv_f32(int n, const float* M, const float* x, float* y, hipStream_t s) {
+  if(n) k_gemv<float><<<(n + 3) / 4, 256, 0, s>>>(n, M, x, y);
+}
+void launch_zero_tail(int n, int n_active, double* y, hipStream_t s) {
+  if(n > n_active) k_zero_tail<double><<<grid_for(n - n_active, 256), 256, 0, s>>>(n, n_active, y);
+}
+void launch_zero_tail_f32(int n, int n_active, float* y, hipStream_t s) {
+  if(n > n_active) k_zero_tail<float><<<grid_for(n - n_active, 256), 256, 0, s>>>(n, n_active, y);
+}
+
+}  // namespace pmcd

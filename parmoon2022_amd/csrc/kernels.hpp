@@ -1,0 +1,110 @@
+// Launch wrappers of the HIP kernels (implemented in k_assemble.hip and
+// k_linalg.hip). All take device pointers and a stream.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "examples.hpp"
+
+namespace pmcd {
+
+// ---- assembly ------------------------------------------------------------
+struct AsmArgs {
+  int shape;        // Shape
+  int dim;
+  int nv;           // vertices per cell
+  int nl;           // local DOFs
+  int nq;           // quadrature points
+  int ncomp;        // derivative components in the reference table (6 or 10)
+  int supg;         // 1: add the SUPG/SD local routine
+  int n_cells;
+  ExampleParams ex;
+  const double* ref;        // [nq][ncomp][nl] reference basis values/derivatives
+  const double* qw;         // [nq] reference weights
+  const double* qp;         // [nq][3] reference points
+  const double* cell_xyz;   // [n_cells][nv][3]
+  double* local_A;          // [n_cells][nl][nl]  (row = test, column = ansatz)
+  double* local_rhs;        // [n_cells][nl]
+};
+void launch_assemble_local(const AsmArgs& a, hipStream_t s);
+
+// Sum local matrices into CSR rows in increasing cell order; Dirichlet rows
+// (r >= n_active) get 1 on the diagonal and 0 elsewhere.
+void launch_gather_matrix(int n_rows, int n_active, int nl, const int* row_ptr,
+                          const int* col, const int* contrib_ptr, const int* contrib,
+                          const uint8_t* slot, const double* local_A, double* val,
+                          hipStream_t s);
+// rhs[r] = sum of local rhs (plus Neumann terms) in cell order for active rows,
+// Dirichlet value for r >= n_active.
+void launch_gather_rhs(int n_rows, int n_active, const int* contrib_ptr, const int* contrib,
+                       const double* local_rhs, const int* extra_ptr, const double* extra,
+                       const double* dirichlet_values, double* rhs, hipStream_t s);
+
+// ---- errors ---------------------------------------------------------------
+// per-cell {L2^2, H1semi^2, SD^2, DG, Linf}
+void launch_cell_errors(const AsmArgs& a, const int* dof, const double* u, double* cell_err,
+                        hipStream_t s);
+
+// ---- sparse / dense linear algebra ----------------------------------------
+// y[r] = beta*z[r] + alpha*(A x)[r] for r < n_active, beta*z[r] + alpha*x[r] otherwise
+void launch_spmv(int n, int n_active, const int* rp, const int* col, const double* val,
+                 const double* x, const double* z, double* y, double alpha, double beta,
+                 hipStream_t s);
+void launch_spmv_f32(int n, int n_active, const int* rp, const int* col, const float* val,
+                     const float* x, const float* z, float* y, float alpha, float beta,
+                     hipStream_t s);
+// damped Jacobi sweep x_out = x + omega * (b - A x) / diag (non-active rows use A = I)
+void launch_jacobi(int n, int n_active, const int* rp, const int* col, const double* val,
+                   const double* inv_diag_unused, const double* diag, const double* b,
+                   const double* x, double* x_out, double omega, hipStream_t s);
+void launch_jacobi_f32(int n, int n_active, const int* rp, const int* col, const float* val,
+                       const float* diag, const float* b, const float* x, float* x_out,
+                       float omega, hipStream_t s);
+// plain CSR product y = A x over all rows (transfer operators)
+void launch_csr(int n, const int* rp, const int* col, const double* val, const double* x,
+                double* y, hipStream_t s);
+void launch_csr_f32(int n, const int* rp, const int* col, const float* val, const float* x,
+                    float* y, hipStream_t s);
+
+// deterministic reductions: result written to out[0] (device)
+int reduce_blocks(int n);
+void launch_dot(int n, const double* x, const double* y, double* partial, double* out,
+                hipStream_t s);
+void launch_sum_max(int n, int stride, const double* v, double* partial, double* out,
+                    hipStream_t s);
+
+// vector kernels
+void launch_axpy_dev(int n, const double* alpha_dev, double alpha_scale, const double* x,
+                     double* y, hipStream_t s);  // y += (alpha_scale * *alpha_dev) x
+void launch_axpy(int n, double alpha, const double* x, double* y, hipStream_t s);
+void launch_scale_copy_dev(int n, const double* num_dev, int invert, const double* x, double* y,
+                           hipStream_t s);  // y = x * (invert ? 1/(*num) : *num)
+void launch_fill(int n, double v, double* y, hipStream_t s);
+void launch_copy(int n, const double* x, double* y, hipStream_t s);
+// fused MGS step: r -= h_k v_k; h_{k+1} = <r, v_{k+1}> partials (v_next may be null)
+void launch_mgs_step(int n, const double* h_dev, double* r, const double* v,
+                     const double* v_next, double* partial, double* out_next, hipStream_t s);
+void launch_d2f(int n, const double* x, float* y, hipStream_t s);
+void launch_f2d(int n, const float* x, double* y, hipStream_t s);
+void launch_fill_f32(int n, float v, float* y, hipStream_t s);
+void launch_axpy_f32(int n, float alpha, const float* x, float* y, int n_active, hipStream_t s);
+void launch_axpy_active(int n_active, double alpha, const double* x, double* y, hipStream_t s);
+// dense y = M x (M row-major n x n)
+void launch_gemv(int n, const double* M, const double* x, double* y, hipStream_t s);
+void launch_gemv_f32(int n, const float* M, const float* x, float* y, hipStream_t s);
+// zero entries [n_active, n)
+void launch_zero_tail(int n, int n_active, double* y, hipStream_t s);
+void launch_zero_tail_f32(int n, int n_active, float* y, hipStream_t s);
+// y = a x ; y *= a ; y = x / d
+void launch_scale_copy(int n, double a, const double* x, double* y, hipStream_t s);
+void launch_scale_inplace(int n, double a, double* y, hipStream_t s);
+void launch_div(int n, const double* x, const double* d, double* y, hipStream_t s);
+// dst[i] = src[idx[i]]
+void launch_gather_idx(int n, const int* idx, const double* src, double* dst, hipStream_t s);
+// scatter CSR rows into a dense row-major n x n matrix (zero-filled first)
+void launch_csr_to_dense(int n, const int* rp, const int* col, const double* val, double* M,
+                         hipStream_t s);
+// in-place Gauss-Jordan inverse with partial pivoting: A is overwritten,
+// Ainv receives the inverse; work holds n + 2 doubles
+void dense_inverse(int n, double* A, double* Ainv, double* work, int* ipiv_dev, hipStream_t s);
+
+}  // namespace pmcd

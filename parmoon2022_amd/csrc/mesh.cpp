@@ -1,0 +1,294 @@
+#include "mesh.hpp"
+#include <algorithm>
+#include <cmath>
+#include <unordered_map>
+
+namespace pmcd {
+
+namespace {
+
+// Refinement table of one shape: the parent-local vertices spanning each
+// refined vertex (its barycentre is the new vertex) and the refined-vertex
+// lists of the children, in child order.
+struct RefTable {
+  std::vector<std::vector<int>> span;
+  std::vector<std::vector<int>> child;
+};
+
+const RefTable& ref_table(Shape s) {
+  static const RefTable tri = {
+      {{0}, {1}, {2}, {0, 1}, {1, 2}, {2, 0}},
+      {{0, 3, 5}, {1, 4, 3}, {2, 5, 4}, {3, 4, 5}}};
+  static const RefTable quad = {
+      {{0}, {1}, {2}, {3}, {0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 1, 2, 3}},
+      {{0, 4, 8, 7}, {1, 5, 8, 4}, {2, 6, 8, 5}, {3, 7, 8, 6}}};
+  static const RefTable tet = {
+      {{0}, {1}, {2}, {3}, {0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}},
+      {{0, 4, 6, 7}, {4, 1, 5, 8}, {6, 5, 2, 9}, {7, 8, 9, 3},
+       {4, 6, 7, 8}, {5, 6, 4, 8}, {6, 7, 8, 9}, {8, 5, 6, 9}}};
+  static const RefTable hex = {
+      {{0}, {1}, {2}, {3}, {4}, {5}, {6}, {7},
+       {0, 1}, {1, 2}, {2, 3}, {3, 0}, {0, 1, 2, 3},
+       {1, 5}, {5, 4}, {4, 0}, {0, 1, 5, 4},
+       {2, 6}, {6, 5}, {1, 2, 6, 5},
+       {3, 7}, {7, 6}, {2, 3, 7, 6},
+       {4, 7}, {3, 0, 4, 7}, {4, 5, 6, 7},
+       {0, 1, 2, 3, 4, 5, 6, 7}},
+      {{0, 8, 12, 11, 15, 16, 26, 24}, {1, 9, 12, 8, 13, 19, 26, 16},
+       {2, 10, 12, 9, 17, 22, 26, 19}, {3, 11, 12, 10, 20, 24, 26, 22},
+       {4, 23, 25, 14, 15, 24, 26, 16}, {5, 14, 25, 18, 13, 16, 26, 19},
+       {6, 18, 25, 21, 17, 19, 26, 22}, {7, 21, 25, 23, 20, 22, 26, 24}}};
+  switch(s) {
+    case Shape::Triangle: return tri;
+    case Shape::Quadrangle: return quad;
+    case Shape::Tetrahedron: return tet;
+    default: return hex;
+  }
+}
+
+struct KeyHash {
+  size_t operator()(const std::array<int, 4>& k) const {
+    uint64_t h = 1469598103934665603ull;
+    for(int v : k) { h ^= (uint64_t)(uint32_t)v; h *= 1099511628211ull; }
+    return (size_t)h;
+  }
+};
+
+// boundary component of a joint of the unit square / unit cube from its
+// centroid (Default_UnitSquare: 0 y=0, 1 x=1, 2 y=1, 3 x=0;
+// Default_UnitCube: 0 z=0, 1 x=0, 2 y=0, 3 z=1, 4 x=1, 5 y=1)
+int unit_box_component(int dim, const double* c) {
+  const double t = 1e-12;
+  if(dim == 2) {
+    if(std::fabs(c[1]) < t) return 0;
+    if(std::fabs(c[0] - 1) < t) return 1;
+    if(std::fabs(c[1] - 1) < t) return 2;
+    if(std::fabs(c[0]) < t) return 3;
+  } else {
+    if(std::fabs(c[2]) < t) return 0;
+    if(std::fabs(c[0]) < t) return 1;
+    if(std::fabs(c[1]) < t) return 2;
+    if(std::fabs(c[2] - 1) < t) return 3;
+    if(std::fabs(c[0] - 1) < t) return 4;
+    if(std::fabs(c[1] - 1) < t) return 5;
+  }
+  return -1;
+}
+
+// mark joints on the boundary of the unit box
+void set_box_boundary(Mesh& m) {
+  const auto& faces = shape_faces(m.shape);
+  m.nj = (int)faces.size();
+  m.joint_comp.assign((size_t)m.n_cells * m.nj, -1);
+  for(int c = 0; c < m.n_cells; ++c)
+    for(int j = 0; j < m.nj; ++j) {
+      double cen[3] = {0, 0, 0};
+      for(int lv : faces[j])
+        for(int k = 0; k < 3; ++k) cen[k] += m.coords[3 * m.cell(c)[lv] + k] / faces[j].size();
+      // a joint is on the boundary only if all its vertices share the plane
+      int comp = unit_box_component(m.dim(), cen);
+      if(comp >= 0) {
+        bool all = true;
+        for(int lv : faces[j]) {
+          const double* x = &m.coords[3 * m.cell(c)[lv]];
+          if(unit_box_component(m.dim(), x) < 0) all = false;
+        }
+        if(!all) comp = -1;
+      }
+      m.joint_comp[(size_t)c * m.nj + j] = comp;
+    }
+}
+
+}  // namespace
+
+Mesh default_mesh(DefaultGeometry g) {
+  Mesh m;
+  auto add_v = [&](double x, double y, double z) {
+    m.coords.push_back(x); m.coords.push_back(y); m.coords.push_back(z);
+  };
+  switch(g) {
+    case DefaultGeometry::UnitSquare:
+      m.shape = Shape::Quadrangle; m.nv = 4;
+      add_v(0, 0, 0); add_v(1, 0, 0); add_v(1, 1, 0); add_v(0, 1, 0);
+      m.cell_vertices = {0, 1, 2, 3};
+      break;
+    case DefaultGeometry::TwoTriangles:
+      m.shape = Shape::Triangle; m.nv = 3;
+      add_v(0, 0, 0); add_v(1, 0, 0); add_v(1, 1, 0); add_v(0, 1, 0);
+      m.cell_vertices = {0, 1, 2, 0, 2, 3};
+      break;
+    case DefaultGeometry::TwoTriangles2:
+      m.shape = Shape::Triangle; m.nv = 3;
+      add_v(0, 0, 0); add_v(1, 0, 0); add_v(1, 1, 0); add_v(0, 1, 0);
+      m.cell_vertices = {0, 1, 3, 1, 2, 3};
+      break;
+    case DefaultGeometry::CubeHexa:
+      m.shape = Shape::Hexahedron; m.nv = 8;
+      add_v(0, 0, 0); add_v(1, 0, 0); add_v(1, 1, 0); add_v(0, 1, 0);
+      add_v(0, 0, 1); add_v(1, 0, 1); add_v(1, 1, 1); add_v(0, 1, 1);
+      m.cell_vertices = {0, 1, 2, 3, 4, 5, 6, 7};
+      break;
+    case DefaultGeometry::CubeTetra:
+      m.shape = Shape::Tetrahedron; m.nv = 4;
+      add_v(0, 0, 0); add_v(1, 0, 0); add_v(1, 1, 0); add_v(0, 1, 0);
+      add_v(0, 0, 1); add_v(1, 0, 1); add_v(1, 1, 1); add_v(0, 1, 1);
+      m.cell_vertices = {0, 1, 2, 6, 0, 5, 1, 6, 0, 2, 3, 6,
+                         0, 3, 7, 6, 0, 4, 5, 6, 0, 7, 4, 6};
+      break;
+  }
+  m.n_cells = (int)(m.cell_vertices.size() / m.nv);
+  m.parent.assign(m.n_cells, -1);
+  m.child_index.assign(m.n_cells, 0);
+  set_box_boundary(m);
+  return m;
+}
+
+Mesh structured_cube(int n, bool tetra) {
+  PMCD_CHECK(n >= 1, "structured_cube needs n >= 1");
+  Mesh m;
+  m.shape = tetra ? Shape::Tetrahedron : Shape::Hexahedron;
+  m.nv = tetra ? 4 : 8;
+  const int n1 = n + 1;
+  m.coords.resize((size_t)3 * n1 * n1 * n1);
+  for(int k = 0; k < n1; ++k)
+    for(int j = 0; j < n1; ++j)
+      for(int i = 0; i < n1; ++i) {
+        size_t v = (size_t)i + (size_t)n1 * (j + (size_t)n1 * k);
+        m.coords[3 * v] = (double)i / n;
+        m.coords[3 * v + 1] = (double)j / n;
+        m.coords[3 * v + 2] = (double)k / n;
+      }
+  // local cube corners in the default unit-cube vertex order
+  const int co[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                        {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  const int tets[6][4] = {{0, 1, 2, 6}, {0, 5, 1, 6}, {0, 2, 3, 6},
+                          {0, 3, 7, 6}, {0, 4, 5, 6}, {0, 7, 4, 6}};
+  for(int k = 0; k < n; ++k)
+    for(int j = 0; j < n; ++j)
+      for(int i = 0; i < n; ++i) {
+        int g[8];
+        for(int a = 0; a < 8; ++a)
+          g[a] = (i + co[a][0]) + n1 * ((j + co[a][1]) + n1 * (k + co[a][2]));
+        if(tetra) {
+          for(auto& t : tets)
+            for(int a = 0; a < 4; ++a) m.cell_vertices.push_back(g[t[a]]);
+        } else {
+          for(int a = 0; a < 8; ++a) m.cell_vertices.push_back(g[a]);
+        }
+      }
+  m.n_cells = (int)(m.cell_vertices.size() / m.nv);
+  m.parent.assign(m.n_cells, -1);
+  m.child_index.assign(m.n_cells, 0);
+  set_box_boundary(m);
+  return m;
+}
+
+Mesh refine_regular(const Mesh& cm) {
+  const RefTable& rt = ref_table(cm.shape);
+  const auto& faces = shape_faces(cm.shape);
+  Mesh fm;
+  fm.shape = cm.shape;
+  fm.nv = cm.nv;
+  fm.nj = cm.nj;
+  fm.coords = cm.coords;  // old vertices keep their ids
+  const int n_children = (int)rt.child.size();
+  const int n_ref = (int)rt.span.size();
+  fm.n_cells = cm.n_cells * n_children;
+  fm.cell_vertices.resize((size_t)fm.n_cells * fm.nv);
+  fm.joint_comp.assign((size_t)fm.n_cells * fm.nj, -1);
+  fm.parent.resize(fm.n_cells);
+  fm.child_index.resize(fm.n_cells);
+
+  // refined vertices shared between cells are keyed by the sorted global ids
+  // of the vertices spanning them; cell-interior vertices are never shared
+  std::unordered_map<std::array<int, 4>, int, KeyHash> shared;
+  shared.reserve((size_t)cm.n_cells * 8);
+
+  // which refined vertices lie on each parent joint
+  std::vector<std::vector<bool>> on_joint(faces.size(), std::vector<bool>(n_ref, false));
+  for(size_t j = 0; j < faces.size(); ++j)
+    for(int r = 0; r < n_ref; ++r) {
+      bool in = true;
+      for(int v : rt.span[r])
+        if(std::find(faces[j].begin(), faces[j].end(), v) == faces[j].end()) in = false;
+      on_joint[j][r] = in;
+    }
+
+  std::vector<int> gid(n_ref);
+  for(int c = 0; c < cm.n_cells; ++c) {
+    const int* cv = cm.cell(c);
+    for(int r = 0; r < n_ref; ++r) {
+      const auto& sp = rt.span[r];
+      if(sp.size() == 1) { gid[r] = cv[sp[0]]; continue; }
+      double x[3] = {0, 0, 0};
+      for(int v : sp)
+        for(int k = 0; k < 3; ++k) x[k] += cm.coords[3 * cv[v] + k];
+      for(int k = 0; k < 3; ++k) x[k] *= 1.0 / sp.size();
+      bool interior = (int)sp.size() == cm.nv;
+      int id = -1;
+      if(!interior) {
+        std::array<int, 4> key = {-1, -1, -1, -1};
+        std::vector<int> g;
+        for(int v : sp) g.push_back(cv[v]);
+        std::sort(g.begin(), g.end());
+        for(size_t a = 0; a < g.size(); ++a) key[a] = g[a];
+        auto it = shared.find(key);
+        if(it != shared.end()) id = it->second;
+        else {
+          id = (int)(fm.coords.size() / 3);
+          fm.coords.push_back(x[0]); fm.coords.push_back(x[1]); fm.coords.push_back(x[2]);
+          shared.emplace(key, id);
+        }
+      } else {
+        id = (int)(fm.coords.size() / 3);
+        fm.coords.push_back(x[0]); fm.coords.push_back(x[1]); fm.coords.push_back(x[2]);
+      }
+      gid[r] = id;
+    }
+    for(int ch = 0; ch < n_children; ++ch) {
+      int fc = c * n_children + ch;
+      for(int a = 0; a < fm.nv; ++a) fm.cell_vertices[(size_t)fc * fm.nv + a] = gid[rt.child[ch][a]];
+      fm.parent[fc] = c;
+      fm.child_index[fc] = ch;
+      // a child joint inherits the component of the parent joint containing it
+      for(int jj = 0; jj < fm.nj; ++jj) {
+        for(size_t j = 0; j < faces.size(); ++j) {
+          int comp = cm.joint_comp[(size_t)c * cm.nj + j];
+          if(comp < 0) continue;
+          bool in = true;
+          for(int lv : faces[jj])
+            if(!on_joint[j][rt.child[ch][lv]]) in = false;
+          if(in) { fm.joint_comp[(size_t)fc * fm.nj + jj] = comp; break; }
+        }
+      }
+    }
+  }
+  return fm;
+}
+
+double cell_diameter(const Mesh& m, int c) {
+  const int* v = m.cell(c);
+  auto d2 = [&](int a, int b) {
+    const double* x = &m.coords[3 * v[a]];
+    const double* y = &m.coords[3 * v[b]];
+    double s = 0;
+    for(int k = 0; k < 3; ++k) s += (x[k] - y[k]) * (x[k] - y[k]);
+    return s;
+  };
+  double r = 0;
+  switch(m.shape) {
+    case Shape::Triangle: r = std::max(d2(0, 1), std::max(d2(1, 2), d2(2, 0))); break;
+    case Shape::Quadrangle: r = std::max(d2(0, 2), d2(1, 3)); break;
+    case Shape::Tetrahedron: {
+      const int e[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+      for(auto& p : e) r = std::max(r, d2(p[0], p[1]));
+      break;
+    }
+    case Shape::Hexahedron:
+      r = std::max(std::max(d2(0, 6), d2(1, 7)), std::max(d2(2, 4), d2(3, 5)));
+      break;
+  }
+  return std::sqrt(r);
+}
+
+}  // namespace pmcd

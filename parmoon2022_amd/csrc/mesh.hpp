@@ -1,0 +1,51 @@
+// Conforming meshes of triangles, quadrilaterals, tetrahedra or hexahedra,
+// generated from the reference's default coarse meshes by uniform regular
+// refinement. Cells of a refined level are stored in the order of a
+// depth-first walk over the refinement tree (src/Geometry/It_EQ.C:16-51,
+// src/Geometry/Domain.C:1961-2000), i.e. the children of each coarse cell,
+// in child order, in the coarse cell order.
+#pragma once
+#include "common.hpp"
+
+namespace pmcd {
+
+struct Mesh {
+  Shape shape = Shape::Triangle;
+  int n_cells = 0;
+  int nv = 0;                      // vertices per cell
+  int nj = 0;                      // joints (edges/faces) per cell
+  std::vector<double> coords;      // 3 per vertex
+  std::vector<int> cell_vertices;  // nv per cell, global vertex ids
+  std::vector<int> joint_comp;     // nj per cell: boundary component id, -1 if interior
+  std::vector<int> parent;         // parent cell in the coarser level, -1 for a root
+  std::vector<int> child_index;    // index of the cell among its parent's children
+
+  int dim() const { return shape_dim(shape); }
+  int n_vertices() const { return (int)(coords.size() / 3); }
+  const int* cell(int c) const { return &cell_vertices[(size_t)c * nv]; }
+};
+
+// Default coarse meshes (src/Geometry/Tests.C:95-188 TwoTriangles(2),
+// :473-517 UnitSquare, :1893-1935 unit-cube hexahedron, :1937-1995 unit-cube
+// tetrahedra). Boundary component ids follow the default boundary files
+// (Tests.C:1704-1736, :1799-1845).
+enum class DefaultGeometry : int {
+  UnitSquare = 0, TwoTriangles = 1, TwoTriangles2 = 2, CubeHexa = 3, CubeTetra = 4
+};
+Mesh default_mesh(DefaultGeometry g);
+
+// Structured N x N x N cube meshes as a mesh file would list them
+// (lexicographic cells, x fastest): hexahedra, or each cube split into the
+// six tetrahedra of the default unit-cube tetrahedral mesh.
+Mesh structured_cube(int n, bool tetra);
+
+// One step of regular refinement of every cell (RefTriRegDesc.C,
+// RefQuadRegDesc.C, RefTetraRegDesc.C, RefHexaRegDesc.C child tables).
+Mesh refine_regular(const Mesh& coarse);
+
+// Cell diameter as used for h_K with CELL_MEASURE=0 (Triangle.C:25-40,
+// Quadrangle.C:26-37, Tetrahedron.C:64-89, longest space diagonal for
+// hexahedra).
+double cell_diameter(const Mesh& m, int c);
+
+}  // namespace pmcd

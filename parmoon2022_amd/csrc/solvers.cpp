@@ -1,0 +1,277 @@
+// Iterative and direct solvers on the device-resident finest level.
+//
+// Reference: Solver<>::solve (src/Solver/Solver.C:601-696),
+// IterativeMethod::converged (include/Solver/IterativeMethod.h:83-162),
+// Iteration_cg::iterate (src/Solver/Iteration_cg.C:27-147),
+// Iteration_gmres::flexible_gmres (src/Solver/Iteration_gmres.C:430-620) with
+// the Givens/Hessenberg utilities (src/Solver/Utilities_gmres.C:6-108).
+// Vectors stay on the device; only the scalars of the Krylov recurrences
+// (dot products, Hessenberg column) come to the host, once per iteration.
+#include "system.hpp"
+#include <chrono>
+#include <cmath>
+
+namespace pmcd {
+
+namespace {
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// IterativeMethod::converged
+struct Convergence {
+  const SolverDesc& sd;
+  double initial = 0, previous = 0;
+  explicit Convergence(const SolverDesc& s) : sd(s) {}
+  bool operator()(double res, int it) {
+    if(it == 0) { initial = res; previous = res; }
+    const bool small = res < sd.res_tol;
+    const bool reduced = res < sd.res_red * initial;
+    const bool enough = it >= sd.min_iter;
+    const bool stagnation = res > 1e10 * previous;
+    const bool diverged = res > sd.div_factor * initial;
+    bool conv = (small || reduced || stagnation || diverged) && enough;
+    if(it == sd.max_iter) conv = true;
+    if(diverged)
+      PMCD_FAIL("Execution of the code stopped because of solver divergence: ", initial, " -> ", res,
+                " at iteration ", it);
+    previous = res;
+    return conv;
+  }
+};
+
+void givens_generate(double dx, double dy, double& cs, double& sn) {
+  if(dy == 0.0) { cs = 1.0; sn = 0.0; }
+  else if(std::abs(dy) > std::abs(dx)) {
+    double t = dx / dy;
+    sn = 1.0 / std::sqrt(1.0 + t * t);
+    cs = t * sn;
+  } else {
+    double t = dy / dx;
+    cs = 1.0 / std::sqrt(1.0 + t * t);
+    sn = t * cs;
+  }
+}
+
+void givens_apply(double& dx, double& dy, double cs, double sn) {
+  double t = cs * dx + sn * dy;
+  dy = -sn * dx + cs * dy;
+  dx = t;
+}
+
+}  // namespace
+
+// Solver<>::update_matrix (Solver.C:423-501): factorize the coarse (or, for
+// the direct solver, the only) level, and make FP32 copies of the level
+// matrices for the mixed-precision V-cycle.
+void CDSystem::prepare(const SolverDesc& s) {
+  sd_ = s;
+  if((s.precond == 2 && n_levels() > 1) || s.method == 0) {
+    if(s.method == 0 && n_levels() > 1) PMCD_FAIL("the direct solver expects a single level");
+    if(!coarse_ready_) coarse_factorize();
+  }
+  if(s.mixed && !mixed_ready_) {
+    for(auto& Lp : levels_) {
+      Level& L = *Lp;
+      L.val32.alloc(L.val.n);
+      L.diag32.alloc(L.n());
+      launch_d2f((int)L.val.n, L.val.p, L.val32.p, stream_);
+      launch_d2f(L.n(), L.diag.p, L.diag32.p, stream_);
+    }
+    mixed_ready_ = true;
+  }
+}
+
+SolveStats CDSystem::solve(const SolverDesc& s) {
+  prepare(s);
+  const double t0 = now_s();
+  SolveStats st;
+  switch(s.method) {
+    case 0: st = solve_direct(); break;
+    case 1: st = solve_cg(); break;
+    case 2: st = solve_fgmres(); break;
+    case 3: st = solve_richardson(); break;
+    default: PMCD_FAIL("unknown solver method ", s.method);
+  }
+  PMCD_HIP(hipStreamSynchronize(stream_));
+  st.seconds = now_s() - t0;
+  return st;
+}
+
+SolveStats CDSystem::solve_direct() {
+  Level& F = finest();
+  SolveStats st;
+  DevBuf<double> b(F.n());
+  launch_copy(F.n(), F.rhs.p, b.p, stream_);
+  launch_gemv(F.n(), coarse_inv_.p, b.p, F.sol.p, stream_);
+  st.converged = 1;
+  return st;
+}
+
+SolveStats CDSystem::solve_richardson() {
+  // x <- x + M^{-1} (b - A x)
+  Level& F = finest();
+  const int n = F.n();
+  Convergence conv(sd_);
+  if(kr_.n != (size_t)n) { kr_.alloc(n); kw_.alloc(n); }
+  SolveStats st;
+  launch_copy(n, F.rhs.p, kr_.p, stream_);
+  apply_scaled_add(F, F.sol.p, kr_.p, -1.0);
+  double res = std::sqrt(dot(n, kr_.p, kr_.p));
+  st.initial_residual = res;
+  int it = 0;
+  if(!conv(res, 0)) {
+    for(it = 1; it <= sd_.max_iter; ++it) {
+      precond(kr_.p, kw_.p);
+      launch_axpy(n, 1.0, kw_.p, F.sol.p, stream_);
+      launch_copy(n, F.rhs.p, kr_.p, stream_);
+      apply_scaled_add(F, F.sol.p, kr_.p, -1.0);
+      res = std::sqrt(dot(n, kr_.p, kr_.p));
+      if(conv(res, it)) break;
+    }
+  }
+  st.iterations = std::min(it, sd_.max_iter);
+  st.final_residual = res;
+  st.converged = res < sd_.res_tol || res < sd_.res_red * st.initial_residual;
+  return st;
+}
+
+SolveStats CDSystem::solve_cg() {
+  Level& F = finest();
+  const int n = F.n();
+  Convergence conv(sd_);
+  DevBuf<double> p(n), z(n), q(n), r(n);
+  double* x = F.sol.p;
+  SolveStats st;
+  launch_copy(n, F.rhs.p, r.p, stream_);
+  apply_scaled_add(F, x, r.p, -1.0);
+  double normb = std::sqrt(dot(n, F.rhs.p, F.rhs.p));
+  if(normb == 0.0) {
+    normb = 1.0;
+    launch_fill(n, 0.0, r.p, stream_);
+    launch_fill(n, 0.0, x, stream_);
+  }
+  double resid = std::sqrt(dot(n, r.p, r.p)) / normb;
+  st.initial_residual = resid;
+  if(conv(resid, 0)) { st.final_residual = resid; st.converged = 1; return st; }
+  double rho_1 = 0.0;
+  for(int i = 1; i <= sd_.max_iter; ++i) {
+    precond(r.p, z.p);
+    const double rho = dot(n, r.p, z.p);
+    if(i == 1) launch_copy(n, z.p, p.p, stream_);
+    else {
+      const double beta = rho / rho_1;
+      launch_scale_inplace(n, beta, p.p, stream_);
+      launch_axpy(n, 1.0, z.p, p.p, stream_);
+    }
+    apply(F, p.p, q.p);
+    const double alpha = rho / dot(n, p.p, q.p);
+    launch_axpy(n, alpha, p.p, x, stream_);
+    launch_axpy(n, -alpha, q.p, r.p, stream_);
+    resid = std::sqrt(dot(n, r.p, r.p)) / normb;
+    st.iterations = i;
+    if(conv(resid, i)) break;
+    rho_1 = rho;
+  }
+  st.final_residual = resid;
+  st.converged = resid < sd_.res_tol || resid < sd_.res_red * st.initial_residual;
+  return st;
+}
+
+SolveStats CDSystem::solve_fgmres() {
+  Level& F = finest();
+  const int n = F.n();
+  const int m = sd_.restart;
+  Convergence conv(sd_);
+  if(kv_.size() != (size_t)m + 1 || kr_.n != (size_t)n) {
+    kv_.clear(); kz_.clear();
+    for(int i = 0; i <= m; ++i) kv_.emplace_back(n);
+    for(int i = 0; i < m; ++i) kz_.emplace_back(n);
+    kr_.alloc(n);
+    kw_.alloc(n);
+  }
+  kh_.alloc(m + 2);
+  std::vector<double> s(m + 1), cs(m + 1), sn(m + 1);
+  // Hessenberg matrix, column-major (m+1) x m
+  std::vector<double> H((size_t)(m + 1) * m, 0.0);
+  auto h = [&](int i, int j) -> double& { return H[(size_t)j * (m + 1) + i]; };
+  double* x = F.sol.p;
+  double* r = kr_.p;
+  SolveStats st;
+  launch_copy(n, F.rhs.p, r, stream_);
+  apply_scaled_add(F, x, r, -1.0);
+  double resid = std::sqrt(dot(n, r, r));
+  double beta = resid;
+  st.initial_residual = resid;
+  if(conv(resid, 0)) { st.final_residual = resid; st.converged = 1; return st; }
+  std::vector<double> hcol(m + 2);
+  int j = 1;
+  auto update = [&](int k) {
+    // Hessenberg back-solve (Utilities_gmres.C:90-108) and x += sum y_j z_j
+    std::vector<double> y(s.begin(), s.end());
+    for(int i = k; i >= 0; --i) {
+      y[i] /= h(i, i);
+      for(int jj = i - 1; jj >= 0; --jj) y[jj] -= h(jj, i) * y[i];
+    }
+    launch_fill(n, 0.0, r, stream_);
+    for(int jj = 0; jj <= k; ++jj) launch_axpy(n, y[jj], kz_[jj].p, r, stream_);
+    launch_axpy(n, 1.0, r, x, stream_);
+  };
+  while(j <= sd_.max_iter) {
+    launch_scale_copy(n, 1.0 / beta, r, kv_[0].p, stream_);
+    std::fill(s.begin(), s.end(), 0.0);
+    s[0] = beta;
+    for(int i = 0; i < m && j <= sd_.max_iter; ++i, ++j) {
+      precond(kv_[i].p, kz_[i].p);
+      apply(F, kz_[i].p, r);
+      // modified Gram-Schmidt: H(k,i) = <r, v_k>; r -= H(k,i) v_k, fused pairwise
+      launch_dot(n, r, kv_[0].p, red_partial_.p, kh_.p, stream_);
+      for(int k = 0; k <= i; ++k)
+        launch_mgs_step(n, kh_.p + k, r, kv_[k].p, k < i ? kv_[k + 1].p : nullptr,
+                        red_partial_.p, kh_.p + k + 1, stream_);
+      launch_dot(n, r, r, red_partial_.p, kh_.p + i + 1, stream_);
+      PMCD_HIP(hipMemcpyAsync(hcol.data(), kh_.p, sizeof(double) * (i + 2), hipMemcpyDeviceToHost,
+                              stream_));
+      PMCD_HIP(hipStreamSynchronize(stream_));
+      for(int k = 0; k <= i; ++k) h(k, i) = hcol[k];
+      h(i + 1, i) = std::sqrt(hcol[i + 1]);
+      if(h(i + 1, i) != 0.0) launch_scale_copy(n, 1.0 / h(i + 1, i), r, kv_[i + 1].p, stream_);
+      else PMCD_FAIL("Unhappy breakdown in flexible gmres at iteration ", j);
+      for(int k = 0; k < i; ++k) givens_apply(h(k, i), h(k + 1, i), cs[k], sn[k]);
+      givens_generate(h(i, i), h(i + 1, i), cs[i], sn[i]);
+      givens_apply(h(i, i), h(i + 1, i), cs[i], sn[i]);
+      givens_apply(s[i], s[i + 1], cs[i], sn[i]);
+      resid = std::abs(s[i + 1]);
+      st.iterations = j;
+      if(conv(resid, j)) {
+        update(i);
+        st.final_residual = resid;
+        st.converged = resid < sd_.res_tol || resid < sd_.res_red * st.initial_residual;
+        return st;
+      }
+    }
+    update(m - 1);
+    launch_copy(n, F.rhs.p, r, stream_);
+    apply_scaled_add(F, x, r, -1.0);
+    beta = std::sqrt(dot(n, r, r));
+    if(conv(resid, j)) break;
+  }
+  st.final_residual = resid;
+  st.converged = resid < sd_.res_tol || resid < sd_.res_red * st.initial_residual;
+  return st;
+}
+
+void CDSystem::errors(double out[5]) {
+  Level& F = finest();
+  AsmArgs a = asm_args(F);
+  launch_cell_errors(a, F.dof.p, F.sol.p, cell_err_.p, stream_);
+  launch_sum_max(F.mesh.n_cells, 5, cell_err_.p, red_partial_.p, red_out_.p, stream_);
+  double e[5];
+  PMCD_HIP(hipMemcpyAsync(e, red_out_.p, sizeof(e), hipMemcpyDeviceToHost, stream_));
+  PMCD_HIP(hipStreamSynchronize(stream_));
+  for(int k = 0; k < 4; ++k) out[k] = std::sqrt(e[k]);
+  out[4] = e[4];
+}
+
+}  // namespace pmcd

@@ -1,0 +1,79 @@
+#include "structure.hpp"
+#include <algorithm>
+#include <climits>
+
+namespace pmcd {
+
+Structure build_structure(const FESpace& sp, int n_cells) {
+  Structure st;
+  const int nl = sp.n_local;
+  const int n = sp.n_dof;
+  st.n_rows = n;
+  // DOF -> (cell, local) list, cells increasing
+  st.contrib_ptr.assign(n + 1, 0);
+  for(size_t p = 0; p < sp.dof.size(); ++p) st.contrib_ptr[sp.dof[p] + 1]++;
+  for(int r = 0; r < n; ++r) st.contrib_ptr[r + 1] += st.contrib_ptr[r];
+  st.contrib.resize(sp.dof.size());
+  {
+    std::vector<int> fill(st.contrib_ptr.begin(), st.contrib_ptr.end() - 1);
+    for(int c = 0; c < n_cells; ++c)
+      for(int m = 0; m < nl; ++m) st.contrib[fill[sp.dof[(size_t)c * nl + m]]++] = c * nl + m;
+  }
+  // rows: union of the DOFs of the cells containing the row DOF, ascending
+  std::vector<int64_t> len(n, 0);
+#pragma omp parallel
+  {
+    std::vector<int> buf;
+#pragma omp for schedule(dynamic, 1024)
+    for(int r = 0; r < n; ++r) {
+      buf.clear();
+      for(int e = st.contrib_ptr[r]; e < st.contrib_ptr[r + 1]; ++e) {
+        int c = st.contrib[e] / nl;
+        const int* d = sp.cell_dof(c);
+        buf.insert(buf.end(), d, d + nl);
+      }
+      std::sort(buf.begin(), buf.end());
+      len[r] = std::unique(buf.begin(), buf.end()) - buf.begin();
+    }
+  }
+  st.row_ptr.assign(n + 1, 0);
+  int64_t tot = 0;
+  for(int r = 0; r < n; ++r) {
+    tot += len[r];
+    PMCD_CHECK(tot < INT_MAX, "matrix has more than 2^31 entries");
+    st.row_ptr[r + 1] = (int)tot;
+    st.max_row = std::max(st.max_row, (int)len[r]);
+  }
+  st.nnz = tot;
+  PMCD_CHECK(st.max_row <= 255, "row with ", st.max_row, " entries exceeds the 8-bit slot map");
+  st.col.resize(tot);
+  st.diag.resize(n);
+  st.slot.resize(st.contrib.size() * (size_t)nl);
+#pragma omp parallel
+  {
+    std::vector<int> buf;
+#pragma omp for schedule(dynamic, 1024)
+    for(int r = 0; r < n; ++r) {
+      buf.clear();
+      for(int e = st.contrib_ptr[r]; e < st.contrib_ptr[r + 1]; ++e) {
+        int c = st.contrib[e] / nl;
+        const int* d = sp.cell_dof(c);
+        buf.insert(buf.end(), d, d + nl);
+      }
+      std::sort(buf.begin(), buf.end());
+      buf.erase(std::unique(buf.begin(), buf.end()), buf.end());
+      std::copy(buf.begin(), buf.end(), st.col.begin() + st.row_ptr[r]);
+      st.diag[r] = st.row_ptr[r] + (int)(std::lower_bound(buf.begin(), buf.end(), r) - buf.begin());
+      for(int e = st.contrib_ptr[r]; e < st.contrib_ptr[r + 1]; ++e) {
+        int c = st.contrib[e] / nl;
+        const int* d = sp.cell_dof(c);
+        for(int k = 0; k < nl; ++k)
+          st.slot[(size_t)e * nl + k] =
+              (uint8_t)(std::lower_bound(buf.begin(), buf.end(), d[k]) - buf.begin());
+      }
+    }
+  }
+  return st;
+}
+
+}  // namespace pmcd

@@ -1,0 +1,494 @@
+#include "system.hpp"
+#include "elements.hpp"
+#include <chrono>
+#include <cmath>
+#include <algorithm>
+
+namespace pmcd {
+
+namespace {
+
+// physical position of the nodal point of local DOF i of cell c
+void node_position(const Mesh& m, int c, const ElementInfo& info, int i, double* x) {
+  const int* v = m.cell(c);
+  const auto& p = info.nodes[i];
+  for(int k = 0; k < 3; ++k) x[k] = 0;
+  if(shape_is_simplex(m.shape)) {
+    double lam[4] = {1 - p[0] - p[1] - (m.dim() == 3 ? p[2] : 0.0), p[0], p[1], p[2]};
+    for(int a = 0; a <= m.dim(); ++a)
+      for(int k = 0; k < 3; ++k) x[k] += lam[a] * m.coords[3 * v[a] + k];
+  } else {
+    // multilinear interpolation of the vertices (tensor reference cell [-1,1]^d)
+    const int nvx = m.nv;
+    for(int a = 0; a < nvx; ++a) {
+      // vertex a's reference corner
+      static const int q2[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+      int cx = q2[a % 4][0], cy = q2[a % 4][1], cz = a / 4;
+      double w = (cx ? (1 + p[0]) : (1 - p[0])) / 2 * ((cy ? (1 + p[1]) : (1 - p[1])) / 2);
+      if(m.dim() == 3) w *= (cz ? (1 + p[2]) : (1 - p[2])) / 2;
+      for(int k = 0; k < 3; ++k) x[k] += w * m.coords[3 * v[a] + k];
+    }
+  }
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+std::vector<std::unique_ptr<Level>> build_levels(const ProblemDesc& d) {
+  PMCD_CHECK(d.order == 1 || d.order == 2, "ansatz order must be 1 or 2");
+  PMCD_CHECK(d.n_refine >= 0 && d.n_refine <= 12, "n_refine out of range");
+  Mesh base;
+  if(d.geometry == 10 || d.geometry == 11) base = structured_cube(d.structured_n, d.geometry == 11);
+  else {
+    PMCD_CHECK(d.geometry >= 0 && d.geometry <= 4, "unknown geometry ", d.geometry);
+    base = default_mesh((DefaultGeometry)d.geometry);
+  }
+  std::vector<Mesh> meshes;
+  meshes.push_back(std::move(base));
+  for(int i = 0; i < d.n_refine; ++i) meshes.push_back(refine_regular(meshes.back()));
+  const int keep = std::max(1, std::min(d.mg_levels, (int)meshes.size()));
+  const ExampleParams ex = d.ex;
+  BoundCondFn bc = [ex](int comp, const double*) { return (BoundCond)example_bc(ex, comp); };
+  std::vector<std::unique_ptr<Level>> levels;
+  for(int l = (int)meshes.size() - keep; l < (int)meshes.size(); ++l) {
+    auto L = std::make_unique<Level>();
+    L->mesh = std::move(meshes[l]);
+    L->space = build_fespace(L->mesh, d.order, bc);
+    L->st = build_structure(L->space, L->mesh.n_cells);
+    levels.push_back(std::move(L));
+  }
+  return levels;
+}
+
+CDSystem::CDSystem(const ProblemDesc& d, hipStream_t stream) : desc_(d), stream_(stream) {
+  levels_ = build_levels(d);
+  // quadrature and reference tables
+  const Shape shape = levels_.back()->mesh.shape;
+  rule_ = cell_rule_for(shape, d.order, d.linear_quad != 0);
+  const QuadFormula& qf = quad_formula(rule_);
+  const Element el = element_for(shape, d.order);
+  const int nl = element_n_dof(el);
+  ncomp_ = shape_dim(shape) == 2 ? kNDeriv2D : kNDeriv3D;
+  std::vector<double> ref((size_t)qf.n() * ncomp_ * nl), qp((size_t)qf.n() * 3);
+  for(int q = 0; q < qf.n(); ++q) {
+    double p[3] = {qf.xi[q], qf.dim >= 2 ? qf.eta[q] : 0.0, qf.dim >= 3 ? qf.zeta[q] : 0.0};
+    eval_basis(el, p, &ref[(size_t)q * ncomp_ * nl]);
+    for(int k = 0; k < 3; ++k) qp[3 * q + k] = p[k];
+  }
+  ref_.upload(ref, stream_);
+  qw_.upload(qf.w, stream_);
+  qp_.upload(qp, stream_);
+  size_t max_cells = 0, max_n = 0;
+  for(auto& L : levels_) {
+    build_level_device(*L);
+    max_cells = std::max(max_cells, (size_t)L->mesh.n_cells);
+    max_n = std::max(max_n, (size_t)L->n());
+  }
+  for(size_t l = 1; l < levels_.size(); ++l) build_transfer(*levels_[l], *levels_[l - 1]);
+  local_A_.alloc(max_cells * nl * nl);
+  local_rhs_.alloc(max_cells * nl);
+  red_partial_.alloc(1024 * 5 + 64);
+  red_out_.alloc(64);
+  cell_err_.alloc(max_cells * 5);
+  PMCD_HIP(hipStreamSynchronize(stream_));
+}
+
+CDSystem::~CDSystem() { (void)hipStreamSynchronize(stream_); }
+
+void CDSystem::build_level_device(Level& L) {
+  const Mesh& m = L.mesh;
+  const FESpace& sp = L.space;
+  const Structure& st = L.st;
+  const ElementInfo& info = element_info(sp.element);
+  // packed vertex coordinates per cell
+  std::vector<double> xyz((size_t)m.n_cells * m.nv * 3);
+  for(int c = 0; c < m.n_cells; ++c)
+    for(int a = 0; a < m.nv; ++a)
+      for(int k = 0; k < 3; ++k) xyz[((size_t)c * m.nv + a) * 3 + k] = m.coords[3 * m.cell(c)[a] + k];
+  L.cell_xyz.upload(xyz, stream_);
+  L.dof.upload(sp.dof, stream_);
+  L.row_ptr.upload(st.row_ptr, stream_);
+  L.col.upload(st.col, stream_);
+  L.diag_pos.upload(st.diag, stream_);
+  L.contrib_ptr.upload(st.contrib_ptr, stream_);
+  L.contrib.upload(st.contrib, stream_);
+  L.slot.upload(st.slot, stream_);
+  // Dirichlet values at the nodal points (Assemble3D.C:472-549, Assemble2D.C:3200-3230)
+  const int n = sp.n_dof, na = sp.n_active, nl = sp.n_local;
+  L.dirichlet_host.assign(n - na, 0.0);
+  std::vector<char> seen(n - na, 0);
+  for(int c = 0; c < m.n_cells; ++c)
+    for(int i = 0; i < nl; ++i) {
+      int r = sp.cell_dof(c)[i];
+      if(r < na || seen[r - na]) continue;
+      seen[r - na] = 1;
+      double x[3];
+      node_position(m, c, info, i, x);
+      L.dirichlet_host[r - na] = example_bvalue(desc_.ex, 0, x[0], x[1], x[2]);
+    }
+  L.dir_vals.upload(L.dirichlet_host, stream_);
+  // Neumann edge integrals in 2D (Assemble2D.C:3214-3255): per (cell, local
+  // DOF) the terms in the order the reference adds them
+  if(m.dim() == 2) {
+    std::vector<std::vector<double>> terms((size_t)m.n_cells * nl);
+    bool any = false;
+    const auto& faces = shape_faces(m.shape);
+    const QuadFormula& lq = quad_formula(line_rule_for_degree(2 * element_order(sp.element)));
+    for(int c = 0; c < m.n_cells; ++c)
+      for(int j = 0; j < m.nj; ++j) {
+        int comp = m.joint_comp[(size_t)c * m.nj + j];
+        if(comp < 0 || example_bc(desc_.ex, comp) != NEUMANN) continue;
+        any = true;
+        const int* v = m.cell(c);
+        const double* x0 = &m.coords[3 * v[faces[j][0]]];
+        const double* x1 = &m.coords[3 * v[faces[j][1]]];
+        const double hE = std::sqrt((x1[0] - x0[0]) * (x1[0] - x0[0]) + (x1[1] - x0[1]) * (x1[1] - x0[1])) / 2;
+        // reference coordinates of the edge end points
+        double r0[3], r1[3];
+        {
+          const auto& A = info.nodes;  // corner nodes are nodes of the element
+          auto corner = [&](int lv, double* r) {
+            for(int i = 0; i < info.n_dof; ++i)
+              if(info.entity_vertices[i].size() == 1 && info.entity_vertices[i][0] == lv) {
+                for(int k = 0; k < 3; ++k) r[k] = A[i][k];
+                return;
+              }
+          };
+          corner(faces[j][0], r0);
+          corner(faces[j][1], r1);
+        }
+        for(int l = 0; l < lq.n(); ++l) {
+          const double s = (1 + lq.xi[l]) / 2;
+          double rp[3], xp[3], phi[kMaxLocalDof * kNDeriv3D];
+          for(int k = 0; k < 3; ++k) { rp[k] = r0[k] + s * (r1[k] - r0[k]); xp[k] = x0[k] + s * (x1[k] - x0[k]); }
+          eval_basis(sp.element, rp, phi);
+          double val = example_bvalue(desc_.ex, comp, xp[0], xp[1], 0.0);
+          val *= hE * lq.w[l];
+          for(int k = 0; k < nl; ++k)
+            if(sp.cell_dof(c)[k] < na) terms[(size_t)c * nl + k].push_back(val * phi[k]);
+        }
+      }
+    if(any) {
+      std::vector<int> ptr((size_t)m.n_cells * nl + 1, 0);
+      std::vector<double> flat;
+      for(size_t p = 0; p < terms.size(); ++p) {
+        ptr[p + 1] = ptr[p] + (int)terms[p].size();
+        flat.insert(flat.end(), terms[p].begin(), terms[p].end());
+      }
+      L.extra_ptr.upload(ptr, stream_);
+      L.extra.upload(flat, stream_);
+    }
+  }
+  L.val.alloc(st.nnz);
+  L.rhs.alloc(n);
+  L.sol.alloc(n);
+  L.diag.alloc(n);
+  L.mg_sol.alloc(n); L.mg_sol2.alloc(n); L.mg_rhs.alloc(n); L.mg_def.alloc(n); L.mg_tmp.alloc(n);
+}
+
+// P[f][c] = value of coarse basis function c at the nodal point of fine DOF f
+// (the interpolation the reference's GridTransfer::Prolongate applies cell by
+// cell and averages, src/Multigrid/GridTransfer.C:603-865); R = P^T
+// (DefectRestriction, :866-1115).
+void CDSystem::build_transfer(Level& F, const Level& C) {
+  const Mesh& fm = F.mesh;
+  const Mesh& cm = C.mesh;
+  const ElementInfo& fi = element_info(F.space.element);
+  const ElementInfo& ci = element_info(C.space.element);
+  const int nf = F.n(), ncd = C.n();
+  std::vector<std::vector<std::pair<int, double>>> rows(nf);
+  std::vector<char> done(nf, 0);
+  std::vector<double> phi(kMaxLocalDof * kNDeriv3D);
+  for(int c = 0; c < fm.n_cells; ++c) {
+    const int p = fm.parent[c];
+    PMCD_CHECK(p >= 0 && p < cm.n_cells, "level hierarchy mismatch");
+    const int* cv = cm.cell(p);
+    // inverse affine map of the parent cell
+    const int dim = cm.dim();
+    double o[3], J[3][3];
+    if(shape_is_simplex(cm.shape)) {
+      for(int k = 0; k < 3; ++k) {
+        o[k] = cm.coords[3 * cv[0] + k];
+        for(int a = 0; a < dim; ++a) J[k][a] = cm.coords[3 * cv[a + 1] + k] - o[k];
+      }
+    } else {
+      // x = x0 + J (xi + 1) with the affine map of vertices 0, 1, 3 (, 4)
+      const int e[3] = {1, 3, 4};
+      for(int k = 0; k < 3; ++k) {
+        o[k] = cm.coords[3 * cv[0] + k];
+        for(int a = 0; a < dim; ++a) J[k][a] = (cm.coords[3 * cv[e[a]] + k] - o[k]) / 2;
+      }
+    }
+    // invert J (dim x dim)
+    double Ji[3][3] = {{0}};
+    if(dim == 2) {
+      double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      Ji[0][0] = J[1][1] / det; Ji[0][1] = -J[0][1] / det;
+      Ji[1][0] = -J[1][0] / det; Ji[1][1] = J[0][0] / det;
+    } else {
+      double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                   J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                   J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+      Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+      Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+      Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+      Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+      Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+      Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+      Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+      Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+      Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+    }
+    for(int i = 0; i < fi.n_dof; ++i) {
+      const int f = F.space.cell_dof(c)[i];
+      if(done[f]) continue;
+      done[f] = 1;
+      double x[3];
+      node_position(fm, c, fi, i, x);
+      double d[3], rp[3] = {0, 0, 0};
+      for(int k = 0; k < 3; ++k) d[k] = x[k] - o[k];
+      for(int a = 0; a < dim; ++a) {
+        double s = 0;
+        for(int k = 0; k < dim; ++k) s += Ji[a][k] * d[k];
+        rp[a] = shape_is_simplex(cm.shape) ? s : s - 1.0;
+      }
+      eval_basis(C.space.element, rp, phi.data());
+      for(int l = 0; l < ci.n_dof; ++l) {
+        double v = phi[l];
+        if(std::fabs(v) > 1e-13) rows[f].push_back({C.space.cell_dof(p)[l], v});
+      }
+    }
+  }
+  std::vector<int> rp(nf + 1, 0), col;
+  std::vector<double> val;
+  for(int f = 0; f < nf; ++f) {
+    auto& r = rows[f];
+    std::sort(r.begin(), r.end());
+    for(auto& e : r) { col.push_back(e.first); val.push_back(e.second); }
+    rp[f + 1] = (int)col.size();
+  }
+  // transpose
+  std::vector<int> trp(ncd + 1, 0), tcol(col.size());
+  std::vector<double> tval(col.size());
+  for(int c : col) trp[c + 1]++;
+  for(int c = 0; c < ncd; ++c) trp[c + 1] += trp[c];
+  std::vector<int> fill(trp.begin(), trp.end() - 1);
+  for(int f = 0; f < nf; ++f)
+    for(int k = rp[f]; k < rp[f + 1]; ++k) {
+      int pos = fill[col[k]]++;
+      tcol[pos] = f;
+      tval[pos] = val[k];
+    }
+  F.n_coarse = ncd;
+  F.p_rp.upload(rp, stream_); F.p_col.upload(col, stream_); F.p_val.upload(val, stream_);
+  F.r_rp.upload(trp, stream_); F.r_col.upload(tcol, stream_); F.r_val.upload(tval, stream_);
+  std::vector<float> v32(val.begin(), val.end()), t32(tval.begin(), tval.end());
+  F.p_val32.upload(v32, stream_);
+  F.r_val32.upload(t32, stream_);
+  PMCD_HIP(hipStreamSynchronize(stream_));
+}
+
+AsmArgs CDSystem::asm_args(const Level& L) {
+  AsmArgs a;
+  a.shape = (int)L.mesh.shape;
+  a.dim = L.mesh.dim();
+  a.nv = L.mesh.nv;
+  a.nl = L.space.n_local;
+  a.nq = quad_formula(rule_).n();
+  a.ncomp = ncomp_;
+  a.supg = desc_.supg;
+  a.n_cells = L.mesh.n_cells;
+  a.ex = desc_.ex;
+  a.ex.dim = a.dim;
+  a.ref = ref_.p;
+  a.qw = qw_.p;
+  a.qp = qp_.p;
+  a.cell_xyz = L.cell_xyz.p;
+  a.local_A = local_A_.p;
+  a.local_rhs = local_rhs_.p;
+  return a;
+}
+
+void CDSystem::assemble_level(Level& L) {
+  AsmArgs a = asm_args(L);
+  launch_assemble_local(a, stream_);
+  launch_gather_matrix(L.n(), L.n_active(), a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p,
+                       L.contrib.p, L.slot.p, local_A_.p, L.val.p, stream_);
+  launch_gather_rhs(L.n(), L.n_active(), L.contrib_ptr.p, L.contrib.p, local_rhs_.p,
+                    L.extra_ptr.p, L.extra.p, L.dir_vals.p, L.rhs.p, stream_);
+  launch_gather_idx(L.n(), L.diag_pos.p, L.val.p, L.diag.p, stream_);
+}
+
+void CDSystem::assemble() {
+  PMCD_HIP(hipStreamSynchronize(stream_));
+  const double t0 = now_s();
+  for(auto& L : levels_) assemble_level(*L);
+  // the solution starts at zero with the Dirichlet values copied from the rhs
+  // (ConvectionDiffusion.C:443, BlockVector::copy_nonactive)
+  Level& F = finest();
+  launch_fill(F.n_active(), 0.0, F.sol.p, stream_);
+  launch_copy(F.n() - F.n_active(), F.rhs.p + F.n_active(), F.sol.p + F.n_active(), stream_);
+  PMCD_HIP(hipStreamSynchronize(stream_));
+  last_assemble_ms = (now_s() - t0) * 1e3;
+  coarse_ready_ = false;
+  mixed_ready_ = false;
+}
+
+void CDSystem::coarse_factorize() {
+  Level& C = *levels_.front();
+  const int n = C.n();
+  PMCD_CHECK(n <= 20000, "coarse direct solve limited to 20000 unknowns, got ", n);
+  DevBuf<double> A((size_t)n * n);
+  coarse_inv_.alloc((size_t)n * n);
+  DevBuf<double> work(n + 2);
+  DevBuf<int> piv(1);
+  launch_csr_to_dense(n, C.row_ptr.p, C.col.p, C.val.p, A.p, stream_);
+  dense_inverse(n, A.p, coarse_inv_.p, work.p, piv.p, stream_);
+  coarse_tmp_.alloc(2 * (size_t)n);
+  PMCD_HIP(hipStreamSynchronize(stream_));
+  coarse_ready_ = true;
+}
+
+void CDSystem::apply(Level& L, const double* x, double* y) {
+  launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, nullptr, y, 1.0, 0.0, stream_);
+}
+
+void CDSystem::apply_scaled_add(Level& L, const double* x, double* y, double a) {
+  launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, y, y, a, 1.0, stream_);
+}
+
+double CDSystem::dot(int n, const double* x, const double* y) {
+  launch_dot(n, x, y, red_partial_.p, red_out_.p, stream_);
+  double h;
+  PMCD_HIP(hipMemcpyAsync(&h, red_out_.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+  PMCD_HIP(hipStreamSynchronize(stream_));
+  return h;
+}
+
+// n damped Jacobi sweeps; the result ends in x (ping-pong through x2)
+void CDSystem::smooth(Level& L, double* x, double* x2, const double* b, int steps) {
+  for(int s = 0; s < steps; ++s) {
+    launch_jacobi(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b, x, x2,
+                  sd_.jacobi_damp, stream_);
+    launch_copy(L.n(), x2, x, stream_);
+  }
+}
+
+void CDSystem::smooth32(Level& L, float* x, float* x2, const float* b, int steps) {
+  for(int s = 0; s < steps; ++s) {
+    launch_jacobi_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x, x2,
+                      (float)sd_.jacobi_damp, stream_);
+    std::swap(x, x2);
+    // keep the iterate in the caller's buffer
+    if(s == steps - 1 && (steps % 2) == 1)
+      PMCD_HIP(hipMemcpyAsync(x2, x, sizeof(float) * L.n(), hipMemcpyDeviceToDevice, stream_));
+  }
+}
+
+// Multigrid::cycle with a V-cycle control (Multigrid.C:157-308): finest
+// iterate 0 with the non-active entries of the rhs, pre-smoothing and defect
+// restriction down to the coarsest level, direct coarse solve, damped
+// prolongation and post-smoothing on the way up.
+void CDSystem::vcycle64(const double* rhs, double* out) {
+  const int Lf = n_levels() - 1;
+  std::vector<const double*> b(n_levels());
+  std::vector<double*> x(n_levels());
+  for(int l = 0; l <= Lf; ++l) { b[l] = levels_[l]->mg_rhs.p; x[l] = levels_[l]->mg_sol.p; }
+  b[Lf] = rhs;
+  {
+    Level& F = finest();
+    launch_fill(F.n_active(), 0.0, x[Lf], stream_);
+    launch_copy(F.n() - F.n_active(), rhs + F.n_active(), x[Lf] + F.n_active(), stream_);
+  }
+  for(int l = Lf; l >= 1; --l) {
+    Level& L = *levels_[l];
+    Level& C = *levels_[l - 1];
+    smooth(L, x[l], L.mg_sol2.p, b[l], sd_.n_pre);
+    launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x[l], b[l], L.mg_def.p, -1.0,
+                1.0, stream_);
+    launch_fill(C.n(), 0.0, x[l - 1], stream_);
+    launch_csr(C.n(), L.r_rp.p, L.r_col.p, L.r_val.p, L.mg_def.p, C.mg_rhs.p, stream_);
+    launch_zero_tail(C.n(), C.n_active(), C.mg_rhs.p, stream_);
+  }
+  {
+    Level& C = *levels_[0];
+    if(n_levels() == 1) {
+      smooth(C, x[0], C.mg_sol2.p, b[0], sd_.n_pre + sd_.n_post);
+    } else {
+      launch_gemv(C.n(), coarse_inv_.p, b[0], x[0], stream_);
+    }
+  }
+  for(int l = 1; l <= Lf; ++l) {
+    Level& L = *levels_[l];
+    launch_csr(L.n(), L.p_rp.p, L.p_col.p, L.p_val.p, x[l - 1], L.mg_tmp.p, stream_);
+    launch_axpy_active(L.n_active(), sd_.mg_damp, L.mg_tmp.p, x[l], stream_);
+    smooth(L, x[l], L.mg_sol2.p, b[l], sd_.n_post);
+  }
+  launch_copy(finest().n(), x[Lf], out, stream_);
+}
+
+void CDSystem::vcycle32(const double* rhs, double* out) {
+  const int Lf = n_levels() - 1;
+  for(auto& Lp : levels_) {
+    Level& L = *Lp;
+    if(L.f_sol.n != (size_t)L.n()) {
+      L.f_sol.alloc(L.n()); L.f_sol2.alloc(L.n()); L.f_rhs.alloc(L.n());
+      L.f_def.alloc(L.n()); L.f_tmp.alloc(L.n());
+    }
+  }
+  Level& F = finest();
+  launch_d2f(F.n(), rhs, F.f_rhs.p, stream_);
+  launch_fill_f32(F.n_active(), 0.0f, F.f_sol.p, stream_);
+  PMCD_HIP(hipMemcpyAsync(F.f_sol.p + F.n_active(), F.f_rhs.p + F.n_active(),
+                          sizeof(float) * (F.n() - F.n_active()), hipMemcpyDeviceToDevice, stream_));
+  for(int l = Lf; l >= 1; --l) {
+    Level& L = *levels_[l];
+    Level& C = *levels_[l - 1];
+    smooth32(L, L.f_sol.p, L.f_sol2.p, L.f_rhs.p, sd_.n_pre);
+    launch_spmv_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.f_sol.p, L.f_rhs.p,
+                    L.f_def.p, -1.0f, 1.0f, stream_);
+    launch_fill_f32(C.n(), 0.0f, C.f_sol.p, stream_);
+    launch_csr_f32(C.n(), L.r_rp.p, L.r_col.p, L.r_val32.p, L.f_def.p, C.f_rhs.p, stream_);
+    launch_zero_tail_f32(C.n(), C.n_active(), C.f_rhs.p, stream_);
+  }
+  {
+    Level& C = *levels_[0];
+    if(n_levels() == 1) {
+      smooth32(C, C.f_sol.p, C.f_sol2.p, C.f_rhs.p, sd_.n_pre + sd_.n_post);
+    } else {
+      // coarse direct solve in FP64
+      launch_f2d(C.n(), C.f_rhs.p, coarse_tmp_.p, stream_);
+      launch_gemv(C.n(), coarse_inv_.p, coarse_tmp_.p, coarse_tmp_.p + C.n(), stream_);
+      launch_d2f(C.n(), coarse_tmp_.p + C.n(), C.f_sol.p, stream_);
+    }
+  }
+  for(int l = 1; l <= Lf; ++l) {
+    Level& L = *levels_[l];
+    launch_csr_f32(L.n(), L.p_rp.p, L.p_col.p, L.p_val32.p, levels_[l - 1]->f_sol.p, L.f_tmp.p,
+                   stream_);
+    launch_axpy_f32(L.n(), (float)sd_.mg_damp, L.f_tmp.p, L.f_sol.p, L.n_active(), stream_);
+    smooth32(L, L.f_sol.p, L.f_sol2.p, L.f_rhs.p, sd_.n_post);
+  }
+  launch_f2d(F.n(), F.f_sol.p, out, stream_);
+}
+
+void CDSystem::vcycle(const double* rhs, double* out) {
+  if(sd_.mixed) vcycle32(rhs, out);
+  else vcycle64(rhs, out);
+}
+
+// Preconditioner application: none (copy), Jacobi (r = z / diag,
+// Iteration_jacobi.C:168-182) or one multigrid cycle.
+void CDSystem::precond(const double* v, double* z) {
+  Level& F = finest();
+  switch(sd_.precond) {
+    case 0: launch_copy(F.n(), v, z, stream_); break;
+    case 1: launch_div(F.n(), v, F.diag.p, z, stream_); break;
+    default: vcycle(v, z); break;
+  }
+}
+
+}  // namespace pmcd

@@ -1,0 +1,144 @@
+// Device-resident convection-diffusion system: mesh hierarchy, FE spaces,
+// CSR matrices and right-hand sides per multigrid level, assembly, the
+// iterative solvers (CG, flexible GMRES) with Jacobi or geometric multigrid
+// preconditioning (FP64 or FP32 V-cycle), and error evaluation.
+//
+// Reference: ConvectionDiffusion<d> (src/System/ConvectionDiffusion.C:72-459)
+// owning one SystemPerGrid per level (finest first there, coarsest first
+// here), Solver<> (src/Solver/Solver.C:423-696), Iteration_cg
+// (Iteration_cg.C:27-147), Iteration_gmres::flexible_gmres
+// (Iteration_gmres.C:430-620), Multigrid (src/Multigrid/Multigrid.C:157-410).
+#pragma once
+#include <memory>
+#include "common.hpp"
+#include "mesh.hpp"
+#include "fespace.hpp"
+#include "structure.hpp"
+#include "quadrature.hpp"
+#include "device.hpp"
+#include "kernels.hpp"
+
+namespace pmcd {
+
+struct ProblemDesc {
+  int geometry;       // DefaultGeometry, or 10 = structured hexa cube, 11 = structured tetra cube
+  int n_refine;       // regular refinement steps of the default mesh
+  int structured_n;   // cells per edge for the structured cubes
+  int order;          // ansatz order 1 or 2
+  int supg;           // 0 Galerkin, 1 SUPG
+  int linear_quad;    // 1: quadrature degree 2p (INTERNAL_PROBLEM_LINEAR), else 3p
+  int mg_levels;      // number of levels kept (1 = finest only)
+  ExampleParams ex;
+};
+
+struct SolverDesc {
+  int method;          // 0 direct, 1 cg, 2 fgmres, 3 richardson
+  int precond;         // 0 none, 1 jacobi, 2 multigrid
+  int max_iter;
+  int min_iter;
+  double res_tol;      // residual_tolerance
+  double res_red;      // residual_reduction
+  double div_factor;   // divergence_factor
+  int restart;         // gmres_restart
+  int n_pre, n_post;   // multigrid smoothing steps
+  double jacobi_damp;  // jacobi_damp
+  double mg_damp;      // multigrid_correction_damp_factor
+  int mixed;           // 1: V-cycle in FP32
+};
+
+struct SolveStats {
+  int iterations = 0;
+  double initial_residual = 0;
+  double final_residual = 0;
+  int converged = 0;
+  double seconds = 0;
+};
+
+struct Level {
+  Mesh mesh;
+  FESpace space;
+  Structure st;
+  std::vector<double> dirichlet_host;
+  // device data
+  DevBuf<double> cell_xyz;
+  DevBuf<int> dof, row_ptr, col, diag_pos, contrib_ptr, contrib;
+  DevBuf<uint8_t> slot;
+  DevBuf<double> dir_vals, extra;
+  DevBuf<int> extra_ptr;
+  DevBuf<double> val, rhs, sol, diag;
+  DevBuf<float> val32, diag32;
+  // multigrid work vectors
+  DevBuf<double> mg_sol, mg_sol2, mg_rhs, mg_def, mg_tmp;
+  DevBuf<float> f_sol, f_sol2, f_rhs, f_def, f_tmp;
+  // transfer from the next coarser level: P (n x n_coarse), R = P^T
+  DevBuf<int> p_rp, p_col, r_rp, r_col;
+  DevBuf<double> p_val, r_val;
+  DevBuf<float> p_val32, r_val32;
+  int n_coarse = 0;
+  int n() const { return space.n_dof; }
+  int n_active() const { return space.n_active; }
+};
+
+// Host-side setup only (meshes, FE spaces, sparsity, slot maps): no GPU use.
+std::vector<std::unique_ptr<Level>> build_levels(const ProblemDesc& d);
+
+class CDSystem {
+ public:
+  CDSystem(const ProblemDesc& d, hipStream_t stream);
+  ~CDSystem();
+
+  void assemble();                        // every level
+  SolveStats solve(const SolverDesc& s);  // finest level
+  void errors(double out[5]);             // L2, H1-semi, SD, DG, Linf on the finest level
+
+  int n_levels() const { return (int)levels_.size(); }
+  Level& level(int l) { return *levels_[l]; }
+  Level& finest() { return *levels_.back(); }
+  const ProblemDesc& desc() const { return desc_; }
+  hipStream_t stream() const { return stream_; }
+  QuadRule cell_rule() const { return rule_; }
+  double last_assemble_ms = 0;
+
+  // building blocks (used by the solvers and exposed through the C ABI)
+  void apply(Level& L, const double* x, double* y);                 // y = A x
+  void apply_scaled_add(Level& L, const double* x, double* y, double a);  // y += a A x
+  double dot(int n, const double* x, const double* y);             // synchronous
+  void vcycle(const double* rhs, double* out);                     // FP64 or FP32 V-cycle
+  void set_solver(const SolverDesc& s) { sd_ = s; }
+  void prepare(const SolverDesc& s);      // coarse factorization, FP32 copies
+
+ private:
+  void build_level_device(Level& L);
+  void build_transfer(Level& fine, const Level& coarse);
+  void assemble_level(Level& L);
+  void coarse_factorize();
+  void smooth(Level& L, double* x, double* x2, const double* b, int steps);
+  void smooth32(Level& L, float* x, float* x2, const float* b, int steps);
+  void vcycle64(const double* rhs, double* out);
+  void vcycle32(const double* rhs, double* out);
+  void precond(const double* v, double* z);
+  SolveStats solve_cg();
+  SolveStats solve_fgmres();
+  SolveStats solve_direct();
+  SolveStats solve_richardson();
+
+  ProblemDesc desc_;
+  SolverDesc sd_{};
+  hipStream_t stream_;
+  std::vector<std::unique_ptr<Level>> levels_;  // coarsest first
+  QuadRule rule_;
+  int ncomp_ = 0;
+  DevBuf<double> ref_, qw_, qp_;
+  DevBuf<double> local_A_, local_rhs_;
+  DevBuf<double> red_partial_, red_out_, cell_err_;
+  // dense inverse of the coarsest matrix (direct coarse solve)
+  DevBuf<double> coarse_inv_, coarse_tmp_;
+  bool coarse_ready_ = false;
+  bool mixed_ready_ = false;
+  // Krylov work space
+  std::vector<DevBuf<double>> kv_, kz_;
+  DevBuf<double> kr_, kw_, kh_;
+  AsmArgs asm_args(const Level& L);
+};
+
+}  // namespace pmcd

@@ -1,0 +1,1750 @@
+/*
+ * ORACLE -- test infrastructure only. CPU restatement of the reference's
+ * convection-diffusion assemble-and-solve path, used by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker and
+ * baseline; never part of the shipped compute path.
+ *
+ * It follows the reference's algorithms and operation order, in code of its
+ * own:
+ *   mesh + regular refinement, depth-first cell order: src/Geometry/Tests.C,
+ *     src/Refinement/Ref{Tri,Quad,Tetra,Hexa}RegDesc.C, It_EQ.C:16-51
+ *   DOF numbering by union-find over shared joints + renumbering:
+ *     src/FE/FESpace3D.C:186-804, FEMapper.C:456-600, FESpace.C:1863-2099
+ *   sparsity: src/Matrix/Structure.C:226-618
+ *   assembly: Assemble3D.C:183-549, Assemble2D.C:2798-3255,
+ *     LocalAssembling.C:734-970, ConvDiff.C:269-340,1776-1999,
+ *     HexaAffin.C/TetraAffin.C/TriaAffin.C/QuadAffin.C
+ *   BlockFEMatrix::apply_scaled_add: BlockFEMatrix.C:1008-1095, FEMatrix.C:209-223
+ *   CG: Iteration_cg.C:27-147; FGMRES: Iteration_gmres.C:430-620,
+ *     Utilities_gmres.C:6-108; stopping test: IterativeMethod.h:83-162
+ *   multigrid V-cycle: Multigrid.C:157-410, MultigridLevel.C:118-151,
+ *     JacobiSmoother.C:22-30, GridTransfer.C:603-1115 (cell-wise
+ *     prolongation/restriction with multiplicity averaging)
+ *   errors: FEFunction3D.C:60-248, ConvDiff.C:2124-2183
+ * Serial sums everywhere (Blas1.C:21-98). Built with -ffp-contract=off.
+ * The coarse/direct solve uses dense LU with partial pivoting in place of
+ * UMFPACK (not present here); both solve the same system exactly up to
+ * rounding.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAXNL 27
+#define MAXQ 64
+
+/* ----------------------------------------------------------------------- */
+/* shapes                                                                   */
+/* ----------------------------------------------------------------------- */
+enum { TRI = 0, QUAD = 1, TET = 2, HEX = 3 };
+static const int NV[4] = {3, 4, 4, 8};
+static const int NJ[4] = {3, 4, 4, 6};
+static const int DIM[4] = {2, 2, 3, 3};
+/* joint vertex lists (Triangle.C/Quadrangle.C edges, Tetrahedron.C:18-19,
+   Hexahedron.C:20-22) */
+static const int JV_TRI[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+static const int JV_QUAD[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+static const int JV_TET[4][3] = {{0, 1, 2}, {0, 3, 1}, {2, 1, 3}, {0, 2, 3}};
+static const int JV_HEX[6][4] = {{0, 1, 2, 3}, {0, 4, 5, 1}, {1, 5, 6, 2},
+                                 {2, 6, 7, 3}, {0, 3, 7, 4}, {4, 7, 6, 5}};
+static int jv_len(int s) { return s == TRI || s == QUAD ? 2 : (s == TET ? 3 : 4); }
+static int jv(int s, int j, int k) {
+  switch(s) {
+    case TRI: return JV_TRI[j][k];
+    case QUAD: return JV_QUAD[j][k];
+    case TET: return JV_TET[j][k];
+    default: return JV_HEX[j][k];
+  }
+}
+
+/* ----------------------------------------------------------------------- */
+/* mesh                                                                     */
+/* ----------------------------------------------------------------------- */
+typedef struct {
+  int shape, ncells, nverts, cap;
+  double* X;   /* 3 per vertex */
+  int* cv;     /* NV per cell */
+  int* parent; /* cell in the coarser mesh */
+} omesh;
+
+/* exact-coordinate vertex dictionary (all coordinates here are dyadic) */
+typedef struct {
+  int cap;
+  int* slot;
+} vdict;
+
+static uint64_t hash3(const double* x) {
+  uint64_t h = 1469598103934665603ull, b;
+  for(int k = 0; k < 3; ++k) {
+    memcpy(&b, &x[k], 8);
+    h ^= b;
+    h *= 1099511628211ull;
+    h ^= h >> 29;
+  }
+  return h;
+}
+
+static int vertex_id(omesh* m, vdict* d, const double* x) {
+  uint64_t h = hash3(x) & (uint64_t)(d->cap - 1);
+  for(;;) {
+    int v = d->slot[h];
+    if(v < 0) break;
+    if(m->X[3 * v] == x[0] && m->X[3 * v + 1] == x[1] && m->X[3 * v + 2] == x[2]) return v;
+    h = (h + 1) & (uint64_t)(d->cap - 1);
+  }
+  if(m->nverts == m->cap) {
+    m->cap = m->cap * 2 + 16;
+    m->X = realloc(m->X, sizeof(double) * 3 * m->cap);
+  }
+  int v = m->nverts++;
+  memcpy(&m->X[3 * v], x, 3 * sizeof(double));
+  d->slot[h] = v;
+  return v;
+}
+
+static void mesh_free(omesh* m) {
+  free(m->X);
+  free(m->cv);
+  free(m->parent);
+}
+
+static void default_mesh(int geometry, omesh* m) {
+  static const double sq[4][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}};
+  static const double cube[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                    {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  memset(m, 0, sizeof(*m));
+  const int* cells = NULL;
+  static const int c_sq[4] = {0, 1, 2, 3};
+  static const int c_tt[6] = {0, 1, 2, 0, 2, 3};
+  static const int c_tt2[6] = {0, 1, 3, 1, 2, 3};
+  static const int c_hex[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+  static const int c_tet[24] = {0, 1, 2, 6, 0, 5, 1, 6, 0, 2, 3, 6, 0, 3, 7, 6, 0, 4, 5, 6, 0, 7, 4, 6};
+  int nc = 0, nvx = 0;
+  switch(geometry) {
+    case 0: m->shape = QUAD; cells = c_sq; nc = 1; nvx = 4; break;
+    case 1: m->shape = TRI; cells = c_tt; nc = 2; nvx = 4; break;
+    case 2: m->shape = TRI; cells = c_tt2; nc = 2; nvx = 4; break;
+    case 3: m->shape = HEX; cells = c_hex; nc = 1; nvx = 8; break;
+    default: m->shape = TET; cells = c_tet; nc = 6; nvx = 8; break;
+  }
+  m->nverts = nvx;
+  m->cap = nvx;
+  m->X = malloc(sizeof(double) * 3 * nvx);
+  for(int v = 0; v < nvx; ++v)
+    for(int k = 0; k < 3; ++k) m->X[3 * v + k] = nvx == 4 ? sq[v][k] : cube[v][k];
+  m->ncells = nc;
+  m->cv = malloc(sizeof(int) * nc * NV[m->shape]);
+  memcpy(m->cv, cells, sizeof(int) * nc * NV[m->shape]);
+  m->parent = malloc(sizeof(int) * nc);
+  for(int c = 0; c < nc; ++c) m->parent[c] = -1;
+}
+
+/* child vertex lists in terms of the parent's refined vertices; refined
+   vertex r is the barycentre of the parent vertices listed in span */
+typedef struct {
+  int nref, nchild;
+  int span[27][9]; /* span[r][0] = count */
+  int child[8][8];
+} reftab;
+
+static void set_span(reftab* t, int r, int n, const int* v) {
+  t->span[r][0] = n;
+  for(int i = 0; i < n; ++i) t->span[r][1 + i] = v[i];
+}
+
+static reftab make_reftab(int shape) {
+  reftab t;
+  memset(&t, 0, sizeof(t));
+  if(shape == TRI) {
+    /* RefTriRegDesc.C: 3 = mid(0,1), 4 = mid(1,2), 5 = mid(2,0) */
+    const int e[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+    const int ch[4][3] = {{0, 3, 5}, {1, 4, 3}, {2, 5, 4}, {3, 4, 5}};
+    t.nref = 6; t.nchild = 4;
+    for(int v = 0; v < 3; ++v) set_span(&t, v, 1, &v);
+    for(int k = 0; k < 3; ++k) set_span(&t, 3 + k, 2, e[k]);
+    for(int c = 0; c < 4; ++c) memcpy(t.child[c], ch[c], sizeof(ch[c]));
+  } else if(shape == QUAD) {
+    /* RefQuadRegDesc.C */
+    const int e[4][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}};
+    const int all[4] = {0, 1, 2, 3};
+    const int ch[4][4] = {{0, 4, 8, 7}, {1, 5, 8, 4}, {2, 6, 8, 5}, {3, 7, 8, 6}};
+    t.nref = 9; t.nchild = 4;
+    for(int v = 0; v < 4; ++v) set_span(&t, v, 1, &v);
+    for(int k = 0; k < 4; ++k) set_span(&t, 4 + k, 2, e[k]);
+    set_span(&t, 8, 4, all);
+    for(int c = 0; c < 4; ++c) memcpy(t.child[c], ch[c], sizeof(ch[c]));
+  } else if(shape == TET) {
+    /* RefTetraRegDesc.C:15-16 and its edge-vertex table */
+    const int e[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+    const int ch[8][4] = {{0, 4, 6, 7}, {4, 1, 5, 8}, {6, 5, 2, 9}, {7, 8, 9, 3},
+                          {4, 6, 7, 8}, {5, 6, 4, 8}, {6, 7, 8, 9}, {8, 5, 6, 9}};
+    t.nref = 10; t.nchild = 8;
+    for(int v = 0; v < 4; ++v) set_span(&t, v, 1, &v);
+    for(int k = 0; k < 6; ++k) set_span(&t, 4 + k, 2, e[k]);
+    for(int c = 0; c < 8; ++c) memcpy(t.child[c], ch[c], sizeof(ch[c]));
+  } else {
+    /* RefHexaRegDesc.C:17-23 and its edge/face vertex tables */
+    static const int spans[27][9] = {
+        {1, 0}, {1, 1}, {1, 2}, {1, 3}, {1, 4}, {1, 5}, {1, 6}, {1, 7},
+        {2, 0, 1}, {2, 1, 2}, {2, 2, 3}, {2, 3, 0}, {4, 0, 1, 2, 3},
+        {2, 1, 5}, {2, 5, 4}, {2, 4, 0}, {4, 0, 1, 5, 4},
+        {2, 2, 6}, {2, 6, 5}, {4, 1, 2, 6, 5},
+        {2, 3, 7}, {2, 7, 6}, {4, 2, 3, 7, 6},
+        {2, 4, 7}, {4, 3, 0, 4, 7}, {4, 4, 5, 6, 7},
+        {8, 0, 1, 2, 3, 4, 5, 6, 7}};
+    const int ch[8][8] = {{0, 8, 12, 11, 15, 16, 26, 24}, {1, 9, 12, 8, 13, 19, 26, 16},
+                          {2, 10, 12, 9, 17, 22, 26, 19}, {3, 11, 12, 10, 20, 24, 26, 22},
+                          {4, 23, 25, 14, 15, 24, 26, 16}, {5, 14, 25, 18, 13, 16, 26, 19},
+                          {6, 18, 25, 21, 17, 19, 26, 22}, {7, 21, 25, 23, 20, 22, 26, 24}};
+    t.nref = 27; t.nchild = 8;
+    memcpy(t.span, spans, sizeof(spans));
+    for(int c = 0; c < 8; ++c) memcpy(t.child[c], ch[c], sizeof(ch[c]));
+  }
+  return t;
+}
+
+static void refine(const omesh* cm, omesh* fm) {
+  reftab t = make_reftab(cm->shape);
+  const int nv = NV[cm->shape];
+  memset(fm, 0, sizeof(*fm));
+  fm->shape = cm->shape;
+  fm->ncells = cm->ncells * t.nchild;
+  fm->cv = malloc(sizeof(int) * fm->ncells * nv);
+  fm->parent = malloc(sizeof(int) * fm->ncells);
+  fm->cap = cm->nverts * 8 + 64;
+  fm->X = malloc(sizeof(double) * 3 * fm->cap);
+  vdict d;
+  d.cap = 1;
+  while(d.cap < 4 * (cm->nverts * 8 + 64)) d.cap <<= 1;
+  d.slot = malloc(sizeof(int) * d.cap);
+  for(int i = 0; i < d.cap; ++i) d.slot[i] = -1;
+  for(int c = 0; c < cm->ncells; ++c) {
+    int gid[27];
+    for(int r = 0; r < t.nref; ++r) {
+      double x[3] = {0, 0, 0};
+      const int n = t.span[r][0];
+      for(int i = 0; i < n; ++i)
+        for(int k = 0; k < 3; ++k) x[k] += cm->X[3 * cm->cv[c * nv + t.span[r][1 + i]] + k];
+      for(int k = 0; k < 3; ++k) x[k] /= n;
+      gid[r] = vertex_id(fm, &d, x);
+    }
+    for(int ch = 0; ch < t.nchild; ++ch) {
+      const int fc = c * t.nchild + ch;
+      for(int a = 0; a < nv; ++a) fm->cv[fc * nv + a] = gid[t.child[ch][a]];
+      fm->parent[fc] = c;
+    }
+  }
+  free(d.slot);
+}
+
+/* boundary component of a joint of the unit box from its vertices
+   (Default_UnitSquare / Default_UnitCube parameter files) */
+static int box_comp(int dim, const double* xs, int n) {
+  for(int plane = 0; plane < (dim == 2 ? 4 : 6); ++plane) {
+    int ok = 1;
+    for(int i = 0; i < n && ok; ++i) {
+      const double* x = xs + 3 * i;
+      if(dim == 2) {
+        const double v[4] = {x[1], x[0] - 1, x[1] - 1, x[0]};
+        ok = v[plane] == 0.0;
+      } else {
+        const double v[6] = {x[2], x[0], x[1], x[2] - 1, x[0] - 1, x[1] - 1};
+        ok = v[plane] == 0.0;
+      }
+    }
+    if(ok) return plane;
+  }
+  return -1;
+}
+
+/* ----------------------------------------------------------------------- */
+/* elements: node sets (FE descriptor order) and Lagrange bases             */
+/* ----------------------------------------------------------------------- */
+typedef struct {
+  int shape, order, nl, dim;
+  double node[MAXNL][3];
+  int simplex_alpha[MAXNL][4]; /* barycentric multi-index (simplices) */
+  int tensor_idx[MAXNL][3];    /* 1D node indices (tensor elements) */
+} oelem;
+
+static oelem make_elem(int shape, int order) {
+  oelem e;
+  memset(&e, 0, sizeof(e));
+  e.shape = shape;
+  e.order = order;
+  e.dim = DIM[shape];
+  if(shape == QUAD || shape == HEX) {
+    const int n1 = order + 1, nz = e.dim == 3 ? n1 : 1;
+    int i = 0;
+    for(int c = 0; c < nz; ++c)
+      for(int b = 0; b < n1; ++b)
+        for(int a = 0; a < n1; ++a, ++i) {
+          e.tensor_idx[i][0] = a;
+          e.tensor_idx[i][1] = b;
+          e.tensor_idx[i][2] = c;
+          e.node[i][0] = -1.0 + 2.0 * a / order;
+          e.node[i][1] = -1.0 + 2.0 * b / order;
+          e.node[i][2] = e.dim == 3 ? -1.0 + 2.0 * c / order : 0.0;
+        }
+    e.nl = i;
+  } else {
+    /* P1: vertices; P2 (FE_C_T_P2_2D.h / FE_C_T_P2_3D.h layout): nodes in
+       lexicographic order of (zeta, eta, xi) on the grid of step 1/2 */
+    int i = 0;
+    const int p = order;
+    const int nz = e.dim == 3 ? p : 0;
+    for(int c = 0; c <= nz; ++c)
+      for(int b = 0; b <= p - c; ++b)
+        for(int a = 0; a <= p - b - c; ++a, ++i) {
+          e.node[i][0] = (double)a / p;
+          e.node[i][1] = (double)b / p;
+          e.node[i][2] = (double)c / p;
+          e.simplex_alpha[i][1] = a;
+          e.simplex_alpha[i][2] = b;
+          e.simplex_alpha[i][3] = c;
+          e.simplex_alpha[i][0] = p - a - b - c;
+        }
+    e.nl = i;
+  }
+  return e;
+}
+
+/* 1D Lagrange polynomial through equidistant nodes of [-1,1]: value and
+   first two derivatives, by the product formula */
+static void lag1d(int order, int a, double x, double* v, double* d1, double* d2) {
+  double nodes[3];
+  for(int k = 0; k <= order; ++k) nodes[k] = -1.0 + 2.0 * k / order;
+  double val = 1, der = 0, der2 = 0;
+  /* expand product incrementally: (f g)' = f' g + f g', (f g)'' = f'' g + 2 f' g' + f g'' */
+  for(int k = 0; k <= order; ++k) {
+    if(k == a) continue;
+    const double den = nodes[a] - nodes[k];
+    const double g = (x - nodes[k]) / den, gd = 1.0 / den;
+    der2 = der2 * g + 2 * der * gd;
+    der = der * g + val * gd;
+    val = val * g;
+  }
+  *v = val;
+  *d1 = der;
+  *d2 = der2;
+}
+
+/* Silvester's factor P_a(l) = prod_{k<a} (p l - k)/(k+1) with derivatives */
+static void silv(int p, int a, double l, double* v, double* d1, double* d2) {
+  double val = 1, der = 0, der2 = 0;
+  for(int k = 0; k < a; ++k) {
+    const double g = (p * l - k) / (k + 1), gd = (double)p / (k + 1);
+    der2 = der2 * g + 2 * der * gd;
+    der = der * g + val * gd;
+    val = val * g;
+  }
+  *v = val;
+  *d1 = der;
+  *d2 = der2;
+}
+
+/* out[c*nl+i], c over D0, first derivatives, then second derivatives in
+   multi-index order (2D: xx, xy, yy; 3D: xx, xy, xz, yy, yz, zz) */
+static void elem_eval(const oelem* e, const double* p, double* out) {
+  const int nl = e->nl, d = e->dim;
+  const int pr2[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+  const int pr3[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+  const int n2 = d == 2 ? 3 : 6;
+  for(int i = 0; i < nl; ++i) {
+    double grad[3] = {0, 0, 0}, hess[6] = {0, 0, 0, 0, 0, 0}, val;
+    if(e->shape == QUAD || e->shape == HEX) {
+      double v[3], a1[3], a2[3];
+      for(int k = 0; k < d; ++k) lag1d(e->order, e->tensor_idx[i][k], p[k], &v[k], &a1[k], &a2[k]);
+      val = 1;
+      for(int k = 0; k < d; ++k) val *= v[k];
+      for(int k = 0; k < d; ++k) {
+        double g = a1[k];
+        for(int m = 0; m < d; ++m)
+          if(m != k) g *= v[m];
+        grad[k] = g;
+      }
+      for(int t = 0; t < n2; ++t) {
+        const int k = d == 2 ? pr2[t][0] : pr3[t][0], l = d == 2 ? pr2[t][1] : pr3[t][1];
+        double h = 1;
+        for(int m = 0; m < d; ++m) {
+          if(k == l && m == k) h *= a2[m];
+          else if(m == k || m == l) h *= a1[m];
+          else h *= v[m];
+        }
+        hess[t] = h;
+      }
+    } else {
+      double lam[4], f[4], f1[4], f2[4];
+      lam[0] = 1;
+      for(int k = 0; k < d; ++k) { lam[0] -= p[k]; lam[k + 1] = p[k]; }
+      for(int j = 0; j <= d; ++j) silv(e->order, e->simplex_alpha[i][j], lam[j], &f[j], &f1[j], &f2[j]);
+      /* d lam_j / d xi_k */
+      double g[4][3] = {{0}};
+      for(int k = 0; k < d; ++k) { g[0][k] = -1; g[k + 1][k] = 1; }
+      val = 1;
+      for(int j = 0; j <= d; ++j) val *= f[j];
+      double dl[4], ddl[4][4];
+      for(int j = 0; j <= d; ++j) {
+        double t = f1[j];
+        for(int m = 0; m <= d; ++m)
+          if(m != j) t *= f[m];
+        dl[j] = t;
+        for(int jj = 0; jj <= d; ++jj) {
+          double h = 1;
+          for(int m = 0; m <= d; ++m) {
+            if(j == jj && m == j) h *= f2[m];
+            else if(m == j || m == jj) h *= f1[m];
+            else h *= f[m];
+          }
+          ddl[j][jj] = h;
+        }
+      }
+      for(int k = 0; k < d; ++k) {
+        double s = 0;
+        for(int j = 0; j <= d; ++j) s += g[j][k] * dl[j];
+        grad[k] = s;
+      }
+      for(int t = 0; t < n2; ++t) {
+        const int k = d == 2 ? pr2[t][0] : pr3[t][0], l = d == 2 ? pr2[t][1] : pr3[t][1];
+        double s = 0;
+        for(int j = 0; j <= d; ++j)
+          for(int jj = 0; jj <= d; ++jj) s += g[j][k] * g[jj][l] * ddl[j][jj];
+        hess[t] = s;
+      }
+    }
+    out[i] = val;
+    for(int k = 0; k < d; ++k) out[(1 + k) * nl + i] = grad[k];
+    for(int t = 0; t < n2; ++t) out[(1 + d + t) * nl + i] = hess[t];
+  }
+}
+
+/* ----------------------------------------------------------------------- */
+/* quadrature (QuadFormula.C data, degree map QuadratureFormulaDatabase.C)  */
+/* ----------------------------------------------------------------------- */
+typedef struct {
+  int n;
+  double w[MAXQ], x[MAXQ][3];
+} oquad;
+
+static oquad make_quad(int shape, int degree) {
+  oquad q;
+  memset(&q, 0, sizeof(q));
+  if(shape == TRI) {
+    if(degree <= 5) {
+      const double a = 0.797426985353087322398025276169754, b = 0.101286507323456338800987361915123;
+      const double c = 0.059715871789769820459117580973106, dd = 0.470142064105115089770441209513447;
+      const double wa = 0.0629695902724135762978419727500906, wc = 0.0661970763942530903688246939165759;
+      const double W[7] = {0.1125, wa, wa, wa, wc, wc, wc};
+      const double X[7] = {0.333333333333333333333333333333333, a, b, b, c, dd, dd};
+      const double Y[7] = {0.333333333333333333333333333333333, b, a, b, dd, c, dd};
+      q.n = 7;
+      for(int i = 0; i < 7; ++i) { q.w[i] = W[i]; q.x[i][0] = X[i]; q.x[i][1] = Y[i]; }
+    } else {
+      const double w0 = 0.0485678981413994169096209912536443, w1 = 0.0156673501135695352684274156436046,
+                   w2 = 0.0389137705023871396583696781497019, w3 = 0.0398238694636051265164458871320226,
+                   w4 = 0.0127888378293490156308393992794999, w5 = 0.0216417696886446886446886446886446;
+      const double a1 = 0.489682519198737627783706924836192, b1 = 0.020634961602524744432586150327616;
+      const double a2 = 0.437089591492936637269930364435354, b2 = 0.125820817014126725460139271129292;
+      const double a3 = 0.188203535619032730240961280467335, b3 = 0.623592928761934539518077439065330;
+      const double a4 = 0.0447295133944527098651065899662763, b4 = 0.910540973211094580269786820067447;
+      const double p = 0.741198598784498020690079873523423, r = 0.0368384120547362836348175987833851,
+                   s = 0.221962989160765695675102527693192, th = 0.333333333333333333333333333333333;
+      const double W[19] = {w0, w1, w1, w1, w2, w2, w2, w3, w3, w3, w4, w4, w4, w5, w5, w5, w5, w5, w5};
+      const double X[19] = {th, a1, a1, b1, a2, a2, b2, a3, a3, b3, a4, a4, b4, p, r, p, s, r, s};
+      const double Y[19] = {th, a1, b1, a1, a2, b2, a2, a3, b3, a3, a4, b4, a4, r, p, s, p, s, r};
+      q.n = 19;
+      for(int i = 0; i < 19; ++i) { q.w[i] = W[i]; q.x[i][0] = X[i]; q.x[i][1] = Y[i]; }
+    }
+  } else if(shape == QUAD) {
+    if(degree <= 2) {
+      const double g = 0.57735026918962576450914878050195746;
+      const double X[4] = {-g, g, -g, g}, Y[4] = {-g, -g, g, g};
+      q.n = 4;
+      for(int i = 0; i < 4; ++i) { q.w[i] = 1; q.x[i][0] = X[i]; q.x[i][1] = Y[i]; }
+    } else if(degree <= 5) {
+      const double g = 0.77459666924148337703585307995647992;
+      const double wa = 0.30864197530864197530864197530864198, wb = 0.49382716049382716049382716049382717,
+                   wc = 0.79012345679012345679012345679012346;
+      const double W[9] = {wa, wb, wa, wb, wc, wb, wa, wb, wa};
+      const double X[9] = {-g, 0., g, -g, 0., g, -g, 0., g}, Y[9] = {-g, -g, -g, 0., 0., 0., g, g, g};
+      q.n = 9;
+      for(int i = 0; i < 9; ++i) { q.w[i] = W[i]; q.x[i][0] = X[i]; q.x[i][1] = Y[i]; }
+    } else {
+      const double a = 0.86113631159405257522394648889280951, b = 0.33998104358485626480266575910324469;
+      const double wa = 0.12100299328560200552121209737014755, wb = 0.22685185185185185185185185185185185,
+                   wc = 0.42529330301069429077508419892614875;
+      const double W[16] = {wa, wb, wb, wa, wb, wc, wc, wb, wb, wc, wc, wb, wa, wb, wb, wa};
+      const double x1[4] = {-a, -b, b, a};
+      q.n = 16;
+      for(int i = 0; i < 16; ++i) { q.w[i] = W[i]; q.x[i][0] = x1[i % 4]; q.x[i][1] = x1[i / 4]; }
+    }
+  } else if(shape == TET) {
+    if(degree <= 2) {
+      const double a = 0.138196601125010515179541316563, b = 0.585410196624968454461376050311;
+      const double X[4] = {a, a, a, b}, Y[4] = {a, a, b, a}, Z[4] = {a, b, a, a};
+      q.n = 4;
+      for(int i = 0; i < 4; ++i) { q.w[i] = 1. / 24.; q.x[i][0] = X[i]; q.x[i][1] = Y[i]; q.x[i][2] = Z[i]; }
+    } else if(degree <= 4) {
+      const double w0 = -.0131555555555555555555555555556, w1 = .00762222222222222222222222222222,
+                   w2 = .0248888888888888888888888888889;
+      const double s = .0714285714285714285714285714286, t = .785714285714285714285714285714;
+      const double u = .399403576166799204996102147462, v = .100596423833200795003897852538;
+      const double W[11] = {w0, w1, w1, w1, w1, w2, w2, w2, w2, w2, w2};
+      const double X[11] = {.25, s, t, s, s, u, u, v, v, v, u};
+      const double Y[11] = {.25, s, s, t, s, u, v, u, v, u, v};
+      const double Z[11] = {.25, s, s, s, t, v, u, u, u, v, v};
+      q.n = 11;
+      for(int i = 0; i < 11; ++i) { q.w[i] = W[i]; q.x[i][0] = X[i]; q.x[i][1] = Y[i]; q.x[i][2] = Z[i]; }
+    } else {
+      const double w0 = 0.602678571428571428571428571428571e-2, w1 = 0.302836780970891758063769725577305e-1,
+                   w2 = 0.116452490860289694108936091443380e-1, w3 = 0.109491415613864593456430191124068e-1;
+      const double th = 0.333333333333333333333333333333333;
+      const double a = 0.909090909090909090909090909090909e-1, b = 0.727272727272727272727272727272727;
+      const double c = 0.665501535736642982398804642263025e-1, d = 0.433449846426335701760119535773697;
+      const double W[15] = {w0, w0, w0, w0, w1, w2, w2, w2, w2, w3, w3, w3, w3, w3, w3};
+      const double X[15] = {th, th, 0, th, 0.25, a, a, b, a, c, d, d, c, c, d};
+      const double Y[15] = {th, 0, th, th, 0.25, a, b, a, a, d, c, d, c, d, c};
+      const double Z[15] = {0, th, th, th, 0.25, b, 0.90909090909090909090909090909091e-1,
+                            0.909090909090909090909090909090911e-1, 0.909090909090909090909090909090911e-1,
+                            0.433449846426335701760119535773699, 0.433449846426335701760119535773699,
+                            0.66550153573664298239880464226304e-1, 0.433449846426335701760119535773699,
+                            0.66550153573664298239880464226304e-1, 0.665501535736642982398804642263035e-1};
+      q.n = 15;
+      for(int i = 0; i < 15; ++i) { q.w[i] = W[i]; q.x[i][0] = X[i]; q.x[i][1] = Y[i]; q.x[i][2] = Z[i]; }
+    }
+  } else {
+    double w1[4], x1[4];
+    int n1;
+    if(degree <= 2) {
+      const double g = 0.5773502691896257645091489;
+      n1 = 2; w1[0] = w1[1] = 1.0; x1[0] = -g; x1[1] = g;
+    } else if(degree <= 5) {
+      n1 = 3;
+      w1[0] = 5. / 9.; w1[1] = 8. / 9.; w1[2] = 5. / 9.;
+      x1[0] = -0.7745966692414833770358530; x1[1] = 0; x1[2] = 0.7745966692414833770358530;
+    } else {
+      n1 = 4;
+      w1[0] = 0.652145154862546142626936051; w1[1] = 0.347854845137453857373063949;
+      w1[2] = 0.347854845137453857373063949; w1[3] = 0.652145154862546142626936051;
+      x1[0] = 0.33998104358485626480266576; x1[1] = 0.86113631159405257522394649;
+      x1[2] = -0.86113631159405257522394649; x1[3] = -0.33998104358485626480266576;
+    }
+    q.n = 0;
+    for(int i = 0; i < n1; ++i)
+      for(int j = 0; j < n1; ++j)
+        for(int k = 0; k < n1; ++k) {
+          q.w[q.n] = w1[i] * w1[j] * w1[k];
+          q.x[q.n][0] = x1[k];
+          q.x[q.n][1] = x1[j];
+          q.x[q.n][2] = x1[i];
+          q.n++;
+        }
+  }
+  return q;
+}
+
+/* ----------------------------------------------------------------------- */
+/* examples (problem data)                                                  */
+/* ----------------------------------------------------------------------- */
+typedef struct {
+  int kind;
+  double eps, b[3], c, f0;
+} oexample;
+
+/* coeffs: eps, b1, b2, b3, c, f */
+static void ex_coeffs(const oexample* ex, double x, double y, double z, double* o) {
+  const double p = M_PI;
+  switch(ex->kind) {
+    case 0:
+      o[0] = ex->eps; o[1] = o[2] = o[3] = 0; o[4] = 0;
+      o[5] = -o[0] * (-2 * p * p * sin(p * x) * sin(p * y));
+      break;
+    case 1: {
+      double ux = p * cos(p * x) * sin(p * y), uy = p * sin(p * x) * cos(p * y);
+      o[0] = ex->eps; o[1] = ex->b[0]; o[2] = ex->b[1]; o[3] = 0; o[4] = 0;
+      o[5] = -o[0] * (-2 * p * p * sin(p * x) * sin(p * y)) + o[1] * ux + o[2] * uy;
+      break;
+    }
+    case 2: {
+      double sx = sin(p * x), sy = sin(p * y), sz = sin(p * z);
+      o[0] = ex->eps; o[1] = ex->b[0]; o[2] = ex->b[1]; o[3] = ex->b[2]; o[4] = 0;
+      o[5] = 3 * p * p * ex->eps * (sx * sy * sz) + ex->b[0] * (p * cos(p * x) * sy * sz) +
+             ex->b[1] * (p * sx * cos(p * y) * sz) + ex->b[2] * (p * sx * sy * cos(p * z));
+      break;
+    }
+    case 3: o[0] = 1; o[1] = 1; o[2] = 1; o[3] = 1; o[4] = 1; o[5] = 1; break;
+    case 4: o[0] = 1; o[1] = 1; o[2] = 2; o[3] = 3; o[4] = 0; o[5] = 20; break;
+    case 5:
+      o[0] = 1; o[1] = 4; o[2] = 3; o[3] = 2; o[4] = 1;
+      o[5] = -1 * 2 + 4 * 2 * x + 3 * z + 2 * y + 1 * (x * x + y * z);
+      break;
+    default:
+      o[0] = ex->eps; o[1] = ex->b[0]; o[2] = ex->b[1]; o[3] = ex->b[2]; o[4] = ex->c; o[5] = ex->f0;
+  }
+}
+
+/* exact: u, ux, uy, uz; returns 0 if unknown */
+static int ex_exact(const oexample* ex, double x, double y, double z, double* o) {
+  const double p = M_PI;
+  switch(ex->kind) {
+    case 0:
+    case 1:
+      o[0] = sin(p * x) * sin(p * y); o[1] = p * cos(p * x) * sin(p * y);
+      o[2] = p * sin(p * x) * cos(p * y); o[3] = 0;
+      return 1;
+    case 2:
+      o[0] = sin(p * x) * sin(p * y) * sin(p * z); o[1] = p * cos(p * x) * sin(p * y) * sin(p * z);
+      o[2] = p * sin(p * x) * cos(p * y) * sin(p * z); o[3] = p * sin(p * x) * sin(p * y) * cos(p * z);
+      return 1;
+    case 3: o[0] = 1; o[1] = o[2] = o[3] = 0; return 1;
+    case 4: o[0] = 1 + 2 * x + 3 * y + 4 * z; o[1] = 2; o[2] = 3; o[3] = 4; return 1;
+    case 5: o[0] = x * x + y * z; o[1] = 2 * x; o[2] = z; o[3] = y; return 1;
+    default: o[0] = o[1] = o[2] = o[3] = 0; return 0;
+  }
+}
+
+static int ex_bc(const oexample* ex, int comp) { return (ex->kind == 0 && comp == 1) ? 1 : 0; }
+
+static double ex_bvalue(const oexample* ex, int comp, double x, double y, double z) {
+  double e[4];
+  if(!ex_exact(ex, x, y, z, e)) return 0;
+  if(ex->kind == 0 && comp == 1) return ex->eps * (1.0 * e[1] + 0.0 * e[2]);
+  return e[0];
+}
+
+/* ----------------------------------------------------------------------- */
+/* level: mesh + space + matrix                                             */
+/* ----------------------------------------------------------------------- */
+typedef struct {
+  omesh m;
+  int nl, ndof, nactive, ndir;
+  int* dof; /* GlobalNumbers */
+  int* rp;
+  int* col;
+  long nnz;
+  double* val;
+  double* rhs;
+  double* sol;
+  /* multigrid work */
+  double *mg_sol, *mg_rhs, *mg_def, *mg_aux;
+  double* lu; /* dense LU of the coarse/direct matrix */
+  int* piv;
+  int* nb_cell; /* face neighbours: nb_cell[c*NJ+j], -1 boundary */
+  int* nb_joint;
+  int* comp;    /* boundary component per joint, -1 interior */
+} olevel;
+
+typedef struct orc_problem {
+  int nlev;
+  olevel* lev; /* 0 = coarsest */
+  oelem el;
+  oquad qd;
+  oexample ex;
+  int supg;
+} orc_problem;
+
+/* face neighbours by matching sorted vertex ids of joints */
+typedef struct { int v[4]; int cell, joint; } facekey;
+static int fk_cmp(const void* a, const void* b) {
+  const facekey* x = a;
+  const facekey* y = b;
+  for(int k = 0; k < 4; ++k)
+    if(x->v[k] != y->v[k]) return x->v[k] < y->v[k] ? -1 : 1;
+  if(x->cell != y->cell) return x->cell < y->cell ? -1 : 1;
+  return x->joint - y->joint;
+}
+
+static void build_neighbours(olevel* L) {
+  const omesh* m = &L->m;
+  const int s = m->shape, nj = NJ[s], nv = NV[s], len = jv_len(s);
+  const int n = m->ncells * nj;
+  facekey* fk = malloc(sizeof(facekey) * n);
+  for(int c = 0; c < m->ncells; ++c)
+    for(int j = 0; j < nj; ++j) {
+      facekey* f = &fk[c * nj + j];
+      int v[4] = {-1, -1, -1, -1};
+      for(int k = 0; k < len; ++k) v[k] = m->cv[c * nv + jv(s, j, k)];
+      /* sort */
+      for(int a = 0; a < len; ++a)
+        for(int b = a + 1; b < len; ++b)
+          if(v[b] < v[a]) { int t = v[a]; v[a] = v[b]; v[b] = t; }
+      memcpy(f->v, v, sizeof(v));
+      f->cell = c;
+      f->joint = j;
+    }
+  qsort(fk, n, sizeof(facekey), fk_cmp);
+  L->nb_cell = malloc(sizeof(int) * n);
+  L->nb_joint = malloc(sizeof(int) * n);
+  L->comp = malloc(sizeof(int) * n);
+  for(int i = 0; i < n; ++i) { L->nb_cell[i] = -1; L->nb_joint[i] = -1; }
+  for(int i = 0; i + 1 < n; ++i)
+    if(memcmp(fk[i].v, fk[i + 1].v, sizeof(fk[i].v)) == 0) {
+      L->nb_cell[fk[i].cell * nj + fk[i].joint] = fk[i + 1].cell;
+      L->nb_joint[fk[i].cell * nj + fk[i].joint] = fk[i + 1].joint;
+      L->nb_cell[fk[i + 1].cell * nj + fk[i + 1].joint] = fk[i].cell;
+      L->nb_joint[fk[i + 1].cell * nj + fk[i + 1].joint] = fk[i].joint;
+    }
+  free(fk);
+  for(int c = 0; c < m->ncells; ++c)
+    for(int j = 0; j < nj; ++j) {
+      L->comp[c * nj + j] = -1;
+      if(L->nb_cell[c * nj + j] >= 0) continue;
+      double xs[12];
+      for(int k = 0; k < len; ++k) memcpy(&xs[3 * k], &m->X[3 * m->cv[c * nv + jv(s, j, k)]], 24);
+      L->comp[c * nj + j] = box_comp(DIM[s], xs, len);
+    }
+}
+
+/* physical coordinates of a reference point (affine maps of the reference) */
+typedef struct {
+  double c0[3], C[3][3], det, K[3][3], hK;
+} ogeo;
+
+static void geo(const omesh* m, int c, ogeo* g) {
+  const int s = m->shape, nv = NV[s];
+  double X[8][3];
+  for(int a = 0; a < nv; ++a) memcpy(X[a], &m->X[3 * m->cv[c * nv + a]], 24);
+  memset(g, 0, sizeof(*g));
+  if(s == TRI || s == QUAD) {
+    for(int a = 0; a < 2; ++a) {
+      if(s == TRI) {
+        g->c0[a] = X[0][a]; g->C[a][0] = X[1][a] - X[0][a]; g->C[a][1] = X[2][a] - X[0][a];
+      } else {
+        g->c0[a] = (X[1][a] + X[3][a]) * 0.5;
+        g->C[a][0] = (X[1][a] - X[0][a]) * 0.5;
+        g->C[a][1] = (X[3][a] - X[0][a]) * 0.5;
+      }
+    }
+    const double xc1 = g->C[0][0], xc2 = g->C[0][1], yc1 = g->C[1][0], yc2 = g->C[1][1];
+    g->det = xc1 * yc2 - xc2 * yc1;
+    g->K[0][0] = yc2; g->K[0][1] = -yc1; g->K[1][0] = -xc2; g->K[1][1] = xc1;
+    double d2 = 0;
+    if(s == TRI) {
+      for(int e = 0; e < 3; ++e) {
+        int a = e, b = (e + 1) % 3;
+        double t = 0;
+        for(int k = 0; k < 3; ++k) t += (X[a][k] - X[b][k]) * (X[a][k] - X[b][k]);
+        if(t > d2) d2 = t;
+      }
+    } else {
+      for(int e = 0; e < 2; ++e) {
+        double t = 0;
+        for(int k = 0; k < 3; ++k) t += (X[e][k] - X[e + 2][k]) * (X[e][k] - X[e + 2][k]);
+        if(t > d2) d2 = t;
+      }
+    }
+    g->hK = sqrt(d2);
+    return;
+  }
+  for(int a = 0; a < 3; ++a) {
+    if(s == TET) {
+      g->c0[a] = X[0][a];
+      for(int k = 0; k < 3; ++k) g->C[a][k] = X[k + 1][a] - X[0][a];
+    } else {
+      g->c0[a] = (X[1][a] + X[3][a] + X[4][a] - X[0][a]) * 0.5;
+      g->C[a][0] = (X[1][a] - X[0][a]) * 0.5;
+      g->C[a][1] = (X[3][a] - X[0][a]) * 0.5;
+      g->C[a][2] = (X[4][a] - X[0][a]) * 0.5;
+    }
+  }
+  const double xc1 = g->C[0][0], xc2 = g->C[0][1], xc3 = g->C[0][2];
+  const double yc1 = g->C[1][0], yc2 = g->C[1][1], yc3 = g->C[1][2];
+  const double zc1 = g->C[2][0], zc2 = g->C[2][1], zc3 = g->C[2][2];
+  g->det = xc1 * yc2 * zc3 + xc2 * yc3 * zc1 + xc3 * yc1 * zc2 - xc3 * yc2 * zc1 - xc2 * yc1 * zc3 -
+           xc1 * yc3 * zc2;
+  g->K[0][0] = yc2 * zc3 - yc3 * zc2; g->K[0][1] = yc3 * zc1 - yc1 * zc3; g->K[0][2] = yc1 * zc2 - yc2 * zc1;
+  g->K[1][0] = xc3 * zc2 - xc2 * zc3; g->K[1][1] = xc1 * zc3 - xc3 * zc1; g->K[1][2] = xc2 * zc1 - xc1 * zc2;
+  g->K[2][0] = xc2 * yc3 - xc3 * yc2; g->K[2][1] = xc3 * yc1 - xc1 * yc3; g->K[2][2] = xc1 * yc2 - xc2 * yc1;
+  double d2 = 0;
+  if(s == TET) {
+    for(int a = 0; a < 4; ++a)
+      for(int b = a + 1; b < 4; ++b) {
+        double t = 0;
+        for(int k = 0; k < 3; ++k) t += (X[a][k] - X[b][k]) * (X[a][k] - X[b][k]);
+        if(t > d2) d2 = t;
+      }
+  } else {
+    const int dg[4][2] = {{0, 6}, {1, 7}, {2, 4}, {3, 5}};
+    for(int e = 0; e < 4; ++e) {
+      double t = 0;
+      for(int k = 0; k < 3; ++k) t += (X[dg[e][0]][k] - X[dg[e][1]][k]) * (X[dg[e][0]][k] - X[dg[e][1]][k]);
+      if(t > d2) d2 = t;
+    }
+  }
+  g->hK = sqrt(d2);
+}
+
+static void geo_point(const ogeo* g, int dim, const double* r, double* x) {
+  for(int a = 0; a < 3; ++a) {
+    x[a] = g->c0[a] + g->C[a][0] * r[0] + g->C[a][1] * r[1];
+    if(dim == 3) x[a] = x[a] + g->C[a][2] * r[2];
+  }
+}
+
+/* node position of local DOF i in cell c: affine image of the reference node */
+static void node_xyz(const orc_problem* P, const omesh* m, int c, int i, double* x) {
+  ogeo g;
+  geo(m, c, &g);
+  geo_point(&g, P->el.dim, P->el.node[i], x);
+}
+
+/* ----------------------------------------------------------------------- */
+/* DOF numbering (union-find restatement)                                   */
+/* ----------------------------------------------------------------------- */
+#define OR_FIRSTMARK (-10)
+#define NBT 7
+
+static int uf_find(const int* G, int w) {
+  int v;
+  while((v = G[w]) > -1) w = v;
+  return w;
+}
+
+static void map_single_dof(int* G, int dof0, int dof1, int* Counter) {
+  int w0 = uf_find(G, dof0), w1 = uf_find(G, dof1);
+  if(G[dof0] == -1 && G[dof1] == -1) {
+    (*Counter)--;
+    if(dof0 < dof1) { G[dof0] = *Counter; G[dof1] = dof0; }
+    else { G[dof1] = *Counter; G[dof0] = dof1; }
+  } else if(G[dof0] == -1) {
+    if(w1 > dof0) { G[dof0] = G[w1]; G[dof1] = dof0; G[w1] = dof0; }
+    else G[dof0] = w1;
+  } else if(G[dof1] == -1) {
+    if(w0 > dof1) { G[dof1] = G[w0]; G[dof0] = dof1; G[w0] = dof1; }
+    else G[dof1] = w0;
+  } else if(G[w0] != G[w1]) {
+    int e = G[w0] > G[w1] ? G[w0] : G[w1];
+    int w = w0 < w1 ? w0 : w1;
+    G[w0] = w; G[w1] = w; G[w] = e;
+  }
+}
+
+static void map_single_boundary_dof(int* G, int dof, int* BoundCounter) {
+  int w = uf_find(G, dof);
+  if(G[w] == -1) {
+    (*BoundCounter)--;
+    G[dof] = *BoundCounter;
+  } else if(G[w] < *BoundCounter) {
+    (*BoundCounter)--;
+    G[w] = *BoundCounter;
+    if(dof != w) G[dof] = w;
+  }
+}
+
+/* local DOFs lying on joint j (nodes inside the joint) */
+static int joint_dofs(const orc_problem* P, int j, int* out) {
+  const oelem* e = &P->el;
+  const int s = e->shape, len = jv_len(s);
+  /* reference vertex coordinates */
+  double rv[8][3];
+  oelem e1 = make_elem(s, 1);
+  for(int v = 0; v < NV[s]; ++v) {
+    /* vertex v of the shape: for tensor shapes the element-1 node index of
+       the corner is the tensor position of the vertex */
+    if(s == QUAD || s == HEX) {
+      static const int q2[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+      rv[v][0] = q2[v % 4][0] ? 1 : -1;
+      rv[v][1] = q2[v % 4][1] ? 1 : -1;
+      rv[v][2] = s == HEX ? (v / 4 ? 1 : -1) : 0;
+    } else {
+      for(int k = 0; k < 3; ++k) rv[v][k] = e1.node[v][k];
+    }
+  }
+  int n = 0;
+  for(int i = 0; i < e->nl; ++i) {
+    /* node inside the convex hull of the joint vertices: solve by checking
+       it is an affine combination with non-negative weights (joint is a
+       segment, triangle or parallelogram on the reference cell) */
+    const double* x = e->node[i];
+    int in = 0;
+    if(len == 2) {
+      const double* a = rv[jv(s, j, 0)];
+      const double* b = rv[jv(s, j, 1)];
+      double cr = (b[0] - a[0]) * (x[1] - a[1]) - (b[1] - a[1]) * (x[0] - a[0]);
+      in = fabs(cr) < 1e-12;
+    } else {
+      const double* a = rv[jv(s, j, 0)];
+      const double* b = rv[jv(s, j, 1)];
+      const double* c = rv[jv(s, j, 2)];
+      double u[3], v[3], w[3], nrm[3];
+      for(int k = 0; k < 3; ++k) { u[k] = b[k] - a[k]; v[k] = c[k] - a[k]; w[k] = x[k] - a[k]; }
+      nrm[0] = u[1] * v[2] - u[2] * v[1];
+      nrm[1] = u[2] * v[0] - u[0] * v[2];
+      nrm[2] = u[0] * v[1] - u[1] * v[0];
+      in = fabs(nrm[0] * w[0] + nrm[1] * w[1] + nrm[2] * w[2]) < 1e-12;
+    }
+    if(in) out[n++] = i;
+  }
+  return n;
+}
+
+static void build_space(orc_problem* P, olevel* L) {
+  const omesh* m = &L->m;
+  const int s = m->shape, nj = NJ[s], nl = P->el.nl;
+  const int N = m->ncells * nl;
+  L->nl = nl;
+  int jd[8][MAXNL], njd[8];
+  for(int j = 0; j < nj; ++j) njd[j] = joint_dofs(P, j, jd[j]);
+  /* BoundaryUpperBound: joint DOFs per boundary joint, by kind */
+  int BUB[NBT] = {0};
+  for(int c = 0; c < m->ncells; ++c)
+    for(int j = 0; j < nj; ++j)
+      if(L->nb_cell[c * nj + j] < 0) BUB[ex_bc(&P->ex, L->comp[c * nj + j])] += njd[j];
+  int Counters[NBT + 1];
+  Counters[0] = OR_FIRSTMARK + 1;
+  for(int k = 0; k < NBT; ++k) Counters[k + 1] = Counters[k] - BUB[k];
+  int* Counter = &Counters[NBT];
+  int* G = malloc(sizeof(int) * N);
+  for(int i = 0; i < N; ++i) G[i] = -1;
+  for(int c = 0; c < m->ncells; ++c) {
+    for(int j = 0; j < nj; ++j) {
+      const int nb = L->nb_cell[c * nj + j];
+      if(nb < 0) {
+        const int kind = ex_bc(&P->ex, L->comp[c * nj + j]);
+        for(int k = 0; k < njd[j]; ++k) map_single_boundary_dof(G, c * nl + jd[j][k], &Counters[kind]);
+      } else if(nb > c) {
+        const int jn = L->nb_joint[c * nj + j];
+        /* pair joint DOFs by coinciding nodal points */
+        for(int k = 0; k < njd[j]; ++k) {
+          double x0[3];
+          node_xyz(P, m, c, jd[j][k], x0);
+          for(int kk = 0; kk < njd[jn]; ++kk) {
+            double x1[3];
+            node_xyz(P, m, nb, jd[jn][kk], x1);
+            if(fabs(x0[0] - x1[0]) + fabs(x0[1] - x1[1]) + fabs(x0[2] - x1[2]) < 1e-12) {
+              map_single_dof(G, c * nl + jd[j][k], nb * nl + jd[jn][kk], Counter);
+              break;
+            }
+          }
+        }
+      }
+    }
+    /* inner DOFs (on no joint) */
+    for(int i = 0; i < nl; ++i) {
+      int on = 0;
+      for(int j = 0; j < nj && !on; ++j)
+        for(int k = 0; k < njd[j]; ++k)
+          if(jd[j][k] == i) on = 1;
+      if(!on) { (*Counter)--; G[c * nl + i] = *Counter; }
+    }
+  }
+  /* compute_various_numbers (no hanging nodes) */
+  int total = 0;
+  for(int k = 0; k < NBT; ++k) total += BUB[k];
+  int BoundMark[NBT];
+  int l = OR_FIRSTMARK;
+  for(int k = 0; k < NBT; ++k) { l -= BUB[k]; BoundMark[k] = l; }
+  const int InnerMark = l;
+  int BoundCounter[NBT];
+  l = 0;
+  for(int k = 0; k < NBT; ++k) { l += BUB[k]; BoundCounter[k] = l; }
+  int count = l, DirichletCounter = 0, N_Dir = 0, N_Inner = 0;
+  int NBnd[NBT] = {0};
+  for(int i = 0; i < N; ++i) {
+    int v = G[i];
+    if(v < -1) {
+      if(v <= InnerMark) { G[i] = count++; N_Inner++; continue; }
+      int mm = NBT - 1, matched = 0;
+      for(; mm >= 0; --mm)
+        if(v <= BoundMark[mm]) { G[i] = BoundCounter[mm]++; NBnd[mm]++; matched = 1; break; }
+      if(!matched && v <= OR_FIRSTMARK) { G[i] = DirichletCounter++; N_Dir++; }
+    } else if(v >= 0) {
+      G[i] = G[v];
+    }
+  }
+  int nb_total = 0;
+  for(int k = 0; k < NBT; ++k) nb_total += NBnd[k];
+  const int DirichletOffset = N_Inner + nb_total;
+  const int InnerOffset = -total;
+  int BoundOffset[NBT];
+  l = N_Inner;
+  {
+    int mm = 0;
+    for(int k = 0; k < NBT; ++k) { mm += BUB[k]; BoundOffset[k] = l - mm; l += NBnd[k]; }
+  }
+  const int DirichletMark = BUB[0];
+  l = BUB[0];
+  for(int k = 0; k < NBT - 1; ++k) { l += BUB[k + 1]; BoundMark[k] = l; }
+  BoundMark[NBT - 1] = l;
+  for(int i = 0; i < N; ++i) {
+    int v = G[i];
+    if(v < DirichletMark) G[i] += DirichletOffset;
+    else if(v < BoundMark[NBT - 1]) {
+      for(int mm = 0; mm < NBT; ++mm)
+        if(v < BoundMark[mm]) { G[i] += BoundOffset[mm]; break; }
+    } else G[i] += InnerOffset;
+  }
+  L->dof = G;
+  L->nactive = N_Inner + nb_total;
+  L->ndir = N_Dir;
+  L->ndof = L->nactive + N_Dir;
+}
+
+/* sorted full pattern (Structure.C) */
+static int icmp(const void* a, const void* b) { return *(const int*)a - *(const int*)b; }
+static void build_structure(olevel* L) {
+  const int n = L->ndof, nl = L->nl, nc = L->m.ncells;
+  int* cnt = calloc(n + 1, sizeof(int));
+  for(int i = 0; i < nc * nl; ++i) cnt[L->dof[i] + 1]++;
+  for(int r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
+  int* cells = malloc(sizeof(int) * nc * nl);
+  int* fill = malloc(sizeof(int) * n);
+  memcpy(fill, cnt, sizeof(int) * n);
+  for(int c = 0; c < nc; ++c)
+    for(int m = 0; m < nl; ++m) cells[fill[L->dof[c * nl + m]]++] = c;
+  L->rp = malloc(sizeof(int) * (n + 1));
+  int* buf = malloc(sizeof(int) * 64 * MAXNL);
+  long cap = (long)n * 32 + 64, used = 0;
+  L->col = malloc(sizeof(int) * cap);
+  L->rp[0] = 0;
+  for(int r = 0; r < n; ++r) {
+    int nb = 0;
+    for(int e = cnt[r]; e < cnt[r + 1]; ++e)
+      for(int k = 0; k < nl; ++k) buf[nb++] = L->dof[cells[e] * nl + k];
+    qsort(buf, nb, sizeof(int), icmp);
+    int u = 0;
+    for(int k = 0; k < nb; ++k)
+      if(u == 0 || buf[k] != buf[u - 1]) buf[u++] = buf[k];
+    if(used + u > cap) { cap = (used + u) * 2; L->col = realloc(L->col, sizeof(int) * cap); }
+    memcpy(&L->col[used], buf, sizeof(int) * u);
+    used += u;
+    L->rp[r + 1] = (int)used;
+  }
+  L->nnz = used;
+  L->val = calloc(used, sizeof(double));
+  L->rhs = calloc(n, sizeof(double));
+  L->sol = calloc(n, sizeof(double));
+  free(buf); free(cells); free(fill); free(cnt);
+}
+
+/* ----------------------------------------------------------------------- */
+/* assembly                                                                 */
+/* ----------------------------------------------------------------------- */
+static double sdfem_delta(int dim, double hK, double eps, double b1, double b2, double b3) {
+  if(fabs(eps) < 1e-20) eps = 1e-20;
+  double nb = b1 * b1 + b2 * b2;
+  if(dim == 3) nb += b3 * b3;
+  nb = sqrt(nb);
+  double delta;
+  const double nu = 1.0 / 1;
+  if(nb > 0) {
+    double alpha = nu * nb * hK / (2 * eps);
+    delta = nu * hK * (1 / tanh(alpha) - 1 / alpha) / (2 * nb);
+  } else delta = 0;
+  return delta * 1.0;
+}
+
+/* physical values at quad point: u, ux, uy, uz, uxx, uyy, uzz for all i */
+static void orig_values(const orc_problem* P, const ogeo* g, const double* ref, double (*o)[MAXNL]) {
+  const int nl = P->el.nl, d = P->el.dim, s = P->el.shape;
+  for(int i = 0; i < nl; ++i) {
+    o[0][i] = ref[i];
+    if(d == 2) {
+      const double rx = ref[nl + i], ry = ref[2 * nl + i];
+      o[1][i] = (g->K[0][0] * rx + g->K[0][1] * ry) / g->det;
+      o[2][i] = (g->K[1][0] * rx + g->K[1][1] * ry) / g->det;
+      o[3][i] = 0;
+      for(int a = 0; a < 2; ++a) {
+        const double j0 = g->K[a][0] / g->det, j1 = g->K[a][1] / g->det;
+        o[4 + a][i] = j0 * j0 * ref[3 * nl + i] + 2 * j0 * j1 * ref[4 * nl + i] + j1 * j1 * ref[5 * nl + i];
+      }
+      o[6][i] = 0;
+    } else {
+      const double rx = ref[nl + i], ry = ref[2 * nl + i], rz = ref[3 * nl + i];
+      for(int a = 0; a < 3; ++a) {
+        if(s == TET)
+          o[1 + a][i] = (g->K[a][0] / g->det) * rx + (g->K[a][1] / g->det) * ry + (g->K[a][2] / g->det) * rz;
+        else
+          o[1 + a][i] = (g->K[a][0] * rx + g->K[a][1] * ry + g->K[a][2] * rz) / g->det;
+      }
+      for(int a = 0; a < 3; ++a) {
+        const double j0 = g->K[a][0] / g->det, j1 = g->K[a][1] / g->det, j2 = g->K[a][2] / g->det;
+        o[4 + a][i] = j0 * j0 * ref[4 * nl + i] + 2 * j0 * j1 * ref[5 * nl + i] +
+                      2 * j0 * j2 * ref[6 * nl + i] + j1 * j1 * ref[7 * nl + i] +
+                      2 * j1 * j2 * ref[8 * nl + i] + j2 * j2 * ref[9 * nl + i];
+      }
+    }
+  }
+}
+
+static void assemble_level(orc_problem* P, olevel* L) {
+  const omesh* m = &L->m;
+  const int nl = L->nl, d = P->el.dim, s = m->shape, nj = NJ[s];
+  const oquad* q = &P->qd;
+  memset(L->val, 0, sizeof(double) * L->nnz);
+  memset(L->rhs, 0, sizeof(double) * L->ndof);
+  static double ref[MAXQ][10 * MAXNL];
+  for(int k = 0; k < q->n; ++k) elem_eval(&P->el, q->x[k], ref[k]);
+  double A[MAXNL][MAXNL], F[MAXNL];
+  double o[7][MAXNL];
+  for(int c = 0; c < m->ncells; ++c) {
+    ogeo g;
+    geo(m, c, &g);
+    memset(A, 0, sizeof(A));
+    memset(F, 0, sizeof(F));
+    for(int k = 0; k < q->n; ++k) {
+      double x[3], cf[6];
+      geo_point(&g, d, q->x[k], x);
+      const double Mult = q->w[k] * fabs(g.det);
+      ex_coeffs(&P->ex, x[0], x[1], d == 3 ? x[2] : 0.0, cf);
+      orig_values(P, &g, ref[k], o);
+      const double eps = cf[0], b1 = cf[1], b2 = cf[2], b3 = cf[3], cr = cf[4], f = cf[5];
+      const double delta = P->supg ? sdfem_delta(d, g.hK, eps, b1, b2, b3) : 0;
+      for(int i = 0; i < nl; ++i) {
+        const double t0 = o[0][i], tx = o[1][i], ty = o[2][i], tz = o[3][i];
+        /* Galerkin, then SD (the reference calls the local routines in this order) */
+        if(d == 3) F[i] += Mult * t0 * f;
+        else F[i] += Mult * t0 * f;
+        for(int j = 0; j < nl; ++j) {
+          const double a0 = o[0][j], ax = o[1][j], ay = o[2][j], az = o[3][j];
+          if(d == 3) {
+            double val = eps * (tx * ax + ty * ay + tz * az);
+            val += (b1 * ax + b2 * ay + b3 * az) * t0;
+            val += cr * a0 * t0;
+            A[i][j] += val * Mult;
+          } else {
+            double val = eps * (tx * ax + ty * ay);
+            val += (b1 * ax + b2 * ay) * t0;
+            val += cr * a0 * t0;
+            val *= Mult;
+            A[i][j] += val;
+          }
+        }
+      }
+      if(P->supg) {
+        for(int i = 0; i < nl; ++i) {
+          const double tx = o[1][i], ty = o[2][i], tz = o[3][i];
+          if(d == 3) {
+            const double bgradv = b1 * tx + b2 * ty + b3 * tz;
+            F[i] += Mult * (delta * bgradv) * f;
+            for(int j = 0; j < nl; ++j) {
+              double val = -eps * (o[4][j] + o[5][j] + o[6][j]);
+              val += b1 * o[1][j] + b2 * o[2][j] + b3 * o[3][j] + cr * o[0][j];
+              A[i][j] += val * delta * bgradv * Mult;
+            }
+          } else {
+            const double bgradv = b1 * tx + b2 * ty;
+            F[i] += Mult * delta * bgradv * f;
+            for(int j = 0; j < nl; ++j) {
+              double val = delta * (-eps * (o[4][j] + o[5][j]) + b1 * o[1][j] + b2 * o[2][j] + cr * o[0][j]) * bgradv;
+              val *= Mult;
+              A[i][j] += val;
+            }
+          }
+        }
+      }
+    }
+    /* scatter by linear row search; Dirichlet rows get set(l,l,1) */
+    const int* DOF = &L->dof[c * nl];
+    for(int mm = 0; mm < nl; ++mm) {
+      const int l = DOF[mm];
+      if(l < L->nactive) {
+        for(int k = 0; k < nl; ++k) {
+          int found = 0;
+          for(int e = L->rp[l]; e < L->rp[l + 1]; ++e)
+            if(L->col[e] == DOF[k]) { L->val[e] += A[mm][k]; found = 1; break; }
+          if(!found) fprintf(stderr, "oracle: entry not found\n");
+        }
+      } else {
+        for(int e = L->rp[l]; e < L->rp[l + 1]; ++e)
+          if(L->col[e] == l) L->val[e] = 1.0;
+      }
+    }
+    for(int mm = 0; mm < nl; ++mm)
+      if(DOF[mm] < L->nactive) L->rhs[DOF[mm]] += F[mm];
+    /* boundary joints: Dirichlet values, Neumann integrals (2D) */
+    for(int j = 0; j < nj; ++j) {
+      if(L->nb_cell[c * nj + j] >= 0) continue;
+      const int comp = L->comp[c * nj + j];
+      int jd[MAXNL];
+      const int n = joint_dofs(P, j, jd);
+      if(ex_bc(&P->ex, comp) == 0) {
+        for(int k = 0; k < n; ++k) {
+          double x[3];
+          node_xyz(P, m, c, jd[k], x);
+          L->rhs[DOF[jd[k]]] = ex_bvalue(&P->ex, comp, x[0], x[1], x[2]);
+        }
+      } else if(d == 2) {
+        /* Gauss rule of degree 2p on the edge from vertex j to vertex j+1 */
+        const int deg = 2 * P->el.order;
+        double lw[4], lx[4];
+        int ln;
+        if(deg <= 3) { ln = 2; lw[0] = lw[1] = 1; lx[0] = 2 * 0.21132486540518711774 - 1; lx[1] = 2 * 0.78867513459481288226 - 1; }
+        else {
+          ln = 3;
+          lw[0] = 0.2777777777777778 * 2; lw[1] = 0.4444444444444444 * 2; lw[2] = 0.2777777777777778 * 2;
+          lx[0] = 2 * 0.11270166537925831149 - 1; lx[1] = 2 * 0.5 - 1; lx[2] = 2 * 0.88729833462074168851 - 1;
+        }
+        const int v0 = m->cv[c * NV[s] + jv(s, j, 0)], v1 = m->cv[c * NV[s] + jv(s, j, 1)];
+        const double* x0 = &m->X[3 * v0];
+        const double* x1 = &m->X[3 * v1];
+        const double hE = sqrt((x1[0] - x0[0]) * (x1[0] - x0[0]) + (x1[1] - x0[1]) * (x1[1] - x0[1])) / 2;
+        /* reference end points of the edge */
+        double r0[3], r1[3];
+        {
+          oelem e1 = make_elem(s, 1);
+          int a0 = jv(s, j, 0), a1 = jv(s, j, 1);
+          if(s == QUAD) {
+            static const int q2[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+            r0[0] = q2[a0][0] ? 1 : -1; r0[1] = q2[a0][1] ? 1 : -1; r0[2] = 0;
+            r1[0] = q2[a1][0] ? 1 : -1; r1[1] = q2[a1][1] ? 1 : -1; r1[2] = 0;
+          } else {
+            memcpy(r0, e1.node[a0], 24);
+            memcpy(r1, e1.node[a1], 24);
+          }
+        }
+        for(int lq = 0; lq < ln; ++lq) {
+          const double t = (1 + lx[lq]) / 2;
+          double rp[3], xp[3], phi[10 * MAXNL];
+          for(int k = 0; k < 3; ++k) { rp[k] = r0[k] + t * (r1[k] - r0[k]); xp[k] = x0[k] + t * (x1[k] - x0[k]); }
+          elem_eval(&P->el, rp, phi);
+          double sv = ex_bvalue(&P->ex, comp, xp[0], xp[1], 0);
+          sv *= hE * lw[lq];
+          for(int k = 0; k < nl; ++k)
+            if(DOF[k] < L->nactive) L->rhs[DOF[k]] += sv * phi[k];
+        }
+      }
+    }
+  }
+}
+
+/* ----------------------------------------------------------------------- */
+/* linear algebra                                                           */
+/* ----------------------------------------------------------------------- */
+static double ddot(int n, const double* x, const double* y) {
+  double r = 0;
+  for(int i = 0; i < n; ++i) r += x[i] * y[i];
+  return r;
+}
+
+/* y += a * A x on active rows, y[na] += a * x[na] */
+static void apply_scaled_add(const olevel* L, const double* x, double* y, double a) {
+  for(int i = 0; i < L->nactive; ++i) {
+    double v = 0;
+    for(int j = L->rp[i]; j < L->rp[i + 1]; ++j) v += L->val[j] * x[L->col[j]];
+    y[i] += a * v;
+  }
+  for(int i = L->nactive; i < L->ndof; ++i) y[i] += a * x[i];
+}
+
+static void apply(const olevel* L, const double* x, double* y) {
+  memset(y, 0, sizeof(double) * L->ndof);
+  apply_scaled_add(L, x, y, 1.0);
+}
+
+static double diag_of(const olevel* L, int r) {
+  for(int j = L->rp[r]; j < L->rp[r + 1]; ++j)
+    if(L->col[j] == r) return L->val[j];
+  return 0;
+}
+
+/* dense LU with partial pivoting of the level matrix (all rows as stored) */
+static void dense_factor(olevel* L) {
+  const int n = L->ndof;
+  free(L->lu);
+  free(L->piv);
+  L->lu = calloc((size_t)n * n, sizeof(double));
+  L->piv = malloc(sizeof(int) * n);
+  for(int r = 0; r < n; ++r)
+    for(int j = L->rp[r]; j < L->rp[r + 1]; ++j) L->lu[(size_t)r * n + L->col[j]] = L->val[j];
+  double* a = L->lu;
+  for(int k = 0; k < n; ++k) {
+    int p = k;
+    for(int i = k + 1; i < n; ++i)
+      if(fabs(a[(size_t)i * n + k]) > fabs(a[(size_t)p * n + k])) p = i;
+    L->piv[k] = p;
+    if(p != k)
+      for(int j = 0; j < n; ++j) { double t = a[(size_t)k * n + j]; a[(size_t)k * n + j] = a[(size_t)p * n + j]; a[(size_t)p * n + j] = t; }
+    for(int i = k + 1; i < n; ++i) {
+      double f = a[(size_t)i * n + k] / a[(size_t)k * n + k];
+      a[(size_t)i * n + k] = f;
+      if(f != 0)
+        for(int j = k + 1; j < n; ++j) a[(size_t)i * n + j] -= f * a[(size_t)k * n + j];
+    }
+  }
+}
+
+static void dense_solve(const olevel* L, const double* b, double* x) {
+  const int n = L->ndof;
+  const double* a = L->lu;
+  memcpy(x, b, sizeof(double) * n);
+  for(int k = 0; k < n; ++k)
+    if(L->piv[k] != k) { double t = x[k]; x[k] = x[L->piv[k]]; x[L->piv[k]] = t; }
+  for(int i = 0; i < n; ++i)
+    for(int j = 0; j < i; ++j) x[i] -= a[(size_t)i * n + j] * x[j];
+  for(int i = n - 1; i >= 0; --i) {
+    for(int j = i + 1; j < n; ++j) x[i] -= a[(size_t)i * n + j] * x[j];
+    x[i] /= a[(size_t)i * n + i];
+  }
+}
+
+/* ----------------------------------------------------------------------- */
+/* grid transfer (GridTransfer.C restatement, cell-wise with averaging)     */
+/* ----------------------------------------------------------------------- */
+/* coarse basis values at the fine nodes of child cell fc (QQ of the reference) */
+static void child_matrix(const orc_problem* P, const olevel* C, const olevel* F, int fc, double* QQ) {
+  const int pc = F->m.parent[fc];
+  ogeo g;
+  geo(&C->m, pc, &g);
+  const int nl = P->el.nl, d = P->el.dim;
+  double phi[10 * MAXNL];
+  for(int k = 0; k < nl; ++k) {
+    double x[3];
+    node_xyz(P, &F->m, fc, k, x);
+    /* reference coordinates in the parent: solve C r = x - c0 */
+    double rhs[3] = {x[0] - g.c0[0], x[1] - g.c0[1], x[2] - g.c0[2]}, r[3] = {0, 0, 0};
+    for(int a = 0; a < d; ++a) {
+      double s = 0;
+      for(int b = 0; b < d; ++b) s += g.K[b][a] * rhs[b];
+      r[a] = s / g.det;
+    }
+    elem_eval(&P->el, r, phi);
+    for(int l = 0; l < nl; ++l) QQ[k * nl + l] = phi[l];
+  }
+}
+
+static void prolongate(const orc_problem* P, const olevel* C, const olevel* F, const double* cf, double* ff) {
+  const int nl = P->el.nl;
+  double* aux = calloc(F->ndof, sizeof(double));
+  memset(ff, 0, sizeof(double) * F->ndof);
+  for(int i = 0; i < F->m.ncells; ++i)
+    for(int j = 0; j < nl; ++j) aux[F->dof[i * nl + j]] += 1;
+  double QQ[MAXNL * MAXNL], Val[MAXNL];
+  for(int fc = 0; fc < F->m.ncells; ++fc) {
+    const int pc = F->m.parent[fc];
+    child_matrix(P, C, F, fc, QQ);
+    for(int l = 0; l < nl; ++l) Val[l] = cf[C->dof[pc * nl + l]];
+    for(int k = 0; k < nl; ++k) {
+      double s = 0;
+      for(int l = 0; l < nl; ++l) s += QQ[k * nl + l] * Val[l];
+      ff[F->dof[fc * nl + k]] += s;
+    }
+  }
+  for(int i = 0; i < F->ndof; ++i) ff[i] /= aux[i];
+  free(aux);
+}
+
+static void restrict_defect(const orc_problem* P, const olevel* C, const olevel* F, double* cf, const double* ff) {
+  const int nl = P->el.nl;
+  double* aux = calloc(F->ndof, sizeof(double));
+  double* copy = malloc(sizeof(double) * F->ndof);
+  memcpy(copy, ff, sizeof(double) * F->ndof);
+  memset(cf, 0, sizeof(double) * C->ndof);
+  for(int i = 0; i < F->m.ncells; ++i)
+    for(int j = 0; j < nl; ++j) aux[F->dof[i * nl + j]] += 1;
+  for(int i = 0; i < F->ndof; ++i) copy[i] /= aux[i];
+  double QQ[MAXNL * MAXNL], Val[MAXNL];
+  for(int fc = 0; fc < F->m.ncells; ++fc) {
+    const int pc = F->m.parent[fc];
+    child_matrix(P, C, F, fc, QQ);
+    for(int l = 0; l < nl; ++l) Val[l] = copy[F->dof[fc * nl + l]];
+    for(int k = 0; k < nl; ++k) {
+      double s = 0;
+      for(int l = 0; l < nl; ++l) s += QQ[l * nl + k] * Val[l];
+      cf[C->dof[pc * nl + k]] += s;
+    }
+  }
+  free(aux);
+  free(copy);
+}
+
+/* ----------------------------------------------------------------------- */
+/* solvers                                                                  */
+/* ----------------------------------------------------------------------- */
+typedef struct {
+  int method, precond, max_iter, min_iter;
+  double res_tol, res_red, div_factor;
+  int restart, n_pre, n_post;
+  double jacobi_damp, mg_damp;
+  int mixed;
+} osolver;
+
+typedef struct {
+  int iterations, converged;
+  double initial_residual, final_residual, seconds;
+} ostats;
+
+typedef struct {
+  const osolver* s;
+  double initial, previous;
+  int diverged;
+} oconv;
+
+static int converged(oconv* c, double res, int it) {
+  if(it == 0) { c->initial = res; c->previous = res; }
+  int small = res < c->s->res_tol, red = res < c->s->res_red * c->initial;
+  int enough = it >= c->s->min_iter;
+  int stag = res > 1e10 * c->previous, div = res > c->s->div_factor * c->initial;
+  int conv = (small || red || stag || div) && enough;
+  if(it == c->s->max_iter) conv = 1;
+  if(div) c->diverged = 1;
+  c->previous = res;
+  return conv;
+}
+
+static void jacobi_smooth(const olevel* L, const double* b, double* x, double omega) {
+  const int n = L->ndof;
+  double* d = malloc(sizeof(double) * n);
+  memcpy(d, b, sizeof(double) * n);
+  apply_scaled_add(L, x, d, -1.0);
+  for(int i = 0; i < n; ++i) d[i] /= diag_of(L, i);
+  for(int i = 0; i < n; ++i) x[i] += omega * d[i];
+  free(d);
+}
+
+/* V-cycle (Multigrid::cycle with V control) */
+static void mg_cycle(orc_problem* P, const osolver* s, const double* v, double* z) {
+  const int Lf = P->nlev - 1;
+  olevel* F = &P->lev[Lf];
+  memcpy(F->mg_rhs, v, sizeof(double) * F->ndof);
+  memset(F->mg_sol, 0, sizeof(double) * F->nactive);
+  memcpy(F->mg_sol + F->nactive, v + F->nactive, sizeof(double) * (F->ndof - F->nactive));
+  for(int l = Lf; l >= 1; --l) {
+    olevel* L = &P->lev[l];
+    olevel* C = &P->lev[l - 1];
+    for(int k = 0; k < s->n_pre; ++k) jacobi_smooth(L, L->mg_rhs, L->mg_sol, s->jacobi_damp);
+    memcpy(L->mg_def, L->mg_rhs, sizeof(double) * L->ndof);
+    apply_scaled_add(L, L->mg_sol, L->mg_def, -1.0);
+    memset(C->mg_sol, 0, sizeof(double) * C->ndof);
+    restrict_defect(P, C, L, C->mg_rhs, L->mg_def);
+    for(int i = C->nactive; i < C->ndof; ++i) C->mg_rhs[i] = 0;
+  }
+  {
+    olevel* C = &P->lev[0];
+    if(P->nlev > 1) dense_solve(C, C->mg_rhs, C->mg_sol);
+    else
+      for(int k = 0; k < s->n_pre + s->n_post; ++k) jacobi_smooth(C, C->mg_rhs, C->mg_sol, s->jacobi_damp);
+  }
+  for(int l = 1; l <= Lf; ++l) {
+    olevel* L = &P->lev[l];
+    olevel* C = &P->lev[l - 1];
+    prolongate(P, C, L, C->mg_sol, L->mg_aux);
+    for(int j = 0; j < L->nactive; ++j) L->mg_sol[j] += s->mg_damp * L->mg_aux[j];
+    for(int k = 0; k < s->n_post; ++k) jacobi_smooth(L, L->mg_rhs, L->mg_sol, s->jacobi_damp);
+  }
+  memcpy(z, F->mg_sol, sizeof(double) * F->ndof);
+}
+
+static void precond(orc_problem* P, const osolver* s, const double* v, double* z) {
+  olevel* F = &P->lev[P->nlev - 1];
+  if(s->precond == 0) memcpy(z, v, sizeof(double) * F->ndof);
+  else if(s->precond == 1)
+    for(int i = 0; i < F->ndof; ++i) z[i] = v[i] / diag_of(F, i);
+  else mg_cycle(P, s, v, z);
+}
+
+static void givens_gen(double dx, double dy, double* cs, double* sn) {
+  if(dy == 0.0) { *cs = 1; *sn = 0; }
+  else if(fabs(dy) > fabs(dx)) { double t = dx / dy; *sn = 1.0 / sqrt(1.0 + t * t); *cs = t * *sn; }
+  else { double t = dy / dx; *cs = 1.0 / sqrt(1.0 + t * t); *sn = t * *cs; }
+}
+static void givens_app(double* dx, double* dy, double cs, double sn) {
+  double t = cs * *dx + sn * *dy;
+  *dy = -sn * *dx + cs * *dy;
+  *dx = t;
+}
+
+static int solve_fgmres(orc_problem* P, const osolver* s, ostats* st) {
+  olevel* F = &P->lev[P->nlev - 1];
+  const int n = F->ndof, m = s->restart;
+  oconv cv = {s, 0, 0, 0};
+  double* x = F->sol;
+  double* r = malloc(sizeof(double) * n);
+  double** v = malloc(sizeof(double*) * (m + 1));
+  double** z = malloc(sizeof(double*) * m);
+  for(int i = 0; i <= m; ++i) v[i] = malloc(sizeof(double) * n);
+  for(int i = 0; i < m; ++i) z[i] = malloc(sizeof(double) * n);
+  double* H = calloc((size_t)(m + 1) * m, sizeof(double));
+#define HH(i, j) H[(size_t)(j) * (m + 1) + (i)]
+  double* sv = calloc(m + 1, sizeof(double));
+  double* cs = calloc(m + 1, sizeof(double));
+  double* sn = calloc(m + 1, sizeof(double));
+  double* y = calloc(m + 1, sizeof(double));
+  memcpy(r, F->rhs, sizeof(double) * n);
+  apply_scaled_add(F, x, r, -1.0);
+  double resid = sqrt(ddot(n, r, r)), beta = resid;
+  st->initial_residual = resid;
+  int done = converged(&cv, resid, 0);
+  int j = 1;
+  while(!done && j <= s->max_iter) {
+    for(int k = 0; k < n; ++k) v[0][k] = r[k] * (1.0 / beta);
+    memset(sv, 0, sizeof(double) * (m + 1));
+    sv[0] = beta;
+    int i;
+    for(i = 0; i < m && j <= s->max_iter; ++i, ++j) {
+      precond(P, s, v[i], z[i]);
+      apply(F, z[i], r);
+      for(int k = 0; k <= i; ++k) {
+        HH(k, i) = ddot(n, r, v[k]);
+        for(int t = 0; t < n; ++t) r[t] += -HH(k, i) * v[k][t];
+      }
+      HH(i + 1, i) = sqrt(ddot(n, r, r));
+      if(HH(i + 1, i) == 0.0) { fprintf(stderr, "oracle: unhappy breakdown\n"); done = 1; break; }
+      for(int t = 0; t < n; ++t) v[i + 1][t] = r[t] * (1.0 / HH(i + 1, i));
+      for(int k = 0; k < i; ++k) givens_app(&HH(k, i), &HH(k + 1, i), cs[k], sn[k]);
+      givens_gen(HH(i, i), HH(i + 1, i), &cs[i], &sn[i]);
+      givens_app(&HH(i, i), &HH(i + 1, i), cs[i], sn[i]);
+      givens_app(&sv[i], &sv[i + 1], cs[i], sn[i]);
+      resid = fabs(sv[i + 1]);
+      st->iterations = j;
+      if(converged(&cv, resid, j)) {
+        /* update with i */
+        memcpy(y, sv, sizeof(double) * (m + 1));
+        for(int a = i; a >= 0; --a) {
+          y[a] /= HH(a, a);
+          for(int b = a - 1; b >= 0; --b) y[b] -= HH(b, a) * y[a];
+        }
+        memset(r, 0, sizeof(double) * n);
+        for(int b = 0; b <= i; ++b)
+          for(int t = 0; t < n; ++t) r[t] += y[b] * z[b][t];
+        for(int t = 0; t < n; ++t) x[t] += r[t];
+        done = 2;
+        ++j;
+        break;
+      }
+    }
+    if(done) break;
+    memcpy(y, sv, sizeof(double) * (m + 1));
+    for(int a = m - 1; a >= 0; --a) {
+      y[a] /= HH(a, a);
+      for(int b = a - 1; b >= 0; --b) y[b] -= HH(b, a) * y[a];
+    }
+    memset(r, 0, sizeof(double) * n);
+    for(int b = 0; b < m; ++b)
+      for(int t = 0; t < n; ++t) r[t] += y[b] * z[b][t];
+    for(int t = 0; t < n; ++t) x[t] += r[t];
+    memcpy(r, F->rhs, sizeof(double) * n);
+    apply_scaled_add(F, x, r, -1.0);
+    beta = sqrt(ddot(n, r, r));
+    if(converged(&cv, resid, j)) break;
+  }
+#undef HH
+  st->final_residual = resid;
+  st->converged = resid < s->res_tol || resid < s->res_red * st->initial_residual;
+  for(int i = 0; i <= m; ++i) free(v[i]);
+  for(int i = 0; i < m; ++i) free(z[i]);
+  free(v); free(z); free(H); free(sv); free(cs); free(sn); free(y); free(r);
+  return cv.diverged ? 2 : 0;
+}
+
+static int solve_cg(orc_problem* P, const osolver* s, ostats* st) {
+  olevel* F = &P->lev[P->nlev - 1];
+  const int n = F->ndof;
+  oconv cv = {s, 0, 0, 0};
+  double* x = F->sol;
+  double *p = calloc(n, sizeof(double)), *zz = calloc(n, sizeof(double)), *qv = calloc(n, sizeof(double)),
+         *r = calloc(n, sizeof(double));
+  memcpy(r, F->rhs, sizeof(double) * n);
+  apply_scaled_add(F, x, r, -1.0);
+  double normb = sqrt(ddot(n, F->rhs, F->rhs));
+  if(normb == 0.0) { normb = 1.0; memset(r, 0, sizeof(double) * n); memset(x, 0, sizeof(double) * n); }
+  double resid = sqrt(ddot(n, r, r)) / normb, rho_1 = 0;
+  st->initial_residual = resid;
+  if(!converged(&cv, resid, 0)) {
+    for(int i = 1; i <= s->max_iter; ++i) {
+      precond(P, s, r, zz);
+      double rho = ddot(n, r, zz);
+      if(i == 1) memcpy(p, zz, sizeof(double) * n);
+      else {
+        double beta = rho / rho_1;
+        for(int t = 0; t < n; ++t) p[t] *= beta;
+        for(int t = 0; t < n; ++t) p[t] += zz[t];
+      }
+      apply(F, p, qv);
+      double alpha = rho / ddot(n, p, qv);
+      for(int t = 0; t < n; ++t) x[t] += alpha * p[t];
+      for(int t = 0; t < n; ++t) r[t] += -alpha * qv[t];
+      resid = sqrt(ddot(n, r, r)) / normb;
+      st->iterations = i;
+      if(converged(&cv, resid, i)) break;
+      rho_1 = rho;
+    }
+  }
+  st->final_residual = resid;
+  st->converged = resid < s->res_tol || resid < s->res_red * st->initial_residual;
+  free(p); free(zz); free(qv); free(r);
+  return cv.diverged ? 2 : 0;
+}
+
+/* ----------------------------------------------------------------------- */
+/* exported API                                                             */
+/* ----------------------------------------------------------------------- */
+orc_problem* orc_create(int geometry, int n_refine, int order, int supg, int linear_quad, int mg_levels,
+                        int example, double eps, const double* b, double c, double f0) {
+  orc_problem* P = calloc(1, sizeof(orc_problem));
+  omesh* meshes = calloc(n_refine + 1, sizeof(omesh));
+  default_mesh(geometry, &meshes[0]);
+  for(int i = 0; i < n_refine; ++i) refine(&meshes[i], &meshes[i + 1]);
+  int keep = mg_levels < 1 ? 1 : mg_levels;
+  if(keep > n_refine + 1) keep = n_refine + 1;
+  P->nlev = keep;
+  P->lev = calloc(keep, sizeof(olevel));
+  const int shape = meshes[0].shape;
+  P->el = make_elem(shape, order);
+  P->qd = make_quad(shape, (linear_quad ? 2 : 3) * order);
+  P->ex.kind = example;
+  P->ex.eps = eps;
+  for(int k = 0; k < 3; ++k) P->ex.b[k] = b ? b[k] : 0.0;
+  P->ex.c = c;
+  P->ex.f0 = f0;
+  P->supg = supg;
+  for(int i = 0; i <= n_refine; ++i) {
+    int l = i - (n_refine + 1 - keep);
+    if(l < 0) { mesh_free(&meshes[i]); continue; }
+    olevel* L = &P->lev[l];
+    L->m = meshes[i];
+    build_neighbours(L);
+    build_space(P, L);
+    build_structure(L);
+    L->mg_sol = calloc(L->ndof, sizeof(double));
+    L->mg_rhs = calloc(L->ndof, sizeof(double));
+    L->mg_def = calloc(L->ndof, sizeof(double));
+    L->mg_aux = calloc(L->ndof, sizeof(double));
+  }
+  free(meshes);
+  return P;
+}
+
+void orc_destroy(orc_problem* P) {
+  if(!P) return;
+  for(int l = 0; l < P->nlev; ++l) {
+    olevel* L = &P->lev[l];
+    mesh_free(&L->m);
+    free(L->dof); free(L->rp); free(L->col); free(L->val); free(L->rhs); free(L->sol);
+    free(L->mg_sol); free(L->mg_rhs); free(L->mg_def); free(L->mg_aux); free(L->lu); free(L->piv);
+    free(L->nb_cell); free(L->nb_joint); free(L->comp);
+  }
+  free(P->lev);
+  free(P);
+}
+
+static olevel* lvl(orc_problem* P, int l) { return &P->lev[l < 0 ? P->nlev + l : l]; }
+
+void orc_info(orc_problem* P, int level, long long* info) {
+  olevel* L = lvl(P, level);
+  info[0] = L->m.ncells; info[1] = L->ndof; info[2] = L->nactive; info[3] = L->nnz;
+  info[4] = L->nl; info[5] = P->qd.n; info[6] = P->nlev; info[7] = L->ndir;
+}
+void orc_dofmap(orc_problem* P, int level, int* out) {
+  olevel* L = lvl(P, level);
+  memcpy(out, L->dof, sizeof(int) * L->m.ncells * L->nl);
+}
+void orc_pattern(orc_problem* P, int level, int* rp, int* col) {
+  olevel* L = lvl(P, level);
+  memcpy(rp, L->rp, sizeof(int) * (L->ndof + 1));
+  memcpy(col, L->col, sizeof(int) * L->nnz);
+}
+void orc_cell_coords(orc_problem* P, int level, double* out) {
+  olevel* L = lvl(P, level);
+  const int nv = NV[L->m.shape];
+  for(int c = 0; c < L->m.ncells; ++c)
+    for(int a = 0; a < nv; ++a) memcpy(&out[(c * nv + a) * 3], &L->m.X[3 * L->m.cv[c * nv + a]], 24);
+}
+void orc_assemble(orc_problem* P) {
+  for(int l = 0; l < P->nlev; ++l) assemble_level(P, &P->lev[l]);
+  olevel* F = lvl(P, -1);
+  memset(F->sol, 0, sizeof(double) * F->nactive);
+  memcpy(F->sol + F->nactive, F->rhs + F->nactive, sizeof(double) * (F->ndof - F->nactive));
+}
+void orc_matrix(orc_problem* P, int level, double* v) { olevel* L = lvl(P, level); memcpy(v, L->val, sizeof(double) * L->nnz); }
+void orc_rhs(orc_problem* P, int level, double* v) { olevel* L = lvl(P, level); memcpy(v, L->rhs, sizeof(double) * L->ndof); }
+void orc_solution(orc_problem* P, double* v) { olevel* L = lvl(P, -1); memcpy(v, L->sol, sizeof(double) * L->ndof); }
+void orc_set_solution(orc_problem* P, const double* v) { olevel* L = lvl(P, -1); memcpy(L->sol, v, sizeof(double) * L->ndof); }
+void orc_apply(orc_problem* P, int level, const double* x, double* y, double a, int mode) {
+  olevel* L = lvl(P, level);
+  if(mode == 0) apply(L, x, y);
+  else apply_scaled_add(L, x, y, a);
+}
+
+int orc_solve(orc_problem* P, const osolver* s, ostats* st) {
+  memset(st, 0, sizeof(*st));
+  if(s->method == 0 || (s->precond == 2 && P->nlev > 1)) dense_factor(&P->lev[0]);
+  if(s->method == 0) {
+    olevel* F = lvl(P, -1);
+    dense_solve(F, F->rhs, F->sol);
+    st->converged = 1;
+    return 0;
+  }
+  if(s->method == 1) return solve_cg(P, s, st);
+  return solve_fgmres(P, s, st);
+}
+
+void orc_vcycle(orc_problem* P, const osolver* s, const double* v, double* z) {
+  if(P->nlev > 1 && !P->lev[0].lu) dense_factor(&P->lev[0]);
+  mg_cycle(P, s, v, z);
+}
+
+/* L2, H1-semi, SD, DG, Linf */
+void orc_errors(orc_problem* P, double* out) {
+  olevel* L = lvl(P, -1);
+  const omesh* m = &L->m;
+  const int nl = L->nl, d = P->el.dim;
+  const oquad* q = &P->qd;
+  static double ref[MAXQ][10 * MAXNL];
+  for(int k = 0; k < q->n; ++k) elem_eval(&P->el, q->x[k], ref[k]);
+  double err[5] = {0, 0, 0, 0, 0};
+  double o[7][MAXNL];
+  for(int c = 0; c < m->ncells; ++c) {
+    ogeo g;
+    geo(m, c, &g);
+    double U[MAXNL], Le[5] = {0, 0, 0, 0, 0};
+    for(int l = 0; l < nl; ++l) U[l] = L->sol[L->dof[c * nl + l]];
+    for(int k = 0; k < q->n; ++k) {
+      orig_values(P, &g, ref[k], o);
+      double v[4] = {0, 0, 0, 0};
+      for(int t = 0; t < 4; ++t)
+        for(int l = 0; l < nl; ++l) v[t] += U[l] * o[t][l];
+      double x[3], ex[4], cf[6];
+      geo_point(&g, d, q->x[k], x);
+      const double w = q->w[k] * fabs(g.det);
+      ex_exact(&P->ex, x[0], x[1], d == 3 ? x[2] : 0, ex);
+      ex_coeffs(&P->ex, x[0], x[1], d == 3 ? x[2] : 0, cf);
+      const double nu = cf[0], b1 = cf[1], b2 = cf[2], b3 = d == 3 ? cf[3] : 0, cr = cf[4];
+      const double delta = sdfem_delta(d, g.hK, nu, b1, b2, b3);
+      const double e0 = v[0] - ex[0], e1 = v[1] - ex[1], e2 = v[2] - ex[2], e3 = d == 2 ? 0. : v[3] - ex[3];
+      double e4 = b1 * e1 + b2 * e2;
+      if(d == 3) e4 += b3 * e3;
+      Le[0] += w * e0 * e0;
+      if(fabs(e0) > Le[4]) Le[4] = fabs(e0);
+      Le[1] += w * e1 * e1;
+      Le[1] += w * e2 * e2;
+      Le[1] += w * e3 * e3;
+      Le[2] += w * (nu * (e1 * e1 + e2 * e2 + e3 * e3) + cr * e0 * e0 + delta * e4 * e4);
+      Le[3] += w * (nu * (e1 * e1 + e2 * e2 + e3 * e3) + (e4 + cr * e0) * e0);
+    }
+    for(int k = 0; k < 4; ++k) err[k] += Le[k];
+    if(err[4] < Le[4]) err[4] = Le[4];
+  }
+  for(int k = 0; k < 4; ++k) out[k] = sqrt(err[k]);
+  out[4] = err[4];
+}
+
+/* standalone kernels on caller-provided CSR data (reference semantics) */
+void orc_spmv_csr(int n, int n_active, const int* rp, const int* col, const double* val, const double* x,
+                  double* y, double a) {
+  for(int i = 0; i < n_active; ++i) {
+    double v = 0;
+    for(int j = rp[i]; j < rp[i + 1]; ++j) v += val[j] * x[col[j]];
+    y[i] += a * v;
+  }
+  for(int i = n_active; i < n; ++i) y[i] += a * x[i];
+}
+
+double orc_ddot(int n, const double* x, const double* y) { return ddot(n, x, y); }
+
+/* basis values at a reference point (for closed-form checks) */
+void orc_basis(int shape, int order, const double* p, double* out) {
+  oelem e = make_elem(shape, order);
+  elem_eval(&e, p, out);
+}
+
+int orc_quad(int shape, int degree, double* w, double* x) {
+  oquad q = make_quad(shape, degree);
+  for(int i = 0; i < q.n; ++i) {
+    w[i] = q.w[i];
+    for(int k = 0; k < 3; ++k) x[3 * i + k] = q.x[i][k];
+  }
+  return q.n;
+}

@@ -1,0 +1,188 @@
+"""Benchmark: CD3D solves/s on one MI355X.
+
+Workload (default): unit cube refined 6 times (64^3 hexahedra), Q2 (2,146,689
+DoF), eps = 1e-3, b = (1,1,1)/sqrt(3), SUPG, manufactured u = sin(pi x)
+sin(pi y) sin(pi z); FGMRES(30) right-preconditioned by a 5-level geometric
+multigrid V(2,2) with damped Jacobi (omega 0.8) and a direct coarse solve;
+residual reduction 1e-10. Quadrature: the reference default (degree 3p, 64
+points) or --quad 27 (degree 2p).
+
+One step = one complete solve on device-resident mesh/space/structure:
+assembly of every multigrid level + FGMRES-MG solve. The mesh, DOF maps and
+sparsity are set up once before the timed region.
+
+Prints one JSON line on stdout.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def c2_db(args):
+    s3 = 1 / math.sqrt(3)
+    return {"geo_file": "Default_UnitCube_Hexa", "ANSATZ_ORDER": 2,
+            "refinement_n_initial_steps": args.refine, "example": "sine_3d",
+            "space_discretization_type": "supg", "diffusion_coefficient": 1e-3,
+            "convection": (s3, s3, s3), "INTERNAL_PROBLEM_LINEAR": 1 if args.quad == 27 else 0,
+            "solver_type": "iterative", "iterative_solver_type": "fgmres",
+            "preconditioner": "multigrid", "multigrid_n_levels": args.levels,
+            "multigrid_n_pre_smooth": 2, "multigrid_n_post_smooth": 2, "jacobi_damp": 0.8,
+            "multigrid_correction_damp_factor": 1.0, "gmres_restart": 30,
+            "residual_reduction": 1e-10, "residual_tolerance": 0.0, "max_n_iterations": args.max_iter,
+            "multigrid_mixed_precision": 1 if args.mixed else 0}
+
+
+def cpu_baseline(db, gpu_iters, sample_div):
+    """Oracle (scalar C port of the reference path) timed on a bounded sample of
+    the same workload: assembly of 1/sample_div of the finest cells and two
+    FGMRES-MG iterations at full size, extrapolated to one full solve."""
+    import numpy as np  # noqa: F401
+    import oracle
+    import parmoon2022_amd as pm
+    t0 = time.time()
+    o = oracle.Oracle(pm.problem_desc(db))
+    setup_s = time.time() - t0
+    info = o.info()
+    ncell = info["n_cells"]
+    nsample = max(1, ncell // sample_div)
+    t0 = time.time()
+    oracle.lib().orc_assemble_cells(o._p, 0, nsample)
+    t_cells = time.time() - t0
+    # full assembly for the solve (untimed) -- the sample above times the cost
+    t0 = time.time()
+    o.assemble()
+    t_full_asm = time.time() - t0
+    sd = pm.solver_desc(db)
+    sd.max_iter = 2
+    t0 = time.time()
+    o.solve(sd)
+    t_two = time.time() - t0
+    per_iter = t_two / 2
+    # all multigrid levels are assembled: scale the finest-level sample by the
+    # cell count of the hierarchy (sum over levels of 8^-k)
+    levels = int(db["multigrid_n_levels"])
+    level_factor = sum(8.0 ** -k for k in range(levels))
+    t_asm = t_cells * (ncell / nsample) * level_factor
+    t_solve = t_asm + gpu_iters * per_iter
+    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": ("oracle (C, -O2, 1 thread) on the same C2 problem: assembly of %d of %d finest "
+                       "cells (%.2f s) scaled to all %d levels, and 2 FGMRES+MG iterations at full size "
+                       "(%.2f s/iteration) scaled to the GPU's %d iterations; setup %.1f s untimed, "
+                       "full oracle assembly %.1f s" % (nsample, ncell, t_cells, levels, per_iter,
+                                                        gpu_iters, setup_s, t_full_asm)),
+            "seconds_per_solve": t_solve}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--refine", type=int, default=6)
+    ap.add_argument("--levels", type=int, default=5)
+    ap.add_argument("--quad", type=int, default=64, choices=[64, 27])
+    ap.add_argument("--mixed", action="store_true", help="FP32 V-cycle inside FP64 FGMRES")
+    ap.add_argument("--max-iter", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(instances shard across GPUs without communication); run with --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import parmoon2022_amd as pm
+
+    db = c2_db(args)
+    t0 = time.time()
+    cd = pm.ConvectionDiffusion(db)
+    log("setup %.1f s" % (time.time() - t0), cd.info())
+    info = cd.info()
+    n, nnz = info["n_dof"], info["nnz"]
+
+    lat = []
+    iters = []
+    for step in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        cd.assemble()
+        st = cd.solve()
+        dt = time.perf_counter() - t0
+        log("step %d: %.3f s (assembly %.2f ms, %d iterations, reduction %.2e)" %
+            (step, dt, cd.assemble_ms, st.iterations, st.final_residual / st.initial_residual))
+        if step >= args.warmup:
+            lat.append(dt)
+            iters.append(st.iterations)
+    total = sum(lat)
+    errs = cd.output()
+    value = args.steps / total
+
+    # dominant kernels, timed with HIP events on the system stream
+    t_jac = cd.time_kernel("jacobi_f32" if args.mixed else "jacobi_f64", -1, 20)
+    t_spmv = cd.time_kernel("spmv_f64", -1, 20)
+    t_asm = cd.time_kernel("assemble_local", -1, 3)
+    vb = 4 if args.mixed else 8
+    ib = 4
+    jac_bytes = (vb + ib) * nnz + 4 * (n + 1) + vb * n * 5
+    spmv_bytes = 12 * nnz + 4 * (n + 1) + 16 * n
+    n_cells = info["n_cells"]
+    nq = info["n_quad"]
+    asm_flops = (2 * 27 * 27 * 4 + 2 * 4 * 5 * 27) * nq * n_cells
+    achieved = jac_bytes / (t_jac * 1e-3) / 1e9
+    result = {
+        "metric": "CD3D solves/s (C2: 64^3 Q2 SUPG, FGMRES(30)+MG V(2,2) Jacobi, rel. res 1e-10)",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32-vcycle/f64" if args.mixed else "f64",
+        "data": "synthetic (manufactured solution, generated mesh)",
+        "config": {"workload": "C2 CD3D Q2 unit cube %d^3, %d DoF, nnz %d, %d-point quadrature, %d MG levels"
+                               % (2 ** args.refine, n, nnz, nq, info["n_levels"]),
+                   "dof": n, "nnz": nnz, "quad_points": nq, "mixed_precision": bool(args.mixed)},
+        "dof_per_s": value * n,
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "iterations": int(np.median(iters)),
+        "l2_error": float(errs[0]),
+        "h1_error": float(errs[1]),
+        "roofline": {"bound": "hbm", "kernel": "k_jacobi (finest level)", "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": None, "algorithmic_bytes": jac_bytes, "avg_ms": t_jac},
+        "kernels": {"spmv_f64_ms": t_spmv, "spmv_f64_GBs": spmv_bytes / (t_spmv * 1e-3) / 1e9,
+                    "assemble_local_ms": t_asm,
+                    "assemble_local_TFLOPs": asm_flops / (t_asm * 1e-3) / 1e12,
+                    "assemble_fp64_frac": asm_flops / (t_asm * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
+    }
+    cd.close()
+    if not args.no_cpu:
+        try:
+            cb = cpu_baseline(db, int(np.median(iters)), args.cpu_sample)
+            result["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+            result["gpu_over_cpu"] = value / cb["value"]
+        except Exception as ex:  # the baseline is reported, not required
+            log("cpu baseline failed:", ex)
+            result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

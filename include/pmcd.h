@@ -149,6 +149,12 @@ int pmcd_apply(pmcd_system* s, int level, const double* x, double* y, double a, 
  * settings (Iteration_multigrid::apply, Iteration_multigrid.C:23-34). */
 int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, double* z);
 
+/* Average device time (HIP events on the system's stream) of `iters`
+ * back-to-back launches of one kernel on level `level`:
+ * 0 FP64 SpMV, 1 FP64 Jacobi sweep, 2 FP32 SpMV, 3 local-matrix assembly,
+ * 4 row gather, 5 FP32 Jacobi sweep. No reference counterpart (measurement). */
+int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* avg_ms);
+
 /* Device-pointer entry points (all pointers are device memory, `stream` a
  * hipStream_t or NULL). FEMatrix::multiplyActive + addScaledNonActive:
  * y[r] = beta*z[r] + alpha*(A x)[r] for r < n_active, beta*z[r] + alpha*x[r]

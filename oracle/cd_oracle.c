@@ -631,6 +631,9 @@ typedef struct {
   double *mg_sol, *mg_rhs, *mg_def, *mg_aux;
   double* lu; /* dense LU of the coarse/direct matrix */
   int* piv;
+  double* qq;   /* per child number: coarse basis at the child's nodes */
+  int qq_ready[8];
+  double* diag; /* diagonal entries (Iteration_jacobi's copy) */
   int* nb_cell; /* face neighbours: nb_cell[c*NJ+j], -1 boundary */
   int* nb_joint;
   int* comp;    /* boundary component per joint, -1 interior */
@@ -898,6 +901,13 @@ static void build_space(orc_problem* P, olevel* L) {
   int* Counter = &Counters[NBT];
   int* G = malloc(sizeof(int) * N);
   for(int i = 0; i < N; ++i) G[i] = -1;
+  /* nodal point coordinates of every (cell, local dof) */
+  double* NX = malloc(sizeof(double) * 3 * (size_t)N);
+  for(int c = 0; c < m->ncells; ++c) {
+    ogeo g;
+    geo(m, c, &g);
+    for(int i = 0; i < nl; ++i) geo_point(&g, P->el.dim, P->el.node[i], &NX[3 * ((size_t)c * nl + i)]);
+  }
   for(int c = 0; c < m->ncells; ++c) {
     for(int j = 0; j < nj; ++j) {
       const int nb = L->nb_cell[c * nj + j];
@@ -908,11 +918,9 @@ static void build_space(orc_problem* P, olevel* L) {
         const int jn = L->nb_joint[c * nj + j];
         /* pair joint DOFs by coinciding nodal points */
         for(int k = 0; k < njd[j]; ++k) {
-          double x0[3];
-          node_xyz(P, m, c, jd[j][k], x0);
+          const double* x0 = &NX[3 * ((size_t)c * nl + jd[j][k])];
           for(int kk = 0; kk < njd[jn]; ++kk) {
-            double x1[3];
-            node_xyz(P, m, nb, jd[jn][kk], x1);
+            const double* x1 = &NX[3 * ((size_t)nb * nl + jd[jn][kk])];
             if(fabs(x0[0] - x1[0]) + fabs(x0[1] - x1[1]) + fabs(x0[2] - x1[2]) < 1e-12) {
               map_single_dof(G, c * nl + jd[j][k], nb * nl + jd[jn][kk], Counter);
               break;
@@ -930,6 +938,7 @@ static void build_space(orc_problem* P, olevel* L) {
       if(!on) { (*Counter)--; G[c * nl + i] = *Counter; }
     }
   }
+  free(NX);
   /* compute_various_numbers (no hanging nodes) */
   int total = 0;
   for(int k = 0; k < NBT; ++k) total += BUB[k];
@@ -1069,7 +1078,7 @@ static void orig_values(const orc_problem* P, const ogeo* g, const double* ref, 
   }
 }
 
-static void assemble_level(orc_problem* P, olevel* L) {
+static void assemble_level(orc_problem* P, olevel* L, int c_begin, int c_end) {
   const omesh* m = &L->m;
   const int nl = L->nl, d = P->el.dim, s = m->shape, nj = NJ[s];
   const oquad* q = &P->qd;
@@ -1079,7 +1088,7 @@ static void assemble_level(orc_problem* P, olevel* L) {
   for(int k = 0; k < q->n; ++k) elem_eval(&P->el, q->x[k], ref[k]);
   double A[MAXNL][MAXNL], F[MAXNL];
   double o[7][MAXNL];
-  for(int c = 0; c < m->ncells; ++c) {
+  for(int c = c_begin; c < c_end; ++c) {
     ogeo g;
     geo(m, c, &g);
     memset(A, 0, sizeof(A));
@@ -1305,16 +1314,34 @@ static void child_matrix(const orc_problem* P, const olevel* C, const olevel* F,
   }
 }
 
-static void prolongate(const orc_problem* P, const olevel* C, const olevel* F, const double* cf, double* ff) {
+/* QQ depends only on the element and the child number on uniformly refined
+   affine meshes (the reference caches it per (element, refinement, child),
+   FEDatabase::GetProlongationMatrix3D); children of a cell are stored
+   consecutively, so the child number is fc % n_children. */
+static const double* child_qq(const orc_problem* P, const olevel* C, olevel* F, int fc) {
+  const int nch = F->m.ncells / C->m.ncells, nl = P->el.nl;
+  const int j = fc % nch;
+  if(!F->qq) {
+    F->qq = malloc(sizeof(double) * 8 * nl * nl);
+    for(int k = 0; k < 8; ++k) F->qq_ready[k] = 0;
+  }
+  if(!F->qq_ready[j]) {
+    child_matrix(P, C, F, fc, &F->qq[j * nl * nl]);
+    F->qq_ready[j] = 1;
+  }
+  return &F->qq[j * nl * nl];
+}
+
+static void prolongate(const orc_problem* P, const olevel* C, olevel* F, const double* cf, double* ff) {
   const int nl = P->el.nl;
   double* aux = calloc(F->ndof, sizeof(double));
   memset(ff, 0, sizeof(double) * F->ndof);
   for(int i = 0; i < F->m.ncells; ++i)
     for(int j = 0; j < nl; ++j) aux[F->dof[i * nl + j]] += 1;
-  double QQ[MAXNL * MAXNL], Val[MAXNL];
+  double Val[MAXNL];
   for(int fc = 0; fc < F->m.ncells; ++fc) {
     const int pc = F->m.parent[fc];
-    child_matrix(P, C, F, fc, QQ);
+    const double* QQ = child_qq(P, C, F, fc);
     for(int l = 0; l < nl; ++l) Val[l] = cf[C->dof[pc * nl + l]];
     for(int k = 0; k < nl; ++k) {
       double s = 0;
@@ -1326,7 +1353,7 @@ static void prolongate(const orc_problem* P, const olevel* C, const olevel* F, c
   free(aux);
 }
 
-static void restrict_defect(const orc_problem* P, const olevel* C, const olevel* F, double* cf, const double* ff) {
+static void restrict_defect(const orc_problem* P, const olevel* C, olevel* F, double* cf, const double* ff) {
   const int nl = P->el.nl;
   double* aux = calloc(F->ndof, sizeof(double));
   double* copy = malloc(sizeof(double) * F->ndof);
@@ -1335,10 +1362,10 @@ static void restrict_defect(const orc_problem* P, const olevel* C, const olevel*
   for(int i = 0; i < F->m.ncells; ++i)
     for(int j = 0; j < nl; ++j) aux[F->dof[i * nl + j]] += 1;
   for(int i = 0; i < F->ndof; ++i) copy[i] /= aux[i];
-  double QQ[MAXNL * MAXNL], Val[MAXNL];
+  double Val[MAXNL];
   for(int fc = 0; fc < F->m.ncells; ++fc) {
     const int pc = F->m.parent[fc];
-    child_matrix(P, C, F, fc, QQ);
+    const double* QQ = child_qq(P, C, F, fc);
     for(int l = 0; l < nl; ++l) Val[l] = copy[F->dof[fc * nl + l]];
     for(int k = 0; k < nl; ++k) {
       double s = 0;
@@ -1389,7 +1416,7 @@ static void jacobi_smooth(const olevel* L, const double* b, double* x, double om
   double* d = malloc(sizeof(double) * n);
   memcpy(d, b, sizeof(double) * n);
   apply_scaled_add(L, x, d, -1.0);
-  for(int i = 0; i < n; ++i) d[i] /= diag_of(L, i);
+  for(int i = 0; i < n; ++i) d[i] /= L->diag[i];
   for(int i = 0; i < n; ++i) x[i] += omega * d[i];
   free(d);
 }
@@ -1431,7 +1458,7 @@ static void precond(orc_problem* P, const osolver* s, const double* v, double* z
   olevel* F = &P->lev[P->nlev - 1];
   if(s->precond == 0) memcpy(z, v, sizeof(double) * F->ndof);
   else if(s->precond == 1)
-    for(int i = 0; i < F->ndof; ++i) z[i] = v[i] / diag_of(F, i);
+    for(int i = 0; i < F->ndof; ++i) z[i] = v[i] / F->diag[i];
   else mg_cycle(P, s, v, z);
 }
 
@@ -1614,7 +1641,7 @@ void orc_destroy(orc_problem* P) {
     mesh_free(&L->m);
     free(L->dof); free(L->rp); free(L->col); free(L->val); free(L->rhs); free(L->sol);
     free(L->mg_sol); free(L->mg_rhs); free(L->mg_def); free(L->mg_aux); free(L->lu); free(L->piv);
-    free(L->nb_cell); free(L->nb_joint); free(L->comp);
+    free(L->nb_cell); free(L->nb_joint); free(L->comp); free(L->qq); free(L->diag);
   }
   free(P->lev);
   free(P);
@@ -1642,8 +1669,13 @@ void orc_cell_coords(orc_problem* P, int level, double* out) {
   for(int c = 0; c < L->m.ncells; ++c)
     for(int a = 0; a < nv; ++a) memcpy(&out[(c * nv + a) * 3], &L->m.X[3 * L->m.cv[c * nv + a]], 24);
 }
+static void extract_diag(olevel* L) {
+  if(!L->diag) L->diag = malloc(sizeof(double) * L->ndof);
+  for(int r = 0; r < L->ndof; ++r) L->diag[r] = diag_of(L, r);
+}
+
 void orc_assemble(orc_problem* P) {
-  for(int l = 0; l < P->nlev; ++l) assemble_level(P, &P->lev[l]);
+  for(int l = 0; l < P->nlev; ++l) { assemble_level(P, &P->lev[l], 0, P->lev[l].m.ncells); extract_diag(&P->lev[l]); }
   olevel* F = lvl(P, -1);
   memset(F->sol, 0, sizeof(double) * F->nactive);
   memcpy(F->sol + F->nactive, F->rhs + F->nactive, sizeof(double) * (F->ndof - F->nactive));
@@ -1747,4 +1779,12 @@ int orc_quad(int shape, int degree, double* w, double* x) {
     for(int k = 0; k < 3; ++k) x[3 * i + k] = q.x[i][k];
   }
   return q.n;
+}
+
+/* timing samples for the CPU baseline: assemble cells [c0, c1) of the finest
+   level into its (zeroed) matrix, and run `iters` FGMRES iterations */
+void orc_assemble_cells(orc_problem* P, int c0, int c1) {
+  olevel* L = lvl(P, -1);
+  if(c1 > L->m.ncells) c1 = L->m.ncells;
+  assemble_level(P, L, c0, c1);
 }

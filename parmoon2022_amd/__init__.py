@@ -60,8 +60,11 @@ EXPORTS = [
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
     "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
     "pmcd_get_rhs", "pmcd_solve", "pmcd_get_solution", "pmcd_set_solution", "pmcd_errors",
-    "pmcd_apply", "pmcd_vcycle", "pmcd_spmv_device", "pmcd_dot_device",
+    "pmcd_apply", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_spmv_device", "pmcd_dot_device",
 ]
+
+KERNELS = {"spmv_f64": 0, "jacobi_f64": 1, "spmv_f32": 2, "assemble_local": 3, "gather_rows": 4,
+           "jacobi_f32": 5}
 
 
 def lib():
@@ -271,6 +274,12 @@ class ConvectionDiffusion:
         _check(lib().pmcd_apply(self._s, level, _ptr(x, ctypes.c_double),
                                 _ptr(out, ctypes.c_double), ctypes.c_double(a), mode))
         return out
+
+    def time_kernel(self, name, level=-1, iters=20):
+        """Average device time (ms) of one kernel from HIP events on the system stream."""
+        ms = ctypes.c_double()
+        _check(lib().pmcd_time_kernel(self._s, KERNELS[name], level, iters, ctypes.byref(ms)))
+        return ms.value
 
     def vcycle(self, v, db=None):
         sd = solver_desc(self.db if db is None else db)

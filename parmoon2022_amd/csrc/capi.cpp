@@ -302,6 +302,13 @@ int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, dou
   });
 }
 
+int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* avg_ms) {
+  return guarded([&] {
+    PMCD_CHECK(iters > 0, "iters must be positive");
+    s->sys->time_kernel(which, level, iters, avg_ms);
+  });
+}
+
 int pmcd_spmv_device(int n, int n_active, const int* row_ptr, const int* col, const double* val,
                      const double* x, const double* z, double* y, double alpha, double beta,
                      void* stream) {

@@ -352,6 +352,52 @@ void CDSystem::coarse_factorize() {
   coarse_ready_ = true;
 }
 
+// HIP-event timing of one kernel class on this system's stream.
+void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
+  if(level < 0) level += n_levels();
+  PMCD_CHECK(level >= 0 && level < n_levels(), "level out of range");
+  Level& L = *levels_[level];
+  const int n = L.n();
+  DevBuf<double> x(n), y(n), b(n);
+  launch_fill(n, 1.0, x.p, stream_);
+  launch_fill(n, 0.5, b.p, stream_);
+  DevBuf<float> xf, yf, bf;
+  if(which == 2 || which == 5) {
+    PMCD_CHECK(L.val32.n == L.val.n, "FP32 matrix copies not prepared");
+    xf.alloc(n); yf.alloc(n); bf.alloc(n);
+    launch_fill_f32(n, 1.0f, xf.p, stream_);
+    launch_fill_f32(n, 0.5f, bf.p, stream_);
+  }
+  AsmArgs a = asm_args(L);
+  auto launch = [&]() {
+    switch(which) {
+      case 0: launch_spmv(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x.p, nullptr, y.p, 1.0, 0.0, stream_); break;
+      case 1: launch_jacobi(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b.p, x.p, y.p, 0.8, stream_); break;
+      case 2: launch_spmv_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, xf.p, nullptr, yf.p, 1.0f, 0.0f, stream_); break;
+      case 3: launch_assemble_local(a, stream_); break;
+      case 4:
+        launch_gather_matrix(n, L.n_active(), a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p, L.contrib.p,
+                             L.slot.p, local_A_.p, L.val.p, stream_);
+        break;
+      case 5: launch_jacobi_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, bf.p, xf.p, yf.p, 0.8f, stream_); break;
+      default: PMCD_FAIL("unknown kernel id ", which);
+    }
+  };
+  launch();  // warm-up
+  hipEvent_t e0, e1;
+  PMCD_HIP(hipEventCreate(&e0));
+  PMCD_HIP(hipEventCreate(&e1));
+  PMCD_HIP(hipEventRecord(e0, stream_));
+  for(int i = 0; i < iters; ++i) launch();
+  PMCD_HIP(hipEventRecord(e1, stream_));
+  PMCD_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  PMCD_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = ms / iters;
+}
+
 void CDSystem::apply(Level& L, const double* x, double* y) {
   launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, nullptr, y, 1.0, 0.0, stream_);
 }

@@ -106,6 +106,7 @@ class CDSystem {
   void vcycle(const double* rhs, double* out);                     // FP64 or FP32 V-cycle
   void set_solver(const SolverDesc& s) { sd_ = s; }
   void prepare(const SolverDesc& s);      // coarse factorization, FP32 copies
+  void time_kernel(int which, int level, int iters, double* avg_ms);
 
  private:
   void build_level_device(Level& L);

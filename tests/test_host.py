@@ -1,0 +1,79 @@
+"""Host-side parity (no GPU): the library loads and exports the C ABI, and the
+host setup (meshes, DOF numbering, CSR pattern) is bit-identical to the
+oracle's union-find restatement of the reference."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+import parmoon2022_amd as pm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = pm.lib()
+    header = open(os.path.join(ROOT, "include", "pmcd.h")).read()
+    names = set(re.findall(r"\b(pmcd_[a-z0-9_]+)\s*\(", header))
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(pm.EXPORTS) <= names
+
+
+CASES = [("TwoTriangles", 1, 4, "sine_laplace_2d"), ("TwoTriangles2", 2, 3, "sine_laplace_2d"),
+         ("UnitSquare", 1, 4, "sine_laplace_2d"), ("UnitSquare", 2, 3, "sine_2d"),
+         ("Default_UnitCube_Hexa", 1, 3, "sine_3d"), ("Default_UnitCube_Hexa", 2, 3, "sine_3d"),
+         ("Default_UnitCube_Tetra", 1, 3, "const_rhs"), ("Default_UnitCube_Tetra", 2, 2, "const_rhs")]
+
+
+@pytest.mark.parametrize("geo,order,nref,ex", CASES)
+def test_dof_numbering_and_pattern_match_oracle(oracle_lib, geo, order, nref, ex):
+    db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": nref, "example": ex,
+          "preconditioner": "multigrid", "solver_type": "iterative", "multigrid_n_levels": 3}
+    h = pm.HostSetup(db)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    for level in range(h.info()["n_levels"]):
+        hi, oi = h.info(level), o.info(level)
+        assert hi == oi
+        assert np.array_equal(h.dofmap(level), o.dofmap(level))
+        rp1, c1 = h.pattern(level)
+        rp2, c2 = o.pattern(level)
+        assert np.array_equal(rp1, rp2) and np.array_equal(c1, c2)
+
+
+def test_c1_counts():
+    """C1: TwoTriangles refined 8 times, P1: 66049 DoF, 1024 Dirichlet, nnz 460289."""
+    h = pm.HostSetup({"geo_file": "TwoTriangles", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 8,
+                      "example": "sine_2d"})
+    i = h.info()
+    assert (i["n_cells"], i["n_dof"], i["n_dirichlet"], i["nnz"]) == (131072, 66049, 1024, 460289)
+
+
+def test_q2_hierarchy_counts():
+    """Q2 unit cube levels 4^3 .. 32^3: DoF (2N+1)^3 and nnz (8N+1)^3 (1D Q2 pattern has 8N+1 entries)."""
+    h = pm.HostSetup({"geo_file": "Default_UnitCube_Hexa", "ANSATZ_ORDER": 2,
+                      "refinement_n_initial_steps": 5, "example": "sine_3d", "solver_type": "iterative",
+                      "preconditioner": "multigrid", "multigrid_n_levels": 4})
+    for level, n in enumerate([4, 8, 16, 32]):
+        i = h.info(level)
+        assert i["n_dof"] == (2 * n + 1) ** 3 and i["nnz"] == (8 * n + 1) ** 3
+
+
+def test_dirichlet_dofs_last_and_sorted_columns():
+    h = pm.HostSetup({"geo_file": "Default_UnitCube_Hexa", "ANSATZ_ORDER": 2,
+                      "refinement_n_initial_steps": 2, "example": "sine_3d"})
+    i = h.info()
+    rp, col = h.pattern()
+    for r in range(i["n_dof"]):
+        seg = col[rp[r]:rp[r + 1]]
+        assert np.all(np.diff(seg) > 0) and r in seg
+    # boundary nodes of the unit cube are the Dirichlet DOFs: (2N+1)^3 - (2N-1)^3 with N = 4
+    assert i["n_dirichlet"] == 9 ** 3 - 7 ** 3
+
+
+def test_invalid_input_raises():
+    with pytest.raises(RuntimeError):
+        pm.HostSetup({"geo_file": "UnitSquare", "ANSATZ_ORDER": 3, "example": "sine_2d"})

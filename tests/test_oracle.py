@@ -1,0 +1,142 @@
+"""Pins the oracle (CPU restatement of the reference path) to the reference's
+own fixtures and to closed forms, so it can serve as the parity checker."""
+import numpy as np
+import pytest
+
+import parmoon2022_amd as pm
+
+# test/cd2d_test.c++:95-96,99-101,159-161,164-166 : L2, H1-semi, SD, L_inf;
+# tolerance 1e-11 absolute (cd2d_test.c++:45)
+CD2D_GOLDEN = {
+    ("UnitSquare", 1): [0.030136750989447, 0.50136473756928, 0.50136473756928, 0.086620472046938],
+    ("UnitSquare", 2): [0.0019271846609828, 0.050971302154831, 0.050971302154831, 0.0033739148851507],
+    ("TwoTriangles", 1): [0.068969534539772, 0.83003777179706, 0.83003777179706, 0.16974869712877],
+    ("TwoTriangles", 2): [0.00421775248811, 0.12664905657584, 0.12664905657584, 0.013686550806003],
+    ("TwoTriangles2", 1): [0.068969534539772, 0.83003777179706, 0.83003777179706, 0.16974869712877],
+    ("TwoTriangles2", 2): [0.00421775248811, 0.12664905657584, 0.12664905657584, 0.013686550806003],
+}
+
+
+def cd2d_db(geo, order):
+    return {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 2,
+            "example": "sine_laplace_2d", "solver_type": "direct"}
+
+
+@pytest.mark.parametrize("geo,order", list(CD2D_GOLDEN))
+def test_oracle_cd2d_golden_direct(oracle_lib, geo, order):
+    db = cd2d_db(geo, order)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    o.solve(pm.solver_desc(db))
+    e = o.errors()
+    g = CD2D_GOLDEN[(geo, order)]
+    got = [e[0], e[1], e[2], e[4]]
+    assert np.allclose(got, g, rtol=0, atol=1e-11), (got, g)
+
+
+@pytest.mark.parametrize("geo,order", [("UnitSquare", 1), ("TwoTriangles", 2)])
+def test_oracle_cd2d_golden_fgmres_mg(oracle_lib, geo, order):
+    """cd2d_test.c++:255-268 uses FGMRES + multigrid (2 levels, Jacobi 2/2,
+    direct coarse); the errors are the same fixtures. This build's cycle is V."""
+    db = cd2d_db(geo, order)
+    db.update({"solver_type": "iterative", "iterative_solver_type": "fgmres",
+               "preconditioner": "multigrid", "multigrid_n_levels": 2,
+               "multigrid_n_pre_smooth": 2, "multigrid_n_post_smooth": 2,
+               "jacobi_damp": 1.0, "multigrid_correction_damp_factor": 1.0,
+               "residual_tolerance": 1e-13, "max_n_iterations": 1000, "gmres_restart": 20})
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    o.solve(pm.solver_desc(db))
+    e = o.errors()
+    g = CD2D_GOLDEN[(geo, order)]
+    assert np.allclose([e[0], e[1], e[2], e[4]], g, rtol=0, atol=1e-11)
+
+
+@pytest.mark.parametrize("geo", ["Default_UnitCube_Hexa", "Default_UnitCube_Tetra"])
+@pytest.mark.parametrize("example,order", [("test_p0", 1), ("test_p1", 1), ("test_p2", 2)])
+def test_oracle_cd3d_solution_in_space(oracle_lib, geo, example, order):
+    """test/cd3d_solvers_test.c++:187-214: L2 and H1 errors vanish (1e-13)."""
+    db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 2,
+          "example": example, "solver_type": "direct"}
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    o.solve(pm.solver_desc(db))
+    e = o.errors()
+    assert e[0] < 1e-13 and e[1] < 1e-13, e
+
+
+def test_oracle_structure_counts(oracle_lib):
+    """test/structure_test.c++:1431-1480: Q2 on the default hexahedron gives
+    27 rows, 27 columns, 729 entries, 27 per row."""
+    db = {"geo_file": "Default_UnitCube_Hexa", "ANSATZ_ORDER": 2, "refinement_n_initial_steps": 0,
+          "example": "test_p2"}
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    rp, col = o.pattern()
+    assert len(rp) - 1 == 27 and len(col) == 729
+    assert np.all(np.diff(rp) == 27)
+
+
+@pytest.mark.parametrize("shape,order,n", [(0, 1, 3), (0, 2, 6), (1, 1, 4), (1, 2, 9),
+                                           (2, 1, 4), (2, 2, 10), (3, 1, 8), (3, 2, 27)])
+def test_oracle_basis_closed_forms(oracle_lib, shape, order, n):
+    """Lagrange bases: partition of unity, zero derivative sums, nodal
+    property at the nodes (checked through the element's own node set), and
+    first derivatives against central differences."""
+    rng = np.random.default_rng(shape * 10 + order)
+    dim = 2 if shape < 2 else 3
+    ncomp = 6 if dim == 2 else 10
+    for _ in range(5):
+        if shape in (1, 3):
+            p = rng.uniform(-1, 1, 3)
+        else:
+            p = rng.dirichlet(np.ones(dim + 1))[:3] if dim == 3 else np.r_[rng.dirichlet(np.ones(3))[:2], 0]
+        if dim == 2:
+            p[2] = 0
+        v = oracle_lib.basis(shape, order, p)[:ncomp * n].reshape(ncomp, n)
+        assert abs(v[0].sum() - 1) < 1e-13
+        assert np.all(np.abs(v[1:].sum(axis=1)) < 1e-12)
+        h = 1e-6
+        for k in range(dim):
+            dp = np.zeros(3)
+            dp[k] = h
+            fd = (oracle_lib.basis(shape, order, p + dp)[:n] - oracle_lib.basis(shape, order, p - dp)[:n]) / (2 * h)
+            assert np.allclose(fd, v[1 + k], atol=1e-7)
+
+
+@pytest.mark.parametrize("shape,degree,exact_deg", [(0, 3, 5), (0, 6, 9), (1, 3, 5), (1, 6, 7),
+                                                    (2, 3, 4), (2, 6, 5), (3, 3, 5), (3, 6, 7)])
+def test_oracle_quadrature_exactness(oracle_lib, shape, degree, exact_deg):
+    """test/quadrature_test.c++: each rule integrates monomials up to its
+    accuracy exactly (reference cells [-1,1]^d or the unit simplex)."""
+    from math import factorial
+    w, x = oracle_lib.quad_rule(shape, degree)
+    dim = 2 if shape < 2 else 3
+    for a in range(exact_deg + 1):
+        for b in range(exact_deg + 1 - a):
+            for c in range((exact_deg + 1 - a - b) if dim == 3 else 1):
+                val = np.sum(w * x[:, 0] ** a * x[:, 1] ** b * (x[:, 2] ** c if dim == 3 else 1))
+                if shape in (1, 3):
+                    ex = 1.0
+                    for e in ([a, b, c] if dim == 3 else [a, b]):
+                        ex *= 0 if e % 2 else 2.0 / (e + 1)
+                else:
+                    if dim == 2:
+                        ex = factorial(a) * factorial(b) / factorial(a + b + 2)
+                    else:
+                        ex = factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + 3)
+                assert abs(val - ex) < 1e-13, (a, b, c, val, ex)
+
+
+def test_oracle_fgmres_mg_solution_in_space(oracle_lib):
+    """FGMRES + two-level multigrid (cell-wise grid transfer, direct coarse
+    solve) recovers a solution that lies in the FE space."""
+    db = {"geo_file": "Default_UnitCube_Hexa", "ANSATZ_ORDER": 2, "refinement_n_initial_steps": 2,
+          "example": "test_p0", "solver_type": "iterative", "iterative_solver_type": "fgmres",
+          "preconditioner": "multigrid", "multigrid_n_levels": 2, "multigrid_n_pre_smooth": 2,
+          "multigrid_n_post_smooth": 2, "jacobi_damp": 0.8, "residual_tolerance": 1e-13,
+          "max_n_iterations": 100}
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    st = o.solve(pm.solver_desc(db))
+    e = o.errors()
+    assert e[0] < 1e-12 and st.iterations < 50

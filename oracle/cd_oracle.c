@@ -1788,3 +1788,20 @@ void orc_assemble_cells(orc_problem* P, int c0, int c1) {
   if(c1 > L->m.ncells) c1 = L->m.ncells;
   assemble_level(P, L, c0, c1);
 }
+
+/* load a matrix / rhs computed elsewhere (same sparsity) into a level, e.g.
+   to time the solver phase on exactly the system another path assembled */
+void orc_set_matrix(orc_problem* P, int level, const double* v) {
+  olevel* L = lvl(P, level);
+  memcpy(L->val, v, sizeof(double) * L->nnz);
+  extract_diag(L);
+  if(L->lu) { free(L->lu); L->lu = NULL; free(L->piv); L->piv = NULL; }
+}
+void orc_set_rhs(orc_problem* P, int level, const double* v) {
+  olevel* L = lvl(P, level);
+  memcpy(L->rhs, v, sizeof(double) * L->ndof);
+  if(L == lvl(P, -1)) {
+    memset(L->sol, 0, sizeof(double) * L->nactive);
+    memcpy(L->sol + L->nactive, L->rhs + L->nactive, sizeof(double) * (L->ndof - L->nactive));
+  }
+}

@@ -155,6 +155,11 @@ int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, dou
  * 4 row gather, 5 FP32 Jacobi sweep. No reference counterpart (measurement). */
 int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* avg_ms);
 
+/* Launch-shape tuning of the CSR kernels: key 0 = SpMV, 1 = Jacobi sweep;
+ * value = lanes per matrix row (4, 8, 16 or 32). Process-wide. No reference
+ * counterpart (tuning knob used by the autotuning sweep). */
+int pmcd_set_tuning(int key, int value);
+
 /* Device-pointer entry points (all pointers are device memory, `stream` a
  * hipStream_t or NULL). FEMatrix::multiplyActive + addScaledNonActive:
  * y[r] = beta*z[r] + alpha*(A x)[r] for r < n_active, beta*z[r] + alpha*x[r]

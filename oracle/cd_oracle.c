@@ -1399,8 +1399,11 @@ typedef struct {
   int diverged;
 } oconv;
 
+static double g_hist[100000];
+static int g_nhist;
 static int converged(oconv* c, double res, int it) {
-  if(it == 0) { c->initial = res; c->previous = res; }
+  if(it == 0) { c->initial = res; c->previous = res; g_nhist = 0; }
+  if(g_nhist < 100000) g_hist[g_nhist++] = res;
   int small = res < c->s->res_tol, red = res < c->s->res_red * c->initial;
   int enough = it >= c->s->min_iter;
   int stag = res > 1e10 * c->previous, div = res > c->s->div_factor * c->initial;
@@ -1804,4 +1807,11 @@ void orc_set_rhs(orc_problem* P, int level, const double* v) {
     memset(L->sol, 0, sizeof(double) * L->nactive);
     memcpy(L->sol + L->nactive, L->rhs + L->nactive, sizeof(double) * (L->ndof - L->nactive));
   }
+}
+
+/* residual history of the last FGMRES/CG solve (LoopInfo output) */
+int orc_history(double* out, int max) {
+  int n = g_nhist < max ? g_nhist : max;
+  memcpy(out, g_hist, sizeof(double) * n);
+  return n;
 }

@@ -60,8 +60,14 @@ EXPORTS = [
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
     "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
     "pmcd_get_rhs", "pmcd_solve", "pmcd_get_solution", "pmcd_set_solution", "pmcd_errors",
-    "pmcd_apply", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_spmv_device", "pmcd_dot_device",
+    "pmcd_apply", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_set_tuning", "pmcd_spmv_device",
+    "pmcd_dot_device",
 ]
+
+
+def set_tuning(kernel, lanes_per_row):
+    """Lanes per CSR row for 'spmv' or 'jacobi' kernels (4, 8, 16, 32)."""
+    _check(lib().pmcd_set_tuning({"spmv": 0, "jacobi": 1}[kernel], int(lanes_per_row)))
 
 KERNELS = {"spmv_f64": 0, "jacobi_f64": 1, "spmv_f32": 2, "assemble_local": 3, "gather_rows": 4,
            "jacobi_f32": 5}
@@ -288,6 +294,24 @@ class ConvectionDiffusion:
         _check(lib().pmcd_vcycle(self._s, ctypes.byref(sd), _ptr(v, ctypes.c_double),
                                  _ptr(z, ctypes.c_double)))
         return z
+
+
+def autotune(cd, level=-1, iters=10, log=None):
+    """Pick lanes-per-row of the SpMV and Jacobi kernels for this matrix by
+    timing each candidate with HIP events; sets the fastest and returns the
+    timings {kernel: {lanes: ms}}."""
+    out = {}
+    for kernel, timed in (("spmv", "spmv_f64"), ("jacobi", "jacobi_f64")):
+        res = {}
+        for w in (4, 8, 16, 32):
+            set_tuning(kernel, w)
+            res[w] = cd.time_kernel(timed, level, iters)
+        best = min(res, key=res.get)
+        set_tuning(kernel, best)
+        out[kernel] = res
+        if log:
+            log("autotune %s: %s -> %d lanes/row" % (kernel, {k: round(v, 4) for k, v in res.items()}, best))
+    return out
 
 
 def gpu_available():

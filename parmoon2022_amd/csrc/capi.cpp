@@ -309,6 +309,10 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
   });
 }
 
+int pmcd_set_tuning(int key, int value) {
+  return guarded([&] { set_tuning(key, value); });
+}
+
 int pmcd_spmv_device(int n, int n_active, const int* row_ptr, const int* col, const double* val,
                      const double* x, const double* z, double* y, double alpha, double beta,
                      void* stream) {

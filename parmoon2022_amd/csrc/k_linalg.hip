@@ -72,6 +72,23 @@ __global__ __launch_bounds__(256) void k_jacobi(int n, int n_active, const int* 
   }
 }
 
+// First Jacobi sweep from x = 0 with b = 0 on the non-active rows: A x = 0,
+// so the sweep reduces to x_out = 0 + omega * ((b + (-1)*0) / diag) on active
+// rows (the same floating-point result as the general sweep) and x_out = b on
+// non-active rows.
+template <class T>
+__global__ void k_jacobi_from_zero(int n, int n_active, const T* __restrict__ diag,
+                                   const T* __restrict__ b, T* __restrict__ x_out, T omega) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if(i < n_active) {
+      T d = b[i] / diag[i];
+      x_out[i] = T(0) + omega * d;
+    } else {
+      x_out[i] = b[i];
+    }
+  }
+}
+
 template <int W, class T>
 __global__ __launch_bounds__(256) void k_csr(int n, const int* __restrict__ rp,
                                              const int* __restrict__ col,
@@ -385,19 +402,30 @@ void dense_inverse(int n, double* A, double* I, double* work, int* piv, hipStrea
   PMCD_HIP(hipGetLastError());
 }
 
-// lanes per row from the mean row length
-static int spmv_width(int n, const int* /*rp*/, long nnz_hint) {
-  (void)n; (void)nnz_hint;
-  return 16;
+// lanes per row for the CSR kernels (tunable; 16 unless set)
+static int g_spmv_width = 16;
+static int g_jacobi_width = 16;
+
+void set_tuning(int key, int value) {
+  PMCD_CHECK(value == 4 || value == 8 || value == 16 || value == 32, "lanes per row must be 4, 8, 16 or 32");
+  if(key == 0) g_spmv_width = value;
+  else if(key == 1) g_jacobi_width = value;
+  else PMCD_FAIL("unknown tuning key ", key);
 }
 
-static int g_spmv_width = 16;
+void launch_jacobi_from_zero(int n, int n_active, const double* diag, const double* b, double* x_out,
+                             double omega, hipStream_t s) {
+  if(n) k_jacobi_from_zero<double><<<grid_for(n, 256 * 4), 256, 0, s>>>(n, n_active, diag, b, x_out, omega);
+}
+void launch_jacobi_from_zero_f32(int n, int n_active, const float* diag, const float* b, float* x_out,
+                                 float omega, hipStream_t s) {
+  if(n) k_jacobi_from_zero<float><<<grid_for(n, 256 * 4), 256, 0, s>>>(n, n_active, diag, b, x_out, omega);
+}
 
 void launch_spmv(int n, int n_active, const int* rp, const int* col, const double* val,
                  const double* x, const double* z, double* y, double alpha, double beta,
                  hipStream_t s) {
   if(n == 0) return;
-  (void)spmv_width;
   dispatch_spmv<double>(g_spmv_width, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
   PMCD_HIP(hipGetLastError());
 }
@@ -410,12 +438,24 @@ void launch_spmv_f32(int n, int n_active, const int* rp, const int* col, const f
   PMCD_HIP(hipGetLastError());
 }
 
+template <class T>
+static void dispatch_jacobi(int width, int n, int n_active, const int* rp, const int* col,
+                            const T* val, const T* diag, const T* b, const T* x, T* x_out, T omega,
+                            hipStream_t s) {
+  const int blocks = (int)(((long)n * width + 255) / 256);
+  switch(width) {
+    case 4: k_jacobi<4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    case 8: k_jacobi<8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    case 16: k_jacobi<16, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    default: k_jacobi<32, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+  }
+}
+
 void launch_jacobi(int n, int n_active, const int* rp, const int* col, const double* val,
                    const double*, const double* diag, const double* b, const double* x,
                    double* x_out, double omega, hipStream_t s) {
   if(n == 0) return;
-  const int blocks = (int)(((long)n * 16 + 255) / 256);
-  k_jacobi<16, double><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega);
+  dispatch_jacobi<double>(g_jacobi_width, n, n_active, rp, col, val, diag, b, x, x_out, omega, s);
   PMCD_HIP(hipGetLastError());
 }
 
@@ -423,8 +463,7 @@ void launch_jacobi_f32(int n, int n_active, const int* rp, const int* col, const
                        const float* diag, const float* b, const float* x, float* x_out,
                        float omega, hipStream_t s) {
   if(n == 0) return;
-  const int blocks = (int)(((long)n * 16 + 255) / 256);
-  k_jacobi<16, float><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega);
+  dispatch_jacobi<float>(g_jacobi_width, n, n_active, rp, col, val, diag, b, x, x_out, omega, s);
   PMCD_HIP(hipGetLastError());
 }
 

@@ -59,6 +59,13 @@ void launch_jacobi(int n, int n_active, const int* rp, const int* col, const dou
 void launch_jacobi_f32(int n, int n_active, const int* rp, const int* col, const float* val,
                        const float* diag, const float* b, const float* x, float* x_out,
                        float omega, hipStream_t s);
+// first sweep from x = 0 when b vanishes on the non-active rows (no SpMV needed)
+void launch_jacobi_from_zero(int n, int n_active, const double* diag, const double* b,
+                             double* x_out, double omega, hipStream_t s);
+void launch_jacobi_from_zero_f32(int n, int n_active, const float* diag, const float* b,
+                                 float* x_out, float omega, hipStream_t s);
+// lanes per row of the CSR kernels: key 0 SpMV, 1 Jacobi; value 4, 8, 16 or 32
+void set_tuning(int key, int value);
 // plain CSR product y = A x over all rows (transfer operators)
 void launch_csr(int n, const int* rp, const int* col, const double* val, const double* x,
                 double* y, hipStream_t s);

@@ -67,6 +67,7 @@ void givens_apply(double& dx, double& dy, double cs, double sn) {
 // matrices for the mixed-precision V-cycle.
 void CDSystem::prepare(const SolverDesc& s) {
   sd_ = s;
+  rhs_nonactive_zero_ = false;
   if((s.precond == 2 && n_levels() > 1) || s.method == 0) {
     if(s.method == 0 && n_levels() > 1) PMCD_FAIL("the direct solver expects a single level");
     if(!coarse_ready_) coarse_factorize();
@@ -119,6 +120,7 @@ SolveStats CDSystem::solve_richardson() {
   launch_copy(n, F.rhs.p, kr_.p, stream_);
   apply_scaled_add(F, F.sol.p, kr_.p, -1.0);
   double res = std::sqrt(dot(n, kr_.p, kr_.p));
+  rhs_nonactive_zero_ = false;
   st.initial_residual = res;
   int it = 0;
   if(!conv(res, 0)) {
@@ -153,6 +155,7 @@ SolveStats CDSystem::solve_cg() {
     launch_fill(n, 0.0, x, stream_);
   }
   double resid = std::sqrt(dot(n, r.p, r.p)) / normb;
+  rhs_nonactive_zero_ = tail_is_zero(r.p);
   st.initial_residual = resid;
   if(conv(resid, 0)) { st.final_residual = resid; st.converged = 1; return st; }
   double rho_1 = 0.0;
@@ -202,6 +205,9 @@ SolveStats CDSystem::solve_fgmres() {
   launch_copy(n, F.rhs.p, r, stream_);
   apply_scaled_add(F, x, r, -1.0);
   double resid = std::sqrt(dot(n, r, r));
+  // every Krylov vector inherits the non-active part of r0 (A is the identity
+  // there and the V-cycle returns the rhs's non-active part)
+  rhs_nonactive_zero_ = tail_is_zero(r);
   double beta = resid;
   st.initial_residual = resid;
   if(conv(resid, 0)) { st.final_residual = resid; st.converged = 1; return st; }

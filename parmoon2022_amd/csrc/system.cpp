@@ -414,66 +414,87 @@ double CDSystem::dot(int n, const double* x, const double* y) {
   return h;
 }
 
-// n damped Jacobi sweeps; the result ends in x (ping-pong through x2)
-void CDSystem::smooth(Level& L, double* x, double* x2, const double* b, int steps) {
+// `steps` damped Jacobi sweeps ping-ponging between x[0] and x[1]; returns the
+// index of the buffer holding the result. With `from_zero` the iterate is 0
+// and b vanishes on the non-active rows, so the first sweep needs no SpMV.
+int CDSystem::smooth(Level& L, double* const x[2], int cur, const double* b, int steps,
+                     bool from_zero) {
   for(int s = 0; s < steps; ++s) {
-    launch_jacobi(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b, x, x2,
-                  sd_.jacobi_damp, stream_);
-    launch_copy(L.n(), x2, x, stream_);
+    if(s == 0 && from_zero)
+      launch_jacobi_from_zero(L.n(), L.n_active(), L.diag.p, b, x[1 - cur], sd_.jacobi_damp, stream_);
+    else
+      launch_jacobi(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b, x[cur],
+                    x[1 - cur], sd_.jacobi_damp, stream_);
+    cur = 1 - cur;
   }
+  return cur;
 }
 
-void CDSystem::smooth32(Level& L, float* x, float* x2, const float* b, int steps) {
+int CDSystem::smooth32(Level& L, float* const x[2], int cur, const float* b, int steps,
+                       bool from_zero) {
   for(int s = 0; s < steps; ++s) {
-    launch_jacobi_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x, x2,
-                      (float)sd_.jacobi_damp, stream_);
-    std::swap(x, x2);
-    // keep the iterate in the caller's buffer
-    if(s == steps - 1 && (steps % 2) == 1)
-      PMCD_HIP(hipMemcpyAsync(x2, x, sizeof(float) * L.n(), hipMemcpyDeviceToDevice, stream_));
+    if(s == 0 && from_zero)
+      launch_jacobi_from_zero_f32(L.n(), L.n_active(), L.diag32.p, b, x[1 - cur],
+                                  (float)sd_.jacobi_damp, stream_);
+    else
+      launch_jacobi_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x[cur],
+                        x[1 - cur], (float)sd_.jacobi_damp, stream_);
+    cur = 1 - cur;
   }
+  return cur;
 }
 
 // Multigrid::cycle with a V-cycle control (Multigrid.C:157-308): finest
 // iterate 0 with the non-active entries of the rhs, pre-smoothing and defect
 // restriction down to the coarsest level, direct coarse solve, damped
-// prolongation and post-smoothing on the way up.
+// prolongation and post-smoothing on the way up. The finest iterate lives in
+// `out` and one work buffer; coarser iterates start at 0 with a rhs that is 0
+// on the non-active rows.
 void CDSystem::vcycle64(const double* rhs, double* out) {
   const int Lf = n_levels() - 1;
   std::vector<const double*> b(n_levels());
-  std::vector<double*> x(n_levels());
-  for(int l = 0; l <= Lf; ++l) { b[l] = levels_[l]->mg_rhs.p; x[l] = levels_[l]->mg_sol.p; }
+  std::vector<std::array<double*, 2>> x(n_levels());
+  std::vector<int> cur(n_levels(), 0);
+  for(int l = 0; l <= Lf; ++l) {
+    b[l] = levels_[l]->mg_rhs.p;
+    x[l] = {levels_[l]->mg_sol.p, levels_[l]->mg_sol2.p};
+  }
   b[Lf] = rhs;
-  {
+  x[Lf] = {out, levels_[Lf]->mg_sol2.p};
+  // the first sweep may skip its SpMV only if it starts from the zero iterate
+  const bool fine_zero = rhs_nonactive_zero_ && sd_.n_pre > 0;
+  if(!fine_zero) {
     Level& F = finest();
-    launch_fill(F.n_active(), 0.0, x[Lf], stream_);
-    launch_copy(F.n() - F.n_active(), rhs + F.n_active(), x[Lf] + F.n_active(), stream_);
+    launch_fill(F.n_active(), 0.0, x[Lf][0], stream_);
+    launch_copy(F.n() - F.n_active(), rhs + F.n_active(), x[Lf][0] + F.n_active(), stream_);
   }
   for(int l = Lf; l >= 1; --l) {
     Level& L = *levels_[l];
     Level& C = *levels_[l - 1];
-    smooth(L, x[l], L.mg_sol2.p, b[l], sd_.n_pre);
-    launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x[l], b[l], L.mg_def.p, -1.0,
-                1.0, stream_);
-    launch_fill(C.n(), 0.0, x[l - 1], stream_);
+    const bool zero = (l == Lf) ? fine_zero : sd_.n_pre > 0;
+    cur[l] = smooth(L, x[l].data(), cur[l], b[l], sd_.n_pre, zero);
+    launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x[l][cur[l]], b[l], L.mg_def.p,
+                -1.0, 1.0, stream_);
+    if(sd_.n_pre == 0 || l - 1 == 0) launch_fill(C.n(), 0.0, x[l - 1][0], stream_);
     launch_csr(C.n(), L.r_rp.p, L.r_col.p, L.r_val.p, L.mg_def.p, C.mg_rhs.p, stream_);
     launch_zero_tail(C.n(), C.n_active(), C.mg_rhs.p, stream_);
   }
   {
     Level& C = *levels_[0];
     if(n_levels() == 1) {
-      smooth(C, x[0], C.mg_sol2.p, b[0], sd_.n_pre + sd_.n_post);
+      cur[0] = smooth(C, x[0].data(), cur[0], b[0], sd_.n_pre + sd_.n_post, false);
     } else {
-      launch_gemv(C.n(), coarse_inv_.p, b[0], x[0], stream_);
+      launch_gemv(C.n(), coarse_inv_.p, b[0], x[0][0], stream_);
+      cur[0] = 0;
     }
   }
   for(int l = 1; l <= Lf; ++l) {
     Level& L = *levels_[l];
-    launch_csr(L.n(), L.p_rp.p, L.p_col.p, L.p_val.p, x[l - 1], L.mg_tmp.p, stream_);
-    launch_axpy_active(L.n_active(), sd_.mg_damp, L.mg_tmp.p, x[l], stream_);
-    smooth(L, x[l], L.mg_sol2.p, b[l], sd_.n_post);
+    launch_csr(L.n(), L.p_rp.p, L.p_col.p, L.p_val.p, x[l - 1][cur[l - 1]], L.mg_tmp.p, stream_);
+    launch_axpy_active(L.n_active(), sd_.mg_damp, L.mg_tmp.p, x[l][cur[l]], stream_);
+    cur[l] = smooth(L, x[l].data(), cur[l], b[l], sd_.n_post, false);
   }
-  launch_copy(finest().n(), x[Lf], out, stream_);
+  if(cur[Lf] != 0) launch_copy(finest().n(), x[Lf][cur[Lf]], out, stream_);
 }
 
 void CDSystem::vcycle32(const double* rhs, double* out) {
@@ -485,45 +506,59 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
       L.f_def.alloc(L.n()); L.f_tmp.alloc(L.n());
     }
   }
+  std::vector<std::array<float*, 2>> x(n_levels());
+  std::vector<int> cur(n_levels(), 0);
+  for(int l = 0; l <= Lf; ++l) x[l] = {levels_[l]->f_sol.p, levels_[l]->f_sol2.p};
   Level& F = finest();
   launch_d2f(F.n(), rhs, F.f_rhs.p, stream_);
-  launch_fill_f32(F.n_active(), 0.0f, F.f_sol.p, stream_);
-  PMCD_HIP(hipMemcpyAsync(F.f_sol.p + F.n_active(), F.f_rhs.p + F.n_active(),
-                          sizeof(float) * (F.n() - F.n_active()), hipMemcpyDeviceToDevice, stream_));
+  const bool fine_zero = rhs_nonactive_zero_ && sd_.n_pre > 0;
+  if(!fine_zero) {
+    launch_fill_f32(F.n_active(), 0.0f, F.f_sol.p, stream_);
+    PMCD_HIP(hipMemcpyAsync(F.f_sol.p + F.n_active(), F.f_rhs.p + F.n_active(),
+                            sizeof(float) * (F.n() - F.n_active()), hipMemcpyDeviceToDevice, stream_));
+  }
   for(int l = Lf; l >= 1; --l) {
     Level& L = *levels_[l];
     Level& C = *levels_[l - 1];
-    smooth32(L, L.f_sol.p, L.f_sol2.p, L.f_rhs.p, sd_.n_pre);
-    launch_spmv_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.f_sol.p, L.f_rhs.p,
+    const bool zero = (l == Lf) ? fine_zero : sd_.n_pre > 0;
+    cur[l] = smooth32(L, x[l].data(), cur[l], L.f_rhs.p, sd_.n_pre, zero);
+    launch_spmv_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, x[l][cur[l]], L.f_rhs.p,
                     L.f_def.p, -1.0f, 1.0f, stream_);
-    launch_fill_f32(C.n(), 0.0f, C.f_sol.p, stream_);
+    if(sd_.n_pre == 0 || l - 1 == 0) launch_fill_f32(C.n(), 0.0f, x[l - 1][0], stream_);
     launch_csr_f32(C.n(), L.r_rp.p, L.r_col.p, L.r_val32.p, L.f_def.p, C.f_rhs.p, stream_);
     launch_zero_tail_f32(C.n(), C.n_active(), C.f_rhs.p, stream_);
   }
   {
     Level& C = *levels_[0];
     if(n_levels() == 1) {
-      smooth32(C, C.f_sol.p, C.f_sol2.p, C.f_rhs.p, sd_.n_pre + sd_.n_post);
+      cur[0] = smooth32(C, x[0].data(), cur[0], C.f_rhs.p, sd_.n_pre + sd_.n_post, false);
     } else {
       // coarse direct solve in FP64
       launch_f2d(C.n(), C.f_rhs.p, coarse_tmp_.p, stream_);
       launch_gemv(C.n(), coarse_inv_.p, coarse_tmp_.p, coarse_tmp_.p + C.n(), stream_);
-      launch_d2f(C.n(), coarse_tmp_.p + C.n(), C.f_sol.p, stream_);
+      launch_d2f(C.n(), coarse_tmp_.p + C.n(), x[0][0], stream_);
+      cur[0] = 0;
     }
   }
   for(int l = 1; l <= Lf; ++l) {
     Level& L = *levels_[l];
-    launch_csr_f32(L.n(), L.p_rp.p, L.p_col.p, L.p_val32.p, levels_[l - 1]->f_sol.p, L.f_tmp.p,
-                   stream_);
-    launch_axpy_f32(L.n(), (float)sd_.mg_damp, L.f_tmp.p, L.f_sol.p, L.n_active(), stream_);
-    smooth32(L, L.f_sol.p, L.f_sol2.p, L.f_rhs.p, sd_.n_post);
+    launch_csr_f32(L.n(), L.p_rp.p, L.p_col.p, L.p_val32.p, x[l - 1][cur[l - 1]], L.f_tmp.p, stream_);
+    launch_axpy_f32(L.n(), (float)sd_.mg_damp, L.f_tmp.p, x[l][cur[l]], L.n_active(), stream_);
+    cur[l] = smooth32(L, x[l].data(), cur[l], L.f_rhs.p, sd_.n_post, false);
   }
-  launch_f2d(F.n(), F.f_sol.p, out, stream_);
+  launch_f2d(F.n(), x[Lf][cur[Lf]], out, stream_);
 }
 
 void CDSystem::vcycle(const double* rhs, double* out) {
   if(sd_.mixed) vcycle32(rhs, out);
   else vcycle64(rhs, out);
+}
+
+bool CDSystem::tail_is_zero(const double* v) {
+  Level& F = finest();
+  const int m = F.n() - F.n_active();
+  if(m == 0) return true;
+  return dot(m, v + F.n_active(), v + F.n_active()) == 0.0;
 }
 
 // Preconditioner application: none (copy), Jacobi (r = z / diag,

@@ -9,6 +9,7 @@
 // (Iteration_cg.C:27-147), Iteration_gmres::flexible_gmres
 // (Iteration_gmres.C:430-620), Multigrid (src/Multigrid/Multigrid.C:157-410).
 #pragma once
+#include <array>
 #include <memory>
 #include "common.hpp"
 #include "mesh.hpp"
@@ -113,8 +114,9 @@ class CDSystem {
   void build_transfer(Level& fine, const Level& coarse);
   void assemble_level(Level& L);
   void coarse_factorize();
-  void smooth(Level& L, double* x, double* x2, const double* b, int steps);
-  void smooth32(Level& L, float* x, float* x2, const float* b, int steps);
+  int smooth(Level& L, double* const x[2], int cur, const double* b, int steps, bool from_zero);
+  int smooth32(Level& L, float* const x[2], int cur, const float* b, int steps, bool from_zero);
+  bool tail_is_zero(const double* v);  // v vanishes on the finest non-active rows
   void vcycle64(const double* rhs, double* out);
   void vcycle32(const double* rhs, double* out);
   void precond(const double* v, double* z);
@@ -136,6 +138,9 @@ class CDSystem {
   DevBuf<double> coarse_inv_, coarse_tmp_;
   bool coarse_ready_ = false;
   bool mixed_ready_ = false;
+  // set by the Krylov solvers when every preconditioner input vanishes on the
+  // non-active rows (true when the initial residual does)
+  bool rhs_nonactive_zero_ = false;
   // Krylov work space
   std::vector<DevBuf<double>> kv_, kz_;
   DevBuf<double> kr_, kw_, kh_;

@@ -155,10 +155,17 @@ int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, dou
  * 4 row gather, 5 FP32 Jacobi sweep. No reference counterpart (measurement). */
 int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* avg_ms);
 
-/* Launch-shape tuning of the CSR kernels: key 0 = SpMV, 1 = Jacobi sweep;
- * value = lanes per matrix row (4, 8, 16 or 32). Process-wide. No reference
- * counterpart (tuning knob used by the autotuning sweep). */
+/* Launch-shape tuning (process-wide; no reference counterpart, knobs of the
+ * autotuning sweep): key 0 / 1 = lanes per matrix row of the SpMV / Jacobi
+ * kernels (4, 8, 16, 32); key 2 / 3 = their variant (0 plain, 1 streaming with
+ * XCD-contiguous rows); key 4 = assembly kernel (0 reference operation order on
+ * vector ALUs, 1 FP64 MFMA). */
 int pmcd_set_tuning(int key, int value);
+
+/* Lane-layout check of the 16x16x4 FP64 MFMA used by the assembly: writes the
+ * 64 lanes x 4 result registers of one product of known operands (256
+ * doubles). No reference counterpart (hardware check). */
+int pmcd_probe_mfma(double* out256);
 
 /* Device-pointer entry points (all pointers are device memory, `stream` a
  * hipStream_t or NULL). FEMatrix::multiplyActive + addScaledNonActive:

@@ -60,14 +60,28 @@ EXPORTS = [
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
     "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
     "pmcd_get_rhs", "pmcd_solve", "pmcd_get_solution", "pmcd_set_solution", "pmcd_errors",
-    "pmcd_apply", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_set_tuning", "pmcd_spmv_device",
-    "pmcd_dot_device",
+    "pmcd_apply", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_set_tuning", "pmcd_probe_mfma",
+    "pmcd_spmv_device", "pmcd_dot_device",
 ]
 
 
-def set_tuning(kernel, lanes_per_row):
-    """Lanes per CSR row for 'spmv' or 'jacobi' kernels (4, 8, 16, 32)."""
-    _check(lib().pmcd_set_tuning({"spmv": 0, "jacobi": 1}[kernel], int(lanes_per_row)))
+def set_assembly_variant(variant):
+    """0: reference operation order on vector ALUs; 1: FP64 MFMA (default)."""
+    _check(lib().pmcd_set_tuning(4, int(variant)))
+
+
+def probe_mfma():
+    out = np.zeros(256)
+    _check(lib().pmcd_probe_mfma(_ptr(out, ctypes.c_double)))
+    return out.reshape(64, 4)
+
+
+def set_tuning(kernel, lanes_per_row, variant=0):
+    """Launch shape of the 'spmv' or 'jacobi' CSR kernels: lanes per row (4, 8,
+    16, 32) and variant (0 plain, 1 streaming with XCD-contiguous rows)."""
+    k = {"spmv": 0, "jacobi": 1}[kernel]
+    _check(lib().pmcd_set_tuning(k, int(lanes_per_row)))
+    _check(lib().pmcd_set_tuning(k + 2, int(variant)))
 
 KERNELS = {"spmv_f64": 0, "jacobi_f64": 1, "spmv_f32": 2, "assemble_local": 3, "gather_rows": 4,
            "jacobi_f32": 5}
@@ -303,14 +317,15 @@ def autotune(cd, level=-1, iters=10, log=None):
     out = {}
     for kernel, timed in (("spmv", "spmv_f64"), ("jacobi", "jacobi_f64")):
         res = {}
-        for w in (4, 8, 16, 32):
-            set_tuning(kernel, w)
-            res[w] = cd.time_kernel(timed, level, iters)
+        for variant in (0, 1):
+            for w in (8, 16, 32):
+                set_tuning(kernel, w, variant)
+                res["%s%d" % ("stream" if variant else "plain", w)] = cd.time_kernel(timed, level, iters)
         best = min(res, key=res.get)
-        set_tuning(kernel, best)
+        set_tuning(kernel, int(best.lstrip("streamplain")), 1 if best.startswith("stream") else 0)
         out[kernel] = res
         if log:
-            log("autotune %s: %s -> %d lanes/row" % (kernel, {k: round(v, 4) for k, v in res.items()}, best))
+            log("autotune %s: %s -> %s" % (kernel, {k: round(v, 4) for k, v in res.items()}, best))
     return out
 
 

@@ -313,6 +313,16 @@ int pmcd_set_tuning(int key, int value) {
   return guarded([&] { set_tuning(key, value); });
 }
 
+int pmcd_probe_mfma(double* out256) {
+  return guarded([&] {
+    int ndev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) PMCD_FAIL("no GPU visible");
+    DevBuf<double> d(256);
+    launch_mfma_probe(d.p, nullptr);
+    PMCD_HIP(hipMemcpy(out256, d.p, 256 * sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
 int pmcd_spmv_device(int n, int n_active, const int* row_ptr, const int* col, const double* val,
                      const double* x, const double* z, double* y, double alpha, double beta,
                      void* stream) {

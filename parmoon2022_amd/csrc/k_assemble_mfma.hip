@@ -1,0 +1,323 @@
+// FP64-MFMA assembly of the convection-diffusion local matrices and right-hand
+// sides (same forms as k_assemble.hip: BilinearAssembleGalerkin<d> +
+// BilinearAssemble_SD<d>, src/AssembleRoutines/ConvDiff.C:1776-1999).
+//
+// For an affine cell the physical derivatives are fixed linear combinations
+// of the reference ones, so with R[q][s][i] the cell-independent reference
+// table (value, first and second derivatives of basis function i at point q)
+// the local matrix factors as
+//     A_K[i][j] = sum_{q, r} R[q][r][i] * V_K[q][r][j],
+//     V_K[q][r][j] = sum_s W_K[q][r][s] * R[q][s][j],
+// with r over the test components (value, d/dxi_k) and s over the ansatz
+// components (value, d/dxi_k, second derivatives). W_K (per point, 1+d by
+// components) holds weight * |det J| times the coefficient and metric terms:
+//     W[0][0] = c,  W[0][1+l] = beta_l,  W[1+k][0] = delta beta_k c,
+//     W[1+k][1+l] = eps M_kl + delta beta_k beta_l,
+//     W[1+k][H_t] = -delta beta_k eps m_t,
+// where G = J^-T (rows: d/dx_a = sum_k G[a][k] d/dxi_k), M = G^T G,
+// beta = G^T b, m_t the Laplacian weights of the second-derivative components,
+// and delta the SUPG parameter (0 for Galerkin). The right-hand side is the
+// extra column j = n_local: V[q][0][n] = w f, V[q][1+k][n] = w delta f beta_k.
+//
+// Stage 1 (vector ALUs) builds V in LDS for a chunk of points; stage 2 runs
+// the (n_local x 4 n_q) x (4 n_q x (n_local+1)) product as 16x16x4 FP64 MFMA
+// tiles, the K dimension being the (point, test component) pairs. One wave
+// per cell, four cells per block sharing the reference-table chunk.
+#include "kernels.hpp"
+#include "device.hpp"
+
+namespace pmcd {
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+constexpr int kCells = 4;
+constexpr int kQC = 8;     // quadrature points per chunk
+constexpr int kPad = 32;   // padded local dimension (two 16-wide tiles)
+
+__device__ inline double sdfem_delta_m(int dim, double hK, double eps, double b1, double b2,
+                                       double b3) {
+  if(fabs(eps) < 1e-20) eps = 1e-20;
+  double nb = b1 * b1 + b2 * b2;
+  if(dim == 3) nb += b3 * b3;
+  nb = sqrt(nb);
+  if(!(nb > 0)) return 0.0;
+  const double alpha = nb * hK / (2 * eps);
+  return hK * (1 / tanh(alpha) - 1 / alpha) / (2 * nb);
+}
+
+// affine map data of a cell (same maps as the vector kernel)
+struct GeoM {
+  double c0[3], C[3][3], G[3][3], det, hK;
+};
+
+__device__ inline void geo_m(int shape, const double* v, GeoM& g) {
+  auto X = [&](int i, int k) { return v[3 * i + k]; };
+  double K[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for(int a = 0; a < 3; ++a)
+    for(int k = 0; k < 3; ++k) g.C[a][k] = 0;
+  double d2 = 0;
+  if(shape == (int)Shape::Triangle || shape == (int)Shape::Quadrangle) {
+    for(int a = 0; a < 2; ++a) {
+      if(shape == (int)Shape::Triangle) {
+        g.c0[a] = X(0, a); g.C[a][0] = X(1, a) - X(0, a); g.C[a][1] = X(2, a) - X(0, a);
+      } else {
+        g.c0[a] = (X(1, a) + X(3, a)) * 0.5;
+        g.C[a][0] = (X(1, a) - X(0, a)) * 0.5;
+        g.C[a][1] = (X(3, a) - X(0, a)) * 0.5;
+      }
+    }
+    g.c0[2] = 0;
+    g.det = g.C[0][0] * g.C[1][1] - g.C[0][1] * g.C[1][0];
+    K[0][0] = g.C[1][1]; K[0][1] = -g.C[1][0]; K[1][0] = -g.C[0][1]; K[1][1] = g.C[0][0];
+    const int nv = shape == (int)Shape::Triangle ? 3 : 4;
+    if(nv == 3) {
+      for(int m = 0; m < 3; ++m) {
+        int p = m, q = (m + 1) % 3;
+        double s = 0;
+        for(int k = 0; k < 3; ++k) s += (X(p, k) - X(q, k)) * (X(p, k) - X(q, k));
+        d2 = fmax(d2, s);
+      }
+    } else {
+      for(int m = 0; m < 2; ++m) {
+        double s = 0;
+        for(int k = 0; k < 3; ++k) s += (X(m, k) - X(m + 2, k)) * (X(m, k) - X(m + 2, k));
+        d2 = fmax(d2, s);
+      }
+    }
+  } else {
+    for(int a = 0; a < 3; ++a) {
+      if(shape == (int)Shape::Tetrahedron) {
+        g.c0[a] = X(0, a);
+        for(int k = 0; k < 3; ++k) g.C[a][k] = X(k + 1, a) - X(0, a);
+      } else {
+        g.c0[a] = (X(1, a) + X(3, a) + X(4, a) - X(0, a)) * 0.5;
+        g.C[a][0] = (X(1, a) - X(0, a)) * 0.5;
+        g.C[a][1] = (X(3, a) - X(0, a)) * 0.5;
+        g.C[a][2] = (X(4, a) - X(0, a)) * 0.5;
+      }
+    }
+    const double xc1 = g.C[0][0], xc2 = g.C[0][1], xc3 = g.C[0][2];
+    const double yc1 = g.C[1][0], yc2 = g.C[1][1], yc3 = g.C[1][2];
+    const double zc1 = g.C[2][0], zc2 = g.C[2][1], zc3 = g.C[2][2];
+    g.det = xc1 * yc2 * zc3 + xc2 * yc3 * zc1 + xc3 * yc1 * zc2 - xc3 * yc2 * zc1 - xc2 * yc1 * zc3 -
+            xc1 * yc3 * zc2;
+    K[0][0] = yc2 * zc3 - yc3 * zc2; K[0][1] = yc3 * zc1 - yc1 * zc3; K[0][2] = yc1 * zc2 - yc2 * zc1;
+    K[1][0] = xc3 * zc2 - xc2 * zc3; K[1][1] = xc1 * zc3 - xc3 * zc1; K[1][2] = xc2 * zc1 - xc1 * zc2;
+    K[2][0] = xc2 * yc3 - xc3 * yc2; K[2][1] = xc3 * yc1 - xc1 * yc3; K[2][2] = xc1 * yc2 - xc2 * yc1;
+    if(shape == (int)Shape::Tetrahedron) {
+      for(int p = 0; p < 4; ++p)
+        for(int q = p + 1; q < 4; ++q) {
+          double s = 0;
+          for(int k = 0; k < 3; ++k) s += (X(p, k) - X(q, k)) * (X(p, k) - X(q, k));
+          d2 = fmax(d2, s);
+        }
+    } else {
+      const int dg[4][2] = {{0, 6}, {1, 7}, {2, 4}, {3, 5}};
+      for(int m = 0; m < 4; ++m) {
+        double s = 0;
+        for(int k = 0; k < 3; ++k) s += (X(dg[m][0], k) - X(dg[m][1], k)) * (X(dg[m][0], k) - X(dg[m][1], k));
+        d2 = fmax(d2, s);
+      }
+    }
+  }
+  for(int a = 0; a < 3; ++a)
+    for(int k = 0; k < 3; ++k) g.G[a][k] = K[a][k] / g.det;
+  g.hK = sqrt(d2);
+}
+
+// DIM: 2 or 3; NC: reference components (6 or 10); NT = 1 + DIM test components
+template <int DIM, int NL>
+__global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
+  constexpr int NC = DIM == 2 ? 6 : 10;
+  constexpr int NT = 1 + DIM;
+  constexpr int NH = NC - NT;               // second-derivative components
+  constexpr int NIT = (NL + 15) / 16;       // i tiles
+  constexpr int NJT = (NL + 1 + 15) / 16;   // j tiles (incl. the rhs column)
+  constexpr int KC = kQC * NT;              // K extent of one chunk
+  static_assert(KC % 4 == 0, "chunk K extent must be a multiple of 4");
+  __shared__ double s_R[kQC][NC][kPad];      // reference table chunk, zero padded
+  __shared__ double s_V[kCells][KC][kPad];   // per-cell V chunk
+  __shared__ double s_W[kCells][kQC][NT][NC + 1];  // last column: rhs coefficient
+
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int cell = blockIdx.x * kCells + wave;
+  const bool live = cell < a.n_cells;
+  GeoM g;
+  if(live) geo_m(a.shape, a.cell_xyz + (size_t)cell * a.nv * 3, g);
+  // metric terms: M = G^T G, m (Laplacian weights), beta needs b per point
+  double M[3][3];
+  for(int k = 0; k < 3; ++k)
+    for(int l = 0; l < 3; ++l) {
+      double s = 0;
+      if(live)
+        for(int d = 0; d < DIM; ++d) s += g.G[d][k] * g.G[d][l];
+      M[k][l] = s;
+    }
+  double mlap[6];
+  if(DIM == 2) { mlap[0] = M[0][0]; mlap[1] = 2 * M[0][1]; mlap[2] = M[1][1]; }
+  else {
+    mlap[0] = M[0][0]; mlap[1] = 2 * M[0][1]; mlap[2] = 2 * M[0][2];
+    mlap[3] = M[1][1]; mlap[4] = 2 * M[1][2]; mlap[5] = M[2][2];
+  }
+
+  v4d acc[NIT][NJT];
+  for(int it = 0; it < NIT; ++it)
+    for(int jt = 0; jt < NJT; ++jt) acc[it][jt] = v4d{0, 0, 0, 0};
+
+  for(int q0 = 0; q0 < a.nq; q0 += kQC) {
+    const int nqc = min(kQC, a.nq - q0);
+    // reference table chunk (points beyond nq and columns beyond NL are zero)
+    for(int t = threadIdx.x; t < kQC * NC * kPad; t += 256) {
+      const int qq = t / (NC * kPad), rem = t % (NC * kPad), s = rem / kPad, i = rem % kPad;
+      double v = 0;
+      if(qq < nqc && i < NL) v = a.ref[((size_t)(q0 + qq) * NC + s) * NL + i];
+      s_R[qq][s][i] = v;
+    }
+    // W per point (lanes over points)
+    if(live && lane < kQC) {
+      const int qq = lane;
+      double (*W)[NC + 1] = s_W[wave][qq];
+      if(qq < nqc) {
+        const int q = q0 + qq;
+        const double xi = a.qp[3 * q], eta = a.qp[3 * q + 1], zeta = a.qp[3 * q + 2];
+        double x[3];
+        for(int d = 0; d < 3; ++d) {
+          x[d] = g.c0[d] + g.C[d][0] * xi + g.C[d][1] * eta;
+          if(DIM == 3) x[d] = x[d] + g.C[d][2] * zeta;
+        }
+        const double w = a.qw[q] * fabs(g.det);
+        double cf[6];
+        example_coeffs(a.ex, x[0], x[1], DIM == 3 ? x[2] : 0.0, cf);
+        const double eps = cf[0], c = cf[4], f = cf[5];
+        const double b[3] = {cf[1], cf[2], DIM == 3 ? cf[3] : 0.0};
+        const double delta = a.supg ? sdfem_delta_m(DIM, g.hK, eps, b[0], b[1], b[2]) : 0.0;
+        double beta[3];
+        for(int k = 0; k < 3; ++k) {
+          double s = 0;
+          for(int d = 0; d < DIM; ++d) s += g.G[d][k] * b[d];
+          beta[k] = s;
+        }
+        W[0][0] = w * c;
+        for(int l = 0; l < DIM; ++l) W[0][1 + l] = w * beta[l];
+        for(int t = 0; t < NH; ++t) W[0][NT + t] = 0.0;
+        W[0][NC] = w * f;
+        for(int k = 0; k < DIM; ++k) {
+          W[1 + k][0] = w * delta * beta[k] * c;
+          for(int l = 0; l < DIM; ++l) W[1 + k][1 + l] = w * (eps * M[k][l] + delta * beta[k] * beta[l]);
+          for(int t = 0; t < NH; ++t) W[1 + k][NT + t] = -w * delta * beta[k] * eps * mlap[t];
+          W[1 + k][NC] = w * delta * f * beta[k];
+        }
+      } else {
+        for(int r = 0; r < NT; ++r)
+          for(int s = 0; s <= NC; ++s) W[r][s] = 0.0;
+      }
+    }
+    __syncthreads();
+    // stage 1: V[q*NT + r][j] = sum_s W[q][r][s] R[q][s][j]; j == NL -> rhs column
+    if(live) {
+      const int j = lane % 32, half = lane / 32;
+      for(int qq = half; qq < kQC; qq += 2) {
+        double out[NT];
+        for(int r = 0; r < NT; ++r) out[r] = 0.0;
+        if(j < NL) {
+#pragma unroll
+          for(int s = 0; s < NC; ++s) {
+            const double rv = s_R[qq][s][j];
+#pragma unroll
+            for(int r = 0; r < NT; ++r) out[r] += s_W[wave][qq][r][s] * rv;
+          }
+        } else if(j == NL) {
+          for(int r = 0; r < NT; ++r) out[r] = s_W[wave][qq][r][NC];
+        }
+        for(int r = 0; r < NT; ++r) s_V[wave][qq * NT + r][j] = out[r];
+      }
+    }
+    __syncthreads();
+    // stage 2: A[i][j] += sum_k R^T[i][k] V[k][j], k = (point, test component)
+    if(live) {
+      const int l16 = lane % 16, kq = lane / 16;
+      for(int k0 = 0; k0 < KC; k0 += 4) {
+        const int k = k0 + kq, qq = k / NT, r = k % NT;
+        double av[NIT], bv[NJT];
+#pragma unroll
+        for(int it = 0; it < NIT; ++it) av[it] = s_R[qq][r][it * 16 + l16];
+#pragma unroll
+        for(int jt = 0; jt < NJT; ++jt) bv[jt] = s_V[wave][k][jt * 16 + l16];
+#pragma unroll
+        for(int it = 0; it < NIT; ++it)
+#pragma unroll
+          for(int jt = 0; jt < NJT; ++jt)
+            acc[it][jt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[it], bv[jt], acc[it][jt], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  if(live) {
+    // D tile layout: lane holds rows 4*(lane/16)+v, column lane%16
+    double* A = a.local_A + (size_t)cell * NL * NL;
+    double* F = a.local_rhs + (size_t)cell * NL;
+    const int l16 = lane % 16, rg = lane / 16;
+#pragma unroll
+    for(int it = 0; it < NIT; ++it)
+#pragma unroll
+      for(int jt = 0; jt < NJT; ++jt)
+#pragma unroll
+        for(int v = 0; v < 4; ++v) {
+          const int i = it * 16 + 4 * rg + v, j = jt * 16 + l16;
+          if(i < NL) {
+            if(j < NL) A[i * NL + j] = acc[it][jt][v];
+            else if(j == NL) F[i] = acc[it][jt][v];
+          }
+        }
+  }
+}
+
+// lane-layout check of the 16x16x4 FP64 MFMA: D = A B with A[i][k] = i + 100 k,
+// B[k][j] = (k == j % 4) ? 1 + j : 0; out[lane*4+v] = D register v of lane
+__global__ void k_mfma_probe(double* out) {
+  const int lane = threadIdx.x;
+  const int i = lane % 16, k = lane / 16;
+  const double av = i + 100.0 * k;
+  const int j = lane % 16;
+  const double bv = (k == j % 4) ? 1.0 + j : 0.0;
+  v4d acc = {0, 0, 0, 0};
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+  for(int v = 0; v < 4; ++v) out[lane * 4 + v] = acc[v];
+}
+
+}  // namespace
+
+bool launch_assemble_mfma(const AsmArgs& a, hipStream_t s) {
+  if(a.n_cells == 0) return true;
+  const int blocks = (a.n_cells + kCells - 1) / kCells;
+  if(a.dim == 3 && a.ncomp == 10) {
+    switch(a.nl) {
+      case 4: k_assemble_mfma<3, 4><<<blocks, 256, 0, s>>>(a); break;
+      case 8: k_assemble_mfma<3, 8><<<blocks, 256, 0, s>>>(a); break;
+      case 10: k_assemble_mfma<3, 10><<<blocks, 256, 0, s>>>(a); break;
+      case 27: k_assemble_mfma<3, 27><<<blocks, 256, 0, s>>>(a); break;
+      default: return false;
+    }
+  } else if(a.dim == 2 && a.ncomp == 6) {
+    switch(a.nl) {
+      case 3: k_assemble_mfma<2, 3><<<blocks, 256, 0, s>>>(a); break;
+      case 4: k_assemble_mfma<2, 4><<<blocks, 256, 0, s>>>(a); break;
+      case 6: k_assemble_mfma<2, 6><<<blocks, 256, 0, s>>>(a); break;
+      case 9: k_assemble_mfma<2, 9><<<blocks, 256, 0, s>>>(a); break;
+      default: return false;
+    }
+  } else {
+    return false;
+  }
+  PMCD_HIP(hipGetLastError());
+  return true;
+}
+
+void launch_mfma_probe(double* out, hipStream_t s) {
+  k_mfma_probe<<<1, 64, 0, s>>>(out);
+  PMCD_HIP(hipGetLastError());
+}
+
+}  // namespace pmcd

@@ -72,6 +72,79 @@ __global__ __launch_bounds__(256) void k_jacobi(int n, int n_active, const int* 
   }
 }
 
+// Streaming variant: blocks are remapped so that each of the 8 XCDs (which
+// receive blocks round-robin) works on one contiguous range of rows, keeping
+// the x entries those rows share in the XCD's own L2; the matrix streams are
+// read with non-temporal loads so they do not evict x; each lane keeps two
+// independent partial sums to keep two loads in flight.
+__device__ inline int xcd_block(int bid, int nb) {
+  const int per = nb / 8;
+  if(bid >= per * 8) return bid;
+  return (bid % 8) * per + bid / 8;
+}
+
+template <int W, class T>
+__device__ inline T row_sum_stream(const int* __restrict__ rp, const int* __restrict__ col,
+                                   const T* __restrict__ val, const T* __restrict__ x, int row,
+                                   int l) {
+  const int e = rp[row + 1];
+  int k = rp[row] + l;
+  T s0 = 0, s1 = 0;
+  for(; k + W < e; k += 2 * W) {
+    const int c0 = __builtin_nontemporal_load(&col[k]);
+    const int c1 = __builtin_nontemporal_load(&col[k + W]);
+    const T v0 = __builtin_nontemporal_load(&val[k]);
+    const T v1 = __builtin_nontemporal_load(&val[k + W]);
+    s0 += v0 * x[c0];
+    s1 += v1 * x[c1];
+  }
+  if(k < e) s0 += __builtin_nontemporal_load(&val[k]) * x[__builtin_nontemporal_load(&col[k])];
+  T s = s0 + s1;
+#pragma unroll
+  for(int off = W / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, W);
+  return s;
+}
+
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_spmv_stream(int n, int n_active, const int* __restrict__ rp,
+                                                     const int* __restrict__ col,
+                                                     const T* __restrict__ val,
+                                                     const T* __restrict__ x,
+                                                     const T* __restrict__ z, T* __restrict__ y,
+                                                     T alpha, T beta) {
+  const int gt = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const int row = gt / W, l = gt % W;
+  if(row >= n) return;
+  if(row < n_active) {
+    const T s = row_sum_stream<W, T>(rp, col, val, x, row, l);
+    if(l == 0) y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * s;
+  } else if(l == 0) {
+    y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * x[row];
+  }
+}
+
+template <int W, class T>
+__global__ __launch_bounds__(256) void k_jacobi_stream(int n, int n_active,
+                                                       const int* __restrict__ rp,
+                                                       const int* __restrict__ col,
+                                                       const T* __restrict__ val,
+                                                       const T* __restrict__ diag,
+                                                       const T* __restrict__ b,
+                                                       const T* __restrict__ x,
+                                                       T* __restrict__ x_out, T omega) {
+  const int gt = xcd_block(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+  const int row = gt / W, l = gt % W;
+  if(row >= n) return;
+  T s;
+  if(row < n_active) s = row_sum_stream<W, T>(rp, col, val, x, row, l);
+  else s = x[row];
+  if(l == 0) {
+    T d = b[row] + T(-1) * s;
+    d = d / diag[row];
+    x_out[row] = x[row] + omega * d;
+  }
+}
+
 // First Jacobi sweep from x = 0 with b = 0 on the non-active rows: A x = 0,
 // so the sweep reduces to x_out = 0 + omega * ((b + (-1)*0) / diag) on active
 // rows (the same floating-point result as the general sweep) and x_out = b on
@@ -256,10 +329,19 @@ inline int grid_for(int n, int per_block) {
 }
 
 template <class T>
-void dispatch_spmv(int width, int n, int n_active, const int* rp, const int* col, const T* val,
-                   const T* x, const T* z, T* y, T alpha, T beta, hipStream_t s) {
+void dispatch_spmv(int width, int variant, int n, int n_active, const int* rp, const int* col,
+                   const T* val, const T* x, const T* z, T* y, T alpha, T beta, hipStream_t s) {
   const long threads = (long)n * width;
   const int blocks = (int)((threads + 255) / 256);
+  if(variant == 1) {
+    switch(width) {
+      case 4: k_spmv_stream<4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+      case 8: k_spmv_stream<8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+      case 16: k_spmv_stream<16, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+      default: k_spmv_stream<32, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    }
+    return;
+  }
   switch(width) {
     case 4: k_spmv<4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
     case 8: k_spmv<8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
@@ -402,15 +484,31 @@ void dense_inverse(int n, double* A, double* I, double* work, int* piv, hipStrea
   PMCD_HIP(hipGetLastError());
 }
 
-// lanes per row for the CSR kernels (tunable; 16 unless set)
-static int g_spmv_width = 16;
-static int g_jacobi_width = 16;
+// launch shape of the CSR kernels (tunable): lanes per row and variant
+// (0 plain, 1 streaming with XCD-contiguous rows)
+static int g_spmv_width = 16, g_spmv_variant = 0;
+static int g_jacobi_width = 16, g_jacobi_variant = 0;
+
+static int g_assembly_variant = 1;
+int assembly_variant() { return g_assembly_variant; }
 
 void set_tuning(int key, int value) {
-  PMCD_CHECK(value == 4 || value == 8 || value == 16 || value == 32, "lanes per row must be 4, 8, 16 or 32");
-  if(key == 0) g_spmv_width = value;
-  else if(key == 1) g_jacobi_width = value;
-  else PMCD_FAIL("unknown tuning key ", key);
+  if(key == 4) {
+    PMCD_CHECK(value == 0 || value == 1, "assembly variant must be 0 or 1");
+    g_assembly_variant = value;
+    return;
+  }
+  if(key == 0 || key == 1)
+    PMCD_CHECK(value == 4 || value == 8 || value == 16 || value == 32, "lanes per row must be 4, 8, 16 or 32");
+  else
+    PMCD_CHECK(value == 0 || value == 1, "variant must be 0 or 1");
+  switch(key) {
+    case 0: g_spmv_width = value; break;
+    case 1: g_jacobi_width = value; break;
+    case 2: g_spmv_variant = value; break;
+    case 3: g_jacobi_variant = value; break;
+    default: PMCD_FAIL("unknown tuning key ", key);
+  }
 }
 
 void launch_jacobi_from_zero(int n, int n_active, const double* diag, const double* b, double* x_out,
@@ -426,7 +524,7 @@ void launch_spmv(int n, int n_active, const int* rp, const int* col, const doubl
                  const double* x, const double* z, double* y, double alpha, double beta,
                  hipStream_t s) {
   if(n == 0) return;
-  dispatch_spmv<double>(g_spmv_width, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
+  dispatch_spmv<double>(g_spmv_width, g_spmv_variant, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
   PMCD_HIP(hipGetLastError());
 }
 
@@ -434,7 +532,7 @@ void launch_spmv_f32(int n, int n_active, const int* rp, const int* col, const f
                      const float* x, const float* z, float* y, float alpha, float beta,
                      hipStream_t s) {
   if(n == 0) return;
-  dispatch_spmv<float>(g_spmv_width, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
+  dispatch_spmv<float>(g_spmv_width, g_spmv_variant, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
   PMCD_HIP(hipGetLastError());
 }
 
@@ -443,6 +541,15 @@ static void dispatch_jacobi(int width, int n, int n_active, const int* rp, const
                             const T* val, const T* diag, const T* b, const T* x, T* x_out, T omega,
                             hipStream_t s) {
   const int blocks = (int)(((long)n * width + 255) / 256);
+  if(g_jacobi_variant == 1) {
+    switch(width) {
+      case 4: k_jacobi_stream<4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+      case 8: k_jacobi_stream<8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+      case 16: k_jacobi_stream<16, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+      default: k_jacobi_stream<32, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    }
+    return;
+  }
   switch(width) {
     case 4: k_jacobi<4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
     case 8: k_jacobi<8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;

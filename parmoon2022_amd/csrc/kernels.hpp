@@ -26,6 +26,11 @@ struct AsmArgs {
   double* local_rhs;        // [n_cells][nl]
 };
 void launch_assemble_local(const AsmArgs& a, hipStream_t s);
+// FP64-MFMA variant; returns false if the element is not covered
+bool launch_assemble_mfma(const AsmArgs& a, hipStream_t s);
+void launch_mfma_probe(double* out, hipStream_t s);
+// assembly variant: 0 reference-order vector kernel, 1 MFMA (default)
+int assembly_variant();
 
 // Sum local matrices into CSR rows in increasing cell order; Dirichlet rows
 // (r >= n_active) get 1 on the diagonal and 0 elsewhere.

@@ -314,7 +314,7 @@ AsmArgs CDSystem::asm_args(const Level& L) {
 
 void CDSystem::assemble_level(Level& L) {
   AsmArgs a = asm_args(L);
-  launch_assemble_local(a, stream_);
+  if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
   launch_gather_matrix(L.n(), L.n_active(), a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p,
                        L.contrib.p, L.slot.p, local_A_.p, L.val.p, stream_);
   launch_gather_rhs(L.n(), L.n_active(), L.contrib_ptr.p, L.contrib.p, local_rhs_.p,
@@ -374,7 +374,9 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
       case 0: launch_spmv(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x.p, nullptr, y.p, 1.0, 0.0, stream_); break;
       case 1: launch_jacobi(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b.p, x.p, y.p, 0.8, stream_); break;
       case 2: launch_spmv_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, xf.p, nullptr, yf.p, 1.0f, 0.0f, stream_); break;
-      case 3: launch_assemble_local(a, stream_); break;
+      case 3:
+        if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
+        break;
       case 4:
         launch_gather_matrix(n, L.n_active(), a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p, L.contrib.p,
                              L.slot.p, local_A_.p, L.val.p, stream_);

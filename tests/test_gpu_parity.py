@@ -47,12 +47,27 @@ def row_rel_err(rp, a, b):
     return worst
 
 
+def test_mfma_f64_lane_layout():
+    """16x16x4 FP64 MFMA operand/result layout assumed by the assembly kernel:
+    A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k, D[i][j] in lane
+    j + 16 (i // 4), register i % 4. Probe: A[i][k] = i + 100 k,
+    B[k][j] = (1 + j) if k == j % 4 else 0, so D[i][j] = (i + 100 (j % 4)) (1 + j)."""
+    out = pm.probe_mfma()
+    for lane in range(64):
+        for v in range(4):
+            i, j = 4 * (lane // 16) + v, lane % 16
+            assert out[lane, v] == (i + 100 * (j % 4)) * (1 + j), (lane, v, out[lane, v])
+
+
+@pytest.mark.parametrize("variant", [0, 1], ids=["vector", "mfma"])
 @pytest.mark.parametrize("case", ASSEMBLY_CASES, ids=lambda c: "%s-p%d-r%d-%s-%s" % c[:5])
-def test_assembly_matches_oracle(oracle_lib, case):
+def test_assembly_matches_oracle(oracle_lib, case, variant):
     db = make_db(*case)
-    print("case", case, flush=True)
+    print("case", case, "variant", variant, flush=True)
+    pm.set_assembly_variant(variant)
     cd = pm.ConvectionDiffusion(db)
     cd.assemble()
+    pm.set_assembly_variant(1)
     o = oracle_lib.Oracle(pm.problem_desc(db))
     o.assemble()
     assert np.array_equal(cd.dofmap(), o.dofmap())
@@ -70,10 +85,14 @@ def test_assembly_matches_oracle(oracle_lib, case):
     cd.close()
 
 
+@pytest.mark.parametrize("shape", [(8, 0), (16, 0), (32, 0), (8, 1), (16, 1), (32, 1)],
+                         ids=lambda s: "w%d-v%d" % s)
 @pytest.mark.parametrize("case", ASSEMBLY_CASES[:8:3], ids=lambda c: "%s-p%d" % c[:2])
-def test_spmv_matches_serial_sum(oracle_lib, case):
-    """BlockFEMatrix::apply / apply_scaled_add against the serial row sums."""
+def test_spmv_matches_serial_sum(oracle_lib, case, shape):
+    """BlockFEMatrix::apply / apply_scaled_add against the serial row sums, for
+    every launch shape the autotuner may pick."""
     db = make_db(*case)
+    pm.set_tuning("spmv", *shape)
     cd = pm.ConvectionDiffusion(db)
     cd.assemble()
     rng = np.random.default_rng(7)
@@ -94,6 +113,26 @@ def test_spmv_matches_serial_sum(oracle_lib, case):
     # non-active rows: identity
     na = i["n_active"]
     assert np.array_equal(got_apply[na:], x[na:])
+    cd.close()
+    pm.set_tuning("spmv", 16, 0)
+
+
+@pytest.mark.parametrize("shape", [(8, 0), (32, 0), (8, 1), (16, 1), (32, 1)], ids=lambda s: "w%d-v%d" % s)
+def test_jacobi_shapes_same_solve(oracle_lib, shape):
+    """The multigrid solve is independent of the Jacobi launch shape up to
+    rounding: same iteration count and solution within 1e-10 of the oracle."""
+    db = c2_db(3, 2)
+    pm.set_tuning("jacobi", *shape)
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    st = cd.solve()
+    u = cd.solution()
+    pm.set_tuning("jacobi", 16, 0)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    so = o.solve(pm.solver_desc(db))
+    rel = np.linalg.norm(u - o.solution()) / np.linalg.norm(o.solution())
+    assert abs(st.iterations - so.iterations) <= 1 and rel < 1e-10, (st.iterations, so.iterations, rel)
     cd.close()
 
 

@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
     __syncthreads();
   }
   if(live) {
-    // D tile layout: lane holds rows 4*(lane/16)+v, column lane%16
+    // D tile layout: register v of lane l holds row l/16 + 4 v, column l%16
     double* A = a.local_A + (size_t)cell * NL * NL;
     double* F = a.local_rhs + (size_t)cell * NL;
     const int l16 = lane % 16, rg = lane / 16;
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
       for(int jt = 0; jt < NJT; ++jt)
 #pragma unroll
         for(int v = 0; v < 4; ++v) {
-          const int i = it * 16 + 4 * rg + v, j = jt * 16 + l16;
+          const int i = it * 16 + rg + 4 * v, j = jt * 16 + l16;
           if(i < NL) {
             if(j < NL) A[i * NL + j] = acc[it][jt][v];
             else if(j == NL) F[i] = acc[it][jt][v];

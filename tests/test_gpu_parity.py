@@ -50,13 +50,20 @@ def row_rel_err(rp, a, b):
 def test_mfma_f64_lane_layout():
     """16x16x4 FP64 MFMA operand/result layout assumed by the assembly kernel:
     A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k, D[i][j] in lane
-    j + 16 (i // 4), register i % 4. Probe: A[i][k] = i + 100 k,
+    j + 16 (i % 4), register i // 4. Probe: A[i][k] = i + 100 k,
     B[k][j] = (1 + j) if k == j % 4 else 0, so D[i][j] = (i + 100 (j % 4)) (1 + j)."""
     out = pm.probe_mfma()
+    bad = []
     for lane in range(64):
         for v in range(4):
-            i, j = 4 * (lane // 16) + v, lane % 16
-            assert out[lane, v] == (i + 100 * (j % 4)) * (1 + j), (lane, v, out[lane, v])
+            i, j = lane // 16 + 4 * v, lane % 16
+            if out[lane, v] != (i + 100 * (j % 4)) * (1 + j):
+                bad.append((lane, v, out[lane, v]))
+    if bad:
+        print("probe table (lane: 4 registers):")
+        for lane in range(64):
+            print(lane, list(out[lane]))
+    assert not bad, bad[:8]
 
 
 @pytest.mark.parametrize("variant", [0, 1], ids=["vector", "mfma"])

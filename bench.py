@@ -91,6 +91,53 @@ def cpu_baseline(db, gpu_iters, sample_div, matrices, rhs):
             "seconds_per_solve": t_solve}
 
 
+def kernel_times(cd, info, mixed):
+    """HIP-event averages of the finest level's dominant kernels on the
+    system's stream, with their algorithmic bytes (CSR values + column
+    indices once, row pointers once, the vectors once each) and flops."""
+    import parmoon2022_amd as pm
+    n, nnz = info["n_dof"], info["nnz"]
+    vb = 4 if mixed else 8
+    t_jac = cd.time_kernel("jacobi_f32" if mixed else "jacobi_f64", -1, 20)
+    t_spmv = cd.time_kernel("spmv_f64", -1, 20)
+    t_copy = cd.time_kernel("stream_copy", -1, 20)
+    t_asm = cd.time_kernel("assemble_local", -1, 3)
+    # Jacobi reads val, col, row_ptr, x (gathered), b, diag, x (own row) and
+    # writes x_out: (vb + 4) nnz + 4 (n + 1) + 5 vb n bytes
+    jac_bytes = (vb + 4) * nnz + 4 * (n + 1) + vb * n * 5
+    spmv_bytes = 12 * nnz + 4 * (n + 1) + 16 * n
+    copy_bytes = 2 * 8 * (nnz - nnz % 2)
+    asm_flops = (2 * 27 * 27 * 4 + 2 * 4 * 5 * 27) * info["n_quad"] * info["n_cells"]
+    lanes, variant = pm.tuning("jacobi")
+    return {
+        "jacobi_kernel": "k_%s%s<%d%s>" % ("jacobi", {0: "", 1: "_stream", 2: "_unrolled"}[variant],
+                                          lanes, ",f32" if mixed else ",f64"),
+        "jacobi_ms": t_jac, "jacobi_bytes": jac_bytes,
+        "spmv_f64_ms": t_spmv, "spmv_f64_GBs": spmv_bytes / (t_spmv * 1e-3) / 1e9,
+        "stream_copy_ms": t_copy, "stream_copy_bytes": copy_bytes,
+        "stream_copy_GBs": copy_bytes / (t_copy * 1e-3) / 1e9,
+        "assemble_local_ms": t_asm,
+        "assemble_local_TFLOPs": asm_flops / (t_asm * 1e-3) / 1e12,
+        "assemble_fp64_frac": asm_flops / (t_asm * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+    }
+
+
+def measured_traffic(kernel, n, nnz):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    hardware-counter summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by
+    the same counters' reading on the calibration copy), or None when no
+    summary covers this kernel and matrix."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "hbm_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    for row in table.get("kernels", []):
+        if row["kernel"] == kernel and row["n_dof"] == n and row["nnz"] == nnz:
+            return {"bytes": row["bytes_per_launch"], "source": "profiles/hbm_traffic.json"}
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +154,12 @@ def main():
     ap.add_argument("--sweep", action="store_true", help="time CSR kernels for each lanes-per-row")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--fixed-shape", default=None, metavar="LANES,VARIANT",
+                    help="skip the autotuner: lanes per row and variant (0 plain, 1 stream, "
+                         "2 unrolled) of both CSR kernels")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="assemble once, time the dominant kernels and the calibration copy, "
+                         "print them as JSON and stop (short runs for hardware counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -126,7 +179,17 @@ def main():
     # the matrix values do not change the launch-shape choice: tune once on
     # the first assembled system, outside the timed region
     cd.assemble()
-    tuning = pm.autotune(cd, log=log)
+    if args.fixed_shape:
+        lanes, variant = (int(v) for v in args.fixed_shape.split(","))
+        pm.set_tuning("spmv", lanes, variant)
+        pm.set_tuning("jacobi", lanes, variant)
+        tuning = {}
+    else:
+        tuning = pm.autotune(cd, log=log)
+    if args.kernels_only:
+        print(json.dumps(kernel_times(cd, info, args.mixed)), flush=True)
+        cd.close()
+        return
 
     lat = []
     iters = []
@@ -144,18 +207,10 @@ def main():
     errs = cd.output()
     value = args.steps / total
 
-    # dominant kernels, timed with HIP events on the system stream
-    t_jac = cd.time_kernel("jacobi_f32" if args.mixed else "jacobi_f64", -1, 20)
-    t_spmv = cd.time_kernel("spmv_f64", -1, 20)
-    t_asm = cd.time_kernel("assemble_local", -1, 3)
-    vb = 4 if args.mixed else 8
-    ib = 4
-    jac_bytes = (vb + ib) * nnz + 4 * (n + 1) + vb * n * 5
-    spmv_bytes = 12 * nnz + 4 * (n + 1) + 16 * n
-    n_cells = info["n_cells"]
-    nq = info["n_quad"]
-    asm_flops = (2 * 27 * 27 * 4 + 2 * 4 * 5 * 27) * nq * n_cells
+    kt = kernel_times(cd, info, args.mixed)
+    t_jac, jac_bytes = kt["jacobi_ms"], kt["jacobi_bytes"]
     achieved = jac_bytes / (t_jac * 1e-3) / 1e9
+    traffic = measured_traffic(kt["jacobi_kernel"], n, nnz)
     result = {
         "metric": "CD3D solves/s (C2: 64^3 Q2 SUPG, FGMRES(30)+MG V(2,2) Jacobi, rel. res 1e-10)",
         "value": value,
@@ -178,15 +233,15 @@ def main():
         "converged": bool(st.converged),
         "l2_error": float(errs[0]),
         "h1_error": float(errs[1]),
-        "roofline": {"bound": "hbm", "kernel": "k_jacobi (finest level)", "achieved": achieved,
-                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": None, "algorithmic_bytes": jac_bytes, "avg_ms": t_jac},
+        "roofline": {"bound": "hbm", "kernel": kt["jacobi_kernel"] + " (finest level)",
+                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": traffic["bytes"] if traffic else None,
+                     "traffic_source": traffic["source"] if traffic else None,
+                     "algorithmic_bytes": jac_bytes, "avg_ms": t_jac},
         "jacobi_damp": args.jacobi_damp,
         "autotune_ms": {k: {str(w): round(t, 5) for w, t in v.items()} for k, v in tuning.items()},
-        "kernels": {"spmv_f64_ms": t_spmv, "spmv_f64_GBs": spmv_bytes / (t_spmv * 1e-3) / 1e9,
-                    "assemble_local_ms": t_asm,
-                    "assemble_local_TFLOPs": asm_flops / (t_asm * 1e-3) / 1e12,
-                    "assemble_fp64_frac": asm_flops / (t_asm * 1e-3) / 1e12 / PEAK_FP64_TFLOPS},
+        "kernels": {k: v for k, v in kt.items() if not k.startswith("jacobi")},
     }
     mats, rhss = [], []
     if not args.no_cpu:

@@ -152,15 +152,19 @@ int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, dou
 /* Average device time (HIP events on the system's stream) of `iters`
  * back-to-back launches of one kernel on level `level`:
  * 0 FP64 SpMV, 1 FP64 Jacobi sweep, 2 FP32 SpMV, 3 local-matrix assembly,
- * 4 row gather, 5 FP32 Jacobi sweep. No reference counterpart (measurement). */
+ * 4 row gather, 5 FP32 Jacobi sweep, 6 streaming copy of the level's matrix
+ * values (HBM calibration). No reference counterpart (measurement). */
 int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* avg_ms);
 
 /* Launch-shape tuning (process-wide; no reference counterpart, knobs of the
  * autotuning sweep): key 0 / 1 = lanes per matrix row of the SpMV / Jacobi
  * kernels (4, 8, 16, 32); key 2 / 3 = their variant (0 plain, 1 streaming with
- * XCD-contiguous rows); key 4 = assembly kernel (0 reference operation order on
+ * XCD-contiguous rows, 2 unrolled with all loads of a lane in flight at once,
+ * taken where the level's longest row fits 16 entries per lane); key 4 = assembly kernel (0 reference operation order on
  * vector ALUs, 1 FP64 MFMA). */
 int pmcd_set_tuning(int key, int value);
+/* Current value of a launch-shape knob (keys as for pmcd_set_tuning). */
+int pmcd_get_tuning(int key, int* value);
 
 /* Lane-layout check of the 16x16x4 FP64 MFMA used by the assembly: writes the
  * 64 lanes x 4 result registers of one product of known operands (256

@@ -60,7 +60,8 @@ EXPORTS = [
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
     "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
     "pmcd_get_rhs", "pmcd_solve", "pmcd_get_solution", "pmcd_set_solution", "pmcd_errors",
-    "pmcd_apply", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_set_tuning", "pmcd_probe_mfma",
+    "pmcd_apply", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_set_tuning", "pmcd_get_tuning",
+    "pmcd_probe_mfma",
     "pmcd_spmv_device", "pmcd_dot_device",
 ]
 
@@ -78,13 +79,26 @@ def probe_mfma():
 
 def set_tuning(kernel, lanes_per_row, variant=0):
     """Launch shape of the 'spmv' or 'jacobi' CSR kernels: lanes per row (4, 8,
-    16, 32) and variant (0 plain, 1 streaming with XCD-contiguous rows)."""
+    16, 32) and variant (0 plain, 1 streaming with XCD-contiguous rows, 2
+    unrolled: all loads of a lane issued before the products, used where the
+    longest row fits 16 entries per lane)."""
     k = {"spmv": 0, "jacobi": 1}[kernel]
     _check(lib().pmcd_set_tuning(k, int(lanes_per_row)))
     _check(lib().pmcd_set_tuning(k + 2, int(variant)))
 
+def tuning(kernel):
+    """(lanes per row, variant) currently set for the 'spmv' or 'jacobi' kernels."""
+    k = {"spmv": 0, "jacobi": 1}[kernel]
+    out = []
+    for key in (k, k + 2):
+        v = ctypes.c_int(0)
+        _check(lib().pmcd_get_tuning(key, ctypes.byref(v)))
+        out.append(v.value)
+    return tuple(out)
+
+
 KERNELS = {"spmv_f64": 0, "jacobi_f64": 1, "spmv_f32": 2, "assemble_local": 3, "gather_rows": 4,
-           "jacobi_f32": 5}
+           "jacobi_f32": 5, "stream_copy": 6}
 
 
 def lib():
@@ -314,15 +328,18 @@ def autotune(cd, level=-1, iters=10, log=None):
     """Pick lanes-per-row of the SpMV and Jacobi kernels for this matrix by
     timing each candidate with HIP events; sets the fastest and returns the
     timings {kernel: {lanes: ms}}."""
+    names = ("plain", "stream", "unrolled")
     out = {}
     for kernel, timed in (("spmv", "spmv_f64"), ("jacobi", "jacobi_f64")):
         res = {}
-        for variant in (0, 1):
+        for variant in (0, 1, 2):
             for w in (8, 16, 32):
                 set_tuning(kernel, w, variant)
-                res["%s%d" % ("stream" if variant else "plain", w)] = cd.time_kernel(timed, level, iters)
+                res[(names[variant], w)] = cd.time_kernel(timed, level, iters)
         best = min(res, key=res.get)
-        set_tuning(kernel, int(best.lstrip("streamplain")), 1 if best.startswith("stream") else 0)
+        set_tuning(kernel, best[1], names.index(best[0]))
+        res = {"%s%d" % k: v for k, v in res.items()}
+        best = "%s%d" % best
         out[kernel] = res
         if log:
             log("autotune %s: %s -> %s" % (kernel, {k: round(v, 4) for k, v in res.items()}, best))

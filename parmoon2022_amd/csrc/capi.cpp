@@ -313,6 +313,13 @@ int pmcd_set_tuning(int key, int value) {
   return guarded([&] { set_tuning(key, value); });
 }
 
+int pmcd_get_tuning(int key, int* value) {
+  return guarded([&] {
+    PMCD_CHECK(value != nullptr, "null output");
+    *value = get_tuning(key);
+  });
+}
+
 int pmcd_probe_mfma(double* out256) {
   return guarded([&] {
     int ndev = 0;

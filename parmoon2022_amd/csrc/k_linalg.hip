@@ -145,6 +145,76 @@ __global__ __launch_bounds__(256) void k_jacobi_stream(int n, int n_active,
   }
 }
 
+// Unrolled variant: every lane owns at most U entries of its row (U W >= the
+// longest row); it issues all its column and value loads, then all the
+// dependent x gathers, then the products, maximising the loads in flight.
+template <int W, int U, class T>
+__device__ inline T row_sum_unrolled(const int* __restrict__ rp, const int* __restrict__ col,
+                                     const T* __restrict__ val, const T* __restrict__ x, int row,
+                                     int l) {
+  const int b = rp[row], e = rp[row + 1];
+  int c[U];
+  T v[U], xv[U];
+#pragma unroll
+  for(int u = 0; u < U; ++u) {
+    const int k = b + l + u * W;
+    c[u] = k < e ? col[k] : -1;
+  }
+#pragma unroll
+  for(int u = 0; u < U; ++u) {
+    const int k = b + l + u * W;
+    v[u] = k < e ? val[k] : T(0);
+  }
+#pragma unroll
+  for(int u = 0; u < U; ++u) xv[u] = c[u] >= 0 ? x[c[u]] : T(0);
+  T s = 0;
+#pragma unroll
+  for(int u = 0; u < U; ++u) s += v[u] * xv[u];
+#pragma unroll
+  for(int off = W / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, W);
+  return s;
+}
+
+template <int W, int U, class T>
+__global__ __launch_bounds__(256) void k_spmv_unrolled(int n, int n_active, const int* __restrict__ rp,
+                                                       const int* __restrict__ col,
+                                                       const T* __restrict__ val,
+                                                       const T* __restrict__ x,
+                                                       const T* __restrict__ z, T* __restrict__ y,
+                                                       T alpha, T beta) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int row = gt / W, l = gt % W;
+  if(row >= n) return;
+  if(row < n_active) {
+    const T s = row_sum_unrolled<W, U, T>(rp, col, val, x, row, l);
+    if(l == 0) y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * s;
+  } else if(l == 0) {
+    y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * x[row];
+  }
+}
+
+template <int W, int U, class T>
+__global__ __launch_bounds__(256) void k_jacobi_unrolled(int n, int n_active,
+                                                         const int* __restrict__ rp,
+                                                         const int* __restrict__ col,
+                                                         const T* __restrict__ val,
+                                                         const T* __restrict__ diag,
+                                                         const T* __restrict__ b,
+                                                         const T* __restrict__ x,
+                                                         T* __restrict__ x_out, T omega) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int row = gt / W, l = gt % W;
+  if(row >= n) return;
+  T s;
+  if(row < n_active) s = row_sum_unrolled<W, U, T>(rp, col, val, x, row, l);
+  else s = x[row];
+  if(l == 0) {
+    T d = b[row] + T(-1) * s;
+    d = d / diag[row];
+    x_out[row] = x[row] + omega * d;
+  }
+}
+
 // First Jacobi sweep from x = 0 with b = 0 on the non-active rows: A x = 0,
 // so the sweep reduces to x_out = 0 + omega * ((b + (-1)*0) / diag) on active
 // rows (the same floating-point result as the general sweep) and x_out = b on
@@ -267,6 +337,13 @@ __global__ void k_fill(int n, T v, T* __restrict__ y) {
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = v;
 }
 
+// Plain streaming copy of 16-byte vectors: the HBM calibration kernel (known
+// bytes read and written per dispatch) and the practical copy ceiling.
+__global__ __launch_bounds__(256) void k_stream_copy(long n2, const double2* __restrict__ x,
+                                                     double2* __restrict__ y) {
+  for(long i = (long)blockIdx.x * 256 + threadIdx.x; i < n2; i += (long)gridDim.x * 256) y[i] = x[i];
+}
+
 __global__ void k_copy(int n, const double* __restrict__ x, double* __restrict__ y) {
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = x[i];
 }
@@ -328,11 +405,54 @@ inline int grid_for(int n, int per_block) {
   return (int)std::max<long>(g, 1);
 }
 
-template <class T>
-void dispatch_spmv(int width, int variant, int n, int n_active, const int* rp, const int* col,
+// per-lane entry count of the unrolled variant for rows of at most max_row
+// entries spread over W lanes: the smallest of 2, 4, 8, 16 that covers them,
+// 0 when max_row is unknown (< 0) or too long
+inline int unroll_for(int max_row, int W) {
+  if(max_row < 0) return 0;
+  for(int u = 2; u <= 16; u *= 2)
+    if(u * W >= max_row) return u;
+  return 0;
+}
+
+template <int W, class T>
+void spmv_unrolled(int u, int blocks, int n, int n_active, const int* rp, const int* col,
                    const T* val, const T* x, const T* z, T* y, T alpha, T beta, hipStream_t s) {
+  switch(u) {
+    case 2: k_spmv_unrolled<W, 2, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    case 4: k_spmv_unrolled<W, 4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    case 8: k_spmv_unrolled<W, 8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    default: k_spmv_unrolled<W, 16, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+  }
+}
+
+template <int W, class T>
+void jacobi_unrolled(int u, int blocks, int n, int n_active, const int* rp, const int* col,
+                     const T* val, const T* diag, const T* b, const T* x, T* x_out, T omega,
+                     hipStream_t s) {
+  switch(u) {
+    case 2: k_jacobi_unrolled<W, 2, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    case 4: k_jacobi_unrolled<W, 4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    case 8: k_jacobi_unrolled<W, 8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    default: k_jacobi_unrolled<W, 16, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+  }
+}
+
+template <class T>
+void dispatch_spmv(int width, int variant, int max_row, int n, int n_active, const int* rp,
+                   const int* col, const T* val, const T* x, const T* z, T* y, T alpha, T beta,
+                   hipStream_t s) {
   const long threads = (long)n * width;
   const int blocks = (int)((threads + 255) / 256);
+  const int u = width >= 8 ? unroll_for(max_row, width) : 0;
+  if(variant == 2 && u) {
+    switch(width) {
+      case 8: spmv_unrolled<8, T>(u, blocks, n, n_active, rp, col, val, x, z, y, alpha, beta, s); break;
+      case 16: spmv_unrolled<16, T>(u, blocks, n, n_active, rp, col, val, x, z, y, alpha, beta, s); break;
+      default: spmv_unrolled<32, T>(u, blocks, n, n_active, rp, col, val, x, z, y, alpha, beta, s); break;
+    }
+    return;
+  }
   if(variant == 1) {
     switch(width) {
       case 4: k_spmv_stream<4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
@@ -485,7 +605,8 @@ void dense_inverse(int n, double* A, double* I, double* work, int* piv, hipStrea
 }
 
 // launch shape of the CSR kernels (tunable): lanes per row and variant
-// (0 plain, 1 streaming with XCD-contiguous rows)
+// (0 plain, 1 streaming with XCD-contiguous rows, 2 unrolled where the
+// longest row is known and fits)
 static int g_spmv_width = 16, g_spmv_variant = 0;
 static int g_jacobi_width = 16, g_jacobi_variant = 0;
 
@@ -501,12 +622,23 @@ void set_tuning(int key, int value) {
   if(key == 0 || key == 1)
     PMCD_CHECK(value == 4 || value == 8 || value == 16 || value == 32, "lanes per row must be 4, 8, 16 or 32");
   else
-    PMCD_CHECK(value == 0 || value == 1, "variant must be 0 or 1");
+    PMCD_CHECK(value >= 0 && value <= 2, "variant must be 0, 1 or 2");
   switch(key) {
     case 0: g_spmv_width = value; break;
     case 1: g_jacobi_width = value; break;
     case 2: g_spmv_variant = value; break;
     case 3: g_jacobi_variant = value; break;
+    default: PMCD_FAIL("unknown tuning key ", key);
+  }
+}
+
+int get_tuning(int key) {
+  switch(key) {
+    case 0: return g_spmv_width;
+    case 1: return g_jacobi_width;
+    case 2: return g_spmv_variant;
+    case 3: return g_jacobi_variant;
+    case 4: return g_assembly_variant;
     default: PMCD_FAIL("unknown tuning key ", key);
   }
 }
@@ -522,25 +654,34 @@ void launch_jacobi_from_zero_f32(int n, int n_active, const float* diag, const f
 
 void launch_spmv(int n, int n_active, const int* rp, const int* col, const double* val,
                  const double* x, const double* z, double* y, double alpha, double beta,
-                 hipStream_t s) {
+                 hipStream_t s, int max_row) {
   if(n == 0) return;
-  dispatch_spmv<double>(g_spmv_width, g_spmv_variant, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
+  dispatch_spmv<double>(g_spmv_width, g_spmv_variant, max_row, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
   PMCD_HIP(hipGetLastError());
 }
 
 void launch_spmv_f32(int n, int n_active, const int* rp, const int* col, const float* val,
                      const float* x, const float* z, float* y, float alpha, float beta,
-                     hipStream_t s) {
+                     hipStream_t s, int max_row) {
   if(n == 0) return;
-  dispatch_spmv<float>(g_spmv_width, g_spmv_variant, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
+  dispatch_spmv<float>(g_spmv_width, g_spmv_variant, max_row, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
   PMCD_HIP(hipGetLastError());
 }
 
 template <class T>
-static void dispatch_jacobi(int width, int n, int n_active, const int* rp, const int* col,
-                            const T* val, const T* diag, const T* b, const T* x, T* x_out, T omega,
-                            hipStream_t s) {
+static void dispatch_jacobi(int width, int max_row, int n, int n_active, const int* rp,
+                            const int* col, const T* val, const T* diag, const T* b, const T* x,
+                            T* x_out, T omega, hipStream_t s) {
   const int blocks = (int)(((long)n * width + 255) / 256);
+  const int u = width >= 8 ? unroll_for(max_row, width) : 0;
+  if(g_jacobi_variant == 2 && u) {
+    switch(width) {
+      case 8: jacobi_unrolled<8, T>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
+      case 16: jacobi_unrolled<16, T>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
+      default: jacobi_unrolled<32, T>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
+    }
+    return;
+  }
   if(g_jacobi_variant == 1) {
     switch(width) {
       case 4: k_jacobi_stream<4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
@@ -560,17 +701,17 @@ static void dispatch_jacobi(int width, int n, int n_active, const int* rp, const
 
 void launch_jacobi(int n, int n_active, const int* rp, const int* col, const double* val,
                    const double*, const double* diag, const double* b, const double* x,
-                   double* x_out, double omega, hipStream_t s) {
+                   double* x_out, double omega, hipStream_t s, int max_row) {
   if(n == 0) return;
-  dispatch_jacobi<double>(g_jacobi_width, n, n_active, rp, col, val, diag, b, x, x_out, omega, s);
+  dispatch_jacobi<double>(g_jacobi_width, max_row, n, n_active, rp, col, val, diag, b, x, x_out, omega, s);
   PMCD_HIP(hipGetLastError());
 }
 
 void launch_jacobi_f32(int n, int n_active, const int* rp, const int* col, const float* val,
                        const float* diag, const float* b, const float* x, float* x_out,
-                       float omega, hipStream_t s) {
+                       float omega, hipStream_t s, int max_row) {
   if(n == 0) return;
-  dispatch_jacobi<float>(g_jacobi_width, n, n_active, rp, col, val, diag, b, x, x_out, omega, s);
+  dispatch_jacobi<float>(g_jacobi_width, max_row, n, n_active, rp, col, val, diag, b, x, x_out, omega, s);
   PMCD_HIP(hipGetLastError());
 }
 
@@ -628,6 +769,13 @@ void launch_fill(int n, double v, double* y, hipStream_t s) {
 void launch_fill_f32(int n, float v, float* y, hipStream_t s) {
   if(n) k_fill<float><<<grid_for(n, 256 * 4), 256, 0, s>>>(n, v, y);
 }
+void launch_stream_copy(long n, const double* x, double* y, hipStream_t s) {
+  PMCD_CHECK(n % 2 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)y % 16) == 0,
+             "stream copy needs an even count of 16-byte aligned doubles");
+  if(n) k_stream_copy<<<2048, 256, 0, s>>>(n / 2, (const double2*)x, (double2*)y);
+  PMCD_HIP(hipGetLastError());
+}
+
 void launch_copy(int n, const double* x, double* y, hipStream_t s) {
   if(n) k_copy<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, x, y);
 }

@@ -53,17 +53,18 @@ void launch_cell_errors(const AsmArgs& a, const int* dof, const double* u, doubl
 // y[r] = beta*z[r] + alpha*(A x)[r] for r < n_active, beta*z[r] + alpha*x[r] otherwise
 void launch_spmv(int n, int n_active, const int* rp, const int* col, const double* val,
                  const double* x, const double* z, double* y, double alpha, double beta,
-                 hipStream_t s);
+                 hipStream_t s, int max_row = -1);
 void launch_spmv_f32(int n, int n_active, const int* rp, const int* col, const float* val,
                      const float* x, const float* z, float* y, float alpha, float beta,
-                     hipStream_t s);
+                     hipStream_t s, int max_row = -1);
 // damped Jacobi sweep x_out = x + omega * (b - A x) / diag (non-active rows use A = I)
 void launch_jacobi(int n, int n_active, const int* rp, const int* col, const double* val,
                    const double* inv_diag_unused, const double* diag, const double* b,
-                   const double* x, double* x_out, double omega, hipStream_t s);
+                   const double* x, double* x_out, double omega, hipStream_t s,
+                   int max_row = -1);
 void launch_jacobi_f32(int n, int n_active, const int* rp, const int* col, const float* val,
                        const float* diag, const float* b, const float* x, float* x_out,
-                       float omega, hipStream_t s);
+                       float omega, hipStream_t s, int max_row = -1);
 // first sweep from x = 0 when b vanishes on the non-active rows (no SpMV needed)
 void launch_jacobi_from_zero(int n, int n_active, const double* diag, const double* b,
                              double* x_out, double omega, hipStream_t s);
@@ -71,6 +72,7 @@ void launch_jacobi_from_zero_f32(int n, int n_active, const float* diag, const f
                                  float* x_out, float omega, hipStream_t s);
 // lanes per row of the CSR kernels: key 0 SpMV, 1 Jacobi; value 4, 8, 16 or 32
 void set_tuning(int key, int value);
+int get_tuning(int key);
 // plain CSR product y = A x over all rows (transfer operators)
 void launch_csr(int n, const int* rp, const int* col, const double* val, const double* x,
                 double* y, hipStream_t s);
@@ -92,6 +94,8 @@ void launch_scale_copy_dev(int n, const double* num_dev, int invert, const doubl
                            hipStream_t s);  // y = x * (invert ? 1/(*num) : *num)
 void launch_fill(int n, double v, double* y, hipStream_t s);
 void launch_copy(int n, const double* x, double* y, hipStream_t s);
+// y = x for n doubles (n even, 16-byte aligned), 16-byte vector accesses
+void launch_stream_copy(long n, const double* x, double* y, hipStream_t s);
 // fused MGS step: r -= h_k v_k; h_{k+1} = <r, v_{k+1}> partials (v_next may be null)
 void launch_mgs_step(int n, const double* h_dev, double* r, const double* v,
                      const double* v_next, double* partial, double* out_next, hipStream_t s);

@@ -368,12 +368,14 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
     launch_fill_f32(n, 1.0f, xf.p, stream_);
     launch_fill_f32(n, 0.5f, bf.p, stream_);
   }
+  DevBuf<double> copy_dst;
+  if(which == 6) copy_dst.alloc(L.val.n & ~(size_t)1);
   AsmArgs a = asm_args(L);
   auto launch = [&]() {
     switch(which) {
-      case 0: launch_spmv(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x.p, nullptr, y.p, 1.0, 0.0, stream_); break;
-      case 1: launch_jacobi(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b.p, x.p, y.p, 0.8, stream_); break;
-      case 2: launch_spmv_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, xf.p, nullptr, yf.p, 1.0f, 0.0f, stream_); break;
+      case 0: launch_spmv(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x.p, nullptr, y.p, 1.0, 0.0, stream_, L.st.max_row); break;
+      case 1: launch_jacobi(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b.p, x.p, y.p, 0.8, stream_, L.st.max_row); break;
+      case 2: launch_spmv_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, xf.p, nullptr, yf.p, 1.0f, 0.0f, stream_, L.st.max_row); break;
       case 3:
         if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
         break;
@@ -381,7 +383,8 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
         launch_gather_matrix(n, L.n_active(), a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p, L.contrib.p,
                              L.slot.p, local_A_.p, L.val.p, stream_);
         break;
-      case 5: launch_jacobi_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, bf.p, xf.p, yf.p, 0.8f, stream_); break;
+      case 5: launch_jacobi_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, bf.p, xf.p, yf.p, 0.8f, stream_, L.st.max_row); break;
+      case 6: launch_stream_copy((long)copy_dst.n, L.val.p, copy_dst.p, stream_); break;
       default: PMCD_FAIL("unknown kernel id ", which);
     }
   };
@@ -401,11 +404,12 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
 }
 
 void CDSystem::apply(Level& L, const double* x, double* y) {
-  launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, nullptr, y, 1.0, 0.0, stream_);
+  launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, nullptr, y, 1.0, 0.0, stream_,
+              L.st.max_row);
 }
 
 void CDSystem::apply_scaled_add(Level& L, const double* x, double* y, double a) {
-  launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, y, y, a, 1.0, stream_);
+  launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, y, y, a, 1.0, stream_, L.st.max_row);
 }
 
 double CDSystem::dot(int n, const double* x, const double* y) {
@@ -426,7 +430,7 @@ int CDSystem::smooth(Level& L, double* const x[2], int cur, const double* b, int
       launch_jacobi_from_zero(L.n(), L.n_active(), L.diag.p, b, x[1 - cur], sd_.jacobi_damp, stream_);
     else
       launch_jacobi(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b, x[cur],
-                    x[1 - cur], sd_.jacobi_damp, stream_);
+                    x[1 - cur], sd_.jacobi_damp, stream_, L.st.max_row);
     cur = 1 - cur;
   }
   return cur;
@@ -440,7 +444,7 @@ int CDSystem::smooth32(Level& L, float* const x[2], int cur, const float* b, int
                                   (float)sd_.jacobi_damp, stream_);
     else
       launch_jacobi_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x[cur],
-                        x[1 - cur], (float)sd_.jacobi_damp, stream_);
+                        x[1 - cur], (float)sd_.jacobi_damp, stream_, L.st.max_row);
     cur = 1 - cur;
   }
   return cur;
@@ -476,7 +480,7 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
     const bool zero = (l == Lf) ? fine_zero : sd_.n_pre > 0;
     cur[l] = smooth(L, x[l].data(), cur[l], b[l], sd_.n_pre, zero);
     launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x[l][cur[l]], b[l], L.mg_def.p,
-                -1.0, 1.0, stream_);
+                -1.0, 1.0, stream_, L.st.max_row);
     if(sd_.n_pre == 0 || l - 1 == 0) launch_fill(C.n(), 0.0, x[l - 1][0], stream_);
     launch_csr(C.n(), L.r_rp.p, L.r_col.p, L.r_val.p, L.mg_def.p, C.mg_rhs.p, stream_);
     launch_zero_tail(C.n(), C.n_active(), C.mg_rhs.p, stream_);
@@ -525,7 +529,7 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
     const bool zero = (l == Lf) ? fine_zero : sd_.n_pre > 0;
     cur[l] = smooth32(L, x[l].data(), cur[l], L.f_rhs.p, sd_.n_pre, zero);
     launch_spmv_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, x[l][cur[l]], L.f_rhs.p,
-                    L.f_def.p, -1.0f, 1.0f, stream_);
+                    L.f_def.p, -1.0f, 1.0f, stream_, L.st.max_row);
     if(sd_.n_pre == 0 || l - 1 == 0) launch_fill_f32(C.n(), 0.0f, x[l - 1][0], stream_);
     launch_csr_f32(C.n(), L.r_rp.p, L.r_col.p, L.r_val32.p, L.f_def.p, C.f_rhs.p, stream_);
     launch_zero_tail_f32(C.n(), C.n_active(), C.f_rhs.p, stream_);

@@ -151,7 +151,6 @@ def main():
     ap.add_argument("--jacobi-damp", type=float, default=0.5,
                     help="Jacobi smoother damping; 0.5 by default because 0.8 does not converge "
                          "for this SUPG operator (measured: 1e-3 reduction in 4000 iterations)")
-    ap.add_argument("--sweep", action="store_true", help="time CSR kernels for each lanes-per-row")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--fixed-shape", default=None, metavar="LANES,VARIANT",
@@ -211,6 +210,7 @@ def main():
     t_jac, jac_bytes = kt["jacobi_ms"], kt["jacobi_bytes"]
     achieved = jac_bytes / (t_jac * 1e-3) / 1e9
     traffic = measured_traffic(kt["jacobi_kernel"], n, nnz)
+    nq = info["n_quad"]
     result = {
         "metric": "CD3D solves/s (C2: 64^3 Q2 SUPG, FGMRES(30)+MG V(2,2) Jacobi, rel. res 1e-10)",
         "value": value,

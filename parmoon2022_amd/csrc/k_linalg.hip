@@ -248,6 +248,55 @@ __global__ __launch_bounds__(256) void k_csr(int n, const int* __restrict__ rp,
   if(l == 0) y[row] = s;
 }
 
+// Grid transfer with the reference's non-active rules fused in (Multigrid.C:
+// 310-410): restriction writes (R d)[r] on active coarse rows and 0 on the
+// Dirichlet tail; prolongation adds a (P e)[r] to the fine iterate on active
+// rows only. U = 0 walks the row in a loop; U > 0 is the unrolled row sum
+// (U W >= the longest row). Same per-lane order and butterfly as k_csr.
+template <int W, int U, class T>
+__device__ inline T transfer_row(const int* __restrict__ rp, const int* __restrict__ col,
+                                 const T* __restrict__ val, const T* __restrict__ x, int row, int l) {
+  if constexpr(U > 0) {
+    return row_sum_unrolled<W, U, T>(rp, col, val, x, row, l);
+  } else {
+    T s = 0;
+    const int e = rp[row + 1];
+    for(int k = rp[row] + l; k < e; k += W) s += val[k] * x[col[k]];
+#pragma unroll
+    for(int off = W / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, W);
+    return s;
+  }
+}
+
+template <int W, int U, class T>
+__global__ __launch_bounds__(256) void k_restrict(int n, int n_active, const int* __restrict__ rp,
+                                                  const int* __restrict__ col,
+                                                  const T* __restrict__ val,
+                                                  const T* __restrict__ x, T* __restrict__ y) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int row = gt / W, l = gt % W;
+  if(row >= n) return;
+  if(row < n_active) {
+    const T s = transfer_row<W, U, T>(rp, col, val, x, row, l);
+    if(l == 0) y[row] = s;
+  } else if(l == 0) {
+    y[row] = T(0);
+  }
+}
+
+template <int W, int U, class T>
+__global__ __launch_bounds__(256) void k_prolong_add(int n_active, const int* __restrict__ rp,
+                                                     const int* __restrict__ col,
+                                                     const T* __restrict__ val,
+                                                     const T* __restrict__ x, T* __restrict__ y,
+                                                     T a) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int row = gt / W, l = gt % W;
+  if(row >= n_active) return;
+  const T s = transfer_row<W, U, T>(rp, col, val, x, row, l);
+  if(l == 0) y[row] += a * s;
+}
+
 __device__ inline double block_sum(double v) {
   __shared__ double s_w[4];
   for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -713,6 +762,59 @@ void launch_jacobi_f32(int n, int n_active, const int* rp, const int* col, const
   if(n == 0) return;
   dispatch_jacobi<float>(g_jacobi_width, max_row, n, n_active, rp, col, val, diag, b, x, x_out, omega, s);
   PMCD_HIP(hipGetLastError());
+}
+
+// lanes per row from the mean row length, unroll depth from the longest row
+template <class T, bool PROLONG>
+static void dispatch_transfer(int n, int n_active, long nnz, int max_row, const int* rp,
+                              const int* col, const T* val, const T* x, T* y, T a, hipStream_t s) {
+  const int rows = PROLONG ? n_active : n;
+  if(rows <= 0) return;
+  const double mean = n > 0 ? (double)nnz / n : 0.0;
+  const int W = mean <= 12 ? 4 : (mean <= 24 ? 8 : 16);
+  const int u = unroll_for(max_row, W);
+  const int blocks = (int)(((long)rows * W + 255) / 256);
+#define PMCD_TRANSFER(WW, UU)                                                                    \
+  do {                                                                                           \
+    if constexpr(PROLONG)                                                                        \
+      k_prolong_add<WW, UU, T><<<blocks, 256, 0, s>>>(n_active, rp, col, val, x, y, a);         \
+    else                                                                                         \
+      k_restrict<WW, UU, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, y);             \
+  } while(0)
+#define PMCD_TRANSFER_U(WW)                      \
+  switch(u) {                                    \
+    case 2: PMCD_TRANSFER(WW, 2); break;         \
+    case 4: PMCD_TRANSFER(WW, 4); break;         \
+    case 8: PMCD_TRANSFER(WW, 8); break;         \
+    case 16: PMCD_TRANSFER(WW, 16); break;       \
+    default: PMCD_TRANSFER(WW, 0); break;        \
+  }
+  switch(W) {
+    case 4: PMCD_TRANSFER_U(4); break;
+    case 8: PMCD_TRANSFER_U(8); break;
+    default: PMCD_TRANSFER_U(16); break;
+  }
+#undef PMCD_TRANSFER_U
+#undef PMCD_TRANSFER
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_restrict(int n, int n_active, long nnz, int max_row, const int* rp, const int* col,
+                     const double* val, const double* x, double* y, hipStream_t s) {
+  dispatch_transfer<double, false>(n, n_active, nnz, max_row, rp, col, val, x, y, 0.0, s);
+}
+void launch_restrict_f32(int n, int n_active, long nnz, int max_row, const int* rp, const int* col,
+                         const float* val, const float* x, float* y, hipStream_t s) {
+  dispatch_transfer<float, false>(n, n_active, nnz, max_row, rp, col, val, x, y, 0.0f, s);
+}
+void launch_prolong_add(int n, int n_active, long nnz, int max_row, const int* rp, const int* col,
+                        const double* val, const double* x, double* y, double a, hipStream_t s) {
+  dispatch_transfer<double, true>(n, n_active, nnz, max_row, rp, col, val, x, y, a, s);
+}
+void launch_prolong_add_f32(int n, int n_active, long nnz, int max_row, const int* rp,
+                            const int* col, const float* val, const float* x, float* y, float a,
+                            hipStream_t s) {
+  dispatch_transfer<float, true>(n, n_active, nnz, max_row, rp, col, val, x, y, a, s);
 }
 
 void launch_csr(int n, const int* rp, const int* col, const double* val, const double* x,

@@ -74,6 +74,17 @@ void launch_jacobi_from_zero_f32(int n, int n_active, const float* diag, const f
 void set_tuning(int key, int value);
 int get_tuning(int key);
 // plain CSR product y = A x over all rows (transfer operators)
+// restriction y = R x on active rows, 0 on rows >= n_active
+void launch_restrict(int n, int n_active, long nnz, int max_row, const int* rp, const int* col,
+                     const double* val, const double* x, double* y, hipStream_t s);
+void launch_restrict_f32(int n, int n_active, long nnz, int max_row, const int* rp, const int* col,
+                         const float* val, const float* x, float* y, hipStream_t s);
+// prolongation y[r] += a (P x)[r] for r < n_active (n rows, nnz entries in P)
+void launch_prolong_add(int n, int n_active, long nnz, int max_row, const int* rp, const int* col,
+                        const double* val, const double* x, double* y, double a, hipStream_t s);
+void launch_prolong_add_f32(int n, int n_active, long nnz, int max_row, const int* rp,
+                            const int* col, const float* val, const float* x, float* y, float a,
+                            hipStream_t s);
 void launch_csr(int n, const int* rp, const int* col, const double* val, const double* x,
                 double* y, hipStream_t s);
 void launch_csr_f32(int n, const int* rp, const int* col, const float* val, const float* x,

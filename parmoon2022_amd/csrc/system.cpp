@@ -283,6 +283,10 @@ void CDSystem::build_transfer(Level& F, const Level& C) {
       tval[pos] = val[k];
     }
   F.n_coarse = ncd;
+  F.p_max_row = 0;
+  for(int f = 0; f < nf; ++f) F.p_max_row = std::max(F.p_max_row, rp[f + 1] - rp[f]);
+  F.r_max_row = 0;
+  for(int c = 0; c < ncd; ++c) F.r_max_row = std::max(F.r_max_row, trp[c + 1] - trp[c]);
   F.p_rp.upload(rp, stream_); F.p_col.upload(col, stream_); F.p_val.upload(val, stream_);
   F.r_rp.upload(trp, stream_); F.r_col.upload(tcol, stream_); F.r_val.upload(tval, stream_);
   std::vector<float> v32(val.begin(), val.end()), t32(tval.begin(), tval.end());
@@ -482,8 +486,8 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
     launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x[l][cur[l]], b[l], L.mg_def.p,
                 -1.0, 1.0, stream_, L.st.max_row);
     if(sd_.n_pre == 0 || l - 1 == 0) launch_fill(C.n(), 0.0, x[l - 1][0], stream_);
-    launch_csr(C.n(), L.r_rp.p, L.r_col.p, L.r_val.p, L.mg_def.p, C.mg_rhs.p, stream_);
-    launch_zero_tail(C.n(), C.n_active(), C.mg_rhs.p, stream_);
+    launch_restrict(C.n(), C.n_active(), (long)L.r_val.n, L.r_max_row, L.r_rp.p, L.r_col.p,
+                    L.r_val.p, L.mg_def.p, C.mg_rhs.p, stream_);
   }
   {
     Level& C = *levels_[0];
@@ -496,8 +500,8 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
   }
   for(int l = 1; l <= Lf; ++l) {
     Level& L = *levels_[l];
-    launch_csr(L.n(), L.p_rp.p, L.p_col.p, L.p_val.p, x[l - 1][cur[l - 1]], L.mg_tmp.p, stream_);
-    launch_axpy_active(L.n_active(), sd_.mg_damp, L.mg_tmp.p, x[l][cur[l]], stream_);
+    launch_prolong_add(L.n(), L.n_active(), (long)L.p_val.n, L.p_max_row, L.p_rp.p, L.p_col.p,
+                       L.p_val.p, x[l - 1][cur[l - 1]], x[l][cur[l]], sd_.mg_damp, stream_);
     cur[l] = smooth(L, x[l].data(), cur[l], b[l], sd_.n_post, false);
   }
   if(cur[Lf] != 0) launch_copy(finest().n(), x[Lf][cur[Lf]], out, stream_);
@@ -531,8 +535,8 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
     launch_spmv_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, x[l][cur[l]], L.f_rhs.p,
                     L.f_def.p, -1.0f, 1.0f, stream_, L.st.max_row);
     if(sd_.n_pre == 0 || l - 1 == 0) launch_fill_f32(C.n(), 0.0f, x[l - 1][0], stream_);
-    launch_csr_f32(C.n(), L.r_rp.p, L.r_col.p, L.r_val32.p, L.f_def.p, C.f_rhs.p, stream_);
-    launch_zero_tail_f32(C.n(), C.n_active(), C.f_rhs.p, stream_);
+    launch_restrict_f32(C.n(), C.n_active(), (long)L.r_val32.n, L.r_max_row, L.r_rp.p, L.r_col.p,
+                        L.r_val32.p, L.f_def.p, C.f_rhs.p, stream_);
   }
   {
     Level& C = *levels_[0];
@@ -548,8 +552,9 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
   }
   for(int l = 1; l <= Lf; ++l) {
     Level& L = *levels_[l];
-    launch_csr_f32(L.n(), L.p_rp.p, L.p_col.p, L.p_val32.p, x[l - 1][cur[l - 1]], L.f_tmp.p, stream_);
-    launch_axpy_f32(L.n(), (float)sd_.mg_damp, L.f_tmp.p, x[l][cur[l]], L.n_active(), stream_);
+    launch_prolong_add_f32(L.n(), L.n_active(), (long)L.p_val32.n, L.p_max_row, L.p_rp.p,
+                           L.p_col.p, L.p_val32.p, x[l - 1][cur[l - 1]], x[l][cur[l]],
+                           (float)sd_.mg_damp, stream_);
     cur[l] = smooth32(L, x[l].data(), cur[l], L.f_rhs.p, sd_.n_post, false);
   }
   launch_f2d(F.n(), x[Lf][cur[Lf]], out, stream_);

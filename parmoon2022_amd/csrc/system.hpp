@@ -76,6 +76,7 @@ struct Level {
   DevBuf<double> p_val, r_val;
   DevBuf<float> p_val32, r_val32;
   int n_coarse = 0;
+  int p_max_row = -1, r_max_row = -1;  // longest rows of P and R
   int n() const { return space.n_dof; }
   int n_active() const { return space.n_active; }
 };

@@ -52,8 +52,8 @@ static const int JV_HEX[6][4] = {{0, 1, 2, 3}, {0, 4, 5, 1}, {1, 5, 6, 2},
 static int jv_len(int s) { return s == TRI || s == QUAD ? 2 : (s == TET ? 3 : 4); }
 static int jv(int s, int j, int k) {
   switch(s) {
-    case TRI: return JV_TRI[j][k];
-    case QUAD: return JV_QUAD[j][k];
+    case TRI: return k < 2 ? JV_TRI[j][k] : -1; /* edges have 2 vertices */
+    case QUAD: return k < 2 ? JV_QUAD[j][k] : -1;
     case TET: return JV_TET[j][k];
     default: return JV_HEX[j][k];
   }

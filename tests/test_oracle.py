@@ -140,3 +140,27 @@ def test_oracle_fgmres_mg_solution_in_space(oracle_lib):
     st = o.solve(pm.solver_desc(db))
     e = o.errors()
     assert e[0] < 1e-12 and st.iterations < 50
+
+
+def test_c2_operator_jacobi_damping_bound(oracle_lib):
+    """Why the C2 benchmark damps its Jacobi smoother with 0.5, not the
+    configured 0.8: for the Q2 SUPG operator (eps 1e-3, b = (1,1,1)/sqrt 3)
+    the eigenvalues of D^-1 A reach modulus ~3.2, so I - 0.8 D^-1 A has
+    spectral radius above 1 (the smoother amplifies those modes), while
+    I - 0.5 D^-1 A stays a contraction on them."""
+    import math
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    s3 = 1 / math.sqrt(3)
+    db = {"geo_file": "Default_UnitCube_Hexa", "ANSATZ_ORDER": 2, "refinement_n_initial_steps": 3,
+          "example": "sine_3d", "space_discretization_type": "supg", "diffusion_coefficient": 1e-3,
+          "convection": (s3, s3, s3)}
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    na = o.info()["n_active"]
+    rp, col = o.pattern()
+    A = sp.csr_matrix((o.matrix_values(), col, rp))[:na, :na]
+    DA = sp.diags(1 / A.diagonal()) @ A
+    ev = sla.eigs(DA, k=6, which="LM", return_eigenvectors=False, tol=1e-8)
+    assert np.max(np.abs(1 - 0.8 * ev)) > 1.2
+    assert np.max(np.abs(1 - 0.5 * ev)) < 1.0

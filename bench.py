@@ -190,6 +190,8 @@ def main():
     else:
         tuning = pm.autotune(cd, log=log)
     if args.kernels_only:
+        if args.mixed:
+            cd.solve()  # the FP32 copies of the matrices are made by the solver's set-up
         print(json.dumps(kernel_times(cd, info, args.mixed)), flush=True)
         cd.close()
         return

@@ -46,7 +46,8 @@ def c2_db(args):
             "multigrid_n_pre_smooth": 2, "multigrid_n_post_smooth": 2, "jacobi_damp": args.jacobi_damp,
             "multigrid_correction_damp_factor": 1.0, "gmres_restart": 30,
             "residual_reduction": 1e-10, "residual_tolerance": 0.0, "max_n_iterations": args.max_iter,
-            "multigrid_mixed_precision": 1 if args.mixed else 0}
+            "multigrid_mixed_precision": 1 if args.mixed else 0,
+            "multigrid_smoother": args.smoother, "sor_omega": args.sor_omega}
 
 
 def cpu_baseline(db, gpu_iters, sample_div, matrices, rhs):
@@ -155,6 +156,9 @@ def main():
     ap.add_argument("--jacobi-damp", type=float, default=0.5,
                     help="Jacobi smoother damping; 0.5 by default because 0.8 does not converge "
                          "for this SUPG operator (measured: 1e-3 reduction in 4000 iterations)")
+    ap.add_argument("--smoother", default="jacobi", choices=["jacobi", "sor", "ssor"],
+                    help="multigrid smoother (multigrid_smoother)")
+    ap.add_argument("--sor-omega", type=float, default=1.0, help="sor_omega for the SOR smoothers")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--fixed-shape", default=None, metavar="LANES,VARIANT",
@@ -246,6 +250,7 @@ def main():
                      "traffic_source": traffic["source"] if traffic else None,
                      "algorithmic_bytes": jac_bytes, "avg_ms": t_jac},
         "jacobi_damp": args.jacobi_damp,
+        "smoother": args.smoother,
         "autotune_ms": {k: {str(w): round(t, 5) for w, t in v.items()} for k, v in tuning.items()},
         "kernels": {k: v for k, v in kt.items() if not k.startswith("jacobi")},
     }

@@ -57,8 +57,15 @@ typedef struct {
   double f0;        /* right-hand side (PMCD_EX_CONST_RHS) */
 } pmcd_problem_desc;
 
-enum { PMCD_DIRECT = 0, PMCD_CG = 1, PMCD_FGMRES = 2, PMCD_RICHARDSON = 3 };
-enum { PMCD_NO_PRECOND = 0, PMCD_JACOBI = 1, PMCD_MULTIGRID = 2 };
+/* solver_type / iterative_solver_type: direct, cg, fgmres, richardson, and the
+ * SOR / SSOR iterations (Iteration_sor, src/Solver/Iteration_sor.C:27-120) */
+enum { PMCD_DIRECT = 0, PMCD_CG = 1, PMCD_FGMRES = 2, PMCD_RICHARDSON = 3, PMCD_SOR_ITER = 4,
+       PMCD_SSOR_ITER = 5 };
+/* preconditioner: no_preconditioner, jacobi, multigrid, sor, ssor
+ * (Solver.C:214-241; SOR/SSOR apply one sweep from zero, Iteration_sor.C:125-140) */
+enum { PMCD_NO_PRECOND = 0, PMCD_JACOBI = 1, PMCD_MULTIGRID = 2, PMCD_SOR = 3, PMCD_SSOR = 4 };
+/* multigrid_smoother: jacobi, sor, ssor (Multigrid.C:19-30, MultigridLevel.C:40-66) */
+enum { PMCD_SMOOTH_JACOBI = 0, PMCD_SMOOTH_SOR = 1, PMCD_SMOOTH_SSOR = 2 };
 
 typedef struct {
   int method;          /* solver_type / iterative_solver_type */
@@ -73,6 +80,8 @@ typedef struct {
   double jacobi_damp;  /* jacobi_damp */
   double mg_damp;      /* multigrid_correction_damp_factor */
   int mixed;           /* 1: multigrid V-cycle in FP32 inside the FP64 outer solver */
+  int smoother;        /* multigrid_smoother: PMCD_SMOOTH_* */
+  double sor_omega;    /* sor_omega (reference default 1.5, Solver.C:154-156) */
 } pmcd_solver_desc;
 
 typedef struct {
@@ -101,6 +110,15 @@ int pmcd_host_info(pmcd_host* h, int level, long long* info);
 int pmcd_host_dofmap(pmcd_host* h, int level, int* out);
 int pmcd_host_pattern(pmcd_host* h, int level, int* row_ptr, int* col);
 int pmcd_host_cell_coords(pmcd_host* h, int level, double* out);
+/* Row classes of the device SOR sweep for a level (host computation): rows
+ * (n_dof ints) grouped by class, class_ptr (n_dof + 1 ints, may be NULL; the
+ * first n_classes + 1 are set). exact = 1 when the classes are the level
+ * schedule of the row-order dependencies (the device sweep then equals the
+ * serial TMatrix::sor_sweep, Matrix.C:1370-1395, bit for bit), 0 for greedy
+ * multicolour order (used when there are more than max_exact_classes levels).
+ * No reference counterpart. */
+int pmcd_host_sweep_order(pmcd_host* h, int level, int max_exact_classes, int* rows, int* class_ptr,
+                          int* n_classes, int* exact);
 
 /* Mesh hierarchy, FE spaces, sparsity, device buffers.
  * Replaces ConvectionDiffusion<d>::ConvectionDiffusion
@@ -148,6 +166,13 @@ int pmcd_apply(pmcd_system* s, int level, const double* x, double* y, double a, 
 /* One multigrid cycle z = MG(v) on the finest level with the given solver
  * settings (Iteration_multigrid::apply, Iteration_multigrid.C:23-34). */
 int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, double* z);
+/* One SOR sweep x <- sweep(b, x) on level `level`, in place on host arrays
+ * (TMatrix::sor_sweep, src/Matrix/Matrix.C:1308-1420): flag 0 forward, 1
+ * backward, 2 forward then backward (SSOR). *exact (may be NULL) is set to 1
+ * when the device visits rows in an order equivalent to the serial sweep
+ * (bit-identical result), 0 when it uses multicolour order. */
+int pmcd_sor_sweep(pmcd_system* s, int level, const double* b, double* x, double omega, int flag,
+                   int* exact);
 
 /* Average device time (HIP events on the system's stream) of `iters`
  * back-to-back launches of one kernel on level `level`:

@@ -23,7 +23,7 @@ class OSolver(ctypes.Structure):
                 ("div_factor", ctypes.c_double), ("restart", ctypes.c_int),
                 ("n_pre", ctypes.c_int), ("n_post", ctypes.c_int),
                 ("jacobi_damp", ctypes.c_double), ("mg_damp", ctypes.c_double),
-                ("mixed", ctypes.c_int)]
+                ("mixed", ctypes.c_int), ("smoother", ctypes.c_int), ("sor_omega", ctypes.c_double)]
 
 
 class OStats(ctypes.Structure):
@@ -151,6 +151,12 @@ class Oracle:
         lib().orc_vcycle(self._p, ctypes.byref(s), _p(v), _p(z))
         return z
 
+    def sor_sweep(self, b, x, omega, flag, level=-1):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.array(x, dtype=np.float64)
+        lib().orc_sor_sweep(self._p, level, _p(b), _p(out), ctypes.c_double(omega), int(flag))
+        return out
+
     def errors(self):
         e = np.zeros(5)
         lib().orc_errors(self._p, _p(e))
@@ -170,6 +176,17 @@ def quad_rule(shape, degree):
     x = np.zeros(64 * 3)
     n = lib().orc_quad(shape, degree, _p(w), _p(x))
     return w[:n], x[:3 * n].reshape(n, 3)
+
+
+def sor_csr(rp, col, val, b, x, omega, flag):
+    """TMatrix::sor_sweep on any CSR matrix; returns the new x."""
+    n = len(rp) - 1
+    out = np.array(x, dtype=np.float64)
+    lib().orc_sor_csr(n, _p(np.ascontiguousarray(rp, np.int32), ctypes.c_int),
+                      _p(np.ascontiguousarray(col, np.int32), ctypes.c_int),
+                      _p(np.ascontiguousarray(val, np.float64)), _p(np.ascontiguousarray(b, np.float64)),
+                      _p(out), ctypes.c_double(omega), int(flag))
+    return out
 
 
 def spmv_csr(n, n_active, rp, col, val, x, y, a=1.0):

@@ -1386,6 +1386,8 @@ typedef struct {
   int restart, n_pre, n_post;
   double jacobi_damp, mg_damp;
   int mixed;
+  int smoother; /* multigrid_smoother: 0 jacobi, 1 sor, 2 ssor */
+  double sor_omega;
 } osolver;
 
 typedef struct {
@@ -1424,6 +1426,42 @@ static void jacobi_smooth(const olevel* L, const double* b, double* x, double om
   free(d);
 }
 
+/* TMatrix::sor_sweep (Matrix.C:1308-1420) on a CSR matrix, in place, over
+   all rows in serial order: forward (flag 0), backward (1) or both (2). Each
+   row sums b - A x in column order, then x += omega * s / diag. */
+static void sor_csr(int n, const int* rp, const int* col, const double* val, const double* b, double* x,
+                    double omega, int flag) {
+  double* d = malloc(sizeof(double) * n);
+  for(int r = 0; r < n; ++r) {
+    d[r] = 0;
+    for(int k = rp[r]; k < rp[r + 1]; ++k)
+      if(col[k] == r) d[r] = val[k];
+  }
+  for(int pass = 0; pass < 2; ++pass) {
+    const int backward = pass == 1;
+    if(backward && flag == 0) break;
+    if(!backward && flag == 1) continue;
+    for(int t = 0; t < n; ++t) {
+      const int r = backward ? n - 1 - t : t;
+      double s = b[r];
+      for(int k = rp[r]; k < rp[r + 1]; ++k) s -= val[k] * x[col[k]];
+      x[r] = x[r] + omega * s / d[r];
+    }
+  }
+  free(d);
+}
+
+static void sor_level(const olevel* L, const double* b, double* x, double omega, int flag) {
+  sor_csr(L->ndof, L->rp, L->col, L->val, b, x, omega, flag);
+}
+
+/* one multigrid smoothing step: JacobiSmoother::smooth or SORSmoother::smooth
+   (SORSmoother.C:29-33: sweep flag 0 for "sor", 2 for "ssor") */
+static void smooth_step(const osolver* s, const olevel* L, const double* b, double* x) {
+  if(s->smoother == 0) jacobi_smooth(L, b, x, s->jacobi_damp);
+  else sor_level(L, b, x, s->sor_omega, s->smoother == 1 ? 0 : 2);
+}
+
 /* V-cycle (Multigrid::cycle with V control) */
 static void mg_cycle(orc_problem* P, const osolver* s, const double* v, double* z) {
   const int Lf = P->nlev - 1;
@@ -1434,7 +1472,7 @@ static void mg_cycle(orc_problem* P, const osolver* s, const double* v, double* 
   for(int l = Lf; l >= 1; --l) {
     olevel* L = &P->lev[l];
     olevel* C = &P->lev[l - 1];
-    for(int k = 0; k < s->n_pre; ++k) jacobi_smooth(L, L->mg_rhs, L->mg_sol, s->jacobi_damp);
+    for(int k = 0; k < s->n_pre; ++k) smooth_step(s, L, L->mg_rhs, L->mg_sol);
     memcpy(L->mg_def, L->mg_rhs, sizeof(double) * L->ndof);
     apply_scaled_add(L, L->mg_sol, L->mg_def, -1.0);
     memset(C->mg_sol, 0, sizeof(double) * C->ndof);
@@ -1445,14 +1483,14 @@ static void mg_cycle(orc_problem* P, const osolver* s, const double* v, double* 
     olevel* C = &P->lev[0];
     if(P->nlev > 1) dense_solve(C, C->mg_rhs, C->mg_sol);
     else
-      for(int k = 0; k < s->n_pre + s->n_post; ++k) jacobi_smooth(C, C->mg_rhs, C->mg_sol, s->jacobi_damp);
+      for(int k = 0; k < s->n_pre + s->n_post; ++k) smooth_step(s, C, C->mg_rhs, C->mg_sol);
   }
   for(int l = 1; l <= Lf; ++l) {
     olevel* L = &P->lev[l];
     olevel* C = &P->lev[l - 1];
     prolongate(P, C, L, C->mg_sol, L->mg_aux);
     for(int j = 0; j < L->nactive; ++j) L->mg_sol[j] += s->mg_damp * L->mg_aux[j];
-    for(int k = 0; k < s->n_post; ++k) jacobi_smooth(L, L->mg_rhs, L->mg_sol, s->jacobi_damp);
+    for(int k = 0; k < s->n_post; ++k) smooth_step(s, L, L->mg_rhs, L->mg_sol);
   }
   memcpy(z, F->mg_sol, sizeof(double) * F->ndof);
 }
@@ -1462,7 +1500,11 @@ static void precond(orc_problem* P, const osolver* s, const double* v, double* z
   if(s->precond == 0) memcpy(z, v, sizeof(double) * F->ndof);
   else if(s->precond == 1)
     for(int i = 0; i < F->ndof; ++i) z[i] = v[i] / F->diag[i];
-  else mg_cycle(P, s, v, z);
+  else if(s->precond == 3 || s->precond == 4) {
+    /* Iteration_sor::apply (Iteration_sor.C:125-140): sweep(s) from zero */
+    memset(z, 0, sizeof(double) * F->ndof);
+    sor_level(F, v, z, s->sor_omega, s->precond == 3 ? 0 : 2);
+  } else mg_cycle(P, s, v, z);
 }
 
 static void givens_gen(double dx, double dy, double* cs, double* sn) {
@@ -1598,6 +1640,35 @@ static int solve_cg(orc_problem* P, const osolver* s, ostats* st) {
   return cv.diverged ? 2 : 0;
 }
 
+/* Iteration_sor::iterate (Iteration_sor.C:27-120) */
+static int solve_sor(orc_problem* P, const osolver* s, ostats* st, int flag) {
+  olevel* F = &P->lev[P->nlev - 1];
+  const int n = F->ndof;
+  oconv cv = {s, 0, 0, 0};
+  double* x = F->sol;
+  double* r = malloc(sizeof(double) * n);
+  memcpy(r, F->rhs, sizeof(double) * n);
+  apply_scaled_add(F, x, r, -1.0);
+  double normb = sqrt(ddot(n, F->rhs, F->rhs));
+  if(normb == 0.0) { normb = 1.0; memset(r, 0, sizeof(double) * n); memset(x, 0, sizeof(double) * n); }
+  double resid = sqrt(ddot(n, r, r)) / normb;
+  st->initial_residual = resid;
+  if(!converged(&cv, resid, 0)) {
+    for(int i = 1; i <= s->max_iter; ++i) {
+      sor_level(F, F->rhs, x, s->sor_omega, flag);
+      memcpy(r, F->rhs, sizeof(double) * n);
+      apply_scaled_add(F, x, r, -1.0);
+      resid = sqrt(ddot(n, r, r)) / normb;
+      st->iterations = i;
+      if(converged(&cv, resid, i)) break;
+    }
+  }
+  st->final_residual = resid;
+  st->converged = resid < s->res_tol || resid < s->res_red * st->initial_residual;
+  free(r);
+  return cv.diverged ? 2 : 0;
+}
+
 /* ----------------------------------------------------------------------- */
 /* exported API                                                             */
 /* ----------------------------------------------------------------------- */
@@ -1703,6 +1774,7 @@ int orc_solve(orc_problem* P, const osolver* s, ostats* st) {
     return 0;
   }
   if(s->method == 1) return solve_cg(P, s, st);
+  if(s->method == 4 || s->method == 5) return solve_sor(P, s, st, s->method == 4 ? 0 : 2);
   return solve_fgmres(P, s, st);
 }
 
@@ -1807,6 +1879,15 @@ void orc_set_rhs(orc_problem* P, int level, const double* v) {
     memset(L->sol, 0, sizeof(double) * L->nactive);
     memcpy(L->sol + L->nactive, L->rhs + L->nactive, sizeof(double) * (L->ndof - L->nactive));
   }
+}
+
+/* TMatrix::sor_sweep on a level's matrix or on any CSR matrix (in place) */
+void orc_sor_sweep(orc_problem* P, int level, const double* b, double* x, double omega, int flag) {
+  sor_level(lvl(P, level), b, x, omega, flag);
+}
+void orc_sor_csr(int n, const int* rp, const int* col, const double* val, const double* b, double* x,
+                 double omega, int flag) {
+  sor_csr(n, rp, col, val, b, x, omega, flag);
 }
 
 /* residual history of the last FGMRES/CG solve (LoopInfo output) */

@@ -24,8 +24,9 @@ GEOMETRIES = {"UnitSquare": 0, "TwoTriangles": 1, "TwoTriangles2": 2,
               "StructuredHexa": 10, "StructuredTetra": 11}
 EXAMPLES = {"sine_laplace_2d": 0, "sine_2d": 1, "sine_3d": 2, "test_p0": 3,
             "test_p1": 4, "test_p2": 5, "const_rhs": 6}
-METHODS = {"direct": 0, "cg": 1, "fgmres": 2, "richardson": 3}
-PRECONDS = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2}
+METHODS = {"direct": 0, "cg": 1, "fgmres": 2, "richardson": 3, "sor": 4, "ssor": 5}
+PRECONDS = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2, "sor": 3, "ssor": 4}
+SMOOTHERS = {"jacobi": 0, "sor": 1, "ssor": 2}
 
 
 class ProblemDesc(ctypes.Structure):
@@ -44,7 +45,7 @@ class SolverDesc(ctypes.Structure):
                 ("div_factor", ctypes.c_double), ("restart", ctypes.c_int),
                 ("n_pre", ctypes.c_int), ("n_post", ctypes.c_int),
                 ("jacobi_damp", ctypes.c_double), ("mg_damp", ctypes.c_double),
-                ("mixed", ctypes.c_int)]
+                ("mixed", ctypes.c_int), ("smoother", ctypes.c_int), ("sor_omega", ctypes.c_double)]
 
 
 class SolveStats(ctypes.Structure):
@@ -56,11 +57,11 @@ class SolveStats(ctypes.Structure):
 EXPORTS = [
     "pmcd_last_error", "pmcd_device_count",
     "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
-    "pmcd_host_pattern", "pmcd_host_cell_coords",
+    "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_sweep_order",
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
     "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
     "pmcd_get_rhs", "pmcd_solve", "pmcd_get_solution", "pmcd_set_solution", "pmcd_errors",
-    "pmcd_apply", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_set_tuning", "pmcd_get_tuning",
+    "pmcd_apply", "pmcd_sor_sweep", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_set_tuning", "pmcd_get_tuning",
     "pmcd_probe_mfma",
     "pmcd_spmv_device", "pmcd_dot_device",
 ]
@@ -163,6 +164,8 @@ def solver_desc(db):
     s.jacobi_damp = float(db.get("jacobi_damp", 1.0))
     s.mg_damp = float(db.get("multigrid_correction_damp_factor", 1.0))
     s.mixed = int(db.get("multigrid_mixed_precision", 0))
+    s.smoother = SMOOTHERS[db.get("multigrid_smoother", "jacobi")]
+    s.sor_omega = float(db.get("sor_omega", 1.5))
     return s
 
 
@@ -197,6 +200,16 @@ class HostSetup:
         col = np.empty(i["nnz"], dtype=np.int32)
         _check(lib().pmcd_host_pattern(self._h, level, _ptr(rp, ctypes.c_int), _ptr(col, ctypes.c_int)))
         return rp, col
+
+    def sweep_order(self, level=-1, max_exact_classes=512):
+        """Device SOR row classes: (rows, class_ptr, exact)."""
+        n = self.info(level)["n_dof"]
+        rows = np.empty(n, dtype=np.int32)
+        ptr = np.empty(n + 1, dtype=np.int32)
+        k, ex = ctypes.c_int(0), ctypes.c_int(0)
+        _check(lib().pmcd_host_sweep_order(self._h, level, int(max_exact_classes), _ptr(rows, ctypes.c_int),
+                                           _ptr(ptr, ctypes.c_int), ctypes.byref(k), ctypes.byref(ex)))
+        return rows, ptr[:k.value + 1].copy(), bool(ex.value)
 
     def cell_coords(self, level=-1):
         i = self.info(level)
@@ -308,6 +321,16 @@ class ConvectionDiffusion:
         _check(lib().pmcd_apply(self._s, level, _ptr(x, ctypes.c_double),
                                 _ptr(out, ctypes.c_double), ctypes.c_double(a), mode))
         return out
+
+    def sor_sweep(self, b, x, omega=1.5, flag=2, level=-1):
+        """TMatrix::sor_sweep on the level's matrix: returns (new x, exact) where
+        exact tells whether the device order reproduces the serial sweep."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.array(x, dtype=np.float64)
+        ex = ctypes.c_int(0)
+        _check(lib().pmcd_sor_sweep(self._s, level, _ptr(b, ctypes.c_double), _ptr(out, ctypes.c_double),
+                                    ctypes.c_double(omega), int(flag), ctypes.byref(ex)))
+        return out, bool(ex.value)
 
     def time_kernel(self, name, level=-1, iters=20):
         """Average device time (ms) of one kernel from HIP events on the system stream."""

@@ -67,6 +67,9 @@ SolverDesc to_solver(const pmcd_solver_desc& s) {
   d.jacobi_damp = s.jacobi_damp;
   d.mg_damp = s.mg_damp;
   d.mixed = s.mixed;
+  d.smoother = s.smoother;
+  d.sor_omega = s.sor_omega;
+  PMCD_CHECK(d.smoother >= 0 && d.smoother <= 2, "unknown multigrid smoother ", d.smoother);
   return d;
 }
 }  // namespace
@@ -137,6 +140,19 @@ int pmcd_host_pattern(pmcd_host* h, int level, int* row_ptr, int* col) {
     Level& L = host_level(h, level);
     std::memcpy(row_ptr, L.st.row_ptr.data(), L.st.row_ptr.size() * sizeof(int));
     std::memcpy(col, L.st.col.data(), L.st.col.size() * sizeof(int));
+  });
+}
+
+int pmcd_host_sweep_order(pmcd_host* h, int level, int max_exact_classes, int* rows, int* class_ptr,
+                          int* n_classes, int* exact) {
+  return guarded([&] {
+    PMCD_CHECK(rows && n_classes && exact, "null argument");
+    Level& L = host_level(h, level);
+    SweepOrder o = build_sweep_order(L.st, max_exact_classes);
+    std::memcpy(rows, o.rows.data(), o.rows.size() * sizeof(int));
+    *n_classes = (int)o.class_ptr.size() - 1;
+    if(class_ptr) std::memcpy(class_ptr, o.class_ptr.data(), o.class_ptr.size() * sizeof(int));
+    *exact = o.exact ? 1 : 0;
   });
 }
 
@@ -286,6 +302,21 @@ int pmcd_apply(pmcd_system* s, int level, const double* x, double* y, double a, 
     else s->sys->apply(L, dx.p, dy.p);
     dy.download(y, L.n(), s->stream);
     PMCD_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
+int pmcd_sor_sweep(pmcd_system* s, int level, const double* b, double* x, double omega, int flag,
+                   int* exact) {
+  return guarded([&] {
+    PMCD_CHECK(b && x, "null argument");
+    Level& L = level_of(s, level);
+    DevBuf<double> db, dx;
+    db.upload(b, L.n(), s->stream);
+    dx.upload(x, L.n(), s->stream);
+    s->sys->sor_sweep(L, db.p, dx.p, omega, flag);
+    dx.download(x, L.n(), s->stream);
+    PMCD_HIP(hipStreamSynchronize(s->stream));
+    if(exact) *exact = L.sweep_exact ? 1 : 0;
   });
 }
 

@@ -297,6 +297,28 @@ __global__ __launch_bounds__(256) void k_prolong_add(int n_active, const int* __
   if(l == 0) y[row] += a * s;
 }
 
+// Rows of one class of a successive over-relaxation sweep
+// (TMatrix::sor_sweep, src/Matrix/Matrix.C:1308-1420): one thread per row, the
+// row's sum taken in column order starting from b, then
+// x[row] += omega * s / diag[row], in place. Rows of one class are never
+// adjacent, so each reads only x entries that no other thread of the launch
+// writes.
+template <class T>
+__global__ __launch_bounds__(256) void k_sor_rows(int count, const int* __restrict__ rows,
+                                                  const int* __restrict__ rp,
+                                                  const int* __restrict__ col,
+                                                  const T* __restrict__ val,
+                                                  const T* __restrict__ diag,
+                                                  const T* __restrict__ b, T* x, T omega) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if(i >= count) return;
+  const int r = rows[i];
+  T s = b[r];
+  const int e = rp[r + 1];
+  for(int k = rp[r]; k < e; ++k) s -= val[k] * x[col[k]];
+  x[r] = x[r] + omega * s / diag[r];
+}
+
 __device__ inline double block_sum(double v) {
   __shared__ double s_w[4];
   for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -815,6 +837,19 @@ void launch_prolong_add_f32(int n, int n_active, long nnz, int max_row, const in
                             const int* col, const float* val, const float* x, float* y, float a,
                             hipStream_t s) {
   dispatch_transfer<float, true>(n, n_active, nnz, max_row, rp, col, val, x, y, a, s);
+}
+
+void launch_sor_rows(int count, const int* rows, const int* rp, const int* col, const double* val,
+                     const double* diag, const double* b, double* x, double omega, hipStream_t s) {
+  if(count <= 0) return;
+  k_sor_rows<double><<<(count + 255) / 256, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega);
+  PMCD_HIP(hipGetLastError());
+}
+void launch_sor_rows_f32(int count, const int* rows, const int* rp, const int* col, const float* val,
+                         const float* diag, const float* b, float* x, float omega, hipStream_t s) {
+  if(count <= 0) return;
+  k_sor_rows<float><<<(count + 255) / 256, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega);
+  PMCD_HIP(hipGetLastError());
 }
 
 void launch_csr(int n, const int* rp, const int* col, const double* val, const double* x,

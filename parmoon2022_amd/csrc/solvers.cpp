@@ -72,6 +72,9 @@ void CDSystem::prepare(const SolverDesc& s) {
     if(s.method == 0 && n_levels() > 1) PMCD_FAIL("the direct solver expects a single level");
     if(!coarse_ready_) coarse_factorize();
   }
+  if(s.precond == 2 && s.smoother != 0)
+    for(auto& Lp : levels_) prepare_sweep(*Lp);
+  if(s.precond == 3 || s.precond == 4 || s.method == 4 || s.method == 5) prepare_sweep(finest());
   if(s.mixed && !mixed_ready_) {
     for(auto& Lp : levels_) {
       Level& L = *Lp;
@@ -93,6 +96,8 @@ SolveStats CDSystem::solve(const SolverDesc& s) {
     case 1: st = solve_cg(); break;
     case 2: st = solve_fgmres(); break;
     case 3: st = solve_richardson(); break;
+    case 4: st = solve_sor(0); break;
+    case 5: st = solve_sor(2); break;
     default: PMCD_FAIL("unknown solver method ", s.method);
   }
   PMCD_HIP(hipStreamSynchronize(stream_));
@@ -130,6 +135,40 @@ SolveStats CDSystem::solve_richardson() {
       launch_copy(n, F.rhs.p, kr_.p, stream_);
       apply_scaled_add(F, F.sol.p, kr_.p, -1.0);
       res = std::sqrt(dot(n, kr_.p, kr_.p));
+      if(conv(res, it)) break;
+    }
+  }
+  st.iterations = std::min(it, sd_.max_iter);
+  st.final_residual = res;
+  st.converged = res < sd_.res_tol || res < sd_.res_red * st.initial_residual;
+  return st;
+}
+
+// Iteration_sor::iterate (Iteration_sor.C:27-120): relative residual
+// |b - A x| / |b|, one (symmetric) sweep per iteration on the finest level.
+SolveStats CDSystem::solve_sor(int flag) {
+  Level& F = finest();
+  const int n = F.n();
+  Convergence conv(sd_);
+  if(kr_.n != (size_t)n) { kr_.alloc(n); kw_.alloc(n); }
+  SolveStats st;
+  launch_copy(n, F.rhs.p, kr_.p, stream_);
+  apply_scaled_add(F, F.sol.p, kr_.p, -1.0);
+  double normb = std::sqrt(dot(n, F.rhs.p, F.rhs.p));
+  if(normb == 0.0) {
+    normb = 1.0;
+    launch_fill(n, 0.0, kr_.p, stream_);
+    launch_fill(n, 0.0, F.sol.p, stream_);
+  }
+  double res = std::sqrt(dot(n, kr_.p, kr_.p)) / normb;
+  st.initial_residual = res;
+  int it = 0;
+  if(!conv(res, 0)) {
+    for(it = 1; it <= sd_.max_iter; ++it) {
+      sor_sweep(F, F.rhs.p, F.sol.p, sd_.sor_omega, flag);
+      launch_copy(n, F.rhs.p, kr_.p, stream_);
+      apply_scaled_add(F, F.sol.p, kr_.p, -1.0);
+      res = std::sqrt(dot(n, kr_.p, kr_.p)) / normb;
       if(conv(res, it)) break;
     }
   }

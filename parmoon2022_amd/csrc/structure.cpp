@@ -76,4 +76,41 @@ Structure build_structure(const FESpace& sp, int n_cells) {
   return st;
 }
 
+SweepOrder build_sweep_order(const Structure& st, int max_exact_classes) {
+  const int n = st.n_rows;
+  std::vector<int> cls(n, 0);
+  int n_cls = n > 0 ? 1 : 0;
+  for(int r = 0; r < n; ++r) {
+    int c = 0;
+    for(int k = st.row_ptr[r]; k < st.row_ptr[r + 1]; ++k)
+      if(st.col[k] < r) c = std::max(c, cls[st.col[k]] + 1);
+    cls[r] = c;
+    n_cls = std::max(n_cls, c + 1);
+  }
+  SweepOrder o;
+  o.exact = n_cls <= max_exact_classes;
+  if(!o.exact) {
+    // greedy colouring in row order: the smallest colour no coloured
+    // neighbour holds
+    std::vector<int> stamp(1, -1);
+    n_cls = 0;
+    for(int r = 0; r < n; ++r) {
+      for(int k = st.row_ptr[r]; k < st.row_ptr[r + 1]; ++k)
+        if(st.col[k] < r) stamp[cls[st.col[k]]] = r;
+      int c = 0;
+      while(c < (int)stamp.size() && stamp[c] == r) ++c;
+      if(c == (int)stamp.size()) stamp.push_back(-1);
+      cls[r] = c;
+      n_cls = std::max(n_cls, c + 1);
+    }
+  }
+  o.class_ptr.assign(n_cls + 1, 0);
+  for(int r = 0; r < n; ++r) o.class_ptr[cls[r] + 1]++;
+  for(int k = 0; k < n_cls; ++k) o.class_ptr[k + 1] += o.class_ptr[k];
+  o.rows.resize(n);
+  std::vector<int> fill(o.class_ptr.begin(), o.class_ptr.end() - 1);
+  for(int r = 0; r < n; ++r) o.rows[fill[cls[r]]++] = r;
+  return o;
+}
+
 }  // namespace pmcd

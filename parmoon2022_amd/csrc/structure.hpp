@@ -35,4 +35,24 @@ struct Structure {
 
 Structure build_structure(const FESpace& sp, int n_cells);
 
+// Row classes for a successive over-relaxation sweep on the device. A forward
+// sweep visits the classes in order, a backward sweep in reverse; rows of one
+// class are never adjacent (the pattern is structurally symmetric), so a
+// class updates in parallel.
+//
+// exact: classes are the levels of the row-order dependency graph
+// (class(r) = 1 + max class of the adjacent rows c < r). Every row then reads
+// new values for its neighbours c < r and old ones for c > r, the values the
+// serial sweep of the reference uses (src/Matrix/Matrix.C:1370-1395), so the
+// result is bit-identical. Otherwise (more than max_exact_classes levels,
+// e.g. Q2 in 3D) greedy colouring in row order gives a multicolour SOR: the
+// same per-row arithmetic in a different row order.
+struct SweepOrder {
+  std::vector<int> rows;       // rows grouped by class, ascending inside a class
+  std::vector<int> class_ptr;  // class k = rows[class_ptr[k] .. class_ptr[k+1])
+  bool exact = false;
+};
+
+SweepOrder build_sweep_order(const Structure& st, int max_exact_classes);
+
 }  // namespace pmcd

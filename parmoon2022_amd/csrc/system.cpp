@@ -429,6 +429,11 @@ double CDSystem::dot(int n, const double* x, const double* y) {
 // and b vanishes on the non-active rows, so the first sweep needs no SpMV.
 int CDSystem::smooth(Level& L, double* const x[2], int cur, const double* b, int steps,
                      bool from_zero) {
+  if(sd_.smoother != 0) {
+    // SORSmoother::smooth (SORSmoother.C:29-33): one sweep (pair) per step, in place
+    for(int s = 0; s < steps; ++s) sor_sweep(L, b, x[cur], sd_.sor_omega, sd_.smoother == 1 ? 0 : 2);
+    return cur;
+  }
   for(int s = 0; s < steps; ++s) {
     if(s == 0 && from_zero)
       launch_jacobi_from_zero(L.n(), L.n_active(), L.diag.p, b, x[1 - cur], sd_.jacobi_damp, stream_);
@@ -442,6 +447,11 @@ int CDSystem::smooth(Level& L, double* const x[2], int cur, const double* b, int
 
 int CDSystem::smooth32(Level& L, float* const x[2], int cur, const float* b, int steps,
                        bool from_zero) {
+  if(sd_.smoother != 0) {
+    for(int s = 0; s < steps; ++s)
+      sor_sweep32(L, b, x[cur], (float)sd_.sor_omega, sd_.smoother == 1 ? 0 : 2);
+    return cur;
+  }
   for(int s = 0; s < steps; ++s) {
     if(s == 0 && from_zero)
       launch_jacobi_from_zero_f32(L.n(), L.n_active(), L.diag32.p, b, x[1 - cur],
@@ -471,8 +481,10 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
   }
   b[Lf] = rhs;
   x[Lf] = {out, levels_[Lf]->mg_sol2.p};
-  // the first sweep may skip its SpMV only if it starts from the zero iterate
-  const bool fine_zero = rhs_nonactive_zero_ && sd_.n_pre > 0;
+  // the first Jacobi sweep may skip its SpMV only if it starts from the zero
+  // iterate; SOR updates in place and needs the iterate initialised
+  const bool jacobi = sd_.smoother == 0;
+  const bool fine_zero = jacobi && rhs_nonactive_zero_ && sd_.n_pre > 0;
   if(!fine_zero) {
     Level& F = finest();
     launch_fill(F.n_active(), 0.0, x[Lf][0], stream_);
@@ -481,11 +493,11 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
   for(int l = Lf; l >= 1; --l) {
     Level& L = *levels_[l];
     Level& C = *levels_[l - 1];
-    const bool zero = (l == Lf) ? fine_zero : sd_.n_pre > 0;
+    const bool zero = (l == Lf) ? fine_zero : (jacobi && sd_.n_pre > 0);
     cur[l] = smooth(L, x[l].data(), cur[l], b[l], sd_.n_pre, zero);
     launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x[l][cur[l]], b[l], L.mg_def.p,
                 -1.0, 1.0, stream_, L.st.max_row);
-    if(sd_.n_pre == 0 || l - 1 == 0) launch_fill(C.n(), 0.0, x[l - 1][0], stream_);
+    if(!jacobi || sd_.n_pre == 0 || l - 1 == 0) launch_fill(C.n(), 0.0, x[l - 1][0], stream_);
     launch_restrict(C.n(), C.n_active(), (long)L.r_val.n, L.r_max_row, L.r_rp.p, L.r_col.p,
                     L.r_val.p, L.mg_def.p, C.mg_rhs.p, stream_);
   }
@@ -521,7 +533,8 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
   for(int l = 0; l <= Lf; ++l) x[l] = {levels_[l]->f_sol.p, levels_[l]->f_sol2.p};
   Level& F = finest();
   launch_d2f(F.n(), rhs, F.f_rhs.p, stream_);
-  const bool fine_zero = rhs_nonactive_zero_ && sd_.n_pre > 0;
+  const bool jacobi = sd_.smoother == 0;
+  const bool fine_zero = jacobi && rhs_nonactive_zero_ && sd_.n_pre > 0;
   if(!fine_zero) {
     launch_fill_f32(F.n_active(), 0.0f, F.f_sol.p, stream_);
     PMCD_HIP(hipMemcpyAsync(F.f_sol.p + F.n_active(), F.f_rhs.p + F.n_active(),
@@ -530,11 +543,11 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
   for(int l = Lf; l >= 1; --l) {
     Level& L = *levels_[l];
     Level& C = *levels_[l - 1];
-    const bool zero = (l == Lf) ? fine_zero : sd_.n_pre > 0;
+    const bool zero = (l == Lf) ? fine_zero : (jacobi && sd_.n_pre > 0);
     cur[l] = smooth32(L, x[l].data(), cur[l], L.f_rhs.p, sd_.n_pre, zero);
     launch_spmv_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, x[l][cur[l]], L.f_rhs.p,
                     L.f_def.p, -1.0f, 1.0f, stream_, L.st.max_row);
-    if(sd_.n_pre == 0 || l - 1 == 0) launch_fill_f32(C.n(), 0.0f, x[l - 1][0], stream_);
+    if(!jacobi || sd_.n_pre == 0 || l - 1 == 0) launch_fill_f32(C.n(), 0.0f, x[l - 1][0], stream_);
     launch_restrict_f32(C.n(), C.n_active(), (long)L.r_val32.n, L.r_max_row, L.r_rp.p, L.r_col.p,
                         L.r_val32.p, L.f_def.p, C.f_rhs.p, stream_);
   }
@@ -579,8 +592,57 @@ void CDSystem::precond(const double* v, double* z) {
   switch(sd_.precond) {
     case 0: launch_copy(F.n(), v, z, stream_); break;
     case 1: launch_div(F.n(), v, F.diag.p, z, stream_); break;
-    default: vcycle(v, z); break;
+    case 2: vcycle(v, z); break;
+    case 3:
+    case 4:
+      // Iteration_sor::apply (Iteration_sor.C:125-140): sweep(s) from zero
+      launch_fill(F.n(), 0.0, z, stream_);
+      sor_sweep(F, v, z, sd_.sor_omega, sd_.precond == 3 ? 0 : 2);
+      break;
+    default: PMCD_FAIL("unknown preconditioner ", sd_.precond);
   }
+}
+
+void CDSystem::prepare_sweep(Level& L) {
+  if(!L.sweep_ptr.empty()) return;
+  // the level schedule is exact; beyond this many classes (deep dependency
+  // chains, e.g. Q2 in 3D) launches per class would dominate and the sweep
+  // falls back to greedy multicolour order
+  constexpr int kMaxExactClasses = 512;
+  SweepOrder o = build_sweep_order(L.st, kMaxExactClasses);
+  L.sweep_ptr = std::move(o.class_ptr);
+  L.sweep_exact = o.exact;
+  L.sweep_rows.upload(o.rows, stream_);
+  PMCD_HIP(hipStreamSynchronize(stream_));
+}
+
+void CDSystem::sor_sweep(Level& L, const double* b, double* x, double omega, int flag) {
+  PMCD_CHECK(flag >= 0 && flag <= 2, "sor_sweep flag must be 0, 1 or 2");
+  prepare_sweep(L);
+  const int K = (int)L.sweep_ptr.size() - 1;
+  auto cls = [&](int k) {
+    launch_sor_rows(L.sweep_ptr[k + 1] - L.sweep_ptr[k], L.sweep_rows.p + L.sweep_ptr[k], L.row_ptr.p,
+                    L.col.p, L.val.p, L.diag.p, b, x, omega, stream_);
+  };
+  if(flag == 0 || flag == 2)
+    for(int k = 0; k < K; ++k) cls(k);
+  if(flag == 1 || flag == 2)
+    for(int k = K - 1; k >= 0; --k) cls(k);
+}
+
+void CDSystem::sor_sweep32(Level& L, const float* b, float* x, float omega, int flag) {
+  PMCD_CHECK(flag >= 0 && flag <= 2, "sor_sweep flag must be 0, 1 or 2");
+  PMCD_CHECK(L.val32.n == L.val.n, "FP32 matrix copies not prepared");
+  prepare_sweep(L);
+  const int K = (int)L.sweep_ptr.size() - 1;
+  auto cls = [&](int k) {
+    launch_sor_rows_f32(L.sweep_ptr[k + 1] - L.sweep_ptr[k], L.sweep_rows.p + L.sweep_ptr[k],
+                        L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x, omega, stream_);
+  };
+  if(flag == 0 || flag == 2)
+    for(int k = 0; k < K; ++k) cls(k);
+  if(flag == 1 || flag == 2)
+    for(int k = K - 1; k >= 0; --k) cls(k);
 }
 
 }  // namespace pmcd

@@ -33,8 +33,8 @@ struct ProblemDesc {
 };
 
 struct SolverDesc {
-  int method;          // 0 direct, 1 cg, 2 fgmres, 3 richardson
-  int precond;         // 0 none, 1 jacobi, 2 multigrid
+  int method;          // 0 direct, 1 cg, 2 fgmres, 3 richardson, 4 sor, 5 ssor
+  int precond;         // 0 none, 1 jacobi, 2 multigrid, 3 sor, 4 ssor
   int max_iter;
   int min_iter;
   double res_tol;      // residual_tolerance
@@ -45,6 +45,8 @@ struct SolverDesc {
   double jacobi_damp;  // jacobi_damp
   double mg_damp;      // multigrid_correction_damp_factor
   int mixed;           // 1: V-cycle in FP32
+  int smoother;        // multigrid smoother: 0 jacobi, 1 sor (forward), 2 ssor
+  double sor_omega;    // sor_omega
 };
 
 struct SolveStats {
@@ -77,6 +79,11 @@ struct Level {
   DevBuf<float> p_val32, r_val32;
   int n_coarse = 0;
   int p_max_row = -1, r_max_row = -1;  // longest rows of P and R
+  // SOR sweep classes (built on first use): class k = sweep_rows[sweep_ptr[k] ..
+  // sweep_ptr[k+1]); exact = level schedule (bit-identical to the serial sweep)
+  std::vector<int> sweep_ptr;
+  DevBuf<int> sweep_rows;
+  bool sweep_exact = false;
   int n() const { return space.n_dof; }
   int n_active() const { return space.n_active; }
 };
@@ -109,6 +116,10 @@ class CDSystem {
   void set_solver(const SolverDesc& s) { sd_ = s; }
   void prepare(const SolverDesc& s);      // coarse factorization, FP32 copies
   void time_kernel(int which, int level, int iters, double* avg_ms);
+  // TMatrix::sor_sweep on level L, in place: flag 0 forward, 1 backward, 2 both
+  void sor_sweep(Level& L, const double* b, double* x, double omega, int flag);
+  void sor_sweep32(Level& L, const float* b, float* x, float omega, int flag);
+  void prepare_sweep(Level& L);
 
  private:
   void build_level_device(Level& L);
@@ -125,6 +136,7 @@ class CDSystem {
   SolveStats solve_fgmres();
   SolveStats solve_direct();
   SolveStats solve_richardson();
+  SolveStats solve_sor(int flag);
 
   ProblemDesc desc_;
   SolverDesc sd_{};

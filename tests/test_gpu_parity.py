@@ -321,3 +321,140 @@ def test_cg_jacobi_c1_shape_matches_oracle(oracle_lib):
     assert rel < 1e-10
     assert abs(eg[0] - eo[0]) <= 1e-3 * eo[0]
     cd.close()
+
+
+def _class_sweep(rp, col, val, b, x, omega, rows, ptr, backward=False):
+    """Serial SOR visiting the rows class by class (python loop)."""
+    x = np.array(x, dtype=np.float64)
+    diag = np.array([val[rp[r]:rp[r + 1]][col[rp[r]:rp[r + 1]] == r][0] for r in range(len(rp) - 1)])
+    ks = range(len(ptr) - 2, -1, -1) if backward else range(len(ptr) - 1)
+    for k in ks:
+        for r in rows[ptr[k]:ptr[k + 1]]:
+            s = b[r]
+            for j in range(rp[r], rp[r + 1]):
+                s -= val[j] * x[col[j]]
+            x[r] = x[r] + omega * s / diag[r]
+    return x
+
+
+SOR_2D = {"geo_file": "TwoTriangles", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 4, "example": "sine_2d",
+          "diffusion_coefficient": 0.01, "convection": (1.0, 0.5, 0.0), "space_discretization_type": "supg"}
+
+
+@pytest.mark.parametrize("flag", [0, 1, 2])
+def test_sor_sweep_exact_matches_oracle(oracle_lib, flag):
+    """Level-scheduled device sweep = the oracle's serial TMatrix::sor_sweep,
+    bit for bit (2D P1, 18 classes)."""
+    cd = pm.ConvectionDiffusion(SOR_2D)
+    cd.assemble()
+    o = oracle_lib.Oracle(pm.problem_desc(SOR_2D))
+    o.assemble()
+    n = cd.info()["n_dof"]
+    rng = np.random.default_rng(10 + flag)
+    b, x0 = rng.standard_normal(n), rng.standard_normal(n)
+    got, exact = cd.sor_sweep(b, x0, 1.5, flag)
+    assert exact
+    assert np.array_equal(got, o.sor_sweep(b, x0, 1.5, flag))
+    cd.close()
+
+
+@pytest.mark.parametrize("flag", [0, 2])
+def test_sor_sweep_multicolour_matches_class_order(flag):
+    """3D Q2: the device visits greedy colour classes; the result equals a
+    serial sweep over the same class order, bit for bit."""
+    db = make_db(*ASSEMBLY_CASES[5])
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    h = pm.HostSetup(db)
+    rows, ptr, exact = h.sweep_order()
+    assert not exact
+    rp, col = cd.pattern()
+    val = cd.matrix_values()
+    n = len(rp) - 1
+    rng = np.random.default_rng(20 + flag)
+    b, x0 = rng.standard_normal(n), rng.standard_normal(n)
+    got, ex = cd.sor_sweep(b, x0, 1.2, flag)
+    assert not ex
+    ref = _class_sweep(rp, col, val, b, x0, 1.2, rows, ptr)
+    if flag == 2:
+        ref = _class_sweep(rp, col, val, b, ref, 1.2, rows, ptr, backward=True)
+    assert np.array_equal(got, ref)
+    cd.close()
+
+
+@pytest.mark.parametrize("key", list(GOLDEN2D))
+def test_cd2d_golden_fgmres_ssor_on_gpu(key):
+    """cd2d_test.c++:243-251 on the device: FGMRES + SSOR preconditioner."""
+    geo, order = key
+    db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 2,
+          "example": "sine_laplace_2d", "solver_type": "iterative", "iterative_solver_type": "fgmres",
+          "preconditioner": "ssor", "max_n_iterations": 1000, "residual_tolerance": 1e-13}
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    st = cd.solve()
+    e = cd.output()
+    assert st.converged
+    assert np.allclose([e[0], e[1], e[2], e[4]], GOLDEN2D[key], rtol=0, atol=1e-11)
+    cd.close()
+
+
+def test_ssor_iteration_matches_oracle(oracle_lib):
+    """iterative_solver_type ssor (Iteration_sor::iterate) in exact class
+    order: same iteration count as the oracle, same solution to rounding."""
+    db = dict(SOR_2D, solver_type="iterative", iterative_solver_type="ssor", sor_omega=1.2,
+              residual_tolerance=1e-10, max_n_iterations=2000)
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    st = cd.solve()
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    so = o.solve(pm.solver_desc(db))
+    rel = np.linalg.norm(cd.solution() - o.solution()) / np.linalg.norm(o.solution())
+    print("ssor iterations gpu %d oracle %d rel %.3e" % (st.iterations, so.iterations, rel), flush=True)
+    assert st.converged and abs(st.iterations - so.iterations) <= 1 and rel < 1e-10
+    cd.close()
+
+
+def test_vcycle_ssor_smoother_matches_oracle(oracle_lib):
+    """Multigrid with the SSOR smoother on 2D Q2 (exact class order on every
+    level): the V-cycle matches the oracle's to rounding."""
+    db = {"geo_file": "UnitSquare", "ANSATZ_ORDER": 2, "refinement_n_initial_steps": 4, "example": "sine_2d",
+          "diffusion_coefficient": 0.01, "convection": (0.6, 0.8, 0.0), "space_discretization_type": "supg",
+          "solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "multigrid",
+          "multigrid_n_levels": 3, "multigrid_smoother": "ssor", "sor_omega": 1.1,
+          "multigrid_n_pre_smooth": 2, "multigrid_n_post_smooth": 2}
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    i = cd.info()
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal(i["n_dof"])
+    v[i["n_active"]:] = 0
+    zg = cd.vcycle(v)
+    zo = o.vcycle(pm.solver_desc(db), v)
+    rel = np.linalg.norm(zg - zo) / np.linalg.norm(zo)
+    print("SSOR V-cycle rel diff %.3e" % rel, flush=True)
+    assert rel < 1e-12
+    cd.close()
+
+
+@pytest.mark.parametrize("smoother,mixed", [("sor", 0), ("ssor", 0), ("ssor", 1)])
+def test_c2_shape_sor_smoother_solution(oracle_lib, smoother, mixed):
+    """C2 shape (8^3 Q2 SUPG) with an SOR/SSOR multigrid smoother: the device
+    sweeps in multicolour order, the oracle serially, so iterates differ;
+    both solve the same system, to 1e-8 relative at a 1e-13 residual
+    reduction (the mixed V-cycle too)."""
+    db = c2_db(3, 3, mixed=mixed)
+    db.update({"multigrid_smoother": smoother, "sor_omega": 1.0, "residual_reduction": 1e-13})
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    st = cd.solve()
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    so = o.solve(pm.solver_desc(db))
+    rel = np.linalg.norm(cd.solution() - o.solution()) / np.linalg.norm(o.solution())
+    print("%s mixed %d: iterations gpu %d oracle %d rel %.3e" % (smoother, mixed, st.iterations,
+                                                                 so.iterations, rel), flush=True)
+    assert st.converged and so.converged and rel < 1e-8
+    cd.close()

@@ -77,3 +77,66 @@ def test_dirichlet_dofs_last_and_sorted_columns():
 def test_invalid_input_raises():
     with pytest.raises(RuntimeError):
         pm.HostSetup({"geo_file": "UnitSquare", "ANSATZ_ORDER": 3, "example": "sine_2d"})
+
+
+def _class_sweep(rp, col, val, b, x, omega, rows, ptr, backward=False):
+    """Serial SOR visiting rows class by class in the given order (python
+    loop; the order inside a class is irrelevant since its rows are not
+    adjacent)."""
+    x = np.array(x, dtype=np.float64)
+    diag = np.array([val[rp[r]:rp[r + 1]][col[rp[r]:rp[r + 1]] == r][0] for r in range(len(rp) - 1)])
+    ks = range(len(ptr) - 2, -1, -1) if backward else range(len(ptr) - 1)
+    for k in ks:
+        for r in rows[ptr[k]:ptr[k + 1]]:
+            s = b[r]
+            for j in range(rp[r], rp[r + 1]):
+                s -= val[j] * x[col[j]]
+            x[r] = x[r] + omega * s / diag[r]
+    return x
+
+
+@pytest.mark.parametrize("geo,order,nref,ex", [("TwoTriangles", 1, 4, "sine_2d"), ("UnitSquare", 2, 3, "sine_2d"),
+                                               ("Default_UnitCube_Hexa", 2, 2, "sine_3d"),
+                                               ("Default_UnitCube_Tetra", 1, 2, "sine_3d")])
+def test_sweep_classes_are_independent_sets(geo, order, nref, ex):
+    """SOR row classes: every row once, no two adjacent rows in one class; in
+    exact mode every neighbour c > r of row r lies in a later class."""
+    h = pm.HostSetup({"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": nref, "example": ex})
+    rp, col = h.pattern()
+    n = len(rp) - 1
+    for max_exact in (100000, 1):
+        rows, ptr, exact = h.sweep_order(max_exact_classes=max_exact)
+        assert exact == (max_exact == 100000)
+        assert np.array_equal(np.sort(rows), np.arange(n))
+        cls = np.empty(n, dtype=np.int64)
+        for k in range(len(ptr) - 1):
+            cls[rows[ptr[k]:ptr[k + 1]]] = k
+        r_of = np.repeat(np.arange(n), np.diff(rp))
+        off = col != r_of
+        assert np.all(cls[r_of[off]] != cls[col[off]])
+        if exact:
+            up = col > r_of
+            assert np.all(cls[col[up]] > cls[r_of[up]])
+
+
+@pytest.mark.parametrize("flag", [0, 1, 2])
+def test_exact_class_sweep_equals_serial_sweep(oracle_lib, flag):
+    """Visiting the level-schedule classes reproduces the oracle's serial
+    TMatrix::sor_sweep bit for bit (forward, backward, symmetric)."""
+    db = {"geo_file": "TwoTriangles", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 4, "example": "sine_2d",
+          "diffusion_coefficient": 0.01, "convection": (1.0, 0.5, 0.0), "space_discretization_type": "supg"}
+    h = pm.HostSetup(db)
+    rows, ptr, exact = h.sweep_order()
+    assert exact and len(ptr) - 1 < 64
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    rp, col = o.pattern()
+    val = o.matrix_values()
+    rng = np.random.default_rng(flag)
+    b, x0 = rng.standard_normal(len(rp) - 1), rng.standard_normal(len(rp) - 1)
+    x = x0
+    if flag in (0, 2):
+        x = _class_sweep(rp, col, val, b, x, 1.5, rows, ptr)
+    if flag in (1, 2):
+        x = _class_sweep(rp, col, val, b, x, 1.5, rows, ptr, backward=True)
+    assert np.array_equal(x, o.sor_sweep(b, x0, 1.5, flag))

@@ -164,3 +164,59 @@ def test_c2_operator_jacobi_damping_bound(oracle_lib):
     ev = sla.eigs(DA, k=6, which="LM", return_eigenvectors=False, tol=1e-8)
     assert np.max(np.abs(1 - 0.8 * ev)) > 1.2
     assert np.max(np.abs(1 - 0.5 * ev)) < 1.0
+
+
+def test_oracle_sor_sweep_known_answer(oracle_lib):
+    """test/matrix_test.c++:74-84,345-371: the 3x3 matrix D = [[0.5,0,0],
+    [0,1,0.1],[0,0.1,1.5]] with rhs (-1,1,3) has the solution (-2, 120/149,
+    290/149); forward, backward and symmetric sweeps (omega 0.5) leave it
+    unchanged, and 10 SSOR sweeps (omega 1.1) from zero reach it to 1e-14."""
+    rp = np.array([0, 1, 3, 5])
+    col = np.array([0, 1, 2, 1, 2])
+    val = np.array([0.5, 1, 0.1, 0.1, 1.5])
+    rhs = np.array([-1.0, 1.0, 3.0])
+    exact = np.array([-2.0, 120 / 149, 290 / 149])
+    for flag in (0, 1, 2):
+        assert np.sum(np.abs(oracle_lib.sor_csr(rp, col, val, rhs, exact, 0.5, flag) - exact)) <= 1e-14
+    x = np.zeros(3)
+    for _ in range(10):
+        x = oracle_lib.sor_csr(rp, col, val, rhs, x, 1.1, 2)
+    assert np.sum(np.abs(x - exact)) <= 1e-14
+
+
+def test_oracle_sor_sweep_is_gauss_seidel(oracle_lib):
+    """A forward sweep with omega 1 is the Gauss-Seidel step x_new =
+    (D + L)^-1 (b - U x_old); a backward one uses (D + U)^-1 (b - L x_old)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(5)
+    n = 40
+    A = (sp.random(n, n, density=0.15, random_state=7, format="csr") + sp.eye(n) * 4).tocsr()
+    A.sort_indices()
+    b, x0 = rng.standard_normal(n), rng.standard_normal(n)
+    Ad = A.toarray()
+    L, U, D = np.tril(Ad, -1), np.triu(Ad, 1), np.diag(np.diag(Ad))
+    fw = np.linalg.solve(D + L, b - U @ x0)
+    bw = np.linalg.solve(D + U, b - L @ x0)
+    assert np.allclose(oracle_lib.sor_csr(A.indptr, A.indices, A.data, b, x0, 1.0, 0), fw, atol=1e-12)
+    assert np.allclose(oracle_lib.sor_csr(A.indptr, A.indices, A.data, b, x0, 1.0, 1), bw, atol=1e-12)
+    # relaxation: omega times the Gauss-Seidel correction, row by row
+    w = 1.3
+    M = D / w + L
+    fw_w = x0 + np.linalg.solve(M, b - Ad @ x0)
+    assert np.allclose(oracle_lib.sor_csr(A.indptr, A.indices, A.data, b, x0, w, 0), fw_w, atol=1e-12)
+
+
+@pytest.mark.parametrize("geo,order", list(CD2D_GOLDEN))
+def test_oracle_cd2d_golden_fgmres_ssor(oracle_lib, geo, order):
+    """cd2d_test.c++:243-251: FGMRES preconditioned by SSOR (sor_omega 1.5,
+    the default) reproduces the golden errors (tolerance 1e-11)."""
+    db = cd2d_db(geo, order)
+    db.update({"solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "ssor",
+               "max_n_iterations": 1000, "residual_tolerance": 1e-13})
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    st = o.solve(pm.solver_desc(db))
+    e = o.errors()
+    g = CD2D_GOLDEN[(geo, order)]
+    assert st.converged
+    assert np.allclose([e[0], e[1], e[2], e[4]], g, rtol=0, atol=1e-11)

@@ -151,6 +151,12 @@ class Oracle:
         lib().orc_vcycle(self._p, ctypes.byref(s), _p(v), _p(z))
         return z
 
+    def set_matrix(self, v, level=-1):
+        """Replace the level's matrix values (same pattern), e.g. by the
+        device-assembled ones, so that later steps start from equal data."""
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        lib().orc_set_matrix(self._p, level, _p(v))
+
     def sor_sweep(self, b, x, omega, flag, level=-1):
         b = np.ascontiguousarray(b, dtype=np.float64)
         out = np.array(x, dtype=np.float64)

@@ -362,8 +362,10 @@ def test_sor_sweep_exact_matches_oracle(oracle_lib, flag):
 @pytest.mark.parametrize("flag", [0, 2])
 def test_sor_sweep_multicolour_matches_class_order(flag):
     """3D Q2: the device visits greedy colour classes; the result equals a
-    serial sweep over the same class order, bit for bit."""
-    db = make_db(*ASSEMBLY_CASES[5])
+    serial sweep over the same class order, bit for bit (8^3 Q2: 1556
+    dependency levels, so the device uses colours)."""
+    db = make_db("Default_UnitCube_Hexa", 2, 3, "sine_3d", "supg",
+                 {"diffusion_coefficient": 1e-3, "convection": B_UNIT})
     cd = pm.ConvectionDiffusion(db)
     cd.assemble()
     h = pm.HostSetup(db)

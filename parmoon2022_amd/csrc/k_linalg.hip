@@ -319,6 +319,24 @@ __global__ __launch_bounds__(256) void k_sor_rows(int count, const int* __restri
   x[r] = x[r] + omega * s / diag[r];
 }
 
+// Multicolour variant: W lanes share a row (unrolled loads, butterfly sum),
+// then x[row] += omega * (b - sum) / diag. The row sum is rounded in a
+// different order than the serial sweep's b - a_1 x_1 - a_2 x_2 ...
+template <int W, int U, class T>
+__global__ __launch_bounds__(256) void k_sor_rows_lanes(int count, const int* __restrict__ rows,
+                                                        const int* __restrict__ rp,
+                                                        const int* __restrict__ col,
+                                                        const T* __restrict__ val,
+                                                        const T* __restrict__ diag,
+                                                        const T* __restrict__ b, T* x, T omega) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int i = gt / W, l = gt % W;
+  if(i >= count) return;
+  const int r = rows[i];
+  const T s = transfer_row<W, U, T>(rp, col, val, x, r, l);
+  if(l == 0) x[r] = x[r] + omega * (b[r] - s) / diag[r];
+}
+
 __device__ inline double block_sum(double v) {
   __shared__ double s_w[4];
   for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -839,17 +857,37 @@ void launch_prolong_add_f32(int n, int n_active, long nnz, int max_row, const in
   dispatch_transfer<float, true>(n, n_active, nnz, max_row, rp, col, val, x, y, a, s);
 }
 
-void launch_sor_rows(int count, const int* rows, const int* rp, const int* col, const double* val,
-                     const double* diag, const double* b, double* x, double omega, hipStream_t s) {
+template <class T>
+static void dispatch_sor_rows(int count, const int* rows, const int* rp, const int* col, const T* val,
+                              const T* diag, const T* b, T* x, T omega, bool serial, int max_row,
+                              hipStream_t s) {
   if(count <= 0) return;
-  k_sor_rows<double><<<(count + 255) / 256, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega);
+  if(serial) {
+    k_sor_rows<T><<<(count + 255) / 256, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega);
+  } else {
+    constexpr int W = 16;
+    const int u = unroll_for(max_row, W);
+    const int blocks = (int)(((long)count * W + 255) / 256);
+    switch(u) {
+      case 2: k_sor_rows_lanes<W, 2, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega); break;
+      case 4: k_sor_rows_lanes<W, 4, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega); break;
+      case 8: k_sor_rows_lanes<W, 8, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega); break;
+      case 16: k_sor_rows_lanes<W, 16, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega); break;
+      default: k_sor_rows_lanes<W, 0, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega); break;
+    }
+  }
   PMCD_HIP(hipGetLastError());
 }
+
+void launch_sor_rows(int count, const int* rows, const int* rp, const int* col, const double* val,
+                     const double* diag, const double* b, double* x, double omega, bool serial,
+                     int max_row, hipStream_t s) {
+  dispatch_sor_rows<double>(count, rows, rp, col, val, diag, b, x, omega, serial, max_row, s);
+}
 void launch_sor_rows_f32(int count, const int* rows, const int* rp, const int* col, const float* val,
-                         const float* diag, const float* b, float* x, float omega, hipStream_t s) {
-  if(count <= 0) return;
-  k_sor_rows<float><<<(count + 255) / 256, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega);
-  PMCD_HIP(hipGetLastError());
+                         const float* diag, const float* b, float* x, float omega, bool serial,
+                         int max_row, hipStream_t s) {
+  dispatch_sor_rows<float>(count, rows, rp, col, val, diag, b, x, omega, serial, max_row, s);
 }
 
 void launch_csr(int n, const int* rp, const int* col, const double* val, const double* x,

@@ -74,11 +74,15 @@ void launch_jacobi_from_zero_f32(int n, int n_active, const float* diag, const f
 void set_tuning(int key, int value);
 int get_tuning(int key);
 // plain CSR product y = A x over all rows (transfer operators)
-// one class of an SOR sweep: rows[0..count) updated in place (one thread per row)
+// one class of an SOR sweep: rows[0..count) updated in place; serial: one
+// thread per row summing in column order (the reference's rounding), else 16
+// lanes per row with a butterfly sum (rows of at most max_row entries)
 void launch_sor_rows(int count, const int* rows, const int* rp, const int* col, const double* val,
-                     const double* diag, const double* b, double* x, double omega, hipStream_t s);
+                     const double* diag, const double* b, double* x, double omega, bool serial,
+                     int max_row, hipStream_t s);
 void launch_sor_rows_f32(int count, const int* rows, const int* rp, const int* col, const float* val,
-                         const float* diag, const float* b, float* x, float omega, hipStream_t s);
+                         const float* diag, const float* b, float* x, float omega, bool serial,
+                         int max_row, hipStream_t s);
 // restriction y = R x on active rows, 0 on rows >= n_active
 void launch_restrict(int n, int n_active, long nnz, int max_row, const int* rp, const int* col,
                      const double* val, const double* x, double* y, hipStream_t s);

@@ -622,7 +622,7 @@ void CDSystem::sor_sweep(Level& L, const double* b, double* x, double omega, int
   const int K = (int)L.sweep_ptr.size() - 1;
   auto cls = [&](int k) {
     launch_sor_rows(L.sweep_ptr[k + 1] - L.sweep_ptr[k], L.sweep_rows.p + L.sweep_ptr[k], L.row_ptr.p,
-                    L.col.p, L.val.p, L.diag.p, b, x, omega, stream_);
+                    L.col.p, L.val.p, L.diag.p, b, x, omega, L.sweep_exact, L.st.max_row, stream_);
   };
   if(flag == 0 || flag == 2)
     for(int k = 0; k < K; ++k) cls(k);
@@ -637,7 +637,8 @@ void CDSystem::sor_sweep32(Level& L, const float* b, float* x, float omega, int 
   const int K = (int)L.sweep_ptr.size() - 1;
   auto cls = [&](int k) {
     launch_sor_rows_f32(L.sweep_ptr[k + 1] - L.sweep_ptr[k], L.sweep_rows.p + L.sweep_ptr[k],
-                        L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x, omega, stream_);
+                        L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x, omega, L.sweep_exact,
+                        L.st.max_row, stream_);
   };
   if(flag == 0 || flag == 2)
     for(int k = 0; k < K; ++k) cls(k);

@@ -361,9 +361,10 @@ def test_sor_sweep_exact_matches_oracle(oracle_lib, flag):
 
 @pytest.mark.parametrize("flag", [0, 2])
 def test_sor_sweep_multicolour_matches_class_order(flag):
-    """3D Q2: the device visits greedy colour classes; the result equals a
-    serial sweep over the same class order, bit for bit (8^3 Q2: 1556
-    dependency levels, so the device uses colours)."""
+    """3D Q2: the device visits greedy colour classes (8^3 Q2 has 1556
+    dependency levels, so the device uses colours) with 16 lanes per row; the
+    result equals a serial sweep over the same class order up to the
+    rounding of the row sums (1e-12 relative)."""
     db = make_db("Default_UnitCube_Hexa", 2, 3, "sine_3d", "supg",
                  {"diffusion_coefficient": 1e-3, "convection": B_UNIT})
     cd = pm.ConvectionDiffusion(db)
@@ -381,7 +382,9 @@ def test_sor_sweep_multicolour_matches_class_order(flag):
     ref = _class_sweep(rp, col, val, b, x0, 1.2, rows, ptr)
     if flag == 2:
         ref = _class_sweep(rp, col, val, b, ref, 1.2, rows, ptr, backward=True)
-    assert np.array_equal(got, ref)
+    rel = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    print("multicolour sweep vs serial class order: %.3e" % rel, flush=True)
+    assert rel < 1e-12
     cd.close()
 
 

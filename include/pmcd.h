@@ -206,6 +206,29 @@ int pmcd_spmv_device(int n, int n_active, const int* row_ptr, const int* col, co
 /* Ddot (Blas1.C:21-38): deterministic two-pass reduction, result to host. */
 int pmcd_dot_device(int n, const double* x, const double* y, double* result, void* stream);
 
+/* ---- batched instances ------------------------------------------------------
+ * Many independent problems (own mesh, coefficients and boundary data; shared
+ * ansatz order, discretisation and quadrature) assembled and solved together:
+ * one device launch per step covers every instance. The reference solves one
+ * ConvectionDiffusion<d> at a time (ConvectionDiffusion.C:112-452); a batch
+ * is the same path for a set of them. Instances are single-level; the solver
+ * is flexible GMRES (Iteration_gmres.C:430-620) with no, Jacobi, SOR or SSOR
+ * preconditioning (FP32 sweeps when sd->mixed), per-instance convergence. */
+typedef struct pmcd_batch pmcd_batch;
+int pmcd_batch_create(const pmcd_problem_desc* descs, int n_inst, pmcd_batch** out);
+int pmcd_batch_destroy(pmcd_batch* b);
+/* inst < 0: info[0..4] = n_inst, n_dof, n_active, nnz, n_cells of the whole
+ * batch; inst >= 0: info[0..3] = 1, n_dof, n_active, nnz of that instance. */
+int pmcd_batch_info(pmcd_batch* b, int inst, long long* info);
+int pmcd_batch_assemble(pmcd_batch* b, double* ms);
+/* stats: one entry per instance (may be NULL). */
+int pmcd_batch_solve(pmcd_batch* b, const pmcd_solver_desc* sd, pmcd_solve_stats* stats);
+/* instance-local numbering (the instance's own DOF order and CSR pattern) */
+int pmcd_batch_get_solution(pmcd_batch* b, int inst, double* out);
+int pmcd_batch_set_solution(pmcd_batch* b, int inst, const double* in);
+int pmcd_batch_get_matrix(pmcd_batch* b, int inst, double* val);
+int pmcd_batch_get_rhs(pmcd_batch* b, int inst, double* rhs);
+
 #ifdef __cplusplus
 }
 #endif

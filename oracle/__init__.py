@@ -44,7 +44,7 @@ def lib():
             build()
         L = ctypes.CDLL(LIB_PATH)
         L.orc_create.restype = ctypes.c_void_p
-        L.orc_create.argtypes = [ctypes.c_int] * 7 + [ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+        L.orc_create.argtypes = [ctypes.c_int] * 8 + [ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                                       ctypes.c_double, ctypes.c_double]
         for name in ["orc_destroy", "orc_info", "orc_dofmap", "orc_pattern", "orc_cell_coords",
                      "orc_assemble", "orc_matrix", "orc_rhs", "orc_solution", "orc_set_solution",
@@ -74,7 +74,8 @@ class Oracle:
     def __init__(self, desc):
         b = (ctypes.c_double * 3)(*list(desc.b))
         self.desc = desc
-        self._p = ctypes.c_void_p(lib().orc_create(desc.geometry, desc.n_refine, desc.order, desc.supg,
+        self._p = ctypes.c_void_p(lib().orc_create(desc.geometry, desc.structured_n, desc.n_refine,
+                                                   desc.order, desc.supg,
                                                    desc.linear_quad, desc.mg_levels, desc.example,
                                                    desc.eps, b, desc.c, desc.f0))
 

@@ -69,29 +69,32 @@ typedef struct {
   int* parent; /* cell in the coarser mesh */
 } omesh;
 
-/* exact-coordinate vertex dictionary (all coordinates here are dyadic) */
+/* refined-vertex dictionary keyed by the sorted ids of the parent vertices
+   spanning the new vertex: cells share a new vertex through their common
+   vertex, edge or face (the refinement's topology), not through equal
+   coordinates */
 typedef struct {
   int cap;
   int* slot;
+  int (*key)[4];
 } vdict;
 
-static uint64_t hash3(const double* x) {
-  uint64_t h = 1469598103934665603ull, b;
-  for(int k = 0; k < 3; ++k) {
-    memcpy(&b, &x[k], 8);
-    h ^= b;
+static uint64_t hash4(const int* k) {
+  uint64_t h = 1469598103934665603ull;
+  for(int i = 0; i < 4; ++i) {
+    h ^= (uint64_t)(uint32_t)k[i];
     h *= 1099511628211ull;
     h ^= h >> 29;
   }
   return h;
 }
 
-static int vertex_id(omesh* m, vdict* d, const double* x) {
-  uint64_t h = hash3(x) & (uint64_t)(d->cap - 1);
+static int vertex_id(omesh* m, vdict* d, const int* key, const double* x) {
+  uint64_t h = hash4(key) & (uint64_t)(d->cap - 1);
   for(;;) {
     int v = d->slot[h];
     if(v < 0) break;
-    if(m->X[3 * v] == x[0] && m->X[3 * v + 1] == x[1] && m->X[3 * v + 2] == x[2]) return v;
+    if(memcmp(d->key[h], key, sizeof(int) * 4) == 0) return v;
     h = (h + 1) & (uint64_t)(d->cap - 1);
   }
   if(m->nverts == m->cap) {
@@ -101,6 +104,7 @@ static int vertex_id(omesh* m, vdict* d, const double* x) {
   int v = m->nverts++;
   memcpy(&m->X[3 * v], x, 3 * sizeof(double));
   d->slot[h] = v;
+  memcpy(d->key[h], key, sizeof(int) * 4);
   return v;
 }
 
@@ -139,6 +143,48 @@ static void default_mesh(int geometry, omesh* m) {
   memcpy(m->cv, cells, sizeof(int) * nc * NV[m->shape]);
   m->parent = malloc(sizeof(int) * nc);
   for(int c = 0; c < nc; ++c) m->parent[c] = -1;
+}
+
+/* structured unit cube: (n+1)^3 grid vertices (x fastest), cubes in the same
+   order, each a hexahedron or split into the 6 tetrahedra of the default
+   tetrahedral cube (cells around the 0-6 diagonal) */
+static void structured_mesh(int n, int tetra, omesh* m) {
+  static const int co[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                               {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  static const int c_tet[6][4] = {{0, 1, 2, 6}, {0, 5, 1, 6}, {0, 2, 3, 6},
+                                  {0, 3, 7, 6}, {0, 4, 5, 6}, {0, 7, 4, 6}};
+  memset(m, 0, sizeof(*m));
+  m->shape = tetra ? TET : HEX;
+  const int n1 = n + 1, nv = NV[m->shape];
+  m->nverts = n1 * n1 * n1;
+  m->cap = m->nverts;
+  m->X = malloc(sizeof(double) * 3 * m->nverts);
+  for(int k = 0; k < n1; ++k)
+    for(int j = 0; j < n1; ++j)
+      for(int i = 0; i < n1; ++i) {
+        const int v = i + n1 * (j + n1 * k);
+        m->X[3 * v] = (double)i / n;
+        m->X[3 * v + 1] = (double)j / n;
+        m->X[3 * v + 2] = (double)k / n;
+      }
+  m->ncells = n * n * n * (tetra ? 6 : 1);
+  m->cv = malloc(sizeof(int) * m->ncells * nv);
+  int c = 0;
+  for(int k = 0; k < n; ++k)
+    for(int j = 0; j < n; ++j)
+      for(int i = 0; i < n; ++i) {
+        int g[8];
+        for(int a = 0; a < 8; ++a) g[a] = (i + co[a][0]) + n1 * ((j + co[a][1]) + n1 * (k + co[a][2]));
+        if(tetra)
+          for(int t = 0; t < 6; ++t, ++c)
+            for(int a = 0; a < 4; ++a) m->cv[c * 4 + a] = g[c_tet[t][a]];
+        else {
+          for(int a = 0; a < 8; ++a) m->cv[c * 8 + a] = g[a];
+          ++c;
+        }
+      }
+  m->parent = malloc(sizeof(int) * m->ncells);
+  for(int q = 0; q < m->ncells; ++q) m->parent[q] = -1;
 }
 
 /* child vertex lists in terms of the parent's refined vertices; refined
@@ -219,6 +265,7 @@ static void refine(const omesh* cm, omesh* fm) {
   d.cap = 1;
   while(d.cap < 4 * (cm->nverts * 8 + 64)) d.cap <<= 1;
   d.slot = malloc(sizeof(int) * d.cap);
+  d.key = malloc(sizeof(int[4]) * d.cap);
   for(int i = 0; i < d.cap; ++i) d.slot[i] = -1;
   for(int c = 0; c < cm->ncells; ++c) {
     int gid[27];
@@ -228,7 +275,15 @@ static void refine(const omesh* cm, omesh* fm) {
       for(int i = 0; i < n; ++i)
         for(int k = 0; k < 3; ++k) x[k] += cm->X[3 * cm->cv[c * nv + t.span[r][1 + i]] + k];
       for(int k = 0; k < 3; ++k) x[k] /= n;
-      gid[r] = vertex_id(fm, &d, x);
+      int key[4] = {-1, -1, -1, -1};
+      if(n > 4) {
+        key[0] = -2 - c; /* cell-interior vertex: never shared */
+      } else {
+        for(int i = 0; i < n; ++i) key[i] = cm->cv[c * nv + t.span[r][1 + i]];
+        for(int a = 1; a < n; ++a) /* sort the spanning vertex ids */
+          for(int b = a; b > 0 && key[b - 1] > key[b]; --b) { int tmp = key[b]; key[b] = key[b - 1]; key[b - 1] = tmp; }
+      }
+      gid[r] = vertex_id(fm, &d, key, x);
     }
     for(int ch = 0; ch < t.nchild; ++ch) {
       const int fc = c * t.nchild + ch;
@@ -237,6 +292,7 @@ static void refine(const omesh* cm, omesh* fm) {
     }
   }
   free(d.slot);
+  free(d.key);
 }
 
 /* boundary component of a joint of the unit box from its vertices
@@ -1672,11 +1728,12 @@ static int solve_sor(orc_problem* P, const osolver* s, ostats* st, int flag) {
 /* ----------------------------------------------------------------------- */
 /* exported API                                                             */
 /* ----------------------------------------------------------------------- */
-orc_problem* orc_create(int geometry, int n_refine, int order, int supg, int linear_quad, int mg_levels,
-                        int example, double eps, const double* b, double c, double f0) {
+orc_problem* orc_create(int geometry, int structured_n, int n_refine, int order, int supg, int linear_quad,
+                        int mg_levels, int example, double eps, const double* b, double c, double f0) {
   orc_problem* P = calloc(1, sizeof(orc_problem));
   omesh* meshes = calloc(n_refine + 1, sizeof(omesh));
-  default_mesh(geometry, &meshes[0]);
+  if(geometry == 10 || geometry == 11) structured_mesh(structured_n, geometry == 11, &meshes[0]);
+  else default_mesh(geometry, &meshes[0]);
   for(int i = 0; i < n_refine; ++i) refine(&meshes[i], &meshes[i + 1]);
   int keep = mg_levels < 1 ? 1 : mg_levels;
   if(keep > n_refine + 1) keep = n_refine + 1;

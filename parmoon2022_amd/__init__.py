@@ -64,6 +64,9 @@ EXPORTS = [
     "pmcd_apply", "pmcd_sor_sweep", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_set_tuning", "pmcd_get_tuning",
     "pmcd_probe_mfma",
     "pmcd_spmv_device", "pmcd_dot_device",
+    "pmcd_batch_create", "pmcd_batch_destroy", "pmcd_batch_info", "pmcd_batch_assemble",
+    "pmcd_batch_solve", "pmcd_batch_get_solution", "pmcd_batch_set_solution", "pmcd_batch_get_matrix",
+    "pmcd_batch_get_rhs",
 ]
 
 
@@ -345,6 +348,83 @@ class ConvectionDiffusion:
         _check(lib().pmcd_vcycle(self._s, ctypes.byref(sd), _ptr(v, ctypes.c_double),
                                  _ptr(z, ctypes.c_double)))
         return z
+
+
+class Batch:
+    """Many CD instances assembled and solved together (one device launch per
+    step for all of them). `dbs` are per-instance parameter dictionaries with
+    the same keys as ConvectionDiffusion's; ansatz order, discretisation and
+    quadrature must agree, each instance is single-level."""
+
+    def __init__(self, dbs):
+        self.dbs = [dict(db) for db in dbs]
+        descs = (ProblemDesc * len(self.dbs))(*[problem_desc(db) for db in self.dbs])
+        self._b = ctypes.c_void_p()
+        _check(lib().pmcd_batch_create(descs, len(self.dbs), ctypes.byref(self._b)))
+        self.assemble_ms = 0.0
+
+    def close(self):
+        if getattr(self, "_b", None):
+            lib().pmcd_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        self.close()
+
+    def info(self, inst=-1):
+        a = (ctypes.c_longlong * 5)()
+        _check(lib().pmcd_batch_info(self._b, int(inst), a))
+        keys = ["n_inst", "n_dof", "n_active", "nnz", "n_cells"]
+        return dict(zip(keys, list(a)))
+
+    def assemble(self):
+        ms = ctypes.c_double()
+        _check(lib().pmcd_batch_assemble(self._b, ctypes.byref(ms)))
+        self.assemble_ms = ms.value
+
+    def solve(self, db=None):
+        """Flexible GMRES on every instance; returns one SolveStats per instance."""
+        sd = solver_desc(self.dbs[0] if db is None else db)
+        st = (SolveStats * len(self.dbs))()
+        _check(lib().pmcd_batch_solve(self._b, ctypes.byref(sd), st))
+        return list(st)
+
+    def _vec(self, fn, inst, n):
+        out = np.empty(n, dtype=np.float64)
+        _check(fn(self._b, int(inst), _ptr(out, ctypes.c_double)))
+        return out
+
+    def solution(self, inst):
+        return self._vec(lib().pmcd_batch_get_solution, inst, self.info(inst)["n_dof"])
+
+    def set_solution(self, inst, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(lib().pmcd_batch_set_solution(self._b, int(inst), _ptr(v, ctypes.c_double)))
+
+    def rhs(self, inst):
+        return self._vec(lib().pmcd_batch_get_rhs, inst, self.info(inst)["n_dof"])
+
+    def matrix_values(self, inst):
+        return self._vec(lib().pmcd_batch_get_matrix, inst, self.info(inst)["nnz"])
+
+
+def c4_instances(n_inst=256, seed=20261202, n_range=(8, 40)):
+    """Batched workload: unit cubes of N^3 cubes split into 6 tetrahedra each,
+    N uniform in n_range, P1, SUPG, eps = 10^U(-6, 0), b a unit vector uniform
+    on the sphere, f = 1, homogeneous Dirichlet data (numpy PCG64 seeded
+    `seed`)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_inst):
+        n = int(rng.integers(n_range[0], n_range[1] + 1))
+        eps = float(10.0 ** rng.uniform(-6, 0))
+        g = rng.standard_normal(3)
+        b = g / np.linalg.norm(g)
+        out.append({"geo_file": "StructuredTetra", "structured_n": n, "ANSATZ_ORDER": 1,
+                    "refinement_n_initial_steps": 0, "example": "const_rhs", "space_discretization_type": "supg",
+                    "diffusion_coefficient": eps, "convection": tuple(float(v) for v in b), "reaction": 0.0,
+                    "rhs_constant": 1.0})
+    return out
 
 
 def autotune(cd, level=-1, iters=10, log=None):

@@ -1,6 +1,7 @@
 // extern "C" boundary of libpmcd.so (declarations in include/pmcd.h).
 #include "../../include/pmcd.h"
 #include "system.hpp"
+#include "batch.hpp"
 #include <cstring>
 #include <string>
 
@@ -9,6 +10,11 @@ using namespace pmcd;
 struct pmcd_system {
   hipStream_t stream = nullptr;
   std::unique_ptr<CDSystem> sys;
+};
+
+struct pmcd_batch {
+  hipStream_t stream = nullptr;
+  std::unique_ptr<BatchSystem> sys;
 };
 
 namespace {
@@ -375,6 +381,102 @@ int pmcd_dot_device(int n, const double* x, const double* y, double* result, voi
     launch_dot(n, x, y, part.p, out.p, (hipStream_t)stream);
     PMCD_HIP(hipMemcpyAsync(result, out.p, sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)stream));
     PMCD_HIP(hipStreamSynchronize((hipStream_t)stream));
+  });
+}
+
+// ---- batched instances -------------------------------------------------------
+int pmcd_batch_create(const pmcd_problem_desc* descs, int n_inst, pmcd_batch** out) {
+  return guarded([&] {
+    PMCD_CHECK(descs && out && n_inst > 0, "null argument or empty batch");
+    int ndev = 0;
+    if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+      PMCD_FAIL("no GPU visible: the HIP path needs an MI355X device");
+    std::vector<ProblemDesc> d(n_inst);
+    for(int i = 0; i < n_inst; ++i) d[i] = to_desc(descs[i]);
+    auto b = std::make_unique<pmcd_batch>();
+    PMCD_HIP(hipStreamCreate(&b->stream));
+    try {
+      b->sys = std::make_unique<BatchSystem>(d, b->stream);
+    } catch(...) {
+      (void)hipStreamDestroy(b->stream);
+      throw;
+    }
+    *out = b.release();
+  });
+}
+
+int pmcd_batch_destroy(pmcd_batch* b) {
+  return guarded([&] {
+    if(!b) return;
+    b->sys.reset();
+    if(b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+  });
+}
+
+int pmcd_batch_info(pmcd_batch* b, int inst, long long* info) {
+  return guarded([&] {
+    PMCD_CHECK(b && b->sys && info, "null argument");
+    BatchSystem& s = *b->sys;
+    if(inst < 0) {
+      info[0] = s.n_inst(); info[1] = s.n_dof(); info[2] = s.n_active(); info[3] = s.nnz();
+      info[4] = s.n_cells();
+    } else {
+      PMCD_CHECK(inst < s.n_inst(), "instance out of range");
+      info[0] = 1; info[1] = s.inst_n_dof(inst); info[2] = s.inst_n_active(inst);
+      info[3] = s.inst_nnz(inst); info[4] = 0;
+    }
+  });
+}
+
+int pmcd_batch_assemble(pmcd_batch* b, double* ms) {
+  return guarded([&] {
+    PMCD_CHECK(b && b->sys, "null argument");
+    b->sys->assemble();
+    if(ms) *ms = b->sys->last_assemble_ms;
+  });
+}
+
+int pmcd_batch_solve(pmcd_batch* b, const pmcd_solver_desc* sd, pmcd_solve_stats* stats) {
+  return guarded([&] {
+    PMCD_CHECK(b && b->sys && sd, "null argument");
+    auto st = b->sys->solve(to_solver(*sd));
+    if(stats)
+      for(size_t i = 0; i < st.size(); ++i) {
+        stats[i].iterations = st[i].iterations;
+        stats[i].converged = st[i].converged;
+        stats[i].initial_residual = st[i].initial_residual;
+        stats[i].final_residual = st[i].final_residual;
+        stats[i].seconds = st[i].seconds;
+      }
+  });
+}
+
+int pmcd_batch_get_solution(pmcd_batch* b, int inst, double* out) {
+  return guarded([&] {
+    PMCD_CHECK(b && b->sys && out, "null argument");
+    b->sys->get_solution(inst, out);
+  });
+}
+
+int pmcd_batch_set_solution(pmcd_batch* b, int inst, const double* in) {
+  return guarded([&] {
+    PMCD_CHECK(b && b->sys && in, "null argument");
+    b->sys->set_solution(inst, in);
+  });
+}
+
+int pmcd_batch_get_matrix(pmcd_batch* b, int inst, double* val) {
+  return guarded([&] {
+    PMCD_CHECK(b && b->sys && val, "null argument");
+    b->sys->get_matrix(inst, val);
+  });
+}
+
+int pmcd_batch_get_rhs(pmcd_batch* b, int inst, double* rhs) {
+  return guarded([&] {
+    PMCD_CHECK(b && b->sys && rhs, "null argument");
+    b->sys->get_rhs(inst, rhs);
   });
 }
 

@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_assemble_local(AsmArgs a) {
     }
     const double mult = a.qw[q] * fabs(g.det);
     double cf[6];
-    example_coeffs(a.ex, x[0], x[1], a.dim == 3 ? x[2] : 0.0, cf);
+    example_coeffs(cell_params(a, cell), x[0], x[1], a.dim == 3 ? x[2] : 0.0, cf);
     double* s = s_coef[wave][q];
     s[0] = mult;
     for(int k = 0; k < 6; ++k) s[1 + k] = cf[k];
@@ -391,8 +391,8 @@ __global__ __launch_bounds__(64) void k_cell_errors(AsmArgs a, const int* __rest
     const double w = a.qw[q] * fabs(g.det);
     double ex[5], cf[6];
     const double z = a.dim == 3 ? x[2] : 0.0;
-    example_exact(a.ex, x[0], x[1], z, ex);
-    example_coeffs(a.ex, x[0], x[1], z, cf);
+    example_exact(cell_params(a, cell), x[0], x[1], z, ex);
+    example_coeffs(cell_params(a, cell), x[0], x[1], z, cf);
     double nu, b1, b2, b3, c;
     if(a.dim == 3) { nu = cf[0]; b1 = cf[1]; b2 = cf[2]; b3 = cf[3]; c = cf[4]; }
     else { nu = cf[0]; b1 = cf[1]; b2 = cf[2]; b3 = 0; c = cf[4]; }

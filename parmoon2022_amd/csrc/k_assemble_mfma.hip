@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
         }
         const double w = a.qw[q] * fabs(g.det);
         double cf[6];
-        example_coeffs(a.ex, x[0], x[1], DIM == 3 ? x[2] : 0.0, cf);
+        example_coeffs(cell_params(a, cell), x[0], x[1], DIM == 3 ? x[2] : 0.0, cf);
         const double eps = cf[0], c = cf[4], f = cf[5];
         const double b[3] = {cf[1], cf[2], DIM == 3 ? cf[3] : 0.0};
         const double delta = a.supg ? sdfem_delta_m(DIM, g.hK, eps, b[0], b[1], b[2]) : 0.0;

@@ -18,6 +18,9 @@ struct AsmArgs {
   int supg;         // 1: add the SUPG/SD local routine
   int n_cells;
   ExampleParams ex;
+  // batched instances: cell c uses cell_ex[cell_inst[c]] instead of ex
+  const ExampleParams* cell_ex = nullptr;
+  const int* cell_inst = nullptr;
   const double* ref;        // [nq][ncomp][nl] reference basis values/derivatives
   const double* qw;         // [nq] reference weights
   const double* qp;         // [nq][3] reference points
@@ -25,6 +28,9 @@ struct AsmArgs {
   double* local_A;          // [n_cells][nl][nl]  (row = test, column = ansatz)
   double* local_rhs;        // [n_cells][nl]
 };
+__host__ __device__ inline const ExampleParams& cell_params(const AsmArgs& a, int cell) {
+  return a.cell_inst ? a.cell_ex[a.cell_inst[cell]] : a.ex;
+}
 void launch_assemble_local(const AsmArgs& a, hipStream_t s);
 // FP64-MFMA variant; returns false if the element is not covered
 bool launch_assemble_mfma(const AsmArgs& a, hipStream_t s);
@@ -142,5 +148,25 @@ void launch_csr_to_dense(int n, const int* rp, const int* col, const double* val
 // in-place Gauss-Jordan inverse with partial pivoting: A is overwritten,
 // Ainv receives the inverse; work holds n + 2 doubles
 void dense_inverse(int n, double* A, double* Ainv, double* work, int* ipiv_dev, hipStream_t s);
+
+// ---- batched (segmented) vector kernels ------------------------------------
+// A chunk is a row range [begin, end) owned by one instance.
+struct SegChunk {
+  int inst, begin, end;
+};
+constexpr int kChunkRows = 4096;
+// out[i] = (x, y) restricted to instance i's rows (partial: one per chunk)
+void launch_seg_dot(int n_chunks, const SegChunk* ch, int n_inst, const int* inst_ptr,
+                    const double* x, const double* y, double* partial, double* out, hipStream_t s);
+// w -= hprev[inst] vprev (vprev may be null), then out[i] = (w, vnext) per
+// instance (vnext may be null)
+void launch_seg_mgs(int n_chunks, const SegChunk* ch, int n_inst, const int* inst_ptr,
+                    const double* hprev, const double* vprev, double* w, const double* vnext,
+                    double* partial, double* out, hipStream_t s);
+// y = a[inst] x ; y += a[inst] x
+void launch_seg_scale(int n_chunks, const SegChunk* ch, const double* a, const double* x, double* y,
+                      hipStream_t s);
+void launch_seg_axpy(int n_chunks, const SegChunk* ch, const double* a, const double* x, double* y,
+                     hipStream_t s);
 
 }  // namespace pmcd

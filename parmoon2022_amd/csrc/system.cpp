@@ -6,8 +6,6 @@
 
 namespace pmcd {
 
-namespace {
-
 // physical position of the nodal point of local DOF i of cell c
 void node_position(const Mesh& m, int c, const ElementInfo& info, int i, double* x) {
   const int* v = m.cell(c);
@@ -31,10 +29,12 @@ void node_position(const Mesh& m, int c, const ElementInfo& info, int i, double*
   }
 }
 
-double now_s() {
+double now_seconds() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+namespace {
+double now_s() { return now_seconds(); }
 }  // namespace
 
 std::vector<std::unique_ptr<Level>> build_levels(const ProblemDesc& d) {
@@ -603,7 +603,17 @@ void CDSystem::precond(const double* v, double* z) {
   }
 }
 
-void CDSystem::prepare_sweep(Level& L) {
+void CDSystem::prepare_sweep(Level& L) { prepare_level_sweep(L, stream_); }
+
+void CDSystem::sor_sweep(Level& L, const double* b, double* x, double omega, int flag) {
+  level_sor_sweep(L, b, x, omega, flag, stream_);
+}
+
+void CDSystem::sor_sweep32(Level& L, const float* b, float* x, float omega, int flag) {
+  level_sor_sweep32(L, b, x, omega, flag, stream_);
+}
+
+void prepare_level_sweep(Level& L, hipStream_t stream) {
   if(!L.sweep_ptr.empty()) return;
   // the level schedule is exact; beyond this many classes (deep dependency
   // chains, e.g. Q2 in 3D) launches per class would dominate and the sweep
@@ -612,17 +622,17 @@ void CDSystem::prepare_sweep(Level& L) {
   SweepOrder o = build_sweep_order(L.st, kMaxExactClasses);
   L.sweep_ptr = std::move(o.class_ptr);
   L.sweep_exact = o.exact;
-  L.sweep_rows.upload(o.rows, stream_);
-  PMCD_HIP(hipStreamSynchronize(stream_));
+  L.sweep_rows.upload(o.rows, stream);
+  PMCD_HIP(hipStreamSynchronize(stream));
 }
 
-void CDSystem::sor_sweep(Level& L, const double* b, double* x, double omega, int flag) {
+void level_sor_sweep(Level& L, const double* b, double* x, double omega, int flag, hipStream_t stream) {
   PMCD_CHECK(flag >= 0 && flag <= 2, "sor_sweep flag must be 0, 1 or 2");
-  prepare_sweep(L);
+  prepare_level_sweep(L, stream);
   const int K = (int)L.sweep_ptr.size() - 1;
   auto cls = [&](int k) {
     launch_sor_rows(L.sweep_ptr[k + 1] - L.sweep_ptr[k], L.sweep_rows.p + L.sweep_ptr[k], L.row_ptr.p,
-                    L.col.p, L.val.p, L.diag.p, b, x, omega, L.sweep_exact, L.st.max_row, stream_);
+                    L.col.p, L.val.p, L.diag.p, b, x, omega, L.sweep_exact, L.st.max_row, stream);
   };
   if(flag == 0 || flag == 2)
     for(int k = 0; k < K; ++k) cls(k);
@@ -630,15 +640,16 @@ void CDSystem::sor_sweep(Level& L, const double* b, double* x, double omega, int
     for(int k = K - 1; k >= 0; --k) cls(k);
 }
 
-void CDSystem::sor_sweep32(Level& L, const float* b, float* x, float omega, int flag) {
+void level_sor_sweep32(Level& L, const float* b, float* x, float omega, int flag,
+                       hipStream_t stream) {
   PMCD_CHECK(flag >= 0 && flag <= 2, "sor_sweep flag must be 0, 1 or 2");
   PMCD_CHECK(L.val32.n == L.val.n, "FP32 matrix copies not prepared");
-  prepare_sweep(L);
+  prepare_level_sweep(L, stream);
   const int K = (int)L.sweep_ptr.size() - 1;
   auto cls = [&](int k) {
     launch_sor_rows_f32(L.sweep_ptr[k + 1] - L.sweep_ptr[k], L.sweep_rows.p + L.sweep_ptr[k],
                         L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x, omega, L.sweep_exact,
-                        L.st.max_row, stream_);
+                        L.st.max_row, stream);
   };
   if(flag == 0 || flag == 2)
     for(int k = 0; k < K; ++k) cls(k);

@@ -464,3 +464,65 @@ def test_c2_shape_sor_smoother_solution(oracle_lib, smoother, mixed):
                                                                  so.iterations, rel), flush=True)
     assert st.converged and so.converged and rel < 1e-8
     cd.close()
+
+
+def _batch_dbs():
+    rng = np.random.default_rng(3)
+    out = []
+    for n, eps in ((3, 1e-2), (5, 1e-5), (4, 1.0), (6, 3e-4), (2, 1e-6)):
+        g = rng.standard_normal(3)
+        out.append({"geo_file": "StructuredTetra", "structured_n": n, "ANSATZ_ORDER": 1, "example": "const_rhs",
+                    "space_discretization_type": "supg", "diffusion_coefficient": eps,
+                    "convection": tuple(g / np.linalg.norm(g)), "reaction": 0.0, "rhs_constant": 1.0,
+                    "solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "ssor",
+                    "sor_omega": 1.0, "gmres_restart": 30, "residual_reduction": 1e-12, "residual_tolerance": 0.0,
+                    "max_n_iterations": 500})
+    return out
+
+
+def test_batch_assembly_equals_single_instances():
+    """Each instance of a batch assembles to the same bits as on its own."""
+    dbs = _batch_dbs()
+    bt = pm.Batch(dbs)
+    bt.assemble()
+    for i, db in enumerate(dbs):
+        cd = pm.ConvectionDiffusion(db)
+        cd.assemble()
+        assert np.array_equal(bt.matrix_values(i), cd.matrix_values()), i
+        assert np.array_equal(bt.rhs(i), cd.rhs()), i
+        cd.close()
+    bt.close()
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_batch_solve_matches_single_and_oracle(oracle_lib, mixed):
+    """Batched FGMRES + SSOR (FP64 or FP32 sweeps): every instance converges
+    and matches its single-instance solve and the oracle's serial-SSOR solve
+    (to 1e-9 relative at a 1e-12 residual reduction; iteration counts within
+    rounding-driven drift of the single-instance run)."""
+    dbs = _batch_dbs()
+    for db in dbs:
+        db["multigrid_mixed_precision"] = mixed
+    bt = pm.Batch(dbs)
+    bt.assemble()
+    sdb = dict(dbs[0])
+    stats = bt.solve(sdb)
+    for i, db in enumerate(dbs):
+        ub = bt.solution(i)
+        cd = pm.ConvectionDiffusion(db)
+        cd.assemble()
+        st = cd.solve(dict(sdb, multigrid_mixed_precision=0))
+        us = cd.solution()
+        o = oracle_lib.Oracle(pm.problem_desc(db))
+        o.assemble()
+        o.solve(pm.solver_desc(dict(sdb, multigrid_mixed_precision=0)))
+        uo = o.solution()
+        rs = np.linalg.norm(ub - us) / np.linalg.norm(us)
+        ro = np.linalg.norm(ub - uo) / np.linalg.norm(uo)
+        print("instance %d: batch its %d single its %d, rel vs single %.2e vs oracle %.2e" %
+              (i, stats[i].iterations, st.iterations, rs, ro), flush=True)
+        assert stats[i].converged and rs < 1e-9 and ro < 1e-9
+        if not mixed:
+            assert abs(stats[i].iterations - st.iterations) <= 1
+        cd.close()
+    bt.close()

@@ -29,10 +29,13 @@ CASES = [("TwoTriangles", 1, 4, "sine_laplace_2d"), ("TwoTriangles2", 2, 3, "sin
          ("Default_UnitCube_Tetra", 1, 3, "const_rhs"), ("Default_UnitCube_Tetra", 2, 2, "const_rhs")]
 
 
-@pytest.mark.parametrize("geo,order,nref,ex", CASES)
+@pytest.mark.parametrize("geo,order,nref,ex", CASES + [("StructuredTetra", 1, 0, "const_rhs"),
+                                                       ("StructuredTetra", 2, 1, "const_rhs"),
+                                                       ("StructuredHexa", 2, 1, "const_rhs")])
 def test_dof_numbering_and_pattern_match_oracle(oracle_lib, geo, order, nref, ex):
     db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": nref, "example": ex,
-          "preconditioner": "multigrid", "solver_type": "iterative", "multigrid_n_levels": 3}
+          "preconditioner": "multigrid", "solver_type": "iterative", "multigrid_n_levels": 3,
+          "structured_n": 5}
     h = pm.HostSetup(db)
     o = oracle_lib.Oracle(pm.problem_desc(db))
     for level in range(h.info()["n_levels"]):

@@ -469,7 +469,7 @@ def test_c2_shape_sor_smoother_solution(oracle_lib, smoother, mixed):
 def _batch_dbs():
     rng = np.random.default_rng(3)
     out = []
-    for n, eps in ((3, 1e-2), (5, 1e-5), (4, 1.0), (6, 3e-4), (2, 1e-6)):
+    for n, eps in ((3, 1e-2), (5, 1e-5), (4, 1.0), (6, 3e-4), (7, 1e-6)):
         g = rng.standard_normal(3)
         out.append({"geo_file": "StructuredTetra", "structured_n": n, "ANSATZ_ORDER": 1, "example": "const_rhs",
                     "space_discretization_type": "supg", "diffusion_coefficient": eps,

@@ -143,9 +143,90 @@ def measured_traffic(kernel, n, nnz):
     return None
 
 
+C4_SOLVER = {"solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "ssor",
+             "sor_omega": 1.0, "gmres_restart": 30, "residual_reduction": 1e-10, "residual_tolerance": 0.0,
+             "max_n_iterations": 2000}
+
+
+def c4_cpu_baseline(dbs, sdb, per_gpu_solves, sample):
+    """Oracle (scalar C port) solving `sample` instances of the batch one by
+    one (FGMRES + serial SSOR in FP64), extrapolated to the whole batch by
+    DoF count."""
+    import numpy as np
+    import oracle
+    import parmoon2022_amd as pm
+    rng = np.random.default_rng(1)
+    idx = rng.choice(len(dbs), size=min(sample, len(dbs)), replace=False)
+    t_sum, dof_sum = 0.0, 0
+    for i in idx:
+        o = oracle.Oracle(pm.problem_desc(dbs[i]))
+        t0 = time.time()
+        o.assemble()
+        o.solve(pm.solver_desc(dict(dbs[i], **dict(sdb, multigrid_mixed_precision=0))))
+        t_sum += time.time() - t0
+        dof_sum += o.info()["n_dof"]
+    total_dof = sum((db["structured_n"] + 1) ** 3 for db in dbs)
+    t_batch = t_sum * total_dof / dof_sum
+    return {"value": len(dbs) / t_batch, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": "oracle (C, -O3, 1 thread): assembly + FGMRES(30)/SSOR of %d of the %d instances "
+                      "(%d DoF, %.1f s), scaled by DoF to the whole batch" % (len(idx), len(dbs), dof_sum, t_sum)}
+
+
+def bench_c4(args):
+    import numpy as np
+    import parmoon2022_amd as pm
+    dbs = pm.c4_instances(args.instances)
+    sdb = dict(C4_SOLVER, multigrid_mixed_precision=1 if args.mixed else 0)
+    t0 = time.time()
+    bt = pm.Batch(dbs)
+    info = bt.info()
+    log("setup %.1f s" % (time.time() - t0), info)
+    lat, its = [], None
+    for step in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        bt.assemble()
+        st = bt.solve(sdb)
+        dt = time.perf_counter() - t0
+        its = np.array([s.iterations for s in st])
+        conv = int(sum(s.converged for s in st))
+        log("step %d: %.3f s (assembly %.1f ms), iterations mean %.1f max %d, converged %d/%d" %
+            (step, dt, bt.assemble_ms, its.mean(), its.max(), conv, len(st)))
+        if step >= args.warmup:
+            lat.append(dt)
+    total = sum(lat)
+    value = args.steps * len(dbs) / total
+    result = {
+        "metric": "CD3D solves/s & DoF/s at fixed L2 err vs FP64 oracle; p50/p99 solve latency",
+        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-ssor/f64" if args.mixed else "f64",
+        "data": "synthetic (seeded instance parameters, generated meshes)",
+        "config": {"workload": "C4 batch of %d P1 SUPG tetrahedral unit cubes, N in 8..40, %d DoF, nnz %d, "
+                               "FGMRES(30) + SSOR, rel. res 1e-10" % (len(dbs), info["n_dof"], info["nnz"]),
+                   "instances": len(dbs), "dof": info["n_dof"], "nnz": info["nnz"], "cells": info["n_cells"]},
+        "dof_per_s": value * info["n_dof"] / len(dbs),
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "iterations": {"mean": float(its.mean()), "max": int(its.max())},
+        "converged": conv,
+    }
+    bt.close()
+    if not args.no_cpu:
+        try:
+            result["cpu_baseline"] = c4_cpu_baseline(dbs, sdb, value, 8)
+            result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
+        except Exception as ex:
+            log("cpu baseline failed:", ex)
+            result["cpu_baseline"] = None
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4"],
+                    help="c2: one 64^3 Q2 SUPG solve per step (default); c4: a batch of --instances P1 "
+                         "tetrahedral instances per step")
+    ap.add_argument("--instances", type=int, default=256, help="c4 batch size")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--refine", type=int, default=6)
@@ -177,6 +258,8 @@ def main():
     import numpy as np
     import parmoon2022_amd as pm
 
+    if args.workload == "c4":
+        return bench_c4(args)
     db = c2_db(args)
     t0 = time.time()
     cd = pm.ConvectionDiffusion(db)

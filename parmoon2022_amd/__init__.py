@@ -70,6 +70,19 @@ EXPORTS = [
 ]
 
 
+def set_sor_exact_limit(n):
+    """SOR sweeps of systems created afterwards follow the exact level
+    schedule (bit-identical to the serial sweep) when it has at most n
+    classes, greedy multicolour order otherwise (default 64)."""
+    _check(lib().pmcd_set_tuning(5, int(n)))
+
+
+def get_sor_exact_limit():
+    v = ctypes.c_int(0)
+    _check(lib().pmcd_get_tuning(5, ctypes.byref(v)))
+    return v.value
+
+
 def set_assembly_variant(variant):
     """0: reference operation order on vector ALUs; 1: FP64 MFMA (default)."""
     _check(lib().pmcd_set_tuning(4, int(variant)))
@@ -204,8 +217,11 @@ class HostSetup:
         _check(lib().pmcd_host_pattern(self._h, level, _ptr(rp, ctypes.c_int), _ptr(col, ctypes.c_int)))
         return rp, col
 
-    def sweep_order(self, level=-1, max_exact_classes=512):
-        """Device SOR row classes: (rows, class_ptr, exact)."""
+    def sweep_order(self, level=-1, max_exact_classes=None):
+        """Device SOR row classes: (rows, class_ptr, exact); the limit defaults
+        to the library's current one (set_sor_exact_limit)."""
+        if max_exact_classes is None:
+            max_exact_classes = get_sor_exact_limit()
         n = self.info(level)["n_dof"]
         rows = np.empty(n, dtype=np.int32)
         ptr = np.empty(n + 1, dtype=np.int32)

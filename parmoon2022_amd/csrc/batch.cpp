@@ -207,6 +207,12 @@ BatchSystem::BatchSystem(const std::vector<ProblemDesc>& descs, hipStream_t stre
   }
   chunks_.upload(chunks_host_, stream_);
   inst_ptr_.upload(iptr, stream_);
+  std::vector<int> rinst(n_);
+  for(int i = 0; i < n_inst(); ++i)
+    for(int r = 0; r < inst_[i].L->n(); ++r) rinst[grow(i, r)] = i;
+  row_inst_.upload(rinst, stream_);
+  active_host_.assign(n_inst(), 1);
+  active_.upload(active_host_, stream_);
   partial_.alloc(chunks_host_.size() + 1);
   PMCD_HIP(hipStreamSynchronize(stream_));
 }
@@ -252,7 +258,18 @@ void BatchSystem::assemble() {
 
 void BatchSystem::seg_dot(const double* x, const double* y, double* out_dev) {
   launch_seg_dot((int)chunks_host_.size(), chunks_.p, n_inst(), inst_ptr_.p, x, y, partial_.p, out_dev,
-                 stream_);
+                 stream_, active_.p);
+}
+
+// y = beta z + alpha A x with the non-active rule; rows of converged
+// instances become 0 (their vectors are all zero from then on)
+void BatchSystem::spmv(const double* x, const double* z, double* y, double alpha, double beta) {
+  if(G_.st.max_row <= 64)
+    launch_spmv_masked(n_, na_, G_.row_ptr.p, G_.col.p, G_.val.p, x, z, y, alpha, beta, G_.st.max_row,
+                       row_inst_.p, active_.p, stream_);
+  else
+    launch_spmv_rowlen(n_, na_, G_.row_ptr.p, G_.col.p, G_.val.p, x, z, y, alpha, beta, stream_,
+                       G_.st.max_row, (double)G_.st.nnz / std::max(1, n_));
 }
 
 void BatchSystem::precond(const double* v, double* z) {
@@ -265,11 +282,11 @@ void BatchSystem::precond(const double* v, double* z) {
       if(sd_.mixed) {
         launch_d2f(n_, v, f_v_.p, stream_);
         launch_fill_f32(n_, 0.0f, f_z_.p, stream_);
-        level_sor_sweep32(G_, f_v_.p, f_z_.p, (float)sd_.sor_omega, flag, stream_);
+        level_sor_sweep32(G_, f_v_.p, f_z_.p, (float)sd_.sor_omega, flag, stream_, row_inst_.p, active_.p);
         launch_f2d(n_, f_z_.p, z, stream_);
       } else {
         launch_fill(n_, 0.0, z, stream_);
-        level_sor_sweep(G_, v, z, sd_.sor_omega, flag, stream_);
+        level_sor_sweep(G_, v, z, sd_.sor_omega, flag, stream_, row_inst_.p, active_.p);
       }
       break;
     }
@@ -317,12 +334,19 @@ std::vector<SolveStats> BatchSystem::solve(const SolverDesc& sd) {
   double* r = kr_.p;
   auto residual = [&]() {  // r = b - A x, beta = |r| per instance
     launch_copy(n, G_.rhs.p, r, stream_);
-    launch_spmv(n, na_, G_.row_ptr.p, G_.col.p, G_.val.p, x, r, r, -1.0, 1.0, stream_, G_.st.max_row);
+    spmv(x, r, r, -1.0, 1.0);
     seg_dot(r, r, kh_.p);
     PMCD_HIP(hipMemcpyAsync(beta.data(), kh_.p, sizeof(double) * ni, hipMemcpyDeviceToHost, stream_));
     PMCD_HIP(hipStreamSynchronize(stream_));
     for(auto& b : beta) b = std::sqrt(b);
   };
+  // active flags on the device follow active_host_ (copied after each change;
+  // the host array changes again only after a later stream synchronisation)
+  auto push_active = [&]() {
+    PMCD_HIP(hipMemcpyAsync(active_.p, active_host_.data(), ni, hipMemcpyHostToDevice, stream_));
+  };
+  std::fill(active_host_.begin(), active_host_.end(), 1);
+  push_active();
   auto set_scale = [&](const std::vector<double>& a) {
     PMCD_HIP(hipMemcpyAsync(kscale_.p, a.data(), sizeof(double) * ni, hipMemcpyHostToDevice, stream_));
   };
@@ -356,6 +380,10 @@ std::vector<SolveStats> BatchSystem::solve(const SolverDesc& sd) {
                     stream_);
     PMCD_HIP(hipStreamSynchronize(stream_));
   };
+  auto sync_active = [&]() {
+    for(int inst = 0; inst < ni; ++inst) active_host_[inst] = active[inst] ? 1 : 0;
+    push_active();
+  };
   residual();
   int n_active_inst = 0;
   for(int inst = 0; inst < ni; ++inst) {
@@ -369,6 +397,7 @@ std::vector<SolveStats> BatchSystem::solve(const SolverDesc& sd) {
       ++n_active_inst;
     }
   }
+  sync_active();
   int j = 1;
   while(j <= sd_.max_iter && n_active_inst > 0) {
     for(int inst = 0; inst < ni; ++inst) {
@@ -377,17 +406,17 @@ std::vector<SolveStats> BatchSystem::solve(const SolverDesc& sd) {
       s[(size_t)inst * (m + 1)] = beta[inst];
     }
     set_scale(scale);
-    launch_seg_scale((int)chunks_host_.size(), chunks_.p, kscale_.p, r, kv_[0].p, stream_);
+    launch_seg_scale((int)chunks_host_.size(), chunks_.p, kscale_.p, r, kv_[0].p, stream_, active_.p);
     int i = 0;
     for(; i < m && j <= sd_.max_iter && n_active_inst > 0; ++i, ++j) {
       precond(kv_[i].p, kz_[i].p);
-      launch_spmv(n, na_, G_.row_ptr.p, G_.col.p, G_.val.p, kz_[i].p, nullptr, r, 1.0, 0.0, stream_,
-                  G_.st.max_row);
+      spmv(kz_[i].p, nullptr, r, 1.0, 0.0);
       // modified Gram-Schmidt, one fused step per basis vector
       seg_dot(r, kv_[0].p, kh_.p);
       for(int k = 0; k <= i; ++k)
         launch_seg_mgs((int)chunks_host_.size(), chunks_.p, ni, inst_ptr_.p, kh_.p + (size_t)k * ni, kv_[k].p,
-                       r, k < i ? kv_[k + 1].p : nullptr, partial_.p, kh_.p + (size_t)(k + 1) * ni, stream_);
+                       r, k < i ? kv_[k + 1].p : nullptr, partial_.p, kh_.p + (size_t)(k + 1) * ni, stream_,
+                       active_.p);
       seg_dot(r, r, kh_.p + (size_t)(i + 1) * ni);
       PMCD_HIP(hipMemcpyAsync(hcol.data(), kh_.p, sizeof(double) * (size_t)(i + 2) * ni,
                               hipMemcpyDeviceToHost, stream_));
@@ -427,9 +456,10 @@ std::vector<SolveStats> BatchSystem::solve(const SolverDesc& sd) {
             st[inst].converged = resid[inst] < sd_.res_tol ||
                                  resid[inst] < sd_.res_red * st[inst].initial_residual;
           }
+        sync_active();
       }
       set_scale(scale);
-      launch_seg_scale((int)chunks_host_.size(), chunks_.p, kscale_.p, r, kv_[i + 1].p, stream_);
+      launch_seg_scale((int)chunks_host_.size(), chunks_.p, kscale_.p, r, kv_[i + 1].p, stream_, active_.p);
     }
     if(n_active_inst == 0) break;
     // restart: update, true residual
@@ -445,6 +475,7 @@ std::vector<SolveStats> BatchSystem::solve(const SolverDesc& sd) {
                              resid[inst] < sd_.res_red * st[inst].initial_residual;
       }
     }
+    sync_active();
   }
   for(int inst = 0; inst < ni; ++inst)
     if(active[inst]) {

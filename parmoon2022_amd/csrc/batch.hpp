@@ -57,6 +57,7 @@ class BatchSystem {
   void to_global(int i, const double* local, double* global_host);
   void precond(const double* v, double* z);
   void seg_dot(const double* x, const double* y, double* out_dev);
+  void spmv(const double* x, const double* z, double* y, double alpha, double beta);
 
   hipStream_t stream_;
   std::vector<Inst> inst_;
@@ -78,6 +79,10 @@ class BatchSystem {
   SolverDesc sd_{};
   std::vector<DevBuf<double>> kv_, kz_;
   DevBuf<double> kr_, kw_, kh_, kscale_, kcoef_;
+  // converged instances are masked out of the matrix kernels
+  DevBuf<int> row_inst_;
+  DevBuf<uint8_t> active_;
+  std::vector<uint8_t> active_host_;
   DevBuf<float> f_v_, f_z_;
   bool mixed_ready_ = false;
 };

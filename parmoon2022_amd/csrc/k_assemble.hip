@@ -363,6 +363,97 @@ __global__ void k_gather_rhs(int n_rows, int n_active, const int* __restrict__ c
 // Error integrals of one cell (FEFunction3D.C:60-248 with
 // conv_diff_l2_h1_linf_error<d>, ConvDiff.C:2124-2183): lanes over points,
 // then lane 0 sums the point contributions in rule order.
+// P1 triangles and tetrahedra: one thread per cell, the whole local matrix
+// ((DIM+1)^2 entries) in registers. Same per-entry operation order as
+// k_assemble_local (the reference's), so both give the same bits; the
+// Laplacian of a P1 basis function is zero.
+template <int DIM>
+__global__ __launch_bounds__(256) void k_assemble_p1(AsmArgs a) {
+  constexpr int NL = DIM + 1;
+  const int cell = blockIdx.x * 256 + threadIdx.x;
+  if(cell >= a.n_cells) return;
+  Geo g;
+  cell_geometry(a.shape, a.cell_xyz + (size_t)cell * a.nv * 3, g);
+  const bool supg = a.supg != 0;
+  const ExampleParams& ex = cell_params(a, cell);
+  double acc[NL * NL], rhs[NL];
+#pragma unroll
+  for(int e = 0; e < NL * NL; ++e) acc[e] = 0.0;
+#pragma unroll
+  for(int i = 0; i < NL; ++i) rhs[i] = 0.0;
+  for(int q = 0; q < a.nq; ++q) {
+    const double xi = a.qp[3 * q], eta = a.qp[3 * q + 1], zeta = a.qp[3 * q + 2];
+    double x[3];
+    for(int d = 0; d < 3; ++d) {
+      x[d] = g.c0[d] + g.C[d][0] * xi + g.C[d][1] * eta;
+      if(DIM == 3) x[d] = x[d] + g.C[d][2] * zeta;
+    }
+    const double Mult = a.qw[q] * fabs(g.det);
+    double cf[6];
+    example_coeffs(ex, x[0], x[1], DIM == 3 ? x[2] : 0.0, cf);
+    const double eps = cf[0], b1 = cf[1], b2 = cf[2], b3 = cf[3], c = cf[4], f = cf[5];
+    const double delta = supg ? sdfem_delta(DIM, g.hK, eps, b1, b2, b3) : 0.0;
+    double P[4][NL];
+#pragma unroll
+    for(int i = 0; i < NL; ++i) {
+      double o[5];
+      phys_values(a, g, q, i, false, o);
+      for(int k = 0; k < 4; ++k) P[k][i] = o[k];
+    }
+#pragma unroll
+    for(int i = 0; i < NL; ++i) {
+      const double t0 = P[0][i], tx = P[1][i], ty = P[2][i], tz = P[3][i];
+#pragma unroll
+      for(int j = 0; j < NL; ++j) {
+        const double a0 = P[0][j], ax = P[1][j], ay = P[2][j], az = P[3][j];
+        double& A = acc[i * NL + j];
+        if(DIM == 3) {
+          double val = eps * (tx * ax + ty * ay + tz * az);
+          val += (b1 * ax + b2 * ay + b3 * az) * t0;
+          val += c * a0 * t0;
+          A += val * Mult;
+          if(supg) {
+            const double bgradv = b1 * tx + b2 * ty + b3 * tz;
+            double v2 = -eps * 0.0;
+            v2 += b1 * ax + b2 * ay + b3 * az + c * a0;
+            A += v2 * delta * bgradv * Mult;
+          }
+        } else {
+          double val = eps * (tx * ax + ty * ay);
+          val += (b1 * ax + b2 * ay) * t0;
+          val += c * a0 * t0;
+          val *= Mult;
+          A += val;
+          if(supg) {
+            const double bgradv = b1 * tx + b2 * ty;
+            double v2 = delta * (-eps * 0.0 + b1 * ax + b2 * ay + c * a0) * bgradv;
+            v2 *= Mult;
+            A += v2;
+          }
+        }
+      }
+      if(DIM == 3) {
+        rhs[i] += Mult * t0 * f;
+        if(supg) {
+          const double bgradv = b1 * tx + b2 * ty + b3 * tz;
+          rhs[i] += Mult * (delta * bgradv) * f;
+        }
+      } else {
+        rhs[i] += Mult * t0 * f;
+        if(supg) {
+          const double bgradv = b1 * tx + b2 * ty;
+          rhs[i] += Mult * delta * bgradv * f;
+        }
+      }
+    }
+  }
+  double* A = a.local_A + (size_t)cell * NL * NL;
+#pragma unroll
+  for(int e = 0; e < NL * NL; ++e) A[e] = acc[e];
+#pragma unroll
+  for(int i = 0; i < NL; ++i) a.local_rhs[(size_t)cell * NL + i] = rhs[i];
+}
+
 // cell_err[5 * cell + k]: L2^2, H1-semi^2, SD^2, DG (volume part), Linf.
 __global__ __launch_bounds__(64) void k_cell_errors(AsmArgs a, const int* __restrict__ dof,
                                                      const double* __restrict__ u,
@@ -428,8 +519,20 @@ __global__ __launch_bounds__(64) void k_cell_errors(AsmArgs a, const int* __rest
 
 }  // namespace
 
+bool p1_simplex(const AsmArgs& a) {
+  return (a.shape == (int)Shape::Triangle && a.nl == 3) || (a.shape == (int)Shape::Tetrahedron && a.nl == 4);
+}
+
 void launch_assemble_local(const AsmArgs& a, hipStream_t s) {
   PMCD_CHECK(a.nq <= kMaxQ, "too many quadrature points ", a.nq);
+  if(a.n_cells == 0) return;
+  if(p1_simplex(a)) {
+    const int blocks = (a.n_cells + 255) / 256;
+    if(a.dim == 3) k_assemble_p1<3><<<blocks, 256, 0, s>>>(a);
+    else k_assemble_p1<2><<<blocks, 256, 0, s>>>(a);
+    PMCD_HIP(hipGetLastError());
+    return;
+  }
   const int blocks = (a.n_cells + kCellsPerBlock - 1) / kCellsPerBlock;
   if(blocks == 0) return;
   switch(a.nl) {

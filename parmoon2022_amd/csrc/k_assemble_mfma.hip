@@ -291,6 +291,9 @@ __global__ void k_mfma_probe(double* out) {
 
 bool launch_assemble_mfma(const AsmArgs& a, hipStream_t s) {
   if(a.n_cells == 0) return true;
+  // P1 simplices: 3 or 4 local DOFs fill too little of a 16x16 tile; the
+  // thread-per-cell vector kernel is faster
+  if(p1_simplex(a)) return false;
   const int blocks = (a.n_cells + kCells - 1) / kCells;
   if(a.dim == 3 && a.ncomp == 10) {
     switch(a.nl) {

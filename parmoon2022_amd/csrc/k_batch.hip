@@ -24,12 +24,17 @@ __device__ inline double chunk_sum(double v) {
   return r;
 }
 
-// partial[c] = sum over chunk c of x*y
+// partial[c] = sum over chunk c of x*y (0 for an inactive instance)
 __global__ __launch_bounds__(256) void k_seg_dot(const SegChunk* __restrict__ ch,
+                                                 const uint8_t* __restrict__ active,
                                                  const double* __restrict__ x,
                                                  const double* __restrict__ y,
                                                  double* __restrict__ partial) {
   const SegChunk c = ch[blockIdx.x];
+  if(active && !active[c.inst]) {
+    if(threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    return;
+  }
   double s = 0;
   for(int i = c.begin + threadIdx.x; i < c.end; i += 256) s += x[i] * y[i];
   s = chunk_sum(s);
@@ -39,12 +44,17 @@ __global__ __launch_bounds__(256) void k_seg_dot(const SegChunk* __restrict__ ch
 // w -= hprev[inst] * vprev (if vprev), then partial = chunk sum of w * vnext
 // (if vnext): one modified Gram-Schmidt step fused with the next projection
 __global__ __launch_bounds__(256) void k_seg_mgs(const SegChunk* __restrict__ ch,
+                                                 const uint8_t* __restrict__ active,
                                                  const double* __restrict__ hprev,
                                                  const double* __restrict__ vprev,
                                                  double* __restrict__ w,
                                                  const double* __restrict__ vnext,
                                                  double* __restrict__ partial) {
   const SegChunk c = ch[blockIdx.x];
+  if(active && !active[c.inst]) {
+    if(vnext && threadIdx.x == 0) partial[blockIdx.x] = 0.0;
+    return;
+  }
   const double h = vprev ? hprev[c.inst] : 0.0;
   double s = 0;
   for(int i = c.begin + threadIdx.x; i < c.end; i += 256) {
@@ -71,12 +81,17 @@ __global__ void k_seg_final(int n_inst, const int* __restrict__ inst_ptr,
   out[i] = s;
 }
 
-// y = a[inst] * x
+// y = a[inst] * x (0 for an inactive instance)
 __global__ __launch_bounds__(256) void k_seg_scale(const SegChunk* __restrict__ ch,
+                                                   const uint8_t* __restrict__ active,
                                                    const double* __restrict__ a,
                                                    const double* __restrict__ x,
                                                    double* __restrict__ y) {
   const SegChunk c = ch[blockIdx.x];
+  if(active && !active[c.inst]) {
+    for(int i = c.begin + threadIdx.x; i < c.end; i += 256) y[i] = 0.0;
+    return;
+  }
   const double s = a[c.inst];
   for(int i = c.begin + threadIdx.x; i < c.end; i += 256) y[i] = s * x[i];
 }
@@ -92,29 +107,89 @@ __global__ __launch_bounds__(256) void k_seg_axpy(const SegChunk* __restrict__ c
   for(int i = c.begin + threadIdx.x; i < c.end; i += 256) y[i] += s * x[i];
 }
 
+// Batched SpMV with the non-active rule: rows of inactive (converged)
+// instances are written as 0 without reading their matrix rows; the others
+// as in k_spmv_unrolled.
+template <int W, int U>
+__global__ __launch_bounds__(256) void k_spmv_masked(int n, int n_active, const int* __restrict__ rp,
+                                                     const int* __restrict__ col,
+                                                     const double* __restrict__ val,
+                                                     const double* __restrict__ x,
+                                                     const double* __restrict__ z, double* __restrict__ y,
+                                                     double alpha, double beta,
+                                                     const int* __restrict__ row_inst,
+                                                     const uint8_t* __restrict__ active) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int row = gt / W, l = gt % W;
+  if(row >= n) return;
+  if(!active[row_inst[row]]) {
+    if(l == 0) y[row] = 0.0;
+    return;
+  }
+  if(row < n_active) {
+    const int b = rp[row], e = rp[row + 1];
+    int cc[U];
+    double v[U], xv[U];
+#pragma unroll
+    for(int u = 0; u < U; ++u) {
+      const int k = b + l + u * W;
+      cc[u] = k < e ? col[k] : -1;
+    }
+#pragma unroll
+    for(int u = 0; u < U; ++u) {
+      const int k = b + l + u * W;
+      v[u] = k < e ? val[k] : 0.0;
+    }
+#pragma unroll
+    for(int u = 0; u < U; ++u) xv[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
+    double s = 0;
+#pragma unroll
+    for(int u = 0; u < U; ++u) s += v[u] * xv[u];
+#pragma unroll
+    for(int off = W / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, W);
+    if(l == 0) y[row] = (beta == 0.0 ? 0.0 : beta * z[row]) + alpha * s;
+  } else if(l == 0) {
+    y[row] = (beta == 0.0 ? 0.0 : beta * z[row]) + alpha * x[row];
+  }
+}
+
 }  // namespace
 
+void launch_spmv_masked(int n, int n_active, const int* rp, const int* col, const double* val,
+                        const double* x, const double* z, double* y, double alpha, double beta,
+                        int max_row, const int* row_inst, const uint8_t* active, hipStream_t s) {
+  if(n == 0) return;
+  constexpr int W = 8;
+  PMCD_CHECK(max_row <= 8 * W, "batched rows longer than ", 8 * W, " entries");
+  const int blocks = (int)(((long)n * W + 255) / 256);
+  if(max_row <= 2 * W) k_spmv_masked<W, 2><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta, row_inst, active);
+  else if(max_row <= 4 * W) k_spmv_masked<W, 4><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta, row_inst, active);
+  else k_spmv_masked<W, 8><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta, row_inst, active);
+  PMCD_HIP(hipGetLastError());
+}
+
 void launch_seg_dot(int n_chunks, const SegChunk* ch, int n_inst, const int* inst_ptr,
-                    const double* x, const double* y, double* partial, double* out, hipStream_t s) {
+                    const double* x, const double* y, double* partial, double* out, hipStream_t s,
+                    const uint8_t* active) {
   if(n_chunks == 0) return;
-  k_seg_dot<<<n_chunks, 256, 0, s>>>(ch, x, y, partial);
+  k_seg_dot<<<n_chunks, 256, 0, s>>>(ch, active, x, y, partial);
   k_seg_final<<<(n_inst + 63) / 64, 64, 0, s>>>(n_inst, inst_ptr, partial, out);
   PMCD_HIP(hipGetLastError());
 }
 
 void launch_seg_mgs(int n_chunks, const SegChunk* ch, int n_inst, const int* inst_ptr,
                     const double* hprev, const double* vprev, double* w, const double* vnext,
-                    double* partial, double* out, hipStream_t s) {
+                    double* partial, double* out, hipStream_t s, const uint8_t* active) {
   if(n_chunks == 0) return;
-  k_seg_mgs<<<n_chunks, 256, 0, s>>>(ch, hprev, vprev, w, vnext, partial);
+  k_seg_mgs<<<n_chunks, 256, 0, s>>>(ch, active, hprev, vprev, w, vnext, partial);
   if(vnext) k_seg_final<<<(n_inst + 63) / 64, 64, 0, s>>>(n_inst, inst_ptr, partial, out);
   PMCD_HIP(hipGetLastError());
 }
 
 void launch_seg_scale(int n_chunks, const SegChunk* ch, const double* a, const double* x, double* y,
-                      hipStream_t s) {
+                      hipStream_t s, const uint8_t* active) {
   if(n_chunks == 0) return;
-  k_seg_scale<<<n_chunks, 256, 0, s>>>(ch, a, x, y);
+  k_seg_scale<<<n_chunks, 256, 0, s>>>(ch, active, a, x, y);
   PMCD_HIP(hipGetLastError());
 }
 

@@ -303,16 +303,21 @@ __global__ __launch_bounds__(256) void k_prolong_add(int n_active, const int* __
 // x[row] += omega * s / diag[row], in place. Rows of one class are never
 // adjacent, so each reads only x entries that no other thread of the launch
 // writes.
+// A row whose owner (row_owner[r]) is flagged inactive is skipped (batched
+// solves: converged instances); row_owner null means every row is active.
 template <class T>
 __global__ __launch_bounds__(256) void k_sor_rows(int count, const int* __restrict__ rows,
                                                   const int* __restrict__ rp,
                                                   const int* __restrict__ col,
                                                   const T* __restrict__ val,
                                                   const T* __restrict__ diag,
-                                                  const T* __restrict__ b, T* x, T omega) {
+                                                  const T* __restrict__ b, T* x, T omega,
+                                                  const int* __restrict__ row_owner,
+                                                  const uint8_t* __restrict__ owner_active) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if(i >= count) return;
   const int r = rows[i];
+  if(row_owner && !owner_active[row_owner[r]]) return;
   T s = b[r];
   const int e = rp[r + 1];
   for(int k = rp[r]; k < e; ++k) s -= val[k] * x[col[k]];
@@ -328,11 +333,14 @@ __global__ __launch_bounds__(256) void k_sor_rows_lanes(int count, const int* __
                                                         const int* __restrict__ col,
                                                         const T* __restrict__ val,
                                                         const T* __restrict__ diag,
-                                                        const T* __restrict__ b, T* x, T omega) {
+                                                        const T* __restrict__ b, T* x, T omega,
+                                                        const int* __restrict__ row_owner,
+                                                        const uint8_t* __restrict__ owner_active) {
   const int gt = blockIdx.x * 256 + threadIdx.x;
   const int i = gt / W, l = gt % W;
   if(i >= count) return;
   const int r = rows[i];
+  if(row_owner && !owner_active[row_owner[r]]) return;
   const T s = transfer_row<W, U, T>(rp, col, val, x, r, l);
   if(l == 0) x[r] = x[r] + omega * (b[r] - s) / diag[r];
 }
@@ -702,7 +710,17 @@ static int g_jacobi_width = 16, g_jacobi_variant = 0;
 static int g_assembly_variant = 1;
 int assembly_variant() { return g_assembly_variant; }
 
+// SOR sweeps follow the exact level schedule when it has at most this many
+// classes, greedy multicolour order otherwise
+static int g_sor_exact_limit = 64;
+int sor_exact_limit() { return g_sor_exact_limit; }
+
 void set_tuning(int key, int value) {
+  if(key == 5) {
+    PMCD_CHECK(value >= 0, "the exact-sweep class limit must be non-negative");
+    g_sor_exact_limit = value;
+    return;
+  }
   if(key == 4) {
     PMCD_CHECK(value == 0 || value == 1, "assembly variant must be 0 or 1");
     g_assembly_variant = value;
@@ -728,6 +746,7 @@ int get_tuning(int key) {
     case 2: return g_spmv_variant;
     case 3: return g_jacobi_variant;
     case 4: return g_assembly_variant;
+    case 5: return g_sor_exact_limit;
     default: PMCD_FAIL("unknown tuning key ", key);
   }
 }
@@ -746,6 +765,16 @@ void launch_spmv(int n, int n_active, const int* rp, const int* col, const doubl
                  hipStream_t s, int max_row) {
   if(n == 0) return;
   dispatch_spmv<double>(g_spmv_width, g_spmv_variant, max_row, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_spmv_rowlen(int n, int n_active, const int* rp, const int* col, const double* val,
+                        const double* x, const double* z, double* y, double alpha, double beta,
+                        hipStream_t s, int max_row, double mean_row) {
+  if(n == 0) return;
+  const int W = mean_row <= 12 ? 4 : (mean_row <= 24 ? 8 : 16);
+  // the unrolled variant needs 8 or more lanes per row
+  dispatch_spmv<double>(W < 8 ? 8 : W, 2, max_row, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
   PMCD_HIP(hipGetLastError());
 }
 
@@ -860,34 +889,49 @@ void launch_prolong_add_f32(int n, int n_active, long nnz, int max_row, const in
 template <class T>
 static void dispatch_sor_rows(int count, const int* rows, const int* rp, const int* col, const T* val,
                               const T* diag, const T* b, T* x, T omega, bool serial, int max_row,
-                              hipStream_t s) {
+                              double mean_row, hipStream_t s, const int* row_owner,
+                              const uint8_t* owner_active) {
   if(count <= 0) return;
   if(serial) {
-    k_sor_rows<T><<<(count + 255) / 256, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega);
+    k_sor_rows<T><<<(count + 255) / 256, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega,
+                                                      row_owner, owner_active);
   } else {
-    constexpr int W = 16;
+    // lanes per row from the mean row length of the class
+    const double mean = mean_row;
+    const int W = mean <= 12 ? 4 : (mean <= 24 ? 8 : 16);
     const int u = unroll_for(max_row, W);
     const int blocks = (int)(((long)count * W + 255) / 256);
-    switch(u) {
-      case 2: k_sor_rows_lanes<W, 2, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega); break;
-      case 4: k_sor_rows_lanes<W, 4, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega); break;
-      case 8: k_sor_rows_lanes<W, 8, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega); break;
-      case 16: k_sor_rows_lanes<W, 16, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega); break;
-      default: k_sor_rows_lanes<W, 0, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega); break;
+#define PMCD_SOR_U(WW)                                                                                     \
+  switch(u) {                                                                                              \
+    case 2: k_sor_rows_lanes<WW, 2, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega, row_owner, owner_active); break; \
+    case 4: k_sor_rows_lanes<WW, 4, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega, row_owner, owner_active); break; \
+    case 8: k_sor_rows_lanes<WW, 8, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega, row_owner, owner_active); break; \
+    case 16: k_sor_rows_lanes<WW, 16, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega, row_owner, owner_active); break; \
+    default: k_sor_rows_lanes<WW, 0, T><<<blocks, 256, 0, s>>>(count, rows, rp, col, val, diag, b, x, omega, row_owner, owner_active); break; \
+  }
+    switch(W) {
+      case 4: PMCD_SOR_U(4); break;
+      case 8: PMCD_SOR_U(8); break;
+      default: PMCD_SOR_U(16); break;
     }
+#undef PMCD_SOR_U
   }
   PMCD_HIP(hipGetLastError());
 }
 
 void launch_sor_rows(int count, const int* rows, const int* rp, const int* col, const double* val,
                      const double* diag, const double* b, double* x, double omega, bool serial,
-                     int max_row, hipStream_t s) {
-  dispatch_sor_rows<double>(count, rows, rp, col, val, diag, b, x, omega, serial, max_row, s);
+                     int max_row, double mean_row, hipStream_t s, const int* row_owner,
+                     const uint8_t* owner_active) {
+  dispatch_sor_rows<double>(count, rows, rp, col, val, diag, b, x, omega, serial, max_row, mean_row, s,
+                            row_owner, owner_active);
 }
 void launch_sor_rows_f32(int count, const int* rows, const int* rp, const int* col, const float* val,
                          const float* diag, const float* b, float* x, float omega, bool serial,
-                         int max_row, hipStream_t s) {
-  dispatch_sor_rows<float>(count, rows, rp, col, val, diag, b, x, omega, serial, max_row, s);
+                         int max_row, double mean_row, hipStream_t s, const int* row_owner,
+                         const uint8_t* owner_active) {
+  dispatch_sor_rows<float>(count, rows, rp, col, val, diag, b, x, omega, serial, max_row, mean_row, s,
+                           row_owner, owner_active);
 }
 
 void launch_csr(int n, const int* rp, const int* col, const double* val, const double* x,

@@ -31,6 +31,8 @@ struct AsmArgs {
 __host__ __device__ inline const ExampleParams& cell_params(const AsmArgs& a, int cell) {
   return a.cell_inst ? a.cell_ex[a.cell_inst[cell]] : a.ex;
 }
+// P1 triangles/tetrahedra take a thread-per-cell kernel (same arithmetic)
+bool p1_simplex(const AsmArgs& a);
 void launch_assemble_local(const AsmArgs& a, hipStream_t s);
 // FP64-MFMA variant; returns false if the element is not covered
 bool launch_assemble_mfma(const AsmArgs& a, hipStream_t s);
@@ -63,6 +65,10 @@ void launch_spmv(int n, int n_active, const int* rp, const int* col, const doubl
 void launch_spmv_f32(int n, int n_active, const int* rp, const int* col, const float* val,
                      const float* x, const float* z, float* y, float alpha, float beta,
                      hipStream_t s, int max_row = -1);
+// same, launch shape from the row lengths (8 or 16 lanes, unrolled loads)
+void launch_spmv_rowlen(int n, int n_active, const int* rp, const int* col, const double* val,
+                        const double* x, const double* z, double* y, double alpha, double beta,
+                        hipStream_t s, int max_row, double mean_row);
 // damped Jacobi sweep x_out = x + omega * (b - A x) / diag (non-active rows use A = I)
 void launch_jacobi(int n, int n_active, const int* rp, const int* col, const double* val,
                    const double* inv_diag_unused, const double* diag, const double* b,
@@ -81,14 +87,20 @@ void set_tuning(int key, int value);
 int get_tuning(int key);
 // plain CSR product y = A x over all rows (transfer operators)
 // one class of an SOR sweep: rows[0..count) updated in place; serial: one
-// thread per row summing in column order (the reference's rounding), else 16
-// lanes per row with a butterfly sum (rows of at most max_row entries)
+// thread per row summing in column order (the reference's rounding), else
+// 4/8/16 lanes per row (from the mean row length) with a butterfly sum
+// (rows of at most max_row entries)
+// (row_owner, owner_active): optional per-row owner and per-owner flags; rows
+// of inactive owners are skipped
 void launch_sor_rows(int count, const int* rows, const int* rp, const int* col, const double* val,
                      const double* diag, const double* b, double* x, double omega, bool serial,
-                     int max_row, hipStream_t s);
+                     int max_row, double mean_row, hipStream_t s, const int* row_owner = nullptr,
+                     const uint8_t* owner_active = nullptr);
 void launch_sor_rows_f32(int count, const int* rows, const int* rp, const int* col, const float* val,
                          const float* diag, const float* b, float* x, float omega, bool serial,
-                         int max_row, hipStream_t s);
+                         int max_row, double mean_row, hipStream_t s, const int* row_owner = nullptr,
+                         const uint8_t* owner_active = nullptr);
+int sor_exact_limit();
 // restriction y = R x on active rows, 0 on rows >= n_active
 void launch_restrict(int n, int n_active, long nnz, int max_row, const int* rp, const int* col,
                      const double* val, const double* x, double* y, hipStream_t s);
@@ -156,16 +168,24 @@ struct SegChunk {
 };
 constexpr int kChunkRows = 4096;
 // out[i] = (x, y) restricted to instance i's rows (partial: one per chunk)
+// `active` (may be null): per-instance flags; chunks of inactive instances
+// contribute 0 and are not read
 void launch_seg_dot(int n_chunks, const SegChunk* ch, int n_inst, const int* inst_ptr,
-                    const double* x, const double* y, double* partial, double* out, hipStream_t s);
+                    const double* x, const double* y, double* partial, double* out, hipStream_t s,
+                    const uint8_t* active = nullptr);
 // w -= hprev[inst] vprev (vprev may be null), then out[i] = (w, vnext) per
 // instance (vnext may be null)
 void launch_seg_mgs(int n_chunks, const SegChunk* ch, int n_inst, const int* inst_ptr,
                     const double* hprev, const double* vprev, double* w, const double* vnext,
-                    double* partial, double* out, hipStream_t s);
+                    double* partial, double* out, hipStream_t s, const uint8_t* active = nullptr);
 // y = a[inst] x ; y += a[inst] x
 void launch_seg_scale(int n_chunks, const SegChunk* ch, const double* a, const double* x, double* y,
-                      hipStream_t s);
+                      hipStream_t s, const uint8_t* active = nullptr);
+// SpMV with the non-active rule over a batch: rows whose instance is inactive
+// are set to 0 without reading the matrix (rows of at most 64 entries)
+void launch_spmv_masked(int n, int n_active, const int* rp, const int* col, const double* val,
+                        const double* x, const double* z, double* y, double alpha, double beta,
+                        int max_row, const int* row_inst, const uint8_t* active, hipStream_t s);
 void launch_seg_axpy(int n_chunks, const SegChunk* ch, const double* a, const double* x, double* y,
                      hipStream_t s);
 

@@ -615,24 +615,25 @@ void CDSystem::sor_sweep32(Level& L, const float* b, float* x, float omega, int 
 
 void prepare_level_sweep(Level& L, hipStream_t stream) {
   if(!L.sweep_ptr.empty()) return;
-  // the level schedule is exact; beyond this many classes (deep dependency
-  // chains, e.g. Q2 in 3D) launches per class would dominate and the sweep
-  // falls back to greedy multicolour order
-  constexpr int kMaxExactClasses = 512;
-  SweepOrder o = build_sweep_order(L.st, kMaxExactClasses);
+  // the level schedule is exact; beyond sor_exact_limit() classes (deep
+  // dependency chains, e.g. 3D meshes) launches per class would dominate and
+  // the sweep uses greedy multicolour order
+  SweepOrder o = build_sweep_order(L.st, sor_exact_limit());
   L.sweep_ptr = std::move(o.class_ptr);
   L.sweep_exact = o.exact;
   L.sweep_rows.upload(o.rows, stream);
   PMCD_HIP(hipStreamSynchronize(stream));
 }
 
-void level_sor_sweep(Level& L, const double* b, double* x, double omega, int flag, hipStream_t stream) {
+void level_sor_sweep(Level& L, const double* b, double* x, double omega, int flag, hipStream_t stream,
+                     const int* row_owner, const uint8_t* owner_active) {
   PMCD_CHECK(flag >= 0 && flag <= 2, "sor_sweep flag must be 0, 1 or 2");
   prepare_level_sweep(L, stream);
   const int K = (int)L.sweep_ptr.size() - 1;
   auto cls = [&](int k) {
     launch_sor_rows(L.sweep_ptr[k + 1] - L.sweep_ptr[k], L.sweep_rows.p + L.sweep_ptr[k], L.row_ptr.p,
-                    L.col.p, L.val.p, L.diag.p, b, x, omega, L.sweep_exact, L.st.max_row, stream);
+                    L.col.p, L.val.p, L.diag.p, b, x, omega, L.sweep_exact, L.st.max_row,
+                    (double)L.st.nnz / std::max(1, L.n()), stream, row_owner, owner_active);
   };
   if(flag == 0 || flag == 2)
     for(int k = 0; k < K; ++k) cls(k);
@@ -641,7 +642,7 @@ void level_sor_sweep(Level& L, const double* b, double* x, double omega, int fla
 }
 
 void level_sor_sweep32(Level& L, const float* b, float* x, float omega, int flag,
-                       hipStream_t stream) {
+                       hipStream_t stream, const int* row_owner, const uint8_t* owner_active) {
   PMCD_CHECK(flag >= 0 && flag <= 2, "sor_sweep flag must be 0, 1 or 2");
   PMCD_CHECK(L.val32.n == L.val.n, "FP32 matrix copies not prepared");
   prepare_level_sweep(L, stream);
@@ -649,7 +650,8 @@ void level_sor_sweep32(Level& L, const float* b, float* x, float omega, int flag
   auto cls = [&](int k) {
     launch_sor_rows_f32(L.sweep_ptr[k + 1] - L.sweep_ptr[k], L.sweep_rows.p + L.sweep_ptr[k],
                         L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x, omega, L.sweep_exact,
-                        L.st.max_row, stream);
+                        L.st.max_row, (double)L.st.nnz / std::max(1, L.n()), stream, row_owner,
+                        owner_active);
   };
   if(flag == 0 || flag == 2)
     for(int k = 0; k < K; ++k) cls(k);

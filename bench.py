@@ -220,12 +220,79 @@ def bench_c4(args):
     print(json.dumps(result), flush=True)
 
 
+def bench_c5(args):
+    """Served stream: for each arrival rate, feed `--requests` requests at
+    their Poisson arrival times into the continuous-batching server and
+    measure per-request latency (arrival to result) and throughput."""
+    import threading
+    import numpy as np
+    import parmoon2022_amd.serve as sv
+    rates = [float(r) for r in args.rates.split(",")]
+    srv = sv.Server(max_slots=args.slots)
+    # resident systems for every mesh of the streams (set-up outside the clock)
+    all_dbs = []
+    for rate in rates:
+        all_dbs += sv.c5_requests(args.requests, rate)[1]
+    t0 = time.time()
+    srv.warm(all_dbs)
+    log("warm-up: %d meshes resident in %.1f s" % (len({sv.mesh_key(d) for d in all_dbs}), time.time() - t0))
+    sweep = []
+    for rate in rates:
+        offs, dbs = sv.c5_requests(args.requests, rate)
+        start = time.perf_counter() + 0.05
+        reqs = []
+
+        def feed():
+            for off, db in zip(offs, dbs):
+                delay = start + off - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                reqs.append(srv.submit(db, arrival=start + off))
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        for r in reqs:
+            r.done.wait()
+        errs = [r for r in reqs if r.error is not None]
+        lat = np.array([r.latency for r in reqs])
+        span = max(r.finished for r in reqs) - start
+        dof = 0
+        for r in reqs:
+            k = r.db["refinement_n_initial_steps"]
+            dof += (2 ** k + 1) ** 3 if r.db["geo_file"] == "Default_UnitCube_Hexa" else (2 ** (k + 1) + 1) ** 2
+        row = {"rate": rate, "solves_per_s": len(reqs) / span, "dof_per_s": dof / span,
+               "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+               "converged": int(sum(1 for r in reqs if r.stats is not None and r.stats.converged)),
+               "errors": len(errs)}
+        log("rate %g/s: %.1f solves/s, p50 %.1f ms, p99 %.1f ms, converged %d/%d" %
+            (rate, row["solves_per_s"], row["latency_ms"]["p50"], row["latency_ms"]["p99"], row["converged"],
+             len(reqs)))
+        sweep.append(row)
+    srv.close()
+    top = max(sweep, key=lambda r: r["solves_per_s"])
+    result = {
+        "metric": "CD3D solves/s & DoF/s at fixed L2 err vs FP64 oracle; p50/p99 solve latency",
+        "value": top["solves_per_s"], "unit": "solves/s", "n_gpus": 1, "steps": args.requests,
+        "warmup": 0, "ms_per_step": 1e3 / top["solves_per_s"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded request stream, generated meshes)",
+        "config": {"workload": "C5 served stream: Poisson arrivals, 50/50 CD2D P2 / CD3D Q1, DoF lognormal "
+                               "(median 2e5, 1.6e4-4e6), FGMRES(30) + MG V(2,2) SSOR, rel. res 1e-10, "
+                               "%d slots" % args.slots, "requests_per_rate": args.requests},
+        "latency_ms": top["latency_ms"], "sweep": sweep,
+    }
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4"],
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"],
                     help="c2: one 64^3 Q2 SUPG solve per step (default); c4: a batch of --instances P1 "
-                         "tetrahedral instances per step")
+                         "tetrahedral instances per step; c5: a served stream of mixed CD2D P2 / CD3D Q1 "
+                         "requests at each of --rates")
+    ap.add_argument("--rates", default="10,30,100,300,1000,2000", help="c5 arrival rates (requests/s)")
+    ap.add_argument("--requests", type=int, default=60, help="c5 requests per rate")
+    ap.add_argument("--slots", type=int, default=8, help="c5 concurrent requests")
     ap.add_argument("--instances", type=int, default=256, help="c4 batch size")
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
@@ -260,6 +327,8 @@ def main():
 
     if args.workload == "c4":
         return bench_c4(args)
+    if args.workload == "c5":
+        return bench_c5(args)
     db = c2_db(args)
     t0 = time.time()
     cd = pm.ConvectionDiffusion(db)

@@ -96,6 +96,9 @@ typedef struct {
 const char* pmcd_last_error(void);
 /* Number of visible GPUs. */
 int pmcd_device_count(int* n);
+/* Free and total device memory in bytes (hipMemGetInfo); no reference
+ * counterpart (admission by free-HBM budget in the request loop). */
+int pmcd_mem_info(unsigned long long* free_bytes, unsigned long long* total_bytes);
 
 /* Host-only setup (no GPU needed): mesh hierarchy, FE spaces and sparsity,
  * the part of ConvectionDiffusion<d>'s constructor that stays on the host:
@@ -126,6 +129,12 @@ int pmcd_host_sweep_order(pmcd_host* h, int level, int max_exact_classes, int* r
  * TFESpace2D/3D (FESpace3D.C:186-804), TStructure (Structure.C:226-618). */
 int pmcd_system_create(const pmcd_problem_desc* desc, pmcd_system** out);
 int pmcd_system_destroy(pmcd_system* s);
+/* Replace the PDE data (example, eps, b, c, f0) of a system, keeping its mesh
+ * hierarchy, spaces, sparsity and device buffers; the other fields of desc
+ * must equal the system's. Reassemble afterwards. Lets a request loop keep
+ * meshes, matrices and hierarchies resident across requests (the reference
+ * builds a new ConvectionDiffusion<d> per problem, ConvectionDiffusion.C:112). */
+int pmcd_system_set_data(pmcd_system* s, const pmcd_problem_desc* desc);
 
 /* info[0..7] = n_cells, n_dof, n_active, nnz, n_local, n_quad_points,
  * n_levels, n_dirichlet of level `level` (0 = coarsest, -1 = finest). */

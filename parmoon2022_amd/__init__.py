@@ -55,7 +55,7 @@ class SolveStats(ctypes.Structure):
 
 
 EXPORTS = [
-    "pmcd_last_error", "pmcd_device_count",
+    "pmcd_last_error", "pmcd_device_count", "pmcd_mem_info", "pmcd_system_set_data",
     "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
     "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_sweep_order",
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
@@ -68,6 +68,13 @@ EXPORTS = [
     "pmcd_batch_solve", "pmcd_batch_get_solution", "pmcd_batch_set_solution", "pmcd_batch_get_matrix",
     "pmcd_batch_get_rhs",
 ]
+
+
+def mem_info():
+    """(free, total) device memory in bytes."""
+    f, t = ctypes.c_ulonglong(0), ctypes.c_ulonglong(0)
+    _check(lib().pmcd_mem_info(ctypes.byref(f), ctypes.byref(t)))
+    return f.value, t.value
 
 
 def set_sor_exact_limit(n):
@@ -271,6 +278,14 @@ class ConvectionDiffusion:
         ms = ctypes.c_double()
         _check(lib().pmcd_assemble(self._s, ctypes.byref(ms)))
         self.assemble_ms = ms.value
+
+    def set_data(self, db):
+        """New PDE data (example, coefficients) on the same mesh hierarchy;
+        reassemble afterwards."""
+        desc = problem_desc(db)
+        _check(lib().pmcd_system_set_data(self._s, ctypes.byref(desc)))
+        self.db = dict(db)
+        self.desc = desc
 
     def solve(self, db=None):
         sd = solver_desc(self.db if db is None else db)

@@ -170,6 +170,29 @@ int pmcd_device_count(int* n) {
   return guarded([&] { PMCD_HIP(hipGetDeviceCount(n)); });
 }
 
+int pmcd_mem_info(unsigned long long* free_bytes, unsigned long long* total_bytes) {
+  return guarded([&] {
+    PMCD_CHECK(free_bytes && total_bytes, "null argument");
+    size_t f = 0, t = 0;
+    PMCD_HIP(hipMemGetInfo(&f, &t));
+    *free_bytes = f;
+    *total_bytes = t;
+  });
+}
+
+int pmcd_system_set_data(pmcd_system* s, const pmcd_problem_desc* desc) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->sys && desc, "null argument");
+    const ProblemDesc& cur = s->sys->desc();
+    const ProblemDesc d = to_desc(*desc);
+    PMCD_CHECK(d.geometry == cur.geometry && d.n_refine == cur.n_refine &&
+                   d.structured_n == cur.structured_n && d.order == cur.order && d.supg == cur.supg &&
+                   d.linear_quad == cur.linear_quad && d.mg_levels == cur.mg_levels,
+               "pmcd_system_set_data changes only the PDE data (example, eps, b, c, f0)");
+    s->sys->set_example(d.ex);
+  });
+}
+
 int pmcd_system_create(const pmcd_problem_desc* desc, pmcd_system** out) {
   return guarded([&] {
     PMCD_CHECK(desc && out, "null argument");

@@ -98,24 +98,11 @@ CDSystem::CDSystem(const ProblemDesc& d, hipStream_t stream) : desc_(d), stream_
 
 CDSystem::~CDSystem() { (void)hipStreamSynchronize(stream_); }
 
-void CDSystem::build_level_device(Level& L) {
+// Dirichlet values and Neumann terms of the current example on level L
+void CDSystem::boundary_data(Level& L) {
   const Mesh& m = L.mesh;
   const FESpace& sp = L.space;
-  const Structure& st = L.st;
   const ElementInfo& info = element_info(sp.element);
-  // packed vertex coordinates per cell
-  std::vector<double> xyz((size_t)m.n_cells * m.nv * 3);
-  for(int c = 0; c < m.n_cells; ++c)
-    for(int a = 0; a < m.nv; ++a)
-      for(int k = 0; k < 3; ++k) xyz[((size_t)c * m.nv + a) * 3 + k] = m.coords[3 * m.cell(c)[a] + k];
-  L.cell_xyz.upload(xyz, stream_);
-  L.dof.upload(sp.dof, stream_);
-  L.row_ptr.upload(st.row_ptr, stream_);
-  L.col.upload(st.col, stream_);
-  L.diag_pos.upload(st.diag, stream_);
-  L.contrib_ptr.upload(st.contrib_ptr, stream_);
-  L.contrib.upload(st.contrib, stream_);
-  L.slot.upload(st.slot, stream_);
   // Dirichlet values at the nodal points (Assemble3D.C:472-549, Assemble2D.C:3200-3230)
   const int n = sp.n_dof, na = sp.n_active, nl = sp.n_local;
   L.dirichlet_host.assign(n - na, 0.0);
@@ -180,8 +167,42 @@ void CDSystem::build_level_device(Level& L) {
       }
       L.extra_ptr.upload(ptr, stream_);
       L.extra.upload(flat, stream_);
+    } else {
+      L.extra_ptr.release();
+      L.extra.release();
     }
   }
+}
+
+// New PDE data (coefficients, source, boundary values) on the same mesh and
+// spaces: the device structures stay, the boundary data is recomputed; the
+// next assemble() uses the new data.
+void CDSystem::set_example(const ExampleParams& ex) {
+  desc_.ex = ex;
+  for(auto& L : levels_) boundary_data(*L);
+  PMCD_HIP(hipStreamSynchronize(stream_));
+}
+
+void CDSystem::build_level_device(Level& L) {
+  const Mesh& m = L.mesh;
+  const FESpace& sp = L.space;
+  const Structure& st = L.st;
+  const ElementInfo& info = element_info(sp.element);
+  // packed vertex coordinates per cell
+  std::vector<double> xyz((size_t)m.n_cells * m.nv * 3);
+  for(int c = 0; c < m.n_cells; ++c)
+    for(int a = 0; a < m.nv; ++a)
+      for(int k = 0; k < 3; ++k) xyz[((size_t)c * m.nv + a) * 3 + k] = m.coords[3 * m.cell(c)[a] + k];
+  L.cell_xyz.upload(xyz, stream_);
+  L.dof.upload(sp.dof, stream_);
+  L.row_ptr.upload(st.row_ptr, stream_);
+  L.col.upload(st.col, stream_);
+  L.diag_pos.upload(st.diag, stream_);
+  L.contrib_ptr.upload(st.contrib_ptr, stream_);
+  L.contrib.upload(st.contrib, stream_);
+  L.slot.upload(st.slot, stream_);
+  boundary_data(L);
+  const int n = sp.n_dof;
   L.val.alloc(st.nnz);
   L.rhs.alloc(n);
   L.sol.alloc(n);

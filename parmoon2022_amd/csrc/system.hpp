@@ -133,9 +133,12 @@ class CDSystem {
   void sor_sweep(Level& L, const double* b, double* x, double omega, int flag);
   void sor_sweep32(Level& L, const float* b, float* x, float omega, int flag);
   void prepare_sweep(Level& L);
+  // same mesh, new coefficients / source / boundary values
+  void set_example(const ExampleParams& ex);
 
  private:
   void build_level_device(Level& L);
+  void boundary_data(Level& L);
   void build_transfer(Level& fine, const Level& coarse);
   void assemble_level(Level& L);
   void coarse_factorize();

@@ -530,3 +530,40 @@ def test_batch_solve_matches_single_and_oracle(oracle_lib, mixed):
             assert abs(stats[i].iterations - st.iterations) <= 1
         cd.close()
     bt.close()
+
+
+def _serve_dbs():
+    import parmoon2022_amd.serve as sv
+    _, dbs = sv.c5_requests(8, 100.0, seed=11)
+    out = []
+    for i, db in enumerate(dbs):
+        db = dict(db)
+        db["refinement_n_initial_steps"] = 3 + (i % 2)  # small meshes
+        db["multigrid_n_levels"] = 2
+        out.append(db)
+    return out
+
+
+@pytest.mark.parametrize("slots,budget", [(4, None), (3, 1)], ids=["4slots", "budget-serialised"])
+def test_served_results_same_alone_and_in_the_mix(slots, budget):
+    """Continuous batching: requests run concurrently on their own streams,
+    reusing resident systems of earlier requests on the same mesh; every
+    result equals the one computed alone on a fresh system, bit for bit. With
+    a one-byte budget requests are admitted one at a time and still finish."""
+    import parmoon2022_amd.serve as sv
+    dbs = _serve_dbs()
+    alone = []
+    for db in dbs:
+        cd = pm.ConvectionDiffusion(db)
+        cd.assemble()
+        st = cd.solve()
+        alone.append((cd.solution(), st.iterations))
+        cd.close()
+    srv = sv.Server(max_slots=slots, budget=budget, keep_solution=True)
+    reqs = [srv.submit(db) for db in dbs]
+    for r in reqs:
+        assert r.done.wait(120), "request %d did not finish" % r.id
+    srv.close()
+    for r, (u, its) in zip(reqs, alone):
+        assert r.error is None, r.error
+        assert r.stats.iterations == its and np.array_equal(r.solution, u), r.id

@@ -234,7 +234,7 @@ def bench_c5(args):
     for rate in rates:
         all_dbs += sv.c5_requests(args.requests, rate)[1]
     t0 = time.time()
-    srv.warm(all_dbs)
+    srv.warm(all_dbs, per_mesh=args.slots)
     log("warm-up: %d meshes resident in %.1f s" % (len({sv.mesh_key(d) for d in all_dbs}), time.time() - t0))
     sweep = []
     for rate in rates:

@@ -276,7 +276,7 @@ def bench_c5(args):
         "warmup": 0, "ms_per_step": 1e3 / top["solves_per_s"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded request stream, generated meshes)",
         "config": {"workload": "C5 served stream: Poisson arrivals, 50/50 CD2D P2 / CD3D Q1, DoF lognormal "
-                               "(median 2e5, 1.6e4-4e6), FGMRES(30) + MG V(2,2) SSOR, rel. res 1e-10, "
+                               "(median 2e5, 1.6e4-4e6), FGMRES(30) + MG V(2,2) SOR, rel. res 1e-10, "
                                "%d slots" % args.slots, "requests_per_rate": args.requests},
         "latency_ms": top["latency_ms"], "sweep": sweep,
     }

@@ -196,7 +196,7 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
  * XCD-contiguous rows, 2 unrolled with all loads of a lane in flight at once,
  * taken where the level's longest row fits 16 entries per lane); key 5 = the
  * largest number of level-schedule classes for which SOR sweeps follow the
- * exact serial order (default 64; deeper schedules use multicolour order),
+ * exact serial order (default 24; deeper schedules use multicolour order),
  * read when a level's sweep order is first built; key 4 = assembly kernel (0 reference operation order on
  * vector ALUs, 1 FP64 MFMA). */
 int pmcd_set_tuning(int key, int value);

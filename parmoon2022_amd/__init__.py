@@ -80,7 +80,7 @@ def mem_info():
 def set_sor_exact_limit(n):
     """SOR sweeps of systems created afterwards follow the exact level
     schedule (bit-identical to the serial sweep) when it has at most n
-    classes, greedy multicolour order otherwise (default 64)."""
+    classes, greedy multicolour order otherwise (default 24)."""
     _check(lib().pmcd_set_tuning(5, int(n)))
 
 

@@ -712,7 +712,7 @@ int assembly_variant() { return g_assembly_variant; }
 
 // SOR sweeps follow the exact level schedule when it has at most this many
 // classes, greedy multicolour order otherwise
-static int g_sor_exact_limit = 64;
+static int g_sor_exact_limit = 24;
 int sor_exact_limit() { return g_sor_exact_limit; }
 
 void set_tuning(int key, int value) {

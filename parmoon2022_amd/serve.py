@@ -265,7 +265,7 @@ def c5_requests(n_req, rate, seed=7, solver=None):
               # hierarchy down to 33^2 (2D P2) or 9^3 (3D Q1) DoF: the dense
               # coarse solve stays small
               "multigrid_n_levels": k - 2 if three_d else k - 3,
-              "multigrid_smoother": "ssor", "sor_omega": 1.0, "multigrid_n_pre_smooth": 2,
+              "multigrid_smoother": "sor", "sor_omega": 1.0, "multigrid_n_pre_smooth": 2,
               "multigrid_n_post_smooth": 2, "gmres_restart": 30, "residual_reduction": 1e-10,
               "residual_tolerance": 0.0, "max_n_iterations": 500}
         db.update(solver)

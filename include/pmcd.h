@@ -187,14 +187,16 @@ int pmcd_sor_sweep(pmcd_system* s, int level, const double* b, double* x, double
  * back-to-back launches of one kernel on level `level`:
  * 0 FP64 SpMV, 1 FP64 Jacobi sweep, 2 FP32 SpMV, 3 local-matrix assembly,
  * 4 row gather, 5 FP32 Jacobi sweep, 6 streaming copy of the level's matrix
- * values (HBM calibration). No reference counterpart (measurement). */
+ * values (HBM calibration), 7 the same copy with four loads in flight per
+ * thread. No reference counterpart (measurement). */
 int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* avg_ms);
 
 /* Launch-shape tuning (process-wide; no reference counterpart, knobs of the
  * autotuning sweep): key 0 / 1 = lanes per matrix row of the SpMV / Jacobi
  * kernels (4, 8, 16, 32); key 2 / 3 = their variant (0 plain, 1 streaming with
  * XCD-contiguous rows, 2 unrolled with all loads of a lane in flight at once,
- * taken where the level's longest row fits 16 entries per lane); key 5 = the
+ * taken where the level's longest row fits 16 entries per lane, 3 unrolled
+ * with non-temporal matrix loads); key 5 = the
  * largest number of level-schedule classes for which SOR sweeps follow the
  * exact serial order (default 24; deeper schedules use multicolour order),
  * read when a level's sweep order is first built; key 4 = assembly kernel (0 reference operation order on

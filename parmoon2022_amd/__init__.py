@@ -105,7 +105,8 @@ def set_tuning(kernel, lanes_per_row, variant=0):
     """Launch shape of the 'spmv' or 'jacobi' CSR kernels: lanes per row (4, 8,
     16, 32) and variant (0 plain, 1 streaming with XCD-contiguous rows, 2
     unrolled: all loads of a lane issued before the products, used where the
-    longest row fits 16 entries per lane)."""
+    longest row fits 16 entries per lane, 3 unrolled with non-temporal matrix
+    loads)."""
     k = {"spmv": 0, "jacobi": 1}[kernel]
     _check(lib().pmcd_set_tuning(k, int(lanes_per_row)))
     _check(lib().pmcd_set_tuning(k + 2, int(variant)))
@@ -122,7 +123,7 @@ def tuning(kernel):
 
 
 KERNELS = {"spmv_f64": 0, "jacobi_f64": 1, "spmv_f32": 2, "assemble_local": 3, "gather_rows": 4,
-           "jacobi_f32": 5, "stream_copy": 6}
+           "jacobi_f32": 5, "stream_copy": 6, "stream_copy4": 7}
 
 
 def lib():
@@ -462,11 +463,11 @@ def autotune(cd, level=-1, iters=10, log=None):
     """Pick lanes-per-row of the SpMV and Jacobi kernels for this matrix by
     timing each candidate with HIP events; sets the fastest and returns the
     timings {kernel: {lanes: ms}}."""
-    names = ("plain", "stream", "unrolled")
+    names = ("plain", "stream", "unrolled", "unrollednt")
     out = {}
     for kernel, timed in (("spmv", "spmv_f64"), ("jacobi", "jacobi_f64")):
         res = {}
-        for variant in (0, 1, 2):
+        for variant in (0, 1, 2, 3):
             for w in (8, 16, 32):
                 set_tuning(kernel, w, variant)
                 res[(names[variant], w)] = cd.time_kernel(timed, level, iters)

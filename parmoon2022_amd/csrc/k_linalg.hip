@@ -175,7 +175,43 @@ __device__ inline T row_sum_unrolled(const int* __restrict__ rp, const int* __re
   return s;
 }
 
+// Same with non-temporal matrix loads (the CSR arrays stream through once
+// per pass; x keeps the cache)
 template <int W, int U, class T>
+__device__ inline T row_sum_unrolled_nt(const int* __restrict__ rp, const int* __restrict__ col,
+                                        const T* __restrict__ val, const T* __restrict__ x, int row,
+                                        int l) {
+  const int b = rp[row], e = rp[row + 1];
+  int c[U];
+  T v[U], xv[U];
+#pragma unroll
+  for(int u = 0; u < U; ++u) {
+    const int k = b + l + u * W;
+    c[u] = k < e ? __builtin_nontemporal_load(&col[k]) : -1;
+  }
+#pragma unroll
+  for(int u = 0; u < U; ++u) {
+    const int k = b + l + u * W;
+    v[u] = k < e ? __builtin_nontemporal_load(&val[k]) : T(0);
+  }
+#pragma unroll
+  for(int u = 0; u < U; ++u) xv[u] = c[u] >= 0 ? x[c[u]] : T(0);
+  T s = 0;
+#pragma unroll
+  for(int u = 0; u < U; ++u) s += v[u] * xv[u];
+#pragma unroll
+  for(int off = W / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, W);
+  return s;
+}
+
+template <int W, int U, class T, bool NT>
+__device__ inline T row_sum_u(const int* __restrict__ rp, const int* __restrict__ col,
+                              const T* __restrict__ val, const T* __restrict__ x, int row, int l) {
+  if constexpr(NT) return row_sum_unrolled_nt<W, U, T>(rp, col, val, x, row, l);
+  else return row_sum_unrolled<W, U, T>(rp, col, val, x, row, l);
+}
+
+template <int W, int U, class T, bool NT = false>
 __global__ __launch_bounds__(256) void k_spmv_unrolled(int n, int n_active, const int* __restrict__ rp,
                                                        const int* __restrict__ col,
                                                        const T* __restrict__ val,
@@ -186,14 +222,14 @@ __global__ __launch_bounds__(256) void k_spmv_unrolled(int n, int n_active, cons
   const int row = gt / W, l = gt % W;
   if(row >= n) return;
   if(row < n_active) {
-    const T s = row_sum_unrolled<W, U, T>(rp, col, val, x, row, l);
+    const T s = row_sum_u<W, U, T, NT>(rp, col, val, x, row, l);
     if(l == 0) y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * s;
   } else if(l == 0) {
     y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * x[row];
   }
 }
 
-template <int W, int U, class T>
+template <int W, int U, class T, bool NT = false>
 __global__ __launch_bounds__(256) void k_jacobi_unrolled(int n, int n_active,
                                                          const int* __restrict__ rp,
                                                          const int* __restrict__ col,
@@ -206,7 +242,7 @@ __global__ __launch_bounds__(256) void k_jacobi_unrolled(int n, int n_active,
   const int row = gt / W, l = gt % W;
   if(row >= n) return;
   T s;
-  if(row < n_active) s = row_sum_unrolled<W, U, T>(rp, col, val, x, row, l);
+  if(row < n_active) s = row_sum_u<W, U, T, NT>(rp, col, val, x, row, l);
   else s = x[row];
   if(l == 0) {
     T d = b[row] + T(-1) * s;
@@ -441,6 +477,18 @@ __global__ __launch_bounds__(256) void k_stream_copy(long n2, const double2* __r
   for(long i = (long)blockIdx.x * 256 + threadIdx.x; i < n2; i += (long)gridDim.x * 256) y[i] = x[i];
 }
 
+// four independent 16-byte loads in flight per thread
+__global__ __launch_bounds__(256) void k_stream_copy4(long n2, const double2* __restrict__ x,
+                                                      double2* __restrict__ y) {
+  const long stride = (long)gridDim.x * 256;
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  for(; i + 3 * stride < n2; i += 4 * stride) {
+    const double2 a = x[i], b = x[i + stride], c = x[i + 2 * stride], d = x[i + 3 * stride];
+    y[i] = a; y[i + stride] = b; y[i + 2 * stride] = c; y[i + 3 * stride] = d;
+  }
+  for(; i < n2; i += stride) y[i] = x[i];
+}
+
 __global__ void k_copy(int n, const double* __restrict__ x, double* __restrict__ y) {
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = x[i];
 }
@@ -512,26 +560,26 @@ inline int unroll_for(int max_row, int W) {
   return 0;
 }
 
-template <int W, class T>
+template <int W, class T, bool NT = false>
 void spmv_unrolled(int u, int blocks, int n, int n_active, const int* rp, const int* col,
                    const T* val, const T* x, const T* z, T* y, T alpha, T beta, hipStream_t s) {
   switch(u) {
-    case 2: k_spmv_unrolled<W, 2, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
-    case 4: k_spmv_unrolled<W, 4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
-    case 8: k_spmv_unrolled<W, 8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
-    default: k_spmv_unrolled<W, 16, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    case 2: k_spmv_unrolled<W, 2, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    case 4: k_spmv_unrolled<W, 4, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    case 8: k_spmv_unrolled<W, 8, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    default: k_spmv_unrolled<W, 16, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
   }
 }
 
-template <int W, class T>
+template <int W, class T, bool NT = false>
 void jacobi_unrolled(int u, int blocks, int n, int n_active, const int* rp, const int* col,
                      const T* val, const T* diag, const T* b, const T* x, T* x_out, T omega,
                      hipStream_t s) {
   switch(u) {
-    case 2: k_jacobi_unrolled<W, 2, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
-    case 4: k_jacobi_unrolled<W, 4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
-    case 8: k_jacobi_unrolled<W, 8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
-    default: k_jacobi_unrolled<W, 16, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    case 2: k_jacobi_unrolled<W, 2, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    case 4: k_jacobi_unrolled<W, 4, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    case 8: k_jacobi_unrolled<W, 8, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    default: k_jacobi_unrolled<W, 16, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
   }
 }
 
@@ -547,6 +595,14 @@ void dispatch_spmv(int width, int variant, int max_row, int n, int n_active, con
       case 8: spmv_unrolled<8, T>(u, blocks, n, n_active, rp, col, val, x, z, y, alpha, beta, s); break;
       case 16: spmv_unrolled<16, T>(u, blocks, n, n_active, rp, col, val, x, z, y, alpha, beta, s); break;
       default: spmv_unrolled<32, T>(u, blocks, n, n_active, rp, col, val, x, z, y, alpha, beta, s); break;
+    }
+    return;
+  }
+  if(variant == 3 && u) {
+    switch(width) {
+      case 8: spmv_unrolled<8, T, true>(u, blocks, n, n_active, rp, col, val, x, z, y, alpha, beta, s); break;
+      case 16: spmv_unrolled<16, T, true>(u, blocks, n, n_active, rp, col, val, x, z, y, alpha, beta, s); break;
+      default: spmv_unrolled<32, T, true>(u, blocks, n, n_active, rp, col, val, x, z, y, alpha, beta, s); break;
     }
     return;
   }
@@ -729,7 +785,7 @@ void set_tuning(int key, int value) {
   if(key == 0 || key == 1)
     PMCD_CHECK(value == 4 || value == 8 || value == 16 || value == 32, "lanes per row must be 4, 8, 16 or 32");
   else
-    PMCD_CHECK(value >= 0 && value <= 2, "variant must be 0, 1 or 2");
+    PMCD_CHECK(value >= 0 && value <= 3, "variant must be 0, 1, 2 or 3");
   switch(key) {
     case 0: g_spmv_width = value; break;
     case 1: g_jacobi_width = value; break;
@@ -797,6 +853,14 @@ static void dispatch_jacobi(int width, int max_row, int n, int n_active, const i
       case 8: jacobi_unrolled<8, T>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
       case 16: jacobi_unrolled<16, T>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
       default: jacobi_unrolled<32, T>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
+    }
+    return;
+  }
+  if(g_jacobi_variant == 3 && u) {
+    switch(width) {
+      case 8: jacobi_unrolled<8, T, true>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
+      case 16: jacobi_unrolled<16, T, true>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
+      default: jacobi_unrolled<32, T, true>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
     }
     return;
   }
@@ -992,6 +1056,13 @@ void launch_stream_copy(long n, const double* x, double* y, hipStream_t s) {
   PMCD_CHECK(n % 2 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)y % 16) == 0,
              "stream copy needs an even count of 16-byte aligned doubles");
   if(n) k_stream_copy<<<2048, 256, 0, s>>>(n / 2, (const double2*)x, (double2*)y);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_stream_copy4(long n, const double* x, double* y, hipStream_t s) {
+  PMCD_CHECK(n % 2 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)y % 16) == 0,
+             "stream copy needs an even count of 16-byte aligned doubles");
+  if(n) k_stream_copy4<<<4096, 256, 0, s>>>(n / 2, (const double2*)x, (double2*)y);
   PMCD_HIP(hipGetLastError());
 }
 

@@ -394,7 +394,7 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
     launch_fill_f32(n, 0.5f, bf.p, stream_);
   }
   DevBuf<double> copy_dst;
-  if(which == 6) copy_dst.alloc(L.val.n & ~(size_t)1);
+  if(which == 6 || which == 7) copy_dst.alloc(L.val.n & ~(size_t)1);
   AsmArgs a = asm_args(L);
   auto launch = [&]() {
     switch(which) {
@@ -410,6 +410,7 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
         break;
       case 5: launch_jacobi_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, bf.p, xf.p, yf.p, 0.8f, stream_, L.st.max_row); break;
       case 6: launch_stream_copy((long)copy_dst.n, L.val.p, copy_dst.p, stream_); break;
+      case 7: launch_stream_copy4((long)copy_dst.n, L.val.p, copy_dst.p, stream_); break;
       default: PMCD_FAIL("unknown kernel id ", which);
     }
   };

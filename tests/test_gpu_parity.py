@@ -93,7 +93,7 @@ def test_assembly_matches_oracle(oracle_lib, case, variant):
 
 
 @pytest.mark.parametrize("shape", [(8, 0), (16, 0), (32, 0), (8, 1), (16, 1), (32, 1),
-                                   (8, 2), (16, 2), (32, 2)],
+                                   (8, 2), (16, 2), (32, 2), (8, 3), (16, 3)],
                          ids=lambda s: "w%d-v%d" % s)
 @pytest.mark.parametrize("case", ASSEMBLY_CASES[:8:3], ids=lambda c: "%s-p%d" % c[:2])
 def test_spmv_matches_serial_sum(oracle_lib, case, shape):
@@ -135,15 +135,16 @@ def test_unrolled_csr_bit_identical(width):
     x = rng.standard_normal(cd.info()["n_dof"])
     y0 = rng.standard_normal(cd.info()["n_dof"])
     out = []
-    for variant in (0, 2):
+    for variant in (0, 2, 3):
         pm.set_tuning("spmv", width, variant)
         out.append((cd.apply(x), cd.apply(x, y=y0, a=-0.7)))
     pm.set_tuning("spmv", 16, 0)
-    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    for k in (1, 2):
+        assert np.array_equal(out[0][0], out[k][0]) and np.array_equal(out[0][1], out[k][1])
     cd.close()
 
 
-@pytest.mark.parametrize("shape", [(8, 0), (32, 0), (8, 1), (16, 1), (32, 1), (8, 2), (16, 2), (32, 2)],
+@pytest.mark.parametrize("shape", [(8, 0), (32, 0), (8, 1), (16, 1), (32, 1), (8, 2), (16, 2), (32, 2), (8, 3)],
                          ids=lambda s: "w%d-v%d" % s)
 def test_jacobi_shapes_same_solve(oracle_lib, shape):
     """The multigrid solve is independent of the Jacobi launch shape up to

@@ -106,6 +106,7 @@ def kernel_times(cd, info, mixed):
     t_jac = cd.time_kernel("jacobi_f32" if mixed else "jacobi_f64", -1, 20)
     t_spmv = cd.time_kernel("spmv_f64", -1, 20)
     t_copy = min(cd.time_kernel("stream_copy", -1, 20), cd.time_kernel("stream_copy4", -1, 20))
+    t_read = cd.time_kernel("stream_read", -1, 20)
     t_asm = cd.time_kernel("assemble_local", -1, 3)
     # Jacobi reads val, col, row_ptr, x (gathered), b, diag, x (own row) and
     # writes x_out: (vb + 4) nnz + 4 (n + 1) + 5 vb n bytes
@@ -121,6 +122,7 @@ def kernel_times(cd, info, mixed):
         "spmv_f64_ms": t_spmv, "spmv_f64_GBs": spmv_bytes / (t_spmv * 1e-3) / 1e9,
         "stream_copy_ms": t_copy, "stream_copy_bytes": copy_bytes,  # the faster of the two copies
         "stream_copy_GBs": copy_bytes / (t_copy * 1e-3) / 1e9,
+        "stream_read_GBs": copy_bytes / 2 / (t_read * 1e-3) / 1e9,
         "assemble_local_ms": t_asm,
         "assemble_local_TFLOPs": asm_flops / (t_asm * 1e-3) / 1e12,
         "assemble_fp64_frac": asm_flops / (t_asm * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,

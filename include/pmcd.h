@@ -188,7 +188,8 @@ int pmcd_sor_sweep(pmcd_system* s, int level, const double* b, double* x, double
  * 0 FP64 SpMV, 1 FP64 Jacobi sweep, 2 FP32 SpMV, 3 local-matrix assembly,
  * 4 row gather, 5 FP32 Jacobi sweep, 6 streaming copy of the level's matrix
  * values (HBM calibration), 7 the same copy with four loads in flight per
- * thread. No reference counterpart (measurement). */
+ * thread, 8 a read-only stream over the values. No reference counterpart
+ * (measurement). */
 int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* avg_ms);
 
 /* Launch-shape tuning (process-wide; no reference counterpart, knobs of the

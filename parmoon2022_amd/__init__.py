@@ -123,7 +123,7 @@ def tuning(kernel):
 
 
 KERNELS = {"spmv_f64": 0, "jacobi_f64": 1, "spmv_f32": 2, "assemble_local": 3, "gather_rows": 4,
-           "jacobi_f32": 5, "stream_copy": 6, "stream_copy4": 7}
+           "jacobi_f32": 5, "stream_copy": 6, "stream_copy4": 7, "stream_read": 8}
 
 
 def lib():

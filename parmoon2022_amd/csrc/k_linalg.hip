@@ -489,6 +489,24 @@ __global__ __launch_bounds__(256) void k_stream_copy4(long n2, const double2* __
   for(; i < n2; i += stride) y[i] = x[i];
 }
 
+// read-only stream (sum of 16-byte vectors, four loads in flight per thread):
+// the HBM read ceiling for kernels that mostly read
+__global__ __launch_bounds__(256) void k_stream_read(long n2, const double2* __restrict__ x,
+                                                     double* __restrict__ out) {
+  const long stride = (long)gridDim.x * 256;
+  long i = (long)blockIdx.x * 256 + threadIdx.x;
+  double s0 = 0, s1 = 0;
+  for(; i + 3 * stride < n2; i += 4 * stride) {
+    const double2 a = x[i], b = x[i + stride], c = x[i + 2 * stride], d = x[i + 3 * stride];
+    s0 += (a.x + b.x) + (c.x + d.x);
+    s1 += (a.y + b.y) + (c.y + d.y);
+  }
+  for(; i < n2; i += stride) { s0 += x[i].x; s1 += x[i].y; }
+  s0 += s1;
+  for(int off = 32; off > 0; off >>= 1) s0 += __shfl_xor(s0, off, 64);
+  if((threadIdx.x & 63) == 0) out[(blockIdx.x * 256 + threadIdx.x) / 64] = s0;
+}
+
 __global__ void k_copy(int n, const double* __restrict__ x, double* __restrict__ y) {
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = x[i];
 }
@@ -1063,6 +1081,12 @@ void launch_stream_copy4(long n, const double* x, double* y, hipStream_t s) {
   PMCD_CHECK(n % 2 == 0 && ((uintptr_t)x % 16) == 0 && ((uintptr_t)y % 16) == 0,
              "stream copy needs an even count of 16-byte aligned doubles");
   if(n) k_stream_copy4<<<4096, 256, 0, s>>>(n / 2, (const double2*)x, (double2*)y);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_stream_read(long n, const double* x, double* out, hipStream_t s) {
+  PMCD_CHECK(n % 2 == 0 && ((uintptr_t)x % 16) == 0, "stream read needs an even count of aligned doubles");
+  if(n) k_stream_read<<<8192, 256, 0, s>>>(n / 2, (const double2*)x, out);
   PMCD_HIP(hipGetLastError());
 }
 

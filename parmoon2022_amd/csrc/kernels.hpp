@@ -135,6 +135,8 @@ void launch_copy(int n, const double* x, double* y, hipStream_t s);
 // y = x for n doubles (n even, 16-byte aligned), 16-byte vector accesses
 void launch_stream_copy(long n, const double* x, double* y, hipStream_t s);
 void launch_stream_copy4(long n, const double* x, double* y, hipStream_t s);
+// out[w] = partial sums of x per wave (8192 x 4 entries)
+void launch_stream_read(long n, const double* x, double* out, hipStream_t s);
 // fused MGS step: r -= h_k v_k; h_{k+1} = <r, v_{k+1}> partials (v_next may be null)
 void launch_mgs_step(int n, const double* h_dev, double* r, const double* v,
                      const double* v_next, double* partial, double* out_next, hipStream_t s);

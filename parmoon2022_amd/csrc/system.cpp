@@ -395,6 +395,7 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
   }
   DevBuf<double> copy_dst;
   if(which == 6 || which == 7) copy_dst.alloc(L.val.n & ~(size_t)1);
+  if(which == 8) copy_dst.alloc(8192 * 4);
   AsmArgs a = asm_args(L);
   auto launch = [&]() {
     switch(which) {
@@ -411,6 +412,7 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
       case 5: launch_jacobi_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, bf.p, xf.p, yf.p, 0.8f, stream_, L.st.max_row); break;
       case 6: launch_stream_copy((long)copy_dst.n, L.val.p, copy_dst.p, stream_); break;
       case 7: launch_stream_copy4((long)copy_dst.n, L.val.p, copy_dst.p, stream_); break;
+      case 8: launch_stream_read((long)(L.val.n & ~(size_t)1), L.val.p, copy_dst.p, stream_); break;
       default: PMCD_FAIL("unknown kernel id ", which);
     }
   };

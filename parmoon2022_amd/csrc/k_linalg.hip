@@ -241,13 +241,21 @@ __global__ __launch_bounds__(256) void k_jacobi_unrolled(int n, int n_active,
   const int gt = blockIdx.x * 256 + threadIdx.x;
   const int row = gt / W, l = gt % W;
   if(row >= n) return;
+  // the row's own b, diag and x are loaded before the row sum so that their
+  // latency overlaps the matrix loads
+  T bb = 0, dd = 1, xr = 0;
+  if(l == 0) {
+    bb = b[row];
+    dd = diag[row];
+    xr = x[row];
+  }
   T s;
   if(row < n_active) s = row_sum_u<W, U, T, NT>(rp, col, val, x, row, l);
-  else s = x[row];
+  else s = xr;
   if(l == 0) {
-    T d = b[row] + T(-1) * s;
-    d = d / diag[row];
-    x_out[row] = x[row] + omega * d;
+    T d = bb + T(-1) * s;
+    d = d / dd;
+    x_out[row] = xr + omega * d;
   }
 }
 

@@ -197,8 +197,7 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
  * kernels (4, 8, 16, 32); key 2 / 3 = their variant (0 plain, 1 streaming with
  * XCD-contiguous rows, 2 unrolled with all loads of a lane in flight at once,
  * taken where the level's longest row fits 16 entries per lane, 3 unrolled
- * with non-temporal matrix loads, 4 unrolled reading 16-bit column offsets
- * where a row's active and Dirichlet column spans are below 65536); key 5 = the
+ * with non-temporal matrix loads); key 5 = the
  * largest number of level-schedule classes for which SOR sweeps follow the
  * exact serial order (default 24; deeper schedules use multicolour order),
  * read when a level's sweep order is first built; key 4 = assembly kernel (0 reference operation order on

@@ -106,8 +106,7 @@ def set_tuning(kernel, lanes_per_row, variant=0):
     16, 32) and variant (0 plain, 1 streaming with XCD-contiguous rows, 2
     unrolled: all loads of a lane issued before the products, used where the
     longest row fits 16 entries per lane, 3 unrolled with non-temporal matrix
-    loads, 4 unrolled reading 16-bit column offsets where a row's column span
-    allows)."""
+    loads)."""
     k = {"spmv": 0, "jacobi": 1}[kernel]
     _check(lib().pmcd_set_tuning(k, int(lanes_per_row)))
     _check(lib().pmcd_set_tuning(k + 2, int(variant)))
@@ -464,11 +463,11 @@ def autotune(cd, level=-1, iters=10, log=None):
     """Pick lanes-per-row of the SpMV and Jacobi kernels for this matrix by
     timing each candidate with HIP events; sets the fastest and returns the
     timings {kernel: {lanes: ms}}."""
-    names = ("plain", "stream", "unrolled", "unrollednt", "idx16")
+    names = ("plain", "stream", "unrolled", "unrollednt")
     out = {}
     for kernel, timed in (("spmv", "spmv_f64"), ("jacobi", "jacobi_f64")):
         res = {}
-        for variant in (0, 1, 2, 3, 4):
+        for variant in (0, 1, 2, 3):
             for w in (8, 16, 32):
                 set_tuning(kernel, w, variant)
                 res[(names[variant], w)] = cd.time_kernel(timed, level, iters)

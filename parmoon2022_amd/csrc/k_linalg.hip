@@ -204,84 +204,6 @@ __device__ inline T row_sum_unrolled_nt(const int* __restrict__ rp, const int* _
   return s;
 }
 
-// Unrolled row sum reading 16-bit column offsets (CsrIdx16) where the row has
-// them: 2 bytes per entry instead of 4; same summation order as
-// row_sum_unrolled, hence the same bits.
-template <int W, int U, class T>
-__device__ inline T row_sum_c16(const int* __restrict__ rp, const int* __restrict__ col,
-                                const uint16_t* __restrict__ off, const int* __restrict__ base,
-                                const uint8_t* __restrict__ split, const T* __restrict__ val,
-                                const T* __restrict__ x, int row, int l) {
-  const int b = rp[row], e = rp[row + 1];
-  const int sp = split[row];
-  int c[U];
-  if(sp != kWideRow) {
-    const int ab = base[2 * row], db = base[2 * row + 1];
-#pragma unroll
-    for(int u = 0; u < U; ++u) {
-      const int k = b + l + u * W;
-      c[u] = k < e ? (k - b < sp ? ab : db) + (int)off[k] : -1;
-    }
-  } else {
-#pragma unroll
-    for(int u = 0; u < U; ++u) {
-      const int k = b + l + u * W;
-      c[u] = k < e ? col[k] : -1;
-    }
-  }
-  T v[U], xv[U];
-#pragma unroll
-  for(int u = 0; u < U; ++u) {
-    const int k = b + l + u * W;
-    v[u] = k < e ? val[k] : T(0);
-  }
-#pragma unroll
-  for(int u = 0; u < U; ++u) xv[u] = c[u] >= 0 ? x[c[u]] : T(0);
-  T s = 0;
-#pragma unroll
-  for(int u = 0; u < U; ++u) s += v[u] * xv[u];
-#pragma unroll
-  for(int o = W / 2; o > 0; o >>= 1) s += __shfl_xor(s, o, W);
-  return s;
-}
-
-template <int W, int U, class T>
-__global__ __launch_bounds__(256) void k_spmv_c16(int n, int n_active, const int* __restrict__ rp,
-                                                  const int* __restrict__ col, CsrIdx16 ci,
-                                                  const T* __restrict__ val, const T* __restrict__ x,
-                                                  const T* __restrict__ z, T* __restrict__ y, T alpha,
-                                                  T beta) {
-  const int gt = blockIdx.x * 256 + threadIdx.x;
-  const int row = gt / W, l = gt % W;
-  if(row >= n) return;
-  if(row < n_active) {
-    const T s = row_sum_c16<W, U, T>(rp, col, ci.off, ci.base, ci.split, val, x, row, l);
-    if(l == 0) y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * s;
-  } else if(l == 0) {
-    y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * x[row];
-  }
-}
-
-template <int W, int U, class T>
-__global__ __launch_bounds__(256) void k_jacobi_c16(int n, int n_active, const int* __restrict__ rp,
-                                                    const int* __restrict__ col, CsrIdx16 ci,
-                                                    const T* __restrict__ val,
-                                                    const T* __restrict__ diag,
-                                                    const T* __restrict__ b, const T* __restrict__ x,
-                                                    T* __restrict__ x_out, T omega) {
-  const int gt = blockIdx.x * 256 + threadIdx.x;
-  const int row = gt / W, l = gt % W;
-  if(row >= n) return;
-  T s;
-  if(row < n_active) s = row_sum_c16<W, U, T>(rp, col, ci.off, ci.base, ci.split, val, x, row, l);
-  else s = x[row];
-  if(l == 0) {
-    T d = b[row] + T(-1) * s;
-    d = d / diag[row];
-    x_out[row] = x[row] + omega * d;
-  }
-}
-
 template <int W, int U, class T, bool NT>
 __device__ inline T row_sum_u(const int* __restrict__ rp, const int* __restrict__ col,
                               const T* __restrict__ val, const T* __restrict__ x, int row, int l) {
@@ -687,44 +609,13 @@ void jacobi_unrolled(int u, int blocks, int n, int n_active, const int* rp, cons
   }
 }
 
-template <int W, class T>
-void spmv_c16(int u, int blocks, int n, int n_active, const int* rp, const int* col, const CsrIdx16& ci,
-              const T* val, const T* x, const T* z, T* y, T alpha, T beta, hipStream_t s) {
-  switch(u) {
-    case 2: k_spmv_c16<W, 2, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, ci, val, x, z, y, alpha, beta); break;
-    case 4: k_spmv_c16<W, 4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, ci, val, x, z, y, alpha, beta); break;
-    case 8: k_spmv_c16<W, 8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, ci, val, x, z, y, alpha, beta); break;
-    default: k_spmv_c16<W, 16, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, ci, val, x, z, y, alpha, beta); break;
-  }
-}
-
-template <int W, class T>
-void jacobi_c16(int u, int blocks, int n, int n_active, const int* rp, const int* col, const CsrIdx16& ci,
-                const T* val, const T* diag, const T* b, const T* x, T* x_out, T omega, hipStream_t s) {
-  switch(u) {
-    case 2: k_jacobi_c16<W, 2, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, ci, val, diag, b, x, x_out, omega); break;
-    case 4: k_jacobi_c16<W, 4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, ci, val, diag, b, x, x_out, omega); break;
-    case 8: k_jacobi_c16<W, 8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, ci, val, diag, b, x, x_out, omega); break;
-    default: k_jacobi_c16<W, 16, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, ci, val, diag, b, x, x_out, omega); break;
-  }
-}
-
 template <class T>
 void dispatch_spmv(int width, int variant, int max_row, int n, int n_active, const int* rp,
                    const int* col, const T* val, const T* x, const T* z, T* y, T alpha, T beta,
-                   hipStream_t s, const CsrIdx16* c16 = nullptr) {
+                   hipStream_t s) {
   const long threads = (long)n * width;
   const int blocks = (int)((threads + 255) / 256);
   const int u = width >= 8 ? unroll_for(max_row, width) : 0;
-  if(variant == 4 && u && c16) {
-    switch(width) {
-      case 8: spmv_c16<8, T>(u, blocks, n, n_active, rp, col, *c16, val, x, z, y, alpha, beta, s); break;
-      case 16: spmv_c16<16, T>(u, blocks, n, n_active, rp, col, *c16, val, x, z, y, alpha, beta, s); break;
-      default: spmv_c16<32, T>(u, blocks, n, n_active, rp, col, *c16, val, x, z, y, alpha, beta, s); break;
-    }
-    return;
-  }
-  if(variant == 4) variant = 2;
   if(variant == 2 && u) {
     switch(width) {
       case 8: spmv_unrolled<8, T>(u, blocks, n, n_active, rp, col, val, x, z, y, alpha, beta, s); break;
@@ -920,7 +811,7 @@ void set_tuning(int key, int value) {
   if(key == 0 || key == 1)
     PMCD_CHECK(value == 4 || value == 8 || value == 16 || value == 32, "lanes per row must be 4, 8, 16 or 32");
   else
-    PMCD_CHECK(value >= 0 && value <= 4, "variant must be 0 to 4");
+    PMCD_CHECK(value >= 0 && value <= 3, "variant must be 0, 1, 2 or 3");
   switch(key) {
     case 0: g_spmv_width = value; break;
     case 1: g_jacobi_width = value; break;
@@ -953,10 +844,9 @@ void launch_jacobi_from_zero_f32(int n, int n_active, const float* diag, const f
 
 void launch_spmv(int n, int n_active, const int* rp, const int* col, const double* val,
                  const double* x, const double* z, double* y, double alpha, double beta,
-                 hipStream_t s, int max_row, const CsrIdx16* c16) {
+                 hipStream_t s, int max_row) {
   if(n == 0) return;
-  dispatch_spmv<double>(g_spmv_width, g_spmv_variant, max_row, n, n_active, rp, col, val, x, z, y, alpha, beta, s,
-                        c16);
+  dispatch_spmv<double>(g_spmv_width, g_spmv_variant, max_row, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
   PMCD_HIP(hipGetLastError());
 }
 
@@ -972,30 +862,19 @@ void launch_spmv_rowlen(int n, int n_active, const int* rp, const int* col, cons
 
 void launch_spmv_f32(int n, int n_active, const int* rp, const int* col, const float* val,
                      const float* x, const float* z, float* y, float alpha, float beta,
-                     hipStream_t s, int max_row, const CsrIdx16* c16) {
+                     hipStream_t s, int max_row) {
   if(n == 0) return;
-  dispatch_spmv<float>(g_spmv_width, g_spmv_variant, max_row, n, n_active, rp, col, val, x, z, y, alpha, beta, s,
-                       c16);
+  dispatch_spmv<float>(g_spmv_width, g_spmv_variant, max_row, n, n_active, rp, col, val, x, z, y, alpha, beta, s);
   PMCD_HIP(hipGetLastError());
 }
 
 template <class T>
 static void dispatch_jacobi(int width, int max_row, int n, int n_active, const int* rp,
                             const int* col, const T* val, const T* diag, const T* b, const T* x,
-                            T* x_out, T omega, hipStream_t s, const CsrIdx16* c16 = nullptr) {
+                            T* x_out, T omega, hipStream_t s) {
   const int blocks = (int)(((long)n * width + 255) / 256);
   const int u = width >= 8 ? unroll_for(max_row, width) : 0;
-  int variant = g_jacobi_variant;
-  if(variant == 4 && u && c16) {
-    switch(width) {
-      case 8: jacobi_c16<8, T>(u, blocks, n, n_active, rp, col, *c16, val, diag, b, x, x_out, omega, s); break;
-      case 16: jacobi_c16<16, T>(u, blocks, n, n_active, rp, col, *c16, val, diag, b, x, x_out, omega, s); break;
-      default: jacobi_c16<32, T>(u, blocks, n, n_active, rp, col, *c16, val, diag, b, x, x_out, omega, s); break;
-    }
-    return;
-  }
-  if(variant == 4) variant = 2;
-  if(variant == 2 && u) {
+  if(g_jacobi_variant == 2 && u) {
     switch(width) {
       case 8: jacobi_unrolled<8, T>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
       case 16: jacobi_unrolled<16, T>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
@@ -1003,7 +882,7 @@ static void dispatch_jacobi(int width, int max_row, int n, int n_active, const i
     }
     return;
   }
-  if(variant == 3 && u) {
+  if(g_jacobi_variant == 3 && u) {
     switch(width) {
       case 8: jacobi_unrolled<8, T, true>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
       case 16: jacobi_unrolled<16, T, true>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
@@ -1011,7 +890,7 @@ static void dispatch_jacobi(int width, int max_row, int n, int n_active, const i
     }
     return;
   }
-  if(variant == 1) {
+  if(g_jacobi_variant == 1) {
     switch(width) {
       case 4: k_jacobi_stream<4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
       case 8: k_jacobi_stream<8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
@@ -1030,17 +909,17 @@ static void dispatch_jacobi(int width, int max_row, int n, int n_active, const i
 
 void launch_jacobi(int n, int n_active, const int* rp, const int* col, const double* val,
                    const double*, const double* diag, const double* b, const double* x,
-                   double* x_out, double omega, hipStream_t s, int max_row, const CsrIdx16* c16) {
+                   double* x_out, double omega, hipStream_t s, int max_row) {
   if(n == 0) return;
-  dispatch_jacobi<double>(g_jacobi_width, max_row, n, n_active, rp, col, val, diag, b, x, x_out, omega, s, c16);
+  dispatch_jacobi<double>(g_jacobi_width, max_row, n, n_active, rp, col, val, diag, b, x, x_out, omega, s);
   PMCD_HIP(hipGetLastError());
 }
 
 void launch_jacobi_f32(int n, int n_active, const int* rp, const int* col, const float* val,
                        const float* diag, const float* b, const float* x, float* x_out,
-                       float omega, hipStream_t s, int max_row, const CsrIdx16* c16) {
+                       float omega, hipStream_t s, int max_row) {
   if(n == 0) return;
-  dispatch_jacobi<float>(g_jacobi_width, max_row, n, n_active, rp, col, val, diag, b, x, x_out, omega, s, c16);
+  dispatch_jacobi<float>(g_jacobi_width, max_row, n, n_active, rp, col, val, diag, b, x, x_out, omega, s);
   PMCD_HIP(hipGetLastError());
 }
 

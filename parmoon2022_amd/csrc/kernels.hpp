@@ -58,23 +58,13 @@ void launch_cell_errors(const AsmArgs& a, const int* dof, const double* u, doubl
                         hipStream_t s);
 
 // ---- sparse / dense linear algebra ----------------------------------------
-// 16-bit column offsets of a CSR matrix (same entry order as its int32
-// columns): a row with split[r] != 255 stores col = base[2r] + off for its
-// first split[r] entries (active columns) and col = base[2r+1] + off for the
-// rest (Dirichlet columns); split[r] == 255 marks a row kept at int32 width.
-struct CsrIdx16 {
-  const uint16_t* off = nullptr;
-  const int* base = nullptr;
-  const uint8_t* split = nullptr;
-};
-constexpr uint8_t kWideRow = 255;
 // y[r] = beta*z[r] + alpha*(A x)[r] for r < n_active, beta*z[r] + alpha*x[r] otherwise
 void launch_spmv(int n, int n_active, const int* rp, const int* col, const double* val,
                  const double* x, const double* z, double* y, double alpha, double beta,
-                 hipStream_t s, int max_row = -1, const CsrIdx16* c16 = nullptr);
+                 hipStream_t s, int max_row = -1);
 void launch_spmv_f32(int n, int n_active, const int* rp, const int* col, const float* val,
                      const float* x, const float* z, float* y, float alpha, float beta,
-                     hipStream_t s, int max_row = -1, const CsrIdx16* c16 = nullptr);
+                     hipStream_t s, int max_row = -1);
 // same, launch shape from the row lengths (8 or 16 lanes, unrolled loads)
 void launch_spmv_rowlen(int n, int n_active, const int* rp, const int* col, const double* val,
                         const double* x, const double* z, double* y, double alpha, double beta,
@@ -83,10 +73,10 @@ void launch_spmv_rowlen(int n, int n_active, const int* rp, const int* col, cons
 void launch_jacobi(int n, int n_active, const int* rp, const int* col, const double* val,
                    const double* inv_diag_unused, const double* diag, const double* b,
                    const double* x, double* x_out, double omega, hipStream_t s,
-                   int max_row = -1, const CsrIdx16* c16 = nullptr);
+                   int max_row = -1);
 void launch_jacobi_f32(int n, int n_active, const int* rp, const int* col, const float* val,
                        const float* diag, const float* b, const float* x, float* x_out,
-                       float omega, hipStream_t s, int max_row = -1, const CsrIdx16* c16 = nullptr);
+                       float omega, hipStream_t s, int max_row = -1);
 // first sweep from x = 0 when b vanishes on the non-active rows (no SpMV needed)
 void launch_jacobi_from_zero(int n, int n_active, const double* diag, const double* b,
                              double* x_out, double omega, hipStream_t s);

@@ -113,29 +113,4 @@ SweepOrder build_sweep_order(const Structure& st, int max_exact_classes) {
   return o;
 }
 
-double build_idx16(const Structure& st, int n_active, std::vector<uint16_t>& off, std::vector<int>& base,
-                   std::vector<uint8_t>& split) {
-  const int n = st.n_rows;
-  off.assign(st.nnz, 0);
-  base.assign(2 * (size_t)n, 0);
-  split.assign(n, 255);
-  int64_t packed = 0;
-  for(int r = 0; r < n; ++r) {
-    const int b = st.row_ptr[r], e = st.row_ptr[r + 1];
-    // columns ascend and the Dirichlet ones (>= n_active) come last
-    int k = b;
-    while(k < e && st.col[k] < n_active) ++k;
-    const int na = k - b;
-    const bool act_ok = na == 0 || st.col[k - 1] - st.col[b] < 65536;
-    const bool dir_ok = k == e || st.col[e - 1] - st.col[k] < 65536;
-    if(!act_ok || !dir_ok || na >= 255) continue;
-    split[r] = (uint8_t)na;
-    base[2 * (size_t)r] = na ? st.col[b] : 0;
-    base[2 * (size_t)r + 1] = k < e ? st.col[k] : 0;
-    for(int q = b; q < e; ++q) off[q] = (uint16_t)(st.col[q] - (q < k ? base[2 * (size_t)r] : base[2 * (size_t)r + 1]));
-    packed += e - b;
-  }
-  return st.nnz ? (double)packed / st.nnz : 0.0;
-}
-
 }  // namespace pmcd

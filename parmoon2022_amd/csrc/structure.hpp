@@ -47,11 +47,6 @@ Structure build_structure(const FESpace& sp, int n_cells);
 // result is bit-identical. Otherwise (more than max_exact_classes levels,
 // e.g. Q2 in 3D) greedy colouring in row order gives a multicolour SOR: the
 // same per-row arithmetic in a different row order.
-// 16-bit column offsets (CsrIdx16 layout): off[nnz], base[2 n], split[n].
-// Returns the fraction of entries stored at 16 bits.
-double build_idx16(const Structure& st, int n_active, std::vector<uint16_t>& off, std::vector<int>& base,
-                   std::vector<uint8_t>& split);
-
 struct SweepOrder {
   std::vector<int> rows;       // rows grouped by class, ascending inside a class
   std::vector<int> class_ptr;  // class k = rows[class_ptr[k] .. class_ptr[k+1])

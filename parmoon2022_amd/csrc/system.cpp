@@ -201,16 +201,6 @@ void CDSystem::build_level_device(Level& L) {
   L.contrib_ptr.upload(st.contrib_ptr, stream_);
   L.contrib.upload(st.contrib, stream_);
   L.slot.upload(st.slot, stream_);
-  {
-    std::vector<uint16_t> off;
-    std::vector<int> base;
-    std::vector<uint8_t> split;
-    build_idx16(st, sp.n_active, off, base, split);
-    L.col16.upload(off, stream_);
-    L.col_base.upload(base, stream_);
-    L.col_split.upload(split, stream_);
-    L.idx16_ = CsrIdx16{L.col16.p, L.col_base.p, L.col_split.p};
-  }
   boundary_data(L);
   const int n = sp.n_dof;
   L.val.alloc(st.nnz);
@@ -409,9 +399,9 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
   AsmArgs a = asm_args(L);
   auto launch = [&]() {
     switch(which) {
-      case 0: launch_spmv(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x.p, nullptr, y.p, 1.0, 0.0, stream_, L.st.max_row, L.idx16()); break;
-      case 1: launch_jacobi(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b.p, x.p, y.p, 0.8, stream_, L.st.max_row, L.idx16()); break;
-      case 2: launch_spmv_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, xf.p, nullptr, yf.p, 1.0f, 0.0f, stream_, L.st.max_row, L.idx16()); break;
+      case 0: launch_spmv(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x.p, nullptr, y.p, 1.0, 0.0, stream_, L.st.max_row); break;
+      case 1: launch_jacobi(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b.p, x.p, y.p, 0.8, stream_, L.st.max_row); break;
+      case 2: launch_spmv_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, xf.p, nullptr, yf.p, 1.0f, 0.0f, stream_, L.st.max_row); break;
       case 3:
         if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
         break;
@@ -419,7 +409,7 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
         launch_gather_matrix(n, L.n_active(), a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p, L.contrib.p,
                              L.slot.p, local_A_.p, L.val.p, stream_);
         break;
-      case 5: launch_jacobi_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, bf.p, xf.p, yf.p, 0.8f, stream_, L.st.max_row, L.idx16()); break;
+      case 5: launch_jacobi_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, bf.p, xf.p, yf.p, 0.8f, stream_, L.st.max_row); break;
       case 6: launch_stream_copy((long)copy_dst.n, L.val.p, copy_dst.p, stream_); break;
       case 7: launch_stream_copy4((long)copy_dst.n, L.val.p, copy_dst.p, stream_); break;
       case 8: launch_stream_read((long)(L.val.n & ~(size_t)1), L.val.p, copy_dst.p, stream_); break;
@@ -443,11 +433,11 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
 
 void CDSystem::apply(Level& L, const double* x, double* y) {
   launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, nullptr, y, 1.0, 0.0, stream_,
-              L.st.max_row, L.idx16());
+              L.st.max_row);
 }
 
 void CDSystem::apply_scaled_add(Level& L, const double* x, double* y, double a) {
-  launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, y, y, a, 1.0, stream_, L.st.max_row, L.idx16());
+  launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, y, y, a, 1.0, stream_, L.st.max_row);
 }
 
 double CDSystem::dot(int n, const double* x, const double* y) {
@@ -473,7 +463,7 @@ int CDSystem::smooth(Level& L, double* const x[2], int cur, const double* b, int
       launch_jacobi_from_zero(L.n(), L.n_active(), L.diag.p, b, x[1 - cur], sd_.jacobi_damp, stream_);
     else
       launch_jacobi(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b, x[cur],
-                    x[1 - cur], sd_.jacobi_damp, stream_, L.st.max_row, L.idx16());
+                    x[1 - cur], sd_.jacobi_damp, stream_, L.st.max_row);
     cur = 1 - cur;
   }
   return cur;
@@ -492,7 +482,7 @@ int CDSystem::smooth32(Level& L, float* const x[2], int cur, const float* b, int
                                   (float)sd_.jacobi_damp, stream_);
     else
       launch_jacobi_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x[cur],
-                        x[1 - cur], (float)sd_.jacobi_damp, stream_, L.st.max_row, L.idx16());
+                        x[1 - cur], (float)sd_.jacobi_damp, stream_, L.st.max_row);
     cur = 1 - cur;
   }
   return cur;
@@ -530,7 +520,7 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
     const bool zero = (l == Lf) ? fine_zero : (jacobi && sd_.n_pre > 0);
     cur[l] = smooth(L, x[l].data(), cur[l], b[l], sd_.n_pre, zero);
     launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x[l][cur[l]], b[l], L.mg_def.p,
-                -1.0, 1.0, stream_, L.st.max_row, L.idx16());
+                -1.0, 1.0, stream_, L.st.max_row);
     if(!jacobi || sd_.n_pre == 0 || l - 1 == 0) launch_fill(C.n(), 0.0, x[l - 1][0], stream_);
     launch_restrict(C.n(), C.n_active(), (long)L.r_val.n, L.r_max_row, L.r_rp.p, L.r_col.p,
                     L.r_val.p, L.mg_def.p, C.mg_rhs.p, stream_);
@@ -580,7 +570,7 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
     const bool zero = (l == Lf) ? fine_zero : (jacobi && sd_.n_pre > 0);
     cur[l] = smooth32(L, x[l].data(), cur[l], L.f_rhs.p, sd_.n_pre, zero);
     launch_spmv_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, x[l][cur[l]], L.f_rhs.p,
-                    L.f_def.p, -1.0f, 1.0f, stream_, L.st.max_row, L.idx16());
+                    L.f_def.p, -1.0f, 1.0f, stream_, L.st.max_row);
     if(!jacobi || sd_.n_pre == 0 || l - 1 == 0) launch_fill_f32(C.n(), 0.0f, x[l - 1][0], stream_);
     launch_restrict_f32(C.n(), C.n_active(), (long)L.r_val32.n, L.r_max_row, L.r_rp.p, L.r_col.p,
                         L.r_val32.p, L.f_def.p, C.f_rhs.p, stream_);

@@ -80,12 +80,6 @@ struct Level {
   DevBuf<float> p_val32, r_val32;
   int n_coarse = 0;
   int p_max_row = -1, r_max_row = -1;  // longest rows of P and R
-  // 16-bit column offsets of the level matrix (rows that fit)
-  DevBuf<uint16_t> col16;
-  DevBuf<int> col_base;
-  DevBuf<uint8_t> col_split;
-  CsrIdx16 idx16_;
-  const CsrIdx16* idx16() const { return idx16_.off ? &idx16_ : nullptr; }
   // SOR sweep classes (built on first use): class k = sweep_rows[sweep_ptr[k] ..
   // sweep_ptr[k+1]); exact = level schedule (bit-identical to the serial sweep)
   std::vector<int> sweep_ptr;

@@ -84,6 +84,12 @@ def set_sor_exact_limit(n):
     _check(lib().pmcd_set_tuning(5, int(n)))
 
 
+def set_vcycle_graphs(on):
+    """Replay V-cycles from captured HIP graphs (default) or launch them
+    kernel by kernel."""
+    _check(lib().pmcd_set_tuning(6, 1 if on else 0))
+
+
 def get_sor_exact_limit():
     v = ctypes.c_int(0)
     _check(lib().pmcd_get_tuning(5, ctypes.byref(v)))

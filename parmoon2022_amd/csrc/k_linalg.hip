@@ -797,7 +797,20 @@ int assembly_variant() { return g_assembly_variant; }
 static int g_sor_exact_limit = 24;
 int sor_exact_limit() { return g_sor_exact_limit; }
 
+// V-cycles replayed from captured HIP graphs (1) or launched kernel by kernel (0)
+static int g_vcycle_graphs = 1;
+int vcycle_graphs() { return g_vcycle_graphs; }
+// changes whenever a launch shape changes, so that captured graphs are rebuilt
+static int g_tuning_epoch = 0;
+int tuning_epoch() { return g_tuning_epoch; }
+
 void set_tuning(int key, int value) {
+  ++g_tuning_epoch;
+  if(key == 6) {
+    PMCD_CHECK(value == 0 || value == 1, "graph switch must be 0 or 1");
+    g_vcycle_graphs = value;
+    return;
+  }
   if(key == 5) {
     PMCD_CHECK(value >= 0, "the exact-sweep class limit must be non-negative");
     g_sor_exact_limit = value;
@@ -829,6 +842,7 @@ int get_tuning(int key) {
     case 3: return g_jacobi_variant;
     case 4: return g_assembly_variant;
     case 5: return g_sor_exact_limit;
+    case 6: return g_vcycle_graphs;
     default: PMCD_FAIL("unknown tuning key ", key);
   }
 }

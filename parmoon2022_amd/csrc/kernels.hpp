@@ -101,6 +101,8 @@ void launch_sor_rows_f32(int count, const int* rows, const int* rp, const int* c
                          int max_row, double mean_row, hipStream_t s, const int* row_owner = nullptr,
                          const uint8_t* owner_active = nullptr);
 int sor_exact_limit();
+int vcycle_graphs();
+int tuning_epoch();
 // restriction y = R x on active rows, 0 on rows >= n_active
 void launch_restrict(int n, int n_active, long nnz, int max_row, const int* rp, const int* col,
                      const double* val, const double* x, double* y, hipStream_t s);

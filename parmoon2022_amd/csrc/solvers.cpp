@@ -75,11 +75,19 @@ void CDSystem::prepare(const SolverDesc& s) {
   if(s.precond == 2 && s.smoother != 0)
     for(auto& Lp : levels_) prepare_sweep(*Lp);
   if(s.precond == 3 || s.precond == 4 || s.method == 4 || s.method == 5) prepare_sweep(finest());
+  if(s.mixed)
+    for(auto& Lp : levels_) {
+      Level& L = *Lp;
+      if(L.f_sol.n != (size_t)L.n()) {
+        L.f_sol.alloc(L.n()); L.f_sol2.alloc(L.n()); L.f_rhs.alloc(L.n());
+        L.f_def.alloc(L.n()); L.f_tmp.alloc(L.n());
+      }
+    }
   if(s.mixed && !mixed_ready_) {
     for(auto& Lp : levels_) {
       Level& L = *Lp;
-      L.val32.alloc(L.val.n);
-      L.diag32.alloc(L.n());
+      if(L.val32.n != L.val.n) L.val32.alloc(L.val.n);
+      if(L.diag32.n != (size_t)L.n()) L.diag32.alloc(L.n());
       launch_d2f((int)L.val.n, L.val.p, L.val32.p, stream_);
       launch_d2f(L.n(), L.diag.p, L.diag32.p, stream_);
     }

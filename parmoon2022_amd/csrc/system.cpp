@@ -96,7 +96,15 @@ CDSystem::CDSystem(const ProblemDesc& d, hipStream_t stream) : desc_(d), stream_
   PMCD_HIP(hipStreamSynchronize(stream_));
 }
 
-CDSystem::~CDSystem() { (void)hipStreamSynchronize(stream_); }
+CDSystem::~CDSystem() {
+  (void)hipStreamSynchronize(stream_);
+  clear_graphs();
+}
+
+void CDSystem::clear_graphs() {
+  for(auto& g : vgraphs_) (void)hipGraphExecDestroy(g.exec);
+  vgraphs_.clear();
+}
 
 // Dirichlet values and Neumann terms of the current example on level L
 void CDSystem::boundary_data(Level& L) {
@@ -367,12 +375,12 @@ void CDSystem::coarse_factorize() {
   const int n = C.n();
   PMCD_CHECK(n <= 20000, "coarse direct solve limited to 20000 unknowns, got ", n);
   DevBuf<double> A((size_t)n * n);
-  coarse_inv_.alloc((size_t)n * n);
+  if(coarse_inv_.n != (size_t)n * n) coarse_inv_.alloc((size_t)n * n);
   DevBuf<double> work(n + 2);
   DevBuf<int> piv(1);
   launch_csr_to_dense(n, C.row_ptr.p, C.col.p, C.val.p, A.p, stream_);
   dense_inverse(n, A.p, coarse_inv_.p, work.p, piv.p, stream_);
-  coarse_tmp_.alloc(2 * (size_t)n);
+  if(coarse_tmp_.n != 2 * (size_t)n) coarse_tmp_.alloc(2 * (size_t)n);
   PMCD_HIP(hipStreamSynchronize(stream_));
   coarse_ready_ = true;
 }
@@ -545,13 +553,8 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
 
 void CDSystem::vcycle32(const double* rhs, double* out) {
   const int Lf = n_levels() - 1;
-  for(auto& Lp : levels_) {
-    Level& L = *Lp;
-    if(L.f_sol.n != (size_t)L.n()) {
-      L.f_sol.alloc(L.n()); L.f_sol2.alloc(L.n()); L.f_rhs.alloc(L.n());
-      L.f_def.alloc(L.n()); L.f_tmp.alloc(L.n());
-    }
-  }
+  for(auto& Lp : levels_)
+    PMCD_CHECK(Lp->f_sol.n == (size_t)Lp->n(), "FP32 work vectors not prepared");
   std::vector<std::array<float*, 2>> x(n_levels());
   std::vector<int> cur(n_levels(), 0);
   for(int l = 0; l <= Lf; ++l) x[l] = {levels_[l]->f_sol.p, levels_[l]->f_sol2.p};
@@ -597,9 +600,54 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
   launch_f2d(F.n(), x[Lf][cur[Lf]], out, stream_);
 }
 
-void CDSystem::vcycle(const double* rhs, double* out) {
+void CDSystem::vcycle_eager(const double* rhs, double* out) {
   if(sd_.mixed) vcycle32(rhs, out);
   else vcycle64(rhs, out);
+}
+
+// The V-cycle is a fixed kernel sequence for given solver parameters, launch
+// shapes and buffers: it is captured once per (rhs, out, zero-tail) and
+// replayed as one graph launch. Graphs are dropped when the parameters or the
+// launch shapes change; matrix values are read from the same buffers, so
+// reassembly keeps them valid. The sequence holds no memset nodes.
+void CDSystem::vcycle(const double* rhs, double* out) {
+  if(!vcycle_graphs()) {
+    vcycle_eager(rhs, out);
+    return;
+  }
+  auto same = [](const SolverDesc& a, const SolverDesc& b) {
+    return a.smoother == b.smoother && a.n_pre == b.n_pre && a.n_post == b.n_post &&
+           a.jacobi_damp == b.jacobi_damp && a.mg_damp == b.mg_damp && a.sor_omega == b.sor_omega &&
+           a.mixed == b.mixed;
+  };
+  if(graph_epoch_ != tuning_epoch() || !same(graph_sd_, sd_) || vgraphs_.size() > 128) {
+    clear_graphs();
+    graph_epoch_ = tuning_epoch();
+    graph_sd_ = sd_;
+  }
+  const bool fz = rhs_nonactive_zero_;
+  for(const auto& g : vgraphs_)
+    if(g.rhs == rhs && g.out == out && g.fine_zero == fz) {
+      PMCD_HIP(hipGraphLaunch(g.exec, stream_));
+      return;
+    }
+  PMCD_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+  try {
+    vcycle_eager(rhs, out);
+  } catch(...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(stream_, &g);
+    if(g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  hipGraph_t graph = nullptr;
+  PMCD_HIP(hipStreamEndCapture(stream_, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  PMCD_HIP(e);
+  vgraphs_.push_back({rhs, out, fz, exec});
+  PMCD_HIP(hipGraphLaunch(exec, stream_));
 }
 
 bool CDSystem::tail_is_zero(const double* v) {

@@ -108,6 +108,8 @@ class CDSystem {
  public:
   CDSystem(const ProblemDesc& d, hipStream_t stream);
   ~CDSystem();
+  CDSystem(const CDSystem&) = delete;
+  CDSystem& operator=(const CDSystem&) = delete;
 
   void assemble();                        // every level
   SolveStats solve(const SolverDesc& s);  // finest level
@@ -147,6 +149,8 @@ class CDSystem {
   bool tail_is_zero(const double* v);  // v vanishes on the finest non-active rows
   void vcycle64(const double* rhs, double* out);
   void vcycle32(const double* rhs, double* out);
+  void vcycle_eager(const double* rhs, double* out);
+  void clear_graphs();
   void precond(const double* v, double* z);
   SolveStats solve_cg();
   SolveStats solve_fgmres();
@@ -170,6 +174,17 @@ class CDSystem {
   // set by the Krylov solvers when every preconditioner input vanishes on the
   // non-active rows (true when the initial residual does)
   bool rhs_nonactive_zero_ = false;
+  // captured V-cycles, one per (input, output, zero-tail) triple, valid for
+  // the solver parameters and launch shapes they were captured with
+  struct VGraph {
+    const double* rhs;
+    double* out;
+    bool fine_zero;
+    hipGraphExec_t exec;
+  };
+  std::vector<VGraph> vgraphs_;
+  SolverDesc graph_sd_{};
+  int graph_epoch_ = -1;
   // Krylov work space
   std::vector<DevBuf<double>> kv_, kz_;
   DevBuf<double> kr_, kw_, kh_;

@@ -568,3 +568,36 @@ def test_served_results_same_alone_and_in_the_mix(slots, budget):
     for r, (u, its) in zip(reqs, alone):
         assert r.error is None, r.error
         assert r.stats.iterations == its and np.array_equal(r.solution, u), r.id
+
+
+@pytest.mark.parametrize("smoother,mixed", [("jacobi", 0), ("ssor", 0), ("jacobi", 1)])
+def test_vcycle_graph_replays_match_eager(smoother, mixed):
+    """Captured V-cycle graphs: every replay (with other device work in
+    between) gives the bits of the kernel-by-kernel V-cycle, and a full solve
+    with graphs equals the solve without them."""
+    db = c2_db(3, 3, mixed=mixed)
+    db.update({"multigrid_smoother": smoother, "sor_omega": 1.0})
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    i = cd.info()
+    rng = np.random.default_rng(21)
+    v = rng.standard_normal(i["n_dof"])
+    v[i["n_active"]:] = 0
+    pm.set_vcycle_graphs(False)
+    ref = cd.vcycle(v)
+    st0 = cd.solve()
+    u0 = cd.solution()
+    pm.set_vcycle_graphs(True)
+    try:
+        for _ in range(4):
+            assert np.array_equal(cd.vcycle(v), ref)
+            cd.apply(rng.standard_normal(i["n_dof"]))
+        cd.assemble()
+        st1 = cd.solve()
+        assert st1.iterations == st0.iterations and np.array_equal(cd.solution(), u0)
+        cd.assemble()
+        st2 = cd.solve()
+        assert st2.iterations == st0.iterations and np.array_equal(cd.solution(), u0)
+    finally:
+        pm.set_vcycle_graphs(True)
+    cd.close()

@@ -201,8 +201,8 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
  * largest number of level-schedule classes for which SOR sweeps follow the
  * exact serial order (default 24; deeper schedules use multicolour order),
  * read when a level's sweep order is first built; key 6 = 1 to replay
- * multigrid V-cycles from captured HIP graphs (default), 0 to launch them
- * kernel by kernel; key 4 = assembly kernel (0 reference operation order on
+ * multigrid V-cycles from captured HIP graphs, 0 to launch them kernel by
+ * kernel (default); key 4 = assembly kernel (0 reference operation order on
  * vector ALUs, 1 FP64 MFMA). */
 int pmcd_set_tuning(int key, int value);
 /* Current value of a launch-shape knob (keys as for pmcd_set_tuning). */

@@ -85,9 +85,15 @@ def set_sor_exact_limit(n):
 
 
 def set_vcycle_graphs(on):
-    """Replay V-cycles from captured HIP graphs (default) or launch them
-    kernel by kernel."""
+    """Replay V-cycles from captured HIP graphs, or launch them kernel by
+    kernel (default)."""
     _check(lib().pmcd_set_tuning(6, 1 if on else 0))
+
+
+def get_vcycle_graphs():
+    v = ctypes.c_int(0)
+    _check(lib().pmcd_get_tuning(6, ctypes.byref(v)))
+    return bool(v.value)
 
 
 def get_sor_exact_limit():

@@ -797,8 +797,9 @@ int assembly_variant() { return g_assembly_variant; }
 static int g_sor_exact_limit = 24;
 int sor_exact_limit() { return g_sor_exact_limit; }
 
-// V-cycles replayed from captured HIP graphs (1) or launched kernel by kernel (0)
-static int g_vcycle_graphs = 1;
+// V-cycles replayed from captured HIP graphs (1) or launched kernel by kernel
+// (0, default: replay measured no faster on these kernel sequences)
+static int g_vcycle_graphs = 0;
 int vcycle_graphs() { return g_vcycle_graphs; }
 // changes whenever a launch shape changes, so that captured graphs are rebuilt
 static int g_tuning_epoch = 0;

@@ -583,6 +583,7 @@ def test_vcycle_graph_replays_match_eager(smoother, mixed):
     rng = np.random.default_rng(21)
     v = rng.standard_normal(i["n_dof"])
     v[i["n_active"]:] = 0
+    old = pm.get_vcycle_graphs()
     pm.set_vcycle_graphs(False)
     ref = cd.vcycle(v)
     st0 = cd.solve()
@@ -599,5 +600,5 @@ def test_vcycle_graph_replays_match_eager(smoother, mixed):
         st2 = cd.solve()
         assert st2.iterations == st0.iterations and np.array_equal(cd.solution(), u0)
     finally:
-        pm.set_vcycle_graphs(True)
+        pm.set_vcycle_graphs(old)
     cd.close()

@@ -254,6 +254,7 @@ void BatchSystem::assemble() {
   PMCD_HIP(hipStreamSynchronize(stream_));
   last_assemble_ms = (now_seconds() - t0) * 1e3;
   mixed_ready_ = false;
+  G_.sw_fresh = G_.sw_fresh32 = false;
 }
 
 void BatchSystem::seg_dot(const double* x, const double* y, double* out_dev) {

@@ -389,6 +389,43 @@ __global__ __launch_bounds__(256) void k_sor_rows_lanes(int count, const int* __
   if(l == 0) x[r] = x[r] + omega * (b[r] - s) / diag[r];
 }
 
+// Multicolour class over the class-ordered copy of the matrix: permuted row i
+// (original row rows[i]) has its entries, in their original order, at
+// sw_rp[i] .. sw_rp[i+1]; a class is a contiguous range of permuted rows, so
+// its matrix data streams like a plain SpMV.
+template <int W, int U, class T>
+__global__ __launch_bounds__(256) void k_sor_rows_perm(int count, int i0, const int* __restrict__ rows,
+                                                       const int* __restrict__ sw_rp,
+                                                       const int* __restrict__ sw_col,
+                                                       const T* __restrict__ sw_val,
+                                                       const T* __restrict__ diag,
+                                                       const T* __restrict__ b, T* x, T omega,
+                                                       const int* __restrict__ row_owner,
+                                                       const uint8_t* __restrict__ owner_active) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int li = gt / W, l = gt % W;
+  if(li >= count) return;
+  const int i = i0 + li;
+  const int r = rows[i];
+  if(row_owner && !owner_active[row_owner[r]]) return;
+  const T s = transfer_row<W, U, T>(sw_rp, sw_col, sw_val, x, i, l);
+  if(l == 0) x[r] = x[r] + omega * (b[r] - s) / diag[r];
+}
+
+// val in class order: permuted row i <- original row rows[i]
+template <class T>
+__global__ __launch_bounds__(256) void k_permute_rows(int n, const int* __restrict__ rows,
+                                                      const int* __restrict__ rp,
+                                                      const int* __restrict__ sw_rp,
+                                                      const T* __restrict__ val, T* __restrict__ sw_val) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int i = gt / 8, l = gt % 8;
+  if(i >= n) return;
+  const int r = rows[i];
+  const int src = rp[r], dst = sw_rp[i], len = rp[r + 1] - src;
+  for(int t = l; t < len; t += 8) sw_val[dst + t] = val[src + t];
+}
+
 __device__ inline double block_sum(double v) {
   __shared__ double s_w[4];
   for(int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
@@ -1021,6 +1058,63 @@ static void dispatch_sor_rows(int count, const int* rows, const int* rp, const i
     }
 #undef PMCD_SOR_U
   }
+  PMCD_HIP(hipGetLastError());
+}
+
+template <class T>
+static void dispatch_sor_perm(int count, int i0, const int* rows, const int* sw_rp, const int* sw_col,
+                              const T* sw_val, const T* diag, const T* b, T* x, T omega, int max_row,
+                              double mean_row, hipStream_t s, const int* row_owner,
+                              const uint8_t* owner_active) {
+  if(count <= 0) return;
+  const int W = mean_row <= 12 ? 4 : (mean_row <= 24 ? 8 : 16);
+  const int u = unroll_for(max_row, W);
+  const int blocks = (int)(((long)count * W + 255) / 256);
+#define PMCD_SORP(WW, UU)                                                                      \
+  k_sor_rows_perm<WW, UU, T><<<blocks, 256, 0, s>>>(count, i0, rows, sw_rp, sw_col, sw_val, diag, b, x, \
+                                                    omega, row_owner, owner_active)
+#define PMCD_SORP_U(WW)                       \
+  switch(u) {                                 \
+    case 2: PMCD_SORP(WW, 2); break;          \
+    case 4: PMCD_SORP(WW, 4); break;          \
+    case 8: PMCD_SORP(WW, 8); break;          \
+    case 16: PMCD_SORP(WW, 16); break;        \
+    default: PMCD_SORP(WW, 0); break;         \
+  }
+  switch(W) {
+    case 4: PMCD_SORP_U(4); break;
+    case 8: PMCD_SORP_U(8); break;
+    default: PMCD_SORP_U(16); break;
+  }
+#undef PMCD_SORP_U
+#undef PMCD_SORP
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_sor_rows_perm(int count, int i0, const int* rows, const int* sw_rp, const int* sw_col,
+                          const double* sw_val, const double* diag, const double* b, double* x, double omega,
+                          int max_row, double mean_row, hipStream_t s, const int* row_owner,
+                          const uint8_t* owner_active) {
+  dispatch_sor_perm<double>(count, i0, rows, sw_rp, sw_col, sw_val, diag, b, x, omega, max_row, mean_row, s,
+                            row_owner, owner_active);
+}
+void launch_sor_rows_perm_f32(int count, int i0, const int* rows, const int* sw_rp, const int* sw_col,
+                              const float* sw_val, const float* diag, const float* b, float* x, float omega,
+                              int max_row, double mean_row, hipStream_t s, const int* row_owner,
+                              const uint8_t* owner_active) {
+  dispatch_sor_perm<float>(count, i0, rows, sw_rp, sw_col, sw_val, diag, b, x, omega, max_row, mean_row, s,
+                           row_owner, owner_active);
+}
+void launch_permute_rows(int n, const int* rows, const int* rp, const int* sw_rp, const double* val,
+                         double* sw_val, hipStream_t s) {
+  if(n <= 0) return;
+  k_permute_rows<double><<<(int)(((long)n * 8 + 255) / 256), 256, 0, s>>>(n, rows, rp, sw_rp, val, sw_val);
+  PMCD_HIP(hipGetLastError());
+}
+void launch_permute_rows_f32(int n, const int* rows, const int* rp, const int* sw_rp, const float* val,
+                             float* sw_val, hipStream_t s) {
+  if(n <= 0) return;
+  k_permute_rows<float><<<(int)(((long)n * 8 + 255) / 256), 256, 0, s>>>(n, rows, rp, sw_rp, val, sw_val);
   PMCD_HIP(hipGetLastError());
 }
 

@@ -101,6 +101,21 @@ void launch_sor_rows_f32(int count, const int* rows, const int* rp, const int* c
                          int max_row, double mean_row, hipStream_t s, const int* row_owner = nullptr,
                          const uint8_t* owner_active = nullptr);
 int sor_exact_limit();
+// multicolour class [i0, i0 + count) over the class-ordered matrix copy
+// (sw_rp, sw_col, sw_val: permuted rows, original entry order)
+void launch_sor_rows_perm(int count, int i0, const int* rows, const int* sw_rp, const int* sw_col,
+                          const double* sw_val, const double* diag, const double* b, double* x, double omega,
+                          int max_row, double mean_row, hipStream_t s, const int* row_owner = nullptr,
+                          const uint8_t* owner_active = nullptr);
+void launch_sor_rows_perm_f32(int count, int i0, const int* rows, const int* sw_rp, const int* sw_col,
+                              const float* sw_val, const float* diag, const float* b, float* x, float omega,
+                              int max_row, double mean_row, hipStream_t s, const int* row_owner = nullptr,
+                              const uint8_t* owner_active = nullptr);
+// sw_val[sw_rp[i] + t] = val[rp[rows[i]] + t]
+void launch_permute_rows(int n, const int* rows, const int* rp, const int* sw_rp, const double* val,
+                         double* sw_val, hipStream_t s);
+void launch_permute_rows_f32(int n, const int* rows, const int* rp, const int* sw_rp, const float* val,
+                             float* sw_val, hipStream_t s);
 int vcycle_graphs();
 int tuning_epoch();
 // restriction y = R x on active rows, 0 on rows >= n_active

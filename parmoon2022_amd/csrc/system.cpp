@@ -368,6 +368,7 @@ void CDSystem::assemble() {
   last_assemble_ms = (now_s() - t0) * 1e3;
   coarse_ready_ = false;
   mixed_ready_ = false;
+  for(auto& L : levels_) L->sw_fresh = L->sw_fresh32 = false;
 }
 
 void CDSystem::coarse_factorize() {
@@ -694,18 +695,51 @@ void prepare_level_sweep(Level& L, hipStream_t stream) {
   L.sweep_ptr = std::move(o.class_ptr);
   L.sweep_exact = o.exact;
   L.sweep_rows.upload(o.rows, stream);
+  if(!o.exact) {
+    const Structure& st = L.st;
+    std::vector<int> rp(st.n_rows + 1, 0), cl(st.nnz);
+    for(int i = 0; i < st.n_rows; ++i) {
+      const int r = o.rows[i], len = st.row_ptr[r + 1] - st.row_ptr[r];
+      rp[i + 1] = rp[i] + len;
+      std::copy(st.col.begin() + st.row_ptr[r], st.col.begin() + st.row_ptr[r + 1], cl.begin() + rp[i]);
+    }
+    L.sw_rp.upload(rp, stream);
+    L.sw_col.upload(cl, stream);
+    L.sw_val.alloc(st.nnz);
+    L.sw_fresh = false;
+  }
   PMCD_HIP(hipStreamSynchronize(stream));
+}
+
+// class-ordered values up to date with val (and val32)
+static void refresh_sweep_values(Level& L, bool f32, hipStream_t stream) {
+  if(L.sweep_exact || L.sw_rp.n == 0) return;
+  if(!f32 && !L.sw_fresh) {
+    launch_permute_rows(L.n(), L.sweep_rows.p, L.row_ptr.p, L.sw_rp.p, L.val.p, L.sw_val.p, stream);
+    L.sw_fresh = true;
+  }
+  if(f32 && !L.sw_fresh32) {
+    if(L.sw_val32.n != L.val.n) L.sw_val32.alloc(L.val.n);
+    launch_permute_rows_f32(L.n(), L.sweep_rows.p, L.row_ptr.p, L.sw_rp.p, L.val32.p, L.sw_val32.p, stream);
+    L.sw_fresh32 = true;
+  }
 }
 
 void level_sor_sweep(Level& L, const double* b, double* x, double omega, int flag, hipStream_t stream,
                      const int* row_owner, const uint8_t* owner_active) {
   PMCD_CHECK(flag >= 0 && flag <= 2, "sor_sweep flag must be 0, 1 or 2");
   prepare_level_sweep(L, stream);
+  refresh_sweep_values(L, false, stream);
   const int K = (int)L.sweep_ptr.size() - 1;
+  const double mean_row = (double)L.st.nnz / std::max(1, L.n());
   auto cls = [&](int k) {
-    launch_sor_rows(L.sweep_ptr[k + 1] - L.sweep_ptr[k], L.sweep_rows.p + L.sweep_ptr[k], L.row_ptr.p,
-                    L.col.p, L.val.p, L.diag.p, b, x, omega, L.sweep_exact, L.st.max_row,
-                    (double)L.st.nnz / std::max(1, L.n()), stream, row_owner, owner_active);
+    const int cnt = L.sweep_ptr[k + 1] - L.sweep_ptr[k];
+    if(L.sweep_exact)
+      launch_sor_rows(cnt, L.sweep_rows.p + L.sweep_ptr[k], L.row_ptr.p, L.col.p, L.val.p, L.diag.p, b, x,
+                      omega, true, L.st.max_row, mean_row, stream, row_owner, owner_active);
+    else
+      launch_sor_rows_perm(cnt, L.sweep_ptr[k], L.sweep_rows.p, L.sw_rp.p, L.sw_col.p, L.sw_val.p, L.diag.p,
+                           b, x, omega, L.st.max_row, mean_row, stream, row_owner, owner_active);
   };
   if(flag == 0 || flag == 2)
     for(int k = 0; k < K; ++k) cls(k);
@@ -718,12 +752,18 @@ void level_sor_sweep32(Level& L, const float* b, float* x, float omega, int flag
   PMCD_CHECK(flag >= 0 && flag <= 2, "sor_sweep flag must be 0, 1 or 2");
   PMCD_CHECK(L.val32.n == L.val.n, "FP32 matrix copies not prepared");
   prepare_level_sweep(L, stream);
+  refresh_sweep_values(L, true, stream);
   const int K = (int)L.sweep_ptr.size() - 1;
+  const double mean_row = (double)L.st.nnz / std::max(1, L.n());
   auto cls = [&](int k) {
-    launch_sor_rows_f32(L.sweep_ptr[k + 1] - L.sweep_ptr[k], L.sweep_rows.p + L.sweep_ptr[k],
-                        L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x, omega, L.sweep_exact,
-                        L.st.max_row, (double)L.st.nnz / std::max(1, L.n()), stream, row_owner,
-                        owner_active);
+    const int cnt = L.sweep_ptr[k + 1] - L.sweep_ptr[k];
+    if(L.sweep_exact)
+      launch_sor_rows_f32(cnt, L.sweep_rows.p + L.sweep_ptr[k], L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b,
+                          x, omega, true, L.st.max_row, mean_row, stream, row_owner, owner_active);
+    else
+      launch_sor_rows_perm_f32(cnt, L.sweep_ptr[k], L.sweep_rows.p, L.sw_rp.p, L.sw_col.p, L.sw_val32.p,
+                               L.diag32.p, b, x, omega, L.st.max_row, mean_row, stream, row_owner,
+                               owner_active);
   };
   if(flag == 0 || flag == 2)
     for(int k = 0; k < K; ++k) cls(k);

@@ -85,6 +85,12 @@ struct Level {
   std::vector<int> sweep_ptr;
   DevBuf<int> sweep_rows;
   bool sweep_exact = false;
+  // multicolour sweeps read a copy of the matrix with the rows in class order
+  // (each row's entries in their original order), refreshed after assembly
+  DevBuf<int> sw_rp, sw_col;
+  DevBuf<double> sw_val;
+  DevBuf<float> sw_val32;
+  bool sw_fresh = false, sw_fresh32 = false;
   int n() const { return space.n_dof; }
   int n_active() const { return space.n_active; }
 };

@@ -174,6 +174,65 @@ def c4_cpu_baseline(dbs, sdb, per_gpu_solves, sample):
                       "(%d DoF, %.1f s), scaled by DoF to the whole batch" % (len(idx), len(dbs), dof_sum, t_sum)}
 
 
+C1_DB = {"geo_file": "TwoTriangles", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 8, "example": "sine_2d",
+         "diffusion_coefficient": 1.0, "convection": (0.0, 0.0, 0.0), "solver_type": "iterative",
+         "iterative_solver_type": "cg", "preconditioner": "jacobi", "residual_tolerance": 1e-12,
+         "max_n_iterations": 100000}
+
+
+def bench_c1(args):
+    """CD2D P1 on TwoTriangles refined 8 times (257^2 vertices), eps 1,
+    manufactured sin(pi x) sin(pi y), Dirichlet on all sides, CG + Jacobi to
+    relative residual 1e-12; the CPU baseline is the oracle on the whole
+    problem."""
+    import numpy as np
+    import parmoon2022_amd as pm
+    t0 = time.time()
+    cd = pm.ConvectionDiffusion(C1_DB)
+    info = cd.info()
+    log("setup %.1f s" % (time.time() - t0), info)
+    lat = []
+    for step in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        cd.assemble()
+        st = cd.solve()
+        dt = time.perf_counter() - t0
+        log("step %d: %.4f s (assembly %.2f ms, %d iterations)" % (step, dt, cd.assemble_ms, st.iterations))
+        if step >= args.warmup:
+            lat.append(dt)
+    errs = cd.output()
+    cd.close()
+    total = sum(lat)
+    value = args.steps / total
+    result = {
+        "metric": "CD3D solves/s & DoF/s at fixed L2 err vs FP64 oracle; p50/p99 solve latency",
+        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (manufactured solution, generated mesh)",
+        "config": {"workload": "C1 CD2D P1 TwoTriangles refined 8 times, %d DoF, nnz %d, CG + Jacobi, "
+                               "rel. res 1e-12" % (info["n_dof"], info["nnz"]), "dof": info["n_dof"]},
+        "dof_per_s": value * info["n_dof"],
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "iterations": int(st.iterations), "l2_error": float(errs[0]), "h1_error": float(errs[1]),
+    }
+    if not args.no_cpu:
+        import oracle
+        t0 = time.time()
+        o = oracle.Oracle(pm.problem_desc(C1_DB))
+        setup = time.time() - t0
+        t0 = time.time()
+        o.assemble()
+        so = o.solve(pm.solver_desc(C1_DB))
+        t_cpu = time.time() - t0
+        eo = o.errors()
+        result["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "solves/s", "cores": 1, "kind": "port",
+                                  "sample": "oracle (C, -O3, 1 thread): the whole C1 problem, assembly + CG "
+                                            "(%d iterations, %.2f s; setup %.1f s untimed), L2 error %.6e"
+                                            % (so.iterations, t_cpu, setup, eo[0])}
+        result["gpu_over_cpu"] = value * t_cpu
+    print(json.dumps(result), flush=True)
+
+
 def bench_c4(args):
     import numpy as np
     import parmoon2022_amd as pm
@@ -288,8 +347,9 @@ def bench_c5(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"],
-                    help="c2: one 64^3 Q2 SUPG solve per step (default); c4: a batch of --instances P1 "
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "c5"],
+                    help="c2: one 64^3 Q2 SUPG solve per step (default); c1: the CD2D P1 CG + Jacobi "
+                         "problem on 257^2 vertices; c4: a batch of --instances P1 "
                          "tetrahedral instances per step; c5: a served stream of mixed CD2D P2 / CD3D Q1 "
                          "requests at each of --rates")
     ap.add_argument("--rates", default="10,30,100,300,1000,2000", help="c5 arrival rates (requests/s)")
@@ -327,6 +387,8 @@ def main():
     import numpy as np
     import parmoon2022_amd as pm
 
+    if args.workload == "c1":
+        return bench_c1(args)
     if args.workload == "c4":
         return bench_c4(args)
     if args.workload == "c5":

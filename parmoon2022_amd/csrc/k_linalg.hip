@@ -276,27 +276,11 @@ __global__ void k_jacobi_from_zero(int n, int n_active, const T* __restrict__ di
   }
 }
 
-template <int W, class T>
-__global__ __launch_bounds__(256) void k_csr(int n, const int* __restrict__ rp,
-                                             const int* __restrict__ col,
-                                             const T* __restrict__ val, const T* __restrict__ x,
-                                             T* __restrict__ y) {
-  const int gt = blockIdx.x * 256 + threadIdx.x;
-  const int row = gt / W, l = gt % W;
-  if(row >= n) return;
-  T s = 0;
-  const int e = rp[row + 1];
-  for(int k = rp[row] + l; k < e; k += W) s += val[k] * x[col[k]];
-#pragma unroll
-  for(int off = W / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, W);
-  if(l == 0) y[row] = s;
-}
-
 // Grid transfer with the reference's non-active rules fused in (Multigrid.C:
 // 310-410): restriction writes (R d)[r] on active coarse rows and 0 on the
 // Dirichlet tail; prolongation adds a (P e)[r] to the fine iterate on active
 // rows only. U = 0 walks the row in a loop; U > 0 is the unrolled row sum
-// (U W >= the longest row). Same per-lane order and butterfly as k_csr.
+// (U W >= the longest row). Same per-lane order and butterfly as k_spmv.
 template <int W, int U, class T>
 __device__ inline T transfer_row(const int* __restrict__ rp, const int* __restrict__ col,
                                  const T* __restrict__ val, const T* __restrict__ x, int row, int l) {
@@ -491,23 +475,9 @@ __global__ void k_sum_max_final(int m, const double* __restrict__ partial, doubl
   }
 }
 
-__global__ void k_axpy_dev(int n, const double* __restrict__ a, double scale,
-                           const double* __restrict__ x, double* __restrict__ y) {
-  const double al = scale * a[0];
-  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    y[i] += al * x[i];
-}
-
 __global__ void k_axpy(int n, double a, const double* __restrict__ x, double* __restrict__ y) {
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     y[i] += a * x[i];
-}
-
-__global__ void k_scale_copy_dev(int n, const double* __restrict__ num, int invert,
-                                 const double* __restrict__ x, double* __restrict__ y) {
-  const double f = invert ? 1.0 / num[0] : num[0];
-  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    y[i] = x[i] * f;
 }
 
 template <class T>
@@ -583,17 +553,6 @@ __global__ void k_f2d(int n, const float* __restrict__ x, double* __restrict__ y
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     y[i] = (double)x[i];
 }
-template <class T>
-__global__ void k_axpy_active(int n_active, T a, const T* __restrict__ x, T* __restrict__ y) {
-  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_active; i += gridDim.x * blockDim.x)
-    y[i] += a * x[i];
-}
-template <class T>
-__global__ void k_zero_tail(int n, int n_active, T* __restrict__ y) {
-  for(int i = n_active + blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    y[i] = T(0);
-}
-
 // one wave per output row
 template <class T>
 __global__ __launch_bounds__(256) void k_gemv(int n, const T* __restrict__ M,
@@ -1133,22 +1092,6 @@ void launch_sor_rows_f32(int count, const int* rows, const int* rp, const int* c
                            row_owner, owner_active);
 }
 
-void launch_csr(int n, const int* rp, const int* col, const double* val, const double* x,
-                double* y, hipStream_t s) {
-  if(n == 0) return;
-  const int blocks = (int)(((long)n * 8 + 255) / 256);
-  k_csr<8, double><<<blocks, 256, 0, s>>>(n, rp, col, val, x, y);
-  PMCD_HIP(hipGetLastError());
-}
-
-void launch_csr_f32(int n, const int* rp, const int* col, const float* val, const float* x,
-                    float* y, hipStream_t s) {
-  if(n == 0) return;
-  const int blocks = (int)(((long)n * 8 + 255) / 256);
-  k_csr<8, float><<<blocks, 256, 0, s>>>(n, rp, col, val, x, y);
-  PMCD_HIP(hipGetLastError());
-}
-
 int reduce_blocks(int n) {
   int b = (n + 255) / 256;
   return std::max(1, std::min(b, kRedBlocks));
@@ -1170,16 +1113,8 @@ void launch_sum_max(int n, int stride, const double* v, double* partial, double*
   PMCD_HIP(hipGetLastError());
 }
 
-void launch_axpy_dev(int n, const double* a, double scale, const double* x, double* y,
-                     hipStream_t s) {
-  k_axpy_dev<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, scale, x, y);
-}
 void launch_axpy(int n, double a, const double* x, double* y, hipStream_t s) {
   k_axpy<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, x, y);
-}
-void launch_scale_copy_dev(int n, const double* num, int invert, const double* x, double* y,
-                           hipStream_t s) {
-  k_scale_copy_dev<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, num, invert, x, y);
 }
 void launch_fill(int n, double v, double* y, hipStream_t s) {
   if(n) k_fill<double><<<grid_for(n, 256 * 4), 256, 0, s>>>(n, v, y);
@@ -1222,24 +1157,7 @@ void launch_d2f(int n, const double* x, float* y, hipStream_t s) {
 void launch_f2d(int n, const float* x, double* y, hipStream_t s) {
   if(n) k_f2d<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, x, y);
 }
-void launch_axpy_f32(int n, float a, const float* x, float* y, int n_active, hipStream_t s) {
-  (void)n;
-  if(n_active) k_axpy_active<float><<<grid_for(n_active, 256 * 4), 256, 0, s>>>(n_active, a, x, y);
-}
-void launch_axpy_active(int n_active, double a, const double* x, double* y, hipStream_t s) {
-  if(n_active) k_axpy_active<double><<<grid_for(n_active, 256 * 4), 256, 0, s>>>(n_active, a, x, y);
-}
 void launch_gemv(int n, const double* M, const double* x, double* y, hipStream_t s) {
   if(n) k_gemv<double><<<(n + 3) / 4, 256, 0, s>>>(n, M, x, y);
 }
-void launch_gemv_f32(int n, const float* M, const float* x, float* y, hipStream_t s) {
-  if(n) k_gemv<float><<<(n + 3) / 4, 256, 0, s>>>(n, M, x, y);
-}
-void launch_zero_tail(int n, int n_active, double* y, hipStream_t s) {
-  if(n > n_active) k_zero_tail<double><<<grid_for(n - n_active, 256), 256, 0, s>>>(n, n_active, y);
-}
-void launch_zero_tail_f32(int n, int n_active, float* y, hipStream_t s) {
-  if(n > n_active) k_zero_tail<float><<<grid_for(n - n_active, 256), 256, 0, s>>>(n, n_active, y);
-}
-
 }  // namespace pmcd

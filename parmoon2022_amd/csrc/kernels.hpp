@@ -129,10 +129,6 @@ void launch_prolong_add(int n, int n_active, long nnz, int max_row, const int* r
 void launch_prolong_add_f32(int n, int n_active, long nnz, int max_row, const int* rp,
                             const int* col, const float* val, const float* x, float* y, float a,
                             hipStream_t s);
-void launch_csr(int n, const int* rp, const int* col, const double* val, const double* x,
-                double* y, hipStream_t s);
-void launch_csr_f32(int n, const int* rp, const int* col, const float* val, const float* x,
-                    float* y, hipStream_t s);
 
 // deterministic reductions: result written to out[0] (device)
 int reduce_blocks(int n);
@@ -142,11 +138,7 @@ void launch_sum_max(int n, int stride, const double* v, double* partial, double*
                     hipStream_t s);
 
 // vector kernels
-void launch_axpy_dev(int n, const double* alpha_dev, double alpha_scale, const double* x,
-                     double* y, hipStream_t s);  // y += (alpha_scale * *alpha_dev) x
 void launch_axpy(int n, double alpha, const double* x, double* y, hipStream_t s);
-void launch_scale_copy_dev(int n, const double* num_dev, int invert, const double* x, double* y,
-                           hipStream_t s);  // y = x * (invert ? 1/(*num) : *num)
 void launch_fill(int n, double v, double* y, hipStream_t s);
 void launch_copy(int n, const double* x, double* y, hipStream_t s);
 // y = x for n doubles (n even, 16-byte aligned), 16-byte vector accesses
@@ -160,14 +152,8 @@ void launch_mgs_step(int n, const double* h_dev, double* r, const double* v,
 void launch_d2f(int n, const double* x, float* y, hipStream_t s);
 void launch_f2d(int n, const float* x, double* y, hipStream_t s);
 void launch_fill_f32(int n, float v, float* y, hipStream_t s);
-void launch_axpy_f32(int n, float alpha, const float* x, float* y, int n_active, hipStream_t s);
-void launch_axpy_active(int n_active, double alpha, const double* x, double* y, hipStream_t s);
 // dense y = M x (M row-major n x n)
 void launch_gemv(int n, const double* M, const double* x, double* y, hipStream_t s);
-void launch_gemv_f32(int n, const float* M, const float* x, float* y, hipStream_t s);
-// zero entries [n_active, n)
-void launch_zero_tail(int n, int n_active, double* y, hipStream_t s);
-void launch_zero_tail_f32(int n, int n_active, float* y, hipStream_t s);
 // y = a x ; y *= a ; y = x / d
 void launch_scale_copy(int n, double a, const double* x, double* y, hipStream_t s);
 void launch_scale_inplace(int n, double a, double* y, hipStream_t s);

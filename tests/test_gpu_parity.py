@@ -602,3 +602,28 @@ def test_vcycle_graph_replays_match_eager(smoother, mixed):
     finally:
         pm.set_vcycle_graphs(old)
     cd.close()
+
+
+def test_c1_full_size_matches_oracle(oracle_lib):
+    """C1 at its full size (TwoTriangles refined 8 times, 66,049 DoF): CG +
+    Jacobi to 1e-12 relative residual on the device and in the oracle: same
+    iteration count within rounding drift, solution to 1e-10, L2/H1 errors to
+    3 significant digits."""
+    db = {"geo_file": "TwoTriangles", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 8, "example": "sine_2d",
+          "diffusion_coefficient": 1.0, "solver_type": "iterative", "iterative_solver_type": "cg",
+          "preconditioner": "jacobi", "residual_tolerance": 1e-12, "max_n_iterations": 100000}
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    st = cd.solve()
+    eg = cd.output()
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    so = o.solve(pm.solver_desc(db))
+    eo = o.errors()
+    rel = np.linalg.norm(cd.solution() - o.solution()) / np.linalg.norm(o.solution())
+    print("C1: CG iterations gpu %d oracle %d, rel %.3e, L2 %.6e vs %.6e" % (st.iterations, so.iterations, rel,
+                                                                             eg[0], eo[0]), flush=True)
+    assert st.converged and abs(st.iterations - so.iterations) <= max(2, 0.01 * so.iterations)
+    assert rel < 1e-10
+    assert abs(eg[0] - eo[0]) <= 1e-3 * eo[0] and abs(eg[1] - eo[1]) <= 1e-3 * eo[1]
+    cd.close()

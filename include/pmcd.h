@@ -203,7 +203,9 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
  * read when a level's sweep order is first built; key 6 = 1 to replay
  * multigrid V-cycles from captured HIP graphs, 0 to launch them kernel by
  * kernel (default); key 4 = assembly kernel (0 reference operation order on
- * vector ALUs, 1 FP64 MFMA). */
+ * vector ALUs, 1 FP64 MFMA); key 7 = 1 (default) to run CG with no or Jacobi
+ * preconditioning as two fused kernels per iteration with its scalars on the
+ * device, 0 for the loop that reads every scalar back to the host. */
 int pmcd_set_tuning(int key, int value);
 /* Current value of a launch-shape knob (keys as for pmcd_set_tuning). */
 int pmcd_get_tuning(int key, int* value);

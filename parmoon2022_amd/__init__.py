@@ -96,6 +96,19 @@ def get_vcycle_graphs():
     return bool(v.value)
 
 
+def set_cg_device(on):
+    """CG with no or Jacobi preconditioning: scalars kept on the device, two
+    fused kernels per iteration (default), or the loop that reads every
+    scalar back to the host."""
+    _check(lib().pmcd_set_tuning(7, 1 if on else 0))
+
+
+def get_cg_device():
+    v = ctypes.c_int(0)
+    _check(lib().pmcd_get_tuning(7, ctypes.byref(v)))
+    return bool(v.value)
+
+
 def get_sor_exact_limit():
     v = ctypes.c_int(0)
     _check(lib().pmcd_get_tuning(5, ctypes.byref(v)))

@@ -797,6 +797,10 @@ int sor_exact_limit() { return g_sor_exact_limit; }
 // (0, default: replay measured no faster on these kernel sequences)
 static int g_vcycle_graphs = 0;
 int vcycle_graphs() { return g_vcycle_graphs; }
+static int g_cg_device = 1;
+int cg_device() { return g_cg_device; }
+static int g_cg_blocks = 1024;
+int cg_block_cap() { return g_cg_blocks; }
 // changes whenever a launch shape changes, so that captured graphs are rebuilt
 static int g_tuning_epoch = 0;
 int tuning_epoch() { return g_tuning_epoch; }
@@ -806,6 +810,16 @@ void set_tuning(int key, int value) {
   if(key == 6) {
     PMCD_CHECK(value == 0 || value == 1, "graph switch must be 0 or 1");
     g_vcycle_graphs = value;
+    return;
+  }
+  if(key == 7) {
+    PMCD_CHECK(value == 0 || value == 1, "CG switch must be 0 or 1");
+    g_cg_device = value;
+    return;
+  }
+  if(key == 8) {
+    PMCD_CHECK(value >= 1 && value <= kCgBlocks, "CG block cap must be in 1..", kCgBlocks);
+    g_cg_blocks = value;
     return;
   }
   if(key == 5) {
@@ -840,6 +854,8 @@ int get_tuning(int key) {
     case 4: return g_assembly_variant;
     case 5: return g_sor_exact_limit;
     case 6: return g_vcycle_graphs;
+    case 7: return g_cg_device;
+    case 8: return g_cg_blocks;
     default: PMCD_FAIL("unknown tuning key ", key);
   }
 }

@@ -117,6 +117,10 @@ void launch_permute_rows(int n, const int* rows, const int* rp, const int* sw_rp
 void launch_permute_rows_f32(int n, const int* rows, const int* rp, const int* sw_rp, const float* val,
                              float* sw_val, hipStream_t s);
 int vcycle_graphs();
+// 1: CG with no or Jacobi preconditioning keeps its scalars on the device
+int cg_device();
+// largest grid of the device CG kernels (1..kCgBlocks)
+int cg_block_cap();
 int tuning_epoch();
 // restriction y = R x on active rows, 0 on rows >= n_active
 void launch_restrict(int n, int n_active, long nnz, int max_row, const int* rp, const int* col,
@@ -194,5 +198,43 @@ void launch_spmv_masked(int n, int n_active, const int* rp, const int* col, cons
                         int max_row, const int* row_inst, const uint8_t* active, hipStream_t s);
 void launch_seg_axpy(int n_chunks, const SegChunk* ch, const double* a, const double* x, double* y,
                      hipStream_t s);
+
+// ---- conjugate gradients with device-resident scalars ----------------------
+// State of one CG solve. Scalars that a kernel both reads and writes sit in
+// two slots (iteration k in slot k & 1), so that no block reads a slot that
+// another block of the same launch writes; block 0 writes, every block
+// computes the same values. The host copies the state back now and then to
+// see whether the iteration has stopped.
+struct CgState {
+  double rho[2];     // rho_k = (r_{k-1}, z_k), z_k = M r_{k-1}
+  double res[2];     // relative residual after iteration k
+  double normb;      // residuals are |r| / normb
+  double initial;    // residual before the first iteration
+  double res_tol, res_red, div_factor;
+  double final_res;  // residual at the stop
+  int iter;          // iterations done at the stop
+  int done;          // 0 running, 1 stopped (converged or limit), 2 diverged
+  int min_iter, max_iter;
+};
+// Grids of the two CG kernels (the direction kernel's is capped by
+// cg_block_cap(), the update kernel's by kCgUpdateBlocks) and the sizes of
+// their partial-sum buffers
+constexpr int kCgBlocks = 4096;
+constexpr int kCgUpdateBlocks = 1024;
+int cg_direction_blocks(int n, int max_row);
+int cg_update_blocks(int n);
+// Iteration `it` (1-based), first kernel: stop test of iteration it - 1 from
+// the update kernel's partials pu ([2][nbu]: |r|^2, (r, z)), beta, then
+// p_new = z (it = 1) or beta p_old + z, q = A p_new (non-active rule) and the
+// per-block partials pd of (p_new, q). No-op once st->done is set; launched
+// with it = max_iter + 1 it only applies the final stop test.
+void launch_cg_direction(int it, int n, int n_active, const int* rp, const int* col, const double* val,
+                         int max_row, const double* z, const double* p_old, double* p_new, double* q,
+                         const double* pu, int nbu, double* pd, CgState* st, hipStream_t s);
+// Iteration `it`, second kernel: alpha = rho_it / (p, q) from the partials pd
+// ([nbd]), x += alpha p, r -= alpha q, z = M r, partials of |r|^2 and (r, z)
+// into pu.
+void launch_cg_update(int it, int n, const double* p, const double* q, const double* diag, double* x,
+                      double* r, double* z, const double* pd, int nbd, double* pu, CgState* st, hipStream_t s);
 
 }  // namespace pmcd

@@ -190,7 +190,8 @@ SolveStats CDSystem::solve_cg() {
   Level& F = finest();
   const int n = F.n();
   Convergence conv(sd_);
-  DevBuf<double> p(n), z(n), q(n), r(n);
+  if(kr_.n != (size_t)n) { kr_.alloc(n); kw_.alloc(n); }
+  DevBuf<double>& r = kr_;
   double* x = F.sol.p;
   SolveStats st;
   launch_copy(n, F.rhs.p, r.p, stream_);
@@ -202,6 +203,15 @@ SolveStats CDSystem::solve_cg() {
     launch_fill(n, 0.0, x, stream_);
   }
   double resid = std::sqrt(dot(n, r.p, r.p)) / normb;
+  if((sd_.precond == 0 || sd_.precond == 1) && cg_device()) {
+    if(conv(resid, 0)) {
+      st.initial_residual = st.final_residual = resid;
+      st.converged = 1;
+      return st;
+    }
+    return solve_cg_device(n, normb, resid);
+  }
+  DevBuf<double> p(n), z(n), q(n);
   rhs_nonactive_zero_ = tail_is_zero(r.p);
   st.initial_residual = resid;
   if(conv(resid, 0)) { st.final_residual = resid; st.converged = 1; return st; }
@@ -226,6 +236,88 @@ SolveStats CDSystem::solve_cg() {
   }
   st.final_residual = resid;
   st.converged = resid < sd_.res_tol || resid < sd_.res_red * st.initial_residual;
+  return st;
+}
+
+// CG with no or Jacobi preconditioning, scalars on the device (k_krylov.hip):
+// the residual r = b - A x is in kr_, x in the finest solution vector.
+// Iterations are launched in chunks; the state after chunk k is read back
+// while chunk k + 1 runs, and iterations launched after the stop do nothing.
+SolveStats CDSystem::solve_cg_device(int n, double normb, double resid) {
+  Level& F = finest();
+  if(cg_z_.n != (size_t)n) {
+    for(auto& b : cg_p_) b.alloc(n);
+    cg_z_.alloc(n);
+    cg_q_.alloc(n);
+  }
+  if(cg_pd_.n == 0) {
+    cg_pd_.alloc(kCgBlocks);
+    cg_pu_.alloc(2 * kCgUpdateBlocks);
+  }
+  if(cg_state_.n == 0) cg_state_.alloc(1);
+  if(!cg_host_) PMCD_HIP(hipHostMalloc((void**)&cg_host_, 2 * sizeof(CgState), hipHostMallocDefault));
+  const double* diag = sd_.precond == 1 ? F.diag.p : nullptr;
+  if(diag) launch_div(n, kr_.p, diag, cg_z_.p, stream_);
+  else launch_copy(n, kr_.p, cg_z_.p, stream_);
+  CgState s0{};
+  s0.rho[1] = dot(n, kr_.p, cg_z_.p);
+  s0.res[0] = resid;
+  s0.normb = normb;
+  s0.initial = resid;
+  s0.res_tol = sd_.res_tol;
+  s0.res_red = sd_.res_red;
+  s0.div_factor = sd_.div_factor;
+  s0.min_iter = sd_.min_iter;
+  s0.max_iter = sd_.max_iter;
+  cg_host_[0] = s0;
+  PMCD_HIP(hipMemcpyAsync(cg_state_.p, cg_host_, sizeof(CgState), hipMemcpyHostToDevice, stream_));
+  PMCD_HIP(hipStreamSynchronize(stream_));
+  const int nbd = cg_direction_blocks(n, F.st.max_row), nbu = cg_update_blocks(n);
+  auto direction = [&](int it) {
+    launch_cg_direction(it, n, F.n_active(), F.row_ptr.p, F.col.p, F.val.p, F.st.max_row, cg_z_.p,
+                        cg_p_[(it - 1) & 1].p, cg_p_[it & 1].p, cg_q_.p, cg_pu_.p, nbu, cg_pd_.p, cg_state_.p,
+                        stream_);
+  };
+  hipEvent_t ev[2];
+  for(auto& e : ev) PMCD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  int launched = 0, chunk = 8, k = 0;
+  bool stopped = false;
+  while(!stopped) {
+    const int m = std::min(chunk, sd_.max_iter - launched);
+    for(int j = 1; j <= m; ++j) {
+      const int it = launched + j;
+      direction(it);
+      launch_cg_update(it, n, cg_p_[it & 1].p, cg_q_.p, diag, F.sol.p, kr_.p, cg_z_.p, cg_pd_.p, nbd, cg_pu_.p,
+                       cg_state_.p, stream_);
+    }
+    launched += m;
+    const bool last = launched >= sd_.max_iter;
+    if(last) direction(sd_.max_iter + 1);  // the stopping test of the final iteration
+    PMCD_HIP(hipMemcpyAsync(&cg_host_[k & 1], cg_state_.p, sizeof(CgState), hipMemcpyDeviceToHost, stream_));
+    PMCD_HIP(hipEventRecord(ev[k & 1], stream_));
+    if(last) {
+      PMCD_HIP(hipEventSynchronize(ev[k & 1]));
+      stopped = true;
+    } else if(k > 0) {
+      PMCD_HIP(hipEventSynchronize(ev[(k - 1) & 1]));
+      stopped = cg_host_[(k - 1) & 1].done != 0;
+    }
+    ++k;
+    chunk = std::min(2 * chunk, 32);
+  }
+  PMCD_HIP(hipStreamSynchronize(stream_));
+  for(auto& e : ev) PMCD_HIP(hipEventDestroy(e));
+  CgState fin;
+  PMCD_HIP(hipMemcpy(&fin, cg_state_.p, sizeof(CgState), hipMemcpyDeviceToHost));
+  PMCD_CHECK(fin.done != 0, "device CG ended without a stopping decision");
+  if(fin.done == 2)
+    PMCD_FAIL("Execution of the code stopped because of solver divergence: ", fin.initial, " -> ",
+              fin.final_res, " at iteration ", fin.iter);
+  SolveStats st;
+  st.initial_residual = resid;
+  st.iterations = fin.iter;
+  st.final_residual = fin.final_res;
+  st.converged = fin.final_res < sd_.res_tol || fin.final_res < sd_.res_red * resid;
   return st;
 }
 

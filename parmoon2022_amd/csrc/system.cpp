@@ -99,6 +99,7 @@ CDSystem::CDSystem(const ProblemDesc& d, hipStream_t stream) : desc_(d), stream_
 CDSystem::~CDSystem() {
   (void)hipStreamSynchronize(stream_);
   clear_graphs();
+  if(cg_host_) (void)hipHostFree(cg_host_);
 }
 
 void CDSystem::clear_graphs() {

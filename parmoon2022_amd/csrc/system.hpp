@@ -159,6 +159,7 @@ class CDSystem {
   void clear_graphs();
   void precond(const double* v, double* z);
   SolveStats solve_cg();
+  SolveStats solve_cg_device(int n, double normb, double resid);
   SolveStats solve_fgmres();
   SolveStats solve_direct();
   SolveStats solve_richardson();
@@ -194,6 +195,11 @@ class CDSystem {
   // Krylov work space
   std::vector<DevBuf<double>> kv_, kz_;
   DevBuf<double> kr_, kw_, kh_;
+  // device-resident CG: direction ping-pong, state, partial sums and a
+  // pinned host copy of the state
+  DevBuf<double> cg_p_[2], cg_z_, cg_q_, cg_pd_, cg_pu_;
+  DevBuf<CgState> cg_state_;
+  CgState* cg_host_ = nullptr;
   AsmArgs asm_args(const Level& L);
 };
 

@@ -627,3 +627,57 @@ def test_c1_full_size_matches_oracle(oracle_lib):
     assert rel < 1e-10
     assert abs(eg[0] - eo[0]) <= 1e-3 * eo[0] and abs(eg[1] - eo[1]) <= 1e-3 * eo[1]
     cd.close()
+
+
+def _cg_both(db):
+    """Solve `db` with the device-resident CG and with the host-scalar CG
+    loop; returns (stats, solution) of each."""
+    old = pm.get_cg_device()
+    out = []
+    try:
+        for on in (True, False):
+            pm.set_cg_device(on)
+            cd = pm.ConvectionDiffusion(db)
+            cd.assemble()
+            st = cd.solve()
+            out.append((st, cd.solution()))
+            cd.close()
+    finally:
+        pm.set_cg_device(old)
+    return out
+
+
+@pytest.mark.parametrize("case", ["p1_jacobi", "q2_hexa_none", "p2_tet_jacobi"])
+def test_cg_device_matches_host_loop(case):
+    """The device-resident CG (two fused kernels per iteration, last-block
+    reductions) against the loop that reads every scalar back: same iteration
+    count up to the summation order of the dot products, same solution."""
+    db = {"p1_jacobi": {"geo_file": "TwoTriangles", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 6,
+                        "example": "sine_2d", "diffusion_coefficient": 1.0, "preconditioner": "jacobi"},
+          "q2_hexa_none": {"geo_file": "Default_UnitCube_Hexa", "ANSATZ_ORDER": 2, "refinement_n_initial_steps": 3,
+                           "example": "sine_3d", "diffusion_coefficient": 1.0, "preconditioner": "no_preconditioner"},
+          "p2_tet_jacobi": {"geo_file": "Default_UnitCube_Tetra", "ANSATZ_ORDER": 2,
+                            "refinement_n_initial_steps": 2, "example": "sine_3d", "diffusion_coefficient": 1.0,
+                            "preconditioner": "jacobi"}}[case]
+    db.update({"solver_type": "iterative", "iterative_solver_type": "cg", "residual_tolerance": 1e-11,
+               "max_n_iterations": 5000})
+    (sd, ud), (sh, uh) = _cg_both(db)
+    rel = np.linalg.norm(ud - uh) / np.linalg.norm(uh)
+    print("%s: iterations device %d host %d, residual %.3e / %.3e, rel %.3e" %
+          (case, sd.iterations, sh.iterations, sd.final_residual, sh.final_residual, rel), flush=True)
+    assert sd.converged and sh.converged
+    assert abs(sd.iterations - sh.iterations) <= max(1, 0.01 * sh.iterations)
+    assert rel < 1e-9
+
+
+def test_cg_device_iteration_limit():
+    """Stopping at max_n_iterations: the device CG runs exactly that many
+    iterations (the ones launched after the stop do nothing) and leaves the
+    same iterate as the host loop."""
+    db = {"geo_file": "TwoTriangles", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 6, "example": "sine_2d",
+          "diffusion_coefficient": 1.0, "preconditioner": "jacobi", "solver_type": "iterative",
+          "iterative_solver_type": "cg", "residual_tolerance": 1e-14, "max_n_iterations": 13}
+    (sd, ud), (sh, uh) = _cg_both(db)
+    assert sd.iterations == sh.iterations == 13
+    assert np.linalg.norm(ud - uh) <= 1e-12 * np.linalg.norm(uh)
+    assert abs(sd.final_residual - sh.final_residual) <= 1e-10 * sh.final_residual

@@ -115,9 +115,13 @@ def kernel_times(cd, info, mixed):
     copy_bytes = 2 * 8 * (nnz - nnz % 2)
     asm_flops = (2 * 27 * 27 * 4 + 2 * 4 * 5 * 27) * info["n_quad"] * info["n_cells"]
     lanes, variant = pm.tuning("jacobi")
+    if variant == 4 and n - info["n_dirichlet"] >= pm.get_sell_min_rows():
+        jac_name = "k_jacobi_sell<%d%s>" % (pm.get_sell_unroll(), ",f32" if mixed else ",f64")
+    else:
+        jac_name = "k_%s%s<%d%s>" % ("jacobi", {0: "", 1: "_stream", 2: "_unrolled", 3: "_unrolled_nt",
+                                                4: "_unrolled"}[variant], lanes, ",f32" if mixed else ",f64")
     return {
-        "jacobi_kernel": "k_%s%s<%d%s>" % ("jacobi", {0: "", 1: "_stream", 2: "_unrolled"}[variant],
-                                          lanes, ",f32" if mixed else ",f64"),
+        "jacobi_kernel": jac_name,
         "jacobi_ms": t_jac, "jacobi_bytes": jac_bytes,
         "spmv_f64_ms": t_spmv, "spmv_f64_GBs": spmv_bytes / (t_spmv * 1e-3) / 1e9,
         "stream_copy_ms": t_copy, "stream_copy_bytes": copy_bytes,  # the faster of the two copies
@@ -408,10 +412,12 @@ def main():
         pm.set_tuning("jacobi", lanes, variant)
         tuning = {}
     else:
-        tuning = pm.autotune(cd, log=log)
-    if args.kernels_only:
         if args.mixed:
             cd.solve()  # the FP32 copies of the matrices are made by the solver's set-up
+        tuning = pm.autotune(cd, log=log, mixed=args.mixed)
+    if args.kernels_only:
+        if args.mixed:
+            cd.solve()
         print(json.dumps(kernel_times(cd, info, args.mixed)), flush=True)
         cd.close()
         return

@@ -122,6 +122,12 @@ int pmcd_host_cell_coords(pmcd_host* h, int level, double* out);
  * No reference counterpart. */
 int pmcd_host_sweep_order(pmcd_host* h, int level, int max_exact_classes, int* rows, int* class_ptr,
                           int* n_classes, int* exact);
+/* Sliced ELL layout of the level's active rows as the device builds it:
+ * perm and len (n_active + 63 ints each: slot -> row, -1 on padding, and the
+ * row length), slice_off ((n_active + 63) / 64 + 1 entries: first entry of
+ * each slice of 64 slots; entry k of the row in slot t sits at
+ * slice_off[t / 64] + 64 k + t % 64). No reference counterpart. */
+int pmcd_host_sell_layout(pmcd_host* h, int level, int* perm, int* len, long long* slice_off, int* n_slots);
 
 /* Mesh hierarchy, FE spaces, sparsity, device buffers.
  * Replaces ConvectionDiffusion<d>::ConvectionDiffusion
@@ -182,6 +188,11 @@ int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, dou
  * (bit-identical result), 0 when it uses multicolour order. */
 int pmcd_sor_sweep(pmcd_system* s, int level, const double* b, double* x, double omega, int flag,
                    int* exact);
+/* One damped Jacobi step x <- x + omega D^{-1} (b - A x) on level `level`, in
+ * place on host arrays (JacobiSmoother::smooth, src/Multigrid/
+ * JacobiSmoother.C:22-30; non-active rows with A = I), with the kernel the
+ * smoother uses (CSR or sliced ELL, tuning key 3). */
+int pmcd_jacobi_sweep(pmcd_system* s, int level, const double* b, double* x, double omega);
 
 /* Average device time (HIP events on the system's stream) of `iters`
  * back-to-back launches of one kernel on level `level`:
@@ -197,7 +208,14 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
  * kernels (4, 8, 16, 32); key 2 / 3 = their variant (0 plain, 1 streaming with
  * XCD-contiguous rows, 2 unrolled with all loads of a lane in flight at once,
  * taken where the level's longest row fits 16 entries per lane, 3 unrolled
- * with non-temporal matrix loads); key 5 = the
+ * with non-temporal matrix loads, 4 sliced ELL: one lane per row, each row
+ * summed serially in column order as TMatrix::multiply does, so the row sums
+ * are bitwise those of the serial loop; calls on raw CSR arrays take variant
+ * 2 instead); key 9 = entries per lane loaded per step by the sliced ELL
+ * kernels (4, 8, 16; default 16); key 10 = rows of a sliced ELL
+ * length-sorting window, in slices of 64 (default 4; read when a level's
+ * layout is first built); key 11 = levels with fewer active rows than this
+ * keep the unrolled CSR kernels under variant 4 (default 0); key 5 = the
  * largest number of level-schedule classes for which SOR sweeps follow the
  * exact serial order (default 24; deeper schedules use multicolour order),
  * read when a level's sweep order is first built; key 6 = 1 to replay
@@ -205,7 +223,8 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
  * kernel (default); key 4 = assembly kernel (0 reference operation order on
  * vector ALUs, 1 FP64 MFMA); key 7 = 1 (default) to run CG with no or Jacobi
  * preconditioning as two fused kernels per iteration with its scalars on the
- * device, 0 for the loop that reads every scalar back to the host. */
+ * device, 0 for the loop that reads every scalar back to the host; key 8 =
+ * the largest grid of the device CG's SpMV kernel (1..4096 blocks). */
 int pmcd_set_tuning(int key, int value);
 /* Current value of a launch-shape knob (keys as for pmcd_set_tuning). */
 int pmcd_get_tuning(int key, int* value);

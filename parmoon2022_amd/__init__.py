@@ -57,7 +57,8 @@ class SolveStats(ctypes.Structure):
 EXPORTS = [
     "pmcd_last_error", "pmcd_device_count", "pmcd_mem_info", "pmcd_system_set_data",
     "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
-    "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_sweep_order",
+    "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_sweep_order", "pmcd_host_sell_layout",
+    "pmcd_jacobi_sweep",
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
     "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
     "pmcd_get_rhs", "pmcd_solve", "pmcd_get_solution", "pmcd_set_solution", "pmcd_errors",
@@ -96,6 +97,17 @@ def get_vcycle_graphs():
     return bool(v.value)
 
 
+def set_sell_unroll(u):
+    """Entries per lane loaded per step by the sliced ELL kernels (4, 8, 16)."""
+    _check(lib().pmcd_set_tuning(9, int(u)))
+
+
+def get_sell_unroll():
+    v = ctypes.c_int(0)
+    _check(lib().pmcd_get_tuning(9, ctypes.byref(v)))
+    return v.value
+
+
 def set_cg_device(on):
     """CG with no or Jacobi preconditioning: scalars kept on the device, two
     fused kernels per iteration (default), or the loop that reads every
@@ -127,11 +139,12 @@ def probe_mfma():
 
 
 def set_tuning(kernel, lanes_per_row, variant=0):
-    """Launch shape of the 'spmv' or 'jacobi' CSR kernels: lanes per row (4, 8,
+    """Launch shape of the 'spmv' or 'jacobi' kernels: lanes per row (4, 8,
     16, 32) and variant (0 plain, 1 streaming with XCD-contiguous rows, 2
     unrolled: all loads of a lane issued before the products, used where the
     longest row fits 16 entries per lane, 3 unrolled with non-temporal matrix
-    loads)."""
+    loads, 4 sliced ELL: one lane per row, rows summed serially in column
+    order; the lane count is not used)."""
     k = {"spmv": 0, "jacobi": 1}[kernel]
     _check(lib().pmcd_set_tuning(k, int(lanes_per_row)))
     _check(lib().pmcd_set_tuning(k + 2, int(variant)))
@@ -263,6 +276,21 @@ class HostSetup:
                                            _ptr(ptr, ctypes.c_int), ctypes.byref(k), ctypes.byref(ex)))
         return rows, ptr[:k.value + 1].copy(), bool(ex.value)
 
+    def sell_layout(self, level=-1):
+        """Sliced ELL layout of the level's active rows: (perm, len, slice_off),
+        slot t of slice t // 64 holding row perm[t] (-1: padding) whose entry k
+        sits at slice_off[t // 64] + 64 k + t % 64."""
+        na = self.info(level)["n_active"]
+        m = (na + 63) // 64 * 64 + 64
+        perm = np.empty(m, dtype=np.int32)
+        ln = np.empty(m, dtype=np.int32)
+        off = np.empty(m // 64 + 1, dtype=np.int64)
+        ns = ctypes.c_int(0)
+        _check(lib().pmcd_host_sell_layout(self._h, level, _ptr(perm, ctypes.c_int), _ptr(ln, ctypes.c_int),
+                                           _ptr(off, ctypes.c_longlong), ctypes.byref(ns)))
+        k = ns.value
+        return perm[:k].copy(), ln[:k].copy(), off[:k // 64 + 1].copy()
+
     def cell_coords(self, level=-1):
         i = self.info(level)
         nv = VERTICES_PER_CELL[self.desc.geometry]
@@ -382,6 +410,14 @@ class ConvectionDiffusion:
                                 _ptr(out, ctypes.c_double), ctypes.c_double(a), mode))
         return out
 
+    def jacobi_sweep(self, b, x, omega=0.8, level=-1):
+        """JacobiSmoother::smooth, one step: x + omega D^{-1} (b - A x)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.array(x, dtype=np.float64)
+        _check(lib().pmcd_jacobi_sweep(self._s, level, _ptr(b, ctypes.c_double), _ptr(out, ctypes.c_double),
+                                       ctypes.c_double(omega)))
+        return out
+
     def sor_sweep(self, b, x, omega=1.5, flag=2, level=-1):
         """TMatrix::sor_sweep on the level's matrix: returns (new x, exact) where
         exact tells whether the device order reproduces the serial sweep."""
@@ -484,26 +520,78 @@ def c4_instances(n_inst=256, seed=20261202, n_range=(8, 40)):
     return out
 
 
-def autotune(cd, level=-1, iters=10, log=None):
-    """Pick lanes-per-row of the SpMV and Jacobi kernels for this matrix by
-    timing each candidate with HIP events; sets the fastest and returns the
-    timings {kernel: {lanes: ms}}."""
-    names = ("plain", "stream", "unrolled", "unrollednt")
-    out = {}
-    for kernel, timed in (("spmv", "spmv_f64"), ("jacobi", "jacobi_f64")):
+def autotune(cd, level=-1, iters=10, log=None, mixed=False):
+    """Pick the launch shape of the SpMV and Jacobi kernels for this matrix by
+    timing each candidate with HIP events: CSR variants x lanes per row, and
+    the sliced ELL kernels. Sets the fastest; when the sliced ELL kernels win
+    on `level`, the coarser levels on which they lose keep the CSR kernels
+    (set_sell_min_rows). With `mixed` the Jacobi sweep is timed in FP32, the
+    precision the V-cycle runs in (the system must have solved once, which
+    makes its FP32 copies). Returns the timings {kernel: {shape: ms}}."""
+    names = ("plain", "stream", "unrolled", "unrollednt", "sell")
+    set_sell_min_rows(0)
+    out, chosen, csr_best = {}, {}, {}
+    kernels = (("spmv", "spmv_f64"), ("jacobi", "jacobi_f32" if mixed else "jacobi_f64"))
+    for kernel, timed in kernels:
         res = {}
         for variant in (0, 1, 2, 3):
             for w in (8, 16, 32):
                 set_tuning(kernel, w, variant)
                 res[(names[variant], w)] = cd.time_kernel(timed, level, iters)
+        set_tuning(kernel, 16, 4)
+        res[("sell", 16)] = cd.time_kernel(timed, level, iters)
         best = min(res, key=res.get)
-        set_tuning(kernel, best[1], names.index(best[0]))
-        res = {"%s%d" % k: v for k, v in res.items()}
-        best = "%s%d" % best
-        out[kernel] = res
+        csr_best[kernel] = min((k for k in res if k[0] != "sell"), key=res.get)
+        # under variant 4 the lane count is that of the CSR fallback on small levels
+        set_tuning(kernel, csr_best[kernel][1] if best[0] == "sell" else best[1], names.index(best[0]))
+        chosen[kernel] = best
+        out[kernel] = {"%s%d" % k: v for k, v in res.items()}
         if log:
-            log("autotune %s: %s -> %s" % (kernel, {k: round(v, 4) for k, v in res.items()}, best))
+            log("autotune %s: %s -> %s%d" % (kernel, {k: round(v, 4) for k, v in out[kernel].items()}, *best))
+    # below `level`, keep the sliced ELL kernels while they stay faster
+    info = cd.info()
+    finest = info["n_levels"] - 1 if level < 0 else level
+    threshold = 0
+    for kernel, timed in kernels:
+        if chosen[kernel][0] != "sell":
+            continue
+        lanes = csr_best[kernel][1]
+        for lev in range(finest - 1, -1, -1):
+            set_tuning(kernel, lanes, 4)
+            t_sell = cd.time_kernel(timed, lev, iters)
+            set_tuning(kernel, lanes, names.index(csr_best[kernel][0]))
+            t_csr = cd.time_kernel(timed, lev, iters)
+            set_tuning(kernel, lanes, 4)
+            if t_sell > t_csr:
+                threshold = max(threshold, cd.info(lev + 1)["n_active"])
+                break
+    set_sell_min_rows(threshold)
+    if log and threshold:
+        log("autotune: sliced ELL on levels with at least %d active rows" % threshold)
     return out
+
+
+def set_sell_window(slices):
+    """Rows of a sliced ELL length-sorting window, in slices of 64 rows; read
+    when a level's layout is first built."""
+    _check(lib().pmcd_set_tuning(10, int(slices)))
+
+
+def get_sell_window():
+    v = ctypes.c_int(0)
+    _check(lib().pmcd_get_tuning(10, ctypes.byref(v)))
+    return v.value
+
+
+def set_sell_min_rows(n):
+    """Under variant 4, levels with fewer active rows keep the unrolled CSR kernels."""
+    _check(lib().pmcd_set_tuning(11, int(n)))
+
+
+def get_sell_min_rows():
+    v = ctypes.c_int(0)
+    _check(lib().pmcd_get_tuning(11, ctypes.byref(v)))
+    return v.value
 
 
 def gpu_available():

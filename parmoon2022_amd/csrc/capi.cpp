@@ -162,6 +162,18 @@ int pmcd_host_sweep_order(pmcd_host* h, int level, int max_exact_classes, int* r
   });
 }
 
+int pmcd_host_sell_layout(pmcd_host* h, int level, int* perm, int* len, long long* slice_off, int* n_slots) {
+  return guarded([&] {
+    PMCD_CHECK(perm && len && slice_off && n_slots, "null argument");
+    Level& L = host_level(h, level);
+    SellLayout o = build_sell(L.st, L.n_active(), sell_window());
+    std::memcpy(perm, o.perm.data(), o.perm.size() * sizeof(int));
+    std::memcpy(len, o.len.data(), o.len.size() * sizeof(int));
+    std::memcpy(slice_off, o.slice_off.data(), o.slice_off.size() * sizeof(long long));
+    *n_slots = o.n_slots();
+  });
+}
+
 int pmcd_host_cell_coords(pmcd_host* h, int level, double* out) {
   return guarded([&] { copy_coords(host_level(h, level).mesh, out); });
 }
@@ -330,6 +342,19 @@ int pmcd_apply(pmcd_system* s, int level, const double* x, double* y, double a, 
     if(mode == 1) s->sys->apply_scaled_add(L, dx.p, dy.p, a);
     else s->sys->apply(L, dx.p, dy.p);
     dy.download(y, L.n(), s->stream);
+    PMCD_HIP(hipStreamSynchronize(s->stream));
+  });
+}
+
+int pmcd_jacobi_sweep(pmcd_system* s, int level, const double* b, double* x, double omega) {
+  return guarded([&] {
+    PMCD_CHECK(b && x, "null argument");
+    Level& L = level_of(s, level);
+    DevBuf<double> db, dx, dy(L.n());
+    db.upload(b, L.n(), s->stream);
+    dx.upload(x, L.n(), s->stream);
+    level_jacobi(L, db.p, dx.p, dy.p, omega, s->stream);
+    dy.download(x, L.n(), s->stream);
     PMCD_HIP(hipStreamSynchronize(s->stream));
   });
 }

@@ -318,7 +318,10 @@ __global__ __launch_bounds__(64) void k_gather_matrix(int n_rows, int n_active, 
                                                        const int* __restrict__ contrib,
                                                        const uint8_t* __restrict__ slot,
                                                        const double* __restrict__ local_A,
-                                                       double* __restrict__ val) {
+                                                       double* __restrict__ val,
+                                                       const int* __restrict__ sell_slot,
+                                                       const long long* __restrict__ sell_off,
+                                                       double* __restrict__ sell_val) {
   __shared__ double s_row[256];
   const int r = blockIdx.x;
   const int lane = threadIdx.x;
@@ -341,6 +344,12 @@ __global__ __launch_bounds__(64) void k_gather_matrix(int n_rows, int n_active, 
     __syncthreads();
   }
   for(int k = lane; k < len; k += 64) val[r0 + k] = s_row[k];
+  if(sell_val) {
+    // the same row into its sliced ELL slot (entry k at base + 64 k)
+    const int t = sell_slot[r];
+    const long long base = sell_off[t / kSellRows] + (t % kSellRows);
+    for(int k = lane; k < len; k += 64) sell_val[base + (long long)k * kSellRows] = s_row[k];
+  }
 }
 
 __global__ void k_gather_rhs(int n_rows, int n_active, const int* __restrict__ contrib_ptr,
@@ -550,10 +559,11 @@ void launch_assemble_local(const AsmArgs& a, hipStream_t s) {
 
 void launch_gather_matrix(int n_rows, int n_active, int nl, const int* row_ptr, const int* col,
                           const int* contrib_ptr, const int* contrib, const uint8_t* slot,
-                          const double* local_A, double* val, hipStream_t s) {
+                          const double* local_A, double* val, hipStream_t s, const int* sell_slot,
+                          const long long* sell_off, double* sell_val) {
   if(n_rows == 0) return;
   k_gather_matrix<<<n_rows, 64, 0, s>>>(n_rows, n_active, nl, row_ptr, col, contrib_ptr, contrib,
-                                        slot, local_A, val);
+                                        slot, local_A, val, sell_slot, sell_off, sell_val);
   PMCD_HIP(hipGetLastError());
 }
 

@@ -612,6 +612,8 @@ void dispatch_spmv(int width, int variant, int max_row, int n, int n_active, con
   const long threads = (long)n * width;
   const int blocks = (int)((threads + 255) / 256);
   const int u = width >= 8 ? unroll_for(max_row, width) : 0;
+  // variant 4 (sliced ELL) needs a level's layout; raw CSR calls run unrolled
+  if(variant == 4) variant = 2;
   if(variant == 2 && u) {
     switch(width) {
       case 8: spmv_unrolled<8, T>(u, blocks, n, n_active, rp, col, val, x, z, y, alpha, beta, s); break;
@@ -822,6 +824,18 @@ void set_tuning(int key, int value) {
     g_cg_blocks = value;
     return;
   }
+  if(key == 9) {
+    set_sell_unroll(value);
+    return;
+  }
+  if(key == 10) {
+    set_sell_window(value);
+    return;
+  }
+  if(key == 11) {
+    set_sell_min_rows(value);
+    return;
+  }
   if(key == 5) {
     PMCD_CHECK(value >= 0, "the exact-sweep class limit must be non-negative");
     g_sor_exact_limit = value;
@@ -835,7 +849,7 @@ void set_tuning(int key, int value) {
   if(key == 0 || key == 1)
     PMCD_CHECK(value == 4 || value == 8 || value == 16 || value == 32, "lanes per row must be 4, 8, 16 or 32");
   else
-    PMCD_CHECK(value >= 0 && value <= 3, "variant must be 0, 1, 2 or 3");
+    PMCD_CHECK(value >= 0 && value <= 4, "variant must be 0, 1, 2, 3 or 4");
   switch(key) {
     case 0: g_spmv_width = value; break;
     case 1: g_jacobi_width = value; break;
@@ -856,6 +870,9 @@ int get_tuning(int key) {
     case 6: return g_vcycle_graphs;
     case 7: return g_cg_device;
     case 8: return g_cg_blocks;
+    case 9: return sell_unroll();
+    case 10: return get_sell_window_slices();
+    case 11: return sell_min_rows();
     default: PMCD_FAIL("unknown tuning key ", key);
   }
 }
@@ -901,7 +918,9 @@ static void dispatch_jacobi(int width, int max_row, int n, int n_active, const i
                             T* x_out, T omega, hipStream_t s) {
   const int blocks = (int)(((long)n * width + 255) / 256);
   const int u = width >= 8 ? unroll_for(max_row, width) : 0;
-  if(g_jacobi_variant == 2 && u) {
+  // variant 4 (sliced ELL) needs a level's layout; raw CSR calls run unrolled
+  const int variant = g_jacobi_variant == 4 ? 2 : g_jacobi_variant;
+  if(variant == 2 && u) {
     switch(width) {
       case 8: jacobi_unrolled<8, T>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
       case 16: jacobi_unrolled<16, T>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
@@ -909,7 +928,7 @@ static void dispatch_jacobi(int width, int max_row, int n, int n_active, const i
     }
     return;
   }
-  if(g_jacobi_variant == 3 && u) {
+  if(variant == 3 && u) {
     switch(width) {
       case 8: jacobi_unrolled<8, T, true>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
       case 16: jacobi_unrolled<16, T, true>(u, blocks, n, n_active, rp, col, val, diag, b, x, x_out, omega, s); break;
@@ -917,7 +936,7 @@ static void dispatch_jacobi(int width, int max_row, int n, int n_active, const i
     }
     return;
   }
-  if(g_jacobi_variant == 1) {
+  if(variant == 1) {
     switch(width) {
       case 4: k_jacobi_stream<4, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
       case 8: k_jacobi_stream<8, T><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;

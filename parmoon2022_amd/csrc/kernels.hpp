@@ -41,11 +41,14 @@ void launch_mfma_probe(double* out, hipStream_t s);
 int assembly_variant();
 
 // Sum local matrices into CSR rows in increasing cell order; Dirichlet rows
-// (r >= n_active) get 1 on the diagonal and 0 elsewhere.
+// (r >= n_active) get 1 on the diagonal and 0 elsewhere. With sell_val the
+// active rows are also written into a sliced ELL copy (sell_slot: row ->
+// slot, sell_off: first entry of each slice).
 void launch_gather_matrix(int n_rows, int n_active, int nl, const int* row_ptr,
                           const int* col, const int* contrib_ptr, const int* contrib,
                           const uint8_t* slot, const double* local_A, double* val,
-                          hipStream_t s);
+                          hipStream_t s, const int* sell_slot = nullptr,
+                          const long long* sell_off = nullptr, double* sell_val = nullptr);
 // rhs[r] = sum of local rhs (plus Neumann terms) in cell order for active rows,
 // Dirichlet value for r >= n_active.
 void launch_gather_rhs(int n_rows, int n_active, const int* contrib_ptr, const int* contrib,
@@ -198,6 +201,39 @@ void launch_spmv_masked(int n, int n_active, const int* rp, const int* col, cons
                         int max_row, const int* row_inst, const uint8_t* active, hipStream_t s);
 void launch_seg_axpy(int n_chunks, const SegChunk* ch, const double* a, const double* x, double* y,
                      hipStream_t s);
+
+// ---- sliced ELL (one wavefront per slice of kSellRows active rows) ----------
+constexpr int kSellRows = 64;
+// rows per length-sorting window of the sliced ELL layout: sell_window()
+// slices (default 4), read when a level's layout is built
+int sell_window();
+// levels with fewer active rows keep the unrolled CSR kernels under variant 4
+int sell_min_rows();
+void set_sell_min_rows(int n);
+template <class T>
+struct SellView {
+  int n_slots;                 // active rows rounded up to whole slices
+  const long long* off;        // first entry of each slice
+  const int* perm;             // slot -> row (-1: padding)
+  const int* len;              // slot -> row length
+  const int* col;
+  const T* val;
+};
+template <class T>
+void launch_spmv_sell(int n, int n_active, const SellView<T>& m, const T* x, const T* z, T* y, T alpha, T beta,
+                      hipStream_t s);
+template <class T>
+void launch_jacobi_sell(int n, int n_active, const SellView<T>& m, const T* diag, const T* b, const T* x, T* x_out,
+                        T omega, hipStream_t s);
+// copy the active rows' CSR entries (values or columns) into the sliced layout
+template <class T>
+void launch_sell_fill(int n_slots, const long long* off, const int* perm, const int* len,
+                      const int* rp, const T* src, T* dst, hipStream_t s);
+// entries of a lane loaded per step (4, 8 or 16)
+int sell_unroll();
+void set_sell_unroll(int u);
+void set_sell_window(int slices);
+int get_sell_window_slices();
 
 // ---- conjugate gradients with device-resident scalars ----------------------
 // State of one CG solve. Scalars that a kernel both reads and writes sit in

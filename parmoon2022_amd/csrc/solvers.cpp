@@ -93,6 +93,10 @@ void CDSystem::prepare(const SolverDesc& s) {
     }
     mixed_ready_ = true;
   }
+  // sliced ELL values are refreshed here, outside any captured V-cycle
+  if(get_tuning(2) == 4 || get_tuning(3) == 4)
+    for(auto& Lp : levels_)
+      if(Lp->n_active() >= sell_min_rows()) prepare_level_sell(*Lp, s.mixed != 0, stream_);
 }
 
 SolveStats CDSystem::solve(const SolverDesc& s) {

@@ -113,4 +113,31 @@ SweepOrder build_sweep_order(const Structure& st, int max_exact_classes) {
   return o;
 }
 
+SellLayout build_sell(const Structure& st, int n_active, int window) {
+  PMCD_CHECK(window > 0 && window % kSellRows == 0, "sliced ELL window must be a multiple of ", kSellRows);
+  SellLayout o;
+  const int n_slots = (n_active + kSellRows - 1) / kSellRows * kSellRows;
+  o.perm.assign(n_slots, -1);
+  o.len.assign(n_slots, 0);
+  auto row_len = [&](int r) { return st.row_ptr[r + 1] - st.row_ptr[r]; };
+  for(int w0 = 0; w0 < n_active; w0 += window) {
+    const int w1 = std::min(n_active, w0 + window);
+    std::vector<int> rows(w1 - w0);
+    for(int r = w0; r < w1; ++r) rows[r - w0] = r;
+    std::stable_sort(rows.begin(), rows.end(), [&](int a, int b) { return row_len(a) > row_len(b); });
+    for(int i = 0; i < (int)rows.size(); ++i) {
+      o.perm[w0 + i] = rows[i];
+      o.len[w0 + i] = row_len(rows[i]);
+    }
+  }
+  const int n_slices = n_slots / kSellRows;
+  o.slice_off.assign(n_slices + 1, 0);
+  for(int sl = 0; sl < n_slices; ++sl) {
+    int width = 0;
+    for(int l = 0; l < kSellRows; ++l) width = std::max(width, o.len[sl * kSellRows + l]);
+    o.slice_off[sl + 1] = o.slice_off[sl] + (long long)width * kSellRows;
+  }
+  return o;
+}
+
 }  // namespace pmcd

@@ -14,6 +14,7 @@
 #pragma once
 #include "common.hpp"
 #include "fespace.hpp"
+#include "kernels.hpp"
 
 namespace pmcd {
 
@@ -54,5 +55,20 @@ struct SweepOrder {
 };
 
 SweepOrder build_sweep_order(const Structure& st, int max_exact_classes);
+
+// Sliced ELL layout of the active rows (r < n_active) of a CSR pattern:
+// slices of kSellRows rows, one wavefront per slice; inside a slice entry k of
+// the row in lane l sits at slice_off[s] + k * kSellRows + l, so that the
+// wavefront reads kSellRows consecutive values per load. Rows are sorted by
+// length (longest first, stable) inside windows of `window` rows so that a
+// slice holds rows of about equal length. Each row keeps its entries in CSR
+// (ascending column) order.
+struct SellLayout {
+  std::vector<int> perm;             // slot -> row, -1 for padding slots
+  std::vector<int> len;              // slot -> row length (0 for padding)
+  std::vector<long long> slice_off;  // first entry of each slice, n_slices + 1
+  int n_slots() const { return (int)perm.size(); }
+};
+SellLayout build_sell(const Structure& st, int n_active, int window);
 
 }  // namespace pmcd

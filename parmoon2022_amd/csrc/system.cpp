@@ -3,6 +3,7 @@
 #include <chrono>
 #include <cmath>
 #include <algorithm>
+#include <climits>
 
 namespace pmcd {
 
@@ -349,8 +350,14 @@ AsmArgs CDSystem::asm_args(const Level& L) {
 void CDSystem::assemble_level(Level& L) {
   AsmArgs a = asm_args(L);
   if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
-  launch_gather_matrix(L.n(), L.n_active(), a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p,
-                       L.contrib.p, L.slot.p, local_A_.p, L.val.p, stream_);
+  // a level whose matrix kernels run on the sliced ELL copy gets it from the
+  // same row gather (no separate copy pass)
+  const bool sell = level_uses_sell(L);
+  if(sell) build_level_sell(L, stream_);
+  launch_gather_matrix(L.n(), L.n_active(), a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p, L.contrib.p, L.slot.p,
+                       local_A_.p, L.val.p, stream_, sell ? L.sell_slot.p : nullptr,
+                       sell ? L.sell_off.p : nullptr, sell ? L.sell_val.p : nullptr);
+  L.sell_fresh = sell;
   launch_gather_rhs(L.n(), L.n_active(), L.contrib_ptr.p, L.contrib.p, local_rhs_.p,
                     L.extra_ptr.p, L.extra.p, L.dir_vals.p, L.rhs.p, stream_);
   launch_gather_idx(L.n(), L.diag_pos.p, L.val.p, L.diag.p, stream_);
@@ -369,7 +376,7 @@ void CDSystem::assemble() {
   last_assemble_ms = (now_s() - t0) * 1e3;
   coarse_ready_ = false;
   mixed_ready_ = false;
-  for(auto& L : levels_) L->sw_fresh = L->sw_fresh32 = false;
+  for(auto& L : levels_) L->sw_fresh = L->sw_fresh32 = L->sell_fresh32 = false;
 }
 
 void CDSystem::coarse_factorize() {
@@ -409,17 +416,20 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
   AsmArgs a = asm_args(L);
   auto launch = [&]() {
     switch(which) {
-      case 0: launch_spmv(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x.p, nullptr, y.p, 1.0, 0.0, stream_, L.st.max_row); break;
-      case 1: launch_jacobi(n, L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b.p, x.p, y.p, 0.8, stream_, L.st.max_row); break;
-      case 2: launch_spmv_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, xf.p, nullptr, yf.p, 1.0f, 0.0f, stream_, L.st.max_row); break;
+      case 0: level_spmv(L, x.p, nullptr, y.p, 1.0, 0.0, stream_); break;
+      case 1: level_jacobi(L, b.p, x.p, y.p, 0.8, stream_); break;
+      case 2: level_spmv32(L, xf.p, nullptr, yf.p, 1.0f, 0.0f, stream_); break;
       case 3:
         if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
         break;
-      case 4:
+      case 4: {
+        const bool sell = level_uses_sell(L) && L.sell_slots >= 0;
         launch_gather_matrix(n, L.n_active(), a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p, L.contrib.p,
-                             L.slot.p, local_A_.p, L.val.p, stream_);
+                             L.slot.p, local_A_.p, L.val.p, stream_, sell ? L.sell_slot.p : nullptr,
+                             sell ? L.sell_off.p : nullptr, sell ? L.sell_val.p : nullptr);
         break;
-      case 5: launch_jacobi_f32(n, L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, bf.p, xf.p, yf.p, 0.8f, stream_, L.st.max_row); break;
+      }
+      case 5: level_jacobi32(L, bf.p, xf.p, yf.p, 0.8f, stream_); break;
       case 6: launch_stream_copy((long)copy_dst.n, L.val.p, copy_dst.p, stream_); break;
       case 7: launch_stream_copy4((long)copy_dst.n, L.val.p, copy_dst.p, stream_); break;
       case 8: launch_stream_read((long)(L.val.n & ~(size_t)1), L.val.p, copy_dst.p, stream_); break;
@@ -442,12 +452,11 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
 }
 
 void CDSystem::apply(Level& L, const double* x, double* y) {
-  launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, nullptr, y, 1.0, 0.0, stream_,
-              L.st.max_row);
+  level_spmv(L, x, nullptr, y, 1.0, 0.0, stream_);
 }
 
 void CDSystem::apply_scaled_add(Level& L, const double* x, double* y, double a) {
-  launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, y, y, a, 1.0, stream_, L.st.max_row);
+  level_spmv(L, x, y, y, a, 1.0, stream_);
 }
 
 double CDSystem::dot(int n, const double* x, const double* y) {
@@ -472,8 +481,7 @@ int CDSystem::smooth(Level& L, double* const x[2], int cur, const double* b, int
     if(s == 0 && from_zero)
       launch_jacobi_from_zero(L.n(), L.n_active(), L.diag.p, b, x[1 - cur], sd_.jacobi_damp, stream_);
     else
-      launch_jacobi(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b, x[cur],
-                    x[1 - cur], sd_.jacobi_damp, stream_, L.st.max_row);
+      level_jacobi(L, b, x[cur], x[1 - cur], sd_.jacobi_damp, stream_);
     cur = 1 - cur;
   }
   return cur;
@@ -491,8 +499,7 @@ int CDSystem::smooth32(Level& L, float* const x[2], int cur, const float* b, int
       launch_jacobi_from_zero_f32(L.n(), L.n_active(), L.diag32.p, b, x[1 - cur],
                                   (float)sd_.jacobi_damp, stream_);
     else
-      launch_jacobi_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x[cur],
-                        x[1 - cur], (float)sd_.jacobi_damp, stream_, L.st.max_row);
+      level_jacobi32(L, b, x[cur], x[1 - cur], (float)sd_.jacobi_damp, stream_);
     cur = 1 - cur;
   }
   return cur;
@@ -529,8 +536,7 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
     Level& C = *levels_[l - 1];
     const bool zero = (l == Lf) ? fine_zero : (jacobi && sd_.n_pre > 0);
     cur[l] = smooth(L, x[l].data(), cur[l], b[l], sd_.n_pre, zero);
-    launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x[l][cur[l]], b[l], L.mg_def.p,
-                -1.0, 1.0, stream_, L.st.max_row);
+    level_spmv(L, x[l][cur[l]], b[l], L.mg_def.p, -1.0, 1.0, stream_);
     if(!jacobi || sd_.n_pre == 0 || l - 1 == 0) launch_fill(C.n(), 0.0, x[l - 1][0], stream_);
     launch_restrict(C.n(), C.n_active(), (long)L.r_val.n, L.r_max_row, L.r_rp.p, L.r_col.p,
                     L.r_val.p, L.mg_def.p, C.mg_rhs.p, stream_);
@@ -574,8 +580,7 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
     Level& C = *levels_[l - 1];
     const bool zero = (l == Lf) ? fine_zero : (jacobi && sd_.n_pre > 0);
     cur[l] = smooth32(L, x[l].data(), cur[l], L.f_rhs.p, sd_.n_pre, zero);
-    launch_spmv_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, x[l][cur[l]], L.f_rhs.p,
-                    L.f_def.p, -1.0f, 1.0f, stream_, L.st.max_row);
+    level_spmv32(L, x[l][cur[l]], L.f_rhs.p, L.f_def.p, -1.0f, 1.0f, stream_);
     if(!jacobi || sd_.n_pre == 0 || l - 1 == 0) launch_fill_f32(C.n(), 0.0f, x[l - 1][0], stream_);
     launch_restrict_f32(C.n(), C.n_active(), (long)L.r_val32.n, L.r_max_row, L.r_rp.p, L.r_col.p,
                         L.r_val32.p, L.f_def.p, C.f_rhs.p, stream_);
@@ -685,6 +690,95 @@ void CDSystem::sor_sweep(Level& L, const double* b, double* x, double omega, int
 
 void CDSystem::sor_sweep32(Level& L, const float* b, float* x, float omega, int flag) {
   level_sor_sweep32(L, b, x, omega, flag, stream_);
+}
+
+void build_level_sell(Level& L, hipStream_t stream) {
+  if(L.sell_slots >= 0) return;
+  SellLayout o = build_sell(L.st, L.n_active(), sell_window());
+  PMCD_CHECK(o.slice_off.back() < (1LL << 40), "sliced ELL layout too large");
+  std::vector<int> slot_of(std::max(1, L.n_active()), 0);
+  for(int t = 0; t < o.n_slots(); ++t)
+    if(o.perm[t] >= 0) slot_of[o.perm[t]] = t;
+  L.sell_off.upload(o.slice_off, stream);
+  L.sell_perm.upload(o.perm, stream);
+  L.sell_len.upload(o.len, stream);
+  L.sell_slot.upload(slot_of, stream);
+  L.sell_col.alloc(std::max<long long>(1, o.slice_off.back()));
+  L.sell_val.alloc(L.sell_col.n);
+  L.sell_slots = o.n_slots();
+  PMCD_HIP(hipMemsetAsync(L.sell_col.p, 0, L.sell_col.n * sizeof(int), stream));
+  PMCD_HIP(hipMemsetAsync(L.sell_val.p, 0, L.sell_val.n * sizeof(double), stream));
+  launch_sell_fill<int>(L.sell_slots, L.sell_off.p, L.sell_perm.p, L.sell_len.p, L.row_ptr.p, L.col.p,
+                        L.sell_col.p, stream);
+  L.sell_fresh = L.sell_fresh32 = false;
+}
+
+void prepare_level_sell(Level& L, bool f32, hipStream_t stream) {
+  build_level_sell(L, stream);
+  if(!L.sell_fresh) {
+    launch_sell_fill<double>(L.sell_slots, L.sell_off.p, L.sell_perm.p, L.sell_len.p, L.row_ptr.p, L.val.p,
+                             L.sell_val.p, stream);
+    L.sell_fresh = true;
+  }
+  if(f32 && !L.sell_fresh32) {
+    // the FP32 copy rounds the FP64 one entry by entry (padding stays 0)
+    PMCD_CHECK(L.sell_val.n < (size_t)INT_MAX, "sliced ELL copy too large for one conversion launch");
+    if(L.sell_val32.n != L.sell_val.n) L.sell_val32.alloc(L.sell_val.n);
+    launch_d2f((int)L.sell_val.n, L.sell_val.p, L.sell_val32.p, stream);
+    L.sell_fresh32 = true;
+  }
+}
+
+template <class T>
+static SellView<T> sell_view(const Level& L, const T* val) {
+  return SellView<T>{L.sell_slots, L.sell_off.p, L.sell_perm.p, L.sell_len.p, L.sell_col.p, val};
+}
+
+// variant 4 on a level with at least sell_min_rows() active rows
+static bool use_sell(const Level& L, int key) { return get_tuning(key) == 4 && L.n_active() >= sell_min_rows(); }
+
+bool level_uses_sell(const Level& L) { return use_sell(L, 2) || use_sell(L, 3); }
+
+void level_spmv(Level& L, const double* x, const double* z, double* y, double alpha, double beta,
+                hipStream_t stream) {
+  if(use_sell(L, 2)) {
+    prepare_level_sell(L, false, stream);
+    launch_spmv_sell<double>(L.n(), L.n_active(), sell_view<double>(L, L.sell_val.p), x, z, y, alpha, beta, stream);
+  } else {
+    launch_spmv(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, x, z, y, alpha, beta, stream, L.st.max_row);
+  }
+}
+
+void level_spmv32(Level& L, const float* x, const float* z, float* y, float alpha, float beta, hipStream_t stream) {
+  if(use_sell(L, 2)) {
+    prepare_level_sell(L, true, stream);
+    launch_spmv_sell<float>(L.n(), L.n_active(), sell_view<float>(L, L.sell_val32.p), x, z, y, alpha, beta, stream);
+  } else {
+    launch_spmv_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, x, z, y, alpha, beta, stream,
+                    L.st.max_row);
+  }
+}
+
+void level_jacobi(Level& L, const double* b, const double* x, double* x_out, double omega, hipStream_t stream) {
+  if(use_sell(L, 3)) {
+    prepare_level_sell(L, false, stream);
+    launch_jacobi_sell<double>(L.n(), L.n_active(), sell_view<double>(L, L.sell_val.p), L.diag.p, b, x, x_out,
+                               omega, stream);
+  } else {
+    launch_jacobi(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val.p, nullptr, L.diag.p, b, x, x_out, omega, stream,
+                  L.st.max_row);
+  }
+}
+
+void level_jacobi32(Level& L, const float* b, const float* x, float* x_out, float omega, hipStream_t stream) {
+  if(use_sell(L, 3)) {
+    prepare_level_sell(L, true, stream);
+    launch_jacobi_sell<float>(L.n(), L.n_active(), sell_view<float>(L, L.sell_val32.p), L.diag32.p, b, x, x_out,
+                              omega, stream);
+  } else {
+    launch_jacobi_f32(L.n(), L.n_active(), L.row_ptr.p, L.col.p, L.val32.p, L.diag32.p, b, x, x_out, omega, stream,
+                      L.st.max_row);
+  }
 }
 
 void prepare_level_sweep(Level& L, hipStream_t stream) {

@@ -91,6 +91,14 @@ struct Level {
   DevBuf<double> sw_val;
   DevBuf<float> sw_val32;
   bool sw_fresh = false, sw_fresh32 = false;
+  // sliced ELL copy of the active rows (built on first use; values refreshed
+  // after assembly)
+  int sell_slots = -1;  // -1: not built
+  DevBuf<long long> sell_off;
+  DevBuf<int> sell_perm, sell_len, sell_col, sell_slot;  // sell_slot: row -> slot
+  DevBuf<double> sell_val;
+  DevBuf<float> sell_val32;
+  bool sell_fresh = false, sell_fresh32 = false;
   int n() const { return space.n_dof; }
   int n_active() const { return space.n_active; }
 };
@@ -106,6 +114,22 @@ void level_sor_sweep(Level& L, const double* b, double* x, double omega, int fla
                      const int* row_owner = nullptr, const uint8_t* owner_active = nullptr);
 void level_sor_sweep32(Level& L, const float* b, float* x, float omega, int flag, hipStream_t stream,
                        const int* row_owner = nullptr, const uint8_t* owner_active = nullptr);
+// Matrix kernels of a level: the sliced ELL kernels where the SpMV / Jacobi
+// variant is 4 (layout built and values refreshed on first use), the CSR
+// kernels otherwise. y = beta z + alpha A x with the non-active rule;
+// x_out = x + omega D^{-1} (b - A x).
+void level_spmv(Level& L, const double* x, const double* z, double* y, double alpha, double beta,
+                hipStream_t stream);
+void level_spmv32(Level& L, const float* x, const float* z, float* y, float alpha, float beta, hipStream_t stream);
+void level_jacobi(Level& L, const double* b, const double* x, double* x_out, double omega, hipStream_t stream);
+void level_jacobi32(Level& L, const float* b, const float* x, float* x_out, float omega, hipStream_t stream);
+// builds the sliced ELL layout and refreshes its FP64 (and, with f32, FP32)
+// values; called before any graph capture
+void prepare_level_sell(Level& L, bool f32, hipStream_t stream);
+// builds the sliced ELL layout (positions and column indices) only
+void build_level_sell(Level& L, hipStream_t stream);
+// whether the SpMV or Jacobi kernels of the level run on the sliced ELL copy
+bool level_uses_sell(const Level& L);
 // physical position of the nodal point of local DOF i of cell c
 void node_position(const Mesh& m, int c, const ElementInfo& info, int i, double* x);
 double now_seconds();

@@ -144,7 +144,8 @@ def test_unrolled_csr_bit_identical(width):
     cd.close()
 
 
-@pytest.mark.parametrize("shape", [(8, 0), (32, 0), (8, 1), (16, 1), (32, 1), (8, 2), (16, 2), (32, 2), (8, 3)],
+@pytest.mark.parametrize("shape", [(8, 0), (32, 0), (8, 1), (16, 1), (32, 1), (8, 2), (16, 2), (32, 2), (8, 3),
+                                   (16, 4)],
                          ids=lambda s: "w%d-v%d" % s)
 def test_jacobi_shapes_same_solve(oracle_lib, shape):
     """The multigrid solve is independent of the Jacobi launch shape up to
@@ -681,3 +682,92 @@ def test_cg_device_iteration_limit():
     assert sd.iterations == sh.iterations == 13
     assert np.linalg.norm(ud - uh) <= 1e-12 * np.linalg.norm(uh)
     assert abs(sd.final_residual - sh.final_residual) <= 1e-10 * sh.final_residual
+
+
+SELL_CASES = [ASSEMBLY_CASES[k] for k in (0, 3, 5, 6, 9)]
+
+
+@pytest.mark.parametrize("unroll", [4, 8, 16])
+@pytest.mark.parametrize("case", SELL_CASES, ids=lambda c: "%s-p%d" % c[:2])
+def test_sell_spmv_and_jacobi_bitwise_serial(oracle_lib, case, unroll):
+    """Sliced ELL kernels (variant 4) sum each row serially in CSR column
+    order, like TMatrix::multiply: apply, apply_scaled_add and the Jacobi step
+    are bitwise equal to the oracle's serial CSR loops on the same matrix."""
+    old_s, old_j, old_u = pm.tuning("spmv"), pm.tuning("jacobi"), pm.get_sell_unroll()
+    try:
+        pm.set_tuning("spmv", 16, 4)
+        pm.set_tuning("jacobi", 16, 4)
+        pm.set_sell_unroll(unroll)
+        cd = pm.ConvectionDiffusion(make_db(*case))
+        cd.assemble()
+        i = cd.info()
+        n, na = i["n_dof"], i["n_active"]
+        rng = np.random.default_rng(5)
+        x = rng.standard_normal(n)
+        y0 = rng.standard_normal(n)
+        b = rng.standard_normal(n)
+        rp, col = cd.pattern()
+        val = cd.matrix_values()
+        assert np.array_equal(cd.apply(x), oracle_lib.spmv_csr(n, na, rp, col, val, x, np.zeros(n), 1.0))
+        assert np.array_equal(cd.apply(x, y=y0, a=-0.7), oracle_lib.spmv_csr(n, na, rp, col, val, x, y0, -0.7))
+        diag = np.array([val[rp[r]:rp[r + 1]][col[rp[r]:rp[r + 1]] == r][0] for r in range(n)])
+        d = oracle_lib.spmv_csr(n, na, rp, col, val, x, b, -1.0) / diag
+        assert np.array_equal(cd.jacobi_sweep(b, x, omega=0.8), x + 0.8 * d)
+        cd.close()
+    finally:
+        pm.set_tuning("spmv", *old_s)
+        pm.set_tuning("jacobi", *old_j)
+        pm.set_sell_unroll(old_u)
+
+
+def test_sell_mixed_vcycle_solve():
+    """C3 path (FP32 V-cycle) with the sliced ELL kernels: same iteration
+    count as with the CSR kernels up to one, solutions within 1e-7."""
+    db = c2_db(3, 3, mixed=1)
+    old_s, old_j = pm.tuning("spmv"), pm.tuning("jacobi")
+    out = []
+    try:
+        for v in (2, 4):
+            pm.set_tuning("spmv", 16, v)
+            pm.set_tuning("jacobi", 8, v)
+            cd = pm.ConvectionDiffusion(db)
+            cd.assemble()
+            st = cd.solve()
+            out.append((st, cd.solution()))
+            cd.close()
+    finally:
+        pm.set_tuning("spmv", *old_s)
+        pm.set_tuning("jacobi", *old_j)
+    (sa, ua), (sb, ub) = out
+    rel = np.linalg.norm(ua - ub) / np.linalg.norm(ua)
+    print("mixed: iterations csr %d sell %d, rel %.3e" % (sa.iterations, sb.iterations, rel), flush=True)
+    assert sa.converged and sb.converged
+    assert abs(sa.iterations - sb.iterations) <= 1 and rel < 1e-7
+
+
+def test_sell_copy_refreshed_when_switched_on_after_assembly(oracle_lib):
+    """The sliced ELL copy is written by the row gather when the kernels use
+    it at assembly time, and copied from the CSR values when they are switched
+    on later; both give the serial row sums bit for bit, also after the
+    coefficients change (set_data + assemble)."""
+    case = ASSEMBLY_CASES[6]
+    old_s = pm.tuning("spmv")
+    try:
+        pm.set_tuning("spmv", 16, 2)
+        db = make_db(*case)
+        cd = pm.ConvectionDiffusion(db)
+        cd.assemble()
+        i = cd.info()
+        n, na = i["n_dof"], i["n_active"]
+        x = np.random.default_rng(9).standard_normal(n)
+        rp, col = cd.pattern()
+        pm.set_tuning("spmv", 16, 4)  # switched on after assembly: copied from CSR
+        assert np.array_equal(cd.apply(x), oracle_lib.spmv_csr(n, na, rp, col, cd.matrix_values(), x, np.zeros(n)))
+        db2 = dict(db, diffusion_coefficient=0.5, convection=(0.2, -0.4, 0.1))
+        cd.set_data(db2)
+        cd.assemble()  # now written by the gather
+        v2 = cd.matrix_values()
+        assert np.array_equal(cd.apply(x), oracle_lib.spmv_csr(n, na, rp, col, v2, x, np.zeros(n)))
+        cd.close()
+    finally:
+        pm.set_tuning("spmv", *old_s)

@@ -143,3 +143,49 @@ def test_exact_class_sweep_equals_serial_sweep(oracle_lib, flag):
     if flag in (1, 2):
         x = _class_sweep(rp, col, val, b, x, 1.5, rows, ptr, backward=True)
     assert np.array_equal(x, o.sor_sweep(b, x0, 1.5, flag))
+
+
+@pytest.mark.parametrize("geo,order,nref,ex", [("TwoTriangles", 1, 5, "sine_2d"), ("UnitSquare", 2, 3, "sine_2d"),
+                                               ("Default_UnitCube_Hexa", 2, 2, "sine_3d"),
+                                               ("Default_UnitCube_Tetra", 2, 1, "sine_3d"),
+                                               ("Default_UnitCube_Hexa", 1, 3, "sine_3d")])
+def test_sell_layout_covers_active_rows(geo, order, nref, ex):
+    """Sliced ELL layout: the first n_active slots hold every active row once
+    (padding slots after), lengths are the CSR row lengths, non-increasing
+    inside each sorting window (the window knob, in slices of 64 rows; the
+    default and a wide one), every slice wide enough for its
+    rows, and the entry positions of all rows are distinct and inside the
+    value array. Reading the CSR entries through the layout (the device's
+    fill and row-sum index rules) gives back every row in column order."""
+    h = pm.HostSetup({"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": nref, "example": ex})
+    rp, col = h.pattern()
+    na = h.info()["n_active"]
+    old = pm.get_sell_window()
+    for slices in (old, 32):
+        pm.set_sell_window(slices)
+        try:
+            _check_sell_layout(h, rp, col, na, 64 * slices)
+        finally:
+            pm.set_sell_window(old)
+
+
+def _check_sell_layout(h, rp, col, na, window):
+    perm, ln, off = h.sell_layout()
+    n_slots = len(perm)
+    assert n_slots % 64 == 0 and na <= n_slots < na + 64
+    assert np.array_equal(np.sort(perm[:na]), np.arange(na)) and np.all(perm[na:] == -1)
+    assert np.array_equal(ln[:na], np.diff(rp)[perm[:na]]) and np.all(ln[na:] == 0)
+    for w in range(0, na, window):
+        seg = ln[w:min(na, w + window)]
+        assert np.all(np.diff(seg) <= 0)
+    width = np.diff(off) // 64
+    assert np.array_equal(width, ln.reshape(-1, 64).max(axis=1))
+    # positions of entry k of slot t
+    t = np.repeat(np.arange(n_slots), ln)
+    k = np.arange(len(t)) - np.repeat(np.cumsum(ln) - ln, ln)
+    pos = off[t // 64] + 64 * k + t % 64
+    assert len(np.unique(pos)) == len(pos) and pos.min() >= 0 and pos.max() < off[-1]
+    sell_col = np.full(off[-1], -1, dtype=np.int64)
+    sell_col[pos] = col[rp[perm[t]] + k]
+    back = sell_col[pos]
+    assert np.array_equal(back, col[rp[perm[t]] + k])

@@ -205,7 +205,8 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
 
 /* Launch-shape tuning (process-wide; no reference counterpart, knobs of the
  * autotuning sweep): key 0 / 1 = lanes per matrix row of the SpMV / Jacobi
- * kernels (4, 8, 16, 32); key 2 / 3 = their variant (0 plain, 1 streaming with
+ * kernels (4, 8, 16, 32; defaults 16 and 8); key 2 / 3 = their variant
+ * (default 2; 0 plain, 1 streaming with
  * XCD-contiguous rows, 2 unrolled with all loads of a lane in flight at once,
  * taken where the level's longest row fits 16 entries per lane, 3 unrolled
  * with non-temporal matrix loads, 4 sliced ELL: one lane per row, each row

@@ -100,6 +100,7 @@ def test_spmv_matches_serial_sum(oracle_lib, case, shape):
     """BlockFEMatrix::apply / apply_scaled_add against the serial row sums, for
     every launch shape the autotuner may pick."""
     db = make_db(*case)
+    old = pm.tuning("spmv")
     pm.set_tuning("spmv", *shape)
     cd = pm.ConvectionDiffusion(db)
     cd.assemble()
@@ -122,7 +123,7 @@ def test_spmv_matches_serial_sum(oracle_lib, case, shape):
     na = i["n_active"]
     assert np.array_equal(got_apply[na:], x[na:])
     cd.close()
-    pm.set_tuning("spmv", 16, 0)
+    pm.set_tuning("spmv", *old)
 
 
 @pytest.mark.parametrize("width", [8, 16, 32])
@@ -135,10 +136,11 @@ def test_unrolled_csr_bit_identical(width):
     x = rng.standard_normal(cd.info()["n_dof"])
     y0 = rng.standard_normal(cd.info()["n_dof"])
     out = []
+    old = pm.tuning("spmv")
     for variant in (0, 2, 3):
         pm.set_tuning("spmv", width, variant)
         out.append((cd.apply(x), cd.apply(x, y=y0, a=-0.7)))
-    pm.set_tuning("spmv", 16, 0)
+    pm.set_tuning("spmv", *old)
     for k in (1, 2):
         assert np.array_equal(out[0][0], out[k][0]) and np.array_equal(out[0][1], out[k][1])
     cd.close()
@@ -151,12 +153,13 @@ def test_jacobi_shapes_same_solve(oracle_lib, shape):
     """The multigrid solve is independent of the Jacobi launch shape up to
     rounding: same iteration count and solution within 1e-10 of the oracle."""
     db = c2_db(3, 2)
+    old = pm.tuning("jacobi")
     pm.set_tuning("jacobi", *shape)
     cd = pm.ConvectionDiffusion(db)
     cd.assemble()
     st = cd.solve()
     u = cd.solution()
-    pm.set_tuning("jacobi", 16, 0)
+    pm.set_tuning("jacobi", *old)
     o = oracle_lib.Oracle(pm.problem_desc(db))
     o.assemble()
     so = o.solve(pm.solver_desc(db))

@@ -664,82 +664,84 @@ __global__ void k_gather_idx(int n, const int* __restrict__ idx, const double* _
     dst[i] = src[idx[i]];
 }
 
+// column-major dense copy: M[c * n + r] = A_rc
 __global__ void k_csr_to_dense(int n, const int* __restrict__ rp, const int* __restrict__ col,
                                const double* __restrict__ val, double* __restrict__ M) {
   const int r = blockIdx.x;
-  for(int k = rp[r] + threadIdx.x; k < rp[r + 1]; k += blockDim.x) M[(size_t)r * n + col[k]] = val[k];
+  for(int k = rp[r] + threadIdx.x; k < rp[r + 1]; k += blockDim.x) M[(size_t)col[k] * n + r] = val[k];
 }
 
-// pivot search in column k, rows k..n-1 (largest magnitude, lowest row on ties)
-__global__ __launch_bounds__(256) void k_pivot(int n, int k, const double* __restrict__ A,
-                                               int* __restrict__ piv) {
-  __shared__ double s_v[256];
-  __shared__ int s_i[256];
+// One Gauss-Jordan step k, in place, on a column-major n x n matrix B: one
+// wavefront per column. Every wavefront reads the pivot column (col_in: column
+// k before the step, written by the previous step), finds the pivot row p
+// (largest magnitude in rows k..n-1, lowest row on ties; the same in every
+// wavefront), swaps rows k and p of its own column and applies the step:
+// column j != k: t = b_kj / piv, b_kj = t, b_ij -= c_i t; column k: b_kk =
+// 1 / piv, b_ik = -c_i / piv (c = pivot column after the swap). The owner of
+// column k + 1 writes its updated column to col_out for the next step.
+__global__ __launch_bounds__(256) void k_gj_step(int n, int k, double* __restrict__ B,
+                                                 const double* __restrict__ col_in, double* __restrict__ col_out,
+                                                 int* __restrict__ perm) {
+  const int j = blockIdx.x * 4 + (int)threadIdx.x / 64, l = threadIdx.x % 64;
+  if(j >= n) return;
   double best = -1.0;
-  int bi = k;
-  for(int i = k + threadIdx.x; i < n; i += 256) {
-    double v = fabs(A[(size_t)i * n + k]);
+  int bi = n;
+  for(int i = k + l; i < n; i += 64) {
+    const double v = fabs(col_in[i]);
     if(v > best) { best = v; bi = i; }
   }
-  s_v[threadIdx.x] = best;
-  s_i[threadIdx.x] = bi;
-  __syncthreads();
-  for(int off = 128; off > 0; off >>= 1) {
-    if(threadIdx.x < off) {
-      double v2 = s_v[threadIdx.x + off];
-      int i2 = s_i[threadIdx.x + off];
-      if(v2 > s_v[threadIdx.x] || (v2 == s_v[threadIdx.x] && i2 < s_i[threadIdx.x])) {
-        s_v[threadIdx.x] = v2;
-        s_i[threadIdx.x] = i2;
+  for(int off = 32; off > 0; off >>= 1) {
+    const double v2 = __shfl_xor(best, off, 64);
+    const int i2 = __shfl_xor(bi, off, 64);
+    if(v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
+  }
+  const int p = bi < n ? bi : k;  // (a column of NaNs has no pivot; stay in bounds)
+  const double ck = col_in[k], cp = col_in[p];
+  const double piv = cp;
+  double* a = B + (size_t)j * n;
+  if(j == k) {
+    for(int i = l; i < n; i += 64) {
+      const double ci = i == k ? cp : (i == p ? ck : col_in[i]);
+      a[i] = i == k ? 1.0 / piv : -ci / piv;
+    }
+  } else {
+    const double akj = a[p], apj = a[k];
+    const double t = akj / piv;
+    for(int i = l; i < n; i += 64) {
+      double ai;
+      if(i == k) {
+        ai = t;
+      } else {
+        const double ci = i == p ? ck : col_in[i];
+        ai = (i == p ? apj : a[i]) - ci * t;
       }
-    }
-    __syncthreads();
-  }
-  if(threadIdx.x == 0) piv[0] = s_i[0];
-}
-
-// swap rows k and piv, then save column k and the pivot value
-__global__ void k_swap_save(int n, int k, double* __restrict__ A, double* __restrict__ I,
-                            const int* __restrict__ piv, double* __restrict__ colk) {
-  const int p = piv[0];
-  for(int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    if(p != k) {
-      double t = A[(size_t)k * n + j]; A[(size_t)k * n + j] = A[(size_t)p * n + j]; A[(size_t)p * n + j] = t;
-      t = I[(size_t)k * n + j]; I[(size_t)k * n + j] = I[(size_t)p * n + j]; I[(size_t)p * n + j] = t;
+      a[i] = ai;
+      if(j == k + 1) col_out[i] = ai;
     }
   }
+  if(j == 0 && l == 0) perm[k] = p;
 }
 
-__global__ void k_save_col(int n, int k, const double* __restrict__ A, double* __restrict__ colk) {
-  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-    colk[i] = A[(size_t)i * n + k];
-}
-
-__global__ void k_scale_row(int n, int k, double* __restrict__ A, double* __restrict__ I,
-                            const double* __restrict__ colk) {
-  const double pv = colk[k];
-  for(int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    A[(size_t)k * n + j] = A[(size_t)k * n + j] / pv;
-    I[(size_t)k * n + j] = I[(size_t)k * n + j] / pv;
+// column order of the inverse: the row interchanges undone as column
+// interchanges in reverse order (one thread)
+__global__ void k_gj_colmap(int n, const int* __restrict__ perm, int* __restrict__ colmap) {
+  if(blockIdx.x != 0 || threadIdx.x != 0) return;
+  for(int j = 0; j < n; ++j) colmap[j] = j;
+  for(int k = n - 1; k >= 0; --k) {
+    const int p = perm[k], t = colmap[k];
+    colmap[k] = colmap[p];
+    colmap[p] = t;
   }
 }
 
-__global__ void k_eliminate(int n, int k, double* __restrict__ A, double* __restrict__ I,
-                            const double* __restrict__ colk) {
-  const int i = blockIdx.y;
-  if(i == k) return;
-  const double f = colk[i];
-  if(f == 0.0) return;
-  for(int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    A[(size_t)i * n + j] -= f * A[(size_t)k * n + j];
-    I[(size_t)i * n + j] -= f * I[(size_t)k * n + j];
-  }
-}
-
-__global__ void k_identity(int n, double* __restrict__ I) {
+// row-major inverse: inv[i][j] = B[i][colmap[j]] (B column-major)
+__global__ void k_gj_store(int n, const double* __restrict__ B, const int* __restrict__ colmap,
+                           double* __restrict__ inv) {
   const size_t nn = (size_t)n * n;
-  for(size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nn; t += (size_t)gridDim.x * blockDim.x)
-    I[t] = (t / n == t % n) ? 1.0 : 0.0;
+  for(size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < nn; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = t / n, j = t % n;
+    inv[t] = B[(size_t)colmap[j] * n + i];
+  }
 }
 
 }  // namespace
@@ -766,18 +768,16 @@ void launch_csr_to_dense(int n, const int* rp, const int* col, const double* val
   k_csr_to_dense<<<n, 64, 0, s>>>(n, rp, col, val, M);
 }
 
-void dense_inverse(int n, double* A, double* I, double* work, int* piv, hipStream_t s) {
+void dense_inverse(int n, double* B, double* inv, double* colbuf, int* perm, int* colmap, hipStream_t s) {
   if(!n) return;
-  const int g = std::max(1, std::min(1024, (n * n + 255) / 256));
-  k_identity<<<g, 256, 0, s>>>(n, I);
-  const int gx = (n + 255) / 256;
-  for(int k = 0; k < n; ++k) {
-    k_pivot<<<1, 256, 0, s>>>(n, k, A, piv);
-    k_swap_save<<<gx, 256, 0, s>>>(n, k, A, I, piv, work);
-    k_save_col<<<gx, 256, 0, s>>>(n, k, A, work);
-    k_scale_row<<<gx, 256, 0, s>>>(n, k, A, I, work);
-    k_eliminate<<<dim3(gx, n), 256, 0, s>>>(n, k, A, I, work);
-  }
+  launch_copy(n, B, colbuf, s);  // column 0 = the first pivot column
+  const int blocks = (n + 3) / 4;
+  for(int k = 0; k < n; ++k)
+    k_gj_step<<<blocks, 256, 0, s>>>(n, k, B, colbuf + (size_t)(k & 1) * n, colbuf + (size_t)((k + 1) & 1) * n,
+                                     perm);
+  k_gj_colmap<<<1, 64, 0, s>>>(n, perm, colmap);
+  const long nn = (long)n * n;
+  k_gj_store<<<(int)std::max(1L, std::min((nn + 255) / 256, 8192L)), 256, 0, s>>>(n, B, colmap, inv);
   PMCD_HIP(hipGetLastError());
 }
 

@@ -167,12 +167,13 @@ void launch_scale_inplace(int n, double a, double* y, hipStream_t s);
 void launch_div(int n, const double* x, const double* d, double* y, hipStream_t s);
 // dst[i] = src[idx[i]]
 void launch_gather_idx(int n, const int* idx, const double* src, double* dst, hipStream_t s);
-// scatter CSR rows into a dense row-major n x n matrix (zero-filled first)
+// scatter CSR rows into a dense column-major n x n matrix (zero-filled first)
 void launch_csr_to_dense(int n, const int* rp, const int* col, const double* val, double* M,
                          hipStream_t s);
-// in-place Gauss-Jordan inverse with partial pivoting: A is overwritten,
-// Ainv receives the inverse; work holds n + 2 doubles
-void dense_inverse(int n, double* A, double* Ainv, double* work, int* ipiv_dev, hipStream_t s);
+// Gauss-Jordan inverse with partial pivoting, one launch per pivot: B
+// (column-major, overwritten) -> inv (row-major); colbuf holds 2 n doubles,
+// perm and colmap n ints each
+void dense_inverse(int n, double* B, double* inv, double* colbuf, int* perm, int* colmap, hipStream_t s);
 
 // ---- batched (segmented) vector kernels ------------------------------------
 // A chunk is a row range [begin, end) owned by one instance.

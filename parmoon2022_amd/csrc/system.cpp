@@ -383,12 +383,11 @@ void CDSystem::coarse_factorize() {
   Level& C = *levels_.front();
   const int n = C.n();
   PMCD_CHECK(n <= 20000, "coarse direct solve limited to 20000 unknowns, got ", n);
-  DevBuf<double> A((size_t)n * n);
+  DevBuf<double> A((size_t)n * n), colbuf(2 * (size_t)n);
+  DevBuf<int> perm(n), colmap(n);
   if(coarse_inv_.n != (size_t)n * n) coarse_inv_.alloc((size_t)n * n);
-  DevBuf<double> work(n + 2);
-  DevBuf<int> piv(1);
   launch_csr_to_dense(n, C.row_ptr.p, C.col.p, C.val.p, A.p, stream_);
-  dense_inverse(n, A.p, coarse_inv_.p, work.p, piv.p, stream_);
+  dense_inverse(n, A.p, coarse_inv_.p, colbuf.p, perm.p, colmap.p, stream_);
   if(coarse_tmp_.n != 2 * (size_t)n) coarse_tmp_.alloc(2 * (size_t)n);
   PMCD_HIP(hipStreamSynchronize(stream_));
   coarse_ready_ = true;

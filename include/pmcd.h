@@ -222,7 +222,8 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
  * read when a level's sweep order is first built; key 6 = 1 to replay
  * multigrid V-cycles from captured HIP graphs, 0 to launch them kernel by
  * kernel (default); key 4 = assembly kernel (0 reference operation order on
- * vector ALUs, 1 FP64 MFMA); key 7 = 1 (default) to run CG with no or Jacobi
+ * vector ALUs, 1 FP64 MFMA for 27-DOF hexahedra and the vector kernels for the
+ * other elements); key 7 = 1 (default) to run CG with no or Jacobi
  * preconditioning as two fused kernels per iteration with its scalars on the
  * device, 0 for the loop that reads every scalar back to the host; key 8 =
  * the largest grid of the device CG's SpMV kernel (1..4096 blocks). */

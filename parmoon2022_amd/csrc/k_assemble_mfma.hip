@@ -291,26 +291,16 @@ __global__ void k_mfma_probe(double* out) {
 
 bool launch_assemble_mfma(const AsmArgs& a, hipStream_t s) {
   if(a.n_cells == 0) return true;
-  // P1 simplices: 3 or 4 local DOFs fill too little of a 16x16 tile; the
-  // thread-per-cell vector kernel is faster
-  if(p1_simplex(a)) return false;
+  // Only a local matrix that fills the 16x16 tiles pays for the MFMA path:
+  // measured per element (SUPG, 2^18-2^21 cells), the vector kernels are
+  // faster for every element with fewer than 16 local DOFs (Q1 hexahedra
+  // 12.9 vs 33.2 ms, P2 tetrahedra 1.2 vs 1.7 ms, P2 triangles 2.1 vs 3.6 ms,
+  // Q2 quadrilaterals within 5%), the MFMA kernel for Q2 hexahedra (10.0 vs
+  // 27.7 ms at 64 points).
+  if(a.nl < 16) return false;
   const int blocks = (a.n_cells + kCells - 1) / kCells;
-  if(a.dim == 3 && a.ncomp == 10) {
-    switch(a.nl) {
-      case 4: k_assemble_mfma<3, 4><<<blocks, 256, 0, s>>>(a); break;
-      case 8: k_assemble_mfma<3, 8><<<blocks, 256, 0, s>>>(a); break;
-      case 10: k_assemble_mfma<3, 10><<<blocks, 256, 0, s>>>(a); break;
-      case 27: k_assemble_mfma<3, 27><<<blocks, 256, 0, s>>>(a); break;
-      default: return false;
-    }
-  } else if(a.dim == 2 && a.ncomp == 6) {
-    switch(a.nl) {
-      case 3: k_assemble_mfma<2, 3><<<blocks, 256, 0, s>>>(a); break;
-      case 4: k_assemble_mfma<2, 4><<<blocks, 256, 0, s>>>(a); break;
-      case 6: k_assemble_mfma<2, 6><<<blocks, 256, 0, s>>>(a); break;
-      case 9: k_assemble_mfma<2, 9><<<blocks, 256, 0, s>>>(a); break;
-      default: return false;
-    }
+  if(a.dim == 3 && a.ncomp == 10 && a.nl == 27) {
+    k_assemble_mfma<3, 27><<<blocks, 256, 0, s>>>(a);
   } else {
     return false;
   }

@@ -37,7 +37,8 @@ void launch_assemble_local(const AsmArgs& a, hipStream_t s);
 // FP64-MFMA variant; returns false if the element is not covered
 bool launch_assemble_mfma(const AsmArgs& a, hipStream_t s);
 void launch_mfma_probe(double* out, hipStream_t s);
-// assembly variant: 0 reference-order vector kernel, 1 MFMA (default)
+// assembly variant: 0 reference-order vector kernels, 1 (default) the MFMA
+// kernel for 27-DOF hexahedra and the vector kernels for the other elements
 int assembly_variant();
 
 // Sum local matrices into CSR rows in increasing cell order; Dirichlet rows

@@ -17,8 +17,8 @@ extern "C" {
 typedef struct pmcd_system pmcd_system;
 
 /* Geometry codes: default coarse meshes of the reference
- * (src/Geometry/Tests.C:95,473,1893,1937) refined n_refine times, or
- * structured N^3 cubes. */
+ * (src/Geometry/Tests.C:95,473,1893,1937), structured N^3 cubes, or a 3D
+ * .mesh file; each refined n_refine times. */
 enum {
   PMCD_UNIT_SQUARE = 0,
   PMCD_TWO_TRIANGLES = 1,
@@ -26,7 +26,8 @@ enum {
   PMCD_CUBE_HEXA = 3,
   PMCD_CUBE_TETRA = 4,
   PMCD_STRUCTURED_HEXA = 10,
-  PMCD_STRUCTURED_TETRA = 11
+  PMCD_STRUCTURED_TETRA = 11,
+  PMCD_MESH_FILE = 12
 };
 
 /* Example codes (PDE data, include/Examples/CD_2D/SineLaplace.h,
@@ -55,6 +56,9 @@ typedef struct {
   double b[3];      /* convection */
   double c;         /* reaction (PMCD_EX_CONST_RHS) */
   double f0;        /* right-hand side (PMCD_EX_CONST_RHS) */
+  const char* mesh_file; /* PMCD_MESH_FILE: path of a 3D medit .mesh file (geo_file ending in
+                            .mesh, read as TDomain::InitFromMesh, src/Geometry/Domain.C:257-270,
+                            878-992, 2912-3330); NULL otherwise */
 } pmcd_problem_desc;
 
 /* solver_type / iterative_solver_type: direct, cg, fgmres, richardson, and the
@@ -113,6 +117,11 @@ int pmcd_host_info(pmcd_host* h, int level, long long* info);
 int pmcd_host_dofmap(pmcd_host* h, int level, int* out);
 int pmcd_host_pattern(pmcd_host* h, int level, int* row_ptr, int* col);
 int pmcd_host_cell_coords(pmcd_host* h, int level, double* out);
+/* Writes the level's mesh as a 3D medit .mesh file (vertices, boundary faces
+ * with marker = boundary component + 1, cells), the format TCollection::
+ * writeMesh writes (src/Geometry/Collection.C:806-); it reads back through
+ * PMCD_MESH_FILE to the same cells, vertices and boundary joints. */
+int pmcd_host_write_mesh(pmcd_host* h, int level, const char* path);
 /* Row classes of the device SOR sweep for a level (host computation): rows
  * (n_dof ints) grouped by class, class_ptr (n_dof + 1 ints, may be NULL; the
  * first n_classes + 1 are set). exact = 1 when the classes are the level

@@ -72,6 +72,8 @@ class Oracle:
     """Reference-order CPU problem: same constructor arguments as ProblemDesc."""
 
     def __init__(self, desc):
+        if desc.geometry == 12:
+            raise ValueError("the oracle builds default and structured meshes only, not mesh files")
         b = (ctypes.c_double * 3)(*list(desc.b))
         self.desc = desc
         self._p = ctypes.c_void_p(lib().orc_create(desc.geometry, desc.structured_n, desc.n_refine,

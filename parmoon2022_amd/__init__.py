@@ -22,6 +22,7 @@ _lib = None
 GEOMETRIES = {"UnitSquare": 0, "TwoTriangles": 1, "TwoTriangles2": 2,
               "Default_UnitCube_Hexa": 3, "Default_UnitCube_Tetra": 4,
               "StructuredHexa": 10, "StructuredTetra": 11}
+MESH_FILE = 12
 EXAMPLES = {"sine_laplace_2d": 0, "sine_2d": 1, "sine_3d": 2, "test_p0": 3,
             "test_p1": 4, "test_p2": 5, "const_rhs": 6}
 METHODS = {"direct": 0, "cg": 1, "fgmres": 2, "richardson": 3, "sor": 4, "ssor": 5}
@@ -35,7 +36,7 @@ class ProblemDesc(ctypes.Structure):
                 ("supg", ctypes.c_int), ("linear_quad", ctypes.c_int),
                 ("mg_levels", ctypes.c_int), ("example", ctypes.c_int),
                 ("eps", ctypes.c_double), ("b", ctypes.c_double * 3),
-                ("c", ctypes.c_double), ("f0", ctypes.c_double)]
+                ("c", ctypes.c_double), ("f0", ctypes.c_double), ("mesh_file", ctypes.c_char_p)]
 
 
 class SolverDesc(ctypes.Structure):
@@ -58,6 +59,7 @@ EXPORTS = [
     "pmcd_last_error", "pmcd_device_count", "pmcd_mem_info", "pmcd_system_set_data",
     "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
     "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_sweep_order", "pmcd_host_sell_layout",
+    "pmcd_host_write_mesh",
     "pmcd_jacobi_sweep",
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
     "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
@@ -189,7 +191,14 @@ def _ptr(a, ctype):
 def problem_desc(db):
     """Build a ProblemDesc from a parameter dictionary (reference key names)."""
     d = ProblemDesc()
-    d.geometry = GEOMETRIES[db.get("geo_file", "UnitSquare")]
+    geo = db.get("geo_file", "UnitSquare")
+    if str(geo).endswith(".mesh"):
+        # the struct holds the path; keep the bytes alive with it
+        d._mesh_path = os.fsencode(os.path.abspath(geo))
+        d.geometry = MESH_FILE
+        d.mesh_file = d._mesh_path
+    else:
+        d.geometry = GEOMETRIES[geo]
     d.n_refine = int(db.get("refinement_n_initial_steps", 0))
     d.structured_n = int(db.get("structured_n", 1))
     d.order = int(db.get("ANSATZ_ORDER", 1))
@@ -291,9 +300,16 @@ class HostSetup:
         k = ns.value
         return perm[:k].copy(), ln[:k].copy(), off[:k // 64 + 1].copy()
 
+    def write_mesh(self, path, level=-1):
+        """The level's mesh as a 3D .mesh file (readable as geo_file)."""
+        _check(lib().pmcd_host_write_mesh(self._h, level, os.fsencode(path)))
+
     def cell_coords(self, level=-1):
         i = self.info(level)
-        nv = VERTICES_PER_CELL[self.desc.geometry]
+        if self.desc.geometry == MESH_FILE:  # tetrahedra (P1 4, P2 10 DOFs) or hexahedra (8, 27)
+            nv = {4: 4, 10: 4, 8: 8, 27: 8}[i["n_local"]]
+        else:
+            nv = VERTICES_PER_CELL[self.desc.geometry]
         out = np.empty(i["n_cells"] * nv * 3, dtype=np.float64)
         _check(lib().pmcd_host_cell_coords(self._h, level, _ptr(out, ctypes.c_double)))
         return out.reshape(i["n_cells"], nv, 3)

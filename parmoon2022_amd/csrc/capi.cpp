@@ -55,6 +55,10 @@ ProblemDesc to_desc(const pmcd_problem_desc& d) {
   for(int k = 0; k < 3; ++k) p.ex.b[k] = d.b[k];
   p.ex.c = d.c;
   p.ex.f0 = d.f0;
+  if(d.geometry == PMCD_MESH_FILE) {
+    PMCD_CHECK(d.mesh_file != nullptr, "geometry PMCD_MESH_FILE needs mesh_file");
+    p.mesh_file = d.mesh_file;
+  }
   return p;
 }
 
@@ -171,6 +175,13 @@ int pmcd_host_sell_layout(pmcd_host* h, int level, int* perm, int* len, long lon
     std::memcpy(len, o.len.data(), o.len.size() * sizeof(int));
     std::memcpy(slice_off, o.slice_off.data(), o.slice_off.size() * sizeof(long long));
     *n_slots = o.n_slots();
+  });
+}
+
+int pmcd_host_write_mesh(pmcd_host* h, int level, const char* path) {
+  return guarded([&] {
+    PMCD_CHECK(path != nullptr, "null path");
+    write_mesh_file(host_level(h, level).mesh, path);
   });
 }
 

@@ -1,6 +1,11 @@
 #include "mesh.hpp"
 #include <algorithm>
+#include <array>
 #include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <map>
+#include <sstream>
 #include <unordered_map>
 
 namespace pmcd {
@@ -289,6 +294,194 @@ double cell_diameter(const Mesh& m, int c) {
       break;
   }
   return std::sqrt(r);
+}
+
+namespace {
+
+// whitespace-separated tokens of a .mesh file, '#' comments removed
+std::vector<std::string> mesh_tokens(const std::string& path) {
+  std::ifstream in(path);
+  PMCD_CHECK(in.good(), "cannot open mesh file ", path);
+  std::vector<std::string> tok;
+  std::string line;
+  while(std::getline(in, line)) {
+    const size_t h = line.find('#');
+    if(h != std::string::npos) line.erase(h);
+    std::istringstream ls(line);
+    std::string t;
+    while(ls >> t) tok.push_back(t);
+  }
+  return tok;
+}
+
+}  // namespace
+
+Mesh read_mesh_file(const std::string& path) {
+  const std::vector<std::string> tok = mesh_tokens(path);
+  size_t p = 0;
+  auto next = [&]() -> const std::string& {
+    PMCD_CHECK(p < tok.size(), "unexpected end of mesh file ", path);
+    return tok[p++];
+  };
+  auto num_i = [&]() {
+    const std::string& t = next();
+    char* end = nullptr;
+    const long v = std::strtol(t.c_str(), &end, 10);
+    PMCD_CHECK(end && *end == 0, "mesh file ", path, ": expected an integer, got '", t, "'");
+    return (int)v;
+  };
+  // an element's reference number; like the reference's stream reader, a
+  // missing one at the end of a section (the next token is not a number)
+  // reads as 0
+  auto ref_i = [&]() {
+    if(p >= tok.size()) return 0;
+    char* end = nullptr;
+    const long v = std::strtol(tok[p].c_str(), &end, 10);
+    if(!(end && *end == 0)) return 0;
+    ++p;
+    return (int)v;
+  };
+  auto num_d = [&]() {
+    const std::string& t = next();
+    char* end = nullptr;
+    const double v = std::strtod(t.c_str(), &end);
+    PMCD_CHECK(end && *end == 0, "mesh file ", path, ": expected a number, got '", t, "'");
+    return v;
+  };
+  int dim = 0;
+  std::vector<double> coords;
+  std::vector<int> tets, hexes, tris, quads;  // 1-based ids
+  while(p < tok.size()) {
+    const std::string key = next();
+    if(key == "MeshVersionFormatted") {
+      num_i();
+    } else if(key == "Dimension") {
+      dim = num_i();
+      PMCD_CHECK(dim == 2 || dim == 3, "mesh file ", path, ": dimension ", dim);
+    } else if(key == "Vertices") {
+      PMCD_CHECK(dim != 0, "mesh file ", path, ": Vertices before Dimension");
+      const int n = num_i();
+      PMCD_CHECK(n >= 0, "mesh file ", path, ": negative vertex count");
+      coords.assign((size_t)3 * n, 0.0);
+      for(int v = 0; v < n; ++v) {
+        for(int k = 0; k < dim; ++k) coords[3 * (size_t)v + k] = num_d();
+        num_i();  // vertex reference
+      }
+    } else if(key == "Edges" || key == "Triangles" || key == "Quadrilaterals" || key == "Tetrahedra" ||
+              key == "Hexahedra") {
+      const int nn = key == "Edges" ? 2 : key == "Triangles" ? 3 : key == "Tetrahedra" || key == "Quadrilaterals" ? 4 : 8;
+      std::vector<int>* dst = key == "Triangles" ? &tris : key == "Quadrilaterals" ? &quads
+                              : key == "Tetrahedra" ? &tets : key == "Hexahedra" ? &hexes : nullptr;
+      const int n = num_i();
+      PMCD_CHECK(n >= 0, "mesh file ", path, ": negative count in ", key);
+      for(int e = 0; e < n; ++e) {
+        for(int k = 0; k < nn; ++k) {
+          const int id = num_i();
+          if(dst) dst->push_back(id);
+        }
+        const int ref = ref_i();
+        if(dst) dst->push_back(ref);
+      }
+    } else if(key == "End") {
+      break;
+    } else {
+      PMCD_FAIL("mesh file ", path, ": unsupported section '", key, "'");
+    }
+  }
+  PMCD_CHECK(!(tets.empty() && hexes.empty()), "mesh file ", path,
+             ": no Tetrahedra or Hexahedra (2D meshes need a .PRM boundary description, not supported)");
+  PMCD_CHECK(tets.empty() || hexes.empty(), "mesh file ", path, ": mixed tetrahedra and hexahedra");
+  Mesh m;
+  const bool tetra = !tets.empty();
+  m.shape = tetra ? Shape::Tetrahedron : Shape::Hexahedron;
+  m.nv = tetra ? 4 : 8;
+  m.coords = std::move(coords);
+  const int nvert = m.n_vertices();
+  const std::vector<int>& cells = tetra ? tets : hexes;
+  m.n_cells = (int)(cells.size() / (m.nv + 1));
+  m.cell_vertices.resize((size_t)m.n_cells * m.nv);
+  for(int c = 0; c < m.n_cells; ++c)
+    for(int k = 0; k < m.nv; ++k) {
+      const int id = cells[(size_t)c * (m.nv + 1) + k];
+      PMCD_CHECK(id >= 1 && id <= nvert, "mesh file ", path, ": vertex id ", id, " out of range");
+      m.cell_vertices[(size_t)c * m.nv + k] = id - 1;
+    }
+  // faces: a face of one cell is a boundary joint
+  const auto& faces = shape_faces(m.shape);
+  m.nj = (int)faces.size();
+  const int fn = tetra ? 3 : 4;
+  auto key_of = [&](const int* v) {
+    std::array<int, 4> k = {v[0], v[1], v[2], fn == 4 ? v[3] : -1};
+    std::sort(k.begin(), k.begin() + fn);
+    return k;
+  };
+  std::map<std::array<int, 4>, int> count;
+  for(int c = 0; c < m.n_cells; ++c)
+    for(int j = 0; j < m.nj; ++j) {
+      int v[4] = {-1, -1, -1, -1};
+      for(int a = 0; a < fn; ++a) v[a] = m.cell(c)[faces[j][a]];
+      ++count[key_of(v)];
+    }
+  // boundary components: file faces on the boundary, in file order
+  std::map<std::array<int, 4>, int> comp_of;
+  int n_comp = 0;
+  const std::vector<int>& bf = tetra ? tris : quads;
+  for(size_t f = 0; f < bf.size(); f += fn + 1) {
+    int v[4] = {-1, -1, -1, -1};
+    for(int a = 0; a < fn; ++a) {
+      PMCD_CHECK(bf[f + a] >= 1 && bf[f + a] <= nvert, "mesh file ", path, ": face vertex out of range");
+      v[a] = bf[f + a] - 1;
+    }
+    const auto k = key_of(v);
+    auto it = count.find(k);
+    if(it != count.end() && it->second == 1 && !comp_of.count(k)) comp_of[k] = n_comp++;
+  }
+  m.joint_comp.assign((size_t)m.n_cells * m.nj, -1);
+  for(int c = 0; c < m.n_cells; ++c)
+    for(int j = 0; j < m.nj; ++j) {
+      int v[4] = {-1, -1, -1, -1};
+      for(int a = 0; a < fn; ++a) v[a] = m.cell(c)[faces[j][a]];
+      const auto k = key_of(v);
+      const int cnt = count[k];
+      PMCD_CHECK(cnt <= 2, "mesh file ", path, ": a face shared by ", cnt, " cells");
+      if(cnt == 2) continue;
+      auto it = comp_of.find(k);
+      if(it == comp_of.end()) it = comp_of.emplace(k, n_comp++).first;
+      m.joint_comp[(size_t)c * m.nj + j] = it->second;
+    }
+  m.parent.assign(m.n_cells, -1);
+  m.child_index.assign(m.n_cells, 0);
+  return m;
+}
+
+void write_mesh_file(const Mesh& m, const std::string& path) {
+  PMCD_CHECK(m.dim() == 3, "only 3D meshes are written");
+  std::ofstream out(path);
+  PMCD_CHECK(out.good(), "cannot write mesh file ", path);
+  out.precision(17);
+  out << "MeshVersionFormatted 1\n\nDimension 3\n\nVertices\n" << m.n_vertices() << "\n";
+  for(int v = 0; v < m.n_vertices(); ++v)
+    out << m.coords[3 * v] << " " << m.coords[3 * v + 1] << " " << m.coords[3 * v + 2] << " 0\n";
+  const auto& faces = shape_faces(m.shape);
+  const bool tetra = m.shape == Shape::Tetrahedron;
+  std::ostringstream bf;
+  int n_bf = 0;
+  for(int c = 0; c < m.n_cells; ++c)
+    for(int j = 0; j < m.nj; ++j) {
+      const int comp = m.joint_comp[(size_t)c * m.nj + j];
+      if(comp < 0) continue;
+      for(int lv : faces[j]) bf << m.cell(c)[lv] + 1 << " ";
+      bf << comp + 1 << "\n";
+      ++n_bf;
+    }
+  out << "\n" << (tetra ? "Triangles" : "Quadrilaterals") << "\n" << n_bf << "\n" << bf.str();
+  out << "\n" << (tetra ? "Tetrahedra" : "Hexahedra") << "\n" << m.n_cells << "\n";
+  for(int c = 0; c < m.n_cells; ++c) {
+    for(int k = 0; k < m.nv; ++k) out << m.cell(c)[k] + 1 << " ";
+    out << "1\n";
+  }
+  out << "\nEnd\n";
+  PMCD_CHECK(out.good(), "error writing mesh file ", path);
 }
 
 }  // namespace pmcd

@@ -39,6 +39,20 @@ Mesh default_mesh(DefaultGeometry g);
 // six tetrahedra of the default unit-cube tetrahedral mesh.
 Mesh structured_cube(int n, bool tetra);
 
+// 3D mesh in the medit .mesh format, read as TDomain::InitFromMesh without a
+// .PRM file reads it (src/Geometry/Mesh.C:48-260, Domain.C:2912-3330):
+// vertices and cells (Tetrahedra or Hexahedra) in file order, each cell's
+// vertices in file order (1-based ids). Every boundary face is its own
+// boundary component, numbered in the order of the file's boundary faces
+// (Triangles / Quadrilaterals); boundary faces the file does not list follow
+// in cell and local face order. Cells with 2D shapes need a .PRM boundary
+// description and are rejected.
+Mesh read_mesh_file(const std::string& path);
+// Writes a 3D mesh in the same format: vertices, the boundary faces (one per
+// boundary joint, in cell and local face order, marker = component + 1) and
+// the cells.
+void write_mesh_file(const Mesh& m, const std::string& path);
+
 // One step of regular refinement of every cell (RefTriRegDesc.C,
 // RefQuadRegDesc.C, RefTetraRegDesc.C, RefHexaRegDesc.C child tables).
 Mesh refine_regular(const Mesh& coarse);

@@ -23,7 +23,7 @@
 namespace pmcd {
 
 struct ProblemDesc {
-  int geometry;       // DefaultGeometry, or 10 = structured hexa cube, 11 = structured tetra cube
+  int geometry;       // DefaultGeometry, 10 = structured hexa cube, 11 = structured tetra cube, 12 = mesh file
   int n_refine;       // regular refinement steps of the default mesh
   int structured_n;   // cells per edge for the structured cubes
   int order;          // ansatz order 1 or 2
@@ -31,6 +31,7 @@ struct ProblemDesc {
   int linear_quad;    // 1: quadrature degree 2p (INTERNAL_PROBLEM_LINEAR), else 3p
   int mg_levels;      // number of levels kept (1 = finest only)
   ExampleParams ex;
+  std::string mesh_file;  // geometry 12: path of a 3D .mesh file
 };
 
 struct SolverDesc {

@@ -41,7 +41,15 @@ def estimate_bytes(db):
     geo = db.get("geo_file")
     k = int(db.get("refinement_n_initial_steps", 0))
     p = int(db.get("ANSATZ_ORDER", 1))
-    if geo in ("Default_UnitCube_Hexa", "Default_UnitCube_Tetra", "StructuredHexa", "StructuredTetra"):
+    if str(geo).endswith(".mesh"):
+        # counts of the file's coarse mesh; refinement multiplies cells by 8
+        nvert, ncell, hexa = _mesh_file_counts(geo)
+        cells = ncell * 8 ** k
+        nl = {(False, 1): 4, (False, 2): 10, (True, 1): 8, (True, 2): 27}[(hexa, p)]
+        n = int(nvert * 8 ** k * (1 if p == 1 else (8 if hexa else 7)))
+        row = {4: 27, 10: 65, 8: 27, 27: 125}[nl]
+        levels = 1.15
+    elif geo in ("Default_UnitCube_Hexa", "Default_UnitCube_Tetra", "StructuredHexa", "StructuredTetra"):
         m = int(db.get("structured_n", 1)) * 2 ** k if geo.startswith("Structured") else 2 ** k
         n = (p * m + 1) ** 3
         cells = m ** 3 * (6 if "Tetra" in geo else 1)
@@ -60,6 +68,21 @@ def estimate_bytes(db):
     nnz = n * row
     per_level = (nnz * (8 + 4 + 4) + cells * nl * nl * 9 + cells * nl * 16 + n * 8 * (2 * restart + 20))
     return int(per_level * levels) + (64 << 20)
+
+
+def _mesh_file_counts(path):
+    """(vertices, cells, hexahedral) of a .mesh file, from its section counts."""
+    nvert = ncell = 0
+    hexa = False
+    with open(path) as f:
+        tok = f.read().split()
+    for i, t in enumerate(tok[:-1]):
+        if t == "Vertices":
+            nvert = int(tok[i + 1])
+        elif t in ("Tetrahedra", "Hexahedra"):
+            ncell = int(tok[i + 1])
+            hexa = t == "Hexahedra"
+    return nvert, ncell, hexa
 
 
 class Request:

@@ -774,3 +774,26 @@ def test_sell_copy_refreshed_when_switched_on_after_assembly(oracle_lib):
         cd.close()
     finally:
         pm.set_tuning("spmv", *old_s)
+
+
+def test_mesh_file_solve_equals_generated_mesh(tmp_path):
+    """Assembly and FGMRES-MG solve on a coarse mesh read from a .mesh file
+    (written from the default unit-cube tetrahedra) give the bits of the
+    same problem on the generated mesh: same matrix, solution and errors."""
+    base = pm.HostSetup({"geo_file": "Default_UnitCube_Tetra", "ANSATZ_ORDER": 2, "example": "sine_3d"})
+    path = str(tmp_path / "cube_tetra.mesh")
+    base.write_mesh(path)
+    db = {"ANSATZ_ORDER": 2, "refinement_n_initial_steps": 2, "example": "sine_3d",
+          "space_discretization_type": "supg", "diffusion_coefficient": 1e-2, "convection": B_UNIT,
+          "solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "multigrid",
+          "multigrid_n_levels": 2, "jacobi_damp": 0.5, "residual_reduction": 1e-10, "residual_tolerance": 0.0,
+          "max_n_iterations": 200}
+    out = []
+    for geo in ("Default_UnitCube_Tetra", path):
+        cd = pm.ConvectionDiffusion(dict(db, geo_file=geo))
+        cd.assemble()
+        st = cd.solve()
+        out.append((cd.matrix_values(), st.iterations, cd.solution(), cd.output()))
+        cd.close()
+    (va, ia, ua, ea), (vb, ib, ub, eb) = out
+    assert np.array_equal(va, vb) and ia == ib and np.array_equal(ua, ub) and np.array_equal(ea, eb)

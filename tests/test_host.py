@@ -189,3 +189,62 @@ def _check_sell_layout(h, rp, col, na, window):
     sell_col[pos] = col[rp[perm[t]] + k]
     back = sell_col[pos]
     assert np.array_equal(back, col[rp[perm[t]] + k])
+
+
+MESH_ROUND_TRIP = [("StructuredTetra", 1, 1, 3), ("StructuredHexa", 2, 1, 2), ("Default_UnitCube_Tetra", 2, 2, 1),
+                   ("Default_UnitCube_Hexa", 1, 2, 1)]
+
+
+@pytest.mark.parametrize("geo,order,nref,sn", MESH_ROUND_TRIP, ids=lambda v: str(v))
+def test_mesh_file_round_trip(tmp_path, geo, order, nref, sn):
+    """Like the reference's mesh3d_test (write the coarse domain to a .mesh
+    file, read it back, refine, solve): a mesh written with write_mesh and
+    read as geo_file gives the same refined cells, coordinates, DOF map and
+    sparsity as the generated mesh."""
+    base = pm.HostSetup({"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 0,
+                         "structured_n": sn, "example": "sine_3d"})
+    path = str(tmp_path / "coarse.mesh")
+    base.write_mesh(path)
+    a = pm.HostSetup({"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": nref,
+                      "structured_n": sn, "example": "sine_3d"})
+    b = pm.HostSetup({"geo_file": path, "ANSATZ_ORDER": order, "refinement_n_initial_steps": nref,
+                      "example": "sine_3d"})
+    assert a.info() == b.info()
+    assert np.array_equal(a.dofmap(), b.dofmap())
+    for x, y in zip(a.pattern(), b.pattern()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.cell_coords(), b.cell_coords())
+
+
+def test_reference_tetrahedron_mesh_file():
+    """data/mesh/ReferenceTetrahedron.mesh of the reference (one tetrahedron,
+    its four faces as boundary triangles; the cell's reference number is
+    missing before End, which the reference's reader takes as 0): P1 has 4
+    Dirichlet DOFs; P2 after one refinement 8 cells and 35 DOFs of which one
+    (the inner edge of the refined octahedron) is active; volumes sum to 1/6."""
+    path = os.path.join(os.path.dirname(__file__), "golden", "reference_tetrahedron.mesh")
+    h = pm.HostSetup({"geo_file": path, "ANSATZ_ORDER": 1, "example": "sine_3d"})
+    i = h.info()
+    assert (i["n_cells"], i["n_dof"], i["n_active"]) == (1, 4, 0)
+    h = pm.HostSetup({"geo_file": path, "ANSATZ_ORDER": 2, "refinement_n_initial_steps": 1, "example": "sine_3d"})
+    i = h.info()
+    assert (i["n_cells"], i["n_dof"], i["n_active"], i["n_dirichlet"]) == (8, 35, 1, 34)
+    x = h.cell_coords()
+    vol = np.abs(np.linalg.det(x[:, 1:, :] - x[:, :1, :])) / 6
+    assert abs(vol.sum() - 1 / 6) < 1e-15
+
+
+def test_mesh_file_errors(tmp_path):
+    """Missing files, 2D meshes (which need a .PRM boundary file) and vertex
+    ids out of range are reported as errors."""
+    with pytest.raises(RuntimeError, match="cannot open"):
+        pm.HostSetup({"geo_file": str(tmp_path / "missing.mesh"), "ANSATZ_ORDER": 1, "example": "sine_3d"})
+    p2d = tmp_path / "square.mesh"
+    p2d.write_text("MeshVersionFormatted 1\nDimension 2\nVertices\n4\n0 0 1\n1 0 1\n1 1 1\n0 1 1\n"
+                   "Triangles\n2\n1 2 3 0\n1 3 4 0\nEnd\n")
+    with pytest.raises(RuntimeError, match="PRM"):
+        pm.HostSetup({"geo_file": str(p2d), "ANSATZ_ORDER": 1, "example": "sine_2d"})
+    bad = tmp_path / "bad.mesh"
+    bad.write_text("Dimension 3\nVertices\n4\n0 0 0 0\n1 0 0 0\n0 1 0 0\n0 0 1 0\nTetrahedra\n1\n1 2 3 9 0\nEnd\n")
+    with pytest.raises(RuntimeError, match="out of range"):
+        pm.HostSetup({"geo_file": str(bad), "ANSATZ_ORDER": 1, "example": "sine_3d"})

@@ -70,6 +70,8 @@ enum { PMCD_DIRECT = 0, PMCD_CG = 1, PMCD_FGMRES = 2, PMCD_RICHARDSON = 3, PMCD_
 enum { PMCD_NO_PRECOND = 0, PMCD_JACOBI = 1, PMCD_MULTIGRID = 2, PMCD_SOR = 3, PMCD_SSOR = 4 };
 /* multigrid_smoother: jacobi, sor, ssor (Multigrid.C:19-30, MultigridLevel.C:40-66) */
 enum { PMCD_SMOOTH_JACOBI = 0, PMCD_SMOOTH_SOR = 1, PMCD_SMOOTH_SSOR = 2 };
+/* multigrid_cycle_type */
+enum { PMCD_CYCLE_V = 0, PMCD_CYCLE_W = 1, PMCD_CYCLE_F = 2 };
 
 typedef struct {
   int method;          /* solver_type / iterative_solver_type */
@@ -86,6 +88,7 @@ typedef struct {
   int mixed;           /* 1: multigrid V-cycle in FP32 inside the FP64 outer solver */
   int smoother;        /* multigrid_smoother: PMCD_SMOOTH_* */
   double sor_omega;    /* sor_omega (reference default 1.5, Solver.C:154-156) */
+  int cycle;           /* multigrid_cycle_type: PMCD_CYCLE_V, _W, _F (CycleControl.C:17-111) */
 } pmcd_solver_desc;
 
 typedef struct {
@@ -188,7 +191,8 @@ int pmcd_errors(pmcd_system* s, double* out5);
  * of the level's length. */
 int pmcd_apply(pmcd_system* s, int level, const double* x, double* y, double a, int mode);
 /* One multigrid cycle z = MG(v) on the finest level with the given solver
- * settings (Iteration_multigrid::apply, Iteration_multigrid.C:23-34). */
+ * settings (Iteration_multigrid::apply, Iteration_multigrid.C:23-34; V, W or
+ * F cycle by sd->cycle). */
 int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, double* z);
 /* One SOR sweep x <- sweep(b, x) on level `level`, in place on host arrays
  * (TMatrix::sor_sweep, src/Matrix/Matrix.C:1308-1420): flag 0 forward, 1
@@ -237,6 +241,10 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
  * device, 0 for the loop that reads every scalar back to the host; key 8 =
  * the largest grid of the device CG's SpMV kernel (1..4096 blocks). */
 int pmcd_set_tuning(int key, int value);
+/* Directions of one multigrid cycle (CycleControl, src/Multigrid/
+ * CycleControl.C:17-111): writes at most cap entries (0 down, 1 up, 2 end,
+ * 3 the leading dummy) and returns the full count in *n. Host computation. */
+int pmcd_cycle_directions(int type, int n_levels, int* out, int cap, int* n);
 /* Current value of a launch-shape knob (keys as for pmcd_set_tuning). */
 int pmcd_get_tuning(int key, int* value);
 

@@ -23,7 +23,8 @@ class OSolver(ctypes.Structure):
                 ("div_factor", ctypes.c_double), ("restart", ctypes.c_int),
                 ("n_pre", ctypes.c_int), ("n_post", ctypes.c_int),
                 ("jacobi_damp", ctypes.c_double), ("mg_damp", ctypes.c_double),
-                ("mixed", ctypes.c_int), ("smoother", ctypes.c_int), ("sor_omega", ctypes.c_double)]
+                ("mixed", ctypes.c_int), ("smoother", ctypes.c_int), ("sor_omega", ctypes.c_double),
+                ("cycle", ctypes.c_int)]
 
 
 class OStats(ctypes.Structure):

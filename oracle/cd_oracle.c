@@ -1444,6 +1444,7 @@ typedef struct {
   int mixed;
   int smoother; /* multigrid_smoother: 0 jacobi, 1 sor, 2 ssor */
   double sor_omega;
+  int cycle; /* multigrid_cycle_type: 0 V, 1 W, 2 F */
 } osolver;
 
 typedef struct {
@@ -1518,36 +1519,105 @@ static void smooth_step(const osolver* s, const olevel* L, const double* b, doub
   else sor_level(L, b, x, s->sor_omega, s->smoother == 1 ? 0 : 2);
 }
 
-/* V-cycle (Multigrid::cycle with V control) */
+/* Cycle directions of CycleControl (CycleControl.C:17-111): a dummy first
+   entry, then for each visit of a level with recursion count r: r times
+   "down" followed by the coarser level's directions, then "up" ("end" on the
+   finest level); the coarsest level contributes one "up". Recursion counts
+   are 1 (V) or 2 (W, F) except 1 on the finest level; an F cycle resets a
+   level's count to 1 after its first visit. */
+enum { MG_DOWN = 0, MG_UP = 1, MG_END = 2, MG_DUMMY = 3 };
+
+static void cycle_fill(int type, int* rec, int level, int finest, int* dirs, int* n) {
+  if(level == 0) {
+    dirs[(*n)++] = MG_UP;
+    return;
+  }
+  for(int j = 0; j < rec[level]; ++j) {
+    dirs[(*n)++] = MG_DOWN;
+    cycle_fill(type, rec, level - 1, finest, dirs, n);
+  }
+  if(type == 2) rec[level] = 1;
+  dirs[(*n)++] = level == finest ? MG_END : MG_UP;
+}
+
+/* directions into a malloc'ed array; returns its length */
+static int cycle_dirs(int type, int nlev, int** out) {
+  int* rec = malloc(sizeof(int) * nlev);
+  for(int l = 0; l < nlev; ++l) rec[l] = type == 0 ? 1 : 2;
+  rec[nlev - 1] = 1;
+  /* at most 2^nlev visits per level */
+  int cap = 4;
+  for(int l = 0; l < nlev; ++l) cap *= 2;
+  int* dirs = malloc(sizeof(int) * (size_t)cap * (nlev + 1));
+  int n = 0;
+  dirs[n++] = MG_DUMMY;
+  cycle_fill(type, rec, nlev - 1, nlev - 1, dirs, &n);
+  free(rec);
+  *out = dirs;
+  return n;
+}
+
+int orc_cycle_dirs(int type, int nlev, int* out, int cap) {
+  int* d;
+  const int n = cycle_dirs(type, nlev, &d);
+  for(int i = 0; i < n && i < cap; ++i) out[i] = d[i];
+  free(d);
+  return n;
+}
+
+/* prolongate level l-1's iterate and add it, damped, to level l's active rows
+   (Multigrid::update_solution_in_finer_grid, Multigrid.C:357-410) */
+static void mg_prolong_add(orc_problem* P, const osolver* s, int l) {
+  olevel* L = &P->lev[l];
+  olevel* C = &P->lev[l - 1];
+  prolongate(P, C, L, C->mg_sol, L->mg_aux);
+  for(int j = 0; j < L->nactive; ++j) L->mg_sol[j] += s->mg_damp * L->mg_aux[j];
+}
+
+/* Multigrid::cycle / cycle_step (Multigrid.C:157-308) with V, W or F control */
 static void mg_cycle(orc_problem* P, const osolver* s, const double* v, double* z) {
   const int Lf = P->nlev - 1;
   olevel* F = &P->lev[Lf];
   memcpy(F->mg_rhs, v, sizeof(double) * F->ndof);
   memset(F->mg_sol, 0, sizeof(double) * F->nactive);
   memcpy(F->mg_sol + F->nactive, v + F->nactive, sizeof(double) * (F->ndof - F->nactive));
-  for(int l = Lf; l >= 1; --l) {
-    olevel* L = &P->lev[l];
-    olevel* C = &P->lev[l - 1];
-    for(int k = 0; k < s->n_pre; ++k) smooth_step(s, L, L->mg_rhs, L->mg_sol);
-    memcpy(L->mg_def, L->mg_rhs, sizeof(double) * L->ndof);
-    apply_scaled_add(L, L->mg_sol, L->mg_def, -1.0);
-    memset(C->mg_sol, 0, sizeof(double) * C->ndof);
-    restrict_defect(P, C, L, C->mg_rhs, L->mg_def);
-    for(int i = C->nactive; i < C->ndof; ++i) C->mg_rhs[i] = 0;
+  if(P->nlev == 1) {
+    for(int k = 0; k < s->n_pre + s->n_post; ++k) smooth_step(s, F, F->mg_rhs, F->mg_sol);
+    memcpy(z, F->mg_sol, sizeof(double) * F->ndof);
+    return;
   }
-  {
-    olevel* C = &P->lev[0];
-    if(P->nlev > 1) dense_solve(C, C->mg_rhs, C->mg_sol);
-    else
-      for(int k = 0; k < s->n_pre + s->n_post; ++k) smooth_step(s, C, C->mg_rhs, C->mg_sol);
+  int* dirs;
+  const int nd = cycle_dirs(s->cycle, P->nlev, &dirs);
+  int level = Lf;
+  for(int i = 1; i < nd; ++i) {
+    if(level == 0) {
+      /* coarsest level: the direct solve, then up */
+      olevel* C = &P->lev[0];
+      dense_solve(C, C->mg_rhs, C->mg_sol);
+      mg_prolong_add(P, s, 1);
+      level = 1;
+      continue;
+    }
+    olevel* L = &P->lev[level];
+    if(dirs[i - 1] == MG_UP)
+      for(int k = 0; k < s->n_post; ++k) smooth_step(s, L, L->mg_rhs, L->mg_sol);
+    if(dirs[i] == MG_DOWN) {
+      olevel* C = &P->lev[level - 1];
+      for(int k = 0; k < s->n_pre; ++k) smooth_step(s, L, L->mg_rhs, L->mg_sol);
+      memcpy(L->mg_def, L->mg_rhs, sizeof(double) * L->ndof);
+      apply_scaled_add(L, L->mg_sol, L->mg_def, -1.0);
+      memset(C->mg_sol, 0, sizeof(double) * C->ndof);
+      restrict_defect(P, C, L, C->mg_rhs, L->mg_def);
+      for(int j = C->nactive; j < C->ndof; ++j) C->mg_rhs[j] = 0;
+      level = level - 1;
+    } else if(dirs[i] == MG_UP) {
+      mg_prolong_add(P, s, level + 1);
+      level = level + 1;
+    } else {
+      break;
+    }
   }
-  for(int l = 1; l <= Lf; ++l) {
-    olevel* L = &P->lev[l];
-    olevel* C = &P->lev[l - 1];
-    prolongate(P, C, L, C->mg_sol, L->mg_aux);
-    for(int j = 0; j < L->nactive; ++j) L->mg_sol[j] += s->mg_damp * L->mg_aux[j];
-    for(int k = 0; k < s->n_post; ++k) smooth_step(s, L, L->mg_rhs, L->mg_sol);
-  }
+  free(dirs);
   memcpy(z, F->mg_sol, sizeof(double) * F->ndof);
 }
 

@@ -28,6 +28,7 @@ EXAMPLES = {"sine_laplace_2d": 0, "sine_2d": 1, "sine_3d": 2, "test_p0": 3,
 METHODS = {"direct": 0, "cg": 1, "fgmres": 2, "richardson": 3, "sor": 4, "ssor": 5}
 PRECONDS = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2, "sor": 3, "ssor": 4}
 SMOOTHERS = {"jacobi": 0, "sor": 1, "ssor": 2}
+CYCLES = {"V": 0, "W": 1, "F": 2}
 
 
 class ProblemDesc(ctypes.Structure):
@@ -46,7 +47,8 @@ class SolverDesc(ctypes.Structure):
                 ("div_factor", ctypes.c_double), ("restart", ctypes.c_int),
                 ("n_pre", ctypes.c_int), ("n_post", ctypes.c_int),
                 ("jacobi_damp", ctypes.c_double), ("mg_damp", ctypes.c_double),
-                ("mixed", ctypes.c_int), ("smoother", ctypes.c_int), ("sor_omega", ctypes.c_double)]
+                ("mixed", ctypes.c_int), ("smoother", ctypes.c_int), ("sor_omega", ctypes.c_double),
+                ("cycle", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -59,7 +61,7 @@ EXPORTS = [
     "pmcd_last_error", "pmcd_device_count", "pmcd_mem_info", "pmcd_system_set_data",
     "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
     "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_sweep_order", "pmcd_host_sell_layout",
-    "pmcd_host_write_mesh",
+    "pmcd_host_write_mesh", "pmcd_cycle_directions",
     "pmcd_jacobi_sweep",
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
     "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
@@ -97,6 +99,16 @@ def get_vcycle_graphs():
     v = ctypes.c_int(0)
     _check(lib().pmcd_get_tuning(6, ctypes.byref(v)))
     return bool(v.value)
+
+
+def cycle_directions(cycle, n_levels):
+    """Directions of one multigrid cycle ('V', 'W', 'F'): 0 down, 1 up, 2 end,
+    3 the leading dummy (CycleControl)."""
+    n = ctypes.c_int(0)
+    out = np.empty(4096, dtype=np.int32)
+    _check(lib().pmcd_cycle_directions(CYCLES[cycle], int(n_levels), _ptr(out, ctypes.c_int), len(out),
+                                       ctypes.byref(n)))
+    return out[:min(n.value, len(out))].copy()
 
 
 def set_sell_unroll(u):
@@ -237,6 +249,7 @@ def solver_desc(db):
     s.mixed = int(db.get("multigrid_mixed_precision", 0))
     s.smoother = SMOOTHERS[db.get("multigrid_smoother", "jacobi")]
     s.sor_omega = float(db.get("sor_omega", 1.5))
+    s.cycle = CYCLES[db.get("multigrid_cycle_type", "V")]
     return s
 
 

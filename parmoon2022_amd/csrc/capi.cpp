@@ -79,7 +79,9 @@ SolverDesc to_solver(const pmcd_solver_desc& s) {
   d.mixed = s.mixed;
   d.smoother = s.smoother;
   d.sor_omega = s.sor_omega;
+  d.cycle = s.cycle;
   PMCD_CHECK(d.smoother >= 0 && d.smoother <= 2, "unknown multigrid smoother ", d.smoother);
+  PMCD_CHECK(d.cycle >= 0 && d.cycle <= 2, "unknown multigrid cycle ", d.cycle);
   return d;
 }
 }  // namespace
@@ -402,6 +404,15 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
   return guarded([&] {
     PMCD_CHECK(iters > 0, "iters must be positive");
     s->sys->time_kernel(which, level, iters, avg_ms);
+  });
+}
+
+int pmcd_cycle_directions(int type, int n_levels, int* out, int cap, int* n) {
+  return guarded([&] {
+    PMCD_CHECK(n && (out || cap == 0), "null argument");
+    const std::vector<int> d = cycle_directions(type, n_levels);
+    for(int i = 0; i < (int)d.size() && i < cap; ++i) out[i] = d[i];
+    *n = (int)d.size();
   });
 }
 

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <algorithm>
 #include <climits>
+#include <functional>
 
 namespace pmcd {
 
@@ -511,6 +512,36 @@ int CDSystem::smooth32(Level& L, float* const x[2], int cur, const float* b, int
 // prolongation and post-smoothing on the way up. The finest iterate lives in
 // `out` and one work buffer; coarser iterates start at 0 with a rhs that is 0
 // on the non-active rows.
+std::vector<int> cycle_directions(int type, int n_levels) {
+  PMCD_CHECK(type >= 0 && type <= 2, "unknown multigrid cycle ", type);
+  PMCD_CHECK(n_levels >= 1 && n_levels <= 16, "multigrid levels out of range");
+  std::vector<int> rec(n_levels, type == 0 ? 1 : 2);
+  rec[n_levels - 1] = 1;
+  std::vector<int> dirs{kMgDummy};
+  const int finest = n_levels - 1;
+  std::function<void(int)> fill = [&](int level) {
+    if(level == 0) {
+      dirs.push_back(kMgUp);
+      return;
+    }
+    for(int j = 0; j < rec[level]; ++j) {
+      dirs.push_back(kMgDown);
+      fill(level - 1);
+    }
+    if(type == 2) rec[level] = 1;
+    dirs.push_back(level == finest ? kMgEnd : kMgUp);
+  };
+  fill(finest);
+  return dirs;
+}
+
+// One multigrid cycle z = MG(v) (Multigrid::cycle / cycle_step,
+// Multigrid.C:157-308): the directions of cycle_directions() walked from the
+// finest level; going down pre-smooths, computes the defect, restarts the
+// coarser iterate at 0 and restricts; the coarsest level is the direct solve;
+// coming up prolongates with damping and post-smooths. Jacobi sweeps
+// ping-pong between two buffers per level; a level whose iterate is known to
+// be 0 takes its first Jacobi sweep without the SpMV.
 void CDSystem::vcycle64(const double* rhs, double* out) {
   const int Lf = n_levels() - 1;
   std::vector<const double*> b(n_levels());
@@ -531,34 +562,59 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
     launch_fill(F.n_active(), 0.0, x[Lf][0], stream_);
     launch_copy(F.n() - F.n_active(), rhs + F.n_active(), x[Lf][0] + F.n_active(), stream_);
   }
-  for(int l = Lf; l >= 1; --l) {
-    Level& L = *levels_[l];
-    Level& C = *levels_[l - 1];
-    const bool zero = (l == Lf) ? fine_zero : (jacobi && sd_.n_pre > 0);
-    cur[l] = smooth(L, x[l].data(), cur[l], b[l], sd_.n_pre, zero);
-    level_spmv(L, x[l][cur[l]], b[l], L.mg_def.p, -1.0, 1.0, stream_);
-    if(!jacobi || sd_.n_pre == 0 || l - 1 == 0) launch_fill(C.n(), 0.0, x[l - 1][0], stream_);
-    launch_restrict(C.n(), C.n_active(), (long)L.r_val.n, L.r_max_row, L.r_rp.p, L.r_col.p,
-                    L.r_val.p, L.mg_def.p, C.mg_rhs.p, stream_);
+  if(n_levels() == 1) {
+    cur[0] = smooth(*levels_[0], x[0].data(), cur[0], b[0], sd_.n_pre + sd_.n_post, false);
+    if(cur[0] != 0) launch_copy(finest().n(), x[0][cur[0]], out, stream_);
+    return;
   }
-  {
-    Level& C = *levels_[0];
-    if(n_levels() == 1) {
-      cur[0] = smooth(C, x[0].data(), cur[0], b[0], sd_.n_pre + sd_.n_post, false);
-    } else {
+  auto prolong = [&](int l) {  // damped addition of P x_{l-1} on level l
+    Level& L = *levels_[l];
+    launch_prolong_add(L.n(), L.n_active(), (long)L.p_val.n, L.p_max_row, L.p_rp.p, L.p_col.p, L.p_val.p,
+                       x[l - 1][cur[l - 1]], x[l][cur[l]], sd_.mg_damp, stream_);
+  };
+  std::vector<char> zero(n_levels(), 0);
+  zero[Lf] = fine_zero;
+  const std::vector<int> dirs = cycle_directions(sd_.cycle, n_levels());
+  int level = Lf;
+  for(size_t i = 1; i < dirs.size(); ++i) {
+    if(level == 0) {
+      Level& C = *levels_[0];
       launch_gemv(C.n(), coarse_inv_.p, b[0], x[0][0], stream_);
       cur[0] = 0;
+      prolong(1);
+      zero[1] = 0;
+      level = 1;
+      continue;
     }
-  }
-  for(int l = 1; l <= Lf; ++l) {
-    Level& L = *levels_[l];
-    launch_prolong_add(L.n(), L.n_active(), (long)L.p_val.n, L.p_max_row, L.p_rp.p, L.p_col.p,
-                       L.p_val.p, x[l - 1][cur[l - 1]], x[l][cur[l]], sd_.mg_damp, stream_);
-    cur[l] = smooth(L, x[l].data(), cur[l], b[l], sd_.n_post, false);
+    Level& L = *levels_[level];
+    if(dirs[i - 1] == kMgUp) cur[level] = smooth(L, x[level].data(), cur[level], b[level], sd_.n_post, false);
+    if(dirs[i] == kMgDown) {
+      Level& C = *levels_[level - 1];
+      cur[level] = smooth(L, x[level].data(), cur[level], b[level], sd_.n_pre, zero[level] != 0);
+      zero[level] = 0;
+      level_spmv(L, x[level][cur[level]], b[level], L.mg_def.p, -1.0, 1.0, stream_);
+      // the coarser iterate restarts at 0; a Jacobi sweep from zero does not read it
+      const bool from_zero = jacobi && sd_.n_pre > 0;
+      if(!from_zero || level - 1 == 0) {
+        launch_fill(C.n(), 0.0, x[level - 1][0], stream_);
+        cur[level - 1] = 0;
+      }
+      zero[level - 1] = from_zero;
+      launch_restrict(C.n(), C.n_active(), (long)L.r_val.n, L.r_max_row, L.r_rp.p, L.r_col.p, L.r_val.p,
+                      L.mg_def.p, C.mg_rhs.p, stream_);
+      --level;
+    } else if(dirs[i] == kMgUp) {
+      prolong(level + 1);
+      zero[level + 1] = 0;
+      ++level;
+    } else {
+      break;
+    }
   }
   if(cur[Lf] != 0) launch_copy(finest().n(), x[Lf][cur[Lf]], out, stream_);
 }
 
+// The same cycle on the FP32 copies (C3): the coarse direct solve stays FP64.
 void CDSystem::vcycle32(const double* rhs, double* out) {
   const int Lf = n_levels() - 1;
   for(auto& Lp : levels_)
@@ -575,34 +631,56 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
     PMCD_HIP(hipMemcpyAsync(F.f_sol.p + F.n_active(), F.f_rhs.p + F.n_active(),
                             sizeof(float) * (F.n() - F.n_active()), hipMemcpyDeviceToDevice, stream_));
   }
-  for(int l = Lf; l >= 1; --l) {
-    Level& L = *levels_[l];
-    Level& C = *levels_[l - 1];
-    const bool zero = (l == Lf) ? fine_zero : (jacobi && sd_.n_pre > 0);
-    cur[l] = smooth32(L, x[l].data(), cur[l], L.f_rhs.p, sd_.n_pre, zero);
-    level_spmv32(L, x[l][cur[l]], L.f_rhs.p, L.f_def.p, -1.0f, 1.0f, stream_);
-    if(!jacobi || sd_.n_pre == 0 || l - 1 == 0) launch_fill_f32(C.n(), 0.0f, x[l - 1][0], stream_);
-    launch_restrict_f32(C.n(), C.n_active(), (long)L.r_val32.n, L.r_max_row, L.r_rp.p, L.r_col.p,
-                        L.r_val32.p, L.f_def.p, C.f_rhs.p, stream_);
-  }
-  {
+  if(n_levels() == 1) {
     Level& C = *levels_[0];
-    if(n_levels() == 1) {
-      cur[0] = smooth32(C, x[0].data(), cur[0], C.f_rhs.p, sd_.n_pre + sd_.n_post, false);
-    } else {
-      // coarse direct solve in FP64
+    cur[0] = smooth32(C, x[0].data(), cur[0], C.f_rhs.p, sd_.n_pre + sd_.n_post, false);
+    launch_f2d(F.n(), x[Lf][cur[Lf]], out, stream_);
+    return;
+  }
+  auto prolong = [&](int l) {
+    Level& L = *levels_[l];
+    launch_prolong_add_f32(L.n(), L.n_active(), (long)L.p_val32.n, L.p_max_row, L.p_rp.p, L.p_col.p,
+                           L.p_val32.p, x[l - 1][cur[l - 1]], x[l][cur[l]], (float)sd_.mg_damp, stream_);
+  };
+  std::vector<char> zero(n_levels(), 0);
+  zero[Lf] = fine_zero;
+  const std::vector<int> dirs = cycle_directions(sd_.cycle, n_levels());
+  int level = Lf;
+  for(size_t i = 1; i < dirs.size(); ++i) {
+    if(level == 0) {
+      Level& C = *levels_[0];
       launch_f2d(C.n(), C.f_rhs.p, coarse_tmp_.p, stream_);
       launch_gemv(C.n(), coarse_inv_.p, coarse_tmp_.p, coarse_tmp_.p + C.n(), stream_);
       launch_d2f(C.n(), coarse_tmp_.p + C.n(), x[0][0], stream_);
       cur[0] = 0;
+      prolong(1);
+      zero[1] = 0;
+      level = 1;
+      continue;
     }
-  }
-  for(int l = 1; l <= Lf; ++l) {
-    Level& L = *levels_[l];
-    launch_prolong_add_f32(L.n(), L.n_active(), (long)L.p_val32.n, L.p_max_row, L.p_rp.p,
-                           L.p_col.p, L.p_val32.p, x[l - 1][cur[l - 1]], x[l][cur[l]],
-                           (float)sd_.mg_damp, stream_);
-    cur[l] = smooth32(L, x[l].data(), cur[l], L.f_rhs.p, sd_.n_post, false);
+    Level& L = *levels_[level];
+    if(dirs[i - 1] == kMgUp) cur[level] = smooth32(L, x[level].data(), cur[level], L.f_rhs.p, sd_.n_post, false);
+    if(dirs[i] == kMgDown) {
+      Level& C = *levels_[level - 1];
+      cur[level] = smooth32(L, x[level].data(), cur[level], L.f_rhs.p, sd_.n_pre, zero[level] != 0);
+      zero[level] = 0;
+      level_spmv32(L, x[level][cur[level]], L.f_rhs.p, L.f_def.p, -1.0f, 1.0f, stream_);
+      const bool from_zero = jacobi && sd_.n_pre > 0;
+      if(!from_zero || level - 1 == 0) {
+        launch_fill_f32(C.n(), 0.0f, x[level - 1][0], stream_);
+        cur[level - 1] = 0;
+      }
+      zero[level - 1] = from_zero;
+      launch_restrict_f32(C.n(), C.n_active(), (long)L.r_val32.n, L.r_max_row, L.r_rp.p, L.r_col.p,
+                          L.r_val32.p, L.f_def.p, C.f_rhs.p, stream_);
+      --level;
+    } else if(dirs[i] == kMgUp) {
+      prolong(level + 1);
+      zero[level + 1] = 0;
+      ++level;
+    } else {
+      break;
+    }
   }
   launch_f2d(F.n(), x[Lf][cur[Lf]], out, stream_);
 }
@@ -625,7 +703,7 @@ void CDSystem::vcycle(const double* rhs, double* out) {
   auto same = [](const SolverDesc& a, const SolverDesc& b) {
     return a.smoother == b.smoother && a.n_pre == b.n_pre && a.n_post == b.n_post &&
            a.jacobi_damp == b.jacobi_damp && a.mg_damp == b.mg_damp && a.sor_omega == b.sor_omega &&
-           a.mixed == b.mixed;
+           a.mixed == b.mixed && a.cycle == b.cycle;
   };
   if(graph_epoch_ != tuning_epoch() || !same(graph_sd_, sd_) || vgraphs_.size() > 128) {
     clear_graphs();

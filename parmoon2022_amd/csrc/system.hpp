@@ -49,6 +49,7 @@ struct SolverDesc {
   int mixed;           // 1: V-cycle in FP32
   int smoother;        // multigrid smoother: 0 jacobi, 1 sor (forward), 2 ssor
   double sor_omega;    // sor_omega
+  int cycle;           // multigrid_cycle_type: 0 V, 1 W, 2 F
 };
 
 struct SolveStats {
@@ -131,6 +132,15 @@ void prepare_level_sell(Level& L, bool f32, hipStream_t stream);
 void build_level_sell(Level& L, hipStream_t stream);
 // whether the SpMV or Jacobi kernels of the level run on the sliced ELL copy
 bool level_uses_sell(const Level& L);
+// Directions of one multigrid cycle as CycleControl builds them
+// (src/Multigrid/CycleControl.C:17-111): a dummy first entry, then for each
+// visit of a level with recursion count r, r times "down" followed by the
+// coarser level's directions, then "up" ("end" on the finest level); the
+// coarsest level contributes one "up". Recursion counts: 1 (V) or 2 (W, F)
+// on every level but the finest (1); an F cycle resets a level's count to 1
+// after its first visit. type: 0 V, 1 W, 2 F.
+enum { kMgDown = 0, kMgUp = 1, kMgEnd = 2, kMgDummy = 3 };
+std::vector<int> cycle_directions(int type, int n_levels);
 // physical position of the nodal point of local DOF i of cell c
 void node_position(const Mesh& m, int c, const ElementInfo& info, int i, double* x);
 double now_seconds();
@@ -158,7 +168,7 @@ class CDSystem {
   void apply(Level& L, const double* x, double* y);                 // y = A x
   void apply_scaled_add(Level& L, const double* x, double* y, double a);  // y += a A x
   double dot(int n, const double* x, const double* y);             // synchronous
-  void vcycle(const double* rhs, double* out);                     // FP64 or FP32 V-cycle
+  void vcycle(const double* rhs, double* out);                     // one V, W or F cycle, FP64 or FP32
   void set_solver(const SolverDesc& s) { sd_ = s; }
   void prepare(const SolverDesc& s);      // coarse factorization, FP32 copies
   void time_kernel(int which, int level, int iters, double* avg_ms);

@@ -797,3 +797,79 @@ def test_mesh_file_solve_equals_generated_mesh(tmp_path):
         cd.close()
     (va, ia, ua, ea), (vb, ib, ub, eb) = out
     assert np.array_equal(va, vb) and ia == ib and np.array_equal(ua, ub) and np.array_equal(ea, eb)
+
+
+@pytest.mark.parametrize("cycle,mixed", [("W", 0), ("F", 0), ("W", 1)])
+def test_w_f_cycles_match_oracle(oracle_lib, cycle, mixed):
+    """FGMRES with W and F multigrid cycles (CycleControl) on the C2 shape at
+    8^3 with 3 levels: iterations within one of the oracle's and the solution
+    within 1e-10 (FP64 cycle) or 1e-7 (FP32 cycle) of it."""
+    db = dict(c2_db(3, 3, mixed=mixed), multigrid_cycle_type=cycle)
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    st = cd.solve()
+    u = cd.solution()
+    cd.close()
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    so = o.solve(pm.solver_desc(db))
+    rel = np.linalg.norm(u - o.solution()) / np.linalg.norm(o.solution())
+    print("%s cycle (mixed %d): iterations gpu %d oracle %d, rel %.3e" % (cycle, mixed, st.iterations,
+                                                                        so.iterations, rel), flush=True)
+    assert st.converged and abs(st.iterations - so.iterations) <= 1
+    assert rel < (1e-7 if mixed else 1e-10)
+
+
+@pytest.mark.parametrize("geo", ["Default_UnitCube_Hexa", "Default_UnitCube_Tetra"])
+def test_cd3d_multigrid_solution_in_space_gpu(geo):
+    """test/cd3d_solvers_test.c++ "multigrid" settings on the GPU path
+    (Richardson + W-cycle, 2 levels, Jacobi 3 + 3, damping 0.8): zero error
+    for the quadratic solution with Q2/P2 (1e-12)."""
+    db = {"geo_file": geo, "ANSATZ_ORDER": 2, "refinement_n_initial_steps": 2, "example": "test_p2",
+          "solver_type": "iterative", "iterative_solver_type": "richardson", "preconditioner": "multigrid",
+          "multigrid_n_levels": 2, "multigrid_cycle_type": "W", "multigrid_smoother": "jacobi",
+          "jacobi_damp": 1.0, "multigrid_correction_damp_factor": 0.8, "multigrid_n_pre_smooth": 3,
+          "multigrid_n_post_smooth": 3, "residual_tolerance": 1e-15, "residual_reduction": 0.0,
+          "max_n_iterations": 100}
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    cd.solve()
+    e = cd.output()
+    cd.close()
+    assert e[0] < 1e-12 and e[1] < 1e-12, e
+
+
+@pytest.mark.parametrize("cycle", ["V", "W", "F"])
+@pytest.mark.parametrize("smoother", ["jacobi", "ssor"])
+def test_multigrid_cycle_matches_oracle(oracle_lib, cycle, smoother):
+    """One multigrid cycle (pmcd_vcycle) with V, W and F control on 4 levels
+    (2D Q2, where the cycles differ: W visits the coarsest level 4 times, F 3
+    times) against the oracle's cycle: relative difference below 1e-12. The
+    SSOR smoother runs in the exact class order (high class limit)."""
+    old = pm.get_sor_exact_limit()
+    pm.set_sor_exact_limit(100000)
+    try:
+        db = {"geo_file": "UnitSquare", "ANSATZ_ORDER": 2, "refinement_n_initial_steps": 5, "example": "sine_2d",
+              "diffusion_coefficient": 0.01, "convection": (0.6, 0.8, 0.0), "space_discretization_type": "supg",
+              "solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "multigrid",
+              "multigrid_n_levels": 4, "multigrid_smoother": smoother, "sor_omega": 1.1, "jacobi_damp": 0.6,
+              "multigrid_n_pre_smooth": 2, "multigrid_n_post_smooth": 1, "multigrid_cycle_type": cycle}
+        cd = pm.ConvectionDiffusion(db)
+        cd.assemble()
+        o = oracle_lib.Oracle(pm.problem_desc(db))
+        o.assemble()
+        i = cd.info()
+        v = np.random.default_rng(5).standard_normal(i["n_dof"])
+        v[i["n_active"]:] = 0
+        zg = cd.vcycle(v)
+        zo = o.vcycle(pm.solver_desc(db), v)
+        zv = o.vcycle(pm.solver_desc(dict(db, multigrid_cycle_type="V")), v)
+        rel = np.linalg.norm(zg - zo) / np.linalg.norm(zo)
+        dv = np.linalg.norm(zo - zv) / np.linalg.norm(zv)
+        print("%s cycle, %s: rel %.3e (differs from V by %.3e)" % (cycle, smoother, rel, dv), flush=True)
+        assert rel < 1e-12
+        if cycle != "V":
+            assert dv > 1e-6
+        cd.close()
+    finally:
+        pm.set_sor_exact_limit(old)

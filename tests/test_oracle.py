@@ -220,3 +220,81 @@ def test_oracle_cd2d_golden_fgmres_ssor(oracle_lib, geo, order):
     g = CD2D_GOLDEN[(geo, order)]
     assert st.converged
     assert np.allclose([e[0], e[1], e[2], e[4]], g, rtol=0, atol=1e-11)
+
+
+def _walk_coarse_visits(dirs, n_levels):
+    """Consume a direction list as Multigrid::cycle does and count the visits
+    of the coarsest level."""
+    level, visits = n_levels - 1, 0
+    for i in range(1, len(dirs)):
+        if level == 0:
+            visits += 1
+            level = 1
+        elif dirs[i] == 0:
+            level -= 1
+        elif dirs[i] == 1:
+            level += 1
+        else:
+            break
+    return visits, level
+
+
+@pytest.mark.parametrize("cycle", ["V", "W", "F"])
+def test_cycle_directions(oracle_lib, cycle):
+    """CycleControl directions (CycleControl.C:17-111) of the library and the
+    oracle agree for 2..6 levels and have the known shape: as many downs as
+    ups, ending on the finest level, and 1 (V), 2^(L-2) (W) or L-1 (F) visits
+    of the coarsest level."""
+    import ctypes
+    for L in range(2, 7):
+        d = pm.cycle_directions(cycle, L)
+        out = np.empty(4096, dtype=np.int32)
+        n = oracle_lib.lib().orc_cycle_dirs(pm.CYCLES[cycle], L, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                            len(out))
+        assert np.array_equal(d, out[:n])
+        assert d[0] == 3 and d[-1] == 2
+        assert np.sum(d[1:] == 0) == np.sum(d[1:] == 1)
+        visits, last = _walk_coarse_visits(d, L)
+        assert last == L - 1
+        assert visits == {"V": 1, "W": 2 ** (L - 2), "F": L - 1}[cycle], (cycle, L, visits)
+
+
+@pytest.mark.parametrize("geo", ["Default_UnitCube_Hexa", "Default_UnitCube_Tetra"])
+@pytest.mark.parametrize("example,order", [("test_p0", 1), ("test_p1", 1), ("test_p2", 2)])
+def test_oracle_cd3d_multigrid_solution_in_space(oracle_lib, geo, example, order):
+    """test/cd3d_solvers_test.c++ "multigrid": Richardson + W-cycle multigrid
+    (2 levels, Jacobi, 3 + 3 smoothing steps, correction damping 0.8) to an
+    absolute residual of 1e-15; L2 and H1 errors vanish (1e-12)."""
+    db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 2, "example": example,
+          "solver_type": "iterative", "iterative_solver_type": "richardson", "preconditioner": "multigrid",
+          "multigrid_n_levels": 2, "multigrid_cycle_type": "W", "multigrid_smoother": "jacobi",
+          "jacobi_damp": 1.0, "multigrid_correction_damp_factor": 0.8, "multigrid_n_pre_smooth": 3,
+          "multigrid_n_post_smooth": 3, "residual_tolerance": 1e-15, "residual_reduction": 0.0,
+          "max_n_iterations": 100}
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    o.solve(pm.solver_desc(db))
+    e = o.errors()
+    assert e[0] < 1e-12 and e[1] < 1e-12, e
+
+
+@pytest.mark.parametrize("cycle", ["W", "F"])
+def test_oracle_w_f_cycles_converge_to_the_v_cycle_solution(oracle_lib, cycle):
+    """FGMRES with W and F cycles on a 3-level SUPG problem reaches the same
+    discrete solution as with V cycles (1e-9) in no more iterations."""
+    base = {"geo_file": "Default_UnitCube_Hexa", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 3,
+            "example": "sine_3d", "space_discretization_type": "supg", "diffusion_coefficient": 1e-2,
+            "convection": (0.6, 0.48, 0.64), "solver_type": "iterative", "iterative_solver_type": "fgmres",
+            "preconditioner": "multigrid", "multigrid_n_levels": 3, "jacobi_damp": 0.6,
+            "multigrid_n_pre_smooth": 2, "multigrid_n_post_smooth": 2, "residual_reduction": 1e-12,
+            "residual_tolerance": 0.0, "max_n_iterations": 200}
+    out = []
+    for cyc in ("V", cycle):
+        db = dict(base, multigrid_cycle_type=cyc)
+        o = oracle_lib.Oracle(pm.problem_desc(db))
+        o.assemble()
+        st = o.solve(pm.solver_desc(db))
+        out.append((st.iterations, o.solution()))
+    (iv, uv), (ic, uc) = out
+    assert ic <= iv
+    assert np.linalg.norm(uc - uv) <= 1e-9 * np.linalg.norm(uv)

@@ -72,6 +72,10 @@ enum { PMCD_NO_PRECOND = 0, PMCD_JACOBI = 1, PMCD_MULTIGRID = 2, PMCD_SOR = 3, P
 enum { PMCD_SMOOTH_JACOBI = 0, PMCD_SMOOTH_SOR = 1, PMCD_SMOOTH_SSOR = 2 };
 /* multigrid_cycle_type */
 enum { PMCD_CYCLE_V = 0, PMCD_CYCLE_W = 1, PMCD_CYCLE_F = 2 };
+/* multigrid_smoother_coarse: the direct solve, or the coarse smoother
+ * repeated while the coarse defect exceeds coarse_residual (Multigrid.C:
+ * 200-240) */
+enum { PMCD_COARSE_DIRECT = 0, PMCD_COARSE_JACOBI = 1, PMCD_COARSE_SOR = 2, PMCD_COARSE_SSOR = 3 };
 
 typedef struct {
   int method;          /* solver_type / iterative_solver_type */
@@ -89,6 +93,9 @@ typedef struct {
   int smoother;        /* multigrid_smoother: PMCD_SMOOTH_* */
   double sor_omega;    /* sor_omega (reference default 1.5, Solver.C:154-156) */
   int cycle;           /* multigrid_cycle_type: PMCD_CYCLE_V, _W, _F (CycleControl.C:17-111) */
+  int coarse_smoother; /* multigrid_smoother_coarse: PMCD_COARSE_* (0, the default, direct solve) */
+  double coarse_residual; /* multigrid_coarse_residual (reference default 0.1, absolute) */
+  int coarse_max_iter; /* multigrid_coarse_max_n_iterations (reference default 10) */
 } pmcd_solver_desc;
 
 typedef struct {

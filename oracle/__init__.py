@@ -24,7 +24,8 @@ class OSolver(ctypes.Structure):
                 ("n_pre", ctypes.c_int), ("n_post", ctypes.c_int),
                 ("jacobi_damp", ctypes.c_double), ("mg_damp", ctypes.c_double),
                 ("mixed", ctypes.c_int), ("smoother", ctypes.c_int), ("sor_omega", ctypes.c_double),
-                ("cycle", ctypes.c_int)]
+                ("cycle", ctypes.c_int), ("coarse_smoother", ctypes.c_int),
+                ("coarse_residual", ctypes.c_double), ("coarse_max_iter", ctypes.c_int)]
 
 
 class OStats(ctypes.Structure):

@@ -1445,6 +1445,9 @@ typedef struct {
   int smoother; /* multigrid_smoother: 0 jacobi, 1 sor, 2 ssor */
   double sor_omega;
   int cycle; /* multigrid_cycle_type: 0 V, 1 W, 2 F */
+  int coarse_smoother; /* multigrid_smoother_coarse: 0 direct solve, 1 jacobi, 2 sor, 3 ssor */
+  double coarse_residual;
+  int coarse_max_iter;
 } osolver;
 
 typedef struct {
@@ -1574,6 +1577,22 @@ static void mg_prolong_add(orc_problem* P, const osolver* s, int l) {
   for(int j = 0; j < L->nactive; ++j) L->mg_sol[j] += s->mg_damp * L->mg_aux[j];
 }
 
+/* the coarsest level's loop with an iterative coarse smoother (Multigrid.C:
+   200-240): the residual starts at 1e10, so at least one step runs; then
+   smooth and recompute the defect norm until it is at most coarse_residual
+   or coarse_max_iter steps are done */
+static void coarse_iterate(const osolver* s, olevel* C) {
+  osolver cs = *s;
+  cs.smoother = s->coarse_smoother - 1;
+  double res = 1e10;
+  for(int i = 0; i < s->coarse_max_iter && res > s->coarse_residual; ++i) {
+    smooth_step(&cs, C, C->mg_rhs, C->mg_sol);
+    memcpy(C->mg_def, C->mg_rhs, sizeof(double) * C->ndof);
+    apply_scaled_add(C, C->mg_sol, C->mg_def, -1.0);
+    res = sqrt(ddot(C->ndof, C->mg_def, C->mg_def));
+  }
+}
+
 /* Multigrid::cycle / cycle_step (Multigrid.C:157-308) with V, W or F control */
 static void mg_cycle(orc_problem* P, const osolver* s, const double* v, double* z) {
   const int Lf = P->nlev - 1;
@@ -1591,9 +1610,10 @@ static void mg_cycle(orc_problem* P, const osolver* s, const double* v, double* 
   int level = Lf;
   for(int i = 1; i < nd; ++i) {
     if(level == 0) {
-      /* coarsest level: the direct solve, then up */
+      /* coarsest level: the direct solve or the coarse smoother loop, then up */
       olevel* C = &P->lev[0];
-      dense_solve(C, C->mg_rhs, C->mg_sol);
+      if(s->coarse_smoother == 0) dense_solve(C, C->mg_rhs, C->mg_sol);
+      else coarse_iterate(s, C);
       mg_prolong_add(P, s, 1);
       level = 1;
       continue;

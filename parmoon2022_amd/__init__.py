@@ -29,6 +29,7 @@ METHODS = {"direct": 0, "cg": 1, "fgmres": 2, "richardson": 3, "sor": 4, "ssor":
 PRECONDS = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2, "sor": 3, "ssor": 4}
 SMOOTHERS = {"jacobi": 0, "sor": 1, "ssor": 2}
 CYCLES = {"V": 0, "W": 1, "F": 2}
+COARSE_SMOOTHERS = {"direct_solve": 0, "jacobi": 1, "sor": 2, "ssor": 3}
 
 
 class ProblemDesc(ctypes.Structure):
@@ -48,7 +49,8 @@ class SolverDesc(ctypes.Structure):
                 ("n_pre", ctypes.c_int), ("n_post", ctypes.c_int),
                 ("jacobi_damp", ctypes.c_double), ("mg_damp", ctypes.c_double),
                 ("mixed", ctypes.c_int), ("smoother", ctypes.c_int), ("sor_omega", ctypes.c_double),
-                ("cycle", ctypes.c_int)]
+                ("cycle", ctypes.c_int), ("coarse_smoother", ctypes.c_int),
+                ("coarse_residual", ctypes.c_double), ("coarse_max_iter", ctypes.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -250,6 +252,9 @@ def solver_desc(db):
     s.smoother = SMOOTHERS[db.get("multigrid_smoother", "jacobi")]
     s.sor_omega = float(db.get("sor_omega", 1.5))
     s.cycle = CYCLES[db.get("multigrid_cycle_type", "V")]
+    s.coarse_smoother = COARSE_SMOOTHERS[db.get("multigrid_smoother_coarse", "direct_solve")]
+    s.coarse_residual = float(db.get("multigrid_coarse_residual", 0.1))
+    s.coarse_max_iter = int(db.get("multigrid_coarse_max_n_iterations", 10))
     return s
 
 

@@ -80,6 +80,10 @@ SolverDesc to_solver(const pmcd_solver_desc& s) {
   d.smoother = s.smoother;
   d.sor_omega = s.sor_omega;
   d.cycle = s.cycle;
+  d.coarse_smoother = s.coarse_smoother;
+  d.coarse_residual = s.coarse_residual;
+  d.coarse_max_iter = s.coarse_max_iter;
+  PMCD_CHECK(d.coarse_smoother >= 0 && d.coarse_smoother <= 3, "unknown coarse smoother ", d.coarse_smoother);
   PMCD_CHECK(d.smoother >= 0 && d.smoother <= 2, "unknown multigrid smoother ", d.smoother);
   PMCD_CHECK(d.cycle >= 0 && d.cycle <= 2, "unknown multigrid cycle ", d.cycle);
   return d;

@@ -68,10 +68,11 @@ void givens_apply(double& dx, double& dy, double cs, double sn) {
 void CDSystem::prepare(const SolverDesc& s) {
   sd_ = s;
   rhs_nonactive_zero_ = false;
-  if((s.precond == 2 && n_levels() > 1) || s.method == 0) {
+  if((s.precond == 2 && n_levels() > 1 && s.coarse_smoother == 0) || s.method == 0) {
     if(s.method == 0 && n_levels() > 1) PMCD_FAIL("the direct solver expects a single level");
     if(!coarse_ready_) coarse_factorize();
   }
+  if(s.precond == 2 && n_levels() > 1 && s.coarse_smoother >= 2) prepare_sweep(*levels_.front());
   if(s.precond == 2 && s.smoother != 0)
     for(auto& Lp : levels_) prepare_sweep(*Lp);
   if(s.precond == 3 || s.precond == 4 || s.method == 4 || s.method == 5) prepare_sweep(finest());

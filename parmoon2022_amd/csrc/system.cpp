@@ -472,10 +472,11 @@ double CDSystem::dot(int n, const double* x, const double* y) {
 // index of the buffer holding the result. With `from_zero` the iterate is 0
 // and b vanishes on the non-active rows, so the first sweep needs no SpMV.
 int CDSystem::smooth(Level& L, double* const x[2], int cur, const double* b, int steps,
-                     bool from_zero) {
-  if(sd_.smoother != 0) {
+                     bool from_zero, int kind) {
+  if(kind < 0) kind = sd_.smoother;
+  if(kind != 0) {
     // SORSmoother::smooth (SORSmoother.C:29-33): one sweep (pair) per step, in place
-    for(int s = 0; s < steps; ++s) sor_sweep(L, b, x[cur], sd_.sor_omega, sd_.smoother == 1 ? 0 : 2);
+    for(int s = 0; s < steps; ++s) sor_sweep(L, b, x[cur], sd_.sor_omega, kind == 1 ? 0 : 2);
     return cur;
   }
   for(int s = 0; s < steps; ++s) {
@@ -535,6 +536,22 @@ std::vector<int> cycle_directions(int type, int n_levels) {
   return dirs;
 }
 
+// The coarsest level's loop with an iterative coarse smoother (Multigrid.C:
+// 200-240): the residual starts at 1e10, so at least one step runs; then
+// smooth and recompute |b - A x| until it is at most coarse_residual or
+// coarse_max_iter steps are done. x[cur] holds the start iterate (0).
+int CDSystem::coarse_iterate(const double* b, double* const x[2], int cur) {
+  Level& C = *levels_[0];
+  const int kind = sd_.coarse_smoother - 1;
+  double res = 1e10;
+  for(int i = 0; i < sd_.coarse_max_iter && res > sd_.coarse_residual; ++i) {
+    cur = smooth(C, x, cur, b, 1, false, kind);
+    level_spmv(C, x[cur], b, C.mg_def.p, -1.0, 1.0, stream_);
+    res = std::sqrt(dot(C.n(), C.mg_def.p, C.mg_def.p));
+  }
+  return cur;
+}
+
 // One multigrid cycle z = MG(v) (Multigrid::cycle / cycle_step,
 // Multigrid.C:157-308): the directions of cycle_directions() walked from the
 // finest level; going down pre-smooths, computes the defect, restarts the
@@ -579,8 +596,12 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
   for(size_t i = 1; i < dirs.size(); ++i) {
     if(level == 0) {
       Level& C = *levels_[0];
-      launch_gemv(C.n(), coarse_inv_.p, b[0], x[0][0], stream_);
-      cur[0] = 0;
+      if(sd_.coarse_smoother == 0) {
+        launch_gemv(C.n(), coarse_inv_.p, b[0], x[0][0], stream_);
+        cur[0] = 0;
+      } else {
+        cur[0] = coarse_iterate(b[0], x[0].data(), cur[0]);
+      }
       prolong(1);
       zero[1] = 0;
       level = 1;
@@ -649,9 +670,18 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
   for(size_t i = 1; i < dirs.size(); ++i) {
     if(level == 0) {
       Level& C = *levels_[0];
-      launch_f2d(C.n(), C.f_rhs.p, coarse_tmp_.p, stream_);
-      launch_gemv(C.n(), coarse_inv_.p, coarse_tmp_.p, coarse_tmp_.p + C.n(), stream_);
-      launch_d2f(C.n(), coarse_tmp_.p + C.n(), x[0][0], stream_);
+      if(sd_.coarse_smoother == 0) {
+        launch_f2d(C.n(), C.f_rhs.p, coarse_tmp_.p, stream_);
+        launch_gemv(C.n(), coarse_inv_.p, coarse_tmp_.p, coarse_tmp_.p + C.n(), stream_);
+        launch_d2f(C.n(), coarse_tmp_.p + C.n(), x[0][0], stream_);
+      } else {
+        // the coarse loop runs in FP64 on the level's FP64 work vectors
+        launch_f2d(C.n(), C.f_rhs.p, C.mg_rhs.p, stream_);
+        launch_fill(C.n(), 0.0, C.mg_sol.p, stream_);
+        double* const xc[2] = {C.mg_sol.p, C.mg_sol2.p};
+        const int c = coarse_iterate(C.mg_rhs.p, xc, 0);
+        launch_d2f(C.n(), xc[c], x[0][0], stream_);
+      }
       cur[0] = 0;
       prolong(1);
       zero[1] = 0;
@@ -696,7 +726,8 @@ void CDSystem::vcycle_eager(const double* rhs, double* out) {
 // launch shapes change; matrix values are read from the same buffers, so
 // reassembly keeps them valid. The sequence holds no memset nodes.
 void CDSystem::vcycle(const double* rhs, double* out) {
-  if(!vcycle_graphs()) {
+  // (a coarse smoother loop reads its residual on the host: not capturable)
+  if(!vcycle_graphs() || sd_.coarse_smoother != 0) {
     vcycle_eager(rhs, out);
     return;
   }

@@ -50,6 +50,9 @@ struct SolverDesc {
   int smoother;        // multigrid smoother: 0 jacobi, 1 sor (forward), 2 ssor
   double sor_omega;    // sor_omega
   int cycle;           // multigrid_cycle_type: 0 V, 1 W, 2 F
+  int coarse_smoother; // multigrid_smoother_coarse: 0 direct solve, 1 jacobi, 2 sor, 3 ssor
+  double coarse_residual;  // multigrid_coarse_residual
+  int coarse_max_iter;     // multigrid_coarse_max_n_iterations
 };
 
 struct SolveStats {
@@ -185,7 +188,10 @@ class CDSystem {
   void build_transfer(Level& fine, const Level& coarse);
   void assemble_level(Level& L);
   void coarse_factorize();
-  int smooth(Level& L, double* const x[2], int cur, const double* b, int steps, bool from_zero);
+  // `kind`: 0 Jacobi, 1 SOR, 2 SSOR; -1 the solver's multigrid smoother
+  int smooth(Level& L, double* const x[2], int cur, const double* b, int steps, bool from_zero, int kind = -1);
+  // the coarsest level's iterative loop (coarse smoother instead of the direct solve)
+  int coarse_iterate(const double* b, double* const x[2], int cur);
   int smooth32(Level& L, float* const x[2], int cur, const float* b, int steps, bool from_zero);
   bool tail_is_zero(const double* v);  // v vanishes on the finest non-active rows
   void vcycle64(const double* rhs, double* out);

@@ -873,3 +873,43 @@ def test_multigrid_cycle_matches_oracle(oracle_lib, cycle, smoother):
         cd.close()
     finally:
         pm.set_sor_exact_limit(old)
+
+
+@pytest.mark.parametrize("coarse", ["jacobi", "ssor"])
+@pytest.mark.parametrize("cycle", ["V", "W"])
+def test_coarse_smoother_loop_matches_oracle(oracle_lib, coarse, cycle):
+    """Iterative coarse smoother instead of the direct solve (Multigrid.C:
+    200-240: at least one step, then repeated while the coarse defect norm
+    exceeds multigrid_coarse_residual, at most
+    multigrid_coarse_max_n_iterations times): one cycle against the oracle's
+    (1e-12), and FGMRES iteration counts within one."""
+    old = pm.get_sor_exact_limit()
+    pm.set_sor_exact_limit(100000)
+    try:
+        db = {"geo_file": "UnitSquare", "ANSATZ_ORDER": 2, "refinement_n_initial_steps": 5, "example": "sine_2d",
+              "diffusion_coefficient": 0.01, "convection": (0.6, 0.8, 0.0), "space_discretization_type": "supg",
+              "solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "multigrid",
+              "multigrid_n_levels": 4, "multigrid_smoother": "jacobi", "jacobi_damp": 0.6, "sor_omega": 1.1,
+              "multigrid_n_pre_smooth": 2, "multigrid_n_post_smooth": 1, "multigrid_cycle_type": cycle,
+              "multigrid_smoother_coarse": coarse, "multigrid_coarse_residual": 1e-3,
+              "multigrid_coarse_max_n_iterations": 6, "residual_reduction": 1e-10, "residual_tolerance": 0.0,
+              "max_n_iterations": 300}
+        cd = pm.ConvectionDiffusion(db)
+        cd.assemble()
+        o = oracle_lib.Oracle(pm.problem_desc(db))
+        o.assemble()
+        i = cd.info()
+        v = np.random.default_rng(8).standard_normal(i["n_dof"])
+        v[i["n_active"]:] = 0
+        zg = cd.vcycle(v)
+        zo = o.vcycle(pm.solver_desc(db), v)
+        rel = np.linalg.norm(zg - zo) / np.linalg.norm(zo)
+        st = cd.solve()
+        so = o.solve(pm.solver_desc(db))
+        print("coarse %s, %s cycle: rel %.3e, iterations gpu %d oracle %d" % (coarse, cycle, rel, st.iterations,
+                                                                            so.iterations), flush=True)
+        assert rel < 1e-12
+        assert st.converged and abs(st.iterations - so.iterations) <= 1
+        cd.close()
+    finally:
+        pm.set_sor_exact_limit(old)

@@ -270,6 +270,20 @@ int pmcd_spmv_device(int n, int n_active, const int* row_ptr, const int* col, co
 /* Ddot (Blas1.C:21-38): deterministic two-pass reduction, result to host. */
 int pmcd_dot_device(int n, const double* x, const double* y, double* result, void* stream);
 
+/* Files in the reference's formats (host arrays; no GPU use):
+ * pmcd_vector_write / pmcd_vector_read = BlockVector::write_to_stream /
+ * read_from_stream (src/Matrix/BlockVector.C:1064-1110: the length on its own
+ * line, then the raw doubles; the restart format of CheckpointIO), reading
+ * fails unless the file holds exactly n entries; pmcd_vector_write_market =
+ * BlockVector::write (:598-615, MatrixMarket array); pmcd_matrix_write_market
+ * = TMatrix::write (src/Matrix/Matrix.C:97-130, MatrixMarket coordinate,
+ * 1-based, 16 significant digits). */
+int pmcd_vector_write(const char* path, long long n, const double* v);
+int pmcd_vector_read(const char* path, long long n, double* v);
+int pmcd_vector_write_market(const char* path, long long n, const double* v);
+int pmcd_matrix_write_market(const char* path, int n_rows, int n_cols, const int* row_ptr, const int* col,
+                             const double* val);
+
 /* ---- batched instances ------------------------------------------------------
  * Many independent problems (own mesh, coefficients and boundary data; shared
  * ansatz order, discretisation and quadrature) assembled and solved together:

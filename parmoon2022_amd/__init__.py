@@ -63,7 +63,8 @@ EXPORTS = [
     "pmcd_last_error", "pmcd_device_count", "pmcd_mem_info", "pmcd_system_set_data",
     "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
     "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_sweep_order", "pmcd_host_sell_layout",
-    "pmcd_host_write_mesh", "pmcd_cycle_directions",
+    "pmcd_host_write_mesh", "pmcd_cycle_directions", "pmcd_vector_write", "pmcd_vector_read",
+    "pmcd_vector_write_market", "pmcd_matrix_write_market",
     "pmcd_jacobi_sweep",
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
     "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
@@ -101,6 +102,36 @@ def get_vcycle_graphs():
     v = ctypes.c_int(0)
     _check(lib().pmcd_get_tuning(6, ctypes.byref(v)))
     return bool(v.value)
+
+
+def write_vector(path, v):
+    """BlockVector::write_to_stream: the length on its own line, then the raw
+    doubles (the reference's checkpoint format)."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    _check(lib().pmcd_vector_write(os.fsencode(path), ctypes.c_longlong(len(v)), _ptr(v, ctypes.c_double)))
+
+
+def read_vector(path, n):
+    """BlockVector::read_from_stream of a vector of length n."""
+    out = np.empty(n, dtype=np.float64)
+    _check(lib().pmcd_vector_read(os.fsencode(path), ctypes.c_longlong(n), _ptr(out, ctypes.c_double)))
+    return out
+
+
+def write_vector_market(path, v):
+    """BlockVector::write (MatrixMarket array format)."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    _check(lib().pmcd_vector_write_market(os.fsencode(path), ctypes.c_longlong(len(v)), _ptr(v, ctypes.c_double)))
+
+
+def write_matrix_market(path, row_ptr, col, val, n_cols=None):
+    """TMatrix::write (MatrixMarket coordinate format, 1-based)."""
+    rp = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    c = np.ascontiguousarray(col, dtype=np.int32)
+    v = np.ascontiguousarray(val, dtype=np.float64)
+    n = len(rp) - 1
+    _check(lib().pmcd_matrix_write_market(os.fsencode(path), n, n if n_cols is None else int(n_cols),
+                                          _ptr(rp, ctypes.c_int), _ptr(c, ctypes.c_int), _ptr(v, ctypes.c_double)))
 
 
 def cycle_directions(cycle, n_levels):
@@ -443,6 +474,19 @@ class ConvectionDiffusion:
         _check(lib().pmcd_apply(self._s, level, _ptr(x, ctypes.c_double),
                                 _ptr(out, ctypes.c_double), ctypes.c_double(a), mode))
         return out
+
+    def write_solution(self, path):
+        """The solution in the reference's checkpoint format (write_to_file)."""
+        write_vector(path, self.solution())
+
+    def read_solution(self, path):
+        """Restart from a solution written by write_solution (read_from_file)."""
+        self.set_solution(read_vector(path, self.info()["n_dof"]))
+
+    def write_matrix(self, path, level=-1):
+        """The level's matrix as MatrixMarket coordinates (TMatrix::write)."""
+        rp, col = self.pattern(level)
+        write_matrix_market(path, rp, col, self.matrix_values(level))
 
     def jacobi_sweep(self, b, x, omega=0.8, level=-1):
         """JacobiSmoother::smooth, one step: x + omega D^{-1} (b - A x)."""

@@ -2,6 +2,7 @@
 #include "../../include/pmcd.h"
 #include "system.hpp"
 #include "batch.hpp"
+#include "io.hpp"
 #include <cstring>
 #include <string>
 
@@ -551,6 +552,36 @@ int pmcd_batch_get_rhs(pmcd_batch* b, int inst, double* rhs) {
   return guarded([&] {
     PMCD_CHECK(b && b->sys && rhs, "null argument");
     b->sys->get_rhs(inst, rhs);
+  });
+}
+
+int pmcd_vector_write(const char* path, long long n, const double* v) {
+  return guarded([&] {
+    PMCD_CHECK(path && (v || n == 0) && n >= 0, "invalid argument");
+    write_vector_stream(path, n, v);
+  });
+}
+
+int pmcd_vector_read(const char* path, long long n, double* v) {
+  return guarded([&] {
+    PMCD_CHECK(path && (v || n == 0) && n >= 0, "invalid argument");
+    read_vector_stream(path, n, v);
+  });
+}
+
+int pmcd_vector_write_market(const char* path, long long n, const double* v) {
+  return guarded([&] {
+    PMCD_CHECK(path && (v || n == 0) && n >= 0, "invalid argument");
+    write_vector_market(path, n, v);
+  });
+}
+
+int pmcd_matrix_write_market(const char* path, int n_rows, int n_cols, const int* row_ptr, const int* col,
+                             const double* val) {
+  return guarded([&] {
+    PMCD_CHECK(path && row_ptr && n_rows >= 0 && n_cols >= 0, "invalid argument");
+    PMCD_CHECK(row_ptr[n_rows] == 0 || (col && val), "invalid argument");
+    write_matrix_market(path, n_rows, n_cols, row_ptr, col, val);
   });
 }
 

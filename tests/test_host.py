@@ -248,3 +248,48 @@ def test_mesh_file_errors(tmp_path):
     bad.write_text("Dimension 3\nVertices\n4\n0 0 0 0\n1 0 0 0\n0 1 0 0\n0 0 1 0\nTetrahedra\n1\n1 2 3 9 0\nEnd\n")
     with pytest.raises(RuntimeError, match="out of range"):
         pm.HostSetup({"geo_file": str(bad), "ANSATZ_ORDER": 1, "example": "sine_3d"})
+
+
+def test_vector_stream_format(tmp_path):
+    """BlockVector::write_to_stream: the length on its own line, then the raw
+    doubles; read back bitwise; a wrong length or a first line with more than
+    the count is an error (BlockVector.C:1075-1110)."""
+    v = np.random.default_rng(1).standard_normal(37)
+    p = tmp_path / "v.dat"
+    pm.write_vector(str(p), v)
+    assert p.read_bytes() == b"37\n" + v.tobytes()
+    assert np.array_equal(pm.read_vector(str(p), 37), v)
+    with pytest.raises(RuntimeError, match="unexpected size"):
+        pm.read_vector(str(p), 36)
+    bad = tmp_path / "bad.dat"
+    bad.write_bytes(b"37 1\n" + v.tobytes())
+    with pytest.raises(RuntimeError, match="formatting error"):
+        pm.read_vector(str(bad), 37)
+    short = tmp_path / "short.dat"
+    short.write_bytes(b"37\n" + v[:10].tobytes())
+    with pytest.raises(RuntimeError, match="ends before"):
+        pm.read_vector(str(short), 37)
+
+
+def test_matrix_market_files(tmp_path):
+    """TMatrix::write / BlockVector::write: MatrixMarket coordinate (1-based)
+    and array files that scipy's reader loads back to the same matrix and
+    vector (16 significant digits)."""
+    import scipy.io
+    import scipy.sparse
+    h = pm.HostSetup({"geo_file": "UnitSquare", "ANSATZ_ORDER": 2, "refinement_n_initial_steps": 2,
+                      "example": "sine_2d"})
+    rp, col = h.pattern()
+    val = np.random.default_rng(2).standard_normal(len(col))
+    p = tmp_path / "A.mtx"
+    pm.write_matrix_market(str(p), rp, col, val)
+    assert p.read_text().splitlines()[0] == "%%MatrixMarket matrix coordinate real general "
+    A = scipy.sparse.csr_matrix(scipy.io.mmread(str(p)))
+    n = len(rp) - 1
+    B = scipy.sparse.csr_matrix((val, col, rp), shape=(n, n))
+    assert abs(A - B).max() <= 1e-15 * np.abs(val).max()
+    x = np.random.default_rng(3).standard_normal(n)
+    q = tmp_path / "x.mtx"
+    pm.write_vector_market(str(q), x)
+    y = scipy.io.mmread(str(q)).ravel()
+    assert np.allclose(y, x, rtol=1e-15, atol=0)

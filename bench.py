@@ -348,14 +348,134 @@ def bench_c5(args):
     print(json.dumps(result), flush=True)
 
 
+def tcd_db(args):
+    return {"geo_file": "Default_UnitCube_Hexa", "ANSATZ_ORDER": 2,
+            "refinement_n_initial_steps": args.refine, "example": "quadratic_space_time",
+            "space_discretization_type": "supg", "INTERNAL_PROBLEM_LINEAR": 1 if args.quad == 27 else 0,
+            "time_discretization": "crank_nicolson", "time_step_length": args.tau, "time_start": 0.0,
+            "solver_type": "iterative", "iterative_solver_type": "fgmres",
+            "preconditioner": "multigrid", "multigrid_n_levels": args.levels,
+            "multigrid_n_pre_smooth": 2, "multigrid_n_post_smooth": 2, "jacobi_damp": args.jacobi_damp,
+            "multigrid_correction_damp_factor": 1.0, "gmres_restart": 30,
+            "residual_reduction": 1e-10, "residual_tolerance": 0.0, "max_n_iterations": args.max_iter,
+            "multigrid_mixed_precision": 1 if args.mixed else 0, "multigrid_smoother": args.smoother,
+            "sor_omega": args.sor_omega}
+
+
+def tcd_cpu_baseline(db, gpu_iters, sample_div, matrices, rhs):
+    """Oracle timed on a bounded sample of one time step of the same problem:
+    the time-dependent assembly (stiffness, mass, rhs) of 1/sample_div of the
+    finest cells scaled to all levels, three finest-level matrix passes for
+    the Crank-Nicolson rhs (M u_old, A u_old) and the system matrix, and two
+    FGMRES-MG iterations on the system the GPU formed, scaled to the GPU's
+    iteration count."""
+    import ctypes
+    import numpy as np
+    import oracle
+    import parmoon2022_amd as pm
+    t0 = time.time()
+    o = oracle.Oracle(pm.problem_desc(db))
+    setup_s = time.time() - t0
+    t = pm.tcd_desc(db)
+    oracle.lib().orc_tcd_config(o._p, t.scheme, t.bdf_first, t.tau, t.t0)
+    ncell = o.info()["n_cells"]
+    nsample = max(1, ncell // sample_div)
+    t0 = time.time()
+    oracle.lib().orc_assemble_cells(o._p, 0, nsample)
+    t_cells = time.time() - t0
+    P = ctypes.POINTER(ctypes.c_double)
+    for level, (v, f) in enumerate(zip(matrices, rhs)):
+        v = np.ascontiguousarray(v)
+        f = np.ascontiguousarray(f)
+        oracle.lib().orc_set_matrix(o._p, level, v.ctypes.data_as(P))
+        oracle.lib().orc_set_rhs(o._p, level, f.ctypes.data_as(P))
+    x = np.ascontiguousarray(rhs[-1])
+    t0 = time.time()
+    for _ in range(3):
+        o.apply(x)
+    t_pass = time.time() - t0
+    sd = pm.solver_desc(db)
+    sd.max_iter = 2
+    t0 = time.time()
+    o.solve(sd)
+    per_iter = (time.time() - t0) / 2
+    levels = int(db["multigrid_n_levels"])
+    level_factor = sum(8.0 ** -k for k in range(levels))
+    t_asm = t_cells * (ncell / nsample) * level_factor
+    t_step = t_asm + t_pass + gpu_iters * per_iter
+    return {"value": 1.0 / t_step, "unit": "time steps/s", "cores": 1, "kind": "port",
+            "sample": ("oracle (C, -O3, 1 thread) on one time step of the same problem: time-dependent "
+                       "assembly of %d of %d finest cells (%.2f s) scaled to all %d levels, 3 finest-level "
+                       "matrix passes (%.2f s), 2 FGMRES+MG iterations (%.2f s/iteration) scaled to the GPU's "
+                       "%d; setup %.1f s untimed" % (nsample, ncell, t_cells, levels, t_pass, per_iter,
+                                                     gpu_iters, setup_s))}
+
+
+def bench_tcd(args):
+    """Time-dependent CD3D: Crank-Nicolson steps of the quadratic space-time
+    example (u = t^2 (x^2 + y z), eps 1, b = (4,3,2), c = 1; the reference's
+    TCD3D example -1) with Q2 SUPG on the refined unit cube. One step =
+    stiffness, mass and rhs assembly on every level at the new time, the
+    scheme's rhs and system matrix, and the FGMRES-MG solve. The solution is
+    in the ansatz space and Crank-Nicolson is exact for it, so the L2 error
+    stays at the solver's tolerance."""
+    import numpy as np
+    import parmoon2022_amd as pm
+    db = tcd_db(args)
+    t0 = time.time()
+    cd = pm.TimeConvectionDiffusion(db)
+    info = cd.info()
+    log("setup %.1f s" % (time.time() - t0), info)
+    cd.assemble_initial_time()
+    lat, iters = [], []
+    for step in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        st = cd.step()
+        dt = time.perf_counter() - t0
+        log("step %d (t = %.3f): %.3f s, %d iterations" % (step, cd.current_time, dt, st.iterations))
+        if step >= args.warmup:
+            lat.append(dt)
+            iters.append(st.iterations)
+    errs = cd.output()
+    total = sum(lat)
+    value = args.steps / total
+    n, nnz = info["n_dof"], info["nnz"]
+    result = {
+        "metric": "CD3D solves/s & DoF/s at fixed L2 err vs FP64 oracle; p50/p99 solve latency",
+        "value": value, "unit": "time steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-vcycle/f64" if args.mixed else "f64",
+        "data": "synthetic (manufactured solution, generated mesh)",
+        "config": {"workload": "TCD3D Q2 SUPG unit cube %d^3, %d DoF, nnz %d, Crank-Nicolson tau %g, "
+                               "FGMRES + %d-level MG" % (2 ** args.refine, n, nnz, args.tau, info["n_levels"]),
+                   "dof": n, "nnz": nnz},
+        "dof_per_s": value * n,
+        "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
+        "iterations": int(np.median(iters)), "l2_error": float(errs[0]), "h1_error": float(errs[1]),
+    }
+    mats, rhss = [], []
+    if not args.no_cpu:
+        for level in range(info["n_levels"]):
+            mats.append(cd.matrix_values(level))
+            rhss.append(cd.rhs(level))
+    cd.close()
+    if not args.no_cpu:
+        cb = tcd_cpu_baseline(db, int(np.median(iters)), args.cpu_sample, mats, rhss)
+        result["cpu_baseline"] = cb
+        result["gpu_over_cpu"] = value / cb["value"]
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "c5"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "c5", "tcd"],
                     help="c2: one 64^3 Q2 SUPG solve per step (default); c1: the CD2D P1 CG + Jacobi "
                          "problem on 257^2 vertices; c4: a batch of --instances P1 "
                          "tetrahedral instances per step; c5: a served stream of mixed CD2D P2 / CD3D Q1 "
-                         "requests at each of --rates")
+                         "requests at each of --rates; tcd: Crank-Nicolson time steps of the "
+                         "quadratic space-time example with Q2 SUPG on the --refine cube")
+    ap.add_argument("--tau", type=float, default=0.01, help="tcd time step length")
     ap.add_argument("--rates", default="10,30,100,300,1000,2000", help="c5 arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=60, help="c5 requests per rate")
     ap.add_argument("--slots", type=int, default=8, help="c5 concurrent requests")
@@ -397,6 +517,8 @@ def main():
         return bench_c4(args)
     if args.workload == "c5":
         return bench_c5(args)
+    if args.workload == "tcd":
+        return bench_tcd(args)
     db = c2_db(args)
     t0 = time.time()
     cd = pm.ConvectionDiffusion(db)

@@ -40,7 +40,9 @@ enum {
   PMCD_EX_TEST_P0_3D = 3,
   PMCD_EX_TEST_P1_3D = 4,
   PMCD_EX_TEST_P2_3D = 5,
-  PMCD_EX_CONST_RHS = 6
+  PMCD_EX_CONST_RHS = 6,
+  PMCD_EX_TCD_LINEAR_3D = 7, /* include/Examples/TCD_3D/linear_space_time.h */
+  PMCD_EX_TCD_QUAD_3D = 8    /* include/Examples/TCD_3D/quadratic_space_time.h */
 };
 
 typedef struct {
@@ -269,6 +271,24 @@ int pmcd_spmv_device(int n, int n_active, const int* row_ptr, const int* col, co
                      void* stream);
 /* Ddot (Blas1.C:21-38): deterministic two-pass reduction, result to host. */
 int pmcd_dot_device(int n, const double* x, const double* y, double* result, void* stream);
+
+/* ---- time-dependent problems ----------------------------------------------
+ * TimeConvectionDiffusion<d> (src/System/TimeConvectionDiffusion.C:259-452)
+ * with TimeDiscretization (src/TimeDiscRoutines/TimeDiscretizations.C):
+ * pmcd_tcd_init = assemble_initial_time (stiffness, mass and rhs at
+ * time_start, initial condition = the example's exact solution interpolated
+ * at the nodes); pmcd_tcd_step = one step of length tau (assemble at the new
+ * time, the scheme's rhs and system matrix on every level, solve); *time
+ * receives the new time. Errors at the current time: pmcd_errors. */
+enum { PMCD_BACKWARD_EULER = 0, PMCD_CRANK_NICOLSON = 1, PMCD_BDF_TWO = 2 };
+typedef struct {
+  int scheme;     /* time_discretization: PMCD_BACKWARD_EULER, PMCD_CRANK_NICOLSON, PMCD_BDF_TWO */
+  int bdf_first;  /* bdf_two_first_step: PMCD_BACKWARD_EULER or PMCD_CRANK_NICOLSON */
+  double tau;     /* time_step_length */
+  double t0;      /* time_start */
+} pmcd_tcd_desc;
+int pmcd_tcd_init(pmcd_system* s, const pmcd_tcd_desc* t);
+int pmcd_tcd_step(pmcd_system* s, const pmcd_solver_desc* sd, pmcd_solve_stats* stats, double* time);
 
 /* Files in the reference's formats (host arrays; no GPU use):
  * pmcd_vector_write / pmcd_vector_read = BlockVector::write_to_stream /

@@ -52,6 +52,11 @@ def lib():
                      "orc_assemble", "orc_matrix", "orc_rhs", "orc_solution", "orc_set_solution",
                      "orc_apply", "orc_solve", "orc_vcycle", "orc_errors"]:
             getattr(L, name).restype = None if name != "orc_solve" else ctypes.c_int
+        L.orc_tcd_init.restype = ctypes.c_int
+        L.orc_tcd_init.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
+        L.orc_tcd_step.restype = ctypes.c_int
+        L.orc_tcd_config.restype = ctypes.c_int
+        L.orc_tcd_config.argtypes = L.orc_tcd_init.argtypes
         L.orc_ddot.restype = ctypes.c_double
         L.orc_quad.restype = ctypes.c_int
         _lib = L
@@ -172,6 +177,52 @@ class Oracle:
         e = np.zeros(5)
         lib().orc_errors(self._p, _p(e))
         return e
+
+    # time-dependent problems: the example's coefficients at the current time
+    def tcd_init(self, scheme, bdf_first, tau, t0):
+        """Assemble stiffness, mass and rhs at t0; the solution becomes the
+        exact solution interpolated at the nodes."""
+        if lib().orc_tcd_init(self._p, int(scheme), int(bdf_first), float(tau), float(t0)):
+            raise ValueError("bad time discretization parameters")
+        self.tcd_errors = np.zeros(6)
+        self.tcd_tau = float(tau)
+
+    def tcd_step(self, sdesc):
+        """One time step; returns (OStats, new time)."""
+        s = solver(sdesc)
+        st = OStats()
+        t = ctypes.c_double()
+        rc = lib().orc_tcd_step(self._p, ctypes.byref(s), ctypes.byref(st), ctypes.byref(t))
+        if rc == 1:
+            raise RuntimeError("call tcd_init before tcd_step")
+        if rc == 2:
+            raise RuntimeError("oracle solver diverged")
+        return st, t.value
+
+    def tcd_matrices(self, level=-1):
+        """(stiffness, mass) values of the level's pattern."""
+        nnz = self.info(level)["nnz"]
+        a, m = np.empty(nnz), np.empty(nnz)
+        lib().orc_tcd_matrices(self._p, level, _p(a), _p(m))
+        return a, m
+
+    def tcd_output(self):
+        """Errors at the current time, accumulated over time as
+        TimeConvectionDiffusion::output does (trapezoidal sums of the squared
+        L2 and H1-semi errors, src/System/TimeConvectionDiffusion.C:938-963)."""
+        e = self.errors()
+        te, tau = self.tcd_errors, self.tcd_tau
+        te[0] += (e[0] * e[0] + te[1] * te[1]) * tau * 0.5
+        te[1] = e[0]
+        te[2] += (e[1] * e[1] + te[3] * te[3]) * tau * 0.5
+        te[3] = e[1]
+        te[4] = max(te[4], e[4])
+        te[5] = max(te[5], e[0])
+        return e
+
+    def tcd_get_errors(self):
+        te = self.tcd_errors
+        return np.array([np.sqrt(te[0]), np.sqrt(te[2]), te[4]])
 
 
 def basis(shape, order, point):

@@ -607,6 +607,7 @@ static oquad make_quad(int shape, int degree) {
 typedef struct {
   int kind;
   double eps, b[3], c, f0;
+  double t; /* time of the time-dependent examples 7, 8 */
 } oexample;
 
 /* coeffs: eps, b1, b2, b3, c, f */
@@ -636,6 +637,19 @@ static void ex_coeffs(const oexample* ex, double x, double y, double z, double* 
       o[0] = 1; o[1] = 4; o[2] = 3; o[3] = 2; o[4] = 1;
       o[5] = -1 * 2 + 4 * 2 * x + 3 * z + 2 * y + 1 * (x * x + y * z);
       break;
+    case 7: { /* include/Examples/TCD_3D/linear_space_time.h */
+      const double t = ex->t;
+      o[0] = 1; o[1] = 1; o[2] = 2; o[3] = 3; o[4] = 0;
+      o[5] = 3 * y + 4 * z + 2 + 6 * t + 12 * t;
+      break;
+    }
+    case 8: { /* include/Examples/TCD_3D/quadratic_space_time.h */
+      const double t = ex->t, eps = 1;
+      o[0] = eps; o[1] = 4; o[2] = 3; o[3] = 2; o[4] = 1;
+      o[5] = 2 * t * (x * x + y * z) - eps * 2 * t * t + 4 * 2 * x * t * t + 3 * z * t * t + 2 * y * t * t +
+             1 * t * t * (x * x + y * z);
+      break;
+    }
     default:
       o[0] = ex->eps; o[1] = ex->b[0]; o[2] = ex->b[1]; o[3] = ex->b[2]; o[4] = ex->c; o[5] = ex->f0;
   }
@@ -657,6 +671,14 @@ static int ex_exact(const oexample* ex, double x, double y, double z, double* o)
     case 3: o[0] = 1; o[1] = o[2] = o[3] = 0; return 1;
     case 4: o[0] = 1 + 2 * x + 3 * y + 4 * z; o[1] = 2; o[2] = 3; o[3] = 4; return 1;
     case 5: o[0] = x * x + y * z; o[1] = 2 * x; o[2] = z; o[3] = y; return 1;
+    case 7:
+      o[0] = 1 + 2 * x + 3 * ex->t * y + 4 * ex->t * z; o[1] = 2; o[2] = 3 * ex->t; o[3] = 4 * ex->t;
+      return 1;
+    case 8: {
+      const double t = ex->t;
+      o[0] = t * t * (x * x + y * z); o[1] = 2 * x * t * t; o[2] = z * t * t; o[3] = y * t * t;
+      return 1;
+    }
     default: o[0] = o[1] = o[2] = o[3] = 0; return 0;
   }
 }
@@ -693,6 +715,7 @@ typedef struct {
   int* nb_cell; /* face neighbours: nb_cell[c*NJ+j], -1 boundary */
   int* nb_joint;
   int* comp;    /* boundary component per joint, -1 interior */
+  double *aval, *mval; /* time-dependent problems: stiffness and mass matrices */
 } olevel;
 
 typedef struct orc_problem {
@@ -702,6 +725,10 @@ typedef struct orc_problem {
   oquad qd;
   oexample ex;
   int supg;
+  /* time-dependent problems (TimeConvectionDiffusion + TimeDiscretization) */
+  int tcd, scheme, bdf_first, step;
+  double tau, time;
+  double *u_old, *u_old2, *f_old;
 } orc_problem;
 
 /* face neighbours by matching sorted vertex ids of joints */
@@ -1138,17 +1165,21 @@ static void assemble_level(orc_problem* P, olevel* L, int c_begin, int c_end) {
   const omesh* m = &L->m;
   const int nl = L->nl, d = P->el.dim, s = m->shape, nj = NJ[s];
   const oquad* q = &P->qd;
-  memset(L->val, 0, sizeof(double) * L->nnz);
+  /* time-dependent forms: stiffness into aval, mass into mval */
+  double* AV = P->tcd ? L->aval : L->val;
+  memset(AV, 0, sizeof(double) * L->nnz);
+  if(P->tcd) memset(L->mval, 0, sizeof(double) * L->nnz);
   memset(L->rhs, 0, sizeof(double) * L->ndof);
   static double ref[MAXQ][10 * MAXNL];
   for(int k = 0; k < q->n; ++k) elem_eval(&P->el, q->x[k], ref[k]);
-  double A[MAXNL][MAXNL], F[MAXNL];
+  double A[MAXNL][MAXNL], F[MAXNL], Mm[MAXNL][MAXNL];
   double o[7][MAXNL];
   for(int c = c_begin; c < c_end; ++c) {
     ogeo g;
     geo(m, c, &g);
     memset(A, 0, sizeof(A));
     memset(F, 0, sizeof(F));
+    memset(Mm, 0, sizeof(Mm));
     for(int k = 0; k < q->n; ++k) {
       double x[3], cf[6];
       geo_point(&g, d, q->x[k], x);
@@ -1157,6 +1188,36 @@ static void assemble_level(orc_problem* P, olevel* L, int c_begin, int c_end) {
       orig_values(P, &g, ref[k], o);
       const double eps = cf[0], b1 = cf[1], b2 = cf[2], b3 = cf[3], cr = cf[4], f = cf[5];
       const double delta = P->supg ? sdfem_delta(d, g.hK, eps, b1, b2, b3) : 0;
+      if(P->tcd) {
+        /* TCDStiff, TCDMass, TCDRhs, then TCDStiffSUPG, TCDMassSUPG,
+           TCDRhsSUPG (CD_local_assembling_routines.C:9-387; SDFEM type 2:
+           the parameter is not scaled with the time step) */
+        for(int i = 0; i < nl; ++i) {
+          const double t0 = o[0][i], tx = o[1][i], ty = o[2][i], tz = d == 3 ? o[3][i] : 0;
+          F[i] += Mult * t0 * f;
+          for(int j = 0; j < nl; ++j) {
+            const double a0 = o[0][j], ax = o[1][j], ay = o[2][j], az = d == 3 ? o[3][j] : 0;
+            double val = eps * (tx * ax + ty * ay + tz * az);
+            val += (b1 * ax + b2 * ay + b3 * az) * t0;
+            val += cr * t0 * a0;
+            A[i][j] += Mult * val;
+            Mm[i][j] += Mult * t0 * a0;
+          }
+          if(P->supg) {
+            const double bgradv = delta * (b1 * tx + b2 * ty + b3 * tz);
+            for(int j = 0; j < nl; ++j) {
+              const double a0 = o[0][j], ax = o[1][j], ay = o[2][j], az = d == 3 ? o[3][j] : 0;
+              const double lap = d == 3 ? o[4][j] + o[5][j] + o[6][j] : o[4][j] + o[5][j];
+              double v2 = (b1 * ax + b2 * ay + b3 * az + cr * a0) * bgradv;
+              v2 -= eps * lap * bgradv;
+              A[i][j] += Mult * v2;
+              Mm[i][j] += Mult * a0 * bgradv;
+            }
+            F[i] += Mult * f * bgradv;
+          }
+        }
+        continue;
+      }
       for(int i = 0; i < nl; ++i) {
         const double t0 = o[0][i], tx = o[1][i], ty = o[2][i], tz = o[3][i];
         /* Galerkin, then SD (the reference calls the local routines in this order) */
@@ -1209,12 +1270,20 @@ static void assemble_level(orc_problem* P, olevel* L, int c_begin, int c_end) {
         for(int k = 0; k < nl; ++k) {
           int found = 0;
           for(int e = L->rp[l]; e < L->rp[l + 1]; ++e)
-            if(L->col[e] == DOF[k]) { L->val[e] += A[mm][k]; found = 1; break; }
+            if(L->col[e] == DOF[k]) {
+              AV[e] += A[mm][k];
+              if(P->tcd) L->mval[e] += Mm[mm][k];
+              found = 1;
+              break;
+            }
           if(!found) fprintf(stderr, "oracle: entry not found\n");
         }
       } else {
         for(int e = L->rp[l]; e < L->rp[l + 1]; ++e)
-          if(L->col[e] == l) L->val[e] = 1.0;
+          if(L->col[e] == l) {
+            AV[e] = 1.0;
+            if(P->tcd) L->mval[e] = 1.0;
+          }
       }
     }
     for(int mm = 0; mm < nl; ++mm)
@@ -1833,6 +1902,7 @@ orc_problem* orc_create(int geometry, int structured_n, int n_refine, int order,
   P->el = make_elem(shape, order);
   P->qd = make_quad(shape, (linear_quad ? 2 : 3) * order);
   P->ex.kind = example;
+  P->ex.t = 0;
   P->ex.eps = eps;
   for(int k = 0; k < 3; ++k) P->ex.b[k] = b ? b[k] : 0.0;
   P->ex.c = c;
@@ -1863,7 +1933,9 @@ void orc_destroy(orc_problem* P) {
     free(L->dof); free(L->rp); free(L->col); free(L->val); free(L->rhs); free(L->sol);
     free(L->mg_sol); free(L->mg_rhs); free(L->mg_def); free(L->mg_aux); free(L->lu); free(L->piv);
     free(L->nb_cell); free(L->nb_joint); free(L->comp); free(L->qq); free(L->diag);
+    free(L->aval); free(L->mval);
   }
+  free(P->u_old); free(P->u_old2); free(P->f_old);
   free(P->lev);
   free(P);
 }
@@ -1923,6 +1995,127 @@ int orc_solve(orc_problem* P, const osolver* s, ostats* st) {
   if(s->method == 1) return solve_cg(P, s, st);
   if(s->method == 4 || s->method == 5) return solve_sor(P, s, st, s->method == 4 ? 0 : 2);
   return solve_fgmres(P, s, st);
+}
+
+/* ---- time-dependent problems ----
+   TimeConvectionDiffusion<d>::assemble_initial_time / assemble / solve
+   (src/System/TimeConvectionDiffusion.C:259-452) with the schemes of
+   TimeDiscretization (src/TimeDiscRoutines/TimeDiscretizations.C:117-284,
+   426-483): every step assembles the stiffness matrix A, the mass matrix M
+   and the rhs f at the new time on every level; backward Euler solves
+   (M + tau A) u = tau f + M u_old, Crank-Nicolson (M + tau/2 A) u =
+   tau/2 (f + f_old) + M u_old - tau/2 A u_old, BDF2 (M + 2/3 tau A) u =
+   2/3 tau f + 4/3 M u_old - 1/3 M u_old2 with its first step by backward
+   Euler or Crank-Nicolson. Only active rows take the scheme; Dirichlet rows
+   keep A's identity rows and the boundary values at the new time. */
+static const double BDF2[3] = {4. / 3., -1. / 3., 2. / 3.};
+
+/* y += a * V x on active rows for a value array V of the level's pattern */
+static void csr_add_active(const olevel* L, const double* V, const double* x, double* y, double a) {
+  for(int i = 0; i < L->nactive; ++i) {
+    double v = 0;
+    for(int j = L->rp[i]; j < L->rp[i + 1]; ++j) v += V[j] * x[L->col[j]];
+    y[i] += a * v;
+  }
+}
+
+static void tcd_assemble_all(orc_problem* P) {
+  for(int l = 0; l < P->nlev; ++l) assemble_level(P, &P->lev[l], 0, P->lev[l].m.ncells);
+}
+
+/* time parameters and matrix storage only, no assembly (orc_assemble_cells
+   then assembles the time-dependent forms of a range of finest cells) */
+int orc_tcd_config(orc_problem* P, int scheme, int bdf_first, double tau, double t0) {
+  if(scheme < 0 || scheme > 2 || (bdf_first != 0 && bdf_first != 1) || !(tau > 0)) return 1;
+  P->tcd = 1;
+  P->scheme = scheme;
+  P->bdf_first = bdf_first;
+  P->tau = tau;
+  P->time = t0;
+  P->step = 0;
+  P->ex.t = t0;
+  for(int l = 0; l < P->nlev; ++l) {
+    olevel* L = &P->lev[l];
+    if(!L->aval) {
+      L->aval = calloc(L->nnz, sizeof(double));
+      L->mval = calloc(L->nnz, sizeof(double));
+    }
+  }
+  return 0;
+}
+
+int orc_tcd_init(orc_problem* P, int scheme, int bdf_first, double tau, double t0) {
+  if(orc_tcd_config(P, scheme, bdf_first, tau, t0)) return 1;
+  tcd_assemble_all(P);
+  olevel* F = lvl(P, -1);
+  const int n = F->ndof, nl = F->nl;
+  /* initial condition: the exact solution at the nodal points */
+  memset(F->sol, 0, sizeof(double) * n);
+  for(int c = 0; c < F->m.ncells; ++c)
+    for(int i = 0; i < nl; ++i) {
+      double x[3], e[5];
+      node_xyz(P, &F->m, c, i, x);
+      if(ex_exact(&P->ex, x[0], x[1], P->el.dim == 3 ? x[2] : 0.0, e)) F->sol[F->dof[c * nl + i]] = e[0];
+    }
+  free(P->u_old); free(P->u_old2); free(P->f_old);
+  P->u_old = malloc(sizeof(double) * n);
+  P->u_old2 = malloc(sizeof(double) * n);
+  P->f_old = malloc(sizeof(double) * n);
+  memcpy(P->u_old, F->sol, sizeof(double) * n);
+  memcpy(P->u_old2, F->sol, sizeof(double) * n);
+  memcpy(P->f_old, F->rhs, sizeof(double) * n);
+  return 0;
+}
+
+int orc_tcd_step(orc_problem* P, const osolver* s, ostats* st, double* time) {
+  if(!P->tcd) return 1;
+  P->step += 1;
+  const int eff = P->scheme == 2 ? (P->step <= 1 ? P->bdf_first : 2) : P->scheme;
+  const double tau = P->tau;
+  P->time += tau;
+  P->ex.t = P->time;
+  tcd_assemble_all(P);
+  olevel* F = lvl(P, -1);
+  const int n = F->ndof, na = F->nactive;
+  double* f_new = malloc(sizeof(double) * n);
+  memcpy(f_new, F->rhs, sizeof(double) * n);
+  double* r = F->rhs;
+  if(eff == 2) {
+    for(int i = 0; i < na; ++i) r[i] *= BDF2[2] * tau;
+    csr_add_active(F, F->mval, P->u_old, r, BDF2[0]);
+    csr_add_active(F, F->mval, P->u_old2, r, BDF2[1]);
+  } else if(eff == 0) {
+    for(int i = 0; i < na; ++i) r[i] *= tau;
+    csr_add_active(F, F->mval, P->u_old, r, 1.0);
+  } else {
+    for(int i = 0; i < na; ++i) r[i] = 0.5 * tau * r[i] + 0.5 * tau * P->f_old[i];
+    csr_add_active(F, F->mval, P->u_old, r, 1.0);
+    csr_add_active(F, F->aval, P->u_old, r, -0.5 * tau);
+  }
+  memcpy(r + na, f_new + na, sizeof(double) * (n - na));
+  const double factor = eff == 0 ? tau : (eff == 1 ? 0.5 * tau : BDF2[2] * tau);
+  for(int l = 0; l < P->nlev; ++l) {
+    olevel* L = &P->lev[l];
+    for(int i = 0; i < L->ndof; ++i)
+      for(int j = L->rp[i]; j < L->rp[i + 1]; ++j)
+        L->val[j] = i < L->nactive ? factor * L->aval[j] + L->mval[j] : L->aval[j];
+    extract_diag(L);
+    if(L->lu) { free(L->lu); L->lu = NULL; free(L->piv); L->piv = NULL; }
+  }
+  memcpy(F->sol + na, r + na, sizeof(double) * (n - na));
+  memcpy(P->f_old, f_new, sizeof(double) * n);
+  free(f_new);
+  const int rc = orc_solve(P, s, st);
+  memcpy(P->u_old2, P->u_old, sizeof(double) * n);
+  memcpy(P->u_old, F->sol, sizeof(double) * n);
+  if(time) *time = P->time;
+  return rc;
+}
+
+void orc_tcd_matrices(orc_problem* P, int level, double* a, double* m) {
+  olevel* L = lvl(P, level);
+  if(a) memcpy(a, L->aval, sizeof(double) * L->nnz);
+  if(m) memcpy(m, L->mval, sizeof(double) * L->nnz);
 }
 
 void orc_vcycle(orc_problem* P, const osolver* s, const double* v, double* z) {

@@ -24,7 +24,7 @@ GEOMETRIES = {"UnitSquare": 0, "TwoTriangles": 1, "TwoTriangles2": 2,
               "StructuredHexa": 10, "StructuredTetra": 11}
 MESH_FILE = 12
 EXAMPLES = {"sine_laplace_2d": 0, "sine_2d": 1, "sine_3d": 2, "test_p0": 3,
-            "test_p1": 4, "test_p2": 5, "const_rhs": 6}
+            "test_p1": 4, "test_p2": 5, "const_rhs": 6, "linear_space_time": 7, "quadratic_space_time": 8}
 METHODS = {"direct": 0, "cg": 1, "fgmres": 2, "richardson": 3, "sor": 4, "ssor": 5}
 PRECONDS = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2, "sor": 3, "ssor": 4}
 SMOOTHERS = {"jacobi": 0, "sor": 1, "ssor": 2}
@@ -64,7 +64,7 @@ EXPORTS = [
     "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
     "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_sweep_order", "pmcd_host_sell_layout",
     "pmcd_host_write_mesh", "pmcd_cycle_directions", "pmcd_vector_write", "pmcd_vector_read",
-    "pmcd_vector_write_market", "pmcd_matrix_write_market",
+    "pmcd_vector_write_market", "pmcd_matrix_write_market", "pmcd_tcd_init", "pmcd_tcd_step",
     "pmcd_jacobi_sweep",
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
     "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
@@ -519,6 +519,83 @@ class ConvectionDiffusion:
         _check(lib().pmcd_vcycle(self._s, ctypes.byref(sd), _ptr(v, ctypes.c_double),
                                  _ptr(z, ctypes.c_double)))
         return z
+
+
+class TcdDesc(ctypes.Structure):
+    _fields_ = [("scheme", ctypes.c_int), ("bdf_first", ctypes.c_int), ("tau", ctypes.c_double),
+                ("t0", ctypes.c_double)]
+
+
+TIME_SCHEMES = {"backward_euler": 0, "crank_nicolson": 1, "bdf_two": 2}
+
+
+def tcd_desc(db):
+    """TcdDesc from the reference's time parameters (time_discretization,
+    bdf_two_first_step, time_step_length, time_start)."""
+    t = TcdDesc()
+    t.scheme = TIME_SCHEMES[db.get("time_discretization", "backward_euler")]
+    t.bdf_first = TIME_SCHEMES[db.get("bdf_two_first_step", "backward_euler")]
+    if t.bdf_first not in (0, 1):
+        raise ValueError("bdf_two_first_step must be backward_euler or crank_nicolson")
+    t.tau = float(db.get("time_step_length", 0.05))
+    t.t0 = float(db.get("time_start", 0.0))
+    return t
+
+
+class TimeConvectionDiffusion(ConvectionDiffusion):
+    """Time-dependent CD problem: TimeConvectionDiffusion<d> of the reference
+    (src/System/TimeConvectionDiffusion.C). Driver loop as in the
+    reference's programs:
+
+        tcd = TimeConvectionDiffusion(db)
+        tcd.assemble_initial_time()
+        while tcd.current_time < db["time_end"] - 1e-10:
+            tcd.step()      # set_time_disc_parameters, assemble, solve
+            tcd.output()    # errors at the current time, L2(0,t;.) sums
+    """
+
+    def __init__(self, db):
+        super().__init__(db)
+        self.current_time = float(self.db.get("time_start", 0.0))
+        self.current_step = 0
+        # L2(0,t;L2)^2, L2 at t, L2(0,t;H1)^2, H1 at t, Linf(0,t;Linf), Linf(0,t;L2)
+        self.time_errors = np.zeros(6)
+
+    def assemble_initial_time(self):
+        t = tcd_desc(self.db)
+        _check(lib().pmcd_tcd_init(self._s, ctypes.byref(t)))
+        self.current_time = t.t0
+        self.current_step = 0
+
+    def step(self, db=None):
+        sd = solver_desc(self.db if db is None else db)
+        st = SolveStats()
+        tm = ctypes.c_double()
+        _check(lib().pmcd_tcd_step(self._s, ctypes.byref(sd), ctypes.byref(st), ctypes.byref(tm)))
+        self.current_time = tm.value
+        self.current_step += 1
+        self.stats = st
+        return st
+
+    def output(self):
+        """Errors at the current time and their accumulation over time
+        (TimeConvectionDiffusion::output: trapezoidal sums of the squared L2
+        and H1-semi errors)."""
+        e = super().output()
+        tau = float(self.db.get("time_step_length", 0.05))
+        te = self.time_errors
+        te[0] += (e[0] * e[0] + te[1] * te[1]) * tau * 0.5
+        te[1] = e[0]
+        te[2] += (e[1] * e[1] + te[3] * te[3]) * tau * 0.5
+        te[3] = e[1]
+        te[4] = max(te[4], e[4])
+        te[5] = max(te[5], e[0])
+        return e
+
+    def get_errors(self):
+        """[L2(0,t;L2), L2(0,t;H1), Linf(0,t;Linf)] as get_errors() reports them."""
+        te = self.time_errors
+        return np.array([np.sqrt(te[0]), np.sqrt(te[2]), te[4]])
 
 
 class Batch:

@@ -555,6 +555,29 @@ int pmcd_batch_get_rhs(pmcd_batch* b, int inst, double* rhs) {
   });
 }
 
+int pmcd_tcd_init(pmcd_system* s, const pmcd_tcd_desc* t) {
+  return guarded([&] {
+    PMCD_CHECK(s && t, "null argument");
+    TcdDesc d{t->scheme, t->bdf_first, t->tau, t->t0};
+    s->sys->tcd_init(d);
+  });
+}
+
+int pmcd_tcd_step(pmcd_system* s, const pmcd_solver_desc* sd, pmcd_solve_stats* stats, double* time) {
+  return guarded([&] {
+    PMCD_CHECK(s && sd, "null argument");
+    const SolveStats st = s->sys->tcd_step(to_solver(*sd));
+    if(stats) {
+      stats->iterations = st.iterations;
+      stats->converged = st.converged;
+      stats->initial_residual = st.initial_residual;
+      stats->final_residual = st.final_residual;
+      stats->seconds = st.seconds;
+    }
+    if(time) *time = s->sys->tcd_time();
+  });
+}
+
 int pmcd_vector_write(const char* path, long long n, const double* v) {
   return guarded([&] {
     PMCD_CHECK(path && (v || n == 0) && n >= 0, "invalid argument");

@@ -34,6 +34,13 @@ enum ExampleKind : int {
   EX_TEST_P2_3D = 5,
   // eps, b, c given, f = f0, homogeneous Dirichlet; no known solution
   EX_CONST_RHS = 6,
+  // time-dependent examples (the time is ExampleParams::t):
+  // include/Examples/TCD_3D/linear_space_time.h (reference TCD3D example -2):
+  // u = 1 + 2x + 3ty + 4tz, eps = 1, b = (1, 2, 3), c = 0, Dirichlet
+  EX_TCD_LINEAR_3D = 7,
+  // include/Examples/TCD_3D/quadratic_space_time.h (example -1):
+  // u = t^2 (x^2 + y z), eps = 1, b = (4, 3, 2), c = 1, Dirichlet
+  EX_TCD_QUAD_3D = 8,
 };
 
 struct ExampleParams {
@@ -43,6 +50,7 @@ struct ExampleParams {
   double b[3];
   double c;
   double f0;
+  double t;  // time of the time-dependent examples
 };
 
 // coefficients at a point: out = {eps, b1, b2, b3, c, f}
@@ -86,6 +94,21 @@ __host__ __device__ inline void example_coeffs(const ExampleParams& ex, double x
       out[5] = -eps * 2 + 4 * 2 * x + 3 * z + 2 * y + 1 * (x * x + y * z);
       break;
     }
+    case EX_TCD_LINEAR_3D: {
+      // linear_space_time.h BilinearCoeffs: f = u_t + b . grad u
+      const double t = ex.t;
+      out[0] = 1; out[1] = 1; out[2] = 2; out[3] = 3; out[4] = 0;
+      out[5] = 3 * y + 4 * z + 2 + 6 * t + 12 * t;
+      break;
+    }
+    case EX_TCD_QUAD_3D: {
+      // quadratic_space_time.h BilinearCoeffs: f = u_t - eps lap u + b . grad u + c u
+      const double t = ex.t, eps = 1;
+      out[0] = eps; out[1] = 4; out[2] = 3; out[3] = 2; out[4] = 1;
+      out[5] = 2 * t * (x * x + y * z) - eps * 2 * t * t + 4 * 2 * x * t * t + 3 * z * t * t + 2 * y * t * t +
+               1 * t * t * (x * x + y * z);
+      break;
+    }
     default:  // EX_CONST_RHS
       out[0] = ex.eps; out[1] = ex.b[0]; out[2] = ex.b[1]; out[3] = ex.b[2];
       out[4] = ex.c; out[5] = ex.f0;
@@ -124,6 +147,17 @@ __host__ __device__ inline bool example_exact(const ExampleParams& ex, double x,
     case EX_TEST_P2_3D:
       out[0] = x * x + y * z; out[1] = 2 * x; out[2] = z; out[3] = y; out[4] = 2;
       return true;
+    case EX_TCD_LINEAR_3D: {
+      const double t = ex.t;
+      out[0] = 1 + 2 * x + 3 * t * y + 4 * t * z; out[1] = 2; out[2] = 3 * t; out[3] = 4 * t; out[4] = 0;
+      return true;
+    }
+    case EX_TCD_QUAD_3D: {
+      const double t = ex.t;
+      out[0] = t * t * (x * x + y * z); out[1] = 2 * x * t * t; out[2] = z * t * t; out[3] = y * t * t;
+      out[4] = 2 * t * t;
+      return true;
+    }
     default:
       out[0] = out[1] = out[2] = out[3] = out[4] = 0;
       return false;

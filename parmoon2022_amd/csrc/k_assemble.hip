@@ -214,9 +214,10 @@ __global__ __launch_bounds__(256) void k_assemble_local(AsmArgs a) {
     for(int k = 0; k < 6; ++k) s[1 + k] = cf[k];
     s[7] = need2 ? sdfem_delta(a.dim, g.hK, cf[0], cf[1], cf[2], cf[3]) : 0.0;
   }
-  double acc[EPL];
-  for(int t = 0; t < EPL; ++t) acc[t] = 0.0;
+  double acc[EPL], accm[EPL];
+  for(int t = 0; t < EPL; ++t) acc[t] = accm[t] = 0.0;
   double rhs = 0.0;
+  const bool tcd = a.tcd != 0;
   __syncthreads();
 
   for(int q0 = 0; q0 < a.nq; q0 += kQChunk) {
@@ -245,7 +246,25 @@ __global__ __launch_bounds__(256) void k_assemble_local(AsmArgs a) {
             const int i = e / NL, j = e % NL;
             const double t0 = P[0][i], tx = P[1][i], ty = P[2][i], tz = P[3][i];
             const double a0 = P[0][j], ax = P[1][j], ay = P[2][j], az = P[3][j];
-            if(a.dim == 3) {
+            if(tcd) {
+              // TCDStiff<d> (CD_local_assembling_routines.C:9-45; in 2D the
+              // z components are 0)
+              double val = eps * (tx * ax + ty * ay + tz * az);
+              val += (b1 * ax + b2 * ay + b3 * az) * t0;
+              val += c * t0 * a0;
+              acc[t] += Mult * val;
+              // TCDMass<d> (:156-168)
+              accm[t] += Mult * t0 * a0;
+              if(need2) {
+                // TCDStiffSUPG<d> / TCDMassSUPG<d> (:183-327) with the
+                // unscaled SDFEM type 2 parameter
+                const double bgradv = delta * (b1 * tx + b2 * ty + b3 * tz);
+                double v2 = (b1 * ax + b2 * ay + b3 * az + c * a0) * bgradv;
+                v2 -= eps * P[4][j] * bgradv;
+                acc[t] += Mult * v2;
+                accm[t] += Mult * a0 * bgradv;
+              }
+            } else if(a.dim == 3) {
               // BilinearAssembleGalerkin<3> (ConvDiff.C:1842-1883)
               double val = eps * (tx * ax + ty * ay + tz * az);
               val += (b1 * ax + b2 * ay + b3 * az) * t0;
@@ -279,7 +298,11 @@ __global__ __launch_bounds__(256) void k_assemble_local(AsmArgs a) {
         if(lane < NL) {
           const int i = lane;
           const double t0 = P[0][i], tx = P[1][i], ty = P[2][i], tz = P[3][i];
-          if(a.dim == 3) {
+          if(tcd) {
+            // TCDRhs<d> (:170-181) and TCDRhsSUPG<d> (:328-387)
+            rhs += Mult * t0 * f;
+            if(need2) rhs += Mult * f * (delta * (b1 * tx + b2 * ty + b3 * tz));
+          } else if(a.dim == 3) {
             rhs += Mult * t0 * f;
             if(need2) {
               const double bgradv = b1 * tx + b2 * ty + b3 * tz;
@@ -300,10 +323,14 @@ __global__ __launch_bounds__(256) void k_assemble_local(AsmArgs a) {
   }
   if(live) {
     double* A = a.local_A + (size_t)cell * NL * NL;
+    double* Mm = a.local_M ? a.local_M + (size_t)cell * NL * NL : nullptr;
 #pragma unroll
     for(int t = 0; t < EPL; ++t) {
       const int e = lane + kWave * t;
-      if(e < NL * NL) A[e] = acc[t];
+      if(e < NL * NL) {
+        A[e] = acc[t];
+        if(Mm) Mm[e] = accm[t];
+      }
     }
     if(lane < NL) a.local_rhs[(size_t)cell * NL + lane] = rhs;
   }
@@ -529,7 +556,7 @@ __global__ __launch_bounds__(64) void k_cell_errors(AsmArgs a, const int* __rest
 }  // namespace
 
 bool p1_simplex(const AsmArgs& a) {
-  return (a.shape == (int)Shape::Triangle && a.nl == 3) || (a.shape == (int)Shape::Tetrahedron && a.nl == 4);
+  return !a.tcd && ((a.shape == (int)Shape::Triangle && a.nl == 3) || (a.shape == (int)Shape::Tetrahedron && a.nl == 4));
 }
 
 void launch_assemble_local(const AsmArgs& a, hipStream_t s) {

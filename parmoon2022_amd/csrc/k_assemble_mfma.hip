@@ -297,7 +297,7 @@ bool launch_assemble_mfma(const AsmArgs& a, hipStream_t s) {
   // 12.9 vs 33.2 ms, P2 tetrahedra 1.2 vs 1.7 ms, P2 triangles 2.1 vs 3.6 ms,
   // Q2 quadrilaterals within 5%), the MFMA kernel for Q2 hexahedra (10.0 vs
   // 27.7 ms at 64 points).
-  if(a.nl < 16) return false;
+  if(a.nl < 16 || a.tcd) return false;
   const int blocks = (a.n_cells + kCells - 1) / kCells;
   if(a.dim == 3 && a.ncomp == 10 && a.nl == 27) {
     k_assemble_mfma<3, 27><<<blocks, 256, 0, s>>>(a);

@@ -744,7 +744,28 @@ __global__ void k_gj_store(int n, const double* __restrict__ B, const int* __res
   }
 }
 
+// S = factor A + M on active rows, S = A on Dirichlet rows; 8 lanes per row
+__global__ __launch_bounds__(256) void k_tcd_system(int n, int n_active, const int* __restrict__ rp,
+                                                    const double* __restrict__ A, const double* __restrict__ M,
+                                                    double factor, double* __restrict__ S) {
+  const int gt = blockIdx.x * 256 + threadIdx.x;
+  const int row = gt / 8, l = gt % 8;
+  if(row >= n) return;
+  const int b = rp[row], e = rp[row + 1];
+  if(row < n_active)
+    for(int k = b + l; k < e; k += 8) S[k] = factor * A[k] + M[k];
+  else
+    for(int k = b + l; k < e; k += 8) S[k] = A[k];
+}
+
 }  // namespace
+
+void launch_tcd_system(int n, int n_active, const int* rp, const double* A, const double* M, double factor,
+                       double* S, hipStream_t s) {
+  if(n == 0) return;
+  k_tcd_system<<<(int)(((long)n * 8 + 255) / 256), 256, 0, s>>>(n, n_active, rp, A, M, factor, S);
+  PMCD_HIP(hipGetLastError());
+}
 
 void launch_scale_copy(int n, double a, const double* x, double* y, hipStream_t s) {
   if(n) k_scale_copy<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, x, y);

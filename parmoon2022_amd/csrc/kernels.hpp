@@ -27,11 +27,17 @@ struct AsmArgs {
   const double* cell_xyz;   // [n_cells][nv][3]
   double* local_A;          // [n_cells][nl][nl]  (row = test, column = ansatz)
   double* local_rhs;        // [n_cells][nl]
+  // time-dependent forms (TCDStiff, TCDMass, TCDRhs and their SUPG parts,
+  // src/AssembleRoutines/CD_local_assembling_routines.C:9-387): local_A gets
+  // the stiffness matrix, local_M the mass matrix
+  int tcd = 0;
+  double* local_M = nullptr;  // [n_cells][nl][nl]
 };
 __host__ __device__ inline const ExampleParams& cell_params(const AsmArgs& a, int cell) {
   return a.cell_inst ? a.cell_ex[a.cell_inst[cell]] : a.ex;
 }
-// P1 triangles/tetrahedra take a thread-per-cell kernel (same arithmetic)
+// P1 triangles/tetrahedra take a thread-per-cell kernel (same arithmetic;
+// not for the time-dependent forms)
 bool p1_simplex(const AsmArgs& a);
 void launch_assemble_local(const AsmArgs& a, hipStream_t s);
 // FP64-MFMA variant; returns false if the element is not covered
@@ -175,6 +181,11 @@ void launch_csr_to_dense(int n, const int* rp, const int* col, const double* val
 // (column-major, overwritten) -> inv (row-major); colbuf holds 2 n doubles,
 // perm and colmap n ints each
 void dense_inverse(int n, double* B, double* inv, double* colbuf, int* perm, int* colmap, hipStream_t s);
+
+// time-dependent system matrix: S = factor A + M on active rows, A on the
+// Dirichlet rows (scale_blocks_actives + add_matrix_actives)
+void launch_tcd_system(int n, int n_active, const int* rp, const double* A, const double* M, double factor,
+                       double* S, hipStream_t s);
 
 // ---- batched (segmented) vector kernels ------------------------------------
 // A chunk is a row range [begin, end) owned by one instance.

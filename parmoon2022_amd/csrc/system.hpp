@@ -55,6 +55,15 @@ struct SolverDesc {
   int coarse_max_iter;     // multigrid_coarse_max_n_iterations
 };
 
+// Time discretization (TimeDiscretization, src/TimeDiscRoutines/
+// TimeDiscretizations.C:26-493)
+struct TcdDesc {
+  int scheme;     // time_discretization: 0 backward_euler, 1 crank_nicolson, 2 bdf_two
+  int bdf_first;  // bdf_two_first_step: 0 backward_euler, 1 crank_nicolson
+  double tau;     // time_step_length
+  double t0;      // time_start
+};
+
 struct SolveStats {
   int iterations = 0;
   double initial_residual = 0;
@@ -96,6 +105,9 @@ struct Level {
   DevBuf<double> sw_val;
   DevBuf<float> sw_val32;
   bool sw_fresh = false, sw_fresh32 = false;
+  // time-dependent problems: stiffness and mass matrices (the system matrix
+  // tau_factor A + M goes to val)
+  DevBuf<double> tcd_A, tcd_M;
   // sliced ELL copy of the active rows (built on first use; values refreshed
   // after assembly)
   int sell_slots = -1;  // -1: not built
@@ -182,6 +194,17 @@ class CDSystem {
   // same mesh, new coefficients / source / boundary values
   void set_example(const ExampleParams& ex);
 
+  // time-dependent problem (TimeConvectionDiffusion<d>,
+  // src/System/TimeConvectionDiffusion.C): tcd_init assembles the stiffness
+  // and mass matrices and the rhs at t0 and sets the initial condition (the
+  // exact solution interpolated at the nodes); tcd_step advances one step of
+  // length tau: assemble at the new time, the right-hand side and system
+  // matrix of the scheme, solve.
+  void tcd_init(const TcdDesc& t);
+  SolveStats tcd_step(const SolverDesc& s);
+  double tcd_time() const { return tcd_t_; }
+  int tcd_n_steps() const { return tcd_n_; }
+
  private:
   void build_level_device(Level& L);
   void boundary_data(Level& L);
@@ -198,6 +221,7 @@ class CDSystem {
   void vcycle32(const double* rhs, double* out);
   void vcycle_eager(const double* rhs, double* out);
   void clear_graphs();
+  void tcd_assemble();  // stiffness, mass and rhs at desc_.ex.t on every level
   void precond(const double* v, double* z);
   SolveStats solve_cg();
   SolveStats solve_cg_device(int n, double normb, double resid);
@@ -233,6 +257,12 @@ class CDSystem {
   std::vector<VGraph> vgraphs_;
   SolverDesc graph_sd_{};
   int graph_epoch_ = -1;
+  // time-dependent state
+  bool tcd_ = false;
+  TcdDesc tcd_desc_{};
+  double tcd_t_ = 0;
+  int tcd_n_ = 0;
+  DevBuf<double> local_M_, tcd_f_old_, tcd_u_old_, tcd_u_old2_;
   // Krylov work space
   std::vector<DevBuf<double>> kv_, kz_;
   DevBuf<double> kr_, kw_, kh_;

@@ -298,3 +298,142 @@ def test_oracle_w_f_cycles_converge_to_the_v_cycle_solution(oracle_lib, cycle):
     (iv, uv), (ic, uc) = out
     assert ic <= iv
     assert np.linalg.norm(uc - uv) <= 1e-9 * np.linalg.norm(uv)
+
+
+# test/tcd3d_solvers_test.c++:170-218: tau = 0.1 up to T = 1, two refinements,
+# L2(0,t;L2) and L2(0,t;H1) stay zero after every step (direct 1e-13, the
+# iterative solvers 1e-10)
+TCD3D_SOLVERS = {
+    "direct": ({"solver_type": "direct"}, 1e-13),
+    "jacobi": ({"solver_type": "iterative", "iterative_solver_type": "richardson", "preconditioner": "jacobi",
+                "residual_tolerance": 1e-13, "residual_reduction": 0.0, "max_n_iterations": 1000,
+                "min_n_iterations": 5}, 1e-10),
+    "multigrid": ({"solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "multigrid",
+                   "multigrid_n_levels": 2, "max_n_iterations": 100, "residual_tolerance": 1e-15,
+                   "residual_reduction": 0.0, "multigrid_cycle_type": "W", "multigrid_smoother": "sor",
+                   "multigrid_smoother_coarse": "direct_solve", "multigrid_correction_damp_factor": 0.8,
+                   "multigrid_n_pre_smooth": 3, "multigrid_n_post_smooth": 3}, 1e-10),
+}
+TCD3D_CASES = [("linear_space_time", 1, "backward_euler"), ("quadratic_space_time", 2, "crank_nicolson")]
+
+
+def tcd3d_db(geo, example, order, scheme, solver="direct", **extra):
+    db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 2, "example": example,
+          "time_discretization": scheme, "time_step_length": 0.1, "time_end": 1.0, "time_start": 0.0,
+          "space_discretization_type": "galerkin"}
+    db.update(TCD3D_SOLVERS[solver][0])
+    db.update(extra)
+    return db
+
+
+def run_oracle_tcd(oracle_lib, db, each_step=None):
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    t = pm.tcd_desc(db)
+    o.tcd_init(t.scheme, t.bdf_first, t.tau, t.t0)
+    sd = pm.solver_desc(db)
+    time = t.t0
+    steps = 0
+    while time < db["time_end"] - 1e-10:
+        st, time = o.tcd_step(sd)
+        o.tcd_output()
+        steps += 1
+        if each_step:
+            each_step(o, st, time)
+    return o, steps
+
+
+@pytest.mark.parametrize("solver", list(TCD3D_SOLVERS))
+@pytest.mark.parametrize("geo", ["Default_UnitCube_Hexa", "Default_UnitCube_Tetra"])
+@pytest.mark.parametrize("example,order,scheme", TCD3D_CASES)
+def test_oracle_tcd3d_solvers_fixture(oracle_lib, geo, example, order, scheme, solver):
+    db = tcd3d_db(geo, example, order, scheme, solver)
+    tol = TCD3D_SOLVERS[solver][1]
+
+    def check(o, st, time):
+        e = o.tcd_get_errors()
+        assert e[0] < tol and e[1] < tol, (time, e)
+
+    o, steps = run_oracle_tcd(oracle_lib, db, check)
+    assert steps == 10
+
+
+@pytest.mark.parametrize("first", ["crank_nicolson"])
+def test_oracle_tcd_bdf2_is_exact_for_quadratics_in_time(oracle_lib, first):
+    """BDF2 differentiates quadratics in time exactly, and so does its
+    Crank-Nicolson first step: on the quadratic example the discrete solution
+    is the exact one at every step. A wrong BDF2 coefficient breaks this."""
+    db = tcd3d_db("Default_UnitCube_Hexa", "quadratic_space_time", 2, "bdf_two", bdf_two_first_step=first)
+
+    def check(o, st, time):
+        e = o.errors()
+        assert e[0] < 1e-12 and e[1] < 1e-12, (time, e)
+
+    run_oracle_tcd(oracle_lib, db, check)
+
+
+def test_oracle_tcd_bdf2_backward_euler_start_is_first_order_only_at_the_start(oracle_lib):
+    """With a backward Euler first step the first solution is off by O(tau)
+    (the quadratic example is not linear in time), and BDF2 carries that error
+    along without amplifying it."""
+    db = tcd3d_db("Default_UnitCube_Hexa", "quadratic_space_time", 2, "bdf_two",
+                  bdf_two_first_step="backward_euler")
+    errs = []
+    run_oracle_tcd(oracle_lib, db, lambda o, st, t: errs.append(o.errors()[0]))
+    assert errs[0] > 1e-4 and max(errs) < 10 * errs[0], errs
+
+
+@pytest.mark.parametrize("geo", ["Default_UnitCube_Hexa", "Default_UnitCube_Tetra"])
+@pytest.mark.parametrize("example,order,scheme", TCD3D_CASES)
+def test_oracle_tcd_supg_is_exact_in_the_ansatz_space(oracle_lib, geo, example, order, scheme):
+    """SUPG is consistent: with the solution in the ansatz space and a scheme
+    that is exact in time for it, the stabilized solution (stiffness, mass and
+    rhs terms tested with delta b.grad v) is the exact one. Dropping the SUPG
+    mass term breaks this."""
+    db = tcd3d_db(geo, example, order, scheme, space_discretization_type="supg")
+
+    def check(o, st, time):
+        e = o.errors()
+        assert e[0] < 1e-12 and e[1] < 1e-12, (time, e)
+
+    run_oracle_tcd(oracle_lib, db, check)
+
+
+def test_oracle_tcd_mass_matrix_closed_forms(oracle_lib):
+    """Q2 on the twice refined cube: p = x(1-x) y(1-y) z(1-z) lies in the
+    space and vanishes on the boundary, so with its interpolant P (zero on the
+    Dirichlet rows) P^T M P = int p^2 = 1/30^3 and P^T M 1 = int p = 1/6^3;
+    the stiffness matrix equals the stationary Galerkin matrix of the same
+    coefficients."""
+    db = tcd3d_db("Default_UnitCube_Hexa", "quadratic_space_time", 2, "backward_euler")
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.tcd_init(0, 0, 0.1, 0.0)
+    a, m = o.tcd_matrices()
+    rp, col = o.pattern()
+    info = o.info()
+    n, na = info["n_dof"], info["n_active"]
+    X = np.zeros(n)
+    cells = o.dofmap()
+    # Q2 nodes: the 3x3x3 lattice of the affine cell map (vertex 0 at
+    # (-1,-1,-1), vertices 1, 3, 4 one step along r, s, t)
+    cc = o.cell_coords(nv=8)
+    for c in range(cells.shape[0]):
+        v = cc[c]
+        for k in range(3):
+            for j in range(3):
+                for i in range(3):
+                    x = v[0] + (v[1] - v[0]) * i / 2 + (v[3] - v[0]) * j / 2 + (v[4] - v[0]) * k / 2
+                    X[cells[c, i + 3 * j + 9 * k]] = np.prod(x * (1 - x))
+    assert np.all(X[na:] == 0)
+    Mx = np.zeros(n)
+    one = np.ones(n)
+    M1 = np.zeros(n)
+    for r in range(na):
+        s = slice(rp[r], rp[r + 1])
+        Mx[r] = m[s] @ X[col[s]]
+        M1[r] = m[s] @ one[col[s]]
+    assert abs(X @ Mx - 1 / 30 ** 3) < 1e-15
+    assert abs(X @ M1 - 1 / 6 ** 3) < 1e-15
+    # the stiffness form is the stationary one
+    s = oracle_lib.Oracle(pm.problem_desc(db))
+    s.assemble()
+    assert np.allclose(a, s.matrix_values(), rtol=1e-14, atol=1e-15)

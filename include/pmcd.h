@@ -42,7 +42,8 @@ enum {
   PMCD_EX_TEST_P2_3D = 5,
   PMCD_EX_CONST_RHS = 6,
   PMCD_EX_TCD_LINEAR_3D = 7, /* include/Examples/TCD_3D/linear_space_time.h */
-  PMCD_EX_TCD_QUAD_3D = 8    /* include/Examples/TCD_3D/quadratic_space_time.h */
+  PMCD_EX_TCD_QUAD_3D = 8,   /* include/Examples/TCD_3D/quadratic_space_time.h */
+  PMCD_EX_TCD_EXP_2D = 9     /* include/Examples/TCD_2D/exp.h; eps = 1 / Reynolds number */
 };
 
 typedef struct {

@@ -650,6 +650,13 @@ static void ex_coeffs(const oexample* ex, double x, double y, double z, double* 
              1 * t * t * (x * x + y * z);
       break;
     }
+    case 9: { /* include/Examples/TCD_2D/exp.h */
+      const double q = 2 * p, et = exp(sin(q * ex->t)), s = sin(q * x) * sin(q * y);
+      o[0] = ex->eps; o[1] = 1; o[2] = -1; o[3] = 0; o[4] = 1;
+      o[5] = et * q * cos(q * ex->t) * s + 2 * ex->eps * q * q * et * s + et * q * cos(q * x) * sin(q * y) -
+             et * q * sin(q * x) * cos(q * y) + et * s;
+      break;
+    }
     default:
       o[0] = ex->eps; o[1] = ex->b[0]; o[2] = ex->b[1]; o[3] = ex->b[2]; o[4] = ex->c; o[5] = ex->f0;
   }
@@ -679,6 +686,12 @@ static int ex_exact(const oexample* ex, double x, double y, double z, double* o)
       o[0] = t * t * (x * x + y * z); o[1] = 2 * x * t * t; o[2] = z * t * t; o[3] = y * t * t;
       return 1;
     }
+    case 9: {
+      const double q = 2 * p, et = exp(sin(q * ex->t));
+      o[0] = et * sin(q * x) * sin(q * y); o[1] = et * q * cos(q * x) * sin(q * y);
+      o[2] = et * q * sin(q * x) * cos(q * y); o[3] = 0;
+      return 1;
+    }
     default: o[0] = o[1] = o[2] = o[3] = 0; return 0;
   }
 }
@@ -687,6 +700,7 @@ static int ex_bc(const oexample* ex, int comp) { return (ex->kind == 0 && comp =
 
 static double ex_bvalue(const oexample* ex, int comp, double x, double y, double z) {
   double e[4];
+  if(ex->kind == 9) return 0; /* exp.h BoundValue */
   if(!ex_exact(ex, x, y, z, e)) return 0;
   if(ex->kind == 0 && comp == 1) return ex->eps * (1.0 * e[1] + 0.0 * e[2]);
   return e[0];

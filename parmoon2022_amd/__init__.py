@@ -24,7 +24,8 @@ GEOMETRIES = {"UnitSquare": 0, "TwoTriangles": 1, "TwoTriangles2": 2,
               "StructuredHexa": 10, "StructuredTetra": 11}
 MESH_FILE = 12
 EXAMPLES = {"sine_laplace_2d": 0, "sine_2d": 1, "sine_3d": 2, "test_p0": 3,
-            "test_p1": 4, "test_p2": 5, "const_rhs": 6, "linear_space_time": 7, "quadratic_space_time": 8}
+            "test_p1": 4, "test_p2": 5, "const_rhs": 6, "linear_space_time": 7, "quadratic_space_time": 8,
+            "exp_sin_cos_2d": 9}
 METHODS = {"direct": 0, "cg": 1, "fgmres": 2, "richardson": 3, "sor": 4, "ssor": 5}
 PRECONDS = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2, "sor": 3, "ssor": 4}
 SMOOTHERS = {"jacobi": 0, "sor": 1, "ssor": 2}
@@ -252,7 +253,10 @@ def problem_desc(db):
     use_mg = db.get("preconditioner") == "multigrid" and db.get("solver_type") == "iterative"
     d.mg_levels = int(db.get("multigrid_n_levels", 1)) if use_mg else 1
     d.example = EXAMPLES[db.get("example", "sine_laplace_2d")]
-    d.eps = float(db.get("diffusion_coefficient", 1.0))
+    if "diffusion_coefficient" in db or "reynolds_number" not in db:
+        d.eps = float(db.get("diffusion_coefficient", 1.0))
+    else:
+        d.eps = 1.0 / float(db["reynolds_number"])  # examples with eps = 1 / RE_NR
     b = db.get("convection", (0.0, 0.0, 0.0))
     for k in range(3):
         d.b[k] = float(b[k]) if k < len(b) else 0.0

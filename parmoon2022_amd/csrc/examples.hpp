@@ -41,6 +41,10 @@ enum ExampleKind : int {
   // include/Examples/TCD_3D/quadratic_space_time.h (example -1):
   // u = t^2 (x^2 + y z), eps = 1, b = (4, 3, 2), c = 1, Dirichlet
   EX_TCD_QUAD_3D = 8,
+  // include/Examples/TCD_2D/exp.h (reference TCD2D example 0): u =
+  // exp(sin(2 pi t)) sin(2 pi x) sin(2 pi y), eps (1 / Reynolds number),
+  // b = (1, -1), c = 1, homogeneous Dirichlet
+  EX_TCD_EXP_2D = 9,
 };
 
 struct ExampleParams {
@@ -109,6 +113,14 @@ __host__ __device__ inline void example_coeffs(const ExampleParams& ex, double x
                1 * t * t * (x * x + y * z);
       break;
     }
+    case EX_TCD_EXP_2D: {
+      // exp.h BilinearCoeffs: f = u_t - eps lap u + b . grad u + c u
+      const double q = 2 * p, et = exp(sin(q * ex.t)), s = sin(q * x) * sin(q * y);
+      out[0] = ex.eps; out[1] = 1; out[2] = -1; out[3] = 0; out[4] = 1;
+      out[5] = et * q * cos(q * ex.t) * s - ex.eps * et * q * q * (-s - s) +
+               1 * et * q * cos(q * x) * sin(q * y) + (-1) * et * q * sin(q * x) * cos(q * y) + 1 * et * s;
+      break;
+    }
     default:  // EX_CONST_RHS
       out[0] = ex.eps; out[1] = ex.b[0]; out[2] = ex.b[1]; out[3] = ex.b[2];
       out[4] = ex.c; out[5] = ex.f0;
@@ -158,6 +170,15 @@ __host__ __device__ inline bool example_exact(const ExampleParams& ex, double x,
       out[4] = 2 * t * t;
       return true;
     }
+    case EX_TCD_EXP_2D: {
+      const double q = 2 * p, et = exp(sin(q * ex.t));
+      out[0] = et * sin(q * x) * sin(q * y);
+      out[1] = et * q * cos(q * x) * sin(q * y);
+      out[2] = et * q * sin(q * x) * cos(q * y);
+      out[3] = 0;
+      out[4] = -2 * q * q * out[0];
+      return true;
+    }
     default:
       out[0] = out[1] = out[2] = out[3] = out[4] = 0;
       return false;
@@ -175,6 +196,7 @@ __host__ __device__ inline int example_bc(const ExampleParams& ex, int comp) {
 __host__ __device__ inline double example_bvalue(const ExampleParams& ex, int comp, double x,
                                                  double y, double z) {
   double e[5];
+  if(ex.kind == EX_TCD_EXP_2D) return 0.0;  // exp.h BoundValue
   if(!example_exact(ex, x, y, z, e)) return 0.0;
   if(ex.kind == EX_SINE_LAPLACE_2D && comp == 1) {
     // outer normal of x = 1 is (1, 0)

@@ -23,6 +23,13 @@
 // the (n_local x 4 n_q) x (4 n_q x (n_local+1)) product as 16x16x4 FP64 MFMA
 // tiles, the K dimension being the (point, test component) pairs. One wave
 // per cell, four cells per block sharing the reference-table chunk.
+//
+// Time-dependent problems (TCDStiff + TCDStiffSUPG, TCDMass + TCDMassSUPG,
+// TCDRhs + TCDRhsSUPG, CD_local_assembling_routines.C:9-387, SDFEM type 2)
+// have the same stiffness and rhs forms; the mass matrix u v + delta u (b.grad
+// v) is the same product with WM[q][0] = w, WM[q][1+k] = w delta beta_k on the
+// value component only. It is appended as columns 32.. of V, so one pass over
+// the reference table yields stiffness, rhs and mass.
 #include "kernels.hpp"
 #include "device.hpp"
 
@@ -127,19 +134,24 @@ __device__ inline void geo_m(int shape, const double* v, GeoM& g) {
   g.hK = sqrt(d2);
 }
 
-// DIM: 2 or 3; NC: reference components (6 or 10); NT = 1 + DIM test components
-template <int DIM, int NL>
+// DIM: 2 or 3; NC: reference components (6 or 10); NT = 1 + DIM test
+// components; TCD: also the mass matrix (columns kPad.. of V)
+template <int DIM, int NL, bool TCD>
 __global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
   constexpr int NC = DIM == 2 ? 6 : 10;
   constexpr int NT = 1 + DIM;
   constexpr int NH = NC - NT;               // second-derivative components
+  constexpr int QC = TCD ? kQC / 2 : kQC;   // points per chunk (LDS: V is twice as wide)
+  constexpr int PV = TCD ? 2 * kPad : kPad; // V width
   constexpr int NIT = (NL + 15) / 16;       // i tiles
-  constexpr int NJT = (NL + 1 + 15) / 16;   // j tiles (incl. the rhs column)
-  constexpr int KC = kQC * NT;              // K extent of one chunk
+  constexpr int NJT = TCD ? (kPad + NL + 15) / 16 : (NL + 1 + 15) / 16;  // j tiles (rhs, mass)
+  constexpr int KC = QC * NT;               // K extent of one chunk
   static_assert(KC % 4 == 0, "chunk K extent must be a multiple of 4");
-  __shared__ double s_R[kQC][NC][kPad];      // reference table chunk, zero padded
-  __shared__ double s_V[kCells][KC][kPad];   // per-cell V chunk
-  __shared__ double s_W[kCells][kQC][NT][NC + 1];  // last column: rhs coefficient
+  static_assert(NL + 1 <= kPad, "local matrix and rhs column must fit the padded width");
+  __shared__ double s_R[QC][NC][kPad];       // reference table chunk, zero padded
+  __shared__ double s_V[kCells][KC][PV];     // per-cell V chunk
+  __shared__ double s_W[kCells][QC][NT][NC + 1];  // last column: rhs coefficient
+  __shared__ double s_WM[kCells][QC][TCD ? NT : 1];  // mass weights
 
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int cell = blockIdx.x * kCells + wave;
@@ -166,17 +178,17 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
   for(int it = 0; it < NIT; ++it)
     for(int jt = 0; jt < NJT; ++jt) acc[it][jt] = v4d{0, 0, 0, 0};
 
-  for(int q0 = 0; q0 < a.nq; q0 += kQC) {
-    const int nqc = min(kQC, a.nq - q0);
+  for(int q0 = 0; q0 < a.nq; q0 += QC) {
+    const int nqc = min(QC, a.nq - q0);
     // reference table chunk (points beyond nq and columns beyond NL are zero)
-    for(int t = threadIdx.x; t < kQC * NC * kPad; t += 256) {
+    for(int t = threadIdx.x; t < QC * NC * kPad; t += 256) {
       const int qq = t / (NC * kPad), rem = t % (NC * kPad), s = rem / kPad, i = rem % kPad;
       double v = 0;
       if(qq < nqc && i < NL) v = a.ref[((size_t)(q0 + qq) * NC + s) * NL + i];
       s_R[qq][s][i] = v;
     }
     // W per point (lanes over points)
-    if(live && lane < kQC) {
+    if(live && lane < QC) {
       const int qq = lane;
       double (*W)[NC + 1] = s_W[wave][qq];
       if(qq < nqc) {
@@ -209,16 +221,22 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
           for(int t = 0; t < NH; ++t) W[1 + k][NT + t] = -w * delta * beta[k] * eps * mlap[t];
           W[1 + k][NC] = w * delta * f * beta[k];
         }
+        if constexpr(TCD) {
+          s_WM[wave][qq][0] = w;
+          for(int k = 0; k < DIM; ++k) s_WM[wave][qq][1 + k] = w * delta * beta[k];
+        }
       } else {
         for(int r = 0; r < NT; ++r)
           for(int s = 0; s <= NC; ++s) W[r][s] = 0.0;
+        if constexpr(TCD)
+          for(int r = 0; r < NT; ++r) s_WM[wave][qq][r] = 0.0;
       }
     }
     __syncthreads();
     // stage 1: V[q*NT + r][j] = sum_s W[q][r][s] R[q][s][j]; j == NL -> rhs column
     if(live) {
       const int j = lane % 32, half = lane / 32;
-      for(int qq = half; qq < kQC; qq += 2) {
+      for(int qq = half; qq < QC; qq += 2) {
         double out[NT];
         for(int r = 0; r < NT; ++r) out[r] = 0.0;
         if(j < NL) {
@@ -232,6 +250,11 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
           for(int r = 0; r < NT; ++r) out[r] = s_W[wave][qq][r][NC];
         }
         for(int r = 0; r < NT; ++r) s_V[wave][qq * NT + r][j] = out[r];
+        if constexpr(TCD) {
+          // mass columns: WM[q][r] times the ansatz value
+          const double rv = j < NL ? s_R[qq][0][j] : 0.0;
+          for(int r = 0; r < NT; ++r) s_V[wave][qq * NT + r][kPad + j] = s_WM[wave][qq][r] * rv;
+        }
       }
     }
     __syncthreads();
@@ -258,6 +281,7 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
     // D tile layout: register v of lane l holds row l/16 + 4 v, column l%16
     double* A = a.local_A + (size_t)cell * NL * NL;
     double* F = a.local_rhs + (size_t)cell * NL;
+    double* Mm = TCD ? a.local_M + (size_t)cell * NL * NL : nullptr;
     const int l16 = lane % 16, rg = lane / 16;
 #pragma unroll
     for(int it = 0; it < NIT; ++it)
@@ -269,6 +293,9 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
           if(i < NL) {
             if(j < NL) A[i * NL + j] = acc[it][jt][v];
             else if(j == NL) F[i] = acc[it][jt][v];
+            else if constexpr(TCD) {
+              if(j >= kPad && j < kPad + NL) Mm[i * NL + (j - kPad)] = acc[it][jt][v];
+            }
           }
         }
   }
@@ -297,10 +324,14 @@ bool launch_assemble_mfma(const AsmArgs& a, hipStream_t s) {
   // 12.9 vs 33.2 ms, P2 tetrahedra 1.2 vs 1.7 ms, P2 triangles 2.1 vs 3.6 ms,
   // Q2 quadrilaterals within 5%), the MFMA kernel for Q2 hexahedra (10.0 vs
   // 27.7 ms at 64 points).
-  if(a.nl < 16 || a.tcd) return false;
+  if(a.nl < 16) return false;
+  if(a.tcd && !a.local_M) return false;
   const int blocks = (a.n_cells + kCells - 1) / kCells;
   if(a.dim == 3 && a.ncomp == 10 && a.nl == 27) {
-    k_assemble_mfma<3, 27><<<blocks, 256, 0, s>>>(a);
+    if(a.tcd)
+      k_assemble_mfma<3, 27, true><<<blocks, 256, 0, s>>>(a);
+    else
+      k_assemble_mfma<3, 27, false><<<blocks, 256, 0, s>>>(a);
   } else {
     return false;
   }

@@ -10,7 +10,7 @@ import pytest
 
 import parmoon2022_amd as pm
 from test_gpu_parity import row_rel_err
-from test_oracle import TCD3D_CASES, TCD3D_SOLVERS, tcd3d_db
+from test_oracle import TCD2D_GOLDEN, TCD2D_STEPS, TCD3D_CASES, TCD3D_SOLVERS, tcd2d_db, tcd3d_db
 
 pytestmark = pytest.mark.gpu
 
@@ -123,3 +123,28 @@ def test_tcd_crank_nicolson_exact_at_larger_size():
         t = cd.current_time
         assert st.converged, (t, st.iterations)
         assert e[0] < 1e-9 * t * t and e[1] < 1e-8 * t * t, (t, e)
+
+
+@pytest.mark.parametrize("disc", ["galerkin", "supg"])
+def test_tcd2d_golden_gpu(oracle_lib, disc):
+    """test/tcd2d_test.c++:47-68 on the device: L2 and H1-semi errors after
+    steps 0-3 and 18-20 of Crank-Nicolson equal the reference's within 1e-10,
+    and every step's solution equals the oracle's."""
+    print("tcd2d golden", disc, flush=True)
+    db = tcd2d_db(disc)
+    cd = pm.TimeConvectionDiffusion(db)
+    cd.assemble_initial_time()
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    t = pm.tcd_desc(db)
+    o.tcd_init(t.scheme, t.bdf_first, t.tau, t.t0)
+    sd = pm.solver_desc(db)
+    got = {0: cd.output()}
+    for m in range(1, 21):
+        cd.step()
+        got[m] = cd.output()
+        o.tcd_step(sd)
+        u, uo = cd.solution(), o.solution()
+        assert np.max(np.abs(u - uo)) <= 1e-10 * np.max(np.abs(uo)), m
+    l2, h1 = TCD2D_GOLDEN[disc]
+    for i, m in enumerate(TCD2D_STEPS):
+        assert abs(got[m][0] - l2[i]) < 1e-10 and abs(got[m][1] - h1[i]) < 1e-10, (m, got[m][:2], l2[i], h1[i])

@@ -437,3 +437,43 @@ def test_oracle_tcd_mass_matrix_closed_forms(oracle_lib):
     s = oracle_lib.Oracle(pm.problem_desc(db))
     s.assemble()
     assert np.allclose(a, s.matrix_values(), rtol=1e-14, atol=1e-15)
+
+
+# test/tcd2d_test.c++:47-68: exp.h example (TCD2D example 0), Re = 1, Q1 on
+# the unit square refined 5 times, Crank-Nicolson, tau = 0.05 up to T = 1,
+# direct solver; L2 and H1-semi errors after steps 0, 1, 2, 3, 18, 19, 20
+# (tolerance 1e-10, tcd2d_test.c++:51)
+TCD2D_STEPS = [0, 1, 2, 3, 18, 19, 20]
+TCD2D_GOLDEN = {
+    "galerkin": ([0.0033601882323796, 0.0021831948624723, 0.0041373036975832, 0.0051536220684535,
+                  0.00084798363062268, 0.0011485020220284, 0.0017171969767027],
+                 [0.25214923985069, 0.34294475156715, 0.45320721802076, 0.56543341117647,
+                  0.13988043445394, 0.18484695459687, 0.25175783286665]),
+    "supg": ([0.0033601882323796, 0.0022877555421121, 0.0042674492028728, 0.0052925302485841,
+              0.00090066884841505, 0.0012154988855681, 0.0018006391409988],
+             [0.25214923985069, 0.34297576782859, 0.45325543222406, 0.56548475252577,
+              0.13989349915631, 0.18486434460595, 0.25178153241465]),
+}
+
+
+def tcd2d_db(disc):
+    return {"geo_file": "UnitSquare", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 5,
+            "example": "exp_sin_cos_2d", "reynolds_number": 1, "time_discretization": "crank_nicolson",
+            "time_step_length": 0.05, "time_end": 1.0, "time_start": 0.0, "solver_type": "direct",
+            "space_discretization_type": disc}
+
+
+@pytest.mark.parametrize("disc", ["galerkin", "supg"])
+def test_oracle_tcd2d_golden(oracle_lib, disc):
+    db = tcd2d_db(disc)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    t = pm.tcd_desc(db)
+    o.tcd_init(t.scheme, t.bdf_first, t.tau, t.t0)
+    got = {0: o.errors()}
+    sd = pm.solver_desc(db)
+    for m in range(1, 21):
+        o.tcd_step(sd)
+        got[m] = o.errors()
+    l2, h1 = TCD2D_GOLDEN[disc]
+    for i, m in enumerate(TCD2D_STEPS):
+        assert abs(got[m][0] - l2[i]) < 1e-10 and abs(got[m][1] - h1[i]) < 1e-10, (m, got[m][:2], l2[i], h1[i])

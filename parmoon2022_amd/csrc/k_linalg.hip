@@ -679,14 +679,16 @@ __global__ void k_csr_to_dense(int n, const int* __restrict__ rp, const int* __r
 // column j != k: t = b_kj / piv, b_kj = t, b_ij -= c_i t; column k: b_kk =
 // 1 / piv, b_ik = -c_i / piv (c = pivot column after the swap). The owner of
 // column k + 1 writes its updated column to col_out for the next step.
-__global__ __launch_bounds__(256) void k_gj_step(int n, int k, double* __restrict__ B,
+// Only rows below m take part (pivot search and update): rows m..n-1 are
+// identity rows that no step changes (see dense_inverse).
+__global__ __launch_bounds__(256) void k_gj_step(int n, int m, int k, double* __restrict__ B,
                                                  const double* __restrict__ col_in, double* __restrict__ col_out,
                                                  int* __restrict__ perm) {
   const int j = blockIdx.x * 4 + (int)threadIdx.x / 64, l = threadIdx.x % 64;
   if(j >= n) return;
   double best = -1.0;
-  int bi = n;
-  for(int i = k + l; i < n; i += 64) {
+  int bi = m;
+  for(int i = k + l; i < m; i += 64) {
     const double v = fabs(col_in[i]);
     if(v > best) { best = v; bi = i; }
   }
@@ -695,19 +697,19 @@ __global__ __launch_bounds__(256) void k_gj_step(int n, int k, double* __restric
     const int i2 = __shfl_xor(bi, off, 64);
     if(v2 > best || (v2 == best && i2 < bi)) { best = v2; bi = i2; }
   }
-  const int p = bi < n ? bi : k;  // (a column of NaNs has no pivot; stay in bounds)
+  const int p = bi < m ? bi : k;  // (a column of NaNs has no pivot; stay in bounds)
   const double ck = col_in[k], cp = col_in[p];
   const double piv = cp;
   double* a = B + (size_t)j * n;
   if(j == k) {
-    for(int i = l; i < n; i += 64) {
+    for(int i = l; i < m; i += 64) {
       const double ci = i == k ? cp : (i == p ? ck : col_in[i]);
       a[i] = i == k ? 1.0 / piv : -ci / piv;
     }
   } else {
     const double akj = a[p], apj = a[k];
     const double t = akj / piv;
-    for(int i = l; i < n; i += 64) {
+    for(int i = l; i < m; i += 64) {
       double ai;
       if(i == k) {
         ai = t;
@@ -720,6 +722,18 @@ __global__ __launch_bounds__(256) void k_gj_step(int n, int k, double* __restric
     }
   }
   if(j == 0 && l == 0) perm[k] = p;
+}
+
+// Gauss-Jordan steps on the pivots m..n-1 of a matrix whose rows m..n-1 are
+// identity rows: the pivot row is e_k, so no row interchange and no other
+// column changes; column k becomes -c (1 on the diagonal). perm[k] = k.
+__global__ void k_gj_identity_steps(int n, int m, double* __restrict__ B, int* __restrict__ perm) {
+  const size_t cnt = (size_t)(n - m) * m;
+  for(size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < cnt; t += (size_t)gridDim.x * blockDim.x) {
+    const size_t j = m + t / m, i = t % m;
+    B[j * n + i] = -B[j * n + i];
+  }
+  for(int k = m + blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) perm[k] = k;
 }
 
 // column order of the inverse: the row interchanges undone as column
@@ -789,13 +803,22 @@ void launch_csr_to_dense(int n, const int* rp, const int* col, const double* val
   k_csr_to_dense<<<n, 64, 0, s>>>(n, rp, col, val, M);
 }
 
-void dense_inverse(int n, double* B, double* inv, double* colbuf, int* perm, int* colmap, hipStream_t s) {
+void dense_inverse(int n, int m, double* B, double* inv, double* colbuf, int* perm, int* colmap,
+                   hipStream_t s) {
   if(!n) return;
+  PMCD_CHECK(m >= 0 && m <= n, "dense_inverse: bad leading block ", m, " of ", n);
+  // rows m..n-1 are identity rows (Dirichlet rows of a level matrix): their
+  // steps come first and only negate the leading part of their columns; the
+  // remaining steps 0..m-1 then change rows 0..m-1 only. The inverse is the
+  // same as with all n steps in order.
+  if(m < n)
+    k_gj_identity_steps<<<(int)std::max(1L, std::min(((long)(n - m) * std::max(m, 1) + 255) / 256, 4096L)), 256, 0,
+                          s>>>(n, m, B, perm);
   launch_copy(n, B, colbuf, s);  // column 0 = the first pivot column
   const int blocks = (n + 3) / 4;
-  for(int k = 0; k < n; ++k)
-    k_gj_step<<<blocks, 256, 0, s>>>(n, k, B, colbuf + (size_t)(k & 1) * n, colbuf + (size_t)((k + 1) & 1) * n,
-                                     perm);
+  for(int k = 0; k < m; ++k)
+    k_gj_step<<<blocks, 256, 0, s>>>(n, m, k, B, colbuf + (size_t)(k & 1) * n,
+                                     colbuf + (size_t)((k + 1) & 1) * n, perm);
   k_gj_colmap<<<1, 64, 0, s>>>(n, perm, colmap);
   const long nn = (long)n * n;
   k_gj_store<<<(int)std::max(1L, std::min((nn + 255) / 256, 8192L)), 256, 0, s>>>(n, B, colmap, inv);

@@ -180,7 +180,10 @@ void launch_csr_to_dense(int n, const int* rp, const int* col, const double* val
 // Gauss-Jordan inverse with partial pivoting, one launch per pivot: B
 // (column-major, overwritten) -> inv (row-major); colbuf holds 2 n doubles,
 // perm and colmap n ints each
-void dense_inverse(int n, double* B, double* inv, double* colbuf, int* perm, int* colmap, hipStream_t s);
+// inverse of the column-major n x n matrix B (overwritten) into row-major inv;
+// rows m..n-1 of B must be identity rows (m = n: a general matrix)
+void dense_inverse(int n, int m, double* B, double* inv, double* colbuf, int* perm, int* colmap,
+                   hipStream_t s);
 
 // time-dependent system matrix: S = factor A + M on active rows, A on the
 // Dirichlet rows (scale_blocks_actives + add_matrix_actives)

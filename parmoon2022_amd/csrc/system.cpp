@@ -389,7 +389,8 @@ void CDSystem::coarse_factorize() {
   DevBuf<int> perm(n), colmap(n);
   if(coarse_inv_.n != (size_t)n * n) coarse_inv_.alloc((size_t)n * n);
   launch_csr_to_dense(n, C.row_ptr.p, C.col.p, C.val.p, A.p, stream_);
-  dense_inverse(n, A.p, coarse_inv_.p, colbuf.p, perm.p, colmap.p, stream_);
+  // rows n_active.. of a level matrix are identity rows (Dirichlet rows)
+  dense_inverse(n, C.n_active(), A.p, coarse_inv_.p, colbuf.p, perm.p, colmap.p, stream_);
   if(coarse_tmp_.n != 2 * (size_t)n) coarse_tmp_.alloc(2 * (size_t)n);
   PMCD_HIP(hipStreamSynchronize(stream_));
   coarse_ready_ = true;

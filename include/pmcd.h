@@ -43,7 +43,9 @@ enum {
   PMCD_EX_CONST_RHS = 6,
   PMCD_EX_TCD_LINEAR_3D = 7, /* include/Examples/TCD_3D/linear_space_time.h */
   PMCD_EX_TCD_QUAD_3D = 8,   /* include/Examples/TCD_3D/quadratic_space_time.h */
-  PMCD_EX_TCD_EXP_2D = 9     /* include/Examples/TCD_2D/exp.h; eps = 1 / Reynolds number */
+  PMCD_EX_TCD_EXP_2D = 9,    /* include/Examples/TCD_2D/exp.h; eps = 1 / Reynolds number */
+  PMCD_EX_SHARP_LAYER_2D = 10, /* include/Examples/CD_2D/SharpBoundaryLayer.h */
+  PMCD_EX_SMOOTH_3D = 11     /* include/Examples/CD_3D/smooth_solution_3d.h; eps = diffusion_coefficient */
 };
 
 typedef struct {
@@ -290,6 +292,21 @@ typedef struct {
 } pmcd_tcd_desc;
 int pmcd_tcd_init(pmcd_system* s, const pmcd_tcd_desc* t);
 int pmcd_tcd_step(pmcd_system* s, const pmcd_solver_desc* sd, pmcd_solve_stats* stats, double* time);
+
+/* ---- algebraic flux correction ---------------------------------------------
+ * ConvectionDiffusion_AFC<d>::assemble(0); solve(0); solve(k) until it
+ * returns true (src/System/ConvectionDiffusion_AFC.C:132-504, the loop of
+ * test/cd2d_afc_test.c++:60-74) with afc_limiter = kuzmin,
+ * afc_iteration_scheme = fixed_point_rhs, dynamic damping, no Anderson
+ * acceleration, afc_initial_iterate = afc_zero. Single level, Galerkin.
+ * prm[11]: afc_nonlinloop_epsilon, afc_nonlinloop_maxit,
+ * afc_nonlinloop_damping_factor, _max, _min, _max_global, _increase,
+ * _decrease, _max_increase, _max_decrease, _min_tol.
+ * out[4]: nonlinear iterations, final residual norm, rejected steps, initial
+ * residual norm. stats: linear iterations summed over the loop. The solution
+ * is left in the system (pmcd_get_solution, pmcd_errors). */
+int pmcd_afc_solve(pmcd_system* s, const pmcd_solver_desc* sd, const double* prm, double* out,
+                   pmcd_solve_stats* stats);
 
 /* Files in the reference's formats (host arrays; no GPU use):
  * pmcd_vector_write / pmcd_vector_read = BlockVector::write_to_stream /

@@ -56,6 +56,7 @@ def lib():
         L.orc_tcd_init.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_tcd_step.restype = ctypes.c_int
         L.orc_tcd_config.restype = ctypes.c_int
+        L.orc_afc_solve.restype = ctypes.c_int
         L.orc_tcd_config.argtypes = L.orc_tcd_init.argtypes
         L.orc_ddot.restype = ctypes.c_double
         L.orc_quad.restype = ctypes.c_int
@@ -198,6 +199,21 @@ class Oracle:
         if rc == 2:
             raise RuntimeError("oracle solver diverged")
         return st, t.value
+
+    def afc_solve(self, sdesc, prm):
+        """Algebraic flux correction (Kuzmin limiter, fixed point rhs, dynamic
+        damping) from the zero initial iterate; prm as
+        parmoon2022_amd.afc_params. Returns (iterations, residual, rejected
+        steps, initial residual)."""
+        s = solver(sdesc)
+        p = np.ascontiguousarray(prm, dtype=np.float64)
+        out = np.zeros(4)
+        rc = lib().orc_afc_solve(self._p, ctypes.byref(s), _p(p), _p(out))
+        if rc == 3:
+            raise ValueError("algebraic flux correction needs a single level")
+        if rc == 2:
+            raise RuntimeError("oracle solver diverged")
+        return int(out[0]), out[1], int(out[2]), out[3]
 
     def tcd_matrices(self, level=-1):
         """(stiffness, mass) values of the level's pattern."""

@@ -650,6 +650,22 @@ static void ex_coeffs(const oexample* ex, double x, double y, double z, double* 
              1 * t * t * (x * x + y * z);
       break;
     }
+    case 11: { /* include/Examples/CD_3D/smooth_solution_3d.h: f = -eps lap u + b.grad u + c u */
+      double e[4];
+      const double lap = -2 * y * z * (1 - y) * (1 - z) - 2 * x * z * (1 - x) * (1 - z) - 2 * x * y * (1 - x) * (1 - y);
+      o[0] = ex->eps; o[1] = 1; o[2] = 1; o[3] = 1; o[4] = 1;
+      e[0] = x * y * z * (1 - x) * (1 - y) * (1 - z);
+      e[1] = y * z * (1 - x) * (1 - y) * (1 - z) - x * y * z * (1 - y) * (1 - z);
+      e[2] = x * z * (1 - x) * (1 - y) * (1 - z) - x * y * z * (1 - x) * (1 - z);
+      e[3] = x * y * (1 - x) * (1 - y) * (1 - z) - x * y * z * (1 - x) * (1 - y);
+      o[5] = -o[0] * lap;
+      o[5] += o[1] * e[1] + o[2] * e[2] + o[3] * e[3];
+      o[5] += o[4] * e[0];
+      break;
+    }
+    case 10: /* include/Examples/CD_2D/SharpBoundaryLayer.h */
+      o[0] = 0.001; o[1] = cos(M_PI / 18); o[2] = sin(M_PI / 18); o[3] = 0; o[4] = 0; o[5] = 0;
+      break;
     case 9: { /* include/Examples/TCD_2D/exp.h */
       const double q = 2 * p, et = exp(sin(q * ex->t)), s = sin(q * x) * sin(q * y);
       o[0] = ex->eps; o[1] = 1; o[2] = -1; o[3] = 0; o[4] = 1;
@@ -686,6 +702,12 @@ static int ex_exact(const oexample* ex, double x, double y, double z, double* o)
       o[0] = t * t * (x * x + y * z); o[1] = 2 * x * t * t; o[2] = z * t * t; o[3] = y * t * t;
       return 1;
     }
+    case 11:
+      o[0] = x * y * z * (1 - x) * (1 - y) * (1 - z);
+      o[1] = y * z * (1 - x) * (1 - y) * (1 - z) - x * y * z * (1 - y) * (1 - z);
+      o[2] = x * z * (1 - x) * (1 - y) * (1 - z) - x * y * z * (1 - x) * (1 - z);
+      o[3] = x * y * (1 - x) * (1 - y) * (1 - z) - x * y * z * (1 - x) * (1 - y);
+      return 1;
     case 9: {
       const double q = 2 * p, et = exp(sin(q * ex->t));
       o[0] = et * sin(q * x) * sin(q * y); o[1] = et * q * cos(q * x) * sin(q * y);
@@ -696,11 +718,16 @@ static int ex_exact(const oexample* ex, double x, double y, double z, double* o)
   }
 }
 
-static int ex_bc(const oexample* ex, int comp) { return (ex->kind == 0 && comp == 1) ? 1 : 0; }
+static int ex_bc(const oexample* ex, int comp) {
+  return ((ex->kind == 0 && comp == 1) || (ex->kind == 10 && comp == 2)) ? 1 : 0;
+}
 
 static double ex_bvalue(const oexample* ex, int comp, double x, double y, double z) {
   double e[4];
-  if(ex->kind == 9) return 0; /* exp.h BoundValue */
+  if(ex->kind == 9 || ex->kind == 11) return 0; /* exp.h, smooth_solution_3d.h BoundValue */
+  /* SharpBoundaryLayer.h BoundValue: 1 on x = 0 for parameter < 0.5 (y > 0.5),
+     0 elsewhere and on the Neumann side y = 1 */
+  if(ex->kind == 10) return (comp != 2 && fabs(x) < 1e-12 && y > 0.5) ? 1.0 : 0.0;
   if(!ex_exact(ex, x, y, z, e)) return 0;
   if(ex->kind == 0 && comp == 1) return ex->eps * (1.0 * e[1] + 0.0 * e[2]);
   return e[0];
@@ -741,6 +768,7 @@ typedef struct orc_problem {
   int supg;
   /* time-dependent problems (TimeConvectionDiffusion + TimeDiscretization) */
   int tcd, scheme, bdf_first, step;
+  int afc; /* algebraic flux correction: Dirichlet rows assembled like the others */
   double tau, time;
   double *u_old, *u_old2, *f_old;
 } orc_problem;
@@ -1280,7 +1308,7 @@ static void assemble_level(orc_problem* P, olevel* L, int c_begin, int c_end) {
     const int* DOF = &L->dof[c * nl];
     for(int mm = 0; mm < nl; ++mm) {
       const int l = DOF[mm];
-      if(l < L->nactive) {
+      if(l < L->nactive || P->afc) {
         for(int k = 0; k < nl; ++k) {
           int found = 0;
           for(int e = L->rp[l]; e < L->rp[l + 1]; ++e)
@@ -2123,6 +2151,201 @@ int orc_tcd_step(orc_problem* P, const osolver* s, ostats* st, double* time) {
   memcpy(P->u_old2, P->u_old, sizeof(double) * n);
   memcpy(P->u_old, F->sol, sizeof(double) * n);
   if(time) *time = P->time;
+  return rc;
+}
+
+/* ---- algebraic flux correction, Kuzmin limiter, fixed point rhs ----
+   ConvectionDiffusion_AFC<d>::assemble / solve / dynamic_damping
+   (src/System/ConvectionDiffusion_AFC.C:132-600, 1032-1180) and
+   AlgebraicFluxCorrection::steady_state_algorithm (non-MPI path,
+   src/AssembleRoutines/AlgebraicFluxCorrection.C:2058-2745),
+   compute_artificial_diffusion_matrix (:82-165), AFC_Compute_New_Iterate
+   (:3042-3080), correct_dirichlet_hanging_rows (:2992-3015). The Galerkin
+   matrix A is assembled with its Dirichlet rows (assemble_dirichlet_rows);
+   D = artificial diffusion, E = A + D, the solver matrix S = E with identity
+   Dirichlet rows; the scheme iterates S u = f + fbar(u) with the limited
+   fluxes fbar and the reference's dynamic damping. */
+static int transposed_pos(const olevel* L, int c, int i) {
+  int e = L->rp[c];
+  for(; e < L->rp[c + 1]; ++e)
+    if(L->col[e] == i) break;
+  return e;
+}
+
+static void afc_diffusion(const olevel* L, const double* A, double* D) {
+  memset(D, 0, sizeof(double) * L->nnz);
+  for(int i = 0; i < L->ndof; ++i)
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
+      const int j = L->col[l];
+      if(j == i) continue;
+      const double kij = A[l];
+      double kji = 0;
+      for(int ll = L->rp[j]; ll < L->rp[j + 1]; ++ll)
+        if(L->col[ll] == i) { kji = A[ll]; break; }
+      double m = -kij; /* std::min({-kij, 0.0, -kji}) keeps the first smallest */
+      if(0.0 < m) m = 0.0;
+      if(-kji < m) m = -kji;
+      D[l] = m;
+    }
+  for(int i = 0; i < L->ndof; ++i) {
+    double val = 0;
+    int ll = -1;
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
+      val += D[l];
+      if(L->col[l] == i) ll = l;
+    }
+    D[ll] = -val;
+  }
+}
+
+typedef struct {
+  double *A, *D, *E, *f, *F, *alpha, *Pp, *Pm, *Qp, *Qm, *Rp, *Rm;
+} oafc;
+
+/* rhs = f + limited fluxes of u (Kuzmin limiter); Dirichlet rows = u */
+static void afc_kuzmin_rhs(const olevel* L, oafc* W, const double* u, double* rhs) {
+  const int n = L->ndof, na = L->nactive;
+  const double* E = W->E;
+  for(int i = 0; i < n; ++i)
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) W->F[l] = W->D[l] * (u[L->col[l]] - u[i]);
+  for(int i = 0; i < n; ++i) W->Pp[i] = W->Pm[i] = W->Qp[i] = W->Qm[i] = 0;
+  for(int i = 0; i < n; ++i)
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
+      const int c = L->col[l];
+      if(c == i) continue;
+      const int jj = transposed_pos(L, c, i);
+      if(E[jj] > E[l]) continue;
+      const double F = W->F[l];
+      if(F > 0) { W->Pp[i] += F; W->Qp[c] += F; W->Qm[i] -= F; }
+      if(F < 0) { W->Pm[i] += F; W->Qp[i] -= F; W->Qm[c] += F; }
+    }
+  for(int i = 0; i < n; ++i) {
+    W->Rp[i] = W->Rm[i] = 1.0;
+    if(fabs(W->Pp[i]) > 0) { W->Rp[i] = W->Qp[i] / W->Pp[i]; if(W->Rp[i] > 1) W->Rp[i] = 1; }
+    if(fabs(W->Pm[i]) > 0) { W->Rm[i] = W->Qm[i] / W->Pm[i]; if(W->Rm[i] > 1) W->Rm[i] = 1; }
+  }
+  for(int i = na; i < n; ++i) W->Rp[i] = W->Rm[i] = 1.0;
+  for(int i = 0; i < n; ++i) {
+    int diag = i;
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
+      const int c = L->col[l];
+      if(c == i) diag = l;
+      const int jj = transposed_pos(L, c, i);
+      if(E[jj] > E[l]) continue;
+      double a = 1;
+      if(W->F[l] > 0) a = W->Rp[i];
+      if(W->F[l] < 0) a = W->Rm[i];
+      W->alpha[l] = a;
+      W->alpha[jj] = a;
+    }
+    W->alpha[diag] = 1.0;
+  }
+  memcpy(rhs, W->f, sizeof(double) * n);
+  for(int i = 0; i < n; ++i)
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
+      const int c = L->col[l];
+      if(c == i) continue;
+      const int jj = transposed_pos(L, c, i);
+      if(E[jj] > E[l]) continue;
+      rhs[i] += W->alpha[l] * W->F[l];
+      rhs[c] -= W->alpha[l] * W->F[l];
+    }
+  memcpy(rhs + na, u + na, sizeof(double) * (n - na));
+}
+
+static double afc_residual(const olevel* L, const double* u, const double* rhs, double* tmp) {
+  memcpy(tmp, rhs, sizeof(double) * L->ndof);
+  apply_scaled_add(L, u, tmp, -1.0);
+  return sqrt(ddot(L->ndof, tmp, tmp));
+}
+
+/* prm: epsilon, maxit, damping, damping_max, damping_min, damping_max_global,
+   increase, decrease, max_increase, max_decrease, min_tol.
+   out: iterations, final residual, rejected steps, initial residual */
+int orc_afc_solve(orc_problem* P, const osolver* s, const double* prm, double* out) {
+  if(P->nlev != 1) return 3;
+  olevel* L = lvl(P, -1);
+  const int n = L->ndof, na = L->nactive, nnz = L->nnz;
+  P->afc = 1;
+  assemble_level(P, L, 0, L->m.ncells);
+  P->afc = 0;
+  oafc W;
+  double* buf = calloc((size_t)6 * nnz + 7 * (size_t)n, sizeof(double));
+  W.A = buf; W.D = W.A + nnz; W.E = W.D + nnz; W.F = W.E + nnz; W.alpha = W.F + nnz;
+  double* S = W.alpha + nnz;
+  W.f = S + nnz; W.Pp = W.f + n; W.Pm = W.Pp + n; W.Qp = W.Pm + n; W.Qm = W.Qp + n; W.Rp = W.Qm + n;
+  W.Rm = W.Rp + n;
+  double* old = malloc(sizeof(double) * n);
+  double* cur_sol = malloc(sizeof(double) * n);
+  double* cur_rhs = malloc(sizeof(double) * n);
+  double* tmp = malloc(sizeof(double) * n);
+  for(int l = 0; l < nnz; ++l) W.alpha[l] = 1.0;
+  memcpy(W.A, L->val, sizeof(double) * nnz);
+  memcpy(W.f, L->rhs, sizeof(double) * n);
+  afc_diffusion(L, W.A, W.D);
+  for(int l = 0; l < nnz; ++l) W.E[l] = W.A[l] + W.D[l];
+  memcpy(S, W.E, sizeof(double) * nnz);
+  for(int i = na; i < n; ++i)
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) S[l] = L->col[l] == i ? 1.0 : 0.0;
+  memcpy(L->val, S, sizeof(double) * nnz);
+  extract_diag(L);
+  if(L->lu) { free(L->lu); L->lu = NULL; free(L->piv); L->piv = NULL; }
+  /* initial iterate: zero, Dirichlet values from the rhs */
+  memset(L->sol, 0, sizeof(double) * na);
+  memcpy(L->sol + na, W.f + na, sizeof(double) * (n - na));
+  const double eps = prm[0] * sqrt((double)n);
+  const int maxit = (int)prm[1];
+  double damp = prm[2], damp_max = prm[3];
+  const double damp_min = prm[4], max_global = prm[5], inc = prm[6], dec = prm[7], max_inc = prm[8],
+               max_dec = prm[9], min_tol = prm[10];
+  /* iteration 0 */
+  afc_kuzmin_rhs(L, &W, L->sol, L->rhs);
+  double residual_old = afc_residual(L, L->sol, L->rhs, tmp), residual = residual_old;
+  out[3] = residual_old;
+  memcpy(old, L->sol, sizeof(double) * n);
+  ostats st;
+  int rc = orc_solve(P, s, &st);
+  int k = 1, rejected = 0;
+  for(;; ++k) {
+    double omega = damp;
+    int first_damp = 1;
+    memcpy(cur_rhs, L->rhs, sizeof(double) * n);
+    memcpy(cur_sol, L->sol, sizeof(double) * n);
+    for(;;) {
+      for(int i = 0; i < n; ++i) L->sol[i] = old[i] + omega * (L->sol[i] - old[i]);
+      memcpy(L->sol + na, old + na, sizeof(double) * (n - na));
+      afc_kuzmin_rhs(L, &W, L->sol, L->rhs);
+      residual = afc_residual(L, L->sol, L->rhs, tmp);
+      if(k == 1) break;
+      if(residual < residual_old || omega <= min_tol * damp_min) {
+        if(residual < residual_old && first_damp == 1) {
+          damp_max = fmin(max_global, max_inc * damp_max);
+          omega = fmin(damp_max, inc * omega);
+        }
+        damp = omega;
+        break;
+      }
+      ++rejected;
+      memcpy(L->sol, cur_sol, sizeof(double) * n);
+      memcpy(L->rhs, cur_rhs, sizeof(double) * n);
+      omega = fmax(damp_min, omega * dec);
+      if(first_damp == 1) {
+        damp_max = fmax(damp_min, max_dec * damp_max);
+        first_damp = 0;
+      }
+      damp = omega;
+    }
+    if(residual < eps && k > 1) break;
+    if(k == maxit) break;
+    memcpy(old, L->sol, sizeof(double) * n);
+    rc = orc_solve(P, s, &st);
+    if(rc) break;
+    residual_old = residual;
+  }
+  out[0] = k;
+  out[1] = residual;
+  out[2] = rejected;
+  free(buf); free(old); free(cur_sol); free(cur_rhs); free(tmp);
   return rc;
 }
 

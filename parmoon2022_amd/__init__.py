@@ -25,7 +25,7 @@ GEOMETRIES = {"UnitSquare": 0, "TwoTriangles": 1, "TwoTriangles2": 2,
 MESH_FILE = 12
 EXAMPLES = {"sine_laplace_2d": 0, "sine_2d": 1, "sine_3d": 2, "test_p0": 3,
             "test_p1": 4, "test_p2": 5, "const_rhs": 6, "linear_space_time": 7, "quadratic_space_time": 8,
-            "exp_sin_cos_2d": 9}
+            "exp_sin_cos_2d": 9, "sharp_boundary_layer_2d": 10, "smooth_solution_3d": 11}
 METHODS = {"direct": 0, "cg": 1, "fgmres": 2, "richardson": 3, "sor": 4, "ssor": 5}
 PRECONDS = {"no_preconditioner": 0, "jacobi": 1, "multigrid": 2, "sor": 3, "ssor": 4}
 SMOOTHERS = {"jacobi": 0, "sor": 1, "ssor": 2}
@@ -65,7 +65,7 @@ EXPORTS = [
     "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
     "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_sweep_order", "pmcd_host_sell_layout",
     "pmcd_host_write_mesh", "pmcd_cycle_directions", "pmcd_vector_write", "pmcd_vector_read",
-    "pmcd_vector_write_market", "pmcd_matrix_write_market", "pmcd_tcd_init", "pmcd_tcd_step",
+    "pmcd_vector_write_market", "pmcd_matrix_write_market", "pmcd_tcd_init", "pmcd_tcd_step", "pmcd_afc_solve",
     "pmcd_jacobi_sweep",
     "pmcd_system_create", "pmcd_system_destroy", "pmcd_system_info", "pmcd_get_dofmap",
     "pmcd_get_pattern", "pmcd_get_cell_coords", "pmcd_assemble", "pmcd_get_matrix",
@@ -533,6 +533,31 @@ class TcdDesc(ctypes.Structure):
 TIME_SCHEMES = {"backward_euler": 0, "crank_nicolson": 1, "bdf_two": 2}
 
 
+AFC_PARAM_KEYS = [("afc_nonlinloop_epsilon", 1e-10), ("afc_nonlinloop_maxit", 1),
+                  ("afc_nonlinloop_damping_factor", 1.0), ("afc_nonlinloop_damping_factor_max", 1.0),
+                  ("afc_nonlinloop_damping_factor_min", 0.01), ("afc_nonlinloop_damping_factor_max_global", 1.0),
+                  ("afc_nonlinloop_damping_factor_increase", 1.1), ("afc_nonlinloop_damping_factor_decrease", 0.5),
+                  ("afc_nonlinloop_damping_factor_max_increase", 1.001),
+                  ("afc_nonlinloop_damping_factor_max_decrease", 0.9),
+                  ("afc_nonlinloop_damping_factor_min_tol", 1.001)]
+
+
+def afc_params(db):
+    """Parameters of the algebraic flux correction loop in the reference's
+    names and defaults (AlgebraicFluxCorrection::default_afc_database,
+    src/AssembleRoutines/AlgebraicFluxCorrection.C:1914-2056), as the 11
+    doubles the C ABI takes. Only the Kuzmin limiter with the fixed point rhs
+    scheme, dynamic damping, no Anderson acceleration and the zero initial
+    iterate are built."""
+    fixed = {"afc_limiter": "kuzmin", "afc_iteration_scheme": "fixed_point_rhs",
+             "afc_nonlinloop_anderson_acc": "no", "afc_initial_iterate": "afc_zero",
+             "afc_nonlinloop_damping_factor_constant": "no"}
+    for k, v in fixed.items():
+        if db.get(k, v) != v:
+            raise ValueError("%s = %s is not built (only %s)" % (k, db.get(k), v))
+    return np.array([float(db.get(k, d)) for k, d in AFC_PARAM_KEYS])
+
+
 def tcd_desc(db):
     """TcdDesc from the reference's time parameters (time_discretization,
     bdf_two_first_step, time_step_length, time_start)."""
@@ -600,6 +625,33 @@ class TimeConvectionDiffusion(ConvectionDiffusion):
         """[L2(0,t;L2), L2(0,t;H1), Linf(0,t;Linf)] as get_errors() reports them."""
         te = self.time_errors
         return np.array([np.sqrt(te[0]), np.sqrt(te[2]), te[4]])
+
+
+class ConvectionDiffusionAFC(ConvectionDiffusion):
+    """ConvectionDiffusion_AFC<d> of the reference
+    (src/System/ConvectionDiffusion_AFC.C) with the Kuzmin limiter and the
+    fixed point rhs scheme. `solve()` runs the whole nonlinear loop of the
+    reference's programs (assemble(0), solve(0), solve(k) until converged)."""
+
+    def __init__(self, db):
+        db = dict(db)
+        if db.get("space_discretization_type", "galerkin") != "galerkin":
+            raise ValueError("algebraic flux correction uses the Galerkin discretization")
+        super().__init__(db)
+        self.afc = None
+
+    def solve(self, db=None):
+        d = self.db if db is None else db
+        sd = solver_desc(d)
+        prm = np.ascontiguousarray(afc_params(d))
+        out = np.zeros(4)
+        st = SolveStats()
+        _check(lib().pmcd_afc_solve(self._s, ctypes.byref(sd), _ptr(prm, ctypes.c_double),
+                                    _ptr(out, ctypes.c_double), ctypes.byref(st)))
+        self.stats = st
+        self.afc = {"iterations": int(out[0]), "residual": out[1], "rejected": int(out[2]),
+                    "initial_residual": out[3]}
+        return st
 
 
 class Batch:

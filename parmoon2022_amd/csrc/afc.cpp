@@ -1,0 +1,121 @@
+// Algebraic flux correction on a device system: ConvectionDiffusion_AFC<d>
+// with the Kuzmin limiter and the fixed point rhs scheme
+// (src/System/ConvectionDiffusion_AFC.C:132-600, 1032-1180;
+// src/AssembleRoutines/AlgebraicFluxCorrection.C:82-165, 2058-2745,
+// 2992-3080). The matrices and the limiter run on the device; the damping
+// control (accept / reject a damped iterate by its residual norm) runs on
+// the host between launches, as in the reference's dynamic_damping.
+#include <cmath>
+
+#include "system.hpp"
+
+namespace pmcd {
+
+SolveStats CDSystem::afc_solve(const SolverDesc& s, const double* prm, double* out) {
+  PMCD_CHECK(n_levels() == 1, "algebraic flux correction needs a single level (the reference refuses multigrid)");
+  PMCD_CHECK(!desc_.supg, "algebraic flux correction uses the Galerkin discretization");
+  PMCD_CHECK(!tcd_, "algebraic flux correction is for the stationary problem");
+  Level& L = finest();
+  const int n = L.n(), na = L.n_active();
+  const size_t nnz = L.val.n;
+  if(afc_A_.n != nnz) {
+    afc_tpos_.alloc(nnz);
+    afc_A_.alloc(nnz); afc_D_.alloc(nnz); afc_E_.alloc(nnz); afc_alpha_.alloc(nnz);
+    afc_f_.alloc(n); afc_Rp_.alloc(n); afc_Rm_.alloc(n); afc_old_.alloc(n);
+    afc_cur_sol_.alloc(n); afc_cur_rhs_.alloc(n); afc_res_.alloc(n);
+    launch_afc_transpose(n, L.row_ptr.p, L.col.p, afc_tpos_.p, stream_);
+  }
+  if(kr_.n != (size_t)n) {
+    kr_.alloc(n);
+    kw_.alloc(n);
+  }
+  // Galerkin matrix with its Dirichlet rows assembled (assemble_dirichlet_rows
+  // of Assemble2D/3D), rhs with the Dirichlet values
+  boundary_data(L);
+  AsmArgs a = asm_args(L);
+  if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
+  launch_gather_matrix(n, n, a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p, L.contrib.p, L.slot.p, local_A_.p,
+                       afc_A_.p, stream_);
+  launch_gather_rhs(n, na, L.contrib_ptr.p, L.contrib.p, local_rhs_.p, L.extra_ptr.p, L.extra.p, L.dir_vals.p,
+                    afc_f_.p, stream_);
+  // D, E = A + D, and the solver matrix S = E with identity Dirichlet rows
+  launch_afc_matrices(n, na, L.row_ptr.p, L.col.p, afc_tpos_.p, afc_A_.p, afc_D_.p, afc_E_.p, L.val.p, stream_);
+  launch_gather_idx(n, L.diag_pos.p, L.val.p, L.diag.p, stream_);
+  L.sw_fresh = L.sw_fresh32 = L.sell_fresh = L.sell_fresh32 = false;
+  coarse_ready_ = false;
+  mixed_ready_ = false;
+  launch_fill((int)nnz, 1.0, afc_alpha_.p, stream_);
+  // initial iterate: zero, Dirichlet values from the rhs
+  launch_fill(na, 0.0, L.sol.p, stream_);
+  launch_copy(n - na, afc_f_.p + na, L.sol.p + na, stream_);
+
+  auto limited_rhs = [&]() {
+    launch_afc_rhs(n, na, L.row_ptr.p, L.col.p, afc_tpos_.p, afc_D_.p, afc_E_.p, L.sol.p, afc_f_.p, afc_Rp_.p,
+                   afc_Rm_.p, afc_alpha_.p, L.rhs.p, stream_);
+  };
+  auto residual_norm = [&]() {
+    launch_spmv(n, na, L.row_ptr.p, L.col.p, L.val.p, L.sol.p, L.rhs.p, afc_res_.p, -1.0, 1.0, stream_,
+                L.st.max_row);
+    return std::sqrt(dot(n, afc_res_.p, afc_res_.p));
+  };
+
+  const double eps = prm[0] * std::sqrt((double)n);
+  const int maxit = (int)prm[1];
+  double damp = prm[2], damp_max = prm[3];
+  const double damp_min = prm[4], max_global = prm[5], inc = prm[6], dec = prm[7], max_inc = prm[8],
+               max_dec = prm[9], min_tol = prm[10];
+  SolveStats total;
+  // iteration 0: fluxes of the initial iterate, residual, first solve
+  limited_rhs();
+  double residual_old = residual_norm(), residual = residual_old;
+  out[3] = residual_old;
+  launch_copy(n, L.sol.p, afc_old_.p, stream_);
+  SolveStats st = solve(s);
+  total.iterations += st.iterations;
+  int k = 1, rejected = 0;
+  for(;; ++k) {
+    double omega = damp;
+    bool first_damp = true;
+    launch_copy(n, L.rhs.p, afc_cur_rhs_.p, stream_);
+    launch_copy(n, L.sol.p, afc_cur_sol_.p, stream_);
+    for(;;) {
+      launch_afc_damp(n, na, omega, afc_old_.p, L.sol.p, stream_);
+      limited_rhs();
+      residual = residual_norm();
+      if(k == 1) break;
+      if(residual < residual_old || omega <= min_tol * damp_min) {
+        if(residual < residual_old && first_damp) {
+          damp_max = std::min(max_global, max_inc * damp_max);
+          omega = std::min(damp_max, inc * omega);
+        }
+        damp = omega;
+        break;
+      }
+      ++rejected;
+      launch_copy(n, afc_cur_sol_.p, L.sol.p, stream_);
+      launch_copy(n, afc_cur_rhs_.p, L.rhs.p, stream_);
+      omega = std::max(damp_min, omega * dec);
+      if(first_damp) {
+        damp_max = std::max(damp_min, max_dec * damp_max);
+        first_damp = false;
+      }
+      damp = omega;
+    }
+    if(residual < eps && k > 1) break;
+    if(k == maxit) break;
+    launch_copy(n, L.sol.p, afc_old_.p, stream_);
+    st = solve(s);
+    total.iterations += st.iterations;
+    residual_old = residual;
+  }
+  PMCD_HIP(hipStreamSynchronize(stream_));
+  out[0] = k;
+  out[1] = residual;
+  out[2] = rejected;
+  total.converged = residual < eps ? 1 : 0;
+  total.initial_residual = out[3];
+  total.final_residual = residual;
+  return total;
+}
+
+}  // namespace pmcd

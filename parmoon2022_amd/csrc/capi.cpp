@@ -578,6 +578,22 @@ int pmcd_tcd_step(pmcd_system* s, const pmcd_solver_desc* sd, pmcd_solve_stats* 
   });
 }
 
+int pmcd_afc_solve(pmcd_system* s, const pmcd_solver_desc* sd, const double* prm, double* out,
+                   pmcd_solve_stats* stats) {
+  return guarded([&] {
+    PMCD_CHECK(s && sd && prm && out, "null argument");
+    PMCD_CHECK(prm[0] > 0 && prm[1] >= 1, "afc_nonlinloop_epsilon must be positive, afc_nonlinloop_maxit >= 1");
+    const SolveStats st = s->sys->afc_solve(to_solver(*sd), prm, out);
+    if(stats) {
+      stats->iterations = st.iterations;
+      stats->converged = st.converged;
+      stats->initial_residual = st.initial_residual;
+      stats->final_residual = st.final_residual;
+      stats->seconds = st.seconds;
+    }
+  });
+}
+
 int pmcd_vector_write(const char* path, long long n, const double* v) {
   return guarded([&] {
     PMCD_CHECK(path && (v || n == 0) && n >= 0, "invalid argument");

@@ -45,6 +45,14 @@ enum ExampleKind : int {
   // exp(sin(2 pi t)) sin(2 pi x) sin(2 pi y), eps (1 / Reynolds number),
   // b = (1, -1), c = 1, homogeneous Dirichlet
   EX_TCD_EXP_2D = 9,
+  // include/Examples/CD_2D/SharpBoundaryLayer.h (reference 2D example 3):
+  // eps = 1e-3, b = (cos 10 deg, sin 10 deg), c = f = 0; Neumann (0) on
+  // component 2 (y = 1), Dirichlet 1 on x = 0 above y = 0.5, else 0; no known
+  // solution (errors are norms of the discrete solution)
+  EX_SHARP_LAYER_2D = 10,
+  // include/Examples/CD_3D/smooth_solution_3d.h (reference 3D example 3):
+  // u = x y z (1-x) (1-y) (1-z), eps, b = (1, 1, 1), c = 1, Dirichlet 0
+  EX_SMOOTH_3D = 11,
 };
 
 struct ExampleParams {
@@ -56,6 +64,15 @@ struct ExampleParams {
   double f0;
   double t;  // time of the time-dependent examples
 };
+
+// u = x y z (1-x) (1-y) (1-z): value, gradient, Laplacian
+__host__ __device__ inline void example_smooth3d(double x, double y, double z, double* e) {
+  e[0] = x * y * z * (1 - x) * (1 - y) * (1 - z);
+  e[1] = y * z * (1 - x) * (1 - y) * (1 - z) - x * y * z * (1 - y) * (1 - z);
+  e[2] = x * z * (1 - x) * (1 - y) * (1 - z) - x * y * z * (1 - x) * (1 - z);
+  e[3] = x * y * (1 - x) * (1 - y) * (1 - z) - x * y * z * (1 - x) * (1 - y);
+  e[4] = -2 * y * z * (1 - y) * (1 - z) - 2 * x * z * (1 - x) * (1 - z) - 2 * x * y * (1 - x) * (1 - y);
+}
 
 // coefficients at a point: out = {eps, b1, b2, b3, c, f}
 __host__ __device__ inline void example_coeffs(const ExampleParams& ex, double x, double y,
@@ -111,6 +128,19 @@ __host__ __device__ inline void example_coeffs(const ExampleParams& ex, double x
       out[0] = eps; out[1] = 4; out[2] = 3; out[3] = 2; out[4] = 1;
       out[5] = 2 * t * (x * x + y * z) - eps * 2 * t * t + 4 * 2 * x * t * t + 3 * z * t * t + 2 * y * t * t +
                1 * t * t * (x * x + y * z);
+      break;
+    }
+    case EX_SHARP_LAYER_2D:
+      out[0] = 0.001; out[1] = cos(p / 18); out[2] = sin(p / 18); out[3] = 0; out[4] = 0; out[5] = 0;
+      break;
+    case EX_SMOOTH_3D: {
+      // smooth_solution_3d.h BilinearCoeffs: f = -eps lap u + b . grad u + c u
+      double e[5];
+      example_smooth3d(x, y, z, e);
+      out[0] = ex.eps; out[1] = 1; out[2] = 1; out[3] = 1; out[4] = 1;
+      out[5] = -out[0] * e[4];
+      out[5] += out[1] * e[1] + out[2] * e[2] + out[3] * e[3];
+      out[5] += out[4] * e[0];
       break;
     }
     case EX_TCD_EXP_2D: {
@@ -170,6 +200,9 @@ __host__ __device__ inline bool example_exact(const ExampleParams& ex, double x,
       out[4] = 2 * t * t;
       return true;
     }
+    case EX_SMOOTH_3D:
+      example_smooth3d(x, y, z, out);
+      return true;
     case EX_TCD_EXP_2D: {
       const double q = 2 * p, et = exp(sin(q * ex.t));
       out[0] = et * sin(q * x) * sin(q * y);
@@ -188,6 +221,7 @@ __host__ __device__ inline bool example_exact(const ExampleParams& ex, double x,
 // boundary condition kind (0 Dirichlet, 1 Neumann) of component `comp`
 __host__ __device__ inline int example_bc(const ExampleParams& ex, int comp) {
   if(ex.kind == EX_SINE_LAPLACE_2D && comp == 1) return 1;
+  if(ex.kind == EX_SHARP_LAYER_2D && comp == 2) return 1;
   return 0;
 }
 
@@ -196,7 +230,10 @@ __host__ __device__ inline int example_bc(const ExampleParams& ex, int comp) {
 __host__ __device__ inline double example_bvalue(const ExampleParams& ex, int comp, double x,
                                                  double y, double z) {
   double e[5];
-  if(ex.kind == EX_TCD_EXP_2D) return 0.0;  // exp.h BoundValue
+  if(ex.kind == EX_TCD_EXP_2D || ex.kind == EX_SMOOTH_3D) return 0.0;  // BoundValue = 0
+  // SharpBoundaryLayer.h BoundValue: 1 on x = 0 for boundary parameter < 0.5
+  // (y > 0.5), 0 elsewhere and on the Neumann side
+  if(ex.kind == EX_SHARP_LAYER_2D) return (comp != 2 && fabs(x) < 1e-12 && y > 0.5) ? 1.0 : 0.0;
   if(!example_exact(ex, x, y, z, e)) return 0.0;
   if(ex.kind == EX_SINE_LAPLACE_2D && comp == 1) {
     // outer normal of x = 1 is (1, 0)

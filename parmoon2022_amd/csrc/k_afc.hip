@@ -1,0 +1,214 @@
+// Algebraic flux correction kernels (Kuzmin limiter, fixed point rhs):
+// AlgebraicFluxCorrection::steady_state_algorithm, non-MPI path
+// (src/AssembleRoutines/AlgebraicFluxCorrection.C:2058-2745) and
+// compute_artificial_diffusion_matrix (:82-165).
+//
+// The reference walks the rows in order and scatters: while row i handles
+// its link to column c (only when E_ci <= E_ic, E = A + D), it adds to
+// P_i, Q_i and Q_c, writes alpha for both (i,c) and (c,i), and adds to
+// rhs_i and rhs_c. Here one thread per row gathers everything row i
+// receives, in the order the sequential loop adds it: first from rows
+// c < i, then its own link terms in entry order, then from rows c > i
+// (columns are sorted ascending). The flux of (c,i) is -F_ic exactly (D is
+// symmetric entry by entry), and an alpha entry written from both sides
+// keeps the value of the larger row index, as in the sequential loop.
+#include "kernels.hpp"
+#include "device.hpp"
+
+namespace pmcd {
+
+namespace {
+
+// position of (c, i) in row c; row_ptr[c+1] if absent (sorted columns)
+__device__ inline int find_in_row(const int* __restrict__ rp, const int* __restrict__ col, int c, int i) {
+  int lo = rp[c], hi = rp[c + 1];
+  const int end = hi;
+  while(lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if(col[mid] < i) lo = mid + 1;
+    else hi = mid;
+  }
+  return (lo < end && col[lo] == i) ? lo : end;
+}
+
+__global__ void k_afc_transpose(int n, const int* __restrict__ rp, const int* __restrict__ col,
+                                int* __restrict__ tpos) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  for(int j = rp[i]; j < rp[i + 1]; ++j) tpos[j] = find_in_row(rp, col, col[j], i);
+}
+
+// D_ic = min(-a_ic, 0, -a_ci) off the diagonal, D_ii = -(row sum);
+// E = A + D; S = E with identity Dirichlet rows (rows >= n_active)
+__global__ void k_afc_matrices(int n, int n_active, const int* __restrict__ rp, const int* __restrict__ col,
+                               const int* __restrict__ tpos, const double* __restrict__ A,
+                               double* __restrict__ D, double* __restrict__ E, double* __restrict__ S) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  double sum = 0.0;
+  int diag = -1;
+  for(int j = rp[i]; j < rp[i + 1]; ++j) {
+    const int c = col[j];
+    double d = 0.0;
+    if(c != i) {
+      const int t = tpos[j];
+      const double kij = A[j], kji = t < rp[c + 1] ? A[t] : 0.0;
+      d = -kij;  // std::min({-kij, 0.0, -kji}): the first smallest
+      if(0.0 < d) d = 0.0;
+      if(-kji < d) d = -kji;
+    } else {
+      diag = j;
+    }
+    D[j] = d;
+    sum += d;
+  }
+  if(diag >= 0) D[diag] = -sum;
+  for(int j = rp[i]; j < rp[i + 1]; ++j) {
+    const double e = A[j] + D[j];
+    E[j] = e;
+    S[j] = i < n_active ? e : (col[j] == i ? 1.0 : 0.0);
+  }
+}
+
+// upwind test of the link (i, c) seen from row i: row i handles it unless
+// E_ci > E_ic
+__device__ inline bool handles(const double* __restrict__ E, int j, int t) { return !(E[t] > E[j]); }
+
+// limiter factors R+ and R- of row i
+__global__ void k_afc_limiter(int n, int n_active, const int* __restrict__ rp, const int* __restrict__ col,
+                              const int* __restrict__ tpos, const double* __restrict__ D,
+                              const double* __restrict__ E, const double* __restrict__ u,
+                              double* __restrict__ Rp, double* __restrict__ Rm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const int b = rp[i], e = rp[i + 1];
+  const double ui = u[i];
+  double Pp = 0, Pm = 0, Qp = 0, Qm = 0;
+  // from rows c < i: their link (c, i) with flux -F_ic
+  for(int j = b; j < e && col[j] < i; ++j) {
+    const int t = tpos[j];
+    if(t >= rp[col[j] + 1] || !handles(E, t, j)) continue;
+    const double Fc = D[t] * (ui - u[col[j]]);
+    if(Fc > 0) Qp += Fc;
+    if(Fc < 0) Qm += Fc;
+  }
+  // own links
+  for(int j = b; j < e; ++j) {
+    const int c = col[j];
+    if(c == i) continue;
+    const int t = tpos[j];
+    if(t < rp[c + 1] && !handles(E, j, t)) continue;
+    const double F = D[j] * (u[c] - ui);
+    if(F > 0) { Pp += F; Qm -= F; }
+    if(F < 0) { Pm += F; Qp -= F; }
+  }
+  // from rows c > i
+  for(int j = b; j < e; ++j) {
+    if(col[j] <= i) continue;
+    const int t = tpos[j];
+    if(t >= rp[col[j] + 1] || !handles(E, t, j)) continue;
+    const double Fc = D[t] * (ui - u[col[j]]);
+    if(Fc > 0) Qp += Fc;
+    if(Fc < 0) Qm += Fc;
+  }
+  double rp_ = 1.0, rm_ = 1.0;
+  if(fabs(Pp) > 0) { rp_ = Qp / Pp; if(rp_ > 1) rp_ = 1; }
+  if(fabs(Pm) > 0) { rm_ = Qm / Pm; if(rm_ > 1) rm_ = 1; }
+  if(i >= n_active) rp_ = rm_ = 1.0;
+  Rp[i] = rp_;
+  Rm[i] = rm_;
+}
+
+// alpha of entry j = (i, c): the value of the last row that handles the link
+__device__ inline double link_alpha(int i, int c, bool from_i, bool from_c, double F, const double* Rp,
+                                    const double* Rm) {
+  const bool use_c = from_c && (!from_i || c > i);
+  double a = 1.0;
+  if(use_c) {
+    const double Fc = -F;
+    if(Fc > 0) a = Rp[c];
+    if(Fc < 0) a = Rm[c];
+  } else {
+    if(F > 0) a = Rp[i];
+    if(F < 0) a = Rm[i];
+  }
+  return a;
+}
+
+// rhs_i = f_i + limited fluxes (Dirichlet rows: u_i); alpha per entry
+__global__ void k_afc_rhs(int n, int n_active, const int* __restrict__ rp, const int* __restrict__ col,
+                          const int* __restrict__ tpos, const double* __restrict__ D,
+                          const double* __restrict__ E, const double* __restrict__ u,
+                          const double* __restrict__ Rp, const double* __restrict__ Rm,
+                          const double* __restrict__ f, double* __restrict__ alpha, double* __restrict__ rhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const int b = rp[i], e = rp[i + 1];
+  const double ui = u[i];
+  double r = f[i];
+  // pass 1: rows c < i subtract alpha (-F_ic); pass 2: own links add
+  // alpha F_ic; pass 3: rows c > i
+  for(int pass = 0; pass < 3; ++pass)
+    for(int j = b; j < e; ++j) {
+      const int c = col[j];
+      if(c == i) {
+        if(pass == 0) alpha[j] = 1.0;
+        continue;
+      }
+      const int t = tpos[j];
+      const bool has_t = t < rp[c + 1];
+      const bool from_i = !has_t || handles(E, j, t);
+      const bool from_c = has_t && handles(E, t, j);
+      const double F = D[j] * (u[c] - ui);
+      const double a = link_alpha(i, c, from_i, from_c, F, Rp, Rm);
+      if(pass == 0) {
+        alpha[j] = (from_i || from_c) ? a : alpha[j];
+        if(c < i && from_c) r -= a * (D[t] * (ui - u[c]));
+      } else if(pass == 1) {
+        if(from_i) r += a * F;
+      } else if(c > i && from_c) {
+        r -= a * (D[t] * (ui - u[c]));
+      }
+    }
+  rhs[i] = i < n_active ? r : ui;
+}
+
+// u = old + omega (u - old) on active rows, u = old on Dirichlet rows
+__global__ void k_afc_damp(int n, int n_active, double omega, const double* __restrict__ old,
+                           double* __restrict__ u) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const double o = old[i];
+  u[i] = i < n_active ? (u[i] - o) * omega + o : o;
+}
+
+inline int blocks_for(int n) { return (n + 255) / 256; }
+
+}  // namespace
+
+void launch_afc_transpose(int n, const int* rp, const int* col, int* tpos, hipStream_t s) {
+  if(n) k_afc_transpose<<<blocks_for(n), 256, 0, s>>>(n, rp, col, tpos);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_afc_matrices(int n, int n_active, const int* rp, const int* col, const int* tpos, const double* A,
+                         double* D, double* E, double* S, hipStream_t s) {
+  if(n) k_afc_matrices<<<blocks_for(n), 256, 0, s>>>(n, n_active, rp, col, tpos, A, D, E, S);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_afc_rhs(int n, int n_active, const int* rp, const int* col, const int* tpos, const double* D,
+                    const double* E, const double* u, const double* f, double* Rp, double* Rm, double* alpha,
+                    double* rhs, hipStream_t s) {
+  if(!n) return;
+  k_afc_limiter<<<blocks_for(n), 256, 0, s>>>(n, n_active, rp, col, tpos, D, E, u, Rp, Rm);
+  k_afc_rhs<<<blocks_for(n), 256, 0, s>>>(n, n_active, rp, col, tpos, D, E, u, Rp, Rm, f, alpha, rhs);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_afc_damp(int n, int n_active, double omega, const double* old, double* u, hipStream_t s) {
+  if(n) k_afc_damp<<<blocks_for(n), 256, 0, s>>>(n, n_active, omega, old, u);
+  PMCD_HIP(hipGetLastError());
+}
+
+}  // namespace pmcd

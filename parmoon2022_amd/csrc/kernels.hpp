@@ -187,6 +187,14 @@ void dense_inverse(int n, int m, double* B, double* inv, double* colbuf, int* pe
 
 // time-dependent system matrix: S = factor A + M on active rows, A on the
 // Dirichlet rows (scale_blocks_actives + add_matrix_actives)
+// algebraic flux correction (k_afc.hip)
+void launch_afc_transpose(int n, const int* rp, const int* col, int* tpos, hipStream_t s);
+void launch_afc_matrices(int n, int n_active, const int* rp, const int* col, const int* tpos, const double* A,
+                         double* D, double* E, double* S, hipStream_t s);
+void launch_afc_rhs(int n, int n_active, const int* rp, const int* col, const int* tpos, const double* D,
+                    const double* E, const double* u, const double* f, double* Rp, double* Rm, double* alpha,
+                    double* rhs, hipStream_t s);
+void launch_afc_damp(int n, int n_active, double omega, const double* old, double* u, hipStream_t s);
 void launch_tcd_system(int n, int n_active, const int* rp, const double* A, const double* M, double factor,
                        double* S, hipStream_t s);
 

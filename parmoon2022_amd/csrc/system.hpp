@@ -205,6 +205,16 @@ class CDSystem {
   double tcd_time() const { return tcd_t_; }
   int tcd_n_steps() const { return tcd_n_; }
 
+  // algebraic flux correction (ConvectionDiffusion_AFC<d>,
+  // src/System/ConvectionDiffusion_AFC.C): assemble the Galerkin matrix
+  // with its Dirichlet rows, add the artificial diffusion D, and iterate the
+  // fixed point rhs scheme with the Kuzmin limiter and dynamic damping from
+  // the zero initial iterate. prm: epsilon, maxit, damping factor, its max,
+  // min, global max, increase, decrease, max increase, max decrease, min
+  // tolerance. out: iterations, final residual, rejected steps, initial
+  // residual. Single level only.
+  SolveStats afc_solve(const SolverDesc& s, const double* prm, double* out);
+
  private:
   void build_level_device(Level& L);
   void boundary_data(Level& L);
@@ -263,6 +273,10 @@ class CDSystem {
   double tcd_t_ = 0;
   int tcd_n_ = 0;
   DevBuf<double> local_M_, tcd_f_old_, tcd_u_old_, tcd_u_old2_;
+  // algebraic flux correction state
+  DevBuf<int> afc_tpos_;
+  DevBuf<double> afc_A_, afc_D_, afc_E_, afc_alpha_, afc_f_, afc_Rp_, afc_Rm_, afc_old_, afc_cur_sol_,
+      afc_cur_rhs_, afc_res_;
   // Krylov work space
   std::vector<DevBuf<double>> kv_, kz_;
   DevBuf<double> kr_, kw_, kh_;

@@ -1,0 +1,76 @@
+"""Algebraic flux correction (Kuzmin limiter, fixed point rhs, dynamic
+damping) on the device against the reference's cd2d/cd3d AFC golden errors
+and against the oracle's iterates.
+
+Tolerances: the golden errors within the reference's own bounds (1e-8 in 2D,
+1e-11 in 3D); the converged solution within 1e-10 relative of the oracle's,
+with the same number of nonlinear iterations and rejected steps.
+"""
+import numpy as np
+import pytest
+
+import parmoon2022_amd as pm
+from test_oracle import AFC_GOLDEN, afc_db, afc_errors
+
+pytestmark = pytest.mark.gpu
+
+
+def run_gpu(db):
+    cd = pm.ConvectionDiffusionAFC(db)
+    cd.solve()
+    return cd
+
+
+@pytest.mark.parametrize("geo,example", list(AFC_GOLDEN))
+def test_afc_golden_gpu(oracle_lib, geo, example):
+    print("afc golden", geo, example, flush=True)
+    db = afc_db(geo, example)
+    cd = run_gpu(db)
+    gold, tol = AFC_GOLDEN[(geo, example)]
+    got = afc_errors(cd.output(), len(gold))
+    assert np.allclose(got, gold, rtol=0, atol=tol), (got, gold)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    it, res, rejected, res0 = o.afc_solve(pm.solver_desc(db), pm.afc_params(db))
+    assert cd.afc["iterations"] == it and cd.afc["rejected"] == rejected, (cd.afc, it, rejected)
+    assert abs(cd.afc["initial_residual"] - res0) <= 1e-12 * res0
+    u, uo = cd.solution(), o.solution()
+    assert np.max(np.abs(u - uo)) <= 1e-10 * np.max(np.abs(uo))
+
+
+@pytest.mark.parametrize("geo,example,n_refine,solver", [
+    ("TwoTriangles", "sharp_boundary_layer_2d", 5, "direct"),
+    ("UnitSquare", "sharp_boundary_layer_2d", 5, "fgmres"),
+    ("Default_UnitCube_Hexa", "smooth_solution_3d", 3, "fgmres"),
+])
+def test_afc_matches_oracle(oracle_lib, geo, example, n_refine, solver):
+    """Finer meshes, and FGMRES with Jacobi preconditioning for the linear
+    systems (S = A + D is fixed over the loop), converged to epsilon 1e-13 so
+    that rounding in the linear solves cannot move the stopping step: the
+    same fixed point as the oracle's, reached in about as many iterations."""
+    print("afc vs oracle", geo, example, n_refine, solver, flush=True)
+    extra = {"afc_nonlinloop_epsilon": 1e-13}
+    if solver == "fgmres":
+        extra.update({"solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "jacobi",
+                      "residual_tolerance": 1e-15, "residual_reduction": 0.0, "max_n_iterations": 2000,
+                      "gmres_restart": 50})
+    db = afc_db(geo, example, n_refine=n_refine, **extra)
+    cd = run_gpu(db)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    it, res, rejected, res0 = o.afc_solve(pm.solver_desc(db), pm.afc_params(db))
+    u, uo = cd.solution(), o.solution()
+    assert abs(cd.afc["iterations"] - it) <= 2, (cd.afc, it)
+    assert np.max(np.abs(u - uo)) <= 1e-10 * np.max(np.abs(uo))
+
+
+def test_afc_bounds_at_larger_size():
+    """TwoTriangles refined 8 times (66,049 DoF), FGMRES: the loop converges
+    and the solution of the sharp boundary layer stays in [0, 1]."""
+    print("afc bounds 8 refinements", flush=True)
+    db = afc_db("TwoTriangles", "sharp_boundary_layer_2d", n_refine=8, solver_type="iterative",
+                iterative_solver_type="fgmres", preconditioner="jacobi", residual_tolerance=1e-13,
+                residual_reduction=0.0, max_n_iterations=5000, gmres_restart=50)
+    cd = run_gpu(db)
+    u = cd.solution()
+    n = cd.info()["n_dof"]
+    assert cd.afc["residual"] < 1e-10 * np.sqrt(n) and cd.afc["iterations"] < 1000, cd.afc
+    assert u.min() >= -1e-10 and u.max() <= 1 + 1e-10, (u.min(), u.max())

@@ -477,3 +477,57 @@ def test_oracle_tcd2d_golden(oracle_lib, disc):
     l2, h1 = TCD2D_GOLDEN[disc]
     for i, m in enumerate(TCD2D_STEPS):
         assert abs(got[m][0] - l2[i]) < 1e-10 and abs(got[m][1] - h1[i]) < 1e-10, (m, got[m][:2], l2[i], h1[i])
+
+
+# test/cd2d_afc_test.c++:36-66,118-131,217-233 (tolerance 1e-8) and
+# test/cd3d_afc_test.c++:37-51,93-101 (tolerance 1e-11): Kuzmin limiter,
+# fixed point rhs, no Anderson acceleration; 2D example 3 (sharp boundary
+# layer, errors = norms of the solution), 3D example 3 (smooth solution,
+# eps 0.1); P1 on TwoTriangles / tetrahedra, Q1 on UnitSquare, 3 refinements
+AFC_GOLDEN = {
+    ("TwoTriangles", "sharp_boundary_layer_2d"): ([0.53300159222151, 2.2679598767339, 0.40567403714271,
+                                                  0.99999999999964], 1e-8),
+    ("UnitSquare", "sharp_boundary_layer_2d"): ([0.52464493314744, 2.2075436785016, 0.38778102739911,
+                                                0.99998511032586], 1e-8),
+    ("Default_UnitCube_Tetra", "smooth_solution_3d"): ([0.0010369623588627, 0.011894930265006], 1e-11),
+}
+
+
+def afc_db(geo, example, n_refine=3, **extra):
+    db = {"geo_file": geo, "ANSATZ_ORDER": 1, "refinement_n_initial_steps": n_refine, "example": example,
+          "solver_type": "direct", "algebraic_flux_correction": "afc", "afc_limiter": "kuzmin",
+          "afc_iteration_scheme": "fixed_point_rhs", "afc_nonlinloop_anderson_acc": "no",
+          "afc_nonlinloop_maxit": 1000}
+    if example == "smooth_solution_3d":
+        db["diffusion_coefficient"] = 0.1
+    db.update(extra)
+    return db
+
+
+def afc_errors(e, n):
+    return [e[0], e[1], e[2], e[4]][:n]
+
+
+@pytest.mark.parametrize("geo,example", list(AFC_GOLDEN))
+def test_oracle_afc_golden(oracle_lib, geo, example):
+    db = afc_db(geo, example)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    it, res, rejected, res0 = o.afc_solve(pm.solver_desc(db), pm.afc_params(db))
+    gold, tol = AFC_GOLDEN[(geo, example)]
+    got = afc_errors(o.errors(), len(gold))
+    assert np.allclose(got, gold, rtol=0, atol=tol), (got, gold)
+    # stopped by the reference's criterion: residual < epsilon sqrt(n_dof)
+    assert res < 1e-10 * np.sqrt(o.info()["n_dof"]) and it > 1
+
+
+def test_oracle_afc_fixed_point_and_bounds(oracle_lib):
+    """With a tighter epsilon the loop still converges (residual below
+    1e-13 sqrt(n_dof)), and for the sharp boundary layer with f = 0 and
+    boundary data in [0, 1] the solution stays in [0, 1]: the limiter keeps
+    the discrete maximum principle of the M-matrix A + D."""
+    db = afc_db("TwoTriangles", "sharp_boundary_layer_2d", n_refine=4, afc_nonlinloop_epsilon=1e-13)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    it, res, _, _ = o.afc_solve(pm.solver_desc(db), pm.afc_params(db))
+    u = o.solution()
+    assert u.min() >= -1e-12 and u.max() <= 1 + 1e-12, (u.min(), u.max())
+    assert res < 1e-13 * np.sqrt(o.info()["n_dof"])

@@ -473,16 +473,85 @@ def bench_tcd(args):
     print(json.dumps(result), flush=True)
 
 
+def afc_db(args):
+    return {"geo_file": "TwoTriangles", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": args.afc_refine,
+            "example": "sharp_boundary_layer_2d", "algebraic_flux_correction": "afc", "afc_limiter": "kuzmin",
+            "afc_iteration_scheme": "fixed_point_rhs", "afc_nonlinloop_maxit": 1000,
+            "solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "ssor",
+            "sor_omega": 1.0, "residual_tolerance": 1e-13, "residual_reduction": 0.0, "max_n_iterations": 5000,
+            "gmres_restart": 50}
+
+
+def bench_afc(args):
+    """Algebraic flux correction: the sharp boundary layer (eps 1e-3, b at 10
+    degrees, discontinuous inflow data) with P1 on TwoTriangles refined
+    --afc-refine times, Kuzmin limiter, fixed point rhs with dynamic damping,
+    FGMRES(50) + SSOR for the linear systems. One step = the whole
+    nonlinear loop (assembly, D, the limited iterations to the reference's
+    stopping criterion). The CPU baseline runs the oracle's loop for 3
+    nonlinear iterations on the same problem and scales to the GPU's count."""
+    import numpy as np
+    import parmoon2022_amd as pm
+    db = afc_db(args)
+    cd = pm.ConvectionDiffusionAFC(db)
+    info = cd.info()
+    log("setup", info)
+    lat = []
+    for step in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        cd.solve()
+        dt = time.perf_counter() - t0
+        log("step %d: %.3f s, %d nonlinear iterations, %d FGMRES iterations, residual %.2e" %
+            (step, dt, cd.afc["iterations"], cd.stats.iterations, cd.afc["residual"]))
+        if step >= args.warmup:
+            lat.append(dt)
+    u = cd.solution()
+    errs = cd.output()
+    total = sum(lat)
+    value = args.steps / total
+    n = info["n_dof"]
+    result = {
+        "metric": "CD3D solves/s & DoF/s at fixed L2 err vs FP64 oracle; p50/p99 solve latency",
+        "value": value, "unit": "AFC solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (generated mesh)",
+        "config": {"workload": "AFC CD2D P1 sharp boundary layer, TwoTriangles refined %d times, %d DoF, "
+                               "Kuzmin limiter, fixed point rhs, FGMRES(50) + SSOR" % (args.afc_refine, n),
+                   "dof": n, "nnz": info["nnz"]},
+        "nonlinear_iterations": cd.afc["iterations"], "linear_iterations": int(cd.stats.iterations),
+        "solution_range": [float(u.min()), float(u.max())], "l2_norm": float(errs[0]),
+    }
+    n_it = cd.afc["iterations"]
+    cd.close()
+    if not args.no_cpu:
+        import oracle
+        o = oracle.Oracle(pm.problem_desc(db))
+        prm = pm.afc_params(db)
+        prm[1] = 3
+        t0 = time.time()
+        o.afc_solve(pm.solver_desc(db), prm)
+        t3 = time.time() - t0
+        per_it = t3 / 3
+        result["cpu_baseline"] = {"value": 1.0 / (per_it * n_it), "unit": "AFC solves/s", "cores": 1,
+                                  "kind": "port",
+                                  "sample": "oracle (C, -O3, 1 thread): assembly and the first 3 nonlinear "
+                                            "iterations of the same loop (%.2f s), scaled to the GPU's %d "
+                                            "iterations" % (t3, n_it)}
+        result["gpu_over_cpu"] = value * per_it * n_it
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "c5", "tcd"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "c5", "tcd", "afc"],
                     help="c2: one 64^3 Q2 SUPG solve per step (default); c1: the CD2D P1 CG + Jacobi "
                          "problem on 257^2 vertices; c4: a batch of --instances P1 "
                          "tetrahedral instances per step; c5: a served stream of mixed CD2D P2 / CD3D Q1 "
                          "requests at each of --rates; tcd: Crank-Nicolson time steps of the "
                          "quadratic space-time example with Q2 SUPG on the --refine cube")
     ap.add_argument("--tau", type=float, default=0.01, help="tcd time step length")
+    ap.add_argument("--afc-refine", type=int, default=8, help="afc: refinements of TwoTriangles")
     ap.add_argument("--rates", default="10,30,100,300,1000,2000", help="c5 arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=60, help="c5 requests per rate")
     ap.add_argument("--slots", type=int, default=8, help="c5 concurrent requests")
@@ -526,6 +595,8 @@ def main():
         return bench_c5(args)
     if args.workload == "tcd":
         return bench_tcd(args)
+    if args.workload == "afc":
+        return bench_afc(args)
     db = c2_db(args)
     t0 = time.time()
     cd = pm.ConvectionDiffusion(db)

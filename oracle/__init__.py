@@ -211,6 +211,8 @@ class Oracle:
         rc = lib().orc_afc_solve(self._p, ctypes.byref(s), _p(p), _p(out))
         if rc == 3:
             raise ValueError("algebraic flux correction needs a single level")
+        if rc == 4:
+            raise ValueError("the BJK17 limiter needs P1 triangles or tetrahedra")
         if rc == 2:
             raise RuntimeError("oracle solver diverged")
         return int(out[0]), out[1], int(out[2]), out[3]

@@ -2199,8 +2199,110 @@ static void afc_diffusion(const olevel* L, const double* A, double* D) {
 }
 
 typedef struct {
-  double *A, *D, *E, *f, *F, *alpha, *Pp, *Pm, *Qp, *Qm, *Rp, *Rm;
+  double *A, *D, *E, *f, *F, *alpha, *Pp, *Pm, *Qp, *Qm, *Rp, *Rm, *gamma;
 } oafc;
+
+/* Compute_Parameter_For_Linearity_Preservation (AlgebraicFluxCorrection.C:
+   1295-1470), P1 triangles and tetrahedra: per vertex the largest distance
+   to another vertex of a cell over the smallest height of the vertex over
+   the opposite side (2D: the distance to a neighbour when the angle at it is
+   not acute) */
+static void afc_gamma(const orc_problem* P, const olevel* L, double* gamma) {
+  const omesh* m = &L->m;
+  const int n = L->ndof, nv = NV[m->shape], d = P->el.dim;
+  double* par = malloc(sizeof(double) * 2 * n);
+  for(int i = 0; i < n; ++i) { par[i] = 0.0; par[n + i] = 4711.0; }
+  for(int c = 0; c < m->ncells; ++c) {
+    double x[4], y[4], z[4];
+    for(int a = 0; a < nv; ++a) {
+      const double* X = &m->X[3 * m->cv[c * nv + a]];
+      x[a] = X[0]; y[a] = X[1]; z[a] = X[2];
+    }
+    ogeo g;
+    geo(m, c, &g);
+    const double area = fabs(g.det) / (d == 2 ? 2.0 : 6.0);
+    for(int j = 0; j < nv; ++j) {
+      const int index = L->dof[c * L->nl + j];
+      double dist, dist_max;
+      const int j1 = (j + 1) % nv, j2 = (j + 2) % nv, j3 = (j + 3) % nv;
+      if(d == 2) {
+        if((x[j2] - x[j1]) * (x[j] - x[j1]) + (y[j2] - y[j1]) * (y[j] - y[j1]) <= 0) {
+          dist = sqrt((x[j] - x[j1]) * (x[j] - x[j1]) + (y[j] - y[j1]) * (y[j] - y[j1]));
+          dist_max = sqrt((x[j] - x[j2]) * (x[j] - x[j2]) + (y[j] - y[j2]) * (y[j] - y[j2]));
+        } else if((x[j1] - x[j2]) * (x[j] - x[j2]) + (y[j1] - y[j2]) * (y[j] - y[j2]) <= 0) {
+          dist = sqrt((x[j] - x[j2]) * (x[j] - x[j2]) + (y[j] - y[j2]) * (y[j] - y[j2]));
+          dist_max = sqrt((x[j] - x[j1]) * (x[j] - x[j1]) + (y[j] - y[j1]) * (y[j] - y[j1]));
+        } else {
+          double edge = (x[j1] - x[j2]) * (x[j1] - x[j2]);
+          edge += (y[j1] - y[j2]) * (y[j1] - y[j2]);
+          edge = sqrt(edge);
+          dist = 2. * area / edge;
+          dist_max = sqrt((x[j] - x[j1]) * (x[j] - x[j1]) + (y[j] - y[j1]) * (y[j] - y[j1]));
+          const double dm2 = sqrt((x[j] - x[j2]) * (x[j] - x[j2]) + (y[j] - y[j2]) * (y[j] - y[j2]));
+          if(dm2 > dist_max) dist_max = dm2;
+        }
+      } else {
+        const double d1 = sqrt((x[j] - x[j1]) * (x[j] - x[j1]) + (y[j] - y[j1]) * (y[j] - y[j1]) +
+                               (z[j] - z[j1]) * (z[j] - z[j1]));
+        const double d2 = sqrt((x[j] - x[j2]) * (x[j] - x[j2]) + (y[j] - y[j2]) * (y[j] - y[j2]) +
+                               (z[j] - z[j2]) * (z[j] - z[j2]));
+        const double d3 = sqrt((x[j] - x[j3]) * (x[j] - x[j3]) + (y[j] - y[j3]) * (y[j] - y[j3]) +
+                               (z[j] - z[j3]) * (z[j] - z[j3]));
+        dist_max = fmax(fmax(d1, d2), d3);
+        const double A = (z[j3] - z[j1]) * (y[j2] - y[j1]) - (z[j2] - z[j1]) * (y[j3] - y[j1]);
+        const double B = (z[j2] - z[j1]) * (x[j3] - x[j1]) - (z[j3] - z[j1]) * (x[j2] - x[j1]);
+        const double C = (y[j3] - y[j1]) * (x[j2] - x[j1]) - (x[j3] - x[j1]) * (y[j2] - y[j1]);
+        dist = (6.0 * area) / sqrt(A * A + B * B + C * C);
+      }
+      if(dist_max > par[index]) par[index] = dist_max;
+      if(dist < par[index + n]) par[index + n] = dist;
+    }
+  }
+  for(int i = 0; i < n; ++i) gamma[i] = par[i] / par[i + n];
+  free(par);
+}
+
+/* BJK17 limiter (steady_state_algorithm, :2200-2270, 2527-2560): local
+   bounds from the row's neighbours, Q from gamma and D, alpha the smaller
+   of the two rows' factors; rhs_i += sum_j alpha_ij F_ij over the row */
+static void afc_bjk_rhs(const olevel* L, oafc* W, const double* u, double* rhs) {
+  const int n = L->ndof, na = L->nactive;
+  for(int i = 0; i < n; ++i) {
+    double umin = u[i], umax = u[i], q = 0;
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
+      const int c = L->col[l];
+      if(u[c] < umin) umin = u[c];
+      if(u[c] > umax) umax = u[c];
+      if(c != i) q += W->D[l];
+    }
+    q *= W->gamma[i];
+    double Pp = 0, Pm = 0;
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
+      const int c = L->col[l];
+      W->F[l] = W->D[l] * (u[c] - u[i]);
+      if(c == i) continue;
+      if(W->F[l] > 0) Pp += W->F[l];
+      if(W->F[l] < 0) Pm += W->F[l];
+    }
+    const double Qp = q * (u[i] - umax), Qm = q * (u[i] - umin);
+    W->Rp[i] = W->Rm[i] = 1.0;
+    if(fabs(Pp) > 0) { W->Rp[i] = Qp / Pp; if(W->Rp[i] > 1) W->Rp[i] = 1; }
+    if(fabs(Pm) > 0) { W->Rm[i] = Qm / Pm; if(W->Rm[i] > 1) W->Rm[i] = 1; }
+  }
+  for(int i = na; i < n; ++i) W->Rp[i] = W->Rm[i] = 1.0;
+  memcpy(rhs, W->f, sizeof(double) * n);
+  for(int i = 0; i < n; ++i) {
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
+      const int c = L->col[l];
+      double a = 1;
+      if(W->F[l] > 0) { a = W->Rp[i]; if(W->Rm[c] < a) a = W->Rm[c]; }
+      if(W->F[l] < 0) { a = W->Rm[i]; if(W->Rp[c] < a) a = W->Rp[c]; }
+      W->alpha[l] = c == i ? 1.0 : a;
+    }
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) rhs[i] += W->alpha[l] * W->F[l];
+  }
+  memcpy(rhs + na, u + na, sizeof(double) * (n - na));
+}
 
 /* rhs = f + limited fluxes of u (Kuzmin limiter); Dirichlet rows = u */
 static void afc_kuzmin_rhs(const olevel* L, oafc* W, const double* u, double* rhs) {
@@ -2270,11 +2372,17 @@ int orc_afc_solve(orc_problem* P, const osolver* s, const double* prm, double* o
   assemble_level(P, L, 0, L->m.ncells);
   P->afc = 0;
   oafc W;
-  double* buf = calloc((size_t)6 * nnz + 7 * (size_t)n, sizeof(double));
+  const int bjk = prm[11] == 1.0;
+  if(bjk && !((L->m.shape == TRI || L->m.shape == TET) && L->nl == NV[L->m.shape])) {
+    P->afc = 0;
+    return 4;
+  }
+  double* buf = calloc((size_t)6 * nnz + 8 * (size_t)n, sizeof(double));
   W.A = buf; W.D = W.A + nnz; W.E = W.D + nnz; W.F = W.E + nnz; W.alpha = W.F + nnz;
   double* S = W.alpha + nnz;
   W.f = S + nnz; W.Pp = W.f + n; W.Pm = W.Pp + n; W.Qp = W.Pm + n; W.Qm = W.Qp + n; W.Rp = W.Qm + n;
   W.Rm = W.Rp + n;
+  W.gamma = W.Rm + n;
   double* old = malloc(sizeof(double) * n);
   double* cur_sol = malloc(sizeof(double) * n);
   double* cur_rhs = malloc(sizeof(double) * n);
@@ -2282,6 +2390,19 @@ int orc_afc_solve(orc_problem* P, const osolver* s, const double* prm, double* o
   for(int l = 0; l < nnz; ++l) W.alpha[l] = 1.0;
   memcpy(W.A, L->val, sizeof(double) * nnz);
   memcpy(W.f, L->rhs, sizeof(double) * n);
+  if(bjk) {
+    /* modify_system_matrix (:35-60): a_di = 0 for a Dirichlet column d of
+       an active row i with a_id < 0 */
+    for(int i = 0; i < na; ++i)
+      for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
+        const int c = L->col[l];
+        if(c >= na && W.A[l] < 0.0) {
+          const int t = transposed_pos(L, c, i);
+          if(t < L->rp[c + 1]) W.A[t] = 0.0;
+        }
+      }
+    afc_gamma(P, L, W.gamma);
+  }
   afc_diffusion(L, W.A, W.D);
   for(int l = 0; l < nnz; ++l) W.E[l] = W.A[l] + W.D[l];
   memcpy(S, W.E, sizeof(double) * nnz);
@@ -2298,8 +2419,9 @@ int orc_afc_solve(orc_problem* P, const osolver* s, const double* prm, double* o
   double damp = prm[2], damp_max = prm[3];
   const double damp_min = prm[4], max_global = prm[5], inc = prm[6], dec = prm[7], max_inc = prm[8],
                max_dec = prm[9], min_tol = prm[10];
+  void (*limited_rhs)(const olevel*, oafc*, const double*, double*) = bjk ? afc_bjk_rhs : afc_kuzmin_rhs;
   /* iteration 0 */
-  afc_kuzmin_rhs(L, &W, L->sol, L->rhs);
+  limited_rhs(L, &W, L->sol, L->rhs);
   double residual_old = afc_residual(L, L->sol, L->rhs, tmp), residual = residual_old;
   out[3] = residual_old;
   memcpy(old, L->sol, sizeof(double) * n);
@@ -2314,7 +2436,7 @@ int orc_afc_solve(orc_problem* P, const osolver* s, const double* prm, double* o
     for(;;) {
       for(int i = 0; i < n; ++i) L->sol[i] = old[i] + omega * (L->sol[i] - old[i]);
       memcpy(L->sol + na, old + na, sizeof(double) * (n - na));
-      afc_kuzmin_rhs(L, &W, L->sol, L->rhs);
+      limited_rhs(L, &W, L->sol, L->rhs);
       residual = afc_residual(L, L->sol, L->rhs, tmp);
       if(k == 1) break;
       if(residual < residual_old || omega <= min_tol * damp_min) {

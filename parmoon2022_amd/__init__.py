@@ -546,16 +546,21 @@ def afc_params(db):
     """Parameters of the algebraic flux correction loop in the reference's
     names and defaults (AlgebraicFluxCorrection::default_afc_database,
     src/AssembleRoutines/AlgebraicFluxCorrection.C:1914-2056), as the 11
-    doubles the C ABI takes. Only the Kuzmin limiter with the fixed point rhs
-    scheme, dynamic damping, no Anderson acceleration and the zero initial
+    doubles the C ABI takes, and a twelfth for the limiter (0 kuzmin, 1 BJK17;
+    BJK17 for P1 triangles and tetrahedra). Only the fixed point rhs scheme
+    with dynamic damping, no Anderson acceleration and the zero initial
     iterate are built."""
-    fixed = {"afc_limiter": "kuzmin", "afc_iteration_scheme": "fixed_point_rhs",
+    fixed = {"afc_iteration_scheme": "fixed_point_rhs",
              "afc_nonlinloop_anderson_acc": "no", "afc_initial_iterate": "afc_zero",
              "afc_nonlinloop_damping_factor_constant": "no"}
     for k, v in fixed.items():
         if db.get(k, v) != v:
             raise ValueError("%s = %s is not built (only %s)" % (k, db.get(k), v))
-    return np.array([float(db.get(k, d)) for k, d in AFC_PARAM_KEYS])
+    limiters = {"kuzmin": 0.0, "BJK17": 1.0}
+    lim = db.get("afc_limiter", "kuzmin")
+    if lim not in limiters:
+        raise ValueError("afc_limiter = %s is not built (kuzmin, BJK17)" % lim)
+    return np.array([float(db.get(k, d)) for k, d in AFC_PARAM_KEYS] + [limiters[lim]])
 
 
 def tcd_desc(db):

@@ -11,6 +11,71 @@
 
 namespace pmcd {
 
+namespace {
+
+// Compute_Parameter_For_Linearity_Preservation (AlgebraicFluxCorrection.C:
+// 1295-1470) for P1 triangles and tetrahedra: per vertex DOF the largest
+// distance to another vertex of a cell over the smallest height of the
+// vertex over the opposite side (in 2D the distance to a neighbour whose
+// angle is not acute). Mesh geometry only: computed once on the host.
+std::vector<double> bjk_gamma(const Mesh& m, const FESpace& sp) {
+  const int n = sp.n_dof, nv = m.nv;
+  const bool d2 = m.dim() == 2;
+  std::vector<double> hi(n, 0.0), lo(n, 4711.0);
+  for(int c = 0; c < m.n_cells; ++c) {
+    double x[4], y[4], z[4];
+    for(int a = 0; a < nv; ++a) {
+      const double* X = &m.coords[3 * (size_t)m.cell(c)[a]];
+      x[a] = X[0]; y[a] = X[1]; z[a] = X[2];
+    }
+    double measure;
+    if(d2) {
+      const double det = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0]);
+      measure = std::fabs(det) / 2.0;
+    } else {
+      const double a1 = x[1] - x[0], a2 = x[2] - x[0], a3 = x[3] - x[0];
+      const double b1 = y[1] - y[0], b2 = y[2] - y[0], b3 = y[3] - y[0];
+      const double c1 = z[1] - z[0], c2 = z[2] - z[0], c3 = z[3] - z[0];
+      const double det = a1 * (b2 * c3 - b3 * c2) - a2 * (b1 * c3 - b3 * c1) + a3 * (b1 * c2 - b2 * c1);
+      measure = std::fabs(det) / 6.0;
+    }
+    auto len = [&](int p, int q) {
+      return std::sqrt((x[p] - x[q]) * (x[p] - x[q]) + (y[p] - y[q]) * (y[p] - y[q]) +
+                       (z[p] - z[q]) * (z[p] - z[q]));
+    };
+    for(int j = 0; j < nv; ++j) {
+      const int dof = sp.cell_dof(c)[j];
+      const int j1 = (j + 1) % nv, j2 = (j + 2) % nv, j3 = (j + 3) % nv;
+      double dist, dist_max;
+      if(d2) {
+        if((x[j2] - x[j1]) * (x[j] - x[j1]) + (y[j2] - y[j1]) * (y[j] - y[j1]) <= 0) {
+          dist = len(j, j1);
+          dist_max = len(j, j2);
+        } else if((x[j1] - x[j2]) * (x[j] - x[j2]) + (y[j1] - y[j2]) * (y[j] - y[j2]) <= 0) {
+          dist = len(j, j2);
+          dist_max = len(j, j1);
+        } else {
+          dist = 2. * measure / len(j1, j2);
+          dist_max = std::max(len(j, j1), len(j, j2));
+        }
+      } else {
+        dist_max = std::max(std::max(len(j, j1), len(j, j2)), len(j, j3));
+        const double A = (z[j3] - z[j1]) * (y[j2] - y[j1]) - (z[j2] - z[j1]) * (y[j3] - y[j1]);
+        const double B = (z[j2] - z[j1]) * (x[j3] - x[j1]) - (z[j3] - z[j1]) * (x[j2] - x[j1]);
+        const double C = (y[j3] - y[j1]) * (x[j2] - x[j1]) - (x[j3] - x[j1]) * (y[j2] - y[j1]);
+        dist = (6.0 * measure) / std::sqrt(A * A + B * B + C * C);
+      }
+      hi[dof] = std::max(hi[dof], dist_max);
+      lo[dof] = std::min(lo[dof], dist);
+    }
+  }
+  std::vector<double> g(n);
+  for(int i = 0; i < n; ++i) g[i] = hi[i] / lo[i];
+  return g;
+}
+
+}  // namespace
+
 SolveStats CDSystem::afc_solve(const SolverDesc& s, const double* prm, double* out) {
   PMCD_CHECK(n_levels() == 1, "algebraic flux correction needs a single level (the reference refuses multigrid)");
   PMCD_CHECK(!desc_.supg, "algebraic flux correction uses the Galerkin discretization");
@@ -18,6 +83,11 @@ SolveStats CDSystem::afc_solve(const SolverDesc& s, const double* prm, double* o
   Level& L = finest();
   const int n = L.n(), na = L.n_active();
   const size_t nnz = L.val.n;
+  PMCD_CHECK(prm[11] == 0.0 || prm[11] == 1.0, "unknown AFC limiter ", prm[11]);
+  const bool bjk = prm[11] == 1.0;
+  PMCD_CHECK(!bjk || ((L.mesh.shape == Shape::Triangle || L.mesh.shape == Shape::Tetrahedron) &&
+                      L.space.n_local == L.mesh.nv),
+             "the BJK17 limiter needs P1 triangles or tetrahedra");
   if(afc_A_.n != nnz) {
     afc_tpos_.alloc(nnz);
     afc_A_.alloc(nnz); afc_D_.alloc(nnz); afc_E_.alloc(nnz); afc_alpha_.alloc(nnz);
@@ -38,6 +108,13 @@ SolveStats CDSystem::afc_solve(const SolverDesc& s, const double* prm, double* o
                        afc_A_.p, stream_);
   launch_gather_rhs(n, na, L.contrib_ptr.p, L.contrib.p, local_rhs_.p, L.extra_ptr.p, L.extra.p, L.dir_vals.p,
                     afc_f_.p, stream_);
+  if(bjk) {
+    launch_afc_bjk_modify(na, L.row_ptr.p, L.col.p, afc_tpos_.p, afc_A_.p, stream_);
+    if(afc_gamma_.n != (size_t)n) afc_gamma_.alloc(n);
+    const std::vector<double> gamma = bjk_gamma(L.mesh, L.space);
+    afc_gamma_.upload(gamma, stream_);
+    PMCD_HIP(hipStreamSynchronize(stream_));
+  }
   // D, E = A + D, and the solver matrix S = E with identity Dirichlet rows
   launch_afc_matrices(n, na, L.row_ptr.p, L.col.p, afc_tpos_.p, afc_A_.p, afc_D_.p, afc_E_.p, L.val.p, stream_);
   launch_gather_idx(n, L.diag_pos.p, L.val.p, L.diag.p, stream_);
@@ -50,8 +127,12 @@ SolveStats CDSystem::afc_solve(const SolverDesc& s, const double* prm, double* o
   launch_copy(n - na, afc_f_.p + na, L.sol.p + na, stream_);
 
   auto limited_rhs = [&]() {
-    launch_afc_rhs(n, na, L.row_ptr.p, L.col.p, afc_tpos_.p, afc_D_.p, afc_E_.p, L.sol.p, afc_f_.p, afc_Rp_.p,
-                   afc_Rm_.p, afc_alpha_.p, L.rhs.p, stream_);
+    if(bjk)
+      launch_afc_bjk_rhs(n, na, L.row_ptr.p, L.col.p, afc_D_.p, afc_gamma_.p, L.sol.p, afc_f_.p, afc_Rp_.p,
+                         afc_Rm_.p, afc_alpha_.p, L.rhs.p, stream_);
+    else
+      launch_afc_rhs(n, na, L.row_ptr.p, L.col.p, afc_tpos_.p, afc_D_.p, afc_E_.p, L.sol.p, afc_f_.p,
+                     afc_Rp_.p, afc_Rm_.p, afc_alpha_.p, L.rhs.p, stream_);
   };
   auto residual_norm = [&]() {
     launch_spmv(n, na, L.row_ptr.p, L.col.p, L.val.p, L.sol.p, L.rhs.p, afc_res_.p, -1.0, 1.0, stream_,

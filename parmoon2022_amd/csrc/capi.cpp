@@ -583,6 +583,7 @@ int pmcd_afc_solve(pmcd_system* s, const pmcd_solver_desc* sd, const double* prm
   return guarded([&] {
     PMCD_CHECK(s && sd && prm && out, "null argument");
     PMCD_CHECK(prm[0] > 0 && prm[1] >= 1, "afc_nonlinloop_epsilon must be positive, afc_nonlinloop_maxit >= 1");
+    PMCD_CHECK(prm[11] == 0.0 || prm[11] == 1.0, "afc limiter must be 0 (kuzmin) or 1 (BJK17)");
     const SolveStats st = s->sys->afc_solve(to_solver(*sd), prm, out);
     if(stats) {
       stats->iterations = st.iterations;

@@ -212,3 +212,95 @@ void launch_afc_damp(int n, int n_active, double omega, const double* old, doubl
 }
 
 }  // namespace pmcd
+
+namespace pmcd {
+
+namespace {
+
+// modify_system_matrix (AlgebraicFluxCorrection.C:35-60): a_di = 0 for a
+// Dirichlet column d of an active row i with a_id < 0 (each (d, i) entry has
+// one writer, the row i)
+__global__ void k_afc_bjk_modify(int n_active, const int* __restrict__ rp, const int* __restrict__ col,
+                                 const int* __restrict__ tpos, double* __restrict__ A) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n_active) return;
+  for(int j = rp[i]; j < rp[i + 1]; ++j) {
+    const int c = col[j];
+    if(c >= n_active && A[j] < 0.0 && tpos[j] < rp[c + 1]) A[tpos[j]] = 0.0;
+  }
+}
+
+// BJK17 factors R+ and R- of row i (steady_state_algorithm :2200-2270)
+__global__ void k_afc_bjk_limiter(int n, int n_active, const int* __restrict__ rp, const int* __restrict__ col,
+                                  const double* __restrict__ D, const double* __restrict__ gamma,
+                                  const double* __restrict__ u, double* __restrict__ Rp, double* __restrict__ Rm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const int b = rp[i], e = rp[i + 1];
+  const double ui = u[i];
+  double umin = ui, umax = ui, q = 0, Pp = 0, Pm = 0;
+  for(int j = b; j < e; ++j) {
+    const int c = col[j];
+    const double uc = u[c];
+    if(uc < umin) umin = uc;
+    if(uc > umax) umax = uc;
+    if(c != i) q += D[j];
+  }
+  q *= gamma[i];
+  for(int j = b; j < e; ++j) {
+    const int c = col[j];
+    if(c == i) continue;
+    const double F = D[j] * (u[c] - ui);
+    if(F > 0) Pp += F;
+    if(F < 0) Pm += F;
+  }
+  const double Qp = q * (ui - umax), Qm = q * (ui - umin);
+  double rp_ = 1.0, rm_ = 1.0;
+  if(fabs(Pp) > 0) { rp_ = Qp / Pp; if(rp_ > 1) rp_ = 1; }
+  if(fabs(Pm) > 0) { rm_ = Qm / Pm; if(rm_ > 1) rm_ = 1; }
+  if(i >= n_active) rp_ = rm_ = 1.0;
+  Rp[i] = rp_;
+  Rm[i] = rm_;
+}
+
+// rhs_i = f_i + sum_j alpha_ij F_ij, alpha_ij = min of the two rows' factors
+// (:2527-2560, 2720-2722); Dirichlet rows: u_i
+__global__ void k_afc_bjk_rhs(int n, int n_active, const int* __restrict__ rp, const int* __restrict__ col,
+                              const double* __restrict__ D, const double* __restrict__ u,
+                              const double* __restrict__ Rp, const double* __restrict__ Rm,
+                              const double* __restrict__ f, double* __restrict__ alpha, double* __restrict__ rhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const double ui = u[i];
+  double r = f[i];
+  for(int j = rp[i]; j < rp[i + 1]; ++j) {
+    const int c = col[j];
+    const double F = D[j] * (u[c] - ui);
+    double a = 1.0;
+    if(F > 0) { a = Rp[i]; if(Rm[c] < a) a = Rm[c]; }
+    if(F < 0) { a = Rm[i]; if(Rp[c] < a) a = Rp[c]; }
+    if(c == i) a = 1.0;
+    alpha[j] = a;
+    r += a * F;
+  }
+  rhs[i] = i < n_active ? r : ui;
+}
+
+}  // namespace
+
+void launch_afc_bjk_modify(int n_active, const int* rp, const int* col, const int* tpos, double* A,
+                           hipStream_t s) {
+  if(n_active) k_afc_bjk_modify<<<(n_active + 255) / 256, 256, 0, s>>>(n_active, rp, col, tpos, A);
+  PMCD_HIP(hipGetLastError());
+}
+
+void launch_afc_bjk_rhs(int n, int n_active, const int* rp, const int* col, const double* D, const double* gamma,
+                        const double* u, const double* f, double* Rp, double* Rm, double* alpha, double* rhs,
+                        hipStream_t s) {
+  if(!n) return;
+  k_afc_bjk_limiter<<<(n + 255) / 256, 256, 0, s>>>(n, n_active, rp, col, D, gamma, u, Rp, Rm);
+  k_afc_bjk_rhs<<<(n + 255) / 256, 256, 0, s>>>(n, n_active, rp, col, D, u, Rp, Rm, f, alpha, rhs);
+  PMCD_HIP(hipGetLastError());
+}
+
+}  // namespace pmcd

@@ -211,8 +211,8 @@ class CDSystem {
   // fixed point rhs scheme with the Kuzmin limiter and dynamic damping from
   // the zero initial iterate. prm: epsilon, maxit, damping factor, its max,
   // min, global max, increase, decrease, max increase, max decrease, min
-  // tolerance. out: iterations, final residual, rejected steps, initial
-  // residual. Single level only.
+  // tolerance, limiter (0 Kuzmin, 1 BJK17). out: iterations, final
+  // residual, rejected steps, initial residual. Single level only.
   SolveStats afc_solve(const SolverDesc& s, const double* prm, double* out);
 
  private:
@@ -276,7 +276,7 @@ class CDSystem {
   // algebraic flux correction state
   DevBuf<int> afc_tpos_;
   DevBuf<double> afc_A_, afc_D_, afc_E_, afc_alpha_, afc_f_, afc_Rp_, afc_Rm_, afc_old_, afc_cur_sol_,
-      afc_cur_rhs_, afc_res_;
+      afc_cur_rhs_, afc_res_, afc_gamma_;
   // Krylov work space
   std::vector<DevBuf<double>> kv_, kz_;
   DevBuf<double> kr_, kw_, kh_;

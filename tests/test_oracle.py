@@ -479,17 +479,21 @@ def test_oracle_tcd2d_golden(oracle_lib, disc):
         assert abs(got[m][0] - l2[i]) < 1e-10 and abs(got[m][1] - h1[i]) < 1e-10, (m, got[m][:2], l2[i], h1[i])
 
 
-# test/cd2d_afc_test.c++:36-66,118-131,217-233 (tolerance 1e-8) and
-# test/cd3d_afc_test.c++:37-51,93-101 (tolerance 1e-11): Kuzmin limiter,
-# fixed point rhs, no Anderson acceleration; 2D example 3 (sharp boundary
-# layer, errors = norms of the solution), 3D example 3 (smooth solution,
-# eps 0.1); P1 on TwoTriangles / tetrahedra, Q1 on UnitSquare, 3 refinements
+# test/cd2d_afc_test.c++:36-66,118-131,148-154,217-233 (tolerance 1e-8) and
+# test/cd3d_afc_test.c++:37-51,93-101,123-128 (tolerance 1e-11): Kuzmin and
+# BJK17 limiters, fixed point rhs, no Anderson acceleration; 2D example 3
+# (sharp boundary layer, errors = norms of the solution), 3D example 3
+# (smooth solution, eps 0.1); P1 on TwoTriangles / tetrahedra, Q1 on
+# UnitSquare, 3 refinements
 AFC_GOLDEN = {
-    ("TwoTriangles", "sharp_boundary_layer_2d"): ([0.53300159222151, 2.2679598767339, 0.40567403714271,
-                                                  0.99999999999964], 1e-8),
-    ("UnitSquare", "sharp_boundary_layer_2d"): ([0.52464493314744, 2.2075436785016, 0.38778102739911,
-                                                0.99998511032586], 1e-8),
-    ("Default_UnitCube_Tetra", "smooth_solution_3d"): ([0.0010369623588627, 0.011894930265006], 1e-11),
+    ("TwoTriangles", "sharp_boundary_layer_2d", "kuzmin"): ([0.53300159222151, 2.2679598767339, 0.40567403714271,
+                                                            0.99999999999964], 1e-8),
+    ("UnitSquare", "sharp_boundary_layer_2d", "kuzmin"): ([0.52464493314744, 2.2075436785016, 0.38778102739911,
+                                                          0.99998511032586], 1e-8),
+    ("Default_UnitCube_Tetra", "smooth_solution_3d", "kuzmin"): ([0.0010369623588627, 0.011894930265006], 1e-11),
+    ("TwoTriangles", "sharp_boundary_layer_2d", "BJK17"): ([0.54553405607575, 2.5382226427268, 0.48949470018912,
+                                                           1.0], 1e-8),
+    ("Default_UnitCube_Tetra", "smooth_solution_3d", "BJK17"): ([0.0004164209086368, 0.010287281106288], 1e-11),
 }
 
 
@@ -508,12 +512,12 @@ def afc_errors(e, n):
     return [e[0], e[1], e[2], e[4]][:n]
 
 
-@pytest.mark.parametrize("geo,example", list(AFC_GOLDEN))
-def test_oracle_afc_golden(oracle_lib, geo, example):
-    db = afc_db(geo, example)
+@pytest.mark.parametrize("geo,example,limiter", list(AFC_GOLDEN))
+def test_oracle_afc_golden(oracle_lib, geo, example, limiter):
+    db = afc_db(geo, example, afc_limiter=limiter)
     o = oracle_lib.Oracle(pm.problem_desc(db))
     it, res, rejected, res0 = o.afc_solve(pm.solver_desc(db), pm.afc_params(db))
-    gold, tol = AFC_GOLDEN[(geo, example)]
+    gold, tol = AFC_GOLDEN[(geo, example, limiter)]
     got = afc_errors(o.errors(), len(gold))
     assert np.allclose(got, gold, rtol=0, atol=tol), (got, gold)
     # stopped by the reference's criterion: residual < epsilon sqrt(n_dof)

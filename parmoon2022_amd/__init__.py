@@ -238,7 +238,7 @@ def problem_desc(db):
     """Build a ProblemDesc from a parameter dictionary (reference key names)."""
     d = ProblemDesc()
     geo = db.get("geo_file", "UnitSquare")
-    if str(geo).endswith(".mesh"):
+    if str(geo).endswith((".mesh", ".GEO", ".geo")):
         # the struct holds the path; keep the bytes alive with it
         d._mesh_path = os.fsencode(os.path.abspath(geo))
         d.geometry = MESH_FILE
@@ -359,7 +359,9 @@ class HostSetup:
 
     def cell_coords(self, level=-1):
         i = self.info(level)
-        if self.desc.geometry == MESH_FILE:  # tetrahedra (P1 4, P2 10 DOFs) or hexahedra (8, 27)
+        if self.desc.geometry == MESH_FILE and self.desc.mesh_file.lower().endswith(b".geo"):
+            nv = {3: 3, 6: 3, 4: 4, 9: 4}[i["n_local"]]  # triangles or quadrilaterals
+        elif self.desc.geometry == MESH_FILE:  # tetrahedra (P1 4, P2 10 DOFs) or hexahedra (8, 27)
             nv = {4: 4, 10: 4, 8: 8, 27: 8}[i["n_local"]]
         else:
             nv = VERTICES_PER_CELL[self.desc.geometry]

@@ -316,6 +316,69 @@ std::vector<std::string> mesh_tokens(const std::string& path) {
 
 }  // namespace
 
+Mesh read_geo_file(const std::string& path) {
+  std::ifstream in(path);
+  PMCD_CHECK(in.good(), "cannot open .GEO file ", path);
+  std::string line;
+  std::getline(in, line);  // title
+  std::getline(in, line);  // parametrisation comment
+  int nel = 0, nvt = 0, nmt = 0, nve = 0, nbct = 0;
+  in >> nel >> nvt >> nmt >> nve >> nbct;
+  PMCD_CHECK(!in.fail() && nel > 0 && nvt > 0, ".GEO file ", path, ": bad counts line");
+  PMCD_CHECK(nve == 3 || nve == 4, ".GEO file ", path, ": ", nve,
+             " vertices per cell; 2D triangles (3) or quadrilaterals (4) are read");
+  std::getline(in, line);  // rest of the counts line
+  std::getline(in, line);  // DCORVG
+  std::vector<double> dc(2 * (size_t)nvt);
+  for(int i = 0; i < nvt; ++i) {
+    in >> dc[2 * i] >> dc[2 * i + 1];
+    std::getline(in, line);
+  }
+  std::getline(in, line);  // KVERT
+  std::vector<int> kv((size_t)nel * nve);
+  for(int c = 0; c < nel; ++c) {
+    for(int j = 0; j < nve; ++j) in >> kv[(size_t)c * nve + j];
+    std::getline(in, line);
+  }
+  std::getline(in, line);  // KNPR
+  std::vector<int> knpr(nvt);
+  for(int i = 0; i < nvt; ++i) in >> knpr[i];
+  PMCD_CHECK(!in.fail(), ".GEO file ", path, " ends before its KNPR section is complete");
+  Mesh m;
+  m.shape = nve == 3 ? Shape::Triangle : Shape::Quadrangle;
+  m.nv = nve;
+  // boundary vertices carry the boundary parameter T: component floor(T),
+  // local parameter T - floor(T) on the unit square's line segments
+  // (Default_UnitSquare: (0,0)-(1,0), (1,0)-(1,1), (1,1)-(0,1), (0,1)-(0,0))
+  static const double seg[4][4] = {{0, 0, 1, 0}, {1, 0, 1, 1}, {1, 1, 0, 1}, {0, 1, 0, 0}};
+  m.coords.resize(3 * (size_t)nvt, 0.0);
+  for(int i = 0; i < nvt; ++i) {
+    if(knpr[i] == 0) {
+      m.coords[3 * i] = dc[2 * i];
+      m.coords[3 * i + 1] = dc[2 * i + 1];
+      continue;
+    }
+    PMCD_CHECK(knpr[i] == 1, ".GEO file ", path, ": boundary part ", knpr[i],
+               "; only the unit square boundary (one part, 4 components) is known");
+    const double T = dc[2 * i];
+    const int comp = (int)std::floor(T);
+    PMCD_CHECK(comp >= 0 && comp < 4, ".GEO file ", path, ": boundary parameter ", T, " outside [0, 4)");
+    const double t = T - comp;
+    m.coords[3 * i] = seg[comp][0] + t * (seg[comp][2] - seg[comp][0]);
+    m.coords[3 * i + 1] = seg[comp][1] + t * (seg[comp][3] - seg[comp][1]);
+  }
+  m.cell_vertices.resize(kv.size());
+  for(size_t p = 0; p < kv.size(); ++p) {
+    PMCD_CHECK(kv[p] >= 1 && kv[p] <= nvt, ".GEO file ", path, ": vertex id ", kv[p], " out of range");
+    m.cell_vertices[p] = kv[p] - 1;
+  }
+  m.n_cells = nel;
+  m.parent.assign(m.n_cells, -1);
+  m.child_index.assign(m.n_cells, 0);
+  set_box_boundary(m);
+  return m;
+}
+
 Mesh read_mesh_file(const std::string& path) {
   const std::vector<std::string> tok = mesh_tokens(path);
   size_t p = 0;

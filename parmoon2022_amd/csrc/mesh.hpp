@@ -48,6 +48,12 @@ Mesh structured_cube(int n, bool tetra);
 // in cell and local face order. Cells with 2D shapes need a .PRM boundary
 // description and are rejected.
 Mesh read_mesh_file(const std::string& path);
+// 2D mesh in the .GEO format as TDomain::ReadGeo reads it
+// (src/Geometry/ReadGeo.C:43-140): counts, DCORVG, KVERT, KNPR; root cells
+// and their vertices in file order. Boundary vertices (KNPR > 0) give the
+// boundary parameter, evaluated on the unit square's boundary
+// (Default_UnitSquare); other boundary descriptions are rejected.
+Mesh read_geo_file(const std::string& path);
 // Writes a 3D mesh in the same format: vertices, the boundary faces (one per
 // boundary joint, in cell and local face order, marker = component + 1) and
 // the cells.

@@ -44,7 +44,11 @@ std::vector<std::unique_ptr<Level>> build_levels(const ProblemDesc& d) {
   PMCD_CHECK(d.n_refine >= 0 && d.n_refine <= 12, "n_refine out of range");
   Mesh base;
   if(d.geometry == 10 || d.geometry == 11) base = structured_cube(d.structured_n, d.geometry == 11);
-  else if(d.geometry == 12) base = read_mesh_file(d.mesh_file);
+  else if(d.geometry == 12) {
+    const std::string& f = d.mesh_file;
+    const bool geo = f.size() > 4 && (f.compare(f.size() - 4, 4, ".GEO") == 0 || f.compare(f.size() - 4, 4, ".geo") == 0);
+    base = geo ? read_geo_file(f) : read_mesh_file(f);
+  }
   else {
     PMCD_CHECK(d.geometry >= 0 && d.geometry <= 4, "unknown geometry ", d.geometry);
     base = default_mesh((DefaultGeometry)d.geometry);

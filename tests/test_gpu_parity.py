@@ -947,3 +947,27 @@ def test_checkpoint_and_matrix_files(tmp_path):
     assert rel < 1e-10
     for s in (a, b, c):
         s.close()
+
+
+@pytest.mark.parametrize("geo_file,builtin", [("UnitSquare_tria.GEO", "TwoTriangles2"),
+                                              ("UnitSquare_quad.GEO", "UnitSquare")])
+@pytest.mark.parametrize("order", [1, 2])
+def test_geo_file_cd2d_golden(geo_file, builtin, order):
+    """test/cd2d_test.c++ fixtures on the reference's .GEO files of the same
+    squares: the errors equal the golden ones (1e-11) and the solution equals
+    the built-in mesh's bit for bit."""
+    import os
+    from test_oracle import CD2D_GOLDEN, cd2d_db
+    print("geo golden", geo_file, order, flush=True)
+    db = cd2d_db(builtin, order)
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", geo_file)
+    a = pm.ConvectionDiffusion(db)
+    a.assemble()
+    a.solve()
+    b = pm.ConvectionDiffusion(dict(db, geo_file=path))
+    b.assemble()
+    b.solve()
+    e = b.output()
+    g = CD2D_GOLDEN[(builtin, order)]
+    assert np.allclose([e[0], e[1], e[2], e[4]], g, rtol=0, atol=1e-11)
+    assert np.array_equal(a.solution(), b.solution())

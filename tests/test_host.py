@@ -293,3 +293,57 @@ def test_matrix_market_files(tmp_path):
     pm.write_vector_market(str(q), x)
     y = scipy.io.mmread(str(q)).ravel()
     assert np.allclose(y, x, rtol=1e-15, atol=0)
+
+
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("geo_file,builtin", [("UnitSquare_tria.GEO", "TwoTriangles2"),
+                                              ("UnitSquare_quad.GEO", "UnitSquare")])
+@pytest.mark.parametrize("order,nref", [(1, 3), (2, 2)])
+def test_geo_file_equals_builtin_square(geo_file, builtin, order, nref):
+    """The reference's data/mesh/UnitSquare_tria.GEO and UnitSquare_quad.GEO
+    (boundary vertices given by their parameter on the unit square) read as
+    geo_file give the built-in TwoTriangles2 / UnitSquare meshes: the same
+    refined cells, coordinates, DOF map and sparsity."""
+    a = pm.HostSetup({"geo_file": builtin, "ANSATZ_ORDER": order, "refinement_n_initial_steps": nref,
+                      "example": "sine_laplace_2d"})
+    b = pm.HostSetup({"geo_file": os.path.join(GOLDEN_DIR, geo_file), "ANSATZ_ORDER": order,
+                      "refinement_n_initial_steps": nref, "example": "sine_laplace_2d"})
+    assert a.info() == b.info()
+    assert np.array_equal(a.dofmap(), b.dofmap())
+    for x, y in zip(a.pattern(), b.pattern()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.cell_coords(), b.cell_coords())
+
+
+def test_geo_file_irregular_square():
+    """data/mesh/UnitSquareIrregular.GEO: 9 triangles, 8 vertices of which 5
+    on the boundary (parameter 2.5 is the midpoint (0.5, 1) of the top side);
+    the triangles tile the square (areas sum to 1). After one refinement P1
+    has the 8 vertices plus the midpoints of the 16 edges (Euler: 8 + 9 - 1)
+    = 24 DOFs, of which the 5 boundary vertices and 5 boundary edge midpoints
+    are Dirichlet."""
+    h = pm.HostSetup({"geo_file": os.path.join(GOLDEN_DIR, "UnitSquareIrregular.GEO"), "ANSATZ_ORDER": 1,
+                      "refinement_n_initial_steps": 0, "example": "sine_2d"})
+    cc = h.cell_coords()
+    assert cc.shape[0] == 9
+    pts = {tuple(np.round(p[:2], 12)) for p in cc.reshape(-1, 3)}
+    assert (0.5, 1.0) in pts and (0.6, 0.4) in pts and len(pts) == 8
+    area = sum(0.5 * abs((c[1][0] - c[0][0]) * (c[2][1] - c[0][1]) - (c[2][0] - c[0][0]) * (c[1][1] - c[0][1]))
+               for c in cc)
+    assert abs(area - 1.0) < 1e-14
+    r = pm.HostSetup({"geo_file": os.path.join(GOLDEN_DIR, "UnitSquareIrregular.GEO"), "ANSATZ_ORDER": 1,
+                      "refinement_n_initial_steps": 1, "example": "sine_2d"}).info()
+    assert r["n_cells"] == 36 and r["n_dof"] == 24 and r["n_dof"] - r["n_active"] == 10, r
+
+
+def test_geo_file_errors(tmp_path):
+    bad = tmp_path / "bad.GEO"
+    bad.write_text("t\nc\n 1 3 0 3 1\nDCORVG\n0 0\n1 0\n0.5 0.5\nKVERT\n1 2 3\nKNPR\n2 1 0\n")
+    with pytest.raises(RuntimeError, match="boundary part"):
+        pm.HostSetup({"geo_file": str(bad), "ANSATZ_ORDER": 1, "example": "sine_2d"})
+    short = tmp_path / "short.GEO"
+    short.write_text("t\nc\n 1 3 0 3 1\nDCORVG\n0 0\n1 0\n0.5 0.5\nKVERT\n1 2 3\nKNPR\n1 1\n")
+    with pytest.raises(RuntimeError, match="ends before"):
+        pm.HostSetup({"geo_file": str(short), "ANSATZ_ORDER": 1, "example": "sine_2d"})

@@ -201,9 +201,9 @@ class Oracle:
         return st, t.value
 
     def afc_solve(self, sdesc, prm):
-        """Algebraic flux correction (Kuzmin limiter, fixed point rhs, dynamic
-        damping) from the zero initial iterate; prm as
-        parmoon2022_amd.afc_params. Returns (iterations, residual, rejected
+        """Algebraic flux correction (Kuzmin or BJK17 limiter, fixed point rhs
+        or fixed point matrix scheme, dynamic damping) from the zero initial
+        iterate; prm as parmoon2022_amd.afc_params. Returns (iterations, residual, rejected
         steps, initial residual)."""
         s = solver(sdesc)
         p = np.ascontiguousarray(prm, dtype=np.float64)

@@ -2154,7 +2154,8 @@ int orc_tcd_step(orc_problem* P, const osolver* s, ostats* st, double* time) {
   return rc;
 }
 
-/* ---- algebraic flux correction, Kuzmin limiter, fixed point rhs ----
+/* ---- algebraic flux correction: Kuzmin / BJK17 limiter, fixed point rhs /
+   fixed point matrix ----
    ConvectionDiffusion_AFC<d>::assemble / solve / dynamic_damping
    (src/System/ConvectionDiffusion_AFC.C:132-600, 1032-1180) and
    AlgebraicFluxCorrection::steady_state_algorithm (non-MPI path,
@@ -2355,6 +2356,42 @@ static void afc_kuzmin_rhs(const olevel* L, oafc* W, const double* u, double* rh
   memcpy(rhs + na, u + na, sizeof(double) * (n - na));
 }
 
+/* fixed point matrix scheme (steady_state_algorithm :2746-2790): with the
+   limiter's alpha, S = E - tau0 alpha D off the diagonal, S_ii = E_ii +
+   tau0 sum_j alpha_ij d_ij; rhs = f + tau1 (sum_j alpha_ij d_ij u_j -
+   (sum_j alpha_ij d_ij) u_i) in entry order; Dirichlet rows identity / u */
+static void afc_fpm(olevel* L, const oafc* W, const double* u, double* rhs, double tau0) {
+  const int n = L->ndof, na = L->nactive;
+  const double tau1 = 1.0 - tau0;
+  for(int i = 0; i < n; ++i) {
+    const int b = L->rp[i], e = L->rp[i + 1];
+    if(i >= na) {
+      for(int l = b; l < e; ++l) L->val[l] = L->col[l] == i ? 1.0 : 0.0;
+      rhs[i] = u[i];
+      continue;
+    }
+    double r = W->f[i];
+    for(int l = b; l < e; ++l) {
+      const int c = L->col[l];
+      double v = W->E[l];
+      if(c != i) {
+        v -= W->alpha[l] * W->D[l] * tau0;
+        r += W->alpha[l] * W->D[l] * u[c] * tau1;
+      } else {
+        double sum_flux = 0.0;
+        for(int ll = b; ll < e; ++ll)
+          if(L->col[ll] != i) sum_flux += W->alpha[ll] * W->D[ll];
+        v += sum_flux * tau0;
+        r -= sum_flux * u[i] * tau1;
+      }
+      L->val[l] = v;
+    }
+    rhs[i] = r;
+  }
+  extract_diag(L);
+  if(L->lu) { free(L->lu); L->lu = NULL; free(L->piv); L->piv = NULL; }
+}
+
 static double afc_residual(const olevel* L, const double* u, const double* rhs, double* tmp) {
   memcpy(tmp, rhs, sizeof(double) * L->ndof);
   apply_scaled_add(L, u, tmp, -1.0);
@@ -2419,7 +2456,11 @@ int orc_afc_solve(orc_problem* P, const osolver* s, const double* prm, double* o
   double damp = prm[2], damp_max = prm[3];
   const double damp_min = prm[4], max_global = prm[5], inc = prm[6], dec = prm[7], max_inc = prm[8],
                max_dec = prm[9], min_tol = prm[10];
-  void (*limited_rhs)(const olevel*, oafc*, const double*, double*) = bjk ? afc_bjk_rhs : afc_kuzmin_rhs;
+  void (*limiter)(const olevel*, oafc*, const double*, double*) = bjk ? afc_bjk_rhs : afc_kuzmin_rhs;
+  const int fpm = prm[12] == 1.0;
+  const double tau0 = prm[13];
+#define limited_rhs(L_, W_, u_, r_) \
+  do { limiter(L_, W_, u_, r_); if(fpm) afc_fpm(L_, W_, u_, r_, tau0); } while(0)
   /* iteration 0 */
   limited_rhs(L, &W, L->sol, L->rhs);
   double residual_old = afc_residual(L, L->sol, L->rhs, tmp), residual = residual_old;
@@ -2464,6 +2505,7 @@ int orc_afc_solve(orc_problem* P, const osolver* s, const double* prm, double* o
     if(rc) break;
     residual_old = residual;
   }
+#undef limited_rhs
   out[0] = k;
   out[1] = residual;
   out[2] = rejected;

@@ -548,12 +548,12 @@ def afc_params(db):
     """Parameters of the algebraic flux correction loop in the reference's
     names and defaults (AlgebraicFluxCorrection::default_afc_database,
     src/AssembleRoutines/AlgebraicFluxCorrection.C:1914-2056), as the 11
-    doubles the C ABI takes, and a twelfth for the limiter (0 kuzmin, 1 BJK17;
-    BJK17 for P1 triangles and tetrahedra). Only the fixed point rhs scheme
-    with dynamic damping, no Anderson acceleration and the zero initial
-    iterate are built."""
-    fixed = {"afc_iteration_scheme": "fixed_point_rhs",
-             "afc_nonlinloop_anderson_acc": "no", "afc_initial_iterate": "afc_zero",
+    doubles the C ABI takes, then the limiter (0 kuzmin, 1 BJK17; BJK17 for
+    P1 triangles and tetrahedra), the iteration scheme (0 fixed_point_rhs,
+    1 fixed_point_matrix, the reference's default) and
+    afc_fixed_point_matrix_weight: 14 doubles. Dynamic damping, no Anderson
+    acceleration and the zero initial iterate."""
+    fixed = {"afc_nonlinloop_anderson_acc": "no", "afc_initial_iterate": "afc_zero",
              "afc_nonlinloop_damping_factor_constant": "no"}
     for k, v in fixed.items():
         if db.get(k, v) != v:
@@ -562,7 +562,12 @@ def afc_params(db):
     lim = db.get("afc_limiter", "kuzmin")
     if lim not in limiters:
         raise ValueError("afc_limiter = %s is not built (kuzmin, BJK17)" % lim)
-    return np.array([float(db.get(k, d)) for k, d in AFC_PARAM_KEYS] + [limiters[lim]])
+    schemes = {"fixed_point_rhs": 0.0, "fixed_point_matrix": 1.0}
+    sch = db.get("afc_iteration_scheme", "fixed_point_matrix")
+    if sch not in schemes:
+        raise ValueError("afc_iteration_scheme = %s is not built (fixed_point_rhs, fixed_point_matrix)" % sch)
+    return np.array([float(db.get(k, d)) for k, d in AFC_PARAM_KEYS] +
+                    [limiters[lim], schemes[sch], float(db.get("afc_fixed_point_matrix_weight", 0.75))])
 
 
 def tcd_desc(db):

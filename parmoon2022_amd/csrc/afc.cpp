@@ -85,6 +85,9 @@ SolveStats CDSystem::afc_solve(const SolverDesc& s, const double* prm, double* o
   const size_t nnz = L.val.n;
   PMCD_CHECK(prm[11] == 0.0 || prm[11] == 1.0, "unknown AFC limiter ", prm[11]);
   const bool bjk = prm[11] == 1.0;
+  PMCD_CHECK(prm[12] == 0.0 || prm[12] == 1.0, "unknown AFC iteration scheme ", prm[12]);
+  const bool fpm = prm[12] == 1.0;  // fixed point matrix: S changes with the iterate
+  const double tau0 = prm[13];
   PMCD_CHECK(!bjk || ((L.mesh.shape == Shape::Triangle || L.mesh.shape == Shape::Tetrahedron) &&
                       L.space.n_local == L.mesh.nv),
              "the BJK17 limiter needs P1 triangles or tetrahedra");
@@ -133,6 +136,16 @@ SolveStats CDSystem::afc_solve(const SolverDesc& s, const double* prm, double* o
     else
       launch_afc_rhs(n, na, L.row_ptr.p, L.col.p, afc_tpos_.p, afc_D_.p, afc_E_.p, L.sol.p, afc_f_.p,
                      afc_Rp_.p, afc_Rm_.p, afc_alpha_.p, L.rhs.p, stream_);
+    if(fpm) {
+      // the system matrix follows the iterate: rebuilt here, refactorized by
+      // the next solve
+      launch_afc_fpm(n, na, L.row_ptr.p, L.col.p, afc_D_.p, afc_E_.p, afc_alpha_.p, L.sol.p, afc_f_.p, tau0,
+                     L.val.p, L.rhs.p, stream_);
+      launch_gather_idx(n, L.diag_pos.p, L.val.p, L.diag.p, stream_);
+      L.sw_fresh = L.sw_fresh32 = L.sell_fresh = L.sell_fresh32 = false;
+      coarse_ready_ = false;
+      mixed_ready_ = false;
+    }
   };
   auto residual_norm = [&]() {
     launch_spmv(n, na, L.row_ptr.p, L.col.p, L.val.p, L.sol.p, L.rhs.p, afc_res_.p, -1.0, 1.0, stream_,

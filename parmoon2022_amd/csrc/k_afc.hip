@@ -304,3 +304,56 @@ void launch_afc_bjk_rhs(int n, int n_active, const int* rp, const int* col, cons
 }
 
 }  // namespace pmcd
+
+namespace pmcd {
+
+namespace {
+
+// fixed point matrix scheme (steady_state_algorithm :2746-2790): with the
+// limiter's alpha, S = E - tau0 alpha D off the diagonal, S_ii = E_ii +
+// tau0 sum_j alpha_ij d_ij; rhs = f + tau1 (alpha_ij d_ij u_j, in entry
+// order, minus (sum_j alpha_ij d_ij) u_i at the diagonal); Dirichlet rows:
+// identity and u_i
+__global__ void k_afc_fpm(int n, int n_active, const int* __restrict__ rp, const int* __restrict__ col,
+                          const double* __restrict__ D, const double* __restrict__ E,
+                          const double* __restrict__ alpha, const double* __restrict__ u,
+                          const double* __restrict__ f, double tau0, double* __restrict__ S,
+                          double* __restrict__ rhs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const int b = rp[i], e = rp[i + 1];
+  if(i >= n_active) {
+    for(int j = b; j < e; ++j) S[j] = col[j] == i ? 1.0 : 0.0;
+    rhs[i] = u[i];
+    return;
+  }
+  const double tau1 = 1.0 - tau0;
+  double sum_flux = 0.0;
+  for(int j = b; j < e; ++j)
+    if(col[j] != i) sum_flux += alpha[j] * D[j];
+  double r = f[i];
+  for(int j = b; j < e; ++j) {
+    const int c = col[j];
+    double v = E[j];
+    if(c != i) {
+      v -= alpha[j] * D[j] * tau0;
+      r += alpha[j] * D[j] * u[c] * tau1;
+    } else {
+      v += sum_flux * tau0;
+      r -= sum_flux * u[i] * tau1;
+    }
+    S[j] = v;
+  }
+  rhs[i] = r;
+}
+
+}  // namespace
+
+void launch_afc_fpm(int n, int n_active, const int* rp, const int* col, const double* D, const double* E,
+                    const double* alpha, const double* u, const double* f, double tau0, double* S, double* rhs,
+                    hipStream_t s) {
+  if(n) k_afc_fpm<<<(n + 255) / 256, 256, 0, s>>>(n, n_active, rp, col, D, E, alpha, u, f, tau0, S, rhs);
+  PMCD_HIP(hipGetLastError());
+}
+
+}  // namespace pmcd

@@ -195,6 +195,9 @@ void launch_afc_rhs(int n, int n_active, const int* rp, const int* col, const in
                     const double* E, const double* u, const double* f, double* Rp, double* Rm, double* alpha,
                     double* rhs, hipStream_t s);
 void launch_afc_damp(int n, int n_active, double omega, const double* old, double* u, hipStream_t s);
+void launch_afc_fpm(int n, int n_active, const int* rp, const int* col, const double* D, const double* E,
+                    const double* alpha, const double* u, const double* f, double tau0, double* S, double* rhs,
+                    hipStream_t s);
 void launch_afc_bjk_modify(int n_active, const int* rp, const int* col, const int* tpos, double* A,
                            hipStream_t s);
 void launch_afc_bjk_rhs(int n, int n_active, const int* rp, const int* col, const double* D, const double* gamma,

@@ -208,11 +208,12 @@ class CDSystem {
   // algebraic flux correction (ConvectionDiffusion_AFC<d>,
   // src/System/ConvectionDiffusion_AFC.C): assemble the Galerkin matrix
   // with its Dirichlet rows, add the artificial diffusion D, and iterate the
-  // fixed point rhs scheme with the Kuzmin limiter and dynamic damping from
-  // the zero initial iterate. prm: epsilon, maxit, damping factor, its max,
+  // fixed point rhs or fixed point matrix scheme with the Kuzmin or BJK17
+  // limiter and dynamic damping from the zero initial iterate. prm: epsilon, maxit, damping factor, its max,
   // min, global max, increase, decrease, max increase, max decrease, min
-  // tolerance, limiter (0 Kuzmin, 1 BJK17). out: iterations, final
-  // residual, rejected steps, initial residual. Single level only.
+  // tolerance, limiter (0 Kuzmin, 1 BJK17), scheme (0 fixed point rhs,
+  // 1 fixed point matrix), fixed point matrix weight. out: iterations,
+  // final residual, rejected steps, initial residual. Single level only.
   SolveStats afc_solve(const SolverDesc& s, const double* prm, double* out);
 
  private:

@@ -1,5 +1,5 @@
-"""Algebraic flux correction (Kuzmin and BJK17 limiters, fixed point rhs, dynamic
-damping) on the device against the reference's cd2d/cd3d AFC golden errors
+"""Algebraic flux correction (Kuzmin and BJK17 limiters, fixed point rhs and
+fixed point matrix schemes, dynamic damping) on the device against the reference's cd2d/cd3d AFC golden errors
 and against the oracle's iterates.
 
 Tolerances: the golden errors within the reference's own bounds (1e-8 in 2D,
@@ -21,12 +21,12 @@ def run_gpu(db):
     return cd
 
 
-@pytest.mark.parametrize("geo,example,limiter", list(AFC_GOLDEN))
-def test_afc_golden_gpu(oracle_lib, geo, example, limiter):
-    print("afc golden", geo, example, limiter, flush=True)
-    db = afc_db(geo, example, afc_limiter=limiter)
+@pytest.mark.parametrize("geo,example,limiter,scheme", list(AFC_GOLDEN))
+def test_afc_golden_gpu(oracle_lib, geo, example, limiter, scheme):
+    print("afc golden", geo, example, limiter, scheme, flush=True)
+    db = afc_db(geo, example, afc_limiter=limiter, afc_iteration_scheme=scheme)
     cd = run_gpu(db)
-    gold, tol = AFC_GOLDEN[(geo, example, limiter)]
+    gold, tol = AFC_GOLDEN[(geo, example, limiter, scheme)]
     got = afc_errors(cd.output(), len(gold))
     assert np.allclose(got, gold, rtol=0, atol=tol), (got, gold)
     o = oracle_lib.Oracle(pm.problem_desc(db))
@@ -37,20 +37,22 @@ def test_afc_golden_gpu(oracle_lib, geo, example, limiter):
     assert np.max(np.abs(u - uo)) <= 1e-10 * np.max(np.abs(uo))
 
 
-@pytest.mark.parametrize("geo,example,n_refine,solver,limiter", [
-    ("TwoTriangles", "sharp_boundary_layer_2d", 5, "direct", "kuzmin"),
-    ("UnitSquare", "sharp_boundary_layer_2d", 5, "fgmres", "kuzmin"),
-    ("Default_UnitCube_Hexa", "smooth_solution_3d", 3, "fgmres", "kuzmin"),
-    ("TwoTriangles", "sharp_boundary_layer_2d", 5, "fgmres", "BJK17"),
-    ("Default_UnitCube_Tetra", "smooth_solution_3d", 4, "fgmres", "BJK17"),
+@pytest.mark.parametrize("geo,example,n_refine,solver,limiter,scheme", [
+    ("TwoTriangles", "sharp_boundary_layer_2d", 5, "direct", "kuzmin", "fixed_point_rhs"),
+    ("UnitSquare", "sharp_boundary_layer_2d", 5, "fgmres", "kuzmin", "fixed_point_rhs"),
+    ("Default_UnitCube_Hexa", "smooth_solution_3d", 3, "fgmres", "kuzmin", "fixed_point_rhs"),
+    ("TwoTriangles", "sharp_boundary_layer_2d", 5, "fgmres", "BJK17", "fixed_point_rhs"),
+    ("Default_UnitCube_Tetra", "smooth_solution_3d", 4, "fgmres", "BJK17", "fixed_point_rhs"),
+    ("TwoTriangles", "sharp_boundary_layer_2d", 5, "fgmres", "kuzmin", "fixed_point_matrix"),
+    ("UnitSquare", "sharp_boundary_layer_2d", 5, "direct", "kuzmin", "fixed_point_matrix"),
 ])
-def test_afc_matches_oracle(oracle_lib, geo, example, n_refine, solver, limiter):
+def test_afc_matches_oracle(oracle_lib, geo, example, n_refine, solver, limiter, scheme):
     """Finer meshes, and FGMRES with Jacobi preconditioning for the linear
     systems (S = A + D is fixed over the loop), converged to epsilon 1e-13 so
     that rounding in the linear solves cannot move the stopping step: the
     same fixed point as the oracle's, reached in about as many iterations."""
-    print("afc vs oracle", geo, example, n_refine, solver, limiter, flush=True)
-    extra = {"afc_nonlinloop_epsilon": 1e-13, "afc_limiter": limiter}
+    print("afc vs oracle", geo, example, n_refine, solver, limiter, scheme, flush=True)
+    extra = {"afc_nonlinloop_epsilon": 1e-13, "afc_limiter": limiter, "afc_iteration_scheme": scheme}
     if solver == "fgmres":
         extra.update({"solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "jacobi",
                       "residual_tolerance": 1e-15, "residual_reduction": 0.0, "max_n_iterations": 2000,

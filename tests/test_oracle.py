@@ -495,6 +495,20 @@ AFC_GOLDEN = {
                                                            1.0], 1e-8),
     ("Default_UnitCube_Tetra", "smooth_solution_3d", "BJK17"): ([0.0004164209086368, 0.010287281106288], 1e-11),
 }
+AFC_GOLDEN = {k + ("fixed_point_rhs",): v for k, v in AFC_GOLDEN.items()}
+# the fixed point matrix scheme (cd2d_afc_test.c++:141-147,164-168;
+# cd3d_afc_test.c++:109-115,130-135); errors of the third and fourth kind are
+# checked only for the fixed point rhs cases above
+AFC_GOLDEN.update({
+    ("TwoTriangles", "sharp_boundary_layer_2d", "kuzmin", "fixed_point_matrix"):
+        ([0.533001591476, 2.2679598714175], 1e-8),
+    ("UnitSquare", "sharp_boundary_layer_2d", "kuzmin", "fixed_point_matrix"): ([0.5246449330415, 2.2075436769812], 1e-8),
+    ("Default_UnitCube_Tetra", "smooth_solution_3d", "kuzmin", "fixed_point_matrix"):
+        ([0.0010369624396964, 0.011894928040666], 1e-11),
+    ("TwoTriangles", "sharp_boundary_layer_2d", "BJK17", "fixed_point_matrix"): ([0.54553405576562, 2.5382226446816], 1e-8),
+    ("Default_UnitCube_Tetra", "smooth_solution_3d", "BJK17", "fixed_point_matrix"):
+        ([0.0004164206690843, 0.010287285812062], 1e-11),
+})
 
 
 def afc_db(geo, example, n_refine=3, **extra):
@@ -512,12 +526,12 @@ def afc_errors(e, n):
     return [e[0], e[1], e[2], e[4]][:n]
 
 
-@pytest.mark.parametrize("geo,example,limiter", list(AFC_GOLDEN))
-def test_oracle_afc_golden(oracle_lib, geo, example, limiter):
-    db = afc_db(geo, example, afc_limiter=limiter)
+@pytest.mark.parametrize("geo,example,limiter,scheme", list(AFC_GOLDEN))
+def test_oracle_afc_golden(oracle_lib, geo, example, limiter, scheme):
+    db = afc_db(geo, example, afc_limiter=limiter, afc_iteration_scheme=scheme)
     o = oracle_lib.Oracle(pm.problem_desc(db))
     it, res, rejected, res0 = o.afc_solve(pm.solver_desc(db), pm.afc_params(db))
-    gold, tol = AFC_GOLDEN[(geo, example, limiter)]
+    gold, tol = AFC_GOLDEN[(geo, example, limiter, scheme)]
     got = afc_errors(o.errors(), len(gold))
     assert np.allclose(got, gold, rtol=0, atol=tol), (got, gold)
     # stopped by the reference's criterion: residual < epsilon sqrt(n_dof)

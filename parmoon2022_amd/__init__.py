@@ -817,6 +817,18 @@ def get_sell_min_rows():
     return v.value
 
 
+def set_sweep_block_rows(n):
+    """Exact SOR sweeps whose largest class has at most n rows run as one
+    single-workgroup launch per sweep (0: one launch per class always)."""
+    _check(lib().pmcd_set_tuning(12, int(n)))
+
+
+def get_sweep_block_rows():
+    v = ctypes.c_int(0)
+    _check(lib().pmcd_get_tuning(12, ctypes.byref(v)))
+    return v.value
+
+
 def gpu_available():
     n = ctypes.c_int(0)
     try:

@@ -352,6 +352,37 @@ __global__ __launch_bounds__(256) void k_sor_rows(int count, const int* __restri
   x[r] = x[r] + omega * s / diag[r];
 }
 
+// A whole exact (level-scheduled) sweep in one workgroup: the classes in
+// order (forward), in reverse order (backward) or both, a barrier between
+// classes. Each row is updated exactly as k_sor_rows updates it, so the
+// result is the same bit for bit; on small levels this replaces one launch
+// per class by one launch per sweep.
+template <class T>
+__global__ __launch_bounds__(1024) void k_sor_sweep_block(int K, const int* __restrict__ class_ptr,
+                                                          const int* __restrict__ rows,
+                                                          const int* __restrict__ rp,
+                                                          const int* __restrict__ col,
+                                                          const T* __restrict__ val,
+                                                          const T* __restrict__ diag,
+                                                          const T* __restrict__ b, T* x, T omega, int flag,
+                                                          const int* __restrict__ row_owner,
+                                                          const uint8_t* __restrict__ owner_active) {
+  const int n_pass = flag == 2 ? 2 * K : K;
+  for(int p = 0; p < n_pass; ++p) {
+    const int k = flag == 1 ? K - 1 - p : (p < K ? p : 2 * K - 1 - p);
+    const int e = class_ptr[k + 1];
+    for(int i = class_ptr[k] + (int)threadIdx.x; i < e; i += 1024) {
+      const int r = rows[i];
+      if(row_owner && !owner_active[row_owner[r]]) continue;
+      T s = b[r];
+      const int re = rp[r + 1];
+      for(int q = rp[r]; q < re; ++q) s -= val[q] * x[col[q]];
+      x[r] = x[r] + omega * s / diag[r];
+    }
+    __syncthreads();
+  }
+}
+
 // Multicolour variant: W lanes share a row (unrolled loads, butterfly sum),
 // then x[row] += omega * (b - sum) / diag. The row sum is rounded in a
 // different order than the serial sweep's b - a_1 x_1 - a_2 x_2 ...
@@ -839,6 +870,10 @@ int assembly_variant() { return g_assembly_variant; }
 // SOR sweeps follow the exact level schedule when it has at most this many
 // classes, greedy multicolour order otherwise
 static int g_sor_exact_limit = 24;
+// exact sweeps whose largest class has at most this many rows run in one
+// workgroup (one row per thread and class); larger classes are latency-bound
+// there and keep one launch per class
+static int g_sweep_block_rows = 1024;
 int sor_exact_limit() { return g_sor_exact_limit; }
 
 // V-cycles replayed from captured HIP graphs (1) or launched kernel by kernel
@@ -882,6 +917,11 @@ void set_tuning(int key, int value) {
     set_sell_min_rows(value);
     return;
   }
+  if(key == 12) {
+    PMCD_CHECK(value >= 0, "the one-launch sweep class-size limit must be non-negative");
+    g_sweep_block_rows = value;
+    return;
+  }
   if(key == 5) {
     PMCD_CHECK(value >= 0, "the exact-sweep class limit must be non-negative");
     g_sor_exact_limit = value;
@@ -919,6 +959,7 @@ int get_tuning(int key) {
     case 9: return sell_unroll();
     case 10: return get_sell_window_slices();
     case 11: return sell_min_rows();
+    case 12: return sweep_block_rows();
     default: PMCD_FAIL("unknown tuning key ", key);
   }
 }
@@ -1165,6 +1206,25 @@ void launch_sor_rows(int count, const int* rows, const int* rp, const int* col, 
   dispatch_sor_rows<double>(count, rows, rp, col, val, diag, b, x, omega, serial, max_row, mean_row, s,
                             row_owner, owner_active);
 }
+int sweep_block_rows() { return g_sweep_block_rows; }
+
+void launch_sor_sweep_block(int K, const int* class_ptr, const int* rows, const int* rp, const int* col,
+                            const double* val, const double* diag, const double* b, double* x, double omega,
+                            int flag, hipStream_t s, const int* row_owner, const uint8_t* owner_active) {
+  if(K <= 0) return;
+  k_sor_sweep_block<double><<<1, 1024, 0, s>>>(K, class_ptr, rows, rp, col, val, diag, b, x, omega, flag,
+                                               row_owner, owner_active);
+  PMCD_HIP(hipGetLastError());
+}
+void launch_sor_sweep_block_f32(int K, const int* class_ptr, const int* rows, const int* rp, const int* col,
+                                const float* val, const float* diag, const float* b, float* x, float omega,
+                                int flag, hipStream_t s, const int* row_owner, const uint8_t* owner_active) {
+  if(K <= 0) return;
+  k_sor_sweep_block<float><<<1, 1024, 0, s>>>(K, class_ptr, rows, rp, col, val, diag, b, x, omega, flag,
+                                              row_owner, owner_active);
+  PMCD_HIP(hipGetLastError());
+}
+
 void launch_sor_rows_f32(int count, const int* rows, const int* rp, const int* col, const float* val,
                          const float* diag, const float* b, float* x, float omega, bool serial,
                          int max_row, double mean_row, hipStream_t s, const int* row_owner,

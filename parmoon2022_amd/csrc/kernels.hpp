@@ -111,6 +111,16 @@ void launch_sor_rows_f32(int count, const int* rows, const int* rp, const int* c
                          int max_row, double mean_row, hipStream_t s, const int* row_owner = nullptr,
                          const uint8_t* owner_active = nullptr);
 int sor_exact_limit();
+// one-workgroup exact sweeps (all classes in one launch) on levels whose
+// largest class has at most this many rows (tuning key 12; 0 switches them
+// off)
+int sweep_block_rows();
+void launch_sor_sweep_block(int K, const int* class_ptr, const int* rows, const int* rp, const int* col,
+                            const double* val, const double* diag, const double* b, double* x, double omega,
+                            int flag, hipStream_t s, const int* row_owner, const uint8_t* owner_active);
+void launch_sor_sweep_block_f32(int K, const int* class_ptr, const int* rows, const int* rp, const int* col,
+                                const float* val, const float* diag, const float* b, float* x, float omega,
+                                int flag, hipStream_t s, const int* row_owner, const uint8_t* owner_active);
 // multicolour class [i0, i0 + count) over the class-ordered matrix copy
 // (sw_rp, sw_col, sw_val: permuted rows, original entry order)
 void launch_sor_rows_perm(int count, int i0, const int* rows, const int* sw_rp, const int* sw_col,

@@ -904,6 +904,10 @@ void prepare_level_sweep(Level& L, hipStream_t stream) {
   L.sweep_ptr = std::move(o.class_ptr);
   L.sweep_exact = o.exact;
   L.sweep_rows.upload(o.rows, stream);
+  L.sweep_ptr_dev.upload(L.sweep_ptr, stream);
+  L.sweep_max_class = 0;
+  for(size_t k = 0; k + 1 < L.sweep_ptr.size(); ++k)
+    L.sweep_max_class = std::max(L.sweep_max_class, L.sweep_ptr[k + 1] - L.sweep_ptr[k]);
   if(!o.exact) {
     const Structure& st = L.st;
     std::vector<int> rp(st.n_rows + 1, 0), cl(st.nnz);
@@ -940,6 +944,11 @@ void level_sor_sweep(Level& L, const double* b, double* x, double omega, int fla
   prepare_level_sweep(L, stream);
   refresh_sweep_values(L, false, stream);
   const int K = (int)L.sweep_ptr.size() - 1;
+  if(L.sweep_exact && L.sweep_max_class <= sweep_block_rows()) {
+    launch_sor_sweep_block(K, L.sweep_ptr_dev.p, L.sweep_rows.p, L.row_ptr.p, L.col.p, L.val.p, L.diag.p, b, x,
+                           omega, flag, stream, row_owner, owner_active);
+    return;
+  }
   const double mean_row = (double)L.st.nnz / std::max(1, L.n());
   auto cls = [&](int k) {
     const int cnt = L.sweep_ptr[k + 1] - L.sweep_ptr[k];
@@ -963,6 +972,11 @@ void level_sor_sweep32(Level& L, const float* b, float* x, float omega, int flag
   prepare_level_sweep(L, stream);
   refresh_sweep_values(L, true, stream);
   const int K = (int)L.sweep_ptr.size() - 1;
+  if(L.sweep_exact && L.sweep_max_class <= sweep_block_rows()) {
+    launch_sor_sweep_block_f32(K, L.sweep_ptr_dev.p, L.sweep_rows.p, L.row_ptr.p, L.col.p, L.val32.p,
+                               L.diag32.p, b, x, omega, flag, stream, row_owner, owner_active);
+    return;
+  }
   const double mean_row = (double)L.st.nnz / std::max(1, L.n());
   auto cls = [&](int k) {
     const int cnt = L.sweep_ptr[k + 1] - L.sweep_ptr[k];

@@ -97,6 +97,8 @@ struct Level {
   // SOR sweep classes (built on first use): class k = sweep_rows[sweep_ptr[k] ..
   // sweep_ptr[k+1]); exact = level schedule (bit-identical to the serial sweep)
   std::vector<int> sweep_ptr;
+  DevBuf<int> sweep_ptr_dev;  // the same class bounds on the device
+  int sweep_max_class = 0;    // rows of the largest class
   DevBuf<int> sweep_rows;
   bool sweep_exact = false;
   // multicolour sweeps read a copy of the matrix with the rows in class order

@@ -971,3 +971,26 @@ def test_geo_file_cd2d_golden(geo_file, builtin, order):
     g = CD2D_GOLDEN[(builtin, order)]
     assert np.allclose([e[0], e[1], e[2], e[4]], g, rtol=0, atol=1e-11)
     assert np.array_equal(a.solution(), b.solution())
+
+
+@pytest.mark.parametrize("flag", [0, 1, 2])
+def test_sor_sweep_one_launch_equals_per_class(flag):
+    """The single-workgroup exact sweep (all classes in one launch) gives the
+    same iterate bit for bit as one launch per class."""
+    print("sor one launch", flag, flush=True)
+    cd = pm.ConvectionDiffusion(SOR_2D)  # 2D P1, 18 classes of at most a few hundred rows
+    cd.assemble()
+    n = cd.info()["n_dof"]
+    rng = np.random.default_rng(7)
+    b = rng.standard_normal(n)
+    x0 = rng.standard_normal(n)
+    keep = pm.get_sweep_block_rows()
+    try:
+        pm.set_sweep_block_rows(1 << 30)
+        one, exact1 = cd.sor_sweep(b, x0, omega=1.2, flag=flag)
+        pm.set_sweep_block_rows(0)
+        per_class, exact2 = cd.sor_sweep(b, x0, omega=1.2, flag=flag)
+    finally:
+        pm.set_sweep_block_rows(keep)
+    assert exact1 and exact2
+    assert np.array_equal(one, per_class)

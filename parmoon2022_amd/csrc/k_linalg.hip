@@ -427,6 +427,35 @@ __global__ __launch_bounds__(256) void k_sor_rows_perm(int count, int i0, const 
   if(l == 0) x[r] = x[r] + omega * (b[r] - s) / diag[r];
 }
 
+// A whole multicolour sweep in one workgroup: the class kernel above run
+// class after class with a barrier between classes (W lanes per row, the
+// same row sums), so the result equals one launch per class bit for bit.
+template <int W, int U, class T>
+__global__ __launch_bounds__(1024) void k_sor_sweep_block_perm(int K, const int* __restrict__ class_ptr,
+                                                               const int* __restrict__ rows,
+                                                               const int* __restrict__ sw_rp,
+                                                               const int* __restrict__ sw_col,
+                                                               const T* __restrict__ sw_val,
+                                                               const T* __restrict__ diag,
+                                                               const T* __restrict__ b, T* x, T omega, int flag,
+                                                               const int* __restrict__ row_owner,
+                                                               const uint8_t* __restrict__ owner_active) {
+  const int n_pass = flag == 2 ? 2 * K : K;
+  const int l = (int)threadIdx.x % W;
+  for(int p = 0; p < n_pass; ++p) {
+    const int k = flag == 1 ? K - 1 - p : (p < K ? p : 2 * K - 1 - p);
+    const int i0 = class_ptr[k], count = class_ptr[k + 1] - i0;
+    for(int li = (int)threadIdx.x / W; li < count; li += 1024 / W) {
+      const int i = i0 + li;
+      const int r = rows[i];
+      if(row_owner && !owner_active[row_owner[r]]) continue;
+      const T s = transfer_row<W, U, T>(sw_rp, sw_col, sw_val, x, i, l);
+      if(l == 0) x[r] = x[r] + omega * (b[r] - s) / diag[r];
+    }
+    __syncthreads();
+  }
+}
+
 // val in class order: permuted row i <- original row rows[i]
 template <class T>
 __global__ __launch_bounds__(256) void k_permute_rows(int n, const int* __restrict__ rows,
@@ -1148,7 +1177,7 @@ static void dispatch_sor_perm(int count, int i0, const int* rows, const int* sw_
                               double mean_row, hipStream_t s, const int* row_owner,
                               const uint8_t* owner_active) {
   if(count <= 0) return;
-  const int W = mean_row <= 12 ? 4 : (mean_row <= 24 ? 8 : 16);
+  const int W = sweep_lanes(mean_row);
   const int u = unroll_for(max_row, W);
   const int blocks = (int)(((long)count * W + 255) / 256);
 #define PMCD_SORP(WW, UU)                                                                      \
@@ -1170,6 +1199,52 @@ static void dispatch_sor_perm(int count, int i0, const int* rows, const int* sw_
 #undef PMCD_SORP_U
 #undef PMCD_SORP
   PMCD_HIP(hipGetLastError());
+}
+
+template <class T>
+static void dispatch_sweep_block_perm(int K, const int* class_ptr, const int* rows, const int* sw_rp,
+                                      const int* sw_col, const T* sw_val, const T* diag, const T* b, T* x, T omega,
+                                      int flag, int max_row, double mean_row, hipStream_t s,
+                                      const int* row_owner, const uint8_t* owner_active) {
+  if(K <= 0) return;
+  const int W = sweep_lanes(mean_row);
+  const int u = unroll_for(max_row, W);
+#define PMCD_SWB(WW, UU)                                                                                   \
+  k_sor_sweep_block_perm<WW, UU, T><<<1, 1024, 0, s>>>(K, class_ptr, rows, sw_rp, sw_col, sw_val, diag, b, x, \
+                                                       omega, flag, row_owner, owner_active)
+#define PMCD_SWB_U(WW)                \
+  switch(u) {                         \
+    case 2: PMCD_SWB(WW, 2); break;   \
+    case 4: PMCD_SWB(WW, 4); break;   \
+    case 8: PMCD_SWB(WW, 8); break;   \
+    case 16: PMCD_SWB(WW, 16); break; \
+    default: PMCD_SWB(WW, 0); break;  \
+  }
+  switch(W) {
+    case 4: PMCD_SWB_U(4); break;
+    case 8: PMCD_SWB_U(8); break;
+    default: PMCD_SWB_U(16); break;
+  }
+#undef PMCD_SWB_U
+#undef PMCD_SWB
+  PMCD_HIP(hipGetLastError());
+}
+
+int sweep_lanes(double mean_row) { return mean_row <= 12 ? 4 : (mean_row <= 24 ? 8 : 16); }
+
+void launch_sor_sweep_block_perm(int K, const int* class_ptr, const int* rows, const int* sw_rp,
+                                 const int* sw_col, const double* sw_val, const double* diag, const double* b,
+                                 double* x, double omega, int flag, int max_row, double mean_row, hipStream_t s,
+                                 const int* row_owner, const uint8_t* owner_active) {
+  dispatch_sweep_block_perm<double>(K, class_ptr, rows, sw_rp, sw_col, sw_val, diag, b, x, omega, flag, max_row,
+                                    mean_row, s, row_owner, owner_active);
+}
+void launch_sor_sweep_block_perm_f32(int K, const int* class_ptr, const int* rows, const int* sw_rp,
+                                     const int* sw_col, const float* sw_val, const float* diag, const float* b,
+                                     float* x, float omega, int flag, int max_row, double mean_row, hipStream_t s,
+                                     const int* row_owner, const uint8_t* owner_active) {
+  dispatch_sweep_block_perm<float>(K, class_ptr, rows, sw_rp, sw_col, sw_val, diag, b, x, omega, flag, max_row,
+                                   mean_row, s, row_owner, owner_active);
 }
 
 void launch_sor_rows_perm(int count, int i0, const int* rows, const int* sw_rp, const int* sw_col,

@@ -111,13 +111,23 @@ void launch_sor_rows_f32(int count, const int* rows, const int* rp, const int* c
                          int max_row, double mean_row, hipStream_t s, const int* row_owner = nullptr,
                          const uint8_t* owner_active = nullptr);
 int sor_exact_limit();
-// one-workgroup exact sweeps (all classes in one launch) on levels whose
-// largest class has at most this many rows (tuning key 12; 0 switches them
-// off)
+// one-workgroup sweeps (all classes in one launch) on levels whose largest
+// class has at most this many rows (exact sweeps) or row lanes (multicolour
+// sweeps, W lanes per row); tuning key 12, 0 switches them off
 int sweep_block_rows();
 void launch_sor_sweep_block(int K, const int* class_ptr, const int* rows, const int* rp, const int* col,
                             const double* val, const double* diag, const double* b, double* x, double omega,
                             int flag, hipStream_t s, const int* row_owner, const uint8_t* owner_active);
+// lanes per row of the multicolour class kernels, from the mean row length
+int sweep_lanes(double mean_row);
+void launch_sor_sweep_block_perm(int K, const int* class_ptr, const int* rows, const int* sw_rp,
+                                 const int* sw_col, const double* sw_val, const double* diag, const double* b,
+                                 double* x, double omega, int flag, int max_row, double mean_row, hipStream_t s,
+                                 const int* row_owner, const uint8_t* owner_active);
+void launch_sor_sweep_block_perm_f32(int K, const int* class_ptr, const int* rows, const int* sw_rp,
+                                     const int* sw_col, const float* sw_val, const float* diag, const float* b,
+                                     float* x, float omega, int flag, int max_row, double mean_row, hipStream_t s,
+                                     const int* row_owner, const uint8_t* owner_active);
 void launch_sor_sweep_block_f32(int K, const int* class_ptr, const int* rows, const int* rp, const int* col,
                                 const float* val, const float* diag, const float* b, float* x, float omega,
                                 int flag, hipStream_t s, const int* row_owner, const uint8_t* owner_active);

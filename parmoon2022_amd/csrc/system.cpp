@@ -944,12 +944,20 @@ void level_sor_sweep(Level& L, const double* b, double* x, double omega, int fla
   prepare_level_sweep(L, stream);
   refresh_sweep_values(L, false, stream);
   const int K = (int)L.sweep_ptr.size() - 1;
+  const double mean_row = (double)L.st.nnz / std::max(1, L.n());
   if(L.sweep_exact && L.sweep_max_class <= sweep_block_rows()) {
     launch_sor_sweep_block(K, L.sweep_ptr_dev.p, L.sweep_rows.p, L.row_ptr.p, L.col.p, L.val.p, L.diag.p, b, x,
                            omega, flag, stream, row_owner, owner_active);
     return;
   }
-  const double mean_row = (double)L.st.nnz / std::max(1, L.n());
+  // multicolour classes: one workgroup per sweep when every class fits one
+  // pass of 1024 threads (W lanes per row)
+  if(!L.sweep_exact && (long)L.sweep_max_class * sweep_lanes(mean_row) <= sweep_block_rows()) {
+    launch_sor_sweep_block_perm(K, L.sweep_ptr_dev.p, L.sweep_rows.p, L.sw_rp.p, L.sw_col.p, L.sw_val.p,
+                                L.diag.p, b, x, omega, flag, L.st.max_row, mean_row, stream, row_owner,
+                                owner_active);
+    return;
+  }
   auto cls = [&](int k) {
     const int cnt = L.sweep_ptr[k + 1] - L.sweep_ptr[k];
     if(L.sweep_exact)
@@ -972,12 +980,18 @@ void level_sor_sweep32(Level& L, const float* b, float* x, float omega, int flag
   prepare_level_sweep(L, stream);
   refresh_sweep_values(L, true, stream);
   const int K = (int)L.sweep_ptr.size() - 1;
+  const double mean_row = (double)L.st.nnz / std::max(1, L.n());
   if(L.sweep_exact && L.sweep_max_class <= sweep_block_rows()) {
     launch_sor_sweep_block_f32(K, L.sweep_ptr_dev.p, L.sweep_rows.p, L.row_ptr.p, L.col.p, L.val32.p,
                                L.diag32.p, b, x, omega, flag, stream, row_owner, owner_active);
     return;
   }
-  const double mean_row = (double)L.st.nnz / std::max(1, L.n());
+  if(!L.sweep_exact && (long)L.sweep_max_class * sweep_lanes(mean_row) <= sweep_block_rows()) {
+    launch_sor_sweep_block_perm_f32(K, L.sweep_ptr_dev.p, L.sweep_rows.p, L.sw_rp.p, L.sw_col.p, L.sw_val32.p,
+                                    L.diag32.p, b, x, omega, flag, L.st.max_row, mean_row, stream, row_owner,
+                                    owner_active);
+    return;
+  }
   auto cls = [&](int k) {
     const int cnt = L.sweep_ptr[k + 1] - L.sweep_ptr[k];
     if(L.sweep_exact)

@@ -994,3 +994,29 @@ def test_sor_sweep_one_launch_equals_per_class(flag):
         pm.set_sweep_block_rows(keep)
     assert exact1 and exact2
     assert np.array_equal(one, per_class)
+
+
+@pytest.mark.parametrize("flag", [0, 2])
+@pytest.mark.parametrize("order,nref", [(1, 2), (2, 2)])
+def test_multicolour_sweep_one_launch_equals_per_class(flag, order, nref):
+    """3D hexahedra (greedy colour classes): the one-workgroup multicolour
+    sweep gives the same iterate bit for bit as one launch per class."""
+    print("multicolour one launch", order, nref, flag, flush=True)
+    db = make_db("Default_UnitCube_Hexa", order, nref, "sine_3d", "supg",
+                 {"diffusion_coefficient": 1e-3, "convection": B_UNIT})
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    n = cd.info()["n_dof"]
+    rng = np.random.default_rng(11)
+    b = rng.standard_normal(n)
+    x0 = rng.standard_normal(n)
+    keep = pm.get_sweep_block_rows()
+    try:
+        pm.set_sweep_block_rows(1 << 30)
+        one, exact1 = cd.sor_sweep(b, x0, omega=1.1, flag=flag)
+        pm.set_sweep_block_rows(0)
+        per_class, exact2 = cd.sor_sweep(b, x0, omega=1.1, flag=flag)
+    finally:
+        pm.set_sweep_block_rows(keep)
+    assert not exact1 and not exact2
+    assert np.array_equal(one, per_class)

@@ -56,6 +56,7 @@ ProblemDesc to_desc(const pmcd_problem_desc& d) {
   for(int k = 0; k < 3; ++k) p.ex.b[k] = d.b[k];
   p.ex.c = d.c;
   p.ex.f0 = d.f0;
+  set_example_time(p.ex, 0.0);
   if(d.geometry == PMCD_MESH_FILE) {
     PMCD_CHECK(d.mesh_file != nullptr, "geometry PMCD_MESH_FILE needs mesh_file");
     p.mesh_file = d.mesh_file;

@@ -46,7 +46,7 @@ void CDSystem::tcd_init(const TcdDesc& t) {
   tcd_desc_ = t;
   tcd_t_ = t.t0;
   tcd_n_ = 0;
-  desc_.ex.t = t.t0;
+  set_example_time(desc_.ex, t.t0);
   if(local_M_.n != local_A_.n) local_M_.alloc(local_A_.n);
   tcd_assemble();
   // initial condition: the exact solution interpolated at the nodal points
@@ -83,7 +83,7 @@ SolveStats CDSystem::tcd_step(const SolverDesc& s) {
   const int eff = sch == 2 ? (tcd_n_ <= 1 ? tcd_desc_.bdf_first : 2) : sch;
   const double tau = tcd_desc_.tau;
   tcd_t_ += tau;
-  desc_.ex.t = tcd_t_;
+  set_example_time(desc_.ex, tcd_t_);
   tcd_assemble();
   Level& F = finest();
   const int n = F.n(), na = F.n_active();

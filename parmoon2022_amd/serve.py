@@ -49,6 +49,21 @@ def estimate_bytes(db):
         n = int(nvert * 8 ** k * (1 if p == 1 else (8 if hexa else 7)))
         row = {4: 27, 10: 65, 8: 27, 27: 125}[nl]
         levels = 1.15
+    elif str(geo).lower().endswith(".geo"):
+        # 2D .GEO: counts line NEL NVT NMT NVE; refinement multiplies cells by 4
+        with open(geo) as f:
+            f.readline()
+            f.readline()
+            nel, nvt, _, nve = (int(v) for v in f.readline().split()[:4])
+        cells = nel * 4 ** k
+        tri = nve == 3
+        nl = {(True, 1): 3, (True, 2): 6, (False, 1): 4, (False, 2): 9}[(tri, p)]
+        # DoF per cell of a refined mesh (vertices, plus edge and cell nodes
+        # for order 2), plus the boundary's share
+        per_cell = {(True, 1): 0.5, (True, 2): 2.0, (False, 1): 1.0, (False, 2): 4.0}[(tri, p)]
+        n = int(cells * per_cell) + nvt * p * 2 ** (k + 1)
+        row = {3: 7, 6: 19, 4: 9, 9: 25}[nl]
+        levels = 1.34
     elif geo in ("Default_UnitCube_Hexa", "Default_UnitCube_Tetra", "StructuredHexa", "StructuredTetra"):
         m = int(db.get("structured_n", 1)) * 2 ** k if geo.startswith("Structured") else 2 ** k
         n = (p * m + 1) ** 3

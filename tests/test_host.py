@@ -347,3 +347,17 @@ def test_geo_file_errors(tmp_path):
     short.write_text("t\nc\n 1 3 0 3 1\nDCORVG\n0 0\n1 0\n0.5 0.5\nKVERT\n1 2 3\nKNPR\n1 1\n")
     with pytest.raises(RuntimeError, match="ends before"):
         pm.HostSetup({"geo_file": str(short), "ANSATZ_ORDER": 1, "example": "sine_2d"})
+
+
+def test_serve_estimate_for_geo_files():
+    """The served stream's memory estimate reads a .GEO file's counts: the
+    unit square as two triangles refined k times has about (2^k + 1)^2 DoF
+    (P1), so the estimate grows fourfold per refinement."""
+    from parmoon2022_amd import serve
+    path = os.path.join(GOLDEN_DIR, "UnitSquare_tria.GEO")
+    e6 = serve.estimate_bytes({"geo_file": path, "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 6})
+    e7 = serve.estimate_bytes({"geo_file": path, "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 7})
+    assert (e7 - (64 << 20)) / (e6 - (64 << 20)) == pytest.approx(4.0, rel=0.01)
+    builtin = serve.estimate_bytes({"geo_file": "TwoTriangles2", "ANSATZ_ORDER": 1,
+                                    "refinement_n_initial_steps": 7})
+    assert e7 == pytest.approx(builtin, rel=0.1)

@@ -351,13 +351,14 @@ def test_geo_file_errors(tmp_path):
 
 def test_serve_estimate_for_geo_files():
     """The served stream's memory estimate reads a .GEO file's counts: the
-    unit square as two triangles refined k times has about (2^k + 1)^2 DoF
-    (P1), so the estimate grows fourfold per refinement."""
+    unit square as two triangles refined k times has (2^k + 1)^2 DoF (P1):
+    the estimate grows about fourfold per refinement and stays an upper
+    estimate of the built-in mesh's, within 20%."""
     from parmoon2022_amd import serve
     path = os.path.join(GOLDEN_DIR, "UnitSquare_tria.GEO")
     e6 = serve.estimate_bytes({"geo_file": path, "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 6})
     e7 = serve.estimate_bytes({"geo_file": path, "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 7})
-    assert (e7 - (64 << 20)) / (e6 - (64 << 20)) == pytest.approx(4.0, rel=0.01)
+    assert 3.5 < (e7 - (64 << 20)) / (e6 - (64 << 20)) <= 4.0
     builtin = serve.estimate_bytes({"geo_file": "TwoTriangles2", "ANSATZ_ORDER": 1,
                                     "refinement_n_initial_steps": 7})
-    assert e7 == pytest.approx(builtin, rel=0.1)
+    assert builtin <= e7 <= 1.2 * builtin

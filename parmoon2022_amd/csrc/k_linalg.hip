@@ -818,26 +818,41 @@ __global__ void k_gj_store(int n, const double* __restrict__ B, const int* __res
   }
 }
 
-// S = factor A + M on active rows, S = A on Dirichlet rows; 8 lanes per row
+// S = factor A + M on active rows, S = A on Dirichlet rows; 8 lanes per row.
+// With sell_val, an active row also goes to its sliced ELL slot (entry k at
+// base + 64 k), as the row gather writes it; the sliced ELL layout holds the
+// active rows only (sell_slot has n_active entries).
 __global__ __launch_bounds__(256) void k_tcd_system(int n, int n_active, const int* __restrict__ rp,
                                                     const double* __restrict__ A, const double* __restrict__ M,
-                                                    double factor, double* __restrict__ S) {
+                                                    double factor, double* __restrict__ S,
+                                                    const int* __restrict__ sell_slot,
+                                                    const long long* __restrict__ sell_off,
+                                                    double* __restrict__ sell_val) {
   const int gt = blockIdx.x * 256 + threadIdx.x;
   const int row = gt / 8, l = gt % 8;
   if(row >= n) return;
   const int b = rp[row], e = rp[row + 1];
-  if(row < n_active)
-    for(int k = b + l; k < e; k += 8) S[k] = factor * A[k] + M[k];
-  else
-    for(int k = b + l; k < e; k += 8) S[k] = A[k];
+  const bool to_sell = sell_val != nullptr && row < n_active;
+  long long base = 0;
+  if(to_sell) {
+    const int t = sell_slot[row];
+    base = sell_off[t / kSellRows] + (t % kSellRows);
+  }
+  for(int k = b + l; k < e; k += 8) {
+    const double v = row < n_active ? factor * A[k] + M[k] : A[k];
+    S[k] = v;
+    if(to_sell) sell_val[base + (long long)(k - b) * kSellRows] = v;
+  }
 }
 
 }  // namespace
 
 void launch_tcd_system(int n, int n_active, const int* rp, const double* A, const double* M, double factor,
-                       double* S, hipStream_t s) {
+                       double* S, hipStream_t s, const int* sell_slot, const long long* sell_off,
+                       double* sell_val) {
   if(n == 0) return;
-  k_tcd_system<<<(int)(((long)n * 8 + 255) / 256), 256, 0, s>>>(n, n_active, rp, A, M, factor, S);
+  k_tcd_system<<<(int)(((long)n * 8 + 255) / 256), 256, 0, s>>>(n, n_active, rp, A, M, factor, S, sell_slot,
+                                                                sell_off, sell_val);
   PMCD_HIP(hipGetLastError());
 }
 

@@ -224,7 +224,8 @@ void launch_afc_bjk_rhs(int n, int n_active, const int* rp, const int* col, cons
                         const double* u, const double* f, double* Rp, double* Rm, double* alpha, double* rhs,
                         hipStream_t s);
 void launch_tcd_system(int n, int n_active, const int* rp, const double* A, const double* M, double factor,
-                       double* S, hipStream_t s);
+                       double* S, hipStream_t s, const int* sell_slot = nullptr,
+                       const long long* sell_off = nullptr, double* sell_val = nullptr);
 
 // ---- batched (segmented) vector kernels ------------------------------------
 // A chunk is a row range [begin, end) owned by one instance.

@@ -115,9 +115,14 @@ SolveStats CDSystem::tcd_step(const SolverDesc& s) {
   const double factor = eff == 0 ? tau : (eff == 1 ? 0.5 * tau : kBdf[2] * tau);
   for(auto& Lp : levels_) {
     Level& L = *Lp;
-    launch_tcd_system(L.n(), L.n_active(), L.row_ptr.p, L.tcd_A.p, L.tcd_M.p, factor, L.val.p, stream_);
+    // a level whose sliced ELL copy exists gets it written here too
+    const bool sell = L.sell_slots >= 0 && L.sell_val.n > 0 &&
+                      L.sell_slot.n == (size_t)std::max(1, L.n_active()) && L.n_active() > 0;
+    launch_tcd_system(L.n(), L.n_active(), L.row_ptr.p, L.tcd_A.p, L.tcd_M.p, factor, L.val.p, stream_,
+                      sell ? L.sell_slot.p : nullptr, sell ? L.sell_off.p : nullptr, sell ? L.sell_val.p : nullptr);
     launch_gather_idx(L.n(), L.diag_pos.p, L.val.p, L.diag.p, stream_);
-    L.sw_fresh = L.sw_fresh32 = L.sell_fresh = L.sell_fresh32 = false;
+    L.sw_fresh = L.sw_fresh32 = L.sell_fresh32 = false;
+    L.sell_fresh = sell;
   }
   coarse_ready_ = false;
   mixed_ready_ = false;

@@ -148,3 +148,24 @@ def test_tcd2d_golden_gpu(oracle_lib, disc):
     l2, h1 = TCD2D_GOLDEN[disc]
     for i, m in enumerate(TCD2D_STEPS):
         assert abs(got[m][0] - l2[i]) < 1e-10 and abs(got[m][1] - h1[i]) < 1e-10, (m, got[m][:2], l2[i], h1[i])
+
+
+def test_tcd_sliced_ell_jacobi_matches_oracle(oracle_lib):
+    """With the sliced ELL Jacobi sweep on every level, the system matrix
+    kernel writes the sliced ELL copy of each step's matrix itself: the
+    device steps still follow the oracle's (W-cycle FGMRES, Jacobi
+    smoothing)."""
+    print("tcd sliced ell", flush=True)
+    db = tcd3d_db("Default_UnitCube_Hexa", "quadratic_space_time", 2, "crank_nicolson", "multigrid",
+                  refinement_n_initial_steps=3, multigrid_n_levels=3, space_discretization_type="supg",
+                  time_step_length=0.05, residual_tolerance=0.0, residual_reduction=1e-14,
+                  multigrid_smoother="jacobi", jacobi_damp=0.8)
+    keep = pm.tuning("jacobi"), pm.get_sell_min_rows()
+    try:
+        pm.set_tuning("jacobi", 16, 4)
+        pm.set_sell_min_rows(0)
+        _, _, diffs = run_pair(oracle_lib, db, n_steps=4)
+    finally:
+        pm.set_tuning("jacobi", *keep[0])
+        pm.set_sell_min_rows(keep[1])
+    assert max(diffs) < 1e-10, diffs

@@ -1170,6 +1170,13 @@ static double sdfem_delta(int dim, double hK, double eps, double b1, double b2, 
   return delta * 1.0;
 }
 
+/* Planted mistakes for the tests that show a check can fail (0: none).
+   1: the SUPG -eps*Laplacian term dropped; 2: the mixed second reference
+   derivatives dropped from the physical Hessian. Set only by tests, through
+   orc_set_fault. */
+static int orc_fault = 0;
+void orc_set_fault(int code) { orc_fault = code; }
+
 /* physical values at quad point: u, ux, uy, uz, uxx, uyy, uzz for all i */
 static void orig_values(const orc_problem* P, const ogeo* g, const double* ref, double (*o)[MAXNL]) {
   const int nl = P->el.nl, d = P->el.dim, s = P->el.shape;
@@ -1182,7 +1189,8 @@ static void orig_values(const orc_problem* P, const ogeo* g, const double* ref, 
       o[3][i] = 0;
       for(int a = 0; a < 2; ++a) {
         const double j0 = g->K[a][0] / g->det, j1 = g->K[a][1] / g->det;
-        o[4 + a][i] = j0 * j0 * ref[3 * nl + i] + 2 * j0 * j1 * ref[4 * nl + i] + j1 * j1 * ref[5 * nl + i];
+        o[4 + a][i] = j0 * j0 * ref[3 * nl + i] + (orc_fault == 2 ? 0.0 : 2 * j0 * j1 * ref[4 * nl + i]) +
+                      j1 * j1 * ref[5 * nl + i];
       }
       o[6][i] = 0;
     } else {
@@ -1195,9 +1203,10 @@ static void orig_values(const orc_problem* P, const ogeo* g, const double* ref, 
       }
       for(int a = 0; a < 3; ++a) {
         const double j0 = g->K[a][0] / g->det, j1 = g->K[a][1] / g->det, j2 = g->K[a][2] / g->det;
-        o[4 + a][i] = j0 * j0 * ref[4 * nl + i] + 2 * j0 * j1 * ref[5 * nl + i] +
-                      2 * j0 * j2 * ref[6 * nl + i] + j1 * j1 * ref[7 * nl + i] +
-                      2 * j1 * j2 * ref[8 * nl + i] + j2 * j2 * ref[9 * nl + i];
+        const double mixed = orc_fault == 2 ? 0.0 : 1.0;
+        o[4 + a][i] = j0 * j0 * ref[4 * nl + i] + mixed * 2 * j0 * j1 * ref[5 * nl + i] +
+                      mixed * 2 * j0 * j2 * ref[6 * nl + i] + j1 * j1 * ref[7 * nl + i] +
+                      mixed * 2 * j1 * j2 * ref[8 * nl + i] + j2 * j2 * ref[9 * nl + i];
       }
     }
   }
@@ -1288,7 +1297,7 @@ static void assemble_level(orc_problem* P, olevel* L, int c_begin, int c_end) {
             const double bgradv = b1 * tx + b2 * ty + b3 * tz;
             F[i] += Mult * (delta * bgradv) * f;
             for(int j = 0; j < nl; ++j) {
-              double val = -eps * (o[4][j] + o[5][j] + o[6][j]);
+              double val = orc_fault == 1 ? 0.0 : -eps * (o[4][j] + o[5][j] + o[6][j]);
               val += b1 * o[1][j] + b2 * o[2][j] + b3 * o[3][j] + cr * o[0][j];
               A[i][j] += val * delta * bgradv * Mult;
             }
@@ -1296,7 +1305,8 @@ static void assemble_level(orc_problem* P, olevel* L, int c_begin, int c_end) {
             const double bgradv = b1 * tx + b2 * ty;
             F[i] += Mult * delta * bgradv * f;
             for(int j = 0; j < nl; ++j) {
-              double val = delta * (-eps * (o[4][j] + o[5][j]) + b1 * o[1][j] + b2 * o[2][j] + cr * o[0][j]) * bgradv;
+              const double lap = orc_fault == 1 ? 0.0 : o[4][j] + o[5][j];
+              double val = delta * (-eps * lap + b1 * o[1][j] + b2 * o[2][j] + cr * o[0][j]) * bgradv;
               val *= Mult;
               A[i][j] += val;
             }

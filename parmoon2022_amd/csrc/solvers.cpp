@@ -94,10 +94,16 @@ void CDSystem::prepare(const SolverDesc& s) {
     }
     mixed_ready_ = true;
   }
-  // sliced ELL values are refreshed here, outside any captured V-cycle
+  // sliced ELL and class-ordered SOR values are refreshed here, outside any
+  // captured V-cycle: a replayed graph reads them but never rewrites them
   if(get_tuning(2) == 4 || get_tuning(3) == 4)
     for(auto& Lp : levels_)
       if(Lp->n_active() >= sell_min_rows()) prepare_level_sell(*Lp, s.mixed != 0, stream_);
+  for(auto& Lp : levels_)
+    if(!Lp->sweep_ptr.empty()) {
+      refresh_sweep_values(*Lp, false, stream_);
+      if(s.mixed && Lp->val32.n == Lp->val.n) refresh_sweep_values(*Lp, true, stream_);
+    }
 }
 
 SolveStats CDSystem::solve(const SolverDesc& s) {

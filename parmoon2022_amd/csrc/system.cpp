@@ -729,7 +729,10 @@ void CDSystem::vcycle_eager(const double* rhs, double* out) {
 // shapes and buffers: it is captured once per (rhs, out, zero-tail) and
 // replayed as one graph launch. Graphs are dropped when the parameters or the
 // launch shapes change; matrix values are read from the same buffers, so
-// reassembly keeps them valid. The sequence holds no memset nodes.
+// reassembly keeps them valid. The derived copies (sliced ELL, class-ordered
+// SOR values, FP32 values) are refreshed by prepare() before any cycle runs,
+// never inside a capture, so a captured graph holds no refresh node and stays
+// valid after the matrix changes. The sequence holds no memset nodes.
 void CDSystem::vcycle(const double* rhs, double* out) {
   // (a coarse smoother loop reads its residual on the host: not capturable)
   if(!vcycle_graphs() || sd_.coarse_smoother != 0) {
@@ -925,7 +928,7 @@ void prepare_level_sweep(Level& L, hipStream_t stream) {
 }
 
 // class-ordered values up to date with val (and val32)
-static void refresh_sweep_values(Level& L, bool f32, hipStream_t stream) {
+void refresh_sweep_values(Level& L, bool f32, hipStream_t stream) {
   if(L.sweep_exact || L.sw_rp.n == 0) return;
   if(!f32 && !L.sw_fresh) {
     launch_permute_rows(L.n(), L.sweep_rows.p, L.row_ptr.p, L.sw_rp.p, L.val.p, L.sw_val.p, stream);

@@ -128,6 +128,8 @@ std::vector<std::unique_ptr<Level>> build_levels(const ProblemDesc& d);
 // SOR sweep of a level's matrix in place (TMatrix::sor_sweep, flag 0 forward,
 // 1 backward, 2 both), on the level's row classes (built on first use)
 void prepare_level_sweep(Level& L, hipStream_t stream);
+// the class-ordered (multicolour) copies of val / val32 brought up to date
+void refresh_sweep_values(Level& L, bool f32, hipStream_t stream);
 // (row_owner, owner_active): rows of inactive owners are skipped (may be null)
 void level_sor_sweep(Level& L, const double* b, double* x, double omega, int flag, hipStream_t stream,
                      const int* row_owner = nullptr, const uint8_t* owner_active = nullptr);

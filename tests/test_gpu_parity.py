@@ -28,6 +28,18 @@ ASSEMBLY_CASES = [
     ("Default_UnitCube_Tetra", 1, 2, "const_rhs", "supg",
      {"diffusion_coefficient": 1e-4, "convection": (0.3, -0.5, 0.81), "reaction": 0.0}),
     ("Default_UnitCube_Tetra", 2, 1, "test_p2", "galerkin", {}),
+    ("Default_UnitCube_Tetra", 2, 1, "const_rhs", "supg",
+     {"diffusion_coefficient": 5e-2, "convection": (0.3, -0.5, 0.81), "reaction": 0.4}),
+    ("TwoTriangles", 2, 3, "const_rhs", "supg",
+     {"diffusion_coefficient": 5e-2, "convection": (0.6, -0.8, 0.0), "reaction": 0.3}),
+    # the benchmarked rule: 27 points (INTERNAL_PROBLEM_LINEAR), whose MFMA
+    # kernel runs a partial last chunk of quadrature points (27 = 3 x 8 + 3)
+    ("Default_UnitCube_Hexa", 2, 1, "sine_3d", "supg",
+     {"diffusion_coefficient": 1e-3, "convection": B_UNIT, "INTERNAL_PROBLEM_LINEAR": 1}),
+    ("Default_UnitCube_Hexa", 2, 3, "sine_3d", "supg",
+     {"diffusion_coefficient": 1e-3, "convection": B_UNIT, "INTERNAL_PROBLEM_LINEAR": 1}),
+    ("Default_UnitCube_Hexa", 1, 2, "sine_3d", "supg",
+     {"diffusion_coefficient": 1e-3, "convection": B_UNIT, "INTERNAL_PROBLEM_LINEAR": 1}),
 ]
 
 
@@ -238,13 +250,15 @@ def c2_db(nref, levels, mixed=0, linear=0, damp=0.5):
             "multigrid_mixed_precision": mixed}
 
 
-@pytest.mark.parametrize("nref,levels,damp", [(3, 2, 0.5), (3, 3, 0.5), (3, 3, 0.8)])
-def test_c2_fgmres_mg_matches_oracle(oracle_lib, nref, levels, damp):
-    """C2 shape at 8^3: Q2 SUPG, FGMRES(30) + V(2,2) Jacobi, direct coarse.
-    Iteration counts agree within rounding-driven drift: 1, or 2% of the
-    count for the slowly converging damping 0.8."""
-    db = c2_db(nref, levels, damp=damp)
-    print("case C2 small", nref, levels, damp, flush=True)
+@pytest.mark.parametrize("nref,levels,damp,linear", [(3, 2, 0.5, 0), (3, 3, 0.5, 0), (3, 3, 0.8, 0),
+                                                     (3, 3, 0.5, 1), (4, 4, 0.5, 1)])
+def test_c2_fgmres_mg_matches_oracle(oracle_lib, nref, levels, damp, linear):
+    """C2 shape at 8^3 and 16^3: Q2 SUPG, FGMRES(30) + V(2,2) Jacobi, direct
+    coarse; linear = 1 is the benchmarked 27-point rule. Iteration counts agree
+    within rounding-driven drift: 1, or 2% of the count for the slowly
+    converging damping 0.8."""
+    db = c2_db(nref, levels, damp=damp, linear=linear)
+    print("case C2 small", nref, levels, damp, linear, flush=True)
     cd = pm.ConvectionDiffusion(db)
     cd.assemble()
     st = cd.solve()
@@ -282,10 +296,13 @@ def test_vcycle_matches_oracle(oracle_lib):
     cd.close()
 
 
-def test_mixed_precision_vcycle_solution():
+@pytest.mark.parametrize("linear", [0, 1])
+def test_mixed_precision_vcycle_solution(oracle_lib, linear):
     """FP32 V-cycle inside FP64 FGMRES: same discrete solution within the
-    outer tolerance, L2 error within 1% of the FP64 path."""
-    d64, d32 = c2_db(3, 3, 0), c2_db(3, 3, 1)
+    outer tolerance, L2 error within 1% of the FP64 path; the mixed solution
+    against the FP64 oracle's to the same bound (linear = 1: the benchmarked
+    27-point rule)."""
+    d64, d32 = c2_db(3, 3, 0, linear=linear), c2_db(3, 3, 1, linear=linear)
     a = pm.ConvectionDiffusion(d64)
     a.assemble()
     sa = a.solve()
@@ -302,6 +319,14 @@ def test_mixed_precision_vcycle_solution():
     assert sb.converged
     assert rel < 1e-7
     assert abs(eb[0] - ea[0]) <= 0.01 * ea[0]
+    o = oracle_lib.Oracle(pm.problem_desc(d64))
+    o.assemble()
+    o.solve(pm.solver_desc(d64))
+    uo = o.solution()
+    rel_o = np.linalg.norm(ub - uo) / np.linalg.norm(uo)
+    print("mixed vs FP64 oracle: rel diff %.3e, L2 %.6e vs %.6e" % (rel_o, eb[0], o.errors()[0]), flush=True)
+    assert rel_o < 1e-7
+    assert abs(eb[0] - o.errors()[0]) <= 0.01 * o.errors()[0]
     a.close()
     b.close()
 
@@ -574,6 +599,47 @@ def test_served_results_same_alone_and_in_the_mix(slots, budget):
         assert r.stats.iterations == its and np.array_equal(r.solution, u), r.id
 
 
+@pytest.mark.parametrize("exact", [True, False], ids=["exact-sor", "multicolour-sor"])
+def test_served_results_match_oracle(oracle_lib, exact):
+    """Every served request (CD2D P2 and CD3D Q1, SUPG, FGMRES + multigrid
+    with the SOR smoother, several running at once) against the oracle's solve
+    of the same problem. With the exact SOR order the device sweeps follow the
+    oracle's serial sweeps, and the solutions agree to 1e-10 relative. In the
+    served default (multicolour classes on deep levels) the sweeps visit rows
+    in another order, so both solve to a 1e-13 residual reduction and agree to
+    1e-8."""
+    import parmoon2022_amd.serve as sv
+    dbs = _serve_dbs()
+    if not exact:
+        dbs = [dict(db, residual_reduction=1e-13) for db in dbs]
+    old = pm.get_sor_exact_limit()
+    if exact:
+        pm.set_sor_exact_limit(1 << 30)
+    try:
+        srv = sv.Server(max_slots=4, keep_solution=True)
+        reqs = [srv.submit(db) for db in dbs]
+        for r in reqs:
+            assert r.done.wait(300), "request %d did not finish" % r.id
+        srv.close()
+    finally:
+        pm.set_sor_exact_limit(old)
+    for r, db in zip(reqs, dbs):
+        assert r.error is None, r.error
+        o = oracle_lib.Oracle(pm.problem_desc(db))
+        o.assemble()
+        so = o.solve(pm.solver_desc(db))
+        uo = o.solution()
+        rel = np.linalg.norm(r.solution - uo) / np.linalg.norm(uo)
+        print("request %d (%s p%d r%d): iterations gpu %d oracle %d, rel %.3e" %
+              (r.id, db["geo_file"], db["ANSATZ_ORDER"], db["refinement_n_initial_steps"], r.stats.iterations,
+               so.iterations, rel), flush=True)
+        assert r.stats.converged and so.converged
+        if exact:
+            assert rel < 1e-10 and abs(r.stats.iterations - so.iterations) <= 1
+        else:
+            assert rel < 1e-8
+
+
 @pytest.mark.parametrize("smoother,mixed", [("jacobi", 0), ("ssor", 0), ("jacobi", 1)])
 def test_vcycle_graph_replays_match_eager(smoother, mixed):
     """Captured V-cycle graphs: every replay (with other device work in
@@ -606,6 +672,39 @@ def test_vcycle_graph_replays_match_eager(smoother, mixed):
     finally:
         pm.set_vcycle_graphs(old)
     cd.close()
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_vcycle_graphs_follow_new_coefficients(mixed):
+    """Graph replay with the SSOR smoother on multicolour levels (whose
+    class-ordered value copies derive from the matrix): after set_data with new
+    coefficients and a reassembly, a solve with graphs gives the bits of the
+    same sequence run kernel by kernel."""
+    db = c2_db(3, 3, mixed=mixed)
+    db.update({"multigrid_smoother": "ssor", "sor_omega": 1.0})
+    db2 = dict(db, diffusion_coefficient=2e-2, convection=(0.8, -0.6, 0.0))
+    old_graphs, old_limit = pm.get_vcycle_graphs(), pm.get_sor_exact_limit()
+    pm.set_sor_exact_limit(4)  # multicolour order on every level
+    try:
+        runs = []
+        for graphs in (False, True):
+            pm.set_vcycle_graphs(graphs)
+            cd = pm.ConvectionDiffusion(db)
+            out = []
+            for d in (db, db2, db):
+                cd.set_data(d)
+                cd.assemble()
+                st = cd.solve(d)
+                out.append((st.iterations, cd.solution()))
+            cd.close()
+            runs.append(out)
+        for (i0, u0), (i1, u1) in zip(*runs):
+            assert i0 == i1 and np.array_equal(u0, u1)
+        # the new coefficients give another solution
+        assert not np.array_equal(runs[0][0][1], runs[0][1][1])
+    finally:
+        pm.set_vcycle_graphs(old_graphs)
+        pm.set_sor_exact_limit(old_limit)
 
 
 def test_c1_full_size_matches_oracle(oracle_lib):
@@ -799,12 +898,14 @@ def test_mesh_file_solve_equals_generated_mesh(tmp_path):
     assert np.array_equal(va, vb) and ia == ib and np.array_equal(ua, ub) and np.array_equal(ea, eb)
 
 
-@pytest.mark.parametrize("cycle,mixed", [("W", 0), ("F", 0), ("W", 1)])
-def test_w_f_cycles_match_oracle(oracle_lib, cycle, mixed):
-    """FGMRES with W and F multigrid cycles (CycleControl) on the C2 shape at
-    8^3 with 3 levels: iterations within one of the oracle's and the solution
-    within 1e-10 (FP64 cycle) or 1e-7 (FP32 cycle) of it."""
-    db = dict(c2_db(3, 3, mixed=mixed), multigrid_cycle_type=cycle)
+@pytest.mark.parametrize("cycle,mixed,linear", [("W", 0, 0), ("F", 0, 0), ("W", 1, 0), ("W", 0, 1), ("F", 1, 1),
+                                                ("V", 1, 1)])
+def test_w_f_cycles_match_oracle(oracle_lib, cycle, mixed, linear):
+    """FGMRES with V, W and F multigrid cycles (CycleControl) on the C2 shape
+    at 8^3 with 3 levels: iterations within one of the oracle's and the
+    solution within 1e-10 (FP64 cycle) or 1e-7 (FP32 cycle) of it; linear = 1
+    is the benchmarked 27-point rule."""
+    db = dict(c2_db(3, 3, mixed=mixed, linear=linear), multigrid_cycle_type=cycle)
     cd = pm.ConvectionDiffusion(db)
     cd.assemble()
     st = cd.solve()

@@ -362,3 +362,31 @@ def test_serve_estimate_for_geo_files():
     builtin = serve.estimate_bytes({"geo_file": "TwoTriangles2", "ANSATZ_ORDER": 1,
                                     "refinement_n_initial_steps": 7})
     assert builtin <= e7 <= 1.2 * builtin
+
+
+# test/structure_test.c++ (test_all_values: rows, columns, entries, entries in
+# row 3) on the unrefined meshes, Dirichlet boundary for the odd problem
+# numbers: UnitSquareIrregular.GEO (:695-740), Default_UnitCube_Hexa
+# (:2036-2060, Neumann) and Default_UnitCube_Tetra (:2109-2150). The entry
+# count of row 3 depends on which DoF the numbering calls 3.
+STRUCTURE_FIXTURES = [
+    ("UnitSquareIrregular.GEO", 1, "sine_2d", (8, 8, 40, 5)),
+    ("UnitSquareIrregular.GEO", 2, "sine_2d", (24, 24, 228, 9)),
+    ("Default_UnitCube_Hexa", 1, "sine_3d", (8, 8, 64, 8)),
+    ("Default_UnitCube_Hexa", 2, "sine_3d", (27, 27, 729, 27)),
+    ("Default_UnitCube_Tetra", 1, "sine_3d", (8, 8, 46, 8)),
+    ("Default_UnitCube_Tetra", 2, "sine_3d", (27, 27, 393, 14)),
+]
+
+
+def _structure_values(rp, col):
+    return (len(rp) - 1, int(col.max()) + 1, len(col), int(rp[4] - rp[3]))
+
+
+@pytest.mark.parametrize("geo,order,ex,expected", STRUCTURE_FIXTURES,
+                         ids=lambda v: str(v).replace(".", "_") if not isinstance(v, tuple) else None)
+def test_structure_test_fixtures_host(geo, order, ex, expected):
+    path = os.path.join(GOLDEN_DIR, geo) if geo.endswith(".GEO") else geo
+    h = pm.HostSetup({"geo_file": path, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 0, "example": ex})
+    rp, col = h.pattern()
+    assert _structure_values(rp, col) == expected

@@ -549,3 +549,17 @@ def test_oracle_afc_fixed_point_and_bounds(oracle_lib):
     u = o.solution()
     assert u.min() >= -1e-12 and u.max() <= 1 + 1e-12, (u.min(), u.max())
     assert res < 1e-13 * np.sqrt(o.info()["n_dof"])
+
+
+@pytest.mark.parametrize("geo,order,expected", [
+    ("Default_UnitCube_Hexa", 1, (8, 8, 64, 8)), ("Default_UnitCube_Hexa", 2, (27, 27, 729, 27)),
+    ("Default_UnitCube_Tetra", 1, (8, 8, 46, 8)), ("Default_UnitCube_Tetra", 2, (27, 27, 393, 14))])
+def test_oracle_structure_test_fixtures(oracle_lib, geo, order, expected):
+    """test/structure_test.c++ (:2036-2150) on the unrefined default cubes,
+    Dirichlet boundary: rows, columns, entries and the entry count of row 3,
+    which depends on the DoF numbering (on the tetrahedral cube the rows hold
+    5 or 8 entries for P1 and 10, 14 or 27 for P2)."""
+    db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 0, "example": "sine_3d"}
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    rp, col = o.pattern()
+    assert (len(rp) - 1, int(col.max()) + 1, len(col), int(rp[4] - rp[3])) == expected

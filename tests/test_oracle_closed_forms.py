@@ -10,7 +10,11 @@ default 1, Database.C:785) is a sum of exact polynomial integrals:
   Lagrange integrals on [-1, 1] (mass, first and second derivative pairs),
   evaluated exactly with numpy polynomials;
 * P1 simplices: textbook barycentric formulas (constant gradients, mass
-  |K| (1 + delta_ij) / ((d + 1)(d + 2))).
+  |K| (1 + delta_ij) / ((d + 1)(d + 2)));
+* P2 simplices: the basis as polynomials in the barycentric coordinates,
+  every term (the SUPG -eps*Laplacian included) integrated exactly; planted
+  mistakes in the oracle (the Laplacian or the mixed second derivatives
+  dropped) make this check fail.
 
 The cell diameter h_K is the longest space diagonal of a hexahedron or
 quadrilateral and the longest edge of a simplex (Hexahedron.C:70-95,
@@ -226,3 +230,174 @@ def test_oracle_supg_matrix_closed_form(oracle_lib, geo, order, nref, ex, eps, b
         scale = np.max(np.abs(Fg[:na]))
         assert np.max(np.abs(rhs[:na] - Fg[:na])) < 1e-12 * scale
         assert np.all(rhs[na:] == 0.0)
+
+
+# ---- P2 simplices: SUPG with the -eps*Laplacian term ---------------------
+#
+# On an affine simplex every P2 basis function is a polynomial of degree 2 in
+# the barycentric coordinates (vertex i: 2 l_i^2 - l_i, edge (i, j): 4 l_i l_j)
+# whose gradient sum_k (d phi / d l_k) grad l_k is linear and whose Laplacian
+# sum_kl (d2 phi / d l_k d l_l) grad l_k . grad l_l is constant. Every term of
+# the SUPG form is then a polynomial in the l_k, integrated exactly by
+# int_K prod l_k^a_k = |K| d! prod a_k! / (d + sum a_k)!.
+
+def _bpoly_mul(p, q):
+    out = {}
+    for a, ca in p.items():
+        for b, cb in q.items():
+            e = tuple(x + y for x, y in zip(a, b))
+            out[e] = out.get(e, 0.0) + ca * cb
+    return out
+
+
+def _bpoly_der(p, k):
+    out = {}
+    for a, ca in p.items():
+        if a[k] > 0:
+            e = list(a)
+            e[k] -= 1
+            out[tuple(e)] = out.get(tuple(e), 0.0) + ca * a[k]
+    return out
+
+
+def _bpoly_int(p, d, vol):
+    tot = 0.0
+    for a, ca in p.items():
+        num = math.factorial(d) * np.prod([math.factorial(x) for x in a])
+        tot += ca * num / math.factorial(d + sum(a))
+    return vol * tot
+
+
+def _bpoly_add(*ps):
+    out = {}
+    for sc, p in ps:
+        for a, c in p.items():
+            out[a] = out.get(a, 0.0) + sc * c
+    return out
+
+
+def p2_simplex_basis(d):
+    """P2 basis (vertices, then edges (i, j), i < j) as barycentric
+    polynomials, with the reference node of each."""
+    nv = d + 1
+    unit = lambda k: tuple(1 if m == k else 0 for m in range(nv))
+    basis, nodes = [], []
+    for i in range(nv):
+        sq = tuple(2 if m == i else 0 for m in range(nv))
+        basis.append({sq: 2.0, unit(i): -1.0})
+        nodes.append(REF_VERTS[0 if d == 2 else 2][i])
+    for i in range(nv):
+        for j in range(i + 1, nv):
+            basis.append({tuple(1 if m in (i, j) else 0 for m in range(nv)): 4.0})
+            nodes.append(0.5 * (REF_VERTS[0 if d == 2 else 2][i] + REF_VERTS[0 if d == 2 else 2][j]))
+    return basis, nodes
+
+
+def p2_simplex_local(V, eps, b, c, f0, delta):
+    d = V.shape[1]
+    nv = d + 1
+    T = np.vstack([np.ones(nv), V.T])
+    g = np.linalg.inv(T)[:, 1:]
+    vol = abs(np.linalg.det(T)) / math.factorial(d)
+    G = g @ g.T
+    bg = g @ b
+    basis, _ = p2_simplex_basis(d)
+    n = len(basis)
+    der = [[_bpoly_der(p, k) for k in range(nv)] for p in basis]
+    bgrad = [_bpoly_add(*[(bg[k], der[i][k]) for k in range(nv)]) for i in range(n)]
+    lap = [sum(G[k, l] * _bpoly_der(der[i][k], l).get((0,) * nv, 0.0) for k in range(nv) for l in range(nv))
+           for i in range(n)]
+    A = np.zeros((n, n))
+    F = np.zeros(n)
+    one = {(0,) * nv: 1.0}
+    for i in range(n):
+        F[i] = f0 * _bpoly_int(basis[i], d, vol) + delta * f0 * _bpoly_int(bgrad[i], d, vol)
+        for j in range(n):
+            stiff = sum(G[k, l] * _bpoly_int(_bpoly_mul(der[j][k], der[i][l]), d, vol)
+                        for k in range(nv) for l in range(nv))
+            conv = _bpoly_int(_bpoly_mul(bgrad[j], basis[i]), d, vol)
+            mass = _bpoly_int(_bpoly_mul(basis[j], basis[i]), d, vol)
+            strong = _bpoly_add((-eps * lap[j], one), (1.0, bgrad[j]), (c, basis[j]))
+            A[i, j] = eps * stiff + conv + c * mass + delta * _bpoly_int(_bpoly_mul(strong, bgrad[i]), d, vol)
+    return A, F
+
+
+P2_SIMPLEX_CASES = [
+    # geo, refinements, eps, b, c, f0 (eps large enough that the Laplacian
+    # term is not negligible next to the convective ones)
+    ("TwoTriangles", 2, 5e-2, (0.6, -0.8, 0.0), 0.3, 1.2),
+    ("Default_UnitCube_Tetra", 1, 5e-2, (0.3, -0.5, 0.81), 0.4, 0.9),
+]
+
+
+def _p2_simplex_worst(oracle_lib, geo, nref, eps, b, c, f0):
+    shape = SHAPE_OF[geo]
+    dim = 3 if shape >= 2 else 2
+    db = {"geo_file": geo, "ANSATZ_ORDER": 2, "refinement_n_initial_steps": nref, "example": "const_rhs",
+          "space_discretization_type": "supg", "diffusion_coefficient": eps, "convection": b,
+          "reaction": c, "rhs_constant": f0}
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    info = o.info()
+    n, na, nl = info["n_dof"], info["n_active"], info["n_local"]
+    nv = dim + 1
+    xyz = o.cell_coords(nv=nv)[:, :, :dim]
+    dof = o.dofmap()
+    rp, col = o.pattern()
+    val = o.matrix_values()
+    rhs = o.rhs()
+    # position of each P2 node in the oracle's local numbering
+    _, nodes = p2_simplex_basis(dim)
+    perm = []
+    for p in nodes:
+        v = oracle_lib.basis(shape, 2, np.r_[p, np.zeros(3 - dim)])[:nl]
+        i = int(np.argmax(np.abs(v)))
+        assert abs(v[i] - 1) < 1e-12 and np.sum(np.abs(v)) - 1 < 1e-12
+        perm.append(i)
+    assert sorted(perm) == list(range(nl))
+    inv = np.argsort(perm)
+    bb = np.array(b[:dim], float)
+    Ag, Fg = {}, np.zeros(n)
+    for cell in range(info["n_cells"]):
+        V = xyz[cell]
+        h = max(np.linalg.norm(V[i] - V[j]) for i in range(nv) for j in range(i + 1, nv))
+        A, F = p2_simplex_local(V, eps, bb, c, f0, supg_delta(h, eps, bb))
+        A, F = A[np.ix_(inv, inv)], F[inv]
+        g = dof[cell]
+        for i in range(nl):
+            if g[i] >= na:
+                continue
+            row = Ag.setdefault(g[i], {})
+            for j in range(nl):
+                row[g[j]] = row.get(g[j], 0.0) + A[i, j]
+            Fg[g[i]] += F[i]
+    worst = 0.0
+    for r in range(na):
+        cols = col[rp[r]:rp[r + 1]]
+        exp = np.array([Ag[r].get(cc, 0.0) for cc in cols])
+        worst = max(worst, np.max(np.abs(val[rp[r]:rp[r + 1]] - exp)) / np.max(np.abs(exp)))
+    rhs_err = np.max(np.abs(rhs[:na] - Fg[:na])) / np.max(np.abs(Fg[:na]))
+    return worst, rhs_err
+
+
+@pytest.mark.parametrize("geo,nref,eps,b,c,f0", P2_SIMPLEX_CASES, ids=[c[0] for c in P2_SIMPLEX_CASES])
+def test_oracle_supg_p2_simplex_closed_form(oracle_lib, geo, nref, eps, b, c, f0):
+    """SUPG matrix and rhs on P2 triangles and tetrahedra against the exact
+    barycentric integrals (the -eps*Laplacian term included)."""
+    worst, rhs_err = _p2_simplex_worst(oracle_lib, geo, nref, eps, b, c, f0)
+    assert worst < 1e-12 and rhs_err < 1e-12, (worst, rhs_err)
+
+
+@pytest.mark.parametrize("fault", [1, 2], ids=["laplacian-dropped", "mixed-derivatives-dropped"])
+@pytest.mark.parametrize("geo,nref,eps,b,c,f0", P2_SIMPLEX_CASES, ids=[c[0] for c in P2_SIMPLEX_CASES])
+def test_oracle_supg_p2_simplex_closed_form_detects_planted_mistakes(oracle_lib, geo, nref, eps, b, c, f0,
+                                                                     fault):
+    """The same check with a mistake planted in the oracle (the SUPG
+    Laplacian term dropped, or the mixed second derivatives dropped from the
+    physical Hessian) fails by many orders of magnitude."""
+    oracle_lib.lib().orc_set_fault(fault)
+    try:
+        worst, _ = _p2_simplex_worst(oracle_lib, geo, nref, eps, b, c, f0)
+    finally:
+        oracle_lib.lib().orc_set_fault(0)
+    assert worst > 1e-6, worst

@@ -50,6 +50,30 @@ def c2_db(args):
             "multigrid_smoother": args.smoother, "sor_omega": args.sor_omega}
 
 
+def oracle_comparison(args, nq, u, errs):
+    """The run's errors and solution against the oracle's full-size FP64
+    results for the same workload (tests/golden/c2_oracle_q{27,64}.*: L2 and
+    H1 errors and the solution at every 256th DoF), or None when this
+    configuration has no such fixture."""
+    import numpy as np
+    if not (args.refine == 6 and args.levels == 5 and args.jacobi_damp == 0.5 and args.smoother == "jacobi"):
+        return None
+    base = os.path.join(ROOT, "tests", "golden", "c2_oracle_q%d" % nq)
+    if not os.path.exists(base + ".json"):
+        return None
+    with open(base + ".json") as f:
+        ref = json.load(f)
+    sample = np.load(base + ".npy")
+    got = u[::ref["sample_stride"]]
+    return {"oracle_l2_error": ref["l2_error"], "oracle_h1_error": ref["h1_error"],
+            "oracle_iterations": ref["iterations"],
+            "l2_error_rel_diff": abs(float(errs[0]) - ref["l2_error"]) / ref["l2_error"],
+            "h1_error_rel_diff": abs(float(errs[1]) - ref["h1_error"]) / ref["h1_error"],
+            "solution_rel_diff_sampled": float(np.linalg.norm(got - sample) / np.linalg.norm(sample)),
+            "sample": "every %dth DoF (%d values) of the FP64 oracle solution" % (ref["sample_stride"], sample.size),
+            "within_1pct_l2": abs(float(errs[0]) - ref["l2_error"]) <= 0.01 * ref["l2_error"]}
+
+
 def cpu_baseline(db, gpu_iters, sample_div, matrices, rhs):
     """Oracle (scalar C port of the reference path) timed on a bounded sample of
     the same workload: assembly of 1/sample_div of the finest cells, and two
@@ -637,6 +661,7 @@ def main():
     total = sum(lat)
     errs = cd.output()
     value = args.steps / total
+    vs_oracle = oracle_comparison(args, info["n_quad"], cd.solution(), errs)
 
     kt = kernel_times(cd, info, args.mixed)
     t_jac, jac_bytes = kt["jacobi_ms"], kt["jacobi_bytes"]
@@ -665,6 +690,7 @@ def main():
         "converged": bool(st.converged),
         "l2_error": float(errs[0]),
         "h1_error": float(errs[1]),
+        "vs_fp64_oracle": vs_oracle,
         "roofline": {"bound": "hbm", "kernel": kt["jacobi_kernel"] + " (finest level)",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS,

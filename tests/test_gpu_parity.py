@@ -1121,3 +1121,40 @@ def test_multicolour_sweep_one_launch_equals_per_class(flag, order, nref):
         pm.set_sweep_block_rows(keep)
     assert not exact1 and not exact2
     assert np.array_equal(one, per_class)
+
+
+def test_device_pointer_spmv_and_dot(oracle_lib):
+    """pmcd_spmv_device on caller-owned device CSR (apply and scaled add, the
+    non-active rows as identity) against the serial row sums, and
+    pmcd_dot_device against a serial dot within its rounding bound."""
+    import ctypes
+    from hip_buffers import DeviceArray
+    cd = pm.ConvectionDiffusion(make_db(*ASSEMBLY_CASES[6]))
+    cd.assemble()
+    i = cd.info()
+    n, na = i["n_dof"], i["n_active"]
+    rp, col = cd.pattern()
+    val = cd.matrix_values()
+    cd.close()
+    rng = np.random.default_rng(17)
+    x, z = rng.standard_normal(n), rng.standard_normal(n)
+    bufs = [DeviceArray(a) for a in (rp.astype(np.int32), col.astype(np.int32), val, x, z, np.zeros(n))]
+    d_rp, d_col, d_val, d_x, d_z, d_y = bufs
+    L = pm.lib()
+    L.pmcd_spmv_device.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 6 + [
+        ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+    for alpha, beta, zz in ((1.0, 0.0, np.zeros(n)), (-0.7, 1.0, z)):
+        assert L.pmcd_spmv_device(n, na, d_rp.ptr, d_col.ptr, d_val.ptr, d_x.ptr, d_z.ptr, d_y.ptr, alpha, beta,
+                                  None) == 0
+        got = d_y.get()
+        ref = oracle_lib.spmv_csr(n, na, rp, col, val, x, zz.copy(), alpha)
+        absrow = oracle_lib.spmv_csr(n, na, rp, col, np.abs(val), np.abs(x), np.zeros(n), 1.0)
+        assert np.all(np.abs(got - ref) <= 4 * 125 * np.finfo(float).eps * (absrow + np.abs(zz)) + 1e-300)
+    L.pmcd_dot_device.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]
+    out = ctypes.c_double()
+    assert L.pmcd_dot_device(n, d_x.ptr, d_z.ptr, ctypes.byref(out), None) == 0
+    exact = float(np.dot(x, z))
+    assert abs(out.value - exact) <= 4 * n * np.finfo(float).eps * float(np.dot(np.abs(x), np.abs(z)))
+    for b in bufs:
+        b.close()

@@ -274,6 +274,15 @@ int pmcd_spmv_device(int n, int n_active, const int* row_ptr, const int* col, co
                      void* stream);
 /* Ddot (Blas1.C:21-38): deterministic two-pass reduction, result to host. */
 int pmcd_dot_device(int n, const double* x, const double* y, double* result, void* stream);
+/* pmcd_dot_device without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * reduction's partial sums and result live in a fixed per-device scratch
+ * array of 2048 doubles (16 KiB, held until the process ends); every n >= 0
+ * fits it, and a reduction that would need more is rejected with an error.
+ * Same kernels and bits as pmcd_dot_device. Calls that overlap on one device
+ * (other streams or threads) share the array and race. */
+int pmcd_dot_device_noalloc(int n, const double* x, const double* y, double* result, void* stream);
 
 /* ---- time-dependent problems ----------------------------------------------
  * TimeConvectionDiffusion<d> (src/System/TimeConvectionDiffusion.C:259-452)

@@ -72,7 +72,7 @@ EXPORTS = [
     "pmcd_get_rhs", "pmcd_solve", "pmcd_get_solution", "pmcd_set_solution", "pmcd_errors",
     "pmcd_apply", "pmcd_sor_sweep", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_set_tuning", "pmcd_get_tuning",
     "pmcd_probe_mfma",
-    "pmcd_spmv_device", "pmcd_dot_device",
+    "pmcd_spmv_device", "pmcd_dot_device", "pmcd_dot_device_noalloc",
     "pmcd_batch_create", "pmcd_batch_destroy", "pmcd_batch_info", "pmcd_batch_assemble",
     "pmcd_batch_solve", "pmcd_batch_get_solution", "pmcd_batch_set_solution", "pmcd_batch_get_matrix",
     "pmcd_batch_get_rhs",

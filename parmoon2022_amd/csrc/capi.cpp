@@ -460,6 +460,22 @@ int pmcd_dot_device(int n, const double* x, const double* y, double* result, voi
   });
 }
 
+int pmcd_dot_device_noalloc(int n, const double* x, const double* y, double* result, void* stream) {
+  return guarded([&] {
+    PMCD_CHECK(n >= 0 && result && (n == 0 || (x && y)), "invalid argument");
+    const int b = reduce_blocks(n);
+    // partials in [0, b), the result at kMaxReduceBlocks
+    PMCD_CHECK(b <= kMaxReduceBlocks && kMaxReduceBlocks + 1 <= kNoAllocScratchDoubles,
+               "reduction of ", n, " entries needs more than the ", kNoAllocScratchDoubles,
+               " scratch doubles");
+    double* scratch = noalloc_scratch();
+    launch_dot(n, x, y, scratch, scratch + kMaxReduceBlocks, (hipStream_t)stream);
+    PMCD_HIP(hipMemcpyAsync(result, scratch + kMaxReduceBlocks, sizeof(double), hipMemcpyDeviceToHost,
+                            (hipStream_t)stream));
+    PMCD_HIP(hipStreamSynchronize((hipStream_t)stream));
+  });
+}
+
 // ---- batched instances -------------------------------------------------------
 int pmcd_batch_create(const pmcd_problem_desc* descs, int n_inst, pmcd_batch** out) {
   return guarded([&] {

@@ -21,7 +21,7 @@ namespace pmcd {
 
 namespace {
 
-constexpr int kRedBlocks = 1024;
+constexpr int kRedBlocks = kMaxReduceBlocks;
 
 // W lanes per row; lanes stride the row, then a butterfly over the W lanes.
 template <int W, class T>

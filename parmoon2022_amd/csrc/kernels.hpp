@@ -164,8 +164,14 @@ void launch_prolong_add_f32(int n, int n_active, long nnz, int max_row, const in
                             const int* col, const float* val, const float* x, float* y, float a,
                             hipStream_t s);
 
-// deterministic reductions: result written to out[0] (device)
+// deterministic reductions: result written to out[0] (device); the first
+// pass writes reduce_blocks(n) partials, at most kMaxReduceBlocks
+constexpr int kMaxReduceBlocks = 1024;
 int reduce_blocks(int n);
+// device scratch of the allocation-free entry point: the reduction partials
+// (at most kMaxReduceBlocks) and the result, per device
+constexpr int kNoAllocScratchDoubles = 2048;
+double* noalloc_scratch();
 void launch_dot(int n, const double* x, const double* y, double* partial, double* out,
                 hipStream_t s);
 void launch_sum_max(int n, int stride, const double* v, double* partial, double* out,

@@ -817,6 +817,31 @@ def get_sell_min_rows():
     return v.value
 
 
+def set_sell_mode(mode):
+    """Sliced ELL launch mode: bit 0 XCD-contiguous slices, bit 1 row-ordered
+    output through LDS (layouts with 4-slice windows). Same bits in every mode."""
+    _check(lib().pmcd_set_tuning(13, int(mode)))
+
+
+def get_sell_mode():
+    v = ctypes.c_int(0)
+    _check(lib().pmcd_get_tuning(13, ctypes.byref(v)))
+    return v.value
+
+
+def set_fgmres_lookahead(factor):
+    """FGMRES queues its next iteration before reading the current Hessenberg
+    column while the residual exceeds `factor` times the stopping threshold
+    (0: never). Results do not depend on it."""
+    _check(lib().pmcd_set_tuning(15, int(factor)))
+
+
+def get_fgmres_lookahead():
+    v = ctypes.c_int(0)
+    _check(lib().pmcd_get_tuning(15, ctypes.byref(v)))
+    return v.value
+
+
 def set_sweep_block_rows(n):
     """Exact SOR sweeps whose largest class has at most n rows run as one
     single-workgroup launch per sweep (0: one launch per class always)."""

@@ -22,6 +22,9 @@ namespace pmcd {
 namespace {
 
 constexpr int kRedBlocks = kMaxReduceBlocks;
+// tuning key 14: the unrolled CSR kernels renumber their blocks to
+// XCD-contiguous row ranges (same bits either way)
+static int g_csr_xcd = 0;
 
 // W lanes per row; lanes stride the row, then a butterfly over the W lanes.
 template <int W, class T>
@@ -211,14 +214,15 @@ __device__ inline T row_sum_u(const int* __restrict__ rp, const int* __restrict_
   else return row_sum_unrolled<W, U, T>(rp, col, val, x, row, l);
 }
 
+// xcd != 0: blocks renumbered to XCD-contiguous row ranges (xcd_block)
 template <int W, int U, class T, bool NT = false>
 __global__ __launch_bounds__(256) void k_spmv_unrolled(int n, int n_active, const int* __restrict__ rp,
                                                        const int* __restrict__ col,
                                                        const T* __restrict__ val,
                                                        const T* __restrict__ x,
                                                        const T* __restrict__ z, T* __restrict__ y,
-                                                       T alpha, T beta) {
-  const int gt = blockIdx.x * 256 + threadIdx.x;
+                                                       T alpha, T beta, int xcd) {
+  const int gt = (xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * 256 + threadIdx.x;
   const int row = gt / W, l = gt % W;
   if(row >= n) return;
   if(row < n_active) {
@@ -237,8 +241,8 @@ __global__ __launch_bounds__(256) void k_jacobi_unrolled(int n, int n_active,
                                                          const T* __restrict__ diag,
                                                          const T* __restrict__ b,
                                                          const T* __restrict__ x,
-                                                         T* __restrict__ x_out, T omega) {
-  const int gt = blockIdx.x * 256 + threadIdx.x;
+                                                         T* __restrict__ x_out, T omega, int xcd) {
+  const int gt = (xcd ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x) * 256 + threadIdx.x;
   const int row = gt / W, l = gt % W;
   if(row >= n) return;
   // the row's own b, diag and x are loaded before the row sum so that their
@@ -605,6 +609,66 @@ __global__ __launch_bounds__(256) void k_mgs_step(int n, const double* __restric
   }
 }
 
+// Sum of the m partials of the previous reduction kernel, in the order of
+// k_final_sum, computed by every block of the consuming kernel (the same bits
+// in every block) and broadcast to its threads: the consumer replaces the
+// one-block k_final_sum launch.
+__device__ inline double block_reduce_partials(int m, const double* __restrict__ partial) {
+  __shared__ double s_tot;
+  double s = 0;
+  for(int i = threadIdx.x; i < m; i += 256) s += partial[i];
+  s = block_sum(s);
+  if(threadIdx.x == 0) s_tot = s;
+  __syncthreads();
+  return s_tot;
+}
+
+// Modified Gram-Schmidt step with the previous dot product's reduction
+// folded in: h = (r, v) from the m partials in `prev` (block 0 stores it in
+// *h_out), r -= h v, then the partials of (r, wn) -- of (r, r) when wn is
+// null -- into `partial`. Same arithmetic as k_final_sum + k_mgs_step.
+__global__ __launch_bounds__(256) void k_mgs_fused(int n, int m, const double* __restrict__ prev,
+                                                   double* __restrict__ h_out, double* __restrict__ r,
+                                                   const double* __restrict__ v,
+                                                   const double* __restrict__ wn,
+                                                   double* __restrict__ partial) {
+  const double h = block_reduce_partials(m, prev);
+  if(blockIdx.x == 0 && threadIdx.x == 0) *h_out = h;
+  const double mh = -h;
+  double s = 0;
+  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const double ri = r[i] + mh * v[i];
+    r[i] = ri;
+    s += ri * (wn ? wn[i] : ri);
+  }
+  s = block_sum(s);
+  if(threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// v = r / |r| with |r|^2 from the m partials in `prev`; block 0 stores |r| in
+// *h_out (the subdiagonal Hessenberg entry).
+__global__ __launch_bounds__(256) void k_scale_norm(int n, int m, const double* __restrict__ prev,
+                                                    double* __restrict__ h_out, const double* __restrict__ r,
+                                                    double* __restrict__ v) {
+  const double hn = sqrt(block_reduce_partials(m, prev));
+  if(blockIdx.x == 0 && threadIdx.x == 0) *h_out = hn;
+  const double a = 1.0 / hn;
+  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) v[i] = r[i] * a;
+}
+
+// x += sum_j y_j z_j (FGMRES update, Utilities_gmres.C:90-108), the terms
+// added in j order into a zero accumulator, then the sum into x: the
+// arithmetic of a fill, k axpys and one more axpy, in one pass.
+__global__ __launch_bounds__(256) void k_fgmres_update(int n, int k, const double* __restrict__ y,
+                                                       const double* const* __restrict__ z,
+                                                       double* __restrict__ x) {
+  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    double acc = 0.0;
+    for(int j = 0; j < k; ++j) acc += y[j] * z[j][i];
+    x[i] += 1.0 * acc;
+  }
+}
+
 __global__ void k_d2f(int n, const double* __restrict__ x, float* __restrict__ y) {
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     y[i] = (float)x[i];
@@ -646,10 +710,10 @@ template <int W, class T, bool NT = false>
 void spmv_unrolled(int u, int blocks, int n, int n_active, const int* rp, const int* col,
                    const T* val, const T* x, const T* z, T* y, T alpha, T beta, hipStream_t s) {
   switch(u) {
-    case 2: k_spmv_unrolled<W, 2, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
-    case 4: k_spmv_unrolled<W, 4, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
-    case 8: k_spmv_unrolled<W, 8, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
-    default: k_spmv_unrolled<W, 16, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta); break;
+    case 2: k_spmv_unrolled<W, 2, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta, g_csr_xcd); break;
+    case 4: k_spmv_unrolled<W, 4, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta, g_csr_xcd); break;
+    case 8: k_spmv_unrolled<W, 8, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta, g_csr_xcd); break;
+    default: k_spmv_unrolled<W, 16, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, x, z, y, alpha, beta, g_csr_xcd); break;
   }
 }
 
@@ -658,10 +722,10 @@ void jacobi_unrolled(int u, int blocks, int n, int n_active, const int* rp, cons
                      const T* val, const T* diag, const T* b, const T* x, T* x_out, T omega,
                      hipStream_t s) {
   switch(u) {
-    case 2: k_jacobi_unrolled<W, 2, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
-    case 4: k_jacobi_unrolled<W, 4, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
-    case 8: k_jacobi_unrolled<W, 8, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
-    default: k_jacobi_unrolled<W, 16, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega); break;
+    case 2: k_jacobi_unrolled<W, 2, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega, g_csr_xcd); break;
+    case 4: k_jacobi_unrolled<W, 4, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega, g_csr_xcd); break;
+    case 8: k_jacobi_unrolled<W, 8, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega, g_csr_xcd); break;
+    default: k_jacobi_unrolled<W, 16, T, NT><<<blocks, 256, 0, s>>>(n, n_active, rp, col, val, diag, b, x, x_out, omega, g_csr_xcd); break;
   }
 }
 
@@ -930,6 +994,11 @@ static int g_cg_blocks = 1024;
 int cg_block_cap() { return g_cg_blocks; }
 // changes whenever a launch shape changes, so that captured graphs are rebuilt
 static int g_tuning_epoch = 0;
+// FGMRES queues the next iteration before reading this one's Hessenberg
+// column while the residual exceeds this many times the stopping threshold
+// (tuning key 15; 0: never)
+static int g_fgmres_lookahead = 4;
+double fgmres_lookahead() { return g_fgmres_lookahead; }
 int tuning_epoch() { return g_tuning_epoch; }
 
 void set_tuning(int key, int value) {
@@ -959,6 +1028,20 @@ void set_tuning(int key, int value) {
   }
   if(key == 11) {
     set_sell_min_rows(value);
+    return;
+  }
+  if(key == 13) {
+    set_sell_mode(value);
+    return;
+  }
+  if(key == 15) {
+    PMCD_CHECK(value >= 0, "FGMRES look-ahead factor must be non-negative");
+    g_fgmres_lookahead = value;
+    return;
+  }
+  if(key == 14) {
+    PMCD_CHECK(value == 0 || value == 1, "CSR XCD switch must be 0 or 1");
+    g_csr_xcd = value;
     return;
   }
   if(key == 12) {
@@ -1004,6 +1087,9 @@ int get_tuning(int key) {
     case 10: return get_sell_window_slices();
     case 11: return sell_min_rows();
     case 12: return sweep_block_rows();
+    case 13: return sell_mode();
+    case 14: return g_csr_xcd;
+    case 15: return g_fgmres_lookahead;
     default: PMCD_FAIL("unknown tuning key ", key);
   }
 }
@@ -1381,6 +1467,25 @@ void launch_mgs_step(int n, const double* h, double* r, const double* v, const d
   const int b = reduce_blocks(n);
   k_mgs_step<<<b, 256, 0, s>>>(n, h, r, v, vn, partial);
   if(vn) k_final_sum<<<1, 256, 0, s>>>(b, partial, out_next);
+}
+void launch_dot_partial(int n, const double* x, const double* y, double* partial, hipStream_t s) {
+  k_dot_partial<<<reduce_blocks(n), 256, 0, s>>>(n, x, y, partial);
+  PMCD_HIP(hipGetLastError());
+}
+void launch_mgs_fused(int n, const double* prev, double* h_out, double* r, const double* v, const double* wn,
+                      double* partial, hipStream_t s) {
+  const int b = reduce_blocks(n);
+  k_mgs_fused<<<b, 256, 0, s>>>(n, b, prev, h_out, r, v, wn, partial);
+  PMCD_HIP(hipGetLastError());
+}
+void launch_scale_norm(int n, const double* prev, double* h_out, const double* r, double* v, hipStream_t s) {
+  const int b = reduce_blocks(n);
+  k_scale_norm<<<b, 256, 0, s>>>(n, b, prev, h_out, r, v);
+  PMCD_HIP(hipGetLastError());
+}
+void launch_fgmres_update(int n, int k, const double* y, const double* const* z, double* x, hipStream_t s) {
+  if(n && k) k_fgmres_update<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, k, y, z, x);
+  PMCD_HIP(hipGetLastError());
 }
 void launch_d2f(int n, const double* x, float* y, hipStream_t s) {
   if(n) k_d2f<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, x, y);

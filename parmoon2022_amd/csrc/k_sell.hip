@@ -38,20 +38,55 @@ __device__ inline T sell_row_sum(const SellView<T>& m, int slot, int len, const 
   return s;
 }
 
+// Launch layout: blocks of 256 threads (4 slices); the first slot_blocks
+// blocks cover the slots, the rest the non-active tail rows in order.
+//
+// mode bit 0 (XCD-contiguous slices): the hardware hands blocks to the 8 XCDs
+// round-robin; the slot blocks are renumbered so that XCD k works through one
+// contiguous range of slices, keeping the x entries its rows share in its own
+// L2 instead of every XCD fetching most of x.
+//
+// mode bit 1 (row-ordered output): a block's 256 slots are the rows of one
+// 256-row window, sorted by length; the row sums go to LDS at the row's
+// position in the window, and the row-wise reads (b, diag, z, the row's own
+// x) and the output are done by thread t for row w0 + t, coalesced, instead
+// of at the sorted positions (partial cache lines, written by several
+// waves). The arithmetic per row is the same, so the bits are too.
+__device__ inline int sell_block(int bid, int slot_blocks, int mode) {
+  if(!(mode & 1) || bid >= slot_blocks) return bid;
+  const int per = slot_blocks / 8;
+  if(bid >= per * 8) return bid;
+  return (bid % 8) * per + bid / 8;
+}
+
 // y[r] = beta z[r] + alpha (A x)[r] on active rows, beta z[r] + alpha x[r] on the tail
 template <int U, class T>
-__global__ __launch_bounds__(256) void k_spmv_sell(int n, int n_active, SellView<T> m, const T* __restrict__ x,
-                                                   const T* __restrict__ z, T* __restrict__ y, T alpha, T beta) {
-  const int gt = blockIdx.x * 256 + threadIdx.x;
-  if(gt < m.n_slots) {
-    const int row = m.perm[gt];
-    if(row < 0) return;
-    const int len = m.len[gt];
-    const T zr = beta == T(0) ? T(0) : z[row];
-    const T s = sell_row_sum<U, T>(m, gt, len, x);
-    y[row] = (beta == T(0) ? T(0) : beta * zr) + alpha * s;
+__global__ __launch_bounds__(256) void k_spmv_sell(int n, int n_active, SellView<T> m, int slot_blocks, int mode,
+                                                   const T* __restrict__ x, const T* __restrict__ z,
+                                                   T* __restrict__ y, T alpha, T beta) {
+  const int blk = sell_block(blockIdx.x, slot_blocks, mode);
+  if(blk < slot_blocks) {
+    const int gt = blk * 256 + threadIdx.x;
+    int row = -1;
+    T s = 0;
+    if(gt < m.n_slots) {
+      row = m.perm[gt];
+      if(row >= 0) s = sell_row_sum<U, T>(m, gt, m.len[gt], x);
+    }
+    if(mode & 2) {
+      __shared__ T sh[256];
+      const int w0 = blk * 256;
+      if(row >= 0) sh[row - w0] = s;
+      __syncthreads();
+      row = w0 + threadIdx.x;
+      if(row >= n_active) return;
+      s = sh[threadIdx.x];
+    } else if(row < 0) {
+      return;
+    }
+    y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * s;
   } else {
-    const int r = n_active + (gt - m.n_slots);
+    const int r = n_active + (blk - slot_blocks) * 256 + threadIdx.x;
     if(r >= n) return;
     y[r] = (beta == T(0) ? T(0) : beta * z[r]) + alpha * x[r];
   }
@@ -59,19 +94,31 @@ __global__ __launch_bounds__(256) void k_spmv_sell(int n, int n_active, SellView
 
 // x_out = x + omega D^{-1} (b - A x), the non-active rows with A = I
 template <int U, class T>
-__global__ __launch_bounds__(256) void k_jacobi_sell(int n, int n_active, SellView<T> m,
+__global__ __launch_bounds__(256) void k_jacobi_sell(int n, int n_active, SellView<T> m, int slot_blocks, int mode,
                                                      const T* __restrict__ diag, const T* __restrict__ b,
                                                      const T* __restrict__ x, T* __restrict__ x_out, T omega) {
-  const int gt = blockIdx.x * 256 + threadIdx.x;
-  int row;
-  T s;
-  if(gt < m.n_slots) {
-    row = m.perm[gt];
-    if(row < 0) return;
-    const int len = m.len[gt];
-    s = sell_row_sum<U, T>(m, gt, len, x);
+  const int blk = sell_block(blockIdx.x, slot_blocks, mode);
+  int row = -1;
+  T s = 0;
+  if(blk < slot_blocks) {
+    const int gt = blk * 256 + threadIdx.x;
+    if(gt < m.n_slots) {
+      row = m.perm[gt];
+      if(row >= 0) s = sell_row_sum<U, T>(m, gt, m.len[gt], x);
+    }
+    if(mode & 2) {
+      __shared__ T sh[256];
+      const int w0 = blk * 256;
+      if(row >= 0) sh[row - w0] = s;
+      __syncthreads();
+      row = w0 + threadIdx.x;
+      if(row >= n_active) return;
+      s = sh[threadIdx.x];
+    } else if(row < 0) {
+      return;
+    }
   } else {
-    row = n_active + (gt - m.n_slots);
+    row = n_active + (blk - slot_blocks) * 256 + threadIdx.x;
     if(row >= n) return;
     s = x[row];
   }
@@ -97,13 +144,24 @@ __global__ __launch_bounds__(256) void k_sell_fill(int n_slots, const long long*
 }
 
 template <class T>
+int sell_slot_blocks(const SellView<T>& m) {
+  return (m.n_slots + 255) / 256;
+}
+
+template <class T>
 int sell_blocks(int n, int n_active, const SellView<T>& m) {
-  return (int)(((long)m.n_slots + (n - n_active) + 255) / 256);
+  return sell_slot_blocks(m) + (n - n_active + 255) / 256;
 }
 
 }  // namespace
 
 static int g_sell_unroll = 16;
+static int g_sell_mode = 3;
+int sell_mode() { return g_sell_mode; }
+void set_sell_mode(int mode) {
+  PMCD_CHECK(mode >= 0 && mode <= 3, "sliced ELL mode must be 0..3");
+  g_sell_mode = mode;
+}
 static int g_sell_window = 4;
 static int g_sell_min_rows = 0;
 int sell_min_rows() { return g_sell_min_rows; }
@@ -123,15 +181,22 @@ void set_sell_unroll(int u) {
   g_sell_unroll = u;
 }
 
+// row-ordered output needs each block's 256 slots to be one whole window of
+// the layout (the window it was built with, not the current setting)
+template <class T>
+static int sell_launch_mode(const SellView<T>& m) {
+  return m.window == 256 ? g_sell_mode : (g_sell_mode & 1);
+}
+
 template <class T>
 void launch_spmv_sell(int n, int n_active, const SellView<T>& m, const T* x, const T* z, T* y, T alpha, T beta,
                       hipStream_t s) {
   if(n == 0) return;
-  const int blocks = sell_blocks(n, n_active, m);
+  const int blocks = sell_blocks(n, n_active, m), sb = sell_slot_blocks(m), mode = sell_launch_mode(m);
   switch(g_sell_unroll) {
-    case 4: k_spmv_sell<4, T><<<blocks, 256, 0, s>>>(n, n_active, m, x, z, y, alpha, beta); break;
-    case 16: k_spmv_sell<16, T><<<blocks, 256, 0, s>>>(n, n_active, m, x, z, y, alpha, beta); break;
-    default: k_spmv_sell<8, T><<<blocks, 256, 0, s>>>(n, n_active, m, x, z, y, alpha, beta); break;
+    case 4: k_spmv_sell<4, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, x, z, y, alpha, beta); break;
+    case 16: k_spmv_sell<16, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, x, z, y, alpha, beta); break;
+    default: k_spmv_sell<8, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, x, z, y, alpha, beta); break;
   }
   PMCD_HIP(hipGetLastError());
 }
@@ -140,11 +205,11 @@ template <class T>
 void launch_jacobi_sell(int n, int n_active, const SellView<T>& m, const T* diag, const T* b, const T* x, T* x_out,
                         T omega, hipStream_t s) {
   if(n == 0) return;
-  const int blocks = sell_blocks(n, n_active, m);
+  const int blocks = sell_blocks(n, n_active, m), sb = sell_slot_blocks(m), mode = sell_launch_mode(m);
   switch(g_sell_unroll) {
-    case 4: k_jacobi_sell<4, T><<<blocks, 256, 0, s>>>(n, n_active, m, diag, b, x, x_out, omega); break;
-    case 16: k_jacobi_sell<16, T><<<blocks, 256, 0, s>>>(n, n_active, m, diag, b, x, x_out, omega); break;
-    default: k_jacobi_sell<8, T><<<blocks, 256, 0, s>>>(n, n_active, m, diag, b, x, x_out, omega); break;
+    case 4: k_jacobi_sell<4, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, diag, b, x, x_out, omega); break;
+    case 16: k_jacobi_sell<16, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, diag, b, x, x_out, omega); break;
+    default: k_jacobi_sell<8, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, diag, b, x, x_out, omega); break;
   }
   PMCD_HIP(hipGetLastError());
 }

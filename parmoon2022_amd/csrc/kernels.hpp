@@ -152,6 +152,8 @@ int cg_device();
 // largest grid of the device CG kernels (1..kCgBlocks)
 int cg_block_cap();
 int tuning_epoch();
+// FGMRES look-ahead factor (tuning key 15): 0 off
+double fgmres_lookahead();
 // restriction y = R x on active rows, 0 on rows >= n_active
 void launch_restrict(int n, int n_active, long nnz, int max_row, const int* rp, const int* col,
                      const double* val, const double* x, double* y, hipStream_t s);
@@ -174,6 +176,17 @@ constexpr int kNoAllocScratchDoubles = 2048;
 double* noalloc_scratch();
 void launch_dot(int n, const double* x, const double* y, double* partial, double* out,
                 hipStream_t s);
+// first pass of launch_dot only: reduce_blocks(n) partials
+void launch_dot_partial(int n, const double* x, const double* y, double* partial, hipStream_t s);
+// modified Gram-Schmidt step on the previous reduction's partials (prev, from
+// a kernel over the same n): h = sum(prev) into *h_out, r -= h v, partials of
+// (r, wn), or of (r, r) when wn is null
+void launch_mgs_fused(int n, const double* prev, double* h_out, double* r, const double* v, const double* wn,
+                      double* partial, hipStream_t s);
+// v = r / sqrt(sum(prev)), the norm into *h_out
+void launch_scale_norm(int n, const double* prev, double* h_out, const double* r, double* v, hipStream_t s);
+// x += sum_{j<k} y[j] z[j] (device arrays y and z)
+void launch_fgmres_update(int n, int k, const double* y, const double* const* z, double* x, hipStream_t s);
 void launch_sum_max(int n, int stride, const double* v, double* partial, double* out,
                     hipStream_t s);
 
@@ -277,6 +290,7 @@ struct SellView {
   const int* len;              // slot -> row length
   const int* col;
   const T* val;
+  int window;                  // rows per length-sorted window of the layout
 };
 template <class T>
 void launch_spmv_sell(int n, int n_active, const SellView<T>& m, const T* x, const T* z, T* y, T alpha, T beta,
@@ -288,6 +302,10 @@ void launch_jacobi_sell(int n, int n_active, const SellView<T>& m, const T* diag
 template <class T>
 void launch_sell_fill(int n_slots, const long long* off, const int* perm, const int* len,
                       const int* rp, const T* src, T* dst, hipStream_t s);
+// sliced ELL launch mode (tuning key 13): bit 0 XCD-contiguous slices, bit 1
+// row-ordered output through LDS (used only with 4-slice windows)
+int sell_mode();
+void set_sell_mode(int mode);
 // entries of a lane loaded per step (4, 8 or 16)
 int sell_unroll();
 void set_sell_unroll(int u);

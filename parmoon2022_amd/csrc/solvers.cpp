@@ -127,9 +127,9 @@ SolveStats CDSystem::solve(const SolverDesc& s) {
 SolveStats CDSystem::solve_direct() {
   Level& F = finest();
   SolveStats st;
-  DevBuf<double> b(F.n());
-  launch_copy(F.n(), F.rhs.p, b.p, stream_);
-  launch_gemv(F.n(), coarse_inv_.p, b.p, F.sol.p, stream_);
+  if(direct_b_.n != (size_t)F.n()) direct_b_.alloc(F.n());
+  launch_copy(F.n(), F.rhs.p, direct_b_.p, stream_);
+  launch_gemv(F.n(), coarse_inv_.p, direct_b_.p, F.sol.p, stream_);
   st.converged = 1;
   return st;
 }
@@ -222,7 +222,10 @@ SolveStats CDSystem::solve_cg() {
     }
     return solve_cg_device(n, normb, resid);
   }
-  DevBuf<double> p(n), z(n), q(n);
+  if(cgh_p_.n != (size_t)n) { cgh_p_.alloc(n); cgh_z_.alloc(n); cgh_q_.alloc(n); }
+  DevBuf<double>& p = cgh_p_;
+  DevBuf<double>& z = cgh_z_;
+  DevBuf<double>& q = cgh_q_;
   rhs_nonactive_zero_ = tail_is_zero(r.p);
   st.initial_residual = resid;
   if(conv(resid, 0)) { st.final_residual = resid; st.converged = 1; return st; }
@@ -332,6 +335,17 @@ SolveStats CDSystem::solve_cg_device(int n, double normb, double resid) {
   return st;
 }
 
+// Flexible GMRES(m) (Iteration_gmres.C:430-620) with modified Gram-Schmidt.
+// Each iteration is one device chain: preconditioner, A z, the Gram-Schmidt
+// steps (each folding the previous dot product's reduction in), the
+// normalised next basis vector, then one copy of the Hessenberg column to
+// pinned host memory. The host applies the Givens rotations and tests
+// convergence. While the residual is far above the stopping threshold
+// (fgmres_lookahead() times it, or below min_n_iterations) the next
+// iteration's chain is queued before the host waits for this one: it needs
+// only device data, and if the test stops the solver first, the queued
+// iteration's results (a basis vector, a Hessenberg column beyond the last
+// one used) are never read.
 SolveStats CDSystem::solve_fgmres() {
   Level& F = finest();
   const int n = F.n();
@@ -343,8 +357,24 @@ SolveStats CDSystem::solve_fgmres() {
     for(int i = 0; i < m; ++i) kz_.emplace_back(n);
     kr_.alloc(n);
     kw_.alloc(n);
+    std::vector<const double*> zp(m);
+    for(int i = 0; i < m; ++i) zp[i] = kz_[i].p;
+    kz_ptr_.upload(zp, stream_);
   }
-  kh_.alloc(m + 2);
+  if(kh_.n != (size_t)m + 2) kh_.alloc(m + 2);
+  if(fg_y_.n != (size_t)m) fg_y_.alloc(m);
+  if(fg_partial_.n != 2 * (size_t)kMaxReduceBlocks) fg_partial_.alloc(2 * kMaxReduceBlocks);
+  if(fg_host_cap_ < m + 2) {
+    if(fg_host_) PMCD_HIP(hipHostFree(fg_host_));
+    fg_host_ = nullptr;
+    PMCD_HIP(hipHostMalloc((void**)&fg_host_, sizeof(double) * 3 * (m + 2), hipHostMallocDefault));
+    fg_host_cap_ = m + 2;
+  }
+  if(!fg_ev_[0])
+    for(auto& e : fg_ev_) PMCD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  double* hcol[2] = {fg_host_, fg_host_ + (m + 2)};
+  double* yhost = fg_host_ + 2 * (m + 2);
+  double* P[2] = {fg_partial_.p, fg_partial_.p + kMaxReduceBlocks};
   std::vector<double> s(m + 1), cs(m + 1), sn(m + 1);
   // Hessenberg matrix, column-major (m+1) x m
   std::vector<double> H((size_t)(m + 1) * m, 0.0);
@@ -361,44 +391,59 @@ SolveStats CDSystem::solve_fgmres() {
   double beta = resid;
   st.initial_residual = resid;
   if(conv(resid, 0)) { st.final_residual = resid; st.converged = 1; return st; }
-  std::vector<double> hcol(m + 2);
+  const double threshold = std::max(sd_.res_tol, sd_.res_red * st.initial_residual);
+  const double lookahead = fgmres_lookahead();
   int j = 1;
   auto update = [&](int k) {
-    // Hessenberg back-solve (Utilities_gmres.C:90-108) and x += sum y_j z_j
-    std::vector<double> y(s.begin(), s.end());
+    // Hessenberg back-solve (Utilities_gmres.C:90-108), then x += sum y_j z_j
+    // in one pass (the host buffer is free again once the copy is done: the
+    // next use follows a synchronising reduction or the end of the solve)
+    for(int i = 0; i <= k; ++i) yhost[i] = s[i];
     for(int i = k; i >= 0; --i) {
-      y[i] /= h(i, i);
-      for(int jj = i - 1; jj >= 0; --jj) y[jj] -= h(jj, i) * y[i];
+      yhost[i] /= h(i, i);
+      for(int jj = i - 1; jj >= 0; --jj) yhost[jj] -= h(jj, i) * yhost[i];
     }
-    launch_fill(n, 0.0, r, stream_);
-    for(int jj = 0; jj <= k; ++jj) launch_axpy(n, y[jj], kz_[jj].p, r, stream_);
-    launch_axpy(n, 1.0, r, x, stream_);
+    PMCD_HIP(hipMemcpyAsync(fg_y_.p, yhost, sizeof(double) * (k + 1), hipMemcpyHostToDevice, stream_));
+    launch_fgmres_update(n, k + 1, fg_y_.p, kz_ptr_.p, x, stream_);
+    PMCD_HIP(hipStreamSynchronize(stream_));
+  };
+  // the device chain of iteration i of the current cycle (v_i given)
+  auto enqueue = [&](int i) {
+    precond(kv_[i].p, kz_[i].p);
+    apply(F, kz_[i].p, r);
+    // modified Gram-Schmidt: H(k,i) = <r, v_k>; r -= H(k,i) v_k
+    launch_dot_partial(n, r, kv_[0].p, P[0], stream_);
+    for(int k = 0; k <= i; ++k)
+      launch_mgs_fused(n, P[k & 1], kh_.p + k, r, kv_[k].p, k < i ? kv_[k + 1].p : nullptr, P[(k + 1) & 1],
+                       stream_);
+    launch_scale_norm(n, P[(i + 1) & 1], kh_.p + i + 1, r, kv_[i + 1].p, stream_);
+    PMCD_HIP(hipMemcpyAsync(hcol[i & 1], kh_.p, sizeof(double) * (i + 2), hipMemcpyDeviceToHost, stream_));
+    PMCD_HIP(hipEventRecord(fg_ev_[i & 1], stream_));
   };
   while(j <= sd_.max_iter) {
     launch_scale_copy(n, 1.0 / beta, r, kv_[0].p, stream_);
     std::fill(s.begin(), s.end(), 0.0);
     s[0] = beta;
+    int queued = -1;
+    double last = beta;
     for(int i = 0; i < m && j <= sd_.max_iter; ++i, ++j) {
-      precond(kv_[i].p, kz_[i].p);
-      apply(F, kz_[i].p, r);
-      // modified Gram-Schmidt: H(k,i) = <r, v_k>; r -= H(k,i) v_k, fused pairwise
-      launch_dot(n, r, kv_[0].p, red_partial_.p, kh_.p, stream_);
-      for(int k = 0; k <= i; ++k)
-        launch_mgs_step(n, kh_.p + k, r, kv_[k].p, k < i ? kv_[k + 1].p : nullptr,
-                        red_partial_.p, kh_.p + k + 1, stream_);
-      launch_dot(n, r, r, red_partial_.p, kh_.p + i + 1, stream_);
-      PMCD_HIP(hipMemcpyAsync(hcol.data(), kh_.p, sizeof(double) * (i + 2), hipMemcpyDeviceToHost,
-                              stream_));
-      PMCD_HIP(hipStreamSynchronize(stream_));
-      for(int k = 0; k <= i; ++k) h(k, i) = hcol[k];
-      h(i + 1, i) = std::sqrt(hcol[i + 1]);
-      if(h(i + 1, i) != 0.0) launch_scale_copy(n, 1.0 / h(i + 1, i), r, kv_[i + 1].p, stream_);
-      else PMCD_FAIL("Unhappy breakdown in flexible gmres at iteration ", j);
+      if(queued < i) { enqueue(i); queued = i; }
+      if(lookahead > 0 && i + 1 < m && j + 1 <= sd_.max_iter &&
+         (j + 1 <= sd_.min_iter || last > lookahead * threshold)) {
+        enqueue(i + 1);
+        queued = i + 1;
+      }
+      PMCD_HIP(hipEventSynchronize(fg_ev_[i & 1]));
+      const double* hc = hcol[i & 1];
+      for(int k = 0; k <= i; ++k) h(k, i) = hc[k];
+      h(i + 1, i) = hc[i + 1];
+      if(!(h(i + 1, i) != 0.0)) PMCD_FAIL("Unhappy breakdown in flexible gmres at iteration ", j);
       for(int k = 0; k < i; ++k) givens_apply(h(k, i), h(k + 1, i), cs[k], sn[k]);
       givens_generate(h(i, i), h(i + 1, i), cs[i], sn[i]);
       givens_apply(h(i, i), h(i + 1, i), cs[i], sn[i]);
       givens_apply(s[i], s[i + 1], cs[i], sn[i]);
       resid = std::abs(s[i + 1]);
+      last = resid;
       st.iterations = j;
       if(conv(resid, j)) {
         update(i);

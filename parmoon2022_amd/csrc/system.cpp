@@ -107,6 +107,9 @@ CDSystem::~CDSystem() {
   (void)hipStreamSynchronize(stream_);
   clear_graphs();
   if(cg_host_) (void)hipHostFree(cg_host_);
+  if(fg_host_) (void)hipHostFree(fg_host_);
+  for(auto& e : fg_ev_)
+    if(e) (void)hipEventDestroy(e);
 }
 
 void CDSystem::clear_graphs() {
@@ -389,12 +392,15 @@ void CDSystem::coarse_factorize() {
   Level& C = *levels_.front();
   const int n = C.n();
   PMCD_CHECK(n <= 20000, "coarse direct solve limited to 20000 unknowns, got ", n);
-  DevBuf<double> A((size_t)n * n), colbuf(2 * (size_t)n);
-  DevBuf<int> perm(n), colmap(n);
+  // work space kept for the next factorization (same size)
+  if(coarse_dense_.n != (size_t)n * n) coarse_dense_.alloc((size_t)n * n);
+  if(coarse_colbuf_.n != 2 * (size_t)n) coarse_colbuf_.alloc(2 * (size_t)n);
+  if(coarse_perm_.n != (size_t)n) { coarse_perm_.alloc(n); coarse_colmap_.alloc(n); }
   if(coarse_inv_.n != (size_t)n * n) coarse_inv_.alloc((size_t)n * n);
-  launch_csr_to_dense(n, C.row_ptr.p, C.col.p, C.val.p, A.p, stream_);
+  launch_csr_to_dense(n, C.row_ptr.p, C.col.p, C.val.p, coarse_dense_.p, stream_);
   // rows n_active.. of a level matrix are identity rows (Dirichlet rows)
-  dense_inverse(n, C.n_active(), A.p, coarse_inv_.p, colbuf.p, perm.p, colmap.p, stream_);
+  dense_inverse(n, C.n_active(), coarse_dense_.p, coarse_inv_.p, coarse_colbuf_.p, coarse_perm_.p,
+                coarse_colmap_.p, stream_);
   if(coarse_tmp_.n != 2 * (size_t)n) coarse_tmp_.alloc(2 * (size_t)n);
   PMCD_HIP(hipStreamSynchronize(stream_));
   coarse_ready_ = true;
@@ -811,7 +817,8 @@ void CDSystem::sor_sweep32(Level& L, const float* b, float* x, float omega, int 
 
 void build_level_sell(Level& L, hipStream_t stream) {
   if(L.sell_slots >= 0) return;
-  SellLayout o = build_sell(L.st, L.n_active(), sell_window());
+  L.sell_window = sell_window();
+  SellLayout o = build_sell(L.st, L.n_active(), L.sell_window);
   PMCD_CHECK(o.slice_off.back() < (1LL << 40), "sliced ELL layout too large");
   std::vector<int> slot_of(std::max(1, L.n_active()), 0);
   for(int t = 0; t < o.n_slots(); ++t)
@@ -848,7 +855,7 @@ void prepare_level_sell(Level& L, bool f32, hipStream_t stream) {
 
 template <class T>
 static SellView<T> sell_view(const Level& L, const T* val) {
-  return SellView<T>{L.sell_slots, L.sell_off.p, L.sell_perm.p, L.sell_len.p, L.sell_col.p, val};
+  return SellView<T>{L.sell_slots, L.sell_off.p, L.sell_perm.p, L.sell_len.p, L.sell_col.p, val, L.sell_window};
 }
 
 // variant 4 on a level with at least sell_min_rows() active rows

@@ -113,6 +113,7 @@ struct Level {
   // sliced ELL copy of the active rows (built on first use; values refreshed
   // after assembly)
   int sell_slots = -1;  // -1: not built
+  int sell_window = 0;  // rows per length-sorted window of the built layout
   DevBuf<long long> sell_off;
   DevBuf<int> sell_perm, sell_len, sell_col, sell_slot;  // sell_slot: row -> slot
   DevBuf<double> sell_val;
@@ -285,6 +286,19 @@ class CDSystem {
   // Krylov work space
   std::vector<DevBuf<double>> kv_, kz_;
   DevBuf<double> kr_, kw_, kh_;
+  // FGMRES: device pointers to the flexible basis, the update coefficients,
+  // ping-pong reduction partials, pinned host Hessenberg columns (two) and
+  // update coefficients, and the events of the two columns' copies
+  DevBuf<const double*> kz_ptr_;
+  DevBuf<double> fg_y_, fg_partial_;
+  double* fg_host_ = nullptr;
+  int fg_host_cap_ = 0;
+  hipEvent_t fg_ev_[2] = {nullptr, nullptr};
+  // CG with a host-side loop (multigrid or SOR preconditioning)
+  DevBuf<double> cgh_p_, cgh_z_, cgh_q_;
+  // coarse inverse and direct solve work space
+  DevBuf<double> coarse_dense_, coarse_colbuf_, direct_b_;
+  DevBuf<int> coarse_perm_, coarse_colmap_;
   // device-resident CG: direction ping-pong, state, partial sums and a
   // pinned host copy of the state
   DevBuf<double> cg_p_[2], cg_z_, cg_q_, cg_pd_, cg_pu_;

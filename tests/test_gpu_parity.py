@@ -789,17 +789,24 @@ def test_cg_device_iteration_limit():
 SELL_CASES = [ASSEMBLY_CASES[k] for k in (0, 3, 5, 6, 9)]
 
 
+@pytest.mark.parametrize("mode,window", [(0, 4), (1, 4), (2, 4), (3, 4), (3, 8), (2, 1)],
+                         ids=lambda v: str(v))
 @pytest.mark.parametrize("unroll", [4, 8, 16])
 @pytest.mark.parametrize("case", SELL_CASES, ids=lambda c: "%s-p%d" % c[:2])
-def test_sell_spmv_and_jacobi_bitwise_serial(oracle_lib, case, unroll):
+def test_sell_spmv_and_jacobi_bitwise_serial(oracle_lib, case, unroll, mode, window):
     """Sliced ELL kernels (variant 4) sum each row serially in CSR column
     order, like TMatrix::multiply: apply, apply_scaled_add and the Jacobi step
-    are bitwise equal to the oracle's serial CSR loops on the same matrix."""
+    are bitwise equal to the oracle's serial CSR loops on the same matrix, in
+    every launch mode (XCD-contiguous slices, row-ordered output through LDS;
+    the latter only on layouts of 4-slice windows, otherwise switched off)."""
     old_s, old_j, old_u = pm.tuning("spmv"), pm.tuning("jacobi"), pm.get_sell_unroll()
+    old_m, old_w = pm.get_sell_mode(), pm.get_sell_window()
     try:
         pm.set_tuning("spmv", 16, 4)
         pm.set_tuning("jacobi", 16, 4)
         pm.set_sell_unroll(unroll)
+        pm.set_sell_mode(mode)
+        pm.set_sell_window(window)
         cd = pm.ConvectionDiffusion(make_db(*case))
         cd.assemble()
         i = cd.info()
@@ -820,6 +827,8 @@ def test_sell_spmv_and_jacobi_bitwise_serial(oracle_lib, case, unroll):
         pm.set_tuning("spmv", *old_s)
         pm.set_tuning("jacobi", *old_j)
         pm.set_sell_unroll(old_u)
+        pm.set_sell_mode(old_m)
+        pm.set_sell_window(old_w)
 
 
 def test_sell_mixed_vcycle_solve():
@@ -1158,3 +1167,27 @@ def test_device_pointer_spmv_and_dot(oracle_lib):
     assert abs(out.value - exact) <= 4 * n * np.finfo(float).eps * float(np.dot(np.abs(x), np.abs(z)))
     for b in bufs:
         b.close()
+
+
+@pytest.mark.parametrize("restart,mixed", [(30, 0), (5, 0), (5, 1)])
+def test_fgmres_lookahead_same_bits(restart, mixed):
+    """FGMRES queues its next iteration ahead of the host's convergence test
+    (a look-ahead factor of the stopping threshold); the solve gives the same
+    iterations and bits with the look-ahead off, at its default, and always
+    on (iterations that turn out not to be needed are discarded), across
+    restarts too."""
+    db = dict(c2_db(3, 3, mixed=mixed), gmres_restart=restart)
+    old = pm.get_fgmres_lookahead()
+    out = []
+    try:
+        for f in (0, old, 1 << 30):
+            pm.set_fgmres_lookahead(f)
+            cd = pm.ConvectionDiffusion(db)
+            cd.assemble()
+            st = cd.solve()
+            out.append((st.iterations, st.final_residual, cd.solution()))
+            cd.close()
+    finally:
+        pm.set_fgmres_lookahead(old)
+    for it, res, u in out[1:]:
+        assert it == out[0][0] and res == out[0][1] and np.array_equal(u, out[0][2])

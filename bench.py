@@ -137,7 +137,11 @@ def kernel_times(cd, info, mixed):
     jac_bytes = (vb + 4) * nnz + 4 * (n + 1) + vb * n * 5
     spmv_bytes = 12 * nnz + 4 * (n + 1) + 16 * n
     copy_bytes = 2 * 8 * (nnz - nnz % 2)
-    asm_flops = (2 * 27 * 27 * 4 + 2 * 4 * 5 * 27) * info["n_quad"] * info["n_cells"]
+    # reference-tensor assembly: local matrices = [cells x 34 component pairs]
+    # x [34 x 27^2] GEMM, rhs = [cells x 4 nq] x [4 nq x 27] GEMM (FP64 MFMA)
+    asm_flops = (2 * 34 * 27 * 27 + 2 * 4 * info["n_quad"] * 27) * info["n_cells"]
+    # its HBM stream: the local matrices and rhs written, vertex coordinates read
+    asm_bytes = (27 * 27 + 27) * 8 * info["n_cells"] + 8 * 3 * 8 * info["n_cells"]
     lanes, variant = pm.tuning("jacobi")
     if variant == 4 and n - info["n_dirichlet"] >= pm.get_sell_min_rows():
         jac_name = "k_jacobi_sell<%d%s>" % (pm.get_sell_unroll(), ",f32" if mixed else ",f64")
@@ -151,9 +155,12 @@ def kernel_times(cd, info, mixed):
         "stream_copy_ms": t_copy, "stream_copy_bytes": copy_bytes,  # the faster of the two copies
         "stream_copy_GBs": copy_bytes / (t_copy * 1e-3) / 1e9,
         "stream_read_GBs": copy_bytes / 2 / (t_read * 1e-3) / 1e9,
+        "assemble_kernel": "k_assemble_tensor<3,27> (finest level)",
         "assemble_local_ms": t_asm,
+        "assemble_flops": asm_flops,
         "assemble_local_TFLOPs": asm_flops / (t_asm * 1e-3) / 1e12,
         "assemble_fp64_frac": asm_flops / (t_asm * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+        "assemble_GBs": asm_bytes / (t_asm * 1e-3) / 1e9,
     }
 
 

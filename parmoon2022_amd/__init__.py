@@ -176,8 +176,16 @@ def get_sor_exact_limit():
 
 
 def set_assembly_variant(variant):
-    """0: reference operation order on vector ALUs; 1: FP64 MFMA (default)."""
+    """0: reference operation order on vector ALUs; 1: per-point FP64 MFMA
+    for Q2 hexahedra; 2 (default): reference tensors, one FP64-MFMA GEMM per
+    level (every element but P1 simplices, which keep a thread per cell)."""
     _check(lib().pmcd_set_tuning(4, int(variant)))
+
+
+def assembly_variant():
+    v = ctypes.c_int(0)
+    _check(lib().pmcd_get_tuning(4, ctypes.byref(v)))
+    return v.value
 
 
 def probe_mfma():

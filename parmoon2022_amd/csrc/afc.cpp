@@ -106,7 +106,7 @@ SolveStats CDSystem::afc_solve(const SolverDesc& s, const double* prm, double* o
   // of Assemble2D/3D), rhs with the Dirichlet values
   boundary_data(L);
   AsmArgs a = asm_args(L);
-  if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
+  assemble_cells(a, stream_);
   launch_gather_matrix(n, n, a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p, L.contrib.p, L.slot.p, local_A_.p,
                        afc_A_.p, stream_);
   launch_gather_rhs(n, na, L.contrib_ptr.p, L.contrib.p, local_rhs_.p, L.extra_ptr.p, L.extra.p, L.dir_vals.p,

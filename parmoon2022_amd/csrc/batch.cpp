@@ -193,6 +193,10 @@ BatchSystem::BatchSystem(const std::vector<ProblemDesc>& descs, hipStream_t stre
   ref_.upload(ref, stream_);
   qw_.upload(qf.w, stream_);
   qp_.upload(qp, stream_);
+  bool constant = true;
+  for(const auto& d : descs) constant = constant && example_constant_matrix_coeffs(d.ex.kind);
+  if(constant)
+    tensor_.upload(build_assembly_tensor(shape_dim(shape), nl, qf.n(), ref.data(), qf.w.data()), stream_);
   local_A_.alloc((size_t)cells * nl * nl);
   local_rhs_.alloc((size_t)cells * nl);
   // reduction chunks: each instance's active range, then its Dirichlet range
@@ -239,10 +243,11 @@ void BatchSystem::assemble() {
   a.ref = ref_.p;
   a.qw = qw_.p;
   a.qp = qp_.p;
+  a.tensor = tensor_.n ? tensor_.p : nullptr;
   a.cell_xyz = G_.cell_xyz.p;
   a.local_A = local_A_.p;
   a.local_rhs = local_rhs_.p;
-  if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
+  assemble_cells(a, stream_);
   launch_gather_matrix(n_, na_, a.nl, G_.row_ptr.p, G_.col.p, G_.contrib_ptr.p, G_.contrib.p,
                        G_.slot.p, local_A_.p, G_.val.p, stream_);
   launch_gather_rhs(n_, na_, G_.contrib_ptr.p, G_.contrib.p, local_rhs_.p, nullptr, nullptr,

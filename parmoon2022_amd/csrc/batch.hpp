@@ -67,7 +67,7 @@ class BatchSystem {
   Level G_;  // the global block-diagonal system
   QuadRule rule_;
   int ncomp_ = 0;
-  DevBuf<double> ref_, qw_, qp_, local_A_, local_rhs_;
+  DevBuf<double> ref_, qw_, qp_, tensor_, local_A_, local_rhs_;
   DevBuf<ExampleParams> cell_ex_;
   DevBuf<int> cell_inst_;
   // segmented reductions

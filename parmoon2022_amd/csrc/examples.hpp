@@ -86,6 +86,19 @@ __host__ __device__ inline void example_smooth3d(double x, double y, double z, d
   e[4] = -2 * y * z * (1 - y) * (1 - z) - 2 * x * z * (1 - x) * (1 - z) - 2 * x * y * (1 - x) * (1 - y);
 }
 
+// whether eps, b and c of the example are constant in space (only f may
+// vary): the local matrices are then combinations of reference tensors
+__host__ __device__ inline bool example_constant_matrix_coeffs(int kind) {
+  switch(kind) {
+    case EX_SINE_LAPLACE_2D: case EX_SINE_2D: case EX_SINE_3D: case EX_CONST_RHS: case EX_TEST_P0_3D:
+    case EX_TEST_P1_3D: case EX_TEST_P2_3D: case EX_TCD_LINEAR_3D: case EX_TCD_QUAD_3D: case EX_TCD_EXP_2D:
+    case EX_SHARP_LAYER_2D: case EX_SMOOTH_3D:
+      return true;
+    default:
+      return false;
+  }
+}
+
 // coefficients at a point: out = {eps, b1, b2, b3, c, f}
 __host__ __device__ inline void example_coeffs(const ExampleParams& ex, double x, double y,
                                                double z, double* out) {

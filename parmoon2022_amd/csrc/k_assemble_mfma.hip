@@ -30,6 +30,8 @@
 // v) is the same product with WM[q][0] = w, WM[q][1+k] = w delta beta_k on the
 // value component only. It is appended as columns 32.. of V, so one pass over
 // the reference table yields stiffness, rhs and mass.
+#include <vector>
+
 #include "kernels.hpp"
 #include "device.hpp"
 
@@ -301,6 +303,191 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
   }
 }
 
+// ---- reference-tensor assembly ---------------------------------------------
+//
+// With constant matrix coefficients (eps, b, c) on an affine cell, W_K of the
+// per-point form above is |det J| times a cell constant Wc_K[r][s] times the
+// quadrature weight, so
+//     A_K[i][j] = |det J| sum_p Wc_K[p] T_p[i][j],
+//     T_p[i][j] = sum_q w_q R[q][r][i] R[q][s][j],   p = (r, s),
+// over the NP component pairs with a nonzero coefficient (every (r, s) but
+// value-test x second-derivative-ansatz). The T_p are cell independent
+// (computed once on the host), and the whole level's local matrices are one
+// GEMM [cells x NP] x [NP x NL^2] on FP64 MFMA. The right-hand side keeps the
+// per-point source f: F_K[i] = sum_{q, r} g_q c_r R[q][r][i] with
+// g_q = w_q |det J| f(x_q), c_0 = 1, c_{1+k} = delta beta_k, a second GEMM
+// [cells x nq NT] x [nq NT x NL].
+//
+// Each wave owns 16 cells (the MFMA M dimension) on its own: 16 lanes build
+// the cells' coefficients in the wave's part of LDS, the wave evaluates the
+// source at the cells' points, then streams the NL^2 output columns in groups
+// of TG 16-wide tiles with the coefficient operand held in registers. MQ
+// bounds the rule's points (the source values of a wave's cells stay in LDS).
+template <int DIM, int NL, int MQ>
+__global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
+  constexpr int NC = DIM == 2 ? 6 : 10;
+  constexpr int NT = 1 + DIM;
+  constexpr int NH = NC - NT;
+  constexpr int NP = NT * NC - NH;            // component pairs
+  constexpr int NPP = (NP + 3) / 4 * 4;       // padded K
+  constexpr int KS = NPP / 4;                 // K steps
+  constexpr int NE = NL * NL;
+  constexpr int NTILE = (NE + 15) / 16;
+  constexpr int TG = NTILE < 6 ? NTILE : 6;    // tiles per group
+  constexpr int RT = (NL + 15) / 16;          // rhs tiles
+  __shared__ double s_W[4][16][NPP];
+  __shared__ double s_c[4][16][NT];
+  __shared__ double s_geo[4][16][13];         // c0[3], C[3][3], |det|
+  __shared__ double s_g[4][16][MQ];
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int cell0 = (blockIdx.x * 4 + wave) * 16;
+  if(cell0 >= a.n_cells) return;  // whole wave: no block-wide barrier follows
+  if(lane < 16) {
+    const int cell = cell0 + lane;
+    double* W = s_W[wave][lane];
+    if(cell < a.n_cells) {
+      GeoM g;
+      geo_m(a.shape, a.cell_xyz + (size_t)cell * a.nv * 3, g);
+      const double ad = fabs(g.det);
+      for(int d = 0; d < 3; ++d) {
+        s_geo[wave][lane][d] = g.c0[d];
+        for(int k = 0; k < 3; ++k) s_geo[wave][lane][3 + 3 * d + k] = g.C[d][k];
+      }
+      s_geo[wave][lane][12] = ad;
+      double M[3][3];
+      for(int k = 0; k < 3; ++k)
+        for(int l = 0; l < 3; ++l) {
+          double t = 0;
+          for(int d = 0; d < DIM; ++d) t += g.G[d][k] * g.G[d][l];
+          M[k][l] = t;
+        }
+      double mlap[6];
+      if(DIM == 2) { mlap[0] = M[0][0]; mlap[1] = 2 * M[0][1]; mlap[2] = M[1][1]; }
+      else {
+        mlap[0] = M[0][0]; mlap[1] = 2 * M[0][1]; mlap[2] = 2 * M[0][2];
+        mlap[3] = M[1][1]; mlap[4] = 2 * M[1][2]; mlap[5] = M[2][2];
+      }
+      // the matrix coefficients are constant: any point of the cell
+      double cf[6];
+      example_coeffs(cell_params(a, cell), g.c0[0], g.c0[1], DIM == 3 ? g.c0[2] : 0.0, cf);
+      const double eps = cf[0], c = cf[4];
+      const double b[3] = {cf[1], cf[2], DIM == 3 ? cf[3] : 0.0};
+      const double delta = a.supg ? sdfem_delta_m(DIM, g.hK, eps, b[0], b[1], b[2]) : 0.0;
+      double beta[3];
+      for(int k = 0; k < 3; ++k) {
+        double t = 0;
+        for(int d = 0; d < DIM; ++d) t += g.G[d][k] * b[d];
+        beta[k] = t;
+      }
+      int p = 0;
+      W[p++] = ad * c;
+      for(int l = 0; l < DIM; ++l) W[p++] = ad * beta[l];
+      for(int k = 0; k < DIM; ++k) {
+        W[p++] = ad * (delta * beta[k] * c);
+        for(int l = 0; l < DIM; ++l) W[p++] = ad * (eps * M[k][l] + delta * beta[k] * beta[l]);
+        for(int t = 0; t < NH; ++t) W[p++] = ad * (-delta * beta[k] * eps * mlap[t]);
+      }
+      for(; p < NPP; ++p) W[p] = 0.0;
+      s_c[wave][lane][0] = 1.0;
+      for(int k = 0; k < DIM; ++k) s_c[wave][lane][1 + k] = delta * beta[k];
+    } else {
+      for(int p = 0; p < NPP; ++p) W[p] = 0.0;
+      for(int r = 0; r < NT; ++r) s_c[wave][lane][r] = 0.0;
+      for(int t = 0; t < 13; ++t) s_geo[wave][lane][t] = 0.0;
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // g_q = w_q |det J| f(x_q) for the wave's cells and points
+  for(int t = lane; t < 16 * a.nq; t += 64) {
+    const int cl = t / a.nq, q = t % a.nq, cell = cell0 + cl;
+    double gq = 0.0;
+    if(cell < a.n_cells) {
+      const double* gm = s_geo[wave][cl];
+      const double xi = a.qp[3 * q], eta = a.qp[3 * q + 1], zeta = a.qp[3 * q + 2];
+      double x[3];
+      for(int d = 0; d < 3; ++d) {
+        x[d] = gm[d] + gm[3 + 3 * d] * xi + gm[3 + 3 * d + 1] * eta;
+        if(DIM == 3) x[d] = x[d] + gm[3 + 3 * d + 2] * zeta;
+      }
+      double cf[6];
+      example_coeffs(cell_params(a, cell), x[0], x[1], DIM == 3 ? x[2] : 0.0, cf);
+      gq = a.qw[q] * gm[12] * cf[5];
+    }
+    s_g[wave][cl][q] = gq;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int l16 = lane % 16, kq = lane / 16;
+  // local matrices: D[cell][e] = sum_p W[cell][p] T[p][e]
+  double av[KS];
+#pragma unroll
+  for(int ks = 0; ks < KS; ++ks) av[ks] = s_W[wave][l16][4 * ks + kq];
+  // the tensor rows of K step ks + 1 are loaded while the MFMAs of step ks run
+  // (the tensor is padded to whole tiles: every load is in bounds)
+  const double* Tbase = a.tensor + (size_t)kq * (NTILE * 16) + l16;
+  for(int t0 = 0; t0 < NTILE; t0 += TG) {
+    v4d acc[TG];
+    double bcur[TG], bnext[TG];
+#pragma unroll
+    for(int u = 0; u < TG; ++u) {
+      acc[u] = v4d{0, 0, 0, 0};
+      bcur[u] = t0 + u < NTILE ? Tbase[(t0 + u) * 16] : 0.0;
+    }
+#pragma unroll
+    for(int ks = 0; ks < KS; ++ks) {
+      if(ks + 1 < KS) {
+        const double* Tn = Tbase + (size_t)(4 * (ks + 1)) * (NTILE * 16);
+#pragma unroll
+        for(int u = 0; u < TG; ++u) bnext[u] = t0 + u < NTILE ? Tn[(t0 + u) * 16] : 0.0;
+      }
+#pragma unroll
+      for(int u = 0; u < TG; ++u)
+        if(t0 + u < NTILE) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[ks], bcur[u], acc[u], 0, 0, 0);
+#pragma unroll
+      for(int u = 0; u < TG; ++u) bcur[u] = bnext[u];
+    }
+#pragma unroll
+    for(int u = 0; u < TG; ++u) {
+      const int e = (t0 + u) * 16 + l16;
+      if(t0 + u >= NTILE || e >= NE) continue;
+#pragma unroll
+      for(int v = 0; v < 4; ++v) {
+        const int cell = cell0 + kq + 4 * v;
+        if(cell < a.n_cells) a.local_A[(size_t)cell * NE + e] = acc[u][v];
+      }
+    }
+  }
+  // right-hand sides: F[cell][i] = sum_{q, r} g[cell][q] c[cell][r] R[q][r][i],
+  // the reference-table operands of 8 K steps loaded before their MFMAs
+  const int K = a.nq * NT;
+#pragma unroll
+  for(int rt = 0; rt < RT; ++rt) {
+    v4d acc = {0, 0, 0, 0};
+    const int i = rt * 16 + l16;
+    for(int k0 = 0; k0 < K; k0 += 32) {
+      double bv[8], avr[8];
+#pragma unroll
+      for(int u = 0; u < 8; ++u) {
+        const int k = k0 + 4 * u + kq, q = k / NT, r = k % NT;
+        bv[u] = (k < K && i < NL) ? a.ref[((size_t)q * NC + r) * NL + i] : 0.0;
+        avr[u] = k < K ? s_g[wave][l16][q] * s_c[wave][l16][r] : 0.0;
+      }
+#pragma unroll
+      for(int u = 0; u < 8; ++u)
+        if(k0 + 4 * u < K) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(avr[u], bv[u], acc, 0, 0, 0);
+    }
+    if(i < NL)
+#pragma unroll
+      for(int v = 0; v < 4; ++v) {
+        const int cell = cell0 + kq + 4 * v;
+        if(cell < a.n_cells) a.local_rhs[(size_t)cell * NL + i] = acc[v];
+      }
+  }
+}
+
 // lane-layout check of the 16x16x4 FP64 MFMA: D = A B with A[i][k] = i + 100 k,
 // B[k][j] = (k == j % 4) ? 1 + j : 0; out[lane*4+v] = D register v of lane
 __global__ void k_mfma_probe(double* out) {
@@ -337,6 +524,70 @@ bool launch_assemble_mfma(const AsmArgs& a, hipStream_t s) {
   }
   PMCD_HIP(hipGetLastError());
   return true;
+}
+
+int tensor_pairs(int dim) { return dim == 2 ? 3 * 6 - 3 : 4 * 10 - 6; }
+
+std::vector<double> build_assembly_tensor(int dim, int nl, int nq, const double* ref, const double* qw) {
+  const int NC = dim == 2 ? 6 : 10, NT = 1 + dim, NH = NC - NT;
+  const int np = tensor_pairs(dim), npp = (np + 3) / 4 * 4;
+  const int ne = nl * nl, nep = (ne + 15) / 16 * 16;
+  std::vector<double> T((size_t)npp * nep, 0.0);
+  int p = 0;
+  for(int r = 0; r < NT; ++r)
+    for(int s = 0; s < NC; ++s) {
+      if(r == 0 && s >= NT) continue;
+      for(int i = 0; i < nl; ++i)
+        for(int j = 0; j < nl; ++j) {
+          double t = 0.0;
+          for(int q = 0; q < nq; ++q)
+            t += qw[q] * ref[((size_t)q * NC + r) * nl + i] * ref[((size_t)q * NC + s) * nl + j];
+          T[(size_t)p * nep + i * nl + j] = t;
+        }
+      ++p;
+    }
+  (void)NH;
+  return T;
+}
+
+template <int MQ>
+static bool launch_tensor_mq(const AsmArgs& a, int blocks, hipStream_t s) {
+  if(a.dim == 3) {
+    switch(a.nl) {
+      case 27: k_assemble_tensor<3, 27, MQ><<<blocks, 256, 0, s>>>(a); break;
+      case 8: k_assemble_tensor<3, 8, MQ><<<blocks, 256, 0, s>>>(a); break;
+      case 10: k_assemble_tensor<3, 10, MQ><<<blocks, 256, 0, s>>>(a); break;
+      case 4: k_assemble_tensor<3, 4, MQ><<<blocks, 256, 0, s>>>(a); break;
+      default: return false;
+    }
+  } else {
+    switch(a.nl) {
+      case 9: k_assemble_tensor<2, 9, MQ><<<blocks, 256, 0, s>>>(a); break;
+      case 6: k_assemble_tensor<2, 6, MQ><<<blocks, 256, 0, s>>>(a); break;
+      case 4: k_assemble_tensor<2, 4, MQ><<<blocks, 256, 0, s>>>(a); break;
+      case 3: k_assemble_tensor<2, 3, MQ><<<blocks, 256, 0, s>>>(a); break;
+      default: return false;
+    }
+  }
+  PMCD_HIP(hipGetLastError());
+  return true;
+}
+
+bool launch_assemble_tensor(const AsmArgs& a, hipStream_t s) {
+  if(a.n_cells == 0) return true;
+  if(!a.tensor || a.tcd || a.nq > 64) return false;
+  // P1 simplices: the thread-per-cell kernel measured faster (P1 tetrahedra
+  // 0.65 vs 0.83 ms for 1.57M cells, P1 triangles 0.45 vs 0.85 ms for 2.1M)
+  if(p1_simplex(a)) return false;
+  const int blocks = (a.n_cells + 63) / 64;
+  return a.nq <= 32 ? launch_tensor_mq<32>(a, blocks, s) : launch_tensor_mq<64>(a, blocks, s);
+}
+
+void assemble_cells(const AsmArgs& a, hipStream_t s) {
+  const int v = assembly_variant();
+  if(v == 2 && launch_assemble_tensor(a, s)) return;
+  if(v >= 1 && launch_assemble_mfma(a, s)) return;
+  launch_assemble_local(a, s);
 }
 
 void launch_mfma_probe(double* out, hipStream_t s) {

@@ -972,7 +972,7 @@ void dense_inverse(int n, int m, double* B, double* inv, double* colbuf, int* pe
 static int g_spmv_width = 16, g_spmv_variant = 2;
 static int g_jacobi_width = 8, g_jacobi_variant = 2;
 
-static int g_assembly_variant = 1;
+static int g_assembly_variant = 2;
 int assembly_variant() { return g_assembly_variant; }
 
 // SOR sweeps follow the exact level schedule when it has at most this many
@@ -1055,7 +1055,7 @@ void set_tuning(int key, int value) {
     return;
   }
   if(key == 4) {
-    PMCD_CHECK(value == 0 || value == 1, "assembly variant must be 0 or 1");
+    PMCD_CHECK(value >= 0 && value <= 2, "assembly variant must be 0, 1 or 2");
     g_assembly_variant = value;
     return;
   }

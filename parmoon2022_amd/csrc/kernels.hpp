@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 #include "examples.hpp"
 
 namespace pmcd {
@@ -32,6 +33,9 @@ struct AsmArgs {
   // the stiffness matrix, local_M the mass matrix
   int tcd = 0;
   double* local_M = nullptr;  // [n_cells][nl][nl]
+  // reference tensors T_p[i][j] of the component pairs, [NP padded to 4]
+  // [nl^2 padded to 16] (null: not available, the per-point kernels run)
+  const double* tensor = nullptr;
 };
 __host__ __device__ inline const ExampleParams& cell_params(const AsmArgs& a, int cell) {
   return a.cell_inst ? a.cell_ex[a.cell_inst[cell]] : a.ex;
@@ -42,9 +46,20 @@ bool p1_simplex(const AsmArgs& a);
 void launch_assemble_local(const AsmArgs& a, hipStream_t s);
 // FP64-MFMA variant; returns false if the element is not covered
 bool launch_assemble_mfma(const AsmArgs& a, hipStream_t s);
+// reference-tensor variant (constant matrix coefficients, affine cells):
+// one FP64-MFMA GEMM per level; returns false if not covered
+bool launch_assemble_tensor(const AsmArgs& a, hipStream_t s);
+// component pairs of the reference tensors in dimension dim, and the tensors
+// themselves ([pairs padded to 4][nl^2 padded to 16]) from the reference
+// table ref[q][component][i] and the weights qw of an nq-point rule
+int tensor_pairs(int dim);
+std::vector<double> build_assembly_tensor(int dim, int nl, int nq, const double* ref, const double* qw);
+// local matrices and rhs of every cell by the variant selected (assembly
+// variant 0: reference-order vector kernels; 1: per-point FP64-MFMA kernel for
+// 27-DOF hexahedra, vector kernels otherwise; 2: reference tensors where
+// available, else as 1)
+void assemble_cells(const AsmArgs& a, hipStream_t s);
 void launch_mfma_probe(double* out, hipStream_t s);
-// assembly variant: 0 reference-order vector kernels, 1 (default) the MFMA
-// kernel for 27-DOF hexahedra and the vector kernels for the other elements
 int assembly_variant();
 
 // Sum local matrices into CSR rows in increasing cell order; Dirichlet rows

@@ -88,6 +88,9 @@ CDSystem::CDSystem(const ProblemDesc& d, hipStream_t stream) : desc_(d), stream_
   ref_.upload(ref, stream_);
   qw_.upload(qf.w, stream_);
   qp_.upload(qp, stream_);
+  // reference tensors: every current example has constant eps, b and c
+  if(example_constant_matrix_coeffs(d.ex.kind))
+    tensor_.upload(build_assembly_tensor(shape_dim(shape), nl, qf.n(), ref.data(), qf.w.data()), stream_);
   size_t max_cells = 0, max_n = 0;
   for(auto& L : levels_) {
     build_level_device(*L);
@@ -350,6 +353,7 @@ AsmArgs CDSystem::asm_args(const Level& L) {
   a.ref = ref_.p;
   a.qw = qw_.p;
   a.qp = qp_.p;
+  a.tensor = tensor_.n && example_constant_matrix_coeffs(desc_.ex.kind) ? tensor_.p : nullptr;
   a.cell_xyz = L.cell_xyz.p;
   a.local_A = local_A_.p;
   a.local_rhs = local_rhs_.p;
@@ -358,7 +362,7 @@ AsmArgs CDSystem::asm_args(const Level& L) {
 
 void CDSystem::assemble_level(Level& L) {
   AsmArgs a = asm_args(L);
-  if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
+  assemble_cells(a, stream_);
   // a level whose matrix kernels run on the sliced ELL copy gets it from the
   // same row gather (no separate copy pass)
   const bool sell = level_uses_sell(L);
@@ -432,7 +436,7 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
       case 1: level_jacobi(L, b.p, x.p, y.p, 0.8, stream_); break;
       case 2: level_spmv32(L, xf.p, nullptr, yf.p, 1.0f, 0.0f, stream_); break;
       case 3:
-        if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
+        assemble_cells(a, stream_);
         break;
       case 4: {
         const bool sell = level_uses_sell(L) && L.sell_slots >= 0;

@@ -252,7 +252,7 @@ class CDSystem {
   std::vector<std::unique_ptr<Level>> levels_;  // coarsest first
   QuadRule rule_;
   int ncomp_ = 0;
-  DevBuf<double> ref_, qw_, qp_;
+  DevBuf<double> ref_, qw_, qp_, tensor_;
   DevBuf<double> local_A_, local_rhs_;
   DevBuf<double> red_partial_, red_out_, cell_err_;
   // dense inverse of the coarsest matrix (direct coarse solve)

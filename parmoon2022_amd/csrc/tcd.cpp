@@ -28,7 +28,7 @@ void CDSystem::tcd_assemble() {
     AsmArgs a = asm_args(L);
     a.tcd = 1;
     a.local_M = local_M_.p;
-    if(!(assembly_variant() == 1 && launch_assemble_mfma(a, stream_))) launch_assemble_local(a, stream_);
+    assemble_cells(a, stream_);
     launch_gather_matrix(L.n(), L.n_active(), a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p, L.contrib.p, L.slot.p,
                          local_A_.p, L.tcd_A.p, stream_);
     launch_gather_matrix(L.n(), L.n_active(), a.nl, L.row_ptr.p, L.col.p, L.contrib_ptr.p, L.contrib.p, L.slot.p,

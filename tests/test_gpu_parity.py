@@ -78,15 +78,21 @@ def test_mfma_f64_lane_layout():
     assert not bad, bad[:8]
 
 
-@pytest.mark.parametrize("variant", [0, 1], ids=["vector", "mfma"])
+@pytest.mark.parametrize("variant", [0, 1, 2], ids=["vector", "mfma", "tensor"])
 @pytest.mark.parametrize("case", ASSEMBLY_CASES, ids=lambda c: "%s-p%d-r%d-%s-%s" % c[:5])
 def test_assembly_matches_oracle(oracle_lib, case, variant):
+    """Every assembly variant against the oracle: 0 the reference-order vector
+    kernels, 1 the per-point FP64-MFMA kernel (Q2 hexahedra), 2 the
+    reference-tensor GEMM on FP64 MFMA (every element but P1 simplices)."""
     db = make_db(*case)
     print("case", case, "variant", variant, flush=True)
+    old = pm.assembly_variant()
     pm.set_assembly_variant(variant)
-    cd = pm.ConvectionDiffusion(db)
-    cd.assemble()
-    pm.set_assembly_variant(1)
+    try:
+        cd = pm.ConvectionDiffusion(db)
+        cd.assemble()
+    finally:
+        pm.set_assembly_variant(old)
     o = oracle_lib.Oracle(pm.problem_desc(db))
     o.assemble()
     assert np.array_equal(cd.dofmap(), o.dofmap())

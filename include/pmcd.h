@@ -357,6 +357,134 @@ int pmcd_batch_set_solution(pmcd_batch* b, int inst, const double* in);
 int pmcd_batch_get_matrix(pmcd_batch* b, int inst, double* val);
 int pmcd_batch_get_rhs(pmcd_batch* b, int inst, double* rhs);
 
+/* ---- Drop-in entries on caller-owned spaces, structures and matrices ----
+ *
+ * These bind at the reference's own call sites: ConvectionDiffusion<d>::
+ * call_assembling_routine (src/System/ConvectionDiffusion.C:410-452) calls
+ * Assemble3D / Assemble2D with the caller's FE space, structure, matrix
+ * entries and rhs; ConvectionDiffusion<d>::solve (:258-281) calls
+ * Solver<BlockFEMatrix, BlockVector>::solve on the caller's matrix. */
+
+/* Cell shapes of a pmcd_fe_space, as in include/Geometry/Enumerations_geometry.h */
+enum { PMCD_SHAPE_TRIANGLE = 0, PMCD_SHAPE_QUADRANGLE = 1, PMCD_SHAPE_TETRAHEDRON = 2, PMCD_SHAPE_HEXAHEDRON = 3 };
+
+/* A caller's FE space (TFESpace2D / TFESpace3D, include/FE/FESpace.h):
+ * GetGlobalNumbers() and GetBeginIndex() (cell c owns global_numbers
+ * [begin_index[c] .. begin_index[c+1])), get_n_dof() and get_n_active()
+ * (inner and non-Dirichlet DOFs first, Dirichlet DOFs last), and the cells'
+ * vertex coordinates in the element's local vertex order (TBaseCell::
+ * GetVertex). Conforming continuous Lagrange elements P1/P2 on simplices,
+ * Q1/Q2 on quadrilaterals and hexahedra; affine cells. */
+typedef struct {
+  int shape;                  /* PMCD_SHAPE_* */
+  int order;                  /* 1 or 2 */
+  int n_cells;
+  int n_dof;
+  int n_active;
+  const double* vertices;     /* [n_cells][vertices per cell][3] */
+  const int* global_numbers;  /* sum over cells of the local DOF counts */
+  const int* begin_index;     /* n_cells + 1 */
+} pmcd_fe_space;
+
+/* A caller's square structure (TStructure, include/Matrix/Structure.h):
+ * get_row_ptr() and get_vector_columns() of its n_rows rows; it must hold
+ * every coupling of two DOFs of one cell (columns in any order). */
+typedef struct {
+  int n_rows;
+  const int* row_ptr;  /* n_rows + 1 */
+  const int* col;      /* row_ptr[n_rows] */
+} pmcd_structure;
+
+/* Assemble3D / Assemble2D with the convection-diffusion LocalAssembling
+ * (include/AssembleRoutines/Assemble3D.h:28-34, LocalAssembling.h:229-231):
+ * the Galerkin (+ SUPG if la->supg) local matrices and rhs of every cell
+ * added into the caller's `entries` (TMatrix::GetEntries, row_ptr[n_rows]
+ * values) and `rhs` (n_dof values), as the reference's scatter does
+ * (Assemble3D.C:258-300,346-360): active rows get the cell contributions
+ * added in increasing cell order, each Dirichlet row gets 1 on its diagonal
+ * (the rest of the row is left unchanged), the rhs gets the contributions
+ * added on active DOFs and the Dirichlet values set on the others.
+ * dirichlet_values: the n_dof - n_active boundary values in DOF order (the
+ * reference evaluates BoundaryValues through the nodal functionals; host
+ * callbacks cannot run on the device), NULL for zero. The caller resets the
+ * entries and rhs first when a fresh system is wanted, as
+ * ConvectionDiffusion.C:429-430 does; the sums are then the reference's.
+ * la: the PDE data and forms (example, eps, b, c, f0, supg, linear_quad);
+ * its geometry fields are not read. Host arrays. Neumann boundary integrals
+ * are not added. */
+int pmcd_assemble_into(const pmcd_fe_space* space, const pmcd_structure* st, const pmcd_problem_desc* la,
+                       const double* dirichlet_values, double* entries, double* rhs);
+/* The same with the space and structure uploaded once: create, then assemble
+ * as often as needed (host arrays, or device arrays on `stream`). */
+typedef struct pmcd_assembler pmcd_assembler;
+int pmcd_assembler_create(const pmcd_fe_space* space, const pmcd_structure* st, const pmcd_problem_desc* la,
+                          pmcd_assembler** out);
+int pmcd_assembler_destroy(pmcd_assembler* a);
+int pmcd_assembler_assemble(pmcd_assembler* a, const double* dirichlet_values, double* entries, double* rhs);
+int pmcd_assembler_assemble_device(pmcd_assembler* a, const double* dirichlet_values, double* entries, double* rhs,
+                                   void* stream);
+
+/* One level of a caller's matrix hierarchy, coarsest first (the
+ * BlockFEMatrix of each SystemPerGrid, ConvectionDiffusion.h:44-78, and the
+ * multigrid levels built from them, MultigridLevel.C:22-28): CSR with the
+ * Dirichlet rows n_active.. last (they act as identity rows, their entries
+ * are not read) and, on every level but the coarsest, the prolongation from
+ * the next coarser level as an n x n_coarse CSR matrix (the cell-wise
+ * interpolation that GridTransfer::Prolongate applies, GridTransfer.C:603-865;
+ * restriction uses its transpose). */
+typedef struct {
+  int n;
+  int n_active;
+  const int* row_ptr;
+  const int* col;
+  const double* entries;
+  int n_coarse;
+  const int* p_row_ptr;
+  const int* p_col;
+  const double* p_entries;
+} pmcd_csr_level;
+
+/* Solver<BlockFEMatrix, BlockVector> (include/Solver/Solver.h:45-94) on a
+ * caller-owned matrix: create = Solver(db) with the solver keys of
+ * pmcd_solver_desc; update_matrix = update_matrix(A) with the multigrid
+ * levels (the coarse factorization and derived copies are redone; values
+ * only when the patterns are unchanged); solve(b, x) with x the initial
+ * iterate on entry and the solution on return (host arrays, finest level);
+ * is_using_multigrid. */
+typedef struct pmcd_solver pmcd_solver;
+int pmcd_solver_create(const pmcd_solver_desc* sd, pmcd_solver** out);
+int pmcd_solver_destroy(pmcd_solver* s);
+int pmcd_solver_update_matrix(pmcd_solver* s, int n_levels, const pmcd_csr_level* levels);
+int pmcd_solver_solve(pmcd_solver* s, const double* b, double* x, pmcd_solve_stats* stats);
+int pmcd_solver_is_using_multigrid(const pmcd_solver* s, int* out);
+
+/* The prolongation of a system's level (level -1 = finest) from the next
+ * coarser one, as pmcd_csr_level takes it: sizes first (null arrays), then
+ * row_ptr (n_fine + 1), col and val (nnz). No reference counterpart
+ * (GridTransfer applies it cell by cell without storing it). */
+int pmcd_get_transfer(pmcd_system* s, int level, int* n_fine, int* n_coarse, int* nnz, int* row_ptr, int* col,
+                      double* val);
+
+/* Device-pointer building blocks (all pointers device memory, `stream` a
+ * hipStream_t or NULL): Daxpy y += a x (src/General/Blas1.C), Dnorm
+ * sqrt(x.x) (two-pass deterministic reduction, result to host); one damped
+ * Jacobi step x_out = x + omega (b - A x) / diag, A acting as the identity on
+ * the rows n_active.. (JacobiSmoother::smooth, src/Multigrid/
+ * JacobiSmoother.C:22-30, with FEMatrix::multiplyActive);
+ * prolongation fine += a P coarse on the active fine rows and restriction
+ * coarse = R fine on the active coarse rows, 0 on the others
+ * (GridTransfer::Prolongate / DefectRestriction with the non-active rules of
+ * Multigrid.C:310-410). */
+int pmcd_axpy_device(int n, double a, const double* x, double* y, void* stream);
+int pmcd_norm_device(int n, const double* x, double* result, void* stream);
+int pmcd_jacobi_device(int n, int n_active, const int* row_ptr, const int* col, const double* entries,
+                       const double* diag, const double* b, const double* x, double* x_out, double omega,
+                       void* stream);
+int pmcd_prolongate_device(int n_fine, int n_active_fine, const int* p_row_ptr, const int* p_col,
+                           const double* p_entries, const double* coarse, double* fine, double a, void* stream);
+int pmcd_restrict_device(int n_coarse, int n_active_coarse, const int* r_row_ptr, const int* r_col,
+                         const double* r_entries, const double* fine, double* coarse, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

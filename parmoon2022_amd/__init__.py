@@ -73,6 +73,10 @@ EXPORTS = [
     "pmcd_apply", "pmcd_sor_sweep", "pmcd_vcycle", "pmcd_time_kernel", "pmcd_set_tuning", "pmcd_get_tuning",
     "pmcd_probe_mfma",
     "pmcd_spmv_device", "pmcd_dot_device", "pmcd_dot_device_noalloc",
+    "pmcd_assemble_into", "pmcd_assembler_create", "pmcd_assembler_destroy", "pmcd_assembler_assemble",
+    "pmcd_assembler_assemble_device", "pmcd_solver_create", "pmcd_solver_destroy", "pmcd_solver_update_matrix",
+    "pmcd_solver_solve", "pmcd_solver_is_using_multigrid", "pmcd_get_transfer", "pmcd_axpy_device",
+    "pmcd_norm_device", "pmcd_jacobi_device", "pmcd_prolongate_device", "pmcd_restrict_device",
     "pmcd_batch_create", "pmcd_batch_destroy", "pmcd_batch_info", "pmcd_batch_assemble",
     "pmcd_batch_solve", "pmcd_batch_get_solution", "pmcd_batch_set_solution", "pmcd_batch_get_matrix",
     "pmcd_batch_get_rhs",
@@ -489,6 +493,19 @@ class ConvectionDiffusion:
                                 _ptr(out, ctypes.c_double), ctypes.c_double(a), mode))
         return out
 
+    def transfer(self, level=-1):
+        """Prolongation of a level from the next coarser one: (row_ptr, col,
+        val, n_coarse)."""
+        nf, nc, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().pmcd_get_transfer(self._s, level, ctypes.byref(nf), ctypes.byref(nc), ctypes.byref(nz),
+                                       None, None, None))
+        rp = np.zeros(nf.value + 1, np.int32)
+        col = np.zeros(nz.value, np.int32)
+        val = np.zeros(nz.value)
+        _check(lib().pmcd_get_transfer(self._s, level, None, None, None, _ptr(rp, ctypes.c_int),
+                                       _ptr(col, ctypes.c_int), _ptr(val, ctypes.c_double)))
+        return rp, col, val, nc.value
+
     def write_solution(self, path):
         """The solution in the reference's checkpoint format (write_to_file)."""
         write_vector(path, self.solution())
@@ -533,6 +550,151 @@ class ConvectionDiffusion:
         _check(lib().pmcd_vcycle(self._s, ctypes.byref(sd), _ptr(v, ctypes.c_double),
                                  _ptr(z, ctypes.c_double)))
         return z
+
+
+# ---- drop-in interface on caller-owned spaces, structures and matrices -------
+
+SHAPES = {"triangle": 0, "quadrangle": 1, "tetrahedron": 2, "hexahedron": 3}
+
+
+class FeSpaceDesc(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int), ("order", ctypes.c_int), ("n_cells", ctypes.c_int),
+                ("n_dof", ctypes.c_int), ("n_active", ctypes.c_int),
+                ("vertices", ctypes.POINTER(ctypes.c_double)), ("global_numbers", ctypes.POINTER(ctypes.c_int)),
+                ("begin_index", ctypes.POINTER(ctypes.c_int))]
+
+
+class StructureDesc(ctypes.Structure):
+    _fields_ = [("n_rows", ctypes.c_int), ("row_ptr", ctypes.POINTER(ctypes.c_int)),
+                ("col", ctypes.POINTER(ctypes.c_int))]
+
+
+class CsrLevel(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int), ("n_active", ctypes.c_int), ("row_ptr", ctypes.POINTER(ctypes.c_int)),
+                ("col", ctypes.POINTER(ctypes.c_int)), ("entries", ctypes.POINTER(ctypes.c_double)),
+                ("n_coarse", ctypes.c_int), ("p_row_ptr", ctypes.POINTER(ctypes.c_int)),
+                ("p_col", ctypes.POINTER(ctypes.c_int)), ("p_entries", ctypes.POINTER(ctypes.c_double))]
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Assembler:
+    """Assemble3D / Assemble2D with the convection-diffusion LocalAssembling
+    (include/AssembleRoutines/Assemble3D.h:28-34) on a caller's FE space and
+    structure: `shape` a key of SHAPES, `vertices` [n_cells, nv, 3] in the
+    element's vertex order, `global_numbers` / `begin_index` the space's
+    local-to-global map, `n_active` its active bound, (`row_ptr`, `col`) the
+    structure, `db` the PDE data and forms (reference keys). assemble() adds
+    into the caller's entries and rhs like the reference's scatter."""
+
+    def __init__(self, shape, order, vertices, global_numbers, begin_index, n_dof, n_active, row_ptr, col, db):
+        self._keep = [_f64(vertices), _i32(global_numbers), _i32(begin_index), _i32(row_ptr), _i32(col)]
+        v, g, b, rp, cl = self._keep
+        self.space = FeSpaceDesc(SHAPES[shape] if isinstance(shape, str) else int(shape), int(order),
+                                 len(b) - 1, int(n_dof), int(n_active), _ptr(v, ctypes.c_double),
+                                 _ptr(g, ctypes.c_int), _ptr(b, ctypes.c_int))
+        self.structure = StructureDesc(int(n_dof), _ptr(rp, ctypes.c_int), _ptr(cl, ctypes.c_int))
+        self.desc = problem_desc(db)
+        self.nnz = int(rp[-1])
+        self._h = ctypes.c_void_p()
+        _check(lib().pmcd_assembler_create(ctypes.byref(self.space), ctypes.byref(self.structure),
+                                           ctypes.byref(self.desc), ctypes.byref(self._h)))
+
+    def assemble(self, entries, rhs, dirichlet_values=None):
+        """entries (float64, nnz) and rhs (float64, n_dof) are updated in place."""
+        assert entries.dtype == np.float64 and entries.flags.c_contiguous and entries.size == self.nnz
+        assert rhs.dtype == np.float64 and rhs.flags.c_contiguous and rhs.size == self.space.n_dof
+        dv = None if dirichlet_values is None else _f64(dirichlet_values)
+        _check(lib().pmcd_assembler_assemble(self._h, None if dv is None else _ptr(dv, ctypes.c_double),
+                                             _ptr(entries, ctypes.c_double), _ptr(rhs, ctypes.c_double)))
+
+    def assemble_device(self, entries_ptr, rhs_ptr, dirichlet_ptr=None, stream=None):
+        """Device-pointer variant (integer addresses)."""
+        fn = lib().pmcd_assembler_assemble_device
+        fn.argtypes = [ctypes.c_void_p] * 5
+        _check(fn(self._h, dirichlet_ptr, entries_ptr, rhs_ptr, stream))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pmcd_assembler_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def assemble_into(shape, order, vertices, global_numbers, begin_index, n_dof, n_active, row_ptr, col, db,
+                  entries, rhs, dirichlet_values=None):
+    """One-shot pmcd_assemble_into (see Assembler)."""
+    a = Assembler(shape, order, vertices, global_numbers, begin_index, n_dof, n_active, row_ptr, col, db)
+    try:
+        a.assemble(entries, rhs, dirichlet_values)
+    finally:
+        a.close()
+
+
+class Solver:
+    """Solver<BlockFEMatrix, BlockVector> on caller-owned matrices
+    (include/Solver/Solver.h:45-94): Solver(db), update_matrix(levels),
+    solve(b, x), is_using_multigrid(). `levels`, coarsest first: dicts with
+    n_active, row_ptr, col, entries and, above the coarsest, the prolongation
+    p_row_ptr, p_col, p_entries from the next coarser level."""
+
+    def __init__(self, db):
+        self.db = dict(db)
+        self.sd = solver_desc(db)
+        self._h = ctypes.c_void_p()
+        _check(lib().pmcd_solver_create(ctypes.byref(self.sd), ctypes.byref(self._h)))
+        self.stats = None
+
+    def update_matrix(self, levels):
+        arr = (CsrLevel * len(levels))()
+        keep = []
+        prev_n = 0
+        for i, lv in enumerate(levels):
+            rp, cl, ent = _i32(lv["row_ptr"]), _i32(lv["col"]), _f64(lv["entries"])
+            keep += [rp, cl, ent]
+            c = arr[i]
+            c.n = len(rp) - 1
+            c.n_active = int(lv["n_active"])
+            c.row_ptr, c.col, c.entries = _ptr(rp, ctypes.c_int), _ptr(cl, ctypes.c_int), _ptr(ent, ctypes.c_double)
+            if i > 0:
+                prp, pcl, pent = _i32(lv["p_row_ptr"]), _i32(lv["p_col"]), _f64(lv["p_entries"])
+                keep += [prp, pcl, pent]
+                c.n_coarse = prev_n
+                c.p_row_ptr, c.p_col = _ptr(prp, ctypes.c_int), _ptr(pcl, ctypes.c_int)
+                c.p_entries = _ptr(pent, ctypes.c_double)
+            prev_n = c.n
+        _check(lib().pmcd_solver_update_matrix(self._h, len(levels), arr))
+
+    def solve(self, b, x):
+        """x: initial iterate on entry (float64, updated in place)."""
+        b = _f64(b)
+        assert x.dtype == np.float64 and x.flags.c_contiguous
+        st = SolveStats()
+        _check(lib().pmcd_solver_solve(self._h, _ptr(b, ctypes.c_double), _ptr(x, ctypes.c_double),
+                                       ctypes.byref(st)))
+        self.stats = st
+        return st
+
+    def is_using_multigrid(self):
+        v = ctypes.c_int(0)
+        _check(lib().pmcd_solver_is_using_multigrid(self._h, ctypes.byref(v)))
+        return bool(v.value)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pmcd_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 class TcdDesc(ctypes.Structure):

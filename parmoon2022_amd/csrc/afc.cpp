@@ -77,6 +77,7 @@ std::vector<double> bjk_gamma(const Mesh& m, const FESpace& sp) {
 }  // namespace
 
 SolveStats CDSystem::afc_solve(const SolverDesc& s, const double* prm, double* out) {
+  PMCD_CHECK(!operator_only_, "an operator-only system has no mesh to assemble on");
   PMCD_CHECK(n_levels() == 1, "algebraic flux correction needs a single level (the reference refuses multigrid)");
   PMCD_CHECK(!desc_.supg, "algebraic flux correction uses the Galerkin discretization");
   PMCD_CHECK(!tcd_, "algebraic flux correction is for the stationary problem");

@@ -180,23 +180,10 @@ BatchSystem::BatchSystem(const std::vector<ProblemDesc>& descs, hipStream_t stre
   cell_ex_.upload(exs, stream_);
   cell_inst_.upload(cinst, stream_);
   // quadrature and reference tables (shared by all instances)
-  rule_ = cell_rule_for(shape, d0.order, d0.linear_quad != 0);
-  const QuadFormula& qf = quad_formula(rule_);
-  const Element el = element_for(shape, d0.order);
-  ncomp_ = shape_dim(shape) == 2 ? kNDeriv2D : kNDeriv3D;
-  std::vector<double> ref((size_t)qf.n() * ncomp_ * nl), qp((size_t)qf.n() * 3);
-  for(int q = 0; q < qf.n(); ++q) {
-    double p[3] = {qf.xi[q], qf.dim >= 2 ? qf.eta[q] : 0.0, qf.dim >= 3 ? qf.zeta[q] : 0.0};
-    eval_basis(el, p, &ref[(size_t)q * ncomp_ * nl]);
-    for(int k = 0; k < 3; ++k) qp[3 * q + k] = p[k];
-  }
-  ref_.upload(ref, stream_);
-  qw_.upload(qf.w, stream_);
-  qp_.upload(qp, stream_);
   bool constant = true;
   for(const auto& d : descs) constant = constant && example_constant_matrix_coeffs(d.ex.kind);
-  if(constant)
-    tensor_.upload(build_assembly_tensor(shape_dim(shape), nl, qf.n(), ref.data(), qf.w.data()), stream_);
+  rule_ = upload_reference_tables(shape, d0.order, d0.linear_quad != 0, constant, ncomp_, ref_, qw_, qp_, tensor_,
+                                  stream_);
   local_A_.alloc((size_t)cells * nl * nl);
   local_rhs_.alloc((size_t)cells * nl);
   // reduction chunks: each instance's active range, then its Dirichlet range

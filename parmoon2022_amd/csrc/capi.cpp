@@ -3,7 +3,10 @@
 #include "system.hpp"
 #include "batch.hpp"
 #include "io.hpp"
+#include "dropin.hpp"
+#include <cmath>
 #include <cstring>
+#include <algorithm>
 #include <string>
 
 using namespace pmcd;
@@ -16,6 +19,17 @@ struct pmcd_system {
 struct pmcd_batch {
   hipStream_t stream = nullptr;
   std::unique_ptr<BatchSystem> sys;
+};
+
+struct pmcd_assembler {
+  hipStream_t stream = nullptr;
+  std::unique_ptr<CallerAssembler> a;
+};
+
+struct pmcd_solver {
+  hipStream_t stream = nullptr;
+  SolverDesc sd{};
+  std::unique_ptr<CDSystem> sys;
 };
 
 namespace {
@@ -641,6 +655,241 @@ int pmcd_matrix_write_market(const char* path, int n_rows, int n_cols, const int
     PMCD_CHECK(path && row_ptr && n_rows >= 0 && n_cols >= 0, "invalid argument");
     PMCD_CHECK(row_ptr[n_rows] == 0 || (col && val), "invalid argument");
     write_matrix_market(path, n_rows, n_cols, row_ptr, col, val);
+  });
+}
+
+
+// ---- drop-in entries on caller-owned spaces, structures and matrices ----------
+static ProblemDesc forms_desc(const pmcd_problem_desc& la) {
+  pmcd_problem_desc d = la;
+  d.geometry = 0;
+  d.mesh_file = nullptr;
+  return to_desc(d);
+}
+
+static void check_visible_gpu() {
+  int ndev = 0;
+  if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    PMCD_FAIL("no GPU visible: the HIP path needs an MI355X device");
+}
+
+int pmcd_assembler_create(const pmcd_fe_space* space, const pmcd_structure* st, const pmcd_problem_desc* la,
+                          pmcd_assembler** out) {
+  return guarded([&] {
+    PMCD_CHECK(space && st && la && out, "null argument");
+    PMCD_CHECK(st->n_rows == space->n_dof, "structure rows ", st->n_rows, " differ from the space's DOFs ",
+               space->n_dof);
+    PMCD_CHECK(space->shape >= 0 && space->shape <= 3, "unknown cell shape ", space->shape);
+    check_visible_gpu();
+    auto h = std::make_unique<pmcd_assembler>();
+    PMCD_HIP(hipStreamCreate(&h->stream));
+    try {
+      h->a = std::make_unique<CallerAssembler>((Shape)space->shape, space->order, space->n_cells, space->n_dof,
+                                               space->n_active, space->vertices, space->global_numbers,
+                                               space->begin_index, st->row_ptr, st->col, forms_desc(*la),
+                                               h->stream);
+    } catch(...) {
+      (void)hipStreamDestroy(h->stream);
+      throw;
+    }
+    *out = h.release();
+  });
+}
+
+int pmcd_assembler_destroy(pmcd_assembler* h) {
+  return guarded([&] {
+    if(!h) return;
+    h->a.reset();
+    if(h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+  });
+}
+
+int pmcd_assembler_assemble(pmcd_assembler* h, const double* dirichlet_values, double* entries, double* rhs) {
+  return guarded([&] {
+    PMCD_CHECK(h && h->a, "null assembler");
+    h->a->assemble(dirichlet_values, false);
+    h->a->add_into_host(entries, rhs);
+  });
+}
+
+int pmcd_assembler_assemble_device(pmcd_assembler* h, const double* dirichlet_values, double* entries,
+                                   double* rhs, void* stream) {
+  return guarded([&] {
+    PMCD_CHECK(h && h->a, "null assembler");
+    // the caller's arrays (and device Dirichlet values) are complete on its stream
+    PMCD_HIP(hipStreamSynchronize((hipStream_t)stream));
+    h->a->assemble(dirichlet_values, true);
+    h->a->add_into_device(entries, rhs, (hipStream_t)stream);
+  });
+}
+
+int pmcd_assemble_into(const pmcd_fe_space* space, const pmcd_structure* st, const pmcd_problem_desc* la,
+                       const double* dirichlet_values, double* entries, double* rhs) {
+  pmcd_assembler* h = nullptr;
+  int rc = pmcd_assembler_create(space, st, la, &h);
+  if(rc != 0) return rc;
+  rc = pmcd_assembler_assemble(h, dirichlet_values, entries, rhs);
+  std::string err = g_err;
+  pmcd_assembler_destroy(h);
+  g_err = err;
+  return rc;
+}
+
+int pmcd_solver_create(const pmcd_solver_desc* sd, pmcd_solver** out) {
+  return guarded([&] {
+    PMCD_CHECK(sd && out, "null argument");
+    check_visible_gpu();
+    auto s = std::make_unique<pmcd_solver>();
+    s->sd = to_solver(*sd);
+    PMCD_HIP(hipStreamCreate(&s->stream));
+    *out = s.release();
+  });
+}
+
+int pmcd_solver_destroy(pmcd_solver* s) {
+  return guarded([&] {
+    if(!s) return;
+    s->sys.reset();
+    if(s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+  });
+}
+
+int pmcd_solver_update_matrix(pmcd_solver* s, int n_levels, const pmcd_csr_level* levels) {
+  return guarded([&] {
+    PMCD_CHECK(s && levels && n_levels >= 1, "null argument or no level");
+    const bool mg = s->sd.precond == 2;
+    PMCD_CHECK(mg || n_levels == 1, "only a multigrid preconditioner takes several levels");
+    // same patterns: only the values change (Solver::update_matrix on a
+    // matrix of the same structure)
+    bool same = s->sys && s->sys->n_levels() == n_levels;
+    for(int l = 0; same && l < n_levels; ++l) {
+      const Level& L = s->sys->level(l);
+      const pmcd_csr_level& o = levels[l];
+      same = o.n == L.n() && o.n_active == L.n_active() && o.row_ptr && o.col &&
+             std::equal(o.row_ptr, o.row_ptr + o.n + 1, L.st.row_ptr.begin()) &&
+             std::equal(o.col, o.col + o.row_ptr[o.n], L.st.col.begin());
+    }
+    if(same) {
+      for(int l = 0; l < n_levels; ++l) {
+        PMCD_CHECK(levels[l].entries, "level ", l, ": null entries");
+        s->sys->set_level_values(l, levels[l].entries);
+      }
+      return;
+    }
+    std::vector<OperatorLevel> lv(n_levels);
+    for(int l = 0; l < n_levels; ++l) {
+      const pmcd_csr_level& o = levels[l];
+      PMCD_CHECK(o.n > 0 && o.row_ptr && o.col && o.entries, "level ", l, ": null or empty matrix");
+      OperatorLevel& d = lv[l];
+      d.n = o.n;
+      d.n_active = o.n_active;
+      d.row_ptr.assign(o.row_ptr, o.row_ptr + o.n + 1);
+      const int nnz = o.row_ptr[o.n];
+      d.col.assign(o.col, o.col + nnz);
+      d.val.assign(o.entries, o.entries + nnz);
+      if(l > 0) {
+        PMCD_CHECK(o.p_row_ptr && o.p_col && o.p_entries, "level ", l, ": no prolongation");
+        d.n_coarse = o.n_coarse;
+        d.p_row_ptr.assign(o.p_row_ptr, o.p_row_ptr + o.n + 1);
+        const int pn = o.p_row_ptr[o.n];
+        d.p_col.assign(o.p_col, o.p_col + pn);
+        d.p_val.assign(o.p_entries, o.p_entries + pn);
+      }
+    }
+    s->sys.reset();
+    s->sys = std::make_unique<CDSystem>(lv, s->stream);
+  });
+}
+
+int pmcd_solver_solve(pmcd_solver* s, const double* b, double* x, pmcd_solve_stats* st) {
+  return guarded([&] {
+    PMCD_CHECK(s && b && x, "null argument");
+    PMCD_CHECK(s->sys, "pmcd_solver_update_matrix has not been called");
+    CDSystem& sys = *s->sys;
+    Level& F = sys.finest();
+    sys.set_rhs(b);
+    F.sol.upload(x, F.sol.n, s->stream);
+    const SolveStats r = sys.solve(s->sd);
+    F.sol.download(x, F.sol.n, s->stream);
+    PMCD_HIP(hipStreamSynchronize(s->stream));
+    if(st) {
+      st->iterations = r.iterations;
+      st->initial_residual = r.initial_residual;
+      st->final_residual = r.final_residual;
+      st->converged = r.converged;
+      st->seconds = r.seconds;
+    }
+  });
+}
+
+int pmcd_solver_is_using_multigrid(const pmcd_solver* s, int* out) {
+  return guarded([&] {
+    PMCD_CHECK(s && out, "null argument");
+    *out = s->sd.precond == 2 ? 1 : 0;
+  });
+}
+
+int pmcd_get_transfer(pmcd_system* s, int level, int* n_fine, int* n_coarse, int* nnz, int* row_ptr, int* col,
+                      double* val) {
+  return guarded([&] {
+    Level& L = level_of(s, level);
+    PMCD_CHECK(L.n_coarse > 0, "level ", level, " has no coarser level");
+    const int nf = L.n();
+    const int pn = (int)L.p_col.n;
+    if(n_fine) *n_fine = nf;
+    if(n_coarse) *n_coarse = L.n_coarse;
+    if(nnz) *nnz = pn;
+    hipStream_t st = s->stream;
+    if(row_ptr) L.p_rp.download(row_ptr, nf + 1, st);
+    if(col) L.p_col.download(col, pn, st);
+    if(val) L.p_val.download(val, pn, st);
+    PMCD_HIP(hipStreamSynchronize(st));
+  });
+}
+
+int pmcd_axpy_device(int n, double a, const double* x, double* y, void* stream) {
+  return guarded([&] {
+    PMCD_CHECK(n >= 0, "negative size");
+    launch_axpy(n, a, x, y, (hipStream_t)stream);
+    PMCD_HIP(hipGetLastError());
+  });
+}
+
+int pmcd_norm_device(int n, const double* x, double* result, void* stream) {
+  return guarded([&] {
+    PMCD_CHECK(result, "null output");
+    double d = 0;
+    if(pmcd_dot_device(n, x, x, &d, stream) != 0) PMCD_FAIL(g_err);
+    *result = std::sqrt(d);
+  });
+}
+
+int pmcd_jacobi_device(int n, int n_active, const int* row_ptr, const int* col, const double* entries,
+                       const double* diag, const double* b, const double* x, double* x_out, double omega,
+                       void* stream) {
+  return guarded([&] {
+    launch_jacobi(n, n_active, row_ptr, col, entries, nullptr, diag, b, x, x_out, omega, (hipStream_t)stream);
+    PMCD_HIP(hipGetLastError());
+  });
+}
+
+int pmcd_prolongate_device(int n_fine, int n_active_fine, const int* p_row_ptr, const int* p_col,
+                           const double* p_entries, const double* coarse, double* fine, double a, void* stream) {
+  return guarded([&] {
+    launch_prolong_add(n_fine, n_active_fine, 0, -1, p_row_ptr, p_col, p_entries, coarse, fine, a,
+                       (hipStream_t)stream);
+    PMCD_HIP(hipGetLastError());
+  });
+}
+
+int pmcd_restrict_device(int n_coarse, int n_active_coarse, const int* r_row_ptr, const int* r_col,
+                         const double* r_entries, const double* fine, double* coarse, void* stream) {
+  return guarded([&] {
+    launch_restrict(n_coarse, n_active_coarse, 0, -1, r_row_ptr, r_col, r_entries, fine, coarse,
+                    (hipStream_t)stream);
+    PMCD_HIP(hipGetLastError());
   });
 }
 

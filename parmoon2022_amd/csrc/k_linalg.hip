@@ -669,6 +669,30 @@ __global__ __launch_bounds__(256) void k_fgmres_update(int n, int k, const doubl
   }
 }
 
+// The reference's update of a caller's matrix and rhs by an assembly
+// (Assemble3D.C:258-300,346-360): entries of the active rows (the first
+// row_ptr[n_active] entries) get the assembled sums added, Dirichlet rows get
+// 1 on the diagonal; rhs entries of active DOFs get the sums added, the
+// others the Dirichlet values.
+__global__ void k_add_assembled(int n, int n_active, long long nnz_active, const int* __restrict__ diag_pos,
+                                const double* __restrict__ val, const double* __restrict__ f,
+                                double* __restrict__ entries, double* __restrict__ rhs) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for(long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < nnz_active + n; k += stride) {
+    if(k < nnz_active) {
+      if(entries) entries[k] += val[k];
+    } else {
+      const int r = (int)(k - nnz_active);
+      if(r < n_active) {
+        if(rhs) rhs[r] += f[r];
+      } else {
+        if(entries) entries[diag_pos[r]] = 1.0;
+        if(rhs) rhs[r] = f[r];
+      }
+    }
+  }
+}
+
 __global__ void k_d2f(int n, const double* __restrict__ x, float* __restrict__ y) {
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
     y[i] = (float)x[i];
@@ -1485,6 +1509,17 @@ void launch_scale_norm(int n, const double* prev, double* h_out, const double* r
 }
 void launch_fgmres_update(int n, int k, const double* y, const double* const* z, double* x, hipStream_t s) {
   if(n && k) k_fgmres_update<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, k, y, z, x);
+  PMCD_HIP(hipGetLastError());
+}
+void launch_add_assembled(int n, int n_active, const int* row_ptr, const int* diag_pos, const double* val,
+                          const double* f, double* entries, double* rhs, hipStream_t s) {
+  int na_entries = 0;
+  PMCD_HIP(hipMemcpyAsync(&na_entries, row_ptr + n_active, sizeof(int), hipMemcpyDeviceToHost, s));
+  PMCD_HIP(hipStreamSynchronize(s));
+  const long long total = (long long)na_entries + n;
+  if(total == 0) return;
+  const long long g = std::min<long long>((total + 255) / 256, 65535LL * 16);
+  k_add_assembled<<<(int)g, 256, 0, s>>>(n, n_active, na_entries, diag_pos, val, f, entries, rhs);
   PMCD_HIP(hipGetLastError());
 }
 void launch_d2f(int n, const double* x, float* y, hipStream_t s) {

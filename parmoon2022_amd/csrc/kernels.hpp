@@ -228,6 +228,11 @@ void launch_scale_inplace(int n, double a, double* y, hipStream_t s);
 void launch_div(int n, const double* x, const double* d, double* y, hipStream_t s);
 // dst[i] = src[idx[i]]
 void launch_gather_idx(int n, const int* idx, const double* src, double* dst, hipStream_t s);
+// caller arrays += an assembly (val, f: the gathered CSR values and rhs):
+// active-row entries and active rhs added, Dirichlet diagonals set to 1 and
+// Dirichlet rhs set (either output may be null)
+void launch_add_assembled(int n, int n_active, const int* row_ptr, const int* diag_pos, const double* val,
+                          const double* f, double* entries, double* rhs, hipStream_t s);
 // scatter CSR rows into a dense column-major n x n matrix (zero-filled first)
 void launch_csr_to_dense(int n, const int* rp, const int* col, const double* val, double* M,
                          hipStream_t s);

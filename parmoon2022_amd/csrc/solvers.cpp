@@ -464,6 +464,7 @@ SolveStats CDSystem::solve_fgmres() {
 }
 
 void CDSystem::errors(double out[5]) {
+  PMCD_CHECK(!operator_only_, "an operator-only system has no mesh for the error norms");
   Level& F = finest();
   AsmArgs a = asm_args(F);
   launch_cell_errors(a, F.dof.p, F.sol.p, cell_err_.p, stream_);

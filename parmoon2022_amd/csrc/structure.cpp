@@ -4,6 +4,56 @@
 
 namespace pmcd {
 
+Structure structure_from_pattern(const FESpace& sp, int n_cells, const int* row_ptr, const int* col) {
+  Structure st;
+  const int nl = sp.n_local;
+  const int n = sp.n_dof;
+  PMCD_CHECK(row_ptr && col, "null structure arrays");
+  PMCD_CHECK(row_ptr[0] == 0, "row_ptr[0] must be 0");
+  st.n_rows = n;
+  st.row_ptr.assign(row_ptr, row_ptr + n + 1);
+  for(int r = 0; r < n; ++r) {
+    const int len = row_ptr[r + 1] - row_ptr[r];
+    PMCD_CHECK(len >= 0, "row_ptr decreases at row ", r);
+    st.max_row = std::max(st.max_row, len);
+  }
+  st.nnz = row_ptr[n];
+  PMCD_CHECK(st.max_row <= 255, "row with ", st.max_row, " entries exceeds the 8-bit slot map");
+  st.col.assign(col, col + st.nnz);
+  for(int64_t k = 0; k < st.nnz; ++k) PMCD_CHECK(col[k] >= 0 && col[k] < n, "column index out of range");
+  st.contrib_ptr.assign(n + 1, 0);
+  for(size_t p = 0; p < sp.dof.size(); ++p) st.contrib_ptr[sp.dof[p] + 1]++;
+  for(int r = 0; r < n; ++r) st.contrib_ptr[r + 1] += st.contrib_ptr[r];
+  st.contrib.resize(sp.dof.size());
+  {
+    std::vector<int> fill(st.contrib_ptr.begin(), st.contrib_ptr.end() - 1);
+    for(int c = 0; c < n_cells; ++c)
+      for(int m = 0; m < nl; ++m) st.contrib[fill[sp.dof[(size_t)c * nl + m]]++] = c * nl + m;
+  }
+  // positions by linear search in each row, as the reference's scatter finds
+  // them (Assemble3D.C:258-292); a coupling missing from the pattern is an
+  // error (the reference prints "not found" and drops the entry)
+  st.diag.assign(n, -1);
+  st.slot.resize(st.contrib.size() * (size_t)nl);
+  for(int r = 0; r < n; ++r) {
+    const int b = row_ptr[r], e = row_ptr[r + 1];
+    for(int k = b; k < e; ++k)
+      if(col[k] == r) { st.diag[r] = k; break; }
+    PMCD_CHECK(st.diag[r] >= 0, "row ", r, " of the structure has no diagonal entry");
+    for(int q = st.contrib_ptr[r]; q < st.contrib_ptr[r + 1]; ++q) {
+      const int* d = sp.cell_dof(st.contrib[q] / nl);
+      for(int k = 0; k < nl; ++k) {
+        int pos = -1;
+        for(int t = b; t < e; ++t)
+          if(col[t] == d[k]) { pos = t - b; break; }
+        PMCD_CHECK(pos >= 0, "coupling (", r, ", ", d[k], ") is not in the structure");
+        st.slot[(size_t)q * nl + k] = (uint8_t)pos;
+      }
+    }
+  }
+  return st;
+}
+
 Structure build_structure(const FESpace& sp, int n_cells) {
   Structure st;
   const int nl = sp.n_local;

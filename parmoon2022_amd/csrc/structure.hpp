@@ -35,6 +35,10 @@ struct Structure {
 };
 
 Structure build_structure(const FESpace& sp, int n_cells);
+// the maps of a caller's pattern (row_ptr, col: n_dof rows, any column order)
+// for the space's cells: diagonal positions, (cell, local row) lists and slot
+// maps; every coupling of two DOFs of one cell must be in the pattern
+Structure structure_from_pattern(const FESpace& sp, int n_cells, const int* row_ptr, const int* col);
 
 // Row classes for a successive over-relaxation sweep on the device. A forward
 // sweep visits the classes in order, a backward sweep in reverse; rows of one

@@ -70,27 +70,35 @@ std::vector<std::unique_ptr<Level>> build_levels(const ProblemDesc& d) {
   return levels;
 }
 
-CDSystem::CDSystem(const ProblemDesc& d, hipStream_t stream) : desc_(d), stream_(stream) {
-  levels_ = build_levels(d);
-  // quadrature and reference tables
-  const Shape shape = levels_.back()->mesh.shape;
-  rule_ = cell_rule_for(shape, d.order, d.linear_quad != 0);
-  const QuadFormula& qf = quad_formula(rule_);
-  const Element el = element_for(shape, d.order);
+QuadRule upload_reference_tables(Shape shape, int order, bool linear_quad, bool tensors, int& ncomp,
+                                 DevBuf<double>& ref_d, DevBuf<double>& qw_d, DevBuf<double>& qp_d,
+                                 DevBuf<double>& tensor_d, hipStream_t stream) {
+  const QuadRule rule = cell_rule_for(shape, order, linear_quad);
+  const QuadFormula& qf = quad_formula(rule);
+  const Element el = element_for(shape, order);
   const int nl = element_n_dof(el);
-  ncomp_ = shape_dim(shape) == 2 ? kNDeriv2D : kNDeriv3D;
-  std::vector<double> ref((size_t)qf.n() * ncomp_ * nl), qp((size_t)qf.n() * 3);
+  ncomp = shape_dim(shape) == 2 ? kNDeriv2D : kNDeriv3D;
+  std::vector<double> ref((size_t)qf.n() * ncomp * nl), qp((size_t)qf.n() * 3);
   for(int q = 0; q < qf.n(); ++q) {
     double p[3] = {qf.xi[q], qf.dim >= 2 ? qf.eta[q] : 0.0, qf.dim >= 3 ? qf.zeta[q] : 0.0};
-    eval_basis(el, p, &ref[(size_t)q * ncomp_ * nl]);
+    eval_basis(el, p, &ref[(size_t)q * ncomp * nl]);
     for(int k = 0; k < 3; ++k) qp[3 * q + k] = p[k];
   }
-  ref_.upload(ref, stream_);
-  qw_.upload(qf.w, stream_);
-  qp_.upload(qp, stream_);
-  // reference tensors: every current example has constant eps, b and c
-  if(example_constant_matrix_coeffs(d.ex.kind))
-    tensor_.upload(build_assembly_tensor(shape_dim(shape), nl, qf.n(), ref.data(), qf.w.data()), stream_);
+  ref_d.upload(ref, stream);
+  qw_d.upload(qf.w, stream);
+  qp_d.upload(qp, stream);
+  if(tensors) tensor_d.upload(build_assembly_tensor(shape_dim(shape), nl, qf.n(), ref.data(), qf.w.data()), stream);
+  return rule;
+}
+
+CDSystem::CDSystem(const ProblemDesc& d, hipStream_t stream) : desc_(d), stream_(stream) {
+  levels_ = build_levels(d);
+  // quadrature and reference tables; reference tensors when eps, b and c are
+  // constant (every current example)
+  const Shape shape = levels_.back()->mesh.shape;
+  const int nl = element_n_dof(element_for(shape, d.order));
+  rule_ = upload_reference_tables(shape, d.order, d.linear_quad != 0, example_constant_matrix_coeffs(d.ex.kind),
+                                  ncomp_, ref_, qw_, qp_, tensor_, stream_);
   size_t max_cells = 0, max_n = 0;
   for(auto& L : levels_) {
     build_level_device(*L);
@@ -103,6 +111,80 @@ CDSystem::CDSystem(const ProblemDesc& d, hipStream_t stream) : desc_(d), stream_
   red_partial_.alloc(1024 * 5 + 64);
   red_out_.alloc(64);
   cell_err_.alloc(max_cells * 5);
+  PMCD_HIP(hipStreamSynchronize(stream_));
+}
+
+CDSystem::CDSystem(const std::vector<OperatorLevel>& lv, hipStream_t stream) : stream_(stream) {
+  PMCD_CHECK(!lv.empty(), "operator hierarchy without levels");
+  desc_ = ProblemDesc{};
+  desc_.mg_levels = (int)lv.size();
+  operator_only_ = true;
+  for(size_t l = 0; l < lv.size(); ++l) {
+    const OperatorLevel& o = lv[l];
+    PMCD_CHECK(o.n > 0 && o.n_active >= 0 && o.n_active <= o.n, "level ", l, ": invalid sizes");
+    PMCD_CHECK((int)o.row_ptr.size() == o.n + 1 && o.row_ptr[0] == 0, "level ", l, ": row_ptr size");
+    PMCD_CHECK((size_t)o.row_ptr[o.n] == o.col.size() && o.col.size() == o.val.size(), "level ", l,
+               ": column and value counts differ from row_ptr");
+    auto L = std::make_unique<Level>();
+    L->space.n_dof = o.n;
+    L->space.n_active = o.n_active;
+    L->space.n_dirichlet = o.n - o.n_active;
+    Structure& st = L->st;
+    st.n_rows = o.n;
+    st.row_ptr = o.row_ptr;
+    st.col = o.col;
+    st.nnz = (int64_t)o.col.size();
+    st.diag.assign(o.n, -1);
+    for(int r = 0; r < o.n; ++r) {
+      st.max_row = std::max(st.max_row, o.row_ptr[r + 1] - o.row_ptr[r]);
+      for(int k = o.row_ptr[r]; k < o.row_ptr[r + 1]; ++k) {
+        PMCD_CHECK(o.col[k] >= 0 && o.col[k] < o.n, "level ", l, ": column index out of range");
+        if(o.col[k] == r) st.diag[r] = k;
+      }
+      PMCD_CHECK(st.diag[r] >= 0, "level ", l, ": row ", r, " has no diagonal entry");
+    }
+    L->row_ptr.upload(st.row_ptr, stream_);
+    L->col.upload(st.col, stream_);
+    L->diag_pos.upload(st.diag, stream_);
+    L->val.upload(o.val, stream_);
+    const int n = o.n;
+    L->rhs.alloc(n);
+    L->sol.alloc(n);
+    L->diag.alloc(n);
+    L->mg_sol.alloc(n); L->mg_sol2.alloc(n); L->mg_rhs.alloc(n); L->mg_def.alloc(n); L->mg_tmp.alloc(n);
+    launch_fill(n, 0.0, L->rhs.p, stream_);
+    launch_fill(n, 0.0, L->sol.p, stream_);
+    launch_gather_idx(n, L->diag_pos.p, L->val.p, L->diag.p, stream_);
+    levels_.push_back(std::move(L));
+  }
+  for(size_t l = 1; l < lv.size(); ++l) {
+    const OperatorLevel& o = lv[l];
+    PMCD_CHECK(o.n_coarse == lv[l - 1].n, "level ", l, ": prolongation columns differ from the coarser level");
+    PMCD_CHECK((int)o.p_row_ptr.size() == o.n + 1 && (size_t)o.p_row_ptr[o.n] == o.p_col.size() &&
+                   o.p_col.size() == o.p_val.size(),
+               "level ", l, ": prolongation CSR sizes");
+    for(int c : o.p_col) PMCD_CHECK(c >= 0 && c < o.n_coarse, "level ", l, ": prolongation column out of range");
+    set_transfer(*levels_[l], o.n_coarse, o.p_row_ptr, o.p_col, o.p_val);
+  }
+  red_partial_.alloc(1024 * 5 + 64);
+  red_out_.alloc(64);
+  PMCD_HIP(hipStreamSynchronize(stream_));
+}
+
+void CDSystem::set_level_values(int level, const double* val) {
+  PMCD_CHECK(level >= 0 && level < n_levels(), "level out of range");
+  Level& L = *levels_[level];
+  L.val.upload(val, L.val.n, stream_);
+  launch_gather_idx(L.n(), L.diag_pos.p, L.val.p, L.diag.p, stream_);
+  L.sw_fresh = L.sw_fresh32 = L.sell_fresh = L.sell_fresh32 = false;
+  coarse_ready_ = false;
+  mixed_ready_ = false;
+  PMCD_HIP(hipStreamSynchronize(stream_));
+}
+
+void CDSystem::set_rhs(const double* b) {
+  Level& F = finest();
+  F.rhs.upload(b, F.rhs.n, stream_);
   PMCD_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -200,6 +282,7 @@ void CDSystem::boundary_data(Level& L) {
 // spaces: the device structures stay, the boundary data is recomputed; the
 // next assemble() uses the new data.
 void CDSystem::set_example(const ExampleParams& ex) {
+  PMCD_CHECK(!operator_only_, "an operator-only system has no PDE data");
   desc_.ex = ex;
   for(auto& L : levels_) boundary_data(*L);
   PMCD_HIP(hipStreamSynchronize(stream_));
@@ -313,6 +396,13 @@ void CDSystem::build_transfer(Level& F, const Level& C) {
     for(auto& e : r) { col.push_back(e.first); val.push_back(e.second); }
     rp[f + 1] = (int)col.size();
   }
+  set_transfer(F, ncd, rp, col, val);
+}
+
+// P (n_fine x ncd, CSR) and its transpose R on the device, FP64 and FP32
+void CDSystem::set_transfer(Level& F, int ncd, const std::vector<int>& rp, const std::vector<int>& col,
+                            const std::vector<double>& val) {
+  const int nf = (int)rp.size() - 1;
   // transpose
   std::vector<int> trp(ncd + 1, 0), tcol(col.size());
   std::vector<double> tval(col.size());
@@ -377,6 +467,7 @@ void CDSystem::assemble_level(Level& L) {
 }
 
 void CDSystem::assemble() {
+  PMCD_CHECK(!operator_only_, "an operator-only system has no mesh to assemble on");
   PMCD_HIP(hipStreamSynchronize(stream_));
   const double t0 = now_s();
   for(auto& L : levels_) assemble_level(*L);

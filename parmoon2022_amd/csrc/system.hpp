@@ -123,6 +123,14 @@ struct Level {
   int n_active() const { return space.n_active; }
 };
 
+// Quadrature rule of the cells and the device reference tables of the
+// element: basis values and derivatives at the rule's points (ncomp
+// components), weights, points, and (if `tensors`) the reference tensors of
+// the tensor assembly.
+QuadRule upload_reference_tables(Shape shape, int order, bool linear_quad, bool tensors, int& ncomp,
+                                 DevBuf<double>& ref, DevBuf<double>& qw, DevBuf<double>& qp,
+                                 DevBuf<double>& tensor, hipStream_t stream);
+
 // Host-side setup only (meshes, FE spaces, sparsity, slot maps): no GPU use.
 std::vector<std::unique_ptr<Level>> build_levels(const ProblemDesc& d);
 
@@ -165,9 +173,29 @@ std::vector<int> cycle_directions(int type, int n_levels);
 void node_position(const Mesh& m, int c, const ElementInfo& info, int i, double* x);
 double now_seconds();
 
+// One level of a caller-owned operator hierarchy (coarsest first): the CSR
+// matrix, whose rows n_active.. are the identity rows of the Dirichlet DOFs
+// (their entries are not read), and, for every level but the coarsest, the
+// prolongation from the next coarser level (n x n_coarse CSR).
+struct OperatorLevel {
+  int n = 0, n_active = 0;
+  std::vector<int> row_ptr, col;
+  std::vector<double> val;
+  int n_coarse = 0;
+  std::vector<int> p_row_ptr, p_col;
+  std::vector<double> p_val;
+};
+
 class CDSystem {
  public:
   CDSystem(const ProblemDesc& d, hipStream_t stream);
+  // operator only (Solver<> on caller-owned matrices): no mesh, no assembly
+  CDSystem(const std::vector<OperatorLevel>& levels, hipStream_t stream);
+  bool operator_only() const { return operator_only_; }
+  // new matrix values of one level (same pattern), from host memory
+  void set_level_values(int level, const double* val);
+  // right-hand side of the finest level, from host memory
+  void set_rhs(const double* b);
   ~CDSystem();
   CDSystem(const CDSystem&) = delete;
   CDSystem& operator=(const CDSystem&) = delete;
@@ -225,6 +253,8 @@ class CDSystem {
   void build_level_device(Level& L);
   void boundary_data(Level& L);
   void build_transfer(Level& fine, const Level& coarse);
+  void set_transfer(Level& fine, int n_coarse, const std::vector<int>& rp, const std::vector<int>& col,
+                    const std::vector<double>& val);
   void assemble_level(Level& L);
   void coarse_factorize();
   // `kind`: 0 Jacobi, 1 SOR, 2 SSOR; -1 the solver's multigrid smoother
@@ -247,6 +277,7 @@ class CDSystem {
   SolveStats solve_sor(int flag);
 
   ProblemDesc desc_;
+  bool operator_only_ = false;
   SolverDesc sd_{};
   hipStream_t stream_;
   std::vector<std::unique_ptr<Level>> levels_;  // coarsest first

@@ -39,6 +39,7 @@ void CDSystem::tcd_assemble() {
 }
 
 void CDSystem::tcd_init(const TcdDesc& t) {
+  PMCD_CHECK(!operator_only_, "an operator-only system has no mesh to assemble on");
   PMCD_CHECK(t.scheme >= 0 && t.scheme <= 2, "unknown time discretization ", t.scheme);
   PMCD_CHECK(t.bdf_first == 0 || t.bdf_first == 1, "unknown first BDF2 step ", t.bdf_first);
   PMCD_CHECK(t.tau > 0, "time step length must be positive");

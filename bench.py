@@ -273,6 +273,8 @@ def bench_c4(args):
     import parmoon2022_amd as pm
     dbs = pm.c4_instances(args.instances)
     sdb = dict(C4_SOLVER, multigrid_mixed_precision=1 if args.mixed else 0)
+    if args.no_graphs:
+        pm.set_tuning_value(16, 0)
     t0 = time.time()
     bt = pm.Batch(dbs)
     info = bt.info()
@@ -601,6 +603,9 @@ def main():
                     help="multigrid smoother (multigrid_smoother)")
     ap.add_argument("--sor-omega", type=float, default=1.0, help="sor_omega for the SOR smoothers")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="c4: launch each batched iteration kernel by kernel instead of replaying its graph "
+                         "(same kernels and results; for tracing tools that do not follow graph launches)")
     ap.add_argument("--cpu-sample", type=int, default=64)
     ap.add_argument("--fixed-shape", default=None, metavar="LANES,VARIANT",
                     help="skip the autotuner: lanes per row and variant (0 plain, 1 stream, "

@@ -485,6 +485,14 @@ int pmcd_prolongate_device(int n_fine, int n_active_fine, const int* p_row_ptr, 
 int pmcd_restrict_device(int n_coarse, int n_active_coarse, const int* r_row_ptr, const int* r_col,
                          const double* r_entries, const double* fine, double* coarse, void* stream);
 
+/* Batched workload instances: std::mt19937_64 seeded `seed`, per instance in
+ * this order N = uniform_int_distribution(n_lo, n_hi), eps = 10^u with u =
+ * uniform_real_distribution(-6, 0), g = three normal_distribution(0, 1)
+ * draws, b = g / |g| (n_out[n_inst], eps_out[n_inst], b_out[3 n_inst]). No
+ * reference counterpart (the synthetic batch of independent problems). */
+int pmcd_batch_instances(unsigned long long seed, int n_inst, int n_lo, int n_hi, int* n_out, double* eps_out,
+                         double* b_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -77,6 +77,7 @@ EXPORTS = [
     "pmcd_assembler_assemble_device", "pmcd_solver_create", "pmcd_solver_destroy", "pmcd_solver_update_matrix",
     "pmcd_solver_solve", "pmcd_solver_is_using_multigrid", "pmcd_get_transfer", "pmcd_axpy_device",
     "pmcd_norm_device", "pmcd_jacobi_device", "pmcd_prolongate_device", "pmcd_restrict_device",
+    "pmcd_batch_instances",
     "pmcd_batch_create", "pmcd_batch_destroy", "pmcd_batch_info", "pmcd_batch_assemble",
     "pmcd_batch_solve", "pmcd_batch_get_solution", "pmcd_batch_set_solution", "pmcd_batch_get_matrix",
     "pmcd_batch_get_rhs",
@@ -897,15 +898,17 @@ class Batch:
 def c4_instances(n_inst=256, seed=20261202, n_range=(8, 40)):
     """Batched workload: unit cubes of N^3 cubes split into 6 tetrahedra each,
     N uniform in n_range, P1, SUPG, eps = 10^U(-6, 0), b a unit vector uniform
-    on the sphere, f = 1, homogeneous Dirichlet data (numpy PCG64 seeded
-    `seed`)."""
-    rng = np.random.default_rng(seed)
+    on the sphere, f = 1, homogeneous Dirichlet data; drawn by
+    std::mt19937_64 seeded `seed` in the library (pmcd_batch_instances)."""
+    ns = np.zeros(n_inst, np.int32)
+    epss = np.zeros(n_inst)
+    bs = np.zeros(3 * n_inst)
+    _check(lib().pmcd_batch_instances(ctypes.c_ulonglong(seed), n_inst, n_range[0], n_range[1],
+                                      _ptr(ns, ctypes.c_int), _ptr(epss, ctypes.c_double),
+                                      _ptr(bs, ctypes.c_double)))
     out = []
-    for _ in range(n_inst):
-        n = int(rng.integers(n_range[0], n_range[1] + 1))
-        eps = float(10.0 ** rng.uniform(-6, 0))
-        g = rng.standard_normal(3)
-        b = g / np.linalg.norm(g)
+    for i in range(n_inst):
+        n, eps, b = int(ns[i]), float(epss[i]), bs[3 * i:3 * i + 3]
         out.append({"geo_file": "StructuredTetra", "structured_n": n, "ANSATZ_ORDER": 1,
                     "refinement_n_initial_steps": 0, "example": "const_rhs", "space_discretization_type": "supg",
                     "diffusion_coefficient": eps, "convection": tuple(float(v) for v in b), "reaction": 0.0,
@@ -996,6 +999,17 @@ def set_sell_mode(mode):
 def get_sell_mode():
     v = ctypes.c_int(0)
     _check(lib().pmcd_get_tuning(13, ctypes.byref(v)))
+    return v.value
+
+
+def set_tuning_value(key, value):
+    """Raw tuning knob (pmcd_set_tuning key and value)."""
+    _check(lib().pmcd_set_tuning(int(key), int(value)))
+
+
+def tuning_value(key):
+    v = ctypes.c_int(0)
+    _check(lib().pmcd_get_tuning(int(key), ctypes.byref(v)))
     return v.value
 
 

@@ -5,49 +5,6 @@
 
 namespace pmcd {
 
-namespace {
-
-// IterativeMethod::converged (include/Solver/IterativeMethod.h:83-162) for one
-// instance; a diverging instance is reported, not thrown, so that the others
-// finish
-struct InstConv {
-  double initial = 0, previous = 0;
-  bool diverged = false;
-  bool operator()(const SolverDesc& sd, double res, int it) {
-    if(it == 0) { initial = res; previous = res; }
-    const bool small = res < sd.res_tol;
-    const bool reduced = res < sd.res_red * initial;
-    const bool enough = it >= sd.min_iter;
-    const bool stagnation = res > 1e10 * previous;
-    const bool div = res > sd.div_factor * initial;
-    bool conv = (small || reduced || stagnation || div) && enough;
-    if(it == sd.max_iter) conv = true;
-    if(div) diverged = true;
-    previous = res;
-    return conv;
-  }
-};
-
-void givens_gen(double dx, double dy, double& cs, double& sn) {
-  if(dy == 0.0) { cs = 1.0; sn = 0.0; }
-  else if(std::abs(dy) > std::abs(dx)) {
-    double t = dx / dy;
-    sn = 1.0 / std::sqrt(1.0 + t * t);
-    cs = t * sn;
-  } else {
-    double t = dy / dx;
-    cs = 1.0 / std::sqrt(1.0 + t * t);
-    sn = t * cs;
-  }
-}
-
-void givens_app(double& dx, double& dy, double cs, double sn) {
-  double t = cs * dx + sn * dy;
-  dy = -sn * dx + cs * dy;
-  dx = t;
-}
-
-}  // namespace
 
 BatchSystem::BatchSystem(const std::vector<ProblemDesc>& descs, hipStream_t stream)
     : stream_(stream), descs_(descs) {
@@ -208,7 +165,13 @@ BatchSystem::BatchSystem(const std::vector<ProblemDesc>& descs, hipStream_t stre
   PMCD_HIP(hipStreamSynchronize(stream_));
 }
 
-BatchSystem::~BatchSystem() { (void)hipStreamSynchronize(stream_); }
+BatchSystem::~BatchSystem() {
+  (void)hipStreamSynchronize(stream_);
+  clear_graphs();
+  if(poll_host_) (void)hipHostFree(poll_host_);
+  for(auto& e : poll_ev_)
+    if(e) (void)hipEventDestroy(e);
+}
 
 void BatchSystem::assemble() {
   PMCD_HIP(hipStreamSynchronize(stream_));
@@ -249,10 +212,6 @@ void BatchSystem::assemble() {
   G_.sw_fresh = G_.sw_fresh32 = false;
 }
 
-void BatchSystem::seg_dot(const double* x, const double* y, double* out_dev) {
-  launch_seg_dot((int)chunks_host_.size(), chunks_.p, n_inst(), inst_ptr_.p, x, y, partial_.p, out_dev,
-                 stream_, active_.p);
-}
 
 // y = beta z + alpha A x with the non-active rule; rows of converged
 // instances become 0 (their vectors are all zero from then on)
@@ -287,17 +246,49 @@ void BatchSystem::precond(const double* v, double* z) {
   }
 }
 
+bool BatchSystem::coop_sweep(const void* b, void* x, bool f32, int flag) {
+  if(!batch_coop() || G_.sweep_exact || G_.sw_rp.n == 0) return false;
+  const int K = (int)G_.sweep_ptr.size() - 1;
+  const double mean_row = (double)G_.st.nnz / std::max(1, G_.n());
+  if(f32)
+    return launch_sor_sweep_coop_f32(K, G_.sweep_ptr_dev.p, G_.sweep_max_class, G_.sweep_rows.p, G_.sw_rp.p,
+                                     G_.sw_col.p, G_.sw_val32.p, G_.diag32.p, (const float*)b, (float*)x,
+                                     (float)sd_.sor_omega, flag, G_.st.max_row, mean_row, row_inst_.p, active_.p,
+                                     stream_);
+  return launch_sor_sweep_coop(K, G_.sweep_ptr_dev.p, G_.sweep_max_class, G_.sweep_rows.p, G_.sw_rp.p, G_.sw_col.p,
+                               G_.sw_val.p, G_.diag.p, (const double*)b, (double*)x, sd_.sor_omega, flag,
+                               G_.st.max_row, mean_row, row_inst_.p, active_.p, stream_);
+}
+
+void BatchSystem::clear_graphs() {
+  for(auto& g : graphs_)
+    if(g) (void)hipGraphExecDestroy(g);
+  graphs_.clear();
+  graph_key_ = -1;
+}
+
 // Iteration_gmres::flexible_gmres (src/Solver/Iteration_gmres.C:430-620) on
 // every instance at once: the Krylov vectors are global vectors whose
-// instance segments are independent; Hessenberg, Givens and convergence are
-// per instance on the host (one device->host copy per iteration). A
-// converged instance's segments are scaled to zero and stay zero.
+// instance segments are independent. Each instance's Hessenberg matrix,
+// Givens rotations, convergence test and update live on the device
+// (k_batch_givens, k_seg_update, k_batch_restart_state), so an iteration is a
+// fixed kernel sequence: with an SOR/SSOR preconditioner on multicolour
+// classes one cooperative launch per sweep, then one replayed graph for the
+// rest of the iteration (the product, the Gram-Schmidt steps with their
+// reductions folded in, the next basis vector, the rotations and the
+// updates of instances that converged). The host polls the active flags one
+// iteration behind; a converged instance is masked out of every kernel.
 std::vector<SolveStats> BatchSystem::solve(const SolverDesc& sd) {
   sd_ = sd;
   PMCD_CHECK(sd.method == 2, "batched solves use flexible GMRES");
+  PMCD_CHECK(sd.precond == 0 || sd.precond == 1 || sd.precond == 3 || sd.precond == 4,
+             "batched solves support no, jacobi, sor and ssor preconditioning");
   const int n = n_, ni = n_inst(), m = sd.restart;
+  const int nch = (int)chunks_host_.size();
   const double t0 = now_seconds();
-  if(sd.precond == 3 || sd.precond == 4) prepare_level_sweep(G_, stream_);
+  const bool sor = sd.precond == 3 || sd.precond == 4;
+  const int flag = sd.precond == 3 ? 0 : 2;
+  if(sor) prepare_level_sweep(G_, stream_);
   if(sd.mixed && !mixed_ready_) {
     G_.val32.alloc(G_.val.n);
     G_.diag32.alloc(n);
@@ -307,177 +298,181 @@ std::vector<SolveStats> BatchSystem::solve(const SolverDesc& sd) {
     f_z_.alloc(n);
     mixed_ready_ = true;
   }
+  // derived matrix copies brought up to date outside any capture
+  if(sor) refresh_sweep_values(G_, sd.mixed != 0, stream_);
   if(kv_.size() != (size_t)m + 1 || kr_.n != (size_t)n) {
+    clear_graphs();
     kv_.clear(); kz_.clear();
     for(int i = 0; i <= m; ++i) kv_.emplace_back(n);
     for(int i = 0; i < m; ++i) kz_.emplace_back(n);
     kr_.alloc(n);
     kw_.alloc(n);
+    std::vector<const double*> zp(m);
+    for(int i = 0; i < m; ++i) zp[i] = kz_[i].p;
+    kz_ptr_.upload(zp, stream_);
   }
-  kh_.alloc((size_t)(m + 2) * ni);
-  kscale_.alloc(ni);
-  kcoef_.alloc((size_t)(m + 2) * ni);
-  std::vector<SolveStats> st(ni);
-  std::vector<InstConv> conv(ni);
-  std::vector<char> active(ni, 1);
-  std::vector<double> H((size_t)ni * (m + 1) * m, 0.0), s((size_t)ni * (m + 1)), cs((size_t)ni * (m + 1)),
-      sn((size_t)ni * (m + 1)), resid(ni), beta(ni), scale(ni), hcol((size_t)(m + 2) * ni);
-  auto h = [&](int inst, int i, int j) -> double& { return H[((size_t)inst * m + j) * (m + 1) + i]; };
+  if(kh_.n != (size_t)(m + 2) * ni) {
+    clear_graphs();
+    kh_.alloc((size_t)(m + 2) * ni);
+    bH_.alloc((size_t)ni * m * (m + 1));
+    bs_.alloc((size_t)ni * (m + 1));
+    bcs_.alloc((size_t)ni * (m + 1));
+    bsn_.alloc((size_t)ni * (m + 1));
+    by_.alloc((size_t)ni * m);
+    bscale_.alloc(ni);
+    bst_.alloc(ni);
+    bbreak_.alloc(1);
+    bpart_.alloc(2 * (size_t)std::max(1, nch));
+  }
+  if(!poll_host_) {
+    PMCD_HIP(hipHostMalloc((void**)&poll_host_, 2 * ((size_t)ni + 8), hipHostMallocDefault));
+    for(auto& e : poll_ev_) PMCD_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  const BatchSolveParams prm{sd.res_tol, sd.res_red, sd.div_factor, sd.min_iter, sd.max_iter, m};
+  double* P[2] = {bpart_.p, bpart_.p + std::max(1, nch)};
   double* x = G_.sol.p;
   double* r = kr_.p;
-  auto residual = [&]() {  // r = b - A x, beta = |r| per instance
+  // a cooperative sweep (multicolour classes, device support) leaves the rest
+  // of an iteration to the graphs
+  const bool coop = sor && batch_coop() && !G_.sweep_exact && G_.sw_rp.n > 0 && cooperative_launch_supported();
+  auto sweep_input = [&](int i) {  // before the sweep of iteration i
+    if(sd.mixed) {
+      launch_d2f(n, kv_[i].p, f_v_.p, stream_);
+      launch_fill_f32(n, 0.0f, f_z_.p, stream_);
+    } else {
+      launch_fill(n, 0.0, kz_[i].p, stream_);
+    }
+  };
+  auto run_sweep = [&](int i) {
+    const bool ran = sd.mixed ? coop_sweep(f_v_.p, f_z_.p, true, flag) : coop_sweep(kv_[i].p, kz_[i].p, false, flag);
+    PMCD_CHECK(ran, "cooperative sweep not launched");
+  };
+  // every instance starts active; no breakdown yet (outside any capture)
+  std::fill(active_host_.begin(), active_host_.end(), 1);
+  PMCD_HIP(hipMemcpyAsync(active_.p, active_host_.data(), ni, hipMemcpyHostToDevice, stream_));
+  PMCD_HIP(hipMemsetAsync(bbreak_.p, 0, sizeof(int), stream_));
+  auto residual = [&](int mode) {  // r = b - A x, beta, tests
     launch_copy(n, G_.rhs.p, r, stream_);
     spmv(x, r, r, -1.0, 1.0);
-    seg_dot(r, r, kh_.p);
-    PMCD_HIP(hipMemcpyAsync(beta.data(), kh_.p, sizeof(double) * ni, hipMemcpyDeviceToHost, stream_));
-    PMCD_HIP(hipStreamSynchronize(stream_));
-    for(auto& b : beta) b = std::sqrt(b);
+    launch_seg_dot_partial(nch, chunks_.p, mode == 0 ? nullptr : active_.p, r, r, P[0], stream_);
+    launch_batch_restart_state(ni, prm, inst_ptr_.p, P[0], bst_.p, bs_.p, bscale_.p, active_.p, mode, stream_);
+    launch_seg_scale(nch, chunks_.p, bscale_.p, r, kv_[0].p, stream_, active_.p);
   };
-  // active flags on the device follow active_host_ (copied after each change;
-  // the host array changes again only after a later stream synchronisation)
-  auto push_active = [&]() {
-    PMCD_HIP(hipMemcpyAsync(active_.p, active_host_.data(), ni, hipMemcpyHostToDevice, stream_));
-  };
-  std::fill(active_host_.begin(), active_host_.end(), 1);
-  push_active();
-  auto set_scale = [&](const std::vector<double>& a) {
-    PMCD_HIP(hipMemcpyAsync(kscale_.p, a.data(), sizeof(double) * ni, hipMemcpyHostToDevice, stream_));
-  };
-  // x += sum_k y_k z_k for the instances in `which`, after a back-solve of
-  // their Hessenberg systems of size k + 1 (Utilities_gmres.C:90-108); the
-  // sum is accumulated in kw_ (r holds the current Arnoldi vector)
-  std::vector<double> coef((size_t)(m + 2) * ni);
-  auto update = [&](int k, const std::vector<char>& which) {
-    std::fill(coef.begin(), coef.end(), 0.0);
-    bool any = false;
-    std::vector<double> yi(k + 1);
-    for(int inst = 0; inst < ni; ++inst) {
-      if(!which[inst]) continue;
-      any = true;
-      for(int i = 0; i <= k; ++i) yi[i] = s[(size_t)inst * (m + 1) + i];
-      for(int i = k; i >= 0; --i) {
-        yi[i] /= h(inst, i, i);
-        for(int jj = i - 1; jj >= 0; --jj) yi[jj] -= h(inst, jj, i) * yi[i];
-      }
-      for(int i = 0; i <= k; ++i) coef[(size_t)i * ni + inst] = yi[i];
-      coef[(size_t)(k + 1) * ni + inst] = 1.0;
-    }
-    if(!any) return;
-    PMCD_HIP(hipMemcpyAsync(kcoef_.p, coef.data(), sizeof(double) * (size_t)(k + 2) * ni,
-                            hipMemcpyHostToDevice, stream_));
-    launch_fill(n, 0.0, kw_.p, stream_);
-    for(int jj = 0; jj <= k; ++jj)
-      launch_seg_axpy((int)chunks_host_.size(), chunks_.p, kcoef_.p + (size_t)jj * ni, kz_[jj].p, kw_.p,
-                      stream_);
-    launch_seg_axpy((int)chunks_host_.size(), chunks_.p, kcoef_.p + (size_t)(k + 1) * ni, kw_.p, x,
-                    stream_);
-    PMCD_HIP(hipStreamSynchronize(stream_));
-  };
-  auto sync_active = [&]() {
-    for(int inst = 0; inst < ni; ++inst) active_host_[inst] = active[inst] ? 1 : 0;
-    push_active();
-  };
-  residual();
-  int n_active_inst = 0;
-  for(int inst = 0; inst < ni; ++inst) {
-    resid[inst] = beta[inst];
-    st[inst].initial_residual = beta[inst];
-    if(conv[inst](sd_, beta[inst], 0)) {
-      active[inst] = 0;
-      st[inst].final_residual = beta[inst];
-      st[inst].converged = 1;
+  // the part of iteration i after the preconditioner's sweep (or all of it)
+  auto tail = [&](int i, bool after_coop) {
+    if(after_coop) {
+      if(sd.mixed) launch_f2d(n, f_z_.p, kz_[i].p, stream_);
     } else {
-      ++n_active_inst;
-    }
-  }
-  sync_active();
-  int j = 1;
-  while(j <= sd_.max_iter && n_active_inst > 0) {
-    for(int inst = 0; inst < ni; ++inst) {
-      scale[inst] = active[inst] ? 1.0 / beta[inst] : 0.0;
-      std::fill(&s[(size_t)inst * (m + 1)], &s[(size_t)inst * (m + 1)] + m + 1, 0.0);
-      s[(size_t)inst * (m + 1)] = beta[inst];
-    }
-    set_scale(scale);
-    launch_seg_scale((int)chunks_host_.size(), chunks_.p, kscale_.p, r, kv_[0].p, stream_, active_.p);
-    int i = 0;
-    for(; i < m && j <= sd_.max_iter && n_active_inst > 0; ++i, ++j) {
       precond(kv_[i].p, kz_[i].p);
-      spmv(kz_[i].p, nullptr, r, 1.0, 0.0);
-      // modified Gram-Schmidt, one fused step per basis vector
-      seg_dot(r, kv_[0].p, kh_.p);
-      for(int k = 0; k <= i; ++k)
-        launch_seg_mgs((int)chunks_host_.size(), chunks_.p, ni, inst_ptr_.p, kh_.p + (size_t)k * ni, kv_[k].p,
-                       r, k < i ? kv_[k + 1].p : nullptr, partial_.p, kh_.p + (size_t)(k + 1) * ni, stream_,
-                       active_.p);
-      seg_dot(r, r, kh_.p + (size_t)(i + 1) * ni);
-      PMCD_HIP(hipMemcpyAsync(hcol.data(), kh_.p, sizeof(double) * (size_t)(i + 2) * ni,
-                              hipMemcpyDeviceToHost, stream_));
-      PMCD_HIP(hipStreamSynchronize(stream_));
-      std::vector<char> done(ni, 0);
-      bool any_done = false;
-      for(int inst = 0; inst < ni; ++inst) {
-        scale[inst] = 0.0;
-        if(!active[inst]) continue;
-        for(int k = 0; k <= i; ++k) h(inst, k, i) = hcol[(size_t)k * ni + inst];
-        h(inst, i + 1, i) = std::sqrt(hcol[(size_t)(i + 1) * ni + inst]);
-        PMCD_CHECK(h(inst, i + 1, i) != 0.0, "Unhappy breakdown in flexible gmres at iteration ", j,
-                   " of batch instance ", inst);
-        scale[inst] = 1.0 / h(inst, i + 1, i);
-        double* si = &s[(size_t)inst * (m + 1)];
-        double* ci = &cs[(size_t)inst * (m + 1)];
-        double* ni_ = &sn[(size_t)inst * (m + 1)];
-        for(int k = 0; k < i; ++k) givens_app(h(inst, k, i), h(inst, k + 1, i), ci[k], ni_[k]);
-        givens_gen(h(inst, i, i), h(inst, i + 1, i), ci[i], ni_[i]);
-        givens_app(h(inst, i, i), h(inst, i + 1, i), ci[i], ni_[i]);
-        givens_app(si[i], si[i + 1], ci[i], ni_[i]);
-        resid[inst] = std::abs(si[i + 1]);
-        st[inst].iterations = j;
-        if(conv[inst](sd_, resid[inst], j)) {
-          done[inst] = 1;
-          any_done = true;
-        }
-      }
-      if(any_done) {
-        update(i, done);
-        for(int inst = 0; inst < ni; ++inst)
-          if(done[inst]) {
-            active[inst] = 0;
-            scale[inst] = 0.0;
-            --n_active_inst;
-            st[inst].final_residual = resid[inst];
-            st[inst].converged = resid[inst] < sd_.res_tol ||
-                                 resid[inst] < sd_.res_red * st[inst].initial_residual;
-          }
-        sync_active();
-      }
-      set_scale(scale);
-      launch_seg_scale((int)chunks_host_.size(), chunks_.p, kscale_.p, r, kv_[i + 1].p, stream_, active_.p);
     }
-    if(n_active_inst == 0) break;
-    // restart: update, true residual
-    update(i - 1, active);
-    residual();
-    for(int inst = 0; inst < ni; ++inst) {
-      if(!active[inst]) continue;
-      if(conv[inst](sd_, resid[inst], j)) {
-        active[inst] = 0;
-        --n_active_inst;
-        st[inst].final_residual = resid[inst];
-        st[inst].converged = resid[inst] < sd_.res_tol ||
-                             resid[inst] < sd_.res_red * st[inst].initial_residual;
-      }
-    }
-    sync_active();
+    spmv(kz_[i].p, nullptr, r, 1.0, 0.0);
+    // modified Gram-Schmidt: H(k,i) = <r, v_k>; r -= H(k,i) v_k; |r|
+    launch_seg_dot_partial(nch, chunks_.p, active_.p, r, kv_[0].p, P[0], stream_);
+    for(int k = 0; k <= i; ++k)
+      launch_seg_mgs2(nch, chunks_.p, active_.p, inst_ptr_.p, P[k & 1], kh_.p + (size_t)k * ni, kv_[k].p, r,
+                      k < i ? kv_[k + 1].p : nullptr, P[(k + 1) & 1], stream_);
+    launch_seg_normscale(nch, chunks_.p, active_.p, inst_ptr_.p, P[(i + 1) & 1], kh_.p + (size_t)(i + 1) * ni, r,
+                         kv_[i + 1].p, stream_);
+    launch_batch_givens(ni, i, prm, kh_.p, bH_.p, bs_.p, bcs_.p, bsn_.p, bst_.p, by_.p, active_.p, bbreak_.p,
+                        stream_);
+    launch_seg_update(nch, chunks_.p, bst_.p, m, by_.p, kz_ptr_.p, x, stream_);
+    if(after_coop && i + 1 < m) sweep_input(i + 1);
+  };
+  // graphs: one per iteration index; valid for the buffers, the solver
+  // parameters and the launch shapes they were captured with
+  const long long key = ((((long long)m * 8 + sd.precond) * 2 + sd.mixed) * 2 + (coop ? 1 : 0)) * 100003LL +
+                        tuning_epoch();
+  if(graph_key_ != key) {
+    clear_graphs();
+    graph_key_ = key;
+    graphs_.assign(m, nullptr);
   }
-  for(int inst = 0; inst < ni; ++inst)
-    if(active[inst]) {
-      st[inst].final_residual = resid[inst];
-      st[inst].converged = resid[inst] < sd_.res_tol || resid[inst] < sd_.res_red * st[inst].initial_residual;
+  auto run_tail = [&](int i, bool after_coop) {
+    if(!batch_graphs()) {
+      tail(i, after_coop);
+      return;
     }
+    if(!graphs_[i]) {
+      PMCD_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      try {
+        tail(i, after_coop);
+      } catch(...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream_, &g);
+        if(g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hipGraph_t graph = nullptr;
+      PMCD_HIP(hipStreamEndCapture(stream_, &graph));
+      hipGraphExec_t exec = nullptr;
+      const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      PMCD_HIP(e);
+      graphs_[i] = exec;
+    }
+    PMCD_HIP(hipGraphLaunch(graphs_[i], stream_));
+  };
+  // host view of the active flags, one iteration behind the device
+  int slot = 0;
+  bool pending = false, all_done = false;
+  auto poll = [&]() {
+    uint8_t* h = poll_host_ + (size_t)slot * (ni + 8);
+    PMCD_HIP(hipMemcpyAsync(h, active_.p, ni, hipMemcpyDeviceToHost, stream_));
+    PMCD_HIP(hipMemcpyAsync(h + ni, bbreak_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    PMCD_HIP(hipEventRecord(poll_ev_[slot], stream_));
+    if(pending) {
+      const int prev = slot ^ 1;
+      PMCD_HIP(hipEventSynchronize(poll_ev_[prev]));
+      const uint8_t* hp = poll_host_ + (size_t)prev * (ni + 8);
+      int brk = 0;
+      std::memcpy(&brk, hp + ni, sizeof(int));
+      PMCD_CHECK(brk == 0, "Unhappy breakdown in flexible gmres of a batch instance");
+      all_done = std::none_of(hp, hp + ni, [](uint8_t a) { return a != 0; });
+    }
+    pending = true;
+    slot ^= 1;
+  };
+  residual(0);
+  poll();
+  int j = 1;
+  while(j <= sd.max_iter && !all_done) {
+    int i = 0;
+    if(coop) sweep_input(0);
+    for(; i < m && j <= sd.max_iter && !all_done; ++i, ++j) {
+      if(coop) run_sweep(i);
+      run_tail(i, coop);
+      poll();
+    }
+    if(all_done || j > sd.max_iter) break;
+    // restart: the update with the cycle's m coefficients, the true residual
+    launch_batch_restart_coef(ni, prm, bH_.p, bs_.p, bst_.p, by_.p, active_.p, stream_);
+    launch_seg_update(nch, chunks_.p, bst_.p, m, by_.p, kz_ptr_.p, x, stream_);
+    residual(1);
+    poll();
+  }
   PMCD_HIP(hipStreamSynchronize(stream_));
+  {
+    // the last flags (breakdown of the final iterations)
+    int brk = 0;
+    PMCD_HIP(hipMemcpy(&brk, bbreak_.p, sizeof(int), hipMemcpyDeviceToHost));
+    PMCD_CHECK(brk == 0, "Unhappy breakdown in flexible gmres of a batch instance");
+  }
+  std::vector<BatchInstState> fin(ni);
+  bst_.download(fin.data(), ni, stream_);
+  PMCD_HIP(hipStreamSynchronize(stream_));
+  // every instance leaves the active set converged or at max_n_iterations
+  std::fill(active_host_.begin(), active_host_.end(), 1);
+  PMCD_HIP(hipMemcpyAsync(active_.p, active_host_.data(), ni, hipMemcpyHostToDevice, stream_));
+  PMCD_HIP(hipStreamSynchronize(stream_));
+  std::vector<SolveStats> st(ni);
   const double secs = now_seconds() - t0;
-  for(auto& x : st) x.seconds = secs;
+  for(int inst = 0; inst < ni; ++inst) {
+    st[inst].iterations = fin[inst].iterations;
+    st[inst].initial_residual = fin[inst].initial;
+    st[inst].final_residual = fin[inst].resid;
+    st[inst].converged = fin[inst].converged;
+    st[inst].seconds = secs;
+  }
   return st;
 }
 

@@ -56,7 +56,6 @@ class BatchSystem {
   }
   void to_global(int i, const double* local, double* global_host);
   void precond(const double* v, double* z);
-  void seg_dot(const double* x, const double* y, double* out_dev);
   void spmv(const double* x, const double* z, double* y, double alpha, double beta);
 
   hipStream_t stream_;
@@ -78,13 +77,33 @@ class BatchSystem {
   // solver work
   SolverDesc sd_{};
   std::vector<DevBuf<double>> kv_, kz_;
-  DevBuf<double> kr_, kw_, kh_, kscale_, kcoef_;
+  DevBuf<double> kr_, kw_, kh_;
   // converged instances are masked out of the matrix kernels
   DevBuf<int> row_inst_;
   DevBuf<uint8_t> active_;
   std::vector<uint8_t> active_host_;
   DevBuf<float> f_v_, f_z_;
   bool mixed_ready_ = false;
+  // device-resident FGMRES state: per-instance Hessenberg matrices, Givens
+  // rotations, rhs of the least-squares problem, update coefficients, scale
+  // factors, instance states, a breakdown flag, ping-pong reduction partials
+  // and the flexible basis' device pointers
+  DevBuf<double> bH_, bs_, bcs_, bsn_, by_, bscale_, bpart_;
+  DevBuf<BatchInstState> bst_;
+  DevBuf<int> bbreak_;
+  DevBuf<const double*> kz_ptr_;
+  // pinned host copies of the active flags and breakdown flag (two slots,
+  // polled one iteration behind), and their events
+  uint8_t* poll_host_ = nullptr;
+  hipEvent_t poll_ev_[2] = {nullptr, nullptr};
+  // one captured graph per iteration index of a restart cycle (the part of
+  // an iteration after a cooperative sweep, or the whole iteration)
+  std::vector<hipGraphExec_t> graphs_;
+  long long graph_key_ = -1;
+  void clear_graphs();
+  // one preconditioner sweep on the current sweep classes; true if it ran as
+  // one cooperative launch
+  bool coop_sweep(const void* b, void* x, bool f32, int flag);
 };
 
 }  // namespace pmcd

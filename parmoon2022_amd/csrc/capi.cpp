@@ -5,6 +5,7 @@
 #include "io.hpp"
 #include "dropin.hpp"
 #include <cmath>
+#include <random>
 #include <cstring>
 #include <algorithm>
 #include <string>
@@ -890,6 +891,26 @@ int pmcd_restrict_device(int n_coarse, int n_active_coarse, const int* r_row_ptr
     launch_restrict(n_coarse, n_active_coarse, 0, -1, r_row_ptr, r_col, r_entries, fine, coarse,
                     (hipStream_t)stream);
     PMCD_HIP(hipGetLastError());
+  });
+}
+
+
+int pmcd_batch_instances(unsigned long long seed, int n_inst, int n_lo, int n_hi, int* n_out, double* eps_out,
+                         double* b_out) {
+  return guarded([&] {
+    PMCD_CHECK(n_inst >= 0 && n_lo >= 1 && n_hi >= n_lo && n_out && eps_out && b_out, "invalid argument");
+    std::mt19937_64 gen(seed);
+    std::uniform_int_distribution<int> cells(n_lo, n_hi);
+    std::uniform_real_distribution<double> expo(-6.0, 0.0);
+    std::normal_distribution<double> normal(0.0, 1.0);
+    for(int i = 0; i < n_inst; ++i) {
+      n_out[i] = cells(gen);
+      eps_out[i] = std::pow(10.0, expo(gen));
+      double g[3];
+      for(double& v : g) v = normal(gen);
+      const double len = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+      for(int k = 0; k < 3; ++k) b_out[3 * i + k] = g[k] / len;
+    }
   });
 }
 

@@ -146,6 +146,18 @@ void launch_sor_sweep_block_perm_f32(int K, const int* class_ptr, const int* row
 void launch_sor_sweep_block_f32(int K, const int* class_ptr, const int* rows, const int* rp, const int* col,
                                 const float* val, const float* diag, const float* b, float* x, float omega,
                                 int flag, hipStream_t s, const int* row_owner, const uint8_t* owner_active);
+bool cooperative_launch_supported();
+// a whole multicolour sweep (class-ordered copy) in one cooperative launch
+// with a grid-wide barrier between classes; false when the device does not
+// support cooperative launches (nothing launched)
+bool launch_sor_sweep_coop(int K, const int* class_ptr, int max_class, const int* rows, const int* sw_rp,
+                           const int* sw_col, const double* sw_val, const double* diag, const double* b, double* x,
+                           double omega, int flag, int max_row, double mean_row, const int* row_owner,
+                           const uint8_t* owner_active, hipStream_t s);
+bool launch_sor_sweep_coop_f32(int K, const int* class_ptr, int max_class, const int* rows, const int* sw_rp,
+                               const int* sw_col, const float* sw_val, const float* diag, const float* b, float* x,
+                               float omega, int flag, int max_row, double mean_row, const int* row_owner,
+                               const uint8_t* owner_active, hipStream_t s);
 // multicolour class [i0, i0 + count) over the class-ordered matrix copy
 // (sw_rp, sw_col, sw_val: permuted rows, original entry order)
 void launch_sor_rows_perm(int count, int i0, const int* rows, const int* sw_rp, const int* sw_col,
@@ -271,6 +283,40 @@ void launch_tcd_system(int n, int n_active, const int* rp, const double* A, cons
 struct SegChunk {
   int inst, begin, end;
 };
+// per-instance state of the device-resident batched FGMRES
+struct BatchInstState {
+  double initial, previous, resid, beta;
+  int iterations;  // FGMRES iterations done
+  int converged;   // residual below the tolerance or reduced enough
+  int update;      // the next update kernel adds sum_{k < n_coef} y_k z_k
+  int n_coef;
+};
+struct BatchSolveParams {
+  double res_tol, res_red, div_factor;
+  int min_iter, max_iter, m;
+};
+void launch_seg_mgs2(int n_chunks, const SegChunk* ch, const uint8_t* active, const int* inst_ptr,
+                     const double* prev, double* hcol_k, const double* v, double* w, const double* vnext,
+                     double* partial, hipStream_t s);
+void launch_seg_normscale(int n_chunks, const SegChunk* ch, const uint8_t* active, const int* inst_ptr,
+                          const double* prev, double* hcol, const double* w, double* vout, hipStream_t s);
+void launch_seg_dot_partial(int n_chunks, const SegChunk* ch, const uint8_t* active, const double* x,
+                            const double* y, double* partial, hipStream_t s);
+void launch_batch_givens(int ni, int i, const BatchSolveParams& p, const double* hcol, double* H, double* sv,
+                         double* cs, double* sn, BatchInstState* st, double* y, uint8_t* active, int* breakdown,
+                         hipStream_t s);
+void launch_batch_restart_coef(int ni, const BatchSolveParams& p, const double* H, const double* sv,
+                               BatchInstState* st, double* y, const uint8_t* active, hipStream_t s);
+void launch_seg_update(int n_chunks, const SegChunk* ch, const BatchInstState* st, int m, const double* y,
+                       const double* const* z, double* x, hipStream_t s);
+// mode 0: start of a solve (test at iteration 0 on beta), 1: restart
+void launch_batch_restart_state(int ni, const BatchSolveParams& p, const int* inst_ptr, const double* partial,
+                                BatchInstState* st, double* sv, double* scale, uint8_t* active, int mode,
+                                hipStream_t s);
+// batched solves: iteration graphs (tuning key 16) and cooperative sweeps
+// (tuning key 17) on or off
+int batch_graphs();
+int batch_coop();
 constexpr int kChunkRows = 4096;
 // out[i] = (x, y) restricted to instance i's rows (partial: one per chunk)
 // `active` (may be null): per-instance flags; chunks of inactive instances

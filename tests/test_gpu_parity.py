@@ -1197,3 +1197,36 @@ def test_fgmres_lookahead_same_bits(restart, mixed):
         pm.set_fgmres_lookahead(old)
     for it, res, u in out[1:]:
         assert it == out[0][0] and res == out[0][1] and np.array_equal(u, out[0][2])
+
+
+@pytest.mark.parametrize("mixed", [0, 1])
+def test_batch_graphs_and_cooperative_sweeps_same_bits(mixed):
+    """The device-resident batched FGMRES gives the same iterations and bits
+    with its iteration graphs on or off and with the multicolour SSOR sweep as
+    one cooperative launch or one launch per colour class."""
+    dbs = _batch_dbs()
+    for db in dbs:
+        db["multigrid_mixed_precision"] = mixed
+    old = [pm.tuning_value(k) for k in (5, 16, 17)]
+    out = []
+    try:
+        pm.set_sor_exact_limit(0)  # multicolour classes
+        for graphs, coop in ((0, 0), (1, 0), (0, 1), (1, 1)):
+            pm.set_tuning_value(16, graphs)
+            pm.set_tuning_value(17, coop)
+            bt = pm.Batch(dbs)
+            # restarts every 5 iterations, and a second solve on the same
+            # batch: the graphs captured first are replayed
+            for rep in range(2):
+                bt.assemble()
+                stats = bt.solve(dict(dbs[0], gmres_restart=5))
+                out.append(([s.iterations for s in stats], [bt.solution(i) for i in range(len(dbs))]))
+            bt.close()
+    finally:
+        for k, v in zip((5, 16, 17), old):
+            pm.set_tuning_value(k, v)
+    for its, sols in out[1:]:
+        assert its == out[0][0]
+        for a, b in zip(sols, out[0][1]):
+            assert np.array_equal(a, b)
+    assert all(i > 0 for i in out[0][0])

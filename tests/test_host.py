@@ -390,3 +390,21 @@ def test_structure_test_fixtures_host(geo, order, ex, expected):
     h = pm.HostSetup({"geo_file": path, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 0, "example": ex})
     rp, col = h.pattern()
     assert _structure_values(rp, col) == expected
+
+
+def test_batch_instances_generator():
+    """Batched workload instances from std::mt19937_64 (seed 20261202): N in
+    8..40, eps = 10^u with u in [-6, 0), b unit vectors; the same seed gives
+    the same instances, another seed others."""
+    a = pm.c4_instances(256)
+    b = pm.c4_instances(256)
+    c = pm.c4_instances(256, seed=7)
+    assert a == b and a != c
+    ns = np.array([d["structured_n"] for d in a])
+    assert ns.min() >= 8 and ns.max() <= 40 and len(set(ns)) > 25
+    eps = np.array([d["diffusion_coefficient"] for d in a])
+    assert np.all(eps <= 1.0) and np.all(eps >= 1e-6)
+    assert -3.8 < np.mean(np.log10(eps)) < -2.2  # mean of U(-6, 0) is -3
+    bv = np.array([d["convection"] for d in a])
+    assert np.allclose(np.linalg.norm(bv, axis=1), 1.0, atol=1e-14)
+    assert np.all(np.abs(bv.mean(axis=0)) < 0.15)  # directions spread over the sphere

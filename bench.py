@@ -74,50 +74,146 @@ def oracle_comparison(args, nq, u, errs):
             "within_1pct_l2": abs(float(errs[0]) - ref["l2_error"]) <= 0.01 * ref["l2_error"]}
 
 
+def machine():
+    """Host description attached to every CPU baseline: logical CPUs, the CPUs
+    this process may run on, the lscpu model line and socket/core counts, and
+    the cpufreq governor."""
+    import subprocess
+    out = {"nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in txt.splitlines():
+            key, _, val = line.partition(":")
+            if key.strip() in ("Model name", "Socket(s)", "Core(s) per socket", "Thread(s) per core",
+                               "CPU max MHz"):
+                out[key.strip()] = val.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/sys/devices/system/cpu/cpu0/cpufreq/scaling_governor") as f:
+            out["governor"] = f.read().strip()
+    except OSError:
+        out["governor"] = "not exposed (no cpufreq entry in this host view)"
+    return out
+
+
+def cpu_workers_count():
+    """Concurrent oracle processes for an all-cores baseline: the CPUs this
+    process may use, at most 16 (one GPU's share of its host)."""
+    return max(1, min(16, len(os.sched_getaffinity(0))))
+
+
+def run_cpu_workers(tasks, tmpdir):
+    """Run one oracle worker process per task, all at once (each a child
+    `python bench.py --cpu-worker`, one thread each), and return their JSON
+    results in task order."""
+    import subprocess
+    path = os.path.join(tmpdir, "tasks.json")
+    with open(path, "w") as f:
+        json.dump(tasks, f)
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    procs = [subprocess.Popen([sys.executable, os.path.abspath(__file__), "--cpu-worker", path,
+                               "--worker-index", str(i)], stdout=subprocess.PIPE, env=env)
+             for i in range(len(tasks))]
+    out = []
+    for p in procs:
+        so, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError("cpu worker failed with status %d" % p.returncode)
+        out.append(json.loads(so.decode().strip().splitlines()[-1]))
+    return out
+
+
+def cpu_worker(path, index):
+    """One oracle worker (no GPU): runs task `index` of the task file and
+    prints its timings as one JSON line."""
+    import ctypes
+    import numpy as np
+    import oracle
+    import parmoon2022_amd as pm
+    with open(path) as f:
+        task = json.load(f)[index]
+    if task["kind"] == "solve":
+        # assembly + solve of each problem, timed separately per problem
+        res = []
+        for db in task["dbs"]:
+            o = oracle.Oracle(pm.problem_desc(db))
+            t0 = time.time()
+            o.assemble()
+            st = o.solve(pm.solver_desc(db))
+            res.append({"seconds": time.time() - t0, "dof": o.info()["n_dof"], "iterations": st.iterations})
+        print(json.dumps({"solves": res}), flush=True)
+        return
+    # "c2": assembly of 1/sample_div of the finest cells and two FGMRES-MG
+    # iterations at full size on the matrices the GPU assembled
+    db = task["db"]
+    t0 = time.time()
+    o = oracle.Oracle(pm.problem_desc(db))
+    setup_s = time.time() - t0
+    ncell = o.info()["n_cells"]
+    nsample = max(1, ncell // task["sample_div"])
+    t0 = time.time()
+    oracle.lib().orc_assemble_cells(o._p, 0, nsample)
+    t_cells = time.time() - t0
+    P = ctypes.POINTER(ctypes.c_double)
+    for level in range(task["levels"]):
+        v = np.load(os.path.join(task["dir"], "m%d.npy" % level), mmap_mode="r")
+        b = np.load(os.path.join(task["dir"], "r%d.npy" % level), mmap_mode="r")
+        v = np.ascontiguousarray(v)
+        b = np.ascontiguousarray(b)
+        oracle.lib().orc_set_matrix(o._p, level, v.ctypes.data_as(P))
+        oracle.lib().orc_set_rhs(o._p, level, b.ctypes.data_as(P))
+        del v, b
+    sd = pm.solver_desc(db)
+    sd.max_iter = 2
+    t0 = time.time()
+    o.solve(sd)
+    per_iter = (time.time() - t0) / 2
+    print(json.dumps({"t_cells": t_cells, "nsample": nsample, "ncell": ncell, "per_iter": per_iter,
+                      "setup_s": setup_s}), flush=True)
+
+
 def cpu_baseline(db, gpu_iters, sample_div, matrices, rhs):
     """Oracle (scalar C port of the reference path) timed on a bounded sample of
     the same workload: assembly of 1/sample_div of the finest cells, and two
     FGMRES-MG iterations at full size on the assembled system, extrapolated to
     one full solve with the GPU's iteration count. To keep the sample short the
     oracle's solver runs on the matrices/rhs the GPU assembled (the assembled
-    values agree to 1e-12, checked by the parity tests)."""
-    import ctypes
+    values agree to 1e-12, checked by the parity tests). Measured once alone
+    (one core) and then in cpu_workers_count() concurrent processes (all
+    cores: the solves/s of the whole host running independent solves)."""
+    import shutil
+    import tempfile
     import numpy as np
-    import oracle
-    import parmoon2022_amd as pm
-    t0 = time.time()
-    o = oracle.Oracle(pm.problem_desc(db))
-    setup_s = time.time() - t0
-    info = o.info()
-    ncell = info["n_cells"]
-    nsample = max(1, ncell // sample_div)
-    t0 = time.time()
-    oracle.lib().orc_assemble_cells(o._p, 0, nsample)
-    t_cells = time.time() - t0
-    P = ctypes.POINTER(ctypes.c_double)
-    for level, (v, f) in enumerate(zip(matrices, rhs)):
-        v = np.ascontiguousarray(v)
-        f = np.ascontiguousarray(f)
-        oracle.lib().orc_set_matrix(o._p, level, v.ctypes.data_as(P))
-        oracle.lib().orc_set_rhs(o._p, level, f.ctypes.data_as(P))
-    sd = pm.solver_desc(db)
-    sd.max_iter = 2
-    t0 = time.time()
-    o.solve(sd)
-    t_two = time.time() - t0
-    per_iter = t_two / 2
-    # all multigrid levels are assembled: scale the finest-level sample by the
-    # cell count of the hierarchy (sum over levels of 8^-k)
     levels = int(db["multigrid_n_levels"])
-    level_factor = sum(8.0 ** -k for k in range(levels))
-    t_asm = t_cells * (ncell / nsample) * level_factor
-    t_solve = t_asm + gpu_iters * per_iter
-    return {"value": 1.0 / t_solve, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": ("oracle (C, -O3, 1 thread) on the same problem: assembly of %d of %d finest "
-                       "cells (%.2f s) scaled to all %d levels, and 2 FGMRES+MG iterations at full size "
-                       "(%.2f s/iteration) scaled to the GPU's %d iterations; setup %.1f s untimed"
-                       % (nsample, ncell, t_cells, levels, per_iter, gpu_iters, setup_s)),
-            "seconds_per_solve": t_solve}
+    tmp = tempfile.mkdtemp(prefix="pmcd_cpu_")
+    try:
+        for level, (v, f) in enumerate(zip(matrices, rhs)):
+            np.save(os.path.join(tmp, "m%d.npy" % level), np.ascontiguousarray(v))
+            np.save(os.path.join(tmp, "r%d.npy" % level), np.ascontiguousarray(f))
+        task = {"kind": "c2", "db": db, "sample_div": sample_div, "levels": len(matrices), "dir": tmp}
+        # all multigrid levels are assembled: scale the finest-level sample by
+        # the cell count of the hierarchy (sum over levels of 8^-k)
+        level_factor = sum(8.0 ** -k for k in range(levels))
+
+        def t_solve(r):
+            return r["t_cells"] * (r["ncell"] / r["nsample"]) * level_factor + gpu_iters * r["per_iter"]
+        one = run_cpu_workers([task], tmp)[0]
+        w = cpu_workers_count()
+        many = run_cpu_workers([task] * w, tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    t1 = t_solve(one)
+    all_rate = sum(1.0 / t_solve(r) for r in many)
+    return {"value": all_rate, "unit": "solves/s", "cores": w, "kind": "port",
+            "sample": ("oracle (C, -O3) in %d concurrent single-thread processes on the same problem, each: "
+                       "assembly of %d of %d finest cells scaled to all %d levels, and 2 FGMRES+MG iterations "
+                       "at full size (%.2f s/iteration under load) scaled to the GPU's %d iterations; setup "
+                       "untimed" % (w, one["nsample"], one["ncell"], levels,
+                                    float(np.mean([r["per_iter"] for r in many])), gpu_iters)),
+            "single_core": {"value": 1.0 / t1, "seconds_per_solve": t1, "per_iter": one["per_iter"],
+                            "assembly_sample_s": one["t_cells"]},
+            "machine": machine()}
 
 
 def kernel_times(cd, info, mixed):
@@ -185,28 +281,99 @@ C4_SOLVER = {"solver_type": "iterative", "iterative_solver_type": "fgmres", "pre
              "max_n_iterations": 2000}
 
 
-def c4_cpu_baseline(dbs, sdb, per_gpu_solves, sample):
+def c4_cpu_baseline(dbs, sdb, sample):
     """Oracle (scalar C port) solving `sample` instances of the batch one by
     one (FGMRES + serial SSOR in FP64), extrapolated to the whole batch by
-    DoF count."""
+    DoF count: once in one process (one core), then in cpu_workers_count()
+    concurrent processes, each on its own sample (all cores)."""
+    import shutil
+    import tempfile
     import numpy as np
-    import oracle
-    import parmoon2022_amd as pm
-    rng = np.random.default_rng(1)
-    idx = rng.choice(len(dbs), size=min(sample, len(dbs)), replace=False)
-    t_sum, dof_sum = 0.0, 0
-    for i in idx:
-        o = oracle.Oracle(pm.problem_desc(dbs[i]))
-        t0 = time.time()
-        o.assemble()
-        o.solve(pm.solver_desc(dict(dbs[i], **dict(sdb, multigrid_mixed_precision=0))))
-        t_sum += time.time() - t0
-        dof_sum += o.info()["n_dof"]
     total_dof = sum((db["structured_n"] + 1) ** 3 for db in dbs)
-    t_batch = t_sum * total_dof / dof_sum
-    return {"value": len(dbs) / t_batch, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "oracle (C, -O3, 1 thread): assembly + FGMRES(30)/SSOR of %d of the %d instances "
-                      "(%d DoF, %.1f s), scaled by DoF to the whole batch" % (len(idx), len(dbs), dof_sum, t_sum)}
+    solver = dict(sdb, multigrid_mixed_precision=0)
+    w = cpu_workers_count()
+
+    def task(seed):
+        idx = np.random.default_rng(seed).choice(len(dbs), size=min(sample, len(dbs)), replace=False)
+        return {"kind": "solve", "dbs": [dict(dbs[i], **solver) for i in idx]}
+
+    def rate(r):
+        t = sum(x["seconds"] for x in r["solves"])
+        d = sum(x["dof"] for x in r["solves"])
+        return len(dbs) / (t * total_dof / d), t, d
+    tmp = tempfile.mkdtemp(prefix="pmcd_cpu_")
+    try:
+        one = run_cpu_workers([task(1)], tmp)[0]
+        many = run_cpu_workers([task(1 + i) for i in range(w)], tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    r1, t1, d1 = rate(one)
+    return {"value": sum(rate(r)[0] for r in many), "unit": "solves/s", "cores": w, "kind": "port",
+            "sample": "oracle (C, -O3) in %d concurrent single-thread processes, each: assembly + FGMRES(30)/SSOR "
+                      "of %d of the %d instances (its own sample), scaled by DoF to the whole batch"
+                      % (w, min(sample, len(dbs)), len(dbs)),
+            "single_core": {"value": r1, "sample_seconds": t1, "sample_dof": d1},
+            "machine": machine()}
+
+
+def c5_cpu_baseline(offs, dbs, per_worker=4, max_dof=300000):
+    """Oracle serving the same request stream: each process solves
+    `per_worker` requests drawn from the stream among those of at most
+    `max_dof` DoF (assembly + the request's own solver in FP64), giving
+    seconds per DoF for each family (CD2D P2, CD3D Q1); the stream's CPU time
+    is its requests' DoF times those rates. Once in one process (one core),
+    then in cpu_workers_count() concurrent processes, each on its own draw
+    (all cores: independent requests in parallel)."""
+    import shutil
+    import tempfile
+    import numpy as np
+
+    def family(db):
+        return "3d" if db["geo_file"] == "Default_UnitCube_Hexa" else "2d"
+
+    def dof(db):
+        k = db["refinement_n_initial_steps"]
+        return (2 ** k + 1) ** 3 if family(db) == "3d" else (2 ** (k + 1) + 1) ** 2
+    small = {}
+    for f in ("2d", "3d"):
+        fam = [db for db in dbs if family(db) == f]
+        # a family with no request that small contributes its smallest one
+        small[f] = [db for db in fam if dof(db) <= max_dof] or sorted(fam, key=dof)[:1]
+
+    def task(seed):
+        rng = np.random.default_rng(seed)
+        pick = []
+        for f in ("2d", "3d"):
+            if small[f]:
+                n = max(1, per_worker // 2)
+                pick += [small[f][i] for i in rng.choice(len(small[f]), size=min(n, len(small[f])), replace=False)]
+        return {"kind": "solve", "dbs": pick}
+
+    def rate(t, r):
+        spd = {}
+        for f in ("2d", "3d"):
+            rows = [x for db, x in zip(t["dbs"], r["solves"]) if family(db) == f]
+            if rows:
+                spd[f] = sum(x["seconds"] for x in rows) / sum(x["dof"] for x in rows)
+        total = sum(dof(db) * spd[family(db)] for db in dbs)
+        return len(dbs) / total, spd
+    w = cpu_workers_count()
+    tasks = [task(100 + i) for i in range(w)]
+    tmp = tempfile.mkdtemp(prefix="pmcd_cpu_")
+    try:
+        one = run_cpu_workers(tasks[:1], tmp)[0]
+        many = run_cpu_workers(tasks, tmp)
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    r1, spd1 = rate(tasks[0], one)
+    return {"value": sum(rate(t, r)[0] for t, r in zip(tasks, many)), "unit": "solves/s", "cores": w,
+            "kind": "port",
+            "sample": "oracle (C, -O3) in %d concurrent single-thread processes, each solving %d requests of the "
+                      "stream with at most %d DoF (assembly + FGMRES(30)/MG SOR, FP64); seconds per DoF per "
+                      "family (CD2D P2, CD3D Q1) applied to all %d requests of the stream"
+                      % (w, per_worker, max_dof, len(dbs)),
+            "single_core": {"value": r1, "seconds_per_dof": spd1},
+            "machine": machine()}
 
 
 C1_DB = {"geo_file": "TwoTriangles", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 8, "example": "sine_2d",
@@ -263,7 +430,8 @@ def bench_c1(args):
         result["cpu_baseline"] = {"value": 1.0 / t_cpu, "unit": "solves/s", "cores": 1, "kind": "port",
                                   "sample": "oracle (C, -O3, 1 thread): the whole C1 problem, assembly + CG "
                                             "(%d iterations, %.2f s; setup %.1f s untimed), L2 error %.6e"
-                                            % (so.iterations, t_cpu, setup, eo[0])}
+                                            % (so.iterations, t_cpu, setup, eo[0]),
+                                  "machine": machine()}
         result["gpu_over_cpu"] = value * t_cpu
     print(json.dumps(result), flush=True)
 
@@ -310,7 +478,7 @@ def bench_c4(args):
     bt.close()
     if not args.no_cpu:
         try:
-            result["cpu_baseline"] = c4_cpu_baseline(dbs, sdb, value, 8)
+            result["cpu_baseline"] = c4_cpu_baseline(dbs, sdb, 8)
             result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
         except Exception as ex:
             log("cpu baseline failed:", ex)
@@ -354,10 +522,7 @@ def bench_c5(args):
         errs = [r for r in reqs if r.error is not None]
         lat = np.array([r.latency for r in reqs])
         span = max(r.finished for r in reqs) - start
-        dof = 0
-        for r in reqs:
-            k = r.db["refinement_n_initial_steps"]
-            dof += (2 ** k + 1) ** 3 if r.db["geo_file"] == "Default_UnitCube_Hexa" else (2 ** (k + 1) + 1) ** 2
+        dof = sum(r.n_dof for r in reqs if r.n_dof)
         row = {"rate": rate, "solves_per_s": len(reqs) / span, "dof_per_s": dof / span,
                "latency_ms": {"p50": 1e3 * float(np.percentile(lat, 50)), "p99": 1e3 * float(np.percentile(lat, 99))},
                "converged": int(sum(1 for r in reqs if r.stats is not None and r.stats.converged)),
@@ -378,6 +543,14 @@ def bench_c5(args):
                                "%d slots" % args.slots, "requests_per_rate": args.requests},
         "latency_ms": top["latency_ms"], "sweep": sweep,
     }
+    if not args.no_cpu:
+        try:
+            offs, dbs = sv.c5_requests(args.requests, top["rate"])
+            result["cpu_baseline"] = c5_cpu_baseline(offs, dbs)
+            result["gpu_over_cpu"] = result["value"] / result["cpu_baseline"]["value"]
+        except Exception as ex:
+            log("cpu baseline failed:", ex)
+            result["cpu_baseline"] = None
     print(json.dumps(result), flush=True)
 
 
@@ -441,7 +614,8 @@ def tcd_cpu_baseline(db, gpu_iters, sample_div, matrices, rhs):
                        "assembly of %d of %d finest cells (%.2f s) scaled to all %d levels, 3 finest-level "
                        "matrix passes (%.2f s), 2 FGMRES+MG iterations (%.2f s/iteration) scaled to the GPU's "
                        "%d; setup %.1f s untimed" % (nsample, ncell, t_cells, levels, t_pass, per_iter,
-                                                     gpu_iters, setup_s))}
+                                                     gpu_iters, setup_s)),
+            "machine": machine()}
 
 
 def bench_tcd(args):
@@ -569,7 +743,8 @@ def bench_afc(args):
                                   "kind": "port",
                                   "sample": "oracle (C, -O3, 1 thread): assembly and the first 3 nonlinear "
                                             "iterations of the same loop (%.2f s), scaled to the GPU's %d "
-                                            "iterations" % (t3, n_it)}
+                                            "iterations" % (t3, n_it),
+                                  "machine": machine()}
         result["gpu_over_cpu"] = value * per_it * n_it
     print(json.dumps(result), flush=True)
 
@@ -613,7 +788,11 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="assemble once, time the dominant kernels and the calibration copy, "
                          "print them as JSON and stop (short runs for hardware counters)")
+    ap.add_argument("--cpu-worker", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--worker-index", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_worker:
+        return cpu_worker(args.cpu_worker, args.worker_index)
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(instances shard across GPUs without communication); run with --gpus 1",
@@ -723,7 +902,7 @@ def main():
     if not args.no_cpu:
         try:
             cb = cpu_baseline(db, int(np.median(iters)), args.cpu_sample, mats, rhss)
-            result["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+            result["cpu_baseline"] = cb
             result["gpu_over_cpu"] = value / cb["value"]
         except Exception as ex:  # the baseline is reported, not required
             log("cpu baseline failed:", ex)

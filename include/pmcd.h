@@ -493,6 +493,83 @@ int pmcd_restrict_device(int n_coarse, int n_active_coarse, const int* r_row_ptr
 int pmcd_batch_instances(unsigned long long seed, int n_inst, int n_lo, int n_hi, int* n_out, double* eps_out,
                          double* b_out);
 
+/* ---- Device memory of a system ----
+ * Bytes the system's device buffers hold now and at most so far (each
+ * allocation counted in whole 4 KiB pages). */
+int pmcd_system_device_bytes(pmcd_system* s, long long* bytes, long long* peak);
+/* Upper estimate of those bytes after a solve with `sd`, from the coarse
+ * mesh's vertex, edge, face and cell counts and the regular-refinement
+ * recurrences (no mesh is refined and nothing is allocated). */
+int pmcd_estimate_device_bytes(const pmcd_problem_desc* pd, const pmcd_solver_desc* sd, long long* bytes);
+/* The finest level's counts behind the estimate: out3 = cells, DoFs (both
+ * exact), matrix-entry bound. */
+int pmcd_estimate_counts(const pmcd_problem_desc* pd, long long* out3);
+
+/* ---- Launch-shape autotuner ----
+ * Times the SpMV and Jacobi kernels of `level` (-1: finest) in every CSR
+ * variant (0 plain, 1 streaming, 2 unrolled, 3 unrolled non-temporal) at 8,
+ * 16 and 32 lanes per row and as sliced ELL, with HIP events over `iters`
+ * launches each, and sets the fastest (pmcd_set_tuning keys 0-3); when
+ * sliced ELL wins, coarser levels on which it loses keep the CSR kernels
+ * (key 11). `mixed`: the Jacobi sweep is timed in FP32 (the system must have
+ * solved with the FP32 V-cycle once). ms_out[26]: SpMV candidates 0..12, then
+ * Jacobi candidates 0..12, candidate 3 * variant + (0, 1, 2 for 8, 16, 32
+ * lanes), candidate 12 sliced ELL. chosen_out[5]: SpMV variant, SpMV lanes,
+ * Jacobi variant, Jacobi lanes, sliced-ELL minimum rows. No reference
+ * counterpart (the reference's kernels have no launch shape). */
+int pmcd_autotune(pmcd_system* s, int level, int iters, int mixed, double* ms_out, int* chosen_out);
+
+/* ---- Request server ----
+ * A request loop over device-resident systems for callers that run many
+ * solves of mixed sizes. The reference runs one problem per process:
+ * ConvectionDiffusion<d>(domain, db), assemble(), solve(), output()
+ * (3DPrograms/CD3D_ParMooN.C:40-53, 2DPrograms/CD2D_ParMooN.C:19-22,
+ * src/System/ConvectionDiffusion.C:112-452); a served request runs the same
+ * assemble and solve on a system that stays resident for the next request on
+ * the same mesh (same geometry, refinement, order, discretization,
+ * quadrature mode, multigrid levels and mesh file), which only receives the
+ * new PDE data (pmcd_system_set_data).
+ *
+ * Requests wait in FIFO order. A worker admits the head request when fewer
+ * than max_slots run and either an idle system for its mesh exists or the
+ * estimated bytes (pmcd_estimate_device_bytes) fit the budget once idle
+ * systems, oldest first, are destroyed; a request larger than the budget
+ * runs when nothing else runs. Each running request works on its own HIP
+ * stream, so requests overlap on the device. Times are seconds on the
+ * server's steady clock (pmcd_server_now). */
+typedef struct pmcd_server pmcd_server;
+typedef struct {
+  long long id;
+  int status;            /* 0 solved, 1 failed (message in error) */
+  int reused;            /* 1: the request ran on a resident system */
+  double arrival, admitted, finished;
+  double assemble_ms;
+  long long n_dof;
+  long long reserved_bytes;  /* device bytes reserved at admission */
+  pmcd_solve_stats stats;
+  char error[256];
+} pmcd_served_result;
+/* budget_bytes <= 0: 90% of the free device memory; keep_solution: keep each
+ * finest-level solution on the host for pmcd_server_solution */
+int pmcd_server_create(int max_slots, long long budget_bytes, int keep_solution, pmcd_server** out);
+/* runs every queued request to the end, then frees the resident systems */
+int pmcd_server_destroy(pmcd_server* s);
+int pmcd_server_now(pmcd_server* s, double* t);
+/* arrival < 0: now */
+int pmcd_server_submit(pmcd_server* s, const pmcd_problem_desc* pd, const pmcd_solver_desc* sd, double arrival,
+                       long long* id);
+/* creates and parks up to `count` idle systems for pd's mesh */
+int pmcd_server_warm(pmcd_server* s, const pmcd_problem_desc* pd, const pmcd_solver_desc* sd, int count);
+/* done = 1 once request `id` finished; waits at most timeout_s (< 0: no limit) */
+int pmcd_server_wait(pmcd_server* s, long long id, double timeout_s, int* done);
+int pmcd_server_result(pmcd_server* s, long long id, pmcd_served_result* out);
+int pmcd_server_solution(pmcd_server* s, long long id, double* out, long long n);
+/* forgets a finished request's record and solution */
+int pmcd_server_release(pmcd_server* s, long long id);
+/* out8: queued, running, reserved bytes, idle bytes, idle systems, budget,
+ * finished requests, systems created */
+int pmcd_server_info(pmcd_server* s, long long* out8);
+
 #ifdef __cplusplus
 }
 #endif

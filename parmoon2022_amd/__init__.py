@@ -81,6 +81,9 @@ EXPORTS = [
     "pmcd_batch_create", "pmcd_batch_destroy", "pmcd_batch_info", "pmcd_batch_assemble",
     "pmcd_batch_solve", "pmcd_batch_get_solution", "pmcd_batch_set_solution", "pmcd_batch_get_matrix",
     "pmcd_batch_get_rhs",
+    "pmcd_system_device_bytes", "pmcd_estimate_device_bytes", "pmcd_estimate_counts", "pmcd_autotune",
+    "pmcd_server_create", "pmcd_server_destroy", "pmcd_server_now", "pmcd_server_submit", "pmcd_server_warm",
+    "pmcd_server_wait", "pmcd_server_result", "pmcd_server_solution", "pmcd_server_release", "pmcd_server_info",
 ]
 
 
@@ -416,6 +419,13 @@ class ConvectionDiffusion:
         ms = ctypes.c_double()
         _check(lib().pmcd_assemble(self._s, ctypes.byref(ms)))
         self.assemble_ms = ms.value
+
+    def device_bytes(self):
+        """(bytes, peak bytes) of the system's device buffers, each counted in
+        whole 4 KiB pages."""
+        b, p = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().pmcd_system_device_bytes(self._s, ctypes.byref(b), ctypes.byref(p)))
+        return b.value, p.value
 
     def set_data(self, db):
         """New PDE data (example, coefficients) on the same mesh hierarchy;
@@ -917,53 +927,29 @@ def c4_instances(n_inst=256, seed=20261202, n_range=(8, 40)):
 
 
 def autotune(cd, level=-1, iters=10, log=None, mixed=False):
-    """Pick the launch shape of the SpMV and Jacobi kernels for this matrix by
-    timing each candidate with HIP events: CSR variants x lanes per row, and
-    the sliced ELL kernels. Sets the fastest; when the sliced ELL kernels win
-    on `level`, the coarser levels on which they lose keep the CSR kernels
-    (set_sell_min_rows). With `mixed` the Jacobi sweep is timed in FP32, the
-    precision the V-cycle runs in (the system must have solved once, which
-    makes its FP32 copies). Returns the timings {kernel: {shape: ms}}."""
-    names = ("plain", "stream", "unrolled", "unrollednt", "sell")
-    set_sell_min_rows(0)
-    out, chosen, csr_best = {}, {}, {}
-    kernels = (("spmv", "spmv_f64"), ("jacobi", "jacobi_f32" if mixed else "jacobi_f64"))
-    for kernel, timed in kernels:
-        res = {}
-        for variant in (0, 1, 2, 3):
-            for w in (8, 16, 32):
-                set_tuning(kernel, w, variant)
-                res[(names[variant], w)] = cd.time_kernel(timed, level, iters)
-        set_tuning(kernel, 16, 4)
-        res[("sell", 16)] = cd.time_kernel(timed, level, iters)
-        best = min(res, key=res.get)
-        csr_best[kernel] = min((k for k in res if k[0] != "sell"), key=res.get)
-        # under variant 4 the lane count is that of the CSR fallback on small levels
-        set_tuning(kernel, csr_best[kernel][1] if best[0] == "sell" else best[1], names.index(best[0]))
-        chosen[kernel] = best
-        out[kernel] = {"%s%d" % k: v for k, v in res.items()}
+    """Pick the launch shape of the SpMV and Jacobi kernels for this matrix
+    (pmcd_autotune): every CSR variant at 8, 16 and 32 lanes per row and the
+    sliced ELL kernels are timed with HIP events and the fastest is set; when
+    the sliced ELL kernels win on `level`, the coarser levels on which they
+    lose keep the CSR kernels (set_sell_min_rows). With `mixed` the Jacobi
+    sweep is timed in FP32, the precision the V-cycle runs in (the system must
+    have solved once, which makes its FP32 copies). Returns the timings
+    {kernel: {shape: ms}}."""
+    names = ("plain", "stream", "unrolled", "unrollednt")
+    ms = np.zeros(26)
+    chosen = (ctypes.c_int * 5)()
+    _check(lib().pmcd_autotune(cd._s, int(level), int(iters), int(bool(mixed)), _ptr(ms, ctypes.c_double), chosen))
+    out = {}
+    for k, kernel in enumerate(("spmv", "jacobi")):
+        res = {"%s%d" % (names[c // 3], (8, 16, 32)[c % 3]): float(ms[13 * k + c]) for c in range(12)}
+        res["sell16"] = float(ms[13 * k + 12])
+        out[kernel] = res
         if log:
-            log("autotune %s: %s -> %s%d" % (kernel, {k: round(v, 4) for k, v in out[kernel].items()}, *best))
-    # below `level`, keep the sliced ELL kernels while they stay faster
-    info = cd.info()
-    finest = info["n_levels"] - 1 if level < 0 else level
-    threshold = 0
-    for kernel, timed in kernels:
-        if chosen[kernel][0] != "sell":
-            continue
-        lanes = csr_best[kernel][1]
-        for lev in range(finest - 1, -1, -1):
-            set_tuning(kernel, lanes, 4)
-            t_sell = cd.time_kernel(timed, lev, iters)
-            set_tuning(kernel, lanes, names.index(csr_best[kernel][0]))
-            t_csr = cd.time_kernel(timed, lev, iters)
-            set_tuning(kernel, lanes, 4)
-            if t_sell > t_csr:
-                threshold = max(threshold, cd.info(lev + 1)["n_active"])
-                break
-    set_sell_min_rows(threshold)
-    if log and threshold:
-        log("autotune: sliced ELL on levels with at least %d active rows" % threshold)
+            v, w = chosen[2 * k], chosen[2 * k + 1]
+            log("autotune %s: %s -> %s%d" % (kernel, {a: round(b, 4) for a, b in res.items()},
+                                             "sell" if v == 4 else names[v], w))
+    if log and chosen[4]:
+        log("autotune: sliced ELL on levels with at least %d active rows" % chosen[4])
     return out
 
 

@@ -4,6 +4,7 @@
 #include "batch.hpp"
 #include "io.hpp"
 #include "dropin.hpp"
+#include "server.hpp"
 #include <cmath>
 #include <random>
 #include <cstring>
@@ -12,25 +13,35 @@
 
 using namespace pmcd;
 
+// Each handle owns a device-byte tally, declared before the object it
+// charges so that it outlives that object's buffers.
 struct pmcd_system {
   hipStream_t stream = nullptr;
+  DevTally tally;
   std::unique_ptr<CDSystem> sys;
 };
 
 struct pmcd_batch {
   hipStream_t stream = nullptr;
+  DevTally tally;
   std::unique_ptr<BatchSystem> sys;
 };
 
 struct pmcd_assembler {
   hipStream_t stream = nullptr;
+  DevTally tally;
   std::unique_ptr<CallerAssembler> a;
 };
 
 struct pmcd_solver {
   hipStream_t stream = nullptr;
   SolverDesc sd{};
+  DevTally tally;
   std::unique_ptr<CDSystem> sys;
+};
+
+struct pmcd_server {
+  std::unique_ptr<Server> srv;
 };
 
 namespace {
@@ -47,6 +58,13 @@ int guarded(F&& f) {
     g_err = "unknown error";
   }
   return 1;
+}
+
+// guarded() with the device allocations of the call charged to handle h
+template <class H, class F>
+int guarded_on(H* h, F&& f) {
+  TallyScope scope(h ? &h->tally : nullptr);
+  return guarded(std::forward<F>(f));
 }
 
 Level& level_of(pmcd_system* s, int level) {
@@ -227,7 +245,7 @@ int pmcd_mem_info(unsigned long long* free_bytes, unsigned long long* total_byte
 }
 
 int pmcd_system_set_data(pmcd_system* s, const pmcd_problem_desc* desc) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(s && s->sys && desc, "null argument");
     const ProblemDesc& cur = s->sys->desc();
     const ProblemDesc d = to_desc(*desc);
@@ -247,6 +265,7 @@ int pmcd_system_create(const pmcd_problem_desc* desc, pmcd_system** out) {
       PMCD_FAIL("no GPU visible: the HIP path needs an MI355X device");
     auto s = std::make_unique<pmcd_system>();
     PMCD_HIP(hipStreamCreate(&s->stream));
+    TallyScope scope(&s->tally);
     try {
       s->sys = std::make_unique<CDSystem>(to_desc(*desc), s->stream);
     } catch(...) {
@@ -267,7 +286,7 @@ int pmcd_system_destroy(pmcd_system* s) {
 }
 
 int pmcd_system_info(pmcd_system* s, int level, long long* info) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     Level& L = level_of(s, level);
     info[0] = L.mesh.n_cells;
     info[1] = L.n();
@@ -281,14 +300,14 @@ int pmcd_system_info(pmcd_system* s, int level, long long* info) {
 }
 
 int pmcd_get_dofmap(pmcd_system* s, int level, int* out) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     Level& L = level_of(s, level);
     std::memcpy(out, L.space.dof.data(), L.space.dof.size() * sizeof(int));
   });
 }
 
 int pmcd_get_pattern(pmcd_system* s, int level, int* row_ptr, int* col) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     Level& L = level_of(s, level);
     std::memcpy(row_ptr, L.st.row_ptr.data(), L.st.row_ptr.size() * sizeof(int));
     std::memcpy(col, L.st.col.data(), L.st.col.size() * sizeof(int));
@@ -296,7 +315,7 @@ int pmcd_get_pattern(pmcd_system* s, int level, int* row_ptr, int* col) {
 }
 
 int pmcd_get_cell_coords(pmcd_system* s, int level, double* out) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     Level& L = level_of(s, level);
     const Mesh& m = L.mesh;
     for(int c = 0; c < m.n_cells; ++c)
@@ -306,7 +325,7 @@ int pmcd_get_cell_coords(pmcd_system* s, int level, double* out) {
 }
 
 int pmcd_assemble(pmcd_system* s, double* ms) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(s && s->sys, "null system");
     s->sys->assemble();
     if(ms) *ms = s->sys->last_assemble_ms;
@@ -314,7 +333,7 @@ int pmcd_assemble(pmcd_system* s, double* ms) {
 }
 
 int pmcd_get_matrix(pmcd_system* s, int level, double* val) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     Level& L = level_of(s, level);
     L.val.download(val, L.val.n, s->stream);
     PMCD_HIP(hipStreamSynchronize(s->stream));
@@ -322,7 +341,7 @@ int pmcd_get_matrix(pmcd_system* s, int level, double* val) {
 }
 
 int pmcd_get_rhs(pmcd_system* s, int level, double* rhs) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     Level& L = level_of(s, level);
     L.rhs.download(rhs, L.n(), s->stream);
     PMCD_HIP(hipStreamSynchronize(s->stream));
@@ -330,7 +349,7 @@ int pmcd_get_rhs(pmcd_system* s, int level, double* rhs) {
 }
 
 int pmcd_solve(pmcd_system* s, const pmcd_solver_desc* sd, pmcd_solve_stats* stats) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(s && s->sys && sd, "null argument");
     SolveStats st = s->sys->solve(to_solver(*sd));
     if(stats) {
@@ -344,7 +363,7 @@ int pmcd_solve(pmcd_system* s, const pmcd_solver_desc* sd, pmcd_solve_stats* sta
 }
 
 int pmcd_get_solution(pmcd_system* s, double* out) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     Level& L = level_of(s, -1);
     L.sol.download(out, L.n(), s->stream);
     PMCD_HIP(hipStreamSynchronize(s->stream));
@@ -352,7 +371,7 @@ int pmcd_get_solution(pmcd_system* s, double* out) {
 }
 
 int pmcd_set_solution(pmcd_system* s, const double* in) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     Level& L = level_of(s, -1);
     PMCD_HIP(hipMemcpyAsync(L.sol.p, in, sizeof(double) * L.n(), hipMemcpyHostToDevice, s->stream));
     PMCD_HIP(hipStreamSynchronize(s->stream));
@@ -360,14 +379,14 @@ int pmcd_set_solution(pmcd_system* s, const double* in) {
 }
 
 int pmcd_errors(pmcd_system* s, double* out5) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(s && s->sys, "null system");
     s->sys->errors(out5);
   });
 }
 
 int pmcd_apply(pmcd_system* s, int level, const double* x, double* y, double a, int mode) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     Level& L = level_of(s, level);
     DevBuf<double> dx, dy;
     dx.upload(x, L.n(), s->stream);
@@ -381,7 +400,7 @@ int pmcd_apply(pmcd_system* s, int level, const double* x, double* y, double a, 
 }
 
 int pmcd_jacobi_sweep(pmcd_system* s, int level, const double* b, double* x, double omega) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(b && x, "null argument");
     Level& L = level_of(s, level);
     DevBuf<double> db, dx, dy(L.n());
@@ -395,7 +414,7 @@ int pmcd_jacobi_sweep(pmcd_system* s, int level, const double* b, double* x, dou
 
 int pmcd_sor_sweep(pmcd_system* s, int level, const double* b, double* x, double omega, int flag,
                    int* exact) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(b && x, "null argument");
     Level& L = level_of(s, level);
     DevBuf<double> db, dx;
@@ -409,7 +428,7 @@ int pmcd_sor_sweep(pmcd_system* s, int level, const double* b, double* x, double
 }
 
 int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, double* z) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(s && s->sys && sd, "null argument");
     Level& F = s->sys->finest();
     s->sys->prepare(to_solver(*sd));
@@ -422,7 +441,7 @@ int pmcd_vcycle(pmcd_system* s, const pmcd_solver_desc* sd, const double* v, dou
 }
 
 int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* avg_ms) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(iters > 0, "iters must be positive");
     s->sys->time_kernel(which, level, iters, avg_ms);
   });
@@ -502,6 +521,7 @@ int pmcd_batch_create(const pmcd_problem_desc* descs, int n_inst, pmcd_batch** o
     for(int i = 0; i < n_inst; ++i) d[i] = to_desc(descs[i]);
     auto b = std::make_unique<pmcd_batch>();
     PMCD_HIP(hipStreamCreate(&b->stream));
+    TallyScope scope(&b->tally);
     try {
       b->sys = std::make_unique<BatchSystem>(d, b->stream);
     } catch(...) {
@@ -522,7 +542,7 @@ int pmcd_batch_destroy(pmcd_batch* b) {
 }
 
 int pmcd_batch_info(pmcd_batch* b, int inst, long long* info) {
-  return guarded([&] {
+  return guarded_on(b, [&] {
     PMCD_CHECK(b && b->sys && info, "null argument");
     BatchSystem& s = *b->sys;
     if(inst < 0) {
@@ -537,7 +557,7 @@ int pmcd_batch_info(pmcd_batch* b, int inst, long long* info) {
 }
 
 int pmcd_batch_assemble(pmcd_batch* b, double* ms) {
-  return guarded([&] {
+  return guarded_on(b, [&] {
     PMCD_CHECK(b && b->sys, "null argument");
     b->sys->assemble();
     if(ms) *ms = b->sys->last_assemble_ms;
@@ -545,7 +565,7 @@ int pmcd_batch_assemble(pmcd_batch* b, double* ms) {
 }
 
 int pmcd_batch_solve(pmcd_batch* b, const pmcd_solver_desc* sd, pmcd_solve_stats* stats) {
-  return guarded([&] {
+  return guarded_on(b, [&] {
     PMCD_CHECK(b && b->sys && sd, "null argument");
     auto st = b->sys->solve(to_solver(*sd));
     if(stats)
@@ -560,35 +580,35 @@ int pmcd_batch_solve(pmcd_batch* b, const pmcd_solver_desc* sd, pmcd_solve_stats
 }
 
 int pmcd_batch_get_solution(pmcd_batch* b, int inst, double* out) {
-  return guarded([&] {
+  return guarded_on(b, [&] {
     PMCD_CHECK(b && b->sys && out, "null argument");
     b->sys->get_solution(inst, out);
   });
 }
 
 int pmcd_batch_set_solution(pmcd_batch* b, int inst, const double* in) {
-  return guarded([&] {
+  return guarded_on(b, [&] {
     PMCD_CHECK(b && b->sys && in, "null argument");
     b->sys->set_solution(inst, in);
   });
 }
 
 int pmcd_batch_get_matrix(pmcd_batch* b, int inst, double* val) {
-  return guarded([&] {
+  return guarded_on(b, [&] {
     PMCD_CHECK(b && b->sys && val, "null argument");
     b->sys->get_matrix(inst, val);
   });
 }
 
 int pmcd_batch_get_rhs(pmcd_batch* b, int inst, double* rhs) {
-  return guarded([&] {
+  return guarded_on(b, [&] {
     PMCD_CHECK(b && b->sys && rhs, "null argument");
     b->sys->get_rhs(inst, rhs);
   });
 }
 
 int pmcd_tcd_init(pmcd_system* s, const pmcd_tcd_desc* t) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(s && t, "null argument");
     TcdDesc d{t->scheme, t->bdf_first, t->tau, t->t0};
     s->sys->tcd_init(d);
@@ -596,7 +616,7 @@ int pmcd_tcd_init(pmcd_system* s, const pmcd_tcd_desc* t) {
 }
 
 int pmcd_tcd_step(pmcd_system* s, const pmcd_solver_desc* sd, pmcd_solve_stats* stats, double* time) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(s && sd, "null argument");
     const SolveStats st = s->sys->tcd_step(to_solver(*sd));
     if(stats) {
@@ -612,7 +632,7 @@ int pmcd_tcd_step(pmcd_system* s, const pmcd_solver_desc* sd, pmcd_solve_stats* 
 
 int pmcd_afc_solve(pmcd_system* s, const pmcd_solver_desc* sd, const double* prm, double* out,
                    pmcd_solve_stats* stats) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(s && sd && prm && out, "null argument");
     PMCD_CHECK(prm[0] > 0 && prm[1] >= 1, "afc_nonlinloop_epsilon must be positive, afc_nonlinloop_maxit >= 1");
     PMCD_CHECK(prm[11] == 0.0 || prm[11] == 1.0, "afc limiter must be 0 (kuzmin) or 1 (BJK17)");
@@ -684,6 +704,7 @@ int pmcd_assembler_create(const pmcd_fe_space* space, const pmcd_structure* st, 
     check_visible_gpu();
     auto h = std::make_unique<pmcd_assembler>();
     PMCD_HIP(hipStreamCreate(&h->stream));
+    TallyScope scope(&h->tally);
     try {
       h->a = std::make_unique<CallerAssembler>((Shape)space->shape, space->order, space->n_cells, space->n_dof,
                                                space->n_active, space->vertices, space->global_numbers,
@@ -707,7 +728,7 @@ int pmcd_assembler_destroy(pmcd_assembler* h) {
 }
 
 int pmcd_assembler_assemble(pmcd_assembler* h, const double* dirichlet_values, double* entries, double* rhs) {
-  return guarded([&] {
+  return guarded_on(h, [&] {
     PMCD_CHECK(h && h->a, "null assembler");
     h->a->assemble(dirichlet_values, false);
     h->a->add_into_host(entries, rhs);
@@ -716,7 +737,7 @@ int pmcd_assembler_assemble(pmcd_assembler* h, const double* dirichlet_values, d
 
 int pmcd_assembler_assemble_device(pmcd_assembler* h, const double* dirichlet_values, double* entries,
                                    double* rhs, void* stream) {
-  return guarded([&] {
+  return guarded_on(h, [&] {
     PMCD_CHECK(h && h->a, "null assembler");
     // the caller's arrays (and device Dirichlet values) are complete on its stream
     PMCD_HIP(hipStreamSynchronize((hipStream_t)stream));
@@ -758,7 +779,7 @@ int pmcd_solver_destroy(pmcd_solver* s) {
 }
 
 int pmcd_solver_update_matrix(pmcd_solver* s, int n_levels, const pmcd_csr_level* levels) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(s && levels && n_levels >= 1, "null argument or no level");
     const bool mg = s->sd.precond == 2;
     PMCD_CHECK(mg || n_levels == 1, "only a multigrid preconditioner takes several levels");
@@ -805,7 +826,7 @@ int pmcd_solver_update_matrix(pmcd_solver* s, int n_levels, const pmcd_csr_level
 }
 
 int pmcd_solver_solve(pmcd_solver* s, const double* b, double* x, pmcd_solve_stats* st) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     PMCD_CHECK(s && b && x, "null argument");
     PMCD_CHECK(s->sys, "pmcd_solver_update_matrix has not been called");
     CDSystem& sys = *s->sys;
@@ -834,7 +855,7 @@ int pmcd_solver_is_using_multigrid(const pmcd_solver* s, int* out) {
 
 int pmcd_get_transfer(pmcd_system* s, int level, int* n_fine, int* n_coarse, int* nnz, int* row_ptr, int* col,
                       double* val) {
-  return guarded([&] {
+  return guarded_on(s, [&] {
     Level& L = level_of(s, level);
     PMCD_CHECK(L.n_coarse > 0, "level ", level, " has no coarser level");
     const int nf = L.n();
@@ -911,6 +932,130 @@ int pmcd_batch_instances(unsigned long long seed, int n_inst, int n_lo, int n_hi
       const double len = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
       for(int k = 0; k < 3; ++k) b_out[3 * i + k] = g[k] / len;
     }
+  });
+}
+
+int pmcd_system_device_bytes(pmcd_system* s, long long* bytes, long long* peak) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->sys, "null system");
+    if(bytes) *bytes = s->tally.bytes.load();
+    if(peak) *peak = s->tally.peak.load();
+  });
+}
+
+int pmcd_estimate_device_bytes(const pmcd_problem_desc* pd, const pmcd_solver_desc* sd, long long* bytes) {
+  return guarded([&] {
+    PMCD_CHECK(pd && sd && bytes, "null argument");
+    *bytes = estimate_device_bytes(to_desc(*pd), to_solver(*sd));
+  });
+}
+
+int pmcd_estimate_counts(const pmcd_problem_desc* pd, long long* out3) {
+  return guarded([&] {
+    PMCD_CHECK(pd && out3, "null argument");
+    const LevelEstimate e = estimate_levels(to_desc(*pd)).back();
+    out3[0] = (long long)e.cells;
+    out3[1] = (long long)e.dofs;
+    out3[2] = (long long)e.entries;
+  });
+}
+
+int pmcd_autotune(pmcd_system* s, int level, int iters, int mixed, double* ms_out, int* chosen_out) {
+  return guarded_on(s, [&] {
+    PMCD_CHECK(s && s->sys && ms_out && chosen_out, "null argument");
+    autotune(*s->sys, level, iters, mixed != 0, ms_out, chosen_out);
+  });
+}
+
+int pmcd_server_create(int max_slots, long long budget_bytes, int keep_solution, pmcd_server** out) {
+  return guarded([&] {
+    PMCD_CHECK(out, "null argument");
+    check_visible_gpu();
+    auto h = std::make_unique<pmcd_server>();
+    h->srv = std::make_unique<Server>(max_slots, budget_bytes, keep_solution != 0);
+    *out = h.release();
+  });
+}
+
+int pmcd_server_destroy(pmcd_server* s) {
+  return guarded([&] {
+    if(!s) return;
+    s->srv.reset();
+    delete s;
+  });
+}
+
+int pmcd_server_now(pmcd_server* s, double* t) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->srv && t, "null argument");
+    *t = s->srv->now();
+  });
+}
+
+int pmcd_server_submit(pmcd_server* s, const pmcd_problem_desc* pd, const pmcd_solver_desc* sd, double arrival,
+                       long long* id) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->srv && pd && sd && id, "null argument");
+    *id = s->srv->submit(to_desc(*pd), to_solver(*sd), arrival);
+  });
+}
+
+int pmcd_server_warm(pmcd_server* s, const pmcd_problem_desc* pd, const pmcd_solver_desc* sd, int count) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->srv && pd && sd, "null argument");
+    s->srv->warm(to_desc(*pd), to_solver(*sd), count);
+  });
+}
+
+int pmcd_server_wait(pmcd_server* s, long long id, double timeout_s, int* done) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->srv, "null argument");
+    const bool d = s->srv->wait(id, timeout_s);
+    if(done) *done = d ? 1 : 0;
+  });
+}
+
+int pmcd_server_result(pmcd_server* s, long long id, pmcd_served_result* out) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->srv && out, "null argument");
+    const ServedRequest r = s->srv->result(id);
+    std::memset(out, 0, sizeof(*out));
+    out->id = r.id;
+    out->status = r.status;
+    out->reused = r.reused;
+    out->arrival = r.arrival;
+    out->admitted = r.admitted;
+    out->finished = r.finished;
+    out->assemble_ms = r.assemble_ms;
+    out->n_dof = r.n_dof;
+    out->reserved_bytes = r.reserved;
+    out->stats.iterations = r.stats.iterations;
+    out->stats.converged = r.stats.converged;
+    out->stats.initial_residual = r.stats.initial_residual;
+    out->stats.final_residual = r.stats.final_residual;
+    out->stats.seconds = r.stats.seconds;
+    std::strncpy(out->error, r.error.c_str(), sizeof(out->error) - 1);
+  });
+}
+
+int pmcd_server_solution(pmcd_server* s, long long id, double* out, long long n) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->srv && out, "null argument");
+    s->srv->solution(id, out, n);
+  });
+}
+
+int pmcd_server_release(pmcd_server* s, long long id) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->srv, "null argument");
+    s->srv->release(id);
+  });
+}
+
+int pmcd_server_info(pmcd_server* s, long long* out8) {
+  return guarded([&] {
+    PMCD_CHECK(s && s->srv && out8, "null argument");
+    s->srv->info(out8);
   });
 }
 

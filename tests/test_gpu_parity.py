@@ -6,6 +6,7 @@ relative to the row's largest entry), discrete solution within 1e-10 relative,
 errors to 3 significant digits; SpMV within a rounding bound of its serial sum.
 """
 import math
+import re
 
 import numpy as np
 import pytest
@@ -603,6 +604,114 @@ def test_served_results_same_alone_and_in_the_mix(slots, budget):
     for r, (u, its) in zip(reqs, alone):
         assert r.error is None, r.error
         assert r.stats.iterations == its and np.array_equal(r.solution, u), r.id
+
+
+_MG_SOR = {"solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "multigrid",
+           "multigrid_smoother": "sor", "sor_omega": 1.0, "gmres_restart": 30, "residual_reduction": 1e-10,
+           "residual_tolerance": 0.0, "max_n_iterations": 200}
+BYTES_CASES = [
+    ("served-2d-p2", dict(_MG_SOR, geo_file="TwoTriangles", ANSATZ_ORDER=2, refinement_n_initial_steps=5,
+                          multigrid_n_levels=3, example="const_rhs", space_discretization_type="supg",
+                          diffusion_coefficient=1e-3, convection=(0.6, 0.8, 0.0))),
+    ("served-3d-q1", dict(_MG_SOR, geo_file="Default_UnitCube_Hexa", ANSATZ_ORDER=1, refinement_n_initial_steps=4,
+                          multigrid_n_levels=3, example="const_rhs", space_discretization_type="supg",
+                          diffusion_coefficient=1e-3, convection=B_UNIT)),
+    ("q2-mixed-jacobi", dict(_MG_SOR, geo_file="Default_UnitCube_Hexa", ANSATZ_ORDER=2,
+                             refinement_n_initial_steps=3, multigrid_n_levels=3, example="sine_3d",
+                             space_discretization_type="supg", diffusion_coefficient=1e-3, convection=B_UNIT,
+                             multigrid_smoother="jacobi", jacobi_damp=0.5, multigrid_mixed_precision=1,
+                             INTERNAL_PROBLEM_LINEAR=1)),
+    ("p1-tet-cg", {"geo_file": "Default_UnitCube_Tetra", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 3,
+                   "example": "sine_3d", "solver_type": "iterative", "iterative_solver_type": "cg",
+                   "preconditioner": "jacobi", "residual_tolerance": 1e-12, "max_n_iterations": 500}),
+    ("q1-quad-direct", {"geo_file": "UnitSquare", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 4,
+                        "example": "sine_laplace_2d", "solver_type": "direct"}),
+    ("p2-tet-ssor", dict(_MG_SOR, geo_file="Default_UnitCube_Tetra", ANSATZ_ORDER=2, refinement_n_initial_steps=2,
+                         multigrid_n_levels=2, example="test_p2", multigrid_smoother="ssor")),
+]
+
+
+@pytest.mark.parametrize("name,db", BYTES_CASES, ids=[c[0] for c in BYTES_CASES])
+def test_device_bytes_within_estimate(name, db):
+    """The admission estimate (pmcd_estimate_device_bytes) is an upper bound
+    of the device bytes a system really holds after assembling and solving
+    (the per-system tally of every device allocation, peak included), and
+    not a loose one: within 4x."""
+    import parmoon2022_amd.serve as sv
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    st = cd.solve()
+    assert st.converged
+    now, peak = cd.device_bytes()
+    cd.close()
+    est = sv.estimate_bytes(db)
+    print("%s: held %.2f MB, peak %.2f MB, estimate %.2f MB (%.2fx)" %
+          (name, now / 1e6, peak / 1e6, est / 1e6, est / peak))
+    assert 0 < now <= peak <= est <= 4 * peak
+
+
+def test_server_reuses_resident_systems():
+    """Requests on a mesh seen before run on its parked system (reused, with
+    the system's exact bytes reserved instead of the estimate); the server
+    builds one system per mesh and concurrent slot, and every request's
+    iterations equal the fresh-system run."""
+    import parmoon2022_amd.serve as sv
+    dbs = _serve_dbs()
+    srv = sv.Server(max_slots=1, keep_solution=True)
+    first = [srv.submit(db) for db in dbs]
+    for r in first:
+        assert r.done.wait(120)
+    again = [srv.submit(db) for db in dbs]
+    for r in again:
+        assert r.done.wait(120)
+    info = srv.info()
+    srv.close()
+    meshes = {sv.mesh_key(db) for db in dbs}
+    assert info["systems_created"] == len(meshes)
+    assert info["finished"] == 2 * len(dbs)
+    assert all(r.reused for r in again)
+    for a, b, db in zip(first, again, dbs):
+        assert a.error is None and b.error is None
+        assert a.stats.iterations == b.stats.iterations and np.array_equal(a.solution, b.solution)
+        if not a.reused:
+            assert a.reserved_bytes == sv.estimate_bytes(db)
+        assert b.reserved_bytes <= sv.estimate_bytes(db)
+
+
+def test_autotune_sets_the_fastest_shape():
+    """pmcd_autotune times 13 launch shapes per kernel and leaves the fastest
+    set; the SpMV under the chosen shape still matches the oracle's serial
+    row sums."""
+    db = make_db(*ASSEMBLY_CASES[6])
+    old = (pm.tuning("spmv"), pm.tuning("jacobi"), pm.get_sell_min_rows())
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    try:
+        t = pm.autotune(cd, iters=3)
+        for kernel in ("spmv", "jacobi"):
+            assert len(t[kernel]) == 13 and all(v > 0 for v in t[kernel].values())
+            best = min(t[kernel], key=t[kernel].get)
+            lanes, variant = pm.tuning(kernel)
+            if best == "sell16":
+                assert variant == 4
+            else:
+                names = ("plain", "stream", "unrolled", "unrollednt")
+                m = re.fullmatch(r"([a-z]+)(\d+)", best)
+                assert (names[variant], lanes) == (m.group(1), int(m.group(2)))
+        x = np.random.default_rng(3).standard_normal(cd.info()["n_dof"])
+        rp, col = cd.pattern()
+        val = cd.matrix_values()
+        i = cd.info()
+        ref = np.zeros_like(x)
+        for r in range(i["n_active"]):
+            ref[r] = np.dot(val[rp[r]:rp[r + 1]], x[col[rp[r]:rp[r + 1]]])
+        ref[i["n_active"]:] = x[i["n_active"]:]
+        assert np.allclose(cd.apply(x), ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+    finally:
+        cd.close()
+        pm.set_tuning("spmv", *old[0])
+        pm.set_tuning("jacobi", *old[1])
+        pm.set_sell_min_rows(old[2])
 
 
 @pytest.mark.parametrize("exact", [True, False], ids=["exact-sor", "multicolour-sor"])

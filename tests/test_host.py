@@ -349,19 +349,48 @@ def test_geo_file_errors(tmp_path):
         pm.HostSetup({"geo_file": str(short), "ANSATZ_ORDER": 1, "example": "sine_2d"})
 
 
-def test_serve_estimate_for_geo_files():
-    """The served stream's memory estimate reads a .GEO file's counts: the
-    unit square as two triangles refined k times has (2^k + 1)^2 DoF (P1):
-    the estimate grows about fourfold per refinement and stays an upper
-    estimate of the built-in mesh's, within 20%."""
+ESTIMATE_CASES = [
+    ("TwoTriangles", 1, 5), ("TwoTriangles", 2, 4), ("TwoTriangles2", 1, 4), ("TwoTriangles2", 2, 3),
+    ("UnitSquare", 1, 4), ("UnitSquare", 2, 3), ("Default_UnitCube_Hexa", 1, 3), ("Default_UnitCube_Hexa", 2, 2),
+    ("Default_UnitCube_Tetra", 1, 3), ("Default_UnitCube_Tetra", 2, 2),
+    ("UnitSquare_tria.GEO", 1, 3), ("UnitSquare_tria.GEO", 2, 2), ("UnitSquare_quad.GEO", 2, 2),
+    ("UnitSquareIrregular.GEO", 1, 3), ("UnitSquareIrregular.GEO", 2, 2), ("UnitSquareCrissCross.GEO", 2, 2),
+    ("reference_tetrahedron.mesh", 1, 3), ("reference_tetrahedron.mesh", 2, 2),
+]
+
+
+@pytest.mark.parametrize("geo,order,k", ESTIMATE_CASES, ids=["%s-p%d-r%d" % c for c in ESTIMATE_CASES])
+def test_memory_estimate_counts_match_host_setup(geo, order, k):
+    """The admission estimate's counts (pmcd_estimate_counts: regular-refinement
+    recurrences of the coarse mesh's vertex, edge, face and cell counts) equal
+    the host set-up's cells and DoFs exactly, and its matrix-entry bound is at
+    least the host structure's entries, on the built-in meshes and the mesh
+    files of the reference's data."""
+    if geo.endswith((".GEO", ".mesh")):
+        geo = os.path.join(GOLDEN_DIR, geo)
+    db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": k}
+    info = pm.HostSetup(db).info()
+    out = (ctypes.c_longlong * 3)()
+    d = pm.problem_desc(db)
+    assert pm.lib().pmcd_estimate_counts(ctypes.byref(d), out) == 0, pm.lib().pmcd_last_error()
+    assert out[0] == info["n_cells"] and out[1] == info["n_dof"]
+    assert info["nnz"] <= out[2] <= 3 * info["nnz"]
+
+
+def test_memory_estimate_grows_with_the_mesh():
+    """The byte estimate of a .GEO problem grows about fourfold per 2D
+    refinement and covers the solver's vectors: FGMRES(30) needs more than
+    FGMRES(5) by at least 50 vectors of DoF length."""
     from parmoon2022_amd import serve
     path = os.path.join(GOLDEN_DIR, "UnitSquare_tria.GEO")
-    e6 = serve.estimate_bytes({"geo_file": path, "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 6})
-    e7 = serve.estimate_bytes({"geo_file": path, "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 7})
-    assert 3.5 < (e7 - (64 << 20)) / (e6 - (64 << 20)) <= 4.0
-    builtin = serve.estimate_bytes({"geo_file": "TwoTriangles2", "ANSATZ_ORDER": 1,
-                                    "refinement_n_initial_steps": 7})
-    assert builtin <= e7 <= 1.2 * builtin
+    base = {"geo_file": path, "ANSATZ_ORDER": 1, "solver_type": "iterative", "iterative_solver_type": "fgmres",
+            "gmres_restart": 5}
+    e6 = serve.estimate_bytes(dict(base, refinement_n_initial_steps=6))
+    e7 = serve.estimate_bytes(dict(base, refinement_n_initial_steps=7))
+    assert 3.0 < e7 / e6 <= 4.2
+    e7b = serve.estimate_bytes(dict(base, refinement_n_initial_steps=7, gmres_restart=30))
+    n = pm.HostSetup(dict(base, refinement_n_initial_steps=7)).info()["n_dof"]
+    assert e7b - e7 >= 50 * 8 * n
 
 
 # test/structure_test.c++ (test_all_values: rows, columns, entries, entries in

@@ -248,10 +248,22 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
  * multigrid V-cycles from captured HIP graphs, 0 to launch them kernel by
  * kernel (default); key 4 = assembly kernel (0 reference operation order on
  * vector ALUs, 1 FP64 MFMA for 27-DOF hexahedra and the vector kernels for the
- * other elements); key 7 = 1 (default) to run CG with no or Jacobi
+ * other elements, 2 (default) reference-tensor GEMMs on FP64 MFMA for every
+ * element but P1 simplices); key 7 = 1 (default) to run CG with no or Jacobi
  * preconditioning as two fused kernels per iteration with its scalars on the
  * device, 0 for the loop that reads every scalar back to the host; key 8 =
- * the largest grid of the device CG's SpMV kernel (1..4096 blocks). */
+ * the largest grid of the device CG's SpMV kernel (1..4096 blocks); key 12 =
+ * largest sweep class handled by the one-launch sweep kernel; key 13 = sliced
+ * ELL launch mode bits (1 XCD-contiguous slices, 2 row-ordered output through
+ * LDS, 4 non-temporal matrix loads, 8 matrix loads pipelined one step ahead;
+ * default 15; same bits in every mode); key 18 = sliced ELL blocks resident
+ * per CU at most (0 no cap, the default, or 3..16);
+ * key 14 = 1 (default) for XCD-contiguous rows in the unrolled CSR kernels;
+ * key 15 = FGMRES look-ahead factor (the next iteration is queued before the
+ * host test while the residual exceeds this many times the threshold;
+ * default 4, 0 off); key 16 = 1 (default) to replay each batched FGMRES
+ * iteration from a captured HIP graph; key 17 = 1 for one cooperative launch
+ * per batched multicolour sweep (default 0: one launch per class). */
 int pmcd_set_tuning(int key, int value);
 /* Directions of one multigrid cycle (CycleControl, src/Multigrid/
  * CycleControl.C:17-111): writes at most cap entries (0 down, 1 up, 2 end,

@@ -978,7 +978,9 @@ def get_sell_min_rows():
 
 def set_sell_mode(mode):
     """Sliced ELL launch mode: bit 0 XCD-contiguous slices, bit 1 row-ordered
-    output through LDS (layouts with 4-slice windows). Same bits in every mode."""
+    output through LDS (layouts with 4-slice windows), bit 2 non-temporal
+    matrix loads, bit 3 matrix loads pipelined one step ahead. Same bits in
+    every mode."""
     _check(lib().pmcd_set_tuning(13, int(mode)))
 
 

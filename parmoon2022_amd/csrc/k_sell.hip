@@ -14,9 +14,18 @@ namespace pmcd {
 
 namespace {
 
+// matrix entries are read once per sweep: with NT the loads are
+// non-temporal, so that they do not displace the gathered x entries, which
+// neighbouring rows read again, from the caches
+template <bool NT, class V>
+__device__ inline V stream_load(const V* p) {
+  if constexpr(NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // sum over the row in `slot`: U entries of the lane loaded per step, then
 // added in order
-template <int U, class T>
+template <int U, bool NT, class T>
 __device__ inline T sell_row_sum(const SellView<T>& m, int slot, int len, const T* __restrict__ x) {
   const long long base = m.off[slot / kSellRows] + (slot % kSellRows);
   T s = 0;
@@ -26,8 +35,8 @@ __device__ inline T sell_row_sum(const SellView<T>& m, int slot, int len, const 
 #pragma unroll
     for(int u = 0; u < U; ++u) {
       const int k = k0 + u;
-      c[u] = k < len ? m.col[base + (long long)k * kSellRows] : -1;
-      v[u] = k < len ? m.val[base + (long long)k * kSellRows] : T(0);
+      c[u] = k < len ? stream_load<NT>(&m.col[base + (long long)k * kSellRows]) : -1;
+      v[u] = k < len ? stream_load<NT>(&m.val[base + (long long)k * kSellRows]) : T(0);
     }
 #pragma unroll
     for(int u = 0; u < U; ++u) xv[u] = c[u] >= 0 ? x[c[u]] : T(0);
@@ -38,6 +47,53 @@ __device__ inline T sell_row_sum(const SellView<T>& m, int slot, int len, const 
   return s;
 }
 
+// The same sum with the matrix loads of the next U entries issued before the
+// products of the current ones: the HBM latency of the matrix stream overlaps
+// the gathers of x and the arithmetic (same products, same order).
+template <int U, bool NT, class T>
+__device__ inline T sell_row_sum_pipe(const SellView<T>& m, int slot, int len, const T* __restrict__ x) {
+  const long long base = m.off[slot / kSellRows] + (slot % kSellRows);
+  T s = 0;
+  int c[U];
+  T v[U];
+#pragma unroll
+  for(int u = 0; u < U; ++u) {
+    c[u] = u < len ? stream_load<NT>(&m.col[base + (long long)u * kSellRows]) : -1;
+    v[u] = u < len ? stream_load<NT>(&m.val[base + (long long)u * kSellRows]) : T(0);
+  }
+  for(int k0 = 0; k0 < len; k0 += U) {
+    T xv[U];
+#pragma unroll
+    for(int u = 0; u < U; ++u) xv[u] = c[u] >= 0 ? x[c[u]] : T(0);
+    int cn[U];
+    T vn[U];
+#pragma unroll
+    for(int u = 0; u < U; ++u) {
+      const int k = k0 + U + u;
+      cn[u] = k < len ? stream_load<NT>(&m.col[base + (long long)k * kSellRows]) : -1;
+      vn[u] = k < len ? stream_load<NT>(&m.val[base + (long long)k * kSellRows]) : T(0);
+    }
+#pragma unroll
+    for(int u = 0; u < U; ++u)
+      if(k0 + u < len) s += v[u] * xv[u];
+#pragma unroll
+    for(int u = 0; u < U; ++u) {
+      c[u] = cn[u];
+      v[u] = vn[u];
+    }
+  }
+  return s;
+}
+
+// PIPE: the pipelined sum (a kernel of its own, since it needs more registers)
+template <int U, bool PIPE, class T>
+__device__ inline T sell_row(const SellView<T>& m, int slot, int len, const T* __restrict__ x, int mode) {
+  if constexpr(PIPE)
+    return (mode & 4) ? sell_row_sum_pipe<U, true, T>(m, slot, len, x) : sell_row_sum_pipe<U, false, T>(m, slot, len, x);
+  else
+    return (mode & 4) ? sell_row_sum<U, true, T>(m, slot, len, x) : sell_row_sum<U, false, T>(m, slot, len, x);
+}
+
 // Launch layout: blocks of 256 threads (4 slices); the first slot_blocks
 // blocks cover the slots, the rest the non-active tail rows in order.
 //
@@ -45,6 +101,9 @@ __device__ inline T sell_row_sum(const SellView<T>& m, int slot, int len, const 
 // round-robin; the slot blocks are renumbered so that XCD k works through one
 // contiguous range of slices, keeping the x entries its rows share in its own
 // L2 instead of every XCD fetching most of x.
+//
+// mode bit 2 (non-temporal matrix loads): see stream_load. mode bit 3
+// (pipelined matrix loads): see sell_row_sum_pipe.
 //
 // mode bit 1 (row-ordered output): a block's 256 slots are the rows of one
 // 256-row window, sorted by length; the row sums go to LDS at the row's
@@ -60,7 +119,7 @@ __device__ inline int sell_block(int bid, int slot_blocks, int mode) {
 }
 
 // y[r] = beta z[r] + alpha (A x)[r] on active rows, beta z[r] + alpha x[r] on the tail
-template <int U, class T>
+template <int U, bool PIPE, class T>
 __global__ __launch_bounds__(256) void k_spmv_sell(int n, int n_active, SellView<T> m, int slot_blocks, int mode,
                                                    const T* __restrict__ x, const T* __restrict__ z,
                                                    T* __restrict__ y, T alpha, T beta) {
@@ -71,7 +130,7 @@ __global__ __launch_bounds__(256) void k_spmv_sell(int n, int n_active, SellView
     T s = 0;
     if(gt < m.n_slots) {
       row = m.perm[gt];
-      if(row >= 0) s = sell_row_sum<U, T>(m, gt, m.len[gt], x);
+      if(row >= 0) s = sell_row<U, PIPE, T>(m, gt, m.len[gt], x, mode);
     }
     if(mode & 2) {
       __shared__ T sh[256];
@@ -93,7 +152,7 @@ __global__ __launch_bounds__(256) void k_spmv_sell(int n, int n_active, SellView
 }
 
 // x_out = x + omega D^{-1} (b - A x), the non-active rows with A = I
-template <int U, class T>
+template <int U, bool PIPE, class T>
 __global__ __launch_bounds__(256) void k_jacobi_sell(int n, int n_active, SellView<T> m, int slot_blocks, int mode,
                                                      const T* __restrict__ diag, const T* __restrict__ b,
                                                      const T* __restrict__ x, T* __restrict__ x_out, T omega) {
@@ -104,7 +163,7 @@ __global__ __launch_bounds__(256) void k_jacobi_sell(int n, int n_active, SellVi
     const int gt = blk * 256 + threadIdx.x;
     if(gt < m.n_slots) {
       row = m.perm[gt];
-      if(row >= 0) s = sell_row_sum<U, T>(m, gt, m.len[gt], x);
+      if(row >= 0) s = sell_row<U, PIPE, T>(m, gt, m.len[gt], x, mode);
     }
     if(mode & 2) {
       __shared__ T sh[256];
@@ -156,10 +215,10 @@ int sell_blocks(int n, int n_active, const SellView<T>& m) {
 }  // namespace
 
 static int g_sell_unroll = 16;
-static int g_sell_mode = 3;
+static int g_sell_mode = 15;
 int sell_mode() { return g_sell_mode; }
 void set_sell_mode(int mode) {
-  PMCD_CHECK(mode >= 0 && mode <= 3, "sliced ELL mode must be 0..3");
+  PMCD_CHECK(mode >= 0 && mode <= 15, "sliced ELL mode must be 0..15");
   g_sell_mode = mode;
 }
 static int g_sell_window = 4;
@@ -175,6 +234,21 @@ void set_sell_window(int slices) {
   g_sell_window = slices;
 }
 int get_sell_window_slices() { return g_sell_window; }
+// blocks of the sliced ELL kernels resident per CU at most (0: no cap): the
+// cap is set through unused dynamic LDS (160 KiB per CU shared by the
+// resident blocks; one 256-thread block is one wave on each of the 4 SIMDs)
+static int g_sell_blocks_per_cu = 0;
+int sell_blocks_per_cu() { return g_sell_blocks_per_cu; }
+void set_sell_blocks_per_cu(int b) {
+  PMCD_CHECK(b == 0 || (b >= 3 && b <= 16), "sliced ELL blocks per CU must be 0 (no cap) or 3..16");
+  g_sell_blocks_per_cu = b;
+}
+static size_t sell_dynamic_lds() {
+  if(!g_sell_blocks_per_cu) return 0;
+  const size_t per_block = (160u * 1024u) / (size_t)g_sell_blocks_per_cu + 1;
+  const size_t fixed = 256 * sizeof(double);  // the kernels' static row buffer
+  return per_block > fixed ? per_block - fixed : 0;
+}
 int sell_unroll() { return g_sell_unroll; }
 void set_sell_unroll(int u) {
   PMCD_CHECK(u == 4 || u == 8 || u == 16, "sliced ELL unroll must be 4, 8 or 16");
@@ -185,7 +259,7 @@ void set_sell_unroll(int u) {
 // the layout (the window it was built with, not the current setting)
 template <class T>
 static int sell_launch_mode(const SellView<T>& m) {
-  return m.window == 256 ? g_sell_mode : (g_sell_mode & 1);
+  return m.window == 256 ? g_sell_mode : (g_sell_mode & 13);
 }
 
 template <class T>
@@ -193,11 +267,16 @@ void launch_spmv_sell(int n, int n_active, const SellView<T>& m, const T* x, con
                       hipStream_t s) {
   if(n == 0) return;
   const int blocks = sell_blocks(n, n_active, m), sb = sell_slot_blocks(m), mode = sell_launch_mode(m);
+  const size_t lds = sell_dynamic_lds();
+#define PMCD_SELL_SPMV(U, P) \
+  k_spmv_sell<U, P, T><<<blocks, 256, lds, s>>>(n, n_active, m, sb, mode, x, z, y, alpha, beta)
+  const bool pipe = (mode & 8) != 0;
   switch(g_sell_unroll) {
-    case 4: k_spmv_sell<4, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, x, z, y, alpha, beta); break;
-    case 16: k_spmv_sell<16, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, x, z, y, alpha, beta); break;
-    default: k_spmv_sell<8, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, x, z, y, alpha, beta); break;
+    case 4: if(pipe) PMCD_SELL_SPMV(4, true); else PMCD_SELL_SPMV(4, false); break;
+    case 16: if(pipe) PMCD_SELL_SPMV(16, true); else PMCD_SELL_SPMV(16, false); break;
+    default: if(pipe) PMCD_SELL_SPMV(8, true); else PMCD_SELL_SPMV(8, false); break;
   }
+#undef PMCD_SELL_SPMV
   PMCD_HIP(hipGetLastError());
 }
 
@@ -206,11 +285,16 @@ void launch_jacobi_sell(int n, int n_active, const SellView<T>& m, const T* diag
                         T omega, hipStream_t s) {
   if(n == 0) return;
   const int blocks = sell_blocks(n, n_active, m), sb = sell_slot_blocks(m), mode = sell_launch_mode(m);
+  const size_t lds = sell_dynamic_lds();
+#define PMCD_SELL_JAC(U, P) \
+  k_jacobi_sell<U, P, T><<<blocks, 256, lds, s>>>(n, n_active, m, sb, mode, diag, b, x, x_out, omega)
+  const bool pipe = (mode & 8) != 0;
   switch(g_sell_unroll) {
-    case 4: k_jacobi_sell<4, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, diag, b, x, x_out, omega); break;
-    case 16: k_jacobi_sell<16, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, diag, b, x, x_out, omega); break;
-    default: k_jacobi_sell<8, T><<<blocks, 256, 0, s>>>(n, n_active, m, sb, mode, diag, b, x, x_out, omega); break;
+    case 4: if(pipe) PMCD_SELL_JAC(4, true); else PMCD_SELL_JAC(4, false); break;
+    case 16: if(pipe) PMCD_SELL_JAC(16, true); else PMCD_SELL_JAC(16, false); break;
+    default: if(pipe) PMCD_SELL_JAC(8, true); else PMCD_SELL_JAC(8, false); break;
   }
+#undef PMCD_SELL_JAC
   PMCD_HIP(hipGetLastError());
 }
 

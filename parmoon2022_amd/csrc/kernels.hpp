@@ -225,6 +225,8 @@ void launch_copy(int n, const double* x, double* y, hipStream_t s);
 void launch_stream_copy(long n, const double* x, double* y, hipStream_t s);
 void launch_stream_copy4(long n, const double* x, double* y, hipStream_t s);
 // out[w] = partial sums of x per wave (8192 x 4 entries)
+// out must hold kStreamReadBlocks * 4 doubles (one partial sum per wavefront)
+constexpr int kStreamReadBlocks = 32768;
 void launch_stream_read(long n, const double* x, double* out, hipStream_t s);
 // fused MGS step: r -= h_k v_k; h_{k+1} = <r, v_{k+1}> partials (v_next may be null)
 void launch_mgs_step(int n, const double* h_dev, double* r, const double* v,
@@ -347,6 +349,8 @@ constexpr int kSellRows = 64;
 int sell_window();
 // levels with fewer active rows keep the unrolled CSR kernels under variant 4
 int sell_min_rows();
+int sell_blocks_per_cu();
+void set_sell_blocks_per_cu(int b);
 void set_sell_min_rows(int n);
 template <class T>
 struct SellView {

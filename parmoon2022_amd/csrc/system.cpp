@@ -519,7 +519,7 @@ void CDSystem::time_kernel(int which, int level, int iters, double* avg_ms) {
   }
   DevBuf<double> copy_dst;
   if(which == 6 || which == 7) copy_dst.alloc(L.val.n & ~(size_t)1);
-  if(which == 8) copy_dst.alloc(8192 * 4);
+  if(which == 8) copy_dst.alloc((size_t)kStreamReadBlocks * 4);
   AsmArgs a = asm_args(L);
   auto launch = [&]() {
     switch(which) {

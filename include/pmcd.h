@@ -316,16 +316,20 @@ int pmcd_tcd_step(pmcd_system* s, const pmcd_solver_desc* sd, pmcd_solve_stats* 
 
 /* ---- algebraic flux correction ---------------------------------------------
  * ConvectionDiffusion_AFC<d>::assemble(0); solve(0); solve(k) until it
- * returns true (src/System/ConvectionDiffusion_AFC.C:132-504, the loop of
+ * returns true (src/System/ConvectionDiffusion_AFC.C:132-686, the loop of
  * test/cd2d_afc_test.c++:60-74) with afc_limiter = kuzmin or BJK17 (P1
- * triangles and tetrahedra), afc_iteration_scheme = fixed_point_rhs or
- * fixed_point_matrix, dynamic damping, no Anderson acceleration,
- * afc_initial_iterate = afc_zero. Single level, Galerkin.
- * prm[14]: afc_nonlinloop_epsilon, afc_nonlinloop_maxit,
+ * triangles and tetrahedra), afc_iteration_scheme = fixed_point_rhs,
+ * fixed_point_matrix or newton (AlgebraicFluxCorrection.C:2794-2981, the
+ * Jacobians of :224-578 and :957-1140), dynamic damping, with or without
+ * Anderson acceleration (anderson.f90), afc_initial_iterate = afc_zero.
+ * Single level, Galerkin.
+ * prm[19]: afc_nonlinloop_epsilon, afc_nonlinloop_maxit,
  * afc_nonlinloop_damping_factor, _max, _min, _max_global, _increase,
  * _decrease, _max_increase, _max_decrease, _min_tol, limiter (0 kuzmin,
- * 1 BJK17), scheme (0 fixed_point_rhs, 1 fixed_point_matrix),
- * afc_fixed_point_matrix_weight.
+ * 1 BJK17), scheme (0 fixed_point_rhs, 1 fixed_point_matrix, 2 newton),
+ * afc_fixed_point_matrix_weight, afc_fixed_point_derivative_weight,
+ * afc_change_method_threshold, afc_nonlinloop_anderson_acc (0 no, 1 yes),
+ * afc_nonlinloop_anderson_acc_vec (1..32), afc_nonlinloop_anderson_acc_start.
  * out[4]: nonlinear iterations, final residual norm, rejected steps, initial
  * residual norm. stats: linear iterations summed over the loop. The solution
  * is left in the system (pmcd_get_solution, pmcd_errors). */

@@ -1172,8 +1172,10 @@ static double sdfem_delta(int dim, double hK, double eps, double b1, double b2, 
 
 /* Planted mistakes for the tests that show a check can fail (0: none).
    1: the SUPG -eps*Laplacian term dropped; 2: the mixed second reference
-   derivatives dropped from the physical Hessian. Set only by tests, through
-   orc_set_fault. */
+   derivatives dropped from the physical Hessian; 3: the Jacobian of the
+   AFC Newton scheme dropped; 4: the Anderson mixing skipped; 5: the
+   downwind branch of the Kuzmin Jacobian compares E_kj unrounded. Set only
+   by tests, through orc_set_fault. */
 static int orc_fault = 0;
 void orc_set_fault(int code) { orc_fault = code; }
 
@@ -2211,6 +2213,7 @@ static void afc_diffusion(const olevel* L, const double* A, double* D) {
 
 typedef struct {
   double *A, *D, *E, *f, *F, *alpha, *Pp, *Pm, *Qp, *Qm, *Rp, *Rm, *gamma;
+  double *umax, *umin, *q, *df; /* BJK17 bounds and q_i; the Newton matrix */
 } oafc;
 
 /* Compute_Parameter_For_Linearity_Preservation (AlgebraicFluxCorrection.C:
@@ -2287,6 +2290,9 @@ static void afc_bjk_rhs(const olevel* L, oafc* W, const double* u, double* rhs) 
       if(c != i) q += W->D[l];
     }
     q *= W->gamma[i];
+    W->umin[i] = umin;
+    W->umax[i] = umax;
+    W->q[i] = q;
     double Pp = 0, Pm = 0;
     for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
       const int c = L->col[l];
@@ -2296,6 +2302,7 @@ static void afc_bjk_rhs(const olevel* L, oafc* W, const double* u, double* rhs) 
       if(W->F[l] < 0) Pm += W->F[l];
     }
     const double Qp = q * (u[i] - umax), Qm = q * (u[i] - umin);
+    W->Pp[i] = Pp; W->Pm[i] = Pm; W->Qp[i] = Qp; W->Qm[i] = Qm;
     W->Rp[i] = W->Rm[i] = 1.0;
     if(fabs(Pp) > 0) { W->Rp[i] = Qp / Pp; if(W->Rp[i] > 1) W->Rp[i] = 1; }
     if(fabs(Pm) > 0) { W->Rm[i] = Qm / Pm; if(W->Rm[i] > 1) W->Rm[i] = 1; }
@@ -2402,6 +2409,277 @@ static void afc_fpm(olevel* L, const oafc* W, const double* u, double* rhs, doub
   if(L->lu) { free(L->lu); L->lu = NULL; free(L->piv); L->piv = NULL; }
 }
 
+/* sums of D over row r for the derivative of Q and P at a diagonal entry
+   (Compute_Jacobian_times_flux_Kuzmin, :296-345): over the entries with
+   F < 0 (sign -1) or F > 0 (sign +1); with `upwind`, only the entries whose
+   E is at least the transposed entry's */
+static double row_d_sum(const olevel* L, const oafc* W, int r, int sign, int upwind) {
+  double s = 0;
+  for(int m = L->rp[r]; m < L->rp[r + 1]; ++m) {
+    const int ok = sign > 0 ? W->F[m] > 0 : W->F[m] < 0;
+    if(!ok) continue;
+    if(upwind && !(W->E[m] >= W->E[transposed_pos(L, L->col[m], r)])) continue;
+    s += W->D[m];
+  }
+  return s;
+}
+
+/* Compute_Jacobian_times_flux_Kuzmin (AlgebraicFluxCorrection.C:224-578):
+   the sum over the row of i of the derivative of the limiter factor of the
+   link (i, k) with respect to u_j times the flux F_ik, with the reference's
+   case distinctions. The reference keeps E_kj in an int (a_kj) in the
+   downwind branch; the comparison below does the same. */
+static double jac_kuzmin(const olevel* L, const oafc* W, int i, int j, int pij) {
+  const double eps = 1e-14, epsinv = 1.0 / eps;
+  const double *E = W->E, *F = W->F, *D = W->D;
+  const int a_ji = transposed_pos(L, j, i);
+  double sum = 0;
+  for(int k = L->rp[i]; k < L->rp[i + 1]; ++k) {
+    const int kk = L->col[k];
+    int a_kj = -1, a_ki = L->rp[kk];
+    for(int jj = L->rp[kk]; jj < L->rp[kk + 1]; ++jj) {
+      if(L->col[jj] == j) a_kj = jj;
+      if(L->col[jj] == i) a_ki = jj;
+    }
+    if(E[k] >= E[a_ki]) {
+      if(F[k] > 0 && fabs(W->Rp[i] - 1) < eps) continue;
+      if(F[k] < 0 && fabs(W->Rm[i] - 1) < eps) continue;
+      if(fabs(F[k]) < eps) continue;
+      if(i != j && fabs(D[pij]) < 1e-14) continue;
+      if(F[k] > 0) {
+        double DQ, DP;
+        if(i != j) DQ = F[pij] >= 0 ? 0 : -D[pij];
+        else DQ = row_d_sum(L, W, i, -1, 0);
+        if(i != j) DP = F[pij] <= 0 ? 0 : (E[pij] >= E[a_ji] ? D[pij] : 0);
+        else DP = -row_d_sum(L, W, i, +1, 1);
+        const double val = DQ / W->Pp[i] - W->Qp[i] * DP / (W->Pp[i] * W->Pp[i]);
+        if(fabs(val) < epsinv) sum += val * F[k];
+        continue;
+      }
+      if(F[k] < 0) {
+        double DQ, DP;
+        if(i != j) DQ = F[pij] <= 0 ? 0 : -D[pij];
+        else DQ = row_d_sum(L, W, i, +1, 0);
+        if(i != j) DP = F[pij] >= 0 ? 0 : (E[pij] >= E[a_ji] ? D[pij] : 0);
+        else DP = -row_d_sum(L, W, i, -1, 1);
+        const double val = DQ / W->Pm[i] - W->Qm[i] * DP / (W->Pm[i] * W->Pm[i]);
+        if(fabs(val) < epsinv) sum += val * F[k];
+        continue;
+      }
+    } else {
+      if(F[k] > 0 && fabs(W->Rm[kk] - 1) < eps) continue;
+      if(F[k] < 0 && fabs(W->Rp[kk] - 1) < eps) continue;
+      if(fabs(F[k]) < eps) continue;
+      if(a_kj < 0) continue;
+      const double f_kj = F[a_kj], d_kj = D[a_kj];
+      const double e_kj = orc_fault == 5 ? E[a_kj] : (double)(int)E[a_kj];
+      if(kk != j && fabs(d_kj) < 1e-14) continue;
+      const int a_jk = transposed_pos(L, j, kk);
+      if(F[k] > 0) {
+        double DQ, DP;
+        if(kk != j) DQ = f_kj <= 0 ? 0 : -d_kj;
+        else DQ = row_d_sum(L, W, kk, +1, 0);
+        if(kk != j) DP = f_kj >= 0 ? 0 : (e_kj >= E[a_jk] ? d_kj : 0);
+        else DP = -row_d_sum(L, W, kk, -1, 1);
+        const double val = DQ / W->Pm[kk] - W->Qm[kk] * DP / (W->Pm[kk] * W->Pm[kk]);
+        if(fabs(val) < epsinv) sum += val * F[k];
+        continue;
+      }
+      if(F[k] < 0) {
+        double DQ, DP;
+        if(kk != j) DQ = f_kj >= 0 ? 0 : -d_kj;
+        else DQ = row_d_sum(L, W, kk, -1, 0);
+        if(kk != j) DP = f_kj <= 0 ? 0 : (e_kj >= E[a_jk] ? d_kj : 0);
+        else DP = -row_d_sum(L, W, kk, +1, 1);
+        const double val = DQ / W->Pp[kk] - W->Qp[kk] * DP / (W->Pp[kk] * W->Pp[kk]);
+        if(fabs(val) < epsinv) sum += val * F[k];
+        continue;
+      }
+    }
+  }
+  return sum;
+}
+
+/* Compute_Jacobian_times_flux_BJK17 (AlgebraicFluxCorrection.C:957-1140) */
+static double jac_bjk(const olevel* L, const oafc* W, double u_j, int i, int j, int pij) {
+  const double eps = 1e-14, epsinv = 1.0 / eps;
+  const double *F = W->F, *D = W->D;
+  double sum = 0;
+  for(int k = L->rp[i]; k < L->rp[i + 1]; ++k) {
+    if(F[k] > 0 && fabs(W->Rp[i] - 1.0) < eps) continue;
+    if(F[k] < 0 && fabs(W->Rm[i] - 1.0) < eps) continue;
+    if(fabs(F[k]) < eps) continue;
+    const int ck = L->col[k];
+    double DQ = 0.0, DP = 0.0;
+    if(F[k] > 0) {
+      if(W->Rp[i] <= W->Rm[ck]) {
+        if(i != j) {
+          if(fabs(W->umax[i] - u_j) < eps) DQ = -W->q[i];
+          if(F[pij] > 0) DP = D[pij];
+        } else {
+          if(fabs(W->umax[i] - u_j) >= eps) DQ = W->q[i];
+          DP = -row_d_sum(L, W, i, +1, 0);
+        }
+        if(W->Pp[i] > 0.0) {
+          const double val = DQ / W->Pp[i] - W->Qp[i] * DP / (W->Pp[i] * W->Pp[i]);
+          if(fabs(val) < epsinv) sum += val * F[k];
+        }
+      } else {
+        if(ck != j) {
+          if(fabs(W->umin[ck] - u_j) < eps) DQ = -W->q[ck];
+          if(F[pij] < 0) DP = D[pij];
+        } else {
+          if(fabs(W->umin[j] - u_j) >= eps) DQ = W->q[j];
+          DP = -row_d_sum(L, W, j, -1, 0);
+        }
+        if(W->Pm[ck] < 0.0) {
+          const double val = DQ / W->Pm[ck] - W->Qm[ck] * DP / (W->Pm[ck] * W->Pm[ck]);
+          if(fabs(val) < epsinv) sum += val * F[k];
+        }
+      }
+      continue;
+    }
+    if(F[k] < 0) {
+      if(W->Rm[i] <= W->Rp[ck]) {
+        if(i != j) {
+          if(fabs(W->umin[i] - u_j) < eps) DQ = -W->q[i];
+          if(F[pij] < 0) DP = D[pij];
+        } else {
+          if(fabs(W->umin[i] - u_j) >= eps) DQ = W->q[i];
+          DP = -row_d_sum(L, W, i, -1, 0);
+        }
+        if(W->Pm[i] < 0.0) {
+          const double val = DQ / W->Pm[i] - W->Qm[i] * DP / (W->Pm[i] * W->Pm[i]);
+          if(fabs(val) < epsinv) sum += val * F[k];
+        }
+      } else {
+        if(ck != j) {
+          if(fabs(W->umax[ck] - u_j) < eps) DQ = -W->q[ck];
+          if(F[pij] > 0) DP = D[pij];
+        } else {
+          if(fabs(W->umax[j] - u_j) >= eps) DQ = W->q[j];
+          DP = -row_d_sum(L, W, j, +1, 0);
+        }
+        if(W->Pp[ck] > 0.0) {
+          const double val = DQ / W->Pp[ck] - W->Qp[ck] * DP / (W->Pp[ck] * W->Pp[ck]);
+          if(fabs(val) < epsinv) sum += val * F[k];
+        }
+      }
+    }
+  }
+  return sum;
+}
+
+/* Newton scheme (steady_state_algorithm :2794-2981), after the limiter:
+   rhs_i = f_i + sum_j (-E_ij u_j + alpha_ij F_ij); the matrix
+   DF_ij = E_ij - tau0 alpha_ij d_ij - tau1 J_ij off the diagonal,
+   DF_ii = E_ii + tau0 sum_{j != i} alpha_ij d_ij - tau1 J_ii, J the
+   limiter's Jacobian times the fluxes; then rhs_i += sum_j DF_ij u_j and
+   the system matrix is DF (Dirichlet rows identity / u). tau1 = 0 skips J
+   (its product with 0 changes nothing). */
+static void afc_newton(olevel* L, oafc* W, const double* u, double* rhs, double tau0, double tau1, int bjk) {
+  const int n = L->ndof, na = L->nactive;
+  const double *E = W->E, *D = W->D;
+  for(int i = 0; i < na; ++i) {
+    double r = W->f[i];
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) r += -(E[l]) * u[L->col[l]] + W->alpha[l] * W->F[l];
+    rhs[i] = r;
+  }
+  for(int i = 0; i < na; ++i)
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
+      const int c = L->col[l];
+      double J = 0;
+      if(tau1 != 0 && orc_fault != 3) J = bjk ? jac_bjk(L, W, u[c], i, c, l) : jac_kuzmin(L, W, i, c, l);
+      if(c != i) {
+        W->df[l] = E[l] - tau0 * W->alpha[l] * D[l] - tau1 * J;
+      } else {
+        double sum_flux = 0.0;
+        for(int jj = L->rp[i]; jj < L->rp[i + 1]; ++jj)
+          if(L->col[jj] != i) sum_flux += W->alpha[jj] * D[jj];
+        W->df[l] = E[l] + tau0 * sum_flux - tau1 * J;
+      }
+    }
+  for(int i = 0; i < n; ++i) {
+    if(i >= na) {
+      for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) L->val[l] = L->col[l] == i ? 1.0 : 0.0;
+      rhs[i] = u[i];
+      continue;
+    }
+    for(int l = L->rp[i]; l < L->rp[i + 1]; ++l) {
+      rhs[i] += W->df[l] * u[L->col[l]];
+      L->val[l] = W->df[l];
+    }
+  }
+  extract_diag(L);
+  if(L->lu) { free(L->lu); L->lu = NULL; free(L->piv); L->piv = NULL; }
+}
+
+/* Anderson acceleration (ConvectionDiffusion_AFC.C:600-686, anderson.f90):
+   with the columns R_1..R_k of the last k updates and X_1..X_k of the last
+   k fixed point outputs, alpha = argmin |R alpha| subject to sum alpha = 1
+   (DGGLSE with B = (1 ... 1), d = 1, c = 0), and the new iterate
+   sum_j alpha_j X_j. Solved as DGGLSE does: the Householder reflector H with
+   (1 ... 1) H = (0 ... 0 r), then a Householder QR of R H; x_k = 1 / r,
+   the first k - 1 unknowns from the triangular system, alpha = H x.
+   Returns 0 when R H has a zero pivot (the iterate is left unchanged). */
+static int anderson_alpha(int n, int k, const double* R, double* alpha) {
+  /* reflector of the ones row: v = (1 ... 1) - s e_k with s = -sqrt(k) */
+  double* v = malloc(sizeof(double) * k);
+  const double r = -sqrt((double)k);
+  for(int j = 0; j < k; ++j) v[j] = 1.0;
+  v[k - 1] -= r;
+  double vv = 0;
+  for(int j = 0; j < k; ++j) vv += v[j] * v[j];
+  /* M = R H (n x k, column major): M = R - (R v) (2 / vv) v^T */
+  double* M = malloc(sizeof(double) * (size_t)n * k);
+  for(int i = 0; i < n; ++i) {
+    double rv = 0;
+    for(int j = 0; j < k; ++j) rv += R[(size_t)j * n + i] * v[j];
+    for(int j = 0; j < k; ++j) M[(size_t)j * n + i] = R[(size_t)j * n + i] - rv * (2.0 / vv) * v[j];
+  }
+  /* Householder QR of M in place: upper triangle = T */
+  double* w = malloc(sizeof(double) * n);
+  int ok = 1;
+  for(int c = 0; c < k && c < n; ++c) {
+    double* x = M + (size_t)c * n;
+    double nrm = 0;
+    for(int i = c; i < n; ++i) nrm += x[i] * x[i];
+    nrm = sqrt(nrm);
+    if(nrm == 0) { ok = c >= k - 1; continue; }
+    const double a = x[c] > 0 ? -nrm : nrm;
+    for(int i = c; i < n; ++i) w[i] = x[i];
+    w[c] -= a;
+    double ww = 0;
+    for(int i = c; i < n; ++i) ww += w[i] * w[i];
+    for(int cc = c; cc < k; ++cc) {
+      double* y = M + (size_t)cc * n;
+      double d = 0;
+      for(int i = c; i < n; ++i) d += w[i] * y[i];
+      const double t = 2.0 * d / ww;
+      for(int i = c; i < n; ++i) y[i] -= t * w[i];
+    }
+  }
+  double* xv = malloc(sizeof(double) * k);
+  if(ok) {
+    xv[k - 1] = 1.0 / r;
+    /* T11 x1 = -T12 x_k */
+    for(int i = k - 2; i >= 0; --i) {
+      double s = -M[(size_t)(k - 1) * n + i] * xv[k - 1];
+      for(int j = i + 1; j < k - 1; ++j) s -= M[(size_t)j * n + i] * xv[j];
+      const double piv = M[(size_t)i * n + i];
+      if(piv == 0) { ok = 0; break; }
+      xv[i] = s / piv;
+    }
+  }
+  if(ok) {
+    double vx = 0;
+    for(int j = 0; j < k; ++j) vx += v[j] * xv[j];
+    for(int j = 0; j < k; ++j) alpha[j] = xv[j] - (2.0 / vv) * vx * v[j];
+  }
+  free(v); free(M); free(w); free(xv);
+  return ok;
+}
+
 static double afc_residual(const olevel* L, const double* u, const double* rhs, double* tmp) {
   memcpy(tmp, rhs, sizeof(double) * L->ndof);
   apply_scaled_add(L, u, tmp, -1.0);
@@ -2424,12 +2702,14 @@ int orc_afc_solve(orc_problem* P, const osolver* s, const double* prm, double* o
     P->afc = 0;
     return 4;
   }
-  double* buf = calloc((size_t)6 * nnz + 8 * (size_t)n, sizeof(double));
+  double* buf = calloc((size_t)7 * nnz + 11 * (size_t)n, sizeof(double));
   W.A = buf; W.D = W.A + nnz; W.E = W.D + nnz; W.F = W.E + nnz; W.alpha = W.F + nnz;
   double* S = W.alpha + nnz;
-  W.f = S + nnz; W.Pp = W.f + n; W.Pm = W.Pp + n; W.Qp = W.Pm + n; W.Qm = W.Qp + n; W.Rp = W.Qm + n;
+  W.df = S + nnz;
+  W.f = W.df + nnz; W.Pp = W.f + n; W.Pm = W.Pp + n; W.Qp = W.Pm + n; W.Qm = W.Qp + n; W.Rp = W.Qm + n;
   W.Rm = W.Rp + n;
   W.gamma = W.Rm + n;
+  W.umax = W.gamma + n; W.umin = W.umax + n; W.q = W.umin + n;
   double* old = malloc(sizeof(double) * n);
   double* cur_sol = malloc(sizeof(double) * n);
   double* cur_rhs = malloc(sizeof(double) * n);
@@ -2467,10 +2747,29 @@ int orc_afc_solve(orc_problem* P, const osolver* s, const double* prm, double* o
   const double damp_min = prm[4], max_global = prm[5], inc = prm[6], dec = prm[7], max_inc = prm[8],
                max_dec = prm[9], min_tol = prm[10];
   void (*limiter)(const olevel*, oafc*, const double*, double*) = bjk ? afc_bjk_rhs : afc_kuzmin_rhs;
-  const int fpm = prm[12] == 1.0;
+  const int fpm = prm[12] == 1.0, newton = prm[12] == 2.0;
   const double tau0 = prm[13];
-#define limited_rhs(L_, W_, u_, r_) \
-  do { limiter(L_, W_, u_, r_); if(fpm) afc_fpm(L_, W_, u_, r_, tau0); } while(0)
+  /* Newton: the derivative's weight is afc_fixed_point_derivative_weight
+     times a factor that starts at 0 and follows the residual
+     (ConvectionDiffusion_AFC.C:427-460) */
+  const double deriv_weight = prm[14], threshold = prm[15];
+  double deriv_factor = 0.0;
+  /* Anderson acceleration: the last kA fixed point outputs and updates */
+  const int anderson = prm[16] == 1.0, kA = (int)prm[17], startA = (int)prm[18];
+  double *histX = NULL, *histR = NULL, *Rmat = NULL, *alphaA = NULL;
+  int n_hist = 0;
+  if(anderson) {
+    histX = malloc(sizeof(double) * (size_t)n * kA);
+    histR = malloc(sizeof(double) * (size_t)n * kA);
+    Rmat = malloc(sizeof(double) * (size_t)n * kA);
+    alphaA = malloc(sizeof(double) * kA);
+  }
+#define limited_rhs(L_, W_, u_, r_)                                          \
+  do {                                                                       \
+    limiter(L_, W_, u_, r_);                                                 \
+    if(fpm) afc_fpm(L_, W_, u_, r_, tau0);                                   \
+    if(newton) afc_newton(L_, W_, u_, r_, tau0, deriv_weight * deriv_factor, bjk); \
+  } while(0)
   /* iteration 0 */
   limited_rhs(L, &W, L->sol, L->rhs);
   double residual_old = afc_residual(L, L->sol, L->rhs, tmp), residual = residual_old;
@@ -2480,6 +2779,30 @@ int orc_afc_solve(orc_problem* P, const osolver* s, const double* prm, double* o
   int rc = orc_solve(P, s, &st);
   int k = 1, rejected = 0;
   for(;; ++k) {
+    const int acc = anderson && k >= startA;
+    if(acc) {
+      /* anderson_acceleration_damping: append the output and the update,
+         drop the oldest beyond kA, mix once kA are there */
+      if(n_hist == kA) {
+        memmove(histX, histX + n, sizeof(double) * (size_t)n * (kA - 1));
+        memmove(histR, histR + n, sizeof(double) * (size_t)n * (kA - 1));
+        --n_hist;
+      }
+      for(int i = 0; i < n; ++i) {
+        histX[(size_t)n_hist * n + i] = L->sol[i];
+        histR[(size_t)n_hist * n + i] = L->sol[i] - old[i];
+      }
+      ++n_hist;
+      if(k > kA + startA && n_hist == kA) {
+        memcpy(Rmat, histR, sizeof(double) * (size_t)n * kA);
+        if(orc_fault != 4 && anderson_alpha(n, kA, Rmat, alphaA))
+          for(int i = 0; i < n; ++i) {
+            double sum = 0.0;
+            for(int j = 0; j < kA; ++j) sum = sum + histX[(size_t)j * n + i] * alphaA[j];
+            L->sol[i] = sum;
+          }
+      }
+    }
     double omega = damp;
     int first_damp = 1;
     memcpy(cur_rhs, L->rhs, sizeof(double) * n);
@@ -2498,24 +2821,45 @@ int orc_afc_solve(orc_problem* P, const osolver* s, const double* prm, double* o
         damp = omega;
         break;
       }
-      ++rejected;
-      memcpy(L->sol, cur_sol, sizeof(double) * n);
-      memcpy(L->rhs, cur_rhs, sizeof(double) * n);
+      if(!acc) {
+        ++rejected;
+        memcpy(L->sol, cur_sol, sizeof(double) * n);
+        memcpy(L->rhs, cur_rhs, sizeof(double) * n);
+      }
       omega = fmax(damp_min, omega * dec);
       if(first_damp == 1) {
         damp_max = fmax(damp_min, max_dec * damp_max);
         first_damp = 0;
       }
       damp = omega;
+      /* with Anderson acceleration a rejected step only lowers the damping */
+      if(acc) break;
     }
     if(residual < eps && k > 1) break;
     if(k == maxit) break;
     memcpy(old, L->sol, sizeof(double) * n);
     rc = orc_solve(P, s, &st);
     if(rc) break;
+    if(newton) {
+      if(residual <= threshold) {
+        double w = deriv_factor;
+        if(w < 1e-3) w = 0.25;
+        if(residual / residual_old > 0.99) {
+          w *= 0.999;
+          if(w < 0.1) w = 0.1;
+        }
+        if(residual / residual_old < 0.99) {
+          w *= 1.001;
+          if(w > 1.0) w = 1.0;
+        }
+        deriv_factor = w;
+      }
+      if(residual > 100 * threshold) deriv_factor = 0.0;
+    }
     residual_old = residual;
   }
 #undef limited_rhs
+  free(histX); free(histR); free(Rmat); free(alphaA);
   out[0] = k;
   out[1] = residual;
   out[2] = rejected;

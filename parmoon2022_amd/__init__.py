@@ -728,14 +728,15 @@ AFC_PARAM_KEYS = [("afc_nonlinloop_epsilon", 1e-10), ("afc_nonlinloop_maxit", 1)
 def afc_params(db):
     """Parameters of the algebraic flux correction loop in the reference's
     names and defaults (AlgebraicFluxCorrection::default_afc_database,
-    src/AssembleRoutines/AlgebraicFluxCorrection.C:1914-2056), as the 11
-    doubles the C ABI takes, then the limiter (0 kuzmin, 1 BJK17; BJK17 for
-    P1 triangles and tetrahedra), the iteration scheme (0 fixed_point_rhs,
-    1 fixed_point_matrix, the reference's default) and
-    afc_fixed_point_matrix_weight: 14 doubles. Dynamic damping, no Anderson
-    acceleration and the zero initial iterate."""
-    fixed = {"afc_nonlinloop_anderson_acc": "no", "afc_initial_iterate": "afc_zero",
-             "afc_nonlinloop_damping_factor_constant": "no"}
+    src/AssembleRoutines/AlgebraicFluxCorrection.C:1914-2056), as the 19
+    doubles the C ABI takes: the 11 damping parameters, then the limiter
+    (0 kuzmin, 1 BJK17; BJK17 for P1 triangles and tetrahedra), the iteration
+    scheme (0 fixed_point_rhs, 1 fixed_point_matrix, the reference's default,
+    2 newton), afc_fixed_point_matrix_weight, afc_fixed_point_derivative_weight,
+    afc_change_method_threshold, afc_nonlinloop_anderson_acc (0/1),
+    afc_nonlinloop_anderson_acc_vec and afc_nonlinloop_anderson_acc_start.
+    Dynamic damping and the zero initial iterate."""
+    fixed = {"afc_initial_iterate": "afc_zero", "afc_nonlinloop_damping_factor_constant": "no"}
     for k, v in fixed.items():
         if db.get(k, v) != v:
             raise ValueError("%s = %s is not built (only %s)" % (k, db.get(k), v))
@@ -743,12 +744,19 @@ def afc_params(db):
     lim = db.get("afc_limiter", "kuzmin")
     if lim not in limiters:
         raise ValueError("afc_limiter = %s is not built (kuzmin, BJK17)" % lim)
-    schemes = {"fixed_point_rhs": 0.0, "fixed_point_matrix": 1.0}
+    schemes = {"fixed_point_rhs": 0.0, "fixed_point_matrix": 1.0, "newton": 2.0}
     sch = db.get("afc_iteration_scheme", "fixed_point_matrix")
     if sch not in schemes:
-        raise ValueError("afc_iteration_scheme = %s is not built (fixed_point_rhs, fixed_point_matrix)" % sch)
+        raise ValueError("afc_iteration_scheme = %s is not built (fixed_point_rhs, fixed_point_matrix, newton)" % sch)
+    acc = db.get("afc_nonlinloop_anderson_acc", "no")
+    if acc not in ("no", "yes"):
+        raise ValueError("afc_nonlinloop_anderson_acc must be yes or no")
     return np.array([float(db.get(k, d)) for k, d in AFC_PARAM_KEYS] +
-                    [limiters[lim], schemes[sch], float(db.get("afc_fixed_point_matrix_weight", 0.75))])
+                    [limiters[lim], schemes[sch], float(db.get("afc_fixed_point_matrix_weight", 0.75)),
+                     float(db.get("afc_fixed_point_derivative_weight", 0.1)),
+                     float(db.get("afc_change_method_threshold", 1e-5)), 1.0 if acc == "yes" else 0.0,
+                     float(db.get("afc_nonlinloop_anderson_acc_vec", 10)),
+                     float(db.get("afc_nonlinloop_anderson_acc_start", 0))])
 
 
 def tcd_desc(db):
@@ -822,9 +830,11 @@ class TimeConvectionDiffusion(ConvectionDiffusion):
 
 class ConvectionDiffusionAFC(ConvectionDiffusion):
     """ConvectionDiffusion_AFC<d> of the reference
-    (src/System/ConvectionDiffusion_AFC.C) with the Kuzmin limiter and the
-    fixed point rhs scheme. `solve()` runs the whole nonlinear loop of the
-    reference's programs (assemble(0), solve(0), solve(k) until converged)."""
+    (src/System/ConvectionDiffusion_AFC.C) with the Kuzmin or BJK17 limiter,
+    the fixed point rhs, fixed point matrix or Newton scheme, and optional
+    Anderson acceleration (afc_params). `solve()` runs the whole nonlinear
+    loop of the reference's programs (assemble(0), solve(0), solve(k) until
+    converged)."""
 
     def __init__(self, db):
         db = dict(db)

@@ -636,8 +636,12 @@ int pmcd_afc_solve(pmcd_system* s, const pmcd_solver_desc* sd, const double* prm
     PMCD_CHECK(s && sd && prm && out, "null argument");
     PMCD_CHECK(prm[0] > 0 && prm[1] >= 1, "afc_nonlinloop_epsilon must be positive, afc_nonlinloop_maxit >= 1");
     PMCD_CHECK(prm[11] == 0.0 || prm[11] == 1.0, "afc limiter must be 0 (kuzmin) or 1 (BJK17)");
-    PMCD_CHECK(prm[12] == 0.0 || prm[12] == 1.0,
-               "afc iteration scheme must be 0 (fixed_point_rhs) or 1 (fixed_point_matrix)");
+    PMCD_CHECK(prm[12] == 0.0 || prm[12] == 1.0 || prm[12] == 2.0,
+               "afc iteration scheme must be 0 (fixed_point_rhs), 1 (fixed_point_matrix) or 2 (newton)");
+    PMCD_CHECK(prm[16] == 0.0 || prm[16] == 1.0, "afc_nonlinloop_anderson_acc must be 0 or 1");
+    PMCD_CHECK(prm[16] == 0.0 || (prm[17] >= 1 && prm[17] <= LinComb::kMax && prm[18] >= 0),
+               "afc_nonlinloop_anderson_acc_vec must be in 1..", LinComb::kMax,
+               ", afc_nonlinloop_anderson_acc_start >= 0");
     const SolveStats st = s->sys->afc_solve(to_solver(*sd), prm, out);
     if(stats) {
       stats->iterations = st.iterations;

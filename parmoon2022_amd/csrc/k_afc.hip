@@ -78,7 +78,7 @@ __device__ inline bool handles(const double* __restrict__ E, int j, int t) { ret
 __global__ void k_afc_limiter(int n, int n_active, const int* __restrict__ rp, const int* __restrict__ col,
                               const int* __restrict__ tpos, const double* __restrict__ D,
                               const double* __restrict__ E, const double* __restrict__ u,
-                              double* __restrict__ Rp, double* __restrict__ Rm) {
+                              double* __restrict__ Rp, double* __restrict__ Rm, AfcLimiterOut lo) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
   const int b = rp[i], e = rp[i + 1];
@@ -110,6 +110,9 @@ __global__ void k_afc_limiter(int n, int n_active, const int* __restrict__ rp, c
     const double Fc = D[t] * (ui - u[col[j]]);
     if(Fc > 0) Qp += Fc;
     if(Fc < 0) Qm += Fc;
+  }
+  if(lo.Pp) {
+    lo.Pp[i] = Pp; lo.Pm[i] = Pm; lo.Qp[i] = Qp; lo.Qm[i] = Qm;
   }
   double rp_ = 1.0, rm_ = 1.0;
   if(fabs(Pp) > 0) { rp_ = Qp / Pp; if(rp_ > 1) rp_ = 1; }
@@ -199,9 +202,9 @@ void launch_afc_matrices(int n, int n_active, const int* rp, const int* col, con
 
 void launch_afc_rhs(int n, int n_active, const int* rp, const int* col, const int* tpos, const double* D,
                     const double* E, const double* u, const double* f, double* Rp, double* Rm, double* alpha,
-                    double* rhs, hipStream_t s) {
+                    double* rhs, hipStream_t s, AfcLimiterOut lo) {
   if(!n) return;
-  k_afc_limiter<<<blocks_for(n), 256, 0, s>>>(n, n_active, rp, col, tpos, D, E, u, Rp, Rm);
+  k_afc_limiter<<<blocks_for(n), 256, 0, s>>>(n, n_active, rp, col, tpos, D, E, u, Rp, Rm, lo);
   k_afc_rhs<<<blocks_for(n), 256, 0, s>>>(n, n_active, rp, col, tpos, D, E, u, Rp, Rm, f, alpha, rhs);
   PMCD_HIP(hipGetLastError());
 }
@@ -233,7 +236,8 @@ __global__ void k_afc_bjk_modify(int n_active, const int* __restrict__ rp, const
 // BJK17 factors R+ and R- of row i (steady_state_algorithm :2200-2270)
 __global__ void k_afc_bjk_limiter(int n, int n_active, const int* __restrict__ rp, const int* __restrict__ col,
                                   const double* __restrict__ D, const double* __restrict__ gamma,
-                                  const double* __restrict__ u, double* __restrict__ Rp, double* __restrict__ Rm) {
+                                  const double* __restrict__ u, double* __restrict__ Rp, double* __restrict__ Rm,
+                                  AfcLimiterOut lo) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
   const int b = rp[i], e = rp[i + 1];
@@ -255,6 +259,10 @@ __global__ void k_afc_bjk_limiter(int n, int n_active, const int* __restrict__ r
     if(F < 0) Pm += F;
   }
   const double Qp = q * (ui - umax), Qm = q * (ui - umin);
+  if(lo.Pp) {
+    lo.Pp[i] = Pp; lo.Pm[i] = Pm; lo.Qp[i] = Qp; lo.Qm[i] = Qm;
+    lo.umax[i] = umax; lo.umin[i] = umin; lo.q[i] = q;
+  }
   double rp_ = 1.0, rm_ = 1.0;
   if(fabs(Pp) > 0) { rp_ = Qp / Pp; if(rp_ > 1) rp_ = 1; }
   if(fabs(Pm) > 0) { rm_ = Qm / Pm; if(rm_ > 1) rm_ = 1; }
@@ -296,9 +304,9 @@ void launch_afc_bjk_modify(int n_active, const int* rp, const int* col, const in
 
 void launch_afc_bjk_rhs(int n, int n_active, const int* rp, const int* col, const double* D, const double* gamma,
                         const double* u, const double* f, double* Rp, double* Rm, double* alpha, double* rhs,
-                        hipStream_t s) {
+                        hipStream_t s, AfcLimiterOut lo) {
   if(!n) return;
-  k_afc_bjk_limiter<<<(n + 255) / 256, 256, 0, s>>>(n, n_active, rp, col, D, gamma, u, Rp, Rm);
+  k_afc_bjk_limiter<<<(n + 255) / 256, 256, 0, s>>>(n, n_active, rp, col, D, gamma, u, Rp, Rm, lo);
   k_afc_bjk_rhs<<<(n + 255) / 256, 256, 0, s>>>(n, n_active, rp, col, D, u, Rp, Rm, f, alpha, rhs);
   PMCD_HIP(hipGetLastError());
 }
@@ -353,6 +361,197 @@ void launch_afc_fpm(int n, int n_active, const int* rp, const int* col, const do
                     const double* alpha, const double* u, const double* f, double tau0, double* S, double* rhs,
                     hipStream_t s) {
   if(n) k_afc_fpm<<<(n + 255) / 256, 256, 0, s>>>(n, n_active, rp, col, D, E, alpha, u, f, tau0, S, rhs);
+  PMCD_HIP(hipGetLastError());
+}
+
+}  // namespace pmcd
+
+namespace pmcd {
+
+namespace {
+
+// flux of entry m of row r: d_rc (u_c - u_r)
+__device__ inline double flux(const AfcNewtonArgs& a, int r, int m) { return a.D[m] * (a.u[a.col[m]] - a.u[r]); }
+
+// position of (r, c) or -1
+__device__ inline int entry_of(const AfcNewtonArgs& a, int r, int c) {
+  const int p = find_in_row(a.rp, a.col, r, c);
+  return p < a.rp[r + 1] ? p : -1;
+}
+
+// sum of d over the entries of row r with F < 0 (sign -1) or F > 0 (+1);
+// `upwind`: only entries whose E is at least the transposed entry's
+__device__ double row_d_sum(const AfcNewtonArgs& a, int r, int sign, bool upwind) {
+  double s = 0;
+  for(int m = a.rp[r]; m < a.rp[r + 1]; ++m) {
+    const double F = flux(a, r, m);
+    if(!(sign > 0 ? F > 0 : F < 0)) continue;
+    if(upwind && !(a.E[m] >= a.E[a.tpos[m]])) continue;
+    s += a.D[m];
+  }
+  return s;
+}
+
+// Compute_Jacobian_times_flux_Kuzmin (AlgebraicFluxCorrection.C:224-578);
+// the reference keeps E_kj in an int in the downwind branch, and so does
+// this code
+__device__ double jac_kuzmin(const AfcNewtonArgs& a, int i, int j, int pij) {
+  const double eps = 1e-14, epsinv = 1.0 / eps;
+  const double *E = a.E, *D = a.D;
+  const int a_ji = entry_of(a, j, i);
+  const double Fij = flux(a, i, pij);
+  double sum = 0;
+  for(int k = a.rp[i]; k < a.rp[i + 1]; ++k) {
+    const int kk = a.col[k];
+    const double Fk = flux(a, i, k);
+    const int a_ki = a.tpos[k];
+    if(E[k] >= E[a_ki]) {
+      if(Fk > 0 && fabs(a.Rp[i] - 1) < eps) continue;
+      if(Fk < 0 && fabs(a.Rm[i] - 1) < eps) continue;
+      if(fabs(Fk) < eps) continue;
+      if(i != j && fabs(D[pij]) < 1e-14) continue;
+      double DQ, DP, val;
+      if(Fk > 0) {
+        DQ = i != j ? (Fij >= 0 ? 0 : -D[pij]) : row_d_sum(a, i, -1, false);
+        DP = i != j ? (Fij <= 0 ? 0 : (E[pij] >= E[a_ji] ? D[pij] : 0)) : -row_d_sum(a, i, +1, true);
+        val = DQ / a.Pp[i] - a.Qp[i] * DP / (a.Pp[i] * a.Pp[i]);
+      } else {
+        DQ = i != j ? (Fij <= 0 ? 0 : -D[pij]) : row_d_sum(a, i, +1, false);
+        DP = i != j ? (Fij >= 0 ? 0 : (E[pij] >= E[a_ji] ? D[pij] : 0)) : -row_d_sum(a, i, -1, true);
+        val = DQ / a.Pm[i] - a.Qm[i] * DP / (a.Pm[i] * a.Pm[i]);
+      }
+      if(fabs(val) < epsinv) sum += val * Fk;
+    } else {
+      if(Fk > 0 && fabs(a.Rm[kk] - 1) < eps) continue;
+      if(Fk < 0 && fabs(a.Rp[kk] - 1) < eps) continue;
+      if(fabs(Fk) < eps) continue;
+      const int a_kj = entry_of(a, kk, j);
+      if(a_kj < 0) continue;
+      const double f_kj = flux(a, kk, a_kj), d_kj = D[a_kj];
+      const int e_kj = (int)E[a_kj];
+      if(kk != j && fabs(d_kj) < 1e-14) continue;
+      const int a_jk = entry_of(a, j, kk);
+      double DQ, DP, val;
+      if(Fk > 0) {
+        DQ = kk != j ? (f_kj <= 0 ? 0 : -d_kj) : row_d_sum(a, kk, +1, false);
+        DP = kk != j ? (f_kj >= 0 ? 0 : (e_kj >= E[a_jk] ? d_kj : 0)) : -row_d_sum(a, kk, -1, true);
+        val = DQ / a.Pm[kk] - a.Qm[kk] * DP / (a.Pm[kk] * a.Pm[kk]);
+      } else {
+        DQ = kk != j ? (f_kj >= 0 ? 0 : -d_kj) : row_d_sum(a, kk, -1, false);
+        DP = kk != j ? (f_kj <= 0 ? 0 : (e_kj >= E[a_jk] ? d_kj : 0)) : -row_d_sum(a, kk, +1, true);
+        val = DQ / a.Pp[kk] - a.Qp[kk] * DP / (a.Pp[kk] * a.Pp[kk]);
+      }
+      if(fabs(val) < epsinv) sum += val * Fk;
+    }
+  }
+  return sum;
+}
+
+// Compute_Jacobian_times_flux_BJK17 (AlgebraicFluxCorrection.C:957-1140)
+__device__ double jac_bjk(const AfcNewtonArgs& a, int i, int j, int pij) {
+  const double eps = 1e-14, epsinv = 1.0 / eps;
+  const double* D = a.D;
+  const double u_j = a.u[j], Fij = flux(a, i, pij);
+  double sum = 0;
+  for(int k = a.rp[i]; k < a.rp[i + 1]; ++k) {
+    const double Fk = flux(a, i, k);
+    if(Fk > 0 && fabs(a.Rp[i] - 1.0) < eps) continue;
+    if(Fk < 0 && fabs(a.Rm[i] - 1.0) < eps) continue;
+    if(fabs(Fk) < eps) continue;
+    const int ck = a.col[k];
+    double DQ = 0.0, DP = 0.0;
+    if(Fk > 0) {
+      if(a.Rp[i] <= a.Rm[ck]) {
+        if(i != j) {
+          if(fabs(a.umax[i] - u_j) < eps) DQ = -a.q[i];
+          if(Fij > 0) DP = D[pij];
+        } else {
+          if(fabs(a.umax[i] - u_j) >= eps) DQ = a.q[i];
+          DP = -row_d_sum(a, i, +1, false);
+        }
+        if(a.Pp[i] > 0.0) {
+          const double val = DQ / a.Pp[i] - a.Qp[i] * DP / (a.Pp[i] * a.Pp[i]);
+          if(fabs(val) < epsinv) sum += val * Fk;
+        }
+      } else {
+        if(ck != j) {
+          if(fabs(a.umin[ck] - u_j) < eps) DQ = -a.q[ck];
+          if(Fij < 0) DP = D[pij];
+        } else {
+          if(fabs(a.umin[j] - u_j) >= eps) DQ = a.q[j];
+          DP = -row_d_sum(a, j, -1, false);
+        }
+        if(a.Pm[ck] < 0.0) {
+          const double val = DQ / a.Pm[ck] - a.Qm[ck] * DP / (a.Pm[ck] * a.Pm[ck]);
+          if(fabs(val) < epsinv) sum += val * Fk;
+        }
+      }
+    } else {
+      if(a.Rm[i] <= a.Rp[ck]) {
+        if(i != j) {
+          if(fabs(a.umin[i] - u_j) < eps) DQ = -a.q[i];
+          if(Fij < 0) DP = D[pij];
+        } else {
+          if(fabs(a.umin[i] - u_j) >= eps) DQ = a.q[i];
+          DP = -row_d_sum(a, i, -1, false);
+        }
+        if(a.Pm[i] < 0.0) {
+          const double val = DQ / a.Pm[i] - a.Qm[i] * DP / (a.Pm[i] * a.Pm[i]);
+          if(fabs(val) < epsinv) sum += val * Fk;
+        }
+      } else {
+        if(ck != j) {
+          if(fabs(a.umax[ck] - u_j) < eps) DQ = -a.q[ck];
+          if(Fij > 0) DP = D[pij];
+        } else {
+          if(fabs(a.umax[j] - u_j) >= eps) DQ = a.q[j];
+          DP = -row_d_sum(a, j, +1, false);
+        }
+        if(a.Pp[ck] > 0.0) {
+          const double val = DQ / a.Pp[ck] - a.Qp[ck] * DP / (a.Pp[ck] * a.Pp[ck]);
+          if(fabs(val) < epsinv) sum += val * Fk;
+        }
+      }
+    }
+  }
+  return sum;
+}
+
+// Newton scheme (steady_state_algorithm :2794-2981), one thread per row:
+// r_i = f_i + sum_j (-E_ij u_j + alpha_ij F_ij) in entry order, then the
+// row of DF (E - tau0 alpha D - tau1 J off the diagonal, E + tau0
+// sum alpha D - tau1 J on it) and r_i += DF_ij u_j in entry order; the
+// system matrix takes DF. Dirichlet rows: identity and u_i.
+__global__ void k_afc_newton(AfcNewtonArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= a.n) return;
+  const int b = a.rp[i], e = a.rp[i + 1];
+  if(i >= a.n_active) {
+    for(int j = b; j < e; ++j) a.S[j] = a.col[j] == i ? 1.0 : 0.0;
+    a.rhs[i] = a.u[i];
+    return;
+  }
+  double r = a.f[i];
+  for(int j = b; j < e; ++j) r += -(a.E[j]) * a.u[a.col[j]] + a.alpha[j] * flux(a, i, j);
+  double sum_flux = 0.0;
+  for(int j = b; j < e; ++j)
+    if(a.col[j] != i) sum_flux += a.alpha[j] * a.D[j];
+  for(int j = b; j < e; ++j) {
+    const int c = a.col[j];
+    double J = 0;
+    if(a.tau1 != 0) J = a.bjk ? jac_bjk(a, i, c, j) : jac_kuzmin(a, i, c, j);
+    const double df = c != i ? a.E[j] - a.tau0 * a.alpha[j] * a.D[j] - a.tau1 * J
+                             : a.E[j] + a.tau0 * sum_flux - a.tau1 * J;
+    a.S[j] = df;
+    r += df * a.u[c];
+  }
+  a.rhs[i] = r;
+}
+
+}  // namespace
+
+void launch_afc_newton(const AfcNewtonArgs& a, hipStream_t s) {
+  if(a.n) k_afc_newton<<<blocks_for(a.n), 256, 0, s>>>(a);
   PMCD_HIP(hipGetLastError());
 }
 

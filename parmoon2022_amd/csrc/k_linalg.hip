@@ -629,6 +629,15 @@ __global__ __launch_bounds__(256) void k_stream_read(long n2, const double2* __r
   if((threadIdx.x & 63) == 0) out[(blockIdx.x * 256 + threadIdx.x) / 64] = s0;
 }
 
+// y = sum_j c_j V_j, the terms added in order j = 0, 1, ...
+__global__ void k_lincomb(int n, LinComb lc, double* __restrict__ y) {
+  for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for(int j = 0; j < lc.k; ++j) s = s + lc.v[j][i] * lc.c[j];
+    y[i] = s;
+  }
+}
+
 __global__ void k_copy(int n, const double* __restrict__ x, double* __restrict__ y) {
   for(int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = x[i];
 }
@@ -1592,6 +1601,11 @@ void launch_sum_max(int n, int stride, const double* v, double* partial, double*
 
 void launch_axpy(int n, double a, const double* x, double* y, hipStream_t s) {
   k_axpy<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, a, x, y);
+}
+void launch_lincomb(int n, const LinComb& lc, double* y, hipStream_t s) {
+  PMCD_CHECK(lc.k >= 1 && lc.k <= LinComb::kMax, "linear combination of 1..", LinComb::kMax, " vectors");
+  if(n) k_lincomb<<<grid_for(n, 256 * 4), 256, 0, s>>>(n, lc, y);
+  PMCD_HIP(hipGetLastError());
 }
 void launch_fill(int n, double v, double* y, hipStream_t s) {
   if(n) k_fill<double><<<grid_for(n, 256 * 4), 256, 0, s>>>(n, v, y);

@@ -219,6 +219,14 @@ void launch_sum_max(int n, int stride, const double* v, double* partial, double*
 
 // vector kernels
 void launch_axpy(int n, double alpha, const double* x, double* y, hipStream_t s);
+// y = sum_{j < k} c[j] v[j] (device vectors of length n)
+struct LinComb {
+  static constexpr int kMax = 32;
+  int k = 0;
+  const double* v[kMax];
+  double c[kMax];
+};
+void launch_lincomb(int n, const LinComb& lc, double* y, hipStream_t s);
 void launch_fill(int n, double v, double* y, hipStream_t s);
 void launch_copy(int n, const double* x, double* y, hipStream_t s);
 // y = x for n doubles (n even, 16-byte aligned), 16-byte vector accesses
@@ -261,12 +269,27 @@ void dense_inverse(int n, int m, double* B, double* inv, double* colbuf, int* pe
 // time-dependent system matrix: S = factor A + M on active rows, A on the
 // Dirichlet rows (scale_blocks_actives + add_matrix_actives)
 // algebraic flux correction (k_afc.hip)
+// the limiter's per-row sums and bounds, kept for the Newton scheme when set
+struct AfcLimiterOut {
+  double *Pp = nullptr, *Pm = nullptr, *Qp = nullptr, *Qm = nullptr;
+  double *umax = nullptr, *umin = nullptr, *q = nullptr;  // BJK17 only
+};
+struct AfcNewtonArgs {
+  int n, n_active;
+  const int *rp, *col, *tpos;
+  const double *D, *E, *alpha, *u, *f;
+  const double *Pp, *Pm, *Qp, *Qm, *Rp, *Rm, *umax, *umin, *q;
+  double tau0, tau1;
+  int bjk;
+  double *S, *rhs;
+};
+void launch_afc_newton(const AfcNewtonArgs& a, hipStream_t s);
 void launch_afc_transpose(int n, const int* rp, const int* col, int* tpos, hipStream_t s);
 void launch_afc_matrices(int n, int n_active, const int* rp, const int* col, const int* tpos, const double* A,
                          double* D, double* E, double* S, hipStream_t s);
 void launch_afc_rhs(int n, int n_active, const int* rp, const int* col, const int* tpos, const double* D,
                     const double* E, const double* u, const double* f, double* Rp, double* Rm, double* alpha,
-                    double* rhs, hipStream_t s);
+                    double* rhs, hipStream_t s, AfcLimiterOut lo = {});
 void launch_afc_damp(int n, int n_active, double omega, const double* old, double* u, hipStream_t s);
 void launch_afc_fpm(int n, int n_active, const int* rp, const int* col, const double* D, const double* E,
                     const double* alpha, const double* u, const double* f, double tau0, double* S, double* rhs,
@@ -275,7 +298,7 @@ void launch_afc_bjk_modify(int n_active, const int* rp, const int* col, const in
                            hipStream_t s);
 void launch_afc_bjk_rhs(int n, int n_active, const int* rp, const int* col, const double* D, const double* gamma,
                         const double* u, const double* f, double* Rp, double* Rm, double* alpha, double* rhs,
-                        hipStream_t s);
+                        hipStream_t s, AfcLimiterOut lo = {});
 void launch_tcd_system(int n, int n_active, const int* rp, const double* A, const double* M, double factor,
                        double* S, hipStream_t s, const int* sell_slot = nullptr,
                        const long long* sell_off = nullptr, double* sell_val = nullptr);

@@ -314,6 +314,9 @@ class CDSystem {
   DevBuf<int> afc_tpos_;
   DevBuf<double> afc_A_, afc_D_, afc_E_, afc_alpha_, afc_f_, afc_Rp_, afc_Rm_, afc_old_, afc_cur_sol_,
       afc_cur_rhs_, afc_res_, afc_gamma_;
+  DevBuf<double> afc_Pp_, afc_Pm_, afc_Qp_, afc_Qm_, afc_umax_, afc_umin_, afc_q_;  // Newton: limiter sums
+  DevBuf<double> afc_hX_, afc_hR_, afc_M_, afc_w_;  // Anderson: outputs, updates, QR work space
+  bool afc_anderson_alpha(int n, int k, const double* const* R, double* alpha);
   // Krylov work space
   std::vector<DevBuf<double>> kv_, kz_;
   DevBuf<double> kr_, kw_, kh_;

@@ -479,36 +479,54 @@ def test_oracle_tcd2d_golden(oracle_lib, disc):
         assert abs(got[m][0] - l2[i]) < 1e-10 and abs(got[m][1] - h1[i]) < 1e-10, (m, got[m][:2], l2[i], h1[i])
 
 
-# test/cd2d_afc_test.c++:36-66,118-131,148-154,217-233 (tolerance 1e-8) and
-# test/cd3d_afc_test.c++:37-51,93-101,123-128 (tolerance 1e-11): Kuzmin and
-# BJK17 limiters, fixed point rhs, no Anderson acceleration; 2D example 3
-# (sharp boundary layer, errors = norms of the solution), 3D example 3
-# (smooth solution, eps 0.1); P1 on TwoTriangles / tetrahedra, Q1 on
-# UnitSquare, 3 refinements
-AFC_GOLDEN = {
-    ("TwoTriangles", "sharp_boundary_layer_2d", "kuzmin"): ([0.53300159222151, 2.2679598767339, 0.40567403714271,
-                                                            0.99999999999964], 1e-8),
-    ("UnitSquare", "sharp_boundary_layer_2d", "kuzmin"): ([0.52464493314744, 2.2075436785016, 0.38778102739911,
-                                                          0.99998511032586], 1e-8),
-    ("Default_UnitCube_Tetra", "smooth_solution_3d", "kuzmin"): ([0.0010369623588627, 0.011894930265006], 1e-11),
-    ("TwoTriangles", "sharp_boundary_layer_2d", "BJK17"): ([0.54553405607575, 2.5382226427268, 0.48949470018912,
-                                                           1.0], 1e-8),
-    ("Default_UnitCube_Tetra", "smooth_solution_3d", "BJK17"): ([0.0004164209086368, 0.010287281106288], 1e-11),
-}
-AFC_GOLDEN = {k + ("fixed_point_rhs",): v for k, v in AFC_GOLDEN.items()}
-# the fixed point matrix scheme (cd2d_afc_test.c++:141-147,164-168;
-# cd3d_afc_test.c++:109-115,130-135); errors of the third and fourth kind are
-# checked only for the fixed point rhs cases above
-AFC_GOLDEN.update({
-    ("TwoTriangles", "sharp_boundary_layer_2d", "kuzmin", "fixed_point_matrix"):
-        ([0.533001591476, 2.2679598714175], 1e-8),
-    ("UnitSquare", "sharp_boundary_layer_2d", "kuzmin", "fixed_point_matrix"): ([0.5246449330415, 2.2075436769812], 1e-8),
-    ("Default_UnitCube_Tetra", "smooth_solution_3d", "kuzmin", "fixed_point_matrix"):
-        ([0.0010369624396964, 0.011894928040666], 1e-11),
-    ("TwoTriangles", "sharp_boundary_layer_2d", "BJK17", "fixed_point_matrix"): ([0.54553405576562, 2.5382226446816], 1e-8),
-    ("Default_UnitCube_Tetra", "smooth_solution_3d", "BJK17", "fixed_point_matrix"):
-        ([0.0004164206690843, 0.010287285812062], 1e-11),
-})
+# test/cd2d_afc_test.c++:36-66,118-210 (tolerance 1e-8; L2, H1-semi, SD and
+# L-infinity errors) and test/cd3d_afc_test.c++:37-51,93-240 (tolerance
+# 1e-11; L2 and H1-semi): Kuzmin and BJK17 limiters, fixed point rhs, fixed
+# point matrix and Newton schemes, without and with Anderson acceleration;
+# 2D example 3 (sharp boundary layer, errors = norms of the solution), 3D
+# example 3 (smooth solution, eps 0.1); P1 on TwoTriangles / tetrahedra, Q1
+# on UnitSquare / hexahedra, 3 refinements. Keys: geometry, example,
+# limiter, scheme, Anderson acceleration.
+_S2, _S3 = "sharp_boundary_layer_2d", "smooth_solution_3d"
+_AFC_ROWS = [
+    ("TwoTriangles", _S2, "kuzmin", "fixed_point_rhs", "no", [0.53300159222151, 2.2679598767339, 0.40567403714271, 0.99999999999964]),
+    ("TwoTriangles", _S2, "kuzmin", "newton", "no", [0.53300159151439, 2.2679598717208, 0.40567403673183, 0.99999999999294]),
+    ("TwoTriangles", _S2, "kuzmin", "fixed_point_matrix", "no", [0.533001591476, 2.2679598714175, 0.40567403666082, 0.99999999999266]),
+    ("TwoTriangles", _S2, "BJK17", "fixed_point_rhs", "no", [0.54553405607575, 2.5382226427268, 0.48949470018912, 1.0]),
+    ("TwoTriangles", _S2, "BJK17", "newton", "no", [0.5455340557437, 2.5382226441468, 0.48949469803866, 1.0]),
+    ("TwoTriangles", _S2, "BJK17", "fixed_point_matrix", "no", [0.54553405576562, 2.5382226446816, 0.48949469801065, 1.0]),
+    ("TwoTriangles", _S2, "kuzmin", "fixed_point_rhs", "yes", [0.53300159242554, 2.2679598769077, 0.40567403742442, 1.0000000009584]),
+    ("TwoTriangles", _S2, "kuzmin", "newton", "yes", [0.53300159208051, 2.2679598729401, 0.4056740371515, 1.0000000000071]),
+    ("TwoTriangles", _S2, "kuzmin", "fixed_point_matrix", "yes", [0.53300159216948, 2.2679598744846, 0.40567403724517, 1.0000000000082]),
+    ("TwoTriangles", _S2, "BJK17", "fixed_point_rhs", "yes", [0.54553405581158, 2.5382226459933, 0.48949469868059, 1.0000000001146]),
+    ("TwoTriangles", _S2, "BJK17", "newton", "yes", [0.54553405625277, 2.5382226482942, 0.48949469963447, 1.0000000000222]),
+    ("TwoTriangles", _S2, "BJK17", "fixed_point_matrix", "yes", [0.54553405606112, 2.5382226468272, 0.48949469950613, 1.0000000000052]),
+    ("UnitSquare", _S2, "kuzmin", "fixed_point_rhs", "no", [0.52464493314744, 2.2075436785016, 0.38778102739911, 0.99998511032586]),
+    ("UnitSquare", _S2, "kuzmin", "newton", "no", [0.52464493317126, 2.2075436776853, 0.38778102729849, 0.99998511033174]),
+    ("UnitSquare", _S2, "kuzmin", "fixed_point_matrix", "no", [0.5246449330415, 2.2075436769812, 0.38778102725893, 0.99998511032616]),
+    ("UnitSquare", _S2, "kuzmin", "fixed_point_rhs", "yes", [0.52464493309035, 2.2075436770111, 0.38778102708239, 0.99998511033224]),
+    ("UnitSquare", _S2, "kuzmin", "newton", "yes", [0.52464493324716, 2.2075436781711, 0.38778102729767, 0.9999851103142]),
+    ("UnitSquare", _S2, "kuzmin", "fixed_point_matrix", "yes", [0.52464493319966, 2.2075436785091, 0.38778102729507, 0.99998511031822]),
+    ("Default_UnitCube_Tetra", _S3, "kuzmin", "fixed_point_rhs", "no", [0.0010369623588627, 0.011894930265006]),
+    ("Default_UnitCube_Tetra", _S3, "kuzmin", "newton", "no", [0.0010369624917919, 0.011894928221049]),
+    ("Default_UnitCube_Tetra", _S3, "kuzmin", "fixed_point_matrix", "no", [0.0010369624396964, 0.011894928040666]),
+    ("Default_UnitCube_Tetra", _S3, "BJK17", "fixed_point_rhs", "no", [0.0004164209086368, 0.010287281106288]),
+    ("Default_UnitCube_Tetra", _S3, "BJK17", "newton", "no", [0.00041642068495158, 0.010287282108791]),
+    ("Default_UnitCube_Tetra", _S3, "BJK17", "fixed_point_matrix", "no", [0.0004164206690843, 0.010287285812062]),
+    ("Default_UnitCube_Tetra", _S3, "kuzmin", "fixed_point_rhs", "yes", [0.0010369627652847, 0.011894929166125]),
+    ("Default_UnitCube_Tetra", _S3, "kuzmin", "newton", "yes", [0.0010369624917919, 0.011894928221049]),
+    ("Default_UnitCube_Tetra", _S3, "kuzmin", "fixed_point_matrix", "yes", [0.0010369624396964, 0.011894928040666]),
+    ("Default_UnitCube_Tetra", _S3, "BJK17", "fixed_point_rhs", "yes", [0.00041642072554058, 0.010287280993875]),
+    ("Default_UnitCube_Tetra", _S3, "BJK17", "newton", "yes", [0.00041642073256592, 0.010287281035637]),
+    ("Default_UnitCube_Tetra", _S3, "BJK17", "fixed_point_matrix", "yes", [0.0004164208403835, 0.01028728162663]),
+    ("Default_UnitCube_Hexa", _S3, "kuzmin", "fixed_point_rhs", "no", [0.00019851552281602, 0.0042946862986406]),
+    ("Default_UnitCube_Hexa", _S3, "kuzmin", "newton", "no", [0.00019851553388223, 0.0042946863306704]),
+    ("Default_UnitCube_Hexa", _S3, "kuzmin", "fixed_point_matrix", "no", [0.00019851557114826, 0.0042946863131265]),
+    ("Default_UnitCube_Hexa", _S3, "kuzmin", "fixed_point_rhs", "yes", [0.00019851553307403, 0.0042946862979766]),
+    ("Default_UnitCube_Hexa", _S3, "kuzmin", "newton", "yes", [0.00019851553697763, 0.004294686306746]),
+    ("Default_UnitCube_Hexa", _S3, "kuzmin", "fixed_point_matrix", "yes", [0.00019851552155721, 0.0042946862932138]),
+]
+AFC_GOLDEN = {r[:5]: (r[5], 1e-11 if r[1] == _S3 else 1e-8) for r in _AFC_ROWS}
 
 
 def afc_db(geo, example, n_refine=3, **extra):
@@ -526,16 +544,62 @@ def afc_errors(e, n):
     return [e[0], e[1], e[2], e[4]][:n]
 
 
-@pytest.mark.parametrize("geo,example,limiter,scheme", list(AFC_GOLDEN))
-def test_oracle_afc_golden(oracle_lib, geo, example, limiter, scheme):
-    db = afc_db(geo, example, afc_limiter=limiter, afc_iteration_scheme=scheme)
+@pytest.mark.parametrize("geo,example,limiter,scheme,anderson", list(AFC_GOLDEN))
+def test_oracle_afc_golden(oracle_lib, geo, example, limiter, scheme, anderson):
+    db = afc_db(geo, example, afc_limiter=limiter, afc_iteration_scheme=scheme, afc_nonlinloop_anderson_acc=anderson)
     o = oracle_lib.Oracle(pm.problem_desc(db))
     it, res, rejected, res0 = o.afc_solve(pm.solver_desc(db), pm.afc_params(db))
-    gold, tol = AFC_GOLDEN[(geo, example, limiter, scheme)]
+    gold, tol = AFC_GOLDEN[(geo, example, limiter, scheme, anderson)]
     got = afc_errors(o.errors(), len(gold))
     assert np.allclose(got, gold, rtol=0, atol=tol), (got, gold)
     # stopped by the reference's criterion: residual < epsilon sqrt(n_dof)
     assert res < 1e-10 * np.sqrt(o.info()["n_dof"]) and it > 1
+
+
+@pytest.mark.parametrize("fault,selector,min_fails", [
+    (3, lambda k: k[3] == "newton", 6),   # Newton without the limiter's Jacobian
+    (4, lambda k: k[4] == "yes", 7),      # Anderson acceleration without its mixing step
+], ids=["newton-jacobian-dropped", "anderson-mixing-skipped"])
+def test_oracle_afc_golden_detects_planted_mistakes(oracle_lib, fault, selector, min_fails):
+    """The golden errors pin the Newton Jacobian and the Anderson step: with
+    either planted out of the oracle, at least this many of the affected
+    cases (the 3D ones, whose bound is 1e-11) miss the reference's errors."""
+    fails = 0
+    oracle_lib.lib().orc_set_fault(fault)
+    try:
+        for key in [k for k in AFC_GOLDEN if selector(k)]:
+            geo, example, limiter, scheme, anderson = key
+            db = afc_db(geo, example, afc_limiter=limiter, afc_iteration_scheme=scheme,
+                        afc_nonlinloop_anderson_acc=anderson)
+            o = oracle_lib.Oracle(pm.problem_desc(db))
+            o.afc_solve(pm.solver_desc(db), pm.afc_params(db))
+            gold, tol = AFC_GOLDEN[key]
+            got = afc_errors(o.errors(), len(gold))
+            fails += not np.allclose(got, gold, rtol=0, atol=tol)
+    finally:
+        oracle_lib.lib().orc_set_fault(0)
+    assert fails >= min_fails, fails
+
+
+def test_oracle_afc_newton_kuzmin_downwind_rounding(oracle_lib):
+    """The reference keeps E_kj in an int in the downwind branch of the
+    Kuzmin Jacobian (AlgebraicFluxCorrection.C:242,432). With that rounding
+    the oracle meets the Q1 hexahedron Newton errors to 1e-15, at the
+    precision the golden digits carry; comparing E_kj unrounded (planted)
+    moves them by about 1e-12."""
+    key = ("Default_UnitCube_Hexa", "smooth_solution_3d", "kuzmin", "newton", "no")
+    db = afc_db(*key[:2], afc_limiter="kuzmin", afc_iteration_scheme="newton")
+    gold = AFC_GOLDEN[key][0]
+    diffs = []
+    for fault in (0, 5):
+        oracle_lib.lib().orc_set_fault(fault)
+        try:
+            o = oracle_lib.Oracle(pm.problem_desc(db))
+            o.afc_solve(pm.solver_desc(db), pm.afc_params(db))
+            diffs.append(np.max(np.abs(np.array(afc_errors(o.errors(), 2)) - gold)))
+        finally:
+            oracle_lib.lib().orc_set_fault(0)
+    assert diffs[0] < 1e-15 < 1e-13 < diffs[1], diffs
 
 
 def test_oracle_afc_fixed_point_and_bounds(oracle_lib):

@@ -52,6 +52,12 @@ def lib():
                      "orc_assemble", "orc_matrix", "orc_rhs", "orc_solution", "orc_set_solution",
                      "orc_apply", "orc_solve", "orc_vcycle", "orc_errors"]:
             getattr(L, name).restype = None if name != "orc_solve" else ctypes.c_int
+        L.orc_create_adaptive.restype = ctypes.c_void_p
+        L.orc_create_adaptive.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_double,
+                                          ctypes.c_double]
+        L.orc_hanging.restype = ctypes.c_int
         L.orc_tcd_init.restype = ctypes.c_int
         L.orc_tcd_init.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_tcd_step.restype = ctypes.c_int
@@ -84,10 +90,27 @@ class Oracle:
             raise ValueError("the oracle builds default and structured meshes only, not mesh files")
         b = (ctypes.c_double * 3)(*list(desc.b))
         self.desc = desc
-        self._p = ctypes.c_void_p(lib().orc_create(desc.geometry, desc.structured_n, desc.n_refine,
-                                                   desc.order, desc.supg,
-                                                   desc.linear_quad, desc.mg_levels, desc.example,
-                                                   desc.eps, b, desc.c, desc.f0))
+        adapt = int(getattr(desc, "adapt_steps", 0))
+        if adapt > 0:
+            # the ProblemDesc's indicator callback (a C function pointer) and context
+            fn = ctypes.cast(desc.indicator, ctypes.c_void_p)
+            p = lib().orc_create_adaptive(desc.geometry, desc.n_refine, adapt, fn, desc.indicator_ctx, desc.order,
+                                          desc.supg, desc.linear_quad, desc.example, desc.eps, b, desc.c, desc.f0)
+            if not p:
+                raise ValueError("adaptive meshes take P1/Q1 on the 2D default meshes")
+            self._p = ctypes.c_void_p(p)
+        else:
+            self._p = ctypes.c_void_p(lib().orc_create(desc.geometry, desc.structured_n, desc.n_refine,
+                                                       desc.order, desc.supg,
+                                                       desc.linear_quad, desc.mg_levels, desc.example,
+                                                       desc.eps, b, desc.c, desc.f0))
+
+    def hanging(self):
+        """(hanging DOF, coupling DOF, coupling DOF) rows in the sorted order."""
+        n = lib().orc_hanging(self._p, None)
+        out = np.zeros(3 * max(n, 1), dtype=np.int32)
+        lib().orc_hanging(self._p, _p(out, ctypes.c_int))
+        return out[:3 * n].reshape(n, 3)
 
     def __del__(self):
         if getattr(self, "_p", None):

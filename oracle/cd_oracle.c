@@ -59,6 +59,18 @@ static int jv(int s, int j, int k) {
   }
 }
 
+/* Planted mistakes for the tests that show a check can fail (0: none).
+   1: the SUPG -eps*Laplacian term dropped; 2: the mixed second reference
+   derivatives dropped from the physical Hessian; 3: the Jacobian of the
+   AFC Newton scheme dropped; 4: the Anderson mixing skipped; 5: the
+   downwind branch of the Kuzmin Jacobian compares E_kj unrounded; 6: the
+   hanging coupling DOFs that hang themselves receive nothing; 7: hanging
+   rows added in list order instead of the sorted order; 8: the 1-irregular
+   closure of the refinement by indicator skipped. Set only by tests, through
+   orc_set_fault. */
+static int orc_fault = 0;
+void orc_set_fault(int code) { orc_fault = code; }
+
 /* ----------------------------------------------------------------------- */
 /* mesh                                                                     */
 /* ----------------------------------------------------------------------- */
@@ -106,6 +118,17 @@ static int vertex_id(omesh* m, vdict* d, const int* key, const double* x) {
   d->slot[h] = v;
   memcpy(d->key[h], key, sizeof(int) * 4);
   return v;
+}
+
+/* id stored for `key`, or -1 */
+static int vertex_find(const vdict* d, const int* key) {
+  uint64_t h = hash4(key) & (uint64_t)(d->cap - 1);
+  for(;;) {
+    int v = d->slot[h];
+    if(v < 0) return -1;
+    if(memcmp(d->key[h], key, sizeof(int) * 4) == 0) return v;
+    h = (h + 1) & (uint64_t)(d->cap - 1);
+  }
 }
 
 static void mesh_free(omesh* m) {
@@ -293,6 +316,123 @@ static void refine(const omesh* cm, omesh* fm) {
   }
   free(d.slot);
   free(d.key);
+}
+
+/* ----------------------------------------------------------------------- */
+/* adaptive refinement with hanging vertices (2D)                           */
+/* ----------------------------------------------------------------------- */
+/* TDomain::RefineByIndicator with conforming_closure = false
+   (src/Geometry/Domain.C:1094-1207, RefinementStrategy.C:260-287,
+   GridCell.C:348-385): cells with corner values of the indicator on both
+   sides of zero (|value| <= 1e-10 counts for both) are refined regularly,
+   then cells next to a neighbour refined two levels deeper are refined
+   until the mesh is 1-irregular. The leaves stay in depth-first tree order
+   (a refined cell is replaced by its children in child order). Edge
+   midpoints are kept in `mids` across steps, keyed by the sorted ids of the
+   edge's end vertices, so that a cell refined later reuses the midpoint its
+   neighbour created: a midpoint present on a leaf's edge is a hanging
+   vertex of that leaf. */
+typedef struct {
+  double (*fn)(const double* x, void* ctx); /* indicator at a vertex */
+  void* ctx;
+} oindicator;
+
+static int edge_mid(const vdict* mids, int u, int v) {
+  int key[4] = {u < v ? u : v, u < v ? v : u, -1, -1};
+  return vertex_find(mids, key);
+}
+
+static void refine_marked(const omesh* cm, const char* mark, omesh* fm, vdict* mids, int* center_count) {
+  reftab t = make_reftab(cm->shape);
+  const int nv = NV[cm->shape];
+  int nref = 0;
+  for(int c = 0; c < cm->ncells; ++c) nref += mark[c] != 0;
+  memset(fm, 0, sizeof(*fm));
+  fm->shape = cm->shape;
+  fm->ncells = cm->ncells + nref * (t.nchild - 1);
+  fm->cv = malloc(sizeof(int) * fm->ncells * nv);
+  fm->parent = malloc(sizeof(int) * fm->ncells);
+  fm->cap = cm->nverts + nref * 8 + 64;
+  fm->X = malloc(sizeof(double) * 3 * fm->cap);
+  memcpy(fm->X, cm->X, sizeof(double) * 3 * cm->nverts);
+  fm->nverts = cm->nverts;
+  int fc = 0;
+  for(int c = 0; c < cm->ncells; ++c) {
+    if(!mark[c]) {
+      memcpy(&fm->cv[fc * nv], &cm->cv[c * nv], sizeof(int) * nv);
+      fm->parent[fc++] = c;
+      continue;
+    }
+    int gid[27];
+    for(int r = 0; r < t.nref; ++r) {
+      const int n = t.span[r][0];
+      if(n == 1) { gid[r] = cm->cv[c * nv + t.span[r][1]]; continue; }
+      double x[3] = {0, 0, 0};
+      for(int i = 0; i < n; ++i)
+        for(int k = 0; k < 3; ++k) x[k] += cm->X[3 * cm->cv[c * nv + t.span[r][1 + i]] + k];
+      for(int k = 0; k < 3; ++k) x[k] /= n;
+      int key[4] = {-1, -1, -1, -1};
+      if(n == 2) {
+        const int u = cm->cv[c * nv + t.span[r][1]], v = cm->cv[c * nv + t.span[r][2]];
+        key[0] = u < v ? u : v;
+        key[1] = u < v ? v : u;
+      } else {
+        key[0] = -2 - (*center_count)++; /* cell-interior vertex: never shared */
+      }
+      gid[r] = vertex_id(fm, mids, key, x);
+    }
+    for(int ch = 0; ch < t.nchild; ++ch) {
+      for(int a = 0; a < nv; ++a) fm->cv[fc * nv + a] = gid[t.child[ch][a]];
+      fm->parent[fc++] = c;
+    }
+  }
+}
+
+/* the leaves after `steps` indicator refinements of `base` (consumed);
+   mids keeps the edge midpoints */
+static void adapt_mesh(omesh* base, const oindicator* ind, int steps, omesh* out, vdict* mids) {
+  mids->cap = 1 << 20;
+  mids->slot = malloc(sizeof(int) * mids->cap);
+  mids->key = malloc(sizeof(int[4]) * mids->cap);
+  for(int i = 0; i < mids->cap; ++i) mids->slot[i] = -1;
+  int centers = 0;
+  omesh cur = *base;
+  const int nv = NV[cur.shape];
+  for(int step = 0; step < steps; ++step) {
+    char* mark = calloc(cur.ncells, 1);
+    for(int c = 0; c < cur.ncells; ++c) {
+      int inner = 0, outer = 0;
+      for(int a = 0; a < nv; ++a) {
+        const double val = ind->fn(&cur.X[3 * cur.cv[c * nv + a]], ind->ctx);
+        if(val <= 1e-10) inner++;
+        if(val >= -1e-10) outer++;
+      }
+      mark[c] = inner > 0 && outer > 0;
+    }
+    for(;;) {
+      omesh next;
+      refine_marked(&cur, mark, &next, mids, &centers);
+      mesh_free(&cur);
+      free(mark);
+      cur = next;
+      /* 1-irregular closure: a leaf whose edge (u, v) has a midpoint m with
+         (u, m) or (m, v) split again is refined */
+      mark = calloc(cur.ncells, 1);
+      int any = 0;
+      for(int c = 0; c < cur.ncells; ++c)
+        for(int j = 0; j < nv; ++j) {
+          const int u = cur.cv[c * nv + j], v = cur.cv[c * nv + (j + 1) % nv];
+          const int mm = edge_mid(mids, u, v);
+          if(mm >= 0 && (edge_mid(mids, u, mm) >= 0 || edge_mid(mids, mm, v) >= 0)) {
+            mark[c] = 1;
+            any = 1;
+          }
+        }
+      if(!any || orc_fault == 8) break;
+    }
+    free(mark);
+  }
+  *out = cur;
 }
 
 /* boundary component of a joint of the unit box from its vertices
@@ -753,10 +893,16 @@ typedef struct {
   double* qq;   /* per child number: coarse basis at the child's nodes */
   int qq_ready[8];
   double* diag; /* diagonal entries (Iteration_jacobi's copy) */
-  int* nb_cell; /* face neighbours: nb_cell[c*NJ+j], -1 boundary */
+  int* nb_cell; /* face neighbours: nb_cell[c*NJ+j], -1 boundary, -2 interior
+                   joint without a conforming neighbour (hanging vertex) */
   int* nb_joint;
   int* comp;    /* boundary component per joint, -1 interior */
   double *aval, *mval; /* time-dependent problems: stiffness and mass matrices */
+  /* hanging vertices: DOFs nactive_nh .. nactive-1, per hanging DOF its two
+     coupling DOFs (weights 1/2), and the topological order of the list */
+  int nhang, nactive_nh;
+  int* hang_cpl;
+  int* hang_order;
 } olevel;
 
 typedef struct orc_problem {
@@ -822,6 +968,7 @@ static void build_neighbours(olevel* L) {
       double xs[12];
       for(int k = 0; k < len; ++k) memcpy(&xs[3 * k], &m->X[3 * m->cv[c * nv + jv(s, j, k)]], 24);
       L->comp[c * nj + j] = box_comp(DIM[s], xs, len);
+      if(L->comp[c * nj + j] < 0) L->nb_cell[c * nj + j] = -2;
     }
 }
 
@@ -1019,7 +1166,7 @@ static void build_space(orc_problem* P, olevel* L) {
   int BUB[NBT] = {0};
   for(int c = 0; c < m->ncells; ++c)
     for(int j = 0; j < nj; ++j)
-      if(L->nb_cell[c * nj + j] < 0) BUB[ex_bc(&P->ex, L->comp[c * nj + j])] += njd[j];
+      if(L->nb_cell[c * nj + j] == -1) BUB[ex_bc(&P->ex, L->comp[c * nj + j])] += njd[j];
   int Counters[NBT + 1];
   Counters[0] = OR_FIRSTMARK + 1;
   for(int k = 0; k < NBT; ++k) Counters[k + 1] = Counters[k] - BUB[k];
@@ -1153,6 +1300,209 @@ static void build_structure(olevel* L) {
   free(buf); free(cells); free(fill); free(cnt);
 }
 
+/* ---- spaces on meshes with hanging vertices (P1/Q1) ----
+   TFESpace::compute_various_numbers (src/FE/FESpace.C:1863-2099) orders the
+   DOFs as inner, non-Dirichlet boundary kinds, hanging, Dirichlet; a DOF on
+   boundary joints of several kinds takes the lowest kind (Dirichlet first,
+   as map_single_boundary_dof keeps the later, larger mark). A P1/Q1 DOF sits
+   at a vertex, so DOFs are identified through vertex ids; within a class
+   they are numbered in first appearance over (cell, local DOF), hanging DOFs
+   in the order their coarse edges are met. A vertex is hanging when it is
+   the midpoint of a leaf's edge (FE2DMapper1Reg.C:43-58,
+   C1_2_C1_2_1Reg: one hanging DOF per refined edge, coupled to the edge's
+   end DOFs with HN_C_P1_2D_0 = {1/2, 1/2}, HNDesc.C:9-11).
+   sort_hanging_nodes (FESpace.C:2101-2160): depth-first post-order over the
+   coupling DOFs that are hanging themselves, then reversed, so that a
+   hanging DOF comes before the hanging DOFs it couples with. */
+static void hang_visit(const olevel* L, int h, char* seen, int* out, int* n) {
+  seen[h] = 1;
+  for(int k = 0; k < 2; ++k) {
+    const int c = L->hang_cpl[2 * h + k] - L->nactive_nh;
+    if(c >= 0 && c < L->nhang && !seen[c]) hang_visit(L, c, seen, out, n);
+  }
+  out[(*n)++] = h;
+}
+
+static int build_space_hanging(orc_problem* P, olevel* L, const vdict* mids) {
+  const omesh* m = &L->m;
+  const int s = m->shape, nj = NJ[s], nv = NV[s], nl = P->el.nl;
+  if(nl != nv) return -1;
+  L->nl = nl;
+  /* local DOF i sits at cell vertex lv[i] (the same for every cell) */
+  int lv[MAXNL];
+  {
+    ogeo g;
+    geo(m, 0, &g);
+    for(int i = 0; i < nl; ++i) {
+      double x[3];
+      geo_point(&g, P->el.dim, P->el.node[i], x);
+      lv[i] = -1;
+      for(int a = 0; a < nv; ++a) {
+        const double* X = &m->X[3 * m->cv[a]];
+        if(fabs(X[0] - x[0]) + fabs(X[1] - x[1]) + fabs(X[2] - x[2]) < 1e-12) lv[i] = a;
+      }
+      if(lv[i] < 0) return -1;
+    }
+  }
+  const int NVt = m->nverts;
+  int* kind = malloc(sizeof(int) * NVt); /* NBT + 1 inner, NBT hanging, else boundary kind */
+  for(int v = 0; v < NVt; ++v) kind[v] = NBT + 1;
+  for(int c = 0; c < m->ncells; ++c)
+    for(int j = 0; j < nj; ++j) {
+      if(L->nb_cell[c * nj + j] != -1) continue;
+      const int k = ex_bc(&P->ex, L->comp[c * nj + j]);
+      for(int a = 0; a < 2; ++a) {
+        const int v = m->cv[c * nv + jv(s, j, a)];
+        if(k < kind[v]) kind[v] = k;
+      }
+    }
+  /* hanging vertices in the order their coarse edges are met */
+  int* hv = malloc(sizeof(int) * 3 * (NVt + 1));
+  int nh = 0;
+  for(int c = 0; c < m->ncells; ++c)
+    for(int j = 0; j < nj; ++j) {
+      if(L->nb_cell[c * nj + j] != -2) continue;
+      const int u = m->cv[c * nv + jv(s, j, 0)], w = m->cv[c * nv + jv(s, j, 1)];
+      const int mm = edge_mid(mids, u, w);
+      if(mm < 0 || kind[mm] == NBT) continue;
+      kind[mm] = NBT;
+      hv[3 * nh] = mm; hv[3 * nh + 1] = u; hv[3 * nh + 2] = w;
+      nh++;
+    }
+  int* num = malloc(sizeof(int) * NVt);
+  for(int v = 0; v < NVt; ++v) num[v] = -1;
+  int next = 0;
+  /* inner, then non-Dirichlet boundary kinds 1..NBT-1 */
+  for(int o = 0; o < NBT; ++o) {
+    const int want = o == 0 ? NBT + 1 : o;
+    for(int c = 0; c < m->ncells; ++c)
+      for(int i = 0; i < nl; ++i) {
+        const int v = m->cv[c * nv + lv[i]];
+        if(kind[v] == want && num[v] < 0) num[v] = next++;
+      }
+  }
+  L->nactive_nh = next;
+  for(int h = 0; h < nh; ++h) num[hv[3 * h]] = next++;
+  L->nactive = next;
+  for(int c = 0; c < m->ncells; ++c)
+    for(int i = 0; i < nl; ++i) {
+      const int v = m->cv[c * nv + lv[i]];
+      if(kind[v] == 0 && num[v] < 0) num[v] = next++;
+    }
+  L->ndof = next;
+  L->ndir = L->ndof - L->nactive;
+  L->dof = malloc(sizeof(int) * m->ncells * nl);
+  for(int c = 0; c < m->ncells; ++c)
+    for(int i = 0; i < nl; ++i) L->dof[c * nl + i] = num[m->cv[c * nv + lv[i]]];
+  L->nhang = nh;
+  L->hang_cpl = malloc(sizeof(int) * 2 * (nh + 1));
+  for(int h = 0; h < nh; ++h) {
+    L->hang_cpl[2 * h] = num[hv[3 * h + 1]];
+    L->hang_cpl[2 * h + 1] = num[hv[3 * h + 2]];
+  }
+  L->hang_order = malloc(sizeof(int) * (nh + 1));
+  char* seen = calloc(nh + 1, 1);
+  int n = 0;
+  for(int h = 0; h < nh; ++h)
+    if(!seen[h]) hang_visit(L, h, seen, L->hang_order, &n);
+  for(int a = 0, b = n - 1; a < b; ++a, --b) { int t = L->hang_order[a]; L->hang_order[a] = L->hang_order[b]; L->hang_order[b] = t; }
+  free(seen); free(num); free(hv); free(kind);
+  return 0;
+}
+
+/* rows of the coupling DOFs take the columns of the hanging rows added to
+   them (ModifyMatrixAccordingToCoupling needs every entry it adds,
+   src/Matrix/FEMatrix.C:244-296; the pattern of Structure.C:75-230 holds
+   them); entries the reference pattern may hold beyond these are zero */
+static void hanging_structure(olevel* L) {
+  const int n = L->ndof;
+  int** rows = malloc(sizeof(int*) * n);
+  int* len = malloc(sizeof(int) * n);
+  for(int r = 0; r < n; ++r) {
+    len[r] = L->rp[r + 1] - L->rp[r];
+    rows[r] = malloc(sizeof(int) * len[r]);
+    memcpy(rows[r], &L->col[L->rp[r]], sizeof(int) * len[r]);
+  }
+  for(int t = 0; t < L->nhang; ++t) {
+    const int h = L->hang_order[t], hd = L->nactive_nh + h;
+    for(int k = 0; k < 2; ++k) {
+      const int c = L->hang_cpl[2 * h + k];
+      if(c >= L->nactive) continue;
+      int* merged = malloc(sizeof(int) * (len[c] + len[hd]));
+      int a = 0, b = 0, u = 0;
+      while(a < len[c] || b < len[hd]) {
+        int v;
+        if(b >= len[hd] || (a < len[c] && rows[c][a] < rows[hd][b])) v = rows[c][a++];
+        else if(a >= len[c] || rows[hd][b] < rows[c][a]) v = rows[hd][b++];
+        else { v = rows[c][a++]; b++; }
+        merged[u++] = v;
+      }
+      free(rows[c]);
+      rows[c] = merged;
+      len[c] = u;
+    }
+  }
+  long total = 0;
+  for(int r = 0; r < n; ++r) total += len[r];
+  free(L->col);
+  L->col = malloc(sizeof(int) * total);
+  L->rp[0] = 0;
+  for(int r = 0; r < n; ++r) {
+    memcpy(&L->col[L->rp[r]], rows[r], sizeof(int) * len[r]);
+    L->rp[r + 1] = L->rp[r] + len[r];
+    free(rows[r]);
+  }
+  free(rows); free(len);
+  L->nnz = total;
+  free(L->val);
+  L->val = calloc(total, sizeof(double));
+}
+
+/* Assemble2D.C:3350-3427 after the cell loop: for every hanging DOF in the
+   sorted order, its row is added with weight 1/2 to the rows of its active
+   (hanging included) coupling DOFs (ModifyMatrixAccordingToCoupling, with
+   bound n_active), so is its rhs entry, then the hanging rhs entries are 0
+   and the hanging rows become u_h - u_a/2 - u_b/2 = 0 (correct_hanging_rows,
+   FEMatrix.C:384-418) */
+static double* entry_at(const olevel* L, int r, int col) {
+  for(int e = L->rp[r]; e < L->rp[r + 1]; ++e)
+    if(L->col[e] == col) return &L->val[e];
+  return NULL;
+}
+
+static void apply_hanging(olevel* L) {
+  /* planted mistakes: 6 = hanging coupling DOFs receive nothing (bound
+     n_active_non_hanging, as Assembler4 uses), 7 = list order, not sorted */
+  const int bound = orc_fault == 6 ? L->nactive_nh : L->nactive;
+  for(int t = 0; t < L->nhang; ++t) {
+    const int h = orc_fault == 7 ? t : L->hang_order[t], hd = L->nactive_nh + h;
+    for(int k = 0; k < 2; ++k) {
+      const int c = L->hang_cpl[2 * h + k];
+      if(c >= bound) continue;
+      for(int e = L->rp[hd]; e < L->rp[hd + 1]; ++e) {
+        double* dst = entry_at(L, c, L->col[e]);
+        if(!dst) { fprintf(stderr, "oracle: hanging entry missing\n"); continue; }
+        *dst += 0.5 * L->val[e];
+      }
+    }
+  }
+  for(int t = 0; t < L->nhang; ++t) {
+    const int h = orc_fault == 7 ? t : L->hang_order[t], hd = L->nactive_nh + h;
+    for(int k = 0; k < 2; ++k) {
+      const int c = L->hang_cpl[2 * h + k];
+      if(c < bound) L->rhs[c] += 0.5 * L->rhs[hd];
+    }
+  }
+  for(int h = 0; h < L->nhang; ++h) L->rhs[L->nactive_nh + h] = 0;
+  for(int h = 0; h < L->nhang; ++h) {
+    const int hd = L->nactive_nh + h;
+    for(int e = L->rp[hd]; e < L->rp[hd + 1]; ++e) {
+      const int col = L->col[e];
+      L->val[e] = col == hd ? 1.0 : (col == L->hang_cpl[2 * h] || col == L->hang_cpl[2 * h + 1] ? -0.5 : 0.0);
+    }
+  }
+}
+
 /* ----------------------------------------------------------------------- */
 /* assembly                                                                 */
 /* ----------------------------------------------------------------------- */
@@ -1170,14 +1520,6 @@ static double sdfem_delta(int dim, double hK, double eps, double b1, double b2, 
   return delta * 1.0;
 }
 
-/* Planted mistakes for the tests that show a check can fail (0: none).
-   1: the SUPG -eps*Laplacian term dropped; 2: the mixed second reference
-   derivatives dropped from the physical Hessian; 3: the Jacobian of the
-   AFC Newton scheme dropped; 4: the Anderson mixing skipped; 5: the
-   downwind branch of the Kuzmin Jacobian compares E_kj unrounded. Set only
-   by tests, through orc_set_fault. */
-static int orc_fault = 0;
-void orc_set_fault(int code) { orc_fault = code; }
 
 /* physical values at quad point: u, ux, uy, uz, uxx, uyy, uzz for all i */
 static void orig_values(const orc_problem* P, const ogeo* g, const double* ref, double (*o)[MAXNL]) {
@@ -1344,7 +1686,7 @@ static void assemble_level(orc_problem* P, olevel* L, int c_begin, int c_end) {
       if(DOF[mm] < L->nactive) L->rhs[DOF[mm]] += F[mm];
     /* boundary joints: Dirichlet values, Neumann integrals (2D) */
     for(int j = 0; j < nj; ++j) {
-      if(L->nb_cell[c * nj + j] >= 0) continue;
+      if(L->nb_cell[c * nj + j] != -1) continue;
       const int comp = L->comp[c * nj + j];
       int jd[MAXNL];
       const int n = joint_dofs(P, j, jd);
@@ -1979,6 +2321,70 @@ orc_problem* orc_create(int geometry, int structured_n, int n_refine, int order,
   return P;
 }
 
+void orc_destroy(orc_problem* P);
+
+/* a problem on the leaves of `adapt_steps` indicator refinements
+   (TDomain::RefineByIndicator without conforming closure) of the coarse
+   mesh refined `n_refine` times: one level, P1 or Q1 only (NULL otherwise);
+   indicator(x, ctx) is evaluated at the cells' vertices */
+orc_problem* orc_create_adaptive(int geometry, int n_refine, int adapt_steps,
+                                 double (*indicator)(const double*, void*), void* ctx, int order,
+                                 int supg, int linear_quad, int example, double eps, const double* b, double c,
+                                 double f0) {
+  omesh cur;
+  default_mesh(geometry, &cur);
+  if(order != 1 || DIM[cur.shape] != 2) { mesh_free(&cur); return NULL; }
+  for(int i = 0; i < n_refine; ++i) {
+    omesh next;
+    refine(&cur, &next);
+    mesh_free(&cur);
+    cur = next;
+  }
+  oindicator ind = {indicator, ctx};
+  omesh leaves;
+  vdict mids;
+  adapt_mesh(&cur, &ind, adapt_steps, &leaves, &mids);
+  orc_problem* P = calloc(1, sizeof(orc_problem));
+  P->nlev = 1;
+  P->lev = calloc(1, sizeof(olevel));
+  P->el = make_elem(leaves.shape, order);
+  P->qd = make_quad(leaves.shape, (linear_quad ? 2 : 3) * order);
+  P->ex.kind = example;
+  P->ex.eps = eps;
+  for(int k = 0; k < 3; ++k) P->ex.b[k] = b ? b[k] : 0.0;
+  P->ex.c = c;
+  P->ex.f0 = f0;
+  P->supg = supg;
+  olevel* L = &P->lev[0];
+  L->m = leaves;
+  build_neighbours(L);
+  const int rc = build_space_hanging(P, L, &mids);
+  free(mids.slot);
+  free(mids.key);
+  if(rc) { orc_destroy(P); return NULL; }
+  build_structure(L);
+  hanging_structure(L);
+  L->mg_sol = calloc(L->ndof, sizeof(double));
+  L->mg_rhs = calloc(L->ndof, sizeof(double));
+  L->mg_def = calloc(L->ndof, sizeof(double));
+  L->mg_aux = calloc(L->ndof, sizeof(double));
+  return P;
+}
+
+/* hanging DOFs: out[3h..3h+2] = hanging DOF and its two coupling DOFs, in
+   the sorted order; returns the count */
+int orc_hanging(orc_problem* P, int* out) {
+  olevel* L = &P->lev[P->nlev - 1];
+  if(out)
+    for(int t = 0; t < L->nhang; ++t) {
+      const int h = L->hang_order[t];
+      out[3 * t] = L->nactive_nh + h;
+      out[3 * t + 1] = L->hang_cpl[2 * h];
+      out[3 * t + 2] = L->hang_cpl[2 * h + 1];
+    }
+  return L->nhang;
+}
+
 void orc_destroy(orc_problem* P) {
   if(!P) return;
   for(int l = 0; l < P->nlev; ++l) {
@@ -1987,7 +2393,7 @@ void orc_destroy(orc_problem* P) {
     free(L->dof); free(L->rp); free(L->col); free(L->val); free(L->rhs); free(L->sol);
     free(L->mg_sol); free(L->mg_rhs); free(L->mg_def); free(L->mg_aux); free(L->lu); free(L->piv);
     free(L->nb_cell); free(L->nb_joint); free(L->comp); free(L->qq); free(L->diag);
-    free(L->aval); free(L->mval);
+    free(L->aval); free(L->mval); free(L->hang_cpl); free(L->hang_order);
   }
   free(P->u_old); free(P->u_old2); free(P->f_old);
   free(P->lev);
@@ -2022,7 +2428,11 @@ static void extract_diag(olevel* L) {
 }
 
 void orc_assemble(orc_problem* P) {
-  for(int l = 0; l < P->nlev; ++l) { assemble_level(P, &P->lev[l], 0, P->lev[l].m.ncells); extract_diag(&P->lev[l]); }
+  for(int l = 0; l < P->nlev; ++l) {
+    assemble_level(P, &P->lev[l], 0, P->lev[l].m.ncells);
+    if(P->lev[l].nhang) apply_hanging(&P->lev[l]);
+    extract_diag(&P->lev[l]);
+  }
   olevel* F = lvl(P, -1);
   memset(F->sol, 0, sizeof(double) * F->nactive);
   memcpy(F->sol + F->nactive, F->rhs + F->nactive, sizeof(double) * (F->ndof - F->nactive));

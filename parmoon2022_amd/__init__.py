@@ -39,7 +39,12 @@ class ProblemDesc(ctypes.Structure):
                 ("supg", ctypes.c_int), ("linear_quad", ctypes.c_int),
                 ("mg_levels", ctypes.c_int), ("example", ctypes.c_int),
                 ("eps", ctypes.c_double), ("b", ctypes.c_double * 3),
-                ("c", ctypes.c_double), ("f0", ctypes.c_double), ("mesh_file", ctypes.c_char_p)]
+                ("c", ctypes.c_double), ("f0", ctypes.c_double), ("mesh_file", ctypes.c_char_p),
+                ("adapt_steps", ctypes.c_int), ("indicator", ctypes.c_void_p), ("indicator_ctx", ctypes.c_void_p)]
+
+
+# double (*)(const double* x, void* ctx): indicator of the refinement by indicator
+INDICATOR_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
 
 
 class SolverDesc(ctypes.Structure):
@@ -63,7 +68,7 @@ class SolveStats(ctypes.Structure):
 EXPORTS = [
     "pmcd_last_error", "pmcd_device_count", "pmcd_mem_info", "pmcd_system_set_data",
     "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
-    "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_sweep_order", "pmcd_host_sell_layout",
+    "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_hanging", "pmcd_get_hanging", "pmcd_host_sweep_order", "pmcd_host_sell_layout",
     "pmcd_host_write_mesh", "pmcd_cycle_directions", "pmcd_vector_write", "pmcd_vector_read",
     "pmcd_vector_write_market", "pmcd_matrix_write_market", "pmcd_tcd_init", "pmcd_tcd_step", "pmcd_afc_solve",
     "pmcd_jacobi_sweep",
@@ -278,7 +283,38 @@ def problem_desc(db):
         d.b[k] = float(b[k]) if k < len(b) else 0.0
     d.c = float(db.get("reaction", 0.0))
     d.f0 = float(db.get("rhs_constant", 1.0))
+    steps = int(db.get("refine_by_indicator_steps", 0))
+    if steps > 0:
+        # test/adaptive_hanging_test.c++:120-135: RefineByIndicator(indicator)
+        # steps times on a domain with conforming_closure false
+        if db.get("conforming_closure", True):
+            raise ValueError("refinement by indicator is implemented with hanging vertices only: "
+                             "set conforming_closure to False")
+        d.adapt_steps = steps
+        d.indicator = ctypes.cast(indicator_callback(db["refine_indicator"]), ctypes.c_void_p)
+        d.indicator_ctx = None
     return d
+
+
+_INDICATORS = {}
+
+
+def indicator_callback(f):
+    """C callback for an indicator: a callable f(x, y, z) -> float, or the
+    coefficients (a, b, c, e) of a x^2 + b y + c z + e. Callbacks are kept for
+    the life of the process (a served request may call one later, on a server
+    thread), and the same indicator gives the same callback, so requests on
+    one adaptive mesh share resident systems."""
+    key = tuple(float(v) for v in f) if isinstance(f, (tuple, list)) else f
+    cb = _INDICATORS.get(key)
+    if cb is None:
+        if isinstance(key, tuple):
+            a, b, c, e = key
+            fn = lambda x, y, z: a * x * x + b * y + c * z + e  # noqa: E731
+        else:
+            fn = f
+        cb = _INDICATORS[key] = INDICATOR_FN(lambda p, ctx: float(fn(p[0], p[1], p[2])))
+    return cb
 
 
 def solver_desc(db):

@@ -94,6 +94,13 @@ ProblemDesc to_desc(const pmcd_problem_desc& d) {
     PMCD_CHECK(d.mesh_file != nullptr, "geometry PMCD_MESH_FILE needs mesh_file");
     p.mesh_file = d.mesh_file;
   }
+  PMCD_CHECK(d.adapt_steps >= 0 && d.adapt_steps <= 30, "adapt_steps out of range");
+  if(d.adapt_steps > 0) {
+    PMCD_CHECK(d.indicator != nullptr, "adapt_steps > 0 needs an indicator function");
+    p.adapt_steps = d.adapt_steps;
+    p.indicator = d.indicator;
+    p.indicator_ctx = d.indicator_ctx;
+  }
   return p;
 }
 
@@ -219,6 +226,24 @@ int pmcd_host_sell_layout(pmcd_host* h, int level, int* perm, int* len, long lon
   });
 }
 
+static int copy_hanging(const Level& L, int* out, int* n) {
+  PMCD_CHECK(n != nullptr, "null argument");
+  const FESpace& sp = L.space;
+  *n = sp.n_hanging;
+  if(out)
+    for(int t = 0; t < sp.n_hanging; ++t) {
+      const int h = sp.hang_order[t];
+      out[3 * t] = sp.n_active_non_hanging() + h;
+      out[3 * t + 1] = sp.hang_cpl[2 * h];
+      out[3 * t + 2] = sp.hang_cpl[2 * h + 1];
+    }
+  return 0;
+}
+
+int pmcd_host_hanging(pmcd_host* h, int level, int* out, int* n) {
+  return guarded([&] { copy_hanging(host_level(h, level), out, n); });
+}
+
 int pmcd_host_write_mesh(pmcd_host* h, int level, const char* path) {
   return guarded([&] {
     PMCD_CHECK(path != nullptr, "null path");
@@ -252,6 +277,9 @@ int pmcd_system_set_data(pmcd_system* s, const pmcd_problem_desc* desc) {
     PMCD_CHECK(d.geometry == cur.geometry && d.n_refine == cur.n_refine &&
                    d.structured_n == cur.structured_n && d.order == cur.order && d.supg == cur.supg &&
                    d.linear_quad == cur.linear_quad && d.mg_levels == cur.mg_levels,
+               "pmcd_system_set_data changes only the PDE data (example, eps, b, c, f0)");
+    PMCD_CHECK(d.adapt_steps == cur.adapt_steps && d.indicator == cur.indicator &&
+                   d.indicator_ctx == cur.indicator_ctx,
                "pmcd_system_set_data changes only the PDE data (example, eps, b, c, f0)");
     s->sys->set_example(d.ex);
   });
@@ -297,6 +325,10 @@ int pmcd_system_info(pmcd_system* s, int level, long long* info) {
     info[6] = s->sys->n_levels();
     info[7] = L.space.n_dirichlet;
   });
+}
+
+int pmcd_get_hanging(pmcd_system* s, int level, int* out, int* n) {
+  return guarded_on(s, [&] { copy_hanging(level_of(s, level), out, n); });
 }
 
 int pmcd_get_dofmap(pmcd_system* s, int level, int* out) {
@@ -518,7 +550,10 @@ int pmcd_batch_create(const pmcd_problem_desc* descs, int n_inst, pmcd_batch** o
     if(hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
       PMCD_FAIL("no GPU visible: the HIP path needs an MI355X device");
     std::vector<ProblemDesc> d(n_inst);
-    for(int i = 0; i < n_inst; ++i) d[i] = to_desc(descs[i]);
+    for(int i = 0; i < n_inst; ++i) {
+      d[i] = to_desc(descs[i]);
+      PMCD_CHECK(d[i].adapt_steps == 0, "batched instances take regularly refined meshes");
+    }
     auto b = std::make_unique<pmcd_batch>();
     PMCD_HIP(hipStreamCreate(&b->stream));
     TallyScope scope(&b->tally);
@@ -610,6 +645,7 @@ int pmcd_batch_get_rhs(pmcd_batch* b, int inst, double* rhs) {
 int pmcd_tcd_init(pmcd_system* s, const pmcd_tcd_desc* t) {
   return guarded_on(s, [&] {
     PMCD_CHECK(s && t, "null argument");
+    PMCD_CHECK(s->sys->desc().adapt_steps == 0, "time-dependent problems take regularly refined meshes");
     TcdDesc d{t->scheme, t->bdf_first, t->tau, t->t0};
     s->sys->tcd_init(d);
   });
@@ -634,6 +670,9 @@ int pmcd_afc_solve(pmcd_system* s, const pmcd_solver_desc* sd, const double* prm
                    pmcd_solve_stats* stats) {
   return guarded_on(s, [&] {
     PMCD_CHECK(s && sd && prm && out, "null argument");
+    PMCD_CHECK(s->sys->desc().adapt_steps == 0,
+               "algebraic flux correction takes regularly refined meshes (ModifyMatrixAccordingToCouplingAFC "
+               "is not implemented)");
     PMCD_CHECK(prm[0] > 0 && prm[1] >= 1, "afc_nonlinloop_epsilon must be positive, afc_nonlinloop_maxit >= 1");
     PMCD_CHECK(prm[11] == 0.0 || prm[11] == 1.0, "afc limiter must be 0 (kuzmin) or 1 (BJK17)");
     PMCD_CHECK(prm[12] == 0.0 || prm[12] == 1.0 || prm[12] == 2.0,
@@ -689,6 +728,7 @@ static ProblemDesc forms_desc(const pmcd_problem_desc& la) {
   pmcd_problem_desc d = la;
   d.geometry = 0;
   d.mesh_file = nullptr;
+  d.adapt_steps = 0;
   return to_desc(d);
 }
 

@@ -15,7 +15,7 @@ struct Key3Hash {
 };
 }  // namespace
 
-FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc) {
+FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc, const std::vector<HangingVertex>* hanging) {
   FESpace sp;
   sp.element = element_for(mesh.shape, order);
   const ElementInfo& info = element_info(sp.element);
@@ -74,25 +74,74 @@ FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc) {
       }
     }
 
+  // hanging vertices: their own kind, numbered in the list's order
+  const int HANGING = INT_MAX - 1;
+  std::vector<int> hang_index(n_classes, -1);
+  if(hanging && !hanging->empty()) {
+    PMCD_CHECK(order == 1 && mesh.dim() == 2, "hanging vertices are implemented for P1/Q1 in 2D");
+    for(size_t h = 0; h < hanging->size(); ++h) {
+      const int k = vertex_class[(*hanging)[h].vertex];
+      PMCD_CHECK(k >= 0 && vertex_class[(*hanging)[h].a] >= 0 && vertex_class[(*hanging)[h].b] >= 0,
+                 "hanging vertex without a DOF");
+      kind[k] = HANGING;
+      hang_index[k] = (int)h;
+    }
+  }
+
   // numbering: classes are already in first-appearance order
   constexpr int NK = 8;
-  std::vector<int> count_kind(NK + 1, 0);  // [0..6] boundary kinds, [7] inner
-  auto slot = [&](int kd) { return kd == INNER ? NK - 1 : kd; };
+  std::vector<int> count_kind(NK + 1, 0);  // [0..6] boundary kinds, [7] inner, [8] hanging
+  auto slot = [&](int kd) { return kd == INNER ? NK - 1 : (kd == HANGING ? NK : kd); };
   for(int k = 0; k < n_classes; ++k) count_kind[slot(kind[k])]++;
   sp.n_inner = count_kind[NK - 1];
   sp.n_dirichlet = count_kind[DIRICHLET];
-  // offsets: inner, NEUMANN, ROBIN, ..., then DIRICHLET
-  std::vector<int> offset(NK, 0);
+  sp.n_hanging = count_kind[NK];
+  // offsets: inner, NEUMANN, ROBIN, ..., hanging, then DIRICHLET
+  std::vector<int> offset(NK + 1, 0);
   int off = 0;
   offset[NK - 1] = off; off += count_kind[NK - 1];
   for(int kd = 1; kd < NK - 1; ++kd) { offset[kd] = off; off += count_kind[kd]; }
+  offset[NK] = off; off += count_kind[NK];
   sp.n_active = off;
   offset[DIRICHLET] = off; off += count_kind[DIRICHLET];
   sp.n_dof = off;
   std::vector<int> number(n_classes);
-  for(int k = 0; k < n_classes; ++k) number[k] = offset[slot(kind[k])]++;
+  for(int k = 0; k < n_classes; ++k)
+    number[k] = kind[k] == HANGING ? offset[NK] + hang_index[k] : offset[slot(kind[k])]++;
   sp.dof.resize((size_t)nc * nl);
   for(size_t p = 0; p < sp.dof.size(); ++p) sp.dof[p] = number[cls[p]];
+  if(sp.n_hanging) {
+    const int nh = sp.n_hanging, base = sp.n_active_non_hanging();
+    sp.hang_cpl.resize(2 * (size_t)nh);
+    for(int h = 0; h < nh; ++h) {
+      sp.hang_cpl[2 * h] = number[vertex_class[(*hanging)[h].a]];
+      sp.hang_cpl[2 * h + 1] = number[vertex_class[(*hanging)[h].b]];
+    }
+    // depth-first post-order over hanging coupling DOFs, reversed
+    std::vector<char> seen(nh, 0);
+    std::vector<int> post;
+    post.reserve(nh);
+    std::vector<std::pair<int, int>> stack;  // (hanging DOF, next coupling)
+    for(int r = 0; r < nh; ++r) {
+      if(seen[r]) continue;
+      seen[r] = 1;
+      stack.push_back({r, 0});
+      while(!stack.empty()) {
+        auto& top = stack.back();
+        if(top.second < 2) {
+          const int c = sp.hang_cpl[2 * top.first + top.second++] - base;
+          if(c >= 0 && c < nh && !seen[c]) {
+            seen[c] = 1;
+            stack.push_back({c, 0});
+          }
+        } else {
+          post.push_back(top.first);
+          stack.pop_back();
+        }
+      }
+    }
+    sp.hang_order.assign(post.rbegin(), post.rend());
+  }
   return sp;
 }
 

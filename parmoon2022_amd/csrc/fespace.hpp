@@ -30,13 +30,27 @@ struct FESpace {
   Element element;
   int n_local = 0;
   int n_dof = 0;
-  int n_active = 0;      // inner + non-Dirichlet boundary DOFs
+  int n_active = 0;      // inner + non-Dirichlet boundary + hanging DOFs
   int n_dirichlet = 0;
   int n_inner = 0;
   std::vector<int> dof;  // n_cells * n_local, global DOF numbers
+  // hanging DOFs n_active - n_hanging .. n_active - 1: hanging DOF h couples
+  // with hang_cpl[2h], hang_cpl[2h+1] (weights 1/2); hang_order lists the
+  // hanging DOFs (0-based) so that each comes before the hanging DOFs it
+  // couples with (sort_hanging_nodes, src/FE/FESpace.C:2101-2160)
+  int n_hanging = 0;
+  std::vector<int> hang_cpl;
+  std::vector<int> hang_order;
   const int* cell_dof(int c) const { return &dof[(size_t)c * n_local]; }
+  int n_active_non_hanging() const { return n_active - n_hanging; }
 };
 
-FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc);
+// `hanging` (P1/Q1 only): hanging vertices of the mesh; their DOFs follow the
+// non-Dirichlet boundary DOFs, in the list's order (FESpace.C:1863-2099
+// numbers them between the active and the Dirichlet DOFs), each coupled to
+// the DOFs at the ends of its coarse edge with the weights 1/2 of
+// HN_C_P1_2D_0 (HNDesc.C:9-11, FE2DMapper1Reg.C:43-58).
+FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc,
+                      const std::vector<HangingVertex>* hanging = nullptr);
 
 }  // namespace pmcd

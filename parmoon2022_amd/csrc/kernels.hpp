@@ -250,6 +250,16 @@ void launch_scale_inplace(int n, double a, double* y, hipStream_t s);
 void launch_div(int n, const double* x, const double* d, double* y, hipStream_t s);
 // dst[i] = src[idx[i]]
 void launch_gather_idx(int n, const int* idx, const double* src, double* dst, hipStream_t s);
+// hanging-DOF coupling: item i < n_mat adds mat_w[s] * val[mat_src[s]],
+// s = mat_sp[i] .. mat_sp[i+1], to val[mat_dst[i]] in list order; items
+// n_mat .. n_mat + n_rhs do the same on rhs (one launch per coupling depth)
+void launch_hanging_gather(int n_mat, const int* mat_dst, const int* mat_sp, const int* mat_src, const double* mat_w,
+                           double* val, int n_rhs, const int* rhs_dst, const int* rhs_sp, const int* rhs_src,
+                           const double* rhs_w, double* rhs, hipStream_t s);
+// val[pos[i]] = v[i] (the hanging rows' constraint entries), rhs[rhs_first ..
+// rhs_first + n_hang) = 0
+void launch_hanging_fix(int n_fix, const int* pos, const double* v, double* val, int rhs_first, int n_hang,
+                        double* rhs, hipStream_t s);
 // caller arrays += an assembly (val, f: the gathered CSR values and rhs):
 // active-row entries and active rhs added, Dirichlet diagonals set to 1 and
 // Dirichlet rhs set (either output may be null)

@@ -271,6 +271,141 @@ Mesh refine_regular(const Mesh& cm) {
   return fm;
 }
 
+namespace {
+
+// edge midpoints created so far, keyed by the edge's sorted end vertices
+using EdgeMids = std::unordered_map<uint64_t, int>;
+uint64_t edge_key(int u, int v) {
+  if(u > v) std::swap(u, v);
+  return ((uint64_t)(uint32_t)u << 32) | (uint32_t)v;
+}
+int edge_mid(const EdgeMids& mids, int u, int v) {
+  auto it = mids.find(edge_key(u, v));
+  return it == mids.end() ? -1 : it->second;
+}
+
+// one regular refinement of the marked cells; unmarked cells stay, children
+// replace their parent in place
+Mesh refine_marked(const Mesh& cm, const std::vector<char>& mark, EdgeMids& mids) {
+  const RefTable& rt = ref_table(cm.shape);
+  const auto& faces = shape_faces(cm.shape);
+  const int n_children = (int)rt.child.size();
+  const int n_ref = (int)rt.span.size();
+  Mesh fm;
+  fm.shape = cm.shape;
+  fm.nv = cm.nv;
+  fm.nj = cm.nj;
+  fm.coords = cm.coords;
+  std::vector<std::vector<bool>> on_joint(faces.size(), std::vector<bool>(n_ref, false));
+  for(size_t j = 0; j < faces.size(); ++j)
+    for(int r = 0; r < n_ref; ++r) {
+      bool in = true;
+      for(int v : rt.span[r])
+        if(std::find(faces[j].begin(), faces[j].end(), v) == faces[j].end()) in = false;
+      on_joint[j][r] = in;
+    }
+  std::vector<int> gid(n_ref);
+  for(int c = 0; c < cm.n_cells; ++c) {
+    const int* cv = cm.cell(c);
+    if(!mark[c]) {
+      fm.cell_vertices.insert(fm.cell_vertices.end(), cv, cv + cm.nv);
+      fm.joint_comp.insert(fm.joint_comp.end(), cm.joint_comp.begin() + (size_t)c * cm.nj,
+                           cm.joint_comp.begin() + (size_t)(c + 1) * cm.nj);
+      fm.parent.push_back(c);
+      fm.child_index.push_back(0);
+      continue;
+    }
+    for(int r = 0; r < n_ref; ++r) {
+      const auto& sp = rt.span[r];
+      if(sp.size() == 1) { gid[r] = cv[sp[0]]; continue; }
+      double x[3] = {0, 0, 0};
+      for(int v : sp)
+        for(int k = 0; k < 3; ++k) x[k] += cm.coords[3 * cv[v] + k];
+      for(int k = 0; k < 3; ++k) x[k] *= 1.0 / sp.size();
+      int id = -1;
+      if(sp.size() == 2) {
+        auto it = mids.find(edge_key(cv[sp[0]], cv[sp[1]]));
+        if(it != mids.end()) id = it->second;
+      }
+      if(id < 0) {
+        id = (int)(fm.coords.size() / 3);
+        fm.coords.push_back(x[0]); fm.coords.push_back(x[1]); fm.coords.push_back(x[2]);
+        if(sp.size() == 2) mids.emplace(edge_key(cv[sp[0]], cv[sp[1]]), id);
+      }
+      gid[r] = id;
+    }
+    for(int ch = 0; ch < n_children; ++ch) {
+      for(int a = 0; a < fm.nv; ++a) fm.cell_vertices.push_back(gid[rt.child[ch][a]]);
+      fm.parent.push_back(c);
+      fm.child_index.push_back(ch);
+      for(int jj = 0; jj < fm.nj; ++jj) {
+        int comp = -1;
+        for(size_t j = 0; j < faces.size() && comp < 0; ++j) {
+          const int pc = cm.joint_comp[(size_t)c * cm.nj + j];
+          if(pc < 0) continue;
+          bool in = true;
+          for(int lv : faces[jj])
+            if(!on_joint[j][rt.child[ch][lv]]) in = false;
+          if(in) comp = pc;
+        }
+        fm.joint_comp.push_back(comp);
+      }
+    }
+  }
+  fm.n_cells = (int)fm.parent.size();
+  return fm;
+}
+
+}  // namespace
+
+Mesh refine_by_indicator(const Mesh& base, int steps, const IndicatorFn& f, std::vector<HangingVertex>& hanging) {
+  PMCD_CHECK(base.dim() == 2, "refinement by indicator with hanging vertices is implemented for 2D meshes");
+  EdgeMids mids;
+  Mesh cur = base;
+  const auto& faces = shape_faces(cur.shape);
+  for(int step = 0; step < steps; ++step) {
+    // RefinementStrategy::apply_indicator: corner values on both sides of zero
+    std::vector<char> mark(cur.n_cells, 0);
+    for(int c = 0; c < cur.n_cells; ++c) {
+      int inner = 0, outer = 0;
+      for(int a = 0; a < cur.nv; ++a) {
+        const double val = f(&cur.coords[3 * cur.cell(c)[a]]);
+        if(val <= 1e-10) inner++;
+        if(val >= -1e-10) outer++;
+      }
+      mark[c] = inner > 0 && outer > 0;
+    }
+    for(;;) {
+      cur = refine_marked(cur, mark, mids);
+      // 1-irregular closure: a leaf edge (u, v) with midpoint m whose half
+      // (u, m) or (m, v) is split again borders a cell two levels finer
+      mark.assign(cur.n_cells, 0);
+      bool any = false;
+      for(int c = 0; c < cur.n_cells; ++c)
+        for(int j = 0; j < cur.nj; ++j) {
+          const int u = cur.cell(c)[faces[j][0]], v = cur.cell(c)[faces[j][1]];
+          const int m = edge_mid(mids, u, v);
+          if(m >= 0 && (edge_mid(mids, u, m) >= 0 || edge_mid(mids, m, v) >= 0)) {
+            mark[c] = 1;
+            any = true;
+          }
+        }
+      if(!any) break;
+    }
+  }
+  hanging.clear();
+  std::vector<char> seen(cur.n_vertices(), 0);
+  for(int c = 0; c < cur.n_cells; ++c)
+    for(int j = 0; j < cur.nj; ++j) {
+      const int u = cur.cell(c)[faces[j][0]], v = cur.cell(c)[faces[j][1]];
+      const int m = edge_mid(mids, u, v);
+      if(m < 0 || seen[m]) continue;
+      seen[m] = 1;
+      hanging.push_back({m, u, v});
+    }
+  return cur;
+}
+
 double cell_diameter(const Mesh& m, int c) {
   const int* v = m.cell(c);
   auto d2 = [&](int a, int b) {

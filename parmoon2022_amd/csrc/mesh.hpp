@@ -5,6 +5,7 @@
 // src/Geometry/Domain.C:1961-2000), i.e. the children of each coarse cell,
 // in child order, in the coarse cell order.
 #pragma once
+#include <functional>
 #include "common.hpp"
 
 namespace pmcd {
@@ -62,6 +63,27 @@ void write_mesh_file(const Mesh& m, const std::string& path);
 // One step of regular refinement of every cell (RefTriRegDesc.C,
 // RefQuadRegDesc.C, RefTetraRegDesc.C, RefHexaRegDesc.C child tables).
 Mesh refine_regular(const Mesh& coarse);
+
+// A hanging vertex of a mesh refined by indicator: the midpoint of the edge
+// (a, b) of a leaf cell whose neighbour across that edge is refined.
+struct HangingVertex {
+  int vertex, a, b;
+};
+
+// indicator value at a point (x, y, z)
+using IndicatorFn = std::function<double(const double* x)>;
+
+// `steps` times TDomain::RefineByIndicator without conforming closure
+// (src/Geometry/Domain.C:1094-1207, RefinementStrategy.C:260-289): a cell
+// whose corner values of the indicator lie on both sides of zero (a value
+// within 1e-10 of zero counts for both) is refined regularly, then every leaf
+// with a neighbour refined two levels deeper across one of its edges is
+// refined until the mesh is 1-irregular (Domain.C:1818-1835, GridCell.C:
+// 348-383, 462-491). Leaves stay in depth-first tree order (a refined cell is
+// replaced by its children, in child order). 2D meshes only. `hanging`
+// receives the hanging vertices in the order their coarse edges are met over
+// (cell, edge).
+Mesh refine_by_indicator(const Mesh& base, int steps, const IndicatorFn& f, std::vector<HangingVertex>& hanging);
 
 // Cell diameter as used for h_K with CELL_MEASURE=0 (Triangle.C:25-40,
 // Quadrangle.C:26-37, Tetrahedron.C:64-89, longest space diagonal for

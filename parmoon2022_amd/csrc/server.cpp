@@ -111,6 +111,17 @@ double page(double count, double size) {
 std::vector<LevelEstimate> estimate_levels(const ProblemDesc& pd, Shape* shape_out) {
   PMCD_CHECK(pd.order == 1 || pd.order == 2, "ansatz order must be 1 or 2");
   PMCD_CHECK(pd.n_refine >= 0 && pd.n_refine <= 12, "n_refine out of range");
+  if(pd.adapt_steps > 0) {
+    // no closed form for a refinement by indicator: count the host set-up
+    const auto levels = build_levels(pd);
+    const Level& L = *levels.back();
+    if(shape_out) *shape_out = L.mesh.shape;
+    LevelEstimate e;
+    e.cells = L.mesh.n_cells;
+    e.dofs = L.n();
+    e.entries = (double)L.st.nnz;
+    return {e};
+  }
   const Mesh base = coarse_mesh(pd);
   const Shape shape = base.shape;
   if(shape_out) *shape_out = shape;
@@ -178,6 +189,12 @@ long long estimate_device_bytes(const ProblemDesc& pd, const SolverDesc& sd) {
     // sliced ELL copy: rows padded to the longest row of each 64-row slice
     const double slots = 1.25 * nnz + 64 * 64;
     b += 3 * page(n, 4) + page(n / 64 + 2, 8) + page(slots, 4) + page(slots, 8) + (mixed ? page(slots, 4) : 0);
+    // hanging-DOF coupling lists: at most one item per entry and two sources
+    // per item (matrix), one item and two sources per DOF (rhs), the
+    // constraint entries
+    if(pd.adapt_steps > 0)
+      b += 2 * page(nnz + 1, 4) + page(2 * nnz, 4) + page(2 * nnz, 8) + 2 * page(n + 1, 4) + page(2 * n, 4) +
+           page(2 * n, 8) + page(nnz, 4) + page(nnz, 8);
     total += b;
     prev_n = n;
   }
@@ -214,7 +231,8 @@ namespace {
 std::string mesh_key(const ProblemDesc& d) {
   std::ostringstream os;
   os << d.geometry << '|' << d.n_refine << '|' << d.structured_n << '|' << d.order << '|' << d.supg << '|'
-     << d.linear_quad << '|' << d.mg_levels << '|' << d.mesh_file;
+     << d.linear_quad << '|' << d.mg_levels << '|' << d.mesh_file << '|' << d.adapt_steps << '|'
+     << (const void*)d.indicator << '|' << d.indicator_ctx;
   return os.str();
 }
 

@@ -1,6 +1,7 @@
 #include "structure.hpp"
 #include <algorithm>
 #include <climits>
+#include <iterator>
 
 namespace pmcd {
 
@@ -123,7 +124,35 @@ Structure build_structure(const FESpace& sp, int n_cells) {
       }
     }
   }
+  if(sp.n_hanging) return hanging_structure(sp, n_cells, st);
   return st;
+}
+
+// Rows of coupling DOFs take the columns of the hanging rows added to them
+// (FEMatrix::ModifyMatrixAccordingToCoupling, src/Matrix/FEMatrix.C:244-296,
+// needs every entry it adds; Structure.C:75-230 puts them in the pattern),
+// in the sorted hanging order so that a hanging coupling DOF passes on what
+// it received. Entries the reference pattern may hold beyond these stay zero.
+Structure hanging_structure(const FESpace& sp, int n_cells, const Structure& base) {
+  const int n = base.n_rows, first = sp.n_active_non_hanging();
+  std::vector<std::vector<int>> rows(n);
+  for(int r = 0; r < n; ++r) rows[r].assign(base.col.begin() + base.row_ptr[r], base.col.begin() + base.row_ptr[r + 1]);
+  std::vector<int> merged;
+  for(int h : sp.hang_order) {
+    const int hd = first + h;
+    for(int k = 0; k < 2; ++k) {
+      const int c = sp.hang_cpl[2 * h + k];
+      if(c >= sp.n_active) continue;
+      merged.clear();
+      std::set_union(rows[c].begin(), rows[c].end(), rows[hd].begin(), rows[hd].end(), std::back_inserter(merged));
+      rows[c].swap(merged);
+    }
+  }
+  std::vector<int> row_ptr(n + 1, 0), col;
+  for(int r = 0; r < n; ++r) row_ptr[r + 1] = row_ptr[r] + (int)rows[r].size();
+  col.reserve(row_ptr[n]);
+  for(int r = 0; r < n; ++r) col.insert(col.end(), rows[r].begin(), rows[r].end());
+  return structure_from_pattern(sp, n_cells, row_ptr.data(), col.data());
 }
 
 SweepOrder build_sweep_order(const Structure& st, int max_exact_classes) {

@@ -35,6 +35,9 @@ struct Structure {
 };
 
 Structure build_structure(const FESpace& sp, int n_cells);
+// a space with hanging DOFs: `base` (the cell pattern) with the columns of
+// each hanging row added to the rows of its active coupling DOFs
+Structure hanging_structure(const FESpace& sp, int n_cells, const Structure& base);
 // the maps of a caller's pattern (row_ptr, col: n_dof rows, any column order)
 // for the space's cells: diagonal positions, (cell, local row) lists and slot
 // maps; every coupling of two DOFs of one cell must be in the pattern

@@ -60,6 +60,19 @@ std::vector<std::unique_ptr<Level>> build_levels(const ProblemDesc& d) {
   const ExampleParams ex = d.ex;
   BoundCondFn bc = [ex](int comp, const double*) { return (BoundCond)example_bc(ex, comp); };
   std::vector<std::unique_ptr<Level>> levels;
+  if(d.adapt_steps > 0) {
+    PMCD_CHECK(d.indicator, "refinement by indicator without an indicator function");
+    PMCD_CHECK(d.order == 1 && meshes.back().dim() == 2, "hanging vertices are implemented for P1/Q1 in 2D");
+    PMCD_CHECK(d.mg_levels <= 1, "multigrid on meshes refined by indicator is not implemented");
+    auto L = std::make_unique<Level>();
+    std::vector<HangingVertex> hanging;
+    const IndicatorFn f = [&d](const double* x) { return d.indicator(x, d.indicator_ctx); };
+    L->mesh = refine_by_indicator(meshes.back(), d.adapt_steps, f, hanging);
+    L->space = build_fespace(L->mesh, d.order, bc, &hanging);
+    L->st = build_structure(L->space, L->mesh.n_cells);
+    levels.push_back(std::move(L));
+    return levels;
+  }
   for(int l = (int)meshes.size() - keep; l < (int)meshes.size(); ++l) {
     auto L = std::make_unique<Level>();
     L->mesh = std::move(meshes[l]);
@@ -288,6 +301,113 @@ void CDSystem::set_example(const ExampleParams& ex) {
   PMCD_HIP(hipStreamSynchronize(stream_));
 }
 
+namespace {
+
+// Gather lists of the hanging-DOF coupling (see k_hanging.hip). Hanging DOF h
+// adds its row, weighted 1/2, to the rows of its coupling DOFs below n_active
+// (hanging ones included); a target entry (c, col) therefore receives the
+// entries (h, col) of the hanging rows coupled to c, in the sorted order. A
+// hanging row is final once all rows coupled to it have been added: depth(h)
+// = 0 without incoming hanging rows, else 1 + the largest depth among them.
+// Launch k (0-based) finishes the hanging rows of depth k + 1, the last launch
+// the other active rows.
+void hanging_plan(Level& L, hipStream_t stream) {
+  const FESpace& sp = L.space;
+  const Structure& st = L.st;
+  const int nh = sp.n_hanging, first = sp.n_active_non_hanging();
+  std::vector<int> depth(nh, 0);
+  std::vector<std::vector<int>> incoming(sp.n_dof);  // target DOF -> hanging DOFs, sorted order
+  for(int h : sp.hang_order)
+    for(int k = 0; k < 2; ++k) {
+      const int c = sp.hang_cpl[2 * h + k];
+      if(c >= sp.n_active) continue;
+      incoming[c].push_back(h);
+      if(c >= first) depth[c - first] = std::max(depth[c - first], depth[h] + 1);
+    }
+  int max_depth = 0;
+  for(int h = 0; h < nh; ++h) max_depth = std::max(max_depth, depth[h]);
+  const int n_launch = max_depth + 1;
+  auto launch_of = [&](int c) { return c >= first && c < sp.n_active ? depth[c - first] - 1 : max_depth; };
+  auto pos_in_row = [&](int r, int col) {
+    const int* b = &st.col[st.row_ptr[r]];
+    const int* e = &st.col[st.row_ptr[r + 1]];
+    const int* p = std::lower_bound(b, e, col);
+    return p != e && *p == col ? (int)(st.row_ptr[r] + (p - b)) : -1;
+  };
+  std::vector<std::vector<int>> mdst(n_launch), msp(n_launch), msrc(n_launch), rdst(n_launch), rsp(n_launch),
+      rsrc(n_launch);
+  for(int c = 0; c < sp.n_active; ++c) {
+    if(incoming[c].empty()) continue;
+    const int k = launch_of(c);
+    for(int p = st.row_ptr[c]; p < st.row_ptr[c + 1]; ++p) {
+      std::vector<int> srcs;
+      for(int h : incoming[c]) {
+        const int q = pos_in_row(first + h, st.col[p]);
+        if(q >= 0) srcs.push_back(q);
+      }
+      if(srcs.empty()) continue;
+      mdst[k].push_back(p);
+      msp[k].push_back((int)msrc[k].size());
+      msrc[k].insert(msrc[k].end(), srcs.begin(), srcs.end());
+    }
+    rdst[k].push_back(c);
+    rsp[k].push_back((int)rsrc[k].size());
+    for(int h : incoming[c]) rsrc[k].push_back(first + h);
+  }
+  // concatenate the launches; source pointers become global
+  std::vector<int> mat_dst, mat_sp, mat_src, rhs_dst, rhs_sp, rhs_src;
+  L.hg_mat_launch.assign(1, 0);
+  L.hg_rhs_launch.assign(1, 0);
+  for(int k = 0; k < n_launch; ++k) {
+    const int mo = (int)mat_src.size(), ro = (int)rhs_src.size();
+    for(size_t i = 0; i < mdst[k].size(); ++i) { mat_dst.push_back(mdst[k][i]); mat_sp.push_back(mo + msp[k][i]); }
+    mat_src.insert(mat_src.end(), msrc[k].begin(), msrc[k].end());
+    for(size_t i = 0; i < rdst[k].size(); ++i) { rhs_dst.push_back(rdst[k][i]); rhs_sp.push_back(ro + rsp[k][i]); }
+    rhs_src.insert(rhs_src.end(), rsrc[k].begin(), rsrc[k].end());
+    L.hg_mat_launch.push_back((int)mat_dst.size());
+    L.hg_rhs_launch.push_back((int)rhs_dst.size());
+  }
+  mat_sp.push_back((int)mat_src.size());
+  rhs_sp.push_back((int)rhs_src.size());
+  // the hanging rows: 1 on the diagonal, -1/2 at the coupling DOFs, 0 elsewhere
+  std::vector<int> fix_pos;
+  std::vector<double> fix_val;
+  for(int h = 0; h < nh; ++h) {
+    const int r = first + h;
+    for(int p = st.row_ptr[r]; p < st.row_ptr[r + 1]; ++p) {
+      const int col = st.col[p];
+      fix_pos.push_back(p);
+      fix_val.push_back(col == r ? 1.0 : (col == sp.hang_cpl[2 * h] || col == sp.hang_cpl[2 * h + 1] ? -0.5 : 0.0));
+    }
+  }
+  L.hg_mat_dst.upload(mat_dst, stream);
+  L.hg_mat_sp.upload(mat_sp, stream);
+  L.hg_mat_src.upload(mat_src, stream);
+  L.hg_mat_w.upload(std::vector<double>(std::max<size_t>(1, mat_src.size()), 0.5), stream);
+  L.hg_rhs_dst.upload(rhs_dst, stream);
+  L.hg_rhs_sp.upload(rhs_sp, stream);
+  L.hg_rhs_src.upload(rhs_src, stream);
+  L.hg_rhs_w.upload(std::vector<double>(std::max<size_t>(1, rhs_src.size()), 0.5), stream);
+  L.hg_fix_pos.upload(fix_pos, stream);
+  L.hg_fix_val.upload(fix_val, stream);
+}
+
+// Assemble2D.C:3350-3427: coupling of the assembled level, then the
+// constraint rows and zero rhs entries of the hanging DOFs
+void apply_hanging(Level& L, hipStream_t stream) {
+  const int n_launch = (int)L.hg_mat_launch.size() - 1;
+  for(int k = 0; k < n_launch; ++k) {
+    const int m0 = L.hg_mat_launch[k], r0 = L.hg_rhs_launch[k];
+    launch_hanging_gather(L.hg_mat_launch[k + 1] - m0, L.hg_mat_dst.p + m0, L.hg_mat_sp.p + m0, L.hg_mat_src.p,
+                          L.hg_mat_w.p, L.val.p, L.hg_rhs_launch[k + 1] - r0, L.hg_rhs_dst.p + r0,
+                          L.hg_rhs_sp.p + r0, L.hg_rhs_src.p, L.hg_rhs_w.p, L.rhs.p, stream);
+  }
+  launch_hanging_fix((int)L.hg_fix_pos.n, L.hg_fix_pos.p, L.hg_fix_val.p, L.val.p, L.space.n_active_non_hanging(),
+                     L.space.n_hanging, L.rhs.p, stream);
+}
+
+}  // namespace
+
 void CDSystem::build_level_device(Level& L) {
   const Mesh& m = L.mesh;
   const FESpace& sp = L.space;
@@ -313,6 +433,7 @@ void CDSystem::build_level_device(Level& L) {
   L.sol.alloc(n);
   L.diag.alloc(n);
   L.mg_sol.alloc(n); L.mg_sol2.alloc(n); L.mg_rhs.alloc(n); L.mg_def.alloc(n); L.mg_tmp.alloc(n);
+  if(sp.n_hanging) hanging_plan(L, stream_);
 }
 
 // P[f][c] = value of coarse basis function c at the nodal point of fine DOF f
@@ -463,6 +584,10 @@ void CDSystem::assemble_level(Level& L) {
   L.sell_fresh = sell;
   launch_gather_rhs(L.n(), L.n_active(), L.contrib_ptr.p, L.contrib.p, local_rhs_.p,
                     L.extra_ptr.p, L.extra.p, L.dir_vals.p, L.rhs.p, stream_);
+  if(L.space.n_hanging) {
+    apply_hanging(L, stream_);
+    L.sell_fresh = false;  // the fused sliced ELL copy predates the coupling
+  }
   launch_gather_idx(L.n(), L.diag_pos.p, L.val.p, L.diag.p, stream_);
 }
 

@@ -32,6 +32,11 @@ struct ProblemDesc {
   int mg_levels;      // number of levels kept (1 = finest only)
   ExampleParams ex;
   std::string mesh_file;  // geometry 12: path of a 3D .mesh file
+  // TDomain::RefineByIndicator steps without conforming closure after the
+  // regular refinements (2D, P1/Q1, one level): leaves with hanging vertices
+  int adapt_steps = 0;
+  double (*indicator)(const double* x, void* ctx) = nullptr;  // evaluated at vertices
+  void* indicator_ctx = nullptr;
 };
 
 struct SolverDesc {
@@ -119,6 +124,12 @@ struct Level {
   DevBuf<double> sell_val;
   DevBuf<float> sell_val32;
   bool sell_fresh = false, sell_fresh32 = false;
+  // hanging-DOF coupling after assembly: launch k takes matrix items
+  // hg_mat_launch[k] .. hg_mat_launch[k+1] and rhs items hg_rhs_launch[k] ..;
+  // hg_fix_* set the hanging rows' constraint entries
+  std::vector<int> hg_mat_launch, hg_rhs_launch;
+  DevBuf<int> hg_mat_dst, hg_mat_sp, hg_mat_src, hg_rhs_dst, hg_rhs_sp, hg_rhs_src, hg_fix_pos;
+  DevBuf<double> hg_mat_w, hg_rhs_w, hg_fix_val;
   int n() const { return space.n_dof; }
   int n_active() const { return space.n_active; }
 };

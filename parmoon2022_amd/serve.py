@@ -34,7 +34,12 @@ MESH_KEYS = ("geo_file", "refinement_n_initial_steps", "structured_n", "ANSATZ_O
 
 
 def mesh_key(db):
-    return tuple(db.get(k) for k in MESH_KEYS)
+    key = tuple(db.get(k) for k in MESH_KEYS)
+    steps = int(db.get("refine_by_indicator_steps", 0))
+    if steps > 0:
+        f = db["refine_indicator"]
+        key += (steps, tuple(f) if isinstance(f, (tuple, list)) else f)
+    return key
 
 
 def estimate_bytes(db):

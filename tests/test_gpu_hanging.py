@@ -1,0 +1,113 @@
+"""Meshes refined by indicator with hanging vertices (P1 on TwoTriangles, Q1
+on UnitSquare, eight steps of 0.5 x^2 + y - 0.5) on the device: assembly
+with the hanging-DOF coupling against the oracle (row-relative 1e-12, the
+constraint rows and hanging rhs entries exact), the reference's golden
+errors of test/adaptive_hanging_test.c++ within its 1e-11 for the direct and
+the FGMRES solvers, the solution against the oracle's, and served requests.
+"""
+import numpy as np
+import pytest
+
+import parmoon2022_amd as pm
+from test_oracle import HANGING_GOLDEN, hanging_db
+from test_gpu_parity import row_rel_err
+
+pytestmark = pytest.mark.gpu
+
+FGMRES = {"solver_type": "iterative", "iterative_solver_type": "fgmres", "max_n_iterations": 5000,
+          "residual_tolerance": 1e-13, "gmres_restart": 50}
+
+
+def _hanging(cd):
+    import ctypes
+    n = ctypes.c_int()
+    pm._check(pm.lib().pmcd_get_hanging(cd._s, -1, None, ctypes.byref(n)))
+    out = np.zeros(3 * max(n.value, 1), dtype=np.int32)
+    pm._check(pm.lib().pmcd_get_hanging(cd._s, -1, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                        ctypes.byref(n)))
+    return out[:3 * n.value].reshape(-1, 3)
+
+
+@pytest.mark.parametrize("variant", [0, 2], ids=["vector", "tensor"])
+@pytest.mark.parametrize("geo", list(HANGING_GOLDEN))
+def test_hanging_assembly_matches_oracle(oracle_lib, geo, variant):
+    db = hanging_db(geo)
+    print("hanging assembly", geo, "variant", variant, flush=True)
+    old = pm.assembly_variant()
+    pm.set_assembly_variant(variant)
+    try:
+        cd = pm.ConvectionDiffusion(db)
+        cd.assemble()
+    finally:
+        pm.set_assembly_variant(old)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    assert np.array_equal(cd.dofmap(), o.dofmap())
+    hg = _hanging(cd)
+    assert np.array_equal(hg, o.hanging())
+    rp, col = cd.pattern()
+    a, b = cd.matrix_values(), o.matrix_values()
+    err = row_rel_err(rp, a, b)
+    print("matrix: row-relative error %.3e, bit-identical fraction %.4f" % (err, np.mean(a == b)), flush=True)
+    assert err <= 1e-12
+    fa, fb = cd.rhs(), o.rhs()
+    assert np.max(np.abs(fa - fb)) <= 1e-12 * np.max(np.abs(fb))
+    # hanging rows: u_h - u_a / 2 - u_b / 2 = 0, rhs 0 (exact)
+    for h, ca, cb in hg:
+        row = {int(c): v for c, v in zip(col[rp[h]:rp[h + 1]], a[rp[h]:rp[h + 1]])}
+        want = {c: 0.0 for c in row}
+        want[h], want[ca], want[cb] = 1.0, -0.5, -0.5
+        assert row == want, h
+    assert np.all(fa[hg[:, 0]] == 0.0)
+    cd.close()
+
+
+@pytest.mark.parametrize("solver", ["direct", "fgmres_ssor", "fgmres_jacobi"])
+@pytest.mark.parametrize("geo", list(HANGING_GOLDEN))
+def test_hanging_golden_errors_on_gpu(oracle_lib, geo, solver):
+    extra = {}
+    if solver != "direct":
+        extra = dict(FGMRES, preconditioner=solver.split("_")[1])
+    db = hanging_db(geo, **extra)
+    print("hanging golden", geo, solver, flush=True)
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    st = cd.solve()
+    assert st.converged
+    e = cd.output()
+    got = [e[0], e[1], e[2], e[4]]
+    assert np.allclose(got, HANGING_GOLDEN[geo], rtol=0, atol=1e-11), (got, HANGING_GOLDEN[geo])
+    u = cd.solution()
+    hg = _hanging(cd)
+    assert np.allclose(u[hg[:, 0]], 0.5 * (u[hg[:, 1]] + u[hg[:, 2]]), rtol=0, atol=1e-12)
+    if solver == "direct":
+        o = oracle_lib.Oracle(pm.problem_desc(db))
+        o.assemble()
+        o.solve(pm.solver_desc(db))
+        ou = o.solution()
+        assert np.max(np.abs(u - ou)) <= 1e-12 * np.max(np.abs(ou))
+    cd.close()
+
+
+def test_hanging_served_requests():
+    """Requests on an adaptive mesh through the request server: the same
+    solution as a system of their own; the second request reuses the
+    resident system (same indicator, same callback)."""
+    import parmoon2022_amd.serve as sv
+    db = hanging_db("UnitSquare", **dict(FGMRES, preconditioner="ssor"))
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    cd.solve()
+    u = cd.solution()
+    cd.close()
+    srv = sv.Server(max_slots=1, keep_solution=True)
+    reqs = [srv.submit(db), srv.submit(db)]
+    for r in reqs:
+        assert r.done.wait(120)
+    info = srv.info()
+    srv.close()
+    assert info["systems_created"] == 1
+    for r in reqs:
+        assert r.error is None
+        assert np.array_equal(r.solution, u)
+    assert reqs[1].reused

@@ -437,3 +437,64 @@ def test_batch_instances_generator():
     bv = np.array([d["convection"] for d in a])
     assert np.allclose(np.linalg.norm(bv, axis=1), 1.0, atol=1e-14)
     assert np.all(np.abs(bv.mean(axis=0)) < 0.15)  # directions spread over the sphere
+
+
+def _hanging_db(geo):
+    return {"geo_file": geo, "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 0, "example": "sine_laplace_2d",
+            "refine_by_indicator_steps": 8, "refine_indicator": (0.5, 1.0, 0.0, -0.5), "conforming_closure": False}
+
+
+def _host_hanging(h):
+    n = ctypes.c_int()
+    pm._check(pm.lib().pmcd_host_hanging(h._h, -1, None, ctypes.byref(n)))
+    out = np.zeros(3 * max(n.value, 1), dtype=np.int32)
+    pm._check(pm.lib().pmcd_host_hanging(h._h, -1, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                         ctypes.byref(n)))
+    return out[:3 * n.value].reshape(-1, 3)
+
+
+@pytest.mark.parametrize("geo", ["TwoTriangles", "UnitSquare"])
+def test_hanging_setup_matches_oracle(oracle_lib, geo):
+    """Refinement by indicator with hanging vertices: the library's host
+    set-up (leaves, numbering, coupled pattern, sorted hanging list) equals
+    the oracle's."""
+    db = _hanging_db(geo)
+    h = pm.HostSetup(db)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    assert h.info() == o.info()
+    assert np.array_equal(h.cell_coords(), o.cell_coords(nv=3 if geo == "TwoTriangles" else 4))
+    assert np.array_equal(h.dofmap(), o.dofmap())
+    rp, col = h.pattern()
+    orp, ocol = o.pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+    hg = _host_hanging(h)
+    assert len(hg) > 0 and np.array_equal(hg, o.hanging())
+    # hanging DOFs are the last active ones, before the Dirichlet DOFs
+    i = h.info()
+    assert hg[:, 0].max() == i["n_active"] - 1 and hg[:, 0].min() == i["n_active"] - len(hg)
+
+
+def test_hanging_setup_rejections():
+    db = _hanging_db("UnitSquare")
+    with pytest.raises(ValueError, match="conforming_closure"):
+        pm.problem_desc(dict(db, conforming_closure=True))
+    with pytest.raises(RuntimeError, match="P1/Q1"):
+        pm.HostSetup(dict(db, ANSATZ_ORDER=2))
+    with pytest.raises(RuntimeError, match="2D"):
+        pm.HostSetup(dict(db, geo_file="Default_UnitCube_Hexa"))
+    with pytest.raises(RuntimeError, match="multigrid"):
+        pm.HostSetup(dict(db, solver_type="iterative", preconditioner="multigrid", multigrid_n_levels=2,
+                          refinement_n_initial_steps=1))
+
+
+def test_hanging_callable_indicator_equals_coefficients():
+    """A Python indicator f(x, y, z) gives the same mesh as the coefficient
+    form of the same function, and the estimate counts the exact set-up."""
+    a = pm.HostSetup(_hanging_db("TwoTriangles"))
+    b = pm.HostSetup(dict(_hanging_db("TwoTriangles"), refine_indicator=lambda x, y, z: 0.5 * x * x + y - 0.5))
+    assert a.info() == b.info() and np.array_equal(a.dofmap(), b.dofmap())
+    out = (ctypes.c_longlong * 3)()
+    pd = pm.problem_desc(_hanging_db("TwoTriangles"))
+    pm._check(pm.lib().pmcd_estimate_counts(ctypes.byref(pd), out))
+    i = a.info()
+    assert list(out) == [i["n_cells"], i["n_dof"], i["nnz"]]

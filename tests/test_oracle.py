@@ -627,3 +627,99 @@ def test_oracle_structure_test_fixtures(oracle_lib, geo, order, expected):
     o = oracle_lib.Oracle(pm.problem_desc(db))
     rp, col = o.pattern()
     assert (len(rp) - 1, int(col.max()) + 1, len(col), int(rp[4] - rp[3])) == expected
+
+
+# test/adaptive_hanging_test.c++:126-135, 283-290, 392-399: eight
+# RefineByIndicator steps with the indicator 0.5 x^2 + y - 0.5 and no
+# conforming closure, then the default CD problem (sine_laplace, direct
+# solver, P1 on TwoTriangles, Q1 on UnitSquare); L2, H1-semi, SD and L_inf
+# errors within 1e-11
+HANGING_GOLDEN = {
+    "TwoTriangles": [0.042380310691282, 0.59129151149837, 0.59129151149837, 0.13495122911489],
+    "UnitSquare": [0.011874554092922, 0.25451703865297, 0.25451703865297, 0.03750007808692],
+}
+
+
+def hanging_db(geo, **extra):
+    db = {"geo_file": geo, "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 0, "example": "sine_laplace_2d",
+          "refine_by_indicator_steps": 8, "refine_indicator": (0.5, 1.0, 0.0, -0.5), "conforming_closure": False,
+          "solver_type": "direct"}
+    db.update(extra)
+    return db
+
+
+def _hanging_errors(oracle_lib, db):
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    st = o.solve(pm.solver_desc(db))
+    assert st.converged
+    e = o.errors()
+    return np.array([e[0], e[1], e[2], e[4]]), o
+
+
+@pytest.mark.parametrize("geo", list(HANGING_GOLDEN))
+@pytest.mark.parametrize("solver", ["direct", "fgmres_ssor"])
+def test_oracle_hanging_golden(oracle_lib, geo, solver):
+    extra = {}
+    if solver == "fgmres_ssor":
+        extra = {"solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "ssor",
+                 "max_n_iterations": 5000, "residual_tolerance": 1e-13, "gmres_restart": 50}
+    got, o = _hanging_errors(oracle_lib, hanging_db(geo, **extra))
+    assert np.allclose(got, HANGING_GOLDEN[geo], rtol=0, atol=1e-11), (got, HANGING_GOLDEN[geo])
+    # the solution satisfies every hanging constraint u_h = (u_a + u_b) / 2
+    u = o.solution()
+    hg = o.hanging()
+    assert len(hg) > 0
+    assert np.allclose(u[hg[:, 0]], 0.5 * (u[hg[:, 1]] + u[hg[:, 2]]), rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("fault,geos", [(6, ["TwoTriangles"]), (7, ["TwoTriangles"]),
+                                        (8, ["TwoTriangles", "UnitSquare"])])
+def test_oracle_hanging_planted_mistakes_fail(oracle_lib, fault, geos):
+    """The goldens catch each planted mistake: 6 hanging coupling DOFs that
+    hang themselves receive nothing, 7 hanging rows added in list order, 8 no
+    1-irregular closure. (The Q1 mesh has no chained hanging DOFs, so 6 and 7
+    cannot show there.)"""
+    try:
+        oracle_lib.lib().orc_set_fault(fault)
+        for geo in geos:
+            got, _ = _hanging_errors(oracle_lib, hanging_db(geo))
+            assert np.max(np.abs(got - HANGING_GOLDEN[geo])) > 1e-6, (fault, geo, got)
+    finally:
+        oracle_lib.lib().orc_set_fault(0)
+
+
+@pytest.mark.parametrize("geo", list(HANGING_GOLDEN))
+def test_oracle_hanging_mesh_invariants(oracle_lib, geo):
+    """Leaves tile the unit square (areas 4^-k summing to 1), each hanging
+    DOF sits at the midpoint of its coupling DOFs and no hanging DOF lies on
+    the boundary; the sorted list puts a hanging DOF before the hanging DOFs
+    it couples with."""
+    o = oracle_lib.Oracle(pm.problem_desc(hanging_db(geo)))
+    i = o.info()
+    nv = 3 if geo == "TwoTriangles" else 4
+    X = o.cell_coords(nv=nv)[:, :, :2]
+    def cross(u, v):
+        return u[:, 0] * v[:, 1] - u[:, 1] * v[:, 0]
+    if nv == 3:
+        area = 0.5 * np.abs(cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]))
+    else:
+        area = np.abs(cross(X[:, 1] - X[:, 0], X[:, 3] - X[:, 0]))
+    assert abs(area.sum() - 1.0) < 1e-13
+    k = np.log(area / area.max()) / np.log(0.25)
+    assert np.allclose(k, np.round(k), atol=1e-9)
+    # DOF coordinates from the P1/Q1 dofmap: local DOF -> vertex
+    dm = o.dofmap()
+    xy = np.zeros((i["n_dof"], 2))
+    local_vertex = [0, 1, 2] if nv == 3 else [0, 1, 3, 2]  # Q1 DOFs in tensor order
+    for j, a in enumerate(local_vertex):
+        xy[dm[:, j]] = X[:, a]
+    hg = o.hanging()
+    assert np.allclose(xy[hg[:, 0]], 0.5 * (xy[hg[:, 1]] + xy[hg[:, 2]]), atol=1e-15)
+    on_bnd = np.any((np.abs(xy[hg[:, 0]]) < 1e-14) | (np.abs(xy[hg[:, 0]] - 1) < 1e-14), axis=1)
+    assert not on_bnd.any()
+    pos = {h: t for t, h in enumerate(hg[:, 0])}
+    for t, (h, a, b) in enumerate(hg):
+        for c in (a, b):
+            if c in pos:
+                assert pos[c] > t

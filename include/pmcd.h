@@ -71,7 +71,7 @@ typedef struct {
    * indicator(x, indicator_ctx) (x = {x, y, z}) lie on both sides of zero are
    * refined regularly, then the mesh is closed to a 1-irregular one. The
    * hanging vertices' DOFs are coupled to their coarse edges as in
-   * Assemble2D.C:3350-3427. 2D, P1/Q1, mg_levels 1; steady problems only
+   * Assemble2D.C:3350-3427. 2D, order 1 or 2, mg_levels 1; steady problems only
    * (no AFC, no time stepping, no batches). The indicator runs on the host
    * during pmcd_system_create / pmcd_host_create (and on a server worker for
    * a served request, so the context must stay valid until it finishes).
@@ -147,12 +147,14 @@ int pmcd_host_info(pmcd_host* h, int level, long long* info);
 int pmcd_host_dofmap(pmcd_host* h, int level, int* out);
 int pmcd_host_pattern(pmcd_host* h, int level, int* row_ptr, int* col);
 int pmcd_host_cell_coords(pmcd_host* h, int level, double* out);
-/* Hanging DOFs of a level (meshes refined by indicator): *n = their count
- * and, if out is not NULL, out[3t .. 3t+2] = hanging DOF, coupling DOF,
- * coupling DOF (weights 1/2) in the sorted order of
- * TFESpace::get_sorted_hanging_nodes (src/FE/FESpace.C:2094-2160; hanging
- * DOFs follow the non-Dirichlet boundary DOFs, FESpace.h:71-92). */
-int pmcd_host_hanging(pmcd_host* h, int level, int* out, int* n);
+/* Hanging DOFs of a level (meshes refined by indicator) in the sorted order
+ * of TFESpace::get_sorted_hanging_nodes (src/FE/FESpace.C:2094-2160;
+ * hanging DOFs follow the non-Dirichlet boundary DOFs, FESpace.h:71-92):
+ * *n = their count; if out is not NULL, out[4t] = hanging DOF and
+ * out[4t+1 .. 4t+3] = its coupling DOFs (THangingNode::GetDOF, -1 beyond
+ * their count: 2 for order 1, 3 for order 2); if w is not NULL,
+ * w[3t .. 3t+2] = the coupling weights (THangingNode::GetCoeff, 0 beyond). */
+int pmcd_host_hanging(pmcd_host* h, int level, int* out, double* w, int* n);
 /* Writes the level's mesh as a 3D medit .mesh file (vertices, boundary faces
  * with marker = boundary component + 1, cells), the format TCollection::
  * writeMesh writes (src/Geometry/Collection.C:806-); it reads back through
@@ -191,7 +193,7 @@ int pmcd_system_set_data(pmcd_system* s, const pmcd_problem_desc* desc);
  * n_levels, n_dirichlet of level `level` (0 = coarsest, -1 = finest). */
 int pmcd_system_info(pmcd_system* s, int level, long long* info);
 /* Hanging DOFs of a level, as pmcd_host_hanging. */
-int pmcd_get_hanging(pmcd_system* s, int level, int* out, int* n);
+int pmcd_get_hanging(pmcd_system* s, int level, int* out, double* w, int* n);
 /* GlobalNumbers of the level's FE space (TFESpace::GetGlobalDOF),
  * n_cells * n_local ints. */
 int pmcd_get_dofmap(pmcd_system* s, int level, int* out);

@@ -97,7 +97,7 @@ class Oracle:
             p = lib().orc_create_adaptive(desc.geometry, desc.n_refine, adapt, fn, desc.indicator_ctx, desc.order,
                                           desc.supg, desc.linear_quad, desc.example, desc.eps, b, desc.c, desc.f0)
             if not p:
-                raise ValueError("adaptive meshes take P1/Q1 on the 2D default meshes")
+                raise ValueError("adaptive meshes take order 1 or 2 on the 2D default meshes")
             self._p = ctypes.c_void_p(p)
         else:
             self._p = ctypes.c_void_p(lib().orc_create(desc.geometry, desc.structured_n, desc.n_refine,
@@ -106,11 +106,13 @@ class Oracle:
                                                        desc.eps, b, desc.c, desc.f0))
 
     def hanging(self):
-        """(hanging DOF, coupling DOF, coupling DOF) rows in the sorted order."""
-        n = lib().orc_hanging(self._p, None)
-        out = np.zeros(3 * max(n, 1), dtype=np.int32)
-        lib().orc_hanging(self._p, _p(out, ctypes.c_int))
-        return out[:3 * n].reshape(n, 3)
+        """Rows (hanging DOF, 3 coupling DOFs, -1 beyond the count) and the
+        coupling weights (0 beyond), in the sorted order."""
+        n = lib().orc_hanging(self._p, None, None)
+        out = np.zeros(4 * max(n, 1), dtype=np.int32)
+        w = np.zeros(3 * max(n, 1))
+        lib().orc_hanging(self._p, _p(out, ctypes.c_int), _p(w))
+        return out[:4 * n].reshape(n, 4), w[:3 * n].reshape(n, 3)
 
     def __del__(self):
         if getattr(self, "_p", None):

@@ -66,7 +66,8 @@ static int jv(int s, int j, int k) {
    downwind branch of the Kuzmin Jacobian compares E_kj unrounded; 6: the
    hanging coupling DOFs that hang themselves receive nothing; 7: hanging
    rows added in list order instead of the sorted order; 8: the 1-irregular
-   closure of the refinement by indicator skipped. Set only by tests, through
+   closure of the refinement by indicator skipped; 9: the hanging-DOF
+   coupling of the assembled system skipped. Set only by tests, through
    orc_set_fault. */
 static int orc_fault = 0;
 void orc_set_fault(int code) { orc_fault = code; }
@@ -898,10 +899,13 @@ typedef struct {
   int* nb_joint;
   int* comp;    /* boundary component per joint, -1 interior */
   double *aval, *mval; /* time-dependent problems: stiffness and mass matrices */
-  /* hanging vertices: DOFs nactive_nh .. nactive-1, per hanging DOF its two
-     coupling DOFs (weights 1/2), and the topological order of the list */
+  /* hanging DOFs nactive_nh .. nactive-1: per hanging DOF hang_ncpl[h]
+     coupling DOFs hang_cpl[3h + k] with weights hang_w[3h + k], and the
+     topological order of the list */
   int nhang, nactive_nh;
+  int* hang_ncpl;
   int* hang_cpl;
+  double* hang_w;
   int* hang_order;
 } olevel;
 
@@ -1300,105 +1304,199 @@ static void build_structure(olevel* L) {
   free(buf); free(cells); free(fill); free(cnt);
 }
 
-/* ---- spaces on meshes with hanging vertices (P1/Q1) ----
+/* ---- spaces on meshes with hanging vertices (P1/Q1, P2/Q2) ----
    TFESpace::compute_various_numbers (src/FE/FESpace.C:1863-2099) orders the
    DOFs as inner, non-Dirichlet boundary kinds, hanging, Dirichlet; a DOF on
    boundary joints of several kinds takes the lowest kind (Dirichlet first,
-   as map_single_boundary_dof keeps the later, larger mark). A P1/Q1 DOF sits
-   at a vertex, so DOFs are identified through vertex ids; within a class
-   they are numbered in first appearance over (cell, local DOF), hanging DOFs
-   in the order their coarse edges are met. A vertex is hanging when it is
-   the midpoint of a leaf's edge (FE2DMapper1Reg.C:43-58,
-   C1_2_C1_2_1Reg: one hanging DOF per refined edge, coupled to the edge's
-   end DOFs with HN_C_P1_2D_0 = {1/2, 1/2}, HNDesc.C:9-11).
+   as map_single_boundary_dof keeps the later, larger mark). DOFs are
+   identified through the mesh entity at their nodal point (vertex, edge,
+   cell interior); within a class they are numbered in first appearance over
+   (cell, local DOF), hanging DOFs in the order their coarse edges are met.
+   A leaf edge (u, w) whose midpoint m exists is refined on the other side
+   (FE2DMapper1Reg.C:43-78, the one-regular mappers):
+   - P1/Q1 (C1_2_C1_2_1Reg): the vertex m hangs, coupled to u and w with
+     HN_C_P1_2D_0 = {1/2, 1/2} (HNDesc.C:9-11);
+   - P2/Q2 (C2_2_C2_2_1Reg): the coarse edge's middle DOF and the fine
+     vertex m are one DOF; the middle DOFs of the fine edges (u, m) and
+     (m, w), at 1/4 and 3/4 of the coarse edge, hang, coupled to the coarse
+     edge's DOFs (u, m, w) with the quadratic Lagrange values there,
+     {3/8, 3/4, -1/8} and {-1/8, 3/4, 3/8} (HN_C_P2_2D_1 and _0,
+     HNDesc.C:12-17).
    sort_hanging_nodes (FESpace.C:2101-2160): depth-first post-order over the
    coupling DOFs that are hanging themselves, then reversed, so that a
    hanging DOF comes before the hanging DOFs it couples with. */
+#define MAXCPL 3
 static void hang_visit(const olevel* L, int h, char* seen, int* out, int* n) {
   seen[h] = 1;
-  for(int k = 0; k < 2; ++k) {
-    const int c = L->hang_cpl[2 * h + k] - L->nactive_nh;
+  for(int k = 0; k < L->hang_ncpl[h]; ++k) {
+    const int c = L->hang_cpl[MAXCPL * h + k] - L->nactive_nh;
     if(c >= 0 && c < L->nhang && !seen[c]) hang_visit(L, c, seen, out, n);
   }
   out[(*n)++] = h;
 }
 
+static void dict_init(vdict* d, int cap) {
+  d->cap = cap;
+  d->slot = malloc(sizeof(int) * cap);
+  d->key = malloc(sizeof(int[4]) * cap);
+  for(int i = 0; i < cap; ++i) d->slot[i] = -1;
+}
+
+/* value stored for the edge (u, w), or -1 */
+static int edge_class(const vdict* d, int u, int w) {
+  int key[4] = {u < w ? u : w, u < w ? w : u, -1, -1};
+  return vertex_find(d, key);
+}
+
+static void edge_class_set(vdict* d, int u, int w, int v) {
+  int key[4] = {u < w ? u : w, u < w ? w : u, -1, -1};
+  uint64_t h = hash4(key) & (uint64_t)(d->cap - 1);
+  while(d->slot[h] >= 0) h = (h + 1) & (uint64_t)(d->cap - 1);
+  d->slot[h] = v;
+  memcpy(d->key[h], key, sizeof(key));
+}
+
 static int build_space_hanging(orc_problem* P, olevel* L, const vdict* mids) {
   const omesh* m = &L->m;
-  const int s = m->shape, nj = NJ[s], nv = NV[s], nl = P->el.nl;
-  if(nl != nv) return -1;
+  const int s = m->shape, nj = NJ[s], nv = NV[s], nl = P->el.nl, order = P->el.order;
+  if(order != 1 && order != 2) return -1;
   L->nl = nl;
-  /* local DOF i sits at cell vertex lv[i] (the same for every cell) */
-  int lv[MAXNL];
+  /* entity of local DOF i: vertex a (etype 0), edge j (1) or interior (2) */
+  int etype[MAXNL], eidx[MAXNL];
   {
     ogeo g;
     geo(m, 0, &g);
     for(int i = 0; i < nl; ++i) {
       double x[3];
       geo_point(&g, P->el.dim, P->el.node[i], x);
-      lv[i] = -1;
+      etype[i] = 2;
+      eidx[i] = -1;
       for(int a = 0; a < nv; ++a) {
         const double* X = &m->X[3 * m->cv[a]];
-        if(fabs(X[0] - x[0]) + fabs(X[1] - x[1]) + fabs(X[2] - x[2]) < 1e-12) lv[i] = a;
+        if(fabs(X[0] - x[0]) + fabs(X[1] - x[1]) + fabs(X[2] - x[2]) < 1e-12) { etype[i] = 0; eidx[i] = a; }
       }
-      if(lv[i] < 0) return -1;
+      for(int j = 0; j < nj && etype[i] == 2; ++j) {
+        const double* A = &m->X[3 * m->cv[jv(s, j, 0)]];
+        const double* B = &m->X[3 * m->cv[jv(s, j, 1)]];
+        double d = 0;
+        for(int k = 0; k < 3; ++k) d += fabs(0.5 * (A[k] + B[k]) - x[k]);
+        if(d < 1e-12) { etype[i] = 1; eidx[i] = j; }
+      }
     }
   }
-  const int NVt = m->nverts;
-  int* kind = malloc(sizeof(int) * NVt); /* NBT + 1 inner, NBT hanging, else boundary kind */
-  for(int v = 0; v < NVt; ++v) kind[v] = NBT + 1;
+  /* classes in first appearance */
+  const int NVt = m->nverts, N = m->ncells * nl;
+  int* vcls = malloc(sizeof(int) * NVt);
+  for(int v = 0; v < NVt; ++v) vcls[v] = -1;
+  vdict ed;
+  {
+    int cap = 1;
+    while(cap < 4 * (m->ncells * nj + 16)) cap <<= 1;
+    dict_init(&ed, cap);
+  }
+  int* cls = malloc(sizeof(int) * N);
+  int ncls = 0;
+  for(int c = 0; c < m->ncells; ++c)
+    for(int i = 0; i < nl; ++i) {
+      int k;
+      if(etype[i] == 0) {
+        const int v = m->cv[c * nv + eidx[i]];
+        if(vcls[v] < 0) vcls[v] = ncls++;
+        k = vcls[v];
+      } else if(etype[i] == 1) {
+        const int u = m->cv[c * nv + jv(s, eidx[i], 0)], w = m->cv[c * nv + jv(s, eidx[i], 1)];
+        const int mm = edge_mid(mids, u, w);
+        if(mm >= 0) { /* the coarse edge's middle DOF is the fine vertex */
+          if(vcls[mm] < 0) vcls[mm] = ncls++;
+          k = vcls[mm];
+        } else {
+          k = edge_class(&ed, u, w);
+          if(k < 0) { k = ncls++; edge_class_set(&ed, u, w, k); }
+        }
+      } else {
+        k = ncls++;
+      }
+      cls[c * nl + i] = k;
+    }
+  int* kind = malloc(sizeof(int) * ncls); /* NBT + 1 inner, NBT hanging, else boundary kind */
+  for(int k = 0; k < ncls; ++k) kind[k] = NBT + 1;
   for(int c = 0; c < m->ncells; ++c)
     for(int j = 0; j < nj; ++j) {
       if(L->nb_cell[c * nj + j] != -1) continue;
-      const int k = ex_bc(&P->ex, L->comp[c * nj + j]);
-      for(int a = 0; a < 2; ++a) {
-        const int v = m->cv[c * nv + jv(s, j, a)];
-        if(k < kind[v]) kind[v] = k;
+      const int b = ex_bc(&P->ex, L->comp[c * nj + j]);
+      int jd[MAXNL];
+      const int n = joint_dofs(P, j, jd);
+      for(int q = 0; q < n; ++q) {
+        const int k = cls[c * nl + jd[q]];
+        if(b < kind[k]) kind[k] = b;
       }
     }
-  /* hanging vertices in the order their coarse edges are met */
-  int* hv = malloc(sizeof(int) * 3 * (NVt + 1));
+  /* hanging classes, couplings and weights in the order the coarse edges are met */
+  int cap_h = 2 * (m->ncells * nj + 1);
+  int* hcls = malloc(sizeof(int) * cap_h);
+  int* hcpl = malloc(sizeof(int) * MAXCPL * cap_h);
+  double* hw = malloc(sizeof(double) * MAXCPL * cap_h);
+  int* hn = malloc(sizeof(int) * cap_h);
+  char* met = calloc(NVt, 1);
   int nh = 0;
   for(int c = 0; c < m->ncells; ++c)
     for(int j = 0; j < nj; ++j) {
       if(L->nb_cell[c * nj + j] != -2) continue;
       const int u = m->cv[c * nv + jv(s, j, 0)], w = m->cv[c * nv + jv(s, j, 1)];
       const int mm = edge_mid(mids, u, w);
-      if(mm < 0 || kind[mm] == NBT) continue;
-      kind[mm] = NBT;
-      hv[3 * nh] = mm; hv[3 * nh + 1] = u; hv[3 * nh + 2] = w;
-      nh++;
+      if(mm < 0 || met[mm]) continue;
+      met[mm] = 1;
+      if(order == 1) {
+        hcls[nh] = vcls[mm];
+        hn[nh] = 2;
+        hcpl[MAXCPL * nh] = vcls[u]; hcpl[MAXCPL * nh + 1] = vcls[w];
+        hw[MAXCPL * nh] = 0.5; hw[MAXCPL * nh + 1] = 0.5;
+        kind[hcls[nh]] = NBT;
+        nh++;
+      } else {
+        const int half[2][2] = {{u, mm}, {mm, w}};
+        const double wt[2][3] = {{0.375, 0.75, -0.125}, {-0.125, 0.75, 0.375}};
+        for(int q = 0; q < 2; ++q) {
+          const int k = edge_class(&ed, half[q][0], half[q][1]);
+          if(k < 0) { free(cls); free(kind); free(vcls); free(hcls); free(hcpl); free(hw); free(hn); free(met);
+                      free(ed.slot); free(ed.key); return -1; }
+          hcls[nh] = k;
+          hn[nh] = 3;
+          hcpl[MAXCPL * nh] = vcls[u]; hcpl[MAXCPL * nh + 1] = vcls[mm]; hcpl[MAXCPL * nh + 2] = vcls[w];
+          for(int t = 0; t < 3; ++t) hw[MAXCPL * nh + t] = wt[q][t];
+          kind[k] = NBT;
+          nh++;
+        }
+      }
     }
-  int* num = malloc(sizeof(int) * NVt);
-  for(int v = 0; v < NVt; ++v) num[v] = -1;
+  int* num = malloc(sizeof(int) * ncls);
+  for(int k = 0; k < ncls; ++k) num[k] = -1;
   int next = 0;
   /* inner, then non-Dirichlet boundary kinds 1..NBT-1 */
   for(int o = 0; o < NBT; ++o) {
     const int want = o == 0 ? NBT + 1 : o;
-    for(int c = 0; c < m->ncells; ++c)
-      for(int i = 0; i < nl; ++i) {
-        const int v = m->cv[c * nv + lv[i]];
-        if(kind[v] == want && num[v] < 0) num[v] = next++;
-      }
+    for(int p = 0; p < N; ++p)
+      if(kind[cls[p]] == want && num[cls[p]] < 0) num[cls[p]] = next++;
   }
   L->nactive_nh = next;
-  for(int h = 0; h < nh; ++h) num[hv[3 * h]] = next++;
+  for(int h = 0; h < nh; ++h) num[hcls[h]] = next++;
   L->nactive = next;
-  for(int c = 0; c < m->ncells; ++c)
-    for(int i = 0; i < nl; ++i) {
-      const int v = m->cv[c * nv + lv[i]];
-      if(kind[v] == 0 && num[v] < 0) num[v] = next++;
-    }
+  for(int p = 0; p < N; ++p)
+    if(kind[cls[p]] == 0 && num[cls[p]] < 0) num[cls[p]] = next++;
   L->ndof = next;
   L->ndir = L->ndof - L->nactive;
-  L->dof = malloc(sizeof(int) * m->ncells * nl);
-  for(int c = 0; c < m->ncells; ++c)
-    for(int i = 0; i < nl; ++i) L->dof[c * nl + i] = num[m->cv[c * nv + lv[i]]];
+  L->dof = malloc(sizeof(int) * N);
+  for(int p = 0; p < N; ++p) L->dof[p] = num[cls[p]];
   L->nhang = nh;
-  L->hang_cpl = malloc(sizeof(int) * 2 * (nh + 1));
+  L->hang_ncpl = malloc(sizeof(int) * (nh + 1));
+  L->hang_cpl = malloc(sizeof(int) * MAXCPL * (nh + 1));
+  L->hang_w = malloc(sizeof(double) * MAXCPL * (nh + 1));
   for(int h = 0; h < nh; ++h) {
-    L->hang_cpl[2 * h] = num[hv[3 * h + 1]];
-    L->hang_cpl[2 * h + 1] = num[hv[3 * h + 2]];
+    L->hang_ncpl[h] = hn[h];
+    for(int t = 0; t < hn[h]; ++t) {
+      L->hang_cpl[MAXCPL * h + t] = num[hcpl[MAXCPL * h + t]];
+      L->hang_w[MAXCPL * h + t] = hw[MAXCPL * h + t];
+    }
   }
   L->hang_order = malloc(sizeof(int) * (nh + 1));
   char* seen = calloc(nh + 1, 1);
@@ -1406,7 +1504,8 @@ static int build_space_hanging(orc_problem* P, olevel* L, const vdict* mids) {
   for(int h = 0; h < nh; ++h)
     if(!seen[h]) hang_visit(L, h, seen, L->hang_order, &n);
   for(int a = 0, b = n - 1; a < b; ++a, --b) { int t = L->hang_order[a]; L->hang_order[a] = L->hang_order[b]; L->hang_order[b] = t; }
-  free(seen); free(num); free(hv); free(kind);
+  free(seen); free(num); free(cls); free(kind); free(vcls); free(hcls); free(hcpl); free(hw); free(hn); free(met);
+  free(ed.slot); free(ed.key);
   return 0;
 }
 
@@ -1420,13 +1519,27 @@ static void hanging_structure(olevel* L) {
   int* len = malloc(sizeof(int) * n);
   for(int r = 0; r < n; ++r) {
     len[r] = L->rp[r + 1] - L->rp[r];
-    rows[r] = malloc(sizeof(int) * len[r]);
+    rows[r] = malloc(sizeof(int) * (len[r] + MAXCPL));
     memcpy(rows[r], &L->col[L->rp[r]], sizeof(int) * len[r]);
+  }
+  /* a hanging row holds its coupling DOFs, also those outside its cells
+     (add_columns_far_hanging_coupling, Structure.C:142-170: P2 and up) */
+  for(int h = 0; h < L->nhang; ++h) {
+    const int hd = L->nactive_nh + h;
+    for(int k = 0; k < L->hang_ncpl[h]; ++k) {
+      const int c = L->hang_cpl[MAXCPL * h + k];
+      int pos = 0;
+      while(pos < len[hd] && rows[hd][pos] < c) pos++;
+      if(pos < len[hd] && rows[hd][pos] == c) continue;
+      memmove(&rows[hd][pos + 1], &rows[hd][pos], sizeof(int) * (len[hd] - pos));
+      rows[hd][pos] = c;
+      len[hd]++;
+    }
   }
   for(int t = 0; t < L->nhang; ++t) {
     const int h = L->hang_order[t], hd = L->nactive_nh + h;
-    for(int k = 0; k < 2; ++k) {
-      const int c = L->hang_cpl[2 * h + k];
+    for(int k = 0; k < L->hang_ncpl[h]; ++k) {
+      const int c = L->hang_cpl[MAXCPL * h + k];
       if(c >= L->nactive) continue;
       int* merged = malloc(sizeof(int) * (len[c] + len[hd]));
       int a = 0, b = 0, u = 0;
@@ -1476,21 +1589,22 @@ static void apply_hanging(olevel* L) {
   const int bound = orc_fault == 6 ? L->nactive_nh : L->nactive;
   for(int t = 0; t < L->nhang; ++t) {
     const int h = orc_fault == 7 ? t : L->hang_order[t], hd = L->nactive_nh + h;
-    for(int k = 0; k < 2; ++k) {
-      const int c = L->hang_cpl[2 * h + k];
+    for(int k = 0; k < L->hang_ncpl[h]; ++k) {
+      const int c = L->hang_cpl[MAXCPL * h + k];
+      const double w = L->hang_w[MAXCPL * h + k];
       if(c >= bound) continue;
       for(int e = L->rp[hd]; e < L->rp[hd + 1]; ++e) {
         double* dst = entry_at(L, c, L->col[e]);
         if(!dst) { fprintf(stderr, "oracle: hanging entry missing\n"); continue; }
-        *dst += 0.5 * L->val[e];
+        *dst += w * L->val[e];
       }
     }
   }
   for(int t = 0; t < L->nhang; ++t) {
     const int h = orc_fault == 7 ? t : L->hang_order[t], hd = L->nactive_nh + h;
-    for(int k = 0; k < 2; ++k) {
-      const int c = L->hang_cpl[2 * h + k];
-      if(c < bound) L->rhs[c] += 0.5 * L->rhs[hd];
+    for(int k = 0; k < L->hang_ncpl[h]; ++k) {
+      const int c = L->hang_cpl[MAXCPL * h + k];
+      if(c < bound) L->rhs[c] += L->hang_w[MAXCPL * h + k] * L->rhs[hd];
     }
   }
   for(int h = 0; h < L->nhang; ++h) L->rhs[L->nactive_nh + h] = 0;
@@ -1498,7 +1612,10 @@ static void apply_hanging(olevel* L) {
     const int hd = L->nactive_nh + h;
     for(int e = L->rp[hd]; e < L->rp[hd + 1]; ++e) {
       const int col = L->col[e];
-      L->val[e] = col == hd ? 1.0 : (col == L->hang_cpl[2 * h] || col == L->hang_cpl[2 * h + 1] ? -0.5 : 0.0);
+      double v = col == hd ? 1.0 : 0.0;
+      for(int k = 0; k < L->hang_ncpl[h]; ++k)
+        if(col == L->hang_cpl[MAXCPL * h + k]) v = -L->hang_w[MAXCPL * h + k];
+      L->val[e] = v;
     }
   }
 }
@@ -2325,7 +2442,7 @@ void orc_destroy(orc_problem* P);
 
 /* a problem on the leaves of `adapt_steps` indicator refinements
    (TDomain::RefineByIndicator without conforming closure) of the coarse
-   mesh refined `n_refine` times: one level, P1 or Q1 only (NULL otherwise);
+   mesh refined `n_refine` times: one level, 2D, order 1 or 2 (NULL otherwise);
    indicator(x, ctx) is evaluated at the cells' vertices */
 orc_problem* orc_create_adaptive(int geometry, int n_refine, int adapt_steps,
                                  double (*indicator)(const double*, void*), void* ctx, int order,
@@ -2333,7 +2450,7 @@ orc_problem* orc_create_adaptive(int geometry, int n_refine, int adapt_steps,
                                  double f0) {
   omesh cur;
   default_mesh(geometry, &cur);
-  if(order != 1 || DIM[cur.shape] != 2) { mesh_free(&cur); return NULL; }
+  if((order != 1 && order != 2) || DIM[cur.shape] != 2) { mesh_free(&cur); return NULL; }
   for(int i = 0; i < n_refine; ++i) {
     omesh next;
     refine(&cur, &next);
@@ -2371,17 +2488,20 @@ orc_problem* orc_create_adaptive(int geometry, int n_refine, int adapt_steps,
   return P;
 }
 
-/* hanging DOFs: out[3h..3h+2] = hanging DOF and its two coupling DOFs, in
-   the sorted order; returns the count */
-int orc_hanging(orc_problem* P, int* out) {
+/* hanging DOFs in the sorted order: out[4t] = hanging DOF, out[4t+1..3] =
+   its coupling DOFs (-1 beyond the count), w[3t..3t+2] their weights (0
+   beyond); returns the count */
+int orc_hanging(orc_problem* P, int* out, double* w) {
   olevel* L = &P->lev[P->nlev - 1];
-  if(out)
-    for(int t = 0; t < L->nhang; ++t) {
-      const int h = L->hang_order[t];
-      out[3 * t] = L->nactive_nh + h;
-      out[3 * t + 1] = L->hang_cpl[2 * h];
-      out[3 * t + 2] = L->hang_cpl[2 * h + 1];
+  for(int t = 0; t < L->nhang; ++t) {
+    const int h = L->hang_order[t];
+    if(out) {
+      out[4 * t] = L->nactive_nh + h;
+      for(int k = 0; k < MAXCPL; ++k) out[4 * t + 1 + k] = k < L->hang_ncpl[h] ? L->hang_cpl[MAXCPL * h + k] : -1;
     }
+    if(w)
+      for(int k = 0; k < MAXCPL; ++k) w[3 * t + k] = k < L->hang_ncpl[h] ? L->hang_w[MAXCPL * h + k] : 0.0;
+  }
   return L->nhang;
 }
 
@@ -2393,7 +2513,7 @@ void orc_destroy(orc_problem* P) {
     free(L->dof); free(L->rp); free(L->col); free(L->val); free(L->rhs); free(L->sol);
     free(L->mg_sol); free(L->mg_rhs); free(L->mg_def); free(L->mg_aux); free(L->lu); free(L->piv);
     free(L->nb_cell); free(L->nb_joint); free(L->comp); free(L->qq); free(L->diag);
-    free(L->aval); free(L->mval); free(L->hang_cpl); free(L->hang_order);
+    free(L->aval); free(L->mval); free(L->hang_cpl); free(L->hang_order); free(L->hang_ncpl); free(L->hang_w);
   }
   free(P->u_old); free(P->u_old2); free(P->f_old);
   free(P->lev);
@@ -2430,7 +2550,7 @@ static void extract_diag(olevel* L) {
 void orc_assemble(orc_problem* P) {
   for(int l = 0; l < P->nlev; ++l) {
     assemble_level(P, &P->lev[l], 0, P->lev[l].m.ncells);
-    if(P->lev[l].nhang) apply_hanging(&P->lev[l]);
+    if(P->lev[l].nhang && orc_fault != 9) apply_hanging(&P->lev[l]);
     extract_diag(&P->lev[l]);
   }
   olevel* F = lvl(P, -1);

@@ -226,22 +226,25 @@ int pmcd_host_sell_layout(pmcd_host* h, int level, int* perm, int* len, long lon
   });
 }
 
-static int copy_hanging(const Level& L, int* out, int* n) {
+static int copy_hanging(const Level& L, int* out, double* w, int* n) {
   PMCD_CHECK(n != nullptr, "null argument");
   const FESpace& sp = L.space;
+  constexpr int MC = FESpace::kMaxCoupling;
   *n = sp.n_hanging;
-  if(out)
-    for(int t = 0; t < sp.n_hanging; ++t) {
-      const int h = sp.hang_order[t];
-      out[3 * t] = sp.n_active_non_hanging() + h;
-      out[3 * t + 1] = sp.hang_cpl[2 * h];
-      out[3 * t + 2] = sp.hang_cpl[2 * h + 1];
+  for(int t = 0; t < sp.n_hanging; ++t) {
+    const int h = sp.hang_order[t];
+    if(out) {
+      out[4 * t] = sp.n_active_non_hanging() + h;
+      for(int k = 0; k < MC; ++k) out[4 * t + 1 + k] = k < sp.hang_n[h] ? sp.hang_cpl[MC * h + k] : -1;
     }
+    if(w)
+      for(int k = 0; k < MC; ++k) w[3 * t + k] = k < sp.hang_n[h] ? sp.hang_w[MC * h + k] : 0.0;
+  }
   return 0;
 }
 
-int pmcd_host_hanging(pmcd_host* h, int level, int* out, int* n) {
-  return guarded([&] { copy_hanging(host_level(h, level), out, n); });
+int pmcd_host_hanging(pmcd_host* h, int level, int* out, double* w, int* n) {
+  return guarded([&] { copy_hanging(host_level(h, level), out, w, n); });
 }
 
 int pmcd_host_write_mesh(pmcd_host* h, int level, const char* path) {
@@ -327,8 +330,8 @@ int pmcd_system_info(pmcd_system* s, int level, long long* info) {
   });
 }
 
-int pmcd_get_hanging(pmcd_system* s, int level, int* out, int* n) {
-  return guarded_on(s, [&] { copy_hanging(level_of(s, level), out, n); });
+int pmcd_get_hanging(pmcd_system* s, int level, int* out, double* w, int* n) {
+  return guarded_on(s, [&] { copy_hanging(level_of(s, level), out, w, n); });
 }
 
 int pmcd_get_dofmap(pmcd_system* s, int level, int* out) {

@@ -28,6 +28,18 @@ FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc, const 
   std::vector<int> vertex_class(mesh.n_vertices(), -1);
   std::unordered_map<std::array<int, 4>, int, Key3Hash> entity_class;
   entity_class.reserve((size_t)nc * 4);
+  auto edge_key = [](int a, int b) {
+    return std::array<int, 4>{std::min(a, b), std::max(a, b), -1, -1};
+  };
+  // order 2 with hanging vertices: the middle DOF of a refined coarse edge
+  // is the DOF of the fine vertex at its midpoint
+  std::unordered_map<std::array<int, 4>, int, Key3Hash> edge_midpoint;
+  const bool has_hanging = hanging && !hanging->empty();
+  if(has_hanging) {
+    PMCD_CHECK((order == 1 || order == 2) && mesh.dim() == 2, "hanging vertices are implemented for 2D, order 1 or 2");
+    if(order == 2)
+      for(const HangingVertex& hv : *hanging) edge_midpoint.emplace(edge_key(hv.a, hv.b), hv.vertex);
+  }
   int n_classes = 0;
   const int full = shape_n_vertices(mesh.shape);
   for(int c = 0; c < nc; ++c) {
@@ -47,9 +59,16 @@ FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc, const 
         for(size_t a = 0; a < ev.size(); ++a) g[a] = cv[ev[a]];
         std::sort(g, g + ev.size());
         for(size_t a = 0; a < ev.size(); ++a) key[a] = g[a];
-        auto it = entity_class.find(key);
-        if(it == entity_class.end()) { k = n_classes++; entity_class.emplace(key, k); }
-        else k = it->second;
+        auto mid = edge_midpoint.find(key);
+        if(mid != edge_midpoint.end()) {
+          int& vc = vertex_class[mid->second];
+          if(vc < 0) vc = n_classes++;
+          k = vc;
+        } else {
+          auto it = entity_class.find(key);
+          if(it == entity_class.end()) { k = n_classes++; entity_class.emplace(key, k); }
+          else k = it->second;
+        }
       }
       cls[(size_t)c * nl + i] = k;
     }
@@ -74,17 +93,39 @@ FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc, const 
       }
     }
 
-  // hanging vertices: their own kind, numbered in the list's order
+  // hanging DOFs: their own kind, numbered in the list's order; couplings
+  // and weights by class
+  constexpr int MC = FESpace::kMaxCoupling;
   const int HANGING = INT_MAX - 1;
   std::vector<int> hang_index(n_classes, -1);
-  if(hanging && !hanging->empty()) {
-    PMCD_CHECK(order == 1 && mesh.dim() == 2, "hanging vertices are implemented for P1/Q1 in 2D");
-    for(size_t h = 0; h < hanging->size(); ++h) {
-      const int k = vertex_class[(*hanging)[h].vertex];
-      PMCD_CHECK(k >= 0 && vertex_class[(*hanging)[h].a] >= 0 && vertex_class[(*hanging)[h].b] >= 0,
-                 "hanging vertex without a DOF");
-      kind[k] = HANGING;
-      hang_index[k] = (int)h;
+  std::vector<int> hang_class, hang_n;
+  std::vector<int> hang_cpl_class;
+  std::vector<double> hang_w;
+  if(has_hanging) {
+    for(const HangingVertex& hv : *hanging) {
+      const int ka = vertex_class[hv.a], kb = vertex_class[hv.b], km = vertex_class[hv.vertex];
+      PMCD_CHECK(ka >= 0 && kb >= 0 && km >= 0, "hanging vertex without a DOF");
+      if(order == 1) {
+        hang_class.push_back(km);
+        hang_n.push_back(2);
+        hang_cpl_class.insert(hang_cpl_class.end(), {ka, kb, -1});
+        hang_w.insert(hang_w.end(), {0.5, 0.5, 0.0});
+      } else {
+        const int half[2][2] = {{hv.a, hv.vertex}, {hv.vertex, hv.b}};
+        const double wt[2][3] = {{0.375, 0.75, -0.125}, {-0.125, 0.75, 0.375}};
+        for(int q = 0; q < 2; ++q) {
+          auto it = entity_class.find(edge_key(half[q][0], half[q][1]));
+          PMCD_CHECK(it != entity_class.end(), "fine edge of a hanging vertex without a DOF");
+          hang_class.push_back(it->second);
+          hang_n.push_back(3);
+          hang_cpl_class.insert(hang_cpl_class.end(), {ka, km, kb});
+          hang_w.insert(hang_w.end(), {wt[q][0], wt[q][1], wt[q][2]});
+        }
+      }
+    }
+    for(size_t h = 0; h < hang_class.size(); ++h) {
+      kind[hang_class[h]] = HANGING;
+      hang_index[hang_class[h]] = (int)h;
     }
   }
 
@@ -112,11 +153,12 @@ FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc, const 
   for(size_t p = 0; p < sp.dof.size(); ++p) sp.dof[p] = number[cls[p]];
   if(sp.n_hanging) {
     const int nh = sp.n_hanging, base = sp.n_active_non_hanging();
-    sp.hang_cpl.resize(2 * (size_t)nh);
-    for(int h = 0; h < nh; ++h) {
-      sp.hang_cpl[2 * h] = number[vertex_class[(*hanging)[h].a]];
-      sp.hang_cpl[2 * h + 1] = number[vertex_class[(*hanging)[h].b]];
-    }
+    PMCD_CHECK(nh == (int)hang_class.size(), "hanging DOF shared by two coarse edges");
+    sp.hang_n = hang_n;
+    sp.hang_w = hang_w;
+    sp.hang_cpl.assign((size_t)MC * nh, -1);
+    for(int h = 0; h < nh; ++h)
+      for(int k = 0; k < hang_n[h]; ++k) sp.hang_cpl[MC * h + k] = number[hang_cpl_class[MC * h + k]];
     // depth-first post-order over hanging coupling DOFs, reversed
     std::vector<char> seen(nh, 0);
     std::vector<int> post;
@@ -128,8 +170,8 @@ FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc, const 
       stack.push_back({r, 0});
       while(!stack.empty()) {
         auto& top = stack.back();
-        if(top.second < 2) {
-          const int c = sp.hang_cpl[2 * top.first + top.second++] - base;
+        if(top.second < sp.hang_n[top.first]) {
+          const int c = sp.hang_cpl[MC * top.first + top.second++] - base;
           if(c >= 0 && c < nh && !seen[c]) {
             seen[c] = 1;
             stack.push_back({c, 0});

@@ -35,21 +35,30 @@ struct FESpace {
   int n_inner = 0;
   std::vector<int> dof;  // n_cells * n_local, global DOF numbers
   // hanging DOFs n_active - n_hanging .. n_active - 1: hanging DOF h couples
-  // with hang_cpl[2h], hang_cpl[2h+1] (weights 1/2); hang_order lists the
-  // hanging DOFs (0-based) so that each comes before the hanging DOFs it
-  // couples with (sort_hanging_nodes, src/FE/FESpace.C:2101-2160)
+  // with hang_n[h] DOFs hang_cpl[kMaxCoupling * h + k], weights
+  // hang_w[kMaxCoupling * h + k]; hang_order lists the hanging DOFs (0-based)
+  // so that each comes before the hanging DOFs it couples with
+  // (sort_hanging_nodes, src/FE/FESpace.C:2101-2160)
+  static constexpr int kMaxCoupling = 3;
   int n_hanging = 0;
+  std::vector<int> hang_n;
   std::vector<int> hang_cpl;
+  std::vector<double> hang_w;
   std::vector<int> hang_order;
   const int* cell_dof(int c) const { return &dof[(size_t)c * n_local]; }
   int n_active_non_hanging() const { return n_active - n_hanging; }
 };
 
-// `hanging` (P1/Q1 only): hanging vertices of the mesh; their DOFs follow the
-// non-Dirichlet boundary DOFs, in the list's order (FESpace.C:1863-2099
-// numbers them between the active and the Dirichlet DOFs), each coupled to
-// the DOFs at the ends of its coarse edge with the weights 1/2 of
-// HN_C_P1_2D_0 (HNDesc.C:9-11, FE2DMapper1Reg.C:43-58).
+// `hanging` (2D, order 1 or 2): hanging vertices of the mesh. Their edges'
+// hanging DOFs follow the non-Dirichlet boundary DOFs, in the list's order
+// (FESpace.C:1863-2099 numbers them between the active and the Dirichlet
+// DOFs), as the one-regular mappers make them (FE2DMapper1Reg.C:43-78):
+// order 1: the vertex m of the coarse edge (a, b) hangs, coupled to a and b
+// with weights 1/2 (HN_C_P1_2D_0, HNDesc.C:9-11); order 2: the coarse edge's
+// middle DOF is the fine vertex m, and the middle DOFs of the fine edges
+// (a, m), (m, b) hang, coupled to (a, m, b) with the quadratic Lagrange
+// values at 1/4 and 3/4 of the edge, {3/8, 3/4, -1/8} and {-1/8, 3/4, 3/8}
+// (HN_C_P2_2D_1 / _0, HNDesc.C:12-17).
 FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc,
                       const std::vector<HangingVertex>* hanging = nullptr);
 

@@ -135,13 +135,24 @@ Structure build_structure(const FESpace& sp, int n_cells) {
 // it received. Entries the reference pattern may hold beyond these stay zero.
 Structure hanging_structure(const FESpace& sp, int n_cells, const Structure& base) {
   const int n = base.n_rows, first = sp.n_active_non_hanging();
+  constexpr int MC = FESpace::kMaxCoupling;
   std::vector<std::vector<int>> rows(n);
   for(int r = 0; r < n; ++r) rows[r].assign(base.col.begin() + base.row_ptr[r], base.col.begin() + base.row_ptr[r + 1]);
+  // a hanging row holds its coupling DOFs, also those outside its cells
+  // (add_columns_far_hanging_coupling, Structure.C:142-170: order 2)
+  for(int h = 0; h < sp.n_hanging; ++h) {
+    auto& row = rows[first + h];
+    for(int k = 0; k < sp.hang_n[h]; ++k) {
+      const int c = sp.hang_cpl[MC * h + k];
+      auto it = std::lower_bound(row.begin(), row.end(), c);
+      if(it == row.end() || *it != c) row.insert(it, c);
+    }
+  }
   std::vector<int> merged;
   for(int h : sp.hang_order) {
     const int hd = first + h;
-    for(int k = 0; k < 2; ++k) {
-      const int c = sp.hang_cpl[2 * h + k];
+    for(int k = 0; k < sp.hang_n[h]; ++k) {
+      const int c = sp.hang_cpl[MC * h + k];
       if(c >= sp.n_active) continue;
       merged.clear();
       std::set_union(rows[c].begin(), rows[c].end(), rows[hd].begin(), rows[hd].end(), std::back_inserter(merged));

@@ -62,7 +62,7 @@ std::vector<std::unique_ptr<Level>> build_levels(const ProblemDesc& d) {
   std::vector<std::unique_ptr<Level>> levels;
   if(d.adapt_steps > 0) {
     PMCD_CHECK(d.indicator, "refinement by indicator without an indicator function");
-    PMCD_CHECK(d.order == 1 && meshes.back().dim() == 2, "hanging vertices are implemented for P1/Q1 in 2D");
+    PMCD_CHECK(meshes.back().dim() == 2, "hanging vertices are implemented for 2D meshes");
     PMCD_CHECK(d.mg_levels <= 1, "multigrid on meshes refined by indicator is not implemented");
     auto L = std::make_unique<Level>();
     std::vector<HangingVertex> hanging;
@@ -304,7 +304,7 @@ void CDSystem::set_example(const ExampleParams& ex) {
 namespace {
 
 // Gather lists of the hanging-DOF coupling (see k_hanging.hip). Hanging DOF h
-// adds its row, weighted 1/2, to the rows of its coupling DOFs below n_active
+// adds its row, weighted, to the rows of its coupling DOFs below n_active
 // (hanging ones included); a target entry (c, col) therefore receives the
 // entries (h, col) of the hanging rows coupled to c, in the sorted order. A
 // hanging row is final once all rows coupled to it have been added: depth(h)
@@ -315,13 +315,15 @@ void hanging_plan(Level& L, hipStream_t stream) {
   const FESpace& sp = L.space;
   const Structure& st = L.st;
   const int nh = sp.n_hanging, first = sp.n_active_non_hanging();
+  constexpr int MC = FESpace::kMaxCoupling;
   std::vector<int> depth(nh, 0);
-  std::vector<std::vector<int>> incoming(sp.n_dof);  // target DOF -> hanging DOFs, sorted order
+  // target DOF -> (hanging DOF, weight), sorted order
+  std::vector<std::vector<std::pair<int, double>>> incoming(sp.n_dof);
   for(int h : sp.hang_order)
-    for(int k = 0; k < 2; ++k) {
-      const int c = sp.hang_cpl[2 * h + k];
+    for(int k = 0; k < sp.hang_n[h]; ++k) {
+      const int c = sp.hang_cpl[MC * h + k];
       if(c >= sp.n_active) continue;
-      incoming[c].push_back(h);
+      incoming[c].push_back({h, sp.hang_w[MC * h + k]});
       if(c >= first) depth[c - first] = std::max(depth[c - first], depth[h] + 1);
     }
   int max_depth = 0;
@@ -336,58 +338,65 @@ void hanging_plan(Level& L, hipStream_t stream) {
   };
   std::vector<std::vector<int>> mdst(n_launch), msp(n_launch), msrc(n_launch), rdst(n_launch), rsp(n_launch),
       rsrc(n_launch);
+  std::vector<std::vector<double>> mw(n_launch), rw(n_launch);
   for(int c = 0; c < sp.n_active; ++c) {
     if(incoming[c].empty()) continue;
     const int k = launch_of(c);
     for(int p = st.row_ptr[c]; p < st.row_ptr[c + 1]; ++p) {
-      std::vector<int> srcs;
-      for(int h : incoming[c]) {
-        const int q = pos_in_row(first + h, st.col[p]);
-        if(q >= 0) srcs.push_back(q);
+      const size_t before = msrc[k].size();
+      for(const auto& hw : incoming[c]) {
+        const int q = pos_in_row(first + hw.first, st.col[p]);
+        if(q >= 0) { msrc[k].push_back(q); mw[k].push_back(hw.second); }
       }
-      if(srcs.empty()) continue;
+      if(msrc[k].size() == before) continue;
       mdst[k].push_back(p);
-      msp[k].push_back((int)msrc[k].size());
-      msrc[k].insert(msrc[k].end(), srcs.begin(), srcs.end());
+      msp[k].push_back((int)before);
     }
     rdst[k].push_back(c);
     rsp[k].push_back((int)rsrc[k].size());
-    for(int h : incoming[c]) rsrc[k].push_back(first + h);
+    for(const auto& hw : incoming[c]) { rsrc[k].push_back(first + hw.first); rw[k].push_back(hw.second); }
   }
   // concatenate the launches; source pointers become global
   std::vector<int> mat_dst, mat_sp, mat_src, rhs_dst, rhs_sp, rhs_src;
+  std::vector<double> mat_w, rhs_w;
   L.hg_mat_launch.assign(1, 0);
   L.hg_rhs_launch.assign(1, 0);
   for(int k = 0; k < n_launch; ++k) {
     const int mo = (int)mat_src.size(), ro = (int)rhs_src.size();
     for(size_t i = 0; i < mdst[k].size(); ++i) { mat_dst.push_back(mdst[k][i]); mat_sp.push_back(mo + msp[k][i]); }
     mat_src.insert(mat_src.end(), msrc[k].begin(), msrc[k].end());
+    mat_w.insert(mat_w.end(), mw[k].begin(), mw[k].end());
     for(size_t i = 0; i < rdst[k].size(); ++i) { rhs_dst.push_back(rdst[k][i]); rhs_sp.push_back(ro + rsp[k][i]); }
     rhs_src.insert(rhs_src.end(), rsrc[k].begin(), rsrc[k].end());
+    rhs_w.insert(rhs_w.end(), rw[k].begin(), rw[k].end());
     L.hg_mat_launch.push_back((int)mat_dst.size());
     L.hg_rhs_launch.push_back((int)rhs_dst.size());
   }
   mat_sp.push_back((int)mat_src.size());
   rhs_sp.push_back((int)rhs_src.size());
-  // the hanging rows: 1 on the diagonal, -1/2 at the coupling DOFs, 0 elsewhere
+  // the hanging rows: 1 on the diagonal, minus the weights at the coupling
+  // DOFs, 0 elsewhere
   std::vector<int> fix_pos;
   std::vector<double> fix_val;
   for(int h = 0; h < nh; ++h) {
     const int r = first + h;
     for(int p = st.row_ptr[r]; p < st.row_ptr[r + 1]; ++p) {
       const int col = st.col[p];
+      double v = col == r ? 1.0 : 0.0;
+      for(int k = 0; k < sp.hang_n[h]; ++k)
+        if(col == sp.hang_cpl[MC * h + k]) v = -sp.hang_w[MC * h + k];
       fix_pos.push_back(p);
-      fix_val.push_back(col == r ? 1.0 : (col == sp.hang_cpl[2 * h] || col == sp.hang_cpl[2 * h + 1] ? -0.5 : 0.0));
+      fix_val.push_back(v);
     }
   }
   L.hg_mat_dst.upload(mat_dst, stream);
   L.hg_mat_sp.upload(mat_sp, stream);
   L.hg_mat_src.upload(mat_src, stream);
-  L.hg_mat_w.upload(std::vector<double>(std::max<size_t>(1, mat_src.size()), 0.5), stream);
+  L.hg_mat_w.upload(mat_w, stream);
   L.hg_rhs_dst.upload(rhs_dst, stream);
   L.hg_rhs_sp.upload(rhs_sp, stream);
   L.hg_rhs_src.upload(rhs_src, stream);
-  L.hg_rhs_w.upload(std::vector<double>(std::max<size_t>(1, rhs_src.size()), 0.5), stream);
+  L.hg_rhs_w.upload(rhs_w, stream);
   L.hg_fix_pos.upload(fix_pos, stream);
   L.hg_fix_val.upload(fix_val, stream);
 }

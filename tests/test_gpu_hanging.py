@@ -1,15 +1,16 @@
-"""Meshes refined by indicator with hanging vertices (P1 on TwoTriangles, Q1
-on UnitSquare, eight steps of 0.5 x^2 + y - 0.5) on the device: assembly
-with the hanging-DOF coupling against the oracle (row-relative 1e-12, the
-constraint rows and hanging rhs entries exact), the reference's golden
-errors of test/adaptive_hanging_test.c++ within its 1e-11 for the direct and
-the FGMRES solvers, the solution against the oracle's, and served requests.
+"""Meshes refined by indicator with hanging vertices (P1/P2 on TwoTriangles,
+Q1/Q2 on UnitSquare, eight steps of 0.5 x^2 + y - 0.5) on the device:
+assembly with the hanging-DOF coupling against the oracle (row-relative
+1e-12, the constraint rows and hanging rhs entries exact), the reference's
+golden errors of test/adaptive_hanging_test.c++ within its 1e-11 for the
+direct and the FGMRES solvers, the solution against the oracle's, and served
+requests.
 """
 import numpy as np
 import pytest
 
 import parmoon2022_amd as pm
-from test_oracle import HANGING_GOLDEN, hanging_db
+from test_oracle import HANGING_GOLDEN, hanging_db, hanging_constraint_residual
 from test_gpu_parity import row_rel_err
 
 pytestmark = pytest.mark.gpu
@@ -21,18 +22,19 @@ FGMRES = {"solver_type": "iterative", "iterative_solver_type": "fgmres", "max_n_
 def _hanging(cd):
     import ctypes
     n = ctypes.c_int()
-    pm._check(pm.lib().pmcd_get_hanging(cd._s, -1, None, ctypes.byref(n)))
-    out = np.zeros(3 * max(n.value, 1), dtype=np.int32)
+    pm._check(pm.lib().pmcd_get_hanging(cd._s, -1, None, None, ctypes.byref(n)))
+    out = np.zeros(4 * max(n.value, 1), dtype=np.int32)
+    w = np.zeros(3 * max(n.value, 1))
     pm._check(pm.lib().pmcd_get_hanging(cd._s, -1, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                        ctypes.byref(n)))
-    return out[:3 * n.value].reshape(-1, 3)
+                                        w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n)))
+    return out[:4 * n.value].reshape(-1, 4), w[:3 * n.value].reshape(-1, 3)
 
 
 @pytest.mark.parametrize("variant", [0, 2], ids=["vector", "tensor"])
-@pytest.mark.parametrize("geo", list(HANGING_GOLDEN))
-def test_hanging_assembly_matches_oracle(oracle_lib, geo, variant):
-    db = hanging_db(geo)
-    print("hanging assembly", geo, "variant", variant, flush=True)
+@pytest.mark.parametrize("geo,order", list(HANGING_GOLDEN))
+def test_hanging_assembly_matches_oracle(oracle_lib, geo, order, variant):
+    db = hanging_db(geo, order)
+    print("hanging assembly", geo, order, "variant", variant, flush=True)
     old = pm.assembly_variant()
     pm.set_assembly_variant(variant)
     try:
@@ -43,8 +45,9 @@ def test_hanging_assembly_matches_oracle(oracle_lib, geo, variant):
     o = oracle_lib.Oracle(pm.problem_desc(db))
     o.assemble()
     assert np.array_equal(cd.dofmap(), o.dofmap())
-    hg = _hanging(cd)
-    assert np.array_equal(hg, o.hanging())
+    hg, w = _hanging(cd)
+    ohg, ow = o.hanging()
+    assert np.array_equal(hg, ohg) and np.array_equal(w, ow)
     rp, col = cd.pattern()
     a, b = cd.matrix_values(), o.matrix_values()
     err = row_rel_err(rp, a, b)
@@ -52,35 +55,39 @@ def test_hanging_assembly_matches_oracle(oracle_lib, geo, variant):
     assert err <= 1e-12
     fa, fb = cd.rhs(), o.rhs()
     assert np.max(np.abs(fa - fb)) <= 1e-12 * np.max(np.abs(fb))
-    # hanging rows: u_h - u_a / 2 - u_b / 2 = 0, rhs 0 (exact)
-    for h, ca, cb in hg:
+    # hanging rows: u_h - sum_k w_k u_{c_k} = 0, rhs 0 (exact)
+    for (h, *cpl), wt in zip(hg, w):
         row = {int(c): v for c, v in zip(col[rp[h]:rp[h + 1]], a[rp[h]:rp[h + 1]])}
         want = {c: 0.0 for c in row}
-        want[h], want[ca], want[cb] = 1.0, -0.5, -0.5
+        want[h] = 1.0
+        for c, x in zip(cpl, wt):
+            if c >= 0:
+                want[c] = -x
         assert row == want, h
     assert np.all(fa[hg[:, 0]] == 0.0)
     cd.close()
 
 
 @pytest.mark.parametrize("solver", ["direct", "fgmres_ssor", "fgmres_jacobi"])
-@pytest.mark.parametrize("geo", list(HANGING_GOLDEN))
-def test_hanging_golden_errors_on_gpu(oracle_lib, geo, solver):
+@pytest.mark.parametrize("geo,order", list(HANGING_GOLDEN))
+def test_hanging_golden_errors_on_gpu(oracle_lib, geo, order, solver):
     extra = {}
     if solver != "direct":
         extra = dict(FGMRES, preconditioner=solver.split("_")[1])
-    db = hanging_db(geo, **extra)
-    print("hanging golden", geo, solver, flush=True)
+    db = hanging_db(geo, order, **extra)
+    print("hanging golden", geo, order, solver, flush=True)
     cd = pm.ConvectionDiffusion(db)
     cd.assemble()
     st = cd.solve()
     assert st.converged
     e = cd.output()
     got = [e[0], e[1], e[2], e[4]]
-    assert np.allclose(got, HANGING_GOLDEN[geo], rtol=0, atol=1e-11), (got, HANGING_GOLDEN[geo])
+    gold = HANGING_GOLDEN[(geo, order)]
+    assert np.allclose(got, gold, rtol=0, atol=1e-11), (got, gold)
     u = cd.solution()
-    hg = _hanging(cd)
-    assert np.allclose(u[hg[:, 0]], 0.5 * (u[hg[:, 1]] + u[hg[:, 2]]), rtol=0, atol=1e-12)
-    if solver == "direct":
+    hg, w = _hanging(cd)
+    assert hanging_constraint_residual(u, hg, w) < 1e-12
+    if solver == "direct" and order == 1:
         o = oracle_lib.Oracle(pm.problem_desc(db))
         o.assemble()
         o.solve(pm.solver_desc(db))

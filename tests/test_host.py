@@ -439,26 +439,28 @@ def test_batch_instances_generator():
     assert np.all(np.abs(bv.mean(axis=0)) < 0.15)  # directions spread over the sphere
 
 
-def _hanging_db(geo):
-    return {"geo_file": geo, "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 0, "example": "sine_laplace_2d",
+def _hanging_db(geo, order=1):
+    return {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 0, "example": "sine_laplace_2d",
             "refine_by_indicator_steps": 8, "refine_indicator": (0.5, 1.0, 0.0, -0.5), "conforming_closure": False}
 
 
 def _host_hanging(h):
     n = ctypes.c_int()
-    pm._check(pm.lib().pmcd_host_hanging(h._h, -1, None, ctypes.byref(n)))
-    out = np.zeros(3 * max(n.value, 1), dtype=np.int32)
+    pm._check(pm.lib().pmcd_host_hanging(h._h, -1, None, None, ctypes.byref(n)))
+    out = np.zeros(4 * max(n.value, 1), dtype=np.int32)
+    w = np.zeros(3 * max(n.value, 1))
     pm._check(pm.lib().pmcd_host_hanging(h._h, -1, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                         ctypes.byref(n)))
-    return out[:3 * n.value].reshape(-1, 3)
+                                         w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n)))
+    return out[:4 * n.value].reshape(-1, 4), w[:3 * n.value].reshape(-1, 3)
 
 
+@pytest.mark.parametrize("order", [1, 2])
 @pytest.mark.parametrize("geo", ["TwoTriangles", "UnitSquare"])
-def test_hanging_setup_matches_oracle(oracle_lib, geo):
+def test_hanging_setup_matches_oracle(oracle_lib, geo, order):
     """Refinement by indicator with hanging vertices: the library's host
-    set-up (leaves, numbering, coupled pattern, sorted hanging list) equals
-    the oracle's."""
-    db = _hanging_db(geo)
+    set-up (leaves, numbering, coupled pattern, sorted hanging list with its
+    weights) equals the oracle's."""
+    db = _hanging_db(geo, order)
     h = pm.HostSetup(db)
     o = oracle_lib.Oracle(pm.problem_desc(db))
     assert h.info() == o.info()
@@ -467,8 +469,9 @@ def test_hanging_setup_matches_oracle(oracle_lib, geo):
     rp, col = h.pattern()
     orp, ocol = o.pattern()
     assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
-    hg = _host_hanging(h)
-    assert len(hg) > 0 and np.array_equal(hg, o.hanging())
+    hg, w = _host_hanging(h)
+    ohg, ow = o.hanging()
+    assert len(hg) > 0 and np.array_equal(hg, ohg) and np.array_equal(w, ow)
     # hanging DOFs are the last active ones, before the Dirichlet DOFs
     i = h.info()
     assert hg[:, 0].max() == i["n_active"] - 1 and hg[:, 0].min() == i["n_active"] - len(hg)
@@ -478,8 +481,8 @@ def test_hanging_setup_rejections():
     db = _hanging_db("UnitSquare")
     with pytest.raises(ValueError, match="conforming_closure"):
         pm.problem_desc(dict(db, conforming_closure=True))
-    with pytest.raises(RuntimeError, match="P1/Q1"):
-        pm.HostSetup(dict(db, ANSATZ_ORDER=2))
+    with pytest.raises(RuntimeError, match="order"):
+        pm.HostSetup(dict(db, ANSATZ_ORDER=3))
     with pytest.raises(RuntimeError, match="2D"):
         pm.HostSetup(dict(db, geo_file="Default_UnitCube_Hexa"))
     with pytest.raises(RuntimeError, match="multigrid"):

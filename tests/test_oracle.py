@@ -629,19 +629,23 @@ def test_oracle_structure_test_fixtures(oracle_lib, geo, order, expected):
     assert (len(rp) - 1, int(col.max()) + 1, len(col), int(rp[4] - rp[3])) == expected
 
 
-# test/adaptive_hanging_test.c++:126-135, 283-290, 392-399: eight
+# test/adaptive_hanging_test.c++:126-135, 283-306, 392-408: eight
 # RefineByIndicator steps with the indicator 0.5 x^2 + y - 0.5 and no
 # conforming closure, then the default CD problem (sine_laplace, direct
-# solver, P1 on TwoTriangles, Q1 on UnitSquare); L2, H1-semi, SD and L_inf
-# errors within 1e-11
+# solver) with P1/P2 on TwoTriangles, Q1/Q2 on UnitSquare (order 2 from the
+# "testall" run); L2, H1-semi, SD and L_inf errors within 1e-11
 HANGING_GOLDEN = {
-    "TwoTriangles": [0.042380310691282, 0.59129151149837, 0.59129151149837, 0.13495122911489],
-    "UnitSquare": [0.011874554092922, 0.25451703865297, 0.25451703865297, 0.03750007808692],
+    ("TwoTriangles", 1): [0.042380310691282, 0.59129151149837, 0.59129151149837, 0.13495122911489],
+    ("UnitSquare", 1): [0.011874554092922, 0.25451703865297, 0.25451703865297, 0.03750007808692],
+    ("TwoTriangles", 2): [0.0027720696069861, 0.081555432454459, 0.081555432454459, 0.012895816013502],
+    ("UnitSquare", 2): [0.001153069846948, 0.032083741477288, 0.032083741477288, 0.0032514526789689],
 }
+HANGING_FGMRES = {"solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "ssor",
+                  "max_n_iterations": 5000, "residual_tolerance": 1e-13, "gmres_restart": 50}
 
 
-def hanging_db(geo, **extra):
-    db = {"geo_file": geo, "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 0, "example": "sine_laplace_2d",
+def hanging_db(geo, order=1, **extra):
+    db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 0, "example": "sine_laplace_2d",
           "refine_by_indicator_steps": 8, "refine_indicator": (0.5, 1.0, 0.0, -0.5), "conforming_closure": False,
           "solver_type": "direct"}
     db.update(extra)
@@ -657,39 +661,51 @@ def _hanging_errors(oracle_lib, db):
     return np.array([e[0], e[1], e[2], e[4]]), o
 
 
-@pytest.mark.parametrize("geo", list(HANGING_GOLDEN))
+def hanging_constraint_residual(u, hg, w):
+    """max |u_h - sum_k w_k u_{c_k}| over the hanging DOFs"""
+    cpl = np.where(hg[:, 1:] >= 0, hg[:, 1:], 0)
+    return np.max(np.abs(u[hg[:, 0]] - np.sum(w * u[cpl], axis=1)))
+
+
+@pytest.mark.parametrize("geo,order", list(HANGING_GOLDEN))
 @pytest.mark.parametrize("solver", ["direct", "fgmres_ssor"])
-def test_oracle_hanging_golden(oracle_lib, geo, solver):
-    extra = {}
-    if solver == "fgmres_ssor":
-        extra = {"solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "ssor",
-                 "max_n_iterations": 5000, "residual_tolerance": 1e-13, "gmres_restart": 50}
-    got, o = _hanging_errors(oracle_lib, hanging_db(geo, **extra))
-    assert np.allclose(got, HANGING_GOLDEN[geo], rtol=0, atol=1e-11), (got, HANGING_GOLDEN[geo])
-    # the solution satisfies every hanging constraint u_h = (u_a + u_b) / 2
-    u = o.solution()
-    hg = o.hanging()
+def test_oracle_hanging_golden(oracle_lib, geo, order, solver):
+    if solver == "direct" and order == 2:
+        pytest.skip("the oracle's dense direct solve of 7k unknowns takes 20 s; FGMRES covers order 2")
+    extra = HANGING_FGMRES if solver == "fgmres_ssor" else {}
+    got, o = _hanging_errors(oracle_lib, hanging_db(geo, order, **extra))
+    gold = HANGING_GOLDEN[(geo, order)]
+    assert np.allclose(got, gold, rtol=0, atol=1e-11), (got, gold)
+    # the solution satisfies every hanging constraint
+    hg, w = o.hanging()
     assert len(hg) > 0
-    assert np.allclose(u[hg[:, 0]], 0.5 * (u[hg[:, 1]] + u[hg[:, 2]]), rtol=0, atol=1e-12)
+    assert hanging_constraint_residual(o.solution(), hg, w) < 1e-12
 
 
-@pytest.mark.parametrize("fault,geos", [(6, ["TwoTriangles"]), (7, ["TwoTriangles"]),
-                                        (8, ["TwoTriangles", "UnitSquare"])])
-def test_oracle_hanging_planted_mistakes_fail(oracle_lib, fault, geos):
+@pytest.mark.parametrize("fault,cases", [(6, [("TwoTriangles", 1)]), (7, [("TwoTriangles", 1)]),
+                                         (8, [("TwoTriangles", 1), ("UnitSquare", 1)]),
+                                         (9, [("TwoTriangles", 1), ("UnitSquare", 2)])])
+def test_oracle_hanging_planted_mistakes_fail(oracle_lib, fault, cases):
     """The goldens catch each planted mistake: 6 hanging coupling DOFs that
     hang themselves receive nothing, 7 hanging rows added in list order, 8 no
-    1-irregular closure. (The Q1 mesh has no chained hanging DOFs, so 6 and 7
-    cannot show there.)"""
+    1-irregular closure, 9 no coupling at all. Only the P1 triangle space has
+    chained hanging DOFs (a coupling DOF that hangs itself: 24 of them), so 6
+    and 7 can only show there; in order 2 the coupling DOFs are vertices and
+    coarse edge midpoints, which never hang."""
     try:
         oracle_lib.lib().orc_set_fault(fault)
-        for geo in geos:
-            got, _ = _hanging_errors(oracle_lib, hanging_db(geo))
-            assert np.max(np.abs(got - HANGING_GOLDEN[geo])) > 1e-6, (fault, geo, got)
+        for geo, order in cases:
+            extra = HANGING_FGMRES if order == 2 else {}
+            try:
+                got, _ = _hanging_errors(oracle_lib, hanging_db(geo, order, **extra))
+            except AssertionError:
+                continue  # no convergence counts as caught
+            assert np.max(np.abs(got - HANGING_GOLDEN[(geo, order)])) > 1e-6, (fault, geo, order, got)
     finally:
         oracle_lib.lib().orc_set_fault(0)
 
 
-@pytest.mark.parametrize("geo", list(HANGING_GOLDEN))
+@pytest.mark.parametrize("geo", ["TwoTriangles", "UnitSquare"])
 def test_oracle_hanging_mesh_invariants(oracle_lib, geo):
     """Leaves tile the unit square (areas 4^-k summing to 1), each hanging
     DOF sits at the midpoint of its coupling DOFs and no hanging DOF lies on
@@ -714,12 +730,57 @@ def test_oracle_hanging_mesh_invariants(oracle_lib, geo):
     local_vertex = [0, 1, 2] if nv == 3 else [0, 1, 3, 2]  # Q1 DOFs in tensor order
     for j, a in enumerate(local_vertex):
         xy[dm[:, j]] = X[:, a]
-    hg = o.hanging()
+    hg, w = o.hanging()
+    assert np.all(hg[:, 3] == -1) and np.all(w[:, :2] == 0.5)
     assert np.allclose(xy[hg[:, 0]], 0.5 * (xy[hg[:, 1]] + xy[hg[:, 2]]), atol=1e-15)
     on_bnd = np.any((np.abs(xy[hg[:, 0]]) < 1e-14) | (np.abs(xy[hg[:, 0]] - 1) < 1e-14), axis=1)
     assert not on_bnd.any()
     pos = {h: t for t, h in enumerate(hg[:, 0])}
-    for t, (h, a, b) in enumerate(hg):
+    for t, (h, a, b, _) in enumerate(hg):
         for c in (a, b):
             if c in pos:
                 assert pos[c] > t
+
+
+@pytest.mark.parametrize("geo", ["TwoTriangles", "UnitSquare"])
+def test_oracle_hanging_order2_reproduces_quadratics(oracle_lib, geo):
+    """Order 2: every hanging constraint holds exactly for the nodal values of
+    a quadratic polynomial (the constrained space contains the quadratics),
+    the weights are the quadratic Lagrange values at 1/4 and 3/4 and sum to 1,
+    and a hanging row carries all its coupling DOFs, also the far one outside
+    its cells (that coupling is what a missing far entry would drop)."""
+    o = oracle_lib.Oracle(pm.problem_desc(hanging_db(geo, 2)))
+    i = o.info()
+    nv = 3 if geo == "TwoTriangles" else 4
+    X = o.cell_coords(nv=nv)[:, :, :2]
+    dm = o.dofmap()
+    # nodal points: for each local DOF, the reference point where it is 1
+    shape = 0 if nv == 3 else 1
+    ref = [(0, 0), (0.5, 0), (1, 0), (0, 0.5), (0.5, 0.5), (0, 1)] if nv == 3 else \
+        [(a, b) for b in (-1, 0, 1) for a in (-1, 0, 1)]
+    nodes = []
+    for j in range(dm.shape[1]):
+        for p in ref:
+            v = oracle_lib.basis(shape, 2, np.array([p[0], p[1], 0.0]))[:dm.shape[1]]
+            if abs(v[j] - 1) < 1e-12 and np.sum(np.abs(v)) < 1 + 1e-12:
+                nodes.append(p)
+                break
+    nodes = np.array(nodes, dtype=float)
+    xy = np.zeros((i["n_dof"], 2))
+    for c in range(len(dm)):
+        if nv == 3:
+            A, B, C = X[c, 0], X[c, 1], X[c, 2]
+            pts = A + nodes[:, :1] * (B - A) + nodes[:, 1:] * (C - A)
+        else:  # parallelograms: x = x0 + (1 + xi) / 2 (x1 - x0) + (1 + eta) / 2 (x3 - x0)
+            A, B, D = X[c, 0], X[c, 1], X[c, 3]
+            pts = A + 0.5 * (1 + nodes[:, :1]) * (B - A) + 0.5 * (1 + nodes[:, 1:]) * (D - A)
+        xy[dm[c]] = pts
+    u = xy[:, 0] ** 2 + xy[:, 0] * xy[:, 1] + 2 * xy[:, 1] ** 2 + 0.3 * xy[:, 0] - 0.1
+    hg, w = o.hanging()
+    assert len(hg) > 0 and np.all(hg[:, 3] >= 0)
+    assert hanging_constraint_residual(u, hg, w) < 1e-14
+    assert np.allclose(w.sum(axis=1), 1.0, rtol=0, atol=0)
+    assert {tuple(r) for r in w} == {(0.375, 0.75, -0.125), (-0.125, 0.75, 0.375)}
+    rp, col = o.pattern()
+    for h, a, m, b in hg:
+        assert {a, m, b} <= set(col[rp[h]:rp[h + 1]])

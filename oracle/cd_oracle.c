@@ -1511,8 +1511,10 @@ static int build_space_hanging(orc_problem* P, olevel* L, const vdict* mids) {
 
 /* rows of the coupling DOFs take the columns of the hanging rows added to
    them (ModifyMatrixAccordingToCoupling needs every entry it adds,
-   src/Matrix/FEMatrix.C:244-296; the pattern of Structure.C:75-230 holds
-   them); entries the reference pattern may hold beyond these are zero */
+   src/Matrix/FEMatrix.C:244-296); the pattern of Structure.C:75-230 gives
+   them to every coupling row, Dirichlet rows included, where they stay
+   zero. (For P1 triangles the reference also adds the transposed couplings,
+   Structure.C:196-218, zeros that only its AFC uses; they are left out.) */
 static void hanging_structure(olevel* L) {
   const int n = L->ndof;
   int** rows = malloc(sizeof(int*) * n);
@@ -1540,7 +1542,6 @@ static void hanging_structure(olevel* L) {
     const int h = L->hang_order[t], hd = L->nactive_nh + h;
     for(int k = 0; k < L->hang_ncpl[h]; ++k) {
       const int c = L->hang_cpl[MAXCPL * h + k];
-      if(c >= L->nactive) continue;
       int* merged = malloc(sizeof(int) * (len[c] + len[hd]));
       int a = 0, b = 0, u = 0;
       while(a < len[c] || b < len[hd]) {

@@ -130,9 +130,12 @@ Structure build_structure(const FESpace& sp, int n_cells) {
 
 // Rows of coupling DOFs take the columns of the hanging rows added to them
 // (FEMatrix::ModifyMatrixAccordingToCoupling, src/Matrix/FEMatrix.C:244-296,
-// needs every entry it adds; Structure.C:75-230 puts them in the pattern),
-// in the sorted hanging order so that a hanging coupling DOF passes on what
-// it received. Entries the reference pattern may hold beyond these stay zero.
+// needs every entry it adds; Structure.C:75-230 puts them in the pattern of
+// every coupling row, Dirichlet rows included, where they stay zero), in the
+// sorted hanging order so that a hanging coupling DOF passes on what it
+// received. For P1 triangles the reference also adds the transposed
+// couplings (Structure.C:196-218), zeros only its AFC uses; they are left
+// out.
 Structure hanging_structure(const FESpace& sp, int n_cells, const Structure& base) {
   const int n = base.n_rows, first = sp.n_active_non_hanging();
   constexpr int MC = FESpace::kMaxCoupling;
@@ -153,7 +156,6 @@ Structure hanging_structure(const FESpace& sp, int n_cells, const Structure& bas
     const int hd = first + h;
     for(int k = 0; k < sp.hang_n[h]; ++k) {
       const int c = sp.hang_cpl[MC * h + k];
-      if(c >= sp.n_active) continue;
       merged.clear();
       std::set_union(rows[c].begin(), rows[c].end(), rows[hd].begin(), rows[hd].end(), std::back_inserter(merged));
       rows[c].swap(merged);

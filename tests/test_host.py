@@ -501,3 +501,44 @@ def test_hanging_callable_indicator_equals_coefficients():
     pm._check(pm.lib().pmcd_estimate_counts(ctypes.byref(pd), out))
     i = a.info()
     assert list(out) == [i["n_cells"], i["n_dof"], i["nnz"]]
+
+
+# test/structure_test.c++:186-230: three RefineByIndicator steps with the
+# indicator -0.5 x^2 + y + 0.5 (:590-597) and no conforming closure, then
+# TStructure(space); the hanging tables (:2631-2790) hold rows, columns,
+# entries and the entries of row 3. UnitSquare_quad.mesh is the unit square's
+# single quadrilateral (the built-in UnitSquare); its fixture has Neumann
+# boundaries everywhere, which no example here has, so its row-3 count (which
+# depends on the boundary DOFs' numbers) is not compared. UnitSquareIrregular.GEO
+# has Dirichlet boundaries (sine_2d is Dirichlet everywhere). For P1
+# triangles the reference pattern also holds the transposed hanging couplings
+# (Structure.C:196-218, zeros used only by its AFC, not built here): 211
+# entries there against 193 here.
+STRUCTURE_HANGING = [
+    ("UnitSquare", 1, (19, 19, 131, None)),
+    ("UnitSquare", 2, (57, 57, 761, None)),
+    ("UnitSquareIrregular.GEO", 1, (29, 29, None, 9)),
+    ("UnitSquareIrregular.GEO", 2, (93, 93, 1017, 19)),
+]
+
+
+@pytest.mark.parametrize("geo,order,want", STRUCTURE_HANGING)
+def test_structure_test_hanging_fixtures(oracle_lib, geo, order, want):
+    path = os.path.join(GOLDEN_DIR, geo) if geo.endswith(".GEO") else geo
+    db = {"geo_file": path, "ANSATZ_ORDER": order, "example": "sine_2d", "refine_by_indicator_steps": 3,
+          "refine_indicator": (-0.5, 1.0, 0.0, 0.5), "conforming_closure": False}
+    h = pm.HostSetup(db)
+    i = h.info()
+    rp, col = h.pattern()
+    got = (i["n_dof"], i["n_dof"], i["nnz"], int(rp[4] - rp[3]))
+    for g, w in zip(got, want):
+        if w is not None:
+            assert g == w, (got, want)
+    hg, _ = _host_hanging(h)
+    assert len(hg) > 0
+    if order == 1 and geo.endswith(".GEO"):
+        assert got[2] == 211 - 18
+    if not geo.endswith(".GEO"):
+        o = oracle_lib.Oracle(pm.problem_desc(db))
+        orp, ocol = o.pattern()
+        assert np.array_equal(rp, orp) and np.array_equal(col, ocol)

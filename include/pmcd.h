@@ -64,14 +64,15 @@ typedef struct {
   const char* mesh_file; /* PMCD_MESH_FILE: path of a 3D medit .mesh file (geo_file ending in
                             .mesh, read as TDomain::InitFromMesh, src/Geometry/Domain.C:257-270,
                             878-992, 2912-3330); NULL otherwise */
-  /* Adaptive refinement with hanging vertices: after the n_refine regular
-   * refinements, adapt_steps times TDomain::RefineByIndicator(indicator) with
-   * conforming_closure = false (src/Geometry/Domain.C:1094-1207,
-   * test/adaptive_hanging_test.c++:120-135): cells whose corner values of
-   * indicator(x, indicator_ctx) (x = {x, y, z}) lie on both sides of zero are
-   * refined regularly, then the mesh is closed to a 1-irregular one. The
-   * hanging vertices' DOFs are coupled to their coarse edges as in
-   * Assemble2D.C:3350-3427. 2D, order 1 or 2, mg_levels 1; steady problems only
+  /* Adaptive refinement: after the n_refine regular refinements,
+   * adapt_steps times TDomain::RefineByIndicator(indicator)
+   * (src/Geometry/Domain.C:1094-1207, test/adaptive_hanging_test.c++:
+   * 120-135): cells whose corner values of indicator(x, indicator_ctx)
+   * (x = {x, y, z}) lie on both sides of zero are refined regularly, then the
+   * mesh is closed to a 1-irregular one. With adapt_conforming = 0
+   * (conforming_closure = false) the hanging vertices' DOFs are coupled to
+   * their coarse edges as in Assemble2D.C:3350-3427. 2D, order 1 or 2,
+   * mg_levels 1; steady problems only
    * (no AFC, no time stepping, no batches). The indicator runs on the host
    * during pmcd_system_create / pmcd_host_create (and on a server worker for
    * a served request, so the context must stay valid until it finishes).
@@ -79,6 +80,12 @@ typedef struct {
   int adapt_steps;
   double (*indicator)(const double* x, void* ctx);
   void* indicator_ctx;
+  /* conforming_closure = true (the reference's default; triangles only):
+   * TDomain::MakeConfClosure (Domain.C:1219-1360) instead of hanging
+   * vertices: before each step the last closure's bisections are removed,
+   * after it triangles with two or three refined edges are refined
+   * regularly and triangles with one are bisected on it. */
+  int adapt_conforming;
 } pmcd_problem_desc;
 
 /* solver_type / iterative_solver_type: direct, cg, fgmres, richardson, and the

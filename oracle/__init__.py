@@ -55,8 +55,8 @@ def lib():
         L.orc_create_adaptive.restype = ctypes.c_void_p
         L.orc_create_adaptive.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                           ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                          ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_double,
-                                          ctypes.c_double]
+                                          ctypes.c_int, ctypes.c_double, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.c_double, ctypes.c_double]
         L.orc_hanging.restype = ctypes.c_int
         L.orc_tcd_init.restype = ctypes.c_int
         L.orc_tcd_init.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
@@ -94,10 +94,12 @@ class Oracle:
         if adapt > 0:
             # the ProblemDesc's indicator callback (a C function pointer) and context
             fn = ctypes.cast(desc.indicator, ctypes.c_void_p)
-            p = lib().orc_create_adaptive(desc.geometry, desc.n_refine, adapt, fn, desc.indicator_ctx, desc.order,
-                                          desc.supg, desc.linear_quad, desc.example, desc.eps, b, desc.c, desc.f0)
+            p = lib().orc_create_adaptive(desc.geometry, desc.n_refine, adapt, fn, desc.indicator_ctx,
+                                          desc.adapt_conforming, desc.order, desc.supg, desc.linear_quad,
+                                          desc.example, desc.eps, b, desc.c, desc.f0)
             if not p:
-                raise ValueError("adaptive meshes take order 1 or 2 on the 2D default meshes")
+                raise ValueError("adaptive meshes take order 1 or 2 on the 2D default meshes "
+                                 "(the conforming closure on triangles)")
             self._p = ctypes.c_void_p(p)
         else:
             self._p = ctypes.c_void_p(lib().orc_create(desc.geometry, desc.structured_n, desc.n_refine,

@@ -67,8 +67,9 @@ static int jv(int s, int j, int k) {
    hanging coupling DOFs that hang themselves receive nothing; 7: hanging
    rows added in list order instead of the sorted order; 8: the 1-irregular
    closure of the refinement by indicator skipped; 9: the hanging-DOF
-   coupling of the assembled system skipped. Set only by tests, through
-   orc_set_fault. */
+   coupling of the assembled system skipped; 11: the conforming closure
+   refines only triangles with three refined edges.
+   Set only by tests, through orc_set_fault. */
 static int orc_fault = 0;
 void orc_set_fault(int code) { orc_fault = code; }
 
@@ -322,8 +323,8 @@ static void refine(const omesh* cm, omesh* fm) {
 /* ----------------------------------------------------------------------- */
 /* adaptive refinement with hanging vertices (2D)                           */
 /* ----------------------------------------------------------------------- */
-/* TDomain::RefineByIndicator with conforming_closure = false
-   (src/Geometry/Domain.C:1094-1207, RefinementStrategy.C:260-287,
+/* TDomain::RefineByIndicator (src/Geometry/Domain.C:1094-1207,
+   RefinementStrategy.C:260-287,
    GridCell.C:348-385): cells with corner values of the indicator on both
    sides of zero (|value| <= 1e-10 counts for both) are refined regularly,
    then cells next to a neighbour refined two levels deeper are refined
@@ -391,7 +392,61 @@ static void refine_marked(const omesh* cm, const char* mark, omesh* fm, vdict* m
 
 /* the leaves after `steps` indicator refinements of `base` (consumed);
    mids keeps the edge midpoints */
-static void adapt_mesh(omesh* base, const oindicator* ind, int steps, omesh* out, vdict* mids) {
+static int straddles(const oindicator* ind, const double* const* pts, int n) {
+  int inner = 0, outer = 0;
+  for(int a = 0; a < n; ++a) {
+    const double val = ind->fn(pts[a], ind->ctx);
+    if(val <= 1e-10) inner++;
+    if(val >= -1e-10) outer++;
+  }
+  return inner > 0 && outer > 0;
+}
+
+/* refines the marked leaves (consumes mark), then every leaf whose edge
+   (u, v) has a midpoint m with (u, m) or (m, v) split again, until the mesh
+   is 1-irregular */
+static void refine_1irregular(omesh* cur, char* mark, vdict* mids, int* centers) {
+  const int nv = NV[cur->shape];
+  for(;;) {
+    omesh next;
+    refine_marked(cur, mark, &next, mids, centers);
+    mesh_free(cur);
+    free(mark);
+    *cur = next;
+    mark = calloc(cur->ncells, 1);
+    int any = 0;
+    for(int c = 0; c < cur->ncells; ++c)
+      for(int j = 0; j < nv; ++j) {
+        const int u = cur->cv[c * nv + j], v = cur->cv[c * nv + (j + 1) % nv];
+        const int mm = edge_mid(mids, u, v);
+        if(mm >= 0 && (edge_mid(mids, u, mm) >= 0 || edge_mid(mids, mm, v) >= 0)) {
+          mark[c] = 1;
+          any = 1;
+        }
+      }
+    if(!any || orc_fault == 8) break;
+  }
+  free(mark);
+}
+
+/* children of a triangle bisected on edge j (vertices v_j, v_j+1, midpoint
+   m): {v_j, m, v_j+2} and {v_j+1, v_j+2, m} (RefTriBis0Desc.C:21-22,
+   RefTriBis1Desc.C:22-23, RefTriBis2Desc.C:22-23) */
+static void bisect_children(const int* v, int j, int m, int ch[2][3]) {
+  ch[0][0] = v[j]; ch[0][1] = m; ch[0][2] = v[(j + 2) % 3];
+  ch[1][0] = v[(j + 1) % 3]; ch[1][1] = v[(j + 2) % 3]; ch[1][2] = m;
+}
+
+/* `steps` indicator refinements of `base` (consumed); mids keeps the edge
+   midpoints. conforming (triangles): TDomain::MakeConfClosure
+   (src/Geometry/Domain.C:1219-1360, GridCell.C:525-575): before a step the
+   closure bisections are removed (a bisected cell is marked when one of its
+   two halves is); after the 1-irregular refinement, every triangle with two
+   or three refined edges (edges whose neighbour across is refined) is
+   refined regularly, until none is left, and every triangle with one is
+   bisected on it (the bisections are the leaves' last level, rebuilt each
+   step). */
+static void adapt_mesh(omesh* base, const oindicator* ind, int steps, omesh* out, vdict* mids, int conforming) {
   mids->cap = 1 << 20;
   mids->slot = malloc(sizeof(int) * mids->cap);
   mids->key = malloc(sizeof(int[4]) * mids->cap);
@@ -399,39 +454,76 @@ static void adapt_mesh(omesh* base, const oindicator* ind, int steps, omesh* out
   int centers = 0;
   omesh cur = *base;
   const int nv = NV[cur.shape];
+  int* bis = NULL; /* conforming: bisected edge of each leaf of cur, or -1 */
   for(int step = 0; step < steps; ++step) {
     char* mark = calloc(cur.ncells, 1);
     for(int c = 0; c < cur.ncells; ++c) {
-      int inner = 0, outer = 0;
-      for(int a = 0; a < nv; ++a) {
-        const double val = ind->fn(&cur.X[3 * cur.cv[c * nv + a]], ind->ctx);
-        if(val <= 1e-10) inner++;
-        if(val >= -1e-10) outer++;
+      const int* v = &cur.cv[c * nv];
+      if(bis && bis[c] >= 0) {
+        const int j = bis[c];
+        const int m = edge_mid(mids, v[j], v[(j + 1) % 3]);
+        int ch[2][3];
+        bisect_children(v, j, m, ch);
+        for(int h = 0; h < 2; ++h) {
+          const double* pts[3] = {&cur.X[3 * ch[h][0]], &cur.X[3 * ch[h][1]], &cur.X[3 * ch[h][2]]};
+          if(straddles(ind, pts, 3)) mark[c] = 1;
+        }
+      } else {
+        const double* pts[8];
+        for(int a = 0; a < nv; ++a) pts[a] = &cur.X[3 * v[a]];
+        mark[c] = straddles(ind, pts, nv);
       }
-      mark[c] = inner > 0 && outer > 0;
     }
+    refine_1irregular(&cur, mark, mids, &centers);
+    free(bis);
+    bis = NULL;
+    if(!conforming) continue;
+    /* closure: regular refinement of triangles with two or more refined edges */
     for(;;) {
-      omesh next;
-      refine_marked(&cur, mark, &next, mids, &centers);
-      mesh_free(&cur);
-      free(mark);
-      cur = next;
-      /* 1-irregular closure: a leaf whose edge (u, v) has a midpoint m with
-         (u, m) or (m, v) split again is refined */
       mark = calloc(cur.ncells, 1);
       int any = 0;
-      for(int c = 0; c < cur.ncells; ++c)
-        for(int j = 0; j < nv; ++j) {
-          const int u = cur.cv[c * nv + j], v = cur.cv[c * nv + (j + 1) % nv];
-          const int mm = edge_mid(mids, u, v);
-          if(mm >= 0 && (edge_mid(mids, u, mm) >= 0 || edge_mid(mids, mm, v) >= 0)) {
-            mark[c] = 1;
-            any = 1;
-          }
-        }
-      if(!any || orc_fault == 8) break;
+      for(int c = 0; c < cur.ncells; ++c) {
+        int n = 0;
+        for(int j = 0; j < 3; ++j) n += edge_mid(mids, cur.cv[c * 3 + j], cur.cv[c * 3 + (j + 1) % 3]) >= 0;
+        if(n >= (orc_fault == 11 ? 3 : 2)) { mark[c] = 1; any = 1; }
+      }
+      if(!any) { free(mark); break; }
+      refine_1irregular(&cur, mark, mids, &centers);
     }
-    free(mark);
+    bis = malloc(sizeof(int) * cur.ncells);
+    for(int c = 0; c < cur.ncells; ++c) {
+      bis[c] = -1;
+      for(int j = 0; j < 3; ++j)
+        if(edge_mid(mids, cur.cv[c * 3 + j], cur.cv[c * 3 + (j + 1) % 3]) >= 0) bis[c] = j;
+    }
+  }
+  if(bis) {
+    /* the leaves with the bisected cells replaced by their halves */
+    int n = 0;
+    for(int c = 0; c < cur.ncells; ++c) n += bis[c] >= 0 ? 2 : 1;
+    omesh fm = cur;
+    fm.ncells = n;
+    fm.cv = malloc(sizeof(int) * 3 * n);
+    fm.parent = malloc(sizeof(int) * n);
+    int f = 0;
+    for(int c = 0; c < cur.ncells; ++c) {
+      const int* v = &cur.cv[c * 3];
+      if(bis[c] < 0) {
+        memcpy(&fm.cv[3 * f], v, sizeof(int) * 3);
+        fm.parent[f++] = c;
+        continue;
+      }
+      int ch[2][3];
+      bisect_children(v, bis[c], edge_mid(mids, v[bis[c]], v[(bis[c] + 1) % 3]), ch);
+      for(int h = 0; h < 2; ++h) {
+        memcpy(&fm.cv[3 * f], ch[h], sizeof(int) * 3);
+        fm.parent[f++] = c;
+      }
+    }
+    free(cur.cv);
+    free(cur.parent);
+    cur = fm;
+    free(bis);
   }
   *out = cur;
 }
@@ -2443,15 +2535,19 @@ void orc_destroy(orc_problem* P);
 
 /* a problem on the leaves of `adapt_steps` indicator refinements
    (TDomain::RefineByIndicator without conforming closure) of the coarse
-   mesh refined `n_refine` times: one level, 2D, order 1 or 2 (NULL otherwise);
-   indicator(x, ctx) is evaluated at the cells' vertices */
+   mesh refined `n_refine` times: one level, 2D, order 1 or 2, the conforming
+   closure on triangles only (NULL otherwise); indicator(x, ctx) is
+   evaluated at the cells' vertices */
 orc_problem* orc_create_adaptive(int geometry, int n_refine, int adapt_steps,
-                                 double (*indicator)(const double*, void*), void* ctx, int order,
+                                 double (*indicator)(const double*, void*), void* ctx, int conforming, int order,
                                  int supg, int linear_quad, int example, double eps, const double* b, double c,
                                  double f0) {
   omesh cur;
   default_mesh(geometry, &cur);
-  if((order != 1 && order != 2) || DIM[cur.shape] != 2) { mesh_free(&cur); return NULL; }
+  if((order != 1 && order != 2) || DIM[cur.shape] != 2 || (conforming && cur.shape != TRI)) {
+    mesh_free(&cur);
+    return NULL;
+  }
   for(int i = 0; i < n_refine; ++i) {
     omesh next;
     refine(&cur, &next);
@@ -2461,7 +2557,7 @@ orc_problem* orc_create_adaptive(int geometry, int n_refine, int adapt_steps,
   oindicator ind = {indicator, ctx};
   omesh leaves;
   vdict mids;
-  adapt_mesh(&cur, &ind, adapt_steps, &leaves, &mids);
+  adapt_mesh(&cur, &ind, adapt_steps, &leaves, &mids, conforming);
   orc_problem* P = calloc(1, sizeof(orc_problem));
   P->nlev = 1;
   P->lev = calloc(1, sizeof(olevel));

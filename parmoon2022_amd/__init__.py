@@ -40,7 +40,8 @@ class ProblemDesc(ctypes.Structure):
                 ("mg_levels", ctypes.c_int), ("example", ctypes.c_int),
                 ("eps", ctypes.c_double), ("b", ctypes.c_double * 3),
                 ("c", ctypes.c_double), ("f0", ctypes.c_double), ("mesh_file", ctypes.c_char_p),
-                ("adapt_steps", ctypes.c_int), ("indicator", ctypes.c_void_p), ("indicator_ctx", ctypes.c_void_p)]
+                ("adapt_steps", ctypes.c_int), ("indicator", ctypes.c_void_p), ("indicator_ctx", ctypes.c_void_p),
+                ("adapt_conforming", ctypes.c_int)]
 
 
 # double (*)(const double* x, void* ctx): indicator of the refinement by indicator
@@ -286,10 +287,9 @@ def problem_desc(db):
     steps = int(db.get("refine_by_indicator_steps", 0))
     if steps > 0:
         # test/adaptive_hanging_test.c++:120-135: RefineByIndicator(indicator)
-        # steps times on a domain with conforming_closure false
-        if db.get("conforming_closure", True):
-            raise ValueError("refinement by indicator is implemented with hanging vertices only: "
-                             "set conforming_closure to False")
+        # steps times; conforming_closure (the domain default, true) closes
+        # the mesh with bisections, false leaves hanging vertices
+        d.adapt_conforming = 1 if db.get("conforming_closure", True) else 0
         d.adapt_steps = steps
         d.indicator = ctypes.cast(indicator_callback(db["refine_indicator"]), ctypes.c_void_p)
         d.indicator_ctx = None

@@ -100,6 +100,8 @@ ProblemDesc to_desc(const pmcd_problem_desc& d) {
     p.adapt_steps = d.adapt_steps;
     p.indicator = d.indicator;
     p.indicator_ctx = d.indicator_ctx;
+    PMCD_CHECK(d.adapt_conforming == 0 || d.adapt_conforming == 1, "adapt_conforming must be 0 or 1");
+    p.adapt_conforming = d.adapt_conforming;
   }
   return p;
 }
@@ -282,7 +284,7 @@ int pmcd_system_set_data(pmcd_system* s, const pmcd_problem_desc* desc) {
                    d.linear_quad == cur.linear_quad && d.mg_levels == cur.mg_levels,
                "pmcd_system_set_data changes only the PDE data (example, eps, b, c, f0)");
     PMCD_CHECK(d.adapt_steps == cur.adapt_steps && d.indicator == cur.indicator &&
-                   d.indicator_ctx == cur.indicator_ctx,
+                   d.indicator_ctx == cur.indicator_ctx && d.adapt_conforming == cur.adapt_conforming,
                "pmcd_system_set_data changes only the PDE data (example, eps, b, c, f0)");
     s->sys->set_example(d.ex);
   });

@@ -356,29 +356,49 @@ Mesh refine_marked(const Mesh& cm, const std::vector<char>& mark, EdgeMids& mids
   return fm;
 }
 
+// children of a triangle bisected on edge j (vertices v_j, v_j+1, midpoint
+// m): {v_j, m, v_j+2} and {v_j+1, v_j+2, m} (RefTriBis0Desc.C:21-22,
+// RefTriBis1Desc.C:22-23, RefTriBis2Desc.C:22-23)
+void bisect_children(const int* v, int j, int m, int ch[2][3]) {
+  ch[0][0] = v[j]; ch[0][1] = m; ch[0][2] = v[(j + 2) % 3];
+  ch[1][0] = v[(j + 1) % 3]; ch[1][1] = v[(j + 2) % 3]; ch[1][2] = m;
+}
+
 }  // namespace
 
-Mesh refine_by_indicator(const Mesh& base, int steps, const IndicatorFn& f, std::vector<HangingVertex>& hanging) {
-  PMCD_CHECK(base.dim() == 2, "refinement by indicator with hanging vertices is implemented for 2D meshes");
+Mesh refine_by_indicator(const Mesh& base, int steps, const IndicatorFn& f, std::vector<HangingVertex>& hanging,
+                         bool conforming) {
+  PMCD_CHECK(base.dim() == 2, "refinement by indicator is implemented for 2D meshes");
+  PMCD_CHECK(!conforming || base.shape == Shape::Triangle,
+             "the conforming closure is implemented for triangles (for quadrilaterals the reference makes "
+             "mixed meshes)");
   EdgeMids mids;
   Mesh cur = base;
   const auto& faces = shape_faces(cur.shape);
-  for(int step = 0; step < steps; ++step) {
-    // RefinementStrategy::apply_indicator: corner values on both sides of zero
-    std::vector<char> mark(cur.n_cells, 0);
-    for(int c = 0; c < cur.n_cells; ++c) {
-      int inner = 0, outer = 0;
-      for(int a = 0; a < cur.nv; ++a) {
-        const double val = f(&cur.coords[3 * cur.cell(c)[a]]);
-        if(val <= 1e-10) inner++;
-        if(val >= -1e-10) outer++;
-      }
-      mark[c] = inner > 0 && outer > 0;
+  auto straddles = [&](const int* v, int n) {
+    int inner = 0, outer = 0;
+    for(int a = 0; a < n; ++a) {
+      const double val = f(&cur.coords[3 * v[a]]);
+      if(val <= 1e-10) inner++;
+      if(val >= -1e-10) outer++;
     }
+    return inner > 0 && outer > 0;
+  };
+  auto refined_edges = [&](int c, int* last) {
+    int n = 0;
+    for(int j = 0; j < cur.nj; ++j)
+      if(edge_mid(mids, cur.cell(c)[faces[j][0]], cur.cell(c)[faces[j][1]]) >= 0) {
+        n++;
+        if(last) *last = j;
+      }
+    return n;
+  };
+  // refine the marked leaves, then every leaf with an edge (u, v) whose
+  // midpoint m has a half (u, m) or (m, v) split again (a neighbour two
+  // levels finer), until the mesh is 1-irregular
+  auto refine_1irregular = [&](std::vector<char>& mark) {
     for(;;) {
       cur = refine_marked(cur, mark, mids);
-      // 1-irregular closure: a leaf edge (u, v) with midpoint m whose half
-      // (u, m) or (m, v) is split again borders a cell two levels finer
       mark.assign(cur.n_cells, 0);
       bool any = false;
       for(int c = 0; c < cur.n_cells; ++c)
@@ -392,8 +412,78 @@ Mesh refine_by_indicator(const Mesh& base, int steps, const IndicatorFn& f, std:
         }
       if(!any) break;
     }
+  };
+  std::vector<int> bis;  // conforming: bisected edge of each leaf of cur, or -1
+  for(int step = 0; step < steps; ++step) {
+    // RefinementStrategy::apply_indicator: corner values on both sides of
+    // zero; a bisected cell (its closure removed first) is marked when one
+    // of its halves is
+    std::vector<char> mark(cur.n_cells, 0);
+    for(int c = 0; c < cur.n_cells; ++c) {
+      const int* v = cur.cell(c);
+      if(!bis.empty() && bis[c] >= 0) {
+        int ch[2][3];
+        bisect_children(v, bis[c], edge_mid(mids, v[bis[c]], v[(bis[c] + 1) % 3]), ch);
+        mark[c] = straddles(ch[0], 3) || straddles(ch[1], 3);
+      } else {
+        mark[c] = straddles(v, cur.nv);
+      }
+    }
+    refine_1irregular(mark);
+    bis.clear();
+    if(!conforming) continue;
+    // TDomain::MakeConfClosure (Domain.C:1219-1360, GridCell.C:525-575):
+    // triangles with two or three refined edges are refined regularly until
+    // none is left; a triangle with one refined edge is bisected on it
+    for(;;) {
+      mark.assign(cur.n_cells, 0);
+      bool any = false;
+      for(int c = 0; c < cur.n_cells; ++c)
+        if(refined_edges(c, nullptr) >= 2) mark[c] = 1, any = true;
+      if(!any) break;
+      refine_1irregular(mark);
+    }
+    bis.assign(cur.n_cells, -1);
+    for(int c = 0; c < cur.n_cells; ++c) {
+      int j = -1;
+      if(refined_edges(c, &j) == 1) bis[c] = j;
+    }
   }
   hanging.clear();
+  if(!bis.empty()) {
+    // the leaves with each bisected cell replaced by its halves; a half's
+    // joints along the parent's edges keep their components
+    Mesh out;
+    out.shape = cur.shape;
+    out.nv = cur.nv;
+    out.nj = cur.nj;
+    out.coords = cur.coords;
+    for(int c = 0; c < cur.n_cells; ++c) {
+      const int* v = cur.cell(c);
+      const int* jc = &cur.joint_comp[(size_t)c * cur.nj];
+      if(bis[c] < 0) {
+        out.cell_vertices.insert(out.cell_vertices.end(), v, v + 3);
+        out.joint_comp.insert(out.joint_comp.end(), jc, jc + 3);
+        out.parent.push_back(c);
+        out.child_index.push_back(0);
+        continue;
+      }
+      const int j = bis[c];
+      int ch[2][3];
+      bisect_children(v, j, edge_mid(mids, v[j], v[(j + 1) % 3]), ch);
+      // edges of {v_j, m, v_j+2}: half of edge j, interior, edge j+2;
+      // of {v_j+1, v_j+2, m}: edge j+1, interior, half of edge j
+      const int comps[2][3] = {{jc[j], -1, jc[(j + 2) % 3]}, {jc[(j + 1) % 3], -1, jc[j]}};
+      for(int h = 0; h < 2; ++h) {
+        out.cell_vertices.insert(out.cell_vertices.end(), ch[h], ch[h] + 3);
+        out.joint_comp.insert(out.joint_comp.end(), comps[h], comps[h] + 3);
+        out.parent.push_back(c);
+        out.child_index.push_back(h);
+      }
+    }
+    out.n_cells = (int)out.parent.size();
+    return out;
+  }
   std::vector<char> seen(cur.n_vertices(), 0);
   for(int c = 0; c < cur.n_cells; ++c)
     for(int j = 0; j < cur.nj; ++j) {

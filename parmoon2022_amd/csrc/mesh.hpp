@@ -73,17 +73,23 @@ struct HangingVertex {
 // indicator value at a point (x, y, z)
 using IndicatorFn = std::function<double(const double* x)>;
 
-// `steps` times TDomain::RefineByIndicator without conforming closure
-// (src/Geometry/Domain.C:1094-1207, RefinementStrategy.C:260-289): a cell
-// whose corner values of the indicator lie on both sides of zero (a value
-// within 1e-10 of zero counts for both) is refined regularly, then every leaf
-// with a neighbour refined two levels deeper across one of its edges is
-// refined until the mesh is 1-irregular (Domain.C:1818-1835, GridCell.C:
-// 348-383, 462-491). Leaves stay in depth-first tree order (a refined cell is
-// replaced by its children, in child order). 2D meshes only. `hanging`
-// receives the hanging vertices in the order their coarse edges are met over
-// (cell, edge).
-Mesh refine_by_indicator(const Mesh& base, int steps, const IndicatorFn& f, std::vector<HangingVertex>& hanging);
+// `steps` times TDomain::RefineByIndicator (src/Geometry/Domain.C:1094-1207,
+// RefinementStrategy.C:260-289): a cell whose corner values of the indicator
+// lie on both sides of zero (a value within 1e-10 of zero counts for both) is
+// refined regularly, then every leaf with a neighbour refined two levels
+// deeper across one of its edges is refined until the mesh is 1-irregular
+// (Domain.C:1818-1835, GridCell.C:348-383, 462-491). Leaves stay in
+// depth-first tree order (a refined cell is replaced by its children, in
+// child order). 2D meshes only. Without `conforming`, `hanging` receives the
+// hanging vertices in the order their coarse edges are met over (cell, edge).
+// With `conforming` (conforming_closure, triangles; MakeConfClosure,
+// Domain.C:1219-1360, GridCell.C:525-575) the closure bisections of the last
+// step are removed before each step (a bisected cell is marked when one of
+// its halves is), triangles with two or three refined edges are refined
+// regularly until none is left, and a triangle with one refined edge is
+// bisected on it: the mesh is conforming and `hanging` stays empty.
+Mesh refine_by_indicator(const Mesh& base, int steps, const IndicatorFn& f, std::vector<HangingVertex>& hanging,
+                         bool conforming = false);
 
 // Cell diameter as used for h_K with CELL_MEASURE=0 (Triangle.C:25-40,
 // Quadrangle.C:26-37, Tetrahedron.C:64-89, longest space diagonal for

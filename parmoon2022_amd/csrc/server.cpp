@@ -232,7 +232,7 @@ std::string mesh_key(const ProblemDesc& d) {
   std::ostringstream os;
   os << d.geometry << '|' << d.n_refine << '|' << d.structured_n << '|' << d.order << '|' << d.supg << '|'
      << d.linear_quad << '|' << d.mg_levels << '|' << d.mesh_file << '|' << d.adapt_steps << '|'
-     << (const void*)d.indicator << '|' << d.indicator_ctx;
+     << (const void*)d.indicator << '|' << d.indicator_ctx << '|' << d.adapt_conforming;
   return os.str();
 }
 

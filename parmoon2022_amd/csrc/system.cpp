@@ -67,7 +67,7 @@ std::vector<std::unique_ptr<Level>> build_levels(const ProblemDesc& d) {
     auto L = std::make_unique<Level>();
     std::vector<HangingVertex> hanging;
     const IndicatorFn f = [&d](const double* x) { return d.indicator(x, d.indicator_ctx); };
-    L->mesh = refine_by_indicator(meshes.back(), d.adapt_steps, f, hanging);
+    L->mesh = refine_by_indicator(meshes.back(), d.adapt_steps, f, hanging, d.adapt_conforming != 0);
     L->space = build_fespace(L->mesh, d.order, bc, &hanging);
     L->st = build_structure(L->space, L->mesh.n_cells);
     levels.push_back(std::move(L));

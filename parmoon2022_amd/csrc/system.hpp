@@ -37,6 +37,7 @@ struct ProblemDesc {
   int adapt_steps = 0;
   double (*indicator)(const double* x, void* ctx) = nullptr;  // evaluated at vertices
   void* indicator_ctx = nullptr;
+  int adapt_conforming = 0;  // 1: conforming closure (triangles), no hanging vertices
 };
 
 struct SolverDesc {

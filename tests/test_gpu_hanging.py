@@ -4,13 +4,14 @@ assembly with the hanging-DOF coupling against the oracle (row-relative
 1e-12, the constraint rows and hanging rhs entries exact), the reference's
 golden errors of test/adaptive_hanging_test.c++ within its 1e-11 for the
 direct and the FGMRES solvers, the solution against the oracle's, and served
-requests.
+requests; the conforming closure (P1/P2 on TwoTriangles) against the same
+test's conforming goldens and the oracle's matrix.
 """
 import numpy as np
 import pytest
 
 import parmoon2022_amd as pm
-from test_oracle import HANGING_GOLDEN, hanging_db, hanging_constraint_residual
+from test_oracle import CONFORMING_GOLDEN, HANGING_GOLDEN, hanging_db, hanging_constraint_residual
 from test_gpu_parity import row_rel_err
 
 pytestmark = pytest.mark.gpu
@@ -118,3 +119,26 @@ def test_hanging_served_requests():
         assert r.error is None
         assert np.array_equal(r.solution, u)
     assert reqs[1].reused
+
+
+@pytest.mark.parametrize("solver", ["direct", "fgmres_ssor"])
+@pytest.mark.parametrize("order", [1, 2])
+def test_conforming_closure_on_gpu(oracle_lib, order, solver):
+    extra = dict(FGMRES, preconditioner="ssor") if solver != "direct" else {}
+    db = hanging_db("TwoTriangles", order, conforming_closure=True, **extra)
+    print("conforming closure", order, solver, flush=True)
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    assert np.array_equal(cd.dofmap(), o.dofmap())
+    rp, col = cd.pattern()
+    assert row_rel_err(rp, cd.matrix_values(), o.matrix_values()) <= 1e-12
+    hg, _ = _hanging(cd)
+    assert len(hg) == 0
+    st = cd.solve()
+    assert st.converged
+    e = cd.output()
+    got = [e[0], e[1], e[2], e[4]]
+    assert np.allclose(got, CONFORMING_GOLDEN[order], rtol=0, atol=1e-11), (got, CONFORMING_GOLDEN[order])
+    cd.close()

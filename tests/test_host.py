@@ -479,8 +479,8 @@ def test_hanging_setup_matches_oracle(oracle_lib, geo, order):
 
 def test_hanging_setup_rejections():
     db = _hanging_db("UnitSquare")
-    with pytest.raises(ValueError, match="conforming_closure"):
-        pm.problem_desc(dict(db, conforming_closure=True))
+    with pytest.raises(RuntimeError, match="triangles"):
+        pm.HostSetup(dict(db, conforming_closure=True))
     with pytest.raises(RuntimeError, match="order"):
         pm.HostSetup(dict(db, ANSATZ_ORDER=3))
     with pytest.raises(RuntimeError, match="2D"):
@@ -542,3 +542,30 @@ def test_structure_test_hanging_fixtures(oracle_lib, geo, order, want):
         o = oracle_lib.Oracle(pm.problem_desc(db))
         orp, ocol = o.pattern()
         assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_conforming_closure_setup_matches_oracle(oracle_lib, order):
+    """The conforming closure on TwoTriangles: the library's leaves, numbering
+    and pattern equal the oracle's; no hanging DOFs remain; every interior
+    edge is shared by exactly two cells (conforming)."""
+    db = dict(_hanging_db("TwoTriangles", order), conforming_closure=True)
+    h = pm.HostSetup(db)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    assert h.info() == o.info()
+    X = h.cell_coords()
+    assert np.array_equal(X, o.cell_coords(nv=3))
+    assert np.array_equal(h.dofmap(), o.dofmap())
+    rp, col = h.pattern()
+    orp, ocol = o.pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+    hg, _ = _host_hanging(h)
+    assert len(hg) == 0
+    edges = {}
+    for c in range(len(X)):
+        for j in range(3):
+            a, b = tuple(np.round(X[c, j, :2], 12)), tuple(np.round(X[c, (j + 1) % 3, :2], 12))
+            edges[tuple(sorted([a, b]))] = edges.get(tuple(sorted([a, b])), 0) + 1
+    for (a, b), n in edges.items():
+        on_bnd = any(abs(a[k] - b[k]) < 1e-14 and (abs(a[k]) < 1e-14 or abs(a[k] - 1) < 1e-14) for k in range(2))
+        assert n == (1 if on_bnd else 2), (a, b, n)

@@ -644,6 +644,14 @@ HANGING_FGMRES = {"solver_type": "iterative", "iterative_solver_type": "fgmres",
                   "max_n_iterations": 5000, "residual_tolerance": 1e-13, "gmres_restart": 50}
 
 
+# the conforming half of the same test (conforming_closure true, the
+# default): P1 on TwoTriangles (:126-135), P2 from its "testall" run
+CONFORMING_GOLDEN = {
+    1: [0.035770938439154, 0.51608967114182, 0.51608967114182, 0.1410584617451],
+    2: [0.0025402849496846, 0.074968557695118, 0.074968557695118, 0.013210208083023],
+}
+
+
 def hanging_db(geo, order=1, **extra):
     db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 0, "example": "sine_laplace_2d",
           "refine_by_indicator_steps": 8, "refine_indicator": (0.5, 1.0, 0.0, -0.5), "conforming_closure": False,
@@ -784,3 +792,30 @@ def test_oracle_hanging_order2_reproduces_quadratics(oracle_lib, geo):
     rp, col = o.pattern()
     for h, a, m, b in hg:
         assert {a, m, b} <= set(col[rp[h]:rp[h + 1]])
+
+
+@pytest.mark.parametrize("order", [1, 2])
+def test_oracle_conforming_closure_golden(oracle_lib, order):
+    """The conforming closure (bisections and regular refinement of
+    triangles with two refined edges) gives the reference's errors; the mesh
+    is conforming (no hanging DOFs) and tiles the square. Planting a closure
+    that only refines triangles with three refined edges fails them. (Marking
+    a bisected cell through its halves rather than its own corners gives the
+    same mesh for this indicator: that rule is followed from the reference's
+    code, not pinned.)"""
+    extra = HANGING_FGMRES if order == 2 else {}
+    db = hanging_db("TwoTriangles", order, conforming_closure=True, **extra)
+    got, o = _hanging_errors(oracle_lib, db)
+    assert np.allclose(got, CONFORMING_GOLDEN[order], rtol=0, atol=1e-11), (got, CONFORMING_GOLDEN[order])
+    hg, _ = o.hanging()
+    assert len(hg) == 0
+    X = o.cell_coords(nv=3)[:, :, :2]
+    e1, e2 = X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]
+    assert abs(np.sum(0.5 * np.abs(e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0])) - 1.0) < 1e-13
+    if order == 1:
+        try:
+            oracle_lib.lib().orc_set_fault(11)
+            bad, _ = _hanging_errors(oracle_lib, db)
+        finally:
+            oracle_lib.lib().orc_set_fault(0)
+        assert np.max(np.abs(bad - CONFORMING_GOLDEN[1])) > 1e-6

@@ -749,10 +749,102 @@ def bench_afc(args):
     print(json.dumps(result), flush=True)
 
 
+def adaptive_db(args):
+    return {"geo_file": "TwoTriangles", "ANSATZ_ORDER": 2, "refinement_n_initial_steps": args.adapt_refine,
+            "example": "sine_laplace_2d", "refine_by_indicator_steps": args.adapt_steps,
+            "refine_indicator": (0.5, 1.0, 0.0, -0.5), "conforming_closure": False,
+            "solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "ssor",
+            "sor_omega": 1.0, "residual_tolerance": 1e-10, "residual_reduction": 0.0,
+            "max_n_iterations": max(args.max_iter, 20000), "gmres_restart": 50}
+
+
+def bench_adaptive(args):
+    """Adaptive refinement with hanging vertices: P2 on TwoTriangles refined
+    --adapt-refine times, then --adapt-steps steps of TDomain::RefineByIndicator
+    with the indicator of test/adaptive_hanging_test.c++ (0.5 x^2 + y - 0.5,
+    no conforming closure), the sine_laplace problem. One step = assembly
+    with the hanging-DOF coupling + FGMRES(50) + SSOR to 1e-10. Roofline: the
+    SpMV of the coupled matrix (HIP events). The CPU baseline runs the
+    oracle's assembly and 3 FGMRES iterations on the same problem and scales
+    to the GPU's iteration count."""
+    import numpy as np
+    import parmoon2022_amd as pm
+    db = adaptive_db(args)
+    t0 = time.perf_counter()
+    cd = pm.ConvectionDiffusion(db)
+    info = cd.info()
+    log("setup %.1f s" % (time.perf_counter() - t0), info)
+    cd.assemble()
+    tuning = pm.autotune(cd, log=log)
+    lat, iters = [], []
+    for step in range(args.warmup + args.steps):
+        t0 = time.perf_counter()
+        cd.assemble()
+        st = cd.solve()
+        dt = time.perf_counter() - t0
+        log("step %d: %.3f s, %d iterations, residual %.2e" % (step, dt, st.iterations, st.final_residual))
+        if step >= args.warmup:
+            lat.append(dt)
+            iters.append(st.iterations)
+    errs = cd.output()
+    n, nnz = info["n_dof"], info["nnz"]
+    t_spmv = cd.time_kernel("spmv_f64", -1, 20)
+    spmv_bytes = 12 * nnz + 4 * (n + 1) + 16 * n
+    achieved = spmv_bytes / (t_spmv * 1e-3) / 1e9
+    lanes, variant = pm.tuning("spmv")
+    spmv_name = ("k_spmv_sell<%d,f64>" % pm.get_sell_unroll() if variant == 4 else
+                 "k_spmv%s<%d,f64>" % ({0: "", 1: "_stream", 2: "_unrolled", 3: "_unrolled_nt"}[variant], lanes))
+    import ctypes
+    nh = ctypes.c_int()
+    pm._check(pm.lib().pmcd_get_hanging(cd._s, -1, None, None, ctypes.byref(nh)))
+    total = sum(lat)
+    value = args.steps / total
+    result = {
+        "metric": "CD3D solves/s & DoF/s at fixed L2 err vs FP64 oracle; p50/p99 solve latency",
+        "value": value, "unit": "solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * total / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (manufactured solution, generated mesh)",
+        "config": {"workload": "adaptive CD2D P2, TwoTriangles refined %d times then %d indicator steps with "
+                               "hanging vertices, %d DoF (%d hanging), FGMRES(50) + SSOR"
+                               % (args.adapt_refine, args.adapt_steps, n, nh.value),
+                   "dof": n, "nnz": nnz, "hanging_dofs": nh.value, "cells": info["n_cells"]},
+        "dof_per_s": value * n,
+        "iterations": int(np.median(iters)), "converged": bool(st.converged),
+        "l2_error": float(errs[0]), "h1_error": float(errs[1]),
+        "roofline": {"bound": "hbm", "kernel": spmv_name + " (coupled matrix)", "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "algorithmic_bytes": spmv_bytes, "avg_ms": t_spmv},
+        "autotune_ms": {k: {str(w): round(t, 5) for w, t in v.items()} for k, v in tuning.items()},
+    }
+    gpu_iters = int(np.median(iters))
+    cd.close()
+    if not args.no_cpu:
+        import oracle
+        o = oracle.Oracle(pm.problem_desc(db))
+        t0 = time.time()
+        o.assemble()
+        t_asm = time.time() - t0
+        sd = pm.solver_desc(db)
+        sd.max_iter = 3
+        sd.min_iter = 3
+        t0 = time.time()
+        o.solve(sd)
+        per_it = (time.time() - t0) / 3
+        per_solve = t_asm + per_it * gpu_iters
+        result["cpu_baseline"] = {"value": 1.0 / per_solve, "unit": "solves/s", "cores": 1, "kind": "port",
+                                  "sample": "oracle (C, -O3, 1 thread) on the same problem: assembly with the "
+                                            "hanging coupling (%.2f s) and 3 FGMRES(50) + SSOR iterations "
+                                            "(%.3f s each, exact SSOR order), scaled to the GPU's %d "
+                                            "iterations; set-up untimed" % (t_asm, per_it, gpu_iters),
+                                  "machine": machine()}
+        result["gpu_over_cpu"] = value * per_solve
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "c5", "tcd", "afc"],
+    ap.add_argument("--workload", default="c2", choices=["c1", "c2", "c4", "c5", "tcd", "afc", "adaptive"],
                     help="c2: one 64^3 Q2 SUPG solve per step (default); c1: the CD2D P1 CG + Jacobi "
                          "problem on 257^2 vertices; c4: a batch of --instances P1 "
                          "tetrahedral instances per step; c5: a served stream of mixed CD2D P2 / CD3D Q1 "
@@ -760,6 +852,8 @@ def main():
                          "quadratic space-time example with Q2 SUPG on the --refine cube")
     ap.add_argument("--tau", type=float, default=0.01, help="tcd time step length")
     ap.add_argument("--afc-refine", type=int, default=8, help="afc: refinements of TwoTriangles")
+    ap.add_argument("--adapt-refine", type=int, default=5, help="adaptive: uniform refinements first")
+    ap.add_argument("--adapt-steps", type=int, default=8, help="adaptive: refinement-by-indicator steps")
     ap.add_argument("--rates", default="10,30,100,300,1000,2000", help="c5 arrival rates (requests/s)")
     ap.add_argument("--requests", type=int, default=60, help="c5 requests per rate")
     ap.add_argument("--slots", type=int, default=8, help="c5 concurrent requests")
@@ -812,6 +906,8 @@ def main():
         return bench_tcd(args)
     if args.workload == "afc":
         return bench_afc(args)
+    if args.workload == "adaptive":
+        return bench_adaptive(args)
     db = c2_db(args)
     t0 = time.time()
     cd = pm.ConvectionDiffusion(db)

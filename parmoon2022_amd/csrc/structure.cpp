@@ -5,6 +5,16 @@
 
 namespace pmcd {
 
+bool pattern_symmetric(int n, const int* row_ptr, const int* col) {
+  for(int r = 0; r < n; ++r)
+    for(int k = row_ptr[r] + 1; k < row_ptr[r + 1]; ++k)
+      if(col[k - 1] >= col[k]) return false;  // unsorted rows are taken as not symmetric
+  for(int r = 0; r < n; ++r)
+    for(int k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+      if(!std::binary_search(col + row_ptr[col[k]], col + row_ptr[col[k] + 1], r)) return false;
+  return true;
+}
+
 Structure structure_from_pattern(const FESpace& sp, int n_cells, const int* row_ptr, const int* col) {
   Structure st;
   const int nl = sp.n_local;
@@ -36,6 +46,7 @@ Structure structure_from_pattern(const FESpace& sp, int n_cells, const int* row_
   // error (the reference prints "not found" and drops the entry)
   st.diag.assign(n, -1);
   st.slot.resize(st.contrib.size() * (size_t)nl);
+  st.symmetric = pattern_symmetric(n, row_ptr, col);
   for(int r = 0; r < n; ++r) {
     const int b = row_ptr[r], e = row_ptr[r + 1];
     for(int k = b; k < e; ++k)
@@ -165,17 +176,37 @@ Structure hanging_structure(const FESpace& sp, int n_cells, const Structure& bas
   for(int r = 0; r < n; ++r) row_ptr[r + 1] = row_ptr[r] + (int)rows[r].size();
   col.reserve(row_ptr[n]);
   for(int r = 0; r < n; ++r) col.insert(col.end(), rows[r].begin(), rows[r].end());
-  return structure_from_pattern(sp, n_cells, row_ptr.data(), col.data());
+  return structure_from_pattern(sp, n_cells, row_ptr.data(), col.data());  // checks the symmetry
 }
 
 SweepOrder build_sweep_order(const Structure& st, int max_exact_classes) {
   const int n = st.n_rows;
+  // rows c < r that hold column r (the transposed half of the adjacency,
+  // needed when the pattern is not symmetric; a neighbour met twice is
+  // harmless)
+  std::vector<int> tp(n + 1, 0), tc;
+  if(!st.symmetric) {
+    for(int c = 0; c < n; ++c)
+      for(int k = st.row_ptr[c]; k < st.row_ptr[c + 1]; ++k)
+        if(st.col[k] > c) tp[st.col[k] + 1]++;
+    for(int r = 0; r < n; ++r) tp[r + 1] += tp[r];
+    tc.resize(tp[n]);
+    std::vector<int> fill(tp.begin(), tp.end() - 1);
+    for(int c = 0; c < n; ++c)
+      for(int k = st.row_ptr[c]; k < st.row_ptr[c + 1]; ++k)
+        if(st.col[k] > c) tc[fill[st.col[k]]++] = c;
+  }
+  // visits every row c < r adjacent to r
+  auto lower = [&](int r, auto&& f) {
+    for(int k = st.row_ptr[r]; k < st.row_ptr[r + 1]; ++k)
+      if(st.col[k] < r) f(st.col[k]);
+    for(int k = tp[r]; k < tp[r + 1]; ++k) f(tc[k]);
+  };
   std::vector<int> cls(n, 0);
   int n_cls = n > 0 ? 1 : 0;
   for(int r = 0; r < n; ++r) {
     int c = 0;
-    for(int k = st.row_ptr[r]; k < st.row_ptr[r + 1]; ++k)
-      if(st.col[k] < r) c = std::max(c, cls[st.col[k]] + 1);
+    lower(r, [&](int j) { c = std::max(c, cls[j] + 1); });
     cls[r] = c;
     n_cls = std::max(n_cls, c + 1);
   }
@@ -187,8 +218,7 @@ SweepOrder build_sweep_order(const Structure& st, int max_exact_classes) {
     std::vector<int> stamp(1, -1);
     n_cls = 0;
     for(int r = 0; r < n; ++r) {
-      for(int k = st.row_ptr[r]; k < st.row_ptr[r + 1]; ++k)
-        if(st.col[k] < r) stamp[cls[st.col[k]]] = r;
+      lower(r, [&](int j) { stamp[cls[j]] = r; });
       int c = 0;
       while(c < (int)stamp.size() && stamp[c] == r) ++c;
       if(c == (int)stamp.size()) stamp.push_back(-1);

@@ -32,9 +32,16 @@ struct Structure {
   // slot[e * n_local + k] = position (relative to row_ptr[r]) of dof(c, k) in
   // row r, for contribution e = (c, m) of row r
   std::vector<uint8_t> slot;
+  // (r, c) in the pattern exactly when (c, r) is: true for the cell pattern,
+  // false after the hanging columns are added (or for a caller's pattern
+  // that is not)
+  bool symmetric = true;
 };
 
 Structure build_structure(const FESpace& sp, int n_cells);
+// true when every row's columns ascend and (r, c) is in the pattern exactly
+// when (c, r) is
+bool pattern_symmetric(int n, const int* row_ptr, const int* col);
 // a space with hanging DOFs: `base` (the cell pattern) with the columns of
 // each hanging row added to the rows of its active coupling DOFs
 Structure hanging_structure(const FESpace& sp, int n_cells, const Structure& base);
@@ -45,8 +52,10 @@ Structure structure_from_pattern(const FESpace& sp, int n_cells, const int* row_
 
 // Row classes for a successive over-relaxation sweep on the device. A forward
 // sweep visits the classes in order, a backward sweep in reverse; rows of one
-// class are never adjacent (the pattern is structurally symmetric), so a
-// class updates in parallel.
+// class are never adjacent, so a class updates in parallel. Adjacency is
+// symmetric: r and c are adjacent when either row holds the other's column
+// (a pattern with hanging columns is not symmetric: row r may read x_c while
+// row c does not read x_r, and c must still not be updated beside r).
 //
 // exact: classes are the levels of the row-order dependency graph
 // (class(r) = 1 + max class of the adjacent rows c < r). Every row then reads

@@ -156,6 +156,7 @@ CDSystem::CDSystem(const std::vector<OperatorLevel>& lv, hipStream_t stream) : s
       }
       PMCD_CHECK(st.diag[r] >= 0, "level ", l, ": row ", r, " has no diagonal entry");
     }
+    st.symmetric = pattern_symmetric(o.n, st.row_ptr.data(), st.col.data());
     L->row_ptr.upload(st.row_ptr, stream_);
     L->col.upload(st.col, stream_);
     L->diag_pos.upload(st.diag, stream_);

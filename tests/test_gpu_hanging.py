@@ -142,3 +142,34 @@ def test_conforming_closure_on_gpu(oracle_lib, order, solver):
     got = [e[0], e[1], e[2], e[4]]
     assert np.allclose(got, CONFORMING_GOLDEN[order], rtol=0, atol=1e-11), (got, CONFORMING_GOLDEN[order])
     cd.close()
+
+
+@pytest.mark.parametrize("flag", [0, 1, 2])
+@pytest.mark.parametrize("geo,order", [("TwoTriangles", 1), ("UnitSquare", 2)])
+def test_hanging_sor_sweep_equals_serial(oracle_lib, geo, order, flag):
+    """The device SOR sweep on a coupled matrix (its pattern is not
+    symmetric) equals the oracle's serial sweep bit for bit, and an
+    SSOR-preconditioned FGMRES solve repeats bit for bit."""
+    db = hanging_db(geo, order, **dict(FGMRES, preconditioner="ssor", sor_omega=1.0))
+    print("hanging sor", geo, order, flag, flush=True)
+    pm.set_sor_exact_limit(100000)  # read when the sweep order is first built
+    try:
+        cd = pm.ConvectionDiffusion(db)
+        cd.assemble()
+        rng = np.random.default_rng(flag)
+        n = cd.info()["n_dof"]
+        b, x0 = rng.standard_normal(n), rng.standard_normal(n)
+        got, exact = cd.sor_sweep(b, x0.copy(), omega=1.5, flag=flag)
+    finally:
+        pm.set_sor_exact_limit(24)
+    assert exact
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    o.assemble()
+    o.set_matrix(cd.matrix_values())  # the same (device-assembled) matrix
+    assert np.array_equal(got, o.sor_sweep(b, x0, 1.5, flag))
+    st1 = cd.solve()
+    u1 = cd.solution()
+    cd.assemble()
+    st2 = cd.solve()
+    assert st1.converged and st1.iterations == st2.iterations and np.array_equal(u1, cd.solution())
+    cd.close()

@@ -585,3 +585,37 @@ def test_conforming_closure_setup_matches_oracle(oracle_lib, order):
     for (a, b), n in edges.items():
         on_bnd = any(abs(a[k] - b[k]) < 1e-14 and (abs(a[k]) < 1e-14 or abs(a[k] - 1) < 1e-14) for k in range(2))
         assert n == (1 if on_bnd else 2), (a, b, n)
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_refinement_by_indicator_random_indicators_match_oracle(oracle_lib, seed):
+    """Random quadratic indicators a x^2 + b y + e whose zero set passes
+    through a random point of the square, both shapes, orders 1 and 2, with
+    and without the conforming closure (triangles): the library's leaves,
+    numbering, pattern and hanging list equal the oracle's."""
+    rng = np.random.default_rng(seed)
+    geo = ["TwoTriangles", "UnitSquare"][seed % 2]
+    order = 1 + (seed // 2) % 2
+    conforming = geo == "TwoTriangles" and seed % 4 == 0
+    a, b = (float(v) for v in rng.uniform(-1, 1, 2))
+    b = b if abs(b) > 0.1 else 0.5
+    px, py = (float(v) for v in rng.uniform(0.1, 0.9, 2))
+    db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": int(rng.integers(0, 3)),
+          "example": "sine_2d", "refine_by_indicator_steps": int(rng.integers(3, 7)),
+          "refine_indicator": (a, b, 0.0, -(a * px * px + b * py)), "conforming_closure": conforming}
+    h = pm.HostSetup(db)
+    o = oracle_lib.Oracle(pm.problem_desc(db))
+    assert h.info() == o.info(), db
+    assert np.array_equal(h.cell_coords(), o.cell_coords(nv=3 if geo == "TwoTriangles" else 4))
+    assert np.array_equal(h.dofmap(), o.dofmap())
+    rp, col = h.pattern()
+    orp, ocol = o.pattern()
+    assert np.array_equal(rp, orp) and np.array_equal(col, ocol)
+    hg, w = _host_hanging(h)
+    ohg, ow = o.hanging()
+    assert np.array_equal(hg, ohg) and np.array_equal(w, ow)
+    assert h.info()["n_cells"] > 8
+    if conforming:
+        assert len(hg) == 0
+    else:
+        assert len(hg) > 0

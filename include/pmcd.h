@@ -422,7 +422,10 @@ enum { PMCD_SHAPE_TRIANGLE = 0, PMCD_SHAPE_QUADRANGLE = 1, PMCD_SHAPE_TETRAHEDRO
  * (inner and non-Dirichlet DOFs first, Dirichlet DOFs last), and the cells'
  * vertex coordinates in the element's local vertex order (TBaseCell::
  * GetVertex). Conforming continuous Lagrange elements P1/P2 on simplices,
- * Q1/Q2 on quadrilaterals and hexahedra; affine cells. */
+ * Q1/Q2 on quadrilaterals and hexahedra; affine cells. A space with hanging
+ * DOFs (get_n_hanging() > 0) is not accepted here: its coupling needs the
+ * hanging DOFs' couplings and weights, which only the library's own
+ * set-up (pmcd_system_create with adapt_steps) has. */
 typedef struct {
   int shape;                  /* PMCD_SHAPE_* */
   int order;                  /* 1 or 2 */

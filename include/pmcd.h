@@ -419,22 +419,30 @@ enum { PMCD_SHAPE_TRIANGLE = 0, PMCD_SHAPE_QUADRANGLE = 1, PMCD_SHAPE_TETRAHEDRO
 /* A caller's FE space (TFESpace2D / TFESpace3D, include/FE/FESpace.h):
  * GetGlobalNumbers() and GetBeginIndex() (cell c owns global_numbers
  * [begin_index[c] .. begin_index[c+1])), get_n_dof() and get_n_active()
- * (inner and non-Dirichlet DOFs first, Dirichlet DOFs last), and the cells'
- * vertex coordinates in the element's local vertex order (TBaseCell::
- * GetVertex). Conforming continuous Lagrange elements P1/P2 on simplices,
- * Q1/Q2 on quadrilaterals and hexahedra; affine cells. A space with hanging
- * DOFs (get_n_hanging() > 0) is not accepted here: its coupling needs the
- * hanging DOFs' couplings and weights, which only the library's own
- * set-up (pmcd_system_create with adapt_steps) has. */
+ * (inner and non-Dirichlet DOFs first, then the hanging DOFs, Dirichlet DOFs
+ * last), and the cells' vertex coordinates in the element's local vertex
+ * order (TBaseCell::GetVertex). Continuous Lagrange elements P1/P2 on
+ * simplices, Q1/Q2 on quadrilaterals and hexahedra; affine cells.
+ * Hanging DOFs (get_n_hanging(), FESpace.h:361-391, in 2D): the last
+ * n_hanging of the n_active DOFs; hanging[4i] is one of them and
+ * hanging[4i+1..4i+3] its coupling DOFs (THangingNode::GetDOF of
+ * get_hanging_node(), HangingNode.h:39-47, -1 beyond GetN_Nodes()),
+ * hanging_weights[3i..3i+2] their weights (GetCoeff). The
+ * structure must then hold each hanging row's columns in the rows of its
+ * active coupling DOFs, as the reference's structure does. n_hanging = 0
+ * for a conforming space (the two pointers are not read). */
 typedef struct {
   int shape;                  /* PMCD_SHAPE_* */
   int order;                  /* 1 or 2 */
   int n_cells;
   int n_dof;
-  int n_active;
+  int n_active;               /* hanging DOFs included */
   const double* vertices;     /* [n_cells][vertices per cell][3] */
   const int* global_numbers;  /* sum over cells of the local DOF counts */
   const int* begin_index;     /* n_cells + 1 */
+  int n_hanging;
+  const int* hanging;              /* [n_hanging][4] */
+  const double* hanging_weights;   /* [n_hanging][3] */
 } pmcd_fe_space;
 
 /* A caller's square structure (TStructure, include/Matrix/Structure.h):
@@ -454,7 +462,13 @@ typedef struct {
  * (Assemble3D.C:258-300,346-360): active rows get the cell contributions
  * added in increasing cell order, each Dirichlet row gets 1 on its diagonal
  * (the rest of the row is left unchanged), the rhs gets the contributions
- * added on active DOFs and the Dirichlet values set on the others.
+ * added on active DOFs and the Dirichlet values set on the others. With
+ * hanging DOFs the sums are coupled first (Assemble2D.C:3350-3427: each
+ * hanging row, weighted, added to the rows of its active coupling DOFs in
+ * the order the couplings require), the coupled sums are added to the
+ * non-hanging active rows, and each hanging row is set to its constraint
+ * (1 on the diagonal, minus the weights at the coupling DOFs, 0 elsewhere)
+ * with rhs 0.
  * dirichlet_values: the n_dof - n_active boundary values in DOF order (the
  * reference evaluates BoundaryValues through the nodal functionals; host
  * callbacks cannot run on the device), NULL for zero. The caller resets the

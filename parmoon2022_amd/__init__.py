@@ -608,7 +608,8 @@ class FeSpaceDesc(ctypes.Structure):
     _fields_ = [("shape", ctypes.c_int), ("order", ctypes.c_int), ("n_cells", ctypes.c_int),
                 ("n_dof", ctypes.c_int), ("n_active", ctypes.c_int),
                 ("vertices", ctypes.POINTER(ctypes.c_double)), ("global_numbers", ctypes.POINTER(ctypes.c_int)),
-                ("begin_index", ctypes.POINTER(ctypes.c_int))]
+                ("begin_index", ctypes.POINTER(ctypes.c_int)), ("n_hanging", ctypes.c_int),
+                ("hanging", ctypes.POINTER(ctypes.c_int)), ("hanging_weights", ctypes.POINTER(ctypes.c_double))]
 
 
 class StructureDesc(ctypes.Structure):
@@ -636,16 +637,24 @@ class Assembler:
     (include/AssembleRoutines/Assemble3D.h:28-34) on a caller's FE space and
     structure: `shape` a key of SHAPES, `vertices` [n_cells, nv, 3] in the
     element's vertex order, `global_numbers` / `begin_index` the space's
-    local-to-global map, `n_active` its active bound, (`row_ptr`, `col`) the
-    structure, `db` the PDE data and forms (reference keys). assemble() adds
-    into the caller's entries and rhs like the reference's scatter."""
+    local-to-global map, `n_active` its active bound (hanging DOFs
+    included), (`row_ptr`, `col`) the structure, `db` the PDE data and forms
+    (reference keys), `hanging` an optional pair (n x 4 ints: hanging DOF and
+    its coupling DOFs, -1 padded; n x 3 weights). assemble() adds into the
+    caller's entries and rhs like the reference's scatter."""
 
-    def __init__(self, shape, order, vertices, global_numbers, begin_index, n_dof, n_active, row_ptr, col, db):
+    def __init__(self, shape, order, vertices, global_numbers, begin_index, n_dof, n_active, row_ptr, col, db,
+                 hanging=None):
         self._keep = [_f64(vertices), _i32(global_numbers), _i32(begin_index), _i32(row_ptr), _i32(col)]
         v, g, b, rp, cl = self._keep
+        nh, hp, hw = 0, None, None
+        if hanging is not None and len(hanging[0]):
+            hd, hwt = _i32(np.reshape(hanging[0], (-1, 4))), _f64(np.reshape(hanging[1], (-1, 3)))
+            self._keep += [hd, hwt]
+            nh, hp, hw = len(hd), _ptr(hd, ctypes.c_int), _ptr(hwt, ctypes.c_double)
         self.space = FeSpaceDesc(SHAPES[shape] if isinstance(shape, str) else int(shape), int(order),
                                  len(b) - 1, int(n_dof), int(n_active), _ptr(v, ctypes.c_double),
-                                 _ptr(g, ctypes.c_int), _ptr(b, ctypes.c_int))
+                                 _ptr(g, ctypes.c_int), _ptr(b, ctypes.c_int), nh, hp, hw)
         self.structure = StructureDesc(int(n_dof), _ptr(rp, ctypes.c_int), _ptr(cl, ctypes.c_int))
         self.desc = problem_desc(db)
         self.nnz = int(rp[-1])
@@ -677,9 +686,10 @@ class Assembler:
 
 
 def assemble_into(shape, order, vertices, global_numbers, begin_index, n_dof, n_active, row_ptr, col, db,
-                  entries, rhs, dirichlet_values=None):
+                  entries, rhs, dirichlet_values=None, hanging=None):
     """One-shot pmcd_assemble_into (see Assembler)."""
-    a = Assembler(shape, order, vertices, global_numbers, begin_index, n_dof, n_active, row_ptr, col, db)
+    a = Assembler(shape, order, vertices, global_numbers, begin_index, n_dof, n_active, row_ptr, col, db,
+                  hanging)
     try:
         a.assemble(entries, rhs, dirichlet_values)
     finally:

@@ -750,6 +750,8 @@ int pmcd_assembler_create(const pmcd_fe_space* space, const pmcd_structure* st, 
     PMCD_CHECK(st->n_rows == space->n_dof, "structure rows ", st->n_rows, " differ from the space's DOFs ",
                space->n_dof);
     PMCD_CHECK(space->shape >= 0 && space->shape <= 3, "unknown cell shape ", space->shape);
+    PMCD_CHECK(space->n_hanging >= 0, "negative hanging DOF count");
+    PMCD_CHECK(space->n_hanging == 0 || space->shape <= 1, "hanging DOFs are supported in 2D only");
     check_visible_gpu();
     auto h = std::make_unique<pmcd_assembler>();
     PMCD_HIP(hipStreamCreate(&h->stream));
@@ -758,7 +760,8 @@ int pmcd_assembler_create(const pmcd_fe_space* space, const pmcd_structure* st, 
       h->a = std::make_unique<CallerAssembler>((Shape)space->shape, space->order, space->n_cells, space->n_dof,
                                                space->n_active, space->vertices, space->global_numbers,
                                                space->begin_index, st->row_ptr, st->col, forms_desc(*la),
-                                               h->stream);
+                                               h->stream, space->n_hanging, space->hanging,
+                                               space->hanging_weights);
     } catch(...) {
       (void)hipStreamDestroy(h->stream);
       throw;

@@ -18,9 +18,13 @@ namespace pmcd {
 
 class CallerAssembler {
  public:
+  // n_hanging > 0: the hanging DOFs n_active - n_hanging .. n_active - 1,
+  // hanging[4i] = the hanging DOF, hanging[4i+1..4i+3] its coupling DOFs (-1
+  // beyond their count), weights[3i..3i+2] their weights
   CallerAssembler(Shape shape, int order, int n_cells, int n_dof, int n_active, const double* vertices,
                   const int* global_numbers, const int* begin_index, const int* row_ptr, const int* col,
-                  const ProblemDesc& forms, hipStream_t stream);
+                  const ProblemDesc& forms, hipStream_t stream, int n_hanging = 0, const int* hanging = nullptr,
+                  const double* weights = nullptr);
   // local matrices and rhs of every cell summed into the device copies (CSR
   // values with identity Dirichlet rows; rhs with the Dirichlet values on the
   // tail); dirichlet_values: n_dof - n_active values (host or device memory,
@@ -30,7 +34,9 @@ class CallerAssembler {
   // 346-360 and its boundary loop): active rows get the sums added, each
   // Dirichlet row's diagonal is set to 1 (the rest of the row is left as it
   // is), the rhs gets the sums added on active DOFs and the Dirichlet values
-  // set on the others
+  // set on the others; with hanging DOFs the coupled sums are added and the
+  // hanging rows and rhs entries are set to the constraints and 0
+  // (Assemble2D.C:3350-3427 on the caller's reset arrays)
   void add_into_host(double* entries, double* rhs);
   void add_into_device(double* entries, double* rhs, hipStream_t caller_stream);
   int n_dof() const { return L_.space.n_dof; }

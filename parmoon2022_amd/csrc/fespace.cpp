@@ -159,32 +159,40 @@ FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc, const 
     sp.hang_cpl.assign((size_t)MC * nh, -1);
     for(int h = 0; h < nh; ++h)
       for(int k = 0; k < hang_n[h]; ++k) sp.hang_cpl[MC * h + k] = number[hang_cpl_class[MC * h + k]];
-    // depth-first post-order over hanging coupling DOFs, reversed
-    std::vector<char> seen(nh, 0);
-    std::vector<int> post;
-    post.reserve(nh);
-    std::vector<std::pair<int, int>> stack;  // (hanging DOF, next coupling)
-    for(int r = 0; r < nh; ++r) {
-      if(seen[r]) continue;
-      seen[r] = 1;
-      stack.push_back({r, 0});
-      while(!stack.empty()) {
-        auto& top = stack.back();
-        if(top.second < sp.hang_n[top.first]) {
-          const int c = sp.hang_cpl[MC * top.first + top.second++] - base;
-          if(c >= 0 && c < nh && !seen[c]) {
-            seen[c] = 1;
-            stack.push_back({c, 0});
-          }
-        } else {
-          post.push_back(top.first);
-          stack.pop_back();
-        }
-      }
-    }
-    sp.hang_order.assign(post.rbegin(), post.rend());
+    (void)base;
+    sort_hanging(sp);
   }
   return sp;
+}
+
+void sort_hanging(FESpace& sp) {
+  constexpr int MC = FESpace::kMaxCoupling;
+  const int nh = sp.n_hanging, base = sp.n_active_non_hanging();
+  // depth-first post-order over hanging coupling DOFs, reversed
+  std::vector<char> seen(nh, 0);
+  std::vector<int> post;
+  post.reserve(nh);
+  std::vector<std::pair<int, int>> stack;  // (hanging DOF, next coupling)
+  for(int r = 0; r < nh; ++r) {
+    if(seen[r]) continue;
+    seen[r] = 1;
+    stack.push_back({r, 0});
+    while(!stack.empty()) {
+      auto& top = stack.back();
+      if(top.second < sp.hang_n[top.first]) {
+        const int c = sp.hang_cpl[MC * top.first + top.second++] - base;
+        if(c >= 0 && c < nh && !seen[c]) {
+          seen[c] = 1;
+          stack.push_back({c, 0});
+        }
+      } else {
+        post.push_back(top.first);
+        stack.pop_back();
+      }
+    }
+  }
+  PMCD_CHECK((int)post.size() == nh, "not all hanging DOFs sorted");
+  sp.hang_order.assign(post.rbegin(), post.rend());
 }
 
 }  // namespace pmcd

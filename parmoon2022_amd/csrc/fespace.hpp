@@ -62,4 +62,9 @@ struct FESpace {
 FESpace build_fespace(const Mesh& mesh, int order, const BoundCondFn& bc,
                       const std::vector<HangingVertex>* hanging = nullptr);
 
+// hang_order from hang_n / hang_cpl: sort_hanging_nodes (FESpace.C:2101-2160),
+// depth-first post-order over the coupling DOFs that hang themselves,
+// reversed
+void sort_hanging(FESpace& sp);
+
 }  // namespace pmcd

@@ -302,8 +302,6 @@ void CDSystem::set_example(const ExampleParams& ex) {
   PMCD_HIP(hipStreamSynchronize(stream_));
 }
 
-namespace {
-
 // Gather lists of the hanging-DOF coupling (see k_hanging.hip). Hanging DOF h
 // adds its row, weighted, to the rows of its coupling DOFs below n_active
 // (hanging ones included); a target entry (c, col) therefore receives the
@@ -331,17 +329,22 @@ void hanging_plan(Level& L, hipStream_t stream) {
   for(int h = 0; h < nh; ++h) max_depth = std::max(max_depth, depth[h]);
   const int n_launch = max_depth + 1;
   auto launch_of = [&](int c) { return c >= first && c < sp.n_active ? depth[c - first] - 1 : max_depth; };
-  auto pos_in_row = [&](int r, int col) {
-    const int* b = &st.col[st.row_ptr[r]];
-    const int* e = &st.col[st.row_ptr[r + 1]];
-    const int* p = std::lower_bound(b, e, col);
-    return p != e && *p == col ? (int)(st.row_ptr[r] + (p - b)) : -1;
+  auto pos_in_row = [&](int r, int col) {  // a caller's rows need not be sorted
+    for(int p = st.row_ptr[r]; p < st.row_ptr[r + 1]; ++p)
+      if(st.col[p] == col) return p;
+    return -1;
   };
   std::vector<std::vector<int>> mdst(n_launch), msp(n_launch), msrc(n_launch), rdst(n_launch), rsp(n_launch),
       rsrc(n_launch);
   std::vector<std::vector<double>> mw(n_launch), rw(n_launch);
   for(int c = 0; c < sp.n_active; ++c) {
     if(incoming[c].empty()) continue;
+    // ModifyMatrixAccordingToCoupling needs each column of the hanging rows
+    // in the coupling row (FEMatrix.C:275-291 throws otherwise)
+    for(const auto& hw : incoming[c])
+      for(int q = st.row_ptr[first + hw.first]; q < st.row_ptr[first + hw.first + 1]; ++q)
+        PMCD_CHECK(pos_in_row(c, st.col[q]) >= 0, "hanging row ", first + hw.first, " column ", st.col[q],
+                   " could not be added to row ", c, " because the structure does not include an entry here");
     const int k = launch_of(c);
     for(int p = st.row_ptr[c]; p < st.row_ptr[c + 1]; ++p) {
       const size_t before = msrc[k].size();
@@ -415,8 +418,6 @@ void apply_hanging(Level& L, hipStream_t stream) {
   launch_hanging_fix((int)L.hg_fix_pos.n, L.hg_fix_pos.p, L.hg_fix_val.p, L.val.p, L.space.n_active_non_hanging(),
                      L.space.n_hanging, L.rhs.p, stream);
 }
-
-}  // namespace
 
 void CDSystem::build_level_device(Level& L) {
   const Mesh& m = L.mesh;

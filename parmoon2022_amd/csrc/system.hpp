@@ -146,6 +146,14 @@ QuadRule upload_reference_tables(Shape shape, int order, bool linear_quad, bool 
 // Host-side setup only (meshes, FE spaces, sparsity, slot maps): no GPU use.
 std::vector<std::unique_ptr<Level>> build_levels(const ProblemDesc& d);
 
+// Hanging-DOF coupling of a level whose space has hanging DOFs
+// (Assemble2D.C:3350-3427): hanging_plan uploads the gather lists (once;
+// every column of a hanging row must be in the rows of its active coupling
+// DOFs), apply_hanging couples the assembled val and rhs and writes the
+// constraint rows and zero hanging rhs entries.
+void hanging_plan(Level& L, hipStream_t stream);
+void apply_hanging(Level& L, hipStream_t stream);
+
 // SOR sweep of a level's matrix in place (TMatrix::sor_sweep, flag 0 forward,
 // 1 backward, 2 both), on the level's row classes (built on first use)
 void prepare_level_sweep(Level& L, hipStream_t stream);

@@ -94,6 +94,114 @@ def test_assemble_into_rejects_a_pattern_without_a_coupling():
                          np.zeros(n))
 
 
+HANGING_CASES = [("TwoTriangles", 1), ("TwoTriangles", 2), ("UnitSquare", 1), ("UnitSquare", 2)]
+
+
+def _host_hanging(h):
+    n = ctypes.c_int()
+    pm._check(pm.lib().pmcd_host_hanging(h._h, -1, None, None, ctypes.byref(n)))
+    out = np.zeros(4 * max(n.value, 1), dtype=np.int32)
+    w = np.zeros(3 * max(n.value, 1))
+    pm._check(pm.lib().pmcd_host_hanging(h._h, -1, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                         w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n)))
+    return out[:4 * n.value].reshape(-1, 4), w[:3 * n.value].reshape(-1, 3)
+
+
+@pytest.mark.parametrize("geo,order", HANGING_CASES)
+def test_assemble_into_hanging_space_equals_system(geo, order):
+    """A caller's 2D space with hanging DOFs (eight refinements by indicator,
+    hanging vertices): its hanging nodes handed over in DOF order
+    (HangingNodeArray, not the sorted order the coupling needs) and the
+    columns of every row in a shuffled order. The coupled entries and rhs
+    are the bits of pmcd_assemble; a second assembly adds into the
+    non-hanging active rows and leaves the constraint rows and zero hanging
+    rhs entries as they are."""
+    # sine_2d: Dirichlet boundary only (the drop-in adds no Neumann integrals)
+    db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": 0, "example": "sine_2d",
+          "space_discretization_type": "supg", "diffusion_coefficient": 1e-2, "convection": (1.0, 0.5, 0),
+          "refine_by_indicator_steps": 8, "refine_indicator": (0.5, 1.0, 0.0, -0.5), "conforming_closure": False}
+    cd = pm.ConvectionDiffusion(db)
+    cd.assemble()
+    a_ref, f_ref = cd.matrix_values(), cd.rhs()
+    cd.close()
+    h = pm.HostSetup(db)
+    hg, w = _host_hanging(h)
+    assert len(hg) > 0
+    order_dof = np.argsort(hg[:, 0], kind="stable")
+    hg, w = hg[order_dof], w[order_dof]
+    verts, gn, bi, n, na, rp, col = caller_space(db)
+    rng = np.random.default_rng(7)
+    perm = np.concatenate([rp[r] + rng.permutation(rp[r + 1] - rp[r]) for r in range(n)])
+    col_c = col[perm]
+    entries = np.zeros(len(col))
+    rhs = np.zeros(n)
+    pm.assemble_into(SHAPE[geo], order, verts, gn, bi, n, na, rp, col_c, db, entries, rhs,
+                     dirichlet_values=f_ref[na:], hanging=(hg, w))
+    assert np.array_equal(entries, a_ref[perm])
+    assert np.array_equal(rhs, f_ref)
+    first = na - len(hg)
+    asm = pm.Assembler(SHAPE[geo], order, verts, gn, bi, n, na, rp, col_c, db, hanging=(hg, w))
+    asm.assemble(entries, rhs, dirichlet_values=f_ref[na:])
+    nh0, nh1 = rp[first], rp[na]
+    twice = a_ref[perm].copy()
+    twice[:nh0] += a_ref[perm][:nh0]
+    assert np.array_equal(entries, twice)
+    assert np.array_equal(rhs[:first], f_ref[:first] + f_ref[:first])
+    assert np.array_equal(rhs[first:], f_ref[first:]) and not np.any(rhs[first:na])
+    # device arrays of the caller
+    d_ent = DeviceArray(np.zeros(len(col)))
+    d_rhs = DeviceArray(np.full(n, 3.0))  # hanging entries are set, not added
+    d_dir = DeviceArray(f_ref[na:].copy())
+    asm.assemble_device(d_ent.ptr, d_rhs.ptr, d_dir.ptr)
+    f_dev = f_ref.copy()
+    f_dev[:first] += 3.0
+    assert np.array_equal(d_ent.get(), a_ref[perm]) and np.array_equal(d_rhs.get(), f_dev)
+    for b in (d_ent, d_rhs, d_dir):
+        b.close()
+    asm.close()
+    assert nh1 > nh0
+
+
+def test_assemble_into_rejects_bad_hanging_lists():
+    db = {"geo_file": "UnitSquare", "ANSATZ_ORDER": 1, "refinement_n_initial_steps": 0,
+          "example": "sine_laplace_2d", "refine_by_indicator_steps": 4, "refine_indicator": (0.5, 1.0, 0.0, -0.5),
+          "conforming_closure": False}
+    hg, w = _host_hanging(pm.HostSetup(db))
+    verts, gn, bi, n, na, rp, col = caller_space(db)
+    args = ("quadrangle", 1, verts, gn, bi, n, na, rp, col, db, np.zeros(len(col)), np.zeros(n))
+    bad = hg.copy()
+    bad[0, 0] = 0  # not one of the last n_hanging active DOFs
+    with pytest.raises(RuntimeError, match="hanging DOF 0 not in"):
+        pm.assemble_into(*args, hanging=(bad, w))
+    bad = hg.copy()
+    bad[0, 1:] = -1
+    with pytest.raises(RuntimeError, match="without coupling DOFs"):
+        pm.assemble_into(*args, hanging=(bad, w))
+    # a structure without one of a hanging row's columns in the row of an
+    # active coupling DOF (a column that no cell of that DOF couples)
+    first = na - len(hg)
+    cells = gn.reshape(len(bi) - 1, -1)
+    drop = []
+    for i in range(len(hg)):
+        hrow = set(col[rp[hg[i, 0]]:rp[hg[i, 0] + 1]].tolist())
+        for c in hg[i, 1:]:
+            if c < 0 or c >= na:
+                continue
+            near = set(cells[np.any(cells == c, axis=1)].reshape(-1).tolist())
+            drop = [q for q in range(rp[c], rp[c + 1]) if col[q] in hrow and col[q] not in near]
+            if drop:
+                break
+        if drop:
+            break
+    assert drop and first < na
+    col2 = np.delete(col, drop[0])
+    rp2 = rp.copy()
+    rp2[c + 1:] -= 1
+    with pytest.raises(RuntimeError, match="could not be added to row %d" % c):
+        pm.assemble_into("quadrangle", 1, verts, gn, bi, n, na, rp2, col2, db, np.zeros(len(col2)), np.zeros(n),
+                         hanging=(hg, w))
+
+
 def _mg_db(solver="fgmres", levels=3, mixed=0, smoother="jacobi"):
     return {"geo_file": "Default_UnitCube_Hexa", "ANSATZ_ORDER": 2, "refinement_n_initial_steps": 3,
             "example": "sine_3d", "space_discretization_type": "supg", "diffusion_coefficient": 1e-3,

@@ -619,3 +619,18 @@ def test_refinement_by_indicator_random_indicators_match_oracle(oracle_lib, seed
         assert len(hg) == 0
     else:
         assert len(hg) > 0
+
+
+def test_assembler_rejects_hanging_dofs_in_3d():
+    """Hanging DOFs of a caller's space are a 2D feature (the reference's
+    adaptive refinement by indicator is 2D): a 3D space that lists some is
+    refused before any GPU use."""
+    verts = np.zeros((1, 4, 3))
+    gn = np.arange(4, dtype=np.int32)
+    bi = np.array([0, 4], dtype=np.int32)
+    rp = np.arange(0, 17, 4, dtype=np.int32)
+    col = np.tile(np.arange(4, dtype=np.int32), 4)
+    hg = np.array([[3, 0, 1, -1]], dtype=np.int32)
+    w = np.array([[0.5, 0.5, 0.0]])
+    with pytest.raises(RuntimeError, match="2D only"):
+        pm.Assembler("tetrahedron", 1, verts, gn, bi, 4, 4, rp, col, {"example": "test_p1"}, hanging=(hg, w))

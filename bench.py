@@ -232,21 +232,35 @@ def kernel_times(cd, info, mixed):
     # writes x_out: (vb + 4) nnz + 4 (n + 1) + 5 vb n bytes
     jac_bytes = (vb + 4) * nnz + 4 * (n + 1) + vb * n * 5
     spmv_bytes = 12 * nnz + 4 * (n + 1) + 16 * n
+    lanes, variant = pm.tuning("jacobi")
+    sell = variant == 4 and n - info["n_dirichlet"] >= pm.get_sell_min_rows()
+    ss = cd.sell_stats(-1) if sell else None
+    fp = bool(ss and ss["mode"] >= 0 and ss["mode"] & 16)
+    if fp:
+        # x staged in LDS per 256-row block: values and 16-bit footprint
+        # positions per entry, each block's footprint columns (4 bytes each)
+        # and offsets, slot -> row and length (8 bytes per slot), x once,
+        # b, diag, the own x and x_out
+        jac_bytes = (vb + 2) * nnz + 4 * ss["fp_entries"] + 4 * (ss["fp_blocks"] + 1) + 8 * ss["slots"] + 5 * vb * n
+        spmv_bytes = 10 * nnz + 4 * ss["fp_entries"] + 4 * (ss["fp_blocks"] + 1) + 8 * ss["slots"] + 16 * n
     copy_bytes = 2 * 8 * (nnz - nnz % 2)
     # reference-tensor assembly: local matrices = [cells x 34 component pairs]
     # x [34 x 27^2] GEMM, rhs = [cells x 4 nq] x [4 nq x 27] GEMM (FP64 MFMA)
     asm_flops = (2 * 34 * 27 * 27 + 2 * 4 * info["n_quad"] * 27) * info["n_cells"]
     # its HBM stream: the local matrices and rhs written, vertex coordinates read
     asm_bytes = (27 * 27 + 27) * 8 * info["n_cells"] + 8 * 3 * 8 * info["n_cells"]
-    lanes, variant = pm.tuning("jacobi")
-    if variant == 4 and n - info["n_dirichlet"] >= pm.get_sell_min_rows():
-        jac_name = "k_jacobi_sell<%d%s>" % (pm.get_sell_unroll(), ",f32" if mixed else ",f64")
+    if sell:
+        jac_name = "k_jacobi_sell<%d%s%s>" % (pm.get_sell_unroll(), ",fp" if fp else "", ",f32" if mixed else ",f64")
     else:
         jac_name = "k_%s%s<%d%s>" % ("jacobi", {0: "", 1: "_stream", 2: "_unrolled", 3: "_unrolled_nt",
                                                 4: "_unrolled"}[variant], lanes, ",f32" if mixed else ",f64")
     return {
         "jacobi_kernel": jac_name,
         "jacobi_ms": t_jac, "jacobi_bytes": jac_bytes,
+        "sell_layout": ss,
+        # the finest sweep's rate counted in CSR bytes (12 per entry), as
+        # for the kernels that read global columns
+        "csr_equivalent_jacobi_GBs": ((vb + 4) * nnz + 4 * (n + 1) + vb * n * 5) / (t_jac * 1e-3) / 1e9,
         "spmv_f64_ms": t_spmv, "spmv_f64_GBs": spmv_bytes / (t_spmv * 1e-3) / 1e9,
         "stream_copy_ms": t_copy, "stream_copy_bytes": copy_bytes,  # the faster of the two copies
         "stream_copy_GBs": copy_bytes / (t_copy * 1e-3) / 1e9,
@@ -790,9 +804,13 @@ def bench_adaptive(args):
     n, nnz = info["n_dof"], info["nnz"]
     t_spmv = cd.time_kernel("spmv_f64", -1, 20)
     spmv_bytes = 12 * nnz + 4 * (n + 1) + 16 * n
-    achieved = spmv_bytes / (t_spmv * 1e-3) / 1e9
     lanes, variant = pm.tuning("spmv")
-    spmv_name = ("k_spmv_sell<%d,f64>" % pm.get_sell_unroll() if variant == 4 else
+    ss = cd.sell_stats(-1) if variant == 4 and n - info["n_dirichlet"] >= pm.get_sell_min_rows() else None
+    fp = bool(ss and ss["mode"] >= 0 and ss["mode"] & 16)
+    if fp:  # x staged in LDS per block (see kernel_times)
+        spmv_bytes = 10 * nnz + 4 * ss["fp_entries"] + 4 * (ss["fp_blocks"] + 1) + 8 * ss["slots"] + 16 * n
+    achieved = spmv_bytes / (t_spmv * 1e-3) / 1e9
+    spmv_name = ("k_spmv_sell<%d%s,f64>" % (pm.get_sell_unroll(), ",fp" if fp else "") if variant == 4 else
                  "k_spmv%s<%d,f64>" % ({0: "", 1: "_stream", 2: "_unrolled", 3: "_unrolled_nt"}[variant], lanes))
     import ctypes
     nh = ctypes.c_int()

@@ -182,6 +182,15 @@ int pmcd_host_sweep_order(pmcd_host* h, int level, int max_exact_classes, int* r
  * each slice of 64 slots; entry k of the row in slot t sits at
  * slice_off[t / 64] + 64 k + t % 64). No reference counterpart. */
 int pmcd_host_sell_layout(pmcd_host* h, int level, int* perm, int* len, long long* slice_off, int* n_slots);
+/* Column footprints of that layout's blocks of 256 slots (4-slice windows
+ * only), as the LDS-staged kernels (sliced ELL mode bit 4) use them: block
+ * b's distinct columns ascending in fp[fp_off[b] .. fp_off[b+1]], and for
+ * each layout entry the position of its column in its block's list (lcol, 0
+ * on padding). sizes: fp_off, fp and lcol lengths and the widest footprint;
+ * all 0 when a block has more than 4096 distinct columns (the kernels then
+ * read x directly). Null arrays: sizes only. No reference counterpart. */
+int pmcd_host_sell_footprint(pmcd_host* h, int level, int* fp_off, int* fp, unsigned short* lcol,
+                             long long* sizes);
 
 /* Mesh hierarchy, FE spaces, sparsity, device buffers.
  * Replaces ConvectionDiffusion<d>::ConvectionDiffusion
@@ -259,6 +268,12 @@ int pmcd_jacobi_sweep(pmcd_system* s, int level, const double* b, double* x, dou
  * thread, 8 a read-only stream over the values. No reference counterpart
  * (measurement). */
 int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* avg_ms);
+/* The level's sliced ELL layout as built (for byte counts of its kernels):
+ * out[0] slots (-1: not built), out[1] layout entries (values, with
+ * padding), out[2] footprint blocks, out[3] footprint entries, out[4] the
+ * widest footprint (out[2..4] 0 without footprints), out[5] the launch mode
+ * its kernels use now. No reference counterpart (measurement). */
+int pmcd_sell_stats(pmcd_system* s, int level, long long* out);
 
 /* Launch-shape tuning (process-wide; no reference counterpart, knobs of the
  * autotuning sweep): key 0 / 1 = lanes per matrix row of the SpMV / Jacobi

@@ -69,7 +69,7 @@ class SolveStats(ctypes.Structure):
 EXPORTS = [
     "pmcd_last_error", "pmcd_device_count", "pmcd_mem_info", "pmcd_system_set_data",
     "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
-    "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_hanging", "pmcd_get_hanging", "pmcd_host_sweep_order", "pmcd_host_sell_layout",
+    "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_hanging", "pmcd_get_hanging", "pmcd_host_sweep_order", "pmcd_host_sell_layout", "pmcd_host_sell_footprint", "pmcd_sell_stats",
     "pmcd_host_write_mesh", "pmcd_cycle_directions", "pmcd_vector_write", "pmcd_vector_read",
     "pmcd_vector_write_market", "pmcd_matrix_write_market", "pmcd_tcd_init", "pmcd_tcd_step", "pmcd_afc_solve",
     "pmcd_jacobi_sweep",
@@ -405,6 +405,23 @@ class HostSetup:
         k = ns.value
         return perm[:k].copy(), ln[:k].copy(), off[:k // 64 + 1].copy()
 
+    def sell_footprint(self, level=-1):
+        """Column footprints of the sliced ELL layout's 256-slot blocks:
+        (fp_off, fp, lcol, fp_max), block b's distinct columns ascending in
+        fp[fp_off[b]:fp_off[b+1]], lcol[e] the position of entry e's column
+        in its block's list; None when a block is wider than the kernels take."""
+        sizes = np.zeros(4, dtype=np.int64)
+        fn = lib().pmcd_host_sell_footprint
+        _check(fn(self._h, level, None, None, None, _ptr(sizes, ctypes.c_longlong)))
+        if sizes[0] == 0:
+            return None
+        fp_off = np.empty(sizes[0], dtype=np.int32)
+        fp = np.empty(max(1, sizes[1]), dtype=np.int32)
+        lcol = np.empty(max(1, sizes[2]), dtype=np.uint16)
+        _check(fn(self._h, level, _ptr(fp_off, ctypes.c_int), _ptr(fp, ctypes.c_int), _ptr(lcol, ctypes.c_ushort),
+                  _ptr(sizes, ctypes.c_longlong)))
+        return fp_off, fp[:sizes[1]], lcol[:sizes[2]], int(sizes[3])
+
     def write_mesh(self, path, level=-1):
         """The level's mesh as a 3D .mesh file (readable as geo_file)."""
         _check(lib().pmcd_host_write_mesh(self._h, level, os.fsencode(path)))
@@ -583,6 +600,13 @@ class ConvectionDiffusion:
         _check(lib().pmcd_sor_sweep(self._s, level, _ptr(b, ctypes.c_double), _ptr(out, ctypes.c_double),
                                     ctypes.c_double(omega), int(flag), ctypes.byref(ex)))
         return out, bool(ex.value)
+
+    def sell_stats(self, level=-1):
+        """The level's sliced ELL layout as built: slots, layout entries,
+        footprint blocks, footprint entries, widest footprint, launch mode."""
+        out = np.zeros(6, dtype=np.int64)
+        _check(lib().pmcd_sell_stats(self._s, level, _ptr(out, ctypes.c_longlong)))
+        return dict(zip(("slots", "entries", "fp_blocks", "fp_entries", "fp_max", "mode"), (int(v) for v in out)))
 
     def time_kernel(self, name, level=-1, iters=20):
         """Average device time (ms) of one kernel from HIP events on the system stream."""
@@ -1035,8 +1059,9 @@ def get_sell_min_rows():
 def set_sell_mode(mode):
     """Sliced ELL launch mode: bit 0 XCD-contiguous slices, bit 1 row-ordered
     output through LDS (layouts with 4-slice windows), bit 2 non-temporal
-    matrix loads, bit 3 matrix loads pipelined one step ahead. Same bits in
-    every mode."""
+    matrix loads, bit 3 matrix loads pipelined one step ahead, bit 4 each
+    block's x entries gathered once into LDS through its column footprint
+    (with bit 1). Same bits in every mode."""
     _check(lib().pmcd_set_tuning(13, int(mode)))
 
 

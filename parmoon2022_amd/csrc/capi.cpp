@@ -228,6 +228,24 @@ int pmcd_host_sell_layout(pmcd_host* h, int level, int* perm, int* len, long lon
   });
 }
 
+int pmcd_host_sell_footprint(pmcd_host* h, int level, int* fp_off, int* fp, unsigned short* lcol,
+                             long long* sizes) {
+  return guarded([&] {
+    PMCD_CHECK(sizes, "null argument");
+    PMCD_CHECK(sell_window() == 4 * kSellRows, "column footprints need 4-slice windows");
+    Level& L = host_level(h, level);
+    SellLayout o = build_sell(L.st, L.n_active(), sell_window());
+    SellFootprint f = build_sell_footprint(L.st, o, 4 * kSellRows, kSellMaxFootprint);
+    sizes[0] = (long long)f.fp_off.size();
+    sizes[1] = (long long)f.fp.size();
+    sizes[2] = (long long)f.lcol.size();
+    sizes[3] = f.fp_max;
+    if(fp_off) std::memcpy(fp_off, f.fp_off.data(), f.fp_off.size() * sizeof(int));
+    if(fp) std::memcpy(fp, f.fp.data(), f.fp.size() * sizeof(int));
+    if(lcol) std::memcpy(lcol, f.lcol.data(), f.lcol.size() * sizeof(unsigned short));
+  });
+}
+
 static int copy_hanging(const Level& L, int* out, double* w, int* n) {
   PMCD_CHECK(n != nullptr, "null argument");
   const FESpace& sp = L.space;
@@ -481,6 +499,23 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
   return guarded_on(s, [&] {
     PMCD_CHECK(iters > 0, "iters must be positive");
     s->sys->time_kernel(which, level, iters, avg_ms);
+  });
+}
+
+int pmcd_sell_stats(pmcd_system* s, int level, long long* out) {
+  return guarded_on(s, [&] {
+    PMCD_CHECK(out, "null argument");
+    const int nl = s->sys->n_levels();
+    const int l = level < 0 ? nl + level : level;
+    PMCD_CHECK(l >= 0 && l < nl, "level ", level, " out of range");
+    const Level& L = s->sys->level(l);
+    const bool fp = L.sell_fp_max > 0;
+    out[0] = L.sell_slots;
+    out[1] = L.sell_slots >= 0 ? (long long)L.sell_val.n : 0;
+    out[2] = fp ? (long long)L.sell_fp_off.n - 1 : 0;
+    out[3] = fp ? (long long)L.sell_fp.n : 0;
+    out[4] = fp ? L.sell_fp_max : 0;
+    out[5] = L.sell_slots >= 0 ? sell_mode_for(L.sell_window, fp ? L.sell_fp_max : 0) : -1;
   });
 }
 

@@ -24,9 +24,11 @@ __device__ inline V stream_load(const V* p) {
 }
 
 // sum over the row in `slot`: U entries of the lane loaded per step, then
-// added in order
-template <int U, bool NT, class T>
-__device__ inline T sell_row_sum(const SellView<T>& m, int slot, int len, const T* __restrict__ x) {
+// added in order. CI: the column array's type (global int columns, or 16-bit
+// positions in the block's footprint with xs the footprint's x entries)
+template <int U, bool NT, class T, class CI>
+__device__ inline T sell_row_sum(const SellView<T>& m, const CI* __restrict__ cols, int slot, int len,
+                                 const T* __restrict__ x) {
   const long long base = m.off[slot / kSellRows] + (slot % kSellRows);
   T s = 0;
   for(int k0 = 0; k0 < len; k0 += U) {
@@ -35,7 +37,7 @@ __device__ inline T sell_row_sum(const SellView<T>& m, int slot, int len, const 
 #pragma unroll
     for(int u = 0; u < U; ++u) {
       const int k = k0 + u;
-      c[u] = k < len ? stream_load<NT>(&m.col[base + (long long)k * kSellRows]) : -1;
+      c[u] = k < len ? (int)stream_load<NT>(&cols[base + (long long)k * kSellRows]) : -1;
       v[u] = k < len ? stream_load<NT>(&m.val[base + (long long)k * kSellRows]) : T(0);
     }
 #pragma unroll
@@ -50,15 +52,16 @@ __device__ inline T sell_row_sum(const SellView<T>& m, int slot, int len, const 
 // The same sum with the matrix loads of the next U entries issued before the
 // products of the current ones: the HBM latency of the matrix stream overlaps
 // the gathers of x and the arithmetic (same products, same order).
-template <int U, bool NT, class T>
-__device__ inline T sell_row_sum_pipe(const SellView<T>& m, int slot, int len, const T* __restrict__ x) {
+template <int U, bool NT, class T, class CI>
+__device__ inline T sell_row_sum_pipe(const SellView<T>& m, const CI* __restrict__ cols, int slot, int len,
+                                      const T* __restrict__ x) {
   const long long base = m.off[slot / kSellRows] + (slot % kSellRows);
   T s = 0;
   int c[U];
   T v[U];
 #pragma unroll
   for(int u = 0; u < U; ++u) {
-    c[u] = u < len ? stream_load<NT>(&m.col[base + (long long)u * kSellRows]) : -1;
+    c[u] = u < len ? (int)stream_load<NT>(&cols[base + (long long)u * kSellRows]) : -1;
     v[u] = u < len ? stream_load<NT>(&m.val[base + (long long)u * kSellRows]) : T(0);
   }
   for(int k0 = 0; k0 < len; k0 += U) {
@@ -70,7 +73,7 @@ __device__ inline T sell_row_sum_pipe(const SellView<T>& m, int slot, int len, c
 #pragma unroll
     for(int u = 0; u < U; ++u) {
       const int k = k0 + U + u;
-      cn[u] = k < len ? stream_load<NT>(&m.col[base + (long long)k * kSellRows]) : -1;
+      cn[u] = k < len ? (int)stream_load<NT>(&cols[base + (long long)k * kSellRows]) : -1;
       vn[u] = k < len ? stream_load<NT>(&m.val[base + (long long)k * kSellRows]) : T(0);
     }
 #pragma unroll
@@ -86,12 +89,37 @@ __device__ inline T sell_row_sum_pipe(const SellView<T>& m, int slot, int len, c
 }
 
 // PIPE: the pipelined sum (a kernel of its own, since it needs more registers)
-template <int U, bool PIPE, class T>
-__device__ inline T sell_row(const SellView<T>& m, int slot, int len, const T* __restrict__ x, int mode) {
+template <int U, bool PIPE, class T, class CI>
+__device__ inline T sell_row(const SellView<T>& m, const CI* cols, int slot, int len, const T* __restrict__ x,
+                             int mode) {
   if constexpr(PIPE)
-    return (mode & 4) ? sell_row_sum_pipe<U, true, T>(m, slot, len, x) : sell_row_sum_pipe<U, false, T>(m, slot, len, x);
+    return (mode & 4) ? sell_row_sum_pipe<U, true, T, CI>(m, cols, slot, len, x)
+                      : sell_row_sum_pipe<U, false, T, CI>(m, cols, slot, len, x);
   else
-    return (mode & 4) ? sell_row_sum<U, true, T>(m, slot, len, x) : sell_row_sum<U, false, T>(m, slot, len, x);
+    return (mode & 4) ? sell_row_sum<U, true, T, CI>(m, cols, slot, len, x)
+                      : sell_row_sum<U, false, T, CI>(m, cols, slot, len, x);
+}
+
+// Row sum of the slot's row. FP (mode bit 4): the block first gathers the x
+// entries of its footprint (the distinct columns of its 256 rows, about 16
+// entries per distinct column on Q2 hexahedra) into LDS with one load each,
+// then the rows read x there by their 16-bit footprint positions, so that a
+// gather instruction no longer costs the vector L1 one access per lane. The
+// products and their order are the same, so are the bits. Every thread of
+// the block must call it (it synchronizes the block).
+template <int U, bool PIPE, bool FP, class T>
+__device__ inline T sell_block_row(const SellView<T>& m, int blk, int gt, int row, const T* __restrict__ x,
+                                   int mode) {
+  if constexpr(FP) {
+    extern __shared__ __align__(16) unsigned char sell_dyn[];
+    T* xs = reinterpret_cast<T*>(sell_dyn);
+    const int f0 = m.fp_off[blk], nf = m.fp_off[blk + 1] - f0;
+    for(int i = threadIdx.x; i < nf; i += 256) xs[i] = x[m.fp[f0 + i]];
+    __syncthreads();
+    return row >= 0 ? sell_row<U, PIPE, T, unsigned short>(m, m.lcol, gt, m.len[gt], xs, mode) : T(0);
+  } else {
+    return row >= 0 ? sell_row<U, PIPE, T, int>(m, m.col, gt, m.len[gt], x, mode) : T(0);
+  }
 }
 
 // Launch layout: blocks of 256 threads (4 slices); the first slot_blocks
@@ -119,32 +147,29 @@ __device__ inline int sell_block(int bid, int slot_blocks, int mode) {
 }
 
 // y[r] = beta z[r] + alpha (A x)[r] on active rows, beta z[r] + alpha x[r] on the tail
-template <int U, bool PIPE, class T>
+template <int U, bool PIPE, bool FP, class T>
 __global__ __launch_bounds__(256) void k_spmv_sell(int n, int n_active, SellView<T> m, int slot_blocks, int mode,
                                                    const T* __restrict__ x, const T* __restrict__ z,
                                                    T* __restrict__ y, T alpha, T beta) {
   const int blk = sell_block(blockIdx.x, slot_blocks, mode);
   if(blk < slot_blocks) {
     const int gt = blk * 256 + threadIdx.x;
-    int row = -1;
-    T s = 0;
-    if(gt < m.n_slots) {
-      row = m.perm[gt];
-      if(row >= 0) s = sell_row<U, PIPE, T>(m, gt, m.len[gt], x, mode);
-    }
+    const int row = gt < m.n_slots ? m.perm[gt] : -1;
+    T s = sell_block_row<U, PIPE, FP, T>(m, blk, gt, row, x, mode);
     if(mode & 2) {
       __shared__ T sh[256];
       const int w0 = blk * 256;
       if(row >= 0) sh[row - w0] = s;
       __syncthreads();
-      row = w0 + threadIdx.x;
-      if(row >= n_active) return;
-      s = sh[threadIdx.x];
-    } else if(row < 0) {
+      const int r = w0 + threadIdx.x;
+      if(r >= n_active) return;
+      y[r] = (beta == T(0) ? T(0) : beta * z[r]) + alpha * sh[threadIdx.x];
       return;
     }
+    if(row < 0) return;
     y[row] = (beta == T(0) ? T(0) : beta * z[row]) + alpha * s;
   } else {
+
     const int r = n_active + (blk - slot_blocks) * 256 + threadIdx.x;
     if(r >= n) return;
     y[r] = (beta == T(0) ? T(0) : beta * z[r]) + alpha * x[r];
@@ -152,7 +177,7 @@ __global__ __launch_bounds__(256) void k_spmv_sell(int n, int n_active, SellView
 }
 
 // x_out = x + omega D^{-1} (b - A x), the non-active rows with A = I
-template <int U, bool PIPE, class T>
+template <int U, bool PIPE, bool FP, class T>
 __global__ __launch_bounds__(256) void k_jacobi_sell(int n, int n_active, SellView<T> m, int slot_blocks, int mode,
                                                      const T* __restrict__ diag, const T* __restrict__ b,
                                                      const T* __restrict__ x, T* __restrict__ x_out, T omega) {
@@ -161,10 +186,8 @@ __global__ __launch_bounds__(256) void k_jacobi_sell(int n, int n_active, SellVi
   T s = 0;
   if(blk < slot_blocks) {
     const int gt = blk * 256 + threadIdx.x;
-    if(gt < m.n_slots) {
-      row = m.perm[gt];
-      if(row >= 0) s = sell_row<U, PIPE, T>(m, gt, m.len[gt], x, mode);
-    }
+    row = gt < m.n_slots ? m.perm[gt] : -1;
+    s = sell_block_row<U, PIPE, FP, T>(m, blk, gt, row, x, mode);
     if(mode & 2) {
       __shared__ T sh[256];
       const int w0 = blk * 256;
@@ -215,10 +238,10 @@ int sell_blocks(int n, int n_active, const SellView<T>& m) {
 }  // namespace
 
 static int g_sell_unroll = 16;
-static int g_sell_mode = 15;
+static int g_sell_mode = 23;
 int sell_mode() { return g_sell_mode; }
 void set_sell_mode(int mode) {
-  PMCD_CHECK(mode >= 0 && mode <= 15, "sliced ELL mode must be 0..15");
+  PMCD_CHECK(mode >= 0 && mode <= 31, "sliced ELL mode must be 0..31");
   g_sell_mode = mode;
 }
 static int g_sell_window = 4;
@@ -256,10 +279,26 @@ void set_sell_unroll(int u) {
 }
 
 // row-ordered output needs each block's 256 slots to be one whole window of
-// the layout (the window it was built with, not the current setting)
+// the layout (the window it was built with, not the current setting); x in
+// LDS (bit 4) needs that too, and the footprints
+int sell_mode_for(int window, int fp_max) {
+  int mode = window == 256 ? g_sell_mode : (g_sell_mode & 13);
+  if(!(mode & 2) || fp_max <= 0 || fp_max > kSellMaxFootprint) mode &= 15;
+  return mode;
+}
+
 template <class T>
 static int sell_launch_mode(const SellView<T>& m) {
-  return m.window == 256 ? g_sell_mode : (g_sell_mode & 13);
+  return sell_mode_for(m.window, m.lcol ? m.fp_max : 0);
+}
+
+// dynamic LDS of a launch: the footprint's x entries under bit 4, at least
+// the occupancy cap's share
+template <class T>
+static size_t sell_launch_lds(const SellView<T>& m, int mode) {
+  const size_t cap = sell_dynamic_lds();
+  const size_t fp = (mode & 16) ? (size_t)m.fp_max * sizeof(T) : 0;
+  return fp > cap ? fp : cap;
 }
 
 template <class T>
@@ -267,15 +306,18 @@ void launch_spmv_sell(int n, int n_active, const SellView<T>& m, const T* x, con
                       hipStream_t s) {
   if(n == 0) return;
   const int blocks = sell_blocks(n, n_active, m), sb = sell_slot_blocks(m), mode = sell_launch_mode(m);
-  const size_t lds = sell_dynamic_lds();
-#define PMCD_SELL_SPMV(U, P) \
-  k_spmv_sell<U, P, T><<<blocks, 256, lds, s>>>(n, n_active, m, sb, mode, x, z, y, alpha, beta)
+  const size_t lds = sell_launch_lds(m, mode);
+#define PMCD_SELL_SPMV(U, P, F) \
+  k_spmv_sell<U, P, F, T><<<blocks, 256, lds, s>>>(n, n_active, m, sb, mode, x, z, y, alpha, beta)
+#define PMCD_SELL_SPMV_FP(U, P) \
+  if(mode & 16) PMCD_SELL_SPMV(U, P, true); else PMCD_SELL_SPMV(U, P, false)
   const bool pipe = (mode & 8) != 0;
   switch(g_sell_unroll) {
-    case 4: if(pipe) PMCD_SELL_SPMV(4, true); else PMCD_SELL_SPMV(4, false); break;
-    case 16: if(pipe) PMCD_SELL_SPMV(16, true); else PMCD_SELL_SPMV(16, false); break;
-    default: if(pipe) PMCD_SELL_SPMV(8, true); else PMCD_SELL_SPMV(8, false); break;
+    case 4: if(pipe) { PMCD_SELL_SPMV_FP(4, true); } else { PMCD_SELL_SPMV_FP(4, false); } break;
+    case 16: if(pipe) { PMCD_SELL_SPMV_FP(16, true); } else { PMCD_SELL_SPMV_FP(16, false); } break;
+    default: if(pipe) { PMCD_SELL_SPMV_FP(8, true); } else { PMCD_SELL_SPMV_FP(8, false); } break;
   }
+#undef PMCD_SELL_SPMV_FP
 #undef PMCD_SELL_SPMV
   PMCD_HIP(hipGetLastError());
 }
@@ -285,15 +327,18 @@ void launch_jacobi_sell(int n, int n_active, const SellView<T>& m, const T* diag
                         T omega, hipStream_t s) {
   if(n == 0) return;
   const int blocks = sell_blocks(n, n_active, m), sb = sell_slot_blocks(m), mode = sell_launch_mode(m);
-  const size_t lds = sell_dynamic_lds();
-#define PMCD_SELL_JAC(U, P) \
-  k_jacobi_sell<U, P, T><<<blocks, 256, lds, s>>>(n, n_active, m, sb, mode, diag, b, x, x_out, omega)
+  const size_t lds = sell_launch_lds(m, mode);
+#define PMCD_SELL_JAC(U, P, F) \
+  k_jacobi_sell<U, P, F, T><<<blocks, 256, lds, s>>>(n, n_active, m, sb, mode, diag, b, x, x_out, omega)
+#define PMCD_SELL_JAC_FP(U, P) \
+  if(mode & 16) PMCD_SELL_JAC(U, P, true); else PMCD_SELL_JAC(U, P, false)
   const bool pipe = (mode & 8) != 0;
   switch(g_sell_unroll) {
-    case 4: if(pipe) PMCD_SELL_JAC(4, true); else PMCD_SELL_JAC(4, false); break;
-    case 16: if(pipe) PMCD_SELL_JAC(16, true); else PMCD_SELL_JAC(16, false); break;
-    default: if(pipe) PMCD_SELL_JAC(8, true); else PMCD_SELL_JAC(8, false); break;
+    case 4: if(pipe) { PMCD_SELL_JAC_FP(4, true); } else { PMCD_SELL_JAC_FP(4, false); } break;
+    case 16: if(pipe) { PMCD_SELL_JAC_FP(16, true); } else { PMCD_SELL_JAC_FP(16, false); } break;
+    default: if(pipe) { PMCD_SELL_JAC_FP(8, true); } else { PMCD_SELL_JAC_FP(8, false); } break;
   }
+#undef PMCD_SELL_JAC_FP
 #undef PMCD_SELL_JAC
   PMCD_HIP(hipGetLastError());
 }

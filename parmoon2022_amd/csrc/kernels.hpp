@@ -394,7 +394,16 @@ struct SellView {
   const int* col;
   const T* val;
   int window;                  // rows per length-sorted window of the layout
+  // column footprints of the 256-slot blocks (null: none): block-local
+  // column of every entry, first footprint entry of each block, the
+  // footprints' global columns, the widest footprint
+  const unsigned short* lcol;
+  const int* fp_off;
+  const int* fp;
+  int fp_max;
 };
+// widest block footprint (distinct columns) the LDS-staged kernels take
+constexpr int kSellMaxFootprint = 4096;
 template <class T>
 void launch_spmv_sell(int n, int n_active, const SellView<T>& m, const T* x, const T* z, T* y, T alpha, T beta,
                       hipStream_t s);
@@ -406,9 +415,14 @@ template <class T>
 void launch_sell_fill(int n_slots, const long long* off, const int* perm, const int* len,
                       const int* rp, const T* src, T* dst, hipStream_t s);
 // sliced ELL launch mode (tuning key 13): bit 0 XCD-contiguous slices, bit 1
-// row-ordered output through LDS (used only with 4-slice windows)
+// row-ordered output through LDS (used only with 4-slice windows), bit 2
+// non-temporal matrix loads, bit 3 pipelined matrix loads, bit 4 x staged in
+// LDS per block through the column footprints (with bit 1 and footprints)
 int sell_mode();
 void set_sell_mode(int mode);
+// the mode a launch uses on a layout of `window` rows per window whose
+// widest block footprint is fp_max (0: none)
+int sell_mode_for(int window, int fp_max);
 // entries of a lane loaded per step (4, 8 or 16)
 int sell_unroll();
 void set_sell_unroll(int u);

@@ -262,4 +262,44 @@ SellLayout build_sell(const Structure& st, int n_active, int window) {
   return o;
 }
 
+SellFootprint build_sell_footprint(const Structure& st, const SellLayout& o, int block_slots, int max_fp) {
+  PMCD_CHECK(block_slots > 0 && block_slots % kSellRows == 0, "footprint blocks must be whole slices");
+  PMCD_CHECK(max_fp > 0 && max_fp <= 65536, "footprint limit must be 1..65536");
+  SellFootprint f;
+  const int n_slots = o.n_slots();
+  const int n_blocks = (n_slots + block_slots - 1) / block_slots;
+  const int n = st.n_rows;
+  std::vector<int> mark(n, -1), pos(n, 0);
+  std::vector<int> cols;
+  f.fp_off.assign(n_blocks + 1, 0);
+  f.lcol.assign((size_t)o.slice_off.back(), 0);
+  for(int b = 0; b < n_blocks; ++b) {
+    const int t0 = b * block_slots, t1 = std::min(n_slots, t0 + block_slots);
+    cols.clear();
+    for(int t = t0; t < t1; ++t) {
+      const int r = o.perm[t];
+      if(r < 0) continue;
+      for(int q = st.row_ptr[r]; q < st.row_ptr[r + 1]; ++q)
+        if(mark[st.col[q]] != b) {
+          mark[st.col[q]] = b;
+          cols.push_back(st.col[q]);
+        }
+    }
+    if((int)cols.size() > max_fp) return SellFootprint{};
+    std::sort(cols.begin(), cols.end());
+    for(int i = 0; i < (int)cols.size(); ++i) pos[cols[i]] = i;
+    for(int t = t0; t < t1; ++t) {
+      const int r = o.perm[t];
+      if(r < 0) continue;
+      const long long base = o.slice_off[t / kSellRows] + t % kSellRows;
+      for(int q = st.row_ptr[r]; q < st.row_ptr[r + 1]; ++q)
+        f.lcol[base + (long long)(q - st.row_ptr[r]) * kSellRows] = (unsigned short)pos[st.col[q]];
+    }
+    f.fp.insert(f.fp.end(), cols.begin(), cols.end());
+    f.fp_off[b + 1] = (int)f.fp.size();
+    f.fp_max = std::max(f.fp_max, (int)cols.size());
+  }
+  return f;
+}
+
 }  // namespace pmcd

@@ -87,4 +87,19 @@ struct SellLayout {
 };
 SellLayout build_sell(const Structure& st, int n_active, int window);
 
+// Column footprint of each block of `block_slots` slots of a sliced ELL
+// layout: the distinct columns of the block's rows, ascending (fp[fp_off[b]
+// .. fp_off[b+1]]), and for every layout entry its column's position in its
+// block's list (lcol, same positions as the layout's values; 0 on padding).
+// A kernel then gathers each distinct x entry of a block once into LDS and
+// reads it there by the 16-bit position. Empty (fp_off empty) when a block
+// has more than max_fp distinct columns.
+struct SellFootprint {
+  std::vector<unsigned short> lcol;
+  std::vector<int> fp_off;
+  std::vector<int> fp;
+  int fp_max = 0;
+};
+SellFootprint build_sell_footprint(const Structure& st, const SellLayout& o, int block_slots, int max_fp);
+
 }  // namespace pmcd

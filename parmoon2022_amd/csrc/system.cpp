@@ -1068,8 +1068,25 @@ void build_level_sell(Level& L, hipStream_t stream) {
   L.sell_fresh = L.sell_fresh32 = false;
 }
 
+// the column footprints of the layout's 256-slot blocks, built when a
+// launch mode first asks for them
+static void build_level_footprint(Level& L, hipStream_t stream) {
+  if(L.sell_fp_max >= 0) return;
+  L.sell_fp_max = 0;
+  if(L.sell_window != 4 * kSellRows) return;
+  SellFootprint f = build_sell_footprint(L.st, build_sell(L.st, L.n_active(), L.sell_window), 4 * kSellRows,
+                                         kSellMaxFootprint);
+  if(f.fp_off.empty() || f.fp_max == 0) return;
+  PMCD_CHECK((long long)f.lcol.size() == (long long)L.sell_col.n || L.sell_col.n == 1, "footprint layout mismatch");
+  L.sell_lcol.upload(f.lcol, stream);
+  L.sell_fp_off.upload(f.fp_off, stream);
+  L.sell_fp.upload(f.fp, stream);
+  L.sell_fp_max = f.fp_max;
+}
+
 void prepare_level_sell(Level& L, bool f32, hipStream_t stream) {
   build_level_sell(L, stream);
+  if(sell_mode() & 16) build_level_footprint(L, stream);
   if(!L.sell_fresh) {
     launch_sell_fill<double>(L.sell_slots, L.sell_off.p, L.sell_perm.p, L.sell_len.p, L.row_ptr.p, L.val.p,
                              L.sell_val.p, stream);
@@ -1086,7 +1103,10 @@ void prepare_level_sell(Level& L, bool f32, hipStream_t stream) {
 
 template <class T>
 static SellView<T> sell_view(const Level& L, const T* val) {
-  return SellView<T>{L.sell_slots, L.sell_off.p, L.sell_perm.p, L.sell_len.p, L.sell_col.p, val, L.sell_window};
+  const bool fp = L.sell_fp_max > 0;
+  return SellView<T>{L.sell_slots, L.sell_off.p, L.sell_perm.p, L.sell_len.p, L.sell_col.p, val, L.sell_window,
+                     fp ? L.sell_lcol.p : nullptr, fp ? L.sell_fp_off.p : nullptr, fp ? L.sell_fp.p : nullptr,
+                     fp ? L.sell_fp_max : 0};
 }
 
 // variant 4 on a level with at least sell_min_rows() active rows

@@ -125,6 +125,11 @@ struct Level {
   DevBuf<double> sell_val;
   DevBuf<float> sell_val32;
   bool sell_fresh = false, sell_fresh32 = false;
+  // column footprints of the layout's 256-slot blocks (sliced ELL mode bit
+  // 4): -1 not built, 0 not available (window not 256, or a block too wide)
+  int sell_fp_max = -1;
+  DevBuf<unsigned short> sell_lcol;
+  DevBuf<int> sell_fp_off, sell_fp;
   // hanging-DOF coupling after assembly: launch k takes matrix items
   // hg_mat_launch[k] .. hg_mat_launch[k+1] and rhs items hg_rhs_launch[k] ..;
   // hg_fix_* set the hanging rows' constraint entries

@@ -905,7 +905,7 @@ SELL_CASES = [ASSEMBLY_CASES[k] for k in (0, 3, 5, 6, 9)]
 
 
 @pytest.mark.parametrize("mode,window", [(0, 4), (1, 4), (2, 4), (3, 4), (3, 8), (2, 1), (4, 4), (7, 4), (7, 8),
-                                       (8, 4), (15, 4), (15, 8)],
+                                       (8, 4), (15, 4), (15, 8), (18, 4), (23, 4), (31, 4), (31, 8), (17, 4)],
                          ids=lambda v: str(v))
 @pytest.mark.parametrize("unroll", [4, 8, 16])
 @pytest.mark.parametrize("case", SELL_CASES, ids=lambda c: "%s-p%d" % c[:2])
@@ -915,7 +915,9 @@ def test_sell_spmv_and_jacobi_bitwise_serial(oracle_lib, case, unroll, mode, win
     are bitwise equal to the oracle's serial CSR loops on the same matrix, in
     every launch mode (XCD-contiguous slices, row-ordered output through LDS,
     the latter only on layouts of 4-slice windows, otherwise switched off;
-    non-temporal matrix loads; matrix loads pipelined one step ahead)."""
+    non-temporal matrix loads; matrix loads pipelined one step ahead; x
+    gathered into LDS per block through the column footprints, only with
+    row-ordered output)."""
     old_s, old_j, old_u = pm.tuning("spmv"), pm.tuning("jacobi"), pm.get_sell_unroll()
     old_m, old_w = pm.get_sell_mode(), pm.get_sell_window()
     try:

@@ -185,6 +185,46 @@ def test_sell_layout_covers_active_rows(geo, order, nref, ex):
             pm.set_sell_window(old)
 
 
+@pytest.mark.parametrize("geo,order,nref,ex", [("TwoTriangles", 1, 5, "sine_2d"), ("UnitSquare", 2, 3, "sine_2d"),
+                                               ("Default_UnitCube_Hexa", 2, 2, "sine_3d"),
+                                               ("Default_UnitCube_Hexa", 2, 3, "sine_3d"),
+                                               ("Default_UnitCube_Tetra", 2, 1, "sine_3d")])
+def test_sell_footprint_reconstructs_rows(geo, order, nref, ex):
+    """Column footprints of the 256-slot blocks: each block's list is its
+    rows' distinct columns, ascending, and every entry's 16-bit position reads
+    its CSR column back through the list (the LDS-staged kernels' index
+    rule); the widest list is reported. A layout of other windows has none."""
+    h = pm.HostSetup({"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": nref, "example": ex})
+    rp, col = h.pattern()
+    na = h.info()["n_active"]
+    assert pm.get_sell_window() == 4
+    perm, ln, off = h.sell_layout()
+    fp_off, fp, lcol, fp_max = h.sell_footprint()
+    n_slots = len(perm)
+    n_blocks = (n_slots + 255) // 256
+    assert len(fp_off) == n_blocks + 1 and fp_off[0] == 0 and fp_off[-1] == len(fp) and len(lcol) == off[-1]
+    widths = np.diff(fp_off)
+    assert fp_max == widths.max() <= 4096
+    for b in range(n_blocks):
+        rows = perm[256 * b:256 * (b + 1)]
+        rows = rows[rows >= 0]
+        cols = np.concatenate([col[rp[r]:rp[r + 1]] for r in rows])
+        assert np.array_equal(fp[fp_off[b]:fp_off[b + 1]], np.unique(cols))
+    t = np.repeat(np.arange(n_slots), ln)
+    k = np.arange(len(t)) - np.repeat(np.cumsum(ln) - ln, ln)
+    pos = off[t // 64] + 64 * k + t % 64
+    blk = t // 256
+    assert np.all(lcol[pos] < widths[blk])
+    assert np.array_equal(fp[fp_off[blk] + lcol[pos]], col[rp[perm[t]] + k])
+    old = pm.get_sell_window()
+    pm.set_sell_window(8)
+    try:
+        with pytest.raises(RuntimeError, match="4-slice windows"):
+            h.sell_footprint()
+    finally:
+        pm.set_sell_window(old)
+
+
 def _check_sell_layout(h, rp, col, na, window):
     perm, ln, off = h.sell_layout()
     n_slots = len(perm)

@@ -274,6 +274,14 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
  * widest footprint (out[2..4] 0 without footprints), out[5] the launch mode
  * its kernels use now. No reference counterpart (measurement). */
 int pmcd_sell_stats(pmcd_system* s, int level, long long* out);
+/* The V-cycle's grid transfer between `level` (-1 = finest) and the next
+ * coarser level, with the kernels the V-cycle launches, on host arrays:
+ * which 0 = prolongation y += a P x on the active fine rows (x: coarse
+ * vector, y: fine vector), 1 = restriction y = R x on the active coarse rows
+ * and 0 on the coarse Dirichlet tail (x fine, y coarse; a not read). The
+ * counterparts of GridTransfer::Prolongate / DefectRestriction
+ * (GridTransfer.C:603-1115) on the system's own hierarchy. */
+int pmcd_apply_transfer(pmcd_system* s, int level, int which, const double* x, double* y, double a);
 
 /* Launch-shape tuning (process-wide; no reference counterpart, knobs of the
  * autotuning sweep): key 0 / 1 = lanes per matrix row of the SpMV / Jacobi
@@ -302,9 +310,12 @@ int pmcd_sell_stats(pmcd_system* s, int level, long long* out);
  * the largest grid of the device CG's SpMV kernel (1..4096 blocks); key 12 =
  * largest sweep class handled by the one-launch sweep kernel; key 13 = sliced
  * ELL launch mode bits (1 XCD-contiguous slices, 2 row-ordered output through
- * LDS, 4 non-temporal matrix loads, 8 matrix loads pipelined one step ahead;
- * default 15; same bits in every mode); key 18 = sliced ELL blocks resident
- * per CU at most (0 no cap, the default, or 3..16);
+ * LDS, 4 non-temporal matrix loads, 8 matrix loads pipelined one step ahead,
+ * 16 x staged in LDS per 256-row block through its column footprint, with 2;
+ * default 23; same bits in every mode); key 18 = sliced ELL blocks resident
+ * per CU at most (0 no cap, the default, or 3..16); key 19 = 1 (default) for
+ * a sliced ELL copy of the prolongation on levels whose Jacobi sweeps use
+ * the sliced ELL kernels (serial row sums), 0 for the CSR kernel;
  * key 14 = 1 (default) for XCD-contiguous rows in the unrolled CSR kernels;
  * key 15 = FGMRES look-ahead factor (the next iteration is queued before the
  * host test while the residual exceeds this many times the threshold;

@@ -69,7 +69,7 @@ class SolveStats(ctypes.Structure):
 EXPORTS = [
     "pmcd_last_error", "pmcd_device_count", "pmcd_mem_info", "pmcd_system_set_data",
     "pmcd_host_create", "pmcd_host_destroy", "pmcd_host_info", "pmcd_host_dofmap",
-    "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_hanging", "pmcd_get_hanging", "pmcd_host_sweep_order", "pmcd_host_sell_layout", "pmcd_host_sell_footprint", "pmcd_sell_stats",
+    "pmcd_host_pattern", "pmcd_host_cell_coords", "pmcd_host_hanging", "pmcd_get_hanging", "pmcd_host_sweep_order", "pmcd_host_sell_layout", "pmcd_host_sell_footprint", "pmcd_sell_stats", "pmcd_apply_transfer",
     "pmcd_host_write_mesh", "pmcd_cycle_directions", "pmcd_vector_write", "pmcd_vector_read",
     "pmcd_vector_write_market", "pmcd_matrix_write_market", "pmcd_tcd_init", "pmcd_tcd_step", "pmcd_afc_solve",
     "pmcd_jacobi_sweep",
@@ -600,6 +600,20 @@ class ConvectionDiffusion:
         _check(lib().pmcd_sor_sweep(self._s, level, _ptr(b, ctypes.c_double), _ptr(out, ctypes.c_double),
                                     ctypes.c_double(omega), int(flag), ctypes.byref(ex)))
         return out, bool(ex.value)
+
+    def apply_transfer(self, which, x, y=None, level=-1, a=1.0):
+        """The V-cycle's transfer from/to `level`: which "prolong" returns
+        y + a P x (x coarse, y fine; y zero when None), "restrict" returns R x
+        (x fine); the kernels the V-cycle launches."""
+        k = {"prolong": 0, "restrict": 1}[which]
+        i_f, i_c = self.info(level), self.info(level - 1)
+        ny = i_f["n_dof"] if k == 0 else i_c["n_dof"]
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(ny) if y is None else np.array(y, dtype=np.float64)
+        fn = lib().pmcd_apply_transfer
+        fn.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double]
+        _check(fn(self._s, level, k, x.ctypes.data, out.ctypes.data, a))
+        return out
 
     def sell_stats(self, level=-1):
         """The level's sliced ELL layout as built: slots, layout entries,

@@ -502,6 +502,13 @@ int pmcd_time_kernel(pmcd_system* s, int which, int level, int iters, double* av
   });
 }
 
+int pmcd_apply_transfer(pmcd_system* s, int level, int which, const double* x, double* y, double a) {
+  return guarded_on(s, [&] {
+    PMCD_CHECK(x && y, "null argument");
+    s->sys->apply_transfer(level, which, x, y, a);
+  });
+}
+
 int pmcd_sell_stats(pmcd_system* s, int level, long long* out) {
   return guarded_on(s, [&] {
     PMCD_CHECK(out, "null argument");

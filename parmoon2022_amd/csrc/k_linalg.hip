@@ -1046,6 +1046,9 @@ void dense_inverse(int n, int m, double* B, double* inv, double* colbuf, int* pe
 // defaults: the unrolled CSR kernels (bit-identical to the plain ones and
 // faster on every matrix measured); the autotuner may pick others
 static int g_spmv_width = 16, g_spmv_variant = 2;
+// grid transfers of levels that smooth with the sliced ELL kernels: 1 = the
+// sliced ELL copies of P and R (serial row sums), 0 = the CSR kernels
+static int g_transfer_sell = 1;
 static int g_jacobi_width = 8, g_jacobi_variant = 2;
 
 static int g_assembly_variant = 2;
@@ -1120,6 +1123,11 @@ void set_tuning(int key, int value) {
     set_sell_blocks_per_cu(value);
     return;
   }
+  if(key == 19) {
+    PMCD_CHECK(value == 0 || value == 1, "transfer switch must be 0 or 1");
+    g_transfer_sell = value;
+    return;
+  }
   if(key == 16 || key == 17) {
     PMCD_CHECK(value == 0 || value == 1, "batch switch must be 0 or 1");
     (key == 16 ? g_batch_graphs : g_batch_coop) = value;
@@ -1184,6 +1192,7 @@ int get_tuning(int key) {
     case 16: return g_batch_graphs;
     case 17: return g_batch_coop;
     case 18: return sell_blocks_per_cu();
+    case 19: return g_transfer_sell;
     default: PMCD_FAIL("unknown tuning key ", key);
   }
 }

@@ -181,6 +181,11 @@ long long estimate_device_bytes(const ProblemDesc& pd, const SolverDesc& sd) {
     if(!coarsest) {
       const double np = n * nl;
       b += page(n + 1, 4) + page(prev_n + 1, 4) + 2 * (page(np, 4) + page(np, 8) + page(np, 4));
+      // P's sliced ELL copy (padded entries: columns, FP64 and FP32 values,
+      // 16-bit footprint positions; footprints; slot arrays)
+      const double ps = 1.25 * np + 64 * 64;
+      b += page(ps, 4) + page(ps, 8) + page(ps, 4) + page(ps, 2) + page(0.5 * np + 256, 4) + 3 * page(n + 64, 4) +
+           page(n / 64 + 2, 8);
     }
     // sweep-ordered copy (SOR/SSOR smoothing or preconditioning)
     const bool sweep = sweeps_all || (mg && coarsest && sd.coarse_smoother >= 2) ||
@@ -189,6 +194,10 @@ long long estimate_device_bytes(const ProblemDesc& pd, const SolverDesc& sd) {
     // sliced ELL copy: rows padded to the longest row of each 64-row slice
     const double slots = 1.25 * nnz + 64 * 64;
     b += 3 * page(n, 4) + page(n / 64 + 2, 8) + page(slots, 4) + page(slots, 8) + (mixed ? page(slots, 4) : 0);
+    // its block column footprints (x staged in LDS): a 16-bit position per
+    // slot entry, the footprint columns (at most one per entry; a fraction
+    // on meshes whose neighbouring rows share columns) and block offsets
+    b += page(slots, 2) + page(0.5 * nnz + 256, 4) + page(n / 256 + 2, 4);
     // hanging-DOF coupling lists: at most one item per entry and two sources
     // per item (matrix), one item and two sources per DOF (rhs), the
     // constraint entries

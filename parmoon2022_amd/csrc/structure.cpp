@@ -262,13 +262,14 @@ SellLayout build_sell(const Structure& st, int n_active, int window) {
   return o;
 }
 
-SellFootprint build_sell_footprint(const Structure& st, const SellLayout& o, int block_slots, int max_fp) {
+SellFootprint build_sell_footprint(const Structure& st, const SellLayout& o, int block_slots, int max_fp,
+                                   int n_cols) {
   PMCD_CHECK(block_slots > 0 && block_slots % kSellRows == 0, "footprint blocks must be whole slices");
   PMCD_CHECK(max_fp > 0 && max_fp <= 65536, "footprint limit must be 1..65536");
   SellFootprint f;
   const int n_slots = o.n_slots();
   const int n_blocks = (n_slots + block_slots - 1) / block_slots;
-  const int n = st.n_rows;
+  const int n = n_cols < 0 ? st.n_rows : n_cols;
   std::vector<int> mark(n, -1), pos(n, 0);
   std::vector<int> cols;
   f.fp_off.assign(n_blocks + 1, 0);

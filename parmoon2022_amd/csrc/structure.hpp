@@ -100,6 +100,8 @@ struct SellFootprint {
   std::vector<int> fp;
   int fp_max = 0;
 };
-SellFootprint build_sell_footprint(const Structure& st, const SellLayout& o, int block_slots, int max_fp);
+// n_cols: columns of the matrix (-1: st.n_rows, a square matrix)
+SellFootprint build_sell_footprint(const Structure& st, const SellLayout& o, int block_slots, int max_fp,
+                                   int n_cols = -1);
 
 }  // namespace pmcd

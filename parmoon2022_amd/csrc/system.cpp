@@ -531,6 +531,114 @@ void CDSystem::build_transfer(Level& F, const Level& C) {
   set_transfer(F, ncd, rp, col, val);
 }
 
+// The sliced ELL copy of the first `rows` rows of a transfer matrix (CSR,
+// columns < n_cols) and the column footprints of its 256-row blocks
+static void build_sell_matrix(SellMatrix& S, int rows, int n_cols, const std::vector<int>& rp,
+                              const std::vector<int>& col, const std::vector<double>& val, hipStream_t stream) {
+  S.slots = 0;
+  S.fp_max = 0;
+  if(rows <= 0) return;
+  Structure st;
+  st.n_rows = (int)rp.size() - 1;
+  st.row_ptr = rp;
+  st.col = col;
+  const int window = 4 * kSellRows;
+  SellLayout o = build_sell(st, rows, window);
+  const long long entries = std::max<long long>(1, o.slice_off.back());
+  std::vector<int> scol(entries, 0);
+  std::vector<double> sval(entries, 0.0);
+  for(int t = 0; t < o.n_slots(); ++t) {
+    const int r = o.perm[t];
+    if(r < 0) continue;
+    const long long base = o.slice_off[t / kSellRows] + t % kSellRows;
+    for(int q = rp[r]; q < rp[r + 1]; ++q) {
+      scol[base + (long long)(q - rp[r]) * kSellRows] = col[q];
+      sval[base + (long long)(q - rp[r]) * kSellRows] = val[q];
+    }
+  }
+  S.off.upload(o.slice_off, stream);
+  S.perm.upload(o.perm, stream);
+  S.len.upload(o.len, stream);
+  S.col.upload(scol, stream);
+  S.val.upload(sval, stream);
+  std::vector<float> s32(sval.begin(), sval.end());
+  S.val32.upload(s32, stream);
+  SellFootprint f = build_sell_footprint(st, o, window, kSellMaxFootprint, n_cols);
+  if(!f.fp_off.empty() && f.fp_max > 0) {
+    f.lcol.resize(entries, 0);
+    S.lcol.upload(f.lcol, stream);
+    S.fp_off.upload(f.fp_off, stream);
+    S.fp.upload(f.fp, stream);
+    S.fp_max = f.fp_max;
+  }
+  S.slots = o.n_slots();
+  // the host vectors are freed on return: wait for their uploads
+  PMCD_HIP(hipStreamSynchronize(stream));
+}
+
+template <class T>
+static SellView<T> sell_matrix_view(const SellMatrix& S, const T* val) {
+  const bool fp = S.fp_max > 0;
+  return SellView<T>{S.slots, S.off.p, S.perm.p, S.len.p, S.col.p, val, 4 * kSellRows,
+                     fp ? S.lcol.p : nullptr, fp ? S.fp_off.p : nullptr, fp ? S.fp.p : nullptr,
+                     fp ? S.fp_max : 0};
+}
+
+// the sliced ELL prolongation (tuning key 19, default on) goes with the fine
+// level's sliced ELL smoother. The restriction keeps the CSR kernel: R's rows
+// are long (up to 125 entries on Q2 hexahedra) and few, and their fine-level
+// footprints are too wide for LDS, so one lane per row loses to 16 (C2:
+// 60 us against 50 us below the finest level, 24.5 against 8.8 two levels down).
+static bool transfer_uses_sell(const Level& F) {
+  return get_tuning(19) == 1 && get_tuning(3) == 4 && F.n_active() >= sell_min_rows() && F.p_sell.slots > 0;
+}
+
+void CDSystem::prolong_add(Level& F, const double* xc, double* y, double a) {
+  if(transfer_uses_sell(F))
+    launch_spmv_sell<double>(F.n_active(), F.n_active(), sell_matrix_view<double>(F.p_sell, F.p_sell.val.p), xc, y,
+                             y, a, 1.0, stream_);
+  else
+    launch_prolong_add(F.n(), F.n_active(), (long)F.p_val.n, F.p_max_row, F.p_rp.p, F.p_col.p, F.p_val.p, xc, y, a,
+                       stream_);
+}
+
+void CDSystem::restrict_to(Level& F, const Level& C, const double* x, double* y) {
+  launch_restrict(C.n(), C.n_active(), (long)F.r_val.n, F.r_max_row, F.r_rp.p, F.r_col.p, F.r_val.p, x, y, stream_);
+}
+
+void CDSystem::prolong_add32(Level& F, const float* xc, float* y, float a) {
+  if(transfer_uses_sell(F))
+    launch_spmv_sell<float>(F.n_active(), F.n_active(), sell_matrix_view<float>(F.p_sell, F.p_sell.val32.p), xc, y,
+                            y, a, 1.0f, stream_);
+  else
+    launch_prolong_add_f32(F.n(), F.n_active(), (long)F.p_val32.n, F.p_max_row, F.p_rp.p, F.p_col.p,
+                           F.p_val32.p, xc, y, a, stream_);
+}
+
+void CDSystem::restrict_to32(Level& F, const Level& C, const float* x, float* y) {
+  launch_restrict_f32(C.n(), C.n_active(), (long)F.r_val32.n, F.r_max_row, F.r_rp.p, F.r_col.p, F.r_val32.p, x, y,
+                      stream_);
+}
+
+void CDSystem::apply_transfer(int level, int which, const double* x, double* y, double a) {
+  const int nl = n_levels();
+  const int l = level < 0 ? nl + level : level;
+  PMCD_CHECK(l >= 1 && l < nl, "level ", level, " has no coarser level");
+  PMCD_CHECK(which == 0 || which == 1, "transfer must be 0 (prolongation) or 1 (restriction)");
+  Level& F = *levels_[l];
+  const Level& C = *levels_[l - 1];
+  const int nx = which == 0 ? C.n() : F.n(), ny = which == 0 ? F.n() : C.n();
+  DevBuf<double> dx(std::max(1, nx)), dy(std::max(1, ny));
+  dx.upload(x, nx, stream_);
+  dy.upload(y, ny, stream_);
+  if(which == 0)
+    prolong_add(F, dx.p, dy.p, a);
+  else
+    restrict_to(F, C, dx.p, dy.p);
+  dy.download(y, ny, stream_);
+  PMCD_HIP(hipStreamSynchronize(stream_));
+}
+
 // P (n_fine x ncd, CSR) and its transpose R on the device, FP64 and FP32
 void CDSystem::set_transfer(Level& F, int ncd, const std::vector<int>& rp, const std::vector<int>& col,
                             const std::vector<double>& val) {
@@ -558,6 +666,7 @@ void CDSystem::set_transfer(Level& F, int ncd, const std::vector<int>& rp, const
   F.p_val32.upload(v32, stream_);
   F.r_val32.upload(t32, stream_);
   PMCD_HIP(hipStreamSynchronize(stream_));
+  build_sell_matrix(F.p_sell, F.n_active(), ncd, rp, col, val, stream_);
 }
 
 AsmArgs CDSystem::asm_args(const Level& L) {
@@ -827,9 +936,7 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
     return;
   }
   auto prolong = [&](int l) {  // damped addition of P x_{l-1} on level l
-    Level& L = *levels_[l];
-    launch_prolong_add(L.n(), L.n_active(), (long)L.p_val.n, L.p_max_row, L.p_rp.p, L.p_col.p, L.p_val.p,
-                       x[l - 1][cur[l - 1]], x[l][cur[l]], sd_.mg_damp, stream_);
+    prolong_add(*levels_[l], x[l - 1][cur[l - 1]], x[l][cur[l]], sd_.mg_damp);
   };
   std::vector<char> zero(n_levels(), 0);
   zero[Lf] = fine_zero;
@@ -863,8 +970,7 @@ void CDSystem::vcycle64(const double* rhs, double* out) {
         cur[level - 1] = 0;
       }
       zero[level - 1] = from_zero;
-      launch_restrict(C.n(), C.n_active(), (long)L.r_val.n, L.r_max_row, L.r_rp.p, L.r_col.p, L.r_val.p,
-                      L.mg_def.p, C.mg_rhs.p, stream_);
+      restrict_to(L, C, L.mg_def.p, C.mg_rhs.p);
       --level;
     } else if(dirs[i] == kMgUp) {
       prolong(level + 1);
@@ -901,9 +1007,7 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
     return;
   }
   auto prolong = [&](int l) {
-    Level& L = *levels_[l];
-    launch_prolong_add_f32(L.n(), L.n_active(), (long)L.p_val32.n, L.p_max_row, L.p_rp.p, L.p_col.p,
-                           L.p_val32.p, x[l - 1][cur[l - 1]], x[l][cur[l]], (float)sd_.mg_damp, stream_);
+    prolong_add32(*levels_[l], x[l - 1][cur[l - 1]], x[l][cur[l]], (float)sd_.mg_damp);
   };
   std::vector<char> zero(n_levels(), 0);
   zero[Lf] = fine_zero;
@@ -943,8 +1047,7 @@ void CDSystem::vcycle32(const double* rhs, double* out) {
         cur[level - 1] = 0;
       }
       zero[level - 1] = from_zero;
-      launch_restrict_f32(C.n(), C.n_active(), (long)L.r_val32.n, L.r_max_row, L.r_rp.p, L.r_col.p,
-                          L.r_val32.p, L.f_def.p, C.f_rhs.p, stream_);
+      restrict_to32(L, C, L.f_def.p, C.f_rhs.p);
       --level;
     } else if(dirs[i] == kMgUp) {
       prolong(level + 1);

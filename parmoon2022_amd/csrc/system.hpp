@@ -78,6 +78,19 @@ struct SolveStats {
   double seconds = 0;
 };
 
+// A transfer matrix (the prolongation) in the sliced ELL layout of its active rows,
+// with the column footprints of its 256-row blocks (x staged in LDS when the
+// launch mode asks for it); slots -1: not built
+struct SellMatrix {
+  int slots = -1;
+  int fp_max = 0;
+  DevBuf<long long> off;
+  DevBuf<int> perm, len, col, fp_off, fp;
+  DevBuf<unsigned short> lcol;
+  DevBuf<double> val;
+  DevBuf<float> val32;
+};
+
 struct Level {
   Mesh mesh;
   FESpace space;
@@ -100,6 +113,8 @@ struct Level {
   DevBuf<float> p_val32, r_val32;
   int n_coarse = 0;
   int p_max_row = -1, r_max_row = -1;  // longest rows of P and R
+  // P's active fine rows in the sliced ELL layout
+  SellMatrix p_sell;
   // SOR sweep classes (built on first use): class k = sweep_rows[sweep_ptr[k] ..
   // sweep_ptr[k+1]); exact = level schedule (bit-identical to the serial sweep)
   std::vector<int> sweep_ptr;
@@ -245,6 +260,9 @@ class CDSystem {
   void set_solver(const SolverDesc& s) { sd_ = s; }
   void prepare(const SolverDesc& s);      // coarse factorization, FP32 copies
   void time_kernel(int which, int level, int iters, double* avg_ms);
+  // the V-cycle's transfer between `level` and the next coarser one on host
+  // arrays: which 0 = prolongation (y += a P x), 1 = restriction (y = R x)
+  void apply_transfer(int level, int which, const double* x, double* y, double a);
   // TMatrix::sor_sweep on level L, in place: flag 0 forward, 1 backward, 2 both
   void sor_sweep(Level& L, const double* b, double* x, double omega, int flag);
   void sor_sweep32(Level& L, const float* b, float* x, float omega, int flag);
@@ -280,6 +298,12 @@ class CDSystem {
   void build_transfer(Level& fine, const Level& coarse);
   void set_transfer(Level& fine, int n_coarse, const std::vector<int>& rp, const std::vector<int>& col,
                     const std::vector<double>& val);
+  // fine += a P coarse on the active fine rows; coarse = R fine on the active
+  // coarse rows, 0 on its Dirichlet tail (the V-cycle's grid transfers)
+  void prolong_add(Level& fine, const double* coarse, double* y, double a);
+  void restrict_to(Level& fine, const Level& coarse, const double* x, double* y);
+  void prolong_add32(Level& fine, const float* coarse, float* y, float a);
+  void restrict_to32(Level& fine, const Level& coarse, const float* x, float* y);
   void assemble_level(Level& L);
   void coarse_factorize();
   // `kind`: 0 Jacobi, 1 SOR, 2 SSOR; -1 the solver's multigrid smoother

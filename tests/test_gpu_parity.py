@@ -950,6 +950,75 @@ def test_sell_spmv_and_jacobi_bitwise_serial(oracle_lib, case, unroll, mode, win
         pm.set_sell_window(old_w)
 
 
+def _serial_rows(rp, col, val, x, rows):
+    """Row sums of the first `rows` CSR rows, each taken serially in column
+    order with separate multiply and add (TMatrix::multiply's order)."""
+    ln = np.diff(rp[:rows + 1])
+    s = np.zeros(rows)
+    for k in range(int(ln.max()) if rows else 0):
+        m = ln > k
+        q = rp[:rows][m] + k
+        s[m] = s[m] + val[q] * x[col[q]]
+    return s
+
+
+TRANSFER_CASES = [("Default_UnitCube_Hexa", 2, 3, "sine_3d"), ("Default_UnitCube_Tetra", 1, 3, "sine_3d"),
+                  ("UnitSquare", 2, 4, "sine_2d")]
+
+
+@pytest.mark.parametrize("mode", [23, 7])
+@pytest.mark.parametrize("case", TRANSFER_CASES, ids=lambda c: "%s-p%d" % c[:2])
+def test_sell_transfers_bitwise_serial(case, mode):
+    """The prolongation of a level that smooths with the sliced ELL kernels
+    runs on a sliced ELL copy of P: it adds a (P x)[r], the row sum taken
+    serially in column order, on every active fine row and leaves the
+    Dirichlet rows, bit for bit, with x staged in LDS (mode 23) and without
+    (mode 7); the CSR kernel (tuning key 19 = 0) agrees to rounding. The
+    restriction (CSR kernel) writes R x with R = P^T on the active coarse
+    rows and 0 on the tail."""
+    geo, order, nref, ex = case
+    old_j, old_m, old_t = pm.tuning("jacobi"), pm.get_sell_mode(), pm.tuning_value(19)
+    db = {"geo_file": geo, "ANSATZ_ORDER": order, "refinement_n_initial_steps": nref, "example": ex,
+          "solver_type": "iterative", "iterative_solver_type": "fgmres", "preconditioner": "multigrid",
+          "multigrid_n_levels": 3}
+    try:
+        pm.set_tuning("jacobi", 16, 4)
+        pm.set_sell_mode(mode)
+        cd = pm.ConvectionDiffusion(db)
+        cd.assemble()
+        rng = np.random.default_rng(11)
+        for l in (1, 2):
+            fi, ci = cd.info(l), cd.info(l - 1)
+            prp, pcol, pval, _ = cd.transfer(l)
+            xc = rng.standard_normal(ci["n_dof"])
+            y0 = rng.standard_normal(fi["n_dof"])
+            a = 0.7
+            y = cd.apply_transfer("prolong", xc, y0, level=l, a=a)
+            want = y0.copy()
+            na = fi["n_active"]
+            want[:na] = y0[:na] + a * _serial_rows(prp, pcol, pval, xc, na)
+            assert np.array_equal(y, want)
+            pm.set_tuning_value(19, 0)
+            y_csr = cd.apply_transfer("prolong", xc, y0, level=l, a=a)
+            pm.set_tuning_value(19, 1)
+            assert np.abs(y_csr - y).max() <= 1e-13 * np.abs(y).max()
+            # R = P^T, each coarse row's entries in increasing fine row order
+            f_of = np.repeat(np.arange(fi["n_dof"]), np.diff(prp))
+            o = np.lexsort((f_of, pcol))
+            rrp = np.concatenate([[0], np.cumsum(np.bincount(pcol, minlength=ci["n_dof"]))]).astype(np.int64)
+            xf = rng.standard_normal(fi["n_dof"])
+            r = cd.apply_transfer("restrict", xf, level=l)
+            nca = ci["n_active"]
+            want_r = _serial_rows(rrp, f_of[o], pval[o], xf, nca)
+            assert np.abs(r[:nca] - want_r).max() <= 1e-13 * np.abs(want_r).max()
+            assert not np.any(r[nca:])
+        cd.close()
+    finally:
+        pm.set_tuning("jacobi", *old_j)
+        pm.set_sell_mode(old_m)
+        pm.set_tuning_value(19, old_t)
+
+
 def test_sell_mixed_vcycle_solve():
     """C3 path (FP32 V-cycle) with the sliced ELL kernels: same iteration
     count as with the CSR kernels up to one, solutions within 1e-7."""
@@ -1331,6 +1400,7 @@ def test_batch_graphs_and_cooperative_sweeps_same_bits(mixed):
             # restarts every 5 iterations, and a second solve on the same
             # batch: the graphs captured first are replayed
             for rep in range(2):
+                print("mixed %d graphs %d coop %d solve %d" % (mixed, graphs, coop, rep), flush=True)
                 bt.assemble()
                 stats = bt.solve(dict(dbs[0], gmres_restart=5))
                 out.append(([s.iterations for s in stats], [bt.solution(i) for i in range(len(dbs))]))

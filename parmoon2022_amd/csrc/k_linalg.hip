@@ -938,14 +938,23 @@ __global__ void k_gj_identity_steps(int n, int m, double* __restrict__ B, int* _
 
 // column order of the inverse: the row interchanges undone as column
 // interchanges in reverse order (one thread)
-__global__ void k_gj_colmap(int n, const int* __restrict__ perm, int* __restrict__ colmap) {
-  if(blockIdx.x != 0 || threadIdx.x != 0) return;
-  for(int j = 0; j < n; ++j) colmap[j] = j;
-  for(int k = n - 1; k >= 0; --k) {
-    const int p = perm[k], t = colmap[k];
-    colmap[k] = colmap[p];
-    colmap[p] = t;
-  }
+__global__ void k_gj_colmap(int n, const int* __restrict__ perm, int* __restrict__ colmap, bool lds) {
+  extern __shared__ int gj_map[];
+  if(blockIdx.x != 0) return;
+  // the interchanges run on a copy in LDS when it fits (one thread: each
+  // step depends on the last)
+  int* cm = lds ? gj_map : colmap;
+  for(int j = threadIdx.x; j < n; j += blockDim.x) cm[j] = j;
+  __syncthreads();
+  if(threadIdx.x == 0)
+    for(int k = n - 1; k >= 0; --k) {
+      const int p = perm[k], t = cm[k];
+      cm[k] = cm[p];
+      cm[p] = t;
+    }
+  __syncthreads();
+  if(lds)
+    for(int j = threadIdx.x; j < n; j += blockDim.x) colmap[j] = cm[j];
 }
 
 // row-major inverse: inv[i][j] = B[i][colmap[j]] (B column-major)
@@ -1034,7 +1043,8 @@ void dense_inverse(int n, int m, double* B, double* inv, double* colbuf, int* pe
   for(int k = 0; k < m; ++k)
     k_gj_step<<<blocks, 256, 0, s>>>(n, m, k, B, colbuf + (size_t)(k & 1) * n,
                                      colbuf + (size_t)((k + 1) & 1) * n, perm);
-  k_gj_colmap<<<1, 64, 0, s>>>(n, perm, colmap);
+  const bool lds_map = (size_t)n * sizeof(int) <= 64 * 1024;
+  k_gj_colmap<<<1, 256, lds_map ? (size_t)n * sizeof(int) : 0, s>>>(n, perm, colmap, lds_map);
   const long nn = (long)n * n;
   k_gj_store<<<(int)std::max(1L, std::min((nn + 255) / 256, 8192L)), 256, 0, s>>>(n, B, colmap, inv);
   PMCD_HIP(hipGetLastError());
@@ -1046,8 +1056,8 @@ void dense_inverse(int n, int m, double* B, double* inv, double* colbuf, int* pe
 // defaults: the unrolled CSR kernels (bit-identical to the plain ones and
 // faster on every matrix measured); the autotuner may pick others
 static int g_spmv_width = 16, g_spmv_variant = 2;
-// grid transfers of levels that smooth with the sliced ELL kernels: 1 = the
-// sliced ELL copies of P and R (serial row sums), 0 = the CSR kernels
+// prolongations of levels that smooth with the sliced ELL kernels: 1 = the
+// sliced ELL copy of P (serial row sums), 0 = the CSR kernel
 static int g_transfer_sell = 1;
 static int g_jacobi_width = 8, g_jacobi_variant = 2;
 

@@ -270,9 +270,8 @@ void launch_csr_to_dense(int n, const int* rp, const int* col, const double* val
                          hipStream_t s);
 // Gauss-Jordan inverse with partial pivoting, one launch per pivot: B
 // (column-major, overwritten) -> inv (row-major); colbuf holds 2 n doubles,
-// perm and colmap n ints each
-// inverse of the column-major n x n matrix B (overwritten) into row-major inv;
-// rows m..n-1 of B must be identity rows (m = n: a general matrix)
+// perm and colmap n ints each; rows m..n-1 of B must be identity rows (m = n:
+// a general matrix)
 void dense_inverse(int n, int m, double* B, double* inv, double* colbuf, int* perm, int* colmap,
                    hipStream_t s);
 

@@ -523,7 +523,20 @@ __global__ __launch_bounds__(256) void k_dot_partial(int n, const double* __rest
                                                      const double* __restrict__ y,
                                                      double* __restrict__ partial) {
   double s = 0;
-  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += x[i] * y[i];
+  const int stride = gridDim.x * 256;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  // four elements per step, loads first, products added in element order
+  for(; i + 3 * stride < n; i += 4 * stride) {
+    double xx[4], yy[4];
+#pragma unroll
+    for(int u = 0; u < 4; ++u) {
+      xx[u] = x[i + u * stride];
+      yy[u] = y[i + u * stride];
+    }
+#pragma unroll
+    for(int u = 0; u < 4; ++u) s += xx[u] * yy[u];
+  }
+  for(; i < n; i += stride) s += x[i] * y[i];
   s = block_sum(s);
   if(threadIdx.x == 0) partial[blockIdx.x] = s;
 }
@@ -688,7 +701,27 @@ __global__ __launch_bounds__(256) void k_mgs_fused(int n, int m, const double* _
   if(blockIdx.x == 0 && threadIdx.x == 0) *h_out = h;
   const double mh = -h;
   double s = 0;
-  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+  const int stride = gridDim.x * 256;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  // four of the thread's elements per step, all loads issued first (more
+  // bytes in flight per CU); the products are added in the same order as
+  // one element per step, so the partials are the same
+  for(; i + 3 * stride < n; i += 4 * stride) {
+    double rr[4], vv[4], ww[4];
+#pragma unroll
+    for(int u = 0; u < 4; ++u) {
+      rr[u] = r[i + u * stride];
+      vv[u] = v[i + u * stride];
+      if(wn) ww[u] = wn[i + u * stride];
+    }
+#pragma unroll
+    for(int u = 0; u < 4; ++u) {
+      const double ri = rr[u] + mh * vv[u];
+      r[i + u * stride] = ri;
+      s += ri * (wn ? ww[u] : ri);
+    }
+  }
+  for(; i < n; i += stride) {
     const double ri = r[i] + mh * v[i];
     r[i] = ri;
     s += ri * (wn ? wn[i] : ri);
@@ -705,7 +738,16 @@ __global__ __launch_bounds__(256) void k_scale_norm(int n, int m, const double* 
   const double hn = sqrt(block_reduce_partials(m, prev));
   if(blockIdx.x == 0 && threadIdx.x == 0) *h_out = hn;
   const double a = 1.0 / hn;
-  for(int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) v[i] = r[i] * a;
+  const int stride = gridDim.x * 256;
+  int i = blockIdx.x * 256 + threadIdx.x;
+  for(; i + 3 * stride < n; i += 4 * stride) {
+    double rr[4];
+#pragma unroll
+    for(int u = 0; u < 4; ++u) rr[u] = r[i + u * stride];
+#pragma unroll
+    for(int u = 0; u < 4; ++u) v[i + u * stride] = rr[u] * a;
+  }
+  for(; i < n; i += stride) v[i] = r[i] * a;
 }
 
 // x += sum_j y_j z_j (FGMRES update, Utilities_gmres.C:90-108), the terms

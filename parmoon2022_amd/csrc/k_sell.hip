@@ -139,6 +139,10 @@ __device__ inline T sell_block_row(const SellView<T>& m, int blk, int gt, int ro
 // x) and the output are done by thread t for row w0 + t, coalesced, instead
 // of at the sorted positions (partial cache lines, written by several
 // waves). The arithmetic per row is the same, so the bits are too.
+//
+// mode bit 4 (x staged in LDS, with bit 1 and column footprints): see
+// sell_block_row; the launch then takes the footprint's x entries as
+// dynamic LDS.
 __device__ inline int sell_block(int bid, int slot_blocks, int mode) {
   if(!(mode & 1) || bid >= slot_blocks) return bid;
   const int per = slot_blocks / 8;

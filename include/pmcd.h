@@ -321,7 +321,8 @@ int pmcd_apply_transfer(pmcd_system* s, int level, int which, const double* x, d
  * host test while the residual exceeds this many times the threshold;
  * default 4, 0 off); key 16 = 1 (default) to replay each batched FGMRES
  * iteration from a captured HIP graph; key 17 = 1 for one cooperative launch
- * per batched multicolour sweep (default 0: one launch per class). */
+ * per batched multicolour sweep (default 0: one launch per class; with 1 the
+ * iterations are launched kernel by kernel, without graphs). */
 int pmcd_set_tuning(int key, int value);
 /* Directions of one multigrid cycle (CycleControl, src/Multigrid/
  * CycleControl.C:17-111): writes at most cap entries (0 down, 1 up, 2 end,

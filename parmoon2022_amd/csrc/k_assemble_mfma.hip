@@ -323,7 +323,14 @@ __global__ __launch_bounds__(256) void k_assemble_mfma(AsmArgs a) {
 // source at the cells' points, then streams the NL^2 output columns in groups
 // of TG 16-wide tiles with the coefficient operand held in registers. MQ
 // bounds the rule's points (the source values of a wave's cells stay in LDS).
-template <int DIM, int NL, int MQ>
+//
+// TCD: also the mass matrix of the time-dependent forms (TCDMass +
+// TCDMassSUPG, CD_local_assembling_routines.C:156-168,183-327), the same
+// reference tensors with the coefficients |det J| on the value-value pair
+// and |det J| delta beta_k on the pairs (d/dxi_k test, value trial), into
+// local_M; the stiffness and rhs of TCDStiff(SUPG) / TCDRhs(SUPG) are the
+// steady forms above.
+template <int DIM, int NL, int MQ, bool TCD>
 __global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
   constexpr int NC = DIM == 2 ? 6 : 10;
   constexpr int NT = 1 + DIM;
@@ -336,6 +343,7 @@ __global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
   constexpr int TG = NTILE < 6 ? NTILE : 6;    // tiles per group
   constexpr int RT = (NL + 15) / 16;          // rhs tiles
   __shared__ double s_W[4][16][NPP];
+  __shared__ double s_Wm[4][16][TCD ? NPP : 1];
   __shared__ double s_c[4][16][NT];
   __shared__ double s_geo[4][16][13];         // c0[3], C[3][3], |det|
   __shared__ double s_g[4][16][MQ];
@@ -388,10 +396,18 @@ __global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
         for(int t = 0; t < NH; ++t) W[p++] = ad * (-delta * beta[k] * eps * mlap[t]);
       }
       for(; p < NPP; ++p) W[p] = 0.0;
+      if constexpr(TCD) {
+        double* Wm = s_Wm[wave][lane];
+        for(int q = 0; q < NPP; ++q) Wm[q] = 0.0;
+        Wm[0] = ad;
+        for(int k = 0; k < DIM; ++k) Wm[1 + DIM + k * NC] = ad * (delta * beta[k]);
+      }
       s_c[wave][lane][0] = 1.0;
       for(int k = 0; k < DIM; ++k) s_c[wave][lane][1 + k] = delta * beta[k];
     } else {
       for(int p = 0; p < NPP; ++p) W[p] = 0.0;
+      if constexpr(TCD)
+        for(int p = 0; p < NPP; ++p) s_Wm[wave][lane][p] = 0.0;
       for(int r = 0; r < NT; ++r) s_c[wave][lane][r] = 0.0;
       for(int t = 0; t < 13; ++t) s_geo[wave][lane][t] = 0.0;
     }
@@ -421,10 +437,14 @@ __global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int l16 = lane % 16, kq = lane / 16;
-  // local matrices: D[cell][e] = sum_p W[cell][p] T[p][e]
+  // local matrices: D[cell][e] = sum_p W[cell][p] T[p][e] (and the mass
+  // matrices with Wm)
+  for(int pass = 0; pass < (TCD ? 2 : 1); ++pass) {
   double av[KS];
+  const double* Wrow = (TCD && pass == 1) ? &s_Wm[wave][l16][0] : &s_W[wave][l16][0];
 #pragma unroll
-  for(int ks = 0; ks < KS; ++ks) av[ks] = s_W[wave][l16][4 * ks + kq];
+  for(int ks = 0; ks < KS; ++ks) av[ks] = Wrow[4 * ks + kq];
+  double* const out = pass == 0 ? a.local_A : a.local_M;
   // the tensor rows of K step ks + 1 are loaded while the MFMAs of step ks run
   // (the tensor is padded to whole tiles: every load is in bounds)
   const double* Tbase = a.tensor + (size_t)kq * (NTILE * 16) + l16;
@@ -456,9 +476,10 @@ __global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
 #pragma unroll
       for(int v = 0; v < 4; ++v) {
         const int cell = cell0 + kq + 4 * v;
-        if(cell < a.n_cells) a.local_A[(size_t)cell * NE + e] = acc[u][v];
+        if(cell < a.n_cells) out[(size_t)cell * NE + e] = acc[u][v];
       }
     }
+  }
   }
   // right-hand sides: F[cell][i] = sum_{q, r} g[cell][q] c[cell][r] R[q][r][i],
   // the reference-table operands of 8 K steps loaded before their MFMAs
@@ -550,22 +571,22 @@ std::vector<double> build_assembly_tensor(int dim, int nl, int nq, const double*
   return T;
 }
 
-template <int MQ>
+template <int MQ, bool TCD>
 static bool launch_tensor_mq(const AsmArgs& a, int blocks, hipStream_t s) {
   if(a.dim == 3) {
     switch(a.nl) {
-      case 27: k_assemble_tensor<3, 27, MQ><<<blocks, 256, 0, s>>>(a); break;
-      case 8: k_assemble_tensor<3, 8, MQ><<<blocks, 256, 0, s>>>(a); break;
-      case 10: k_assemble_tensor<3, 10, MQ><<<blocks, 256, 0, s>>>(a); break;
-      case 4: k_assemble_tensor<3, 4, MQ><<<blocks, 256, 0, s>>>(a); break;
+      case 27: k_assemble_tensor<3, 27, MQ, TCD><<<blocks, 256, 0, s>>>(a); break;
+      case 8: k_assemble_tensor<3, 8, MQ, TCD><<<blocks, 256, 0, s>>>(a); break;
+      case 10: k_assemble_tensor<3, 10, MQ, TCD><<<blocks, 256, 0, s>>>(a); break;
+      case 4: k_assemble_tensor<3, 4, MQ, TCD><<<blocks, 256, 0, s>>>(a); break;
       default: return false;
     }
   } else {
     switch(a.nl) {
-      case 9: k_assemble_tensor<2, 9, MQ><<<blocks, 256, 0, s>>>(a); break;
-      case 6: k_assemble_tensor<2, 6, MQ><<<blocks, 256, 0, s>>>(a); break;
-      case 4: k_assemble_tensor<2, 4, MQ><<<blocks, 256, 0, s>>>(a); break;
-      case 3: k_assemble_tensor<2, 3, MQ><<<blocks, 256, 0, s>>>(a); break;
+      case 9: k_assemble_tensor<2, 9, MQ, TCD><<<blocks, 256, 0, s>>>(a); break;
+      case 6: k_assemble_tensor<2, 6, MQ, TCD><<<blocks, 256, 0, s>>>(a); break;
+      case 4: k_assemble_tensor<2, 4, MQ, TCD><<<blocks, 256, 0, s>>>(a); break;
+      case 3: k_assemble_tensor<2, 3, MQ, TCD><<<blocks, 256, 0, s>>>(a); break;
       default: return false;
     }
   }
@@ -575,12 +596,14 @@ static bool launch_tensor_mq(const AsmArgs& a, int blocks, hipStream_t s) {
 
 bool launch_assemble_tensor(const AsmArgs& a, hipStream_t s) {
   if(a.n_cells == 0) return true;
-  if(!a.tensor || a.tcd || a.nq > 64) return false;
+  if(!a.tensor || a.nq > 64 || (a.tcd && !a.local_M)) return false;
   // P1 simplices: the thread-per-cell kernel measured faster (P1 tetrahedra
   // 0.65 vs 0.83 ms for 1.57M cells, P1 triangles 0.45 vs 0.85 ms for 2.1M)
   if(p1_simplex(a)) return false;
   const int blocks = (a.n_cells + 63) / 64;
-  return a.nq <= 32 ? launch_tensor_mq<32>(a, blocks, s) : launch_tensor_mq<64>(a, blocks, s);
+  if(a.tcd)
+    return a.nq <= 32 ? launch_tensor_mq<32, true>(a, blocks, s) : launch_tensor_mq<64, true>(a, blocks, s);
+  return a.nq <= 32 ? launch_tensor_mq<32, false>(a, blocks, s) : launch_tensor_mq<64, false>(a, blocks, s);
 }
 
 void assemble_cells(const AsmArgs& a, hipStream_t s) {

@@ -1385,7 +1385,8 @@ def test_fgmres_lookahead_same_bits(restart, mixed):
 def test_batch_graphs_and_cooperative_sweeps_same_bits(mixed):
     """The device-resident batched FGMRES gives the same iterations and bits
     with its iteration graphs on or off and with the multicolour SSOR sweep as
-    one cooperative launch or one launch per colour class."""
+    one cooperative launch or one launch per colour class (with cooperative
+    sweeps the iterations run without graphs, whatever key 16 says)."""
     dbs = _batch_dbs()
     for db in dbs:
         db["multigrid_mixed_precision"] = mixed

@@ -343,7 +343,6 @@ __global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
   constexpr int TG = NTILE < 6 ? NTILE : 6;    // tiles per group
   constexpr int RT = (NL + 15) / 16;          // rhs tiles
   __shared__ double s_W[4][16][NPP];
-  __shared__ double s_Wm[4][16][TCD ? NPP : 1];
   __shared__ double s_c[4][16][NT];
   __shared__ double s_geo[4][16][13];         // c0[3], C[3][3], |det|
   __shared__ double s_g[4][16][MQ];
@@ -396,18 +395,10 @@ __global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
         for(int t = 0; t < NH; ++t) W[p++] = ad * (-delta * beta[k] * eps * mlap[t]);
       }
       for(; p < NPP; ++p) W[p] = 0.0;
-      if constexpr(TCD) {
-        double* Wm = s_Wm[wave][lane];
-        for(int q = 0; q < NPP; ++q) Wm[q] = 0.0;
-        Wm[0] = ad;
-        for(int k = 0; k < DIM; ++k) Wm[1 + DIM + k * NC] = ad * (delta * beta[k]);
-      }
       s_c[wave][lane][0] = 1.0;
       for(int k = 0; k < DIM; ++k) s_c[wave][lane][1 + k] = delta * beta[k];
     } else {
       for(int p = 0; p < NPP; ++p) W[p] = 0.0;
-      if constexpr(TCD)
-        for(int p = 0; p < NPP; ++p) s_Wm[wave][lane][p] = 0.0;
       for(int r = 0; r < NT; ++r) s_c[wave][lane][r] = 0.0;
       for(int t = 0; t < 13; ++t) s_geo[wave][lane][t] = 0.0;
     }
@@ -441,9 +432,26 @@ __global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
   // matrices with Wm)
   for(int pass = 0; pass < (TCD ? 2 : 1); ++pass) {
   double av[KS];
-  const double* Wrow = (TCD && pass == 1) ? &s_Wm[wave][l16][0] : &s_W[wave][l16][0];
 #pragma unroll
-  for(int ks = 0; ks < KS; ++ks) av[ks] = Wrow[4 * ks + kq];
+  for(int ks = 0; ks < KS; ++ks) av[ks] = s_W[wave][l16][4 * ks + kq];
+  if(TCD && pass == 0) {
+    // the mass coefficients replace the stiffness ones in LDS (every lane
+    // holds its operand in registers by now): |det J| on the value pair,
+    // |det J| delta beta_k on the pairs (d/dxi_k test, value trial)
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if(lane < 16) {
+      double* W = s_W[wave][lane];
+      const double ad = s_geo[wave][lane][12];
+      for(int p = 0; p < NPP; ++p) W[p] = 0.0;
+      W[0] = ad;
+      for(int k = 0; k < DIM; ++k) W[1 + DIM + k * NC] = ad * s_c[wave][lane][1 + k];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
   double* const out = pass == 0 ? a.local_A : a.local_M;
   // the tensor rows of K step ks + 1 are loaded while the MFMAs of step ks run
   // (the tensor is padded to whole tiles: every load is in bounds)

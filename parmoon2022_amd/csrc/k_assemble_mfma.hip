@@ -430,6 +430,7 @@ __global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
   const int l16 = lane % 16, kq = lane / 16;
   // local matrices: D[cell][e] = sum_p W[cell][p] T[p][e] (and the mass
   // matrices with Wm)
+#pragma unroll 1
   for(int pass = 0; pass < (TCD ? 2 : 1); ++pass) {
   double av[KS];
 #pragma unroll
@@ -456,7 +457,7 @@ __global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
   // the tensor rows of K step ks + 1 are loaded while the MFMAs of step ks run
   // (the tensor is padded to whole tiles: every load is in bounds)
   const double* Tbase = a.tensor + (size_t)kq * (NTILE * 16) + l16;
-  for(int t0 = 0; t0 < NTILE; t0 += TG) {
+  auto tile_group = [&](int t0) {
     v4d acc[TG];
     double bcur[TG], bnext[TG];
 #pragma unroll
@@ -487,6 +488,14 @@ __global__ __launch_bounds__(256) void k_assemble_tensor(AsmArgs a) {
         if(cell < a.n_cells) out[(size_t)cell * NE + e] = acc[u][v];
       }
     }
+  };
+  if constexpr(TCD) {
+    // the time-dependent kernel (two passes) keeps the tile loop rolled:
+    // unrolled, it spills registers
+#pragma unroll 1
+    for(int t0 = 0; t0 < NTILE; t0 += TG) tile_group(t0);
+  } else {
+    for(int t0 = 0; t0 < NTILE; t0 += TG) tile_group(t0);
   }
   }
   // right-hand sides: F[cell][i] = sum_{q, r} g[cell][q] c[cell][r] R[q][r][i],

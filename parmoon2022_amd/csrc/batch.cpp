@@ -389,9 +389,9 @@ std::vector<SolveStats> BatchSystem::solve(const SolverDesc& sd) {
   }
   auto run_tail = [&](int i, bool after_coop) {
     // with cooperative sweeps the rest of the iteration is launched kernel
-    // by kernel: the test of graphs and cooperative sweeps together stopped
-    // once in about ten runs with a breakdown on a replayed solve, which no
-    // ordering in this code explains (see the design notes)
+    // by kernel: graph replay and cooperative sweeps together have ended a
+    // replayed solve with a breakdown now and then (FP32 sweeps), which no
+    // ordering in this code explains; every launch here is on one stream
     if(!batch_graphs() || coop) {
       tail(i, after_coop);
       return;
